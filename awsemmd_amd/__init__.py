@@ -1,0 +1,296 @@
+"""awsemmd_amd -- MI355X (gfx950) FixBackbone force path of AWSEM-MD.
+
+Python mirror of the reference's FixBackbone interface (src/fix_backbone.h:
+33-49) over the C ABI in include/awsem.h, implemented by the in-tree HIP
+library libawsem_hip.so.  There is no CPU fallback: loading fails loudly if
+the library is missing.
+
+    fix = FixBackbone("fix_backbone_coeff.data", "protein.seq", workdir)
+    fix.set_topology(group, mol, res)        # Construct_Computational_Arrays
+    e = fix.pre_force(x, f)                  # f += forces; per-term energies
+    fix.compute_vector(k)                    # == energy[k+1] (fb.cpp:8223-8232)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libawsem_hip.so")
+
+ET_NAMES = ["Total", "Chain", "Shake", "Chi", "Rama", "Excluded", "DSSP", "P_AP", "Water", "Burial",
+            "Helix", "AMH-Go", "Frag_Mem", "Vec_FM", "Contact_Restraints", "Membrane", "SSB", "Electro",
+            "Bond"]
+ET = {name: k for k, name in enumerate(ET_NAMES)}
+N_ET = len(ET_NAMES)
+PREC_FP64, PREC_FP32 = 0, 1
+
+ERRORS = {-1: "AWSEM_E_ARG", -2: "AWSEM_E_FILE", -3: "AWSEM_E_FORMAT", -4: "AWSEM_E_UNSUPPORTED",
+          -5: "AWSEM_E_TOPOLOGY", -6: "AWSEM_E_MISSING_ATOM", -7: "AWSEM_E_DEVICE", -8: "AWSEM_E_NOMEM",
+          -9: "AWSEM_E_NONFINITE", -10: "AWSEM_E_STATE"}
+
+EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_compute", "awsem_compute_res",
+           "awsem_set_vexcluded", "awsem_set_bonds", "awsem_set_term_mask", "awsem_list", "awsem_run",
+           "awsem_n_residues", "awsem_pair_list_cutoff", "awsem_last_error", "awsem_destroy",
+           "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
+           "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_last_error",
+           "awsem_batch_destroy"]
+
+
+class AwsemError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Langevin(C.Structure):
+    _fields_ = [("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double), ("damp", C.c_double),
+                ("seed", C.c_uint64), ("step0", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libawsem_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `python -m awsemmd_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.awsem_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(vp)]
+        L.awsem_create_arrays.argtypes = [C.c_char_p, C.c_char_p, C.c_int, ip, C.c_char_p, dp, C.c_int, ip, ip, dp,
+                                          dp, C.c_char_p, C.c_int, C.POINTER(vp)]
+        L.awsem_set_topology.argtypes = [vp, C.c_int, ip, ip, ip]
+        L.awsem_compute.argtypes = [vp, dp, dp, dp]
+        L.awsem_compute_res.argtypes = [vp, dp, dp, dp]
+        L.awsem_set_vexcluded.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
+        L.awsem_set_bonds.argtypes = [vp, C.c_int, C.c_double, dp]
+        L.awsem_set_term_mask.argtypes = [vp, C.c_uint32]
+        L.awsem_list.argtypes = [vp, C.c_int, C.c_int, dp, C.POINTER(C.c_int64), C.c_int, ip]
+        L.awsem_run.argtypes = [vp, C.c_int, C.POINTER(Langevin), dp, dp, dp, dp]
+        L.awsem_n_residues.argtypes = [vp, ip]
+        L.awsem_pair_list_cutoff.argtypes = [vp]
+        L.awsem_pair_list_cutoff.restype = C.c_double
+        L.awsem_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.awsem_destroy.argtypes = [vp]
+        L.awsem_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp)]
+        L.awsem_batch_size.argtypes = [vp, ip, ip]
+        L.awsem_batch_set_state.argtypes = [vp, dp, dp]
+        L.awsem_batch_get_state.argtypes = [vp, dp, dp, dp, dp]
+        L.awsem_batch_compute.argtypes = [vp]
+        L.awsem_batch_run.argtypes = [vp, C.c_int, C.POINTER(Langevin)]
+        L.awsem_batch_timing.argtypes = [vp, dp, dp, ip]
+        L.awsem_batch_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.awsem_batch_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _dp(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class FixBackbone:
+    """One system (the LAMMPS `fix ID CA backbone CB O coeff seq` instance)."""
+
+    def __init__(self, coeff: str, seq: str, workdir: str = ".", precision: int = PREC_FP64):
+        h = C.c_void_p()
+        rc = lib().awsem_create(coeff.encode(), seq.encode(), workdir.encode(), precision, C.byref(h))
+        self._finish(h, rc)
+
+    @classmethod
+    def from_arrays(cls, coeff: str, param_dir: str, chains: Sequence[str], aps: Optional[np.ndarray],
+                    mem_pos, mem_len, mem_weight, mem_xyz_nm, mem_seq: str, precision: int = PREC_FP32):
+        self = cls.__new__(cls)
+        chain_len = np.array([len(c) for c in chains], dtype=np.int32)
+        seq = "".join(chains).encode()
+        a = None if aps is None else np.ascontiguousarray(aps, dtype=np.float64)
+        mp = np.ascontiguousarray(mem_pos, dtype=np.int32)
+        ml = np.ascontiguousarray(mem_len, dtype=np.int32)
+        mw = np.ascontiguousarray(mem_weight, dtype=np.float64)
+        mx = np.ascontiguousarray(mem_xyz_nm, dtype=np.float64)
+        h = C.c_void_p()
+        rc = lib().awsem_create_arrays(coeff.encode(), param_dir.encode(), len(chains), _ip(chain_len), seq, _dp(a),
+                                       len(mp), _ip(mp), _ip(ml), _dp(mw), _dp(mx), mem_seq.encode(), precision,
+                                       C.byref(h))
+        self._finish(h, rc)
+        return self
+
+    def _finish(self, h, rc):
+        self.h = h
+        if rc != 0:
+            msg = self.last_error() if h.value else ""
+            if h.value:
+                lib().awsem_destroy(h)
+            self.h = C.c_void_p()
+            raise AwsemError(rc, msg)
+        n = C.c_int()
+        lib().awsem_n_residues(h, C.byref(n))
+        self.n = n.value
+        self.energy = np.zeros(N_ET)
+
+    def last_error(self) -> str:
+        buf = C.create_string_buffer(1024)
+        lib().awsem_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AwsemError(rc, self.last_error())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().awsem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def pair_list_cutoff(self) -> float:
+        return lib().awsem_pair_list_cutoff(self.h)
+
+    def set_topology(self, group, mol, res):
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        m = np.ascontiguousarray(mol, dtype=np.int32)
+        r = np.ascontiguousarray(res, dtype=np.int32)
+        self._check(lib().awsem_set_topology(self.h, len(g), _ip(g), _ip(m), _ip(r)))
+
+    def set_term_mask(self, mask: int):
+        self._check(lib().awsem_set_term_mask(self.h, mask & 0xFFFFFFFF))
+
+    def set_vexcluded(self, enable: bool = True, p: int = 2, lam=None, cut_short=None, cut_long=None):
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(36) for a in (lam, cut_short, cut_long)]
+        self._check(lib().awsem_set_vexcluded(self.h, int(enable), p, *[_dp(a) for a in arr]))
+
+    def set_bonds(self, enable: bool = True, k: float = 200.0, r0=None):
+        r = None if r0 is None else np.ascontiguousarray(r0, dtype=np.float64)
+        self._check(lib().awsem_set_bonds(self.h, int(enable), k, _dp(r)))
+
+    def pre_force(self, x: np.ndarray, f: np.ndarray) -> np.ndarray:
+        """FixBackbone::pre_force: f[n_atoms,3] += forces (atom layout)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert f.dtype == np.float64 and f.flags.c_contiguous
+        e = np.zeros(N_ET)
+        self._check(lib().awsem_compute(self.h, _dp(x), _dp(f), _dp(e)))
+        self.energy = e
+        return e
+
+    def compute_res(self, xres: np.ndarray):
+        """Residue layout (3, n, 3) [CA, CB, O]; returns (forces, energies)."""
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        f = np.zeros_like(x)
+        e = np.zeros(N_ET)
+        self._check(lib().awsem_compute_res(self.h, _dp(x), _dp(f), _dp(e)))
+        self.energy = e
+        return f, e
+
+    def compute_scalar(self) -> float:
+        return float(self.energy[0])
+
+    def compute_vector(self, k: int) -> float:
+        return float(self.energy[k + 1])
+
+    def list(self, kind: int, xres: np.ndarray, arg: int = 0, cap: int = 1 << 20) -> np.ndarray:
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        out = np.zeros(cap, dtype=np.int64)
+        cnt = C.c_int()
+        self._check(lib().awsem_list(self.h, kind, arg, _dp(x), out.ctypes.data_as(C.POINTER(C.c_int64)), cap,
+                                     C.byref(cnt)))
+        return out[: cnt.value].copy()
+
+    def run(self, xres, vres, nsteps, dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=1, step0=0):
+        x = np.ascontiguousarray(xres, dtype=np.float64).copy()
+        v = np.ascontiguousarray(vres, dtype=np.float64).copy()
+        f = np.zeros_like(x)
+        e = np.zeros(N_ET)
+        lg = Langevin(dt, t_start, t_stop, damp, seed, step0)
+        self._check(lib().awsem_run(self.h, nsteps, C.byref(lg), _dp(x), _dp(v), _dp(f), _dp(e)))
+        self.energy = e
+        return x, v, f, e
+
+
+class Batch:
+    """Independent replicas evaluated by the resident (workgroup-per-replica) engine."""
+
+    def __init__(self, fixes: Sequence[FixBackbone], precision: int = PREC_FP32):
+        arr = (C.c_void_p * len(fixes))(*[fx.h.value for fx in fixes])
+        h = C.c_void_p()
+        rc = lib().awsem_batch_create(len(fixes), arr, precision, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.last_error() if h.value else ""
+            raise AwsemError(rc, msg)
+        nrep, nres = C.c_int(), C.c_int()
+        lib().awsem_batch_size(h, C.byref(nrep), C.byref(nres))
+        self.nrep, self.nres = nrep.value, nres.value
+        self.sizes = [fx.n for fx in fixes]
+
+    def last_error(self) -> str:
+        buf = C.create_string_buffer(1024)
+        lib().awsem_batch_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AwsemError(rc, self.last_error())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().awsem_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, x: Optional[np.ndarray] = None, v: Optional[np.ndarray] = None):
+        xs = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        vs = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        self._check(lib().awsem_batch_set_state(self.h, _dp(xs), _dp(vs)))
+
+    def get_state(self):
+        n9 = self.nres * 9
+        x, v, f = np.zeros(n9), np.zeros(n9), np.zeros(n9)
+        e = np.zeros(self.nrep * N_ET)
+        self._check(lib().awsem_batch_get_state(self.h, _dp(x), _dp(v), _dp(f), _dp(e)))
+        return x, v, f, e.reshape(self.nrep, N_ET)
+
+    def compute(self):
+        self._check(lib().awsem_batch_compute(self.h))
+
+    def run(self, nsteps, dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=1, step0=0):
+        lg = Langevin(dt, t_start, t_stop, damp, seed, step0)
+        self._check(lib().awsem_batch_run(self.h, nsteps, C.byref(lg)))
+
+    def timing(self):
+        a, b, k = C.c_double(), C.c_double(), C.c_int()
+        lib().awsem_batch_timing(self.h, C.byref(a), C.byref(b), C.byref(k))
+        return a.value, b.value, k.value
+
+    def split(self, flat: np.ndarray):
+        """Splits a concatenated residue-layout array into per-replica (3, n, 3) arrays."""
+        out, off = [], 0
+        for n in self.sizes:
+            out.append(flat[off * 9:(off + n) * 9].reshape(3, n, 3))
+            off += n
+        return out
+
+
+def pack_states(states: Sequence[np.ndarray]) -> np.ndarray:
+    """Concatenates per-replica (3, n, 3) arrays in batch order."""
+    return np.concatenate([np.ascontiguousarray(s, dtype=np.float64).reshape(-1) for s in states])

@@ -1,0 +1,649 @@
+// awsem_api.cpp -- the C ABI of include/awsem.h over the host parsers and
+// the gfx950 kernels.
+#include "../../include/awsem.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "awsem_host.h"
+#include "awsem_launch.h"
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return AwStatus::err(AWSEM_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device copy of a set of systems
+
+struct DevBatch {
+  int nrep = 0, max_n = 0, total_res = 0, fp32 = 0, lds = 0;
+  std::vector<ReplicaDesc> reps;
+  std::vector<int> res_off;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void *d_rep = nullptr, *d_params = nullptr, *d_groups = nullptr, *d_pairs = nullptr;
+  void *d_coef = nullptr, *d_rf = nullptr, *d_items = nullptr;
+  int *d_rinfo = nullptr, *d_chs = nullptr, *d_chl = nullptr, *d_queue = nullptr, *d_order = nullptr;
+  double *d_aps = nullptr, *d_x = nullptr, *d_v = nullptr, *d_f = nullptr, *d_e = nullptr, *d_scr = nullptr;
+  long long *d_list = nullptr;
+  int *d_count = nullptr;
+  int list_cap = 0;
+  FFParams P;
+  double last_ms = 0.0;
+
+  ~DevBatch()
+  {
+    void *ptrs[] = {d_rep, d_params, d_groups, d_pairs, d_coef, d_rf, d_items, d_rinfo, d_chs, d_chl,
+                    d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count};
+    for (void *p : ptrs) if (p) hipFree(p);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  AwBatchPtrs ptrs() const
+  {
+    AwBatchPtrs p;
+    p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds;
+    p.params = d_params; p.rinfo = d_rinfo; p.aps = d_aps; p.ch_start = d_chs; p.ch_len = d_chl;
+    p.x = d_x; p.v = d_v; p.f = d_f; p.energy = d_e; p.scratch = d_scr;
+    p.groups = d_groups; p.pairs = d_pairs; p.coef = d_coef; p.rf = d_rf; p.items = d_items;
+    p.queue = d_queue; p.order = d_order;
+    return p;
+  }
+};
+
+template <class T>
+static AwStatus upload(T **dst, const std::vector<T> &src, size_t min_elems = 1)
+{
+  size_t n = std::max(src.size(), min_elems);
+  HIPCHK(hipMalloc((void **)dst, n * sizeof(T)));
+  if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return AwStatus();
+}
+static AwStatus upload_raw(void **dst, const void *src, size_t bytes)
+{
+  HIPCHK(hipMalloc(dst, std::max<size_t>(bytes, 16)));
+  if (bytes) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return AwStatus();
+}
+
+static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int precision, bool want_lds,
+                               std::unique_ptr<DevBatch> &out)
+{
+  auto D = std::unique_ptr<DevBatch>(new DevBatch());
+  D->nrep = (int)sys.size();
+  D->fp32 = precision == AWSEM_PREC_FP32;
+  D->P = sys[0]->P;
+  std::vector<int> rinfo, chs, chl;
+  std::vector<double> aps;
+  AwFMPack fm;
+  long long scr = 0;
+  int res = 0, ch = 0;
+  for (int r = 0; r < D->nrep; ++r) {
+    const AwSystem &S = *sys[r];
+    ReplicaDesc R;
+    std::memset(&R, 0, sizeof R);
+    R.n = S.n; R.nch = S.nch; R.res_off = res; R.ch_off = ch;
+    R.fm_group_off = (int)fm.groups.size();
+    int item_base = (int)(fm.items.size() / 4);
+    R.fm_item_base = item_base;
+    if (S.P.fm_flag) {
+      AwStatus s = aw_build_fm(S, fm, item_base);
+      if (!s.ok()) return s;
+    }
+    R.fm_ngroups = (int)fm.groups.size() - R.fm_group_off;
+    R.fm_items = (int)(fm.items.size() / 4) - item_base;
+    R.scratch_off = scr;
+    scr += aw_slab_doubles(S.n);
+    D->reps.push_back(R);
+    D->res_off.push_back(res);
+    rinfo.insert(rinfo.end(), S.rinfo.begin(), S.rinfo.end());
+    aps.insert(aps.end(), S.aps.begin(), S.aps.end());
+    chs.insert(chs.end(), S.ch_start.begin(), S.ch_start.end());
+    chl.insert(chl.end(), S.ch_len.begin(), S.ch_len.end());
+    res += S.n; ch += S.nch;
+    D->max_n = std::max(D->max_n, S.n);
+  }
+  D->total_res = res;
+  D->lds = want_lds && D->max_n <= aw_lds_max_residues();
+  AwStatus s;
+  HIPCHK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&D->ev0));
+  HIPCHK(hipEventCreate(&D->ev1));
+  if (!(s = upload_raw(&D->d_rep, D->reps.data(), D->reps.size() * sizeof(ReplicaDesc))).ok()) return s;
+  if (!(s = upload_raw(&D->d_params, &D->P, sizeof(FFParams))).ok()) return s;
+  if (!(s = upload(&D->d_rinfo, rinfo)).ok()) return s;
+  if (!(s = upload(&D->d_aps, aps)).ok()) return s;
+  if (!(s = upload(&D->d_chs, chs)).ok()) return s;
+  if (!(s = upload(&D->d_chl, chl)).ok()) return s;
+  if (!(s = upload_raw(&D->d_groups, fm.groups.data(), fm.groups.size() * sizeof(FMGroup))).ok()) return s;
+  if (!(s = upload_raw(&D->d_pairs, fm.pairs.data(), fm.pairs.size() * sizeof(int))).ok()) return s;
+  if (D->fp32) {
+    std::vector<float> c(fm.coef.begin(), fm.coef.end()), rf(fm.rf.begin(), fm.rf.end()), it(fm.items.begin(), fm.items.end());
+    if (!(s = upload_raw(&D->d_coef, c.data(), c.size() * sizeof(float))).ok()) return s;
+    if (!(s = upload_raw(&D->d_rf, rf.data(), rf.size() * sizeof(float))).ok()) return s;
+    if (!(s = upload_raw(&D->d_items, it.data(), it.size() * sizeof(float))).ok()) return s;
+  } else {
+    if (!(s = upload_raw(&D->d_coef, fm.coef.data(), fm.coef.size() * sizeof(double))).ok()) return s;
+    if (!(s = upload_raw(&D->d_rf, fm.rf.data(), fm.rf.size() * sizeof(double))).ok()) return s;
+    if (!(s = upload_raw(&D->d_items, fm.items.data(), fm.items.size() * sizeof(double))).ok()) return s;
+  }
+  size_t st = (size_t)res * 9 * sizeof(double);
+  HIPCHK(hipMalloc((void **)&D->d_x, st));
+  HIPCHK(hipMalloc((void **)&D->d_v, st));
+  HIPCHK(hipMalloc((void **)&D->d_f, st));
+  HIPCHK(hipMemset(D->d_x, 0, st));
+  HIPCHK(hipMemset(D->d_v, 0, st));
+  HIPCHK(hipMemset(D->d_f, 0, st));
+  HIPCHK(hipMalloc((void **)&D->d_e, (size_t)D->nrep * AW_N_ET * sizeof(double)));
+  HIPCHK(hipMemset(D->d_e, 0, (size_t)D->nrep * AW_N_ET * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&D->d_scr, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
+  HIPCHK(hipMemset(D->d_scr, 0, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&D->d_queue, 64));
+  HIPCHK(hipMalloc((void **)&D->d_count, 64));
+  // longest-first order for the resident engine's work queue
+  std::vector<int> order(D->nrep);
+  for (int r = 0; r < D->nrep; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return D->reps[a].n > D->reps[b].n; });
+  if (!(s = upload(&D->d_order, order)).ok()) return s;
+  HIPCHK(hipDeviceSynchronize());
+  out = std::move(D);
+  return AwStatus();
+}
+
+// ---------------------------------------------------------------------------
+
+struct awsem_ctx {
+  AwSystem sys;
+  int precision = AWSEM_PREC_FP64;
+  std::string err;
+  bool topo = false;
+  int n_atoms = 0;
+  std::vector<int> atom_of;   // [3*n]: atom index of (kind, residue)
+  std::unique_ptr<DevBatch> dev;
+  bool dirty = true;          // parameters changed since dev was built
+};
+
+struct awsem_batch {
+  std::unique_ptr<DevBatch> dev;
+  std::string err;
+  double ms_total = 0.0;
+  int n_kernels = 0;
+};
+
+static int fail(awsem_ctx *c, const AwStatus &s)
+{
+  if (c) c->err = s.msg;
+  return s.code;
+}
+
+static AwStatus load_system(const std::string &coeff, const std::string &seq_path, const std::string &dir, AwSystem &S,
+                            bool from_files)
+{
+  std::string mem_file, gamma_file;
+  AwStatus s = aw_parse_coeff(coeff, S.P, mem_file, gamma_file);
+  if (!s.ok()) return s;
+  if (from_files) {
+    if (!(s = aw_read_sequence(seq_path, S)).ok()) return s;
+  }
+  if (!(s = aw_read_param_tables(dir, S)).ok()) return s;
+  if (from_files) {
+    if (!(s = aw_read_ssweight(dir, S)).ok()) return s;
+  }
+  if (S.P.fm_flag) {
+    std::string gp = gamma_file.empty() || gamma_file[0] == '/' ? gamma_file : dir + "/" + gamma_file;
+    if (!(s = aw_read_gamma_array(gp, S.fmg)).ok()) return s;
+    if (from_files && !(s = aw_read_mems(dir, mem_file, S)).ok()) return s;
+  }
+  return AwStatus();
+}
+
+extern "C" {
+
+int awsem_create(const char *coeff_path, const char *seq_path, const char *workdir, int precision, awsem_ctx **out)
+{
+  if (!out || !coeff_path || !seq_path) return AWSEM_E_ARG;
+  *out = nullptr;
+  std::unique_ptr<awsem_ctx> c(new awsem_ctx());
+  c->precision = precision;
+  std::string dir = workdir ? workdir : ".";
+  AwStatus s = load_system(coeff_path, seq_path, dir, c->sys, true);
+  if (s.ok()) s = aw_finish_system(c->sys);
+  if (!s.ok()) {
+    // keep the message reachable for the caller through a context-less path
+    *out = c.release();
+    (*out)->err = s.msg;
+    (*out)->sys.n = -1;
+    return s.code;
+  }
+  *out = c.release();
+  return AWSEM_OK;
+}
+
+int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, const int *chain_len, const char *seq,
+                        const double *aps, int n_mem, const int *mem_pos, const int *mem_len, const double *mem_weight,
+                        const double *mem_xyz_nm, const char *mem_seq, int precision, awsem_ctx **out)
+{
+  if (!out || !coeff_path || nch <= 0 || !chain_len || !seq) return AWSEM_E_ARG;
+  *out = nullptr;
+  std::unique_ptr<awsem_ctx> c(new awsem_ctx());
+  c->precision = precision;
+  AwSystem &S = c->sys;
+  AwStatus s = load_system(coeff_path, "", param_dir ? param_dir : ".", S, false);
+  if (s.ok()) {
+    S.nch = nch;
+    int off = 0;
+    for (int k = 0; k < nch; ++k) {
+      if (chain_len[k] <= 0) { s = AwStatus::err(AWSEM_E_ARG, "chain length must be positive"); break; }
+      S.ch_start.push_back(off);
+      S.ch_len.push_back(chain_len[k]);
+      off += chain_len[k];
+    }
+    S.n = off;
+    if (s.ok()) {
+      S.se.assign(seq, seq + S.n);
+      for (char ch : S.se)
+        if (ch < 'A' || ch > 'Z') { s = AwStatus::err(AWSEM_E_FORMAT, "Sequence: unknown residue letter"); break; }
+    }
+    if (s.ok()) {
+      S.aps.assign((size_t)S.n * 12, 0.0);
+      if (aps && S.P.ssweight_flag)
+        for (int i = 0; i < S.n; ++i)
+          for (int k = 0; k < 12; ++k)
+            if (S.P.ssweight[k]) S.aps[(size_t)i * 12 + k] = aps[(size_t)i * 12 + k];
+    }
+    if (s.ok() && S.P.fm_flag) {
+      long long off2 = 0;
+      for (int k = 0; k < n_mem; ++k) {
+        AwMemory m;
+        m.pos = mem_pos[k]; m.len = mem_len[k]; m.weight = mem_weight[k];
+        if (m.len <= 0 || m.pos < 0 || m.pos + m.len > S.n) { s = AwStatus::err(AWSEM_E_FORMAT, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); break; }
+        m.se.assign(mem_seq + off2, mem_seq + off2 + m.len);
+        m.ca.resize((size_t)m.len * 3);
+        m.cb.resize((size_t)m.len * 3);
+        for (int r = 0; r < m.len; ++r)
+          for (int l = 0; l < 3; ++l) {
+            m.ca[3 * r + l] = mem_xyz_nm[((off2 + r) * 2 + 0) * 3 + l] * 10;
+            m.cb[3 * r + l] = m.se[r] == 'G' ? 0.0 : mem_xyz_nm[((off2 + r) * 2 + 1) * 3 + l] * 10;
+          }
+        off2 += m.len;
+        S.mems.push_back(std::move(m));
+      }
+    }
+  }
+  if (s.ok()) s = aw_finish_system(S);
+  *out = c.release();
+  if (!s.ok()) {
+    (*out)->err = s.msg;
+    (*out)->sys.n = -1;
+    return s.code;
+  }
+  return AWSEM_OK;
+}
+
+int awsem_destroy(awsem_ctx *c)
+{
+  delete c;
+  return AWSEM_OK;
+}
+
+int awsem_last_error(const awsem_ctx *c, char *buf, size_t n)
+{
+  if (!c || !buf || n == 0) return AWSEM_E_ARG;
+  std::snprintf(buf, n, "%s", c->err.c_str());
+  return AWSEM_OK;
+}
+
+int awsem_n_residues(const awsem_ctx *c, int *n)
+{
+  if (!c || !n) return AWSEM_E_ARG;
+  *n = c->sys.n;
+  return AWSEM_OK;
+}
+
+double awsem_pair_list_cutoff(const awsem_ctx *c) { return c ? c->sys.P.pair_list_cutoff : 0.0; }
+
+int awsem_set_vexcluded(awsem_ctx *c, int enable, int p, const double *lambda, const double *cs, const double *cl)
+{
+  if (!c || c->sys.n < 0) return AWSEM_E_ARG;
+  FFParams &P = c->sys.P;
+  P.ev_on = enable ? 1 : 0;
+  if (enable && lambda && cs && cl) {
+    P.ev_p = p;
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) {
+        P.ev_lambda[a][b] = lambda[a * 6 + b];
+        P.ev_cut_short[a][b] = cs[a * 6 + b];
+        P.ev_cut_long[a][b] = cl[a * 6 + b];
+      }
+  }
+  AwStatus s = aw_finish_system(c->sys);
+  c->dirty = true;
+  return s.ok() ? AWSEM_OK : fail(c, s);
+}
+
+int awsem_set_bonds(awsem_ctx *c, int enable, double k, const double *r0)
+{
+  if (!c || c->sys.n < 0) return AWSEM_E_ARG;
+  FFParams &P = c->sys.P;
+  P.bond_on = enable ? 1 : 0;
+  if (enable) {
+    P.bond_k = k;
+    if (r0) for (int q = 0; q < 5; ++q) P.bond_r0[q] = r0[q];
+  }
+  c->dirty = true;
+  return AWSEM_OK;
+}
+
+int awsem_set_term_mask(awsem_ctx *c, uint32_t mask)
+{
+  if (!c || c->sys.n < 0) return AWSEM_E_ARG;
+  c->sys.P.term_mask = mask;
+  c->dirty = true;
+  return AWSEM_OK;
+}
+
+// Construct_Computational_Arrays, fb.cpp:1590-1698, single rank (every atom
+// local): residues are the distinct residue tags in increasing order; the
+// library requires them to be 1..n, each with one atom in each group.
+int awsem_set_topology(awsem_ctx *c, int n_atoms, const int *group, const int *mol, const int *res)
+{
+  if (!c || c->sys.n < 0 || n_atoms < 0 || (n_atoms > 0 && (!group || !mol || !res))) return AWSEM_E_ARG;
+  const AwSystem &S = c->sys;
+  int n = S.n;
+  int cnt[3] = {0, 0, 0};
+  for (int a = 0; a < n_atoms; ++a) {
+    int g = group[a];
+    if (g < -1 || g > 2) return fail(c, AwStatus::err(AWSEM_E_ARG, "group code must be -1, 0, 1 or 2"));
+    if (g >= 0) {
+      cnt[g]++;
+      if (res[a] <= 0) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Molecular tag must be positive in fix backbone"));
+    }
+  }
+  if (cnt[0] != cnt[1] || cnt[1] != cnt[2])
+    return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "All groups must contain the same # of atoms in fix backbone"));
+  if (cnt[0] != n) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Number of CA atoms differs from the sequence length"));
+  std::vector<int> map(3 * (size_t)n, -1);
+  for (int a = 0; a < n_atoms; ++a) {
+    int g = group[a];
+    if (g < 0) continue;
+    int r = res[a];
+    if (r > n) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Residue tag is out of range"));
+    int i = r - 1;
+    if (map[(size_t)g * n + i] != -1) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Two atoms of one group share a residue tag"));
+    map[(size_t)g * n + i] = a;
+  }
+  for (int i = 0; i < n; ++i) {
+    int ca = map[i], cb = map[(size_t)n + i], o = map[2 * (size_t)n + i];
+    if (ca == -1) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Missing neighbor atoms in fix backbone (Code 004)"));
+    if (cb == -1 || o == -1) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Missing neighbor atoms in fix backbone (Code 001)"));
+    int chain = mol[ca];
+    if (chain <= 0 || chain > S.nch) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Chain tag is out of range"));
+    if (mol[cb] != chain || mol[o] != chain)
+      return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Atoms in a residue have different chain tag"));
+    int c0 = S.ch_start[chain - 1];
+    if (i < c0 || i > c0 + S.ch_len[chain - 1] - 1) return fail(c, AwStatus::err(AWSEM_E_TOPOLOGY, "Residue tag is out of range"));
+  }
+  c->atom_of = map;
+  c->n_atoms = n_atoms;
+  c->topo = true;
+  return AWSEM_OK;
+}
+
+static AwStatus ensure_dev(awsem_ctx *c)
+{
+  if (c->dev && !c->dirty) return AwStatus();
+  c->dev.reset();
+  std::vector<const AwSystem *> v{&c->sys};
+  AwStatus s = make_dev_batch(v, c->precision, false, c->dev);
+  if (s.ok()) c->dirty = false;
+  return s;
+}
+
+static bool finite_all(const double *x, size_t n)
+{
+  for (size_t k = 0; k < n; ++k) if (!std::isfinite(x[k])) return false;
+  return true;
+}
+
+static AwStatus compute_res(awsem_ctx *c, const double *xres, double *fres, double *energy)
+{
+  AwStatus s = ensure_dev(c);
+  if (!s.ok()) return s;
+  DevBatch &D = *c->dev;
+  size_t bytes = (size_t)c->sys.n * 9 * sizeof(double);
+  HIPCHK(hipMemcpyAsync(D.d_x, xres, bytes, hipMemcpyHostToDevice, D.stream));
+  AwRun run;
+  std::memset(&run, 0, sizeof run);
+  run.mode = 0;
+  HIPCHK(hipEventRecord(D.ev0, D.stream));
+  if (aw_launch_phases(D.ptrs(), run, D.stream)) return AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed");
+  HIPCHK(hipEventRecord(D.ev1, D.stream));
+  std::vector<double> f(c->sys.n * 9);
+  HIPCHK(hipMemcpyAsync(f.data(), D.d_f, bytes, hipMemcpyDeviceToHost, D.stream));
+  double e[AW_N_ET];
+  HIPCHK(hipMemcpyAsync(e, D.d_e, sizeof e, hipMemcpyDeviceToHost, D.stream));
+  HIPCHK(hipStreamSynchronize(D.stream));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, D.ev0, D.ev1);
+  D.last_ms = ms;
+  for (size_t k = 0; k < f.size(); ++k) fres[k] += f[k];
+  if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
+  if (!std::isfinite(e[0])) return AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy");
+  return AwStatus();
+}
+
+int awsem_compute_res(awsem_ctx *c, const double *xres, double *fres, double *energy)
+{
+  if (!c || c->sys.n < 0 || !xres || !fres) return AWSEM_E_ARG;
+  if (!finite_all(xres, (size_t)c->sys.n * 9)) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  AwStatus s = compute_res(c, xres, fres, energy);
+  return s.ok() ? AWSEM_OK : fail(c, s);
+}
+
+int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy)
+{
+  if (!c || c->sys.n < 0 || !x || !f) return AWSEM_E_ARG;
+  if (!c->topo) return fail(c, AwStatus::err(AWSEM_E_STATE, "awsem_set_topology must be called first"));
+  int n = c->sys.n;
+  std::vector<double> xr((size_t)n * 9), fr((size_t)n * 9, 0.0);
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < n; ++i) {
+      int a = c->atom_of[(size_t)k * n + i];
+      for (int l = 0; l < 3; ++l) xr[((size_t)k * n + i) * 3 + l] = x[(size_t)a * 3 + l];
+    }
+  if (!finite_all(xr.data(), xr.size())) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  AwStatus s = compute_res(c, xr.data(), fr.data(), energy);
+  if (!s.ok()) return fail(c, s);
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < n; ++i) {
+      int a = c->atom_of[(size_t)k * n + i];
+      for (int l = 0; l < 3; ++l) f[(size_t)a * 3 + l] += fr[((size_t)k * n + i) * 3 + l];
+    }
+  return AWSEM_OK;
+}
+
+int awsem_list(awsem_ctx *c, int kind, int arg, const double *xres, int64_t *out, int cap, int *count)
+{
+  if (!c || c->sys.n < 0 || !xres || !count || kind < 0 || kind > 6 || cap < 0) return AWSEM_E_ARG;
+  AwStatus s = ensure_dev(c);
+  if (!s.ok()) return fail(c, s);
+  DevBatch &D = *c->dev;
+  if (D.list_cap < std::max(cap, 1)) {
+    if (D.d_list) hipFree(D.d_list);
+    D.list_cap = std::max(cap, 1);
+    if (hipMalloc((void **)&D.d_list, (size_t)D.list_cap * sizeof(long long)) != hipSuccess) return fail(c, AwStatus::err(AWSEM_E_NOMEM, "list buffer"));
+  }
+  size_t bytes = (size_t)c->sys.n * 9 * sizeof(double);
+  if (hipMemcpyAsync(D.d_x, xres, bytes, hipMemcpyHostToDevice, D.stream) != hipSuccess) return fail(c, AwStatus::err(AWSEM_E_DEVICE, "copy"));
+  if (aw_launch_list(D.ptrs(), 0, kind, arg, D.d_list, D.list_cap, D.d_count, D.stream)) return fail(c, AwStatus::err(AWSEM_E_DEVICE, "list launch"));
+  int cnt = 0;
+  if (hipMemcpyAsync(&cnt, D.d_count, sizeof(int), hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
+      hipStreamSynchronize(D.stream) != hipSuccess)
+    return fail(c, AwStatus::err(AWSEM_E_DEVICE, "list copy"));
+  *count = cnt;
+  if (cnt > cap) return fail(c, AwStatus::err(AWSEM_E_NOMEM, "list capacity too small"));
+  std::vector<long long> tmp(cnt);
+  if (cnt && hipMemcpy(tmp.data(), D.d_list, (size_t)cnt * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(c, AwStatus::err(AWSEM_E_DEVICE, "list copy"));
+  std::sort(tmp.begin(), tmp.end());
+  for (int k = 0; k < cnt; ++k) out[k] = tmp[k];
+  return AWSEM_OK;
+}
+
+int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres, double *vres, double *fres, double *energy)
+{
+  if (!c || c->sys.n < 0 || !lg || !xres || !vres || nsteps < 0) return AWSEM_E_ARG;
+  AwStatus s = ensure_dev(c);
+  if (!s.ok()) return fail(c, s);
+  DevBatch &D = *c->dev;
+  size_t bytes = (size_t)c->sys.n * 9 * sizeof(double);
+  if (hipMemcpyAsync(D.d_x, xres, bytes, hipMemcpyHostToDevice, D.stream) != hipSuccess ||
+      hipMemcpyAsync(D.d_v, vres, bytes, hipMemcpyHostToDevice, D.stream) != hipSuccess)
+    return fail(c, AwStatus::err(AWSEM_E_DEVICE, "copy"));
+  AwRun run;
+  std::memset(&run, 0, sizeof run);
+  run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.t_start = lg->t_start; run.t_stop = lg->t_stop;
+  run.damp = lg->damp; run.seed = lg->seed; run.step0 = lg->step0;
+  hipEventRecord(D.ev0, D.stream);
+  if (aw_launch_phases(D.ptrs(), run, D.stream)) return fail(c, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
+  hipEventRecord(D.ev1, D.stream);
+  double e[AW_N_ET];
+  if (hipMemcpyAsync(xres, D.d_x, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
+      hipMemcpyAsync(vres, D.d_v, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
+      (fres && hipMemcpyAsync(fres, D.d_f, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess) ||
+      hipMemcpyAsync(e, D.d_e, sizeof e, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
+      hipStreamSynchronize(D.stream) != hipSuccess)
+    return fail(c, AwStatus::err(AWSEM_E_DEVICE, "copy back"));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, D.ev0, D.ev1);
+  D.last_ms = ms;
+  if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
+  if (!std::isfinite(e[0])) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy"));
+  return AWSEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+
+static int bfail(awsem_batch *b, const AwStatus &s)
+{
+  if (b) b->err = s.msg;
+  return s.code;
+}
+
+int awsem_batch_create(int n_ctx, awsem_ctx *const *ctxs, int precision, awsem_batch **out)
+{
+  if (!out || n_ctx <= 0 || !ctxs) return AWSEM_E_ARG;
+  *out = nullptr;
+  std::vector<const AwSystem *> v;
+  for (int k = 0; k < n_ctx; ++k) {
+    if (!ctxs[k] || ctxs[k]->sys.n < 0) return AWSEM_E_ARG;
+    v.push_back(&ctxs[k]->sys);
+  }
+  std::unique_ptr<awsem_batch> b(new awsem_batch());
+  AwStatus s = make_dev_batch(v, precision, true, b->dev);
+  if (!s.ok()) { *out = b.release(); (*out)->err = s.msg; return s.code; }
+  *out = b.release();
+  return AWSEM_OK;
+}
+
+int awsem_batch_size(const awsem_batch *b, int *nrep, int *nres)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  if (nrep) *nrep = b->dev->nrep;
+  if (nres) *nres = b->dev->total_res;
+  return AWSEM_OK;
+}
+
+int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  DevBatch &D = *b->dev;
+  size_t bytes = (size_t)D.total_res * 9 * sizeof(double);
+  if (x && hipMemcpy(D.d_x, x, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy x"));
+  if (v && hipMemcpy(D.d_v, v, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy v"));
+  return AWSEM_OK;
+}
+
+int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, double *energy)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  DevBatch &D = *b->dev;
+  if (hipStreamSynchronize(D.stream) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "sync"));
+  size_t bytes = (size_t)D.total_res * 9 * sizeof(double);
+  if ((x && hipMemcpy(x, D.d_x, bytes, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (v && hipMemcpy(v, D.d_v, bytes, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (f && hipMemcpy(f, D.d_f, bytes, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (energy && hipMemcpy(energy, D.d_e, (size_t)D.nrep * AW_N_ET * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy back"));
+  return AWSEM_OK;
+}
+
+static int batch_launch(awsem_batch *b, const AwRun &run)
+{
+  DevBatch &D = *b->dev;
+  hipEventRecord(D.ev0, D.stream);
+  if (aw_launch_resident(D.ptrs(), run, D.stream)) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
+  hipEventRecord(D.ev1, D.stream);
+  hipError_t e = hipEventSynchronize(D.ev1);
+  if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, D.ev0, D.ev1);
+  b->ms_total = ms;
+  b->n_kernels = 1;
+  return AWSEM_OK;
+}
+
+int awsem_batch_compute(awsem_batch *b)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  AwRun run;
+  std::memset(&run, 0, sizeof run);
+  run.mode = 0;
+  return batch_launch(b, run);
+}
+
+int awsem_batch_run(awsem_batch *b, int nsteps, const awsem_langevin *lg)
+{
+  if (!b || !b->dev || !lg || nsteps < 0) return AWSEM_E_ARG;
+  AwRun run;
+  std::memset(&run, 0, sizeof run);
+  run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.t_start = lg->t_start; run.t_stop = lg->t_stop;
+  run.damp = lg->damp; run.seed = lg->seed; run.step0 = lg->step0;
+  return batch_launch(b, run);
+}
+
+int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel_avg, int *n_kernels)
+{
+  if (!b) return AWSEM_E_ARG;
+  if (ms_total) *ms_total = b->ms_total;
+  if (ms_kernel_avg) *ms_kernel_avg = b->n_kernels ? b->ms_total / b->n_kernels : 0.0;
+  if (n_kernels) *n_kernels = b->n_kernels;
+  return AWSEM_OK;
+}
+
+int awsem_batch_last_error(const awsem_batch *b, char *buf, size_t n)
+{
+  if (!b || !buf || n == 0) return AWSEM_E_ARG;
+  std::snprintf(buf, n, "%s", b->err.c_str());
+  return AWSEM_OK;
+}
+
+int awsem_batch_destroy(awsem_batch *b)
+{
+  delete b;
+  return AWSEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// awsem_host.h -- host-side system description: the force-field parameters
+// and one system's sequence, chains, ssweight and fragment memories, as the
+// FixBackbone constructor builds them (src/fix_backbone.cpp:86-1072).
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+#include "awsem_types.h"
+
+struct AwStatus {
+  int code = 0;
+  std::string msg;
+  bool ok() const { return code == 0; }
+  static AwStatus err(int c, const std::string &m) { AwStatus s; s.code = c; s.msg = m; return s; }
+};
+
+// Gamma_Array (src/fragment_memory.cpp:275-519)
+struct AwGamma {
+  int nseq_cl = 0, nres_cl = 0;
+  bool frag_resty = false;
+  int i_sep[10] = {0};
+  std::vector<double> gamma;
+  int min_sep() const { return i_sep[0]; }
+  int max_sep() const { return i_sep[nseq_cl]; }
+  double get(int it, int jt, int ires, int jres) const;
+};
+
+struct AwMemory {
+  int pos = 0, len = 0;
+  double weight = 1.0;
+  std::string se;               // one-letter names from the .gro CA records
+  std::vector<double> ca, cb;   // len*3, Angstrom
+};
+
+struct AwSystem {
+  FFParams P;
+  int n = 0, nch = 0;
+  std::string se;
+  std::vector<int> ch_start, ch_len;   // 0-based start
+  std::vector<double> aps;             // n*12
+  std::vector<int> rinfo;
+  AwGamma fmg;
+  std::vector<AwMemory> mems;
+  char frag_mems_file[256] = {0}, fm_gamma_file[256] = {0};
+};
+
+int aw_restype(char c);
+AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file);
+AwStatus aw_read_param_tables(const std::string &dir, AwSystem &S);
+AwStatus aw_read_sequence(const std::string &path, AwSystem &S);
+AwStatus aw_read_ssweight(const std::string &dir, AwSystem &S);
+AwStatus aw_read_gamma_array(const std::string &path, AwGamma &g);
+AwStatus aw_read_mems(const std::string &dir, const std::string &mem_file, AwSystem &S);
+AwStatus aw_finish_system(AwSystem &S);
+
+struct AwFMPack {
+  std::vector<FMGroup> groups;
+  std::vector<int> pairs;     // (di, dj) per pair
+  std::vector<double> coef;   // epsilon * k_frag_mem * weight per memory
+  std::vector<double> rf;     // 4 per (memory, pair)
+  std::vector<double> items;  // gamma, 1/(2 sigma^2), 1/sigma^2, 0 per item
+};
+AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B, int item_base);
+void aw_default_pair_styles(FFParams &P);

@@ -1,0 +1,482 @@
+// awsem_kernels.hip -- gfx950 kernels of the FixBackbone force path.
+//
+// One force evaluation is a fixed sequence of phases over one replica
+// (a system of one or more chains):
+//   A  virtual atoms N, H, C' (fb.cpp:7113-7138), per-residue extents,
+//      zero accumulators
+//   B  fragment memory (fb.cpp:4079-4177), chain/chi/rama (2062-2546),
+//      harmonic bonds, and one sweep over residue pairs for the density
+//      rho (DL1, 7729-7786), DSSP (2902-3277), P_AP (3279-3383) and the
+//      excluded-volume pair style (pev.cpp:50-138)
+//   C  sigma_h and burial (SL, 7795-7841)
+//   D  helix (i,i+4) pairs (7843-7868, 3706-3785) and water xi (DL2, 7879-7937)
+//   E  xi *= dsigma/drho (7939-7950)
+//   F  water / burial / helix pair forces (DL3, 7996-8057)
+// Two engines run the same phase bodies:
+//   * resident: one workgroup owns one replica, its coordinates, forces and
+//     per-residue scalars live in LDS (or in a global scratch slab for
+//     replicas too large for LDS), phases are separated by workgroup
+//     barriers and nsteps of FP64 Langevin dynamics run inside one launch;
+//   * phase: every phase is its own grid-wide launch over one replica's
+//     items with state in global memory (the plain single-chain drop-in).
+// Pair and switching arithmetic runs in T (float or double); coordinates,
+// force accumulation, energies, list predicates and the integrator are FP64.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "awsem_terms.h"
+
+// ------------------------------------------------------------------------
+// block energy reduction
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// sums EAcc over the block; thread 0..N_EACC-1 then adds into out[slot]
+__device__ __forceinline__ void block_energy(const EAcc &E, double *red, double *out, bool atomic)
+{
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  for (int k = 0; k < N_EACC; ++k) {
+    double s = wave_sum(eacc_get(E, k));
+    if (lane == 0) red[wid * N_EACC + k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < N_EACC) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w * N_EACC + threadIdx.x];
+    if (atomic) { if (s != 0.0) aadd(&out[eacc_slot[threadIdx.x]], s); }
+    else out[eacc_slot[threadIdx.x]] = s;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------
+// batch description and views
+
+struct BatchDev {
+  const ReplicaDesc *rep;
+  int nrep;
+  const FFParams *P;
+  const int *rinfo;
+  const double *aps;
+  const int *ch_start, *ch_len;
+  double *x, *v, *f;          // state [res_off*9 ...]
+  double *energy;             // [nrep][AW_N_ET]
+  double *scratch;            // global-mode slabs
+  const FMGroup *groups;
+  const int2 *pairs;
+  const void *coef;
+  const void *rf;
+  const void *items;
+  int *queue;                 // work-queue counter (resident engine)
+  const int *order;           // replica order (longest first)
+};
+
+// doubles of per-replica working storage (x, xv, f, 10 scalars, ext+pad)
+__host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + (n + 1) / 2 + 8; }
+
+template <class T>
+__device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDesc &R, double *slab)
+{
+  View<T> V;
+  int n = R.n;
+  V.n = n; V.nch = R.nch;
+  V.rinfo = B.rinfo + R.res_off;
+  V.aps = B.aps + (long long)R.res_off * 12;
+  V.ch_start = B.ch_start + R.ch_off;
+  V.ch_len = B.ch_len + R.ch_off;
+  V.x = slab;
+  V.xv = slab + 9LL * n;
+  V.f = slab + 18LL * n;
+  double *s = slab + 27LL * n;
+  V.rho_w = s; V.rho_h = s + n; V.sig_w = s + 2 * n; V.sigp_w = s + 3 * n; V.sig_h = s + 4 * n;
+  V.sigp_h = s + 5 * n; V.xi_w = s + 6 * n; V.bf = s + 7 * n; V.hxi1 = s + 8 * n; V.hxi2 = s + 9 * n;
+  V.ext = (float *)(s + 10 * n);
+  V.P = B.P;
+  V.groups = B.groups + R.fm_group_off;
+  V.ngroups = R.fm_ngroups;
+  V.fm_items = R.fm_items;
+  V.fm_item_base = R.fm_item_base;
+  V.pairs = B.pairs;
+  V.coef = (const T *)B.coef;
+  V.rf = B.rf;
+  V.items = B.items;
+  return V;
+}
+
+// ------------------------------------------------------------------------
+// resident engine: one workgroup per replica, nsteps inside the launch
+
+template <class T>
+__device__ __forceinline__ void force_eval_block(const View<T> &V, EAcc &E)
+{
+  int t0 = threadIdx.x, ts = blockDim.x;
+  phase_A(V, t0, ts);
+  __syncthreads();
+  phase_B(V, t0, ts, E);
+  __syncthreads();
+  phase_C(V, t0, ts, E);
+  __syncthreads();
+  phase_D(V, t0, ts, E);
+  __syncthreads();
+  phase_E(V, t0, ts);
+  __syncthreads();
+  phase_F(V, t0, ts, E);
+  __syncthreads();
+}
+
+template <class T, bool LDSM>
+__global__ void __launch_bounds__(1024) k_resident(BatchDev B, LgParams lg, int mode)
+{
+  extern __shared__ double smem[];
+  __shared__ int s_rep;
+  double *red = smem;                 // 16 waves x N_EACC
+  for (;;) {
+    if (threadIdx.x == 0) s_rep = atomicAdd(B.queue, 1);
+    __syncthreads();
+    int slot = s_rep;
+    __syncthreads();
+    if (slot >= B.nrep) break;
+    int r = B.order[slot];
+    const ReplicaDesc R = B.rep[r];
+    double *slab = LDSM ? (smem + 16 * N_EACC) : (B.scratch + R.scratch_off);
+    View<T> V = make_view<T>(B, R, slab);
+    int n = R.n;
+    double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
+    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x[k] = gx[k];
+    __syncthreads();
+    EAcc E;
+    E.clear();
+    force_eval_block(V, E);
+    if (mode == 1) {
+      for (int s = 0; s < lg.nsteps; ++s) {
+        integrate1(V, gv, lg, threadIdx.x, blockDim.x);
+        __syncthreads();
+        E.clear();
+        force_eval_block(V, E);
+        integrate2(V, gv, lg, s, r, threadIdx.x, blockDim.x);
+        __syncthreads();
+      }
+      for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gx[k] = V.x[k];
+    }
+    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gf[k] = V.f[k];
+    block_energy(E, red, B.energy + (long long)r * AW_N_ET, false);
+  }
+}
+
+// ------------------------------------------------------------------------
+// phase engine: each phase is a grid-wide launch over one replica
+
+template <class T>
+__device__ __forceinline__ View<T> phase_view(const BatchDev &B, int r)
+{
+  const ReplicaDesc R = B.rep[r];
+  View<T> V = make_view<T>(B, R, B.scratch + R.scratch_off);
+  return V;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ph_load(BatchDev B, int r)
+{
+  const ReplicaDesc R = B.rep[r];
+  View<T> V = phase_view<T>(B, r);
+  const double *gx = B.x + 9LL * R.res_off;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) V.x[k] = gx[k];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ph_A(BatchDev B, int r)
+{
+  View<T> V = phase_view<T>(B, r);
+  phase_A(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  if (blockIdx.x == 0 && threadIdx.x < AW_N_ET) B.energy[(long long)r * AW_N_ET + threadIdx.x] = 0.0;
+}
+
+#define PH_KERNEL(NAME, CALL)                                                          \
+  template <class T>                                                                   \
+  __global__ void __launch_bounds__(256) NAME(BatchDev B, int r)                       \
+  {                                                                                    \
+    __shared__ double red[4 * N_EACC];                                                 \
+    View<T> V = phase_view<T>(B, r);                                                   \
+    EAcc E;                                                                            \
+    E.clear();                                                                         \
+    CALL;                                                                              \
+    block_energy(E, red, B.energy + (long long)r * AW_N_ET, true);                     \
+  }
+
+PH_KERNEL(k_ph_B, phase_B(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
+PH_KERNEL(k_ph_C, phase_C(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
+PH_KERNEL(k_ph_D, phase_D(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
+PH_KERNEL(k_ph_F, phase_F(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ph_E(BatchDev B, int r)
+{
+  View<T> V = phase_view<T>(B, r);
+  phase_E(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ph_store(BatchDev B, int r, int with_x)
+{
+  const ReplicaDesc R = B.rep[r];
+  View<T> V = phase_view<T>(B, r);
+  double *gf = B.f + 9LL * R.res_off, *gx = B.x + 9LL * R.res_off;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) {
+    gf[k] = V.f[k];
+    if (with_x) gx[k] = V.x[k];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ph_int1(BatchDev B, int r, LgParams lg)
+{
+  const ReplicaDesc R = B.rep[r];
+  View<T> V = phase_view<T>(B, r);
+  integrate1(V, B.v + 9LL * R.res_off, lg, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ph_int2(BatchDev B, int r, LgParams lg, int s)
+{
+  const ReplicaDesc R = B.rep[r];
+  View<T> V = phase_view<T>(B, r);
+  integrate2(V, B.v + 9LL * R.res_off, lg, s, r, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// total energy slot
+__global__ void k_energy_total(double *energy, int nrep)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  double *e = energy + (long long)r * AW_N_ET;
+  double s = 0.0;
+  for (int k = 1; k < AW_N_ET; ++k) s += e[k];
+  e[0] = s;
+}
+
+// ------------------------------------------------------------------------
+// integer lists for parity tests (same predicates as the force passes)
+
+template <class T>
+__global__ void __launch_bounds__(256) k_list(BatchDev B, int r, int kind, int arg, long long *out, int cap, int *count)
+{
+  View<T> V = phase_view<T>(B, r);
+  const FFParams *P = V.P;
+  int n = V.n;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+  if (kind == 4) {
+    for (int i = gt; i < n; i += gs) {
+      int j = i + P->helix_i_diff;
+      if (!P->helix_flag || j >= n) continue;
+      if (rchain(V.rinfo[i]) != rchain(V.rinfo[j])) continue;
+      double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
+      if (r2 < P->helix_cutoff_sq && !(sqrt(r2) > P->helix_cutoff)) {
+        int k = atomicAdd(count, 1);
+        if (k < cap) out[k] = (long long)i * n + j;
+      }
+    }
+    return;
+  }
+  int np = pair_count(n);
+  for (int k = gt; k < np; k += gs) {
+    int i, j;
+    pair_decode(k, n, i, j);
+    int ri = V.rinfo[i], rj = V.rinfo[j];
+    bool same = rchain(ri) == rchain(rj);
+    int dres = j - i;
+    long long code = -1, code2 = -1;
+    if (kind == 0 || kind == 1 || kind == 6) {
+      double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
+      if (kind == 0 && P->water_flag && (!same || dres > 1) && arg < P->n_wells && rsq > P->w_rmin_th_sq[arg] && rsq < P->w_rmax_th_sq[arg])
+        code = (long long)i * n + j;
+      if (kind == 1 && P->helix_flag && (!same || dres > 1) && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq)
+        code = (long long)i * n + j;
+      if (kind == 6 && P->water_flag && (!same || dres >= P->contact_cutoff) && arg < P->n_wells && P->well_flag[arg]) {
+        int it = rtype(ri), jt = rtype(rj);
+        double g0 = P->water_gamma[arg][it][jt][0], g1 = P->water_gamma[arg][it][jt][1];
+        if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > P->w_rmin_th_sq[arg] && rsq < P->w_rmax_th_sq[arg])
+          code = (long long)i * n + j;
+      }
+    } else if (kind == 2) {
+      if (P->dssp_flag && (!same || dres > 2)) {
+        if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST)) {
+          double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
+          if (r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut)) code = (long long)i * n + j;
+        }
+        if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST)) {
+          double r2 = rsq_exact(XA(V, K_O, j), XV(V, K_N, i));
+          if (r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut)) code2 = (long long)j * n + i;
+        }
+      }
+    } else if (kind == 3) {
+      if (P->pap_flag && rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j)) < P->pap_cutoff_sq) code = (long long)i * n + j;
+    } else if (kind == 5) {
+      bool nb = same && (j == i + 1), shortc = same && (j - i < 5);
+      int na = 3 * n;
+      for (int ka = 0; ka < 3; ++ka)
+        for (int kb = 0; kb < 3; ++kb) {
+          if (nb && kb == K_CA && (ka == K_CA || ka == K_O)) continue;
+          int ta = atype(ka, ri), tb = atype(kb, rj);
+          if (P->ev_lambda[ta][tb] == 0.0) continue;
+          double rc = shortc ? P->ev_cut_short[ta][tb] : P->ev_cut_long[ta][tb];
+          double rr = sqrt(rsq_exact(XA(V, ka, i), XA(V, kb, j)));
+          if (rr < rc) {
+            long long a = 3LL * i + ka, b = 3LL * j + kb;
+            long long cc = a < b ? a * na + b : b * na + a;
+            int q = atomicAdd(count, 1);
+            if (q < cap) out[q] = cc;
+          }
+        }
+    }
+    if (code >= 0) { int q = atomicAdd(count, 1); if (q < cap) out[q] = code; }
+    if (code2 >= 0) { int q = atomicAdd(count, 1); if (q < cap) out[q] = code2; }
+  }
+  if (kind == 5 && gt < n) {
+    // same-residue CB-O
+    int i = gt;
+    int ri = V.rinfo[i];
+    int ta = atype(K_CB, ri);
+    if (P->ev_lambda[ta][3] != 0.0) {
+      double rr = sqrt(rsq_exact(XA(V, K_CB, i), XA(V, K_O, i)));
+      if (rr < P->ev_cut_short[ta][3]) {
+        int na = 3 * n;
+        int q = atomicAdd(count, 1);
+        if (q < cap) out[q] = (3LL * i + 1) * na + 3LL * i + 2;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// host-side launchers (internal C++ interface used by awsem_api.cpp)
+
+#include "awsem_launch.h"
+
+static int g_lds_max = 0;
+
+int aw_lds_max_residues()
+{
+  if (!g_lds_max) {
+    int dev = 0, lim = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
+    long long avail = lim / 8 - 16 * N_EACC - 16;
+    int n = 1;
+    while (slab_doubles(n + 1) <= avail) ++n;
+    g_lds_max = n;
+  }
+  return g_lds_max;
+}
+
+long long aw_slab_doubles(int n) { return slab_doubles(n); }
+
+static BatchDev to_dev(const AwBatchPtrs &p)
+{
+  BatchDev B;
+  B.rep = (const ReplicaDesc *)p.rep;
+  B.nrep = p.nrep;
+  B.P = (const FFParams *)p.params;
+  B.rinfo = p.rinfo;
+  B.aps = p.aps;
+  B.ch_start = p.ch_start;
+  B.ch_len = p.ch_len;
+  B.x = p.x; B.v = p.v; B.f = p.f;
+  B.energy = p.energy;
+  B.scratch = p.scratch;
+  B.groups = (const FMGroup *)p.groups;
+  B.pairs = (const int2 *)p.pairs;
+  B.coef = p.coef;
+  B.rf = p.rf;
+  B.items = p.items;
+  B.queue = p.queue;
+  B.order = p.order;
+  return B;
+}
+
+template <class T>
+static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
+{
+  BatchDev B = to_dev(p);
+  LgParams lg;
+  lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
+  lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
+  int mode = run.mode;
+  hipMemsetAsync(p.queue, 0, sizeof(int), st);
+  int threads = run.threads > 0 ? run.threads : 1024;
+  int grid = run.grid > 0 ? run.grid : p.nrep;
+  if (p.lds_resident) {
+    size_t lds = (size_t)(16 * N_EACC + slab_doubles(p.max_n)) * sizeof(double);
+    hipFuncSetAttribute((const void *)k_resident<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_resident<T, true>), dim3(grid), dim3(threads), lds, st, B, lg, mode);
+  } else {
+    size_t lds = (size_t)(16 * N_EACC) * sizeof(double);
+    hipLaunchKernelGGL((k_resident<T, false>), dim3(grid), dim3(threads), lds, st, B, lg, mode);
+  }
+  hipLaunchKernelGGL(k_energy_total, dim3((p.nrep + 63) / 64), dim3(64), 0, st, p.energy, p.nrep);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
+{
+  return p.fp32 ? launch_resident_t<float>(p, run, st) : launch_resident_t<double>(p, run, st);
+}
+
+template <class T>
+static void force_phases(const BatchDev &B, int r, int grid, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_ph_A<T>, dim3(grid), dim3(256), 0, st, B, r);
+  hipLaunchKernelGGL(k_ph_B<T>, dim3(grid), dim3(256), 0, st, B, r);
+  hipLaunchKernelGGL(k_ph_C<T>, dim3(grid), dim3(256), 0, st, B, r);
+  hipLaunchKernelGGL(k_ph_D<T>, dim3(grid), dim3(256), 0, st, B, r);
+  hipLaunchKernelGGL(k_ph_E<T>, dim3(grid), dim3(256), 0, st, B, r);
+  hipLaunchKernelGGL(k_ph_F<T>, dim3(grid), dim3(256), 0, st, B, r);
+}
+
+template <class T>
+static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
+{
+  BatchDev B = to_dev(p);
+  LgParams lg;
+  lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
+  lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
+  int grid = run.grid > 0 ? run.grid : 1024;
+  for (int r = 0; r < p.nrep; ++r) {
+    hipLaunchKernelGGL(k_ph_load<T>, dim3(grid), dim3(256), 0, st, B, r);
+    force_phases<T>(B, r, grid, st);
+    if (run.mode == 1) {
+      for (int s = 0; s < run.nsteps; ++s) {
+        hipLaunchKernelGGL(k_ph_int1<T>, dim3(grid), dim3(256), 0, st, B, r, lg);
+        force_phases<T>(B, r, grid, st);
+        hipLaunchKernelGGL(k_ph_int2<T>, dim3(grid), dim3(256), 0, st, B, r, lg, s);
+      }
+    }
+    hipLaunchKernelGGL(k_ph_store<T>, dim3(grid), dim3(256), 0, st, B, r, run.mode == 1 ? 1 : 0);
+  }
+  hipLaunchKernelGGL(k_energy_total, dim3((p.nrep + 63) / 64), dim3(64), 0, st, p.energy, p.nrep);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
+{
+  return p.fp32 ? launch_phases_t<float>(p, run, st) : launch_phases_t<double>(p, run, st);
+}
+
+int aw_launch_list(const AwBatchPtrs &p, int r, int kind, int arg, long long *out, int cap, int *count, hipStream_t st)
+{
+  BatchDev B = to_dev(p);
+  hipMemsetAsync(count, 0, sizeof(int), st);
+  // virtual atoms first (phase A of this replica)
+  if (p.fp32) {
+    hipLaunchKernelGGL(k_ph_load<float>, dim3(256), dim3(256), 0, st, B, r);
+    hipLaunchKernelGGL(k_ph_A<float>, dim3(256), dim3(256), 0, st, B, r);
+    hipLaunchKernelGGL(k_list<float>, dim3(512), dim3(256), 0, st, B, r, kind, arg, out, cap, count);
+  } else {
+    hipLaunchKernelGGL(k_ph_load<double>, dim3(256), dim3(256), 0, st, B, r);
+    hipLaunchKernelGGL(k_ph_A<double>, dim3(256), dim3(256), 0, st, B, r);
+    hipLaunchKernelGGL(k_list<double>, dim3(512), dim3(256), 0, st, B, r, kind, arg, out, cap, count);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -1,0 +1,660 @@
+// awsem_parse.cpp -- readers for the reference's input formats.
+//
+// Semantics follow the FixBackbone constructor (src/fix_backbone.cpp):
+// the coeff file is a stream of whitespace tokens where a token equal to a
+// section tag ("[Water]") starts that section and anything else is skipped
+// (a "-" suffix on a tag therefore disables a section, fb.cpp:165-497);
+// numeric fields use istream extraction.  Side files are read from the
+// working directory by their fixed names (fb.cpp:555-693); fragment
+// memories use the .mem / .gro formats (fb.cpp:1966-2042,
+// src/fragment_memory.cpp:34-176) and the gamma-class file format
+// (src/fragment_memory.cpp:275-378).
+#include "awsem_host.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <unordered_map>
+#include <map>
+#include <algorithm>
+
+static const int kSeMap[26] = {0, 0, 4, 3, 6, 13, 7, 8, 9, 0, 11, 10, 12, 2, 0, 14, 5, 1, 15, 16, 0, 19, 17, 0, 18, 0};
+static const int kFour[20] = {1, 3, 2, 2, 4, 2, 2, 1, 3, 4, 4, 3, 4, 4, 1, 1, 1, 4, 4, 4};
+
+int aw_restype(char c) { return (c >= 'A' && c <= 'Z') ? kSeMap[c - 'A'] : 0; }
+
+static std::string join(const std::string &dir, const std::string &name)
+{
+  if (name.empty() || name[0] == '/' || dir.empty()) return name;
+  return dir + "/" + name;
+}
+
+static const char *kUnsupported[] = {
+    "[AMH-Go]", "[Fragment_Memory_Table]", "[Contact_Restraints]", "[Vector_Fragment_Memory]",
+    "[Solvent_Barrier]", "[Membrane]", "[Fragment_Frustratometer]", "[Tertiary_Frustratometer]",
+    "[Nmer_Frustratometer]", "[Phosphorylation]", "[Amylometer]", "[Selection_Temperature]",
+    "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]", "[DebyeHuckel]",
+    "[DebyeHuckel_Optimization]", nullptr};
+
+AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file)
+{
+  std::ifstream in(path);
+  if (!in) return AwStatus::err(-2, "Coefficient file was not found!");
+  std::memset(&P, 0, sizeof(P));
+  P.epsilon = 1.0;
+  P.an = 0.4831806; P.bn = 0.7032820; P.cn = -0.1864262;   // fb.cpp:141-143
+  P.ap = 0.4436538; P.bp = 0.2352006; P.cp = 0.3211455;
+  P.ah = 0.8409657; P.bh = 0.8929599; P.ch = -0.7338894;
+  P.fm_sigma_exp = 0.15;
+  P.term_mask = 0xffffffffu;
+  double sigma[12] = {0}, sigma_HO = 0, sigma_NO = 0, hsig_HO = 0, hsig_NO = 0;
+  std::string sec;
+  while (in >> sec) {
+    if (sec == "[ABC]") {
+      in >> P.an >> P.bn >> P.cn >> P.ap >> P.bp >> P.cp >> P.ah >> P.bh >> P.ch;
+    } else if (sec == "[Chain]") {
+      P.chain_flag = 1;
+      in >> P.k_chain[0] >> P.k_chain[1] >> P.k_chain[2] >> P.r_ncb0 >> P.r_cpcb0 >> P.r_ncp0;
+    } else if (sec == "[Shake]") {
+      P.shake_flag = 1;
+      in >> P.k_shake >> P.r_sh1 >> P.r_sh2 >> P.r_sh3;
+    } else if (sec == "[Chi]") {
+      P.chi_flag = 1;
+      in >> P.k_chi >> P.chi0;
+    } else if (sec == "[Excluded]" || sec == "[Excluded_R6]") {
+      double d;
+      for (int k = 0; k < 4; ++k) in >> d;  // only used by the DEBUGFORCES build
+    } else if (sec == "[Excluded_P]") {
+      int p; double d;
+      in >> p;
+      for (int k = 0; k < 4; ++k) in >> d;
+    } else if (sec == "[Rama]") {
+      P.rama_flag = 1;
+      in >> P.k_rama >> P.n_rama_par;
+      if (P.n_rama_par < 0 || P.n_rama_par > 6) return AwStatus::err(-3, "Rama: too many wells");
+      for (int j = 0; j < P.n_rama_par; ++j) {
+        in >> P.w[j] >> sigma[j] >> P.phiw[j] >> P.phi0[j] >> P.psiw[j] >> P.psi0[j];
+        P.phiw[j] *= sigma[j];
+        P.psiw[j] *= sigma[j];
+      }
+    } else if (sec == "[Rama_P]") {
+      P.rama_p_flag = 1;
+      in >> P.n_rama_p_par;
+      if (P.n_rama_p_par < 0 || P.n_rama_p_par > 6) return AwStatus::err(-3, "Rama_P: too many wells");
+      for (int j = 6; j < 6 + P.n_rama_p_par; ++j) {
+        in >> P.w[j] >> sigma[j] >> P.phiw[j] >> P.phi0[j] >> P.psiw[j] >> P.psi0[j];
+        P.phiw[j] *= sigma[j];
+        P.psiw[j] *= sigma[j];
+      }
+    } else if (sec == "[SSWeight]") {
+      P.ssweight_flag = 1;
+      for (int j = 0; j < 12; ++j) { bool b = false; in >> b; P.ssweight[j] = b ? 1 : 0; }
+    } else if (sec == "[Dssp_Hdrgn]") {
+      P.dssp_flag = 1;
+      in >> P.k_dssp >> P.hbscl[0][0] >> P.hbscl[0][1];
+      for (int j = 0; j < 7; ++j) in >> P.hbscl[1][j];
+      for (int j = 0; j < 9; ++j) in >> P.hbscl[2][j];
+      for (int j = 0; j < 9; ++j) in >> P.hbscl[3][j];
+      in >> sigma_HO >> sigma_NO >> P.HO_zero >> P.NO_zero >> P.dssp_cut >> P.pref[0] >> P.pref[1] >> P.d_nu0;
+      P.dssp_cut_sq = P.dssp_cut * P.dssp_cut;
+      P.nu_cut1_sq = std::pow(P.d_nu0 + std::atanh(2.0 * 1e-4 - 1.0) / P.pref[0], 2);
+      P.nu_cut2_sq = std::pow(P.d_nu0 + std::atanh(2.0 * 1e-4 - 1.0) / P.pref[1], 2);
+      P.sigma_HO_sqinv = 1.0 / (sigma_HO * sigma_HO);
+      P.sigma_NO_sqinv = 1.0 / (sigma_NO * sigma_NO);
+    } else if (sec == "[P_AP]") {
+      P.pap_flag = 1;
+      in >> P.k_global_pap >> P.k_betapred_pap >> P.k_pap[0] >> P.k_pap[1] >> P.k_pap[2] >> P.pap_cut >> P.pap_pref;
+      in >> P.i_med_min >> P.i_med_max >> P.i_diff_pap;
+      P.pap_cutoff_sq = std::pow(P.pap_cut + std::atanh(1.0 - 2.0 * 1e-12) / P.pap_pref, 2);
+    } else if (sec == "[Water]") {
+      P.water_flag = 1;
+      in >> P.k_water >> P.water_kappa >> P.water_kappa_sigma >> P.treshold >> P.contact_cutoff >> P.n_wells;
+      if (P.n_wells < 0 || P.n_wells > 5) return AwStatus::err(-3, "Water: too many wells");
+      for (int j = 0; j < P.n_wells; ++j) in >> P.well_r_min[j] >> P.well_r_max[j] >> P.well_flag[j];
+    } else if (sec == "[Burial]") {
+      P.burial_flag = 1;
+      in >> P.k_burial >> P.burial_kappa;
+      for (int j = 0; j < 3; ++j) in >> P.burial_ro_min[j] >> P.burial_ro_max[j];
+    } else if (sec == "[Helix]") {
+      P.helix_flag = 1;
+      in >> P.k_helix >> P.helix_gamma_p >> P.helix_gamma_w >> P.helix_kappa >> P.helix_kappa_sigma;
+      in >> P.helix_treshold >> P.helix_i_diff >> P.helix_cutoff;
+      P.helix_cutoff_sq = P.helix_cutoff * P.helix_cutoff;
+      in >> P.helix_r_min >> P.helix_r_max;
+      for (int j = 0; j < 20; ++j) in >> P.h4prob[j];
+      in >> P.pro_accepter_flag >> P.h4prob_pro_accepter;
+      in >> hsig_HO >> hsig_NO >> P.helix_HO_zero >> P.helix_NO_zero;
+      P.helix_sigma_HO_sqinv = 1.0 / (hsig_HO * hsig_HO);
+      P.helix_sigma_NO_sqinv = 1.0 / (hsig_NO * hsig_NO);
+    } else if (sec == "[Fragment_Memory]") {
+      P.fm_flag = 1;
+      in >> P.k_frag_mem >> mem_file >> gamma_file;
+    } else if (sec == "[Epsilon]") {
+      in >> P.epsilon;
+    } else if (sec == "[Shuffler]" || sec == "[Mutate_Sequence]") {
+      int fl = 0;
+      std::string name;
+      in >> fl >> name;
+      if (fl == 1) return AwStatus::err(-4, "Section " + sec + " is enabled but is not part of this build's force path");
+    } else {
+      for (int k = 0; kUnsupported[k]; ++k)
+        if (sec == kUnsupported[k])
+          return AwStatus::err(-4, "Section " + sec + " is enabled but is not part of this build's force path");
+    }
+    if (in.fail()) return AwStatus::err(-3, "Coefficient file format error in section " + sec);
+  }
+  // fb.cpp:502-516: scale strengths by epsilon
+  double e = P.epsilon;
+  for (int j = 0; j < 3; ++j) P.k_chain[j] *= e;
+  P.k_chi *= e;
+  P.k_rama *= e;
+  P.k_water *= e;
+  P.k_burial *= e;
+  P.k_helix *= e;
+  P.k_dssp *= e;
+  P.k_global_pap *= e;
+  for (int j = 0; j < P.n_rama_par; ++j) P.w[j] *= P.k_rama;
+  for (int j = 6; j < 6 + P.n_rama_p_par; ++j) P.w[j] *= P.k_rama;
+  // cWell ranges (sml.h:363-368), cP_AP ranges (sml.h:111-115)
+  for (int k = 0; k < P.n_wells; ++k) {
+    P.w_rmin_th[k] = P.well_r_min[k] - 8.0 * 2.302585 / P.water_kappa;
+    P.w_rmax_th[k] = P.well_r_max[k] + 8.0 * 2.302585 / P.water_kappa;
+    P.w_rmin_th_sq[k] = P.w_rmin_th[k] * P.w_rmin_th[k];
+    P.w_rmax_th_sq[k] = P.w_rmax_th[k] * P.w_rmax_th[k];
+  }
+  if (P.helix_flag) {
+    P.h_rmin_th = P.helix_r_min - 8.0 * 2.302585 / P.helix_kappa;
+    P.h_rmax_th = P.helix_r_max + 8.0 * 2.302585 / P.helix_kappa;
+    P.h_rmin_th_sq = P.h_rmin_th * P.h_rmin_th;
+    P.h_rmax_th_sq = P.h_rmax_th * P.h_rmax_th;
+  }
+  if (P.pap_flag) {
+    double dd = (double)((int)(8.0 * 2.302585 / P.pap_pref) + 1);
+    P.pap_drmax_sq = (P.pap_cut + dd) * (P.pap_cut + dd);
+    P.pap_drmin_sq = (P.pap_cut - dd) * (P.pap_cut - dd);
+  }
+  // pair_list_cutoff, fb.cpp:1048-1104
+  double plc = 8.0;
+  if (P.dssp_flag) plc = std::fmax(plc, P.dssp_cut);
+  if (P.pap_flag) plc = std::fmax(plc, std::sqrt(P.pap_cutoff_sq));
+  if (P.water_flag) for (int i = 0; i < P.n_wells; ++i) plc = std::fmax(plc, P.w_rmax_th[i]);
+  if (P.helix_flag) {
+    plc = std::fmax(plc, P.h_rmax_th);
+    double r0[3] = {3.77, 2.5, 2.41};
+    auto three = [&](double a, double b, double c) {
+      double r[3];
+      r[0] = std::sqrt(std::pow(a * r0[0], 2) + std::pow(c * r0[1], 2) + a * c * (r0[0] * r0[0] + r0[1] * r0[1] - r0[2] * r0[2]));
+      r[1] = std::sqrt(std::pow(a * r0[2], 2) + std::pow(b * r0[1], 2) + a * b * (r0[2] * r0[2] + r0[1] * r0[1] - r0[0] * r0[0]));
+      r[2] = std::sqrt(std::pow(b * r0[0], 2) + std::pow(c * r0[2], 2) + b * c * (r0[0] * r0[0] + r0[2] * r0[2] - r0[1] * r0[1]));
+      return std::fmax(r[0], std::fmax(r[1], r[2]));
+    };
+    double r1 = 1.1 * three(P.an, P.bn, P.cn) + P.helix_NO_zero + hsig_NO * std::sqrt(6.0 * std::log(10.0));
+    double r2 = 1.1 * three(P.ah, P.bh, P.ch) + P.helix_HO_zero + hsig_HO * std::sqrt(6.0 * std::log(10.0));
+    plc = std::fmax(plc, std::fmax(r1, r2));
+    plc = std::fmax(plc, P.helix_cutoff);
+  }
+  P.pair_list_cutoff = plc;
+  aw_default_pair_styles(P);
+  return AwStatus();
+}
+
+void aw_default_pair_styles(FFParams &P)
+{
+  // excluded volume: pair_style vexcluded 2 3.5 3.5; pair_coeff * * 0.0;
+  // 1 1 / 1 4 / 4 4: 20.0 3.5 4.5; 3 3: 20.0 3.5 3.5
+  // (tools/create_project_tools/CoordinatesToWorkLammpsDataFile.py:314-322)
+  P.ev_on = 1;
+  P.ev_p = 2;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) { P.ev_lambda[a][b] = 0.0; P.ev_cut_short[a][b] = 3.5; P.ev_cut_long[a][b] = 3.5; }
+  const int tp[4][2] = {{1, 1}, {1, 4}, {4, 4}, {3, 3}};
+  const double cl[4] = {4.5, 4.5, 4.5, 3.5};
+  for (int k = 0; k < 4; ++k) {
+    int a = tp[k][0], b = tp[k][1];
+    P.ev_lambda[a][b] = P.ev_lambda[b][a] = 20.0;
+    P.ev_cut_long[a][b] = P.ev_cut_long[b][a] = cl[k];
+  }
+  // bond_style harmonic, K = 200 (CoordinatesToWorkLammpsDataFile.py:289-294)
+  P.bond_on = 1;
+  P.bond_k = 200.0;
+  P.bond_r0[0] = 3.77; P.bond_r0[1] = 2.41; P.bond_r0[2] = 2.50; P.bond_r0[3] = 1.54; P.bond_r0[4] = 1.54;
+}
+
+static AwStatus read_numbers(const std::string &path, const char *name, double *dst, int count)
+{
+  std::ifstream in(path);
+  if (!in) return AwStatus::err(-2, std::string("File ") + name + " doesn't exist");
+  for (int k = 0; k < count; ++k) {
+    if (!(in >> dst[k])) return AwStatus::err(-3, std::string(name) + ": file format error");
+  }
+  return AwStatus();
+}
+
+AwStatus aw_read_param_tables(const std::string &dir, AwSystem &S)
+{
+  FFParams &P = S.P;
+  if (P.dssp_flag) {  // fb.cpp:555-593
+    std::vector<double> t(800);
+    const char *names[3] = {"anti_HB", "anti_NHB", "para_HB"};
+    double(*dst[3])[20][2] = {P.anti_HB, P.anti_NHB, P.para_HB};
+    for (int f = 0; f < 3; ++f) {
+      AwStatus s = read_numbers(join(dir, names[f]), names[f], t.data(), 800);
+      if (!s.ok()) return s;
+      for (int i = 0; i < 20; ++i)
+        for (int j = 0; j < 20; ++j) { dst[f][i][j][0] = t[i * 20 + j]; dst[f][i][j][1] = t[400 + i * 20 + j]; }
+    }
+    AwStatus s = read_numbers(join(dir, "para_one"), "para_one", P.para_one, 20);
+    if (!s.ok()) return s;
+    s = read_numbers(join(dir, "anti_one"), "anti_one", P.anti_one, 20);
+    if (!s.ok()) return s;
+  }
+  double k_water = P.k_water;
+  if (P.water_flag) {  // fb.cpp:623-641
+    std::ifstream in(join(dir, "gamma.dat"));
+    if (!in) return AwStatus::err(-2, "File gamma.dat doesn't exist");
+    for (int w = 0; w < P.n_wells; ++w)
+      for (int i = 0; i < 20; ++i)
+        for (int j = i; j < 20; ++j) {
+          double g0, g1;
+          if (!(in >> g0 >> g1)) return AwStatus::err(-3, "Water potential gamma.dat file format error");
+          P.water_gamma[w][i][j][0] = P.water_gamma[w][j][i][0] = g0 * k_water;
+          P.water_gamma[w][i][j][1] = P.water_gamma[w][j][i][1] = g1 * k_water;
+        }
+  }
+  if (P.burial_flag) {  // fb.cpp:684-693
+    if (!P.water_flag) return AwStatus::err(-3, "Cannot use burial without water potential");
+    double t[60];
+    AwStatus s = read_numbers(join(dir, "burial_gamma.dat"), "burial_gamma.dat", t, 60);
+    if (!s.ok()) return AwStatus::err(s.code, s.code == -2 ? s.msg : "Burial potential burial_gamma.dat file format error");
+    for (int i = 0; i < 20; ++i) for (int k = 0; k < 3; ++k) P.burial_gamma[i][k] = t[i * 3 + k];
+  }
+  return AwStatus();
+}
+
+AwStatus aw_read_sequence(const std::string &path, AwSystem &S)
+{
+  std::ifstream in(path);
+  if (!in) return AwStatus::err(-2, "Sequence file was not found");
+  std::string tok;
+  S.se.clear(); S.ch_start.clear(); S.ch_len.clear();
+  while (in >> tok) {  // fb.cpp:538-553: one whitespace token per chain
+    if (tok.empty() || tok[0] == '#') continue;
+    for (char c : tok)
+      if (c < 'A' || c > 'Z') return AwStatus::err(-3, "Sequence file: unknown residue letter");
+    if ((int)S.ch_len.size() >= AW_MAXCH) return AwStatus::err(-3, "Too many chains");
+    S.ch_start.push_back((int)S.se.size());
+    S.ch_len.push_back((int)tok.size());
+    S.se += tok;
+  }
+  S.nch = (int)S.ch_len.size();
+  S.n = (int)S.se.size();
+  if (S.n == 0) return AwStatus::err(-3, "Empty sequence");
+  return AwStatus();
+}
+
+AwStatus aw_read_ssweight(const std::string &dir, AwSystem &S)
+{
+  S.aps.assign((size_t)S.n * 12, 0.0);
+  if (!S.P.ssweight_flag) return AwStatus();
+  std::ifstream in(join(dir, "ssweight"));  // fb.cpp:595-604
+  if (!in) return AwStatus::err(-2, "File ssweight doesn't exist");
+  for (int j = 0; j < S.n; ++j)
+    for (int i = 0; i < 12; ++i)
+      if (S.P.ssweight[i]) { double v = 0.0; in >> v; S.aps[(size_t)j * 12 + i] = v; }
+  return AwStatus();
+}
+
+// ---------------------------------------------------------------------------
+// Gamma_Array
+
+double AwGamma::get(int it, int jt, int ires, int jres) const
+{
+  int dij = std::abs(ires - jres);
+  if (dij < i_sep[0] || (i_sep[nseq_cl] != -1 && dij > i_sep[nseq_cl])) return 0.0;
+  int cl = 1;
+  while (cl < nseq_cl && dij >= i_sep[cl]) cl++;
+  int r = nres_cl;
+  if (r == 1) return gamma[cl - 1];
+  int a = it, b = jt;
+  if (r == 4) { a = kFour[it] - 1; b = kFour[jt] - 1; }
+  return gamma[(size_t)(cl - 1) * r * r + a * r + b];
+}
+
+static bool blank(const char *s)
+{
+  for (; *s; ++s) if (*s != ' ' && *s != '\t' && *s != '\n') return false;
+  return true;
+}
+
+static int class_indices(const AwGamma &g, const std::string &t, int *a)
+{
+  if (t == "ALL") { for (int i = 0; i < g.nres_cl; ++i) a[i] = i; return g.nres_cl; }
+  if (t.size() == 1) {
+    int code = t[0] - 'A';
+    if (code < 0 || code >= 26 || (kSeMap[code] == 0 && t[0] != 'A')) return -1;
+    if (g.nres_cl < 20) return -2;
+    a[0] = kSeMap[code];
+    return 1;
+  }
+  if (t.size() == 3) {
+    if (g.nres_cl < 4) return -2;
+    int cl = t == "SHL" ? 1 : t == "AHL" ? 2 : t == "BAS" ? 3 : t == "HPB" ? 4 : 0;
+    if (!cl) return -1;
+    if (g.nres_cl == 4) { a[0] = cl - 1; return 1; }
+    int n = 0;
+    for (int i = 0; i < 20; ++i) if (kFour[i] == cl) a[n++] = i;
+    return n;
+  }
+  return -1;
+}
+
+AwStatus aw_read_gamma_array(const std::string &path, AwGamma &g)
+{
+  g = AwGamma();
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return AwStatus::err(-2, "Fragment_Memory: Cannot read gamma file");
+  char line[100];
+  std::vector<std::vector<std::string>> rows;
+  int iline = 0;
+  bool set = false;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || blank(line)) continue;
+    if (iline == 0) {
+      int ns = 0;
+      char *st = std::strtok(line, " \t");
+      while (st) {
+        ns++;
+        if (ns > 10) break;
+        if (std::strcmp(st, "inf") == 0) { g.i_sep[ns - 1] = -1; break; }
+        g.i_sep[ns - 1] = std::atoi(st);
+        st = std::strtok(nullptr, " \t\n");
+      }
+      g.nseq_cl = ns - 1;
+      if (g.nseq_cl <= 0) { std::fclose(f); return AwStatus::err(-3, "Fragment_Memory: Wrong definition of sequance separation classes"); }
+    } else {
+      std::vector<std::string> tk;
+      for (char *st = std::strtok(line, " \t\n"); st; st = std::strtok(nullptr, " \t\n")) tk.push_back(st);
+      if (tk.size() != 4 && tk.size() != 6) { std::fclose(f); return AwStatus::err(-3, "Fragment_Memory: Incorrect entery in gamma file"); }
+      bool six = tk.size() == 6;
+      if (!set) { g.frag_resty = six; set = true; }
+      else if (six != g.frag_resty) { std::fclose(f); return AwStatus::err(-3, "Fragment_Memory: Incorrect entery in gamma file"); }
+      for (size_t i = 0; i + 2 < tk.size(); ++i) {
+        size_t L = tk[i].size();
+        if (L != 1 && L != 3) { std::fclose(f); return AwStatus::err(-3, "Fragment_Memory: Incorrect entery in gamma file"); }
+        if (tk[i] == "ALL") g.nres_cl = std::max(g.nres_cl, 1);
+        else if (L == 3) g.nres_cl = std::max(g.nres_cl, 4);
+        else g.nres_cl = std::max(g.nres_cl, 20);
+      }
+      rows.push_back(tk);
+    }
+    iline++;
+  }
+  std::fclose(f);
+  if (g.nres_cl == 0) return AwStatus::err(-3, "Fragment_Memory: Incorrect entery in gamma file");
+  if (g.frag_resty)
+    return AwStatus::err(-4, "Fragment_Memory: gamma files with fragment residue classes are not part of this build's force path");
+  int r = g.nres_cl;
+  g.gamma.assign((size_t)g.nseq_cl * r * r, 0.0);
+  for (auto &tk : rows) {
+    int cl = std::atoi(tk[2].c_str());
+    double gm = std::atof(tk[3].c_str());
+    if (cl > g.nseq_cl) return AwStatus::err(-3, "Fragment_Memory: Wrong sequance separation class tag");
+    int is[20], js[20];
+    int in = class_indices(g, tk[0], is), jn = class_indices(g, tk[1], js);
+    if (in < 0 || jn < 0)
+      return AwStatus::err(-3, (in == -2 || jn == -2) ? "Fragment_Memory: Cannot build gamma array" : "Fragment_Memory: Incorrect entery in gamma file");
+    for (int i = 0; i < in; ++i)
+      for (int j = 0; j < jn; ++j) {
+        g.gamma[(size_t)(cl - 1) * r * r + is[i] * r + js[j]] = gm;
+        if (is[i] != js[j]) g.gamma[(size_t)(cl - 1) * r * r + js[j] * r + is[i]] = gm;
+      }
+  }
+  return AwStatus();
+}
+
+// ---------------------------------------------------------------------------
+// .gro files are parsed once and cached; each memory then applies the
+// reference's record loop (fm.cpp:61-98) to the cached records.
+
+struct GroRec { int ires; char resty[64], atomty[64]; double x, y, z; };
+struct GroFile { bool ok = false; std::vector<GroRec> recs; bool sorted = true; };
+
+static char three_to_one(const char *s)
+{
+  static const char *nm[20] = {"ALA", "ARG", "ASN", "ASP", "CYS", "GLN", "GLU", "GLY", "HIS", "ILE",
+                               "LEU", "LYS", "MET", "PHE", "PRO", "SER", "THR", "TRP", "TYR", "VAL"};
+  static const char one[] = "ARNDCQEGHILKMFPSTWYV";
+  if (std::strlen(s) == 3)
+    for (int i = 0; i < 20; ++i) if (std::strcmp(s, nm[i]) == 0) return one[i];
+  return '?';
+}
+
+static GroFile load_gro(const std::string &path)
+{
+  GroFile g;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return g;
+  char buff[201];
+  int nAtoms = 0;
+  if (!std::fgets(buff, 200, f) || std::fscanf(f, "%d", &nAtoms) != 1) nAtoms = 0;
+  if (!std::fgets(buff, 200, f)) nAtoms = 0;
+  g.recs.reserve(nAtoms > 0 ? nAtoms : 0);
+  for (int i = 0; i < nAtoms; ++i) {
+    GroRec r;
+    char rt[64], at[64];
+    int iatom;
+    if (std::fscanf(f, "%d %63s %63s %d %lf %lf %lf", &r.ires, rt, at, &iatom, &r.x, &r.y, &r.z) != 7) break;
+    std::snprintf(r.resty, sizeof r.resty, "%s", rt);
+    std::snprintf(r.atomty, sizeof r.atomty, "%s", at);
+    if (!g.recs.empty() && r.ires < g.recs.back().ires) g.sorted = false;
+    g.recs.push_back(r);
+  }
+  std::fclose(f);
+  g.ok = true;
+  return g;
+}
+
+// returns 0 or Fragment_Memory error code (1 file, 2 atom count)
+static int build_memory(const GroFile &g, int pos, int fpos, int len, double weight, AwMemory &m)
+{
+  if (!g.ok) return 1;
+  m.pos = pos; m.len = len; m.weight = weight;
+  m.se.assign(len, '\0');
+  m.ca.assign((size_t)len * 3, 0.0);
+  m.cb.assign((size_t)len * 3, 0.0);
+  int nca = 0, ncb = 0;
+  size_t k0 = 0;
+  if (g.sorted) {  // first record with ires > fpos
+    size_t lo = 0, hi = g.recs.size();
+    while (lo < hi) { size_t mid = (lo + hi) / 2; if (g.recs[mid].ires > fpos) hi = mid; else lo = mid + 1; }
+    k0 = lo;
+  }
+  for (size_t k = k0; k < g.recs.size(); ++k) {
+    const GroRec &r = g.recs[k];
+    int ires = r.ires;
+    if (ires > fpos + len) break;
+    if (ires > fpos && ires <= fpos + len) {
+      ires -= fpos + 1;
+      double x = r.x * 10, y = r.y * 10, z = r.z * 10;
+      if (std::strcmp(r.atomty, "CA") == 0) {
+        if (ires >= len || nca >= len) return 2;
+        m.se[ires] = three_to_one(r.resty);
+        m.ca[3 * ires] = x; m.ca[3 * ires + 1] = y; m.ca[3 * ires + 2] = z;
+        nca++;
+      }
+      if (std::strcmp(r.atomty, "CB") == 0) {
+        if (ires >= len || ncb >= len) return 2;
+        m.cb[3 * ires] = x; m.cb[3 * ires + 1] = y; m.cb[3 * ires + 2] = z;
+        ncb++;
+      }
+    }
+  }
+  return nca == len ? 0 : 2;
+}
+
+static char *trim(char *s)
+{
+  while (std::isspace((unsigned char)*s)) s++;
+  size_t n = std::strlen(s);
+  while (n > 0 && std::isspace((unsigned char)s[n - 1])) s[--n] = '\0';
+  return s;
+}
+
+AwStatus aw_read_mems(const std::string &dir, const std::string &mem_file, AwSystem &S)
+{
+  FILE *f = std::fopen(join(dir, mem_file).c_str(), "r");
+  if (!f) return AwStatus::err(-2, "Fragment_Memory: Error opening mem file");
+  std::unordered_map<std::string, GroFile> cache;
+  char ln[500];
+  int state = 0;  // 0 none, 1 target, 2 memories
+  S.mems.clear();
+  while (std::fgets(ln, sizeof ln, f)) {
+    char *line = trim(ln);
+    if (line[0] == '#') continue;
+    if (line[0] == '[') state = 0;
+    if (blank(line)) { state = 0; continue; }
+    if (state == 2) {
+      char *str[6];
+      int nstr = 0;
+      str[nstr] = std::strtok(line, " \t\n");
+      while (str[nstr]) {
+        nstr++;
+        if (nstr > 5) break;
+        str[nstr] = std::strtok(nullptr, " \t\n");
+      }
+      if (nstr != 5) { std::fclose(f); return AwStatus::err(-3, "Fragment_Memory: Error reading mem file"); }
+      int tpos = std::atoi(str[1]) - 1, fpos = std::atoi(str[2]) - 1, len = std::atoi(str[3]);
+      double w = std::atof(str[4]);
+      std::string gname = str[0];
+      auto it = cache.find(gname);
+      if (it == cache.end()) it = cache.emplace(gname, load_gro(join(dir, gname))).first;
+      AwMemory m;
+      if (len <= 0 || build_memory(it->second, tpos, fpos, len, w, m)) {
+        std::fclose(f);
+        return AwStatus::err(-3, "read_mems: Fragment_Memory: Error reading memory fragment " + gname);
+      }
+      if (tpos < 0 || tpos + len > S.n) {
+        std::fclose(f);
+        return AwStatus::err(-3, "read_mems: Fragment_Memory: Incorrectly defined memory fragment");
+      }
+      S.mems.push_back(std::move(m));
+    } else if (state == 0) {
+      if (std::strcmp(line, "[Target]") == 0) state = 1;
+      else if (std::strcmp(line, "[Memories]") == 0) state = 2;
+    }
+  }
+  std::fclose(f);
+  return AwStatus();
+}
+
+AwStatus aw_finish_system(AwSystem &S)
+{
+  FFParams &P = S.P;
+  S.rinfo.assign(S.n, 0);
+  for (int c = 0; c < S.nch; ++c)
+    for (int k = 0; k < S.ch_len[c]; ++k) {
+      int i = S.ch_start[c] + k;
+      int ri = aw_restype(S.se[i]) | (c << RI_CHAIN_SHIFT);
+      if (k == 0) ri |= RI_FIRST;
+      if (k == S.ch_len[c] - 1) ri |= RI_LAST;
+      if (S.se[i] == 'G') ri |= RI_GLY;
+      if (S.se[i] == 'P') ri |= RI_PRO;
+      S.rinfo[i] = ri;
+    }
+  if (P.fm_flag) {
+    P.fm_min_sep = S.fmg.min_sep();
+    P.fm_max_sep = S.fmg.max_sep();
+    for (auto &m : S.mems) {
+      int i0 = m.pos, i1 = m.pos + m.len - 1;
+      for (int i = i0; i < i1; ++i)
+        if (S.rinfo[i] >> RI_CHAIN_SHIFT != S.rinfo[i1] >> RI_CHAIN_SHIFT && i + P.fm_min_sep <= i1)
+          return AwStatus::err(-3, "Fragment Memory: Interaction between residues of different chains");
+    }
+  }
+  double dens = 0.0;
+  if (P.water_flag) for (int w = 0; w < P.n_wells; ++w) dens = std::fmax(dens, P.w_rmax_th[w]);
+  if (P.helix_flag) dens = std::fmax(dens, P.h_rmax_th);
+  P.dens_reach = dens;
+  double reach = dens;
+  if (P.dssp_flag) reach = std::fmax(reach, P.dssp_cut);
+  if (P.pap_flag) reach = std::fmax(reach, std::sqrt(P.pap_cutoff_sq));
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b)
+      if (P.ev_lambda[a][b] != 0.0) reach = std::fmax(reach, std::fmax(P.ev_cut_short[a][b], P.ev_cut_long[a][b]));
+  P.pair_reach = reach;
+  return AwStatus();
+}
+
+// ---------------------------------------------------------------------------
+// Fragment-memory work description of one system: memories are grouped by
+// (pos, len); a group's (i, j) pairs are the ones
+// compute_fragment_memory_potential visits (fb.cpp:4110-4125, frag_mem_map
+// 784-788) with a non-zero gamma.  rf holds, per (memory, pair), the
+// memory distances CA(i)-CA(j), CA(i)-CB(j), CB(i)-CA(j), CB(i)-CB(j)
+// (Fragment_Memory::Rf, fm.cpp:190-202); a CB entry of a glycine memory
+// residue is stored as -1 (skipped, fb.cpp:4148-4149).
+
+AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B, int item_base)
+{
+  const FFParams &P = S.P;
+  std::map<std::pair<int, int>, std::vector<int>> by;
+  for (size_t k = 0; k < S.mems.size(); ++k) by[{S.mems[k].pos, S.mems[k].len}].push_back((int)k);
+  int min_sep = S.fmg.min_sep(), max_sep = S.fmg.max_sep();
+  int item = item_base;
+  for (auto &kv : by) {
+    int pos = kv.first.first, len = kv.first.second;
+    // candidate pairs of this window (fb.cpp:4110-4125, frag_mem_map 784-788)
+    std::vector<std::pair<int, int>> prs;
+    for (int di = 0; di < len - min_sep; ++di) {
+      int i = pos + di;
+      int je = pos + len - 1;
+      if (max_sep != -1) je = std::min(je, i + max_sep);
+      if (je >= S.n) return AwStatus::err(-3, "Missing residues in memory potential");
+      for (int j = i + min_sep; j <= je; ++j) {
+        double g = S.fmg.get(aw_restype(S.se[i]), aw_restype(S.se[j]), i, j);
+        if (g == 0.0) continue;
+        prs.push_back({i - pos, j - pos});
+      }
+    }
+    if (prs.empty()) continue;
+    FMGroup G;
+    std::memset(&G, 0, sizeof G);
+    G.pos = pos; G.len = len; G.npairs = (int)prs.size(); G.nmem = (int)kv.second.size();
+    G.pair_off = (int)(B.pairs.size() / 2);
+    G.item_off = item;
+    G.mem_off = (int)B.coef.size();
+    G.rf_off = (long long)(B.rf.size() / 4);
+    for (auto &pr : prs) { B.pairs.push_back(pr.first); B.pairs.push_back(pr.second); }
+    for (auto &pr : prs) {
+      int i = pos + pr.first, j = pos + pr.second;
+      double sig2 = std::pow((double)std::abs(i - j), 2 * P.fm_sigma_exp);
+      B.items.push_back(S.fmg.get(aw_restype(S.se[i]), aw_restype(S.se[j]), i, j));
+      B.items.push_back(1.0 / (2.0 * sig2));
+      B.items.push_back(1.0 / sig2);
+      B.items.push_back(0.0);
+    }
+    for (int mk : kv.second) {
+      const AwMemory &m = S.mems[mk];
+      B.coef.push_back(P.epsilon * P.k_frag_mem * m.weight);
+      auto R = [](const double *a, const double *b) {
+        return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+      };
+      for (auto &pr : prs) {
+        int a = pr.first, b = pr.second;  // a < b
+        const double *cai = &m.ca[3 * a], *caj = &m.ca[3 * b], *cbi = &m.cb[3 * a], *cbj = &m.cb[3 * b];
+        bool gi = m.se[a] == 'G', gj = m.se[b] == 'G';
+        B.rf.push_back(R(cai, caj));                 // rf[0][a][b], a<b: CA-CA
+        B.rf.push_back(gj ? -1.0 : R(cai, cbj));     // rf[1][a][b]: CA(i)-CB(j)
+        B.rf.push_back(gi ? -1.0 : R(caj, cbi));     // rf[1][b][a]: CA(j)-CB(i)
+        B.rf.push_back((gi || gj) ? -1.0 : R(cbi, cbj));  // rf[0][b][a]: CB-CB
+      }
+    }
+    item += G.npairs;
+    B.groups.push_back(G);
+  }
+  return AwStatus();
+}
+

@@ -1,0 +1,1172 @@
+// awsem_terms.h -- the FixBackbone force terms and the Langevin update as
+// per-item functions over one replica view.  Included by the gfx950 kernel
+// file; every function is also valid host C++ (AW_HOST_EMU), which lets
+// the exact kernel arithmetic be stepped through on a CPU when debugging.
+#pragma once
+#include <stdint.h>
+#include <math.h>
+#include "awsem_types.h"
+
+#ifdef AW_HOST_EMU
+#include <algorithm>
+#define AW_DEV inline
+#define AW_DEVCONST static const
+struct int2 { int x, y; };
+struct float4 { float x, y, z, w; };
+struct double4 { double x, y, z, w; };
+inline float aw_fast_exp(float a) { return expf(a); }
+inline float aw_fsqrt(float a) { return sqrtf(a); }
+inline void aadd(double *p, double v) { *p += v; }
+#else
+#include <hip/hip_runtime.h>
+#define AW_DEV __device__ __forceinline__
+#define AW_DEVCONST __device__ __constant__
+__device__ __forceinline__ float aw_fast_exp(float a) { return __expf(a); }
+__device__ __forceinline__ float aw_fsqrt(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
+#endif
+
+#define AW_PAP_DELTA 1e-12
+#define AW_DSSP_NU_DELTA 1e-4
+#define AW_DELTA 0.00001
+#define AW_DELTA_WATER_XI 1e-8
+#define AW_DELTA_HELIX_XI 1e-6
+
+// ------------------------------------------------------------------------
+// small math helpers
+
+template <class T> struct V3 { T x, y, z; };
+
+AW_DEV float t_exp(float a) { return aw_fast_exp(a); }
+AW_DEV double t_exp(double a) { return exp(a); }
+AW_DEV float t_sqrt(float a) { return aw_fsqrt(a); }
+AW_DEV double t_sqrt(double a) { return sqrt(a); }
+AW_DEV float t_tanh(float a) { return tanhf(a); }
+AW_DEV double t_tanh(double a) { return tanh(a); }
+AW_DEV float t_cos(float a) { return cosf(a); }
+AW_DEV double t_cos(double a) { return cos(a); }
+AW_DEV float t_sin(float a) { return sinf(a); }
+AW_DEV double t_sin(double a) { return sin(a); }
+AW_DEV float t_atan2(float y, float x) { return atan2f(y, x); }
+AW_DEV double t_atan2(double y, double x) { return atan2(y, x); }
+
+// exact (no FMA contraction) squared distance: the list predicates must
+// decide exactly as the reference's FP64 host code does
+AW_DEV double rsq_exact(const double *a, const double *b)
+{
+#pragma clang fp contract(off)
+  double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+AW_DEV double lin3_exact(double a, double xa, double b, double xb, double c, double xc)
+{
+#pragma clang fp contract(off)
+  return a * xa + b * xb + c * xc;
+}
+
+template <class T>
+AW_DEV V3<T> dvec(const double *a, const double *b)
+{
+  V3<T> d;
+  d.x = (T)(a[0] - b[0]);
+  d.y = (T)(a[1] - b[1]);
+  d.z = (T)(a[2] - b[2]);
+  return d;
+}
+template <class T> AW_DEV T dot(const V3<T> &a, const V3<T> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> AW_DEV V3<T> crs(const V3<T> &a, const V3<T> &b)
+{
+  V3<T> c;
+  c.x = a.y * b.z - a.z * b.y;
+  c.y = a.z * b.x - a.x * b.z;
+  c.z = a.x * b.y - a.y * b.x;
+  return c;
+}
+
+
+// ------------------------------------------------------------------------
+// per-replica view: every pointer is either LDS or a global scratch slab
+
+template <class T>
+struct View {
+  int n, nch;
+  const int *rinfo;     // [n] (global)
+  const double *aps;    // [n][12] (global)
+  const int *ch_start;  // [nch] 0-based
+  const int *ch_len;
+  double *x;            // [3][n][3] CA, CB, O
+  double *xv;           // [3][n][3] N, H, C'
+  double *f;            // [3][n][3]
+  double *rho_w, *rho_h, *sig_w, *sigp_w, *sig_h, *sigp_h, *xi_w, *bf, *hxi1, *hxi2;
+  float *ext;           // [n] distance from CA to the farthest own atom
+  const FFParams *P;
+  // fragment memory
+  const FMGroup *groups;
+  int ngroups, fm_items, fm_item_base;
+  const int2 *pairs;
+  const T *coef;
+  const void *rf;        // T4 per (memory, pair)
+  const void *items;     // FMItem / FMItemD per item
+};
+
+#define XA(V, kind, i) ((V).x + ((kind) * (V).n + (i)) * 3)
+#define XV(V, kind, i) ((V).xv + ((kind) * (V).n + (i)) * 3)
+#define FA(V, kind, i) ((V).f + ((kind) * (V).n + (i)) * 3)
+enum { K_CA = 0, K_CB = 1, K_O = 2 };
+enum { K_N = 0, K_H = 1, K_CP = 2 };
+
+template <class T>
+AW_DEV void fadd(const View<T> &V, int kind, int i, T s, const V3<T> &d)
+{
+  double *p = FA(V, kind, i);
+  aadd(p + 0, (double)(s * d.x));
+  aadd(p + 1, (double)(s * d.y));
+  aadd(p + 2, (double)(s * d.z));
+}
+template <class T>
+AW_DEV void fadd3(const View<T> &V, int kind, int i, T fx, T fy, T fz)
+{
+  double *p = FA(V, kind, i);
+  aadd(p + 0, (double)fx);
+  aadd(p + 1, (double)fy);
+  aadd(p + 2, (double)fz);
+}
+
+// energy accumulators (per thread, FP64)
+struct EAcc {
+  double chain, chi, rama, ev, dssp, pap, water, burial, helix, fm, bond, shake;
+  AW_DEV void clear() { chain = chi = rama = ev = dssp = pap = water = burial = helix = fm = bond = shake = 0.0; }
+};
+#define N_EACC 12
+AW_DEV double eacc_get(const EAcc &e, int k)
+{
+  switch (k) {
+  case 0: return e.chain; case 1: return e.chi; case 2: return e.rama; case 3: return e.ev;
+  case 4: return e.dssp; case 5: return e.pap; case 6: return e.water; case 7: return e.burial;
+  case 8: return e.helix; case 9: return e.fm; case 10: return e.bond; default: return e.shake;
+  }
+}
+AW_DEVCONST int eacc_slot[N_EACC] = {AW_ET_CHAIN, AW_ET_CHI, AW_ET_RAMA, AW_ET_VEXCLUDED, AW_ET_DSSP,
+                                                 AW_ET_PAP, AW_ET_WATER, AW_ET_BURIAL, AW_ET_HELIX, AW_ET_FRAGMEM,
+                                                 AW_ET_BOND, AW_ET_SHAKE};
+
+#define ON(P, slot) ((P)->term_mask & (1u << (slot)))
+
+AW_DEV int rtype(int ri) { return ri & RI_TYPE_MASK; }
+AW_DEV int rchain(int ri) { return ri >> RI_CHAIN_SHIFT; }
+
+// unordered residue pair k -> (i, j), i < j (each pair exactly once)
+AW_DEV void pair_decode(int k, int n, int &i, int &j)
+{
+  int H = (n - 1) >> 1;
+  int a, b;
+  if (k < n * H) {
+    a = k % n;
+    int d = 1 + k / n;
+    b = a + d;
+    if (b >= n) b -= n;
+  } else {
+    a = k - n * H;
+    b = a + (n >> 1);
+  }
+  i = a < b ? a : b;
+  j = a < b ? b : a;
+}
+AW_DEV int pair_count(int n) { return n * (n - 1) / 2; }
+
+// ------------------------------------------------------------------------
+// Phase A: virtual atoms (fb.cpp:7113-7138), extents, zeroing
+
+template <class T>
+AW_DEV void phase_A(const View<T> &V, int t0, int ts)
+{
+  const FFParams *P = V.P;
+  int n = V.n;
+  for (int i = t0; i < n; i += ts) {
+    int ri = V.rinfo[i];
+    const double *ca = XA(V, K_CA, i), *o = XA(V, K_O, i), *cb = XA(V, K_CB, i);
+    double *xn = XV(V, K_N, i), *xh = XV(V, K_H, i), *xcp = XV(V, K_CP, i);
+    if (!(ri & RI_FIRST)) {
+      const double *cam = XA(V, K_CA, i - 1), *om = XA(V, K_O, i - 1);
+      for (int k = 0; k < 3; ++k) {
+        xn[k] = lin3_exact(P->an, cam[k], P->bn, ca[k], P->cn, om[k]);
+        xh[k] = lin3_exact(P->ah, cam[k], P->bh, ca[k], P->ch, om[k]);
+      }
+    } else {
+      xn[0] = xn[1] = xn[2] = 0.0;
+      xh[0] = xh[1] = xh[2] = 0.0;
+    }
+    if (!(ri & RI_LAST)) {
+      const double *cap = XA(V, K_CA, i + 1);
+      for (int k = 0; k < 3; ++k) xcp[k] = lin3_exact(P->ap, ca[k], P->bp, cap[k], P->cp, o[k]);
+    } else {
+      xcp[0] = xcp[1] = xcp[2] = 0.0;
+    }
+    double e2 = rsq_exact(ca, cb);
+    e2 = fmax(e2, rsq_exact(ca, o));
+    if (!(ri & RI_FIRST)) { e2 = fmax(e2, rsq_exact(ca, xn)); e2 = fmax(e2, rsq_exact(ca, xh)); }
+    if (!(ri & RI_LAST)) e2 = fmax(e2, rsq_exact(ca, xcp));
+    V.ext[i] = (float)(sqrt(e2) * (1.0 + 1e-6) + 1e-4);
+    double *f = V.f;
+    for (int k = 0; k < 3; ++k) {
+      f[(0 * n + i) * 3 + k] = 0.0;
+      f[(1 * n + i) * 3 + k] = 0.0;
+      f[(2 * n + i) * 3 + k] = 0.0;
+    }
+    V.rho_w[i] = 0.0; V.rho_h[i] = 0.0; V.xi_w[i] = 0.0;
+    V.hxi1[i] = 0.0; V.hxi2[i] = 0.0; V.bf[i] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// local terms
+
+// compute_chain_potential, fb.cpp:2062-2170
+template <class T>
+AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
+{
+  const FFParams *P = V.P;
+  bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
+  if (!first && !gly) {
+    V3<T> d = dvec<T>(XV(V, K_N, i), XA(V, K_CB, i));
+    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_ncb0;
+    T force = (T)2 * (T)P->k_chain[0] * dr / r;
+    E.chain += (double)((T)P->k_chain[0] * dr * dr);
+    fadd(V, K_CA, i - 1, -(T)P->an * force, d);
+    fadd(V, K_O, i - 1, -(T)P->cn * force, d);
+    fadd(V, K_CA, i, -(T)P->bn * force, d);
+    fadd(V, K_CB, i, force, d);
+  }
+  if (!last && !gly) {
+    V3<T> d = dvec<T>(XV(V, K_CP, i), XA(V, K_CB, i));
+    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_cpcb0;
+    T force = (T)2 * (T)P->k_chain[1] * dr / r;
+    E.chain += (double)((T)P->k_chain[1] * dr * dr);
+    fadd(V, K_CA, i + 1, -(T)P->bp * force, d);
+    fadd(V, K_CA, i, -(T)P->ap * force, d);
+    fadd(V, K_O, i, -(T)P->cp * force, d);
+    fadd(V, K_CB, i, force, d);
+  }
+  if (!first && !last) {
+    V3<T> d = dvec<T>(XV(V, K_N, i), XV(V, K_CP, i));
+    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_ncp0;
+    T force = (T)2 * (T)P->k_chain[2] * dr / r;
+    E.chain += (double)((T)P->k_chain[2] * dr * dr);
+    fadd(V, K_CA, i - 1, -(T)P->an * force, d);
+    fadd(V, K_O, i - 1, -(T)P->cn * force, d);
+    fadd(V, K_CA, i + 1, (T)P->bp * force, d);
+    fadd(V, K_CA, i, -(T)(P->bn - P->ap) * force, d);
+    fadd(V, K_O, i, (T)P->cp * force, d);
+  }
+}
+
+// compute_shake, fb.cpp:2172-2237
+template <class T>
+AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E)
+{
+  const FFParams *P = V.P;
+  T ks = (T)(P->epsilon * P->k_shake);
+  bool last = ri & RI_LAST;
+  if (!last) {
+    V3<T> d = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, i + 1));
+    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_sh1, force = (T)2 * ks * dr / r;
+    E.shake += (double)(ks * dr * dr);
+    fadd(V, K_CA, i, -force, d);
+    fadd(V, K_CA, i + 1, force, d);
+  }
+  {
+    V3<T> d = dvec<T>(XA(V, K_CA, i), XA(V, K_O, i));
+    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_sh2, force = (T)2 * ks * dr / r;
+    E.shake += (double)(ks * dr * dr);
+    fadd(V, K_CA, i, -force, d);
+    fadd(V, K_O, i, force, d);
+  }
+  if (!last) {
+    V3<T> d = dvec<T>(XA(V, K_O, i), XA(V, K_CA, i + 1));
+    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_sh3, force = (T)2 * ks * dr / r;
+    E.shake += (double)(ks * dr * dr);
+    fadd(V, K_O, i, -force, d);
+    fadd(V, K_CA, i + 1, force, d);
+  }
+}
+
+// compute_chi_potential, fb.cpp:2239-2339
+template <class T>
+AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
+{
+  const FFParams *P = V.P;
+  V3<T> a = dvec<T>(XV(V, K_CP, i), XA(V, K_CA, i));
+  V3<T> b = dvec<T>(XA(V, K_CA, i), XV(V, K_N, i));
+  V3<T> c = dvec<T>(XA(V, K_CA, i), XA(V, K_CB, i));
+  T arvsq = (T)1 / dot(a, a), brvsq = (T)1 / dot(b, b), crvsq = (T)1 / dot(c, c);
+  T norm = t_sqrt(arvsq * brvsq * crvsq);
+  V3<T> axb = crs(a, b), cxa = crs(c, a), bxc = crs(b, c);
+  T chi = dot(axb, c) * norm;
+  V3<T> ap_, bp_, cp_;
+  ap_.x = norm * bxc.x - arvsq * chi * a.x; ap_.y = norm * bxc.y - arvsq * chi * a.y; ap_.z = norm * bxc.z - arvsq * chi * a.z;
+  bp_.x = norm * cxa.x - brvsq * chi * b.x; bp_.y = norm * cxa.y - brvsq * chi * b.y; bp_.z = norm * cxa.z - brvsq * chi * b.z;
+  cp_.x = norm * axb.x - crvsq * chi * c.x; cp_.y = norm * axb.y - crvsq * chi * c.y; cp_.z = norm * axb.z - crvsq * chi * c.z;
+  T dchi = chi - (T)P->chi0;
+  T force = (T)2 * (T)P->k_chi * dchi;
+  E.chi += (double)((T)P->k_chi * dchi * dchi);
+  fadd(V, K_CA, i - 1, (T)P->an * force, bp_);
+  fadd(V, K_O, i - 1, (T)P->cn * force, bp_);
+  fadd(V, K_CA, i + 1, -(T)P->bp * force, ap_);
+  T c1 = (T)(1 - P->bn), c2 = (T)(P->ap - 1);
+  fadd3(V, K_CA, i, -(cp_.x + c1 * bp_.x + c2 * ap_.x) * force, -(cp_.y + c1 * bp_.y + c2 * ap_.y) * force,
+        -(cp_.z + c1 * bp_.z + c2 * ap_.z) * force);
+  fadd(V, K_O, i, -(T)P->cp * force, ap_);
+  fadd(V, K_CB, i, force, cp_);
+}
+
+// calcDihedralAndSlopes, fb.cpp:2341-2453 (slope[atom]: CA(i-1), CA(i),
+// CA(i+1), O(i-1), O(i)); the x and y slopes are summed since the force
+// uses their sum (fb.cpp:2467-2504)
+template <class T>
+AW_DEV T dihedral(const View<T> &V, int i, bool phi, V3<T> s[5])
+{
+  const FFParams *P = V.P;
+  T an = (T)P->an, bn = (T)P->bn, cn = (T)P->cn, ap = (T)P->ap, bp = (T)P->bp, cp = (T)P->cp;
+  V3<T> a, b, c;
+  if (phi) {
+    a = dvec<T>(XV(V, K_CP, i), XA(V, K_CA, i));
+    b = dvec<T>(XA(V, K_CA, i), XV(V, K_N, i));
+    c = dvec<T>(XV(V, K_N, i), XV(V, K_CP, i - 1));
+  } else {
+    a = dvec<T>(XV(V, K_N, i + 1), XV(V, K_CP, i));
+    b = dvec<T>(XV(V, K_CP, i), XA(V, K_CA, i));
+    c = dvec<T>(XA(V, K_CA, i), XV(V, K_N, i));
+  }
+  T adb = dot(a, b), bdc = dot(b, c), adc = dot(a, c), b2 = dot(b, b), bm = t_sqrt(b2);
+  V3<T> bxa = crs(b, a), cxa = crs(c, a), cxb = crs(c, b);
+  T cdbxa = dot(c, bxa);
+  T Y = bm * cdbxa, X = adb * bdc - b2 * adc;
+  T angle = t_atan2(Y, X);
+  T X2Y2 = (T)1 / (X * X + Y * Y);
+  T dy = X * X2Y2, dx = -Y * X2Y2;
+  b2 *= dx; adb *= dx; adc *= dx; bdc *= dx;
+  cdbxa *= dy / bm;
+  bm *= dy;
+  const T av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
+  const T bxav[3] = {bxa.x, bxa.y, bxa.z}, cxav[3] = {cxa.x, cxa.y, cxa.z}, cxbv[3] = {cxb.x, cxb.y, cxb.z};
+  T out[5][3];
+  for (int l = 0; l < 3; l++) {
+    T h1, h2, h3, h4, g1, g2, g3;
+    g1 = bv[l] * bdc - cv[l] * b2;
+    g2 = av[l] * bdc - (T)2 * bv[l] * adc + cv[l] * adb;
+    g3 = bv[l] * adb - av[l] * b2;
+    if (phi) {
+      h1 = cxbv[l] * bm; h2 = cxav[l] * bm; h3 = bxav[l] * bm; h4 = bv[l] * cdbxa;
+      out[0][l] = -an * h4 + (an - ap) * h3 + an * h2 + (-an * g2 + (an - ap) * g3);
+      out[1][l] = ((T)1 - bn) * h4 + (bn - bp) * h3 + (ap - (T)1) * h1 - ((T)1 - bn) * h2 +
+                  ((ap - (T)1) * g1 + ((T)1 - bn) * g2 + (bn - bp) * g3);
+      out[2][l] = bp * h1 + bp * g1;
+      out[3][l] = -cn * h4 + (cn - cp) * h3 + cn * h2 + (-cn * g2 + (cn - cp) * g3);
+      out[4][l] = cp * h1 + cp * g1;
+    } else {
+      h1 = bxav[l] * bm; h2 = cxbv[l] * bm; h3 = cxav[l] * bm; h4 = bv[l] * cdbxa;
+      out[0][l] = -an * h1 + (-an * g3);
+      out[1][l] = (ap - (T)1) * h4 + ((T)1 - bn) * h1 + (an - ap) * h2 - (ap - (T)1) * h3 +
+                  ((an - ap) * g1 + (ap - (T)1) * g2 + ((T)1 - bn) * g3);
+      out[2][l] = bp * h4 + (bn - bp) * h2 - bp * h3 + ((bn - bp) * g1 + bp * g2);
+      out[3][l] = -cn * h1 + (-cn * g3);
+      out[4][l] = cp * h4 + (cn - cp) * h2 - cp * h3 + ((cn - cp) * g1 + cp * g2);
+    }
+  }
+  for (int k = 0; k < 5; ++k) { s[k].x = out[k][0]; s[k].y = out[k][1]; s[k].z = out[k][2]; }
+  return angle;
+}
+
+// compute_rama_potential / compute_rama_force, fb.cpp:2455-2546
+template <class T>
+AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
+{
+  const FFParams *P = V.P;
+  V3<T> sphi[5], spsi[5];
+  T phi = dihedral(V, i, true, sphi);
+  T psi = dihedral(V, i, false, spsi);
+  int js = 0, je = P->n_rama_par;
+  if ((ri & RI_PRO) && P->rama_p_flag) { js = 6; je = 6 + P->n_rama_p_par; }
+  T fphi = 0, fpsi = 0;
+  double ev = 0.0;
+  for (int j = js; j < je; j++) {
+    double apsv = V.aps[i * 12 + j];
+    if (P->ssweight[j] && apsv == 0.0) continue;
+    T cphi = t_cos(phi + (T)P->phi0[j]) - (T)1, cpsi = t_cos(psi + (T)P->psi0[j]) - (T)1;
+    T pc = (T)P->phiw[j] * cphi, sc = (T)P->psiw[j] * cpsi;
+    T Vv = (T)P->w[j] * t_exp(-cphi * pc - cpsi * sc);
+    if (P->ssweight[j]) Vv *= (T)apsv;
+    T force = (T)2 * Vv;
+    fphi += force * pc * t_sin(phi + (T)P->phi0[j]);
+    fpsi += force * sc * t_sin(psi + (T)P->psi0[j]);
+    ev += (double)(-Vv);
+  }
+  E.rama += ev;
+  const int kinds[5] = {K_CA, K_CA, K_CA, K_O, K_O};
+  const int res[5] = {i - 1, i, i + 1, i - 1, i};
+  for (int k = 0; k < 5; ++k)
+    fadd3(V, kinds[k], res[k], fphi * sphi[k].x + fpsi * spsi[k].x, fphi * sphi[k].y + fpsi * spsi[k].y,
+          fphi * sphi[k].z + fpsi * spsi[k].z);
+}
+
+// LAMMPS bond_style harmonic on the deck's bonds (outside the reference).
+// Always FP64: r - r0 cancels for bonds near their rest length.
+template <class T>
+AW_DEV void bond_one(const View<T> &V, int ka, int ia, int kb, int ib, double r0, EAcc &E)
+{
+  V3<double> d = dvec<double>(XA(V, ka, ia), XA(V, kb, ib));
+  double r = sqrt(dot(d, d)), dr = r - r0;
+  double k = V.P->bond_k;
+  E.bond += k * dr * dr;
+  double fb = r > 0.0 ? -2.0 * k * dr / r : 0.0;
+  double *pa = FA(V, ka, ia), *pb = FA(V, kb, ib);
+  aadd(pa + 0, fb * d.x); aadd(pa + 1, fb * d.y); aadd(pa + 2, fb * d.z);
+  aadd(pb + 0, -fb * d.x); aadd(pb + 1, -fb * d.y); aadd(pb + 2, -fb * d.z);
+}
+
+template <class T>
+AW_DEV void bonds_term(const View<T> &V, int i, int ri, EAcc &E)
+{
+  const FFParams *P = V.P;
+  bond_one(V, K_CA, i, K_O, i, P->bond_r0[1], E);
+  bond_one(V, K_CA, i, K_CB, i, (ri & RI_GLY) ? P->bond_r0[4] : P->bond_r0[3], E);
+  if (!(ri & RI_LAST)) {
+    bond_one(V, K_CA, i, K_CA, i + 1, P->bond_r0[0], E);
+    bond_one(V, K_O, i, K_CA, i + 1, P->bond_r0[2], E);
+  }
+}
+
+// ------------------------------------------------------------------------
+// fragment memory: one item = one (i, j) pair of one memory group, looping
+// over the group's memories (fb.cpp:4079-4177, Rf fm.cpp:190-202)
+
+template <class T> struct T4 {};
+template <> struct T4<float> { typedef float4 type; typedef FMItem item; };
+template <> struct T4<double> { typedef double4 type; typedef FMItemD item; };
+
+template <class T>
+AW_DEV void fm_item(const View<T> &V, int item, EAcc &E)
+{
+  typedef typename T4<T>::type T4t;
+  typedef typename T4<T>::item ItemT;
+  int lo = 0, hi = V.ngroups - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (V.groups[mid].item_off <= item) lo = mid; else hi = mid - 1;
+  }
+  const FMGroup G = V.groups[lo];
+  int q = item - G.item_off;
+  int2 pr = V.pairs[G.pair_off + q];
+  int i = G.pos + pr.x, j = G.pos + pr.y;
+  const ItemT it = ((const ItemT *)V.items)[item];
+  bool gi = V.rinfo[i] & RI_GLY, gj = V.rinfo[j] & RI_GLY;
+  V3<T> d[4];
+  d[0] = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j));
+  d[1] = dvec<T>(XA(V, K_CA, i), XA(V, K_CB, j));
+  d[2] = dvec<T>(XA(V, K_CB, i), XA(V, K_CA, j));
+  d[3] = dvec<T>(XA(V, K_CB, i), XA(V, K_CB, j));
+  T r[4];
+  for (int k = 0; k < 4; ++k) r[k] = t_sqrt(dot(d[k], d[k]));
+  bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
+  T inv2s = (T)it.inv2s;
+  T S0 = 0, S1 = 0, S2 = 0, S3 = 0, Es = 0;
+  const T4t *rf = (const T4t *)V.rf + G.rf_off + q;
+  const T *coef = V.coef + G.mem_off;
+  int np = G.npairs;
+#pragma unroll 4
+  for (int m = 0; m < G.nmem; ++m) {
+    T4t f4 = rf[(long long)m * np];
+    T cf = coef[m];
+    {
+      T dr = r[0] - f4.x;
+      T v = -cf * t_exp(-dr * dr * inv2s);
+      S0 += v * dr; Es += v;
+    }
+    if (act1 && f4.y >= (T)0) {
+      T dr = r[1] - f4.y;
+      T v = -cf * t_exp(-dr * dr * inv2s);
+      S1 += v * dr; Es += v;
+    }
+    if (act2 && f4.z >= (T)0) {
+      T dr = r[2] - f4.z;
+      T v = -cf * t_exp(-dr * dr * inv2s);
+      S2 += v * dr; Es += v;
+    }
+    if (act3 && f4.w >= (T)0) {
+      T dr = r[3] - f4.w;
+      T v = -cf * t_exp(-dr * dr * inv2s);
+      S3 += v * dr; Es += v;
+    }
+  }
+  T g = (T)it.gamma, invs = (T)it.invs;
+  E.fm += (double)(g * Es);
+  T f0 = g * S0 * invs / r[0];
+  T f1 = act1 ? g * S1 * invs / r[1] : (T)0;
+  T f2 = act2 ? g * S2 * invs / r[2] : (T)0;
+  T f3 = act3 ? g * S3 * invs / r[3] : (T)0;
+  fadd3(V, K_CA, i, f0 * d[0].x + f1 * d[1].x, f0 * d[0].y + f1 * d[1].y, f0 * d[0].z + f1 * d[1].z);
+  fadd3(V, K_CA, j, -(f0 * d[0].x + f2 * d[2].x), -(f0 * d[0].y + f2 * d[2].y), -(f0 * d[0].z + f2 * d[2].z));
+  if (act2) fadd3(V, K_CB, i, f2 * d[2].x + f3 * d[3].x, f2 * d[2].y + f3 * d[3].y, f2 * d[2].z + f3 * d[3].z);
+  if (act1) fadd3(V, K_CB, j, -(f1 * d[1].x + f3 * d[3].x), -(f1 * d[1].y + f3 * d[3].y), -(f1 * d[1].z + f3 * d[3].z));
+}
+
+// ------------------------------------------------------------------------
+// switching functions
+
+// cWell::compute_theta_pair, sml.h:594-610 (theta and theta'/r)
+template <class T>
+AW_DEV void well_theta(T kappa, T rmin, T rmax, T r, T &th, T &prd);
+
+template <>
+AW_DEV void well_theta<double>(double kappa, double rmin, double rmax, double r, double &th, double &prd)
+{
+  double tmin = tanh(kappa * (r - rmin)), tmax = tanh(kappa * (rmax - r));
+  th = 0.25 * (1.0 + tmin) * (1.0 + tmax);
+  prd = kappa * th * (tmax - tmin) / r;
+}
+// FP32: the same function through exponentials, without the cancellation
+// of 1 + tanh(x) for x << 0: theta = 1/((1+e^-2a)(1+e^-2b)),
+// theta'/r = 2 kappa (e^-2a - e^-2b) / (D^2 r)
+template <>
+AW_DEV void well_theta<float>(float kappa, float rmin, float rmax, float r, float &th, float &prd)
+{
+  float ea = aw_fast_exp(-2.0f * kappa * (r - rmin)), eb = aw_fast_exp(-2.0f * kappa * (rmax - r));
+  float D = (1.0f + ea) * (1.0f + eb);
+  float iD = 1.0f / D;
+  th = iD;
+  prd = 2.0f * kappa * (ea - eb) * iD * iD / r;
+}
+
+template <class T>
+AW_DEV void wtheta(const FFParams *P, int w, T r, T &th, T &prd)
+{
+  well_theta<T>((T)P->water_kappa, (T)P->well_r_min[w], (T)P->well_r_max[w], r, th, prd);
+}
+template <class T>
+AW_DEV void htheta(const FFParams *P, T r, T &th, T &prd)
+{
+  well_theta<T>((T)P->helix_kappa, (T)P->helix_r_min, (T)P->helix_r_max, r, th, prd);
+}
+
+AW_DEV const double *site(const View<float> &V, int i, int ri) { return (ri & RI_GLY) ? XA(V, K_CA, i) : XA(V, K_CB, i); }
+AW_DEV const double *site(const View<double> &V, int i, int ri) { return (ri & RI_GLY) ? XA(V, K_CA, i) : XA(V, K_CB, i); }
+
+// cP_AP::compute (sml.h:176-203) in FP64: the nu < pap_delta early exit of
+// fb.cpp:3281 is a hard threshold
+template <class T>
+AW_DEV void pap_nu(const View<T> &V, int i, int j, double &nu, double &prd)
+{
+  const FFParams *P = V.P;
+  double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
+  if (drsq > P->pap_drmax_sq) { nu = 0.0; prd = 0.0; }
+  else if (drsq < P->pap_drmin_sq) { nu = 1.0; prd = 0.0; }
+  else {
+    double dr = sqrt(drsq);
+    double th = tanh(P->pap_pref * (P->pap_cut - dr));
+    nu = 0.5 * (1 + th);
+    prd = 0.5 * P->pap_pref * (1 - th * th) / dr;
+  }
+}
+
+// ------------------------------------------------------------------------
+// DSSP hydrogen bond (acceptor O of i, donor N-H of j), fb.cpp:2902-3277
+
+template <class T>
+AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
+{
+  const FFParams *P = V.P;
+  const int *RI = V.rinfo;
+  int ri = RI[i], rj = RI[j];
+  int ic = rchain(ri), jc = rchain(rj);
+  int i_ch_start = V.ch_start[ic] + 1, j_ch_start = V.ch_start[jc] + 1;          // 1-based like ch_pos
+  int i_ch_end = V.ch_start[ic] + V.ch_len[ic], j_ch_end = V.ch_start[jc] + V.ch_len[jc];
+  bool jlast = rj & RI_LAST;
+  bool i_rep = !(jlast || (RI[j + (jlast ? 0 : 1)] & RI_PRO));
+  bool i_AP = !((ri & RI_FIRST) || jlast || (ri & RI_PRO));
+  bool i_P = !(i >= i_ch_end - 2 || jlast || (RI[i + 2 < V.n ? i + 2 : i] & RI_PRO));
+  int ti = rtype(ri), tj = rtype(rj);
+  T tsa_HB[2] = {0, 0}, tsa_NHB[2] = {0, 0}, tsp_HB[2] = {0, 0};
+  for (int k = 0; k < 2; ++k) {
+    if (i_AP) {
+      tsa_HB[k] = (T)(0.5 * P->anti_HB[ti][tj][k]);
+      tsa_NHB[k] = (T)(0.25 * (P->anti_NHB[rtype(RI[i + 1])][rtype(RI[j - 1])][k] + P->anti_NHB[rtype(RI[i - 1])][rtype(RI[j + 1])][k]));
+    }
+    if (i_P) tsp_HB[k] = (T)P->para_HB[rtype(RI[i + 1])][tj][k];
+  }
+  T lambda[4] = {0, 0, 0, 0};
+  int hb_class;
+  int nr = P->n_rama_par;
+  int dij = j - i < 0 ? i - j : j - i;
+  const double(*h)[9] = P->hbscl;
+  if (ic == jc && dij < 45) {
+    if (dij < 4) {
+      lambda[0] = (T)-h[0][0]; lambda[1] = (T)-h[0][1];
+      hb_class = 1;
+    } else if (dij < 18) {
+      if (nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0) {
+        lambda[0] = (T)-h[1][0];
+        lambda[1] = (T)-h[1][1];
+        lambda[2] = (T)-h[1][2] - (T)h[1][3] * tsa_HB[0] - (T)h[1][4] * tsa_NHB[0] - (T)(h[1][5] * (P->anti_one[ti] + P->anti_one[tj]));
+        lambda[3] = (T)-h[1][6];
+      } else {
+        lambda[1] = (T)-h[1][1];
+      }
+      hb_class = 2;
+    } else {
+      lambda[0] = (T)-h[2][0];
+      lambda[1] = (T)-h[2][1];
+      lambda[2] = (T)-h[2][2] - (T)h[2][3] * tsa_HB[1] - (T)h[2][4] * tsa_NHB[1] - (T)(h[2][5] * (P->anti_one[ti] + P->anti_one[tj]));
+      lambda[3] = (T)-h[2][6] - (T)h[2][7] * tsp_HB[1] - (T)(h[2][8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
+      hb_class = 3;
+    }
+  } else {
+    lambda[0] = (T)-h[3][0];
+    lambda[1] = (T)-h[3][1];
+    lambda[2] = (T)-h[3][2] - (T)h[3][3] * tsa_HB[1] - (T)h[3][4] * tsa_NHB[1] - (T)(h[3][5] * (P->anti_one[ti] + P->anti_one[tj]));
+    lambda[3] = (T)-h[3][6] - (T)h[3][7] * tsp_HB[1] - (T)(h[3][8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
+    hb_class = 4;
+  }
+  // nu switches in FP64: the nu < 1e-4 early exit (fb.cpp:3052) is a hard threshold
+  double nu0 = 0.0, prdnu0 = 0.0, nu1 = 0.0, prdnu1 = 0.0;
+  bool nu0_on = (i - 2 >= i_ch_start - 1 && i + 2 < i_ch_end && hb_class != 2);
+  bool nu1_on = (j - 2 >= j_ch_start - 1 && j + 2 < j_ch_end && hb_class != 2);
+  if (nu0_on) {
+    double rr = rsq_exact(XA(V, K_CA, i + 2), XA(V, K_CA, i - 2));
+    if (rr > P->nu_cut1_sq) {
+      rr = sqrt(rr);
+      double th = tanh(P->pref[0] * (rr - P->d_nu0));
+      nu0 = 0.5 * (1.0 + th);
+      prdnu0 = P->pref[0] * nu0 * (1.0 - th) / rr;
+    }
+  } else nu0 = 1.0;
+  if (nu1_on) {
+    double rr = rsq_exact(XA(V, K_CA, j + 2), XA(V, K_CA, j - 2));
+    if (rr > P->nu_cut2_sq) {
+      rr = sqrt(rr);
+      double th = tanh(P->pref[1] * (rr - P->d_nu0));
+      nu1 = 0.5 * (1.0 + th);
+      prdnu1 = P->pref[1] * nu1 * (1.0 - th) / rr;
+    }
+  } else nu1 = 1.0;
+  if (nu0 < AW_DSSP_NU_DELTA || nu1 < AW_DSSP_NU_DELTA) return;
+
+  bool on[4] = {true, i_rep, i_AP, i_P};
+  int acc[4] = {i, i, j, j}, don[4] = {j, j + 1, i, i + 2};
+  T theta[4], p0[4], p1[4];
+  V3<T> xNO[4], xHO[4];
+  for (int k = 0; k < 4; ++k) {
+    theta[k] = 0; p0[k] = 0; p1[k] = 0;
+    if (!on[k]) continue;
+    xNO[k] = dvec<T>(XA(V, K_O, acc[k]), XV(V, K_N, don[k]));
+    xHO[k] = dvec<T>(XA(V, K_O, acc[k]), XV(V, K_H, don[k]));
+    T RNO = t_sqrt(dot(xNO[k], xNO[k])), RHO = t_sqrt(dot(xHO[k], xHO[k]));
+    T dNO = (RNO - (T)P->NO_zero) * (T)P->sigma_NO_sqinv;
+    T dHO = (RHO - (T)P->HO_zero) * (T)P->sigma_HO_sqinv;
+    theta[k] = t_exp((T)-0.5 * ((RNO - (T)P->NO_zero) * dNO + (RHO - (T)P->HO_zero) * dHO));
+    p0[k] = -dNO / RNO;
+    p1[k] = -dHO / RHO;
+  }
+  T ff1 = (T)P->k_dssp * theta[0];
+  T Vv[4] = {lambda[0], lambda[1] * theta[1], lambda[2] * theta[2], lambda[3] * theta[3]};
+  T theta_sum = ff1 * (Vv[0] + Vv[1] + Vv[2] + Vv[3]);
+  ff1 *= (T)(nu0 * nu1);
+  for (int k = 0; k < 4; ++k) Vv[k] *= ff1;
+  T VT = Vv[0] + Vv[1] + Vv[2] + Vv[3];
+  E.dssp += (double)VT;
+  if (nu0_on) {
+    V3<T> dn = dvec<T>(XA(V, K_CA, i + 2), XA(V, K_CA, i - 2));
+    T fq = theta_sum * (T)(prdnu0 * nu1);
+    fadd(V, K_CA, i - 2, fq, dn);
+    fadd(V, K_CA, i + 2, -fq, dn);
+  }
+  if (nu1_on) {
+    V3<T> dn = dvec<T>(XA(V, K_CA, j + 2), XA(V, K_CA, j - 2));
+    T fq = theta_sum * (T)(nu0 * prdnu1);
+    fadd(V, K_CA, j - 2, fq, dn);
+    fadd(V, K_CA, j + 2, -fq, dn);
+  }
+  T Vk[4] = {VT, Vv[1], Vv[2], Vv[3]};
+  T an = (T)P->an, bn = (T)P->bn, cn = (T)P->cn, ah = (T)P->ah, bh = (T)P->bh, ch = (T)P->ch;
+  for (int k = 0; k < 4; ++k) {
+    if (!on[k]) continue;
+    T q0 = p0[k] * Vk[k], q1 = p1[k] * Vk[k];
+    int a = acc[k], d = don[k];
+    const V3<T> &u = xNO[k], &w = xHO[k];
+    fadd3(V, K_O, a, -(q0 * u.x + q1 * w.x), -(q0 * u.y + q1 * w.y), -(q0 * u.z + q1 * w.z));
+    fadd3(V, K_CA, d - 1, an * q0 * u.x + ah * q1 * w.x, an * q0 * u.y + ah * q1 * w.y, an * q0 * u.z + ah * q1 * w.z);
+    fadd3(V, K_CA, d, bn * q0 * u.x + bh * q1 * w.x, bn * q0 * u.y + bh * q1 * w.y, bn * q0 * u.z + bh * q1 * w.z);
+    fadd3(V, K_O, d - 1, cn * q0 * u.x + ch * q1 * w.x, cn * q0 * u.y + ch * q1 * w.y, cn * q0 * u.z + ch * q1 * w.z);
+  }
+}
+
+// ------------------------------------------------------------------------
+// P_AP (i < j by residue index), fb.cpp:3279-3383
+
+template <class T>
+AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
+{
+  const FFParams *P = V.P;
+  double nu_ij, prd_ij;
+  pap_nu(V, i, j, nu_ij, prd_ij);
+  if (nu_ij < AW_PAP_DELTA) return;
+  const int *RI = V.rinfo;
+  int n = V.n, d = P->i_diff_pap;
+  int ic = rchain(RI[i]), jc = rchain(RI[j]);
+  int i_ch_end = V.ch_start[ic] + V.ch_len[ic], j_ch_end = V.ch_start[jc] + V.ch_len[jc];
+  int mn = P->i_med_min, mx = P->i_med_max;
+  int jhi = (i + mx + 2 * d) < (i_ch_end - 1) ? (i + mx + 2 * d) : (i_ch_end - 1);
+  bool med = ic == jc && i < i_ch_end - (mn + 2 * d) && j >= i + (mn + 2 * d) && j <= jhi;
+  bool lng = (ic == jc && i < i_ch_end - (mx + 2 * d + 1) && j >= i + (mx + 2 * d + 1) && j < j_ch_end) ||
+             (ic != jc && i + d < n && j - d >= 0 && rchain(RI[i + d]) == ic && rchain(RI[j - d]) == jc);
+  bool par = (ic == jc && i < i_ch_end - (mx + 1 + d) && j >= i + (mx + 1) && j < i_ch_end - d) ||
+             (ic != jc && i + d < n && j + d < n && rchain(RI[i + d]) == ic && rchain(RI[j + d]) == jc);
+  int nr = P->n_rama_par;
+  bool beta = nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0;
+  if (med || lng) {
+    double K = beta ? (med ? P->k_pap[0] : 0.0) + (lng ? P->k_pap[1] * P->k_betapred_pap : 0.0)
+                    : (med ? P->k_pap[0] : 0.0) + (lng ? P->k_pap[1] : 0.0);
+    double nu2, prd2;
+    pap_nu(V, i + d, j - d, nu2, prd2);
+    double kk = P->k_global_pap * K;
+    E.pap += -kk * nu_ij * nu2;
+    V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j - d));
+    T f0 = (T)(kk * prd_ij * nu2), f1 = (T)(kk * nu_ij * prd2);
+    fadd(V, K_CA, i, -f0, d0); fadd(V, K_CA, j, f0, d0);
+    fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j - d, f1, d1);
+  }
+  if (par) {
+    double K = beta ? P->k_pap[2] * P->k_betapred_pap : P->k_pap[2];
+    double nu2, prd2;
+    pap_nu(V, i + d, j + d, nu2, prd2);
+    double kk = P->k_global_pap * K;
+    E.pap += -kk * nu_ij * nu2;
+    V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j + d));
+    T f0 = (T)(kk * prd_ij * nu2), f1 = (T)(kk * nu_ij * prd2);
+    fadd(V, K_CA, i, -f0, d0); fadd(V, K_CA, j, f0, d0);
+    fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j + d, f1, d1);
+  }
+}
+
+// ------------------------------------------------------------------------
+// excluded volume pair style, pev.cpp:50-138 (1-2 bonded pairs removed by
+// special_bonds fene: CA(i)-CA(i+1), CA(i)-O(i), O(i)-CA(i+1), CA(i)-CB(i))
+
+AW_DEV int atype(int kind, int ri) { return kind == K_CA ? 1 : (kind == K_O ? 3 : ((ri & RI_GLY) ? 5 : 4)); }
+
+template <class T>
+AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
+{
+  const FFParams *P = V.P;
+  bool same = rchain(ri) == rchain(rj);
+  bool nb = same && (j == i + 1);
+  bool shortc = same && (j - i < 5);
+  for (int ka = 0; ka < 3; ++ka)
+    for (int kb = 0; kb < 3; ++kb) {
+      if (nb && kb == K_CA && (ka == K_CA || ka == K_O)) continue;
+      int ta = atype(ka, ri), tb = atype(kb, rj);
+      double lam = P->ev_lambda[ta][tb];
+      if (lam == 0.0) continue;
+      double rc = shortc ? P->ev_cut_short[ta][tb] : P->ev_cut_long[ta][tb];
+      double rsq = rsq_exact(XA(V, ka, i), XA(V, kb, j));
+      double r = sqrt(rsq);
+      if (!(r < rc)) continue;
+      V3<T> d = dvec<T>(XA(V, ka, i), XA(V, kb, j));
+      T rr = (T)r, dr = rr - (T)rc;
+      int p = P->ev_p;
+      T pw = (T)1;
+      for (int q = 0; q < p - 1; ++q) pw *= dr;
+      T pref = (T)((p % 2 == 0) ? lam : -lam) * pw;
+      T fp = rr > (T)0 ? -pref * (T)p / rr : (T)0;
+      fadd(V, ka, i, fp, d);
+      fadd(V, kb, j, -fp, d);
+      E.ev += (double)(pref * dr);
+    }
+}
+
+// same-residue excluded volume (only CB-O can be unbonded within a residue)
+template <class T>
+AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E)
+{
+  const FFParams *P = V.P;
+  int ta = atype(K_CB, ri), tb = 3;
+  double lam = P->ev_lambda[ta][tb];
+  if (lam == 0.0) return;
+  double rc = P->ev_cut_short[ta][tb];
+  double r = sqrt(rsq_exact(XA(V, K_CB, i), XA(V, K_O, i)));
+  if (!(r < rc)) return;
+  V3<T> d = dvec<T>(XA(V, K_CB, i), XA(V, K_O, i));
+  T rr = (T)r, dr = rr - (T)rc;
+  int p = P->ev_p;
+  T pw = (T)1;
+  for (int q = 0; q < p - 1; ++q) pw *= dr;
+  T pref = (T)((p % 2 == 0) ? lam : -lam) * pw;
+  T fp = rr > (T)0 ? -pref * (T)p / rr : (T)0;
+  fadd(V, K_CB, i, fp, d);
+  fadd(V, K_O, i, -fp, d);
+  E.ev += (double)(pref * dr);
+}
+
+// conservative pair prefilter on CA-CA distance: every atom of residue i
+// lies within ext[i] of CA(i)
+template <class T>
+AW_DEV bool near_pair(const View<T> &V, int i, int j, float reach)
+{
+  const double *a = XA(V, K_CA, i), *b = XA(V, K_CA, j);
+  float dx = (float)(a[0] - b[0]), dy = (float)(a[1] - b[1]), dz = (float)(a[2] - b[2]);
+  float lim = reach + V.ext[i] + V.ext[j] + 1e-3f;
+  return dx * dx + dy * dy + dz * dz <= lim * lim * 1.0001f;
+}
+
+// ------------------------------------------------------------------------
+// Phase B
+
+template <class T>
+AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
+{
+  const FFParams *P = V.P;
+  int ri = V.rinfo[i];
+  bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
+  if (P->chain_flag && ON(P, AW_ET_CHAIN)) chain_term(V, i, ri, E);
+  if (!first && !last && P->chi_flag && !gly && ON(P, AW_ET_CHI)) chi_term(V, i, E);
+  if (P->shake_flag && ON(P, AW_ET_SHAKE)) shake_term(V, i, ri, E);
+  if (!first && !last && P->rama_flag && !gly && ON(P, AW_ET_RAMA)) rama_term(V, i, ri, E);
+  if (P->bond_on && ON(P, AW_ET_BOND)) bonds_term(V, i, ri, E);
+  if (P->ev_on && ON(P, AW_ET_VEXCLUDED)) ev_self(V, i, ri, E);
+}
+
+template <class T>
+AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E)
+{
+  const FFParams *P = V.P;
+  if (!near_pair(V, i, j, (float)P->pair_reach)) return;
+  int ri = V.rinfo[i], rj = V.rinfo[j];
+  bool same = rchain(ri) == rchain(rj);
+  int dres = j - i;
+  // DL1 densities (fb.cpp:7729-7786)
+  if (!same || dres > 1) {
+    double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
+    if (P->water_flag && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
+      T th, prd;
+      wtheta<T>(P, 0, (T)sqrt(rsq), th, prd);
+      aadd(&V.rho_w[i], (double)th);
+      aadd(&V.rho_w[j], (double)th);
+    }
+    if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
+      T th, prd;
+      htheta<T>(P, (T)sqrt(rsq), th, prd);
+      aadd(&V.rho_h[i], (double)th);
+      aadd(&V.rho_h[j], (double)th);
+    }
+  }
+  // DSSP in both orientations (fb.cpp:8064-8094)
+  if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2)) {
+    if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST)) {
+      double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
+      if (r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut)) dssp_term(V, i, j, E);
+    }
+    if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST)) {
+      double r2 = rsq_exact(XA(V, K_O, j), XV(V, K_N, i));
+      if (r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut)) dssp_term(V, j, i, E);
+    }
+  }
+  // P_AP (fb.cpp:8096-8103)
+  if (P->pap_flag && ON(P, AW_ET_PAP)) {
+    if (rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j)) < P->pap_cutoff_sq) pap_term(V, i, j, E);
+  }
+  if (P->ev_on && ON(P, AW_ET_VEXCLUDED)) ev_pair(V, i, j, ri, rj, E);
+}
+
+template <class T>
+AW_DEV void phase_B(const View<T> &V, int t0, int ts, EAcc &E)
+{
+  const FFParams *P = V.P;
+  int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM)) ? V.fm_items : 0;
+  int n = V.n;
+  int np = pair_count(n);
+  int total = nfm + n + np;
+  for (int k = t0; k < total; k += ts) {
+    if (k < nfm) fm_item(V, V.fm_item_base + k, E);
+    else if (k < nfm + n) residue_local(V, k - nfm, E);
+    else {
+      int i, j;
+      pair_decode(k - nfm - n, n, i, j);
+      pair_B(V, i, j, E);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Phase C: sigma_h and burial (SL, fb.cpp:7795-7841)
+
+template <class T>
+AW_DEV void phase_C(const View<T> &V, int t0, int ts, EAcc &E)
+{
+  const FFParams *P = V.P;
+  for (int i = t0; i < V.n; i += ts) {
+    int it = rtype(V.rinfo[i]);
+    if (P->water_flag) {
+      double th = tanh(P->water_kappa_sigma * (V.rho_w[i] - P->treshold));
+      double s = 0.5 * (1.0 - th);
+      V.sig_w[i] = s;
+      V.sigp_w[i] = -P->water_kappa_sigma * s * (1.0 + th);
+    }
+    if (P->helix_flag) {
+      double th = tanh(P->helix_kappa_sigma * (V.rho_h[i] - P->helix_treshold));
+      double s = 0.5 * (1.0 - th);
+      V.sig_h[i] = s;
+      V.sigp_h[i] = -P->helix_kappa_sigma * s * (1.0 + th);
+    }
+    if (P->burial_flag) {
+      double rho = V.rho_w[i], kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
+      for (int b = 0; b < 3; ++b) {
+        double t0_ = tanh(kb * (rho - P->burial_ro_min[b])), t1_ = tanh(kb * (P->burial_ro_max[b] - rho));
+        eb += -0.5 * P->k_burial * P->burial_gamma[it][b] * (t0_ + t1_);
+        bfv += P->burial_gamma[it][b] * (t1_ * t1_ - t0_ * t0_);
+      }
+      V.bf[i] = bfv * 0.5 * P->k_burial * kb;
+      if (ON(P, AW_ET_BURIAL)) E.burial += eb;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Phase D: helix pairs (fb.cpp:7843-7868, 3706-3785) and water xi (DL2)
+
+template <class T>
+AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
+{
+  const FFParams *P = V.P;
+  int j = i + P->helix_i_diff;
+  if (j >= V.n) return;
+  int ri = V.rinfo[i], rj = V.rinfo[j];
+  if (rchain(ri) != rchain(rj)) return;
+  double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
+  if (!(r2 < P->helix_cutoff_sq)) return;
+  if (sqrt(r2) > P->helix_cutoff) return;
+  V3<T> xNO = dvec<T>(XA(V, K_O, i), XV(V, K_N, j)), xHO = dvec<T>(XA(V, K_O, i), XV(V, K_H, j));
+  T RNO = t_sqrt(dot(xNO, xNO)), RHO = t_sqrt(dot(xHO, xHO));
+  int it = rtype(ri), jt = rtype(rj);
+  double h4i = ((ri & RI_PRO) && P->pro_accepter_flag) ? P->h4prob_pro_accepter : P->h4prob[it];
+  T prob_sum = (T)(h4i + P->h4prob[jt]);
+  T dNO = RNO - (T)P->helix_NO_zero, dHO = RHO - (T)P->helix_HO_zero;
+  T dNOs = dNO * (T)P->helix_sigma_NO_sqinv, dHOs = dHO * (T)P->helix_sigma_HO_sqinv;
+  T pt = -(T)P->k_helix * prob_sum * t_exp((T)-0.5 * (dNO * dNOs + dHO * dHOs));
+  T p0 = -dNOs / RNO, p1 = -dHOs / RHO;
+  double gdiff = P->helix_gamma_w - P->helix_gamma_p;
+  double sg = P->helix_gamma_p + gdiff * V.sig_h[i] * V.sig_h[j];
+  V.hxi1[i] = gdiff * (double)pt * V.sigp_h[i] * V.sig_h[j];
+  V.hxi2[i] = gdiff * (double)pt * V.sig_h[i] * V.sigp_h[j];
+  if (!ON(P, AW_ET_HELIX)) return;
+  T Vv = (T)sg * pt;
+  E.helix += (double)Vv;
+  T an = (T)P->an, bn = (T)P->bn, cn = (T)P->cn, ah = (T)P->ah, bh = (T)P->bh, ch = (T)P->ch;
+  T q0 = Vv * p0, q1 = Vv * p1;
+  fadd3(V, K_CA, j - 1, an * q0 * xNO.x + ah * q1 * xHO.x, an * q0 * xNO.y + ah * q1 * xHO.y, an * q0 * xNO.z + ah * q1 * xHO.z);
+  fadd3(V, K_CA, j, bn * q0 * xNO.x + bh * q1 * xHO.x, bn * q0 * xNO.y + bh * q1 * xHO.y, bn * q0 * xNO.z + bh * q1 * xHO.z);
+  fadd3(V, K_O, j - 1, cn * q0 * xNO.x + ch * q1 * xHO.x, cn * q0 * xNO.y + ch * q1 * xHO.y, cn * q0 * xNO.z + ch * q1 * xHO.z);
+  fadd3(V, K_O, i, -(q0 * xNO.x + q1 * xHO.x), -(q0 * xNO.y + q1 * xHO.y), -(q0 * xNO.z + q1 * xHO.z));
+}
+
+template <class T>
+AW_DEV void xi_pair(const View<T> &V, int i, int j)
+{
+  const FFParams *P = V.P;
+  int ri = V.rinfo[i], rj = V.rinfo[j];
+  if (!(rchain(ri) != rchain(rj) || j - i >= P->contact_cutoff)) return;
+  if (!near_pair(V, i, j, (float)P->dens_reach)) return;
+  double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
+  int it = rtype(ri), jt = rtype(rj);
+  double acc_i = 0.0, acc_j = 0.0;
+  for (int w = 0; w < P->n_wells; ++w) {
+    if (!P->well_flag[w]) continue;
+    double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1];
+    // fb.cpp:7925 reads !fabs(g0-g1)<delta, i.e. (!fabs(g0-g1)) < delta: true iff g0 != g1
+    if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
+      T th, prd;
+      wtheta<T>(P, w, (T)sqrt(rsq), th, prd);
+      double tg = (g1 - g0) * (double)th;
+      acc_i += tg * V.sig_w[j];
+      acc_j += tg * V.sig_w[i];
+    }
+  }
+  if (acc_i != 0.0) aadd(&V.xi_w[i], acc_i);
+  if (acc_j != 0.0) aadd(&V.xi_w[j], acc_j);
+}
+
+template <class T>
+AW_DEV void phase_D(const View<T> &V, int t0, int ts, EAcc &E)
+{
+  const FFParams *P = V.P;
+  int n = V.n;
+  int nh = P->helix_flag ? n : 0;
+  int np = P->water_flag ? pair_count(n) : 0;
+  for (int k = t0; k < nh + np; k += ts) {
+    if (k < nh) helix_item(V, k, E);
+    else {
+      int i, j;
+      pair_decode(k - nh, n, i, j);
+      xi_pair(V, i, j);
+    }
+  }
+}
+
+template <class T>
+AW_DEV void phase_E(const View<T> &V, int t0, int ts)
+{
+  for (int i = t0; i < V.n; i += ts) V.xi_w[i] *= V.sigp_w[i];
+}
+
+// ------------------------------------------------------------------------
+// Phase F: DL3 density-site pair forces (fb.cpp:7996-8057, scatter 8116-8128)
+
+template <class T>
+AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
+{
+  const FFParams *P = V.P;
+  int ri = V.rinfo[i], rj = V.rinfo[j];
+  bool same = rchain(ri) == rchain(rj);
+  int dres = j - i;
+  if (!(!same || dres > 1)) return;
+  if (!near_pair(V, i, j, (float)P->dens_reach)) return;
+  const double *si = site(V, i, ri), *sj = site(V, j, rj);
+  double rsq = rsq_exact(si, sj);
+  T r = (T)sqrt(rsq);
+  T force = 0;
+  bool in0 = rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0];
+  if (P->water_flag && ON(P, AW_ET_WATER)) {
+    int it = rtype(ri), jt = rtype(rj);
+    double ew = 0.0;
+    for (int w = 0; w < P->n_wells; ++w) {
+      if (!P->well_flag[w]) continue;
+      if ((!same || dres >= P->contact_cutoff) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
+        double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1], sg;
+        if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
+        else sg = g0 + (g1 - g0) * V.sig_w[i] * V.sig_w[j];
+        T th, prd;
+        wtheta<T>(P, w, r, th, prd);
+        ew += -sg * (double)th;
+        force += (T)sg * prd;
+      }
+    }
+    E.water += ew;
+    if (in0) {
+      double xi = 0.0;
+      if (fabs(V.xi_w[i]) > AW_DELTA_WATER_XI) xi += V.xi_w[i];
+      if (fabs(V.xi_w[j]) > AW_DELTA_WATER_XI) xi += V.xi_w[j];
+      if (xi != 0.0) {
+        T th, prd;
+        wtheta<T>(P, 0, r, th, prd);
+        force += prd * (T)xi;
+      }
+    }
+  }
+  if (P->burial_flag && in0 && ON(P, AW_ET_BURIAL)) {
+    T th, prd;
+    wtheta<T>(P, 0, r, th, prd);
+    force += (T)(V.bf[i] + V.bf[j]) * prd;
+  }
+  if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq && ON(P, AW_ET_HELIX)) {
+    int hd = P->helix_i_diff;
+    double factor = V.hxi1[i] + V.hxi1[j];
+    if (i - hd >= 0) factor += V.hxi2[i - hd];
+    if (j - hd >= 0) factor += V.hxi2[j - hd];
+    if (fabs(factor) > AW_DELTA_HELIX_XI) {
+      T th, prd;
+      htheta<T>(P, r, th, prd);
+      force += -(T)factor * prd;
+    }
+  }
+  if (force != (T)0) {
+    V3<T> d = dvec<T>(si, sj);
+    int ki = (ri & RI_GLY) ? K_CA : K_CB, kj = (rj & RI_GLY) ? K_CA : K_CB;
+    fadd(V, ki, i, force, d);
+    fadd(V, kj, j, -force, d);
+  }
+}
+
+template <class T>
+AW_DEV void phase_F(const View<T> &V, int t0, int ts, EAcc &E)
+{
+  int n = V.n;
+  int np = pair_count(n);
+  for (int k = t0; k < np; k += ts) {
+    int i, j;
+    pair_decode(k, n, i, j);
+    force_pair(V, i, j, E);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Langevin dynamics (LAMMPS fix nve + fix langevin, real units)
+
+#define AW_FTM2V (1.0 / 48.88821291 / 48.88821291)
+#define AW_MVV2E (48.88821291 * 48.88821291)
+#define AW_BOLTZ 0.0019872067
+
+AW_DEV uint64_t mix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+// counter hash (seed, replica, step, atom, component) -> U[0,1)
+AW_DEV double urand(uint64_t seed, int replica, long long step, int atom, int comp)
+{
+  uint64_t h = mix64(seed ^ 0x9e3779b97f4a7c15ULL);
+  h = mix64(h ^ ((uint64_t)(uint32_t)replica << 32 | (uint32_t)comp));
+  h = mix64(h ^ (uint64_t)step);
+  h = mix64(h ^ (uint64_t)(uint32_t)atom);
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+AW_DEV double atom_mass(int kind, int ri) { return kind == K_CA ? 27.0 : (kind == K_O ? 28.0 : ((ri & RI_GLY) ? 2.0 : 60.0)); }
+
+struct LgParams {
+  double dt, t_start, t_stop, damp;
+  unsigned long long seed;
+  long long step0;
+  int nsteps;
+};
+
+// initial_integrate of fix nve
+template <class T>
+AW_DEV void integrate1(const View<T> &V, double *v, const LgParams &lg, int t0, int ts)
+{
+  int n = V.n;
+  double dtf = 0.5 * lg.dt * AW_FTM2V;
+  for (int a = t0; a < 3 * n; a += ts) {
+    int kind = a / n, i = a - kind * n;
+    double dtfm = dtf / atom_mass(kind, V.rinfo[i]);
+    for (int l = 0; l < 3; ++l) {
+      double vv = v[3 * a + l] + dtfm * V.f[3 * a + l];
+      v[3 * a + l] = vv;
+      V.x[3 * a + l] += lg.dt * vv;
+    }
+  }
+}
+
+// fix langevin post_force + fix nve final_integrate
+template <class T>
+AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, int s, int replica, int t0, int ts)
+{
+  int n = V.n;
+  double dtf = 0.5 * lg.dt * AW_FTM2V;
+  double delta = lg.nsteps > 0 ? (double)(s + 1) / (double)lg.nsteps : 0.0;
+  double Tt = lg.t_start + delta * (lg.t_stop - lg.t_start);
+  double tsq = sqrt(Tt);
+  long long step = lg.step0 + s;
+  for (int a = t0; a < 3 * n; a += ts) {
+    int kind = a / n, i = a - kind * n;
+    double m = atom_mass(kind, V.rinfo[i]);
+    double g1 = -m / lg.damp / AW_FTM2V;
+    double g2 = sqrt(m) * sqrt(24.0 * AW_BOLTZ / lg.damp / lg.dt / AW_MVV2E) / AW_FTM2V * tsq;
+    double dtfm = dtf / m;
+    for (int l = 0; l < 3; ++l) {
+      double fv = V.f[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand(lg.seed, replica, step, a, l) - 0.5);
+      V.f[3 * a + l] = fv;
+      v[3 * a + l] += dtfm * fv;
+    }
+  }
+}
+

@@ -1,0 +1,106 @@
+// awsem_types.h -- plain-data structures shared by the host code and the
+// gfx950 kernels of the FixBackbone force path.
+#pragma once
+#include <stdint.h>
+
+#define AW_N_ET 19
+#define AW_MAXCH 1000
+
+// energy slots (FixBackbone::EnergyTerms, src/fix_backbone.h:331-332) + bonds
+enum {
+  AW_ET_TOTAL = 0, AW_ET_CHAIN, AW_ET_SHAKE, AW_ET_CHI, AW_ET_RAMA, AW_ET_VEXCLUDED,
+  AW_ET_DSSP, AW_ET_PAP, AW_ET_WATER, AW_ET_BURIAL, AW_ET_HELIX, AW_ET_AMHGO,
+  AW_ET_FRAGMEM, AW_ET_VFRAGMEM, AW_ET_CONT_REST, AW_ET_MEMB, AW_ET_SSB, AW_ET_DH,
+  AW_ET_BOND
+};
+
+// per-residue info word
+#define RI_TYPE_MASK 31
+#define RI_FIRST 32
+#define RI_LAST 64
+#define RI_GLY 128
+#define RI_PRO 256
+#define RI_CHAIN_SHIFT 16
+
+// Force-field parameters after the constructor's parsing and epsilon
+// scaling (src/fix_backbone.cpp:165-516, src/smart_matrix_lib.h:363-368,
+// 111-115).  Lives in device global memory; kernels read it through a
+// pointer (it is ~55 KB, too large for kernel arguments).
+struct FFParams {
+  int chain_flag, shake_flag, chi_flag, rama_flag, rama_p_flag, ssweight_flag;
+  int dssp_flag, pap_flag, water_flag, burial_flag, helix_flag, fm_flag;
+  double epsilon;
+  double an, bn, cn, ap, bp, cp, ah, bh, ch;
+  double k_chain[3], r_ncb0, r_cpcb0, r_ncp0;
+  double k_shake, r_sh1, r_sh2, r_sh3;
+  double k_chi, chi0, k_rama, k_water;
+  int n_rama_par, n_rama_p_par;
+  int ssweight[12];
+  double w[12], phiw[12], phi0[12], psiw[12], psi0[12];
+  // DSSP hydrogen bonds
+  double k_dssp, hbscl[4][9], NO_zero, HO_zero, sigma_NO_sqinv, sigma_HO_sqinv;
+  double dssp_cut, dssp_cut_sq, pref[2], d_nu0, nu_cut1_sq, nu_cut2_sq;
+  double anti_HB[20][20][2], anti_NHB[20][20][2], para_HB[20][20][2];
+  double para_one[20], anti_one[20];
+  // P_AP
+  double k_global_pap, k_betapred_pap, k_pap[3], pap_cut, pap_pref, pap_cutoff_sq;
+  double pap_drmax_sq, pap_drmin_sq;
+  int i_med_min, i_med_max, i_diff_pap;
+  // water
+  int contact_cutoff, n_wells, well_flag[5];
+  double water_kappa, water_kappa_sigma, treshold;
+  double well_r_min[5], well_r_max[5], w_rmin_th[5], w_rmax_th[5], w_rmin_th_sq[5], w_rmax_th_sq[5];
+  double water_gamma[5][20][20][2];
+  // burial
+  double k_burial, burial_kappa, burial_ro_min[3], burial_ro_max[3], burial_gamma[20][3];
+  // helix
+  int helix_i_diff, pro_accepter_flag;
+  double k_helix, helix_gamma_p, helix_gamma_w, helix_kappa, helix_kappa_sigma, helix_treshold;
+  double helix_cutoff, helix_cutoff_sq, helix_r_min, helix_r_max;
+  double h_rmin_th, h_rmax_th, h_rmin_th_sq, h_rmax_th_sq;
+  double h4prob[20], h4prob_pro_accepter;
+  double helix_NO_zero, helix_HO_zero, helix_sigma_NO_sqinv, helix_sigma_HO_sqinv;
+  // fragment memory
+  double k_frag_mem, fm_sigma_exp;
+  int fm_min_sep, fm_max_sep;
+  // excluded volume pair style (src/pair_excluded_volume.cpp)
+  int ev_on, ev_p;
+  double ev_lambda[6][6], ev_cut_short[6][6], ev_cut_long[6][6];
+  // harmonic bonds of the deck
+  int bond_on;
+  double bond_k, bond_r0[5];
+  // derived cutoffs
+  double pair_list_cutoff;   // src/fix_backbone.cpp:1048-1066
+  double pair_reach;         // largest pair cutoff of any term (density, DSSP, P_AP, EV)
+  double dens_reach;         // largest density-well outer cutoff (water/helix)
+  uint32_t term_mask;
+};
+
+// One group of fragment memories sharing (pos, len): every memory of the
+// group covers the same (i, j) residue pairs, so one work item = one pair
+// of one group, looping over the group's memories.
+struct FMGroup {
+  int pos, len, npairs, nmem;
+  int pair_off;   // into the (di, dj) pair table
+  int item_off;   // first work item of this group (batch-global item index)
+  int mem_off;    // first memory (global index) of this group
+  int pad;
+  long long rf_off;  // first (memory, pair) rf entry
+};
+
+// per-item constants: gamma and the Gaussian widths 1/(2 sigma^2), 1/sigma^2
+struct FMItem {
+  float gamma, inv2s, invs, pad;
+};
+struct FMItemD {
+  double gamma, inv2s, invs, pad;
+};
+
+struct ReplicaDesc {
+  int n;          // residues
+  int nch;        // chains
+  int res_off;    // first residue in the concatenated per-residue arrays
+  int ch_off;     // first chain in the chain tables
+  int fm_group_off, fm_ngroups, fm_items, fm_item_base;  // FM item_off values are batch-global
+  long long scratch_off;  // doubles, global-memory scratch (when not LDS resident)
+};

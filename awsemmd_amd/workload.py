@@ -1,0 +1,288 @@
+"""Seeded synthetic AWSEM-MD workloads and their reference-format file sets.
+
+A system is one or more protein chains with:
+  * a sequence drawn uniformly from the 20 amino acids;
+  * a C-alpha trace that is a self-avoiding random walk with 3.8 A bonds;
+  * CB (H-beta for glycine) and O placed by ideal virtual-bond geometry
+    (the harmonic-bond lengths of the reference deck, CA-O 2.41 A, O-CA(i+1)
+    2.50 A, CA-CB 1.54 A: tools/create_project_tools/CoordinatesToWorkLammpsDataFile.py:289-294);
+  * an ssweight column pair per residue drawn from {(1,0),(0,1),(0,0)} so the
+    rama / DSSP / P_AP ssweight branches are all exercised;
+  * a fragment-memory library: n_struct copies of the trace perturbed by
+    Gaussian noise (sigma 1 A) written as .gro files, and one memory per copy
+    at every 9-residue window start (the .mem format of
+    src/fix_backbone.cpp:1966-2042).
+
+`write_system` writes the exact file set the reference's FixBackbone
+constructor reads (coeff, sequence, ssweight, gamma.dat, burial_gamma.dat,
+DSSP tables, fragment-memory gamma, .mem, .gro).
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import shutil
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+AA1 = "ARNDCQEGHILKMFPSTWYV"
+AA3 = {"A": "ALA", "R": "ARG", "N": "ASN", "D": "ASP", "C": "CYS", "Q": "GLN", "E": "GLU",
+       "G": "GLY", "H": "HIS", "I": "ILE", "L": "LEU", "K": "LYS", "M": "MET", "F": "PHE",
+       "P": "PRO", "S": "SER", "T": "THR", "W": "TRP", "Y": "TYR", "V": "VAL"}
+
+# virtual-atom coefficients of the shipped coeff file [ABC] section
+ABC = dict(an=0.483, bn=0.703, cn=-0.186, ap=0.444, bp=0.235, cp=0.321)
+PARAM_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "params")
+
+
+@dataclasses.dataclass
+class Memory:
+    """One fragment memory: target window [pos, pos+length) taken from
+    structure `struct` (0-based), residues fpos.. of that structure."""
+    struct: int
+    pos: int
+    fpos: int
+    length: int
+    weight: float = 1.0
+
+
+@dataclasses.dataclass
+class System:
+    chains: List[str]                    # one sequence string per chain
+    xca: np.ndarray                      # (n, 3) float64, Angstrom
+    xcb: np.ndarray
+    xo: np.ndarray
+    ssw: np.ndarray                      # (n, 2) float64: ssweight columns 4 and 5 of [SSWeight]
+    mem_structs: List[np.ndarray]        # per structure (n, 2, 3) CA/CB in nm, rounded to 1e-3
+    memories: List[Memory]
+    seed: int = 0
+
+    @property
+    def n(self) -> int:
+        return len(self.seq)
+
+    @property
+    def seq(self) -> str:
+        return "".join(self.chains)
+
+    def chain_index(self) -> np.ndarray:
+        return np.concatenate([np.full(len(c), k, dtype=np.int32) for k, c in enumerate(self.chains)])
+
+    def coords(self) -> np.ndarray:
+        """(3, n, 3) array [kind][residue][xyz] with kind 0 CA, 1 CB, 2 O."""
+        return np.stack([self.xca, self.xcb, self.xo]).astype(np.float64)
+
+
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+def _rand_unit(rng):
+    while True:
+        v = rng.normal(size=3)
+        nv = np.linalg.norm(v)
+        if nv > 1e-8:
+            return v / nv
+
+
+def saw_trace(n: int, rng, start=None, bond=3.8, excl=4.0, excl13=5.0, others: Optional[np.ndarray] = None,
+              center=None, radius=None) -> np.ndarray:
+    """Self-avoiding C-alpha walk: |CA_i - CA_j| >= excl for |i-j|>=3 (and
+    against `others`), |CA_i - CA_{i-2}| >= excl13, optional confinement."""
+    pts = np.zeros((n, 3))
+    pts[0] = np.zeros(3) if start is None else np.asarray(start, dtype=float)
+    i = 1
+    fails = 0
+    while i < n:
+        ok = False
+        for _ in range(200):
+            cand = pts[i - 1] + bond * _rand_unit(rng)
+            if radius is not None and np.linalg.norm(cand - center) > radius:
+                continue
+            if i >= 2 and np.linalg.norm(cand - pts[i - 2]) < excl13:
+                continue
+            if i >= 3 and np.min(np.linalg.norm(pts[: i - 2] - cand, axis=1)) < excl:
+                continue
+            if others is not None and len(others) and np.min(np.linalg.norm(others - cand, axis=1)) < excl:
+                continue
+            ok = True
+            break
+        if ok:
+            pts[i] = cand
+            i += 1
+            fails = 0
+        else:
+            fails += 1
+            i = max(1, i - 1 - min(fails, 8))  # backtrack
+    return pts
+
+
+def place_side_atoms(xca: np.ndarray, seq: str, chi0=-0.83):
+    """Ideal O and CB placement for one chain's CA trace."""
+    n = len(xca)
+    xo = np.zeros_like(xca)
+    for i in range(n):
+        if i < n - 1:
+            e = xca[i + 1] - xca[i]
+            d = np.linalg.norm(e)
+            e = e / d
+            a = (2.41 ** 2 - 2.50 ** 2 + d * d) / (2 * d)
+            h = np.sqrt(max(2.41 ** 2 - a * a, 0.25))
+            ref = (xca[i - 1] - xca[i]) if i > 0 else np.array([0.3, 1.0, -0.2])
+            q = ref - np.dot(ref, e) * e
+            if np.linalg.norm(q) < 1e-6:
+                q = np.cross(e, [1.0, 0.0, 0.0])
+            q = -_unit(q)
+            xo[i] = xca[i] + a * e + h * q
+        else:
+            e = _unit(xca[i] - xca[i - 1]) if n > 1 else np.array([1.0, 0.0, 0.0])
+            perp = _unit(np.cross(e, [0.2, 1.0, 0.3]))
+            xo[i] = xca[i] + 2.41 * _unit(0.5 * e + perp)
+    xcb = np.zeros_like(xca)
+    for i in range(n):
+        if i > 0:
+            pn = ABC["an"] * xca[i - 1] + ABC["bn"] * xca[i] + ABC["cn"] * xo[i - 1]
+        else:
+            pn = xca[i] + 1.46 * _unit(xca[i] - xca[min(i + 1, n - 1)] + np.array([1e-3, 0, 0]))
+        if i < n - 1:
+            pc = ABC["ap"] * xca[i] + ABC["bp"] * xca[i + 1] + ABC["cp"] * xo[i]
+        else:
+            pc = xca[i] + 1.52 * _unit(xca[i] - xca[max(i - 1, 0)] + np.array([0, 1e-3, 0]))
+        b1 = _unit(pn - xca[i])
+        b2 = _unit(pc - xca[i])
+        bis = b1 + b2
+        bis = _unit(bis) if np.linalg.norm(bis) > 1e-6 else _unit(np.cross(b1, [0.0, 0.0, 1.0]))
+        nrm = np.cross(b1, b2)
+        nrm = _unit(nrm) if np.linalg.norm(nrm) > 1e-6 else _unit(np.cross(bis, [0.0, 1.0, 0.0]))
+        best = None
+        for s in (1.0, -1.0):
+            th = np.deg2rad(54.75)
+            cb = xca[i] + 1.54 * _unit(-bis * np.cos(th) + s * nrm * np.sin(th))
+            a = pc - xca[i]
+            b = xca[i] - pn
+            c = xca[i] - cb
+            chi = np.dot(np.cross(a, b), c) / (np.linalg.norm(a) * np.linalg.norm(b) * np.linalg.norm(c))
+            if best is None or abs(chi - chi0) < best[0]:
+                best = (abs(chi - chi0), cb)
+        xcb[i] = best[1]
+    return xo, xcb
+
+
+def make_system(lengths: Sequence[int], seed: int, n_struct: int = 20, mem_len: int = 9,
+                noise: float = 1.0, compact: bool = False, spacing=(10.0, 15.0)) -> System:
+    """Seeded system of len(lengths) chains.  compact=True confines each chain
+    to a sphere and places chain centroids `spacing` apart (multimers)."""
+    rng = np.random.default_rng(seed)
+    chains = ["".join(rng.choice(list(AA1), size=L)) for L in lengths]
+    traces = []
+    all_pts = np.zeros((0, 3))
+    centers = []
+    for k, L in enumerate(lengths):
+        if compact:
+            if k == 0:
+                c = np.zeros(3)
+            else:
+                for _ in range(1000):
+                    c = centers[rng.integers(len(centers))] + rng.uniform(*spacing) * _rand_unit(rng)
+                    if all(np.linalg.norm(c - cc) >= spacing[0] for cc in centers):
+                        break
+            centers.append(c)
+            radius = 3.8 * (L ** (1.0 / 3.0)) * 1.6 + 8.0
+            start = c + rng.normal(size=3)
+            tr = saw_trace(L, rng, start=start, others=all_pts, center=c, radius=radius)
+        else:
+            start = np.array([100.0, 100.0, 100.0]) + 40.0 * k * np.array([1.0, 0.0, 0.0])
+            tr = saw_trace(L, rng, start=start, others=all_pts)
+        traces.append(tr)
+        all_pts = np.vstack([all_pts, tr])
+    xca_l, xcb_l, xo_l = [], [], []
+    for seq, tr in zip(chains, traces):
+        xo, xcb = place_side_atoms(tr, seq)
+        xca_l.append(tr)
+        xcb_l.append(xcb)
+        xo_l.append(xo)
+    xca = np.vstack(xca_l)
+    xcb = np.vstack(xcb_l)
+    xo = np.vstack(xo_l)
+    n = len(xca)
+    if compact:  # shift into a positive box
+        shift = 150.0 - xca.mean(axis=0)
+        xca, xcb, xo = xca + shift, xcb + shift, xo + shift
+    choices = np.array([[1.0, 0.0], [0.0, 1.0], [0.0, 0.0]])
+    ssw = choices[rng.integers(0, 3, size=n)]
+    mem_structs = []
+    for _ in range(n_struct):
+        ca = xca + rng.normal(scale=noise, size=xca.shape)
+        cb = xcb + rng.normal(scale=noise, size=xcb.shape)
+        # nm with 3 decimals, exactly what a .gro file stores
+        s = np.stack([ca, cb], axis=1) / 10.0
+        s = np.array([[float("%.3f" % v) for v in row] for row in s.reshape(-1, 3)]).reshape(n, 2, 3)
+        mem_structs.append(s)
+    memories = []
+    off = 0
+    for seq in chains:
+        L = len(seq)
+        for p in range(0, L - mem_len + 1):
+            for s in range(n_struct):
+                memories.append(Memory(struct=s, pos=off + p, fpos=off + p, length=mem_len, weight=1.0))
+        off += L
+    return System(chains=chains, xca=xca, xcb=xcb, xo=xo, ssw=ssw, mem_structs=mem_structs,
+                  memories=memories, seed=seed)
+
+
+def coeff_text(fm: bool = True, mem_file: str = "fragsLAMW.mem", gamma_file: str = "uniform.gamma",
+               base: Optional[str] = None) -> str:
+    """The shipped coeff file with the [Fragment_Memory] section enabled."""
+    base = base or os.path.join(PARAM_DIR, "fix_backbone_coeff.data")
+    txt = open(base).read()
+    if fm:
+        txt = txt.replace("[Fragment_Memory]-\n1.0\n1r69.mem\n1r69.gamma",
+                          "[Fragment_Memory]\n1.0\n%s\n%s" % (mem_file, gamma_file))
+        assert "[Fragment_Memory]\n1.0\n" in txt
+    return txt
+
+
+def write_gro(path: str, seq: str, struct_nm: np.ndarray) -> None:
+    """.gro with CA and CB lines (no CB for glycine), coordinates in nm."""
+    lines = []
+    k = 0
+    for i, aa in enumerate(seq):
+        name = AA3[aa]
+        atoms = [("CA", struct_nm[i, 0])]
+        if aa != "G":
+            atoms.append(("CB", struct_nm[i, 1]))
+        for an, xyz in atoms:
+            k += 1
+            lines.append("%5d  %s %-3s %5d %8.3f %8.3f %8.3f" % (i + 1, name, an, k, xyz[0], xyz[1], xyz[2]))
+    with open(path, "w") as f:
+        f.write(" Synthetic memory structure\n")
+        f.write("%10d\n" % len(lines))
+        f.write("\n".join(lines) + "\n")
+        f.write("   0.00000   0.00000   0.00000\n")
+
+
+def write_system(sys: System, workdir: str, fm: bool = True, coeff_base: Optional[str] = None) -> dict:
+    """Writes the reference file set for `sys` into workdir; returns paths."""
+    os.makedirs(workdir, exist_ok=True)
+    for name in ("anti_HB", "anti_NHB", "para_HB", "anti_one", "para_one", "gamma.dat",
+                 "burial_gamma.dat", "uniform.gamma"):
+        shutil.copyfile(os.path.join(PARAM_DIR, name), os.path.join(workdir, name))
+    coeff = os.path.join(workdir, "fix_backbone_coeff.data")
+    with open(coeff, "w") as f:
+        f.write(coeff_text(fm=fm, base=coeff_base))
+    seqf = os.path.join(workdir, "protein.seq")
+    with open(seqf, "w") as f:
+        f.write("\n".join(sys.chains) + "\n")
+    with open(os.path.join(workdir, "ssweight"), "w") as f:
+        for a, b in sys.ssw:
+            f.write("%.1f %.1f\n" % (a, b))
+    if fm:
+        seq = sys.seq
+        for s, st in enumerate(sys.mem_structs):
+            write_gro(os.path.join(workdir, "mem%03d.gro" % s), seq, st)
+        with open(os.path.join(workdir, "fragsLAMW.mem"), "w") as f:
+            f.write("[Target]\nquery\n\n[Memories]\n")
+            for m in sys.memories:
+                f.write("mem%03d.gro %d %d %d %g\n" % (m.struct, m.pos + 1, m.fpos + 1, m.length, m.weight))
+    return dict(coeff=coeff, seq=seqf, workdir=workdir)

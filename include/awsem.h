@@ -1,0 +1,181 @@
+/*
+ * awsem.h -- C ABI of the MI355X (gfx950) FixBackbone force path.
+ *
+ * Plain C99: pointers, sizes and status codes only.  Every call returns 0
+ * on success or a negative AWSEM_E_* code; awsem_last_error() gives the
+ * message (the reference's error->all text where one exists).  No call
+ * aborts the process.
+ *
+ * Each entry point cites the reference interface it replaces (paths
+ * relative to the reference repository root; fb.cpp = src/fix_backbone.cpp,
+ * fb.h = src/fix_backbone.h, pev.cpp = src/pair_excluded_volume.cpp).
+ *
+ * Coordinate layouts
+ *   atom layout    : x[n_atoms][3] FP64 in the caller's (LAMMPS) atom order,
+ *                    mapped to residues by awsem_set_topology().
+ *   residue layout : x[3][n][3] FP64, kind 0 = CA, 1 = CB (H-beta for Gly),
+ *                    2 = O, then residue, then xyz.
+ * Forces are ADDED into f (pre_force semantics: LAMMPS clears f before
+ * modify->pre_force and the fix accumulates with +=, fb.cpp:8181-8189).
+ */
+#ifndef AWSEM_H
+#define AWSEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define AWSEM_OK 0
+#define AWSEM_E_ARG (-1)          /* invalid argument / shape                       */
+#define AWSEM_E_FILE (-2)         /* missing or unreadable input file               */
+#define AWSEM_E_FORMAT (-3)       /* malformed coeff / table / memory file          */
+#define AWSEM_E_UNSUPPORTED (-4)  /* coeff enables a term outside this force path   */
+#define AWSEM_E_TOPOLOGY (-5)     /* bad residue / chain tags (fb.cpp:1651-1696)    */
+#define AWSEM_E_MISSING_ATOM (-6) /* DSSP/P_AP/helix missing atom (fb.cpp:2937 ...) */
+#define AWSEM_E_DEVICE (-7)       /* HIP runtime failure                            */
+#define AWSEM_E_NOMEM (-8)        /* allocation / HBM budget exceeded               */
+#define AWSEM_E_NONFINITE (-9)    /* NaN/Inf coordinates or forces                  */
+#define AWSEM_E_STATE (-10)       /* call out of order                              */
+
+/* energy slots: FixBackbone::EnergyTerms order (fb.h:331-332) + bond style */
+enum awsem_energy_slot {
+  AWSEM_ET_TOTAL = 0, AWSEM_ET_CHAIN, AWSEM_ET_SHAKE, AWSEM_ET_CHI, AWSEM_ET_RAMA,
+  AWSEM_ET_VEXCLUDED, AWSEM_ET_DSSP, AWSEM_ET_PAP, AWSEM_ET_WATER, AWSEM_ET_BURIAL,
+  AWSEM_ET_HELIX, AWSEM_ET_AMHGO, AWSEM_ET_FRAGMEM, AWSEM_ET_VFRAGMEM,
+  AWSEM_ET_CONT_REST, AWSEM_ET_MEMB, AWSEM_ET_SSB, AWSEM_ET_DH, AWSEM_ET_BOND,
+  AWSEM_N_ET
+};
+
+/* precision of the pair / switching arithmetic; force and energy
+ * accumulation, reductions and the integrator are FP64 in both */
+#define AWSEM_PREC_FP64 0
+#define AWSEM_PREC_FP32 1
+
+typedef struct awsem_ctx awsem_ctx;
+typedef struct awsem_batch awsem_batch;
+
+/* ---- single system: the FixBackbone drop-in ---------------------------- */
+
+/* FixBackbone::FixBackbone(lmp, narg, arg) with arg[5] = coeff file and
+ * arg[6] = sequence file (fb.cpp:86-1072).  Side files (anti_HB, anti_NHB,
+ * para_HB, para_one, anti_one, ssweight, gamma.dat, burial_gamma.dat and
+ * the fragment-memory .mem/.gro/gamma files) are read from workdir, which
+ * plays the role of the LAMMPS working directory. */
+int awsem_create(const char *coeff_path, const char *seq_path, const char *workdir,
+                 int precision, awsem_ctx **out);
+
+/* Same constructor with the per-system inputs given as arrays instead of
+ * the sequence / ssweight / .mem / .gro files (coeff and the shared
+ * parameter tables still come from files in param_dir).
+ *   seq        : all chains concatenated (n letters), chain_len[nch]
+ *   aps        : ssweight values [n][12] (row per residue, column per
+ *                [SSWeight] flag; ignored columns may hold anything), or NULL
+ *   memories   : n_mem windows; mem_pos (0-based target residue),
+ *                mem_len, mem_weight; mem_xyz_nm[sum(len)][2][3] holds the
+ *                CA and CB coordinates in nm as a .gro file stores them
+ *                (multiplied by 10 like fm.cpp:87); mem_seq[sum(len)] the
+ *                one-letter residue names ('G' marks a missing CB). */
+int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch,
+                        const int *chain_len, const char *seq, const double *aps,
+                        int n_mem, const int *mem_pos, const int *mem_len,
+                        const double *mem_weight, const double *mem_xyz_nm,
+                        const char *mem_seq, int precision, awsem_ctx **out);
+
+/* Construct_Computational_Arrays (fb.cpp:1590-1698): per atom, the fix
+ * group (0 = CA group, 1 = beta group, 2 = oxygen group, -1 none), the
+ * molecule tag and the residue tag (both 1-based, atom_style awsemmd,
+ * src/atom_vec_awsemmd.cpp:56). */
+int awsem_set_topology(awsem_ctx *c, int n_atoms, const int *group, const int *mol,
+                       const int *res);
+
+/* FixBackbone::pre_force -> compute_backbone (fb.cpp:8181, 7047-7673) on
+ * the device: f[n_atoms][3] += forces; energy[AWSEM_N_ET] receives the
+ * per-term energies (compute_vector(k) == energy[k+1], fb.cpp:8223-8232).
+ * When the excluded-volume pair style is enabled on the context
+ * (awsem_set_vexcluded) its forces and energy (slot AWSEM_ET_VEXCLUDED)
+ * are included, as PairExcludedVolume::compute (pev.cpp:50-138) would add
+ * them in the same force evaluation. */
+int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy);
+
+/* Residue-layout variant of awsem_compute (no topology needed). */
+int awsem_compute_res(awsem_ctx *c, const double *xres, double *fres, double *energy);
+
+/* pair_style vexcluded p rc_short rc_long + pair_coeff (pev.cpp:175-265):
+ * tables are [6][6] indexed by atom type (1 CA, 2 N, 3 O, 4 CB, 5 H-beta).
+ * enable=0 removes the pair style; NULL tables keep the deck defaults
+ * (tools/create_project_tools/CoordinatesToWorkLammpsDataFile.py:314-322). */
+int awsem_set_vexcluded(awsem_ctx *c, int enable, int p, const double *lambda,
+                        const double *cut_short, const double *cut_long);
+
+/* Harmonic bond style of the deck (K, r0 for CA-CA, CA-O, O-CA+1, CA-CB,
+ * CA-Hbeta; CoordinatesToWorkLammpsDataFile.py:289-294); enable=0 removes it. */
+int awsem_set_bonds(awsem_ctx *c, int enable, double k, const double *r0_5);
+
+/* Restrict accumulation to energy slots in mask (bit k = slot k); the
+ * coeff file still decides which terms exist.  Test/debug aid. */
+int awsem_set_term_mask(awsem_ctx *c, uint32_t mask);
+
+/* Integer lists built on the device for one coordinate set (residue
+ * layout).  kind: 0 density pairs of water well `arg`, 1 helix-well density
+ * pairs, 2 DSSP (acceptor, donor) calls, 3 P_AP pairs, 4 helix (i,i+4)
+ * pairs, 5 excluded-volume atom pairs, 6 water xi pairs of well `arg`.
+ * Codes i*n+j (atom codes for kind 5), sorted ascending.  Returns the
+ * count through *count; AWSEM_E_NOMEM if cap is too small. */
+int awsem_list(awsem_ctx *c, int kind, int arg, const double *xres, int64_t *out,
+               int cap, int *count);
+
+/* FP64 velocity-Verlet + Langevin dynamics (LAMMPS fix nve + fix langevin
+ * in real units; not part of the reference) with all force terms of the
+ * context, kept resident on the device for nsteps.  t_start -> t_stop ramps
+ * linearly over the call.  xres/vres are updated in place; fres holds the
+ * last force; energy the last step's per-term energies. */
+typedef struct awsem_langevin {
+  double dt;        /* fs */
+  double t_start;   /* K */
+  double t_stop;    /* K */
+  double damp;      /* fs */
+  uint64_t seed;
+  int64_t step0;    /* step counter of the first step (random stream) */
+} awsem_langevin;
+
+int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres,
+              double *vres, double *fres, double *energy);
+
+int awsem_n_residues(const awsem_ctx *c, int *n);
+double awsem_pair_list_cutoff(const awsem_ctx *c);
+int awsem_last_error(const awsem_ctx *c, char *buf, size_t n);
+int awsem_destroy(awsem_ctx *c);
+
+/* ---- batches of independent replicas ------------------------------------ */
+
+/* A batch owns device copies of every replica's system and state.  The
+ * contexts may be destroyed after the call.  All replicas must share one
+ * coeff parameter set. */
+int awsem_batch_create(int n_ctx, awsem_ctx *const *ctxs, int precision,
+                       awsem_batch **out);
+int awsem_batch_size(const awsem_batch *b, int *n_replicas, int *n_residues_total);
+/* host <-> device state, residue layout per replica concatenated in
+ * replica order ([3][n_r][3] blocks); any pointer may be NULL */
+int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v);
+int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f,
+                          double *energy /* [n_rep][AWSEM_N_ET] */);
+/* one force evaluation of every replica (state x), results kept on device */
+int awsem_batch_compute(awsem_batch *b);
+/* nsteps of dynamics for every replica in one persistent launch
+ * sequence; seed stream per replica = (seed, replica index). */
+int awsem_batch_run(awsem_batch *b, int nsteps, const awsem_langevin *lg);
+/* milliseconds of the last compute/run measured with HIP events on the
+ * batch stream, and the main kernel's average duration */
+int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel_avg,
+                       int *n_kernels);
+int awsem_batch_last_error(const awsem_batch *b, char *buf, size_t n);
+int awsem_batch_destroy(awsem_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1887 @@
+/*
+ * awsem_oracle.c -- CPU FP64 restatement of AWSEM-MD's FixBackbone path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): see
+ * awsem_oracle.h.  Function headers cite the reference file:line each one
+ * restates; fb.cpp = src/fix_backbone.cpp, sml.h = src/smart_matrix_lib.h,
+ * fm.cpp = src/fragment_memory.cpp, pev.cpp = src/pair_excluded_volume.cpp.
+ *
+ * Single-rank, non-periodic semantics: all residues LOCAL, no ghosts, the
+ * LAMMPS half neighbour list is replaced by exhaustive enumeration of every
+ * unordered atom pair (the reference's predicates select the same pairs
+ * because every cutoff used is <= pair_list_cutoff).
+ */
+#include "awsem_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <ctype.h>
+
+#define MAXCH 1000
+#define DELTA 0.00001          /* fb.cpp:40 */
+#define DELTA_WATER_XI 1e-8    /* fb.cpp:41 */
+#define DELTA_HELIX_XI 1e-6    /* fb.cpp:42 */
+#define DSSP_NU_DELTA 1e-4     /* fb.cpp:43 */
+#define PAP_DELTA 1e-12        /* fb.cpp:44 */
+
+/* fb.cpp:54 residue letter -> type index */
+static const int se_map[26] = {0, 0, 4, 3, 6, 13, 7, 8, 9, 0, 11, 10, 12, 2, 0, 14, 5, 1, 15, 16, 0, 19, 17, 0, 18, 0};
+/* fm.cpp:30 four-letter classes */
+static const int four_letter_map[20] = {1, 3, 2, 2, 4, 2, 2, 1, 3, 4, 4, 3, 4, 4, 1, 1, 1, 4, 4, 4};
+
+enum { FM_CA = 1, FM_CB = 2 };
+
+typedef struct {
+  int pos, len;
+  double weight;
+  char *se;
+  double *rf0, *rf1; /* len*len: rf0 CA-CA (i<j) / CB-CB (i>j), rf1 CA(i)-CB(j) */
+} or_mem;
+
+typedef struct {
+  int nseq_cl, nres_cl, frag_resty;
+  int i_sep[10];
+  double *gamma;
+} or_gamma;
+
+struct or_ctx {
+  /* flags (fb.cpp:124-130) */
+  int abc_flag, chain_flag, shake_flag, chi_flag, rama_flag, rama_p_flag;
+  int ssweight_flag, dssp_hdrgn_flag, p_ap_flag, water_flag, burial_flag;
+  int helix_flag, frag_mem_flag;
+  double epsilon;
+  double k_chain[3], r_ncb0, r_cpcb0, r_ncp0;
+  double k_shake, r_sh1, r_sh2, r_sh3;
+  double k_chi, chi0, k_rama;
+  int n_rama_par, n_rama_p_par;
+  int ssweight[12];
+  double w[12], sigma[12], phiw[12], phi0[12], psiw[12], psi0[12];
+  double an, bn, cn, ap, bp, cp, ah, bh, ch;
+  /* DSSP */
+  double k_dssp, hbscl[4][9], sigma_HO, sigma_NO, HO_zero, NO_zero;
+  double dssp_hdrgn_cut, dssp_hdrgn_cut_sq, pref[2], d_nu0;
+  double dssp_nu_cut1_sq, dssp_nu_cut2_sq, sigma_HO_sqinv, sigma_NO_sqinv;
+  double m_anti_HB[20][20][2], m_anti_NHB[20][20][2], m_para_HB[20][20][2];
+  double m_para_one[20], m_anti_one[20];
+  /* P_AP */
+  double k_global_P_AP, k_betapred_P_AP, k_P_AP[3], P_AP_cut, P_AP_pref, pap_cutoff_sq;
+  int i_med_min, i_med_max, i_diff_P_AP;
+  double pap_drmax_sq, pap_drmin_sq;
+  /* water / burial */
+  double k_water, water_kappa, water_kappa_sigma, treshold;
+  int contact_cutoff, n_wells, well_flag[5];
+  double well_r_min[5], well_r_max[5];
+  double w_rmin_theta[5], w_rmax_theta[5], w_rmin_theta_sq[5], w_rmax_theta_sq[5];
+  double water_gamma[5][20][20][2];
+  double k_burial, burial_kappa, burial_ro_min[3], burial_ro_max[3];
+  double burial_gamma[20][3];
+  /* helix */
+  double k_helix, helix_gamma_p, helix_gamma_w, h4prob[20];
+  double helix_kappa, helix_kappa_sigma, helix_treshold, helix_cutoff, helix_cutoff_sq;
+  double helix_well_r_min[5], helix_well_r_max[5];
+  double helix_sigma_HO, helix_sigma_NO, helix_HO_zero, helix_NO_zero;
+  double helix_sigma_HO_sqinv, helix_sigma_NO_sqinv;
+  int n_helix_wells, helix_i_diff, pro_accepter_flag;
+  double h4prob_pro_accepter;
+  double h_rmin_theta, h_rmax_theta, h_rmin_theta_sq, h_rmax_theta_sq;
+  /* fragment memory */
+  double k_frag_mem, fm_sigma_exp;
+  char frag_mems_file[256], fm_gamma_file[256];
+  or_gamma fmg;
+  int n_mems;
+  or_mem *mems;
+  int *ilen_fm_map, **frag_mem_map;
+  double pair_list_cutoff;
+  /* system */
+  int n;
+  char *se;
+  int nch, ch_len[MAXCH], ch_pos[MAXCH];
+  int *chain_no; /* 1-based chain tag per residue */
+  double *aps[12];
+  /* excluded volume pair style (pev.cpp) */
+  int ev_p;
+  double ev_lambda[6][6], ev_cut_short[6][6], ev_cut_long[6][6];
+  /* bonds (LAMMPS harmonic, deck of CoordinatesToWorkLammpsDataFile.py:289-294) */
+  double bond_k, bond_r0[5];
+  /* scratch */
+  double *xn, *xh, *xcp;
+  double *water_ro, *helix_ro, *w_sig, *w_sig_prd, *h_sig, *h_sig_prd;
+  double *water_xi, *helix_xi_1, *helix_xi_2, *burial_force;
+  /* current call */
+  const double *xca, *xcb, *xo;
+  double *fca, *fcb, *fo;
+  double *energy;
+  unsigned mask;
+};
+
+/* ------------------------------------------------------------------ */
+/* small helpers                                                        */
+
+static void set_err(char *err, int errlen, const char *fmt, ...)
+{
+  if (!err || errlen <= 0) return;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(err, (size_t)errlen, fmt, ap);
+  va_end(ap);
+}
+
+static char *read_file(const char *path, long *len)
+{
+  FILE *f = fopen(path, "rb");
+  if (!f) return NULL;
+  fseek(f, 0, SEEK_END);
+  long n = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  char *buf = (char *)malloc((size_t)n + 1);
+  if (!buf) { fclose(f); return NULL; }
+  size_t got = fread(buf, 1, (size_t)n, f);
+  buf[got] = '\0';
+  fclose(f);
+  if (len) *len = (long)got;
+  return buf;
+}
+
+static void join_path(char *out, size_t n, const char *dir, const char *name)
+{
+  if (name[0] == '/' || !dir || !dir[0]) snprintf(out, n, "%s", name);
+  else snprintf(out, n, "%s/%s", dir, name);
+}
+
+/* istream-like extraction over a text buffer: skip whitespace, then take
+ * the longest prefix valid for the requested type (operator>> semantics). */
+typedef struct { const char *p; int fail; int eof; } tokin;
+
+static void tk_skip(tokin *t)
+{
+  while (*t->p && isspace((unsigned char)*t->p)) t->p++;
+  if (!*t->p) t->eof = 1;
+}
+
+static int tk_str(tokin *t, char *out, size_t n)
+{
+  if (t->fail) return 0;
+  tk_skip(t);
+  if (!*t->p) { t->fail = 1; return 0; }
+  size_t k = 0;
+  while (*t->p && !isspace((unsigned char)*t->p)) {
+    if (k + 1 < n) out[k++] = *t->p;
+    t->p++;
+  }
+  out[k] = '\0';
+  if (!*t->p) t->eof = 1;
+  return 1;
+}
+
+static double tk_double(tokin *t)
+{
+  if (t->fail) return 0.0;
+  tk_skip(t);
+  if (!*t->p) { t->fail = 1; return 0.0; }
+  char *end;
+  double v = strtod(t->p, &end);
+  if (end == t->p) { t->fail = 1; return 0.0; }
+  t->p = end;
+  if (!*t->p) t->eof = 1;
+  return v;
+}
+
+static int tk_int(tokin *t)
+{
+  if (t->fail) return 0;
+  tk_skip(t);
+  if (!*t->p) { t->fail = 1; return 0; }
+  char *end;
+  long v = strtol(t->p, &end, 10);
+  if (end == t->p) { t->fail = 1; return 0; }
+  t->p = end;
+  if (!*t->p) t->eof = 1;
+  return (int)v;
+}
+
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void cross3(const double *a, const double *b, double *c)
+{
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+#define XCA(i) (c->xca + 3 * (i))
+#define XCB(i) (c->xcb + 3 * (i))
+#define XO(i) (c->xo + 3 * (i))
+#define XN(i) (c->xn + 3 * (i))
+#define XH(i) (c->xh + 3 * (i))
+#define XCP(i) (c->xcp + 3 * (i))
+#define FCA(i) (c->fca + 3 * (i))
+#define FCB(i) (c->fcb + 3 * (i))
+#define FO(i) (c->fo + 3 * (i))
+#define ON(t) (c->mask & (1u << (t)))
+
+static inline void fadd(double *f, double s, const double *v)
+{
+  f[0] += s * v[0];
+  f[1] += s * v[1];
+  f[2] += s * v[2];
+}
+
+/* letters outside A..Z index out of se_map in the reference (undefined);
+ * they map to type 0 here */
+static inline int restype(char ch) { return (ch >= 'A' && ch <= 'Z') ? se_map[ch - 'A'] : 0; }
+
+/* isFirst / isLast, fb.cpp:1570-1583 (1-based chain positions, 0-based index) */
+static inline int is_first(const or_ctx *c, int i) { return c->ch_pos[c->chain_no[i] - 1] == i + 1; }
+static inline int is_last(const or_ctx *c, int i)
+{
+  int ch = c->chain_no[i] - 1;
+  return c->ch_pos[ch] + c->ch_len[ch] - 1 == i + 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Gamma_Array (fm.cpp:275-519)                                         */
+
+static int gm_index_array(or_gamma *g, const char *resty, int *a)
+{
+  if (strcmp(resty, "ALL") == 0) {
+    for (int i = 0; i < g->nres_cl; ++i) a[i] = i;
+    return g->nres_cl;
+  } else if (strlen(resty) == 1) {
+    int code = resty[0] - 'A';
+    if (code < 0 || code >= 26) return -1;
+    if (se_map[code] == 0 && resty[0] != 'A') return -1;
+    if (g->nres_cl < 20) return -2;
+    a[0] = se_map[code];
+    return 1;
+  } else if (strlen(resty) == 3) {
+    int cl;
+    if (g->nres_cl < 4) return -2;
+    if (strcmp(resty, "SHL") == 0) cl = 1;
+    else if (strcmp(resty, "AHL") == 0) cl = 2;
+    else if (strcmp(resty, "HPB") == 0) cl = 4;
+    else if (strcmp(resty, "BAS") == 0) cl = 3;
+    else return -1;
+    if (g->nres_cl == 4) { a[0] = cl - 1; return 1; }
+    int n = 0;
+    for (int i = 0; i < 20; ++i) if (four_letter_map[i] == cl) a[n++] = i;
+    return n;
+  }
+  return -1;
+}
+
+static int gm_load(or_gamma *g, const char *path, char *err, int errlen)
+{
+  memset(g, 0, sizeof(*g));
+  FILE *f = fopen(path, "r");
+  if (!f) { set_err(err, errlen, "Fragment_Memory: Cannot read gamma file"); return -1; }
+  char line[100];
+  int iline = 0, frag_set = 0;
+  long datapos = 0;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int empty = 1;
+    for (char *q = line; *q; ++q) if (*q != ' ' && *q != '\t' && *q != '\n') { empty = 0; break; }
+    if (empty) continue;
+    if (iline == 0) {
+      int ns = 0;
+      char *st = strtok(line, " \t");
+      while (st) {
+        ns++;
+        if (strcmp(st, "inf") == 0) { g->i_sep[ns - 1] = -1; break; }
+        g->i_sep[ns - 1] = atoi(st);
+        st = strtok(NULL, " \t\n");
+      }
+      g->nseq_cl = ns - 1;
+      if (g->nseq_cl == 0) { fclose(f); set_err(err, errlen, "Fragment_Memory: Wrong definition of sequance separation classes"); return -1; }
+      datapos = ftell(f);
+    } else {
+      char buf[8][64];
+      int nbuf = 0;
+      char *st = strtok(line, " \t\n");
+      while (st && nbuf < 8) { snprintf(buf[nbuf++], 64, "%s", st); st = strtok(NULL, " \t\n"); }
+      if (nbuf != 4 && nbuf != 6) { fclose(f); set_err(err, errlen, "Fragment_Memory: Incorrect entery in gamma file"); return -1; }
+      if (!frag_set) { g->frag_resty = (nbuf == 6); frag_set = 1; }
+      else if ((nbuf == 4 && g->frag_resty) || (nbuf == 6 && !g->frag_resty)) {
+        fclose(f); set_err(err, errlen, "Fragment_Memory: Incorrect entery in gamma file"); return -1;
+      }
+      for (int i = 0; i < nbuf - 2; i++) {
+        size_t bl = strlen(buf[i]);
+        if (bl != 1 && bl != 3) { fclose(f); set_err(err, errlen, "Fragment_Memory: Incorrect entery in gamma file"); return -1; }
+        if (strcmp(buf[i], "ALL") == 0) { if (g->nres_cl < 1) g->nres_cl = 1; }
+        else if (bl == 3) { if (g->nres_cl < 4) g->nres_cl = 4; }
+        else { if (g->nres_cl < 20) g->nres_cl = 20; }
+      }
+    }
+    iline++;
+  }
+  if (g->nres_cl == 0) { fclose(f); set_err(err, errlen, "Fragment_Memory: Incorrect entery in gamma file"); return -1; }
+  int r = g->nres_cl;
+  long ng = g->frag_resty ? (long)g->nseq_cl * r * r * r * r : (long)g->nseq_cl * r * r;
+  g->gamma = (double *)calloc((size_t)ng, sizeof(double));
+  fseek(f, datapos, SEEK_SET);
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int empty = 1;
+    for (char *q = line; *q; ++q) if (*q != ' ' && *q != '\t' && *q != '\n') { empty = 0; break; }
+    if (empty) continue;
+    char *t1 = strtok(line, " \t\n");
+    char *t2 = strtok(NULL, " \t\n");
+    if (!g->frag_resty) {
+      int cl = atoi(strtok(NULL, " \t\n"));
+      double gm = atof(strtok(NULL, " \t\n"));
+      if (cl > g->nseq_cl) { fclose(f); set_err(err, errlen, "Fragment_Memory: Wrong sequance separation class tag"); return -1; }
+      int is[20], js[20];
+      int in = gm_index_array(g, t1, is), jn = gm_index_array(g, t2, js);
+      if (in < 0 || jn < 0) {
+        fclose(f);
+        set_err(err, errlen, (in == -2 || jn == -2) ? "Fragment_Memory: Cannot build gamma array" : "Fragment_Memory: Incorrect entery in gamma file");
+        return -1;
+      }
+      for (int i = 0; i < in; ++i)
+        for (int j = 0; j < jn; ++j) {
+          g->gamma[(cl - 1) * r * r + is[i] * r + js[j]] = gm;
+          if (is[i] != js[j]) g->gamma[(cl - 1) * r * r + js[j] * r + is[i]] = gm;
+        }
+    } else {
+      char *t3 = strtok(NULL, " \t\n");
+      (void)strtok(NULL, " \t\n"); /* jfresty: the reference passes ifresty twice (fm.cpp:371) */
+      int cl = atoi(strtok(NULL, " \t\n"));
+      double gm = atof(strtok(NULL, " \t\n"));
+      if (cl > g->nseq_cl) { fclose(f); set_err(err, errlen, "Fragment_Memory: Cannot build gamma array"); return -1; }
+      int is[20], js[20], ks[20], ls[20];
+      int in = gm_index_array(g, t1, is), jn = gm_index_array(g, t2, js);
+      int kn = gm_index_array(g, t3, ks), ln = gm_index_array(g, t3, ls);
+      if (in < 0 || jn < 0 || kn < 0 || ln < 0) { fclose(f); set_err(err, errlen, "Fragment_Memory: Incorrect entery in gamma file"); return -1; }
+      long b = (long)(cl - 1) * r * r * r * r;
+      for (int i = 0; i < in; ++i)
+        for (int j = 0; j < jn; ++j)
+          for (int k = 0; k < kn; ++k)
+            for (int l = 0; l < ln; ++l) {
+              g->gamma[b + is[i] * r * r * r + js[j] * r * r + ks[k] * r + ls[l]] = gm;
+              if (is[i] != js[j]) g->gamma[b + js[j] * r * r * r + is[i] * r * r + ks[k] * r + ls[l]] = gm;
+              if (ks[k] != ls[l]) g->gamma[b + is[i] * r * r * r + js[j] * r * r + ls[l] * r + ks[k]] = gm;
+              if (is[i] != js[j] && ks[k] != ls[l]) g->gamma[b + js[j] * r * r * r + is[i] * r * r + ls[l] * r + ks[k]] = gm;
+            }
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
+/* Gamma_Array::getGamma(ires_type, jres_type, ires, jres), fm.cpp:400-424 */
+static double gm_get2(const or_gamma *g, int it, int jt, int ires, int jres)
+{
+  int dij = abs(ires - jres);
+  if (dij < g->i_sep[0] || (g->i_sep[g->nseq_cl] != -1 && dij > g->i_sep[g->nseq_cl])) return 0.0;
+  int seq_cl;
+  for (seq_cl = 1; seq_cl < g->nseq_cl && dij >= g->i_sep[seq_cl]; seq_cl++) {}
+  int r = g->nres_cl;
+  if (r == 1) return g->gamma[seq_cl - 1];
+  int icl = it, jcl = jt;
+  if (r == 4) { icl = four_letter_map[it] - 1; jcl = four_letter_map[jt] - 1; }
+  return g->gamma[(seq_cl - 1) * r * r + icl * r + jcl];
+}
+
+/* getGamma with fragment residue types, fm.cpp:426-454 */
+static double gm_get4(const or_gamma *g, int it, int jt, int ift, int jft, int ires, int jres)
+{
+  int dij = abs(ires - jres);
+  if (dij < g->i_sep[0] || (g->i_sep[g->nseq_cl] != -1 && dij > g->i_sep[g->nseq_cl])) return 0.0;
+  int seq_cl;
+  for (seq_cl = 1; seq_cl < g->nseq_cl && dij >= g->i_sep[seq_cl]; seq_cl++) {}
+  int r = g->nres_cl;
+  if (r == 1) return g->gamma[seq_cl - 1];
+  int a = it, b = jt, cc = ift, d = jft;
+  if (r == 4) { a = four_letter_map[it] - 1; b = four_letter_map[jt] - 1; cc = four_letter_map[ift] - 1; d = four_letter_map[jft] - 1; }
+  return g->gamma[(long)(seq_cl - 1) * r * r * r * r + a * r * r * r + b * r * r + cc * r + d];
+}
+
+/* ------------------------------------------------------------------ */
+/* Fragment_Memory (.gro reader, fm.cpp:34-176) and read_mems (fb.cpp:1966-2042) */
+
+static char three_to_one(const char *s)
+{
+  static const char *names[20] = {"ALA", "ARG", "ASN", "ASP", "CYS", "GLN", "GLU", "GLY", "HIS", "ILE",
+                                  "LEU", "LYS", "MET", "PHE", "PRO", "SER", "THR", "TRP", "TYR", "VAL"};
+  static const char one[20] = {'A', 'R', 'N', 'D', 'C', 'Q', 'E', 'G', 'H', 'I', 'L', 'K', 'M', 'F', 'P', 'S', 'T', 'W', 'Y', 'V'};
+  if (strlen(s) == 3)
+    for (int i = 0; i < 20; ++i) if (strcmp(s, names[i]) == 0) return one[i];
+  return '?';
+}
+
+static int mem_load(or_mem *m, int pos, int fpos, int len, double weight, const char *path)
+{
+  memset(m, 0, sizeof(*m));
+  m->pos = pos; m->len = len; m->weight = weight;
+  m->se = (char *)calloc((size_t)len + 1, 1);
+  double *xca = (double *)calloc((size_t)len * 3, sizeof(double));
+  double *xcb = (double *)calloc((size_t)len * 3, sizeof(double));
+  FILE *f = fopen(path, "r");
+  if (!f) { free(xca); free(xcb); return 1; /* ERR_FILE */ }
+  char buff[201];
+  int nAtoms = 0, nca = 0, ncb = 0;
+  if (!fgets(buff, 200, f)) nAtoms = 0;
+  if (fscanf(f, "%d", &nAtoms) != 1) nAtoms = 0;
+  if (!fgets(buff, 200, f)) nAtoms = 0;
+  for (int i = 0; i < nAtoms; ++i) {
+    int ires, iatom;
+    char resty[64], atomty[64];
+    double x, y, z;
+    if (fscanf(f, "%d %63s %63s %d %lf %lf %lf", &ires, resty, atomty, &iatom, &x, &y, &z) != 7) break;
+    if (ires > fpos + len) break;
+    if (ires > fpos && ires <= fpos + len) {
+      ires -= fpos + 1;
+      x *= 10; y *= 10; z *= 10;
+      if (strcmp(atomty, "CA") == 0) {
+        if (ires >= len || nca >= len) { fclose(f); free(xca); free(xcb); return 2; }
+        m->se[ires] = three_to_one(resty);
+        if (m->se[ires] == '-') { fclose(f); free(xca); free(xcb); return 3; }
+        xca[3 * ires] = x; xca[3 * ires + 1] = y; xca[3 * ires + 2] = z;
+        nca++;
+      }
+      if (strcmp(atomty, "CB") == 0) {
+        if (ires >= len || ncb >= len) { fclose(f); free(xca); free(xcb); return 2; }
+        xcb[3 * ires] = x; xcb[3 * ires + 1] = y; xcb[3 * ires + 2] = z;
+        ncb++;
+      }
+    }
+  }
+  fclose(f);
+  if (nca != len) { free(xca); free(xcb); return 2; }
+  m->rf0 = (double *)calloc((size_t)len * len, sizeof(double));
+  m->rf1 = (double *)calloc((size_t)len * len, sizeof(double));
+  for (int i = 0; i < len; ++i)
+    for (int j = 0; j < len; ++j) {
+      const double *a, *b;
+      if (i < j) { a = xca + 3 * i; b = xca + 3 * j; }
+      else if (i > j) { a = xcb + 3 * i; b = xcb + 3 * j; }
+      else { a = b = NULL; }
+      if (a) {
+        double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+        m->rf0[i * len + j] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      }
+      const double *p = xca + 3 * i, *q = xcb + 3 * j;
+      double e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
+      m->rf1[i * len + j] = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    }
+  free(xca); free(xcb);
+  return 0;
+}
+
+/* Fragment_Memory::Rf, fm.cpp:190-202 */
+static double mem_rf(const or_mem *m, int ires, int iatom, int jres, int jatom)
+{
+  ires -= m->pos; jres -= m->pos;
+  if (ires < 0 || ires >= m->len || jres < 0 || jres >= m->len) return NAN;
+  int lo = ires < jres ? ires : jres, hi = ires < jres ? jres : ires;
+  if (iatom == FM_CA && jatom == FM_CA) return m->rf0[lo * m->len + hi];
+  if (iatom == FM_CB && jatom == FM_CB) return m->rf0[hi * m->len + lo];
+  return iatom == FM_CA ? m->rf1[ires * m->len + jres] : m->rf1[jres * m->len + ires];
+}
+
+static char *trim(char *s)
+{
+  while (isspace((unsigned char)*s)) s++;
+  size_t n = strlen(s);
+  if (n == 0) return s;
+  char *b = s + n;
+  while (b > s && isspace((unsigned char)*--b)) {}
+  if (!isspace((unsigned char)*b)) b[1] = '\0';
+  else *b = '\0';
+  return s;
+}
+
+static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char *err, int errlen)
+{
+  char path[1024];
+  join_path(path, sizeof path, workdir, mems_file);
+  FILE *f = fopen(path, "r");
+  if (!f) { set_err(err, errlen, "Fragment_Memory: Error opening mem file"); return -1; }
+  char ln[500];
+  int state = 0; /* 0 none, 1 target, 2 mems */
+  c->n_mems = 0;
+  int cap = 0;
+  while (fgets(ln, sizeof ln, f)) {
+    char *line = trim(ln);
+    if (line[0] == '#') continue;
+    if (line[0] == '[') state = 0;
+    int empty = 1;
+    for (char *q = line; *q; ++q) if (*q != ' ' && *q != '\t' && *q != '\n') { empty = 0; break; }
+    if (empty) { state = 0; continue; }
+    if (state == 2) {
+      char *str[6];
+      int nstr = 0;
+      str[nstr] = strtok(line, " \t\n");
+      while (str[nstr] != NULL) {
+        nstr++;
+        if (nstr > 5) break;
+        str[nstr] = strtok(NULL, " \t\n");
+      }
+      if (nstr != 5) { fclose(f); set_err(err, errlen, "Fragment_Memory: Error reading mem file"); return -1; }
+      int tpos = atoi(str[1]) - 1, fpos = atoi(str[2]) - 1, len = atoi(str[3]);
+      double weight = atof(str[4]);
+      if (c->n_mems == cap) { cap = cap ? 2 * cap : 64; c->mems = (or_mem *)realloc(c->mems, (size_t)cap * sizeof(or_mem)); }
+      char gro[1024];
+      join_path(gro, sizeof gro, workdir, str[0]);
+      int e = mem_load(&c->mems[c->n_mems], tpos, fpos, len, weight, gro);
+      c->n_mems++;
+      if (e) { fclose(f); set_err(err, errlen, "read_mems: Fragment_Memory: Error reading memory fragment"); return -1; }
+      if (tpos + len > c->n) { fclose(f); set_err(err, errlen, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); return -1; }
+    } else if (state == 0) {
+      if (strcmp(line, "[Target]") == 0) state = 1;
+      else if (strcmp(line, "[Memories]") == 0) state = 2;
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Constructor: coeff parse (fb.cpp:165-497), scaling (502-516), side   */
+/* files (538-791), pair_list_cutoff (1048-1071)                        */
+
+static double calc_exp_helix_cutoff(const or_ctx *c) /* fb.cpp:1074-1104 */
+{
+  double r0[3] = {3.77, 2.5, 2.41}, r1[3], r2[3];
+  double an = c->an, bn = c->bn, cn = c->cn, ah = c->ah, bh = c->bh, ch = c->ch;
+  r1[0] = sqrt(pow(an * r0[0], 2) + pow(cn * r0[1], 2) + an * cn * (r0[0] * r0[0] + r0[1] * r0[1] - r0[2] * r0[2]));
+  r1[1] = sqrt(pow(an * r0[2], 2) + pow(bn * r0[1], 2) + an * bn * (r0[2] * r0[2] + r0[1] * r0[1] - r0[0] * r0[0]));
+  r1[2] = sqrt(pow(bn * r0[0], 2) + pow(cn * r0[2], 2) + bn * cn * (r0[0] * r0[0] + r0[2] * r0[2] - r0[1] * r0[1]));
+  r2[0] = sqrt(pow(ah * r0[0], 2) + pow(ch * r0[1], 2) + ah * ch * (r0[0] * r0[0] + r0[1] * r0[1] - r0[2] * r0[2]));
+  r2[1] = sqrt(pow(ah * r0[2], 2) + pow(bh * r0[1], 2) + ah * bh * (r0[2] * r0[2] + r0[1] * r0[1] - r0[0] * r0[0]));
+  r2[2] = sqrt(pow(bh * r0[0], 2) + pow(ch * r0[2], 2) + bh * ch * (r0[0] * r0[0] + r0[2] * r0[2] - r0[1] * r0[1]));
+  double r1m = fmax(r1[0], fmax(r1[1], r1[2])), r2m = fmax(r2[0], fmax(r2[1], r2[2]));
+  r1m = 1.1 * r1m + c->helix_NO_zero + c->helix_sigma_NO * sqrt(6.0 * log(10.0));
+  r2m = 1.1 * r2m + c->helix_HO_zero + c->helix_sigma_HO * sqrt(6.0 * log(10.0));
+  return fmax(r1m, r2m);
+}
+
+static int parse_coeff(or_ctx *c, const char *path, char *err, int errlen)
+{
+  char *text = read_file(path, NULL);
+  if (!text) { set_err(err, errlen, "Coefficient file was not found!"); return -1; }
+  tokin in = {text, 0, 0};
+  char sec[256], sbuf[256];
+  int j;
+  while (!in.eof && !in.fail) {
+    sec[0] = '\0';
+    tk_str(&in, sec, sizeof sec);
+    if (strcmp(sec, "[ABC]") == 0) {
+      c->abc_flag = 1;
+      c->an = tk_double(&in); c->bn = tk_double(&in); c->cn = tk_double(&in);
+      c->ap = tk_double(&in); c->bp = tk_double(&in); c->cp = tk_double(&in);
+      c->ah = tk_double(&in); c->bh = tk_double(&in); c->ch = tk_double(&in);
+    } else if (strcmp(sec, "[Chain]") == 0) {
+      c->chain_flag = 1;
+      for (j = 0; j < 3; j++) c->k_chain[j] = tk_double(&in);
+      c->r_ncb0 = tk_double(&in); c->r_cpcb0 = tk_double(&in); c->r_ncp0 = tk_double(&in);
+    } else if (strcmp(sec, "[Shake]") == 0) {
+      c->shake_flag = 1;
+      c->k_shake = tk_double(&in); c->r_sh1 = tk_double(&in); c->r_sh2 = tk_double(&in); c->r_sh3 = tk_double(&in);
+    } else if (strcmp(sec, "[Chi]") == 0) {
+      c->chi_flag = 1;
+      c->k_chi = tk_double(&in); c->chi0 = tk_double(&in);
+    } else if (strcmp(sec, "[Excluded]") == 0 || strcmp(sec, "[Excluded_R6]") == 0) {
+      /* production build: parsed, used only under DEBUGFORCES */
+      for (j = 0; j < 4; j++) tk_double(&in);
+    } else if (strcmp(sec, "[Excluded_P]") == 0) {
+      tk_int(&in);
+      for (j = 0; j < 4; j++) tk_double(&in);
+    } else if (strcmp(sec, "[Rama]") == 0) {
+      c->rama_flag = 1;
+      c->k_rama = tk_double(&in);
+      c->n_rama_par = tk_int(&in);
+      if (c->n_rama_par > 6 || c->n_rama_par < 0) { free(text); set_err(err, errlen, "Rama: too many wells"); return -1; }
+      for (j = 0; j < c->n_rama_par; j++) {
+        c->w[j] = tk_double(&in); c->sigma[j] = tk_double(&in); c->phiw[j] = tk_double(&in);
+        c->phi0[j] = tk_double(&in); c->psiw[j] = tk_double(&in); c->psi0[j] = tk_double(&in);
+        c->phiw[j] *= c->sigma[j];
+        c->psiw[j] *= c->sigma[j];
+      }
+    } else if (strcmp(sec, "[Rama_P]") == 0) {
+      c->rama_p_flag = 1;
+      c->n_rama_p_par = tk_int(&in);
+      if (c->n_rama_p_par > 6 || c->n_rama_p_par < 0) { free(text); set_err(err, errlen, "Rama_P: too many wells"); return -1; }
+      for (j = 0; j < c->n_rama_p_par; j++) {
+        int k = j + 6;
+        c->w[k] = tk_double(&in); c->sigma[k] = tk_double(&in); c->phiw[k] = tk_double(&in);
+        c->phi0[k] = tk_double(&in); c->psiw[k] = tk_double(&in); c->psi0[k] = tk_double(&in);
+        c->phiw[k] *= c->sigma[k];
+        c->psiw[k] *= c->sigma[k];
+      }
+    } else if (strcmp(sec, "[SSWeight]") == 0) {
+      c->ssweight_flag = 1;
+      for (j = 0; j < 12; ++j) c->ssweight[j] = tk_int(&in) != 0;
+    } else if (strcmp(sec, "[Dssp_Hdrgn]") == 0) {
+      c->dssp_hdrgn_flag = 1;
+      c->k_dssp = tk_double(&in);
+      c->hbscl[0][0] = tk_double(&in); c->hbscl[0][1] = tk_double(&in);
+      for (j = 0; j < 7; ++j) c->hbscl[1][j] = tk_double(&in);
+      for (j = 0; j < 9; ++j) c->hbscl[2][j] = tk_double(&in);
+      for (j = 0; j < 9; ++j) c->hbscl[3][j] = tk_double(&in);
+      c->sigma_HO = tk_double(&in); c->sigma_NO = tk_double(&in);
+      c->HO_zero = tk_double(&in); c->NO_zero = tk_double(&in);
+      c->dssp_hdrgn_cut = tk_double(&in);
+      c->pref[0] = tk_double(&in); c->pref[1] = tk_double(&in);
+      c->d_nu0 = tk_double(&in);
+      c->dssp_hdrgn_cut_sq = c->dssp_hdrgn_cut * c->dssp_hdrgn_cut;
+      c->dssp_nu_cut1_sq = pow(c->d_nu0 + atanh(2.0 * DSSP_NU_DELTA - 1.0) / c->pref[0], 2);
+      c->dssp_nu_cut2_sq = pow(c->d_nu0 + atanh(2.0 * DSSP_NU_DELTA - 1.0) / c->pref[1], 2);
+      c->sigma_HO_sqinv = 1.0 / (c->sigma_HO * c->sigma_HO);
+      c->sigma_NO_sqinv = 1.0 / (c->sigma_NO * c->sigma_NO);
+    } else if (strcmp(sec, "[P_AP]") == 0) {
+      c->p_ap_flag = 1;
+      c->k_global_P_AP = tk_double(&in);
+      c->k_betapred_P_AP = tk_double(&in);
+      for (j = 0; j < 3; j++) c->k_P_AP[j] = tk_double(&in);
+      c->P_AP_cut = tk_double(&in);
+      c->P_AP_pref = tk_double(&in);
+      c->i_med_min = tk_int(&in); c->i_med_max = tk_int(&in);
+      c->i_diff_P_AP = tk_int(&in);
+      c->pap_cutoff_sq = pow(c->P_AP_cut + atanh(1.0 - 2.0 * PAP_DELTA) / c->P_AP_pref, 2);
+    } else if (strcmp(sec, "[Water]") == 0) {
+      c->water_flag = 1;
+      c->k_water = tk_double(&in);
+      c->water_kappa = tk_double(&in); c->water_kappa_sigma = tk_double(&in);
+      c->treshold = tk_double(&in);
+      c->contact_cutoff = tk_int(&in);
+      c->n_wells = tk_int(&in);
+      if (c->n_wells > 5 || c->n_wells < 0) { free(text); set_err(err, errlen, "Water: too many wells"); return -1; }
+      for (j = 0; j < c->n_wells; ++j) {
+        c->well_r_min[j] = tk_double(&in); c->well_r_max[j] = tk_double(&in); c->well_flag[j] = tk_int(&in);
+      }
+    } else if (strcmp(sec, "[Burial]") == 0) {
+      c->burial_flag = 1;
+      c->k_burial = tk_double(&in);
+      c->burial_kappa = tk_double(&in);
+      for (j = 0; j < 3; j++) { c->burial_ro_min[j] = tk_double(&in); c->burial_ro_max[j] = tk_double(&in); }
+    } else if (strcmp(sec, "[Helix]") == 0) {
+      c->helix_flag = 1;
+      c->k_helix = tk_double(&in);
+      c->helix_gamma_p = tk_double(&in); c->helix_gamma_w = tk_double(&in);
+      c->helix_kappa = tk_double(&in); c->helix_kappa_sigma = tk_double(&in);
+      c->helix_treshold = tk_double(&in);
+      c->helix_i_diff = tk_int(&in);
+      c->helix_cutoff = tk_double(&in);
+      c->helix_cutoff_sq = c->helix_cutoff * c->helix_cutoff;
+      c->n_helix_wells = 1;
+      c->helix_well_r_min[0] = tk_double(&in); c->helix_well_r_max[0] = tk_double(&in);
+      for (j = 0; j < 20; ++j) c->h4prob[j] = tk_double(&in);
+      c->pro_accepter_flag = tk_int(&in); c->h4prob_pro_accepter = tk_double(&in);
+      c->helix_sigma_HO = tk_double(&in); c->helix_sigma_NO = tk_double(&in);
+      c->helix_HO_zero = tk_double(&in); c->helix_NO_zero = tk_double(&in);
+      c->helix_sigma_HO_sqinv = 1.0 / (c->helix_sigma_HO * c->helix_sigma_HO);
+      c->helix_sigma_NO_sqinv = 1.0 / (c->helix_sigma_NO * c->helix_sigma_NO);
+    } else if (strcmp(sec, "[Fragment_Memory]") == 0) {
+      c->frag_mem_flag = 1;
+      c->k_frag_mem = tk_double(&in);
+      tk_str(&in, c->frag_mems_file, sizeof c->frag_mems_file);
+      tk_str(&in, c->fm_gamma_file, sizeof c->fm_gamma_file);
+    } else if (strcmp(sec, "[Epsilon]") == 0) {
+      c->epsilon = tk_double(&in);
+    } else if (strcmp(sec, "[AMH-Go]") == 0 || strcmp(sec, "[Fragment_Memory_Table]") == 0 ||
+               strcmp(sec, "[Contact_Restraints]") == 0 || strcmp(sec, "[Vector_Fragment_Memory]") == 0 ||
+               strcmp(sec, "[Solvent_Barrier]") == 0 || strcmp(sec, "[Membrane]") == 0 ||
+               strcmp(sec, "[Fragment_Frustratometer]") == 0 || strcmp(sec, "[Tertiary_Frustratometer]") == 0 ||
+               strcmp(sec, "[Nmer_Frustratometer]") == 0 || strcmp(sec, "[Phosphorylation]") == 0 ||
+               strcmp(sec, "[Amylometer]") == 0 || strcmp(sec, "[Selection_Temperature]") == 0 ||
+               strcmp(sec, "[Monte_Carlo_Seq_Opt]") == 0 || strcmp(sec, "[Optimization]") == 0 ||
+               strcmp(sec, "[Burial_Optimization]") == 0 || strcmp(sec, "[DebyeHuckel]") == 0 ||
+               strcmp(sec, "[DebyeHuckel_Optimization]") == 0) {
+      free(text);
+      set_err(err, errlen, "Section %s is enabled but is not part of this build's force path", sec);
+      return -2;
+    } else if (strcmp(sec, "[Shuffler]") == 0) {
+      int fl = tk_int(&in);
+      tk_str(&in, sbuf, sizeof sbuf);
+      if (fl == 1) { free(text); set_err(err, errlen, "Section [Shuffler] is enabled but is not part of this build's force path"); return -2; }
+    } else if (strcmp(sec, "[Mutate_Sequence]") == 0) {
+      int fl = tk_int(&in);
+      tk_str(&in, sbuf, sizeof sbuf);
+      if (fl == 1) { free(text); set_err(err, errlen, "Section [Mutate_Sequence] is enabled but is not part of this build's force path"); return -2; }
+    }
+  }
+  free(text);
+  return 0;
+}
+
+static int read_table_file(const char *dir, const char *name, double *dst, int count, int stride, char *err, int errlen)
+{
+  (void)stride;
+  char path[1024];
+  join_path(path, sizeof path, dir, name);
+  char *t = read_file(path, NULL);
+  if (!t) { set_err(err, errlen, "File %s doesn't exist", name); return -1; }
+  tokin in = {t, 0, 0};
+  for (int k = 0; k < count; ++k) dst[k] = tk_double(&in);
+  int bad = in.fail;
+  free(t);
+  if (bad) { set_err(err, errlen, "DSSP file format error"); return -1; }
+  return 0;
+}
+
+int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
+              or_ctx **out, char *err, int errlen)
+{
+  *out = NULL;
+  or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
+  c->epsilon = 1.0;
+  /* fb.cpp:141-146 defaults */
+  c->an = 0.4831806; c->bn = 0.7032820; c->cn = -0.1864262;
+  c->ap = 0.4436538; c->bp = 0.2352006; c->cp = 0.3211455;
+  c->ah = 0.8409657; c->bh = 0.8929599; c->ch = -0.7338894;
+  c->fm_sigma_exp = 0.15;
+  int rc = parse_coeff(c, coeff_path, err, errlen);
+  if (rc) { or_destroy(c); return rc; }
+
+  /* fb.cpp:502-516 */
+  double e = c->epsilon;
+  for (int j = 0; j < 3; ++j) c->k_chain[j] *= e;
+  c->k_chi *= e; c->k_rama *= e; c->k_water *= e; c->k_burial *= e; c->k_helix *= e;
+  c->k_dssp *= e; c->k_global_P_AP *= e;
+  for (int j = 0; j < c->n_rama_par; j++) c->w[j] *= c->k_rama;
+  for (int j = 0; j < c->n_rama_p_par; j++) c->w[j + 6] *= c->k_rama;
+
+  /* cWell ranges, sml.h:363-368; cP_AP ranges, sml.h:111-115 */
+  for (int k = 0; k < c->n_wells; ++k) {
+    c->w_rmin_theta[k] = c->well_r_min[k] - 8.0 * 2.302585 / c->water_kappa;
+    c->w_rmax_theta[k] = c->well_r_max[k] + 8.0 * 2.302585 / c->water_kappa;
+    c->w_rmin_theta_sq[k] = c->w_rmin_theta[k] * c->w_rmin_theta[k];
+    c->w_rmax_theta_sq[k] = c->w_rmax_theta[k] * c->w_rmax_theta[k];
+  }
+  if (c->helix_flag) {
+    c->h_rmin_theta = c->helix_well_r_min[0] - 8.0 * 2.302585 / c->helix_kappa;
+    c->h_rmax_theta = c->helix_well_r_max[0] + 8.0 * 2.302585 / c->helix_kappa;
+    c->h_rmin_theta_sq = c->h_rmin_theta * c->h_rmin_theta;
+    c->h_rmax_theta_sq = c->h_rmax_theta * c->h_rmax_theta;
+  }
+  if (c->p_ap_flag) {
+    double drmax = c->P_AP_cut + (double)((int)(8.0 * 2.302585 / c->P_AP_pref) + 1);
+    double drmin = c->P_AP_cut - (double)((int)(8.0 * 2.302585 / c->P_AP_pref) + 1);
+    c->pap_drmax_sq = drmax * drmax;
+    c->pap_drmin_sq = drmin * drmin;
+  }
+
+  /* sequence file, fb.cpp:538-553 */
+  char *st = read_file(seq_path, NULL);
+  if (!st) { or_destroy(c); set_err(err, errlen, "Sequence file was not found"); return -1; }
+  tokin in = {st, 0, 0};
+  size_t secap = 1024, selen = 0;
+  c->se = (char *)calloc(secap, 1);
+  char buf[65536];
+  while (!in.eof && !in.fail) {
+    buf[0] = '\0';
+    tk_str(&in, buf, sizeof buf);
+    if (buf[0] == '#' || buf[0] == '\0') continue;
+    if (c->nch >= MAXCH) { free(st); or_destroy(c); set_err(err, errlen, "Too many chains"); return -1; }
+    size_t L = strlen(buf);
+    for (size_t q = 0; q < L; ++q) {
+      if (buf[q] < 'A' || buf[q] > 'Z') { free(st); or_destroy(c); set_err(err, errlen, "Sequence file: unknown residue letter"); return -1; }
+    }
+    c->ch_pos[c->nch] = (int)selen + 1;
+    while (selen + L + 2 > secap) { secap *= 2; c->se = (char *)realloc(c->se, secap); }
+    memcpy(c->se + selen, buf, L);
+    selen += L;
+    c->se[selen] = '\0';
+    c->ch_len[c->nch] = (int)L;
+    c->nch++;
+  }
+  free(st);
+  c->n = (int)selen;
+  if (c->n < 1) { or_destroy(c); set_err(err, errlen, "Empty sequence"); return -1; }
+  int n = c->n;
+  c->chain_no = (int *)calloc((size_t)n, sizeof(int));
+  for (int ch = 0; ch < c->nch; ++ch)
+    for (int k = 0; k < c->ch_len[ch]; ++k) c->chain_no[c->ch_pos[ch] - 1 + k] = ch + 1;
+  for (int k = 0; k < 12; ++k) c->aps[k] = (double *)calloc((size_t)n, sizeof(double));
+
+  char path[1024];
+  if (c->dssp_hdrgn_flag) { /* fb.cpp:555-593 */
+    double tmp[800];
+    if (read_table_file(workdir, "anti_HB", tmp, 800, 0, err, errlen)) { or_destroy(c); return -1; }
+    for (int i = 0; i < 20; ++i) for (int j = 0; j < 20; ++j) { c->m_anti_HB[i][j][0] = tmp[i * 20 + j]; c->m_anti_HB[i][j][1] = tmp[400 + i * 20 + j]; }
+    if (read_table_file(workdir, "anti_NHB", tmp, 800, 0, err, errlen)) { or_destroy(c); return -1; }
+    for (int i = 0; i < 20; ++i) for (int j = 0; j < 20; ++j) { c->m_anti_NHB[i][j][0] = tmp[i * 20 + j]; c->m_anti_NHB[i][j][1] = tmp[400 + i * 20 + j]; }
+    if (read_table_file(workdir, "para_HB", tmp, 800, 0, err, errlen)) { or_destroy(c); return -1; }
+    for (int i = 0; i < 20; ++i) for (int j = 0; j < 20; ++j) { c->m_para_HB[i][j][0] = tmp[i * 20 + j]; c->m_para_HB[i][j][1] = tmp[400 + i * 20 + j]; }
+    if (read_table_file(workdir, "para_one", c->m_para_one, 20, 0, err, errlen)) { or_destroy(c); return -1; }
+    if (read_table_file(workdir, "anti_one", c->m_anti_one, 20, 0, err, errlen)) { or_destroy(c); return -1; }
+  }
+  if (c->ssweight_flag) { /* fb.cpp:595-604 */
+    join_path(path, sizeof path, workdir, "ssweight");
+    char *t = read_file(path, NULL);
+    if (!t) { or_destroy(c); set_err(err, errlen, "File ssweight doesn't exist"); return -1; }
+    tokin s = {t, 0, 0};
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < 12; ++i) c->aps[i][j] = c->ssweight[i] ? tk_double(&s) : 0.0;
+    free(t);
+  }
+  if (c->water_flag) { /* fb.cpp:623-641 */
+    join_path(path, sizeof path, workdir, "gamma.dat");
+    char *t = read_file(path, NULL);
+    if (!t) { or_destroy(c); set_err(err, errlen, "File gamma.dat doesn't exist"); return -1; }
+    tokin s = {t, 0, 0};
+    for (int w = 0; w < c->n_wells; ++w)
+      for (int i = 0; i < 20; ++i)
+        for (int j = i; j < 20; ++j) {
+          double g0 = tk_double(&s), g1 = tk_double(&s);
+          if (s.fail) { free(t); or_destroy(c); set_err(err, errlen, "Water potential gamma.dat file format error"); return -1; }
+          c->water_gamma[w][i][j][0] = c->water_gamma[w][j][i][0] = g0 * c->k_water;
+          c->water_gamma[w][i][j][1] = c->water_gamma[w][j][i][1] = g1 * c->k_water;
+        }
+    free(t);
+  }
+  if (c->burial_flag) { /* fb.cpp:684-693 */
+    if (!c->water_flag) { or_destroy(c); set_err(err, errlen, "Cannot use burial without water potential"); return -1; }
+    join_path(path, sizeof path, workdir, "burial_gamma.dat");
+    char *t = read_file(path, NULL);
+    if (!t) { or_destroy(c); set_err(err, errlen, "File burial_gamma.dat doesn't exist"); return -1; }
+    tokin s = {t, 0, 0};
+    for (int i = 0; i < 20; ++i) for (int k = 0; k < 3; ++k) c->burial_gamma[i][k] = tk_double(&s);
+    int bad = s.fail;
+    free(t);
+    if (bad) { or_destroy(c); set_err(err, errlen, "Burial potential burial_gamma.dat file format error"); return -1; }
+  }
+  if (c->frag_mem_flag) { /* fb.cpp:749-791 */
+    join_path(path, sizeof path, workdir, c->fm_gamma_file);
+    if (gm_load(&c->fmg, path, err, errlen)) { or_destroy(c); return -1; }
+    if (read_mems(c, workdir, c->frag_mems_file, err, errlen)) { or_destroy(c); return -1; }
+    c->ilen_fm_map = (int *)calloc((size_t)n, sizeof(int));
+    c->frag_mem_map = (int **)calloc((size_t)n, sizeof(int *));
+    int min_sep = c->fmg.i_sep[0];
+    for (int k = 0; k < c->n_mems; ++k) {
+      int pos = c->mems[k].pos, len = c->mems[k].len;
+      if (pos + len > n) { or_destroy(c); set_err(err, errlen, "Fragment_Memory: Incorrectly defined memory fragment"); return -1; }
+      for (int i = pos; i < pos + len - min_sep; ++i) {
+        c->ilen_fm_map[i]++;
+        c->frag_mem_map[i] = (int *)realloc(c->frag_mem_map[i], (size_t)c->ilen_fm_map[i] * sizeof(int));
+        c->frag_mem_map[i][c->ilen_fm_map[i] - 1] = k;
+      }
+    }
+  }
+  /* fb.cpp:1048-1066 */
+  double plc = 8.0;
+  if (c->dssp_hdrgn_flag) plc = fmax(plc, c->dssp_hdrgn_cut);
+  if (c->p_ap_flag) plc = fmax(plc, sqrt(c->pap_cutoff_sq));
+  if (c->water_flag) for (int i = 0; i < c->n_wells; ++i) plc = fmax(plc, c->w_rmax_theta[i]);
+  if (c->helix_flag) plc = fmax(plc, c->h_rmax_theta);
+  if (c->helix_flag) plc = fmax(plc, calc_exp_helix_cutoff(c));
+  if (c->helix_flag) plc = fmax(plc, c->helix_cutoff);
+  c->pair_list_cutoff = plc;
+
+  /* excluded volume deck defaults (CoordinatesToWorkLammpsDataFile.py:314-322) */
+  c->ev_p = 2;
+  memset(c->ev_lambda, 0, sizeof c->ev_lambda);
+  for (int a = 0; a < 6; ++a) for (int b = 0; b < 6; ++b) { c->ev_cut_short[a][b] = 3.5; c->ev_cut_long[a][b] = 3.5; }
+  int tp[4][2] = {{1, 1}, {1, 4}, {4, 4}, {3, 3}};
+  double cl[4] = {4.5, 4.5, 4.5, 3.5};
+  for (int k = 0; k < 4; ++k) {
+    int a = tp[k][0], b = tp[k][1];
+    c->ev_lambda[a][b] = c->ev_lambda[b][a] = 20.0;
+    c->ev_cut_short[a][b] = c->ev_cut_short[b][a] = 3.5;
+    c->ev_cut_long[a][b] = c->ev_cut_long[b][a] = cl[k];
+  }
+  c->bond_k = 200.0;
+  c->bond_r0[0] = 3.77; c->bond_r0[1] = 2.41; c->bond_r0[2] = 2.50; c->bond_r0[3] = 1.54; c->bond_r0[4] = 1.54;
+
+  c->xn = (double *)calloc((size_t)n * 3, sizeof(double));
+  c->xh = (double *)calloc((size_t)n * 3, sizeof(double));
+  c->xcp = (double *)calloc((size_t)n * 3, sizeof(double));
+  double **arrs[] = {&c->water_ro, &c->helix_ro, &c->w_sig, &c->w_sig_prd, &c->h_sig, &c->h_sig_prd,
+                     &c->water_xi, &c->helix_xi_1, &c->helix_xi_2, &c->burial_force};
+  for (size_t k = 0; k < sizeof arrs / sizeof arrs[0]; ++k) *arrs[k] = (double *)calloc((size_t)n, sizeof(double));
+  *out = c;
+  return 0;
+}
+
+void or_destroy(or_ctx *c)
+{
+  if (!c) return;
+  for (int k = 0; k < c->n_mems; ++k) { free(c->mems[k].se); free(c->mems[k].rf0); free(c->mems[k].rf1); }
+  free(c->mems);
+  if (c->frag_mem_map) for (int i = 0; i < c->n; ++i) free(c->frag_mem_map[i]);
+  free(c->frag_mem_map); free(c->ilen_fm_map);
+  free(c->fmg.gamma);
+  for (int k = 0; k < 12; ++k) free(c->aps[k]);
+  free(c->se); free(c->chain_no);
+  free(c->xn); free(c->xh); free(c->xcp);
+  free(c->water_ro); free(c->helix_ro); free(c->w_sig); free(c->w_sig_prd); free(c->h_sig); free(c->h_sig_prd);
+  free(c->water_xi); free(c->helix_xi_1); free(c->helix_xi_2); free(c->burial_force);
+  free(c);
+}
+
+int or_n_residues(const or_ctx *c) { return c->n; }
+int or_chain_of(const or_ctx *c, int i) { return c->chain_no[i] - 1; }
+char or_seq_at(const or_ctx *c, int i) { return c->se[i]; }
+double or_pair_list_cutoff(const or_ctx *c) { return c->pair_list_cutoff; }
+
+void or_set_vexcluded(or_ctx *c, int p, const double *lam, const double *cs, const double *clg)
+{
+  c->ev_p = p;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      c->ev_lambda[a][b] = lam[a * 6 + b];
+      c->ev_cut_short[a][b] = cs[a * 6 + b];
+      c->ev_cut_long[a][b] = clg[a * 6 + b];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* virtual atoms, fb.cpp:7113-7138                                      */
+
+static void build_virtual(or_ctx *c)
+{
+  int n = c->n;
+  for (int i = 0; i < n; ++i) {
+    int im1 = i > 0 ? i - 1 : -1;
+    if (im1 != -1 && !is_first(c, i)) {
+      for (int k = 0; k < 3; ++k) {
+        XN(i)[k] = c->an * XCA(im1)[k] + c->bn * XCA(i)[k] + c->cn * XO(im1)[k];
+        XH(i)[k] = c->ah * XCA(im1)[k] + c->bh * XCA(i)[k] + c->ch * XO(im1)[k];
+        XCP(im1)[k] = c->ap * XCA(im1)[k] + c->bp * XCA(i)[k] + c->cp * XO(im1)[k];
+      }
+    } else {
+      XN(i)[0] = XN(i)[1] = XN(i)[2] = 0.0;
+      XH(i)[0] = XH(i)[1] = XH(i)[2] = 0.0;
+      if (im1 != -1) XCP(im1)[0] = XCP(im1)[1] = XCP(im1)[2] = 0.0;
+    }
+  }
+  XCP(n - 1)[0] = XCP(n - 1)[1] = XCP(n - 1)[2] = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* local terms                                                          */
+
+/* compute_chain_potential, fb.cpp:2062-2170 */
+static void chain_term(or_ctx *c, int i)
+{
+  double dx[3], r, dr, force;
+  double *E = &c->energy[OR_ET_CHAIN];
+  if (!is_first(c, i) && c->se[i] != 'G') {
+    int im1 = i - 1;
+    for (int k = 0; k < 3; ++k) dx[k] = XN(i)[k] - XCB(i)[k];
+    r = sqrt(dot3(dx, dx));
+    dr = r - c->r_ncb0;
+    force = 2 * c->k_chain[0] * dr / r;
+    *E += c->k_chain[0] * dr * dr;
+    fadd(FCA(im1), -c->an * force, dx);
+    fadd(FO(im1), -c->cn * force, dx);
+    fadd(FCA(i), -c->bn * force, dx);
+    fadd(FCB(i), force, dx);
+  }
+  if (!is_last(c, i) && c->se[i] != 'G') {
+    int ip1 = i + 1;
+    for (int k = 0; k < 3; ++k) dx[k] = XCP(i)[k] - XCB(i)[k];
+    r = sqrt(dot3(dx, dx));
+    dr = r - c->r_cpcb0;
+    force = 2 * c->k_chain[1] * dr / r;
+    *E += c->k_chain[1] * dr * dr;
+    fadd(FCA(ip1), -c->bp * force, dx);
+    fadd(FCA(i), -c->ap * force, dx);
+    fadd(FO(i), -c->cp * force, dx);
+    fadd(FCB(i), force, dx);
+  }
+  if (!is_first(c, i) && !is_last(c, i)) {
+    int im1 = i - 1, ip1 = i + 1;
+    for (int k = 0; k < 3; ++k) dx[k] = XN(i)[k] - XCP(i)[k];
+    r = sqrt(dot3(dx, dx));
+    dr = r - c->r_ncp0;
+    force = 2 * c->k_chain[2] * dr / r;
+    *E += c->k_chain[2] * dr * dr;
+    fadd(FCA(im1), -c->an * force, dx);
+    fadd(FO(im1), -c->cn * force, dx);
+    fadd(FCA(ip1), c->bp * force, dx);
+    fadd(FCA(i), -(c->bn - c->ap) * force, dx);
+    fadd(FO(i), c->cp * force, dx);
+  }
+}
+
+/* compute_shake, fb.cpp:2172-2237 (k_shake is not pre-scaled: 2*epsilon*k) */
+static void shake_term(or_ctx *c, int i)
+{
+  double dx[3], r, dr, force;
+  double ks = c->epsilon * c->k_shake;
+  double *E = &c->energy[OR_ET_SHAKE];
+  if (!is_last(c, i)) {
+    for (int k = 0; k < 3; ++k) dx[k] = XCA(i)[k] - XCA(i + 1)[k];
+    r = sqrt(dot3(dx, dx)); dr = r - c->r_sh1; force = 2 * ks * dr / r;
+    *E += ks * dr * dr;
+    fadd(FCA(i), -force, dx); fadd(FCA(i + 1), force, dx);
+  }
+  for (int k = 0; k < 3; ++k) dx[k] = XCA(i)[k] - XO(i)[k];
+  r = sqrt(dot3(dx, dx)); dr = r - c->r_sh2; force = 2 * ks * dr / r;
+  *E += ks * dr * dr;
+  fadd(FCA(i), -force, dx); fadd(FO(i), force, dx);
+  if (!is_last(c, i)) {
+    for (int k = 0; k < 3; ++k) dx[k] = XO(i)[k] - XCA(i + 1)[k];
+    r = sqrt(dot3(dx, dx)); dr = r - c->r_sh3; force = 2 * ks * dr / r;
+    *E += ks * dr * dr;
+    fadd(FO(i), -force, dx); fadd(FCA(i + 1), force, dx);
+  }
+}
+
+/* compute_chi_potential, fb.cpp:2239-2339 */
+static void chi_term(or_ctx *c, int i)
+{
+  double a[3], b[3], cc[3], axb[3], cxa[3], bxc[3], aprl[3], bprl[3], cprl[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = XCP(i)[k] - XCA(i)[k];
+    b[k] = XCA(i)[k] - XN(i)[k];
+    cc[k] = XCA(i)[k] - XCB(i)[k];
+  }
+  double arvsq = 1.0 / dot3(a, a), brvsq = 1.0 / dot3(b, b), crvsq = 1.0 / dot3(cc, cc);
+  double norm = sqrt(arvsq * brvsq * crvsq);
+  cross3(a, b, axb); cross3(cc, a, cxa); cross3(b, cc, bxc);
+  double chi = dot3(axb, cc) * norm;
+  for (int k = 0; k < 3; ++k) {
+    aprl[k] = norm * bxc[k] - arvsq * chi * a[k];
+    bprl[k] = norm * cxa[k] - brvsq * chi * b[k];
+    cprl[k] = norm * axb[k] - crvsq * chi * cc[k];
+  }
+  double dchi = chi - c->chi0;
+  double force = 2 * c->k_chi * dchi;
+  c->energy[OR_ET_CHI] += c->k_chi * dchi * dchi;
+  if (!is_first(c, i)) {
+    fadd(FCA(i - 1), c->an * force, bprl);
+    fadd(FO(i - 1), c->cn * force, bprl);
+  }
+  if (!is_last(c, i)) fadd(FCA(i + 1), -c->bp * force, aprl);
+  for (int k = 0; k < 3; ++k) FCA(i)[k] -= (cprl[k] + (1 - c->bn) * bprl[k] + (c->ap - 1) * aprl[k]) * force;
+  fadd(FO(i), -c->cp * force, aprl);
+  fadd(FCB(i), force, cprl);
+}
+
+/* calcDihedralAndSlopes, fb.cpp:2341-2453.  slopes[ang][atom][xyz],
+ * atoms CA0=CA(i-1), CA1=CA(i), CA2=CA(i+1), O0=O(i-1), O1=O(i) */
+static double dihedral_slopes(or_ctx *c, int i, int phi, double ys[5][3], double xs[5][3])
+{
+  double a[3], b[3], cc[3], bxa[3], cxa[3], cxb[3];
+  double an = c->an, bn = c->bn, cn = c->cn, ap = c->ap, bp = c->bp, cp = c->cp;
+  for (int k = 0; k < 3; ++k) {
+    if (phi) {
+      a[k] = XCP(i)[k] - XCA(i)[k];
+      b[k] = XCA(i)[k] - XN(i)[k];
+      cc[k] = XN(i)[k] - XCP(i - 1)[k];
+    } else {
+      a[k] = XN(i + 1)[k] - XCP(i)[k];
+      b[k] = XCP(i)[k] - XCA(i)[k];
+      cc[k] = XCA(i)[k] - XN(i)[k];
+    }
+  }
+  double adb = dot3(a, b), bdc = dot3(b, cc), adc = dot3(a, cc), b2 = dot3(b, b), bm = sqrt(b2);
+  cross3(b, a, bxa); cross3(cc, a, cxa); cross3(cc, b, cxb);
+  double cdbxa = dot3(cc, bxa);
+  double Y = bm * cdbxa, X = adb * bdc - b2 * adc;
+  double angle = atan2(Y, X);
+  double X2Y2 = 1.0 / (X * X + Y * Y);
+  double dy = X * X2Y2, dxx = -Y * X2Y2;
+  b2 *= dxx; adb *= dxx; adc *= dxx; bdc *= dxx;
+  cdbxa *= dy / bm;
+  bm *= dy;
+  for (int l = 0; l < 3; l++) {
+    double h1, h2, h3, h4;
+    if (phi) {
+      h1 = cxb[l] * bm; h2 = cxa[l] * bm; h3 = bxa[l] * bm; h4 = b[l] * cdbxa;
+      ys[0][l] = -an * h4 + (an - ap) * h3 + an * h2;
+      ys[1][l] = (1.0 - bn) * h4 + (bn - bp) * h3 + (ap - 1.0) * h1 - (1.0 - bn) * h2;
+      ys[2][l] = bp * h1;
+      ys[3][l] = -cn * h4 + (cn - cp) * h3 + cn * h2;
+      ys[4][l] = cp * h1;
+      h1 = b[l] * bdc - cc[l] * b2;
+      h2 = a[l] * bdc - 2.0 * b[l] * adc + cc[l] * adb;
+      h3 = b[l] * adb - a[l] * b2;
+      xs[0][l] = -an * h2 + (an - ap) * h3;
+      xs[1][l] = (ap - 1.0) * h1 + (1.0 - bn) * h2 + (bn - bp) * h3;
+      xs[2][l] = bp * h1;
+      xs[3][l] = -cn * h2 + (cn - cp) * h3;
+      xs[4][l] = cp * h1;
+    } else {
+      h1 = bxa[l] * bm; h2 = cxb[l] * bm; h3 = cxa[l] * bm; h4 = b[l] * cdbxa;
+      ys[0][l] = -an * h1;
+      ys[1][l] = (ap - 1.0) * h4 + (1.0 - bn) * h1 + (an - ap) * h2 - (ap - 1) * h3;
+      ys[2][l] = bp * h4 + (bn - bp) * h2 - bp * h3;
+      ys[3][l] = -cn * h1;
+      ys[4][l] = cp * h4 + (cn - cp) * h2 - cp * h3;
+      h1 = b[l] * bdc - cc[l] * b2;
+      h2 = a[l] * bdc - 2.0 * b[l] * adc + cc[l] * adb;
+      h3 = b[l] * adb - a[l] * b2;
+      xs[0][l] = -an * h3;
+      xs[1][l] = (an - ap) * h1 + (ap - 1.0) * h2 + (1.0 - bn) * h3;
+      xs[2][l] = (bn - bp) * h1 + bp * h2;
+      xs[3][l] = -cn * h3;
+      xs[4][l] = (cn - cp) * h1 + cp * h2;
+    }
+  }
+  return angle;
+}
+
+/* compute_rama_potential / compute_rama_force, fb.cpp:2455-2546 */
+static void rama_term(or_ctx *c, int i)
+{
+  double ys[2][5][3], xs[2][5][3];
+  double phi = dihedral_slopes(c, i, 1, ys[0], xs[0]);
+  double psi = dihedral_slopes(c, i, 0, ys[1], xs[1]);
+  int js = 0, je = c->n_rama_par;
+  if (c->se[i] == 'P' && c->rama_p_flag) { js = 6; je = 6 + c->n_rama_p_par; }
+  double *atoms[5] = {FCA(i - 1), FCA(i), FCA(i + 1), FO(i - 1), FO(i)};
+  for (int j = js; j < je; j++) {
+    if (c->ssweight[j] && c->aps[j][i] == 0.0) continue;
+    double cphi = cos(phi + c->phi0[j]) - 1.0, cpsi = cos(psi + c->psi0[j]) - 1.0;
+    double pc = c->phiw[j] * cphi, sc = c->psiw[j] * cpsi;
+    double V = c->w[j] * exp(-cphi * pc - cpsi * sc);
+    if (c->ssweight[j]) V *= c->aps[j][i];
+    double force = 2.0 * V;
+    double f1[2] = {force * pc * sin(phi + c->phi0[j]), force * sc * sin(psi + c->psi0[j])};
+    c->energy[OR_ET_RAMA] += -V;
+    for (int ia = 0; ia < 2; ++ia)
+      for (int at = 0; at < 5; ++at)
+        for (int l = 0; l < 3; ++l) atoms[at][l] += f1[ia] * (ys[ia][at][l] + xs[ia][at][l]);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* fragment memory, fb.cpp:4079-4177                                    */
+
+static void fm_term(or_ctx *c, int i)
+{
+  int n = c->n;
+  int itype = restype(c->se[i]);
+  int maxsep = c->fmg.i_sep[c->fmg.nseq_cl], minsep = c->fmg.i_sep[0];
+  const int ity[4] = {FM_CA, FM_CA, FM_CB, FM_CB}, jty[4] = {FM_CA, FM_CB, FM_CA, FM_CB};
+  for (int ifm = 0; ifm < c->ilen_fm_map[i]; ++ifm) {
+    const or_mem *m = &c->mems[c->frag_mem_map[i][ifm]];
+    double ekw = c->epsilon * c->k_frag_mem * m->weight;
+    int js = i + minsep, je = m->pos + m->len - 1;
+    if (maxsep != -1 && je > i + maxsep) je = i + maxsep;
+    if (je >= n) continue; /* reference aborts: "Missing residues in memory potential" */
+    for (int j = js; j <= je; ++j) {
+      int jtype = restype(c->se[j]);
+      double sig2 = pow(abs(i - j), 2 * c->fm_sigma_exp);
+      double g;
+      if (!c->fmg.frag_resty) g = gm_get2(&c->fmg, itype, jtype, i, j);
+      else g = gm_get4(&c->fmg, itype, jtype, restype(m->se[i - m->pos]), restype(m->se[j - m->pos]), i, j);
+      double ekwg = ekw * g;
+      const double *xi[4] = {XCA(i), XCA(i), XCB(i), XCB(i)}, *xj[4] = {XCA(j), XCB(j), XCA(j), XCB(j)};
+      double *fi[4] = {FCA(i), FCA(i), FCB(i), FCB(i)}, *fj[4] = {FCA(j), FCB(j), FCA(j), FCB(j)};
+      for (int k = 0; k < 4; ++k) {
+        if (ity[k] == FM_CB && (c->se[i] == 'G' || m->se[i - m->pos] == 'G')) continue;
+        if (jty[k] == FM_CB && (c->se[j] == 'G' || m->se[j - m->pos] == 'G')) continue;
+        double dx[3] = {xi[k][0] - xj[k][0], xi[k][1] - xj[k][1], xi[k][2] - xj[k][2]};
+        double r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        double rf = mem_rf(m, i, ity[k], j, jty[k]);
+        double dr = r - rf, drsq = dr * dr;
+        double V = -ekwg * exp(-drsq / (2.0 * sig2));
+        c->energy[OR_ET_FRAGMEM] += V;
+        double force = V * dr / (sig2 * r);
+        fadd(fi[k], force, dx);
+        fadd(fj[k], -force, dx);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* pair machinery                                                       */
+
+/* cWell::compute_theta_pair, sml.h:594-610 */
+static void theta_pair(double kappa, double rmin, double rmax, double rmin_th, double rmax_th, double r,
+                       double *th, double *prd)
+{
+  if (r < rmin_th || r > rmax_th) { *th = 0.0; *prd = 0.0; return; }
+  double tmin = tanh(kappa * (r - rmin)), tmax = tanh(kappa * (rmax - r));
+  *th = 0.25 * (1.0 + tmin) * (1.0 + tmax);
+  *prd = kappa * (*th) * (tmax - tmin) / r;
+}
+
+static inline void wtheta(const or_ctx *c, int w, double r, double *th, double *prd)
+{
+  theta_pair(c->water_kappa, c->well_r_min[w], c->well_r_max[w], c->w_rmin_theta[w], c->w_rmax_theta[w], r, th, prd);
+}
+static inline void htheta(const or_ctx *c, double r, double *th, double *prd)
+{
+  theta_pair(c->helix_kappa, c->helix_well_r_min[0], c->helix_well_r_max[0], c->h_rmin_theta, c->h_rmax_theta, r, th, prd);
+}
+
+void or_theta_pair(const or_ctx *c, int helix, int well, double r, double *theta, double *prd_theta)
+{
+  if (helix) htheta(c, r, theta, prd_theta);
+  else wtheta(c, well, r, theta, prd_theta);
+}
+
+/* density site: CB, or CA for Gly (fb.cpp:7738) */
+static inline const double *site(const or_ctx *c, int i) { return c->se[i] == 'G' ? XCA(i) : XCB(i); }
+static inline double *fsite(or_ctx *c, int i) { return c->se[i] == 'G' ? FCA(i) : FCB(i); }
+
+static inline double rsq_of(const double *a, const double *b)
+{
+  double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+/* cP_AP::compute, sml.h:176-203 */
+static void pap_nu_pair(const or_ctx *c, int i, int j, double *nu, double *prd)
+{
+  double d[3] = {XCA(i)[0] - XCA(j)[0], XCA(i)[1] - XCA(j)[1], XCA(i)[2] - XCA(j)[2]};
+  double drsq = pow(d[0], 2) + pow(d[1], 2) + pow(d[2], 2);
+  or_pap_nu(c, drsq, nu, prd);
+}
+
+void or_pap_nu(const or_ctx *c, double drsq, double *nu, double *prd)
+{
+  if (drsq > c->pap_drmax_sq) { *nu = 0.0; *prd = 0.0; }
+  else if (drsq < c->pap_drmin_sq) { *nu = 1.0; *prd = 0.0; }
+  else {
+    double dr = sqrt(drsq);
+    double th = tanh(c->P_AP_pref * (c->P_AP_cut - dr));
+    *nu = 0.5 * (1 + th);
+    *prd = 0.5 * c->P_AP_pref * (1 - pow(th, 2)) / dr;
+  }
+}
+
+/* cR::rNO / rHO, sml.h:273-301 */
+static inline double rNO(const or_ctx *c, int i, int j)
+{
+  return sqrt(pow(XO(i)[0] - XN(j)[0], 2) + pow(XO(i)[1] - XN(j)[1], 2) + pow(XO(i)[2] - XN(j)[2], 2));
+}
+static inline double rHO(const or_ctx *c, int i, int j)
+{
+  return sqrt(pow(XO(i)[0] - XH(j)[0], 2) + pow(XO(i)[1] - XH(j)[1], 2) + pow(XO(i)[2] - XH(j)[2], 2));
+}
+
+/* DSSP hydrogen bond, compute_dssp_hdrgn fb.cpp:2902-3277 */
+static void dssp_term(or_ctx *c, int i, int j)
+{
+  if (rNO(c, i, j) > c->dssp_hdrgn_cut) return;
+  int i_rep = 1, i_AP = 1, i_P = 1, itheta[4];
+  double lambda[4] = {0, 0, 0, 0}, R_NO[4], R_HO[4], theta[4], nu[2], prdnu[2], prd_theta[4][2], V[4];
+  double dxnu[2][3], xNO[4][3], xHO[4][3], th;
+  double tsa_HB[2], tsa_NHB[2], tsp_HB[2];
+  int hb_class = 0;
+  const char *se = c->se;
+  int ic = c->chain_no[i] - 1, jc = c->chain_no[j] - 1;
+  int i_ch_start = c->ch_pos[ic], j_ch_start = c->ch_pos[jc];
+  int i_ch_end = c->ch_pos[ic] + c->ch_len[ic] - 1, j_ch_end = c->ch_pos[jc] + c->ch_len[jc] - 1;
+  if (is_last(c, j) || se[j + 1] == 'P') i_rep = 0;
+  if (is_first(c, i) || is_last(c, j) || se[i] == 'P') i_AP = 0;
+  if (i >= i_ch_end - 2 || is_last(c, j) || se[i + 2] == 'P') i_P = 0;
+#define ANTI_HB(a, b, k) c->m_anti_HB[restype(a)][restype(b)][k]
+#define ANTI_NHB(a, b, k) c->m_anti_NHB[restype(a)][restype(b)][k]
+#define PARA_HB(a, b, k) c->m_para_HB[restype(a)][restype(b)][k]
+#define ANTI_ONE(a) c->m_anti_one[restype(a)]
+#define PARA_ONE(a) c->m_para_one[restype(a)]
+  for (int k = 0; k < 2; ++k) {
+    if (i_AP) {
+      tsa_HB[k] = 0.5 * ANTI_HB(se[i], se[j], k);
+      tsa_NHB[k] = 0.25 * (ANTI_NHB(se[i + 1], se[j - 1], k) + ANTI_NHB(se[i - 1], se[j + 1], k));
+    } else tsa_HB[k] = tsa_NHB[k] = 0.0;
+    tsp_HB[k] = i_P ? PARA_HB(se[i + 1], se[j], k) : 0.0;
+  }
+  const double (*h)[9] = c->hbscl;
+  int nr = c->n_rama_par;
+  if (ic == jc && abs(j - i) < 45) {
+    if (abs(j - i) < 4) {
+      lambda[0] = -h[0][0]; lambda[1] = -h[0][1]; lambda[2] = 0.0; lambda[3] = 0.0;
+      hb_class = 1;
+    } else if (abs(j - i) < 18) {
+      if (nr > 0 && c->aps[nr - 1][i] == 1.0 && c->aps[nr - 1][j] == 1.0) {
+        lambda[0] = -h[1][0];
+        lambda[1] = -h[1][1];
+        lambda[2] = -h[1][2] - h[1][3] * tsa_HB[0] - h[1][4] * tsa_NHB[0] - h[1][5] * (ANTI_ONE(se[i]) + ANTI_ONE(se[j]));
+        lambda[3] = -h[1][6];
+      } else {
+        lambda[0] = 0.0; lambda[1] = -h[1][1]; lambda[2] = 0.0; lambda[3] = 0.0;
+      }
+      hb_class = 2;
+    } else {
+      lambda[0] = -h[2][0];
+      lambda[1] = -h[2][1];
+      lambda[2] = -h[2][2] - h[2][3] * tsa_HB[1] - h[2][4] * tsa_NHB[1] - h[2][5] * (ANTI_ONE(se[i]) + ANTI_ONE(se[j]));
+      lambda[3] = -h[2][6] - h[2][7] * tsp_HB[1] - h[2][8] * (PARA_ONE(se[i + 1]) + PARA_ONE(se[j]));
+      hb_class = 3;
+    }
+  } else {
+    lambda[0] = -h[3][0];
+    lambda[1] = -h[3][1];
+    lambda[2] = -h[3][2] - h[3][3] * tsa_HB[1] - h[3][4] * tsa_NHB[1] - h[3][5] * (ANTI_ONE(se[i]) + ANTI_ONE(se[j]));
+    lambda[3] = -h[3][6] - h[3][7] * tsp_HB[1] - h[3][8] * (PARA_ONE(se[i + 1]) + PARA_ONE(se[j]));
+    hb_class = 4;
+  }
+  nu[0] = prdnu[0] = 0.0;
+  int nu0_on = (i - 2 >= i_ch_start - 1 && i + 2 < i_ch_end && hb_class != 2);
+  if (nu0_on) {
+    for (int k = 0; k < 3; ++k) dxnu[0][k] = XCA(i + 2)[k] - XCA(i - 2)[k];
+    double rr = dot3(dxnu[0], dxnu[0]);
+    if (rr > c->dssp_nu_cut1_sq) {
+      rr = sqrt(rr);
+      th = tanh(c->pref[0] * (rr - c->d_nu0));
+      nu[0] = 0.5 * (1.0 + th);
+      prdnu[0] = c->pref[0] * nu[0] * (1.0 - th) / rr;
+    }
+  } else nu[0] = 1.0;
+  nu[1] = prdnu[1] = 0.0;
+  int nu1_on = (j - 2 >= j_ch_start - 1 && j + 2 < j_ch_end && hb_class != 2);
+  if (nu1_on) {
+    for (int k = 0; k < 3; ++k) dxnu[1][k] = XCA(j + 2)[k] - XCA(j - 2)[k];
+    double rr = dot3(dxnu[1], dxnu[1]);
+    if (rr > c->dssp_nu_cut2_sq) {
+      rr = sqrt(rr);
+      th = tanh(c->pref[1] * (rr - c->d_nu0));
+      nu[1] = 0.5 * (1.0 + th);
+      prdnu[1] = c->pref[1] * nu[1] * (1.0 - th) / rr;
+    }
+  } else nu[1] = 1.0;
+  if (nu[0] < DSSP_NU_DELTA || nu[1] < DSSP_NU_DELTA) return;
+
+  itheta[0] = 1; itheta[1] = i_rep; itheta[2] = i_AP; itheta[3] = i_P;
+  /* geometry of the four H-bond channels: (acceptor O, donor N/H) */
+  int acc[4] = {i, i, j, j}, don[4] = {j, j + 1, i, i + 2};
+  for (int k = 0; k < 4; ++k) {
+    if (!itheta[k]) continue;
+    R_NO[k] = rNO(c, acc[k], don[k]);
+    R_HO[k] = rHO(c, acc[k], don[k]);
+    for (int l = 0; l < 3; ++l) {
+      xNO[k][l] = XO(acc[k])[l] - XN(don[k])[l];
+      xHO[k][l] = XO(acc[k])[l] - XH(don[k])[l];
+    }
+  }
+  for (int k = 0; k < 4; ++k) {
+    if (itheta[k]) {
+      double dNO = (R_NO[k] - c->NO_zero) * c->sigma_NO_sqinv;
+      double dHO = (R_HO[k] - c->HO_zero) * c->sigma_HO_sqinv;
+      theta[k] = exp(-0.5 * ((R_NO[k] - c->NO_zero) * dNO + (R_HO[k] - c->HO_zero) * dHO));
+      prd_theta[k][0] = -dNO / R_NO[k];
+      prd_theta[k][1] = -dHO / R_HO[k];
+    } else {
+      theta[k] = 0.0;
+      prd_theta[k][0] = prd_theta[k][1] = 0.0;
+    }
+  }
+  double ff1 = c->k_dssp * theta[0];
+  V[0] = lambda[0];
+  V[1] = lambda[1] * theta[1];
+  V[2] = lambda[2] * theta[2];
+  V[3] = lambda[3] * theta[3];
+  double theta_sum = ff1 * (V[0] + V[1] + V[2] + V[3]);
+  ff1 *= nu[0] * nu[1];
+  for (int k = 0; k < 4; ++k) V[k] *= ff1;
+  double VT = V[0] + V[1] + V[2] + V[3];
+  if (ON(OR_ET_DSSP)) c->energy[OR_ET_DSSP] += VT;
+  if (!ON(OR_ET_DSSP)) return;
+  if (nu0_on) {
+    double f = theta_sum * prdnu[0] * nu[1];
+    fadd(FCA(i - 2), f, dxnu[0]);
+    fadd(FCA(i + 2), -f, dxnu[0]);
+  }
+  if (nu1_on) {
+    double f = theta_sum * nu[0] * prdnu[1];
+    fadd(FCA(j - 2), f, dxnu[1]);
+    fadd(FCA(j + 2), -f, dxnu[1]);
+  }
+  double Vk[4] = {VT, V[1], V[2], V[3]};
+  for (int k = 0; k < 4; ++k) {
+    if (!itheta[k]) continue;
+    double p0 = prd_theta[k][0] * Vk[k], p1 = prd_theta[k][1] * Vk[k];
+    int a = acc[k], d = don[k];
+    for (int l = 0; l < 3; ++l) {
+      FO(a)[l] -= p0 * xNO[k][l] + p1 * xHO[k][l];
+      FCA(d - 1)[l] += c->an * p0 * xNO[k][l] + c->ah * p1 * xHO[k][l];
+      FCA(d)[l] += c->bn * p0 * xNO[k][l] + c->bh * p1 * xHO[k][l];
+      FO(d - 1)[l] += c->cn * p0 * xNO[k][l] + c->ch * p1 * xHO[k][l];
+    }
+  }
+}
+
+/* P_AP, compute_P_AP_potential fb.cpp:3279-3383 (i<j by residue index) */
+static void pap_term(or_ctx *c, int i, int j)
+{
+  double nu_ij, prd_ij;
+  pap_nu_pair(c, i, j, &nu_ij, &prd_ij);
+  if (nu_ij < PAP_DELTA) return;
+  int n = c->n, d = c->i_diff_P_AP;
+  int ic = c->chain_no[i] - 1, jc = c->chain_no[j] - 1;
+  int i_ch_end = c->ch_pos[ic] + c->ch_len[ic] - 1, j_ch_end = c->ch_pos[jc] + c->ch_len[jc] - 1;
+  int mn = c->i_med_min, mx = c->i_med_max;
+  int jmed_hi = (i + mx + 2 * d) < (i_ch_end - 1) ? (i + mx + 2 * d) : (i_ch_end - 1);
+  int i_AP_med = ic == jc && i < i_ch_end - (mn + 2 * d) && j >= i + (mn + 2 * d) && j <= jmed_hi;
+  int i_AP_long = (ic == jc && i < i_ch_end - (mx + 2 * d + 1) && j >= i + (mx + 2 * d + 1) && j < j_ch_end) ||
+                  (ic != jc && i + d < n && j - d >= 0 && (c->chain_no[i + d] - 1) == ic && (c->chain_no[j - d] - 1) == jc);
+  int i_P = (ic == jc && i < i_ch_end - (mx + 1 + d) && j >= i + (mx + 1) && j < i_ch_end - d) ||
+            (ic != jc && i + d < n && j + d < n && (c->chain_no[i + d] - 1) == ic && (c->chain_no[j + d] - 1) == jc);
+  int nr = c->n_rama_par;
+  int beta = nr > 0 && c->aps[nr - 1][i] == 1.0 && c->aps[nr - 1][j] == 1.0;
+  if (!ON(OR_ET_PAP)) return;
+  if (i_AP_med || i_AP_long) {
+    double K = beta ? (i_AP_med ? c->k_P_AP[0] : 0.0) + (i_AP_long ? c->k_P_AP[1] * c->k_betapred_P_AP : 0.0)
+                    : (i_AP_med ? c->k_P_AP[0] : 0.0) + (i_AP_long ? c->k_P_AP[1] : 0.0);
+    double nu2, prd2;
+    pap_nu_pair(c, i + d, j - d, &nu2, &prd2);
+    c->energy[OR_ET_PAP] += -c->k_global_P_AP * K * nu_ij * nu2;
+    double dx0[3], dx1[3];
+    for (int l = 0; l < 3; ++l) { dx0[l] = XCA(i)[l] - XCA(j)[l]; dx1[l] = XCA(i + d)[l] - XCA(j - d)[l]; }
+    double f0 = c->k_global_P_AP * K * prd_ij * nu2, f1 = c->k_global_P_AP * K * nu_ij * prd2;
+    fadd(FCA(i), -f0, dx0); fadd(FCA(j), f0, dx0);
+    fadd(FCA(i + d), -f1, dx1); fadd(FCA(j - d), f1, dx1);
+  }
+  if (i_P) {
+    double K = beta ? c->k_P_AP[2] * c->k_betapred_P_AP : c->k_P_AP[2];
+    double nu2, prd2;
+    pap_nu_pair(c, i + d, j + d, &nu2, &prd2);
+    c->energy[OR_ET_PAP] += -c->k_global_P_AP * K * nu_ij * nu2;
+    double dx0[3], dx1[3];
+    for (int l = 0; l < 3; ++l) { dx0[l] = XCA(i)[l] - XCA(j)[l]; dx1[l] = XCA(i + d)[l] - XCA(j + d)[l]; }
+    double f0 = c->k_global_P_AP * K * prd_ij * nu2, f1 = c->k_global_P_AP * K * nu_ij * prd2;
+    fadd(FCA(i), -f0, dx0); fadd(FCA(j), f0, dx0);
+    fadd(FCA(i + d), -f1, dx1); fadd(FCA(j + d), f1, dx1);
+  }
+}
+
+/* compute_helix_dtheta_pair, fb.cpp:3706-3785 */
+static void helix_term(or_ctx *c, int i, int j)
+{
+  if (rNO(c, i, j) > c->helix_cutoff) return;
+  if (c->chain_no[i] != c->chain_no[j]) return;
+  int it = restype(c->se[i]), jt = restype(c->se[j]);
+  double R_NO = rNO(c, i, j), R_HO = rHO(c, i, j), xNO[3], xHO[3];
+  for (int l = 0; l < 3; ++l) { xNO[l] = XO(i)[l] - XN(j)[l]; xHO[l] = XO(i)[l] - XH(j)[l]; }
+  double h4i = c->h4prob[it];
+  if (c->se[i] == 'P' && c->pro_accepter_flag) h4i = c->h4prob_pro_accepter;
+  double prob_sum = h4i + c->h4prob[jt];
+  double dNO = R_NO - c->helix_NO_zero, dHO = R_HO - c->helix_HO_zero;
+  double dNOs = dNO * c->helix_sigma_NO_sqinv, dHOs = dHO * c->helix_sigma_HO_sqinv;
+  double pt = -c->k_helix * prob_sum * exp(-0.5 * (dNO * dNOs + dHO * dHOs));
+  double p0 = -dNOs / R_NO, p1 = -dHOs / R_HO;
+  double gdiff = c->helix_gamma_w - c->helix_gamma_p;
+  double sg = c->helix_gamma_p + gdiff * c->h_sig[i] * c->h_sig[j];
+  c->helix_xi_1[i] = gdiff * pt * c->h_sig_prd[i] * c->h_sig[j];
+  c->helix_xi_2[i] = gdiff * pt * c->h_sig[i] * c->h_sig_prd[j];
+  double V = sg * pt;
+  if (!ON(OR_ET_HELIX)) return;
+  c->energy[OR_ET_HELIX] += V;
+  for (int l = 0; l < 3; ++l) {
+    FCA(j - 1)[l] += V * (c->an * p0 * xNO[l] + c->ah * p1 * xHO[l]);
+    FCA(j)[l] += V * (c->bn * p0 * xNO[l] + c->bh * p1 * xHO[l]);
+    FO(j - 1)[l] += V * (c->cn * p0 * xNO[l] + c->ch * p1 * xHO[l]);
+    FO(i)[l] -= V * (p0 * xNO[l] + p1 * xHO[l]);
+  }
+}
+
+/* helix pair selection, fb.cpp:7843-7868 (j = i + helix_i_diff contiguous
+ * on the same chain; the reference also requires the O-N distance below
+ * helix_cutoff before calling compute_helix_dtheta_pair) */
+static int helix_pair_ok(const or_ctx *c, int i)
+{
+  int j = i + c->helix_i_diff;
+  if (j >= c->n) return 0;
+  if (c->chain_no[i] != c->chain_no[j]) return 0;
+  if (rsq_of(XO(i), XN(j)) >= c->helix_cutoff_sq) return 0;
+  return 1;
+}
+
+/* compute_pair, fb.cpp:7677-8135 */
+static void pair_terms(or_ctx *c)
+{
+  int n = c->n;
+  for (int i = 0; i < n; ++i) {
+    c->water_ro[i] = c->helix_ro[i] = c->w_sig[i] = c->w_sig_prd[i] = c->h_sig[i] = c->h_sig_prd[i] = 0.0;
+    c->water_xi[i] = c->helix_xi_1[i] = c->helix_xi_2[i] = c->burial_force[i] = 0.0;
+  }
+  /* DL1: densities, fb.cpp:7729-7786 */
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      if (!(c->chain_no[i] != c->chain_no[j] || abs(i - j) > 1)) continue;
+      double rsq = rsq_of(site(c, i), site(c, j)), th, prd;
+      if (c->water_flag && rsq > c->w_rmin_theta_sq[0] && rsq < c->w_rmax_theta_sq[0]) {
+        wtheta(c, 0, sqrt(rsq), &th, &prd);
+        c->water_ro[i] += th; c->water_ro[j] += th;
+      }
+      if (c->helix_flag && rsq > c->h_rmin_theta_sq && rsq < c->h_rmax_theta_sq) {
+        htheta(c, sqrt(rsq), &th, &prd);
+        c->helix_ro[i] += th; c->helix_ro[j] += th;
+      }
+    }
+  /* SL: sigma and burial, fb.cpp:7795-7841 */
+  for (int i = 0; i < n; ++i) {
+    int it = restype(c->se[i]);
+    double th;
+    if (c->water_flag) {
+      th = tanh(c->water_kappa_sigma * (c->water_ro[i] - c->treshold));
+      c->w_sig[i] = 0.5 * (1.0 - th);
+      c->w_sig_prd[i] = -c->water_kappa_sigma * c->w_sig[i] * (1.0 + th);
+    }
+    if (c->helix_flag) {
+      th = tanh(c->helix_kappa_sigma * (c->helix_ro[i] - c->helix_treshold));
+      c->h_sig[i] = 0.5 * (1.0 - th);
+      c->h_sig_prd[i] = -c->helix_kappa_sigma * c->h_sig[i] * (1.0 + th);
+    }
+    if (c->burial_flag) {
+      double t[3][2], rho = c->water_ro[i], kb = c->burial_kappa;
+      for (int b = 0; b < 3; ++b) {
+        t[b][0] = tanh(kb * (rho - c->burial_ro_min[b]));
+        t[b][1] = tanh(kb * (c->burial_ro_max[b] - rho));
+      }
+      if (ON(OR_ET_BURIAL))
+        for (int b = 0; b < 3; ++b) c->energy[OR_ET_BURIAL] += -0.5 * c->k_burial * c->burial_gamma[it][b] * (t[b][0] + t[b][1]);
+      double bf = 0.0;
+      for (int b = 0; b < 3; ++b) bf += c->burial_gamma[it][b] * (t[b][1] * t[b][1] - t[b][0] * t[b][0]);
+      c->burial_force[i] = bf * 0.5 * c->k_burial * kb;
+    }
+  }
+  /* helix (i, i+4), fb.cpp:7843-7868 */
+  if (c->helix_flag)
+    for (int i = 0; i < n; ++i)
+      if (helix_pair_ok(c, i)) helix_term(c, i, i + c->helix_i_diff);
+  /* DL2: water xi, fb.cpp:7879-7952 */
+  if (c->water_flag) {
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        if (!(c->chain_no[i] != c->chain_no[j] || abs(i - j) >= c->contact_cutoff)) continue;
+        double rsq = rsq_of(site(c, i), site(c, j));
+        int it = restype(c->se[i]), jt = restype(c->se[j]);
+        for (int w = 0; w < c->n_wells; ++w) {
+          if (!c->well_flag[w]) continue;
+          double g0 = c->water_gamma[w][it][jt][0], g1 = c->water_gamma[w][it][jt][1];
+          /* the reference predicate !fabs(g0-g1)<delta parses as (!fabs(g0-g1)) < delta */
+          if (((double)(!fabs(g0 - g1)) < DELTA) && rsq > c->w_rmin_theta_sq[w] && rsq < c->w_rmax_theta_sq[w]) {
+            double th, prd;
+            wtheta(c, w, sqrt(rsq), &th, &prd);
+            double tg = (g1 - g0) * th;
+            c->water_xi[i] += tg * c->w_sig[j];
+            c->water_xi[j] += tg * c->w_sig[i];
+          }
+        }
+      }
+    for (int i = 0; i < n; ++i) c->water_xi[i] *= c->w_sig_prd[i];
+  }
+  /* DL3: main loop, fb.cpp:7958-8132 */
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      int same = c->chain_no[i] == c->chain_no[j];
+      int dres = j - i;
+      if (!same || dres > 1) {
+        const double *si = site(c, i), *sj = site(c, j);
+        double dx[3] = {si[0] - sj[0], si[1] - sj[1], si[2] - sj[2]};
+        double rsq = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        double force = 0.0, r = 0.0;
+        int br = 0;
+        int it = restype(c->se[i]), jt = restype(c->se[j]);
+        if (c->water_flag) {
+          for (int w = 0; w < c->n_wells; ++w) {
+            if (!c->well_flag[w]) continue;
+            if ((!same || dres >= c->contact_cutoff) && rsq > c->w_rmin_theta_sq[w] && rsq < c->w_rmax_theta_sq[w]) {
+              double g0 = c->water_gamma[w][it][jt][0], g1 = c->water_gamma[w][it][jt][1], sg;
+              if (fabs(g0 - g1) < DELTA) sg = 0.5 * (g0 + g1);
+              else sg = g0 + (g1 - g0) * c->w_sig[i] * c->w_sig[j];
+              if (!br) { r = sqrt(rsq); br = 1; }
+              double th, prd;
+              wtheta(c, w, r, &th, &prd);
+              if (ON(OR_ET_WATER)) { c->energy[OR_ET_WATER] += -sg * th; force += sg * prd; }
+            }
+          }
+          if (rsq > c->w_rmin_theta_sq[0] && rsq < c->w_rmax_theta_sq[0] && ON(OR_ET_WATER)) {
+            double th, prd;
+            if (!br) { r = sqrt(rsq); br = 1; }
+            wtheta(c, 0, r, &th, &prd);
+            if (fabs(c->water_xi[i]) > DELTA_WATER_XI) force += prd * c->water_xi[i];
+            if (fabs(c->water_xi[j]) > DELTA_WATER_XI) force += prd * c->water_xi[j];
+          }
+        }
+        if (c->burial_flag && rsq > c->w_rmin_theta_sq[0] && rsq < c->w_rmax_theta_sq[0] && ON(OR_ET_BURIAL)) {
+          double th, prd;
+          if (!br) { r = sqrt(rsq); br = 1; }
+          wtheta(c, 0, r, &th, &prd);
+          force += (c->burial_force[i] + c->burial_force[j]) * prd;
+        }
+        if (c->helix_flag && rsq > c->h_rmin_theta_sq && rsq < c->h_rmax_theta_sq && ON(OR_ET_HELIX)) {
+          int hd = c->helix_i_diff;
+          double factor = c->helix_xi_1[i] + c->helix_xi_1[j];
+          if (i - hd >= 0) factor += c->helix_xi_2[i - hd];
+          if (j - hd >= 0) factor += c->helix_xi_2[j - hd];
+          if (fabs(factor) > DELTA_HELIX_XI) {
+            double th, prd;
+            if (!br) { r = sqrt(rsq); br = 1; }
+            htheta(c, r, &th, &prd);
+            force += -factor * prd;
+          }
+        }
+        if (force != 0.0) {
+          fadd(fsite(c, i), force, dx);
+          fadd(fsite(c, j), -force, dx);
+        }
+      }
+      /* DSSP: (O of i, CA of j) and (O of j, CA of i), fb.cpp:8064-8094 */
+      if (c->dssp_hdrgn_flag && (!same || dres > 2)) {
+        /* i acceptor, j donor */
+        if (c->se[j] != 'P' && !is_last(c, i) && !is_first(c, j) && j > 0 &&
+            rsq_of(XO(i), XN(j)) < c->dssp_hdrgn_cut_sq)
+          dssp_term(c, i, j);
+        /* j acceptor, i donor */
+        if (c->se[i] != 'P' && !is_last(c, j) && !is_first(c, i) && i > 0 &&
+            rsq_of(XO(j), XN(i)) < c->dssp_hdrgn_cut_sq)
+          dssp_term(c, j, i);
+      }
+      /* P_AP: CA-CA pairs, fb.cpp:8096-8103 */
+      if (c->p_ap_flag && rsq_of(XCA(i), XCA(j)) < c->pap_cutoff_sq) pap_term(c, i, j);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* PairExcludedVolume::compute (pev.cpp:50-138) and harmonic bonds      */
+
+static inline int atom_type(const or_ctx *c, int res, int kind)
+{
+  if (kind == 0) return 1;
+  if (kind == 2) return 3;
+  return c->se[res] == 'G' ? 5 : 4;
+}
+
+/* 1-2 bonded pairs removed by special_bonds fene (deck folding.in:17):
+ * CA(i)-CA(i+1), CA(i)-O(i), O(i)-CA(i+1), CA(i)-CB(i) on one chain. */
+static int bonded12(const or_ctx *c, int ra, int ka, int rb, int kb)
+{
+  if (ra > rb || (ra == rb && ka > kb)) { int t = ra; ra = rb; rb = t; t = ka; ka = kb; kb = t; }
+  if (ra == rb) return (ka == 0 && (kb == 1 || kb == 2));
+  if (rb != ra + 1 || c->chain_no[ra] != c->chain_no[rb]) return 0;
+  return (ka == 0 && kb == 0) || (ka == 2 && kb == 0);
+}
+
+static void ev_terms(or_ctx *c, long long *list, int cap, int *count)
+{
+  int n = c->n, p = c->ev_p;
+  double sign = (p % 2 == 0) ? 1.0 : -1.0;
+  const double *X[3] = {c->xca, c->xcb, c->xo};
+  double *F[3] = {c->fca, c->fcb, c->fo};
+  int na = 3 * n;
+  for (int a = 0; a < na; ++a) {
+    int ra = a / 3, ka = a % 3;
+    for (int b = a + 1; b < na; ++b) {
+      int rb = b / 3, kb = b % 3;
+      if (bonded12(c, ra, ka, rb, kb)) continue;
+      int ta = atom_type(c, ra, ka), tb = atom_type(c, rb, kb);
+      if (c->ev_lambda[ta][tb] == 0.0) continue; /* contributes exactly zero */
+      const double *xa = X[ka] + 3 * ra, *xb = X[kb] + 3 * rb;
+      double d[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+      double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double r = sqrt(rsq);
+      double rc = (abs(ra - rb) < 5 && c->chain_no[ra] == c->chain_no[rb]) ? c->ev_cut_short[ta][tb] : c->ev_cut_long[ta][tb];
+      if (r < rc) {
+        if (list) {
+          if (*count < cap) list[*count] = (long long)a * na + b;
+          (*count)++;
+          continue;
+        }
+        double dr = r - rc;
+        double pref = sign * c->ev_lambda[ta][tb] * pow(dr, p - 1);
+        double fpair = r > 0.0 ? -pref * p / r : 0.0;
+        fadd(F[ka] + 3 * ra, fpair, d);
+        fadd(F[kb] + 3 * rb, -fpair, d);
+        c->energy[OR_ET_VEXCLUDED] += pref * dr;
+      }
+    }
+  }
+}
+
+static void bond_one(or_ctx *c, double *fa, double *fb, const double *xa, const double *xb, double r0)
+{
+  double d[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+  double r = sqrt(dot3(d, d)), dr = r - r0;
+  c->energy[OR_ET_BOND] += c->bond_k * dr * dr;
+  double fb_ = r > 0.0 ? -2.0 * c->bond_k * dr / r : 0.0;
+  fadd(fa, fb_, d);
+  fadd(fb, -fb_, d);
+}
+
+static void bond_terms(or_ctx *c)
+{
+  for (int i = 0; i < c->n; ++i) {
+    bond_one(c, FCA(i), FO(i), XCA(i), XO(i), c->bond_r0[1]);
+    bond_one(c, FCA(i), FCB(i), XCA(i), XCB(i), c->se[i] == 'G' ? c->bond_r0[4] : c->bond_r0[3]);
+    if (!is_last(c, i)) {
+      bond_one(c, FCA(i), FCA(i + 1), XCA(i), XCA(i + 1), c->bond_r0[0]);
+      bond_one(c, FO(i), FCA(i + 1), XO(i), XCA(i + 1), c->bond_r0[2]);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* compute_backbone, fb.cpp:7047-7673 (non-DEBUGFORCES path)            */
+
+int or_compute(or_ctx *c, const double *xca, const double *xcb, const double *xo,
+               double *fca, double *fcb, double *fo, double *energy, unsigned mask)
+{
+  c->xca = xca; c->xcb = xcb; c->xo = xo;
+  c->fca = fca; c->fcb = fcb; c->fo = fo;
+  c->energy = energy;
+  c->mask = mask;
+  for (int k = 0; k < OR_N_ET; ++k) energy[k] = 0.0;
+  build_virtual(c);
+  for (int i = 0; i < c->n; ++i) {
+    int first = is_first(c, i), last = is_last(c, i);
+    if (c->chain_flag && ON(OR_ET_CHAIN)) chain_term(c, i);
+    if (!first && !last && c->chi_flag && c->se[i] != 'G' && ON(OR_ET_CHI)) chi_term(c, i);
+    if (c->shake_flag && ON(OR_ET_SHAKE)) shake_term(c, i);
+    if (!first && !last && c->rama_flag && c->se[i] != 'G' && ON(OR_ET_RAMA)) rama_term(c, i);
+    if (c->frag_mem_flag && ON(OR_ET_FRAGMEM)) fm_term(c, i);
+  }
+  pair_terms(c);
+  if (ON(OR_ET_VEXCLUDED)) ev_terms(c, NULL, 0, NULL);
+  if (ON(OR_ET_BOND)) bond_terms(c);
+  for (int k = 1; k < OR_N_ET; ++k) energy[OR_ET_TOTAL] += energy[k];
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* integer lists                                                        */
+
+static int cmp_ll(const void *a, const void *b)
+{
+  long long x = *(const long long *)a, y = *(const long long *)b;
+  return x < y ? -1 : (x > y);
+}
+
+int or_list(or_ctx *c, int kind, int arg, const double *xca, const double *xcb, const double *xo,
+            long long *out, int cap)
+{
+  c->xca = xca; c->xcb = xcb; c->xo = xo;
+  build_virtual(c);
+  int n = c->n, cnt = 0;
+#define PUSH(v) do { if (cnt < cap) out[cnt] = (v); cnt++; } while (0)
+  if (kind == 5) {
+    ev_terms(c, out, cap, &cnt);
+  } else if (kind == 4) {
+    if (c->helix_flag)
+      for (int i = 0; i < n; ++i)
+        if (helix_pair_ok(c, i) && rNO(c, i, i + c->helix_i_diff) <= c->helix_cutoff) PUSH((long long)i * n + i + c->helix_i_diff);
+  } else {
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        int same = c->chain_no[i] == c->chain_no[j], dres = j - i;
+        double rsq = rsq_of(site(c, i), site(c, j));
+        if (kind == 0 && c->water_flag && (!same || dres > 1) && arg < c->n_wells &&
+            rsq > c->w_rmin_theta_sq[arg] && rsq < c->w_rmax_theta_sq[arg]) PUSH((long long)i * n + j);
+        if (kind == 1 && c->helix_flag && (!same || dres > 1) && rsq > c->h_rmin_theta_sq && rsq < c->h_rmax_theta_sq)
+          PUSH((long long)i * n + j);
+        if (kind == 6 && c->water_flag && (!same || dres >= c->contact_cutoff) && arg < c->n_wells && c->well_flag[arg]) {
+          int it = restype(c->se[i]), jt = restype(c->se[j]);
+          double g0 = c->water_gamma[arg][it][jt][0], g1 = c->water_gamma[arg][it][jt][1];
+          if (((double)(!fabs(g0 - g1)) < DELTA) && rsq > c->w_rmin_theta_sq[arg] && rsq < c->w_rmax_theta_sq[arg])
+            PUSH((long long)i * n + j);
+        }
+        if (kind == 2 && c->dssp_hdrgn_flag && (!same || dres > 2)) {
+          if (c->se[j] != 'P' && !is_last(c, i) && !is_first(c, j) && rsq_of(XO(i), XN(j)) < c->dssp_hdrgn_cut_sq &&
+              !(rNO(c, i, j) > c->dssp_hdrgn_cut)) PUSH((long long)i * n + j);
+          if (c->se[i] != 'P' && !is_last(c, j) && !is_first(c, i) && rsq_of(XO(j), XN(i)) < c->dssp_hdrgn_cut_sq &&
+              !(rNO(c, j, i) > c->dssp_hdrgn_cut)) PUSH((long long)j * n + i);
+        }
+        if (kind == 3 && c->p_ap_flag && rsq_of(XCA(i), XCA(j)) < c->pap_cutoff_sq) PUSH((long long)i * n + j);
+      }
+  }
+#undef PUSH
+  if (cnt > cap) return -cnt;
+  qsort(out, (size_t)cnt, sizeof(long long), cmp_ll);
+  return cnt;
+}
+
+/* ------------------------------------------------------------------ */
+/* fragment-memory introspection                                        */
+
+int or_fm_count(const or_ctx *c) { return c->n_mems; }
+int or_fm_info(const or_ctx *c, int k, int *pos, int *len, double *weight)
+{
+  if (k < 0 || k >= c->n_mems) return -1;
+  *pos = c->mems[k].pos; *len = c->mems[k].len; *weight = c->mems[k].weight;
+  return 0;
+}
+double or_fm_rf(const or_ctx *c, int k, int ires, int iatom, int jres, int jatom)
+{
+  return mem_rf(&c->mems[k], ires, iatom, jres, jatom);
+}
+char or_fm_se(const or_ctx *c, int k, int resno) { return c->mems[k].se[resno - c->mems[k].pos]; }
+double or_fm_gamma(const or_ctx *c, int it, int jt, int ires, int jres) { return gm_get2(&c->fmg, it, jt, ires, jres); }
+int or_fm_minsep(const or_ctx *c) { return c->fmg.i_sep[0]; }
+int or_fm_maxsep(const or_ctx *c) { return c->fmg.i_sep[c->fmg.nseq_cl]; }
+int or_fm_map(const or_ctx *c, int i, int *out, int cap)
+{
+  if (!c->ilen_fm_map) return 0;
+  int n = c->ilen_fm_map[i];
+  for (int k = 0; k < n && k < cap; ++k) out[k] = c->frag_mem_map[i][k];
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Langevin dynamics: LAMMPS fix nve + fix langevin, real units.        */
+/* Counter hash: splitmix64 finaliser over (seed, replica, step, atom,  */
+/* component); uniform in [0,1) from the top 53 bits.                   */
+
+static inline uint64_t mix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+static inline double urand(uint64_t seed, int replica, long long step, int atom, int comp)
+{
+  uint64_t h = mix64(seed ^ 0x9e3779b97f4a7c15ULL);
+  h = mix64(h ^ ((uint64_t)(uint32_t)replica << 32 | (uint32_t)comp));
+  h = mix64(h ^ (uint64_t)step);
+  h = mix64(h ^ (uint64_t)(uint32_t)atom);
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+#define FTM2V (1.0 / 48.88821291 / 48.88821291)
+#define MVV2E (48.88821291 * 48.88821291)
+#define BOLTZ 0.0019872067
+
+static inline double atom_mass(const or_ctx *c, int res, int kind)
+{
+  if (kind == 0) return 27.0;
+  if (kind == 2) return 28.0;
+  return c->se[res] == 'G' ? 2.0 : 60.0;
+}
+
+int or_langevin_run(or_ctx *c, int nsteps, double dt, double t_start, double t_stop, double damp,
+                    unsigned long long seed, int replica, long long step0, double *x, double *v, double *f,
+                    double *energy)
+{
+  int n = c->n;
+  int na = 3 * n;
+  double dtf = 0.5 * dt * FTM2V;
+  unsigned all = 0xffffffffu;
+  for (int s = 0; s < nsteps; ++s) {
+    long long step = step0 + s;
+    /* initial_integrate (fix nve) */
+    for (int a = 0; a < na; ++a) {
+      int kind = a / n, res = a % n;
+      double m = atom_mass(c, res, kind);
+      double dtfm = dtf / m;
+      for (int l = 0; l < 3; ++l) {
+        v[3 * a + l] += dtfm * f[3 * a + l];
+        x[3 * a + l] += dt * v[3 * a + l];
+      }
+    }
+    /* force */
+    for (int k = 0; k < 3 * na; ++k) f[k] = 0.0;
+    or_compute(c, x, x + 3 * n, x + 6 * n, f, f + 3 * n, f + 6 * n, energy, all);
+    /* fix langevin post_force: target T ramps linearly over the run */
+    double delta = nsteps > 0 ? (double)(s + 1) / (double)nsteps : 0.0;
+    double T = t_start + delta * (t_stop - t_start);
+    double tsqrt = sqrt(T);
+    for (int a = 0; a < na; ++a) {
+      int kind = a / n, res = a % n;
+      double m = atom_mass(c, res, kind);
+      double g1 = -m / damp / FTM2V;
+      double g2 = sqrt(m) * sqrt(24.0 * BOLTZ / damp / dt / MVV2E) / FTM2V * tsqrt;
+      for (int l = 0; l < 3; ++l) f[3 * a + l] += g1 * v[3 * a + l] + g2 * (urand(seed, replica, step, a, l) - 0.5);
+    }
+    /* final_integrate */
+    for (int a = 0; a < na; ++a) {
+      int kind = a / n, res = a % n;
+      double dtfm = dtf / atom_mass(c, res, kind);
+      for (int l = 0; l < 3; ++l) v[3 * a + l] += dtfm * f[3 * a + l];
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,165 @@
+"""ctypes wrapper of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product path never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_fm.so")
+
+ET_NAMES = ["Total", "Chain", "Shake", "Chi", "Rama", "Excluded", "DSSP", "P_AP", "Water", "Burial",
+            "Helix", "AMH-Go", "Frag_Mem", "Vec_FM", "Contact_Restraints", "Membrane", "SSB", "Electro",
+            "Bond"]
+N_ET = 19
+ET = {name: k for k, name in enumerate(ET_NAMES)}
+ALL_TERMS = 0xFFFFFFFF
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.or_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
+        L.or_create.restype = C.c_int
+        L.or_destroy.argtypes = [C.c_void_p]
+        L.or_n_residues.argtypes = [C.c_void_p]
+        L.or_compute.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp, dp, C.c_uint]
+        L.or_compute.restype = C.c_int
+        L.or_list.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp,
+                              np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
+        L.or_list.restype = C.c_int
+        L.or_pair_list_cutoff.argtypes = [C.c_void_p]
+        L.or_pair_list_cutoff.restype = C.c_double
+        L.or_fm_count.argtypes = [C.c_void_p]
+        L.or_fm_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.or_fm_rf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_fm_rf.restype = C.c_double
+        L.or_fm_se.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.or_fm_se.restype = C.c_char
+        L.or_fm_gamma.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_fm_gamma.restype = C.c_double
+        L.or_fm_minsep.argtypes = [C.c_void_p]
+        L.or_fm_maxsep.argtypes = [C.c_void_p]
+        L.or_fm_map.argtypes = [C.c_void_p, C.c_int, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_int]
+        L.or_theta_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_pap_nu.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_set_vexcluded.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
+        L.or_langevin_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_ulonglong, C.c_int, C.c_longlong, dp, dp, dp, dp]
+        L.or_langevin_run.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """FP64 FixBackbone restatement on one system (all residues local)."""
+
+    def __init__(self, coeff: str, seq: str, workdir: str):
+        L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = L.or_create(coeff.encode(), seq.encode(), workdir.encode(), C.byref(h), err, 512)
+        if rc != 0:
+            raise OracleError(err.value.decode())
+        self.h = h
+        self.n = L.or_n_residues(h)
+
+    def close(self):
+        if self.h:
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def pair_list_cutoff(self) -> float:
+        return lib().or_pair_list_cutoff(self.h)
+
+    def compute(self, coords: np.ndarray, mask: int = ALL_TERMS):
+        """coords (3, n, 3) [CA, CB, O]; returns forces (3, n, 3), energy (19,)."""
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        f = np.zeros_like(x)
+        e = np.zeros(N_ET)
+        fca, fcb, fo = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        lib().or_compute(self.h, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]), np.ascontiguousarray(x[2]),
+                         fca, fcb, fo, e, mask & 0xFFFFFFFF)
+        f[0], f[1], f[2] = fca, fcb, fo
+        return f, e
+
+    def energy(self, coords, mask=ALL_TERMS):
+        return self.compute(coords, mask)[1]
+
+    def list(self, kind: int, coords: np.ndarray, arg: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        cap = 1 << 16
+        while True:
+            out = np.zeros(cap, dtype=np.int64)
+            k = lib().or_list(self.h, kind, arg, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]),
+                              np.ascontiguousarray(x[2]), out, cap)
+            if k >= 0:
+                return out[:k].copy()
+            cap = -k + 16
+
+    def theta_pair(self, r, helix=False, well=0):
+        t, p = C.c_double(), C.c_double()
+        lib().or_theta_pair(self.h, int(helix), well, r, C.byref(t), C.byref(p))
+        return t.value, p.value
+
+    def pap_nu(self, drsq):
+        a, b = C.c_double(), C.c_double()
+        lib().or_pap_nu(self.h, drsq, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def fm_count(self):
+        return lib().or_fm_count(self.h)
+
+    def fm_info(self, k):
+        p, l, w = C.c_int(), C.c_int(), C.c_double()
+        lib().or_fm_info(self.h, k, C.byref(p), C.byref(l), C.byref(w))
+        return p.value, l.value, w.value
+
+    def fm_rf(self, k, ires, iatom, jres, jatom):
+        return lib().or_fm_rf(self.h, k, ires, iatom, jres, jatom)
+
+    def fm_se(self, k, resno):
+        return lib().or_fm_se(self.h, k, resno).decode()
+
+    def fm_map(self, i):
+        out = np.zeros(4096, dtype=np.int32)
+        k = lib().or_fm_map(self.h, i, out, 4096)
+        return out[:k].copy()
+
+    def langevin(self, coords, vel, force, nsteps, dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0,
+                 seed=1, replica=0, step0=0):
+        x = np.ascontiguousarray(coords, dtype=np.float64).copy()
+        v = np.ascontiguousarray(vel, dtype=np.float64).copy()
+        f = np.ascontiguousarray(force, dtype=np.float64).copy()
+        e = np.zeros(N_ET)
+        lib().or_langevin_run(self.h, nsteps, dt, t_start, t_stop, damp, seed, replica, step0,
+                              x.reshape(-1), v.reshape(-1), f.reshape(-1), e)
+        return x, v, f, e
