@@ -63,8 +63,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
         L.awsem_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(vp)]
-        L.awsem_create_arrays.argtypes = [C.c_char_p, C.c_char_p, C.c_int, ip, C.c_char_p, dp, C.c_int, ip, ip, dp,
-                                          dp, C.c_char_p, C.c_int, C.POINTER(vp)]
+        L.awsem_create_arrays.argtypes = [C.c_char_p, C.c_char_p, C.c_int, ip, C.c_char_p, dp, C.c_int, ip, dp,
+                                          C.c_char_p, C.c_int, ip, ip, ip, ip, dp, C.c_int, C.POINTER(vp)]
         L.awsem_set_topology.argtypes = [vp, C.c_int, ip, ip, ip]
         L.awsem_compute.argtypes = [vp, dp, dp, dp]
         L.awsem_compute_res.argtypes = [vp, dp, dp, dp]
@@ -109,19 +109,25 @@ class FixBackbone:
 
     @classmethod
     def from_arrays(cls, coeff: str, param_dir: str, chains: Sequence[str], aps: Optional[np.ndarray],
-                    mem_pos, mem_len, mem_weight, mem_xyz_nm, mem_seq: str, precision: int = PREC_FP32):
+                    structs_nm: Sequence[np.ndarray], struct_seqs: Sequence[str], mem_struct, mem_tpos, mem_fpos,
+                    mem_len, mem_weight, precision: int = PREC_FP32):
+        """Arrays in place of the sequence/ssweight/.mem/.gro files:
+        structs_nm[s] is (len_s, 2, 3) CA/CB in nm, memories reference
+        (structure, target pos, structure pos, length, weight), 0-based."""
         self = cls.__new__(cls)
         chain_len = np.array([len(c) for c in chains], dtype=np.int32)
         seq = "".join(chains).encode()
         a = None if aps is None else np.ascontiguousarray(aps, dtype=np.float64)
-        mp = np.ascontiguousarray(mem_pos, dtype=np.int32)
-        ml = np.ascontiguousarray(mem_len, dtype=np.int32)
+        slen = np.array([len(q) for q in struct_seqs], dtype=np.int32)
+        sx = np.ascontiguousarray(np.concatenate([np.asarray(q, dtype=np.float64).reshape(-1) for q in structs_nm])
+                                  if len(structs_nm) else np.zeros(1))
+        sseq = "".join(struct_seqs).encode()
+        arrs = [np.ascontiguousarray(v, dtype=np.int32) for v in (mem_struct, mem_tpos, mem_fpos, mem_len)]
         mw = np.ascontiguousarray(mem_weight, dtype=np.float64)
-        mx = np.ascontiguousarray(mem_xyz_nm, dtype=np.float64)
         h = C.c_void_p()
         rc = lib().awsem_create_arrays(coeff.encode(), param_dir.encode(), len(chains), _ip(chain_len), seq, _dp(a),
-                                       len(mp), _ip(mp), _ip(ml), _dp(mw), _dp(mx), mem_seq.encode(), precision,
-                                       C.byref(h))
+                                       len(slen), _ip(slen), _dp(sx), sseq, len(mw), *[_ip(v) for v in arrs], _dp(mw),
+                                       precision, C.byref(h))
         self._finish(h, rc)
         return self
 

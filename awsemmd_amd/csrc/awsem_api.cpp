@@ -231,8 +231,10 @@ int awsem_create(const char *coeff_path, const char *seq_path, const char *workd
 }
 
 int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, const int *chain_len, const char *seq,
-                        const double *aps, int n_mem, const int *mem_pos, const int *mem_len, const double *mem_weight,
-                        const double *mem_xyz_nm, const char *mem_seq, int precision, awsem_ctx **out)
+                        const double *aps, int n_struct, const int *struct_len, const double *struct_xyz_nm,
+                        const char *struct_seq, int n_mem, const int *mem_struct, const int *mem_tpos,
+                        const int *mem_fpos, const int *mem_len, const double *mem_weight, int precision,
+                        awsem_ctx **out)
 {
   if (!out || !coeff_path || nch <= 0 || !chain_len || !seq) return AWSEM_E_ARG;
   *out = nullptr;
@@ -263,20 +265,31 @@ int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, 
             if (S.P.ssweight[k]) S.aps[(size_t)i * 12 + k] = aps[(size_t)i * 12 + k];
     }
     if (s.ok() && S.P.fm_flag) {
-      long long off2 = 0;
-      for (int k = 0; k < n_mem; ++k) {
+      if (n_mem > 0 && (!struct_len || !struct_xyz_nm || !struct_seq || !mem_struct || !mem_tpos || !mem_fpos ||
+                        !mem_len || !mem_weight))
+        s = AwStatus::err(AWSEM_E_ARG, "memory arrays missing");
+      std::vector<long long> soff(n_struct + 1, 0);
+      for (int k = 0; k < n_struct && s.ok(); ++k) soff[k + 1] = soff[k] + struct_len[k];
+      for (int k = 0; k < n_mem && s.ok(); ++k) {
+        int st = mem_struct[k], tp = mem_tpos[k], fp = mem_fpos[k], len = mem_len[k];
+        if (st < 0 || st >= n_struct || len <= 0 || fp < 0 || fp + len > struct_len[st])
+          s = AwStatus::err(AWSEM_E_FORMAT, "read_mems: Fragment_Memory: Error reading memory fragment");
+        else if (tp < 0 || tp + len > S.n)
+          s = AwStatus::err(AWSEM_E_FORMAT, "read_mems: Fragment_Memory: Incorrectly defined memory fragment");
+        if (!s.ok()) break;
         AwMemory m;
-        m.pos = mem_pos[k]; m.len = mem_len[k]; m.weight = mem_weight[k];
-        if (m.len <= 0 || m.pos < 0 || m.pos + m.len > S.n) { s = AwStatus::err(AWSEM_E_FORMAT, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); break; }
-        m.se.assign(mem_seq + off2, mem_seq + off2 + m.len);
-        m.ca.resize((size_t)m.len * 3);
-        m.cb.resize((size_t)m.len * 3);
-        for (int r = 0; r < m.len; ++r)
+        m.pos = tp; m.len = len; m.weight = mem_weight[k];
+        m.se.resize(len);
+        m.ca.assign((size_t)len * 3, 0.0);
+        m.cb.assign((size_t)len * 3, 0.0);
+        for (int r = 0; r < len; ++r) {
+          long long q = soff[st] + fp + r;
+          m.se[r] = struct_seq[q];
           for (int l = 0; l < 3; ++l) {
-            m.ca[3 * r + l] = mem_xyz_nm[((off2 + r) * 2 + 0) * 3 + l] * 10;
-            m.cb[3 * r + l] = m.se[r] == 'G' ? 0.0 : mem_xyz_nm[((off2 + r) * 2 + 1) * 3 + l] * 10;
+            m.ca[3 * r + l] = struct_xyz_nm[(q * 2 + 0) * 3 + l] * 10;
+            if (m.se[r] != 'G') m.cb[3 * r + l] = struct_xyz_nm[(q * 2 + 1) * 3 + l] * 10;
           }
-        off2 += m.len;
+        }
         S.mems.push_back(std::move(m));
       }
     }

@@ -116,16 +116,16 @@ struct View {
 enum { K_CA = 0, K_CB = 1, K_O = 2 };
 enum { K_N = 0, K_H = 1, K_CP = 2 };
 
-template <class T>
-AW_DEV void fadd(const View<T> &V, int kind, int i, T s, const V3<T> &d)
+template <class T, class R>
+AW_DEV void fadd(const View<T> &V, int kind, int i, R s, const V3<R> &d)
 {
   double *p = FA(V, kind, i);
   aadd(p + 0, (double)(s * d.x));
   aadd(p + 1, (double)(s * d.y));
   aadd(p + 2, (double)(s * d.z));
 }
-template <class T>
-AW_DEV void fadd3(const View<T> &V, int kind, int i, T fx, T fy, T fz)
+template <class T, class R>
+AW_DEV void fadd3(const View<T> &V, int kind, int i, R fx, R fy, R fz)
 {
   double *p = FA(V, kind, i);
   aadd(p + 0, (double)fx);
@@ -222,6 +222,9 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
 // ------------------------------------------------------------------------
 // local terms
 
+// Local (per-residue) terms run in FP64 in both precisions: they are O(n)
+// and the dihedral slopes lose accuracy in FP32 for near-collinear atoms.
+
 // compute_chain_potential, fb.cpp:2062-2170
 template <class T>
 AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
@@ -229,35 +232,35 @@ AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
   const FFParams *P = V.P;
   bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
   if (!first && !gly) {
-    V3<T> d = dvec<T>(XV(V, K_N, i), XA(V, K_CB, i));
-    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_ncb0;
-    T force = (T)2 * (T)P->k_chain[0] * dr / r;
-    E.chain += (double)((T)P->k_chain[0] * dr * dr);
-    fadd(V, K_CA, i - 1, -(T)P->an * force, d);
-    fadd(V, K_O, i - 1, -(T)P->cn * force, d);
-    fadd(V, K_CA, i, -(T)P->bn * force, d);
+    V3<double> d = dvec<double>(XV(V, K_N, i), XA(V, K_CB, i));
+    double r = sqrt(dot(d, d)), dr = r - (double)P->r_ncb0;
+    double force = (double)2 * (double)P->k_chain[0] * dr / r;
+    E.chain += (double)((double)P->k_chain[0] * dr * dr);
+    fadd(V, K_CA, i - 1, -(double)P->an * force, d);
+    fadd(V, K_O, i - 1, -(double)P->cn * force, d);
+    fadd(V, K_CA, i, -(double)P->bn * force, d);
     fadd(V, K_CB, i, force, d);
   }
   if (!last && !gly) {
-    V3<T> d = dvec<T>(XV(V, K_CP, i), XA(V, K_CB, i));
-    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_cpcb0;
-    T force = (T)2 * (T)P->k_chain[1] * dr / r;
-    E.chain += (double)((T)P->k_chain[1] * dr * dr);
-    fadd(V, K_CA, i + 1, -(T)P->bp * force, d);
-    fadd(V, K_CA, i, -(T)P->ap * force, d);
-    fadd(V, K_O, i, -(T)P->cp * force, d);
+    V3<double> d = dvec<double>(XV(V, K_CP, i), XA(V, K_CB, i));
+    double r = sqrt(dot(d, d)), dr = r - (double)P->r_cpcb0;
+    double force = (double)2 * (double)P->k_chain[1] * dr / r;
+    E.chain += (double)((double)P->k_chain[1] * dr * dr);
+    fadd(V, K_CA, i + 1, -(double)P->bp * force, d);
+    fadd(V, K_CA, i, -(double)P->ap * force, d);
+    fadd(V, K_O, i, -(double)P->cp * force, d);
     fadd(V, K_CB, i, force, d);
   }
   if (!first && !last) {
-    V3<T> d = dvec<T>(XV(V, K_N, i), XV(V, K_CP, i));
-    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_ncp0;
-    T force = (T)2 * (T)P->k_chain[2] * dr / r;
-    E.chain += (double)((T)P->k_chain[2] * dr * dr);
-    fadd(V, K_CA, i - 1, -(T)P->an * force, d);
-    fadd(V, K_O, i - 1, -(T)P->cn * force, d);
-    fadd(V, K_CA, i + 1, (T)P->bp * force, d);
-    fadd(V, K_CA, i, -(T)(P->bn - P->ap) * force, d);
-    fadd(V, K_O, i, (T)P->cp * force, d);
+    V3<double> d = dvec<double>(XV(V, K_N, i), XV(V, K_CP, i));
+    double r = sqrt(dot(d, d)), dr = r - (double)P->r_ncp0;
+    double force = (double)2 * (double)P->k_chain[2] * dr / r;
+    E.chain += (double)((double)P->k_chain[2] * dr * dr);
+    fadd(V, K_CA, i - 1, -(double)P->an * force, d);
+    fadd(V, K_O, i - 1, -(double)P->cn * force, d);
+    fadd(V, K_CA, i + 1, (double)P->bp * force, d);
+    fadd(V, K_CA, i, -(double)(P->bn - P->ap) * force, d);
+    fadd(V, K_O, i, (double)P->cp * force, d);
   }
 }
 
@@ -266,25 +269,25 @@ template <class T>
 AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E)
 {
   const FFParams *P = V.P;
-  T ks = (T)(P->epsilon * P->k_shake);
+  double ks = (double)(P->epsilon * P->k_shake);
   bool last = ri & RI_LAST;
   if (!last) {
-    V3<T> d = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, i + 1));
-    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_sh1, force = (T)2 * ks * dr / r;
+    V3<double> d = dvec<double>(XA(V, K_CA, i), XA(V, K_CA, i + 1));
+    double r = sqrt(dot(d, d)), dr = r - (double)P->r_sh1, force = (double)2 * ks * dr / r;
     E.shake += (double)(ks * dr * dr);
     fadd(V, K_CA, i, -force, d);
     fadd(V, K_CA, i + 1, force, d);
   }
   {
-    V3<T> d = dvec<T>(XA(V, K_CA, i), XA(V, K_O, i));
-    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_sh2, force = (T)2 * ks * dr / r;
+    V3<double> d = dvec<double>(XA(V, K_CA, i), XA(V, K_O, i));
+    double r = sqrt(dot(d, d)), dr = r - (double)P->r_sh2, force = (double)2 * ks * dr / r;
     E.shake += (double)(ks * dr * dr);
     fadd(V, K_CA, i, -force, d);
     fadd(V, K_O, i, force, d);
   }
   if (!last) {
-    V3<T> d = dvec<T>(XA(V, K_O, i), XA(V, K_CA, i + 1));
-    T r = t_sqrt(dot(d, d)), dr = r - (T)P->r_sh3, force = (T)2 * ks * dr / r;
+    V3<double> d = dvec<double>(XA(V, K_O, i), XA(V, K_CA, i + 1));
+    double r = sqrt(dot(d, d)), dr = r - (double)P->r_sh3, force = (double)2 * ks * dr / r;
     E.shake += (double)(ks * dr * dr);
     fadd(V, K_O, i, -force, d);
     fadd(V, K_CA, i + 1, force, d);
@@ -296,27 +299,27 @@ template <class T>
 AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
 {
   const FFParams *P = V.P;
-  V3<T> a = dvec<T>(XV(V, K_CP, i), XA(V, K_CA, i));
-  V3<T> b = dvec<T>(XA(V, K_CA, i), XV(V, K_N, i));
-  V3<T> c = dvec<T>(XA(V, K_CA, i), XA(V, K_CB, i));
-  T arvsq = (T)1 / dot(a, a), brvsq = (T)1 / dot(b, b), crvsq = (T)1 / dot(c, c);
-  T norm = t_sqrt(arvsq * brvsq * crvsq);
-  V3<T> axb = crs(a, b), cxa = crs(c, a), bxc = crs(b, c);
-  T chi = dot(axb, c) * norm;
-  V3<T> ap_, bp_, cp_;
+  V3<double> a = dvec<double>(XV(V, K_CP, i), XA(V, K_CA, i));
+  V3<double> b = dvec<double>(XA(V, K_CA, i), XV(V, K_N, i));
+  V3<double> c = dvec<double>(XA(V, K_CA, i), XA(V, K_CB, i));
+  double arvsq = (double)1 / dot(a, a), brvsq = (double)1 / dot(b, b), crvsq = (double)1 / dot(c, c);
+  double norm = sqrt(arvsq * brvsq * crvsq);
+  V3<double> axb = crs(a, b), cxa = crs(c, a), bxc = crs(b, c);
+  double chi = dot(axb, c) * norm;
+  V3<double> ap_, bp_, cp_;
   ap_.x = norm * bxc.x - arvsq * chi * a.x; ap_.y = norm * bxc.y - arvsq * chi * a.y; ap_.z = norm * bxc.z - arvsq * chi * a.z;
   bp_.x = norm * cxa.x - brvsq * chi * b.x; bp_.y = norm * cxa.y - brvsq * chi * b.y; bp_.z = norm * cxa.z - brvsq * chi * b.z;
   cp_.x = norm * axb.x - crvsq * chi * c.x; cp_.y = norm * axb.y - crvsq * chi * c.y; cp_.z = norm * axb.z - crvsq * chi * c.z;
-  T dchi = chi - (T)P->chi0;
-  T force = (T)2 * (T)P->k_chi * dchi;
-  E.chi += (double)((T)P->k_chi * dchi * dchi);
-  fadd(V, K_CA, i - 1, (T)P->an * force, bp_);
-  fadd(V, K_O, i - 1, (T)P->cn * force, bp_);
-  fadd(V, K_CA, i + 1, -(T)P->bp * force, ap_);
-  T c1 = (T)(1 - P->bn), c2 = (T)(P->ap - 1);
+  double dchi = chi - (double)P->chi0;
+  double force = (double)2 * (double)P->k_chi * dchi;
+  E.chi += (double)((double)P->k_chi * dchi * dchi);
+  fadd(V, K_CA, i - 1, (double)P->an * force, bp_);
+  fadd(V, K_O, i - 1, (double)P->cn * force, bp_);
+  fadd(V, K_CA, i + 1, -(double)P->bp * force, ap_);
+  double c1 = (double)(1 - P->bn), c2 = (double)(P->ap - 1);
   fadd3(V, K_CA, i, -(cp_.x + c1 * bp_.x + c2 * ap_.x) * force, -(cp_.y + c1 * bp_.y + c2 * ap_.y) * force,
         -(cp_.z + c1 * bp_.z + c2 * ap_.z) * force);
-  fadd(V, K_O, i, -(T)P->cp * force, ap_);
+  fadd(V, K_O, i, -(double)P->cp * force, ap_);
   fadd(V, K_CB, i, force, cp_);
 }
 
@@ -324,51 +327,51 @@ AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
 // CA(i+1), O(i-1), O(i)); the x and y slopes are summed since the force
 // uses their sum (fb.cpp:2467-2504)
 template <class T>
-AW_DEV T dihedral(const View<T> &V, int i, bool phi, V3<T> s[5])
+AW_DEV double dihedral(const View<T> &V, int i, bool phi, V3<double> s[5])
 {
   const FFParams *P = V.P;
-  T an = (T)P->an, bn = (T)P->bn, cn = (T)P->cn, ap = (T)P->ap, bp = (T)P->bp, cp = (T)P->cp;
-  V3<T> a, b, c;
+  double an = (double)P->an, bn = (double)P->bn, cn = (double)P->cn, ap = (double)P->ap, bp = (double)P->bp, cp = (double)P->cp;
+  V3<double> a, b, c;
   if (phi) {
-    a = dvec<T>(XV(V, K_CP, i), XA(V, K_CA, i));
-    b = dvec<T>(XA(V, K_CA, i), XV(V, K_N, i));
-    c = dvec<T>(XV(V, K_N, i), XV(V, K_CP, i - 1));
+    a = dvec<double>(XV(V, K_CP, i), XA(V, K_CA, i));
+    b = dvec<double>(XA(V, K_CA, i), XV(V, K_N, i));
+    c = dvec<double>(XV(V, K_N, i), XV(V, K_CP, i - 1));
   } else {
-    a = dvec<T>(XV(V, K_N, i + 1), XV(V, K_CP, i));
-    b = dvec<T>(XV(V, K_CP, i), XA(V, K_CA, i));
-    c = dvec<T>(XA(V, K_CA, i), XV(V, K_N, i));
+    a = dvec<double>(XV(V, K_N, i + 1), XV(V, K_CP, i));
+    b = dvec<double>(XV(V, K_CP, i), XA(V, K_CA, i));
+    c = dvec<double>(XA(V, K_CA, i), XV(V, K_N, i));
   }
-  T adb = dot(a, b), bdc = dot(b, c), adc = dot(a, c), b2 = dot(b, b), bm = t_sqrt(b2);
-  V3<T> bxa = crs(b, a), cxa = crs(c, a), cxb = crs(c, b);
-  T cdbxa = dot(c, bxa);
-  T Y = bm * cdbxa, X = adb * bdc - b2 * adc;
-  T angle = t_atan2(Y, X);
-  T X2Y2 = (T)1 / (X * X + Y * Y);
-  T dy = X * X2Y2, dx = -Y * X2Y2;
+  double adb = dot(a, b), bdc = dot(b, c), adc = dot(a, c), b2 = dot(b, b), bm = sqrt(b2);
+  V3<double> bxa = crs(b, a), cxa = crs(c, a), cxb = crs(c, b);
+  double cdbxa = dot(c, bxa);
+  double Y = bm * cdbxa, X = adb * bdc - b2 * adc;
+  double angle = atan2(Y, X);
+  double X2Y2 = (double)1 / (X * X + Y * Y);
+  double dy = X * X2Y2, dx = -Y * X2Y2;
   b2 *= dx; adb *= dx; adc *= dx; bdc *= dx;
   cdbxa *= dy / bm;
   bm *= dy;
-  const T av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
-  const T bxav[3] = {bxa.x, bxa.y, bxa.z}, cxav[3] = {cxa.x, cxa.y, cxa.z}, cxbv[3] = {cxb.x, cxb.y, cxb.z};
-  T out[5][3];
+  const double av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
+  const double bxav[3] = {bxa.x, bxa.y, bxa.z}, cxav[3] = {cxa.x, cxa.y, cxa.z}, cxbv[3] = {cxb.x, cxb.y, cxb.z};
+  double out[5][3];
   for (int l = 0; l < 3; l++) {
-    T h1, h2, h3, h4, g1, g2, g3;
+    double h1, h2, h3, h4, g1, g2, g3;
     g1 = bv[l] * bdc - cv[l] * b2;
-    g2 = av[l] * bdc - (T)2 * bv[l] * adc + cv[l] * adb;
+    g2 = av[l] * bdc - (double)2 * bv[l] * adc + cv[l] * adb;
     g3 = bv[l] * adb - av[l] * b2;
     if (phi) {
       h1 = cxbv[l] * bm; h2 = cxav[l] * bm; h3 = bxav[l] * bm; h4 = bv[l] * cdbxa;
       out[0][l] = -an * h4 + (an - ap) * h3 + an * h2 + (-an * g2 + (an - ap) * g3);
-      out[1][l] = ((T)1 - bn) * h4 + (bn - bp) * h3 + (ap - (T)1) * h1 - ((T)1 - bn) * h2 +
-                  ((ap - (T)1) * g1 + ((T)1 - bn) * g2 + (bn - bp) * g3);
+      out[1][l] = ((double)1 - bn) * h4 + (bn - bp) * h3 + (ap - (double)1) * h1 - ((double)1 - bn) * h2 +
+                  ((ap - (double)1) * g1 + ((double)1 - bn) * g2 + (bn - bp) * g3);
       out[2][l] = bp * h1 + bp * g1;
       out[3][l] = -cn * h4 + (cn - cp) * h3 + cn * h2 + (-cn * g2 + (cn - cp) * g3);
       out[4][l] = cp * h1 + cp * g1;
     } else {
       h1 = bxav[l] * bm; h2 = cxbv[l] * bm; h3 = cxav[l] * bm; h4 = bv[l] * cdbxa;
       out[0][l] = -an * h1 + (-an * g3);
-      out[1][l] = (ap - (T)1) * h4 + ((T)1 - bn) * h1 + (an - ap) * h2 - (ap - (T)1) * h3 +
-                  ((an - ap) * g1 + (ap - (T)1) * g2 + ((T)1 - bn) * g3);
+      out[1][l] = (ap - (double)1) * h4 + ((double)1 - bn) * h1 + (an - ap) * h2 - (ap - (double)1) * h3 +
+                  ((an - ap) * g1 + (ap - (double)1) * g2 + ((double)1 - bn) * g3);
       out[2][l] = bp * h4 + (bn - bp) * h2 - bp * h3 + ((bn - bp) * g1 + bp * g2);
       out[3][l] = -cn * h1 + (-cn * g3);
       out[4][l] = cp * h4 + (cn - cp) * h2 - cp * h3 + ((cn - cp) * g1 + cp * g2);
@@ -383,23 +386,23 @@ template <class T>
 AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
 {
   const FFParams *P = V.P;
-  V3<T> sphi[5], spsi[5];
-  T phi = dihedral(V, i, true, sphi);
-  T psi = dihedral(V, i, false, spsi);
+  V3<double> sphi[5], spsi[5];
+  double phi = dihedral(V, i, true, sphi);
+  double psi = dihedral(V, i, false, spsi);
   int js = 0, je = P->n_rama_par;
   if ((ri & RI_PRO) && P->rama_p_flag) { js = 6; je = 6 + P->n_rama_p_par; }
-  T fphi = 0, fpsi = 0;
+  double fphi = 0, fpsi = 0;
   double ev = 0.0;
   for (int j = js; j < je; j++) {
     double apsv = V.aps[i * 12 + j];
     if (P->ssweight[j] && apsv == 0.0) continue;
-    T cphi = t_cos(phi + (T)P->phi0[j]) - (T)1, cpsi = t_cos(psi + (T)P->psi0[j]) - (T)1;
-    T pc = (T)P->phiw[j] * cphi, sc = (T)P->psiw[j] * cpsi;
-    T Vv = (T)P->w[j] * t_exp(-cphi * pc - cpsi * sc);
-    if (P->ssweight[j]) Vv *= (T)apsv;
-    T force = (T)2 * Vv;
-    fphi += force * pc * t_sin(phi + (T)P->phi0[j]);
-    fpsi += force * sc * t_sin(psi + (T)P->psi0[j]);
+    double cphi = cos(phi + (double)P->phi0[j]) - (double)1, cpsi = cos(psi + (double)P->psi0[j]) - (double)1;
+    double pc = (double)P->phiw[j] * cphi, sc = (double)P->psiw[j] * cpsi;
+    double Vv = (double)P->w[j] * exp(-cphi * pc - cpsi * sc);
+    if (P->ssweight[j]) Vv *= (double)apsv;
+    double force = (double)2 * Vv;
+    fphi += force * pc * sin(phi + (double)P->phi0[j]);
+    fpsi += force * sc * sin(psi + (double)P->psi0[j]);
     ev += (double)(-Vv);
   }
   E.rama += ev;
@@ -514,9 +517,11 @@ AW_DEV void fm_item(const View<T> &V, int item, EAcc &E)
 // ------------------------------------------------------------------------
 // switching functions
 
-// cWell::compute_theta_pair, sml.h:594-610 (theta and theta'/r)
+// cWell::compute_theta_pair, sml.h:594-610 (theta and theta'/r).  r is
+// passed in FP64 so that the small arguments kappa*(r - r_min) and
+// kappa*(r_max - r) are formed before any rounding to FP32.
 template <class T>
-AW_DEV void well_theta(T kappa, T rmin, T rmax, T r, T &th, T &prd);
+AW_DEV void well_theta(double kappa, double rmin, double rmax, double r, T &th, T &prd);
 
 template <>
 AW_DEV void well_theta<double>(double kappa, double rmin, double rmax, double r, double &th, double &prd)
@@ -529,24 +534,24 @@ AW_DEV void well_theta<double>(double kappa, double rmin, double rmax, double r,
 // of 1 + tanh(x) for x << 0: theta = 1/((1+e^-2a)(1+e^-2b)),
 // theta'/r = 2 kappa (e^-2a - e^-2b) / (D^2 r)
 template <>
-AW_DEV void well_theta<float>(float kappa, float rmin, float rmax, float r, float &th, float &prd)
+AW_DEV void well_theta<float>(double kappa, double rmin, double rmax, double r, float &th, float &prd)
 {
-  float ea = aw_fast_exp(-2.0f * kappa * (r - rmin)), eb = aw_fast_exp(-2.0f * kappa * (rmax - r));
+  float ea = aw_fast_exp((float)(-2.0 * kappa * (r - rmin))), eb = aw_fast_exp((float)(-2.0 * kappa * (rmax - r)));
   float D = (1.0f + ea) * (1.0f + eb);
   float iD = 1.0f / D;
   th = iD;
-  prd = 2.0f * kappa * (ea - eb) * iD * iD / r;
+  prd = 2.0f * (float)kappa * (ea - eb) * iD * iD / (float)r;
 }
 
 template <class T>
-AW_DEV void wtheta(const FFParams *P, int w, T r, T &th, T &prd)
+AW_DEV void wtheta(const FFParams *P, int w, double r, T &th, T &prd)
 {
-  well_theta<T>((T)P->water_kappa, (T)P->well_r_min[w], (T)P->well_r_max[w], r, th, prd);
+  well_theta<T>(P->water_kappa, P->well_r_min[w], P->well_r_max[w], r, th, prd);
 }
 template <class T>
-AW_DEV void htheta(const FFParams *P, T r, T &th, T &prd)
+AW_DEV void htheta(const FFParams *P, double r, T &th, T &prd)
 {
-  well_theta<T>((T)P->helix_kappa, (T)P->helix_r_min, (T)P->helix_r_max, r, th, prd);
+  well_theta<T>(P->helix_kappa, P->helix_r_min, P->helix_r_max, r, th, prd);
 }
 
 AW_DEV const double *site(const View<float> &V, int i, int ri) { return (ri & RI_GLY) ? XA(V, K_CA, i) : XA(V, K_CB, i); }
@@ -660,12 +665,16 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     if (!on[k]) continue;
     xNO[k] = dvec<T>(XA(V, K_O, acc[k]), XV(V, K_N, don[k]));
     xHO[k] = dvec<T>(XA(V, K_O, acc[k]), XV(V, K_H, don[k]));
-    T RNO = t_sqrt(dot(xNO[k], xNO[k])), RHO = t_sqrt(dot(xHO[k], xHO[k]));
-    T dNO = (RNO - (T)P->NO_zero) * (T)P->sigma_NO_sqinv;
-    T dHO = (RHO - (T)P->HO_zero) * (T)P->sigma_HO_sqinv;
-    theta[k] = t_exp((T)-0.5 * ((RNO - (T)P->NO_zero) * dNO + (RHO - (T)P->HO_zero) * dHO));
-    p0[k] = -dNO / RNO;
-    p1[k] = -dHO / RHO;
+    // distances and their offsets from the well centres in FP64 (the
+    // offsets are small differences of ~3 A quantities)
+    double RNO = sqrt(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_N, don[k])));
+    double RHO = sqrt(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_H, don[k])));
+    T eNO = (T)(RNO - P->NO_zero), eHO = (T)(RHO - P->HO_zero);
+    T dNO = eNO * (T)P->sigma_NO_sqinv;
+    T dHO = eHO * (T)P->sigma_HO_sqinv;
+    theta[k] = t_exp((T)-0.5 * (eNO * dNO + eHO * dHO));
+    p0[k] = -dNO / (T)RNO;
+    p1[k] = -dHO / (T)RHO;
   }
   T ff1 = (T)P->k_dssp * theta[0];
   T Vv[4] = {lambda[0], lambda[1] * theta[1], lambda[2] * theta[2], lambda[3] * theta[3]};
@@ -848,13 +857,13 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E)
     double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
     if (P->water_flag && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
       T th, prd;
-      wtheta<T>(P, 0, (T)sqrt(rsq), th, prd);
+      wtheta<T>(P, 0, sqrt(rsq), th, prd);
       aadd(&V.rho_w[i], (double)th);
       aadd(&V.rho_w[j], (double)th);
     }
     if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
       T th, prd;
-      htheta<T>(P, (T)sqrt(rsq), th, prd);
+      htheta<T>(P, sqrt(rsq), th, prd);
       aadd(&V.rho_h[i], (double)th);
       aadd(&V.rho_h[j], (double)th);
     }
@@ -945,11 +954,12 @@ AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
   if (!(r2 < P->helix_cutoff_sq)) return;
   if (sqrt(r2) > P->helix_cutoff) return;
   V3<T> xNO = dvec<T>(XA(V, K_O, i), XV(V, K_N, j)), xHO = dvec<T>(XA(V, K_O, i), XV(V, K_H, j));
-  T RNO = t_sqrt(dot(xNO, xNO)), RHO = t_sqrt(dot(xHO, xHO));
+  double RNOd = sqrt(r2), RHOd = sqrt(rsq_exact(XA(V, K_O, i), XV(V, K_H, j)));
+  T RNO = (T)RNOd, RHO = (T)RHOd;
   int it = rtype(ri), jt = rtype(rj);
   double h4i = ((ri & RI_PRO) && P->pro_accepter_flag) ? P->h4prob_pro_accepter : P->h4prob[it];
   T prob_sum = (T)(h4i + P->h4prob[jt]);
-  T dNO = RNO - (T)P->helix_NO_zero, dHO = RHO - (T)P->helix_HO_zero;
+  T dNO = (T)(RNOd - P->helix_NO_zero), dHO = (T)(RHOd - P->helix_HO_zero);
   T dNOs = dNO * (T)P->helix_sigma_NO_sqinv, dHOs = dHO * (T)P->helix_sigma_HO_sqinv;
   T pt = -(T)P->k_helix * prob_sum * t_exp((T)-0.5 * (dNO * dNOs + dHO * dHOs));
   T p0 = -dNOs / RNO, p1 = -dHOs / RHO;
@@ -984,7 +994,7 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j)
     // fb.cpp:7925 reads !fabs(g0-g1)<delta, i.e. (!fabs(g0-g1)) < delta: true iff g0 != g1
     if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
       T th, prd;
-      wtheta<T>(P, w, (T)sqrt(rsq), th, prd);
+      wtheta<T>(P, w, sqrt(rsq), th, prd);
       double tg = (g1 - g0) * (double)th;
       acc_i += tg * V.sig_w[j];
       acc_j += tg * V.sig_w[i];
@@ -1031,7 +1041,7 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
   if (!near_pair(V, i, j, (float)P->dens_reach)) return;
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
   double rsq = rsq_exact(si, sj);
-  T r = (T)sqrt(rsq);
+  double r = sqrt(rsq);
   T force = 0;
   bool in0 = rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0];
   if (P->water_flag && ON(P, AW_ET_WATER)) {

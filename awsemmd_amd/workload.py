@@ -86,7 +86,7 @@ def _rand_unit(rng):
             return v / nv
 
 
-def saw_trace(n: int, rng, start=None, bond=3.8, excl=4.0, excl13=5.0, others: Optional[np.ndarray] = None,
+def saw_trace(n: int, rng, start=None, bond=3.8, excl=4.6, excl13=5.0, others: Optional[np.ndarray] = None,
               center=None, radius=None) -> np.ndarray:
     """Self-avoiding C-alpha walk: |CA_i - CA_j| >= excl for |i-j|>=3 (and
     against `others`), |CA_i - CA_{i-2}| >= excl13, optional confinement."""
@@ -215,9 +215,9 @@ def make_system(lengths: Sequence[int], seed: int, n_struct: int = 20, mem_len: 
     for _ in range(n_struct):
         ca = xca + rng.normal(scale=noise, size=xca.shape)
         cb = xcb + rng.normal(scale=noise, size=xcb.shape)
-        # nm with 3 decimals, exactly what a .gro file stores
-        s = np.stack([ca, cb], axis=1) / 10.0
-        s = np.array([[float("%.3f" % v) for v in row] for row in s.reshape(-1, 3)]).reshape(n, 2, 3)
+        # nm with 3 decimals, exactly what a .gro file stores: k/1000 with
+        # integer k is the double that "%8.3f" prints and strtod reads back
+        s = np.round(np.stack([ca, cb], axis=1) * 100.0) / 1000.0
         mem_structs.append(s)
     memories = []
     off = 0
@@ -229,6 +229,22 @@ def make_system(lengths: Sequence[int], seed: int, n_struct: int = 20, mem_len: 
         off += L
     return System(chains=chains, xca=xca, xcb=xcb, xo=xo, ssw=ssw, mem_structs=mem_structs,
                   memories=memories, seed=seed)
+
+
+def arrays(sys: System) -> dict:
+    """Keyword arguments of FixBackbone.from_arrays for `sys` (same content
+    as the file set write_system produces)."""
+    n = sys.n
+    aps = np.zeros((n, 12))
+    aps[:, 3] = sys.ssw[:, 0]
+    aps[:, 4] = sys.ssw[:, 1]
+    seq = sys.seq
+    return dict(chains=sys.chains, aps=aps, structs_nm=sys.mem_structs, struct_seqs=[seq] * len(sys.mem_structs),
+                mem_struct=np.array([m.struct for m in sys.memories], dtype=np.int32),
+                mem_tpos=np.array([m.pos for m in sys.memories], dtype=np.int32),
+                mem_fpos=np.array([m.fpos for m in sys.memories], dtype=np.int32),
+                mem_len=np.array([m.length for m in sys.memories], dtype=np.int32),
+                mem_weight=np.array([m.weight for m in sys.memories], dtype=np.float64))
 
 
 def coeff_text(fm: bool = True, mem_file: str = "fragsLAMW.mem", gamma_file: str = "uniform.gamma",

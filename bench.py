@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched FixBackbone path on one MI355X.
+
+Workload: 256 independent chains, lengths ~ U[50, 400], each with its own
+seeded sequence, self-avoiding C-alpha trace and 20-memory fragment library
+(20 perturbed copies of the trace, sigma 1 A, one memory per copy at every
+9-residue window).  All FixBackbone terms of the shipped coeff file are on
+plus fragment memory, the excluded-volume pair style and the harmonic bonds
+of the reference deck; FP64 Langevin dynamics at 300 K with a 2 fs step.
+
+A "step" is one Langevin step of every replica (one batch-step).  `value`
+is replica-steps/s of the batched path (FP32 pair arithmetic, FP64
+accumulation), whole job, inputs resident in HBM.  Reported beside it:
+  * the plain single-chain FP64 GPU drop-in on the same hardware (one chain
+    at a time through the phase engine), replica-steps/s on a sample of the
+    same chains;
+  * the max relative force error of the batched path against that FP64
+    path (max-norm over atoms);
+  * p50 / p99 latency of one batch-step;
+  * the HBM roofline of the resident kernel (algorithmic bytes / kernel time);
+  * the CPU oracle (FP64 restatement, 1 core) on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MVV2E = 48.88821291 ** 2
+BOLTZ = 0.0019872067
+
+
+def _gen(args):
+    L, seed = args
+    from awsemmd_amd import workload as W
+    return W.make_system([L], seed=seed)
+
+
+def make_workload(n_rep: int, seed: int, lmin=50, lmax=400, workers=None):
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(lmin, lmax + 1, size=n_rep)
+    jobs = [(int(L), seed * 100003 + k) for k, L in enumerate(lengths)]
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers > 1:
+        with mp.get_context("fork").Pool(workers) as pool:
+            systems = pool.map(_gen, jobs, chunksize=4)
+    else:
+        systems = [_gen(j) for j in jobs]
+    return systems
+
+
+def velocities(sys_, T, seed):
+    """Maxwell-Boltzmann velocities (A/fs) at T for CA 27, CB 60 (H-beta 2), O 28."""
+    rng = np.random.default_rng(seed)
+    n = sys_.n
+    m = np.zeros((3, n))
+    m[0] = 27.0
+    m[1] = np.array([2.0 if c == "G" else 60.0 for c in sys_.seq])
+    m[2] = 28.0
+    sd = np.sqrt(BOLTZ * T / (m * MVV2E))
+    v = rng.normal(size=(3, n, 3)) * sd[:, :, None]
+    v -= (v * m[:, :, None]).sum(axis=(0, 1)) / m.sum()
+    return v
+
+
+def pair_counts(s, P_cut=13.184136, dssp_cut=7.0, pap_cut=9.9736, ev_cut=4.5):
+    """Contact-list entries per evaluation (for the algorithmic byte count)."""
+    x = s.coords()
+    site = np.where(np.array([c == "G" for c in s.seq])[:, None], x[0], x[1])
+    def cnt(a, b, cut, minsep):
+        d2 = ((a[:, None, :] - b[None, :, :]) ** 2).sum(-1)
+        i, j = np.triu_indices(len(a), k=minsep)
+        return int((d2[i, j] < cut * cut).sum())
+    p_dens = cnt(site, site, P_cut, 2)
+    p_pap = cnt(x[0], x[0], pap_cut, 1)
+    p_dssp = 2 * cnt(x[2], x[0], dssp_cut, 3)
+    allx = x.reshape(-1, 3)
+    p_ev = cnt(allx, allx, ev_cut, 1)
+    return p_dens, p_dssp, p_pap, p_ev
+
+
+def algorithmic_bytes(systems):
+    """Per batch-step: 360 L + 336 M + 64 L + 4 (3 P_dens + P_dssp + P_pap + P_ev)."""
+    tot = 0
+    for s in systems:
+        L = s.n
+        M = len(s.memories)
+        pd, pdssp, ppap, pev = pair_counts(s)
+        tot += 360 * L + 336 * M + 64 * L + 4 * (3 * pd + pdssp + ppap + pev)
+    return tot
+
+
+def rel_force_err(f, fref):
+    d = np.linalg.norm((f - fref).reshape(-1, 3), axis=1).max()
+    m = np.linalg.norm(fref.reshape(-1, 3), axis=1).max()
+    return float(d / m) if m > 0 else float(d)
+
+
+def cpu_baseline(systems, budget_s=15.0):
+    """The FP64 oracle (1 core) running Langevin steps on a sample of chains."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from awsemmd_amd import workload as W
+    order = np.argsort([s.n for s in systems])
+    pick = [systems[order[len(order) // 2]]]  # a median-length chain
+    rs, tot_steps, t_used = 0, 0, 0.0
+    for s in pick:
+        d = tempfile.mkdtemp(prefix="awsem_cpu_")
+        p = W.write_system(s, d)
+        o = oracle.Oracle(p["coeff"], p["seq"], d)
+        x = s.coords()
+        f, _ = o.compute(x)
+        v = velocities(s, 300.0, 7)
+        o.langevin(x, v, f, 1)  # warm-up
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < budget_s:
+            x, v, f, e = o.langevin(x, v, f, 5, step0=k)
+            k += 5
+        t_used += time.perf_counter() - t0
+        tot_steps += k
+        rs += 1
+        L = s.n
+    return {"value": tot_steps / t_used, "unit": "replica-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 FixBackbone+pair style+bonds+Langevin, {rs} chain of L={L} "
+                      f"(median length), {tot_steps} steps in {t_used:.1f}s on one core"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--chunk", type=int, default=10, help="batch-steps per resident launch")
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: every replica is independent, so a multi-GPU run "
+              "would only shard replicas across GPUs (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import awsemmd_amd as A
+    from awsemmd_amd import workload as W
+    A.lib()
+
+    t0 = time.perf_counter()
+    systems = make_workload(a.replicas, a.seed)
+    t_gen = time.perf_counter() - t0
+    log(f"generated {len(systems)} chains in {t_gen:.1f}s")
+    coeff_dir = tempfile.mkdtemp(prefix="awsem_bench_")
+    W.write_system(systems[0], coeff_dir, fm=False)  # parameter tables
+    coeff = os.path.join(coeff_dir, "fix_backbone_coeff.data")
+    with open(coeff, "w") as f:
+        f.write(W.coeff_text(fm=True, mem_file="unused.mem", gamma_file="uniform.gamma"))
+
+    def ctx(s, prec):
+        return A.FixBackbone.from_arrays(coeff, coeff_dir, precision=prec, **W.arrays(s))
+
+    t0 = time.perf_counter()
+    fixes = [ctx(s, A.PREC_FP32) for s in systems]
+    batch = A.Batch(fixes, A.PREC_FP32)
+    del fixes
+    t_build = time.perf_counter() - t0
+    log(f"batch built in {t_build:.1f}s")
+    x0 = A.pack_states([s.coords() for s in systems])
+    v0 = A.pack_states([velocities(s, 300.0, 1000 + k) for k, s in enumerate(systems)])
+    batch.set_state(x0, v0)
+    lg = dict(dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=a.seed)
+
+    # warm-up
+    batch.run(a.warmup, step0=0, **lg)
+    log(f"warm-up done ({a.warmup} steps)")
+    # timed region: K batch-steps in launches of `chunk` steps
+    chunks = []
+    kernel_ms = []
+    step = a.warmup
+    left = a.steps
+    t_start = time.perf_counter()
+    while left > 0:
+        k = min(a.chunk, left)
+        c0 = time.perf_counter()
+        batch.run(k, step0=step, **lg)
+        chunks.append((time.perf_counter() - c0, k))
+        kernel_ms.append(batch.timing()[0])
+        step += k
+        left -= k
+    t_total = time.perf_counter() - t_start
+    replica_steps = a.replicas * a.steps
+    value = replica_steps / t_total
+    per_step_ms = np.array([1e3 * t / k for t, k in chunks])
+    res_total = sum(s.n for s in systems)
+    log(f"timed {a.steps} steps: {value:.1f} replica-steps/s")
+
+    # force error of the batched FP32 path against the FP64 path (sample)
+    batch.compute()
+    xs, vs, fs, es = batch.get_state()
+    xs_l, fs_l = batch.split(xs), batch.split(fs)
+    rng = np.random.default_rng(5)
+    sample = sorted(rng.choice(a.replicas, size=min(a.plain_sample, a.replicas), replace=False).tolist())
+    errs = []
+    plain_time, plain_steps = 0.0, 0
+    for r in sample:
+        fx = ctx(systems[r], A.PREC_FP64)
+        f64, e64 = fx.compute_res(xs_l[r])
+        errs.append(rel_force_err(fs_l[r], f64))
+        # plain FP64 drop-in throughput: device-resident dynamics, one chain
+        xv = xs_l[r].copy()
+        vv = batch.split(vs)[r].copy()
+        fx.run(xv, vv, 1, step0=0, **lg)  # warm-up
+        c0 = time.perf_counter()
+        fx.run(xv, vv, a.plain_steps, step0=1, **lg)
+        plain_time += time.perf_counter() - c0
+        plain_steps += a.plain_steps
+        fx.close()
+    plain_value = plain_steps / plain_time
+    log(f"plain FP64 sample: {plain_value:.1f} replica-steps/s, force err {max(errs):.2e}")
+    force_err = max(errs)
+
+    bytes_step = algorithmic_bytes(systems)
+    k_avg_ms = float(np.mean(kernel_ms))
+    achieved = bytes_step * a.chunk / (k_avg_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(systems, a.cpu_budget)
+
+    out = {
+        "metric": "replica-steps/s batched vs plain FP64 GPU at force err<=1e-5; p50/p99 lat",
+        "value": value,
+        "unit": "replica-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_total / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math, f64 accumulation/integration",
+        "data": "synthetic",
+        "config": {"workload": "256 chains, L~U[50,400], 20 memories per 9-residue window, Langevin 300 K, dt 2 fs",
+                   "replicas": a.replicas, "residues": res_total, "memories": int(sum(len(s.memories) for s in systems)),
+                   "steps_per_launch": a.chunk, "seed": a.seed},
+        "plain_fp64_gpu": {"value": plain_value, "unit": "replica-steps/s",
+                           "sample": f"{len(sample)} chains x {a.plain_steps} steps, phase engine, one chain at a time"},
+        "speedup_vs_plain_fp64": value / plain_value,
+        "force_err_max_vs_fp64": force_err,
+        "latency_ms_per_batch_step": {"p50": float(np.percentile(per_step_ms, 50)),
+                                      "p99": float(np.percentile(per_step_ms, 99))},
+        "ns_per_day_per_replica": value / a.replicas * 2.0 * 86400 / 1e6,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_resident<float,LDS>", "kernel_avg_ms": k_avg_ms,
+                     "algorithmic_bytes_per_batch_step": bytes_step},
+        "cpu_baseline": cpu,
+        "setup_s": {"generate": t_gen, "build": t_build},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -73,17 +73,21 @@ int awsem_create(const char *coeff_path, const char *seq_path, const char *workd
  * parameter tables still come from files in param_dir).
  *   seq        : all chains concatenated (n letters), chain_len[nch]
  *   aps        : ssweight values [n][12] (row per residue, column per
- *                [SSWeight] flag; ignored columns may hold anything), or NULL
- *   memories   : n_mem windows; mem_pos (0-based target residue),
- *                mem_len, mem_weight; mem_xyz_nm[sum(len)][2][3] holds the
- *                CA and CB coordinates in nm as a .gro file stores them
- *                (multiplied by 10 like fm.cpp:87); mem_seq[sum(len)] the
- *                one-letter residue names ('G' marks a missing CB). */
+ *                [SSWeight] flag; unflagged columns are ignored), or NULL
+ *   structures : n_struct memory structures, the contents of .gro files:
+ *                struct_len[s] residues, struct_xyz_nm[sum][2][3] the CA
+ *                and CB coordinates in nm (multiplied by 10 like
+ *                fm.cpp:87), struct_seq[sum] one-letter residue names
+ *                ('G' residues carry no CB)
+ *   memories   : n_mem lines of the [Memories] section (fb.cpp:1997-2008):
+ *                structure index, target position, position in the
+ *                structure (both 0-based), length, weight. */
 int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch,
                         const int *chain_len, const char *seq, const double *aps,
-                        int n_mem, const int *mem_pos, const int *mem_len,
-                        const double *mem_weight, const double *mem_xyz_nm,
-                        const char *mem_seq, int precision, awsem_ctx **out);
+                        int n_struct, const int *struct_len, const double *struct_xyz_nm,
+                        const char *struct_seq, int n_mem, const int *mem_struct,
+                        const int *mem_tpos, const int *mem_fpos, const int *mem_len,
+                        const double *mem_weight, int precision, awsem_ctx **out);
 
 /* Construct_Computational_Arrays (fb.cpp:1590-1698): per atom, the fix
  * group (0 = CA group, 1 = beta group, 2 = oxygen group, -1 none), the

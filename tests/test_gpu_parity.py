@@ -1,14 +1,21 @@
-"""GPU parity: the HIP force path against the FP64 oracle, per term."""
+"""GPU parity: the HIP force path against the FP64 oracle, per term.
+
+Force error of a term: max over atoms |F_gpu - F_oracle| divided by
+max(max over atoms |F_oracle,term|, 1e-4 * max over atoms |F_oracle,total|);
+the floor keeps terms whose forces are physically zero (e.g. water on a
+chain with no |i-j|>=10 contacts, ~1e-12 kcal/mol/A) from dividing by
+rounding noise.  Energy error: |E_gpu - E_oracle| / max(1, |E_oracle|).
+"""
 import numpy as np
 import pytest
 
-from conftest import make_case, rel_force_err
+from conftest import make_case
 
 pytestmark = pytest.mark.gpu
 
 TERMS = ["Chain", "Chi", "Rama", "Excluded", "DSSP", "P_AP", "Water", "Burial", "Helix", "Frag_Mem", "Bond"]
-# tolerances: FP64 path agrees to rounding; FP32 pair arithmetic with FP64
-# accumulation must stay within 1e-5 relative (force: max-norm over atoms)
+# FP64 path: rounding-level agreement; FP32 pair arithmetic with FP64
+# accumulation: 1e-5 relative per term
 TOL = {0: 1e-9, 1: 1e-5}
 
 
@@ -18,52 +25,68 @@ def _gpu():
     return A
 
 
-@pytest.mark.parametrize("lengths,seed", [([40, 25], 5), ([120], 11)])
-def test_plain_fp64_per_term(oracle_mod, lengths, seed):
+def term_err(f, fo, f_total_max):
+    d = np.linalg.norm((f - fo).reshape(-1, 3), axis=1).max()
+    m = np.linalg.norm(fo.reshape(-1, 3), axis=1).max()
+    return d / max(m, 1e-4 * f_total_max)
+
+
+@pytest.mark.parametrize("lengths,seed,compact", [([40, 25], 5, False), ([120], 11, False), ([60, 60], 17, True)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_plain_per_term(oracle_mod, lengths, seed, compact, prec):
     A = _gpu()
-    s, p = make_case(lengths, seed)
+    s, p = make_case(lengths, seed, compact=compact)
     o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
-    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], prec)
     x = s.coords()
+    ftot = np.linalg.norm(o.compute(x)[0].reshape(-1, 3), axis=1).max()
     for name in TERMS + ["ALL"]:
         mask = 0xFFFFFFFF if name == "ALL" else (1 << A.ET[name])
         fix.set_term_mask(mask)
         f, e = fix.compute_res(x)
         fo, eo = o.compute(x, mask)
         k = 0 if name == "ALL" else A.ET[name]
-        print(name, e[k], eo[k])
-        assert abs(e[k] - eo[k]) <= TOL[0] * max(1.0, abs(eo[k])), name
-        if np.abs(fo).max() > 0:
-            assert rel_force_err(f, fo) <= TOL[0], name
+        ferr = term_err(f, fo, ftot)
+        eerr = abs(e[k] - eo[k]) / max(1.0, abs(eo[k]))
+        print(f"{name:9s} E={e[k]:.9g} Eref={eo[k]:.9g} eerr={eerr:.2e} ferr={ferr:.2e}")
+        assert eerr <= TOL[prec], name
+        assert ferr <= TOL[prec], name
 
 
-def test_batch_fp32_resident(oracle_mod):
+def test_batch_fp32_resident_per_term(oracle_mod):
     A = _gpu()
-    cases = [make_case([L], 100 + k) for k, L in enumerate([60, 37, 90])]
-    fixes = [A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32) for _, p in cases]
-    b = A.Batch(fixes, A.PREC_FP32)
-    b.set_state(A.pack_states([s.coords() for s, _ in cases]))
-    b.compute()
-    x, v, f, e = b.get_state()
-    fs = b.split(f)
-    for r, (s, p) in enumerate(cases):
-        o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
-        fo, eo = o.compute(s.coords())
-        err = rel_force_err(fs[r], fo)
-        print("replica", r, "E", e[r, 0], eo[0], "ferr", err)
-        assert err <= TOL[1]
-        for name in TERMS:
-            k = A.ET[name]
-            assert abs(e[r, k] - eo[k]) <= TOL[1] * max(1.0, abs(eo[k])), (r, name, e[r, k], eo[k])
+    cases = [make_case([L], 100 + k) for k, L in enumerate([60, 37, 90])] + [make_case([30, 44], 7, compact=True)]
+    xs = A.pack_states([s.coords() for s, _ in cases])
+    oracles = [oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"]) for _, p in cases]
+    ftots = [np.linalg.norm(o.compute(s.coords())[0].reshape(-1, 3), axis=1).max() for o, (s, _) in zip(oracles, cases)]
+    for name in TERMS + ["ALL"]:
+        mask = 0xFFFFFFFF if name == "ALL" else (1 << A.ET[name])
+        fixes = [A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32) for _, p in cases]
+        fixes[0].set_term_mask(mask)  # the batch takes the parameters of its first system
+        b = A.Batch(fixes, A.PREC_FP32)
+        b.set_state(xs)
+        b.compute()
+        x, v, f, e = b.get_state()
+        fs = b.split(f)
+        k = 0 if name == "ALL" else A.ET[name]
+        for r, (s, p) in enumerate(cases):
+            fo, eo = oracles[r].compute(s.coords(), mask)
+            ferr = term_err(fs[r], fo, ftots[r])
+            eerr = abs(e[r, k] - eo[k]) / max(1.0, abs(eo[k]))
+            print(f"replica {r} {name:9s} E={e[r, k]:.9g} Eref={eo[k]:.9g} eerr={eerr:.2e} ferr={ferr:.2e}")
+            assert ferr <= TOL[1], (r, name)
+            assert eerr <= TOL[1], (r, name)
 
 
 @pytest.mark.parametrize("kind,arg", [(0, 0), (0, 1), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (6, 1)])
-def test_lists_bit_exact(oracle_mod, kind, arg):
+@pytest.mark.parametrize("prec", [0, 1])
+def test_lists_bit_exact(oracle_mod, kind, arg, prec):
     A = _gpu()
-    s, p = make_case([70, 30], 21)
-    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
-    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
-    x = s.coords()
-    g = fix.list(kind, x, arg)
-    r = o.list(kind, x, arg)
-    assert np.array_equal(g, r), (kind, arg, len(g), len(r))
+    for lengths, seed, compact in (([70, 30], 21, False), ([50, 50, 40], 4, True)):
+        s, p = make_case(lengths, seed, compact=compact)
+        o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+        fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], prec)
+        x = s.coords()
+        g = fix.list(kind, x, arg)
+        r = o.list(kind, x, arg)
+        assert np.array_equal(g, r), (kind, arg, len(g), len(r))
