@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -603,9 +604,11 @@ int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, doubl
   return AWSEM_OK;
 }
 
-static int batch_launch(awsem_batch *b, const AwRun &run)
+static int batch_launch(awsem_batch *b, AwRun run)
 {
   DevBatch &D = *b->dev;
+  // workgroup size of the resident engine (512 default; 1024 for comparison)
+  if (const char *e = std::getenv("AWSEM_RESIDENT_THREADS")) run.threads = std::atoi(e);
   hipEventRecord(D.ev0, D.stream);
   if (aw_launch_resident(D.ptrs(), run, D.stream)) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
   hipEventRecord(D.ev1, D.stream);

@@ -25,31 +25,17 @@
 #include <stdint.h>
 #include "awsem_terms.h"
 
+
 // ------------------------------------------------------------------------
-// block energy reduction
+// energy epilogue: ered[N_EACC] (LDS) -> out[slot]
 
-__device__ __forceinline__ double wave_sum(double v)
+__device__ __forceinline__ void energy_out(const double *ered, double *out, bool atomic)
 {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-
-// sums EAcc over the block; thread 0..N_EACC-1 then adds into out[slot]
-__device__ __forceinline__ void block_energy(const EAcc &E, double *red, double *out, bool atomic)
-{
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  for (int k = 0; k < N_EACC; ++k) {
-    double s = wave_sum(eacc_get(E, k));
-    if (lane == 0) red[wid * N_EACC + k] = s;
-  }
-  __syncthreads();
   if (threadIdx.x < N_EACC) {
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += red[w * N_EACC + threadIdx.x];
-    if (atomic) { if (s != 0.0) aadd(&out[eacc_slot[threadIdx.x]], s); }
-    else out[eacc_slot[threadIdx.x]] = s;
+    double v = ered[threadIdx.x];
+    if (atomic) { if (v != 0.0) atomicAdd(&out[eacc_slot[threadIdx.x]], v); }
+    else out[eacc_slot[threadIdx.x]] = v;
   }
-  __syncthreads();
 }
 
 // ------------------------------------------------------------------------
@@ -87,13 +73,8 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
   V.aps = B.aps + (long long)R.res_off * 12;
   V.ch_start = B.ch_start + R.ch_off;
   V.ch_len = B.ch_len + R.ch_off;
-  V.x = slab;
-  V.xv = slab + 9LL * n;
-  V.f = slab + 18LL * n;
-  double *s = slab + 27LL * n;
-  V.rho_w = s; V.rho_h = s + n; V.sig_w = s + 2 * n; V.sigp_w = s + 3 * n; V.sig_h = s + 4 * n;
-  V.sigp_h = s + 5 * n; V.xi_w = s + 6 * n; V.bf = s + 7 * n; V.hxi1 = s + 8 * n; V.hxi2 = s + 9 * n;
-  V.ext = (float *)(s + 10 * n);
+  V.s = slab;
+  V.ered = nullptr;
   V.P = B.P;
   V.groups = B.groups + R.fm_group_off;
   V.ngroups = R.fm_ngroups;
@@ -110,29 +91,30 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
 // resident engine: one workgroup per replica, nsteps inside the launch
 
 template <class T>
-__device__ __forceinline__ void force_eval_block(const View<T> &V, EAcc &E)
+__device__ __forceinline__ void force_eval_block(const View<T> &V)
 {
   int t0 = threadIdx.x, ts = blockDim.x;
+  if (threadIdx.x < N_EACC) V.ered[threadIdx.x] = 0.0;
   phase_A(V, t0, ts);
   __syncthreads();
-  phase_B(V, t0, ts, E);
+  phase_B(V, t0, ts);
   __syncthreads();
-  phase_C(V, t0, ts, E);
+  phase_C(V, t0, ts);
   __syncthreads();
-  phase_D(V, t0, ts, E);
+  phase_D(V, t0, ts);
   __syncthreads();
   phase_E(V, t0, ts);
   __syncthreads();
-  phase_F(V, t0, ts, E);
+  phase_F(V, t0, ts);
   __syncthreads();
 }
 
-template <class T, bool LDSM>
-__global__ void __launch_bounds__(1024) k_resident(BatchDev B, LgParams lg, int mode)
+template <class T, bool LDSM, int NT>
+__global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mode)
 {
   extern __shared__ double smem[];
   __shared__ int s_rep;
-  double *red = smem;                 // 16 waves x N_EACC
+  __shared__ double ered[N_EACC];
   for (;;) {
     if (threadIdx.x == 0) s_rep = atomicAdd(B.queue, 1);
     __syncthreads();
@@ -141,28 +123,27 @@ __global__ void __launch_bounds__(1024) k_resident(BatchDev B, LgParams lg, int 
     if (slot >= B.nrep) break;
     int r = B.order[slot];
     const ReplicaDesc R = B.rep[r];
-    double *slab = LDSM ? (smem + 16 * N_EACC) : (B.scratch + R.scratch_off);
+    double *slab = LDSM ? smem : (B.scratch + R.scratch_off);
     View<T> V = make_view<T>(B, R, slab);
+    V.ered = ered;
     int n = R.n;
     double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
-    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x[k] = gx[k];
+    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
     __syncthreads();
-    EAcc E;
-    E.clear();
-    force_eval_block(V, E);
+    force_eval_block(V);
     if (mode == 1) {
       for (int s = 0; s < lg.nsteps; ++s) {
         integrate1(V, gv, lg, threadIdx.x, blockDim.x);
         __syncthreads();
-        E.clear();
-        force_eval_block(V, E);
+        force_eval_block(V);
         integrate2(V, gv, lg, s, r, threadIdx.x, blockDim.x);
         __syncthreads();
       }
-      for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gx[k] = V.x[k];
+      for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gx[k] = V.x()[k];
     }
-    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gf[k] = V.f[k];
-    block_energy(E, red, B.energy + (long long)r * AW_N_ET, false);
+    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gf[k] = V.f()[k];
+    energy_out(ered, B.energy + (long long)r * AW_N_ET, false);
+    __syncthreads();
   }
 }
 
@@ -183,7 +164,7 @@ __global__ void __launch_bounds__(256) k_ph_load(BatchDev B, int r)
   const ReplicaDesc R = B.rep[r];
   View<T> V = phase_view<T>(B, r);
   const double *gx = B.x + 9LL * R.res_off;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) V.x[k] = gx[k];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) V.x()[k] = gx[k];
 }
 
 template <class T>
@@ -198,18 +179,20 @@ __global__ void __launch_bounds__(256) k_ph_A(BatchDev B, int r)
   template <class T>                                                                   \
   __global__ void __launch_bounds__(256) NAME(BatchDev B, int r)                       \
   {                                                                                    \
-    __shared__ double red[4 * N_EACC];                                                 \
+    __shared__ double ered[N_EACC];                                                    \
     View<T> V = phase_view<T>(B, r);                                                   \
-    EAcc E;                                                                            \
-    E.clear();                                                                         \
+    V.ered = ered;                                                                     \
+    if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;                                 \
+    __syncthreads();                                                                   \
     CALL;                                                                              \
-    block_energy(E, red, B.energy + (long long)r * AW_N_ET, true);                     \
+    __syncthreads();                                                                   \
+    energy_out(ered, B.energy + (long long)r * AW_N_ET, true);                         \
   }
 
-PH_KERNEL(k_ph_B, phase_B(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
-PH_KERNEL(k_ph_C, phase_C(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
-PH_KERNEL(k_ph_D, phase_D(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
-PH_KERNEL(k_ph_F, phase_F(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, E))
+PH_KERNEL(k_ph_B, phase_B(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
+PH_KERNEL(k_ph_C, phase_C(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
+PH_KERNEL(k_ph_D, phase_D(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
+PH_KERNEL(k_ph_F, phase_F(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
 
 template <class T>
 __global__ void __launch_bounds__(256) k_ph_E(BatchDev B, int r)
@@ -225,8 +208,8 @@ __global__ void __launch_bounds__(256) k_ph_store(BatchDev B, int r, int with_x)
   View<T> V = phase_view<T>(B, r);
   double *gf = B.f + 9LL * R.res_off, *gx = B.x + 9LL * R.res_off;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) {
-    gf[k] = V.f[k];
-    if (with_x) gx[k] = V.x[k];
+    gf[k] = V.f()[k];
+    if (with_x) gx[k] = V.x()[k];
   }
 }
 
@@ -363,7 +346,7 @@ int aw_lds_max_residues()
     int dev = 0, lim = 0;
     hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
-    long long avail = lim / 8 - 16 * N_EACC - 16;
+    long long avail = lim / 8 - 16;
     int n = 1;
     while (slab_doubles(n + 1) <= avail) ++n;
     g_lds_max = n;
@@ -405,15 +388,25 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
   int mode = run.mode;
   hipMemsetAsync(p.queue, 0, sizeof(int), st);
-  int threads = run.threads > 0 ? run.threads : 1024;
+  // 512 threads: the whole step fits the 256-VGPR budget without spills;
+  // 1024 threads (128 VGPRs) is kept for comparison
+  int threads = run.threads == 1024 ? 1024 : 512;
   int grid = run.grid > 0 ? run.grid : p.nrep;
-  if (p.lds_resident) {
-    size_t lds = (size_t)(16 * N_EACC + slab_doubles(p.max_n)) * sizeof(double);
-    hipFuncSetAttribute((const void *)k_resident<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_resident<T, true>), dim3(grid), dim3(threads), lds, st, B, lg, mode);
+  size_t lds = p.lds_resident ? (size_t)slab_doubles(p.max_n) * sizeof(double) : 0;
+  if (threads == 1024) {
+    if (p.lds_resident) {
+      hipFuncSetAttribute((const void *)k_resident<T, true, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_resident<T, true, 1024>), dim3(grid), dim3(1024), lds, st, B, lg, mode);
+    } else {
+      hipLaunchKernelGGL((k_resident<T, false, 1024>), dim3(grid), dim3(1024), 0, st, B, lg, mode);
+    }
   } else {
-    size_t lds = (size_t)(16 * N_EACC) * sizeof(double);
-    hipLaunchKernelGGL((k_resident<T, false>), dim3(grid), dim3(threads), lds, st, B, lg, mode);
+    if (p.lds_resident) {
+      hipFuncSetAttribute((const void *)k_resident<T, true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_resident<T, true, 512>), dim3(grid), dim3(512), lds, st, B, lg, mode);
+    } else {
+      hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, lg, mode);
+    }
   }
   hipLaunchKernelGGL(k_energy_total, dim3((p.nrep + 63) / 64), dim3(64), 0, st, p.energy, p.nrep);
   return hipGetLastError() == hipSuccess ? 0 : -1;

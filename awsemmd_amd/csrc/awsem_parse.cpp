@@ -158,6 +158,10 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
   P.k_global_pap *= e;
   for (int j = 0; j < P.n_rama_par; ++j) P.w[j] *= P.k_rama;
   for (int j = 6; j < 6 + P.n_rama_p_par; ++j) P.w[j] *= P.k_rama;
+  for (int j = 0; j < 12; ++j) {
+    P.cphi0[j] = std::cos(P.phi0[j]); P.sphi0[j] = std::sin(P.phi0[j]);
+    P.cpsi0[j] = std::cos(P.psi0[j]); P.spsi0[j] = std::sin(P.psi0[j]);
+  }
   // cWell ranges (sml.h:363-368), cP_AP ranges (sml.h:111-115)
   for (int k = 0; k < P.n_wells; ++k) {
     P.w_rmin_th[k] = P.well_r_min[k] - 8.0 * 2.302585 / P.water_kappa;

@@ -95,11 +95,8 @@ struct View {
   const double *aps;    // [n][12] (global)
   const int *ch_start;  // [nch] 0-based
   const int *ch_len;
-  double *x;            // [3][n][3] CA, CB, O
-  double *xv;           // [3][n][3] N, H, C'
-  double *f;            // [3][n][3]
-  double *rho_w, *rho_h, *sig_w, *sigp_w, *sig_h, *sigp_h, *xi_w, *bf, *hxi1, *hxi2;
-  float *ext;           // [n] distance from CA to the farthest own atom
+  double *s;            // working slab (LDS or global): see the accessors
+  double *ered;         // energy accumulators [N_EACC] (LDS)
   const FFParams *P;
   // fragment memory
   const FMGroup *groups;
@@ -108,11 +105,28 @@ struct View {
   const T *coef;
   const void *rf;        // T4 per (memory, pair)
   const void *items;     // FMItem / FMItemD per item
+  // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
+  // f[3][n][3], ten per-residue scalars, ext (float)
+  AW_DEV double *x() const { return s; }
+  AW_DEV double *xv() const { return s + 9 * n; }
+  AW_DEV double *f() const { return s + 18 * n; }
+  AW_DEV double *sc(int k) const { return s + 27 * n + k * n; }
+  AW_DEV double *rho_w() const { return sc(0); }
+  AW_DEV double *rho_h() const { return sc(1); }
+  AW_DEV double *sig_w() const { return sc(2); }
+  AW_DEV double *sigp_w() const { return sc(3); }
+  AW_DEV double *sig_h() const { return sc(4); }
+  AW_DEV double *sigp_h() const { return sc(5); }
+  AW_DEV double *xi_w() const { return sc(6); }
+  AW_DEV double *bf() const { return sc(7); }
+  AW_DEV double *hxi1() const { return sc(8); }
+  AW_DEV double *hxi2() const { return sc(9); }
+  AW_DEV float *ext() const { return (float *)sc(10); }
 };
 
-#define XA(V, kind, i) ((V).x + ((kind) * (V).n + (i)) * 3)
-#define XV(V, kind, i) ((V).xv + ((kind) * (V).n + (i)) * 3)
-#define FA(V, kind, i) ((V).f + ((kind) * (V).n + (i)) * 3)
+#define XA(V, kind, i) ((V).x() + ((kind) * (V).n + (i)) * 3)
+#define XV(V, kind, i) ((V).xv() + ((kind) * (V).n + (i)) * 3)
+#define FA(V, kind, i) ((V).f() + ((kind) * (V).n + (i)) * 3)
 enum { K_CA = 0, K_CB = 1, K_O = 2 };
 enum { K_N = 0, K_H = 1, K_CP = 2 };
 
@@ -150,6 +164,29 @@ AW_DEV double eacc_get(const EAcc &e, int k)
 AW_DEVCONST int eacc_slot[N_EACC] = {AW_ET_CHAIN, AW_ET_CHI, AW_ET_RAMA, AW_ET_VEXCLUDED, AW_ET_DSSP,
                                                  AW_ET_PAP, AW_ET_WATER, AW_ET_BURIAL, AW_ET_HELIX, AW_ET_FRAGMEM,
                                                  AW_ET_BOND, AW_ET_SHAKE};
+
+// adds a thread's energy accumulators into ered[N_EACC] (LDS): a wave
+// reduction, then one LDS atomic per wave and term
+#ifdef AW_HOST_EMU
+AW_DEV void eflush(const EAcc &E, double *ered)
+{
+  for (int k = 0; k < N_EACC; ++k) ered[k] += eacc_get(E, k);
+}
+#else
+AW_DEV double wave_sum_d(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+AW_DEV void eflush(const EAcc &E, double *ered)
+{
+  int lane = threadIdx.x & 63;
+  for (int k = 0; k < N_EACC; ++k) {
+    double v = wave_sum_d(eacc_get(E, k));
+    if (lane == 0 && v != 0.0) atomicAdd(&ered[k], v);
+  }
+}
+#endif
 
 #define ON(P, slot) ((P)->term_mask & (1u << (slot)))
 
@@ -207,15 +244,15 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
     e2 = fmax(e2, rsq_exact(ca, o));
     if (!(ri & RI_FIRST)) { e2 = fmax(e2, rsq_exact(ca, xn)); e2 = fmax(e2, rsq_exact(ca, xh)); }
     if (!(ri & RI_LAST)) e2 = fmax(e2, rsq_exact(ca, xcp));
-    V.ext[i] = (float)(sqrt(e2) * (1.0 + 1e-6) + 1e-4);
-    double *f = V.f;
+    V.ext()[i] = (float)(sqrt(e2) * (1.0 + 1e-6) + 1e-4);
+    double *f = V.f();
     for (int k = 0; k < 3; ++k) {
       f[(0 * n + i) * 3 + k] = 0.0;
       f[(1 * n + i) * 3 + k] = 0.0;
       f[(2 * n + i) * 3 + k] = 0.0;
     }
-    V.rho_w[i] = 0.0; V.rho_h[i] = 0.0; V.xi_w[i] = 0.0;
-    V.hxi1[i] = 0.0; V.hxi2[i] = 0.0; V.bf[i] = 0.0;
+    V.rho_w()[i] = 0.0; V.rho_h()[i] = 0.0; V.xi_w()[i] = 0.0;
+    V.hxi1()[i] = 0.0; V.hxi2()[i] = 0.0; V.bf()[i] = 0.0;
   }
 }
 
@@ -323,14 +360,18 @@ AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
   fadd(V, K_CB, i, force, cp_);
 }
 
-// calcDihedralAndSlopes, fb.cpp:2341-2453 (slope[atom]: CA(i-1), CA(i),
-// CA(i+1), O(i-1), O(i)); the x and y slopes are summed since the force
-// uses their sum (fb.cpp:2467-2504)
+// calcDihedralAndSlopes, fb.cpp:2341-2453.  The angle and its slopes are
+// produced in two calls so that the 5x3 slope table never has to be held:
+// dihedral(..., 0, cs, sn) returns cos and sin of the angle (X/|(X,Y)|,
+// Y/|(X,Y)|: the angle is atan2(Y, X)); dihedral(..., fscale, ...)
+// recomputes the geometry and adds fscale * (y_slope + x_slope) of each of
+// the five atoms CA(i-1), CA(i), CA(i+1), O(i-1), O(i) into the forces
+// (compute_rama_force, fb.cpp:2455-2506, summed over the wells).
 template <class T>
-AW_DEV double dihedral(const View<T> &V, int i, bool phi, V3<double> s[5])
+AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &cs, double &sn)
 {
   const FFParams *P = V.P;
-  double an = (double)P->an, bn = (double)P->bn, cn = (double)P->cn, ap = (double)P->ap, bp = (double)P->bp, cp = (double)P->cp;
+  double an = P->an, bn = P->bn, cn = P->cn, ap = P->ap, bp = P->bp, cp = P->cp;
   V3<double> a, b, c;
   if (phi) {
     a = dvec<double>(XV(V, K_CP, i), XA(V, K_CA, i));
@@ -342,43 +383,56 @@ AW_DEV double dihedral(const View<T> &V, int i, bool phi, V3<double> s[5])
     c = dvec<double>(XA(V, K_CA, i), XV(V, K_N, i));
   }
   double adb = dot(a, b), bdc = dot(b, c), adc = dot(a, c), b2 = dot(b, b), bm = sqrt(b2);
-  V3<double> bxa = crs(b, a), cxa = crs(c, a), cxb = crs(c, b);
+  V3<double> bxa = crs(b, a);
   double cdbxa = dot(c, bxa);
   double Y = bm * cdbxa, X = adb * bdc - b2 * adc;
-  double angle = atan2(Y, X);
-  double X2Y2 = (double)1 / (X * X + Y * Y);
+  if (fscale == 0.0) {
+    double h = sqrt(X * X + Y * Y);
+    if (h > 0.0) { cs = X / h; sn = Y / h; }
+    else { cs = 1.0; sn = 0.0; }  // atan2(0, 0) = 0
+    return;
+  }
+  V3<double> cxa = crs(c, a), cxb = crs(c, b);
+  double X2Y2 = 1.0 / (X * X + Y * Y);
   double dy = X * X2Y2, dx = -Y * X2Y2;
   b2 *= dx; adb *= dx; adc *= dx; bdc *= dx;
   cdbxa *= dy / bm;
   bm *= dy;
-  const double av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
-  const double bxav[3] = {bxa.x, bxa.y, bxa.z}, cxav[3] = {cxa.x, cxa.y, cxa.z}, cxbv[3] = {cxb.x, cxb.y, cxb.z};
-  double out[5][3];
+  double *fca_m = FA(V, K_CA, i - 1), *fca_0 = FA(V, K_CA, i), *fca_p = FA(V, K_CA, i + 1);
+  double *fo_m = FA(V, K_O, i - 1), *fo_0 = FA(V, K_O, i);
   for (int l = 0; l < 3; l++) {
-    double h1, h2, h3, h4, g1, g2, g3;
-    g1 = bv[l] * bdc - cv[l] * b2;
-    g2 = av[l] * bdc - (double)2 * bv[l] * adc + cv[l] * adb;
-    g3 = bv[l] * adb - av[l] * b2;
+    double av = l == 0 ? a.x : (l == 1 ? a.y : a.z), bv = l == 0 ? b.x : (l == 1 ? b.y : b.z);
+    double cv = l == 0 ? c.x : (l == 1 ? c.y : c.z);
+    double bxav = l == 0 ? bxa.x : (l == 1 ? bxa.y : bxa.z), cxav = l == 0 ? cxa.x : (l == 1 ? cxa.y : cxa.z);
+    double cxbv = l == 0 ? cxb.x : (l == 1 ? cxb.y : cxb.z);
+    double h1, h2, h3, h4;
+    double g1 = bv * bdc - cv * b2;
+    double g2 = av * bdc - 2.0 * bv * adc + cv * adb;
+    double g3 = bv * adb - av * b2;
+    double o0, o1, o2, o3, o4;
     if (phi) {
-      h1 = cxbv[l] * bm; h2 = cxav[l] * bm; h3 = bxav[l] * bm; h4 = bv[l] * cdbxa;
-      out[0][l] = -an * h4 + (an - ap) * h3 + an * h2 + (-an * g2 + (an - ap) * g3);
-      out[1][l] = ((double)1 - bn) * h4 + (bn - bp) * h3 + (ap - (double)1) * h1 - ((double)1 - bn) * h2 +
-                  ((ap - (double)1) * g1 + ((double)1 - bn) * g2 + (bn - bp) * g3);
-      out[2][l] = bp * h1 + bp * g1;
-      out[3][l] = -cn * h4 + (cn - cp) * h3 + cn * h2 + (-cn * g2 + (cn - cp) * g3);
-      out[4][l] = cp * h1 + cp * g1;
+      h1 = cxbv * bm; h2 = cxav * bm; h3 = bxav * bm; h4 = bv * cdbxa;
+      o0 = -an * h4 + (an - ap) * h3 + an * h2 + (-an * g2 + (an - ap) * g3);
+      o1 = (1.0 - bn) * h4 + (bn - bp) * h3 + (ap - 1.0) * h1 - (1.0 - bn) * h2 +
+           ((ap - 1.0) * g1 + (1.0 - bn) * g2 + (bn - bp) * g3);
+      o2 = bp * h1 + bp * g1;
+      o3 = -cn * h4 + (cn - cp) * h3 + cn * h2 + (-cn * g2 + (cn - cp) * g3);
+      o4 = cp * h1 + cp * g1;
     } else {
-      h1 = bxav[l] * bm; h2 = cxbv[l] * bm; h3 = cxav[l] * bm; h4 = bv[l] * cdbxa;
-      out[0][l] = -an * h1 + (-an * g3);
-      out[1][l] = (ap - (double)1) * h4 + ((double)1 - bn) * h1 + (an - ap) * h2 - (ap - (double)1) * h3 +
-                  ((an - ap) * g1 + (ap - (double)1) * g2 + ((double)1 - bn) * g3);
-      out[2][l] = bp * h4 + (bn - bp) * h2 - bp * h3 + ((bn - bp) * g1 + bp * g2);
-      out[3][l] = -cn * h1 + (-cn * g3);
-      out[4][l] = cp * h4 + (cn - cp) * h2 - cp * h3 + ((cn - cp) * g1 + cp * g2);
+      h1 = bxav * bm; h2 = cxbv * bm; h3 = cxav * bm; h4 = bv * cdbxa;
+      o0 = -an * h1 + (-an * g3);
+      o1 = (ap - 1.0) * h4 + (1.0 - bn) * h1 + (an - ap) * h2 - (ap - 1.0) * h3 +
+           ((an - ap) * g1 + (ap - 1.0) * g2 + (1.0 - bn) * g3);
+      o2 = bp * h4 + (bn - bp) * h2 - bp * h3 + ((bn - bp) * g1 + bp * g2);
+      o3 = -cn * h1 + (-cn * g3);
+      o4 = cp * h4 + (cn - cp) * h2 - cp * h3 + ((cn - cp) * g1 + cp * g2);
     }
+    aadd(fca_m + l, fscale * o0);
+    aadd(fca_0 + l, fscale * o1);
+    aadd(fca_p + l, fscale * o2);
+    aadd(fo_m + l, fscale * o3);
+    aadd(fo_0 + l, fscale * o4);
   }
-  for (int k = 0; k < 5; ++k) { s[k].x = out[k][0]; s[k].y = out[k][1]; s[k].z = out[k][2]; }
-  return angle;
 }
 
 // compute_rama_potential / compute_rama_force, fb.cpp:2455-2546
@@ -386,9 +440,9 @@ template <class T>
 AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
 {
   const FFParams *P = V.P;
-  V3<double> sphi[5], spsi[5];
-  double phi = dihedral(V, i, true, sphi);
-  double psi = dihedral(V, i, false, spsi);
+  double cphi, sphi, cpsi, spsi, dummy;
+  dihedral(V, i, true, 0.0, cphi, sphi);
+  dihedral(V, i, false, 0.0, cpsi, spsi);
   int js = 0, je = P->n_rama_par;
   if ((ri & RI_PRO) && P->rama_p_flag) { js = 6; je = 6 + P->n_rama_p_par; }
   double fphi = 0, fpsi = 0;
@@ -396,21 +450,21 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
   for (int j = js; j < je; j++) {
     double apsv = V.aps[i * 12 + j];
     if (P->ssweight[j] && apsv == 0.0) continue;
-    double cphi = cos(phi + (double)P->phi0[j]) - (double)1, cpsi = cos(psi + (double)P->psi0[j]) - (double)1;
-    double pc = (double)P->phiw[j] * cphi, sc = (double)P->psiw[j] * cpsi;
-    double Vv = (double)P->w[j] * exp(-cphi * pc - cpsi * sc);
-    if (P->ssweight[j]) Vv *= (double)apsv;
-    double force = (double)2 * Vv;
-    fphi += force * pc * sin(phi + (double)P->phi0[j]);
-    fpsi += force * sc * sin(psi + (double)P->psi0[j]);
-    ev += (double)(-Vv);
+    // cos/sin(phi + phi0) by angle addition
+    double cp_ = cphi * P->cphi0[j] - sphi * P->sphi0[j], sp_ = sphi * P->cphi0[j] + cphi * P->sphi0[j];
+    double cq_ = cpsi * P->cpsi0[j] - spsi * P->spsi0[j], sq_ = spsi * P->cpsi0[j] + cpsi * P->spsi0[j];
+    double cph = cp_ - 1.0, cps = cq_ - 1.0;
+    double pc = P->phiw[j] * cph, sc = P->psiw[j] * cps;
+    double Vv = P->w[j] * exp(-cph * pc - cps * sc);
+    if (P->ssweight[j]) Vv *= apsv;
+    double force = 2.0 * Vv;
+    fphi += force * pc * sp_;
+    fpsi += force * sc * sq_;
+    ev += -Vv;
   }
   E.rama += ev;
-  const int kinds[5] = {K_CA, K_CA, K_CA, K_O, K_O};
-  const int res[5] = {i - 1, i, i + 1, i - 1, i};
-  for (int k = 0; k < 5; ++k)
-    fadd3(V, kinds[k], res[k], fphi * sphi[k].x + fpsi * spsi[k].x, fphi * sphi[k].y + fpsi * spsi[k].y,
-          fphi * sphi[k].z + fpsi * spsi[k].z);
+  if (fphi != 0.0) dihedral(V, i, true, fphi, dummy, dummy);
+  if (fpsi != 0.0) dihedral(V, i, false, fpsi, dummy, dummy);
 }
 
 // LAMMPS bond_style harmonic on the deck's bonds (outside the reference).
@@ -659,12 +713,9 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   bool on[4] = {true, i_rep, i_AP, i_P};
   int acc[4] = {i, i, j, j}, don[4] = {j, j + 1, i, i + 2};
   T theta[4], p0[4], p1[4];
-  V3<T> xNO[4], xHO[4];
   for (int k = 0; k < 4; ++k) {
     theta[k] = 0; p0[k] = 0; p1[k] = 0;
     if (!on[k]) continue;
-    xNO[k] = dvec<T>(XA(V, K_O, acc[k]), XV(V, K_N, don[k]));
-    xHO[k] = dvec<T>(XA(V, K_O, acc[k]), XV(V, K_H, don[k]));
     // distances and their offsets from the well centres in FP64 (the
     // offsets are small differences of ~3 A quantities)
     double RNO = sqrt(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_N, don[k])));
@@ -695,13 +746,13 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     fadd(V, K_CA, j - 2, fq, dn);
     fadd(V, K_CA, j + 2, -fq, dn);
   }
-  T Vk[4] = {VT, Vv[1], Vv[2], Vv[3]};
+  Vv[0] = VT;
   T an = (T)P->an, bn = (T)P->bn, cn = (T)P->cn, ah = (T)P->ah, bh = (T)P->bh, ch = (T)P->ch;
   for (int k = 0; k < 4; ++k) {
     if (!on[k]) continue;
-    T q0 = p0[k] * Vk[k], q1 = p1[k] * Vk[k];
+    T q0 = p0[k] * Vv[k], q1 = p1[k] * Vv[k];
     int a = acc[k], d = don[k];
-    const V3<T> &u = xNO[k], &w = xHO[k];
+    V3<T> u = dvec<T>(XA(V, K_O, a), XV(V, K_N, d)), w = dvec<T>(XA(V, K_O, a), XV(V, K_H, d));
     fadd3(V, K_O, a, -(q0 * u.x + q1 * w.x), -(q0 * u.y + q1 * w.y), -(q0 * u.z + q1 * w.z));
     fadd3(V, K_CA, d - 1, an * q0 * u.x + ah * q1 * w.x, an * q0 * u.y + ah * q1 * w.y, an * q0 * u.z + ah * q1 * w.z);
     fadd3(V, K_CA, d, bn * q0 * u.x + bh * q1 * w.x, bn * q0 * u.y + bh * q1 * w.y, bn * q0 * u.z + bh * q1 * w.z);
@@ -823,7 +874,7 @@ AW_DEV bool near_pair(const View<T> &V, int i, int j, float reach)
 {
   const double *a = XA(V, K_CA, i), *b = XA(V, K_CA, j);
   float dx = (float)(a[0] - b[0]), dy = (float)(a[1] - b[1]), dz = (float)(a[2] - b[2]);
-  float lim = reach + V.ext[i] + V.ext[j] + 1e-3f;
+  float lim = reach + V.ext()[i] + V.ext()[j] + 1e-3f;
   return dx * dx + dy * dy + dz * dz <= lim * lim * 1.0001f;
 }
 
@@ -858,14 +909,14 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E)
     if (P->water_flag && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
       T th, prd;
       wtheta<T>(P, 0, sqrt(rsq), th, prd);
-      aadd(&V.rho_w[i], (double)th);
-      aadd(&V.rho_w[j], (double)th);
+      aadd(&V.rho_w()[i], (double)th);
+      aadd(&V.rho_w()[j], (double)th);
     }
     if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
       T th, prd;
       htheta<T>(P, sqrt(rsq), th, prd);
-      aadd(&V.rho_h[i], (double)th);
-      aadd(&V.rho_h[j], (double)th);
+      aadd(&V.rho_h()[i], (double)th);
+      aadd(&V.rho_h()[j], (double)th);
     }
   }
   // DSSP in both orientations (fb.cpp:8064-8094)
@@ -887,56 +938,59 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E)
 }
 
 template <class T>
-AW_DEV void phase_B(const View<T> &V, int t0, int ts, EAcc &E)
+AW_DEV void phase_B(const View<T> &V, int t0, int ts)
 {
+  EAcc E;
+  E.clear();
   const FFParams *P = V.P;
   int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM)) ? V.fm_items : 0;
   int n = V.n;
   int np = pair_count(n);
-  int total = nfm + n + np;
-  for (int k = t0; k < total; k += ts) {
-    if (k < nfm) fm_item(V, V.fm_item_base + k, E);
-    else if (k < nfm + n) residue_local(V, k - nfm, E);
-    else {
-      int i, j;
-      pair_decode(k - nfm - n, n, i, j);
-      pair_B(V, i, j, E);
-    }
+  for (int k = t0; k < nfm; k += ts) fm_item(V, V.fm_item_base + k, E);
+  for (int k = t0; k < n; k += ts) residue_local(V, k, E);
+  for (int k = t0; k < np; k += ts) {
+    int i, j;
+    pair_decode(k, n, i, j);
+    pair_B(V, i, j, E);
   }
+  eflush(E, V.ered);
 }
 
 // ------------------------------------------------------------------------
 // Phase C: sigma_h and burial (SL, fb.cpp:7795-7841)
 
 template <class T>
-AW_DEV void phase_C(const View<T> &V, int t0, int ts, EAcc &E)
+AW_DEV void phase_C(const View<T> &V, int t0, int ts)
 {
+  EAcc E;
+  E.clear();
   const FFParams *P = V.P;
   for (int i = t0; i < V.n; i += ts) {
     int it = rtype(V.rinfo[i]);
     if (P->water_flag) {
-      double th = tanh(P->water_kappa_sigma * (V.rho_w[i] - P->treshold));
+      double th = tanh(P->water_kappa_sigma * (V.rho_w()[i] - P->treshold));
       double s = 0.5 * (1.0 - th);
-      V.sig_w[i] = s;
-      V.sigp_w[i] = -P->water_kappa_sigma * s * (1.0 + th);
+      V.sig_w()[i] = s;
+      V.sigp_w()[i] = -P->water_kappa_sigma * s * (1.0 + th);
     }
     if (P->helix_flag) {
-      double th = tanh(P->helix_kappa_sigma * (V.rho_h[i] - P->helix_treshold));
+      double th = tanh(P->helix_kappa_sigma * (V.rho_h()[i] - P->helix_treshold));
       double s = 0.5 * (1.0 - th);
-      V.sig_h[i] = s;
-      V.sigp_h[i] = -P->helix_kappa_sigma * s * (1.0 + th);
+      V.sig_h()[i] = s;
+      V.sigp_h()[i] = -P->helix_kappa_sigma * s * (1.0 + th);
     }
     if (P->burial_flag) {
-      double rho = V.rho_w[i], kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
+      double rho = V.rho_w()[i], kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
       for (int b = 0; b < 3; ++b) {
         double t0_ = tanh(kb * (rho - P->burial_ro_min[b])), t1_ = tanh(kb * (P->burial_ro_max[b] - rho));
         eb += -0.5 * P->k_burial * P->burial_gamma[it][b] * (t0_ + t1_);
         bfv += P->burial_gamma[it][b] * (t1_ * t1_ - t0_ * t0_);
       }
-      V.bf[i] = bfv * 0.5 * P->k_burial * kb;
+      V.bf()[i] = bfv * 0.5 * P->k_burial * kb;
       if (ON(P, AW_ET_BURIAL)) E.burial += eb;
     }
   }
+  eflush(E, V.ered);
 }
 
 // ------------------------------------------------------------------------
@@ -964,9 +1018,9 @@ AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
   T pt = -(T)P->k_helix * prob_sum * t_exp((T)-0.5 * (dNO * dNOs + dHO * dHOs));
   T p0 = -dNOs / RNO, p1 = -dHOs / RHO;
   double gdiff = P->helix_gamma_w - P->helix_gamma_p;
-  double sg = P->helix_gamma_p + gdiff * V.sig_h[i] * V.sig_h[j];
-  V.hxi1[i] = gdiff * (double)pt * V.sigp_h[i] * V.sig_h[j];
-  V.hxi2[i] = gdiff * (double)pt * V.sig_h[i] * V.sigp_h[j];
+  double sg = P->helix_gamma_p + gdiff * V.sig_h()[i] * V.sig_h()[j];
+  V.hxi1()[i] = gdiff * (double)pt * V.sigp_h()[i] * V.sig_h()[j];
+  V.hxi2()[i] = gdiff * (double)pt * V.sig_h()[i] * V.sigp_h()[j];
   if (!ON(P, AW_ET_HELIX)) return;
   T Vv = (T)sg * pt;
   E.helix += (double)Vv;
@@ -996,17 +1050,19 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j)
       T th, prd;
       wtheta<T>(P, w, sqrt(rsq), th, prd);
       double tg = (g1 - g0) * (double)th;
-      acc_i += tg * V.sig_w[j];
-      acc_j += tg * V.sig_w[i];
+      acc_i += tg * V.sig_w()[j];
+      acc_j += tg * V.sig_w()[i];
     }
   }
-  if (acc_i != 0.0) aadd(&V.xi_w[i], acc_i);
-  if (acc_j != 0.0) aadd(&V.xi_w[j], acc_j);
+  if (acc_i != 0.0) aadd(&V.xi_w()[i], acc_i);
+  if (acc_j != 0.0) aadd(&V.xi_w()[j], acc_j);
 }
 
 template <class T>
-AW_DEV void phase_D(const View<T> &V, int t0, int ts, EAcc &E)
+AW_DEV void phase_D(const View<T> &V, int t0, int ts)
 {
+  EAcc E;
+  E.clear();
   const FFParams *P = V.P;
   int n = V.n;
   int nh = P->helix_flag ? n : 0;
@@ -1019,12 +1075,13 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts, EAcc &E)
       xi_pair(V, i, j);
     }
   }
+  eflush(E, V.ered);
 }
 
 template <class T>
 AW_DEV void phase_E(const View<T> &V, int t0, int ts)
 {
-  for (int i = t0; i < V.n; i += ts) V.xi_w[i] *= V.sigp_w[i];
+  for (int i = t0; i < V.n; i += ts) V.xi_w()[i] *= V.sigp_w()[i];
 }
 
 // ------------------------------------------------------------------------
@@ -1052,7 +1109,7 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
       if ((!same || dres >= P->contact_cutoff) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
         double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1], sg;
         if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
-        else sg = g0 + (g1 - g0) * V.sig_w[i] * V.sig_w[j];
+        else sg = g0 + (g1 - g0) * V.sig_w()[i] * V.sig_w()[j];
         T th, prd;
         wtheta<T>(P, w, r, th, prd);
         ew += -sg * (double)th;
@@ -1062,8 +1119,8 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
     E.water += ew;
     if (in0) {
       double xi = 0.0;
-      if (fabs(V.xi_w[i]) > AW_DELTA_WATER_XI) xi += V.xi_w[i];
-      if (fabs(V.xi_w[j]) > AW_DELTA_WATER_XI) xi += V.xi_w[j];
+      if (fabs(V.xi_w()[i]) > AW_DELTA_WATER_XI) xi += V.xi_w()[i];
+      if (fabs(V.xi_w()[j]) > AW_DELTA_WATER_XI) xi += V.xi_w()[j];
       if (xi != 0.0) {
         T th, prd;
         wtheta<T>(P, 0, r, th, prd);
@@ -1074,13 +1131,13 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
   if (P->burial_flag && in0 && ON(P, AW_ET_BURIAL)) {
     T th, prd;
     wtheta<T>(P, 0, r, th, prd);
-    force += (T)(V.bf[i] + V.bf[j]) * prd;
+    force += (T)(V.bf()[i] + V.bf()[j]) * prd;
   }
   if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq && ON(P, AW_ET_HELIX)) {
     int hd = P->helix_i_diff;
-    double factor = V.hxi1[i] + V.hxi1[j];
-    if (i - hd >= 0) factor += V.hxi2[i - hd];
-    if (j - hd >= 0) factor += V.hxi2[j - hd];
+    double factor = V.hxi1()[i] + V.hxi1()[j];
+    if (i - hd >= 0) factor += V.hxi2()[i - hd];
+    if (j - hd >= 0) factor += V.hxi2()[j - hd];
     if (fabs(factor) > AW_DELTA_HELIX_XI) {
       T th, prd;
       htheta<T>(P, r, th, prd);
@@ -1096,8 +1153,10 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
 }
 
 template <class T>
-AW_DEV void phase_F(const View<T> &V, int t0, int ts, EAcc &E)
+AW_DEV void phase_F(const View<T> &V, int t0, int ts)
 {
+  EAcc E;
+  E.clear();
   int n = V.n;
   int np = pair_count(n);
   for (int k = t0; k < np; k += ts) {
@@ -1105,6 +1164,7 @@ AW_DEV void phase_F(const View<T> &V, int t0, int ts, EAcc &E)
     pair_decode(k, n, i, j);
     force_pair(V, i, j, E);
   }
+  eflush(E, V.ered);
 }
 
 // ------------------------------------------------------------------------
@@ -1149,9 +1209,9 @@ AW_DEV void integrate1(const View<T> &V, double *v, const LgParams &lg, int t0, 
     int kind = a / n, i = a - kind * n;
     double dtfm = dtf / atom_mass(kind, V.rinfo[i]);
     for (int l = 0; l < 3; ++l) {
-      double vv = v[3 * a + l] + dtfm * V.f[3 * a + l];
+      double vv = v[3 * a + l] + dtfm * V.f()[3 * a + l];
       v[3 * a + l] = vv;
-      V.x[3 * a + l] += lg.dt * vv;
+      V.x()[3 * a + l] += lg.dt * vv;
     }
   }
 }
@@ -1173,8 +1233,8 @@ AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, int s, i
     double g2 = sqrt(m) * sqrt(24.0 * AW_BOLTZ / lg.damp / lg.dt / AW_MVV2E) / AW_FTM2V * tsq;
     double dtfm = dtf / m;
     for (int l = 0; l < 3; ++l) {
-      double fv = V.f[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand(lg.seed, replica, step, a, l) - 0.5);
-      V.f[3 * a + l] = fv;
+      double fv = V.f()[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand(lg.seed, replica, step, a, l) - 0.5);
+      V.f()[3 * a + l] = fv;
       v[3 * a + l] += dtfm * fv;
     }
   }

@@ -37,6 +37,7 @@ struct FFParams {
   int n_rama_par, n_rama_p_par;
   int ssweight[12];
   double w[12], phiw[12], phi0[12], psiw[12], psi0[12];
+  double cphi0[12], sphi0[12], cpsi0[12], spsi0[12];  // cos/sin of phi0, psi0
   // DSSP hydrogen bonds
   double k_dssp, hbscl[4][9], NO_zero, HO_zero, sigma_NO_sqinv, sigma_HO_sqinv;
   double dssp_cut, dssp_cut_sq, pref[2], d_nu0, nu_cut1_sq, nu_cut2_sq;

@@ -138,6 +138,24 @@ def cpu_baseline(systems, budget_s=15.0):
                       f"(median length), {tot_steps} steps in {t_used:.1f}s on one core"}
 
 
+def calib_copy(nbytes):
+    """Device-to-device copy of nbytes: the reference kernel for scaling the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    hip.hipMalloc(C.byref(a), C.c_size_t(nbytes))
+    hip.hipMalloc(C.byref(b), C.c_size_t(nbytes))
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy of {nbytes} bytes x3 done")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -154,7 +172,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--threads", type=int, default=0, help="resident workgroup size (512 or 1024)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this size (HBM counter calibration)")
     a = ap.parse_args()
+    if a.threads:
+        os.environ["AWSEM_RESIDENT_THREADS"] = str(a.threads)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every replica is independent, so a multi-GPU run "
               "would only shard replicas across GPUs (see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -241,6 +264,8 @@ def main():
     k_avg_ms = float(np.mean(kernel_ms))
     achieved = bytes_step * a.chunk / (k_avg_ms * 1e-3) / 1e9
 
+    if a.calib_copy_mb:
+        calib_copy(a.calib_copy_mb << 20)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(systems, a.cpu_budget)
