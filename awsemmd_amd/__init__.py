@@ -36,7 +36,8 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_set_vexcluded", "awsem_set_bonds", "awsem_set_term_mask", "awsem_list", "awsem_run",
            "awsem_n_residues", "awsem_pair_list_cutoff", "awsem_last_error", "awsem_destroy",
            "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
-           "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_last_error",
+           "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_phase_clock",
+           "awsem_batch_last_error",
            "awsem_batch_destroy"]
 
 
@@ -86,6 +87,7 @@ def lib():
         L.awsem_batch_run.argtypes = [vp, C.c_int, C.POINTER(Langevin)]
         L.awsem_batch_timing.argtypes = [vp, dp, dp, ip]
         L.awsem_batch_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.awsem_batch_phase_clock.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.awsem_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
@@ -287,6 +289,12 @@ class Batch:
         a, b, k = C.c_double(), C.c_double(), C.c_int()
         lib().awsem_batch_timing(self.h, C.byref(a), C.byref(b), C.byref(k))
         return a.value, b.value, k.value
+
+    def phase_clock(self):
+        """Per-phase wall-clock ticks (100 MHz) when run with AWSEM_PHASE_PROF=1."""
+        out = (C.c_uint64 * 16)()
+        self._check(lib().awsem_batch_phase_clock(self.h, out))
+        return np.array(out[:], dtype=np.float64)
 
     def split(self, flat: np.ndarray):
         """Splits a concatenated residue-layout array into per-replica (3, n, 3) arrays."""

@@ -37,6 +37,7 @@ struct DevBatch {
   double *d_aps = nullptr, *d_x = nullptr, *d_v = nullptr, *d_f = nullptr, *d_e = nullptr, *d_scr = nullptr;
   long long *d_list = nullptr;
   int *d_count = nullptr;
+  unsigned long long *d_prof = nullptr;  // phase clock (AWSEM_PHASE_PROF=1)
   int list_cap = 0;
   FFParams P;
   double last_ms = 0.0;
@@ -44,7 +45,7 @@ struct DevBatch {
   ~DevBatch()
   {
     void *ptrs[] = {d_rep, d_params, d_groups, d_pairs, d_coef, d_rf, d_items, d_rinfo, d_chs, d_chl,
-                    d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count};
+                    d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count, d_prof};
     for (void *p : ptrs) if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
@@ -58,7 +59,7 @@ struct DevBatch {
     p.params = d_params; p.rinfo = d_rinfo; p.aps = d_aps; p.ch_start = d_chs; p.ch_len = d_chl;
     p.x = d_x; p.v = d_v; p.f = d_f; p.energy = d_e; p.scratch = d_scr;
     p.groups = d_groups; p.pairs = d_pairs; p.coef = d_coef; p.rf = d_rf; p.items = d_items;
-    p.queue = d_queue; p.order = d_order;
+    p.queue = d_queue; p.order = d_order; p.prof = d_prof;
     return p;
   }
 };
@@ -152,6 +153,12 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMemset(D->d_scr, 0, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
   HIPCHK(hipMalloc((void **)&D->d_queue, 64));
   HIPCHK(hipMalloc((void **)&D->d_count, 64));
+  if (const char *e = std::getenv("AWSEM_PHASE_PROF")) {
+    if (std::atoi(e) > 0) {
+      HIPCHK(hipMalloc((void **)&D->d_prof, 16 * sizeof(unsigned long long)));
+      HIPCHK(hipMemset(D->d_prof, 0, 16 * sizeof(unsigned long long)));
+    }
+  }
   // longest-first order for the resident engine's work queue
   std::vector<int> order(D->nrep);
   for (int r = 0; r < D->nrep; ++r) order[r] = r;
@@ -646,6 +653,14 @@ int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel
   if (ms_total) *ms_total = b->ms_total;
   if (ms_kernel_avg) *ms_kernel_avg = b->n_kernels ? b->ms_total / b->n_kernels : 0.0;
   if (n_kernels) *n_kernels = b->n_kernels;
+  return AWSEM_OK;
+}
+
+int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks16)
+{
+  if (!b || !b->dev || !ticks16) return AWSEM_E_ARG;
+  if (!b->dev->d_prof) return AWSEM_E_STATE;
+  if (hipMemcpy(ticks16, b->dev->d_prof, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess) return AWSEM_E_DEVICE;
   return AWSEM_OK;
 }
 

@@ -58,6 +58,7 @@ struct BatchDev {
   const void *items;
   int *queue;                 // work-queue counter (resident engine)
   const int *order;           // replica order (longest first)
+  unsigned long long *prof;   // optional phase clock: [2][8] (first replica, all replicas)
 };
 
 // doubles of per-replica working storage (x, xv, f, 10 scalars, ext+pad)
@@ -90,23 +91,40 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
 // ------------------------------------------------------------------------
 // resident engine: one workgroup per replica, nsteps inside the launch
 
+__device__ __forceinline__ void prof_mark(unsigned long long *prof, bool first, int k, unsigned long long &t)
+{
+  if (prof && threadIdx.x == 0) {
+    unsigned long long now = wall_clock64();
+    atomicAdd(&prof[8 + k], now - t);
+    if (first) atomicAdd(&prof[k], now - t);
+    t = now;
+  }
+}
+
 template <class T>
-__device__ __forceinline__ void force_eval_block(const View<T> &V)
+__device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long long *prof, bool first)
 {
   int t0 = threadIdx.x, ts = blockDim.x;
+  unsigned long long t = prof ? wall_clock64() : 0;
   if (threadIdx.x < N_EACC) V.ered[threadIdx.x] = 0.0;
   phase_A(V, t0, ts);
   __syncthreads();
+  prof_mark(prof, first, 0, t);
   phase_B(V, t0, ts);
   __syncthreads();
+  prof_mark(prof, first, 1, t);
   phase_C(V, t0, ts);
   __syncthreads();
+  prof_mark(prof, first, 2, t);
   phase_D(V, t0, ts);
   __syncthreads();
+  prof_mark(prof, first, 3, t);
   phase_E(V, t0, ts);
   __syncthreads();
+  prof_mark(prof, first, 4, t);
   phase_F(V, t0, ts);
   __syncthreads();
+  prof_mark(prof, first, 5, t);
 }
 
 template <class T, bool LDSM, int NT>
@@ -130,14 +148,19 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mo
     double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
     __syncthreads();
-    force_eval_block(V);
+    bool first = slot == 0;
+    force_eval_block(V, B.prof, first);
     if (mode == 1) {
       for (int s = 0; s < lg.nsteps; ++s) {
+        unsigned long long t = B.prof ? wall_clock64() : 0;
         integrate1(V, gv, lg, threadIdx.x, blockDim.x);
         __syncthreads();
-        force_eval_block(V);
+        prof_mark(B.prof, first, 6, t);
+        force_eval_block(V, B.prof, first);
+        t = B.prof ? wall_clock64() : 0;
         integrate2(V, gv, lg, s, r, threadIdx.x, blockDim.x);
         __syncthreads();
+        prof_mark(B.prof, first, 7, t);
       }
       for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gx[k] = V.x()[k];
     }
@@ -376,6 +399,7 @@ static BatchDev to_dev(const AwBatchPtrs &p)
   B.items = p.items;
   B.queue = p.queue;
   B.order = p.order;
+  B.prof = p.prof;
   return B;
 }
 

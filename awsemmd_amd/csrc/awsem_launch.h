@@ -17,6 +17,7 @@ struct AwBatchPtrs {
   const void *groups, *pairs, *coef, *rf, *items;
   int *queue;
   const int *order;
+  unsigned long long *prof;  // optional per-phase clock accumulators [16]
 };
 
 struct AwRun {

@@ -235,6 +235,12 @@ def main():
     res_total = sum(s.n for s in systems)
     log(f"timed {a.steps} steps: {value:.1f} replica-steps/s")
 
+    if os.environ.get("AWSEM_PHASE_PROF"):
+        ticks = batch.phase_clock() / 100.0  # microseconds
+        names = ["A", "B", "C", "D", "E", "F", "int1", "int2"]
+        nsteps_all = a.warmup + a.steps + 1
+        log("phase us/step, longest replica: " + " ".join(f"{n}={t / nsteps_all:.1f}" for n, t in zip(names, ticks[:8])))
+        log("phase us/step, mean replica:    " + " ".join(f"{n}={t / nsteps_all / a.replicas:.1f}" for n, t in zip(names, ticks[8:])))
     # force error of the batched FP32 path against the FP64 path (sample)
     batch.compute()
     xs, vs, fs, es = batch.get_state()

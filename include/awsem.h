@@ -176,6 +176,11 @@ int awsem_batch_run(awsem_batch *b, int nsteps, const awsem_langevin *lg);
  * batch stream, and the main kernel's average duration */
 int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel_avg,
                        int *n_kernels);
+/* Diagnostic: wall-clock ticks (100 MHz) accumulated per phase of the
+ * resident engine when the process runs with AWSEM_PHASE_PROF=1:
+ * [0..7] the first (longest) replica, [8..15] summed over replicas; phases
+ * A, B, C, D, E, F, integrate-1, integrate-2.  AWSEM_E_STATE otherwise. */
+int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks16);
 int awsem_batch_last_error(const awsem_batch *b, char *buf, size_t n);
 int awsem_batch_destroy(awsem_batch *b);
 

@@ -1,0 +1,145 @@
+"""Host-side tests of the C ABI (no GPU calls): exported symbols, the
+reference-format readers, error codes and the topology map."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import make_case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _A():
+    import awsemmd_amd as A
+    return A
+
+
+def test_library_exports_every_declared_symbol():
+    A = _A()
+    hdr = open(os.path.join(ROOT, "include", "awsem.h")).read()
+    declared = sorted(set(re.findall(r"\bint\s+(awsem_\w+)\s*\(", hdr) + re.findall(r"\bdouble\s+(awsem_\w+)\s*\(", hdr)))
+    out = subprocess.run(["nm", "-D", "--defined-only", A.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (awsem_\w+)", out))
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    assert set(A.EXPORTS) <= exported
+
+
+def test_create_reads_reference_file_set(oracle_mod):
+    A = _A()
+    s, p = make_case([30, 20], 3)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    assert fix.n == 50 == o.n
+    # fb.cpp:1048-1066 with the shipped coeff: the water well-2 outer edge
+    assert abs(fix.pair_list_cutoff - o.pair_list_cutoff) < 1e-12
+    assert abs(fix.pair_list_cutoff - (9.5 + 8 * 2.302585 / 5.0)) < 1e-12
+
+
+def test_create_from_arrays_matches_files():
+    A = _A()
+    from awsemmd_amd import workload as W
+    s, p = make_case([40], 8)
+    f1 = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    f2 = A.FixBackbone.from_arrays(p["coeff"], p["workdir"], precision=A.PREC_FP64, **W.arrays(s))
+    assert f1.n == f2.n == 40
+
+
+def test_error_codes(tmp_path):
+    A = _A()
+    s, p = make_case([20], 2)
+    with pytest.raises(A.AwsemError) as e:
+        A.FixBackbone(str(tmp_path / "nope.data"), p["seq"], p["workdir"])
+    assert e.value.code == -2
+    txt = open(p["coeff"]).read().replace("[Solvent_Barrier]-", "[Solvent_Barrier]")
+    bad = tmp_path / "coeff_ssb"
+    bad.write_text(txt)
+    with pytest.raises(A.AwsemError) as e:
+        A.FixBackbone(str(bad), p["seq"], p["workdir"])
+    assert e.value.code == -4 and "Solvent_Barrier" in str(e.value)
+    # gamma.dat truncated
+    d2 = tmp_path / "w2"
+    d2.mkdir()
+    for f in os.listdir(p["workdir"]):
+        data = open(os.path.join(p["workdir"], f), "rb").read()
+        if f == "gamma.dat":
+            data = data[: len(data) // 3]
+        (d2 / f).write_bytes(data)
+    with pytest.raises(A.AwsemError) as e:
+        A.FixBackbone(p["coeff"], p["seq"], str(d2))
+    assert e.value.code == -3 and "gamma.dat" in str(e.value)
+    # memory fragment past the end of the sequence
+    memf = d2 / "fragsLAMW.mem"
+    (d2 / "gamma.dat").write_bytes(open(os.path.join(p["workdir"], "gamma.dat"), "rb").read())
+    memf.write_text("[Target]\nquery\n\n[Memories]\nmem000.gro 15 15 9 1\n")
+    with pytest.raises(A.AwsemError) as e:
+        A.FixBackbone(p["coeff"], p["seq"], str(d2))
+    assert e.value.code == -3
+
+
+def _atoms(s, rng=None):
+    """LAMMPS-style atom table: per residue CA, O, CB (the data-file order of
+    CoordinatesToWorkLammpsDataFile.py), optionally shuffled."""
+    n = s.n
+    ch = s.chain_index()
+    group, mol, res, xyz = [], [], [], []
+    for i in range(n):
+        for g, arr in ((0, s.xca), (2, s.xo), (1, s.xcb)):
+            group.append(g)
+            mol.append(ch[i] + 1)
+            res.append(i + 1)
+            xyz.append(arr[i])
+    group, mol, res, xyz = map(np.array, (group, mol, res, xyz))
+    if rng is not None:
+        perm = rng.permutation(len(group))
+        group, mol, res, xyz = group[perm], mol[perm], res[perm], xyz[perm]
+    return group, mol, res, np.ascontiguousarray(xyz, dtype=np.float64)
+
+
+def test_topology_errors():
+    """Construct_Computational_Arrays error branches (fb.cpp:1640-1696)."""
+    A = _A()
+    s, p = make_case([12, 10], 4)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"])
+    g, m, r, x = _atoms(s)
+    fix.set_topology(g, m, r)
+    bad = m.copy()
+    bad[0] = 3
+    with pytest.raises(A.AwsemError) as e:
+        fix.set_topology(g, bad, r)
+    assert e.value.code == -5 and "Chain tag is out of range" in str(e.value)
+    bad = m.copy()
+    bad[1] = 2  # oxygen of residue 1 on the other chain
+    with pytest.raises(A.AwsemError) as e:
+        fix.set_topology(g, bad, r)
+    assert "different chain tag" in str(e.value)
+    bad = r.copy()
+    bad[0] = 0
+    with pytest.raises(A.AwsemError) as e:
+        fix.set_topology(g, m, bad)
+    assert "must be positive" in str(e.value)
+    gg = g.copy()
+    gg[2] = -1  # drop one CB
+    with pytest.raises(A.AwsemError) as e:
+        fix.set_topology(gg, m, r)
+    assert "same # of atoms" in str(e.value)
+    rr = r.copy()
+    rr[3 * 12] = 5  # a chain-2 residue claims a chain-1 tag
+    with pytest.raises(A.AwsemError):
+        fix.set_topology(g, m, rr)
+
+
+def test_workload_generator_is_seeded():
+    from awsemmd_amd import workload as W
+    a = W.make_system([30, 12], seed=77)
+    b = W.make_system([30, 12], seed=77)
+    assert a.chains == b.chains and np.array_equal(a.coords(), b.coords())
+    assert all(np.array_equal(u, v) for u, v in zip(a.mem_structs, b.mem_structs))
+    # self-avoidance and bond lengths of the trace
+    d = np.linalg.norm(np.diff(a.xca[:30], axis=0), axis=1)
+    assert np.allclose(d, 3.8)
+    # .gro round trip is exact: coordinates are k/1000 nm
+    assert np.array_equal(np.round(a.mem_structs[0] * 1000) / 1000, a.mem_structs[0])
