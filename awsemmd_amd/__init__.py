@@ -37,7 +37,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_n_residues", "awsem_pair_list_cutoff", "awsem_last_error", "awsem_destroy",
            "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
            "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_phase_clock",
-           "awsem_batch_last_error",
+           "awsem_batch_list_stats", "awsem_batch_last_error",
            "awsem_batch_destroy"]
 
 
@@ -88,6 +88,7 @@ def lib():
         L.awsem_batch_timing.argtypes = [vp, dp, dp, ip]
         L.awsem_batch_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.awsem_batch_phase_clock.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.awsem_batch_list_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
         L.awsem_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
@@ -295,6 +296,12 @@ class Batch:
         out = (C.c_uint64 * 16)()
         self._check(lib().awsem_batch_phase_clock(self.h, out))
         return np.array(out[:], dtype=np.float64)
+
+    def list_stats(self):
+        """(list builds so far, pairs in the current lists, replicas without a valid list)."""
+        b, p, k = C.c_longlong(), C.c_longlong(), C.c_int()
+        self._check(lib().awsem_batch_list_stats(self.h, C.byref(b), C.byref(p), C.byref(k)))
+        return b.value, p.value, k.value
 
     def split(self, flat: np.ndarray):
         """Splits a concatenated residue-layout array into per-replica (3, n, 3) arrays."""

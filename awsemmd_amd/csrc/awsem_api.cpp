@@ -38,6 +38,9 @@ struct DevBatch {
   long long *d_list = nullptr;
   int *d_count = nullptr;
   unsigned long long *d_prof = nullptr;  // phase clock (AWSEM_PHASE_PROF=1)
+  int *d_nl = nullptr;                   // Verlet lists (resident engine)
+  double *d_nlx = nullptr;
+  bool nl_dirty = true;
   int list_cap = 0;
   FFParams P;
   double last_ms = 0.0;
@@ -45,7 +48,8 @@ struct DevBatch {
   ~DevBatch()
   {
     void *ptrs[] = {d_rep, d_params, d_groups, d_pairs, d_coef, d_rf, d_items, d_rinfo, d_chs, d_chl,
-                    d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count, d_prof};
+                    d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count, d_prof,
+                    d_nl, d_nlx};
     for (void *p : ptrs) if (p) hipFree(p);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
@@ -60,6 +64,7 @@ struct DevBatch {
     p.x = d_x; p.v = d_v; p.f = d_f; p.energy = d_e; p.scratch = d_scr;
     p.groups = d_groups; p.pairs = d_pairs; p.coef = d_coef; p.rf = d_rf; p.items = d_items;
     p.queue = d_queue; p.order = d_order; p.prof = d_prof;
+    p.nl = d_nl; p.nlx = d_nlx;
     return p;
   }
 };
@@ -89,7 +94,7 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   std::vector<int> rinfo, chs, chl;
   std::vector<double> aps;
   AwFMPack fm;
-  long long scr = 0;
+  long long scr = 0, nl_ints = 0, nlx_dbl = 0;
   int res = 0, ch = 0;
   for (int r = 0; r < D->nrep; ++r) {
     const AwSystem &S = *sys[r];
@@ -107,6 +112,14 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
     R.fm_items = (int)(fm.items.size() / 4) - item_base;
     R.scratch_off = scr;
     scr += aw_slab_doubles(S.n);
+    // Verlet list capacity: all pairs for small systems, else 128 per residue
+    long long allp = (long long)S.n * (S.n - 1) / 2;
+    R.nl_cap = (int)std::min<long long>(std::max<long long>(allp, 1), 128LL * S.n);
+    if (S.n >= 32768) R.nl_cap = 0;  // pairs are packed as (i << 16) | j
+    R.nl_off = nl_ints;
+    nl_ints += 4 + R.nl_cap;
+    R.nlx_off = nlx_dbl;
+    nlx_dbl += 4LL * S.n;
     D->reps.push_back(R);
     D->res_off.push_back(res);
     rinfo.insert(rinfo.end(), S.rinfo.begin(), S.rinfo.end());
@@ -151,6 +164,10 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMemset(D->d_e, 0, (size_t)D->nrep * AW_N_ET * sizeof(double)));
   HIPCHK(hipMalloc((void **)&D->d_scr, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
   HIPCHK(hipMemset(D->d_scr, 0, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&D->d_nl, (size_t)std::max<long long>(nl_ints, 4) * sizeof(int)));
+  HIPCHK(hipMemset(D->d_nl, 0, (size_t)std::max<long long>(nl_ints, 4) * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&D->d_nlx, (size_t)std::max<long long>(nlx_dbl, 4) * sizeof(double)));
+  HIPCHK(hipMemset(D->d_nlx, 0, (size_t)std::max<long long>(nlx_dbl, 4) * sizeof(double)));
   HIPCHK(hipMalloc((void **)&D->d_queue, 64));
   HIPCHK(hipMalloc((void **)&D->d_count, 64));
   if (const char *e = std::getenv("AWSEM_PHASE_PROF")) {
@@ -593,6 +610,7 @@ int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v)
   DevBatch &D = *b->dev;
   size_t bytes = (size_t)D.total_res * 9 * sizeof(double);
   if (x && hipMemcpy(D.d_x, x, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy x"));
+  if (x) D.nl_dirty = true;
   if (v && hipMemcpy(D.d_v, v, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy v"));
   return AWSEM_OK;
 }
@@ -616,6 +634,8 @@ static int batch_launch(awsem_batch *b, AwRun run)
   DevBatch &D = *b->dev;
   // workgroup size of the resident engine (512 default; 1024 for comparison)
   if (const char *e = std::getenv("AWSEM_RESIDENT_THREADS")) run.threads = std::atoi(e);
+  run.nl_reset = D.nl_dirty ? 1 : 0;
+  D.nl_dirty = false;
   hipEventRecord(D.ev0, D.stream);
   if (aw_launch_resident(D.ptrs(), run, D.stream)) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
   hipEventRecord(D.ev1, D.stream);
@@ -661,6 +681,27 @@ int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks16)
   if (!b || !b->dev || !ticks16) return AWSEM_E_ARG;
   if (!b->dev->d_prof) return AWSEM_E_STATE;
   if (hipMemcpy(ticks16, b->dev->d_prof, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess) return AWSEM_E_DEVICE;
+  return AWSEM_OK;
+}
+
+int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *pairs, int *n_invalid)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  const DevBatch &D = *b->dev;
+  if (hipStreamSynchronize(D.stream) != hipSuccess) return AWSEM_E_DEVICE;
+  long long nb = 0, np = 0;
+  int bad = 0;
+  for (const ReplicaDesc &R : D.reps) {
+    int h[4] = {0, 0, 0, 0};
+    if (R.nl_cap > 0 && hipMemcpy(h, D.d_nl + R.nl_off, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+      return AWSEM_E_DEVICE;
+    nb += h[2];
+    np += h[0];
+    bad += h[1] ? 0 : 1;
+  }
+  if (builds) *builds = nb;
+  if (pairs) *pairs = np;
+  if (n_invalid) *n_invalid = bad;
   return AWSEM_OK;
 }
 

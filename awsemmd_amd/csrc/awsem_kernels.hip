@@ -59,6 +59,8 @@ struct BatchDev {
   int *queue;                 // work-queue counter (resident engine)
   const int *order;           // replica order (longest first)
   unsigned long long *prof;   // optional phase clock: [2][8] (first replica, all replicas)
+  int *nl;                    // Verlet lists (resident engine)
+  double *nlx;
 };
 
 // doubles of per-replica working storage (x, xv, f, 10 scalars, ext+pad)
@@ -85,6 +87,10 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
   V.coef = (const T *)B.coef;
   V.rf = B.rf;
   V.items = B.items;
+  V.nl = nullptr;
+  V.nlx = nullptr;
+  V.nl_cap = 0;
+  V.nl_state = nullptr;
   return V;
 }
 
@@ -109,6 +115,13 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
   if (threadIdx.x < N_EACC) V.ered[threadIdx.x] = 0.0;
   phase_A(V, t0, ts);
   __syncthreads();
+  if (V.nl) {
+    bool build = nl_needs_build(V);
+    if (build) phase_nl_build(V, t0, ts);
+    __syncthreads();
+    if (threadIdx.x == 0) nl_publish(V, build);
+    __syncthreads();
+  }
   prof_mark(prof, first, 0, t);
   phase_B(V, t0, ts);
   __syncthreads();
@@ -133,6 +146,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mo
   extern __shared__ double smem[];
   __shared__ int s_rep;
   __shared__ double ered[N_EACC];
+  __shared__ int nl_state[2];
   for (;;) {
     if (threadIdx.x == 0) s_rep = atomicAdd(B.queue, 1);
     __syncthreads();
@@ -144,6 +158,15 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mo
     double *slab = LDSM ? smem : (B.scratch + R.scratch_off);
     View<T> V = make_view<T>(B, R, slab);
     V.ered = ered;
+    V.nl = R.nl_cap > 0 ? B.nl + R.nl_off : nullptr;
+    V.nlx = B.nlx + R.nlx_off;
+    V.nl_cap = R.nl_cap;
+    V.nl_state = nl_state;
+    if (threadIdx.x == 0) {
+      nl_state[0] = 0;
+      nl_state[1] = 0;
+      if (lg.nl_reset && V.nl) V.nl[1] = 0;
+    }
     int n = R.n;
     double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
@@ -400,6 +423,8 @@ static BatchDev to_dev(const AwBatchPtrs &p)
   B.queue = p.queue;
   B.order = p.order;
   B.prof = p.prof;
+  B.nl = p.nl;
+  B.nlx = p.nlx;
   return B;
 }
 
@@ -410,6 +435,7 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   LgParams lg;
   lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
   lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
+  lg.nl_reset = run.nl_reset;
   int mode = run.mode;
   hipMemsetAsync(p.queue, 0, sizeof(int), st);
   // 512 threads: the whole step fits the 256-VGPR budget without spills;
@@ -459,6 +485,7 @@ static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t s
   LgParams lg;
   lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
   lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
+  lg.nl_reset = 0;
   int grid = run.grid > 0 ? run.grid : 1024;
   for (int r = 0; r < p.nrep; ++r) {
     hipLaunchKernelGGL(k_ph_load<T>, dim3(grid), dim3(256), 0, st, B, r);

@@ -18,6 +18,8 @@ struct AwBatchPtrs {
   int *queue;
   const int *order;
   unsigned long long *prof;  // optional per-phase clock accumulators [16]
+  int *nl;                   // Verlet lists of the resident engine
+  double *nlx;
 };
 
 struct AwRun {
@@ -27,6 +29,7 @@ struct AwRun {
   unsigned long long seed;
   long long step0;
   int threads, grid;    // 0 = defaults
+  int nl_reset;         // state replaced since the last launch
 };
 
 int aw_lds_max_residues();

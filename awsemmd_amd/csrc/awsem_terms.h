@@ -9,6 +9,7 @@
 
 #ifdef AW_HOST_EMU
 #include <algorithm>
+#include <string.h>
 #define AW_DEV inline
 #define AW_DEVCONST static const
 struct int2 { int x, y; };
@@ -17,6 +18,10 @@ struct double4 { double x, y, z, w; };
 inline float aw_fast_exp(float a) { return expf(a); }
 inline float aw_fsqrt(float a) { return sqrtf(a); }
 inline void aadd(double *p, double v) { *p += v; }
+inline void amax_i(int *p, int v) { if (v > *p) *p = v; }
+inline int aadd_i(int *p, int v) { int o = *p; *p += v; return o; }
+inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
+inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #else
 #include <hip/hip_runtime.h>
 #define AW_DEV __device__ __forceinline__
@@ -24,6 +29,10 @@ inline void aadd(double *p, double v) { *p += v; }
 __device__ __forceinline__ float aw_fast_exp(float a) { return __expf(a); }
 __device__ __forceinline__ float aw_fsqrt(float a) { return __fsqrt_rn(a); }
 __device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void amax_i(int *p, int v) { atomicMax(p, v); }
+__device__ __forceinline__ int aadd_i(int *p, int v) { return atomicAdd(p, v); }
+__device__ __forceinline__ int f2i(float f) { return __float_as_int(f); }
+__device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
 #endif
 
 #define AW_PAP_DELTA 1e-12
@@ -105,6 +114,11 @@ struct View {
   const T *coef;
   const void *rf;        // T4 per (memory, pair)
   const void *items;     // FMItem / FMItemD per item
+  // Verlet list of residue pairs (resident engine; nl == nullptr: sweep all pairs)
+  int *nl;               // global: [0] count, [1] valid, [2] builds, [3] -, then (i<<16 | j)
+  int nl_cap;
+  double *nlx;           // global: CA positions [n][3] and extents [n] at the last build
+  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
   // f[3][n][3], ten per-residue scalars, ext (float)
   AW_DEV double *x() const { return s; }
@@ -244,7 +258,15 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
     e2 = fmax(e2, rsq_exact(ca, o));
     if (!(ri & RI_FIRST)) { e2 = fmax(e2, rsq_exact(ca, xn)); e2 = fmax(e2, rsq_exact(ca, xh)); }
     if (!(ri & RI_LAST)) e2 = fmax(e2, rsq_exact(ca, xcp));
-    V.ext()[i] = (float)(sqrt(e2) * (1.0 + 1e-6) + 1e-4);
+    float ei = (float)(sqrt(e2) * (1.0 + 1e-6) + 1e-4);
+    V.ext()[i] = ei;
+    if (V.nl) {
+      // CA drift plus growth of the extent since the last list build
+      const double *c0 = V.nlx + 3 * i;
+      double d2 = rsq_exact(ca, c0);
+      float drift = (float)sqrt(d2) * 1.0001f + 1e-4f + fmaxf(0.0f, ei - (float)V.nlx[3 * n + i]);
+      amax_i(&V.nl_state[0], f2i(drift));
+    }
     double *f = V.f();
     for (int k = 0; k < 3; ++k) {
       f[(0 * n + i) * 3 + k] = 0.0;
@@ -531,29 +553,41 @@ AW_DEV void fm_item(const View<T> &V, int item, EAcc &E)
   const T4t *rf = (const T4t *)V.rf + G.rf_off + q;
   const T *coef = V.coef + G.mem_off;
   int np = G.npairs;
-#pragma unroll 4
-  for (int m = 0; m < G.nmem; ++m) {
-    T4t f4 = rf[(long long)m * np];
-    T cf = coef[m];
-    {
-      T dr = r[0] - f4.x;
-      T v = -cf * t_exp(-dr * dr * inv2s);
-      S0 += v * dr; Es += v;
+  // every memory of a batch is loaded before any is used, so that a lane
+  // keeps FMB independent HBM loads in flight
+  const int FMB = sizeof(T) == 4 ? 20 : 8;
+  for (int m0 = 0; m0 < G.nmem; m0 += FMB) {
+    int cnt = G.nmem - m0 < FMB ? G.nmem - m0 : FMB;
+    T4t f4[FMB];
+    T cf[FMB];
+#pragma unroll
+    for (int u = 0; u < FMB; ++u) {
+      if (u < cnt) { f4[u] = rf[(long long)(m0 + u) * np]; cf[u] = coef[m0 + u]; }
     }
-    if (act1 && f4.y >= (T)0) {
-      T dr = r[1] - f4.y;
-      T v = -cf * t_exp(-dr * dr * inv2s);
-      S1 += v * dr; Es += v;
-    }
-    if (act2 && f4.z >= (T)0) {
-      T dr = r[2] - f4.z;
-      T v = -cf * t_exp(-dr * dr * inv2s);
-      S2 += v * dr; Es += v;
-    }
-    if (act3 && f4.w >= (T)0) {
-      T dr = r[3] - f4.w;
-      T v = -cf * t_exp(-dr * dr * inv2s);
-      S3 += v * dr; Es += v;
+#pragma unroll
+    for (int u = 0; u < FMB; ++u) {
+      if (u < cnt) {
+        {
+          T dr = r[0] - f4[u].x;
+          T v = -cf[u] * t_exp(-dr * dr * inv2s);
+          S0 += v * dr; Es += v;
+        }
+        if (act1 && f4[u].y >= (T)0) {
+          T dr = r[1] - f4[u].y;
+          T v = -cf[u] * t_exp(-dr * dr * inv2s);
+          S1 += v * dr; Es += v;
+        }
+        if (act2 && f4[u].z >= (T)0) {
+          T dr = r[2] - f4[u].z;
+          T v = -cf[u] * t_exp(-dr * dr * inv2s);
+          S2 += v * dr; Es += v;
+        }
+        if (act3 && f4[u].w >= (T)0) {
+          T dr = r[3] - f4[u].w;
+          T v = -cf[u] * t_exp(-dr * dr * inv2s);
+          S3 += v * dr; Es += v;
+        }
+      }
     }
   }
   T g = (T)it.gamma, invs = (T)it.invs;
@@ -879,6 +913,77 @@ AW_DEV bool near_pair(const View<T> &V, int i, int j, float reach)
 }
 
 // ------------------------------------------------------------------------
+// Verlet list of residue pairs: every pair whose CA-CA distance is within
+// pair_reach + ext_i + ext_j + skin at the build.  It stays a superset of
+// the pairs any term can select while every residue's CA drift plus extent
+// growth is below skin/2; the passes still apply each term's exact
+// predicate.  Rebuilt on demand (phase_nl_build) by the resident engine.
+
+#define AW_NL_SKIN 2.0f
+
+// true when the list must be rebuilt (uniform; after phase A's barrier)
+template <class T>
+AW_DEV bool nl_needs_build(const View<T> &V)
+{
+  return V.nl && (V.nl[1] == 0 || i2f(V.nl_state[0]) > 0.5f * AW_NL_SKIN);
+}
+
+template <class T>
+AW_DEV void phase_nl_build(const View<T> &V, int t0, int ts)
+{
+  int n = V.n, np = pair_count(n);
+  float reach = (float)V.P->pair_reach + AW_NL_SKIN;
+  for (int k = t0; k < np; k += ts) {
+    int i, j;
+    pair_decode(k, n, i, j);
+    if (near_pair(V, i, j, reach)) {
+      int q = aadd_i(&V.nl_state[1], 1);
+      if (q < V.nl_cap) V.nl[4 + q] = (i << 16) | j;
+    }
+  }
+  for (int i = t0; i < n; i += ts) {
+    const double *ca = XA(V, K_CA, i);
+    V.nlx[3 * i] = ca[0]; V.nlx[3 * i + 1] = ca[1]; V.nlx[3 * i + 2] = ca[2];
+    V.nlx[3 * n + i] = (double)V.ext()[i];
+  }
+}
+
+// after the build's barrier: publish count and validity (one thread)
+template <class T>
+AW_DEV void nl_publish(const View<T> &V, bool built)
+{
+  if (built) {
+    int c = V.nl_state[1];
+    V.nl[0] = c < V.nl_cap ? c : V.nl_cap;
+    V.nl[1] = c <= V.nl_cap ? 1 : 0;
+    V.nl[2] += 1;
+  }
+  V.nl_state[0] = 0;
+  V.nl_state[1] = 0;
+}
+
+// iterate the residue pairs of a pass: the Verlet list when valid, else all
+template <class T, class F>
+AW_DEV void for_pairs(const View<T> &V, int t0, int ts, F &&body)
+{
+  if (V.nl && V.nl[1]) {
+    int cnt = V.nl[0];
+    const int *lst = V.nl + 4;
+    for (int k = t0; k < cnt; k += ts) {
+      int c = lst[k];
+      body(c >> 16, c & 0xffff);
+    }
+  } else {
+    int n = V.n, np = pair_count(n);
+    for (int k = t0; k < np; k += ts) {
+      int i, j;
+      pair_decode(k, n, i, j);
+      body(i, j);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // Phase B
 
 template <class T>
@@ -948,11 +1053,8 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts)
   int np = pair_count(n);
   for (int k = t0; k < nfm; k += ts) fm_item(V, V.fm_item_base + k, E);
   for (int k = t0; k < n; k += ts) residue_local(V, k, E);
-  for (int k = t0; k < np; k += ts) {
-    int i, j;
-    pair_decode(k, n, i, j);
-    pair_B(V, i, j, E);
-  }
+  (void)np;
+  for_pairs(V, t0, ts, [&](int i, int j) { pair_B(V, i, j, E); });
   eflush(E, V.ered);
 }
 
@@ -1066,15 +1168,8 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts)
   const FFParams *P = V.P;
   int n = V.n;
   int nh = P->helix_flag ? n : 0;
-  int np = P->water_flag ? pair_count(n) : 0;
-  for (int k = t0; k < nh + np; k += ts) {
-    if (k < nh) helix_item(V, k, E);
-    else {
-      int i, j;
-      pair_decode(k - nh, n, i, j);
-      xi_pair(V, i, j);
-    }
-  }
+  for (int k = t0; k < nh; k += ts) helix_item(V, k, E);
+  if (P->water_flag) for_pairs(V, t0, ts, [&](int i, int j) { xi_pair(V, i, j); });
   eflush(E, V.ered);
 }
 
@@ -1157,13 +1252,7 @@ AW_DEV void phase_F(const View<T> &V, int t0, int ts)
 {
   EAcc E;
   E.clear();
-  int n = V.n;
-  int np = pair_count(n);
-  for (int k = t0; k < np; k += ts) {
-    int i, j;
-    pair_decode(k, n, i, j);
-    force_pair(V, i, j, E);
-  }
+  for_pairs(V, t0, ts, [&](int i, int j) { force_pair(V, i, j, E); });
   eflush(E, V.ered);
 }
 
@@ -1197,6 +1286,7 @@ struct LgParams {
   unsigned long long seed;
   long long step0;
   int nsteps;
+  int nl_reset;   // coordinates were replaced: rebuild the Verlet lists
 };
 
 // initial_integrate of fix nve

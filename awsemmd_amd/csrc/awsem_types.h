@@ -104,4 +104,7 @@ struct ReplicaDesc {
   int ch_off;     // first chain in the chain tables
   int fm_group_off, fm_ngroups, fm_items, fm_item_base;  // FM item_off values are batch-global
   long long scratch_off;  // doubles, global-memory scratch (when not LDS resident)
+  long long nl_off;       // ints, Verlet list block: 4 header ints + nl_cap pairs
+  long long nlx_off;      // doubles, list reference positions [n][3] + extents [n]
+  int nl_cap, pad2;
 };

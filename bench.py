@@ -180,7 +180,7 @@ def main():
         os.environ["AWSEM_RESIDENT_THREADS"] = str(a.threads)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every replica is independent, so a multi-GPU run "
-              "would only shard replicas across GPUs (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "would only shard replicas across GPUs; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import awsemmd_amd as A
@@ -234,6 +234,8 @@ def main():
     per_step_ms = np.array([1e3 * t / k for t, k in chunks])
     res_total = sum(s.n for s in systems)
     log(f"timed {a.steps} steps: {value:.1f} replica-steps/s")
+    nb, npairs, nbad = batch.list_stats()
+    log(f"Verlet lists: {nb} builds over {a.replicas} replicas, {npairs} pairs now, {nbad} invalid")
 
     if os.environ.get("AWSEM_PHASE_PROF"):
         ticks = batch.phase_clock() / 100.0  # microseconds

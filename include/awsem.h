@@ -181,6 +181,11 @@ int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel
  * [0..7] the first (longest) replica, [8..15] summed over replicas; phases
  * A, B, C, D, E, F, integrate-1, integrate-2.  AWSEM_E_STATE otherwise. */
 int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks16);
+/* Diagnostic: Verlet residue-pair lists of the resident engine summed over
+ * replicas: list builds since creation, pairs in the current lists, and
+ * replicas whose current list is invalid (overflowed or never built). */
+int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *pairs,
+                           int *n_invalid);
 int awsem_batch_last_error(const awsem_batch *b, char *buf, size_t n);
 int awsem_batch_destroy(awsem_batch *b);
 

@@ -90,3 +90,42 @@ def test_lists_bit_exact(oracle_mod, kind, arg, prec):
         g = fix.list(kind, x, arg)
         r = o.list(kind, x, arg)
         assert np.array_equal(g, r), (kind, arg, len(g), len(r))
+
+
+def test_batch_lists_follow_dynamics(oracle_mod):
+    """Resident engine after Langevin dynamics: the Verlet pair lists are
+    rebuilt when residues drift, and forces on the moved coordinates still
+    match the oracle; replacing the state invalidates the lists."""
+    A = _gpu()
+    cases = [make_case([L], 300 + k) for k, L in enumerate([70, 45])] + [make_case([40, 30], 9, compact=True)]
+    x0 = A.pack_states([s.coords() for s, _ in cases])
+    oracles = [oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"]) for _, p in cases]
+    b = A.Batch([A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32) for _, p in cases], A.PREC_FP32)
+    b.set_state(x0, np.zeros_like(x0))
+    b.compute()
+    builds0, pairs0, bad0 = b.list_stats()
+    assert builds0 == len(cases) and bad0 == 0 and pairs0 > 0
+    for rnd in range(2):
+        b.run(400, dt=2.0, t_start=600.0, t_stop=600.0, damp=100.0, seed=11, step0=400 * rnd)
+        b.compute()
+        x, v, f, e = b.get_state()
+        xs, fs = b.split(x), b.split(f)
+        for r in range(len(cases)):
+            fo, eo = oracles[r].compute(xs[r])
+            ftot = np.linalg.norm(fo.reshape(-1, 3), axis=1).max()
+            ferr = term_err(fs[r], fo, ftot)
+            eerr = abs(e[r, 0] - eo[0]) / max(1.0, abs(eo[0]))
+            print(f"round {rnd} replica {r} ferr={ferr:.2e} eerr={eerr:.2e} E={e[r, 0]:.6g}")
+            assert ferr <= TOL[1] and eerr <= TOL[1], (rnd, r)
+    builds, pairs, bad = b.list_stats()
+    print("list builds", builds, "pairs", pairs)
+    assert bad == 0 and builds >= len(cases) + 2
+    # new coordinates: the lists are rebuilt before use
+    b.set_state(x0, np.zeros_like(x0))
+    b.compute()
+    _, _, f, e = b.get_state()
+    for r, (s, _) in enumerate(cases):
+        fo, eo = oracles[r].compute(s.coords())
+        ftot = np.linalg.norm(fo.reshape(-1, 3), axis=1).max()
+        assert term_err(b.split(f)[r], fo, ftot) <= TOL[1]
+    assert b.list_stats()[0] == builds + len(cases)
