@@ -19,7 +19,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libawsem_hip.so")
+# AWSEM_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("AWSEM_LIB") or os.path.join(HERE, "libawsem_hip.so")
 
 ET_NAMES = ["Total", "Chain", "Shake", "Chi", "Rama", "Excluded", "DSSP", "P_AP", "Water", "Burial",
             "Helix", "AMH-Go", "Frag_Mem", "Vec_FM", "Contact_Restraints", "Membrane", "SSB", "Electro",
@@ -293,7 +294,7 @@ class Batch:
 
     def phase_clock(self):
         """Per-phase wall-clock ticks (100 MHz) when run with AWSEM_PHASE_PROF=1."""
-        out = (C.c_uint64 * 16)()
+        out = (C.c_uint64 * 32)()
         self._check(lib().awsem_batch_phase_clock(self.h, out))
         return np.array(out[:], dtype=np.float64)
 

@@ -31,7 +31,7 @@ struct DevBatch {
   std::vector<int> res_off;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  void *d_rep = nullptr, *d_params = nullptr, *d_groups = nullptr, *d_pairs = nullptr;
+  void *d_rep = nullptr, *d_params = nullptr;
   void *d_coef = nullptr, *d_rf = nullptr, *d_items = nullptr;
   int *d_rinfo = nullptr, *d_chs = nullptr, *d_chl = nullptr, *d_queue = nullptr, *d_order = nullptr;
   double *d_aps = nullptr, *d_x = nullptr, *d_v = nullptr, *d_f = nullptr, *d_e = nullptr, *d_scr = nullptr;
@@ -47,7 +47,7 @@ struct DevBatch {
 
   ~DevBatch()
   {
-    void *ptrs[] = {d_rep, d_params, d_groups, d_pairs, d_coef, d_rf, d_items, d_rinfo, d_chs, d_chl,
+    void *ptrs[] = {d_rep, d_params, d_coef, d_rf, d_items, d_rinfo, d_chs, d_chl,
                     d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count, d_prof,
                     d_nl, d_nlx};
     for (void *p : ptrs) if (p) hipFree(p);
@@ -62,7 +62,7 @@ struct DevBatch {
     p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds;
     p.params = d_params; p.rinfo = d_rinfo; p.aps = d_aps; p.ch_start = d_chs; p.ch_len = d_chl;
     p.x = d_x; p.v = d_v; p.f = d_f; p.energy = d_e; p.scratch = d_scr;
-    p.groups = d_groups; p.pairs = d_pairs; p.coef = d_coef; p.rf = d_rf; p.items = d_items;
+    p.coef = d_coef; p.rf = d_rf; p.items = d_items;
     p.queue = d_queue; p.order = d_order; p.prof = d_prof;
     p.nl = d_nl; p.nlx = d_nlx;
     return p;
@@ -101,23 +101,20 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
     ReplicaDesc R;
     std::memset(&R, 0, sizeof R);
     R.n = S.n; R.nch = S.nch; R.res_off = res; R.ch_off = ch;
-    R.fm_group_off = (int)fm.groups.size();
-    int item_base = (int)(fm.items.size() / 4);
+    int item_base = fm.n_items();
     R.fm_item_base = item_base;
     if (S.P.fm_flag) {
-      AwStatus s = aw_build_fm(S, fm, item_base);
+      AwStatus s = aw_build_fm(S, fm);
       if (!s.ok()) return s;
     }
-    R.fm_ngroups = (int)fm.groups.size() - R.fm_group_off;
-    R.fm_items = (int)(fm.items.size() / 4) - item_base;
+    R.fm_items = fm.n_items() - item_base;
     R.scratch_off = scr;
     scr += aw_slab_doubles(S.n);
     // Verlet list capacity: all pairs for small systems, else 128 per residue
     long long allp = (long long)S.n * (S.n - 1) / 2;
     R.nl_cap = (int)std::min<long long>(std::max<long long>(allp, 1), 128LL * S.n);
-    if (S.n >= 32768) R.nl_cap = 0;  // pairs are packed as (i << 16) | j
     R.nl_off = nl_ints;
-    nl_ints += 4 + R.nl_cap;
+    nl_ints += 4 + 5LL * R.nl_cap;  // header, list, then work queues DSSP [2 cap], P_AP [cap], EV [cap]
     R.nlx_off = nlx_dbl;
     nlx_dbl += 4LL * S.n;
     D->reps.push_back(R);
@@ -141,17 +138,17 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   if (!(s = upload(&D->d_aps, aps)).ok()) return s;
   if (!(s = upload(&D->d_chs, chs)).ok()) return s;
   if (!(s = upload(&D->d_chl, chl)).ok()) return s;
-  if (!(s = upload_raw(&D->d_groups, fm.groups.data(), fm.groups.size() * sizeof(FMGroup))).ok()) return s;
-  if (!(s = upload_raw(&D->d_pairs, fm.pairs.data(), fm.pairs.size() * sizeof(int))).ok()) return s;
   if (D->fp32) {
-    std::vector<float> c(fm.coef.begin(), fm.coef.end()), rf(fm.rf.begin(), fm.rf.end()), it(fm.items.begin(), fm.items.end());
+    std::vector<float> c(fm.coef.begin(), fm.coef.end()), rf(fm.rf.begin(), fm.rf.end());
+    std::vector<FMItem> it = aw_fm_items<FMItem>(fm);
     if (!(s = upload_raw(&D->d_coef, c.data(), c.size() * sizeof(float))).ok()) return s;
     if (!(s = upload_raw(&D->d_rf, rf.data(), rf.size() * sizeof(float))).ok()) return s;
-    if (!(s = upload_raw(&D->d_items, it.data(), it.size() * sizeof(float))).ok()) return s;
+    if (!(s = upload_raw(&D->d_items, it.data(), it.size() * sizeof(FMItem))).ok()) return s;
   } else {
     if (!(s = upload_raw(&D->d_coef, fm.coef.data(), fm.coef.size() * sizeof(double))).ok()) return s;
     if (!(s = upload_raw(&D->d_rf, fm.rf.data(), fm.rf.size() * sizeof(double))).ok()) return s;
-    if (!(s = upload_raw(&D->d_items, fm.items.data(), fm.items.size() * sizeof(double))).ok()) return s;
+    std::vector<FMItemD> it = aw_fm_items<FMItemD>(fm);
+    if (!(s = upload_raw(&D->d_items, it.data(), it.size() * sizeof(FMItemD))).ok()) return s;
   }
   size_t st = (size_t)res * 9 * sizeof(double);
   HIPCHK(hipMalloc((void **)&D->d_x, st));
@@ -172,8 +169,8 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMalloc((void **)&D->d_count, 64));
   if (const char *e = std::getenv("AWSEM_PHASE_PROF")) {
     if (std::atoi(e) > 0) {
-      HIPCHK(hipMalloc((void **)&D->d_prof, 16 * sizeof(unsigned long long)));
-      HIPCHK(hipMemset(D->d_prof, 0, 16 * sizeof(unsigned long long)));
+      HIPCHK(hipMalloc((void **)&D->d_prof, 32 * sizeof(unsigned long long)));
+      HIPCHK(hipMemset(D->d_prof, 0, 32 * sizeof(unsigned long long)));
     }
   }
   // longest-first order for the resident engine's work queue
@@ -590,6 +587,13 @@ int awsem_batch_create(int n_ctx, awsem_ctx *const *ctxs, int precision, awsem_b
     v.push_back(&ctxs[k]->sys);
   }
   std::unique_ptr<awsem_batch> b(new awsem_batch());
+  for (const AwSystem *S : v)
+    if (S->n >= 32768) {
+      // residue pairs are packed as (i << 16) | j in the batch engine's lists
+      b->err = "batched systems are limited to 32767 residues each";
+      *out = b.release();
+      return AWSEM_E_ARG;
+    }
   AwStatus s = make_dev_batch(v, precision, true, b->dev);
   if (!s.ok()) { *out = b.release(); (*out)->err = s.msg; return s.code; }
   *out = b.release();
@@ -633,7 +637,6 @@ static int batch_launch(awsem_batch *b, AwRun run)
 {
   DevBatch &D = *b->dev;
   // workgroup size of the resident engine (512 default; 1024 for comparison)
-  if (const char *e = std::getenv("AWSEM_RESIDENT_THREADS")) run.threads = std::atoi(e);
   run.nl_reset = D.nl_dirty ? 1 : 0;
   D.nl_dirty = false;
   hipEventRecord(D.ev0, D.stream);
@@ -676,11 +679,11 @@ int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel
   return AWSEM_OK;
 }
 
-int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks16)
+int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks32)
 {
-  if (!b || !b->dev || !ticks16) return AWSEM_E_ARG;
+  if (!b || !b->dev || !ticks32) return AWSEM_E_ARG;
   if (!b->dev->d_prof) return AWSEM_E_STATE;
-  if (hipMemcpy(ticks16, b->dev->d_prof, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess) return AWSEM_E_DEVICE;
+  if (hipMemcpy(ticks32, b->dev->d_prof, 32 * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess) return AWSEM_E_DEVICE;
   return AWSEM_OK;
 }
 

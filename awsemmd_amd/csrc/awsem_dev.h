@@ -1,0 +1,231 @@
+// awsem_dev.h -- device-side batch description, replica views and the
+// resident engine, shared by the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "awsem_terms.h"
+#include "awsem_launch.h"
+
+// ------------------------------------------------------------------------
+// energy epilogue: ered[N_EACC] (LDS) -> out[slot]
+
+__device__ __forceinline__ void energy_out(const double *ered, double *out, bool atomic)
+{
+  if (threadIdx.x < N_EACC) {
+    double v = ered[threadIdx.x];
+    if (atomic) { if (v != 0.0) atomicAdd(&out[eacc_slot[threadIdx.x]], v); }
+    else out[eacc_slot[threadIdx.x]] = v;
+  }
+}
+
+// ------------------------------------------------------------------------
+// batch description and views
+
+struct BatchDev {
+  const ReplicaDesc *rep;
+  int nrep;
+  const FFParams *P;
+  const int *rinfo;
+  const double *aps;
+  const int *ch_start, *ch_len;
+  double *x, *v, *f;          // state [res_off*9 ...]
+  double *energy;             // [nrep][AW_N_ET]
+  double *scratch;            // global-mode slabs
+  const void *coef;
+  const void *rf;
+  const void *items;
+  int *queue;                 // work-queue counter (resident engine)
+  const int *order;           // replica order (longest first)
+  unsigned long long *prof;   // optional phase clock: [2][16] (first replica, all replicas)
+  int *nl;                    // Verlet lists (resident engine)
+  double *nlx;
+};
+
+// doubles of per-replica working storage (x, xv, f, 10 scalars, ext+pad)
+__host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + (n + 1) / 2 + 8; }
+
+template <class T>
+__device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDesc &R, double *slab)
+{
+  View<T> V;
+  int n = R.n;
+  V.n = n; V.nch = R.nch;
+  V.rinfo = B.rinfo + R.res_off;
+  V.aps = B.aps + (long long)R.res_off * 12;
+  V.ch_start = B.ch_start + R.ch_off;
+  V.ch_len = B.ch_len + R.ch_off;
+  V.s = slab;
+  V.ered = nullptr;
+  V.P = B.P;
+  V.fm_items = R.fm_items;
+  V.fm_item_base = R.fm_item_base;
+  V.coef = (const T *)B.coef;
+  V.rf = B.rf;
+  V.items = B.items;
+  V.nl = nullptr;
+  V.nlx = nullptr;
+  V.nl_cap = 0;
+  V.nl_state = nullptr;
+  V.wq = nullptr;
+  return V;
+}
+
+static inline BatchDev to_dev(const AwBatchPtrs &p)
+{
+  BatchDev B;
+  B.rep = (const ReplicaDesc *)p.rep;
+  B.nrep = p.nrep;
+  B.P = (const FFParams *)p.params;
+  B.rinfo = p.rinfo;
+  B.aps = p.aps;
+  B.ch_start = p.ch_start;
+  B.ch_len = p.ch_len;
+  B.x = p.x; B.v = p.v; B.f = p.f;
+  B.energy = p.energy;
+  B.scratch = p.scratch;
+  B.coef = p.coef;
+  B.rf = p.rf;
+  B.items = p.items;
+  B.queue = p.queue;
+  B.order = p.order;
+  B.prof = p.prof;
+  B.nl = p.nl;
+  B.nlx = p.nlx;
+  return B;
+}
+
+
+// ------------------------------------------------------------------------
+// resident engine: one workgroup per replica, nsteps inside the launch
+
+__device__ __forceinline__ void prof_mark(unsigned long long *prof, bool first, int k, unsigned long long &t)
+{
+  if (prof && threadIdx.x == 0) {
+    unsigned long long now = wall_clock64();
+    atomicAdd(&prof[16 + k], now - t);
+    if (first) atomicAdd(&prof[k], now - t);
+    t = now;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long long *prof, bool first)
+{
+  int t0 = threadIdx.x, ts = blockDim.x;
+  unsigned long long t = prof ? wall_clock64() : 0;
+  if (threadIdx.x < N_EACC) V.ered[threadIdx.x] = 0.0;
+  phase_A(V, t0, ts);
+  __syncthreads();
+  prof_mark(prof, first, 0, t);
+  if (V.nl) {
+    bool build = nl_needs_build(V);
+    if (build) phase_nl_build(V, t0, ts);
+    __syncthreads();
+    if (threadIdx.x == 0) nl_publish(V, build);
+    __syncthreads();
+  }
+  prof_mark(prof, first, 11, t);
+  // phase B in one part, or (clocked run) its three parts separately; one
+  // call site keeps a single copy of the code
+  int nparts = prof ? 3 : 1;
+  for (int q = 0; q < nparts; ++q) {
+    phase_B<T, true>(V, t0, ts, prof ? (1 << q) : 7);
+    __syncthreads();
+    prof_mark(prof, first, prof ? 8 + q : 10, t);
+  }
+  phase_C(V, t0, ts);
+  __syncthreads();
+  prof_mark(prof, first, 2, t);
+  phase_D(V, t0, ts);
+  __syncthreads();
+  prof_mark(prof, first, 3, t);
+  phase_E(V, t0, ts);
+  __syncthreads();
+  prof_mark(prof, first, 4, t);
+  phase_F(V, t0, ts);
+  __syncthreads();
+  prof_mark(prof, first, 5, t);
+}
+
+template <class T, bool LDSM, int NT>
+__global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mode)
+{
+  extern __shared__ double smem[];
+  __shared__ int s_rep;
+  __shared__ double ered[N_EACC];
+  __shared__ int nl_state[8];
+  for (;;) {
+    if (threadIdx.x == 0) s_rep = atomicAdd(B.queue, 1);
+    __syncthreads();
+    int slot = s_rep;
+    __syncthreads();
+    if (slot >= B.nrep) break;
+    int r = B.order[slot];
+    const ReplicaDesc R = B.rep[r];
+    double *slab = LDSM ? smem : (B.scratch + R.scratch_off);
+    View<T> V = make_view<T>(B, R, slab);
+    V.ered = ered;
+    V.nl = R.nl_cap > 0 ? B.nl + R.nl_off : nullptr;
+    V.nlx = B.nlx + R.nlx_off;
+    V.nl_cap = R.nl_cap;
+    V.nl_state = nl_state;
+    V.wq = V.nl ? V.nl + 4 + R.nl_cap : nullptr;
+    if (threadIdx.x < 8) nl_state[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      if (lg.nl_reset && V.nl) V.nl[1] = 0;
+    }
+    int n = R.n;
+    double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
+    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
+    __syncthreads();
+    bool first = slot == 0;
+    // s = -1: the force evaluation before the first step (one call site)
+    int nst = mode == 1 ? lg.nsteps : 0;
+    for (int s = -1; s < nst; ++s) {
+      unsigned long long t = B.prof ? wall_clock64() : 0;
+      if (s >= 0) {
+        integrate1(V, gv, lg, threadIdx.x, blockDim.x);
+        __syncthreads();
+        prof_mark(B.prof, first, 6, t);
+      }
+      force_eval_block(V, B.prof, first);
+      if (s >= 0) {
+        t = B.prof ? wall_clock64() : 0;
+        integrate2(V, gv, lg, s, r, threadIdx.x, blockDim.x);
+        __syncthreads();
+        prof_mark(B.prof, first, 7, t);
+      }
+    }
+    if (mode == 1)
+      for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gx[k] = V.x()[k];
+    for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gf[k] = V.f()[k];
+    energy_out(ered, B.energy + (long long)r * AW_N_ET, false);
+    __syncthreads();
+  }
+}
+
+// total energy slot of every replica (awsem_kernels.hip)
+void aw_energy_total(double *energy, int nrep, hipStream_t st);
+
+template <class T>
+static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
+{
+  BatchDev B = to_dev(p);
+  LgParams lg;
+  lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
+  lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
+  lg.nl_reset = run.nl_reset;
+  int mode = run.mode;
+  (void)hipMemsetAsync(p.queue, 0, sizeof(int), st);
+  // 512 threads: 8 waves per CU with up to 256 VGPRs each
+  int grid = run.grid > 0 ? run.grid : p.nrep;
+  size_t lds = p.lds_resident ? (size_t)slab_doubles(p.max_n) * sizeof(double) : 0;
+  if (p.lds_resident) {
+    (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_resident<T, true, 512>), dim3(grid), dim3(512), lds, st, B, lg, mode);
+  } else {
+    hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, lg, mode);
+  }
+  aw_energy_total(p.energy, p.nrep, st);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -4,6 +4,7 @@
 #pragma once
 #include <map>
 #include <string>
+#include <cstring>
 #include <vector>
 #include "awsem_types.h"
 
@@ -54,11 +55,25 @@ AwStatus aw_read_mems(const std::string &dir, const std::string &mem_file, AwSys
 AwStatus aw_finish_system(AwSystem &S);
 
 struct AwFMPack {
-  std::vector<FMGroup> groups;
-  std::vector<int> pairs;     // (di, dj) per pair
   std::vector<double> coef;   // epsilon * k_frag_mem * weight per memory
-  std::vector<double> rf;     // 4 per (memory, pair)
-  std::vector<double> items;  // gamma, 1/(2 sigma^2), 1/sigma^2, 0 per item
+  std::vector<double> rf;     // 4 per (memory, pair): CA-CA, CA(i)-CB(j), CA(j)-CB(i), CB-CB; -1 = no atom
+  std::vector<double> items;  // gamma, log2(e)/(2 sigma^2), 1/sigma^2 per item
+  std::vector<int> meta;      // ij, rf0, stride, nmem, coef0 per item
+  int n_items() const { return (int)(items.size() / 3); }
 };
-AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B, int item_base);
+// appends one system's FM items (residue indices local to the system)
+AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B);
+template <class ItemT>
+std::vector<ItemT> aw_fm_items(const AwFMPack &B)
+{
+  std::vector<ItemT> out(B.n_items());
+  for (int k = 0; k < B.n_items(); ++k) {
+    ItemT &it = out[k];
+    std::memset(&it, 0, sizeof it);
+    it.gamma = B.items[3 * k]; it.c2 = B.items[3 * k + 1]; it.invs = B.items[3 * k + 2];
+    const int *m = &B.meta[5 * k];
+    it.ij = m[0]; it.rf0 = m[1]; it.stride = m[2]; it.nmem = m[3]; it.coef0 = m[4];
+  }
+  return out;
+}
 void aw_default_pair_styles(FFParams &P);

@@ -14,10 +14,10 @@ struct AwBatchPtrs {
   const double *aps;
   const int *ch_start, *ch_len;
   double *x, *v, *f, *energy, *scratch;
-  const void *groups, *pairs, *coef, *rf, *items;
+  const void *coef, *rf, *items;
   int *queue;
   const int *order;
-  unsigned long long *prof;  // optional per-phase clock accumulators [16]
+  unsigned long long *prof;  // optional per-phase clock accumulators [32]
   int *nl;                   // Verlet lists of the resident engine
   double *nlx;
 };

@@ -585,10 +585,13 @@ AwStatus aw_finish_system(AwSystem &S)
   double reach = dens;
   if (P.dssp_flag) reach = std::fmax(reach, P.dssp_cut);
   if (P.pap_flag) reach = std::fmax(reach, std::sqrt(P.pap_cutoff_sq));
-  for (int a = 0; a < 6; ++a)
-    for (int b = 0; b < 6; ++b)
-      if (P.ev_lambda[a][b] != 0.0) reach = std::fmax(reach, std::fmax(P.ev_cut_short[a][b], P.ev_cut_long[a][b]));
-  P.pair_reach = reach;
+  double evr = 0.0;
+  if (P.ev_on)
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b)
+        if (P.ev_lambda[a][b] != 0.0) evr = std::fmax(evr, std::fmax(P.ev_cut_short[a][b], P.ev_cut_long[a][b]));
+  P.ev_reach = evr;
+  P.pair_reach = std::fmax(reach, evr);
   return AwStatus();
 }
 
@@ -601,13 +604,13 @@ AwStatus aw_finish_system(AwSystem &S)
 // (Fragment_Memory::Rf, fm.cpp:190-202); a CB entry of a glycine memory
 // residue is stored as -1 (skipped, fb.cpp:4148-4149).
 
-AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B, int item_base)
+AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B)
 {
   const FFParams &P = S.P;
   std::map<std::pair<int, int>, std::vector<int>> by;
   for (size_t k = 0; k < S.mems.size(); ++k) by[{S.mems[k].pos, S.mems[k].len}].push_back((int)k);
   int min_sep = S.fmg.min_sep(), max_sep = S.fmg.max_sep();
-  int item = item_base;
+  if (S.n >= 32768) return AwStatus::err(-1, "fragment memory supports at most 32767 residues per system");
   for (auto &kv : by) {
     int pos = kv.first.first, len = kv.first.second;
     // candidate pairs of this window (fb.cpp:4110-4125, frag_mem_map 784-788)
@@ -624,25 +627,33 @@ AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B, int item_base)
       }
     }
     if (prs.empty()) continue;
-    FMGroup G;
-    std::memset(&G, 0, sizeof G);
-    G.pos = pos; G.len = len; G.npairs = (int)prs.size(); G.nmem = (int)kv.second.size();
-    G.pair_off = (int)(B.pairs.size() / 2);
-    G.item_off = item;
-    G.mem_off = (int)B.coef.size();
-    G.rf_off = (long long)(B.rf.size() / 4);
-    for (auto &pr : prs) { B.pairs.push_back(pr.first); B.pairs.push_back(pr.second); }
-    for (auto &pr : prs) {
-      int i = pos + pr.first, j = pos + pr.second;
+    // by separation, then first residue: neighbouring items of a group touch
+    // distinct residues, so their force updates do not collide
+    std::stable_sort(prs.begin(), prs.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+      return a.second - a.first < b.second - b.first;
+    });
+    int np = (int)prs.size(), nmem = (int)kv.second.size();
+    long long rf_base = (long long)(B.rf.size() / 4);
+    if (rf_base + (long long)np * nmem >= (1LL << 31))
+      return AwStatus::err(-8, "fragment memory table exceeds 2^31 entries");
+    if (nmem > AW_FM_NMEM_MASK) return AwStatus::err(-1, "too many memories in one fragment window");
+    int coef0 = (int)B.coef.size();
+    for (int q = 0; q < np; ++q) {
+      int i = pos + prs[q].first, j = pos + prs[q].second;
       double sig2 = std::pow((double)std::abs(i - j), 2 * P.fm_sigma_exp);
       B.items.push_back(S.fmg.get(aw_restype(S.se[i]), aw_restype(S.se[j]), i, j));
-      B.items.push_back(1.0 / (2.0 * sig2));
+      B.items.push_back(1.4426950408889634 / (2.0 * sig2));  // log2(e) / (2 sigma^2)
       B.items.push_back(1.0 / sig2);
-      B.items.push_back(0.0);
+      int flags = (S.se[i] == 'G' ? AW_FM_GLY_I : 0) | (S.se[j] == 'G' ? AW_FM_GLY_J : 0);
+      int meta[5] = {(i << 16) | j, (int)(rf_base + q), np, nmem | flags, coef0};
+      B.meta.insert(B.meta.end(), meta, meta + 5);
     }
+    // weights of the group, zero-padded to a multiple of AW_FM_PAD (the
+    // kernels read them in aligned blocks)
+    for (int mk : kv.second) B.coef.push_back(P.epsilon * P.k_frag_mem * S.mems[mk].weight);
+    while (B.coef.size() % AW_FM_PAD) B.coef.push_back(0.0);
     for (int mk : kv.second) {
       const AwMemory &m = S.mems[mk];
-      B.coef.push_back(P.epsilon * P.k_frag_mem * m.weight);
       auto R = [](const double *a, const double *b) {
         return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
       };
@@ -656,8 +667,6 @@ AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B, int item_base)
         B.rf.push_back((gi || gj) ? -1.0 : R(cbi, cbj));  // rf[0][b][a]: CB-CB
       }
     }
-    item += G.npairs;
-    B.groups.push_back(G);
   }
   return AwStatus();
 }

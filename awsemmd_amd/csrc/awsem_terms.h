@@ -16,23 +16,42 @@ struct int2 { int x, y; };
 struct float4 { float x, y, z, w; };
 struct double4 { double x, y, z, w; };
 inline float aw_fast_exp(float a) { return expf(a); }
+inline float aw_fast_exp2(float a) { return exp2f(a); }
 inline float aw_fsqrt(float a) { return sqrtf(a); }
 inline void aadd(double *p, double v) { *p += v; }
 inline void amax_i(int *p, int v) { if (v > *p) *p = v; }
 inline int aadd_i(int *p, int v) { int o = *p; *p += v; return o; }
 inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
 inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
+#define AW_BARRIER() ((void)0)
+// slot in a work queue for lanes with pred set, -1 otherwise
+inline int q_push(int *counter, bool pred) { return pred ? (*counter)++ : -1; }
 #else
 #include <hip/hip_runtime.h>
 #define AW_DEV __device__ __forceinline__
 #define AW_DEVCONST __device__ __constant__
 __device__ __forceinline__ float aw_fast_exp(float a) { return __expf(a); }
+__device__ __forceinline__ float aw_fast_exp2(float a) { return __builtin_amdgcn_exp2f(a); }
 __device__ __forceinline__ float aw_fsqrt(float a) { return __fsqrt_rn(a); }
 __device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void amax_i(int *p, int v) { atomicMax(p, v); }
 __device__ __forceinline__ int aadd_i(int *p, int v) { return atomicAdd(p, v); }
 __device__ __forceinline__ int f2i(float f) { return __float_as_int(f); }
 __device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
+#define AW_BARRIER() __syncthreads()
+// slot in a work queue (LDS counter) for the active lanes with pred set, -1
+// otherwise: one atomic per wave, slots in lane order
+__device__ __forceinline__ int q_push(int *counter, bool pred)
+{
+  unsigned long long m = __ballot(pred);
+  if (m == 0ull) return -1;
+  int lane = __lane_id();
+  int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
 #endif
 
 #define AW_PAP_DELTA 1e-12
@@ -48,6 +67,8 @@ template <class T> struct V3 { T x, y, z; };
 
 AW_DEV float t_exp(float a) { return aw_fast_exp(a); }
 AW_DEV double t_exp(double a) { return exp(a); }
+AW_DEV float t_exp2(float a) { return aw_fast_exp2(a); }
+AW_DEV double t_exp2(double a) { return exp2(a); }
 AW_DEV float t_sqrt(float a) { return aw_fsqrt(a); }
 AW_DEV double t_sqrt(double a) { return sqrt(a); }
 AW_DEV float t_tanh(float a) { return tanhf(a); }
@@ -108,9 +129,7 @@ struct View {
   double *ered;         // energy accumulators [N_EACC] (LDS)
   const FFParams *P;
   // fragment memory
-  const FMGroup *groups;
-  int ngroups, fm_items, fm_item_base;
-  const int2 *pairs;
+  int fm_items, fm_item_base;
   const T *coef;
   const void *rf;        // T4 per (memory, pair)
   const void *items;     // FMItem / FMItemD per item
@@ -118,7 +137,8 @@ struct View {
   int *nl;               // global: [0] count, [1] valid, [2] builds, [3] -, then (i<<16 | j)
   int nl_cap;
   double *nlx;           // global: CA positions [n][3] and extents [n] at the last build
-  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter
+  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..4] queue counts
+  int *wq;               // global work queues beside a valid list: DSSP [2 cap], P_AP [cap], EV [cap]
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
   // f[3][n][3], ten per-residue scalars, ext (float)
   AW_DEV double *x() const { return s; }
@@ -518,28 +538,40 @@ AW_DEV void bonds_term(const View<T> &V, int i, int ri, EAcc &E)
 
 // ------------------------------------------------------------------------
 // fragment memory: one item = one (i, j) pair of one memory group, looping
-// over the group's memories (fb.cpp:4079-4177, Rf fm.cpp:190-202)
+// over the group's memories (fb.cpp:4079-4177, Rf fm.cpp:190-202).  Pairs
+// are grouped by window because all memories of a window share the target
+// distances r_ij; the Rf of memory m sits at rf0 + m * stride.
 
 template <class T> struct T4 {};
 template <> struct T4<float> { typedef float4 type; typedef FMItem item; };
 template <> struct T4<double> { typedef double4 type; typedef FMItemD item; };
 
+// one memory's four Gaussian wells of one pair: rf = (CA-CA, CA(i)-CB(j),
+// CA(j)-CB(i), CB-CB), -1 where the memory lacks the atom
+template <class T, class T4t>
+AW_DEV void fm_gauss(const T4t &f, T cf, const T *r, T c2, bool act1, bool act2, bool act3,
+                     T &S0, T &S1, T &S2, T &S3, T &Es)
+{
+  T w = -cf;
+  T w1 = (act1 && f.y >= (T)0) ? w : (T)0;
+  T w2 = (act2 && f.z >= (T)0) ? w : (T)0;
+  T w3 = (act3 && f.w >= (T)0) ? w : (T)0;
+  T d0 = r[0] - f.x, d1 = r[1] - f.y, d2 = r[2] - f.z, d3 = r[3] - f.w;
+  T v0 = w * t_exp2(-(d0 * d0) * c2);
+  T v1 = w1 * t_exp2(-(d1 * d1) * c2);
+  T v2 = w2 * t_exp2(-(d2 * d2) * c2);
+  T v3 = w3 * t_exp2(-(d3 * d3) * c2);
+  S0 += v0 * d0; S1 += v1 * d1; S2 += v2 * d2; S3 += v3 * d3;
+  Es += (v0 + v1) + (v2 + v3);
+}
+
 template <class T>
-AW_DEV void fm_item(const View<T> &V, int item, EAcc &E)
+AW_DEV void fm_item(const View<T> &V, const typename T4<T>::item &it, EAcc &E)
 {
   typedef typename T4<T>::type T4t;
-  typedef typename T4<T>::item ItemT;
-  int lo = 0, hi = V.ngroups - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (V.groups[mid].item_off <= item) lo = mid; else hi = mid - 1;
-  }
-  const FMGroup G = V.groups[lo];
-  int q = item - G.item_off;
-  int2 pr = V.pairs[G.pair_off + q];
-  int i = G.pos + pr.x, j = G.pos + pr.y;
-  const ItemT it = ((const ItemT *)V.items)[item];
-  bool gi = V.rinfo[i] & RI_GLY, gj = V.rinfo[j] & RI_GLY;
+  int i = it.ij >> 16, j = it.ij & 0xffff;
+  int nmem = it.nmem & AW_FM_NMEM_MASK;
+  bool gi = it.nmem & AW_FM_GLY_I, gj = it.nmem & AW_FM_GLY_J;
   V3<T> d[4];
   d[0] = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j));
   d[1] = dvec<T>(XA(V, K_CA, i), XA(V, K_CB, j));
@@ -548,47 +580,42 @@ AW_DEV void fm_item(const View<T> &V, int item, EAcc &E)
   T r[4];
   for (int k = 0; k < 4; ++k) r[k] = t_sqrt(dot(d[k], d[k]));
   bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
-  T inv2s = (T)it.inv2s;
+  T c2 = (T)it.c2;
   T S0 = 0, S1 = 0, S2 = 0, S3 = 0, Es = 0;
-  const T4t *rf = (const T4t *)V.rf + G.rf_off + q;
-  const T *coef = V.coef + G.mem_off;
-  int np = G.npairs;
-  // every memory of a batch is loaded before any is used, so that a lane
-  // keeps FMB independent HBM loads in flight
-  const int FMB = sizeof(T) == 4 ? 20 : 8;
-  for (int m0 = 0; m0 < G.nmem; m0 += FMB) {
-    int cnt = G.nmem - m0 < FMB ? G.nmem - m0 : FMB;
+  const T4t *rf = (const T4t *)V.rf + it.rf0;
+  const T4t *cf4 = (const T4t *)(V.coef + it.coef0);  // zero-padded, 4-aligned
+  int np = it.stride;
+  // memories in batches: the batch's weights, then its Rf entries, are all
+  // loaded before any is used.  Full batches of FMB step a pointer; the tail
+  // runs in batches of 4 where memories past nmem re-read the last one and
+  // carry weight 0 (no branches on the data)
+  const int FMB = sizeof(T) == 4 ? 16 : 8;
+  int m0 = 0;
+  const T4t *p = rf;
+  for (; m0 + FMB <= nmem; m0 += FMB) {
     T4t f4[FMB];
     T cf[FMB];
 #pragma unroll
-    for (int u = 0; u < FMB; ++u) {
-      if (u < cnt) { f4[u] = rf[(long long)(m0 + u) * np]; cf[u] = coef[m0 + u]; }
+    for (int u = 0; u < FMB / 4; ++u) {
+      T4t c = cf4[(m0 >> 2) + u];
+      cf[4 * u] = c.x; cf[4 * u + 1] = c.y; cf[4 * u + 2] = c.z; cf[4 * u + 3] = c.w;
     }
 #pragma unroll
-    for (int u = 0; u < FMB; ++u) {
-      if (u < cnt) {
-        {
-          T dr = r[0] - f4[u].x;
-          T v = -cf[u] * t_exp(-dr * dr * inv2s);
-          S0 += v * dr; Es += v;
-        }
-        if (act1 && f4[u].y >= (T)0) {
-          T dr = r[1] - f4[u].y;
-          T v = -cf[u] * t_exp(-dr * dr * inv2s);
-          S1 += v * dr; Es += v;
-        }
-        if (act2 && f4[u].z >= (T)0) {
-          T dr = r[2] - f4[u].z;
-          T v = -cf[u] * t_exp(-dr * dr * inv2s);
-          S2 += v * dr; Es += v;
-        }
-        if (act3 && f4[u].w >= (T)0) {
-          T dr = r[3] - f4[u].w;
-          T v = -cf[u] * t_exp(-dr * dr * inv2s);
-          S3 += v * dr; Es += v;
-        }
-      }
+    for (int u = 0; u < FMB; ++u) { f4[u] = *p; p += np; }
+#pragma unroll
+    for (int u = 0; u < FMB; ++u) fm_gauss(f4[u], cf[u], r, c2, act1, act2, act3, S0, S1, S2, S3, Es);
+  }
+  for (; m0 < nmem; m0 += 4) {
+    T4t f4[4];
+    T4t c = cf4[m0 >> 2];
+    T cf[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int m = m0 + u < nmem ? m0 + u : nmem - 1;
+      f4[u] = rf[m * np];
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) fm_gauss(f4[u], cf[u], r, c2, act1, act2, act3, S0, S1, S2, S3, Es);
   }
   T g = (T)it.gamma, invs = (T)it.invs;
   E.fm += (double)(g * Es);
@@ -960,27 +987,34 @@ AW_DEV void nl_publish(const View<T> &V, bool built)
   }
   V.nl_state[0] = 0;
   V.nl_state[1] = 0;
+  V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
 }
 
 // iterate the residue pairs of a pass: the Verlet list when valid, else all
+// pairs; pairs [k0, k1) of that sequence (one call site of the body)
+template <class T>
+AW_DEV int pair_total(const View<T> &V)
+{
+  return (V.nl && V.nl[1]) ? V.nl[0] : pair_count(V.n);
+}
+
+template <class T, class F>
+AW_DEV void for_pairs_range(const View<T> &V, int k0, int k1, int t0, int ts, F &&body)
+{
+  const int *lst = (V.nl && V.nl[1]) ? V.nl + 4 : nullptr;
+  int n = V.n;
+  for (int k = k0 + t0; k < k1; k += ts) {
+    int i, j;
+    if (lst) { int c = lst[k]; i = c >> 16; j = c & 0xffff; }
+    else pair_decode(k, n, i, j);
+    body(i, j);
+  }
+}
+
 template <class T, class F>
 AW_DEV void for_pairs(const View<T> &V, int t0, int ts, F &&body)
 {
-  if (V.nl && V.nl[1]) {
-    int cnt = V.nl[0];
-    const int *lst = V.nl + 4;
-    for (int k = t0; k < cnt; k += ts) {
-      int c = lst[k];
-      body(c >> 16, c & 0xffff);
-    }
-  } else {
-    int n = V.n, np = pair_count(n);
-    for (int k = t0; k < np; k += ts) {
-      int i, j;
-      pair_decode(k, n, i, j);
-      body(i, j);
-    }
-  }
+  for_pairs_range(V, 0, pair_total(V), t0, ts, body);
 }
 
 // ------------------------------------------------------------------------
@@ -1000,8 +1034,12 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
   if (P->ev_on && ON(P, AW_ET_VEXCLUDED)) ev_self(V, i, ri, E);
 }
 
+// Residue-pair pass of phase B.  With queues (qd/qp/qe non-null) the
+// DSSP (directed, acceptor << 16 | donor), P_AP and excluded-volume pairs
+// that pass their cutoffs are queued and evaluated afterwards in dense
+// passes; otherwise they are evaluated in place.
 template <class T>
-AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E)
+AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr)
 {
   const FFParams *P = V.P;
   if (!near_pair(V, i, j, (float)P->pair_reach)) return;
@@ -1025,36 +1063,88 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E)
     }
   }
   // DSSP in both orientations (fb.cpp:8064-8094)
-  if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2)) {
+  bool dij = false, dji = false;
+  if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2) && near_pair(V, i, j, (float)P->dssp_cut)) {
     if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST)) {
       double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
-      if (r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut)) dssp_term(V, i, j, E);
+      dij = r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut);
     }
     if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST)) {
       double r2 = rsq_exact(XA(V, K_O, j), XV(V, K_N, i));
-      if (r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut)) dssp_term(V, j, i, E);
+      dji = r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut);
     }
   }
   // P_AP (fb.cpp:8096-8103)
-  if (P->pap_flag && ON(P, AW_ET_PAP)) {
-    if (rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j)) < P->pap_cutoff_sq) pap_term(V, i, j, E);
+  bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j)) < P->pap_cutoff_sq;
+  bool ev = P->ev_on && ON(P, AW_ET_VEXCLUDED) && near_pair(V, i, j, (float)P->ev_reach);
+  if (qd) {
+    int c = (i << 16) | j;
+    int s1 = q_push(&V.nl_state[2], dij);
+    if (s1 >= 0) qd[s1] = c;
+    int s2 = q_push(&V.nl_state[2], dji);
+    if (s2 >= 0) qd[s2] = (j << 16) | i;
+    int s3 = q_push(&V.nl_state[3], pap);
+    if (s3 >= 0) qp[s3] = c;
+    int s4 = q_push(&V.nl_state[4], ev);
+    if (s4 >= 0) qe[s4] = c;
+  } else {
+    if (dij) dssp_term(V, i, j, E);
+    if (dji) dssp_term(V, j, i, E);
+    if (pap) pap_term(V, i, j, E);
+    if (ev) ev_pair(V, i, j, ri, rj, E);
   }
-  if (P->ev_on && ON(P, AW_ET_VEXCLUDED)) ev_pair(V, i, j, ri, rj, E);
 }
 
-template <class T>
-AW_DEV void phase_B(const View<T> &V, int t0, int ts)
+// part: bit 0 fragment memory, bit 1 local terms, bit 2 residue pairs.
+// QUEUED (resident engine, one workgroup per replica): the pair sweep runs
+// in chunks of at most nl_cap pairs; each chunk is scanned into the DSSP,
+// P_AP and excluded-volume work queues, which are then evaluated densely.
+// Unqueued (grid-wide phase engine): every term is evaluated in place.
+template <class T, bool QUEUED>
+AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part)
 {
   EAcc E;
   E.clear();
   const FFParams *P = V.P;
-  int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM)) ? V.fm_items : 0;
-  int n = V.n;
-  int np = pair_count(n);
-  for (int k = t0; k < nfm; k += ts) fm_item(V, V.fm_item_base + k, E);
-  for (int k = t0; k < n; k += ts) residue_local(V, k, E);
-  (void)np;
-  for_pairs(V, t0, ts, [&](int i, int j) { pair_B(V, i, j, E); });
+  int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM) && (part & 1)) ? V.fm_items : 0;
+  if (t0 < nfm) {
+    // the next item's descriptor is loaded while the current one computes
+    typedef typename T4<T>::item ItemT;
+    const ItemT *items = (const ItemT *)V.items + V.fm_item_base;
+    ItemT cur = items[t0];
+    for (int k = t0; k < nfm; k += ts) {
+      ItemT nxt = cur;
+      if (k + ts < nfm) nxt = items[k + ts];
+      fm_item(V, cur, E);
+      cur = nxt;
+    }
+  }
+  if (part & 2)
+    for (int k = t0; k < V.n; k += ts) residue_local(V, k, E);
+  if (part & 4) {
+    if (QUEUED) {
+      int cap = V.nl_cap;
+      int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
+      int total = pair_total(V);
+      for (int c0 = 0; c0 < total && cap > 0; c0 += cap) {
+        int c1 = total - c0 < cap ? total : c0 + cap;
+        for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe); });
+        AW_BARRIER();
+        int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
+        for (int k = t0; k < nd; k += ts) { int c = qd[k]; dssp_term(V, c >> 16, c & 0xffff, E); }
+        for (int k = t0; k < npap; k += ts) { int c = qp[k]; pap_term(V, c >> 16, c & 0xffff, E); }
+        for (int k = t0; k < nev; k += ts) {
+          int c = qe[k], i = c >> 16, j = c & 0xffff;
+          ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E);
+        }
+        AW_BARRIER();
+        if (t0 == 0) V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
+        AW_BARRIER();
+      }
+    } else {
+      for_pairs(V, t0, ts, [&](int i, int j) { pair_B(V, i, j, E); });
+    }
+  }
   eflush(E, V.ered);
 }
 

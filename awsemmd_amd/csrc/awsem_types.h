@@ -74,27 +74,31 @@ struct FFParams {
   double pair_list_cutoff;   // src/fix_backbone.cpp:1048-1066
   double pair_reach;         // largest pair cutoff of any term (density, DSSP, P_AP, EV)
   double dens_reach;         // largest density-well outer cutoff (water/helix)
+  double ev_reach;           // largest excluded-volume cutoff
   uint32_t term_mask;
 };
 
-// One group of fragment memories sharing (pos, len): every memory of the
-// group covers the same (i, j) residue pairs, so one work item = one pair
-// of one group, looping over the group's memories.
-struct FMGroup {
-  int pos, len, npairs, nmem;
-  int pair_off;   // into the (di, dj) pair table
-  int item_off;   // first work item of this group (batch-global item index)
-  int mem_off;    // first memory (global index) of this group
-  int pad;
-  long long rf_off;  // first (memory, pair) rf entry
-};
-
-// per-item constants: gamma and the Gaussian widths 1/(2 sigma^2), 1/sigma^2
+// Fragment memory work item: one (i, j) residue pair of one group of
+// memories sharing a window (pos, len); every memory of the group covers
+// the same pairs.  An item carries its residues, the index of its Rf entry
+// for the group's first memory (float4/double4 units; the entry of memory m
+// is rf0 + m * stride) and the group's memory weights (coef0: start of a
+// zero-padded run of AW_FM_PAD weights or more), so a work item needs one
+// dependent load before its Rf loads.  The Gaussian width enters as
+// c2 = log2(e) / (2 sigma^2): exp(-dr^2 / (2 sigma^2)) = exp2(-dr^2 c2).
+#define AW_FM_PAD 16
+#define AW_FM_GLY_I (1 << 30)  // flags in nmem: residue i / j is glycine
+#define AW_FM_GLY_J (1 << 29)
+#define AW_FM_NMEM_MASK ((1 << 29) - 1)
 struct FMItem {
-  float gamma, inv2s, invs, pad;
+  float gamma, c2, invs;     // gamma, log2(e)/(2 sigma^2), 1/sigma^2
+  int ij;                    // (i << 16) | j, replica-local residue indices
+  int rf0, stride, nmem, coef0;
 };
 struct FMItemD {
-  double gamma, inv2s, invs, pad;
+  double gamma, c2, invs;
+  int ij;
+  int rf0, stride, nmem, coef0, pad;
 };
 
 struct ReplicaDesc {
@@ -102,7 +106,7 @@ struct ReplicaDesc {
   int nch;        // chains
   int res_off;    // first residue in the concatenated per-residue arrays
   int ch_off;     // first chain in the chain tables
-  int fm_group_off, fm_ngroups, fm_items, fm_item_base;  // FM item_off values are batch-global
+  int fm_items, fm_item_base;  // this replica's FM items: [fm_item_base, fm_item_base + fm_items)
   long long scratch_off;  // doubles, global-memory scratch (when not LDS resident)
   long long nl_off;       // ints, Verlet list block: 4 header ints + nl_cap pairs
   long long nlx_off;      // doubles, list reference positions [n][3] + extents [n]

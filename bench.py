@@ -172,12 +172,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--threads", type=int, default=0, help="resident workgroup size (512 or 1024)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (HBM counter calibration)")
     a = ap.parse_args()
-    if a.threads:
-        os.environ["AWSEM_RESIDENT_THREADS"] = str(a.threads)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every replica is independent, so a multi-GPU run "
               "would only shard replicas across GPUs; run with --gpus 1", file=sys.stderr)
@@ -239,10 +236,10 @@ def main():
 
     if os.environ.get("AWSEM_PHASE_PROF"):
         ticks = batch.phase_clock() / 100.0  # microseconds
-        names = ["A", "B", "C", "D", "E", "F", "int1", "int2"]
-        nsteps_all = a.warmup + a.steps + 1
-        log("phase us/step, longest replica: " + " ".join(f"{n}={t / nsteps_all:.1f}" for n, t in zip(names, ticks[:8])))
-        log("phase us/step, mean replica:    " + " ".join(f"{n}={t / nsteps_all / a.replicas:.1f}" for n, t in zip(names, ticks[8:])))
+        names = ["A", "B", "C", "D", "E", "F", "int1", "int2", "B_fm", "B_local", "B_pair", "list"]
+        nsteps_all = (a.warmup + 1) + a.steps + (a.steps + a.chunk - 1) // a.chunk  # force evaluations
+        log("phase us/eval, longest replica: " + " ".join(f"{n}={t / nsteps_all:.1f}" for n, t in zip(names, ticks[:12])))
+        log("phase us/eval, mean replica:    " + " ".join(f"{n}={t / nsteps_all / a.replicas:.1f}" for n, t in zip(names, ticks[16:28])))
     # force error of the batched FP32 path against the FP64 path (sample)
     batch.compute()
     xs, vs, fs, es = batch.get_state()

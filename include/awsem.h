@@ -178,9 +178,11 @@ int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel
                        int *n_kernels);
 /* Diagnostic: wall-clock ticks (100 MHz) accumulated per phase of the
  * resident engine when the process runs with AWSEM_PHASE_PROF=1:
- * [0..7] the first (longest) replica, [8..15] summed over replicas; phases
- * A, B, C, D, E, F, integrate-1, integrate-2.  AWSEM_E_STATE otherwise. */
-int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks16);
+ * [0..15] the first (longest) replica, [16..31] summed over replicas; slots
+ * A, B (remainder), C, D, E, F, integrate-1, integrate-2, B fragment memory,
+ * B local terms, B residue pairs, pair-list check/build, 4 unused.
+ * AWSEM_E_STATE otherwise. */
+int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks32);
 /* Diagnostic: Verlet residue-pair lists of the resident engine summed over
  * replicas: list builds since creation, pairs in the current lists, and
  * replicas whose current list is invalid (overflowed or never built). */
