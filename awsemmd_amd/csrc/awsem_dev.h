@@ -42,7 +42,7 @@ struct BatchDev {
 };
 
 // doubles of per-replica working storage (x, xv, f, 10 scalars, ext+pad)
-__host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + (n + 1) / 2 + 8; }
+__host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + 2LL * ((n + 1) / 2) + 8; }
 
 template <class T>
 __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDesc &R, double *slab)
@@ -148,7 +148,7 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
 }
 
 template <class T, bool LDSM, int NT>
-__global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mode)
+__global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__restrict__ Pr, LgParams lg, int mode)
 {
   extern __shared__ double smem[];
   __shared__ int s_rep;
@@ -164,6 +164,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mo
     const ReplicaDesc R = B.rep[r];
     double *slab = LDSM ? smem : (B.scratch + R.scratch_off);
     View<T> V = make_view<T>(B, R, slab);
+    V.P = Pr;  // read-only, unaliased: uniform parameter loads go through the scalar cache
     V.ered = ered;
     V.nl = R.nl_cap > 0 ? B.nl + R.nl_off : nullptr;
     V.nlx = B.nlx + R.nlx_off;
@@ -177,7 +178,10 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, LgParams lg, int mo
     int n = R.n;
     double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
+    int *ri = V.rinfo_copy();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) ri[k] = V.rinfo[k];
     __syncthreads();
+    V.rinfo = ri;
     bool first = slot == 0;
     // s = -1: the force evaluation before the first step (one call site)
     int nst = mode == 1 ? lg.nsteps : 0;
@@ -222,9 +226,9 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   size_t lds = p.lds_resident ? (size_t)slab_doubles(p.max_n) * sizeof(double) : 0;
   if (p.lds_resident) {
     (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_resident<T, true, 512>), dim3(grid), dim3(512), lds, st, B, lg, mode);
+    hipLaunchKernelGGL((k_resident<T, true, 512>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
   } else {
-    hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, lg, mode);
+    hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   }
   aw_energy_total(p.energy, p.nrep, st);
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -89,6 +89,25 @@ AW_DEV double rsq_exact(const double *a, const double *b)
   return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
+// FP32 estimate of |a - b|^2 (relative error below 1e-6): predicates on
+// rsq_exact are decided with it unless it lies within AW_RSQ_MARGIN of the
+// threshold, where the exact FP64 value decides
+#define AW_RSQ_MARGIN 1e-5f
+AW_DEV float rsq_f(const double *a, const double *b)
+{
+  float d0 = (float)(a[0] - b[0]), d1 = (float)(a[1] - b[1]), d2 = (float)(a[2] - b[2]);
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// exact (rsq_exact(a, b) < t2) for t2 > 0
+AW_DEV bool rsq_lt(const double *a, const double *b, double t2)
+{
+  float e = rsq_f(a, b), tf = (float)t2;
+  if (e < tf * (1.0f - AW_RSQ_MARGIN)) return true;
+  if (e > tf * (1.0f + AW_RSQ_MARGIN)) return false;
+  return rsq_exact(a, b) < t2;
+}
+
 AW_DEV double lin3_exact(double a, double xa, double b, double xb, double c, double xc)
 {
 #pragma clang fp contract(off)
@@ -156,6 +175,8 @@ struct View {
   AW_DEV double *hxi1() const { return sc(8); }
   AW_DEV double *hxi2() const { return sc(9); }
   AW_DEV float *ext() const { return (float *)sc(10); }
+  // resident engine: a copy of rinfo after the extents
+  AW_DEV int *rinfo_copy() const { return (int *)(sc(10) + (n + 1) / 2); }
 };
 
 #define XA(V, kind, i) ((V).x() + ((kind) * (V).n + (i)) * 3)
@@ -1042,41 +1063,56 @@ template <class T>
 AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr)
 {
   const FFParams *P = V.P;
-  if (!near_pair(V, i, j, (float)P->pair_reach)) return;
+  // CA-CA distance (FP32) against each term's reach (the near_pair test)
+  float rca = rsq_f(XA(V, K_CA, i), XA(V, K_CA, j));
+  float eij = V.ext()[i] + V.ext()[j] + 1e-3f;
+  float lim = (float)P->pair_reach + eij;
+  if (!(rca <= lim * lim * 1.0001f)) return;
   int ri = V.rinfo[i], rj = V.rinfo[j];
   bool same = rchain(ri) == rchain(rj);
   int dres = j - i;
-  // DL1 densities (fb.cpp:7729-7786)
+  // DL1 densities (fb.cpp:7729-7786); the exact FP64 distance only for
+  // pairs near a well
   if (!same || dres > 1) {
-    double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
-    if (P->water_flag && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
-      T th, prd;
-      wtheta<T>(P, 0, sqrt(rsq), th, prd);
-      aadd(&V.rho_w()[i], (double)th);
-      aadd(&V.rho_w()[j], (double)th);
-    }
-    if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
-      T th, prd;
-      htheta<T>(P, sqrt(rsq), th, prd);
-      aadd(&V.rho_h()[i], (double)th);
-      aadd(&V.rho_h()[j], (double)th);
+    const double *si = site(V, i, ri), *sj = site(V, j, rj);
+    float e = rsq_f(si, sj);
+    bool wc = P->water_flag && e > (float)P->w_rmin_th_sq[0] * (1.0f - AW_RSQ_MARGIN) &&
+              e < (float)P->w_rmax_th_sq[0] * (1.0f + AW_RSQ_MARGIN);
+    bool hc = P->helix_flag && e > (float)P->h_rmin_th_sq * (1.0f - AW_RSQ_MARGIN) &&
+              e < (float)P->h_rmax_th_sq * (1.0f + AW_RSQ_MARGIN);
+    if (wc || hc) {
+      double rsq = rsq_exact(si, sj);
+      if (wc && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
+        T th, prd;
+        wtheta<T>(P, 0, sqrt(rsq), th, prd);
+        aadd(&V.rho_w()[i], (double)th);
+        aadd(&V.rho_w()[j], (double)th);
+      }
+      if (hc && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
+        T th, prd;
+        htheta<T>(P, sqrt(rsq), th, prd);
+        aadd(&V.rho_h()[i], (double)th);
+        aadd(&V.rho_h()[j], (double)th);
+      }
     }
   }
   // DSSP in both orientations (fb.cpp:8064-8094)
   bool dij = false, dji = false;
-  if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2) && near_pair(V, i, j, (float)P->dssp_cut)) {
-    if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST)) {
+  float ld = (float)P->dssp_cut + eij;
+  if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
+    if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST) && rsq_lt(XA(V, K_O, i), XV(V, K_N, j), P->dssp_cut_sq)) {
       double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
-      dij = r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut);
+      dij = !(sqrt(r2) > P->dssp_cut);
     }
-    if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST)) {
+    if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST) && rsq_lt(XA(V, K_O, j), XV(V, K_N, i), P->dssp_cut_sq)) {
       double r2 = rsq_exact(XA(V, K_O, j), XV(V, K_N, i));
-      dji = r2 < P->dssp_cut_sq && !(sqrt(r2) > P->dssp_cut);
+      dji = !(sqrt(r2) > P->dssp_cut);
     }
   }
   // P_AP (fb.cpp:8096-8103)
-  bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j)) < P->pap_cutoff_sq;
-  bool ev = P->ev_on && ON(P, AW_ET_VEXCLUDED) && near_pair(V, i, j, (float)P->ev_reach);
+  bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_lt(XA(V, K_CA, i), XA(V, K_CA, j), P->pap_cutoff_sq);
+  float le = (float)P->ev_reach + eij;
+  bool ev = P->ev_on && ON(P, AW_ET_VEXCLUDED) && rca <= le * le * 1.0001f;
   if (qd) {
     int c = (i << 16) | j;
     int s1 = q_push(&V.nl_state[2], dij);
