@@ -41,6 +41,7 @@ struct DevBatch {
   int *d_nl = nullptr;                   // Verlet lists (resident engine)
   double *d_nlx = nullptr;
   bool nl_dirty = true;
+  bool f_valid = false;  // d_f holds the forces of d_x (after a completed launch)
   int list_cap = 0;
   FFParams P;
   double last_ms = 0.0;
@@ -614,7 +615,7 @@ int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v)
   DevBatch &D = *b->dev;
   size_t bytes = (size_t)D.total_res * 9 * sizeof(double);
   if (x && hipMemcpy(D.d_x, x, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy x"));
-  if (x) D.nl_dirty = true;
+  if (x) { D.nl_dirty = true; D.f_valid = false; }
   if (v && hipMemcpy(D.d_v, v, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy v"));
   return AWSEM_OK;
 }
@@ -636,14 +637,16 @@ int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, doubl
 static int batch_launch(awsem_batch *b, AwRun run)
 {
   DevBatch &D = *b->dev;
-  // workgroup size of the resident engine (512 default; 1024 for comparison)
   run.nl_reset = D.nl_dirty ? 1 : 0;
+  run.f_valid = D.f_valid ? 1 : 0;
   D.nl_dirty = false;
-  hipEventRecord(D.ev0, D.stream);
+  D.f_valid = false;
+  (void)hipEventRecord(D.ev0, D.stream);
   if (aw_launch_resident(D.ptrs(), run, D.stream)) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
-  hipEventRecord(D.ev1, D.stream);
+  (void)hipEventRecord(D.ev1, D.stream);
   hipError_t e = hipEventSynchronize(D.ev1);
   if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));
+  D.f_valid = true;
   float ms = 0.f;
   hipEventElapsedTime(&ms, D.ev0, D.ev1);
   b->ms_total = ms;

@@ -180,12 +180,17 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
     int *ri = V.rinfo_copy();
     for (int k = threadIdx.x; k < n; k += blockDim.x) ri[k] = V.rinfo[k];
+    // dynamics continuing from the previous launch: its final forces are
+    // still valid, so the first step integrates with them directly
+    bool have_f = mode == 1 && lg.f_valid && lg.nsteps > 0;
+    if (have_f)
+      for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.f()[k] = gf[k];
     __syncthreads();
     V.rinfo = ri;
     bool first = slot == 0;
     // s = -1: the force evaluation before the first step (one call site)
     int nst = mode == 1 ? lg.nsteps : 0;
-    for (int s = -1; s < nst; ++s) {
+    for (int s = have_f ? 0 : -1; s < nst; ++s) {
       unsigned long long t = B.prof ? wall_clock64() : 0;
       if (s >= 0) {
         integrate1(V, gv, lg, threadIdx.x, blockDim.x);
@@ -219,6 +224,7 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
   lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
   lg.nl_reset = run.nl_reset;
+  lg.f_valid = run.f_valid;
   int mode = run.mode;
   (void)hipMemsetAsync(p.queue, 0, sizeof(int), st);
   // 512 threads: 8 waves per CU with up to 256 VGPRs each

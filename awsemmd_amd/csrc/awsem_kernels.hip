@@ -254,6 +254,7 @@ static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t s
   lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
   lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
   lg.nl_reset = 0;
+  lg.f_valid = 0;
   int grid = run.grid > 0 ? run.grid : 1024;
   for (int r = 0; r < p.nrep; ++r) {
     hipLaunchKernelGGL(k_ph_load<T>, dim3(grid), dim3(256), 0, st, B, r);

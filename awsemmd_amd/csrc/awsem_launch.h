@@ -30,6 +30,7 @@ struct AwRun {
   long long step0;
   int threads, grid;    // 0 = defaults
   int nl_reset;         // state replaced since the last launch
+  int f_valid;          // device forces belong to the current coordinates
 };
 
 int aw_lds_max_residues();

@@ -1413,6 +1413,7 @@ struct LgParams {
   long long step0;
   int nsteps;
   int nl_reset;   // coordinates were replaced: rebuild the Verlet lists
+  int f_valid;    // the stored forces belong to the stored coordinates
 };
 
 // initial_integrate of fix nve

@@ -129,3 +129,31 @@ def test_batch_lists_follow_dynamics(oracle_mod):
         ftot = np.linalg.norm(fo.reshape(-1, 3), axis=1).max()
         assert term_err(b.split(f)[r], fo, ftot) <= TOL[1]
     assert b.list_stats()[0] == builds + len(cases)
+
+
+def test_batch_fp64_dynamics_matches_oracle(oracle_mod):
+    """FP64 resident engine: Langevin dynamics (fix nve + fix langevin, the
+    shared counter-based random stream) against the oracle's integrator,
+    over two consecutive runs (the second continues from the stored forces)."""
+    A = _gpu()
+    cases = [make_case([40], 21), make_case([25, 20], 22)]
+    rng = np.random.default_rng(4)
+    x0 = [s.coords() for s, _ in cases]
+    v0 = [rng.normal(0.0, 0.005, size=x.shape) for x in x0]
+    b = A.Batch([A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64) for _, p in cases], A.PREC_FP64)
+    b.set_state(A.pack_states(x0), A.pack_states(v0))
+    b.compute()
+    lg = dict(dt=2.0, t_start=300.0, t_stop=300.0, damp=100.0, seed=3)
+    b.run(15, step0=0, **lg)
+    b.run(15, step0=15, **lg)
+    x, v, f, e = b.get_state()
+    xs, vs = b.split(x), b.split(v)
+    for r, (s, p) in enumerate(cases):
+        o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+        fo, _ = o.compute(x0[r])
+        xo, vo, fo, eo = o.langevin(x0[r], v0[r], fo, 15, replica=r, step0=0, **lg)
+        xo, vo, fo, eo = o.langevin(xo, vo, fo, 15, replica=r, step0=15, **lg)
+        dx = np.abs(xs[r] - xo).max()
+        dv = np.abs(vs[r] - vo).max() / np.abs(vo).max()
+        print(f"replica {r}: max|dx|={dx:.2e} A, rel dv={dv:.2e}")
+        assert dx < 1e-8 and dv < 1e-7, r
