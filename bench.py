@@ -102,6 +102,22 @@ def algorithmic_bytes(systems):
     return tot
 
 
+def profiled_traffic(chunk):
+    """HBM bytes per launch of the resident kernel from the committed rocprofv3
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated),
+    rescaled to `chunk` steps per launch; None without the summary."""
+    path = os.path.join(ROOT, "profiles", "k_resident_256chains.md")
+    try:
+        kv = {}
+        for line in open(path):
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        return float(kv["traffic_bytes_per_launch"]) * chunk / float(kv["steps_per_launch"]), float(kv["valu_busy_pct"])
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def rel_force_err(f, fref):
     d = np.linalg.norm((f - fref).reshape(-1, 3), axis=1).max()
     m = np.linalg.norm(fref.reshape(-1, 3), axis=1).max()
@@ -268,6 +284,7 @@ def main():
     bytes_step = algorithmic_bytes(systems)
     k_avg_ms = float(np.mean(kernel_ms))
     achieved = bytes_step * a.chunk / (k_avg_ms * 1e-3) / 1e9
+    traffic, valu_busy = profiled_traffic(a.chunk)
 
     if a.calib_copy_mb:
         calib_copy(a.calib_copy_mb << 20)
@@ -299,7 +316,9 @@ def main():
                                       "p99": float(np.percentile(per_step_ms, 99))},
         "ns_per_day_per_replica": value / a.replicas * 2.0 * 86400 / 1e6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/k_resident_256chains.md (rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00, per launch)",
+                     "valu_busy_pct": valu_busy,
                      "kernel": "k_resident<float,LDS>", "kernel_avg_ms": k_avg_ms,
                      "algorithmic_bytes_per_batch_step": bytes_step},
         "cpu_baseline": cpu,
