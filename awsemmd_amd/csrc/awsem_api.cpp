@@ -23,23 +23,22 @@
   } while (0)
 
 // ---------------------------------------------------------------------------
-// device copy of a set of systems
+// device copy of a set of systems (the static batch, and the single-system
+// context as a batch of one)
 
 struct DevBatch {
   int nrep = 0, max_n = 0, total_res = 0, fp32 = 0, lds = 0;
-  std::vector<ReplicaDesc> reps;
+  std::vector<std::unique_ptr<AwDevSystem>> sys;
+  std::vector<ReplicaDesc> reps;   // host copy; schedule fields set per launch
   std::vector<int> res_off;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  void *d_rep = nullptr, *d_params = nullptr;
-  void *d_coef = nullptr, *d_rf = nullptr, *d_items = nullptr;
-  int *d_rinfo = nullptr, *d_chs = nullptr, *d_chl = nullptr, *d_queue = nullptr, *d_order = nullptr;
-  double *d_aps = nullptr, *d_x = nullptr, *d_v = nullptr, *d_f = nullptr, *d_e = nullptr, *d_scr = nullptr;
+  ReplicaDesc *d_rep = nullptr;
+  void *d_params = nullptr;
+  int *d_queue = nullptr, *d_order = nullptr, *d_count = nullptr;
+  double *d_x = nullptr, *d_v = nullptr, *d_f = nullptr, *d_e = nullptr;  // concatenated state
   long long *d_list = nullptr;
-  int *d_count = nullptr;
   unsigned long long *d_prof = nullptr;  // phase clock (AWSEM_PHASE_PROF=1)
-  int *d_nl = nullptr;                   // Verlet lists (resident engine)
-  double *d_nlx = nullptr;
   bool nl_dirty = true;
   bool f_valid = false;  // d_f holds the forces of d_x (after a completed launch)
   int list_cap = 0;
@@ -48,25 +47,34 @@ struct DevBatch {
 
   ~DevBatch()
   {
-    void *ptrs[] = {d_rep, d_params, d_coef, d_rf, d_items, d_rinfo, d_chs, d_chl,
-                    d_queue, d_order, d_aps, d_x, d_v, d_f, d_e, d_scr, d_list, d_count, d_prof,
-                    d_nl, d_nlx};
-    for (void *p : ptrs) if (p) hipFree(p);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    if (stream) hipStreamDestroy(stream);
+    void *ptrs[] = {d_rep, d_params, d_queue, d_order, d_count, d_x, d_v, d_f, d_e, d_list, d_prof};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    sys.clear();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 
   AwBatchPtrs ptrs() const
   {
     AwBatchPtrs p;
     p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds;
-    p.params = d_params; p.rinfo = d_rinfo; p.aps = d_aps; p.ch_start = d_chs; p.ch_len = d_chl;
-    p.x = d_x; p.v = d_v; p.f = d_f; p.energy = d_e; p.scratch = d_scr;
-    p.coef = d_coef; p.rf = d_rf; p.items = d_items;
-    p.queue = d_queue; p.order = d_order; p.prof = d_prof;
-    p.nl = d_nl; p.nlx = d_nlx;
+    p.params = d_params; p.queue = d_queue; p.order = d_order; p.prof = d_prof;
     return p;
+  }
+
+  // schedule of every replica for one launch, then the descriptors to the device
+  AwStatus set_schedule(int nsteps, const awsem_langevin *lg)
+  {
+    for (int r = 0; r < nrep; ++r) {
+      ReplicaDesc &R = reps[r];
+      R.nsteps = nsteps; R.ramp0 = 0; R.run_steps = nsteps; R.rng_id = r;
+      R.t_start = lg ? lg->t_start : 0.0; R.t_stop = lg ? lg->t_stop : 0.0;
+      R.seed = lg ? lg->seed : 0; R.step0 = lg ? lg->step0 : 0;
+      R.flags = (f_valid ? AW_RF_F_VALID : 0) | (nl_dirty ? AW_RF_NL_RESET : 0);
+    }
+    HIPCHK(hipMemcpyAsync(d_rep, reps.data(), reps.size() * sizeof(ReplicaDesc), hipMemcpyHostToDevice, stream));
+    return AwStatus();
   }
 };
 
@@ -78,12 +86,6 @@ static AwStatus upload(T **dst, const std::vector<T> &src, size_t min_elems = 1)
   if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
   return AwStatus();
 }
-static AwStatus upload_raw(void **dst, const void *src, size_t bytes)
-{
-  HIPCHK(hipMalloc(dst, std::max<size_t>(bytes, 16)));
-  if (bytes) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
-  return AwStatus();
-}
 
 static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int precision, bool want_lds,
                                std::unique_ptr<DevBatch> &out)
@@ -92,66 +94,14 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   D->nrep = (int)sys.size();
   D->fp32 = precision == AWSEM_PREC_FP32;
   D->P = sys[0]->P;
-  std::vector<int> rinfo, chs, chl;
-  std::vector<double> aps;
-  AwFMPack fm;
-  long long scr = 0, nl_ints = 0, nlx_dbl = 0;
-  int res = 0, ch = 0;
-  for (int r = 0; r < D->nrep; ++r) {
-    const AwSystem &S = *sys[r];
-    ReplicaDesc R;
-    std::memset(&R, 0, sizeof R);
-    R.n = S.n; R.nch = S.nch; R.res_off = res; R.ch_off = ch;
-    int item_base = fm.n_items();
-    R.fm_item_base = item_base;
-    if (S.P.fm_flag) {
-      AwStatus s = aw_build_fm(S, fm);
-      if (!s.ok()) return s;
-    }
-    R.fm_items = fm.n_items() - item_base;
-    R.scratch_off = scr;
-    scr += aw_slab_doubles(S.n);
-    // Verlet list capacity: all pairs for small systems, else 128 per residue
-    long long allp = (long long)S.n * (S.n - 1) / 2;
-    R.nl_cap = (int)std::min<long long>(std::max<long long>(allp, 1), 128LL * S.n);
-    R.nl_off = nl_ints;
-    nl_ints += 4 + 5LL * R.nl_cap;  // header, list, then work queues DSSP [2 cap], P_AP [cap], EV [cap]
-    R.nlx_off = nlx_dbl;
-    nlx_dbl += 4LL * S.n;
-    D->reps.push_back(R);
-    D->res_off.push_back(res);
-    rinfo.insert(rinfo.end(), S.rinfo.begin(), S.rinfo.end());
-    aps.insert(aps.end(), S.aps.begin(), S.aps.end());
-    chs.insert(chs.end(), S.ch_start.begin(), S.ch_start.end());
-    chl.insert(chl.end(), S.ch_len.begin(), S.ch_len.end());
-    res += S.n; ch += S.nch;
-    D->max_n = std::max(D->max_n, S.n);
-  }
+  int res = 0;
+  for (const AwSystem *S : sys) { D->res_off.push_back(res); res += S->n; D->max_n = std::max(D->max_n, S->n); }
   D->total_res = res;
   D->lds = want_lds && D->max_n <= aw_lds_max_residues();
-  AwStatus s;
   HIPCHK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&D->ev0));
   HIPCHK(hipEventCreate(&D->ev1));
-  if (!(s = upload_raw(&D->d_rep, D->reps.data(), D->reps.size() * sizeof(ReplicaDesc))).ok()) return s;
-  if (!(s = upload_raw(&D->d_params, &D->P, sizeof(FFParams))).ok()) return s;
-  if (!(s = upload(&D->d_rinfo, rinfo)).ok()) return s;
-  if (!(s = upload(&D->d_aps, aps)).ok()) return s;
-  if (!(s = upload(&D->d_chs, chs)).ok()) return s;
-  if (!(s = upload(&D->d_chl, chl)).ok()) return s;
-  if (D->fp32) {
-    std::vector<float> c(fm.coef.begin(), fm.coef.end()), rf(fm.rf.begin(), fm.rf.end());
-    std::vector<FMItem> it = aw_fm_items<FMItem>(fm);
-    if (!(s = upload_raw(&D->d_coef, c.data(), c.size() * sizeof(float))).ok()) return s;
-    if (!(s = upload_raw(&D->d_rf, rf.data(), rf.size() * sizeof(float))).ok()) return s;
-    if (!(s = upload_raw(&D->d_items, it.data(), it.size() * sizeof(FMItem))).ok()) return s;
-  } else {
-    if (!(s = upload_raw(&D->d_coef, fm.coef.data(), fm.coef.size() * sizeof(double))).ok()) return s;
-    if (!(s = upload_raw(&D->d_rf, fm.rf.data(), fm.rf.size() * sizeof(double))).ok()) return s;
-    std::vector<FMItemD> it = aw_fm_items<FMItemD>(fm);
-    if (!(s = upload_raw(&D->d_items, it.data(), it.size() * sizeof(FMItemD))).ok()) return s;
-  }
-  size_t st = (size_t)res * 9 * sizeof(double);
+  size_t st = std::max<size_t>((size_t)res * 9, 1) * sizeof(double);
   HIPCHK(hipMalloc((void **)&D->d_x, st));
   HIPCHK(hipMalloc((void **)&D->d_v, st));
   HIPCHK(hipMalloc((void **)&D->d_f, st));
@@ -160,12 +110,24 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMemset(D->d_f, 0, st));
   HIPCHK(hipMalloc((void **)&D->d_e, (size_t)D->nrep * AW_N_ET * sizeof(double)));
   HIPCHK(hipMemset(D->d_e, 0, (size_t)D->nrep * AW_N_ET * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&D->d_scr, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
-  HIPCHK(hipMemset(D->d_scr, 0, (size_t)std::max<long long>(scr, 16) * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&D->d_nl, (size_t)std::max<long long>(nl_ints, 4) * sizeof(int)));
-  HIPCHK(hipMemset(D->d_nl, 0, (size_t)std::max<long long>(nl_ints, 4) * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&D->d_nlx, (size_t)std::max<long long>(nlx_dbl, 4) * sizeof(double)));
-  HIPCHK(hipMemset(D->d_nlx, 0, (size_t)std::max<long long>(nlx_dbl, 4) * sizeof(double)));
+  for (int r = 0; r < D->nrep; ++r) {
+    const AwSystem &S = *sys[r];
+    AwDevPlan plan;
+    AwStatus s = aw_devsys_plan(S, D->fp32, !D->lds, false, plan);
+    if (!s.ok()) return s;
+    std::unique_ptr<AwDevSystem> ds(new AwDevSystem());
+    if (!(s = aw_devsys_create(S, plan, *ds)).ok()) return s;
+    ReplicaDesc R = ds->d;
+    R.x = D->d_x + 9LL * D->res_off[r];
+    R.v = D->d_v + 9LL * D->res_off[r];
+    R.f = D->d_f + 9LL * D->res_off[r];
+    R.energy = D->d_e + (long long)r * AW_N_ET;
+    D->reps.push_back(R);
+    D->sys.push_back(std::move(ds));
+  }
+  AwStatus s;
+  if (!(s = upload_raw((void **)&D->d_rep, D->reps.data(), D->reps.size() * sizeof(ReplicaDesc))).ok()) return s;
+  if (!(s = upload_raw(&D->d_params, &D->P, sizeof(FFParams))).ok()) return s;
   HIPCHK(hipMalloc((void **)&D->d_queue, 64));
   HIPCHK(hipMalloc((void **)&D->d_count, 64));
   if (const char *e = std::getenv("AWSEM_PHASE_PROF")) {
@@ -462,6 +424,8 @@ static AwStatus compute_res(awsem_ctx *c, const double *xres, double *fres, doub
   AwRun run;
   std::memset(&run, 0, sizeof run);
   run.mode = 0;
+  AwStatus ss = D.set_schedule(0, nullptr);
+  if (!ss.ok()) return ss;
   HIPCHK(hipEventRecord(D.ev0, D.stream));
   if (aw_launch_phases(D.ptrs(), run, D.stream)) return AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed");
   HIPCHK(hipEventRecord(D.ev1, D.stream));
@@ -549,11 +513,12 @@ int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres, 
     return fail(c, AwStatus::err(AWSEM_E_DEVICE, "copy"));
   AwRun run;
   std::memset(&run, 0, sizeof run);
-  run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.t_start = lg->t_start; run.t_stop = lg->t_stop;
-  run.damp = lg->damp; run.seed = lg->seed; run.step0 = lg->step0;
-  hipEventRecord(D.ev0, D.stream);
+  run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.damp = lg->damp;
+  s = D.set_schedule(nsteps, lg);
+  if (!s.ok()) return fail(c, s);
+  (void)hipEventRecord(D.ev0, D.stream);
   if (aw_launch_phases(D.ptrs(), run, D.stream)) return fail(c, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
-  hipEventRecord(D.ev1, D.stream);
+  (void)hipEventRecord(D.ev1, D.stream);
   double e[AW_N_ET];
   if (hipMemcpyAsync(xres, D.d_x, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
       hipMemcpyAsync(vres, D.d_v, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
@@ -634,11 +599,11 @@ int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, doubl
   return AWSEM_OK;
 }
 
-static int batch_launch(awsem_batch *b, AwRun run)
+static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
 {
   DevBatch &D = *b->dev;
-  run.nl_reset = D.nl_dirty ? 1 : 0;
-  run.f_valid = D.f_valid ? 1 : 0;
+  AwStatus s = D.set_schedule(run.mode == 1 ? run.nsteps : 0, lg);
+  if (!s.ok()) return bfail(b, s);
   D.nl_dirty = false;
   D.f_valid = false;
   (void)hipEventRecord(D.ev0, D.stream);
@@ -660,7 +625,7 @@ int awsem_batch_compute(awsem_batch *b)
   AwRun run;
   std::memset(&run, 0, sizeof run);
   run.mode = 0;
-  return batch_launch(b, run);
+  return batch_launch(b, run, nullptr);
 }
 
 int awsem_batch_run(awsem_batch *b, int nsteps, const awsem_langevin *lg)
@@ -668,9 +633,8 @@ int awsem_batch_run(awsem_batch *b, int nsteps, const awsem_langevin *lg)
   if (!b || !b->dev || !lg || nsteps < 0) return AWSEM_E_ARG;
   AwRun run;
   std::memset(&run, 0, sizeof run);
-  run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.t_start = lg->t_start; run.t_stop = lg->t_stop;
-  run.damp = lg->damp; run.seed = lg->seed; run.step0 = lg->step0;
-  return batch_launch(b, run);
+  run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.damp = lg->damp;
+  return batch_launch(b, run, lg);
 }
 
 int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel_avg, int *n_kernels)
@@ -699,7 +663,7 @@ int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *p
   int bad = 0;
   for (const ReplicaDesc &R : D.reps) {
     int h[4] = {0, 0, 0, 0};
-    if (R.nl_cap > 0 && hipMemcpy(h, D.d_nl + R.nl_off, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+    if (R.nl_cap > 0 && hipMemcpy(h, R.nl, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
       return AWSEM_E_DEVICE;
     nb += h[2];
     np += h[0];

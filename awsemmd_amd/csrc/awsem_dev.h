@@ -7,14 +7,30 @@
 #include "awsem_launch.h"
 
 // ------------------------------------------------------------------------
-// energy epilogue: ered[N_EACC] (LDS) -> out[slot]
+// energy epilogue: ered[N_EACC] (LDS) -> out[AW_N_ET]
 
-__device__ __forceinline__ void energy_out(const double *ered, double *out, bool atomic)
+// phase engine: blocks add their partial sums
+__device__ __forceinline__ void energy_add(const double *ered, double *out)
 {
   if (threadIdx.x < N_EACC) {
     double v = ered[threadIdx.x];
-    if (atomic) { if (v != 0.0) atomicAdd(&out[eacc_slot[threadIdx.x]], v); }
-    else out[eacc_slot[threadIdx.x]] = v;
+    if (v != 0.0) atomicAdd(&out[eacc_slot[threadIdx.x]], v);
+  }
+}
+
+// resident engine: every slot written, slot 0 the total
+__device__ __forceinline__ void energy_write(const double *ered, double *out)
+{
+  int t = threadIdx.x;
+  if (t < AW_N_ET) {
+    double v = 0.0;
+    if (t == 0) {
+      for (int k = 0; k < N_EACC; ++k) v += ered[k];
+    } else {
+      for (int k = 0; k < N_EACC; ++k)
+        if (eacc_slot[k] == t) v = ered[k];
+    }
+    out[t] = v;
   }
 }
 
@@ -22,46 +38,34 @@ __device__ __forceinline__ void energy_out(const double *ered, double *out, bool
 // batch description and views
 
 struct BatchDev {
-  const ReplicaDesc *rep;
+  const ReplicaDesc *rep;     // [nrep]
   int nrep;
   const FFParams *P;
-  const int *rinfo;
-  const double *aps;
-  const int *ch_start, *ch_len;
-  double *x, *v, *f;          // state [res_off*9 ...]
-  double *energy;             // [nrep][AW_N_ET]
-  double *scratch;            // global-mode slabs
-  const void *coef;
-  const void *rf;
-  const void *items;
   int *queue;                 // work-queue counter (resident engine)
   const int *order;           // replica order (longest first)
   unsigned long long *prof;   // optional phase clock: [2][16] (first replica, all replicas)
-  int *nl;                    // Verlet lists (resident engine)
-  double *nlx;
 };
 
-// doubles of per-replica working storage (x, xv, f, 10 scalars, ext+pad)
+// doubles of per-replica working storage (x, xv, f, 10 scalars, ext, rinfo copy)
 __host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + 2LL * ((n + 1) / 2) + 8; }
 
 template <class T>
 __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDesc &R, double *slab)
 {
   View<T> V;
-  int n = R.n;
-  V.n = n; V.nch = R.nch;
-  V.rinfo = B.rinfo + R.res_off;
-  V.aps = B.aps + (long long)R.res_off * 12;
-  V.ch_start = B.ch_start + R.ch_off;
-  V.ch_len = B.ch_len + R.ch_off;
+  V.n = R.n; V.nch = R.nch;
+  V.rinfo = R.rinfo;
+  V.aps = R.aps;
+  V.ch_start = R.ch_start;
+  V.ch_len = R.ch_len;
   V.s = slab;
   V.ered = nullptr;
   V.P = B.P;
   V.fm_items = R.fm_items;
-  V.fm_item_base = R.fm_item_base;
-  V.coef = (const T *)B.coef;
-  V.rf = B.rf;
-  V.items = B.items;
+  V.fm_item_base = 0;
+  V.coef = (const T *)R.coef;
+  V.rf = R.rf;
+  V.items = R.items;
   V.nl = nullptr;
   V.nlx = nullptr;
   V.nl_cap = 0;
@@ -76,24 +80,11 @@ static inline BatchDev to_dev(const AwBatchPtrs &p)
   B.rep = (const ReplicaDesc *)p.rep;
   B.nrep = p.nrep;
   B.P = (const FFParams *)p.params;
-  B.rinfo = p.rinfo;
-  B.aps = p.aps;
-  B.ch_start = p.ch_start;
-  B.ch_len = p.ch_len;
-  B.x = p.x; B.v = p.v; B.f = p.f;
-  B.energy = p.energy;
-  B.scratch = p.scratch;
-  B.coef = p.coef;
-  B.rf = p.rf;
-  B.items = p.items;
   B.queue = p.queue;
   B.order = p.order;
   B.prof = p.prof;
-  B.nl = p.nl;
-  B.nlx = p.nlx;
   return B;
 }
-
 
 // ------------------------------------------------------------------------
 // resident engine: one workgroup per replica, nsteps inside the launch
@@ -162,34 +153,34 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     if (slot >= B.nrep) break;
     int r = B.order[slot];
     const ReplicaDesc R = B.rep[r];
-    double *slab = LDSM ? smem : (B.scratch + R.scratch_off);
+    double *slab = LDSM ? smem : R.scratch;
     View<T> V = make_view<T>(B, R, slab);
     V.P = Pr;  // read-only, unaliased: uniform parameter loads go through the scalar cache
     V.ered = ered;
-    V.nl = R.nl_cap > 0 ? B.nl + R.nl_off : nullptr;
-    V.nlx = B.nlx + R.nlx_off;
+    V.nl = R.nl_cap > 0 ? R.nl : nullptr;
+    V.nlx = R.nlx;
     V.nl_cap = R.nl_cap;
     V.nl_state = nl_state;
     V.wq = V.nl ? V.nl + 4 + R.nl_cap : nullptr;
     if (threadIdx.x < 8) nl_state[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
-      if (lg.nl_reset && V.nl) V.nl[1] = 0;
+      if ((R.flags & AW_RF_NL_RESET) && V.nl) V.nl[1] = 0;
     }
     int n = R.n;
-    double *gx = B.x + 9LL * R.res_off, *gv = B.v + 9LL * R.res_off, *gf = B.f + 9LL * R.res_off;
+    double *gx = R.x, *gv = R.v, *gf = R.f;
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
     int *ri = V.rinfo_copy();
     for (int k = threadIdx.x; k < n; k += blockDim.x) ri[k] = V.rinfo[k];
     // dynamics continuing from the previous launch: its final forces are
     // still valid, so the first step integrates with them directly
-    bool have_f = mode == 1 && lg.f_valid && lg.nsteps > 0;
+    int nst = mode == 1 ? R.nsteps : 0;
+    bool have_f = mode == 1 && (R.flags & AW_RF_F_VALID) && nst > 0;
     if (have_f)
       for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.f()[k] = gf[k];
     __syncthreads();
     V.rinfo = ri;
     bool first = slot == 0;
     // s = -1: the force evaluation before the first step (one call site)
-    int nst = mode == 1 ? lg.nsteps : 0;
     for (int s = have_f ? 0 : -1; s < nst; ++s) {
       unsigned long long t = B.prof ? wall_clock64() : 0;
       if (s >= 0) {
@@ -200,7 +191,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
       force_eval_block(V, B.prof, first);
       if (s >= 0) {
         t = B.prof ? wall_clock64() : 0;
-        integrate2(V, gv, lg, s, r, threadIdx.x, blockDim.x);
+        integrate2(V, gv, lg, R, s, threadIdx.x, blockDim.x);
         __syncthreads();
         prof_mark(B.prof, first, 7, t);
       }
@@ -208,25 +199,29 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     if (mode == 1)
       for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gx[k] = V.x()[k];
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) gf[k] = V.f()[k];
-    energy_out(ered, B.energy + (long long)r * AW_N_ET, false);
+    energy_write(ered, R.energy);
     __syncthreads();
   }
 }
 
-// total energy slot of every replica (awsem_kernels.hip)
-void aw_energy_total(double *energy, int nrep, hipStream_t st);
+template <int D>
+__global__ void k_zero_int(int *p)
+{
+  if (threadIdx.x == 0) *p = 0;
+}
 
+// launches the resident engine over p.rep[0..nrep) (descriptors already on
+// the device, p.queue zeroed by the launcher)
 template <class T>
 static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 {
   BatchDev B = to_dev(p);
   LgParams lg;
-  lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
-  lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
-  lg.nl_reset = run.nl_reset;
-  lg.f_valid = run.f_valid;
+  lg.dt = run.dt; lg.damp = run.damp;
   int mode = run.mode;
-  (void)hipMemsetAsync(p.queue, 0, sizeof(int), st);
+  if (p.nrep <= 0) return 0;
+  // a kernel, not a memset node, so that the sequence can be graph-captured
+  hipLaunchKernelGGL(k_zero_int<0>, dim3(1), dim3(64), 0, st, p.queue);
   // 512 threads: 8 waves per CU with up to 256 VGPRs each
   int grid = run.grid > 0 ? run.grid : p.nrep;
   size_t lds = p.lds_resident ? (size_t)slab_doubles(p.max_n) * sizeof(double) : 0;
@@ -236,6 +231,5 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   } else {
     hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   }
-  aw_energy_total(p.energy, p.nrep, st);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

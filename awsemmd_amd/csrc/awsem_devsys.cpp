@@ -1,0 +1,118 @@
+// awsem_devsys.cpp -- one system's device image: layout, packing and upload.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "awsem_host.h"
+
+#define DS_HIPCHK(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return AwStatus::err(-7, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+AwStatus upload_raw(void **dst, const void *src, size_t bytes)
+{
+  DS_HIPCHK(hipMalloc(dst, std::max<size_t>(bytes, 16)));
+  if (bytes) DS_HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return AwStatus();
+}
+
+AwDevSystem::~AwDevSystem()
+{
+  if (arena) (void)hipFree(arena);
+}
+
+static size_t take(size_t &cur, size_t bytes)
+{
+  size_t o = cur;
+  cur += (bytes + 255) & ~(size_t)255;
+  return o;
+}
+
+AwStatus aw_devsys_plan(const AwSystem &S, bool fp32, bool scratch, bool own_state, AwDevPlan &plan)
+{
+  plan = AwDevPlan();
+  plan.fp32 = fp32; plan.scratch = scratch; plan.own_state = own_state;
+  if (S.P.fm_flag) {
+    AwStatus s = aw_build_fm(S, plan.fm);
+    if (!s.ok()) return s;
+  }
+  int n = S.n;
+  long long allp = (long long)n * (n - 1) / 2;
+  plan.nl_cap = (int)std::min<long long>(std::max<long long>(allp, 1), 128LL * n);
+  size_t tsz = fp32 ? sizeof(float) : sizeof(double);
+  size_t cur = 0;
+  if (own_state) {
+    plan.off_x = take(cur, 9 * (size_t)n * sizeof(double));
+    plan.off_v = take(cur, 9 * (size_t)n * sizeof(double));
+    plan.off_f = take(cur, 9 * (size_t)n * sizeof(double));
+    plan.off_e = take(cur, AW_N_ET * sizeof(double));
+  }
+  plan.off_rinfo = take(cur, (size_t)n * sizeof(int));
+  plan.off_aps = take(cur, 12 * (size_t)n * sizeof(double));
+  plan.off_chs = take(cur, (size_t)S.nch * sizeof(int));
+  plan.off_chl = take(cur, (size_t)S.nch * sizeof(int));
+  plan.off_items = take(cur, (size_t)plan.fm.n_items() * (fp32 ? sizeof(FMItem) : sizeof(FMItemD)));
+  plan.off_rf = take(cur, plan.fm.rf.size() * tsz);
+  plan.off_coef = take(cur, plan.fm.coef.size() * tsz);
+  plan.off_nl = take(cur, (4 + 5 * (size_t)plan.nl_cap) * sizeof(int));
+  plan.off_nlx = take(cur, 4 * (size_t)n * sizeof(double));
+  if (scratch) plan.off_scr = take(cur, (size_t)aw_slab_doubles(n) * sizeof(double));
+  plan.bytes = std::max<size_t>(cur, 256);
+  return AwStatus();
+}
+
+template <class T>
+static void put(std::vector<unsigned char> &img, size_t off, const std::vector<T> &v)
+{
+  if (!v.empty()) std::memcpy(img.data() + off, v.data(), v.size() * sizeof(T));
+}
+
+AwStatus aw_devsys_create(const AwSystem &S, const AwDevPlan &plan, AwDevSystem &out)
+{
+  int n = S.n;
+  std::vector<unsigned char> img(plan.bytes, 0);
+  put(img, plan.off_rinfo, S.rinfo);
+  put(img, plan.off_aps, S.aps);
+  put(img, plan.off_chs, S.ch_start);
+  put(img, plan.off_chl, S.ch_len);
+  const AwFMPack &fm = plan.fm;
+  if (plan.fp32) {
+    put(img, plan.off_items, aw_fm_items<FMItem>(fm));
+    put(img, plan.off_rf, std::vector<float>(fm.rf.begin(), fm.rf.end()));
+    put(img, plan.off_coef, std::vector<float>(fm.coef.begin(), fm.coef.end()));
+  } else {
+    put(img, plan.off_items, aw_fm_items<FMItemD>(fm));
+    put(img, plan.off_rf, fm.rf);
+    put(img, plan.off_coef, fm.coef);
+  }
+  DS_HIPCHK(hipMalloc(&out.arena, plan.bytes));
+  out.bytes = plan.bytes;
+  DS_HIPCHK(hipMemcpy(out.arena, img.data(), plan.bytes, hipMemcpyHostToDevice));
+  unsigned char *b = (unsigned char *)out.arena;
+  ReplicaDesc &R = out.d;
+  std::memset(&R, 0, sizeof R);
+  R.n = n; R.nch = S.nch;
+  R.fm_items = fm.n_items();
+  R.nl_cap = plan.nl_cap;
+  if (plan.own_state) {
+    R.x = (double *)(b + plan.off_x);
+    R.v = (double *)(b + plan.off_v);
+    R.f = (double *)(b + plan.off_f);
+    R.energy = (double *)(b + plan.off_e);
+  }
+  R.rinfo = (const int *)(b + plan.off_rinfo);
+  R.aps = (const double *)(b + plan.off_aps);
+  R.ch_start = (const int *)(b + plan.off_chs);
+  R.ch_len = (const int *)(b + plan.off_chl);
+  R.items = b + plan.off_items;
+  R.rf = b + plan.off_rf;
+  R.coef = b + plan.off_coef;
+  R.nl = (int *)(b + plan.off_nl);
+  R.nlx = (double *)(b + plan.off_nlx);
+  R.scratch = plan.scratch ? (double *)(b + plan.off_scr) : nullptr;
+  return AwStatus();
+}

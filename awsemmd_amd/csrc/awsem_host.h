@@ -77,3 +77,29 @@ std::vector<ItemT> aw_fm_items(const AwFMPack &B)
   return out;
 }
 void aw_default_pair_styles(FFParams &P);
+// ---------------------------------------------------------------------------
+// One system's device image (awsem_devsys.cpp): a single allocation holding
+// its per-residue constants, fragment-memory tables, Verlet list block and,
+// optionally, its state (x, v, f, energy) and a global working slab.
+struct AwDevPlan {
+  AwFMPack fm;
+  bool fp32 = false, scratch = false, own_state = false;
+  int nl_cap = 0;
+  size_t off_x = 0, off_v = 0, off_f = 0, off_e = 0, off_rinfo = 0, off_aps = 0, off_chs = 0, off_chl = 0;
+  size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0;
+  size_t bytes = 0;
+};
+struct AwDevSystem {
+  void *arena = nullptr;
+  size_t bytes = 0;
+  ReplicaDesc d;  // device pointers into the arena; schedule fields unset
+  AwDevSystem() { std::memset(&d, 0, sizeof d); }
+  ~AwDevSystem();
+  AwDevSystem(const AwDevSystem &) = delete;
+  AwDevSystem &operator=(const AwDevSystem &) = delete;
+};
+AwStatus aw_devsys_plan(const AwSystem &S, bool fp32, bool scratch, bool own_state, AwDevPlan &plan);
+AwStatus aw_devsys_create(const AwSystem &S, const AwDevPlan &plan, AwDevSystem &out);
+AwStatus upload_raw(void **dst, const void *src, size_t bytes);
+long long aw_slab_doubles(int n);
+

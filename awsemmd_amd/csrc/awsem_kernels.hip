@@ -32,18 +32,16 @@
 template <class T>
 __device__ __forceinline__ View<T> phase_view(const BatchDev &B, int r)
 {
-  const ReplicaDesc R = B.rep[r];
-  View<T> V = make_view<T>(B, R, B.scratch + R.scratch_off);
-  return V;
+  const ReplicaDesc &R = B.rep[r];
+  return make_view<T>(B, R, R.scratch);
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_ph_load(BatchDev B, int r)
 {
-  const ReplicaDesc R = B.rep[r];
+  const ReplicaDesc &R = B.rep[r];
   View<T> V = phase_view<T>(B, r);
-  const double *gx = B.x + 9LL * R.res_off;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) V.x()[k] = gx[k];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) V.x()[k] = R.x[k];
 }
 
 template <class T>
@@ -51,7 +49,7 @@ __global__ void __launch_bounds__(256) k_ph_A(BatchDev B, int r)
 {
   View<T> V = phase_view<T>(B, r);
   phase_A(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-  if (blockIdx.x == 0 && threadIdx.x < AW_N_ET) B.energy[(long long)r * AW_N_ET + threadIdx.x] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x < AW_N_ET) B.rep[r].energy[threadIdx.x] = 0.0;
 }
 
 #define PH_KERNEL(NAME, CALL)                                                          \
@@ -65,7 +63,7 @@ __global__ void __launch_bounds__(256) k_ph_A(BatchDev B, int r)
     __syncthreads();                                                                   \
     CALL;                                                                              \
     __syncthreads();                                                                   \
-    energy_out(ered, B.energy + (long long)r * AW_N_ET, true);                         \
+    energy_add(ered, B.rep[r].energy);                                                 \
   }
 
 PH_KERNEL(k_ph_B, (phase_B<T, false>(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, 7)))
@@ -83,21 +81,19 @@ __global__ void __launch_bounds__(256) k_ph_E(BatchDev B, int r)
 template <class T>
 __global__ void __launch_bounds__(256) k_ph_store(BatchDev B, int r, int with_x)
 {
-  const ReplicaDesc R = B.rep[r];
+  const ReplicaDesc &R = B.rep[r];
   View<T> V = phase_view<T>(B, r);
-  double *gf = B.f + 9LL * R.res_off, *gx = B.x + 9LL * R.res_off;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 9 * R.n; k += gridDim.x * blockDim.x) {
-    gf[k] = V.f()[k];
-    if (with_x) gx[k] = V.x()[k];
+    R.f[k] = V.f()[k];
+    if (with_x) R.x[k] = V.x()[k];
   }
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_ph_int1(BatchDev B, int r, LgParams lg)
 {
-  const ReplicaDesc R = B.rep[r];
   View<T> V = phase_view<T>(B, r);
-  integrate1(V, B.v + 9LL * R.res_off, lg, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  integrate1(V, B.rep[r].v, lg, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 template <class T>
@@ -105,15 +101,14 @@ __global__ void __launch_bounds__(256) k_ph_int2(BatchDev B, int r, LgParams lg,
 {
   const ReplicaDesc R = B.rep[r];
   View<T> V = phase_view<T>(B, r);
-  integrate2(V, B.v + 9LL * R.res_off, lg, s, r, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  integrate2(V, R.v, lg, R, s, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
-// total energy slot
-__global__ void k_energy_total(double *energy, int nrep)
+// total energy slot of one replica
+__global__ void k_energy_total(BatchDev B, int r)
 {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrep) return;
-  double *e = energy + (long long)r * AW_N_ET;
+  if (threadIdx.x != 0) return;
+  double *e = B.rep[r].energy;
   double s = 0.0;
   for (int k = 1; k < AW_N_ET; ++k) s += e[k];
   e[0] = s;
@@ -235,6 +230,14 @@ int aw_lds_max_residues()
 
 long long aw_slab_doubles(int n) { return slab_doubles(n); }
 
+int aw_launch_resident_f32(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
+int aw_launch_resident_f64(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
+
+int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
+{
+  return p.fp32 ? aw_launch_resident_f32(p, run, st) : aw_launch_resident_f64(p, run, st);
+}
+
 template <class T>
 static void force_phases(const BatchDev &B, int r, int grid, hipStream_t st)
 {
@@ -244,17 +247,17 @@ static void force_phases(const BatchDev &B, int r, int grid, hipStream_t st)
   hipLaunchKernelGGL(k_ph_D<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_E<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_F<T>, dim3(grid), dim3(256), 0, st, B, r);
+  hipLaunchKernelGGL(k_energy_total, dim3(1), dim3(64), 0, st, B, r);
 }
 
+// the phase engine: replicas one after another, each phase a grid-wide
+// launch; dynamics follow the replicas' descriptors (schedule, seed)
 template <class T>
 static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 {
   BatchDev B = to_dev(p);
   LgParams lg;
-  lg.dt = run.dt; lg.t_start = run.t_start; lg.t_stop = run.t_stop; lg.damp = run.damp;
-  lg.seed = run.seed; lg.step0 = run.step0; lg.nsteps = run.nsteps;
-  lg.nl_reset = 0;
-  lg.f_valid = 0;
+  lg.dt = run.dt; lg.damp = run.damp;
   int grid = run.grid > 0 ? run.grid : 1024;
   for (int r = 0; r < p.nrep; ++r) {
     hipLaunchKernelGGL(k_ph_load<T>, dim3(grid), dim3(256), 0, st, B, r);
@@ -268,21 +271,7 @@ static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t s
     }
     hipLaunchKernelGGL(k_ph_store<T>, dim3(grid), dim3(256), 0, st, B, r, run.mode == 1 ? 1 : 0);
   }
-  hipLaunchKernelGGL(k_energy_total, dim3((p.nrep + 63) / 64), dim3(64), 0, st, p.energy, p.nrep);
   return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int aw_launch_resident_f32(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
-int aw_launch_resident_f64(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
-
-int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
-{
-  return p.fp32 ? aw_launch_resident_f32(p, run, st) : aw_launch_resident_f64(p, run, st);
-}
-
-void aw_energy_total(double *energy, int nrep, hipStream_t st)
-{
-  hipLaunchKernelGGL(k_energy_total, dim3((nrep + 63) / 64), dim3(64), 0, st, energy, nrep);
 }
 
 int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)

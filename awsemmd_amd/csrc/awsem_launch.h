@@ -4,33 +4,22 @@
 #include <hip/hip_runtime.h>
 
 struct AwBatchPtrs {
-  const void *rep;      // ReplicaDesc[nrep]
+  const void *rep;      // ReplicaDesc[nrep] on the device
   int nrep;
   int max_n;            // largest replica (residues)
   int fp32;             // pair arithmetic in FP32 (else FP64)
   int lds_resident;     // resident engine keeps replicas in LDS
-  const void *params;   // FFParams
-  const int *rinfo;
-  const double *aps;
-  const int *ch_start, *ch_len;
-  double *x, *v, *f, *energy, *scratch;
-  const void *coef, *rf, *items;
-  int *queue;
-  const int *order;
+  const void *params;   // FFParams on the device
+  int *queue;           // one int of device scratch (work-queue counter)
+  const int *order;     // [nrep] replica order for the resident engine
   unsigned long long *prof;  // optional per-phase clock accumulators [32]
-  int *nl;                   // Verlet lists of the resident engine
-  double *nlx;
 };
 
 struct AwRun {
   int mode;             // 0 force evaluation, 1 dynamics
-  int nsteps;
-  double dt, t_start, t_stop, damp;
-  unsigned long long seed;
-  long long step0;
-  int threads, grid;    // 0 = defaults
-  int nl_reset;         // state replaced since the last launch
-  int f_valid;          // device forces belong to the current coordinates
+  int nsteps;           // phase engine: steps of this launch
+  double dt, damp;
+  int grid;             // 0 = default
 };
 
 int aw_lds_max_residues();

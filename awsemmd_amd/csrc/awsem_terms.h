@@ -1408,12 +1408,7 @@ AW_DEV double urand(uint64_t seed, int replica, long long step, int atom, int co
 AW_DEV double atom_mass(int kind, int ri) { return kind == K_CA ? 27.0 : (kind == K_O ? 28.0 : ((ri & RI_GLY) ? 2.0 : 60.0)); }
 
 struct LgParams {
-  double dt, t_start, t_stop, damp;
-  unsigned long long seed;
-  long long step0;
-  int nsteps;
-  int nl_reset;   // coordinates were replaced: rebuild the Verlet lists
-  int f_valid;    // the stored forces belong to the stored coordinates
+  double dt, damp;  // fs; fix langevin damping time (fs)
 };
 
 // initial_integrate of fix nve
@@ -1435,14 +1430,16 @@ AW_DEV void integrate1(const View<T> &V, double *v, const LgParams &lg, int t0, 
 
 // fix langevin post_force + fix nve final_integrate
 template <class T>
-AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, int s, int replica, int t0, int ts)
+AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, const ReplicaDesc &R, int s, int t0, int ts)
 {
   int n = V.n;
   double dtf = 0.5 * lg.dt * AW_FTM2V;
-  double delta = lg.nsteps > 0 ? (double)(s + 1) / (double)lg.nsteps : 0.0;
-  double Tt = lg.t_start + delta * (lg.t_stop - lg.t_start);
+  // the target temperature ramps over the replica's whole run (which may
+  // span several launches)
+  double delta = R.run_steps > 0 ? (double)(R.ramp0 + s + 1) / (double)R.run_steps : 0.0;
+  double Tt = R.t_start + delta * (R.t_stop - R.t_start);
   double tsq = sqrt(Tt);
-  long long step = lg.step0 + s;
+  long long step = R.step0 + s;
   for (int a = t0; a < 3 * n; a += ts) {
     int kind = a / n, i = a - kind * n;
     double m = atom_mass(kind, V.rinfo[i]);
@@ -1450,7 +1447,7 @@ AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, int s, i
     double g2 = sqrt(m) * sqrt(24.0 * AW_BOLTZ / lg.damp / lg.dt / AW_MVV2E) / AW_FTM2V * tsq;
     double dtfm = dtf / m;
     for (int l = 0; l < 3; ++l) {
-      double fv = V.f()[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand(lg.seed, replica, step, a, l) - 0.5);
+      double fv = V.f()[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand(R.seed, R.rng_id, step, a, l) - 0.5);
       V.f()[3 * a + l] = fv;
       v[3 * a + l] += dtfm * fv;
     }

@@ -101,14 +101,28 @@ struct FMItemD {
   int rf0, stride, nmem, coef0, pad;
 };
 
+// One replica (system) as the kernels see it: device pointers to its own
+// data and its dynamics schedule for the current launch.
+#define AW_RF_F_VALID 1   // f holds the forces of x: dynamics continue from them
+#define AW_RF_NL_RESET 2  // x was replaced: rebuild the Verlet list first
 struct ReplicaDesc {
-  int n;          // residues
-  int nch;        // chains
-  int res_off;    // first residue in the concatenated per-residue arrays
-  int ch_off;     // first chain in the chain tables
-  int fm_items, fm_item_base;  // this replica's FM items: [fm_item_base, fm_item_base + fm_items)
-  long long scratch_off;  // doubles, global-memory scratch (when not LDS resident)
-  long long nl_off;       // ints, Verlet list block: 4 header ints + nl_cap pairs
-  long long nlx_off;      // doubles, list reference positions [n][3] + extents [n]
-  int nl_cap, pad2;
+  int n, nch;             // residues, chains
+  int fm_items, nl_cap;   // fragment-memory items; Verlet list capacity (pairs)
+  const int *rinfo;       // [n]
+  const double *aps;      // [n][12]
+  const int *ch_start, *ch_len;  // [nch]
+  double *x, *v, *f;      // [3][n][3] each (CA, CB, O)
+  double *energy;         // [AW_N_ET]
+  double *scratch;        // working slab in global memory (phase engine; replicas too large for LDS)
+  const void *items, *rf, *coef;  // FMItem/FMItemD [fm_items], float4/double4 Rf, weights
+  int *nl;                // Verlet list block: 4 header ints, nl_cap pairs, 4 nl_cap queue slots
+  double *nlx;            // [4 n]: CA positions and extents at the last list build
+  // dynamics of this replica in the current launch
+  double t_start, t_stop;      // target temperature, ramped linearly over the run
+  unsigned long long seed;     // random stream (seed, rng_id, step, atom, component)
+  long long step0;             // stream step index of the launch's first step
+  int ramp0, run_steps;        // steps of the run before this launch; length of the run
+  int nsteps;                  // steps of this replica in this launch
+  int rng_id;
+  int flags, pad;              // AW_RF_*
 };
