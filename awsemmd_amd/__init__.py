@@ -39,6 +39,9 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
            "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_phase_clock",
            "awsem_batch_list_stats", "awsem_batch_last_error",
+           "awsem_server_create", "awsem_server_submit", "awsem_server_pump", "awsem_server_start",
+           "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
+           "awsem_server_release", "awsem_server_stats", "awsem_server_last_error", "awsem_server_destroy",
            "awsem_batch_destroy"]
 
 
@@ -51,6 +54,11 @@ class AwsemError(RuntimeError):
 class Langevin(C.Structure):
     _fields_ = [("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double), ("damp", C.c_double),
                 ("seed", C.c_uint64), ("step0", C.c_int64)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_jobs", C.c_int), ("steps_per_round", C.c_int),
+                ("hbm_budget_bytes", C.c_double), ("dt", C.c_double), ("damp", C.c_double)]
 
 
 _lib = None
@@ -90,6 +98,19 @@ def lib():
         L.awsem_batch_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.awsem_batch_phase_clock.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.awsem_batch_list_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
+        u64 = C.c_uint64
+        L.awsem_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
+        L.awsem_server_submit.argtypes = [vp, vp, dp, dp, C.c_int, C.c_double, C.c_double, u64, C.POINTER(u64)]
+        L.awsem_server_pump.argtypes = [vp, ip, ip]
+        L.awsem_server_start.argtypes = [vp]
+        L.awsem_server_stop.argtypes = [vp]
+        L.awsem_server_poll.argtypes = [vp, u64, ip]
+        L.awsem_server_wait.argtypes = [vp, u64, C.c_double]
+        L.awsem_server_result.argtypes = [vp, u64, dp, dp, dp]
+        L.awsem_server_release.argtypes = [vp, u64]
+        L.awsem_server_stats.argtypes = [vp, dp]
+        L.awsem_server_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.awsem_server_destroy.argtypes = [vp]
         L.awsem_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
@@ -311,6 +332,94 @@ class Batch:
             out.append(flat[off * 9:(off + n) * 9].reshape(3, n, 3))
             off += n
         return out
+
+
+class Server:
+    """Continuous batching of annealing jobs (awsem_server_*): jobs are
+    admitted between rounds as job slots and device memory free up; every
+    round advances all active jobs in one resident launch."""
+
+    STATES = {0: "queued", 1: "running", 2: "done"}
+
+    def __init__(self, forcefield: FixBackbone, precision: int = PREC_FP32, max_jobs: int = 512,
+                 steps_per_round: int = 50, hbm_budget_bytes: float = 0.0, dt: float = 2.0, damp: float = 1000.0):
+        o = ServerOpts(precision, max_jobs, steps_per_round, hbm_budget_bytes, dt, damp)
+        h = C.c_void_p()
+        rc = lib().awsem_server_create(forcefield.h, C.byref(o), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.last_error()
+            self.close()
+            raise AwsemError(rc, msg)
+
+    def last_error(self) -> str:
+        if not self.h:
+            return ""
+        buf = C.create_string_buffer(1024)
+        lib().awsem_server_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AwsemError(rc, self.last_error())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().awsem_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, system: FixBackbone, xres: np.ndarray, vres: Optional[np.ndarray], nsteps: int,
+               t_start: float = 600.0, t_stop: float = 300.0, seed: int = 1) -> int:
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        v = None if vres is None else np.ascontiguousarray(vres, dtype=np.float64)
+        jid = C.c_uint64()
+        self._check(lib().awsem_server_submit(self.h, system.h, _dp(x), _dp(v), nsteps, t_start, t_stop,
+                                              seed, C.byref(jid)))
+        return jid.value
+
+    def pump(self):
+        """One scheduling round on this thread: (active, queued) afterwards."""
+        a, q = C.c_int(), C.c_int()
+        self._check(lib().awsem_server_pump(self.h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def start(self):
+        self._check(lib().awsem_server_start(self.h))
+
+    def stop(self):
+        self._check(lib().awsem_server_stop(self.h))
+
+    def poll(self, jid: int) -> int:
+        st = C.c_int()
+        self._check(lib().awsem_server_poll(self.h, jid, C.byref(st)))
+        return st.value
+
+    def wait(self, jid: int, timeout: float = 0.0):
+        self._check(lib().awsem_server_wait(self.h, jid, timeout))
+
+    def result(self, jid: int, n: int):
+        """(coordinates (3, n, 3), 19 energies, {queued, device, total} ms)."""
+        x = np.zeros((3, n, 3))
+        e = np.zeros(N_ET)
+        t = np.zeros(3)
+        self._check(lib().awsem_server_result(self.h, jid, _dp(x), _dp(e), _dp(t)))
+        return x, e, {"queued_ms": t[0], "device_ms": t[1], "total_ms": t[2]}
+
+    def release(self, jid: int):
+        self._check(lib().awsem_server_release(self.h, jid))
+
+    def stats(self) -> dict:
+        s = np.zeros(8)
+        self._check(lib().awsem_server_stats(self.h, _dp(s)))
+        keys = ["rounds", "jobs_done", "bytes_in_use", "peak_active", "replica_steps", "device_ms", "budget_bytes",
+                "active"]
+        return dict(zip(keys, s.tolist()))
 
 
 def pack_states(states: Sequence[np.ndarray]) -> np.ndarray:

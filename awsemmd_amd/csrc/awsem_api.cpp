@@ -166,6 +166,8 @@ struct awsem_batch {
   int n_kernels = 0;
 };
 
+const AwSystem *aw_ctx_system(const awsem_ctx *c) { return c ? &c->sys : nullptr; }
+
 static int fail(awsem_ctx *c, const AwStatus &s)
 {
   if (c) c->err = s.msg;

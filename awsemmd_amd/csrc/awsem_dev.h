@@ -152,7 +152,9 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     __syncthreads();
     if (slot >= B.nrep) break;
     int r = B.order[slot];
-    const ReplicaDesc R = B.rep[r];
+    // fields are read where used: holding the whole descriptor in registers
+    // across the step loop costs spills
+    const ReplicaDesc &R = B.rep[r];
     double *slab = LDSM ? smem : R.scratch;
     View<T> V = make_view<T>(B, R, slab);
     V.P = Pr;  // read-only, unaliased: uniform parameter loads go through the scalar cache

@@ -191,6 +191,52 @@ int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *p
 int awsem_batch_last_error(const awsem_batch *b, char *buf, size_t n);
 int awsem_batch_destroy(awsem_batch *b);
 
+/* ---- serving: continuous batching of annealing jobs ---------------------
+ * No counterpart in the reference (one LAMMPS run per process there).  Jobs
+ * queue on the host; a scheduling round admits queued jobs in arrival order
+ * while job slots and the device-memory budget allow, advances every active
+ * job by up to steps_per_round steps in one resident launch, and retires
+ * finished jobs (final coordinates and per-term energies copied back, device
+ * memory freed).  Each job runs fix nve + fix langevin with the target
+ * temperature ramped from t_start to t_stop over its nsteps, with its own
+ * random stream (seed), so its result does not depend on the other jobs in
+ * the launches beyond floating-point accumulation order. */
+typedef struct awsem_server awsem_server;
+typedef struct {
+  int precision;            /* AWSEM_PREC_FP32 or AWSEM_PREC_FP64 */
+  int max_jobs;             /* job slots on the device */
+  int steps_per_round;      /* steps an active job advances per round */
+  double hbm_budget_bytes;  /* device memory for job images; <= 0: 80% of free */
+  double dt, damp;          /* timestep and langevin damping (fs), all jobs */
+} awsem_server_opts;
+/* the force field (coeff + tables) of `forcefield` applies to every job */
+int awsem_server_create(const awsem_ctx *forcefield, const awsem_server_opts *opts,
+                        awsem_server **out);
+/* queues a job: `system` gives sequence, memories and ssweight; xres/vres the
+ * residue-layout start state (vres NULL = 0); returns its id */
+int awsem_server_submit(awsem_server *s, const awsem_ctx *system, const double *xres,
+                        const double *vres, int nsteps, double t_start, double t_stop,
+                        uint64_t seed, uint64_t *job_id);
+/* one scheduling round on the calling thread (not while started) */
+int awsem_server_pump(awsem_server *s, int *n_active, int *n_queued);
+/* a background thread runs rounds while jobs are queued or active */
+int awsem_server_start(awsem_server *s);
+int awsem_server_stop(awsem_server *s);
+/* state: 0 queued, 1 running, 2 done, < 0 failed with that AWSEM_E_* code */
+int awsem_server_poll(awsem_server *s, uint64_t job_id, int *state);
+/* blocks until the job is done or failed (timeout_s <= 0: no limit) */
+int awsem_server_wait(awsem_server *s, uint64_t job_id, double timeout_s);
+/* final residue-layout coordinates, 19 energies and times_ms = {queued,
+ * on device, total} in milliseconds */
+int awsem_server_result(awsem_server *s, uint64_t job_id, double *xres, double *energy,
+                        double *times_ms);
+int awsem_server_release(awsem_server *s, uint64_t job_id);
+/* stats[8]: rounds, jobs done, device bytes in use, peak active jobs,
+ * replica-steps, device ms, budget bytes, active jobs */
+int awsem_server_stats(awsem_server *s, double *stats);
+int awsem_server_last_error(awsem_server *s, char *buf, size_t n);
+int awsem_server_destroy(awsem_server *s);
+
 #ifdef __cplusplus
 }
 #endif
