@@ -113,10 +113,10 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   for (int r = 0; r < D->nrep; ++r) {
     const AwSystem &S = *sys[r];
     AwDevPlan plan;
-    AwStatus s = aw_devsys_plan(S, D->fp32, !D->lds, false, plan);
+    AwStatus s = aw_devsys_plan(S, nullptr, D->fp32, !D->lds, false, plan);
     if (!s.ok()) return s;
     std::unique_ptr<AwDevSystem> ds(new AwDevSystem());
-    if (!(s = aw_devsys_create(S, plan, *ds)).ok()) return s;
+    if (!(s = aw_devsys_create(plan, *ds)).ok()) return s;
     ReplicaDesc R = ds->d;
     R.x = D->d_x + 9LL * D->res_off[r];
     R.v = D->d_v + 9LL * D->res_off[r];

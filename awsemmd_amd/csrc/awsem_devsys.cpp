@@ -32,12 +32,19 @@ static size_t take(size_t &cur, size_t bytes)
   return o;
 }
 
-AwStatus aw_devsys_plan(const AwSystem &S, bool fp32, bool scratch, bool own_state, AwDevPlan &plan)
+template <class T>
+static void put(std::vector<unsigned char> &img, size_t off, const std::vector<T> &v)
+{
+  if (!v.empty()) std::memcpy(img.data() + off, v.data(), v.size() * sizeof(T));
+}
+
+AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool scratch, bool own_state, AwDevPlan &plan)
 {
   plan = AwDevPlan();
   plan.fp32 = fp32; plan.scratch = scratch; plan.own_state = own_state;
-  if (S.P.fm_flag) {
-    AwStatus s = aw_build_fm(S, plan.fm);
+  const FFParams &FP = P ? *P : S.P;
+  if (FP.fm_flag) {
+    AwStatus s = aw_build_fm(S, FP, plan.fm);
     if (!s.ok()) return s;
   }
   int n = S.n;
@@ -62,25 +69,15 @@ AwStatus aw_devsys_plan(const AwSystem &S, bool fp32, bool scratch, bool own_sta
   plan.off_nlx = take(cur, 4 * (size_t)n * sizeof(double));
   if (scratch) plan.off_scr = take(cur, (size_t)aw_slab_doubles(n) * sizeof(double));
   plan.bytes = std::max<size_t>(cur, 256);
-  return AwStatus();
-}
-
-template <class T>
-static void put(std::vector<unsigned char> &img, size_t off, const std::vector<T> &v)
-{
-  if (!v.empty()) std::memcpy(img.data() + off, v.data(), v.size() * sizeof(T));
-}
-
-AwStatus aw_devsys_create(const AwSystem &S, const AwDevPlan &plan, AwDevSystem &out)
-{
-  int n = S.n;
-  std::vector<unsigned char> img(plan.bytes, 0);
+  // the image: zeros (state, list headers, slab) and the packed constants
+  std::vector<unsigned char> &img = plan.image;
+  img.assign(plan.bytes, 0);
   put(img, plan.off_rinfo, S.rinfo);
   put(img, plan.off_aps, S.aps);
   put(img, plan.off_chs, S.ch_start);
   put(img, plan.off_chl, S.ch_len);
   const AwFMPack &fm = plan.fm;
-  if (plan.fp32) {
+  if (fp32) {
     put(img, plan.off_items, aw_fm_items<FMItem>(fm));
     put(img, plan.off_rf, std::vector<float>(fm.rf.begin(), fm.rf.end()));
     put(img, plan.off_coef, std::vector<float>(fm.coef.begin(), fm.coef.end()));
@@ -89,14 +86,26 @@ AwStatus aw_devsys_create(const AwSystem &S, const AwDevPlan &plan, AwDevSystem 
     put(img, plan.off_rf, fm.rf);
     put(img, plan.off_coef, fm.coef);
   }
+  plan.n_items = fm.n_items();
+  plan.n = n;
+  plan.nch = S.nch;
+  plan.fm = AwFMPack();  // packed into the image
+  return AwStatus();
+}
+
+
+AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out)
+{
+  int n = plan.n;
+  if (plan.image.size() != plan.bytes) return AwStatus::err(-10, "device image already consumed");
   DS_HIPCHK(hipMalloc(&out.arena, plan.bytes));
   out.bytes = plan.bytes;
-  DS_HIPCHK(hipMemcpy(out.arena, img.data(), plan.bytes, hipMemcpyHostToDevice));
+  DS_HIPCHK(hipMemcpy(out.arena, plan.image.data(), plan.bytes, hipMemcpyHostToDevice));
   unsigned char *b = (unsigned char *)out.arena;
   ReplicaDesc &R = out.d;
   std::memset(&R, 0, sizeof R);
-  R.n = n; R.nch = S.nch;
-  R.fm_items = fm.n_items();
+  R.n = n; R.nch = plan.nch;
+  R.fm_items = plan.n_items;
   R.nl_cap = plan.nl_cap;
   if (plan.own_state) {
     R.x = (double *)(b + plan.off_x);

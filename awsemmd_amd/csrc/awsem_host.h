@@ -62,7 +62,8 @@ struct AwFMPack {
   int n_items() const { return (int)(items.size() / 3); }
 };
 // appends one system's FM items (residue indices local to the system)
-AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B);
+// P: the force field (fragment-memory strength, sigma exponent, epsilon)
+AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B);
 template <class ItemT>
 std::vector<ItemT> aw_fm_items(const AwFMPack &B)
 {
@@ -84,10 +85,11 @@ void aw_default_pair_styles(FFParams &P);
 struct AwDevPlan {
   AwFMPack fm;
   bool fp32 = false, scratch = false, own_state = false;
-  int nl_cap = 0;
+  int nl_cap = 0, n = 0, nch = 0, n_items = 0;
   size_t off_x = 0, off_v = 0, off_f = 0, off_e = 0, off_rinfo = 0, off_aps = 0, off_chs = 0, off_chl = 0;
   size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0;
   size_t bytes = 0;
+  std::vector<unsigned char> image;  // host copy of the whole device image
 };
 struct AwDevSystem {
   void *arena = nullptr;
@@ -98,8 +100,10 @@ struct AwDevSystem {
   AwDevSystem(const AwDevSystem &) = delete;
   AwDevSystem &operator=(const AwDevSystem &) = delete;
 };
-AwStatus aw_devsys_plan(const AwSystem &S, bool fp32, bool scratch, bool own_state, AwDevPlan &plan);
-AwStatus aw_devsys_create(const AwSystem &S, const AwDevPlan &plan, AwDevSystem &out);
+// P: the force field to pack with (S.P when null)
+AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool scratch, bool own_state, AwDevPlan &plan);
+// allocates the device image and uploads plan.image (the plan's host copy)
+AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out);
 AwStatus upload_raw(void **dst, const void *src, size_t bytes);
 long long aw_slab_doubles(int n);
 

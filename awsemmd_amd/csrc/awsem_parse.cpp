@@ -604,9 +604,8 @@ AwStatus aw_finish_system(AwSystem &S)
 // (Fragment_Memory::Rf, fm.cpp:190-202); a CB entry of a glycine memory
 // residue is stored as -1 (skipped, fb.cpp:4148-4149).
 
-AwStatus aw_build_fm(const AwSystem &S, AwFMPack &B)
+AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B)
 {
-  const FFParams &P = S.P;
   std::map<std::pair<int, int>, std::vector<int>> by;
   for (size_t k = 0; k < S.mems.size(); ++k) by[{S.mems[k].pos, S.mems[k].len}].push_back((int)k);
   int min_sep = S.fmg.min_sep(), max_sep = S.fmg.max_sep();

@@ -3,9 +3,10 @@
 // Jobs (a system, its start coordinates, a number of steps and a
 // temperature ramp) queue on the host.  The scheduler works in rounds: it
 // admits queued jobs in arrival order while job slots and the device-memory
-// budget allow, uploads their device images, advances every active job by up
-// to steps_per_round steps in one resident-engine launch (jobs too large for
-// LDS in a second launch on a second stream, overlapping the first), then
+// budget allow, uploads their device images, advances every active job in
+// one resident-engine launch (jobs too large for LDS in a second launch on a
+// second stream, overlapping the first) by a number of steps inversely
+// proportional to its size, so that all workgroups finish together, then
 // retires finished jobs -- final coordinates and per-term energies copied
 // back, device memory freed -- which makes room for the next admissions.
 // A job's coordinates, velocities, forces and Verlet list stay on the device
@@ -49,10 +50,10 @@ enum JobState { JOB_QUEUED = 0, JOB_RUNNING = 1, JOB_DONE = 2 };
 
 struct Job {
   uint64_t id = 0;
-  AwSystem sys;
-  AwDevPlan plan;
+  int n = 0;  // residues
+  AwDevPlan plan;  // device layout and host image
   std::vector<double> x0, v0;
-  int nsteps = 0, done = 0;
+  int nsteps = 0, done = 0, round_steps = 0;
   double t_start = 300.0, t_stop = 300.0;
   uint64_t seed = 1;
   int state = JOB_QUEUED;  // or a negative AWSEM_E_* code
@@ -163,8 +164,9 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
   }
   for (Job *j : admit) {
     std::unique_ptr<AwDevSystem> ds(new AwDevSystem());
-    AwStatus s = aw_devsys_create(j->sys, j->plan, *ds);
-    size_t bytes = (size_t)j->sys.n * 9 * sizeof(double);
+    AwStatus s = aw_devsys_create(j->plan, *ds);
+    j->plan.image = std::vector<unsigned char>();  // uploaded
+    size_t bytes = (size_t)j->n * 9 * sizeof(double);
     if (s.ok() && (hipMemcpy(ds->d.x, j->x0.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
                    hipMemcpy(ds->d.v, j->v0.data(), bytes, hipMemcpyHostToDevice) != hipSuccess))
       s = AwStatus::err(AWSEM_E_DEVICE, "job upload");
@@ -189,18 +191,34 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
     peak_active = std::max(peak_active, (double)act.size());
   }
   if (!act.empty()) {
+    // steps of each job this round: a job of median cost runs steps_per_round
+    // steps, cheaper jobs proportionally more and dearer ones fewer, so that
+    // every workgroup of the launch is busy for about the same time and small
+    // jobs are not held back by the largest one (cost ~ residues, twice for
+    // jobs working from global memory)
+    std::vector<double> cost;
+    for (Job *j : act) cost.push_back((double)j->n * (j->n <= lds_max ? 1.0 : 2.0));
+    std::vector<double> sorted_cost = cost;
+    std::nth_element(sorted_cost.begin(), sorted_cost.begin() + sorted_cost.size() / 2, sorted_cost.end());
+    double c_med = sorted_cost[sorted_cost.size() / 2];
+    for (size_t k = 0; k < act.size(); ++k) {
+      Job *j = act[k];
+      double want = o.steps_per_round * c_med / cost[k];
+      int steps = (int)std::lround(std::min(want, 8.0 * o.steps_per_round));
+      j->round_steps = std::max(1, std::min(steps, j->nsteps - j->done));
+    }
     // descriptors: LDS-resident jobs first, then the rest; longest first in each
     std::vector<Job *> grp[2];
-    for (Job *j : act) grp[j->sys.n <= lds_max ? 0 : 1].push_back(j);
+    for (Job *j : act) grp[j->n <= lds_max ? 0 : 1].push_back(j);
     int base[2] = {0, (int)grp[0].size()};
     int max_n[2] = {0, 0};
     long long steps = 0;
     for (int g = 0; g < 2; ++g) {
-      std::stable_sort(grp[g].begin(), grp[g].end(), [](Job *a, Job *b) { return a->sys.n > b->sys.n; });
+      std::stable_sort(grp[g].begin(), grp[g].end(), [](Job *a, Job *b) { return a->n > b->n; });
       for (size_t k = 0; k < grp[g].size(); ++k) {
         Job *j = grp[g][k];
         ReplicaDesc R = j->dev->d;
-        R.nsteps = std::min(o.steps_per_round, j->nsteps - j->done);
+        R.nsteps = j->round_steps;
         R.ramp0 = j->done;
         R.run_steps = j->nsteps;
         R.step0 = j->done;
@@ -211,7 +229,7 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
         R.flags = (j->f_valid ? AW_RF_F_VALID : 0) | (j->fresh ? AW_RF_NL_RESET : 0);
         h_rep[base[g] + k] = R;
         h_order[base[g] + k] = (int)k;
-        max_n[g] = std::max(max_n[g], j->sys.n);
+        max_n[g] = std::max(max_n[g], j->n);
         steps += R.nsteps;
       }
     }
@@ -249,14 +267,14 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
     // retire
     std::vector<Job *> finished;
     for (Job *j : act) {
-      j->done += std::min(o.steps_per_round, j->nsteps - j->done);
+      j->done += j->round_steps;
       j->f_valid = true;
       j->fresh = false;
       if (j->done >= j->nsteps) finished.push_back(j);
     }
     for (Job *j : finished) {
-      size_t bytes = (size_t)j->sys.n * 9 * sizeof(double);
-      j->x_out.resize((size_t)j->sys.n * 9);
+      size_t bytes = (size_t)j->n * 9 * sizeof(double);
+      j->x_out.resize((size_t)j->n * 9);
       AwStatus s;
       if (hipMemcpy(j->x_out.data(), j->dev->d.x, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(j->energy, j->dev->d.energy, sizeof j->energy, hipMemcpyDeviceToHost) != hipSuccess)
@@ -343,8 +361,7 @@ int awsem_server_submit(awsem_server *s, const awsem_ctx *system, const double *
   if (!S || S->n <= 0) return AWSEM_E_ARG;
   if (S->n >= 32768) return sfail(s, AwStatus::err(AWSEM_E_ARG, "served systems are limited to 32767 residues each"));
   std::unique_ptr<Job> j(new Job());
-  j->sys = *S;
-  j->sys.P = s->P;  // the server's force field applies to every job
+  j->n = S->n;
   size_t n9 = (size_t)S->n * 9;
   for (size_t k = 0; k < n9; ++k)
     if (!std::isfinite(xres[k]) || (vres && !std::isfinite(vres[k])))
@@ -356,10 +373,11 @@ int awsem_server_submit(awsem_server *s, const awsem_ctx *system, const double *
   j->t_start = t_start;
   j->t_stop = t_stop;
   j->seed = seed;
-  AwStatus st = aw_devsys_plan(j->sys, s->fp32, S->n > s->lds_max, true, j->plan);
+  // the server's force field applies to every job; the image is built here,
+  // on the submitting thread, so that admission is an allocation and a copy
+  AwStatus st = aw_devsys_plan(*S, &s->P, s->fp32, S->n > s->lds_max, true, j->plan);
   if (!st.ok()) return sfail(s, st);
   if (j->plan.bytes > s->budget) return sfail(s, AwStatus::err(AWSEM_E_NOMEM, "job exceeds the device-memory budget"));
-  j->sys.mems.clear();  // packed into the plan; the image needs no more
   j->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

@@ -154,6 +154,71 @@ def cpu_baseline(systems, budget_s=15.0):
                       f"(median length), {tot_steps} steps in {t_used:.1f}s on one core"}
 
 
+def serving_run(A, W, ctx, n_jobs, rate_hz, seed, steps=500, round_steps=50, sigma=0.6):
+    """The serving mix: a Poisson stream of annealing jobs (600 -> 300 K over
+    `steps` steps, then the final per-term energies) with lengths lognormal
+    around a median of 150 residues (log-sd `sigma`), capped at 1500, each
+    generated like the single-chain workload with its own seed.  Jobs are
+    submitted at their arrival times to a background-scheduled server and
+    admitted by slots and device memory while others run."""
+    rng = np.random.default_rng(seed)
+    lengths = np.clip(np.round(rng.lognormal(np.log(150.0), sigma, n_jobs)), 20, 1500).astype(int)
+    jobs = [(int(L), seed * 7919 + k) for k, L in enumerate(lengths)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        systems = pool.map(_gen, jobs, chunksize=2)
+    fixes = [ctx(s, A.PREC_FP32) for s in systems]
+    log(f"serving: {n_jobs} jobs generated in {time.perf_counter() - t0:.1f}s "
+        f"(L median {int(np.median(lengths))}, max {lengths.max()})")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_jobs))
+    srv = A.Server(fixes[0], A.PREC_FP32, max_jobs=1024, steps_per_round=round_steps)
+    # warm-up job (kernel load, first allocations), not counted
+    w = srv.submit(fixes[0], systems[0].coords(), None, round_steps, seed=1)
+    srv.pump()
+    srv.result(w, systems[0].n)
+    srv.release(w)
+    srv.start()
+    # requests arrive on their own threads (the library packs a job's device
+    # image on the submitting thread); latency runs from the arrival time
+    from concurrent.futures import ThreadPoolExecutor
+
+    def submit(k):
+        t_arr = t_start + arrivals[k]
+        jid = srv.submit(fixes[k], systems[k].coords(), None, steps, t_start=600.0, t_stop=300.0, seed=1000 + k)
+        return jid, time.perf_counter() - t_arr
+
+    t_start = time.perf_counter()
+    futs = []
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        for k in range(n_jobs):
+            dt = t_start + arrivals[k] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(pool.submit(submit, k))
+        subs = [f.result() for f in futs]
+    t_sub = time.perf_counter() - t_start
+    lat, energies_ok = [], True
+    for (jid, t_pack), s in zip(subs, systems):
+        srv.wait(jid, 600.0)
+        x, e, t = srv.result(jid, s.n)
+        energies_ok &= bool(np.all(np.isfinite(e)))
+        lat.append(1e3 * t_pack + t["total_ms"])
+    t_all = time.perf_counter() - t_start
+    st = srv.stats()
+    srv.stop()
+    srv.close()
+    lat = np.array(lat)
+    return {"jobs": n_jobs, "arrival_rate_hz": rate_hz, "achieved_arrival_rate_hz": n_jobs / t_sub,
+            "steps_per_job": steps, "steps_per_round": round_steps,
+            "lengths": {"median": int(np.median(lengths)), "mean": float(lengths.mean()), "max": int(lengths.max()),
+                        "lognormal_sigma": sigma, "cap": 1500},
+            "jobs_per_s": n_jobs / t_all, "replica_steps_per_s": n_jobs * steps / t_all,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "peak_active_jobs": st["peak_active"], "rounds": st["rounds"], "device_ms": st["device_ms"],
+            "final_energies_finite": energies_ok, "wall_s": t_all}
+
+
 def calib_copy(nbytes):
     """Device-to-device copy of nbytes: the reference kernel for scaling the
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -188,6 +253,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--serving-jobs", type=int, default=512, help="jobs in the serving-mix run (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=1000.0, help="Poisson arrival rate of the serving mix (1/s)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (HBM counter calibration)")
     a = ap.parse_args()
@@ -288,6 +355,11 @@ def main():
 
     if a.calib_copy_mb:
         calib_copy(a.calib_copy_mb << 20)
+    serving = None
+    if a.serving_jobs > 0:
+        serving = serving_run(A, W, ctx, a.serving_jobs, a.serving_rate, a.seed + 1)
+        log(f"serving: {serving['jobs_per_s']:.1f} jobs/s, latency p50 {serving['latency_ms']['p50']:.1f} ms "
+            f"p99 {serving['latency_ms']['p99']:.1f} ms, peak {serving['peak_active_jobs']:.0f} jobs active")
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(systems, a.cpu_budget)
@@ -322,6 +394,7 @@ def main():
                      "kernel": "k_resident<float,LDS>", "kernel_avg_ms": k_avg_ms,
                      "algorithmic_bytes_per_batch_step": bytes_step},
         "cpu_baseline": cpu,
+        "serving": serving,
         "setup_s": {"generate": t_gen, "build": t_build},
     }
     print(json.dumps(out))

@@ -195,7 +195,7 @@ int awsem_batch_destroy(awsem_batch *b);
  * No counterpart in the reference (one LAMMPS run per process there).  Jobs
  * queue on the host; a scheduling round admits queued jobs in arrival order
  * while job slots and the device-memory budget allow, advances every active
- * job by up to steps_per_round steps in one resident launch, and retires
+ * job in one resident launch, and retires
  * finished jobs (final coordinates and per-term energies copied back, device
  * memory freed).  Each job runs fix nve + fix langevin with the target
  * temperature ramped from t_start to t_stop over its nsteps, with its own
@@ -205,7 +205,9 @@ typedef struct awsem_server awsem_server;
 typedef struct {
   int precision;            /* AWSEM_PREC_FP32 or AWSEM_PREC_FP64 */
   int max_jobs;             /* job slots on the device */
-  int steps_per_round;      /* steps an active job advances per round */
+  int steps_per_round;      /* steps per round of a job of median size; smaller
+                               jobs run proportionally more (up to 8x), larger
+                               ones fewer (at least 1) */
   double hbm_budget_bytes;  /* device memory for job images; <= 0: 80% of free */
   double dt, damp;          /* timestep and langevin damping (fs), all jobs */
 } awsem_server_opts;
