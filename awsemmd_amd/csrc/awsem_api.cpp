@@ -44,9 +44,16 @@ struct DevBatch {
   int list_cap = 0;
   FFParams P;
   double last_ms = 0.0;
+  // the resident launch sequence captured as a graph, replayed per call;
+  // recaptured when the kernel arguments (mode, dt, damp) change
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;
+  int g_mode = -1;
+  double g_dt = 0.0, g_damp = 0.0;
 
   ~DevBatch()
   {
+    if (graph) (void)hipGraphExecDestroy(graph);
     void *ptrs[] = {d_rep, d_params, d_queue, d_order, d_count, d_x, d_v, d_f, d_e, d_list, d_prof};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     sys.clear();
@@ -98,6 +105,7 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   for (const AwSystem *S : sys) { D->res_off.push_back(res); res += S->n; D->max_n = std::max(D->max_n, S->n); }
   D->total_res = res;
   D->lds = want_lds && D->max_n <= aw_lds_max_residues();
+  if (want_lds) aw_resident_prepare();
   HIPCHK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&D->ev0));
   HIPCHK(hipEventCreate(&D->ev1));
@@ -609,7 +617,24 @@ static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
   D.nl_dirty = false;
   D.f_valid = false;
   (void)hipEventRecord(D.ev0, D.stream);
-  if (aw_launch_resident(D.ptrs(), run, D.stream)) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
+  if (D.use_graph) {
+    if (!D.graph || D.g_mode != run.mode || D.g_dt != run.dt || D.g_damp != run.damp) {
+      if (D.graph) { (void)hipGraphExecDestroy(D.graph); D.graph = nullptr; }
+      hipGraph_t g = nullptr;
+      if (hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph capture"));
+      int lrc = aw_launch_resident(D.ptrs(), run, D.stream);
+      hipError_t ce = hipStreamEndCapture(D.stream, &g);
+      if (lrc || ce != hipSuccess || !g) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph capture of the resident launch"));
+      hipError_t ie = hipGraphInstantiate(&D.graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) { D.graph = nullptr; return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph instantiate")); }
+      D.g_mode = run.mode; D.g_dt = run.dt; D.g_damp = run.damp;
+    }
+    if (hipGraphLaunch(D.graph, D.stream) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph launch"));
+  } else if (aw_launch_resident(D.ptrs(), run, D.stream)) {
+    return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
+  }
   (void)hipEventRecord(D.ev1, D.stream);
   hipError_t e = hipEventSynchronize(D.ev1);
   if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));
@@ -637,6 +662,85 @@ int awsem_batch_run(awsem_batch *b, int nsteps, const awsem_langevin *lg)
   std::memset(&run, 0, sizeof run);
   run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.damp = lg->damp;
   return batch_launch(b, run, lg);
+}
+
+int awsem_batch_use_graph(awsem_batch *b, int on)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  b->dev->use_graph = on != 0;
+  return AWSEM_OK;
+}
+
+int awsem_batch_set_skin(awsem_batch *b, double skin)
+{
+  if (!b || !b->dev || !(skin > 0.0) || skin > 20.0) return AWSEM_E_ARG;
+  DevBatch &D = *b->dev;
+  D.P.nl_skin = skin;
+  if (hipMemcpy(D.d_params, &D.P, sizeof(FFParams), hipMemcpyHostToDevice) != hipSuccess)
+    return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "parameter upload"));
+  D.nl_dirty = true;  // lists were built with the old skin
+  return AWSEM_OK;
+}
+
+// Times short dynamics runs of this batch for each (skin, steps per launch)
+// candidate, from the current state each time, and keeps the fastest skin.
+// The state (x, v, f) is restored afterwards.
+int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_steps, double *skin_out,
+                         int *steps_per_launch_out, double *ms_per_step_out)
+{
+  if (!b || !b->dev || !lg || probe_steps <= 0) return AWSEM_E_ARG;
+  DevBatch &D = *b->dev;
+  size_t bytes = (size_t)D.total_res * 9 * sizeof(double);
+  double *save = nullptr;
+  if (hipMalloc((void **)&save, 3 * std::max<size_t>(bytes, 8)) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_NOMEM, "autotune"));
+  auto copy3 = [&](bool to_save) {
+    double *src[3] = {D.d_x, D.d_v, D.d_f};
+    for (int k = 0; k < 3; ++k) {
+      double *a = save + k * (bytes / sizeof(double)), *c = src[k];
+      (void)hipMemcpyAsync(to_save ? a : c, to_save ? c : a, bytes, hipMemcpyDeviceToDevice, D.stream);
+    }
+    (void)hipStreamSynchronize(D.stream);
+  };
+  copy3(true);
+  bool fv0 = D.f_valid;
+  double skin0 = D.P.nl_skin;
+  const double skins[] = {1.0, 1.5, 2.0, 2.5, 3.0};
+  const int chunks[] = {5, 10, 20, 40};
+  double best = 1e300, best_skin = skin0;
+  int best_chunk = 10;
+  int rc = AWSEM_OK;
+  for (double sk : skins) {
+    if ((rc = awsem_batch_set_skin(b, sk)) != AWSEM_OK) break;
+    for (int ch : chunks) {
+      copy3(false);
+      D.f_valid = fv0;
+      D.nl_dirty = true;
+      awsem_langevin l = *lg;
+      if ((rc = awsem_batch_run(b, ch, &l)) != AWSEM_OK) break;  // warm-up: lists, caches
+      double ms = 0.0;
+      int done = 0;
+      while (done < probe_steps) {
+        l.step0 += ch;
+        if ((rc = awsem_batch_run(b, ch, &l)) != AWSEM_OK) break;
+        ms += b->ms_total;
+        done += ch;
+      }
+      if (rc != AWSEM_OK) break;
+      double per = ms / done;
+      if (per < best) { best = per; best_skin = sk; best_chunk = ch; }
+    }
+    if (rc != AWSEM_OK) break;
+  }
+  copy3(false);
+  (void)hipFree(save);
+  D.f_valid = fv0;
+  int rc2 = awsem_batch_set_skin(b, rc == AWSEM_OK ? best_skin : skin0);
+  if (rc != AWSEM_OK) return rc;
+  if (rc2 != AWSEM_OK) return rc2;
+  if (skin_out) *skin_out = best_skin;
+  if (steps_per_launch_out) *steps_per_launch_out = best_chunk;
+  if (ms_per_step_out) *ms_per_step_out = best;
+  return AWSEM_OK;
 }
 
 int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel_avg, int *n_kernels)

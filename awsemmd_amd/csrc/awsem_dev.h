@@ -46,6 +46,9 @@ struct BatchDev {
   unsigned long long *prof;   // optional phase clock: [2][16] (first replica, all replicas)
 };
 
+// bytes of a workgroup's LDS left to the resident kernel's static arrays
+#define AW_LDS_STATIC_RESERVE 256
+
 // doubles of per-replica working storage (x, xv, f, 10 scalars, ext, rinfo copy)
 __host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + 2LL * ((n + 1) / 2) + 8; }
 
@@ -212,6 +215,16 @@ __global__ void k_zero_int(int *p)
   if (threadIdx.x == 0) *p = 0;
 }
 
+template <class T>
+static void prepare_resident_t()
+{
+  int dev = 0, lim = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
+  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lim - AW_LDS_STATIC_RESERVE);
+}
+
 // launches the resident engine over p.rep[0..nrep) (descriptors already on
 // the device, p.queue zeroed by the launcher)
 template <class T>
@@ -228,7 +241,8 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   int grid = run.grid > 0 ? run.grid : p.nrep;
   size_t lds = p.lds_resident ? (size_t)slab_doubles(p.max_n) * sizeof(double) : 0;
   if (p.lds_resident) {
-    (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // the dynamic-LDS limit is raised once, outside any graph capture
+    // (aw_resident_prepare)
     hipLaunchKernelGGL((k_resident<T, true, 512>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
   } else {
     hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);

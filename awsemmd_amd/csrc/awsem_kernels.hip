@@ -220,7 +220,7 @@ int aw_lds_max_residues()
     int dev = 0, lim = 0;
     hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
-    long long avail = lim / 8 - 16;
+    long long avail = (lim - AW_LDS_STATIC_RESERVE) / 8;  // dynamic LDS, doubles
     int n = 1;
     while (slab_doubles(n + 1) <= avail) ++n;
     g_lds_max = n;
@@ -232,6 +232,18 @@ long long aw_slab_doubles(int n) { return slab_doubles(n); }
 
 int aw_launch_resident_f32(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
 int aw_launch_resident_f64(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
+
+void aw_resident_prepare_f32();
+void aw_resident_prepare_f64();
+
+void aw_resident_prepare()
+{
+  static bool done = false;
+  if (done) return;
+  aw_resident_prepare_f32();
+  aw_resident_prepare_f64();
+  done = true;
+}
 
 int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 {

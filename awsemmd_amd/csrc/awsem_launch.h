@@ -24,6 +24,9 @@ struct AwRun {
 
 int aw_lds_max_residues();
 long long aw_slab_doubles(int n);
+// raises the resident kernels' dynamic-LDS limit (once per process, before
+// any launch or graph capture)
+void aw_resident_prepare();
 int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
 int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
 int aw_launch_list(const AwBatchPtrs &p, int r, int kind, int arg, long long *out, int cap, int *count, hipStream_t st);

@@ -202,6 +202,7 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
   }
   P.pair_list_cutoff = plc;
   aw_default_pair_styles(P);
+  P.nl_skin = 2.0;
   return AwStatus();
 }
 

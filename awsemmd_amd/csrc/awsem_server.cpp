@@ -127,6 +127,7 @@ AwStatus awsem_server::init(const FFParams &p, const awsem_server_opts &opts)
   o = opts;
   fp32 = o.precision == AWSEM_PREC_FP32;
   lds_max = aw_lds_max_residues();
+  aw_resident_prepare();
   if (o.hbm_budget_bytes > 0) {
     budget = (size_t)o.hbm_budget_bytes;
   } else {

@@ -967,20 +967,19 @@ AW_DEV bool near_pair(const View<T> &V, int i, int j, float reach)
 // growth is below skin/2; the passes still apply each term's exact
 // predicate.  Rebuilt on demand (phase_nl_build) by the resident engine.
 
-#define AW_NL_SKIN 2.0f
 
 // true when the list must be rebuilt (uniform; after phase A's barrier)
 template <class T>
 AW_DEV bool nl_needs_build(const View<T> &V)
 {
-  return V.nl && (V.nl[1] == 0 || i2f(V.nl_state[0]) > 0.5f * AW_NL_SKIN);
+  return V.nl && (V.nl[1] == 0 || i2f(V.nl_state[0]) > 0.5f * (float)V.P->nl_skin);
 }
 
 template <class T>
 AW_DEV void phase_nl_build(const View<T> &V, int t0, int ts)
 {
   int n = V.n, np = pair_count(n);
-  float reach = (float)V.P->pair_reach + AW_NL_SKIN;
+  float reach = (float)V.P->pair_reach + (float)V.P->nl_skin;
   for (int k = t0; k < np; k += ts) {
     int i, j;
     pair_decode(k, n, i, j);

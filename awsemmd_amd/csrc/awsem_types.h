@@ -72,6 +72,7 @@ struct FFParams {
   double bond_k, bond_r0[5];
   // derived cutoffs
   double pair_list_cutoff;   // src/fix_backbone.cpp:1048-1066
+  double nl_skin;            // Verlet-list skin of the resident engine (A)
   double pair_reach;         // largest pair cutoff of any term (density, DSSP, P_AP, EV)
   double dens_reach;         // largest density-well outer cutoff (water/helix)
   double ev_reach;           // largest excluded-volume cutoff

@@ -183,6 +183,19 @@ int awsem_batch_timing(const awsem_batch *b, double *ms_total, double *ms_kernel
  * B local terms, B residue pairs, pair-list check/build, 4 unused.
  * AWSEM_E_STATE otherwise. */
 int awsem_batch_phase_clock(const awsem_batch *b, uint64_t *ticks32);
+/* Replays the resident launch from a captured HIP graph (default on; the
+ * graph is recaptured when mode, dt or damp change). */
+int awsem_batch_use_graph(awsem_batch *b, int on);
+/* Verlet-list skin (Angstrom, default 2): lists hold residue pairs within
+ * the largest term cutoff + extents + skin and are rebuilt when a residue
+ * has moved by more than skin/2.  Any value gives the same forces. */
+int awsem_batch_set_skin(awsem_batch *b, double skin);
+/* Autotuner: times probe_steps of dynamics (from the current state, which
+ * is restored afterwards) for each skin in {1, 1.5, 2, 2.5, 3} and steps per
+ * launch in {5, 10, 20, 40}; keeps the fastest skin and reports it with the
+ * best steps per launch and its ms per batch-step. */
+int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_steps,
+                         double *skin, int *steps_per_launch, double *ms_per_step);
 /* Diagnostic: Verlet residue-pair lists of the resident engine summed over
  * replicas: list builds since creation, pairs in the current lists, and
  * replicas whose current list is invalid (overflowed or never built). */
