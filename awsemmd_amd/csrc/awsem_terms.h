@@ -156,8 +156,8 @@ struct View {
   int *nl;               // global: [0] count, [1] valid, [2] builds, [3] -, then (i<<16 | j)
   int nl_cap;
   double *nlx;           // global: CA positions [n][3] and extents [n] at the last build
-  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..4] queue counts
-  int *wq;               // global work queues beside a valid list: DSSP [2 cap], P_AP [cap], EV [cap]
+  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts
+  int *wq;               // global work queues beside the list: DSSP [2 cap], P_AP [cap], EV [cap], density pairs [cap]
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
   // f[3][n][3], ten per-residue scalars, ext (float)
   AW_DEV double *x() const { return s; }
@@ -1007,7 +1007,7 @@ AW_DEV void nl_publish(const View<T> &V, bool built)
   }
   V.nl_state[0] = 0;
   V.nl_state[1] = 0;
-  V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
+  V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = V.nl_state[5] = 0;
 }
 
 // iterate the residue pairs of a pass: the Verlet list when valid, else all
@@ -1037,6 +1037,23 @@ AW_DEV void for_pairs(const View<T> &V, int t0, int ts, F &&body)
   for_pairs_range(V, 0, pair_total(V), t0, ts, body);
 }
 
+// pairs for the water-xi and force passes: the density-pair queue filled by
+// phase B's list scan (resident engine, valid list), else every pair
+template <class T, class F>
+AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
+{
+  const int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
+  const int *lst = (V.nl && V.nl[1]) ? V.nl + 4 : nullptr;
+  int total = qs ? V.nl_state[5] : pair_total(V);
+  int n = V.n;
+  for (int k = t0; k < total; k += ts) {
+    int i, j;
+    if (qs || lst) { int c = qs ? qs[k] : lst[k]; i = c >> 16; j = c & 0xffff; }
+    else pair_decode(k, n, i, j);
+    body(i, j);
+  }
+}
+
 // ------------------------------------------------------------------------
 // Phase B
 
@@ -1059,7 +1076,8 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
 // that pass their cutoffs are queued and evaluated afterwards in dense
 // passes; otherwise they are evaluated in place.
 template <class T>
-AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr)
+AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr,
+                   int *qs = nullptr)
 {
   const FFParams *P = V.P;
   // CA-CA distance (FP32) against each term's reach (the near_pair test)
@@ -1075,6 +1093,12 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
   if (!same || dres > 1) {
     const double *si = site(V, i, ri), *sj = site(V, j, rj);
     float e = rsq_f(si, sj);
+    // the pairs the water-xi and force passes (phases D, F) can use
+    if (qs) {
+      float dr = (float)P->dens_reach;
+      int q = q_push(&V.nl_state[5], e < dr * dr * (1.0f + AW_RSQ_MARGIN));
+      if (q >= 0) qs[q] = (i << 16) | j;
+    }
     bool wc = P->water_flag && e > (float)P->w_rmin_th_sq[0] * (1.0f - AW_RSQ_MARGIN) &&
               e < (float)P->w_rmax_th_sq[0] * (1.0f + AW_RSQ_MARGIN);
     bool hc = P->helix_flag && e > (float)P->h_rmin_th_sq * (1.0f - AW_RSQ_MARGIN) &&
@@ -1163,7 +1187,10 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part)
       int total = pair_total(V);
       for (int c0 = 0; c0 < total && cap > 0; c0 += cap) {
         int c1 = total - c0 < cap ? total : c0 + cap;
-        for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe); });
+        // density pairs are kept for phases D and F when the list holds
+        // every pair (one chunk)
+        int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
+        for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe, qs); });
         AW_BARRIER();
         int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
         for (int k = t0; k < nd; k += ts) { int c = qd[k]; dssp_term(V, c >> 16, c & 0xffff, E); }
@@ -1294,7 +1321,7 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts)
   int n = V.n;
   int nh = P->helix_flag ? n : 0;
   for (int k = t0; k < nh; k += ts) helix_item(V, k, E);
-  if (P->water_flag) for_pairs(V, t0, ts, [&](int i, int j) { xi_pair(V, i, j); });
+  if (P->water_flag) for_density_pairs(V, t0, ts, [&](int i, int j) { xi_pair(V, i, j); });
   eflush(E, V.ered);
 }
 
@@ -1377,7 +1404,7 @@ AW_DEV void phase_F(const View<T> &V, int t0, int ts)
 {
   EAcc E;
   E.clear();
-  for_pairs(V, t0, ts, [&](int i, int j) { force_pair(V, i, j, E); });
+  for_density_pairs(V, t0, ts, [&](int i, int j) { force_pair(V, i, j, E); });
   eflush(E, V.ered);
 }
 

@@ -116,7 +116,7 @@ struct ReplicaDesc {
   double *energy;         // [AW_N_ET]
   double *scratch;        // working slab in global memory (phase engine; replicas too large for LDS)
   const void *items, *rf, *coef;  // FMItem/FMItemD [fm_items], float4/double4 Rf, weights
-  int *nl;                // Verlet list block: 4 header ints, nl_cap pairs, 4 nl_cap queue slots
+  int *nl;                // Verlet list block: 4 header ints, nl_cap pairs, 5 nl_cap queue slots
   double *nlx;            // [4 n]: CA positions and extents at the last list build
   // dynamics of this replica in the current launch
   double t_start, t_stop;      // target temperature, ramped linearly over the run

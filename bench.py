@@ -247,7 +247,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--chunk", type=int, default=10, help="batch-steps per resident launch")
+    ap.add_argument("--chunk", type=int, default=0, help="batch-steps per resident launch (0: autotuned)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default Verlet skin (2 A)")
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -291,9 +292,18 @@ def main():
     batch.set_state(x0, v0)
     lg = dict(dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=a.seed)
 
-    # warm-up
+    # warm-up, then the autotuner (skin, steps per launch) from the warmed state
     batch.run(a.warmup, step0=0, **lg)
     log(f"warm-up done ({a.warmup} steps)")
+    tuned = None
+    if not a.no_autotune:
+        t0 = time.perf_counter()
+        tuned = batch.autotune(probe_steps=40, step0=a.warmup, **lg)
+        tuned["tuning_s"] = time.perf_counter() - t0
+        log(f"autotuned: skin {tuned['skin']} A, {tuned['steps_per_launch']} steps per launch "
+            f"({tuned['ms_per_step']:.3f} ms/step) in {tuned['tuning_s']:.1f}s")
+    if a.chunk <= 0:
+        a.chunk = tuned["steps_per_launch"] if tuned else 10
     # timed region: K batch-steps in launches of `chunk` steps
     chunks = []
     kernel_ms = []
@@ -379,7 +389,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "256 chains, L~U[50,400], 20 memories per 9-residue window, Langevin 300 K, dt 2 fs",
                    "replicas": a.replicas, "residues": res_total, "memories": int(sum(len(s.memories) for s in systems)),
-                   "steps_per_launch": a.chunk, "seed": a.seed},
+                   "steps_per_launch": a.chunk, "seed": a.seed,
+                   "verlet_skin_A": tuned["skin"] if tuned else 2.0, "autotuned": tuned is not None},
         "plain_fp64_gpu": {"value": plain_value, "unit": "replica-steps/s",
                            "sample": f"{len(sample)} chains x {a.plain_steps} steps, phase engine, one chain at a time"},
         "speedup_vs_plain_fp64": value / plain_value,
