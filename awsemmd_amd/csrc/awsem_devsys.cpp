@@ -49,7 +49,7 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   }
   int n = S.n;
   long long allp = (long long)n * (n - 1) / 2;
-  plan.nl_cap = (int)std::min<long long>(std::max<long long>(allp, 1), 128LL * n);
+  plan.nl_cap = (int)std::min<long long>(std::max<long long>(allp, 1), 256LL * n);
   size_t tsz = fp32 ? sizeof(float) : sizeof(double);
   size_t cur = 0;
   if (own_state) {

@@ -86,6 +86,17 @@ def _rand_unit(rng):
             return v / nv
 
 
+def _rand_units(rng, m):
+    u = rng.normal(size=(m, 3))
+    return u / np.linalg.norm(u, axis=1, keepdims=True)
+
+
+def _min_dist(c, pts):
+    """min over pts of |c_k - p| for each candidate c_k."""
+    d2 = (c * c).sum(1)[:, None] + (pts * pts).sum(1)[None, :] - 2.0 * c @ pts.T
+    return np.sqrt(np.maximum(d2.min(axis=1), 0.0))
+
+
 def saw_trace(n: int, rng, start=None, bond=3.8, excl=4.6, excl13=5.0, others: Optional[np.ndarray] = None,
               center=None, radius=None) -> np.ndarray:
     """Self-avoiding C-alpha walk: |CA_i - CA_j| >= excl for |i-j|>=3 (and
@@ -96,7 +107,23 @@ def saw_trace(n: int, rng, start=None, bond=3.8, excl=4.6, excl13=5.0, others: O
     fails = 0
     while i < n:
         ok = False
-        for _ in range(200):
+        if radius is not None:
+            # confined (compact) walks: candidates tested in growing batches,
+            # the first admissible one taken
+            for m in (4, 16, 64, 256):
+                c = pts[i - 1] + bond * _rand_units(rng, m)
+                good = np.linalg.norm(c - center, axis=1) <= radius
+                if i >= 2:
+                    good &= np.linalg.norm(c - pts[i - 2], axis=1) >= excl13
+                if i >= 3 and good.any():
+                    good &= _min_dist(c, pts[: i - 2]) >= excl
+                if others is not None and len(others) and good.any():
+                    good &= _min_dist(c, others) >= excl
+                k = np.flatnonzero(good)
+                if len(k):
+                    cand, ok = c[k[0]], True
+                    break
+        for _ in range(0 if radius is not None else 200):
             cand = pts[i - 1] + bond * _rand_unit(rng)
             if radius is not None and np.linalg.norm(cand - center) > radius:
                 continue
@@ -114,6 +141,14 @@ def saw_trace(n: int, rng, start=None, bond=3.8, excl=4.6, excl13=5.0, others: O
             fails = 0
         else:
             fails += 1
+            if fails > 40:
+                # trapped (crowded confinement): restart the chain from a
+                # shifted start with a 5% larger confining sphere
+                pts[0] = pts[0] + rng.normal(size=3)
+                if radius is not None:
+                    radius *= 1.05
+                i, fails = 1, 0
+                continue
             i = max(1, i - 1 - min(fails, 8))  # backtrack
     return pts
 

@@ -241,6 +241,118 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def force_field(A, W, system):
+    """Shipped coeff with fragment memory on, parameter tables in a temp dir;
+    returns ctx(system, precision) -> FixBackbone."""
+    coeff_dir = tempfile.mkdtemp(prefix="awsem_bench_")
+    W.write_system(system, coeff_dir, fm=False)  # parameter tables
+    coeff = os.path.join(coeff_dir, "fix_backbone_coeff.data")
+    with open(coeff, "w") as f:
+        f.write(W.coeff_text(fm=True, mem_file="unused.mem", gamma_file="uniform.gamma"))
+
+    def ctx(s, prec):
+        return A.FixBackbone.from_arrays(coeff, coeff_dir, precision=prec, **W.arrays(s))
+    return ctx
+
+
+def run_single_chain(a, A, W):
+    """Plain FP64 GPU drop-in on one N=1000 chain: `steps` Langevin steps
+    through awsem_run (phase engine), and every `check_every` steps the FP64
+    per-term energies and total forces at the current coordinates against the
+    oracle (per-term forces every 10th check)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    s = W.make_system([a.n_single], seed=a.seed)
+    d = tempfile.mkdtemp(prefix="awsem_c1_")
+    p = W.write_system(s, d)
+    fix = A.FixBackbone(p["coeff"], p["seq"], d, A.PREC_FP64)
+    o = oracle.Oracle(p["coeff"], p["seq"], d)
+    x = s.coords()
+    v = velocities(s, 300.0, 3)
+    terms = ["Chain", "Chi", "Rama", "Excluded", "DSSP", "P_AP", "Water", "Burial", "Helix", "Frag_Mem", "Bond"]
+    worst_e, worst_f, worst_tf, t_gpu, done, checks = 0.0, 0.0, 0.0, 0.0, 0, 0
+    while done < a.steps:
+        k = min(a.check_every, a.steps - done)
+        t0 = time.perf_counter()
+        x, v, f, e = fix.run(x, v, k, dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=a.seed, step0=done)
+        t_gpu += time.perf_counter() - t0
+        done += k
+        fg, eg = fix.compute_res(x)
+        fo, eo = o.compute(x)
+        ftot = np.linalg.norm(fo.reshape(-1, 3), axis=1).max()
+        worst_f = max(worst_f, rel_force_err(fg, fo))
+        worst_e = max(worst_e, float(np.max(np.abs(eg - eo) / np.maximum(1.0, np.abs(eo)))))
+        if checks % 10 == 0:
+            for name in terms:
+                m = 1 << A.ET[name]
+                fix.set_term_mask(m)
+                ft, _ = fix.compute_res(x)
+                fot, _ = o.compute(x, m)
+                dd = np.linalg.norm((ft - fot).reshape(-1, 3), axis=1).max()
+                mm = max(np.linalg.norm(fot.reshape(-1, 3), axis=1).max(), 1e-4 * ftot)
+                worst_tf = max(worst_tf, dd / mm)
+            fix.set_term_mask(0xFFFFFFFF)
+        checks += 1
+        if checks % 10 == 0:
+            log(f"c1: {done} steps, worst rel energy err {worst_e:.2e}, force err {worst_f:.2e}, per-term {worst_tf:.2e}")
+    return {"metric": "plain FP64 GPU drop-in, one chain, steps/s (oracle-checked)", "value": done / t_gpu,
+            "unit": "steps/s", "n_gpus": 1, "steps": done, "warmup": 0, "ms_per_step": 1e3 * t_gpu / done,
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"single chain N={a.n_single}, FP64 end-to-end, Langevin 300 K, dt 2 fs",
+                       "check_every": a.check_every, "checks": checks},
+            "ns_per_day": done / t_gpu * 2.0 * 86400 / 1e6,
+            "max_rel_energy_err_vs_oracle": worst_e, "max_rel_force_err_vs_oracle": worst_f,
+            "max_rel_per_term_force_err_vs_oracle": worst_tf}
+
+
+def run_multimer(a, A, W):
+    """64 replicas of one 8 x 300-residue compact multimer (centroids 10-15 A
+    apart, seeded), each with its own random stream, FP32 pair arithmetic,
+    annealing 600 -> 300 K; a bounded number of steps is timed, with the
+    annealing ramp laid out over the full run (a.anneal_steps)."""
+    t0 = time.perf_counter()
+    complex_ = _gen_multimer(a.seed * 31)
+    systems = [complex_] * a.replicas
+    log(f"c3: multimer of {complex_.n} residues generated in {time.perf_counter() - t0:.1f}s, {a.replicas} replicas")
+    ctx = force_field(A, W, systems[0])
+    batch = A.Batch([ctx(s, A.PREC_FP32) for s in systems], A.PREC_FP32)
+    x0 = A.pack_states([s.coords() for s in systems])
+    batch.set_state(x0, np.zeros_like(x0))
+    lg = dict(dt=2.0, damp=1000.0, seed=a.seed)
+    ramp = lambda step: 600.0 - 300.0 * step / a.anneal_steps
+    batch.run(a.warmup, t_start=ramp(0), t_stop=ramp(a.warmup), step0=0, **lg)
+    tuned = None if a.no_autotune else batch.autotune(probe_steps=20, t_start=ramp(a.warmup), t_stop=ramp(a.warmup),
+                                                       step0=a.warmup, **lg)
+    chunk = a.chunk if a.chunk > 0 else (tuned["steps_per_launch"] if tuned else 10)
+    step, ms = a.warmup, []
+    t0 = time.perf_counter()
+    while step < a.warmup + a.steps:
+        k = min(chunk, a.warmup + a.steps - step)
+        c0 = time.perf_counter()
+        batch.run(k, t_start=ramp(step), t_stop=ramp(step + k), step0=step, **lg)
+        ms.append(1e3 * (time.perf_counter() - c0) / k)
+        step += k
+    t = time.perf_counter() - t0
+    _, _, _, e = batch.get_state()
+    value = a.replicas * a.steps / t
+    return {"metric": "replica-steps/s, multimer annealing batch", "value": value, "unit": "replica-steps/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * t / a.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 pair math, f64 accumulation/integration", "data": "synthetic",
+            "config": {"workload": "64 replicas of an 8-chain x 300-residue compact complex, centroids 10-15 A "
+                                   "apart, annealing 600->300 K", "replicas": a.replicas, "residues_per_replica": systems[0].n,
+                       "anneal_steps": a.anneal_steps, "steps_per_launch": chunk,
+                       "verlet_skin_A": tuned["skin"] if tuned else 2.0},
+            "ns_per_day_per_replica": value / a.replicas * 2.0 * 86400 / 1e6,
+            "latency_ms_per_batch_step": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
+            "final_energies_finite": bool(np.all(np.isfinite(e)))}
+
+
+def _gen_multimer(seed):
+    from awsemmd_amd import workload as W
+    return W.make_system([300] * 8, seed=seed, compact=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +368,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--serving-jobs", type=int, default=512, help="jobs in the serving-mix run (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=1000.0, help="Poisson arrival rate of the serving mix (1/s)")
+    ap.add_argument("--workload", choices=["batch", "single", "multimer"], default="batch",
+                    help="batch: 256 chains (default, the headline); single: one N=1000 chain through the "
+                         "FP64 drop-in, oracle-checked; multimer: 64 x (8 x 300) annealing")
+    ap.add_argument("--n-single", type=int, default=1000)
+    ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--anneal-steps", type=int, default=20000)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (HBM counter calibration)")
     a = ap.parse_args()
@@ -267,6 +385,17 @@ def main():
     import awsemmd_amd as A
     from awsemmd_amd import workload as W
     A.lib()
+
+    if a.workload == "single":
+        if a.steps == 200:
+            a.steps = 10000
+        print(json.dumps(run_single_chain(a, A, W)), flush=True)
+        return
+    if a.workload == "multimer":
+        if a.replicas == 256:
+            a.replicas = 64
+        print(json.dumps(run_multimer(a, A, W)), flush=True)
+        return
 
     t0 = time.perf_counter()
     systems = make_workload(a.replicas, a.seed)

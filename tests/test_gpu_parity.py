@@ -157,3 +157,24 @@ def test_batch_fp64_dynamics_matches_oracle(oracle_mod):
         dv = np.abs(vs[r] - vo).max() / np.abs(vo).max()
         print(f"replica {r}: max|dx|={dx:.2e} A, rel dv={dv:.2e}")
         assert dx < 1e-8 and dv < 1e-7, r
+
+
+def test_plain_fp64_run_checked_along_trajectory(oracle_mod):
+    """The FP64 drop-in (awsem_run, phase engine) over 500 Langevin steps of a
+    300-residue chain; every 100 steps the GPU energies and forces at the
+    trajectory's coordinates against the oracle."""
+    A = _gpu()
+    s, p = make_case([300], 81)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    x = s.coords()
+    v = np.random.default_rng(8).normal(0.0, 0.004, size=x.shape)
+    for k in range(5):
+        x, v, f, e = fix.run(x, v, 100, seed=2, step0=100 * k)
+        assert np.all(np.isfinite(x))
+        fg, eg = fix.compute_res(x)
+        fo, eo = o.compute(x)
+        ferr = term_err(fg, fo, np.linalg.norm(fo.reshape(-1, 3), axis=1).max())
+        eerr = np.max(np.abs(eg - eo) / np.maximum(1.0, np.abs(eo)))
+        print(f"step {100 * (k + 1)}: ferr {ferr:.2e} eerr {eerr:.2e}")
+        assert ferr <= TOL[0] and eerr <= TOL[0]
