@@ -39,7 +39,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
            "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_phase_clock",
            "awsem_batch_list_stats", "awsem_batch_last_error", "awsem_batch_use_graph", "awsem_batch_set_skin",
-           "awsem_batch_autotune",
+           "awsem_batch_autotune", "awsem_batch_set_engine",
            "awsem_server_create", "awsem_server_submit", "awsem_server_pump", "awsem_server_start",
            "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
            "awsem_server_release", "awsem_server_stats", "awsem_server_last_error", "awsem_server_destroy",
@@ -101,7 +101,8 @@ def lib():
         L.awsem_batch_list_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
         L.awsem_batch_use_graph.argtypes = [vp, C.c_int]
         L.awsem_batch_set_skin.argtypes = [vp, C.c_double]
-        L.awsem_batch_autotune.argtypes = [vp, C.POINTER(Langevin), C.c_int, dp, ip, dp]
+        L.awsem_batch_autotune.argtypes = [vp, C.POINTER(Langevin), C.c_int, dp, ip, ip, dp]
+        L.awsem_batch_set_engine.argtypes = [vp, C.c_int]
         u64 = C.c_uint64
         L.awsem_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
         L.awsem_server_submit.argtypes = [vp, vp, dp, dp, C.c_int, C.c_double, C.c_double, u64, C.POINTER(u64)]
@@ -330,13 +331,19 @@ class Batch:
         self._check(lib().awsem_batch_set_skin(self.h, skin))
 
     def autotune(self, probe_steps=40, dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=1, step0=0):
-        """Fastest (skin, steps per launch) on this batch: dict with skin,
-        steps_per_launch, ms_per_step.  The batch keeps the skin."""
+        """Fastest (engine, skin, steps per launch) on this batch: dict with
+        skin, steps_per_launch, engine, ms_per_step.  The batch keeps the
+        engine and the skin."""
         lg = Langevin(dt, t_start, t_stop, damp, seed, step0)
-        sk, ch, ms = C.c_double(), C.c_int(), C.c_double()
+        sk, ch, en, ms = C.c_double(), C.c_int(), C.c_int(), C.c_double()
         self._check(lib().awsem_batch_autotune(self.h, C.byref(lg), probe_steps, C.byref(sk), C.byref(ch),
-                                               C.byref(ms)))
-        return {"skin": sk.value, "steps_per_launch": ch.value, "ms_per_step": ms.value}
+                                               C.byref(en), C.byref(ms)))
+        return {"skin": sk.value, "steps_per_launch": ch.value, "engine": ["resident", "wide"][en.value],
+                "ms_per_step": ms.value}
+
+    def set_engine(self, engine: str):
+        """'resident' (default) or 'wide'."""
+        self._check(lib().awsem_batch_set_engine(self.h, {"resident": 0, "wide": 1}[engine]))
 
     def list_stats(self):
         """(list builds so far, pairs in the current lists, replicas without a valid list)."""

@@ -48,8 +48,9 @@ struct DevBatch {
   // recaptured when the kernel arguments (mode, dt, damp) change
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
-  int g_mode = -1;
+  int g_mode = -1, g_engine = -1, g_nsteps = -1;
   double g_dt = 0.0, g_damp = 0.0;
+  int engine = 0;  // 0 resident (workgroup per replica), 1 wide (phase launches over all replicas)
 
   ~DevBatch()
   {
@@ -121,7 +122,9 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   for (int r = 0; r < D->nrep; ++r) {
     const AwSystem &S = *sys[r];
     AwDevPlan plan;
-    AwStatus s = aw_devsys_plan(S, nullptr, D->fp32, !D->lds, false, plan);
+    // a global working slab for every replica: the wide engine, and
+    // replicas too large for LDS, work from it
+    AwStatus s = aw_devsys_plan(S, nullptr, D->fp32, want_lds || !D->lds, false, plan);
     if (!s.ok()) return s;
     std::unique_ptr<AwDevSystem> ds(new AwDevSystem());
     if (!(s = aw_devsys_create(plan, *ds)).ok()) return s;
@@ -617,22 +620,29 @@ static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
   D.nl_dirty = false;
   D.f_valid = false;
   (void)hipEventRecord(D.ev0, D.stream);
+  int nst = run.mode == 1 ? run.nsteps : 0;
+  auto launch = [&]() {
+    return D.engine == 1 ? aw_launch_wide(D.ptrs(), run, nst, D.stream) : aw_launch_resident(D.ptrs(), run, D.stream);
+  };
   if (D.use_graph) {
-    if (!D.graph || D.g_mode != run.mode || D.g_dt != run.dt || D.g_damp != run.damp) {
+    // the wide engine's launch sequence depends on the step count
+    int gkey_steps = D.engine == 1 ? nst : -1;
+    if (!D.graph || D.g_mode != run.mode || D.g_dt != run.dt || D.g_damp != run.damp || D.g_engine != D.engine ||
+        D.g_nsteps != gkey_steps) {
       if (D.graph) { (void)hipGraphExecDestroy(D.graph); D.graph = nullptr; }
       hipGraph_t g = nullptr;
       if (hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph capture"));
-      int lrc = aw_launch_resident(D.ptrs(), run, D.stream);
+      int lrc = launch();
       hipError_t ce = hipStreamEndCapture(D.stream, &g);
       if (lrc || ce != hipSuccess || !g) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph capture of the resident launch"));
       hipError_t ie = hipGraphInstantiate(&D.graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (ie != hipSuccess) { D.graph = nullptr; return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph instantiate")); }
-      D.g_mode = run.mode; D.g_dt = run.dt; D.g_damp = run.damp;
+      D.g_mode = run.mode; D.g_dt = run.dt; D.g_damp = run.damp; D.g_engine = D.engine; D.g_nsteps = gkey_steps;
     }
     if (hipGraphLaunch(D.graph, D.stream) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph launch"));
-  } else if (aw_launch_resident(D.ptrs(), run, D.stream)) {
+  } else if (launch()) {
     return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
   }
   (void)hipEventRecord(D.ev1, D.stream);
@@ -671,6 +681,13 @@ int awsem_batch_use_graph(awsem_batch *b, int on)
   return AWSEM_OK;
 }
 
+int awsem_batch_set_engine(awsem_batch *b, int engine)
+{
+  if (!b || !b->dev || engine < 0 || engine > 1) return AWSEM_E_ARG;
+  b->dev->engine = engine;
+  return AWSEM_OK;
+}
+
 int awsem_batch_set_skin(awsem_batch *b, double skin)
 {
   if (!b || !b->dev || !(skin > 0.0) || skin > 20.0) return AWSEM_E_ARG;
@@ -686,7 +703,7 @@ int awsem_batch_set_skin(awsem_batch *b, double skin)
 // candidate, from the current state each time, and keeps the fastest skin.
 // The state (x, v, f) is restored afterwards.
 int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_steps, double *skin_out,
-                         int *steps_per_launch_out, double *ms_per_step_out)
+                         int *steps_per_launch_out, int *engine_out, double *ms_per_step_out)
 {
   if (!b || !b->dev || !lg || probe_steps <= 0) return AWSEM_E_ARG;
   DevBatch &D = *b->dev;
@@ -707,9 +724,11 @@ int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_ste
   const double skins[] = {1.0, 1.5, 2.0, 2.5, 3.0};
   const int chunks[] = {5, 10, 20, 40};
   double best = 1e300, best_skin = skin0;
-  int best_chunk = 10;
+  int best_chunk = 10, best_engine = D.engine;
   int rc = AWSEM_OK;
+  for (int eng = 0; eng < 2 && rc == AWSEM_OK; ++eng)
   for (double sk : skins) {
+    D.engine = eng;
     if ((rc = awsem_batch_set_skin(b, sk)) != AWSEM_OK) break;
     for (int ch : chunks) {
       copy3(false);
@@ -727,18 +746,20 @@ int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_ste
       }
       if (rc != AWSEM_OK) break;
       double per = ms / done;
-      if (per < best) { best = per; best_skin = sk; best_chunk = ch; }
+      if (per < best) { best = per; best_skin = sk; best_chunk = ch; best_engine = eng; }
     }
     if (rc != AWSEM_OK) break;
   }
   copy3(false);
   (void)hipFree(save);
   D.f_valid = fv0;
+  D.engine = best_engine;
   int rc2 = awsem_batch_set_skin(b, rc == AWSEM_OK ? best_skin : skin0);
   if (rc != AWSEM_OK) return rc;
   if (rc2 != AWSEM_OK) return rc2;
   if (skin_out) *skin_out = best_skin;
   if (steps_per_launch_out) *steps_per_launch_out = best_chunk;
+  if (engine_out) *engine_out = best_engine;
   if (ms_per_step_out) *ms_per_step_out = best;
   return AWSEM_OK;
 }

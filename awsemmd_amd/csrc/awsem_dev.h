@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     V.nlx = R.nlx;
     V.nl_cap = R.nl_cap;
     V.nl_state = nl_state;
-    V.wq = V.nl ? V.nl + 4 + R.nl_cap : nullptr;
+    V.wq = V.nl ? V.nl + AW_NL_HDR + R.nl_cap : nullptr;
     if (threadIdx.x < 8) nl_state[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
       if ((R.flags & AW_RF_NL_RESET) && V.nl) V.nl[1] = 0;

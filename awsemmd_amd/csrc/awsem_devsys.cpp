@@ -65,7 +65,7 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   plan.off_items = take(cur, (size_t)plan.fm.n_items() * (fp32 ? sizeof(FMItem) : sizeof(FMItemD)));
   plan.off_rf = take(cur, plan.fm.rf.size() * tsz);
   plan.off_coef = take(cur, plan.fm.coef.size() * tsz);
-  plan.off_nl = take(cur, (4 + 6 * (size_t)plan.nl_cap) * sizeof(int));
+  plan.off_nl = take(cur, (AW_NL_HDR + 6 * (size_t)plan.nl_cap) * sizeof(int));
   plan.off_nlx = take(cur, 4 * (size_t)n * sizeof(double));
   if (scratch) plan.off_scr = take(cur, (size_t)aw_slab_doubles(n) * sizeof(double));
   plan.bytes = std::max<size_t>(cur, 256);

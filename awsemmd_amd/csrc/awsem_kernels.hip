@@ -286,6 +286,14 @@ static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t s
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int aw_launch_wide_f32(const AwBatchPtrs &p, const AwRun &run, int max_steps, hipStream_t st);
+int aw_launch_wide_f64(const AwBatchPtrs &p, const AwRun &run, int max_steps, hipStream_t st);
+
+int aw_launch_wide(const AwBatchPtrs &p, const AwRun &run, int max_steps, hipStream_t st)
+{
+  return p.fp32 ? aw_launch_wide_f32(p, run, max_steps, st) : aw_launch_wide_f64(p, run, max_steps, st);
+}
+
 int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 {
   return p.fp32 ? launch_phases_t<float>(p, run, st) : launch_phases_t<double>(p, run, st);

@@ -29,4 +29,7 @@ long long aw_slab_doubles(int n);
 void aw_resident_prepare();
 int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
 int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
+// the wide engine: every phase one launch over all replicas (working slabs,
+// lists and queues in global memory); max_steps = largest replica nsteps
+int aw_launch_wide(const AwBatchPtrs &p, const AwRun &run, int max_steps, hipStream_t st);
 int aw_launch_list(const AwBatchPtrs &p, int r, int kind, int arg, long long *out, int cap, int *count, hipStream_t st);

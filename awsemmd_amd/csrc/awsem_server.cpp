@@ -257,7 +257,11 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
       run.mode = 1;
       run.dt = o.dt;
       run.damp = o.damp;
-      if (aw_launch_resident(p, run, st[g])) return AwStatus::err(AWSEM_E_DEVICE, "server launch failed");
+      // jobs too large for LDS: the wide engine spreads each over many CUs
+      int max_steps = 0;
+      for (Job *j : grp[g]) max_steps = std::max(max_steps, j->round_steps);
+      int lrc = g == 0 ? aw_launch_resident(p, run, st[g]) : aw_launch_wide(p, run, max_steps, st[g]);
+      if (lrc) return AwStatus::err(AWSEM_E_DEVICE, "server launch failed");
     }
     SV_HIPCHK(hipEventRecord(ev2, st[0]));
     SV_HIPCHK(hipStreamWaitEvent(st[1], ev2, 0));

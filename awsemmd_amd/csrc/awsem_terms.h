@@ -153,7 +153,7 @@ struct View {
   const void *rf;        // T4 per (memory, pair)
   const void *items;     // FMItem / FMItemD per item
   // Verlet list of residue pairs (resident engine; nl == nullptr: sweep all pairs)
-  int *nl;               // global: [0] count, [1] valid, [2] builds, [3] -, then (i<<16 | j)
+  int *nl;               // global: header (AW_NL_HDR ints: [0] count, [1] valid, [2] builds), then (i<<16 | j)
   int nl_cap;
   double *nlx;           // global: CA positions [n][3] and extents [n] at the last build
   int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts
@@ -985,7 +985,7 @@ AW_DEV void phase_nl_build(const View<T> &V, int t0, int ts)
     pair_decode(k, n, i, j);
     if (near_pair(V, i, j, reach)) {
       int q = aadd_i(&V.nl_state[1], 1);
-      if (q < V.nl_cap) V.nl[4 + q] = (i << 16) | j;
+      if (q < V.nl_cap) V.nl[AW_NL_HDR + q] = (i << 16) | j;
     }
   }
   for (int i = t0; i < n; i += ts) {
@@ -1021,7 +1021,7 @@ AW_DEV int pair_total(const View<T> &V)
 template <class T, class F>
 AW_DEV void for_pairs_range(const View<T> &V, int k0, int k1, int t0, int ts, F &&body)
 {
-  const int *lst = (V.nl && V.nl[1]) ? V.nl + 4 : nullptr;
+  const int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
   int n = V.n;
   for (int k = k0 + t0; k < k1; k += ts) {
     int i, j;
@@ -1043,7 +1043,7 @@ template <class T, class F>
 AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
 {
   const int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
-  const int *lst = (V.nl && V.nl[1]) ? V.nl + 4 : nullptr;
+  const int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
   int total = qs ? V.nl_state[5] : pair_total(V);
   int n = V.n;
   for (int k = t0; k < total; k += ts) {

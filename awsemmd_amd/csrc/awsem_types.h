@@ -104,6 +104,10 @@ struct FMItemD {
 
 // One replica (system) as the kernels see it: device pointers to its own
 // data and its dynamics schedule for the current launch.
+// Verlet list block: [0] count, [1] valid, [2] builds, [3] -, [4..11] list
+// and queue counters when kept in global memory, then the list (nl_cap
+// packed pairs) and the work queues (5 nl_cap)
+#define AW_NL_HDR 16
 #define AW_RF_F_VALID 1   // f holds the forces of x: dynamics continue from them
 #define AW_RF_NL_RESET 2  // x was replaced: rebuild the Verlet list first
 struct ReplicaDesc {

@@ -1,0 +1,232 @@
+// awsem_wide.h -- the wide engine: every phase is one launch over all
+// replicas at once (blockIdx.y = replica, blockIdx.x spreads a replica's
+// items over several workgroups), with each replica's working slab, Verlet
+// list, work queues and their counters in global memory.  It keeps the whole
+// GPU busy when replicas are few and large (multimer complexes), where one
+// workgroup per replica (the resident engine) would leave most CUs idle.
+//
+// Step s of a launch: int1, A, list check/build, list publish, B (fragment
+// memory and local terms), B (list scan into the work queues), B (queues),
+// C, D, E, F, int2.  Kernel boundaries are the barriers between phases.
+#pragma once
+#include "awsem_dev.h"
+
+// replica of this block; false when it has no work in evaluation s
+// (s = -1: the evaluation before the first step, skipped for replicas
+// whose stored forces are valid; s = -2: a force evaluation on request)
+__device__ __forceinline__ bool w_active(const ReplicaDesc &R, int s)
+{
+  if (s == -2) return true;
+  if (s == -1) return !(R.flags & AW_RF_F_VALID) || R.nsteps == 0;
+  return s < R.nsteps;
+}
+
+template <class T>
+__device__ __forceinline__ View<T> w_view(const BatchDev &B, const ReplicaDesc &R)
+{
+  View<T> V = make_view<T>(B, R, R.scratch);
+  V.nl = R.nl_cap > 0 ? R.nl : nullptr;
+  V.nlx = R.nlx;
+  V.nl_cap = R.nl_cap;
+  V.nl_state = R.nl + 4;
+  V.wq = V.nl ? V.nl + AW_NL_HDR + R.nl_cap : nullptr;
+  return V;
+}
+
+#define W_TID (blockIdx.x * blockDim.x + threadIdx.x)
+#define W_NT (gridDim.x * blockDim.x)
+
+// launch start: state into the slab, list reset on new coordinates
+template <class T>
+__global__ void __launch_bounds__(256) k_w_begin(BatchDev B, int mode)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V = w_view<T>(B, R);
+  int n = R.n;
+  bool have_f = mode == 1 && (R.flags & AW_RF_F_VALID) && R.nsteps > 0;
+  for (int k = W_TID; k < 9 * n; k += W_NT) {
+    V.x()[k] = R.x[k];
+    if (have_f) V.f()[k] = R.f[k];
+  }
+  if (W_TID == 0 && V.nl) {
+    if (R.flags & AW_RF_NL_RESET) V.nl[1] = 0;
+    for (int k = 0; k < 8; ++k) V.nl_state[k] = 0;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_end(BatchDev B, int mode)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V = w_view<T>(B, R);
+  int n = R.n;
+  for (int k = W_TID; k < 9 * n; k += W_NT) {
+    if (mode == 1) R.x[k] = V.x()[k];
+    R.f[k] = V.f()[k];
+  }
+  if (W_TID == 0) {
+    double s = 0.0;
+    for (int k = 1; k < AW_N_ET; ++k) s += R.energy[k];
+    R.energy[0] = s;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_A(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  phase_A(V, W_TID, W_NT);
+  if (W_TID < AW_N_ET) R.energy[W_TID] = 0.0;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_nl_build(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  if (nl_needs_build(V)) phase_nl_build(V, W_TID, W_NT);
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) k_w_nl_publish(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s) || threadIdx.x != 0) return;
+  View<T> V = w_view<T>(B, R);
+  if (V.nl) nl_publish(V, nl_needs_build(V));
+}
+
+// a phase body with per-block energy accumulators added to the replica's slots
+#define W_KERNEL(NAME, BODY)                                                           \
+  template <class T>                                                                   \
+  __global__ void __launch_bounds__(256) NAME(BatchDev B, int s)                       \
+  {                                                                                    \
+    __shared__ double ered[N_EACC];                                                    \
+    const ReplicaDesc &R = B.rep[blockIdx.y];                                          \
+    if (!w_active(R, s)) return;                                                       \
+    View<T> V = w_view<T>(B, R);                                                       \
+    V.ered = ered;                                                                     \
+    if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;                                 \
+    __syncthreads();                                                                   \
+    BODY;                                                                              \
+    __syncthreads();                                                                   \
+    energy_add(ered, R.energy);                                                        \
+  }
+
+// fragment memory and local terms
+W_KERNEL(k_w_B1, (phase_B<T, false>(V, W_TID, W_NT, 3)))
+
+// the list scan: densities in place, DSSP / P_AP / excluded-volume pairs
+// into their queues and the density pairs for D and F; without a valid
+// list every pair is visited and evaluated in place
+template <class T>
+__device__ __forceinline__ void w_scan(const View<T> &V)
+{
+  EAcc E;
+  E.clear();
+  if (V.wq && V.nl[1]) {
+    int cap = V.nl_cap;
+    int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
+    for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe, qs); });
+  } else {
+    for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B(V, i, j, E); });
+  }
+  eflush(E, V.ered);
+}
+W_KERNEL(k_w_B2, w_scan(V))
+
+template <class T>
+__device__ __forceinline__ void w_queues(const View<T> &V)
+{
+  EAcc E;
+  E.clear();
+  if (V.wq && V.nl[1]) {
+    int cap = V.nl_cap;
+    const int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
+    int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
+    for (int k = W_TID; k < nd; k += W_NT) { int c = qd[k]; dssp_term(V, c >> 16, c & 0xffff, E); }
+    for (int k = W_TID; k < npap; k += W_NT) { int c = qp[k]; pap_term(V, c >> 16, c & 0xffff, E); }
+    for (int k = W_TID; k < nev; k += W_NT) {
+      int c = qe[k], i = c >> 16, j = c & 0xffff;
+      ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E);
+    }
+  }
+  eflush(E, V.ered);
+}
+W_KERNEL(k_w_B3, w_queues(V))
+W_KERNEL(k_w_C, phase_C(V, W_TID, W_NT))
+W_KERNEL(k_w_D, phase_D(V, W_TID, W_NT))
+W_KERNEL(k_w_F, phase_F(V, W_TID, W_NT))
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_E(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  phase_E(V, W_TID, W_NT);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_int1(BatchDev B, LgParams lg, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  integrate1(V, R.v, lg, W_TID, W_NT);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_int2(BatchDev B, LgParams lg, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  integrate2(V, R.v, lg, R, s, W_TID, W_NT);
+}
+
+template <class T>
+static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s)
+{
+  dim3 g1(1, g.y);
+  hipLaunchKernelGGL(k_w_A<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_nl_build<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_nl_publish<T>, g1, dim3(64), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_B1<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_B2<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_B3<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_C<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_D<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_E<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_F<T>, g, dim3(256), 0, st, B, s);
+}
+
+// K steps of every replica (mode 1), or one force evaluation (mode 0);
+// run.grid = workgroups per replica (0: enough for about 2048 in total)
+template <class T>
+static int launch_wide_t(const AwBatchPtrs &p, const AwRun &run, int max_steps, hipStream_t st)
+{
+  if (p.nrep <= 0) return 0;
+  BatchDev B = to_dev(p);
+  LgParams lg;
+  lg.dt = run.dt; lg.damp = run.damp;
+  int bx = run.grid > 0 ? run.grid : (2048 + p.nrep - 1) / p.nrep;
+  bx = bx < 1 ? 1 : (bx > 64 ? 64 : bx);
+  dim3 g(bx, p.nrep);
+  hipLaunchKernelGGL(k_w_begin<T>, g, dim3(256), 0, st, B, run.mode);
+  if (run.mode == 1) {
+    w_eval<T>(B, g, st, -1);
+    for (int s = 0; s < max_steps; ++s) {
+      hipLaunchKernelGGL(k_w_int1<T>, g, dim3(256), 0, st, B, lg, s);
+      w_eval<T>(B, g, st, s);
+      hipLaunchKernelGGL(k_w_int2<T>, g, dim3(256), 0, st, B, lg, s);
+    }
+  } else {
+    w_eval<T>(B, g, st, -2);
+  }
+  hipLaunchKernelGGL(k_w_end<T>, g, dim3(256), 0, st, B, run.mode);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -317,6 +317,7 @@ def run_multimer(a, A, W):
     ctx = force_field(A, W, systems[0])
     batch = A.Batch([ctx(s, A.PREC_FP32) for s in systems], A.PREC_FP32)
     x0 = A.pack_states([s.coords() for s in systems])
+    batch.set_engine(a.engine)
     batch.set_state(x0, np.zeros_like(x0))
     lg = dict(dt=2.0, damp=1000.0, seed=a.seed)
     ramp = lambda step: 600.0 - 300.0 * step / a.anneal_steps
@@ -342,7 +343,8 @@ def run_multimer(a, A, W):
             "config": {"workload": "64 replicas of an 8-chain x 300-residue compact complex, centroids 10-15 A "
                                    "apart, annealing 600->300 K", "replicas": a.replicas, "residues_per_replica": systems[0].n,
                        "anneal_steps": a.anneal_steps, "steps_per_launch": chunk,
-                       "verlet_skin_A": tuned["skin"] if tuned else 2.0},
+                       "verlet_skin_A": tuned["skin"] if tuned else 2.0,
+                       "engine": tuned["engine"] if tuned else a.engine},
             "ns_per_day_per_replica": value / a.replicas * 2.0 * 86400 / 1e6,
             "latency_ms_per_batch_step": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
             "final_energies_finite": bool(np.all(np.isfinite(e)))}
@@ -360,7 +362,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--chunk", type=int, default=0, help="batch-steps per resident launch (0: autotuned)")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default Verlet skin (2 A)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default Verlet skin (2 A) and --engine")
+    ap.add_argument("--engine", choices=["resident", "wide"], default="resident",
+                    help="batch engine when not autotuned")
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -418,6 +422,7 @@ def main():
     log(f"batch built in {t_build:.1f}s")
     x0 = A.pack_states([s.coords() for s in systems])
     v0 = A.pack_states([velocities(s, 300.0, 1000 + k) for k, s in enumerate(systems)])
+    batch.set_engine(a.engine)
     batch.set_state(x0, v0)
     lg = dict(dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=a.seed)
 
@@ -429,7 +434,7 @@ def main():
         t0 = time.perf_counter()
         tuned = batch.autotune(probe_steps=40, step0=a.warmup, **lg)
         tuned["tuning_s"] = time.perf_counter() - t0
-        log(f"autotuned: skin {tuned['skin']} A, {tuned['steps_per_launch']} steps per launch "
+        log(f"autotuned: {tuned['engine']} engine, skin {tuned['skin']} A, {tuned['steps_per_launch']} steps per launch "
             f"({tuned['ms_per_step']:.3f} ms/step) in {tuned['tuning_s']:.1f}s")
     if a.chunk <= 0:
         a.chunk = tuned["steps_per_launch"] if tuned else 10
@@ -519,7 +524,8 @@ def main():
         "config": {"workload": "256 chains, L~U[50,400], 20 memories per 9-residue window, Langevin 300 K, dt 2 fs",
                    "replicas": a.replicas, "residues": res_total, "memories": int(sum(len(s.memories) for s in systems)),
                    "steps_per_launch": a.chunk, "seed": a.seed,
-                   "verlet_skin_A": tuned["skin"] if tuned else 2.0, "autotuned": tuned is not None},
+                   "verlet_skin_A": tuned["skin"] if tuned else 2.0, "autotuned": tuned is not None,
+                   "engine": tuned["engine"] if tuned else a.engine},
         "plain_fp64_gpu": {"value": plain_value, "unit": "replica-steps/s",
                            "sample": f"{len(sample)} chains x {a.plain_steps} steps, phase engine, one chain at a time"},
         "speedup_vs_plain_fp64": value / plain_value,

@@ -190,12 +190,16 @@ int awsem_batch_use_graph(awsem_batch *b, int on);
  * the largest term cutoff + extents + skin and are rebuilt when a residue
  * has moved by more than skin/2.  Any value gives the same forces. */
 int awsem_batch_set_skin(awsem_batch *b, double skin);
+/* Engine: 0 resident (one workgroup owns a replica for K steps, state in
+ * LDS; default), 1 wide (every phase one launch over all replicas, state in
+ * global memory; for few large replicas). */
+int awsem_batch_set_engine(awsem_batch *b, int engine);
 /* Autotuner: times probe_steps of dynamics (from the current state, which
- * is restored afterwards) for each skin in {1, 1.5, 2, 2.5, 3} and steps per
- * launch in {5, 10, 20, 40}; keeps the fastest skin and reports it with the
- * best steps per launch and its ms per batch-step. */
+ * is restored afterwards) for each engine, skin in {1, 1.5, 2, 2.5, 3} and
+ * steps per launch in {5, 10, 20, 40}; keeps the fastest engine and skin and
+ * reports them with the best steps per launch and its ms per batch-step. */
 int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_steps,
-                         double *skin, int *steps_per_launch, double *ms_per_step);
+                         double *skin, int *steps_per_launch, int *engine, double *ms_per_step);
 /* Diagnostic: Verlet residue-pair lists of the resident engine summed over
  * replicas: list builds since creation, pairs in the current lists, and
  * replicas whose current list is invalid (overflowed or never built). */

@@ -53,7 +53,8 @@ def test_plain_per_term(oracle_mod, lengths, seed, compact, prec):
         assert ferr <= TOL[prec], name
 
 
-def test_batch_fp32_resident_per_term(oracle_mod):
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+def test_batch_fp32_per_term(oracle_mod, engine):
     A = _gpu()
     cases = [make_case([L], 100 + k) for k, L in enumerate([60, 37, 90])] + [make_case([30, 44], 7, compact=True)]
     xs = A.pack_states([s.coords() for s, _ in cases])
@@ -64,6 +65,7 @@ def test_batch_fp32_resident_per_term(oracle_mod):
         fixes = [A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32) for _, p in cases]
         fixes[0].set_term_mask(mask)  # the batch takes the parameters of its first system
         b = A.Batch(fixes, A.PREC_FP32)
+        b.set_engine(engine)
         b.set_state(xs)
         b.compute()
         x, v, f, e = b.get_state()
@@ -92,7 +94,8 @@ def test_lists_bit_exact(oracle_mod, kind, arg, prec):
         assert np.array_equal(g, r), (kind, arg, len(g), len(r))
 
 
-def test_batch_lists_follow_dynamics(oracle_mod):
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+def test_batch_lists_follow_dynamics(oracle_mod, engine):
     """Resident engine after Langevin dynamics: the Verlet pair lists are
     rebuilt when residues drift, and forces on the moved coordinates still
     match the oracle; replacing the state invalidates the lists."""
@@ -101,6 +104,7 @@ def test_batch_lists_follow_dynamics(oracle_mod):
     x0 = A.pack_states([s.coords() for s, _ in cases])
     oracles = [oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"]) for _, p in cases]
     b = A.Batch([A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32) for _, p in cases], A.PREC_FP32)
+    b.set_engine(engine)
     b.set_state(x0, np.zeros_like(x0))
     b.compute()
     builds0, pairs0, bad0 = b.list_stats()
@@ -131,7 +135,8 @@ def test_batch_lists_follow_dynamics(oracle_mod):
     assert b.list_stats()[0] == builds + len(cases)
 
 
-def test_batch_fp64_dynamics_matches_oracle(oracle_mod):
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+def test_batch_fp64_dynamics_matches_oracle(oracle_mod, engine):
     """FP64 resident engine: Langevin dynamics (fix nve + fix langevin, the
     shared counter-based random stream) against the oracle's integrator,
     over two consecutive runs (the second continues from the stored forces)."""
@@ -141,6 +146,7 @@ def test_batch_fp64_dynamics_matches_oracle(oracle_mod):
     x0 = [s.coords() for s, _ in cases]
     v0 = [rng.normal(0.0, 0.005, size=x.shape) for x in x0]
     b = A.Batch([A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64) for _, p in cases], A.PREC_FP64)
+    b.set_engine(engine)
     b.set_state(A.pack_states(x0), A.pack_states(v0))
     b.compute()
     lg = dict(dt=2.0, t_start=300.0, t_stop=300.0, damp=100.0, seed=3)
