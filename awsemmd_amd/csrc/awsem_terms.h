@@ -586,8 +586,16 @@ AW_DEV void fm_gauss(const T4t &f, T cf, const T *r, T c2, bool act1, bool act2,
   Es += (v0 + v1) + (v2 + v3);
 }
 
+// force sink of fm_item: adds into the replica's force array
 template <class T>
-AW_DEV void fm_item(const View<T> &V, const typename T4<T>::item &it, EAcc &E)
+struct FmSinkView {
+  const View<T> &V;
+  template <class R>
+  AW_DEV void operator()(int kind, int i, R fx, R fy, R fz) const { fadd3(V, kind, i, fx, fy, fz); }
+};
+
+template <class T, class Sink>
+AW_DEV void fm_item_to(const View<T> &V, const typename T4<T>::item &it, EAcc &E, const Sink &sink)
 {
   typedef typename T4<T>::type T4t;
   int i = it.ij >> 16, j = it.ij & 0xffff;
@@ -644,11 +652,18 @@ AW_DEV void fm_item(const View<T> &V, const typename T4<T>::item &it, EAcc &E)
   T f1 = act1 ? g * S1 * invs / r[1] : (T)0;
   T f2 = act2 ? g * S2 * invs / r[2] : (T)0;
   T f3 = act3 ? g * S3 * invs / r[3] : (T)0;
-  fadd3(V, K_CA, i, f0 * d[0].x + f1 * d[1].x, f0 * d[0].y + f1 * d[1].y, f0 * d[0].z + f1 * d[1].z);
-  fadd3(V, K_CA, j, -(f0 * d[0].x + f2 * d[2].x), -(f0 * d[0].y + f2 * d[2].y), -(f0 * d[0].z + f2 * d[2].z));
-  if (act2) fadd3(V, K_CB, i, f2 * d[2].x + f3 * d[3].x, f2 * d[2].y + f3 * d[3].y, f2 * d[2].z + f3 * d[3].z);
-  if (act1) fadd3(V, K_CB, j, -(f1 * d[1].x + f3 * d[3].x), -(f1 * d[1].y + f3 * d[3].y), -(f1 * d[1].z + f3 * d[3].z));
+  sink(K_CA, i, f0 * d[0].x + f1 * d[1].x, f0 * d[0].y + f1 * d[1].y, f0 * d[0].z + f1 * d[1].z);
+  sink(K_CA, j, -(f0 * d[0].x + f2 * d[2].x), -(f0 * d[0].y + f2 * d[2].y), -(f0 * d[0].z + f2 * d[2].z));
+  if (act2) sink(K_CB, i, f2 * d[2].x + f3 * d[3].x, f2 * d[2].y + f3 * d[3].y, f2 * d[2].z + f3 * d[3].z);
+  if (act1) sink(K_CB, j, -(f1 * d[1].x + f3 * d[3].x), -(f1 * d[1].y + f3 * d[3].y), -(f1 * d[1].z + f3 * d[3].z));
 }
+
+template <class T>
+AW_DEV void fm_item(const View<T> &V, const typename T4<T>::item &it, EAcc &E)
+{
+  fm_item_to(V, it, E, FmSinkView<T>{V});
+}
+
 
 // ------------------------------------------------------------------------
 // switching functions

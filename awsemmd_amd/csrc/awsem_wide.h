@@ -116,8 +116,80 @@ __global__ void __launch_bounds__(64) k_w_nl_publish(BatchDev B, int s)
     energy_add(ered, R.energy);                                                        \
   }
 
-// fragment memory and local terms
-W_KERNEL(k_w_B1, (phase_B<T, false>(V, W_TID, W_NT, 3)))
+// force sink of the wide fragment-memory kernel: a block's CA/CB forces for
+// residues [rlo, rlo + span) in LDS, flushed to the replica once
+template <class T>
+struct FmSinkLds {
+  double *fl;
+  int rlo, span;
+  template <class R>
+  __device__ void operator()(int kind, int i, R fx, R fy, R fz) const
+  {
+    double *p = fl + ((size_t)kind * span + (i - rlo)) * 3;
+    aadd(p + 0, (double)fx);
+    aadd(p + 1, (double)fy);
+    aadd(p + 2, (double)fz);
+  }
+};
+
+#define W_FM_SPAN 640  // residues of the block-local force window (30 KB of LDS)
+
+// fragment memory: each block takes a contiguous range of the replica's
+// items (ordered by window), so its forces fall on a short residue window
+// accumulated in LDS and written to the replica's forces once per block;
+// then the local terms
+template <class T>
+__global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
+{
+  __shared__ double ered[N_EACC];
+  __shared__ double fl[2 * W_FM_SPAN * 3];
+  __shared__ int lim[2];
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  V.ered = ered;
+  if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
+  if (threadIdx.x == 0) { lim[0] = 1 << 30; lim[1] = -1; }
+  __syncthreads();
+  EAcc E;
+  E.clear();
+  const FFParams *P = V.P;
+  int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM)) ? R.fm_items : 0;
+  typedef typename T4<T>::item ItemT;
+  const ItemT *items = (const ItemT *)R.items;
+  int chunk = (nfm + gridDim.x - 1) / gridDim.x;
+  int k0 = blockIdx.x * chunk, k1 = min(nfm, k0 + chunk);
+  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+    int ij = items[k].ij;
+    atomicMin(&lim[0], ij >> 16);
+    atomicMax(&lim[1], ij & 0xffff);
+  }
+  __syncthreads();
+  int rlo = lim[0], span = lim[1] - lim[0] + 1;
+  bool local = k0 < k1 && span <= W_FM_SPAN;
+  if (local)
+    for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) fl[k] = 0.0;
+  __syncthreads();
+  if (local) {
+    FmSinkLds<T> sink{fl, rlo, span};
+    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) fm_item_to(V, items[k], E, sink);
+  } else {
+    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) fm_item(V, items[k], E);
+  }
+  __syncthreads();
+  if (local)
+    for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) {
+      double v = fl[k];
+      if (v != 0.0) {
+        int kind = k / (3 * span), rem = k - kind * 3 * span;
+        atomicAdd(&V.f()[((size_t)kind * V.n + rlo) * 3 + rem], v);
+      }
+    }
+  for (int k = W_TID; k < V.n; k += W_NT) residue_local(V, k, E);
+  eflush(E, V.ered);
+  __syncthreads();
+  energy_add(ered, R.energy);
+}
 
 // the list scan: densities in place, DSSP / P_AP / excluded-volume pairs
 // into their queues and the density pairs for D and F; without a valid
