@@ -1033,15 +1033,33 @@ AW_DEV int pair_total(const View<T> &V)
   return (V.nl && V.nl[1]) ? V.nl[0] : pair_count(V.n);
 }
 
+// body(i, j) for the packed pairs src[k0 + t0], src[k0 + t0 + ts], ...: the
+// next entry is loaded while the current pair is evaluated, so a lane does
+// not wait on a memory round trip per pair
+template <class F>
+AW_DEV void for_packed(const int *src, int k0, int k1, int t0, int ts, F &&body)
+{
+  int k = k0 + t0;
+  int cur = k < k1 ? src[k] : 0;
+  for (; k < k1; k += ts) {
+    int nxt = k + ts < k1 ? src[k + ts] : 0;
+    body(cur >> 16, cur & 0xffff);
+    cur = nxt;
+  }
+}
+
 template <class T, class F>
 AW_DEV void for_pairs_range(const View<T> &V, int k0, int k1, int t0, int ts, F &&body)
 {
   const int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
+  if (lst) {
+    for_packed(lst, k0, k1, t0, ts, body);
+    return;
+  }
   int n = V.n;
   for (int k = k0 + t0; k < k1; k += ts) {
     int i, j;
-    if (lst) { int c = lst[k]; i = c >> 16; j = c & 0xffff; }
-    else pair_decode(k, n, i, j);
+    pair_decode(k, n, i, j);
     body(i, j);
   }
 }
@@ -1058,15 +1076,11 @@ template <class T, class F>
 AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
 {
   const int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
-  const int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
-  int total = qs ? V.nl_state[5] : pair_total(V);
-  int n = V.n;
-  for (int k = t0; k < total; k += ts) {
-    int i, j;
-    if (qs || lst) { int c = qs ? qs[k] : lst[k]; i = c >> 16; j = c & 0xffff; }
-    else pair_decode(k, n, i, j);
-    body(i, j);
+  if (qs) {
+    for_packed(qs, 0, V.nl_state[5], t0, ts, body);
+    return;
   }
+  for_pairs(V, t0, ts, body);
 }
 
 // ------------------------------------------------------------------------
@@ -1208,12 +1222,9 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part)
         for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe, qs); });
         AW_BARRIER();
         int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
-        for (int k = t0; k < nd; k += ts) { int c = qd[k]; dssp_term(V, c >> 16, c & 0xffff, E); }
-        for (int k = t0; k < npap; k += ts) { int c = qp[k]; pap_term(V, c >> 16, c & 0xffff, E); }
-        for (int k = t0; k < nev; k += ts) {
-          int c = qe[k], i = c >> 16, j = c & 0xffff;
-          ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E);
-        }
+        for_packed(qd, 0, nd, t0, ts, [&](int i, int j) { dssp_term(V, i, j, E); });
+        for_packed(qp, 0, npap, t0, ts, [&](int i, int j) { pap_term(V, i, j, E); });
+        for_packed(qe, 0, nev, t0, ts, [&](int i, int j) { ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E); });
         AW_BARRIER();
         if (t0 == 0) V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
         AW_BARRIER();

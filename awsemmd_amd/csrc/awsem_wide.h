@@ -219,12 +219,9 @@ __device__ __forceinline__ void w_queues(const View<T> &V)
     int cap = V.nl_cap;
     const int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
     int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
-    for (int k = W_TID; k < nd; k += W_NT) { int c = qd[k]; dssp_term(V, c >> 16, c & 0xffff, E); }
-    for (int k = W_TID; k < npap; k += W_NT) { int c = qp[k]; pap_term(V, c >> 16, c & 0xffff, E); }
-    for (int k = W_TID; k < nev; k += W_NT) {
-      int c = qe[k], i = c >> 16, j = c & 0xffff;
-      ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E);
-    }
+    for_packed(qd, 0, nd, W_TID, W_NT, [&](int i, int j) { dssp_term(V, i, j, E); });
+    for_packed(qp, 0, npap, W_TID, W_NT, [&](int i, int j) { pap_term(V, i, j, E); });
+    for_packed(qe, 0, nev, W_TID, W_NT, [&](int i, int j) { ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E); });
   }
   eflush(E, V.ered);
 }
