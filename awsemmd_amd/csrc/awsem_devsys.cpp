@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -67,6 +68,16 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   plan.off_coef = take(cur, plan.fm.coef.size() * tsz);
   plan.off_nl = take(cur, (AW_NL_HDR + 6 * (size_t)plan.nl_cap) * sizeof(int));
   plan.off_nlx = take(cur, 4 * (size_t)n * sizeof(double));
+  // cell-list table for large systems (the wide engine builds their lists
+  // through it): buckets = a power of two >= 2n
+  int cell_min = AW_CELL_MIN_N;
+  if (const char *e = std::getenv("AWSEM_CELL_MIN_N")) cell_min = std::atoi(e);
+  if (n > cell_min) {
+    int h = 1;
+    while (h < 2 * n) h <<= 1;
+    plan.cell_h = h;
+    plan.off_cells = take(cur, (2 * (size_t)h + 4 * (size_t)n) * sizeof(int));
+  }
   if (scratch) plan.off_scr = take(cur, (size_t)aw_slab_doubles(n) * sizeof(double));
   plan.bytes = std::max<size_t>(cur, 256);
   // the image: zeros (state, list headers, slab) and the packed constants
@@ -123,5 +134,7 @@ AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out)
   R.nl = (int *)(b + plan.off_nl);
   R.nlx = (double *)(b + plan.off_nlx);
   R.scratch = plan.scratch ? (double *)(b + plan.off_scr) : nullptr;
+  R.cells = plan.cell_h ? (int *)(b + plan.off_cells) : nullptr;
+  R.cell_h = plan.cell_h;
   return AwStatus();
 }

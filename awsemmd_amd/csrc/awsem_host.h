@@ -87,7 +87,8 @@ struct AwDevPlan {
   bool fp32 = false, scratch = false, own_state = false;
   int nl_cap = 0, n = 0, nch = 0, n_items = 0;
   size_t off_x = 0, off_v = 0, off_f = 0, off_e = 0, off_rinfo = 0, off_aps = 0, off_chs = 0, off_chl = 0;
-  size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0;
+  size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0, off_cells = 0;
+  int cell_h = 0;
   size_t bytes = 0;
   std::vector<unsigned char> image;  // host copy of the whole device image
 };

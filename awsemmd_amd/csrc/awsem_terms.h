@@ -24,6 +24,8 @@ inline int aadd_i(int *p, int v) { int o = *p; *p += v; return o; }
 inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
 inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #define AW_BARRIER() ((void)0)
+// max of non-negative floats into *p (as int bits)
+inline void wave_amax(int *p, float v) { amax_i(p, f2i(v)); }
 // slot in a work queue for lanes with pred set, -1 otherwise
 inline int q_push(int *counter, bool pred) { return pred ? (*counter)++ : -1; }
 #else
@@ -39,6 +41,17 @@ __device__ __forceinline__ int aadd_i(int *p, int v) { return atomicAdd(p, v); }
 __device__ __forceinline__ int f2i(float f) { return __float_as_int(f); }
 __device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
 #define AW_BARRIER() __syncthreads()
+// max of non-negative floats into *p (as int bits): reduced across the
+// wave first (every lane active), then one atomic per wave
+__device__ __forceinline__ void wave_amax(int *p, float v)
+{
+  int b = __float_as_int(v);
+  for (int o = 32; o > 0; o >>= 1) {
+    int u = __shfl_xor(b, o);
+    b = u > b ? u : b;
+  }
+  if (__lane_id() == 0) atomicMax(p, b);
+}
 // slot in a work queue (LDS counter) for the active lanes with pred set, -1
 // otherwise: one atomic per wave, slots in lane order
 __device__ __forceinline__ int q_push(int *counter, bool pred)
@@ -275,6 +288,7 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
 {
   const FFParams *P = V.P;
   int n = V.n;
+  float dmax = 0.0f, emax = 0.0f;  // this lane's largest drift and extent
   for (int i = t0; i < n; i += ts) {
     int ri = V.rinfo[i];
     const double *ca = XA(V, K_CA, i), *o = XA(V, K_O, i), *cb = XA(V, K_CB, i);
@@ -306,7 +320,8 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
       const double *c0 = V.nlx + 3 * i;
       double d2 = rsq_exact(ca, c0);
       float drift = (float)sqrt(d2) * 1.0001f + 1e-4f + fmaxf(0.0f, ei - (float)V.nlx[3 * n + i]);
-      amax_i(&V.nl_state[0], f2i(drift));
+      dmax = fmaxf(dmax, drift);
+      emax = fmaxf(emax, ei);
     }
     double *f = V.f();
     for (int k = 0; k < 3; ++k) {
@@ -316,6 +331,12 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
     }
     V.rho_w()[i] = 0.0; V.rho_h()[i] = 0.0; V.xi_w()[i] = 0.0;
     V.hxi1()[i] = 0.0; V.hxi2()[i] = 0.0; V.bf()[i] = 0.0;
+  }
+  // every lane of the wave is back here: one atomic per wave for the
+  // largest drift and the largest extent (cell size of a cell-list build)
+  if (V.nl) {
+    wave_amax(&V.nl_state[0], dmax);
+    wave_amax(&V.nl_state[6], emax);
   }
 }
 
@@ -1022,7 +1043,7 @@ AW_DEV void nl_publish(const View<T> &V, bool built)
   }
   V.nl_state[0] = 0;
   V.nl_state[1] = 0;
-  V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = V.nl_state[5] = 0;
+  V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = V.nl_state[5] = V.nl_state[6] = 0;
 }
 
 // iterate the residue pairs of a pass: the Verlet list when valid, else all

@@ -108,6 +108,10 @@ struct FMItemD {
 // and queue counters when kept in global memory, then the list (nl_cap
 // packed pairs) and the work queues (5 nl_cap)
 #define AW_NL_HDR 16
+// systems above this size get a cell-list table for their list builds
+// (AWSEM_CELL_MIN_N overrides it).  Below it the all-pairs build wins: its
+// pair order keeps neighbouring lanes on neighbouring residues.
+#define AW_CELL_MIN_N 4096
 #define AW_RF_F_VALID 1   // f holds the forces of x: dynamics continue from them
 #define AW_RF_NL_RESET 2  // x was replaced: rebuild the Verlet list first
 struct ReplicaDesc {
@@ -120,8 +124,12 @@ struct ReplicaDesc {
   double *energy;         // [AW_N_ET]
   double *scratch;        // working slab in global memory (phase engine; replicas too large for LDS)
   const void *items, *rf, *coef;  // FMItem/FMItemD [fm_items], float4/double4 Rf, weights
-  int *nl;                // Verlet list block: 4 header ints, nl_cap pairs, 5 nl_cap queue slots
+  int *nl;                // Verlet list block: AW_NL_HDR header ints, nl_cap pairs, 5 nl_cap queue slots
   double *nlx;            // [4 n]: CA positions and extents at the last list build
+  int *cells;             // cell-list table for the list build (wide engine): counts [cell_h],
+                          // offsets [cell_h], members [n], bucket [n], slot [n], packed cell [n];
+                          // null: all-pairs build
+  int cell_h, pad_c;      // buckets (a power of two)
   // dynamics of this replica in the current launch
   double t_start, t_stop;      // target temperature, ramped linearly over the run
   unsigned long long seed;     // random stream (seed, rng_id, step, atom, component)

@@ -5,7 +5,8 @@
 // GPU busy when replicas are few and large (multimer complexes), where one
 // workgroup per replica (the resident engine) would leave most CUs idle.
 //
-// Step s of a launch: int1, A, list check/build, list publish, B (fragment
+// Step s of a launch: int1, A, list check/build (all pairs, or through a
+// cell list for large replicas), list publish, B (fragment
 // memory and local terms), B (list scan into the work queues), B (queues),
 // C, D, E, F, int2.  Kernel boundaries are the barriers between phases.
 #pragma once
@@ -81,13 +82,157 @@ __global__ void __launch_bounds__(256) k_w_A(BatchDev B, int s)
   if (W_TID < AW_N_ET) R.energy[W_TID] = 0.0;
 }
 
+// all-pairs list build (replicas without a cell table)
 template <class T>
 __global__ void __launch_bounds__(256) k_w_nl_build(BatchDev B, int s)
 {
   const ReplicaDesc &R = B.rep[blockIdx.y];
-  if (!w_active(R, s)) return;
+  if (!w_active(R, s) || R.cells) return;
   View<T> V = w_view<T>(B, R);
   if (nl_needs_build(V)) phase_nl_build(V, W_TID, W_NT);
+}
+
+// ---- cell-list build: residues binned by CA into cubic cells no smaller
+// than the list reach (pair reach + skin + twice the largest extent), cells
+// hashed into cell_h buckets; each residue then visits the 27 cells around
+// its own and keeps the pairs near_pair accepts.  A pair is seen once: j > i
+// and j's own cell must be the visited cell (buckets may mix cells).
+
+__device__ __forceinline__ int w_cell_bucket(int cx, int cy, int cz, int h)
+{
+  unsigned u = (unsigned)cx * 73856093u ^ (unsigned)cy * 19349663u ^ (unsigned)cz * 83492791u;
+  return (int)(u & (unsigned)(h - 1));
+}
+
+template <class T>
+__device__ __forceinline__ float w_cell_size(const View<T> &V)
+{
+  float reach = (float)V.P->pair_reach + (float)V.P->nl_skin;
+  return (reach + 2.0f * i2f(V.nl_state[6]) + 1e-3f) * 1.0002f + 0.01f;
+}
+
+template <class T>
+__device__ __forceinline__ void w_cell_of(const View<T> &V, int i, float cs, int &cx, int &cy, int &cz)
+{
+  const double *a = XA(V, K_CA, i);
+  cx = (int)floorf((float)a[0] / cs);
+  cy = (int)floorf((float)a[1] / cs);
+  cz = (int)floorf((float)a[2] / cs);
+}
+
+// cell coordinates packed 10 bits each (cells are >= 10 A, so +-512 cells
+// spans kilometres)
+__device__ __forceinline__ int w_cell_pack(int cx, int cy, int cz)
+{
+  return ((cx + 512) & 1023) | (((cy + 512) & 1023) << 10) | (((cz + 512) & 1023) << 20);
+}
+
+template <class T>
+__device__ __forceinline__ bool w_cell_build(const BatchDev &B, const ReplicaDesc &R, int s, View<T> &V)
+{
+  if (!w_active(R, s) || !R.cells) return false;
+  V = w_view<T>(B, R);
+  return nl_needs_build(V);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_cell_zero(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V;
+  if (!w_cell_build(B, R, s, V)) return;
+  for (int k = W_TID; k < R.cell_h; k += W_NT) R.cells[k] = 0;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_cell_count(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V;
+  if (!w_cell_build(B, R, s, V)) return;
+  int h = R.cell_h, n = R.n;
+  int *counts = R.cells, *bucket = R.cells + 2 * h + n, *slot = bucket + n, *cellc = slot + n;
+  float cs = w_cell_size(V);
+  for (int i = W_TID; i < n; i += W_NT) {
+    int cx, cy, cz;
+    w_cell_of(V, i, cs, cx, cy, cz);
+    int b = w_cell_bucket(cx, cy, cz, h);
+    bucket[i] = b;
+    cellc[i] = w_cell_pack(cx, cy, cz);
+    slot[i] = atomicAdd(&counts[b], 1);
+  }
+}
+
+// exclusive prefix sum of the bucket counts (one workgroup per replica)
+template <class T>
+__global__ void __launch_bounds__(256) k_w_cell_scan(BatchDev B, int s)
+{
+  __shared__ int part[256];
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V;
+  if (!w_cell_build(B, R, s, V)) return;
+  int h = R.cell_h;
+  const int *counts = R.cells;
+  int *offs = R.cells + h;
+  int seg = (h + 255) / 256, b0 = threadIdx.x * seg, b1 = min(h, b0 + seg);
+  int sum = 0;
+  for (int b = b0; b < b1; ++b) sum += counts[b];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < 256; ++t) { int v = part[t]; part[t] = acc; acc += v; }
+  }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int b = b0; b < b1; ++b) { offs[b] = acc; acc += counts[b]; }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_w_cell_fill(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V;
+  if (!w_cell_build(B, R, s, V)) return;
+  int h = R.cell_h, n = R.n;
+  const int *offs = R.cells + h, *bucket = R.cells + 2 * h + n, *slot = bucket + n;
+  int *members = R.cells + 2 * h;
+  for (int i = W_TID; i < n; i += W_NT) members[offs[bucket[i]] + slot[i]] = i;
+}
+
+// one work item per (residue, neighbouring cell): 27 n items spread over the grid
+template <class T>
+__global__ void __launch_bounds__(256) k_w_cell_pairs(BatchDev B, int s)
+{
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  View<T> V;
+  if (!w_cell_build(B, R, s, V)) return;
+  int h = R.cell_h, n = R.n;
+  const int *counts = R.cells, *offs = R.cells + h, *members = R.cells + 2 * h;
+  const int *cellc = R.cells + 2 * h + 3 * n;
+  float reach = (float)V.P->pair_reach + (float)V.P->nl_skin;
+  for (int k = W_TID; k < 27 * n; k += W_NT) {
+    int i = k / 27, d = k - 27 * i;
+    int c = cellc[i];
+    int cx = (c & 1023) - 512 + d % 3 - 1, cy = ((c >> 10) & 1023) - 512 + (d / 3) % 3 - 1,
+        cz = ((c >> 20) & 1023) - 512 + d / 9 - 1;
+    int target = w_cell_pack(cx, cy, cz);
+    int b = w_cell_bucket(cx, cy, cz, h);
+    int m0 = offs[b], m1 = m0 + counts[b];
+    for (int m = m0; m < m1; ++m) {
+      int j = members[m];
+      if (j <= i || cellc[j] != target) continue;
+      if (near_pair(V, i, j, reach)) {
+        int q = aadd_i(&V.nl_state[1], 1);
+        if (q < V.nl_cap) V.nl[AW_NL_HDR + q] = (i << 16) | j;
+      }
+    }
+  }
+  for (int i = W_TID; i < n; i += W_NT) {
+    const double *ca = XA(V, K_CA, i);
+    V.nlx[3 * i] = ca[0]; V.nlx[3 * i + 1] = ca[1]; V.nlx[3 * i + 2] = ca[2];
+    V.nlx[3 * n + i] = (double)V.ext()[i];
+  }
 }
 
 template <class T>
@@ -263,6 +408,11 @@ static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s)
   dim3 g1(1, g.y);
   hipLaunchKernelGGL(k_w_A<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_nl_build<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_cell_zero<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_cell_count<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_cell_scan<T>, g1, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_cell_fill<T>, g, dim3(256), 0, st, B, s);
+  hipLaunchKernelGGL(k_w_cell_pairs<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_nl_publish<T>, g1, dim3(64), 0, st, B, s);
   hipLaunchKernelGGL(k_w_B1<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_B2<T>, g, dim3(256), 0, st, B, s);

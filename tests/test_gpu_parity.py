@@ -6,6 +6,8 @@ the floor keeps terms whose forces are physically zero (e.g. water on a
 chain with no |i-j|>=10 contacts, ~1e-12 kcal/mol/A) from dividing by
 rounding noise.  Energy error: |E_gpu - E_oracle| / max(1, |E_oracle|).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -184,3 +186,34 @@ def test_plain_fp64_run_checked_along_trajectory(oracle_mod):
         eerr = np.max(np.abs(eg - eo) / np.maximum(1.0, np.abs(eo)))
         print(f"step {100 * (k + 1)}: ferr {ferr:.2e} eerr {eerr:.2e}")
         assert ferr <= TOL[0] and eerr <= TOL[0]
+
+
+def test_wide_engine_cell_list_build(oracle_mod):
+    """A 600-residue two-chain complex (above the cell-list threshold) on the
+    wide engine: lists built through the cell list, before and after
+    Langevin dynamics that forces rebuilds, give the oracle's forces."""
+    A = _gpu()
+    os.environ["AWSEM_CELL_MIN_N"] = "512"  # force the cell-list build for this 600-residue system
+    s, p = make_case([300, 300], 91, compact=True)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    try:
+        b = A.Batch([A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32)], A.PREC_FP32)
+    finally:
+        del os.environ["AWSEM_CELL_MIN_N"]
+    b.set_engine("wide")
+    x0 = s.coords()
+    b.set_state(x0, np.zeros_like(x0))
+    b.compute()
+    for rnd in range(3):
+        if rnd:
+            b.run(150, dt=2.0, t_start=600.0, t_stop=600.0, damp=100.0, seed=5, step0=150 * rnd)
+            b.compute()
+        x, v, f, e = b.get_state()
+        fo, eo = o.compute(x.reshape(3, -1, 3))
+        ftot = np.linalg.norm(fo.reshape(-1, 3), axis=1).max()
+        ferr = term_err(f.reshape(fo.shape), fo, ftot)
+        eerr = np.max(np.abs(e[0] - eo) / np.maximum(1.0, np.abs(eo)))
+        builds, pairs, bad = b.list_stats()
+        print(f"round {rnd}: ferr {ferr:.2e} eerr {eerr:.2e} builds {builds} pairs {pairs}")
+        assert ferr <= TOL[1] and eerr <= TOL[1] and bad == 0
+    assert builds >= 2
