@@ -592,6 +592,8 @@ int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v)
   if (!b || !b->dev) return AWSEM_E_ARG;
   DevBatch &D = *b->dev;
   size_t bytes = (size_t)D.total_res * 9 * sizeof(double);
+  if ((x && !finite_all(x, (size_t)D.total_res * 9)) || (v && !finite_all(v, (size_t)D.total_res * 9)))
+    return bfail(b, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates or velocities"));
   if (x && hipMemcpy(D.d_x, x, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy x"));
   if (x) { D.nl_dirty = true; D.f_valid = false; }
   if (v && hipMemcpy(D.d_v, v, bytes, hipMemcpyHostToDevice) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "copy v"));

@@ -143,3 +143,30 @@ def test_workload_generator_is_seeded():
     assert np.allclose(d, 3.8)
     # .gro round trip is exact: coordinates are k/1000 nm
     assert np.array_equal(np.round(a.mem_structs[0] * 1000) / 1000, a.mem_structs[0])
+
+
+def test_non_finite_input_is_refused():
+    """NaN/Inf coordinates are refused before any device work (AWSEM_E_NONFINITE)."""
+    A = _A()
+    s, p = make_case([20], 12)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"])
+    x = s.coords().copy()
+    x[1, 3, 2] = np.nan
+    with pytest.raises(A.AwsemError) as e:
+        fix.compute_res(x)
+    assert e.value.code == -9
+    x[1, 3, 2] = np.inf
+    g, m, r, xa = _atoms(s)
+    fix.set_topology(g, m, r)
+    xa[5, 1] = np.inf
+    with pytest.raises(A.AwsemError) as e:
+        fix.pre_force(xa, np.zeros_like(xa))
+    assert e.value.code == -9
+
+
+def test_server_and_batch_symbols_are_bound():
+    A = _A()
+    L = A.lib()
+    for name in ("awsem_server_create", "awsem_server_submit", "awsem_batch_autotune", "awsem_batch_set_engine",
+                 "awsem_batch_use_graph", "awsem_batch_set_skin"):
+        assert getattr(L, name).argtypes is not None
