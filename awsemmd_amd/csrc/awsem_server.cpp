@@ -74,7 +74,10 @@ struct awsem_server {
   int lds_max = 0;
   size_t budget = 0, used = 0;
   hipStream_t st[2] = {nullptr, nullptr};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  // steps of wide-engine jobs relative to the cost model, adapted each round
+  // so that the two launches of a round take about the same time
+  double wide_boost = 1.0;
   void *d_params = nullptr;
   ReplicaDesc *d_rep = nullptr;
   int *d_order = nullptr, *d_queue = nullptr;
@@ -107,6 +110,7 @@ struct awsem_server {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev2) (void)hipEventDestroy(ev2);
+    if (ev3) (void)hipEventDestroy(ev3);
     for (hipStream_t s : st) if (s) (void)hipStreamDestroy(s);
   }
 
@@ -139,6 +143,7 @@ AwStatus awsem_server::init(const FFParams &p, const awsem_server_opts &opts)
   SV_HIPCHK(hipEventCreate(&ev0));
   SV_HIPCHK(hipEventCreate(&ev1));
   SV_HIPCHK(hipEventCreate(&ev2));
+  SV_HIPCHK(hipEventCreate(&ev3));
   AwStatus s = upload_raw(&d_params, &P, sizeof(FFParams));
   if (!s.ok()) return s;
   SV_HIPCHK(hipMalloc((void **)&d_rep, (size_t)o.max_jobs * sizeof(ReplicaDesc)));
@@ -204,7 +209,7 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
     double c_med = sorted_cost[sorted_cost.size() / 2];
     for (size_t k = 0; k < act.size(); ++k) {
       Job *j = act[k];
-      double want = o.steps_per_round * c_med / cost[k];
+      double want = o.steps_per_round * c_med / cost[k] * (j->n <= lds_max ? 1.0 : wide_boost);
       int steps = (int)std::lround(std::min(want, 8.0 * o.steps_per_round));
       j->round_steps = std::max(1, std::min(steps, j->nsteps - j->done));
     }
@@ -263,12 +268,19 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
       int lrc = g == 0 ? aw_launch_resident(p, run, st[g]) : aw_launch_wide(p, run, max_steps, st[g]);
       if (lrc) return AwStatus::err(AWSEM_E_DEVICE, "server launch failed");
     }
+    SV_HIPCHK(hipEventRecord(ev3, st[1]));
     SV_HIPCHK(hipEventRecord(ev2, st[0]));
     SV_HIPCHK(hipStreamWaitEvent(st[1], ev2, 0));
     SV_HIPCHK(hipEventRecord(ev1, st[1]));
     SV_HIPCHK(hipStreamSynchronize(st[1]));
-    float ms = 0.f;
+    float ms = 0.f, ms_res = 0.f, ms_wide = 0.f;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventElapsedTime(&ms_res, ev0, ev2);
+    (void)hipEventElapsedTime(&ms_wide, ev0, ev3);
+    if (!grp[1].empty()) {
+      double r = grp[0].empty() ? 2.0 : (double)ms_res / std::max(1e-3, (double)ms_wide);
+      wide_boost = std::min(16.0, std::max(0.25, wide_boost * std::min(2.0, std::max(0.5, r))));
+    }
     // retire
     std::vector<Job *> finished;
     for (Job *j : act) {
