@@ -102,6 +102,14 @@ __device__ __forceinline__ void prof_mark(unsigned long long *prof, bool first, 
   }
 }
 
+// sub-phase clock of the queued pair pass
+struct PairMarks {
+  unsigned long long *prof;
+  bool first;
+  unsigned long long t;
+  __device__ __forceinline__ void mark(int k) { prof_mark(prof, first, k, t); }
+};
+
 template <class T>
 __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long long *prof, bool first)
 {
@@ -123,7 +131,8 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
   // call site keeps a single copy of the code
   int nparts = prof ? 3 : 1;
   for (int q = 0; q < nparts; ++q) {
-    phase_B<T, true>(V, t0, ts, prof ? (1 << q) : 7);
+    PairMarks pm{prof, first, t};
+    phase_B<T, true, PairMarks>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr);
     __syncthreads();
     prof_mark(prof, first, prof ? 8 + q : 10, t);
   }
@@ -147,7 +156,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
   extern __shared__ double smem[];
   __shared__ int s_rep;
   __shared__ double ered[N_EACC];
-  __shared__ int nl_state[8];
+  __shared__ __attribute__((aligned(8))) int nl_state[8];  // [2..5]: two 64-bit queue-counter words
   for (;;) {
     if (threadIdx.x == 0) s_rep = atomicAdd(B.queue, 1);
     __syncthreads();

@@ -89,12 +89,18 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   put(img, plan.off_chl, S.ch_len);
   const AwFMPack &fm = plan.fm;
   if (fp32) {
-    put(img, plan.off_items, aw_fm_items<FMItem>(fm));
-    put(img, plan.off_rf, std::vector<float>(fm.rf.begin(), fm.rf.end()));
+    std::vector<FMItem> items;
+    std::vector<float> rf;
+    aw_fm_device(fm, items, rf);
+    put(img, plan.off_items, items);
+    put(img, plan.off_rf, rf);
     put(img, plan.off_coef, std::vector<float>(fm.coef.begin(), fm.coef.end()));
   } else {
-    put(img, plan.off_items, aw_fm_items<FMItemD>(fm));
-    put(img, plan.off_rf, fm.rf);
+    std::vector<FMItemD> items;
+    std::vector<double> rf;
+    aw_fm_device(fm, items, rf);
+    put(img, plan.off_items, items);
+    put(img, plan.off_rf, rf);
     put(img, plan.off_coef, fm.coef);
   }
   plan.n_items = fm.n_items();

@@ -2,6 +2,7 @@
 // and one system's sequence, chains, ssweight and fragment memories, as the
 // FixBackbone constructor builds them (src/fix_backbone.cpp:86-1072).
 #pragma once
+#include <cmath>
 #include <map>
 #include <string>
 #include <cstring>
@@ -64,18 +65,36 @@ struct AwFMPack {
 // appends one system's FM items (residue indices local to the system)
 // P: the force field (fragment-memory strength, sigma exponent, epsilon)
 AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B);
-template <class ItemT>
-std::vector<ItemT> aw_fm_items(const AwFMPack &B)
+// the device form of a pack in precision T: work items with the width
+// factor sc = sqrt(c2) rounded to T, and the Rf entries pre-multiplied by
+// it (in T, as the kernels scale the distances), skipped entries (memory or
+// target glycine CB) set to AW_FM_FAR.  Every Rf entry belongs to exactly
+// one item.
+template <class T, class ItemT>
+void aw_fm_device(const AwFMPack &B, std::vector<ItemT> &items, std::vector<T> &rf)
 {
-  std::vector<ItemT> out(B.n_items());
+  items.assign(B.n_items(), ItemT());
+  rf.assign(B.rf.size(), (T)AW_FM_FAR);
   for (int k = 0; k < B.n_items(); ++k) {
-    ItemT &it = out[k];
+    ItemT &it = items[k];
     std::memset(&it, 0, sizeof it);
-    it.gamma = B.items[3 * k]; it.c2 = B.items[3 * k + 1]; it.invs = B.items[3 * k + 2];
     const int *m = &B.meta[5 * k];
+    T sc = (T)std::sqrt(B.items[3 * k + 1]);
+    it.gamma = (T)B.items[3 * k];
+    it.sc = sc;
+    it.invs = (T)(B.items[3 * k + 2] / (double)sc);
     it.ij = m[0]; it.rf0 = m[1]; it.stride = m[2]; it.nmem = m[3]; it.coef0 = m[4];
+    bool gi = m[3] & AW_FM_GLY_I, gj = m[3] & AW_FM_GLY_J;
+    bool live[4] = {true, !gj, !gi, !gi && !gj};
+    int nmem = m[3] & AW_FM_NMEM_MASK;
+    for (int q = 0; q < nmem; ++q) {
+      size_t e = 4 * ((size_t)m[1] + (size_t)q * (size_t)m[2]);
+      for (int c = 0; c < 4; ++c) {
+        double v = B.rf[e + c];
+        if (live[c] && v >= 0.0) rf[e + c] = (T)v * sc;
+      }
+    }
   }
-  return out;
 }
 void aw_default_pair_styles(FFParams &P);
 // ---------------------------------------------------------------------------

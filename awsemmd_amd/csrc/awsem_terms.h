@@ -3,6 +3,7 @@
 // file; every function is also valid host C++ (AW_HOST_EMU), which lets
 // the exact kernel arithmetic be stepped through on a CPU when debugging.
 #pragma once
+#include <type_traits>
 #include <stdint.h>
 #include <math.h>
 #include "awsem_types.h"
@@ -26,8 +27,16 @@ inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #define AW_BARRIER() ((void)0)
 // max of non-negative floats into *p (as int bits)
 inline void wave_amax(int *p, float v) { amax_i(p, f2i(v)); }
-// slot in a work queue for lanes with pred set, -1 otherwise
-inline int q_push(int *counter, bool pred) { return pred ? (*counter)++ : -1; }
+// wave-level queue reservation (one lane here): a lane's ballot is its own
+// predicate and it ranks first among the lanes that set it
+typedef unsigned long long lmask;
+inline lmask w_ballot(bool p) { return p ? 1ull : 0ull; }
+inline int w_count(lmask m) { return (int)m; }
+inline int w_rank(lmask) { return 0; }
+inline void q_reserve4(int *ctr, const int n[4], int b[4])
+{
+  for (int k = 0; k < 4; ++k) { b[k] = ctr[k]; ctr[k] += n[k]; }
+}
 #else
 #include <hip/hip_runtime.h>
 #define AW_DEV __device__ __forceinline__
@@ -52,18 +61,30 @@ __device__ __forceinline__ void wave_amax(int *p, float v)
   }
   if (__lane_id() == 0) atomicMax(p, b);
 }
-// slot in a work queue (LDS counter) for the active lanes with pred set, -1
-// otherwise: one atomic per wave, slots in lane order
-__device__ __forceinline__ int q_push(int *counter, bool pred)
+// wave-level work-queue reservation: w_ballot / w_count over the active
+// lanes, w_rank = the number of lanes below this one in a ballot
+typedef unsigned long long lmask;
+__device__ __forceinline__ lmask w_ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int w_count(lmask m) { return __popcll(m); }
+__device__ __forceinline__ int w_rank(lmask m) { return __popcll(m & ((1ull << __lane_id()) - 1ull)); }
+// reserves n[k] (wave-uniform) slots in each of four queues whose counters
+// ctr[0..3] form two 8-byte-aligned 64-bit words (counts stay below 2^31,
+// so the low half never carries): both atomics are issued by the first
+// active lane back to back, and the bases come back through readfirstlane
+// (no LDS round trip for the broadcast)
+__device__ __forceinline__ void q_reserve4(int *ctr, const int n[4], int b[4])
 {
-  unsigned long long m = __ballot(pred);
-  if (m == 0ull) return -1;
-  int lane = __lane_id();
-  int leader = __ffsll((unsigned long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, __popcll(m));
-  base = __shfl(base, leader);
-  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+  int first = __ffsll((unsigned long long)__ballot(1)) - 1;
+  unsigned long long o0 = 0, o1 = 0;
+  if (__lane_id() == first) {
+    unsigned long long *c = (unsigned long long *)ctr;
+    o0 = atomicAdd(c, (unsigned long long)(unsigned)n[0] | ((unsigned long long)(unsigned)n[1] << 32));
+    o1 = atomicAdd(c + 1, (unsigned long long)(unsigned)n[2] | ((unsigned long long)(unsigned)n[3] << 32));
+  }
+  b[0] = __builtin_amdgcn_readfirstlane((int)(unsigned)o0);
+  b[1] = __builtin_amdgcn_readfirstlane((int)(unsigned)(o0 >> 32));
+  b[2] = __builtin_amdgcn_readfirstlane((int)(unsigned)o1);
+  b[3] = __builtin_amdgcn_readfirstlane((int)(unsigned)(o1 >> 32));
 }
 #endif
 
@@ -112,14 +133,15 @@ AW_DEV float rsq_f(const double *a, const double *b)
   return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
-// exact (rsq_exact(a, b) < t2) for t2 > 0
-AW_DEV bool rsq_lt(const double *a, const double *b, double t2)
+// exact (rsq_exact(a, b) < t2) for t2 > 0, given e = rsq_f(a, b)
+AW_DEV bool rsq_lt_est(float e, const double *a, const double *b, double t2)
 {
-  float e = rsq_f(a, b), tf = (float)t2;
+  float tf = (float)t2;
   if (e < tf * (1.0f - AW_RSQ_MARGIN)) return true;
   if (e > tf * (1.0f + AW_RSQ_MARGIN)) return false;
   return rsq_exact(a, b) < t2;
 }
+AW_DEV bool rsq_lt(const double *a, const double *b, double t2) { return rsq_lt_est(rsq_f(a, b), a, b, t2); }
 
 AW_DEV double lin3_exact(double a, double xa, double b, double xb, double c, double xc)
 {
@@ -588,21 +610,32 @@ template <class T> struct T4 {};
 template <> struct T4<float> { typedef float4 type; typedef FMItem item; };
 template <> struct T4<double> { typedef double4 type; typedef FMItemD item; };
 
-// one memory's four Gaussian wells of one pair: rf = (CA-CA, CA(i)-CB(j),
-// CA(j)-CB(i), CB-CB), -1 where the memory lacks the atom
-template <class T, class T4t>
-AW_DEV void fm_gauss(const T4t &f, T cf, const T *r, T c2, bool act1, bool act2, bool act3,
-                     T &S0, T &S1, T &S2, T &S3, T &Es)
+// a 4-vector load from device memory as a global (not flat) access: the
+// fragment-memory tables never live in LDS
+template <class T>
+AW_DEV typename T4<T>::type ld_global4(const typename T4<T>::type *p)
 {
-  T w = -cf;
-  T w1 = (act1 && f.y >= (T)0) ? w : (T)0;
-  T w2 = (act2 && f.z >= (T)0) ? w : (T)0;
-  T w3 = (act3 && f.w >= (T)0) ? w : (T)0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  V4 v = *(const __attribute__((address_space(1))) V4 *)p;
+  return typename T4<T>::type(v.x, v.y, v.z, v.w);
+#else
+  return *p;
+#endif
+}
+
+// one memory's four Gaussian wells of one pair: rf = (CA-CA, CA(i)-CB(j),
+// CA(j)-CB(i), CB-CB) pre-scaled by the item's sc, AW_FM_FAR where the
+// Gaussian is skipped
+template <class T4t, class T>
+AW_DEV void fm_gauss(const T4t &f, T w, const T *r, T &S0, T &S1, T &S2, T &S3, T &Es)
+{
+  // r and f are pre-scaled by the item's sc: exp2(-d^2) is the Gaussian
   T d0 = r[0] - f.x, d1 = r[1] - f.y, d2 = r[2] - f.z, d3 = r[3] - f.w;
-  T v0 = w * t_exp2(-(d0 * d0) * c2);
-  T v1 = w1 * t_exp2(-(d1 * d1) * c2);
-  T v2 = w2 * t_exp2(-(d2 * d2) * c2);
-  T v3 = w3 * t_exp2(-(d3 * d3) * c2);
+  T v0 = w * t_exp2(-(d0 * d0));
+  T v1 = w * t_exp2(-(d1 * d1));
+  T v2 = w * t_exp2(-(d2 * d2));
+  T v3 = w * t_exp2(-(d3 * d3));
   S0 += v0 * d0; S1 += v1 * d1; S2 += v2 * d2; S3 += v3 * d3;
   Es += (v0 + v1) + (v2 + v3);
 }
@@ -627,10 +660,13 @@ AW_DEV void fm_item_to(const View<T> &V, const typename T4<T>::item &it, EAcc &E
   d[1] = dvec<T>(XA(V, K_CA, i), XA(V, K_CB, j));
   d[2] = dvec<T>(XA(V, K_CB, i), XA(V, K_CA, j));
   d[3] = dvec<T>(XA(V, K_CB, i), XA(V, K_CB, j));
-  T r[4];
-  for (int k = 0; k < 4; ++k) r[k] = t_sqrt(dot(d[k], d[k]));
+  T r[4], rs[4];
+  T sc = (T)it.sc;
+  for (int k = 0; k < 4; ++k) {
+    r[k] = t_sqrt(dot(d[k], d[k]));
+    rs[k] = r[k] * sc;
+  }
   bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
-  T c2 = (T)it.c2;
   T S0 = 0, S1 = 0, S2 = 0, S3 = 0, Es = 0;
   const T4t *rf = (const T4t *)V.rf + it.rf0;
   const T4t *cf4 = (const T4t *)(V.coef + it.coef0);  // zero-padded, 4-aligned
@@ -647,27 +683,28 @@ AW_DEV void fm_item_to(const View<T> &V, const typename T4<T>::item &it, EAcc &E
     T cf[FMB];
 #pragma unroll
     for (int u = 0; u < FMB / 4; ++u) {
-      T4t c = cf4[(m0 >> 2) + u];
+      T4t c = ld_global4<T>(cf4 + (m0 >> 2) + u);
       cf[4 * u] = c.x; cf[4 * u + 1] = c.y; cf[4 * u + 2] = c.z; cf[4 * u + 3] = c.w;
     }
 #pragma unroll
-    for (int u = 0; u < FMB; ++u) { f4[u] = *p; p += np; }
+    for (int u = 0; u < FMB; ++u) { f4[u] = ld_global4<T>(p); p += np; }
 #pragma unroll
-    for (int u = 0; u < FMB; ++u) fm_gauss(f4[u], cf[u], r, c2, act1, act2, act3, S0, S1, S2, S3, Es);
+    for (int u = 0; u < FMB; ++u) fm_gauss(f4[u], cf[u], rs, S0, S1, S2, S3, Es);
   }
   for (; m0 < nmem; m0 += 4) {
     T4t f4[4];
-    T4t c = cf4[m0 >> 2];
+    T4t c = ld_global4<T>(cf4 + (m0 >> 2));
     T cf[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       int m = m0 + u < nmem ? m0 + u : nmem - 1;
-      f4[u] = rf[m * np];
+      f4[u] = ld_global4<T>(rf + m * np);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) fm_gauss(f4[u], cf[u], r, c2, act1, act2, act3, S0, S1, S2, S3, Es);
+    for (int u = 0; u < 4; ++u) fm_gauss(f4[u], cf[u], rs, S0, S1, S2, S3, Es);
   }
-  T g = (T)it.gamma, invs = (T)it.invs;
+  // the weights entered with the sign of cf; the energy is -cf e
+  T g = -(T)it.gamma, invs = (T)it.invs;
   E.fm += (double)(g * Es);
   T f0 = g * S0 * invs / r[0];
   T f1 = act1 ? g * S1 * invs / r[1] : (T)0;
@@ -1140,14 +1177,13 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
   int dres = j - i;
   // DL1 densities (fb.cpp:7729-7786); the exact FP64 distance only for
   // pairs near a well
+  bool dq = false;  // a pair the water-xi and force passes (phases D, F) can use
   if (!same || dres > 1) {
     const double *si = site(V, i, ri), *sj = site(V, j, rj);
     float e = rsq_f(si, sj);
-    // the pairs the water-xi and force passes (phases D, F) can use
     if (qs) {
       float dr = (float)P->dens_reach;
-      int q = q_push(&V.nl_state[5], e < dr * dr * (1.0f + AW_RSQ_MARGIN));
-      if (q >= 0) qs[q] = (i << 16) | j;
+      dq = e < dr * dr * (1.0f + AW_RSQ_MARGIN);
     }
     bool wc = P->water_flag && e > (float)P->w_rmin_th_sq[0] * (1.0f - AW_RSQ_MARGIN) &&
               e < (float)P->w_rmax_th_sq[0] * (1.0f + AW_RSQ_MARGIN);
@@ -1183,19 +1219,22 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
     }
   }
   // P_AP (fb.cpp:8096-8103)
-  bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_lt(XA(V, K_CA, i), XA(V, K_CA, j), P->pap_cutoff_sq);
+  bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), P->pap_cutoff_sq);
   float le = (float)P->ev_reach + eij;
   bool ev = P->ev_on && ON(P, AW_ET_VEXCLUDED) && rca <= le * le * 1.0001f;
   if (qd) {
+    // queue counters nl_state[2..5]: DSSP (both orientations), P_AP,
+    // excluded volume, density pairs
+    lmask m1 = w_ballot(dij), m2 = w_ballot(dji), mp = w_ballot(pap), me = w_ballot(ev), ms = w_ballot(dq);
+    int n1 = w_count(m1);
+    int cnt[4] = {n1 + w_count(m2), w_count(mp), w_count(me), w_count(ms)}, base[4];
+    q_reserve4(&V.nl_state[2], cnt, base);
     int c = (i << 16) | j;
-    int s1 = q_push(&V.nl_state[2], dij);
-    if (s1 >= 0) qd[s1] = c;
-    int s2 = q_push(&V.nl_state[2], dji);
-    if (s2 >= 0) qd[s2] = (j << 16) | i;
-    int s3 = q_push(&V.nl_state[3], pap);
-    if (s3 >= 0) qp[s3] = c;
-    int s4 = q_push(&V.nl_state[4], ev);
-    if (s4 >= 0) qe[s4] = c;
+    if (dij) qd[base[0] + w_rank(m1)] = c;
+    if (dji) qd[base[0] + n1 + w_rank(m2)] = (j << 16) | i;
+    if (pap) qp[base[1] + w_rank(mp)] = c;
+    if (ev) qe[base[2] + w_rank(me)] = c;
+    if (dq) qs[base[3] + w_rank(ms)] = c;
   } else {
     if (dij) dssp_term(V, i, j, E);
     if (dji) dssp_term(V, j, i, E);
@@ -1209,8 +1248,10 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
 // in chunks of at most nl_cap pairs; each chunk is scanned into the DSSP,
 // P_AP and excluded-volume work queues, which are then evaluated densely.
 // Unqueued (grid-wide phase engine): every term is evaluated in place.
-template <class T, bool QUEUED>
-AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part)
+// pm (clocked runs only): pm->mark(k) after each barrier of the queued pair
+// pass, k = 12 scan, 13 DSSP, 14 P_AP, 15 excluded volume
+template <class T, bool QUEUED, class PM = void>
+AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr)
 {
   EAcc E;
   E.clear();
@@ -1242,11 +1283,15 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part)
         int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
         for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe, qs); });
         AW_BARRIER();
+        if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(12);
         int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
         for_packed(qd, 0, nd, t0, ts, [&](int i, int j) { dssp_term(V, i, j, E); });
+        if constexpr (!std::is_void<PM>::value) if (pm) { AW_BARRIER(); pm->mark(13); }
         for_packed(qp, 0, npap, t0, ts, [&](int i, int j) { pap_term(V, i, j, E); });
+        if constexpr (!std::is_void<PM>::value) if (pm) { AW_BARRIER(); pm->mark(14); }
         for_packed(qe, 0, nev, t0, ts, [&](int i, int j) { ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E); });
         AW_BARRIER();
+        if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(15);
         if (t0 == 0) V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
         AW_BARRIER();
       }

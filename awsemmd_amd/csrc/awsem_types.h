@@ -85,19 +85,26 @@ struct FFParams {
 // for the group's first memory (float4/double4 units; the entry of memory m
 // is rf0 + m * stride) and the group's memory weights (coef0: start of a
 // zero-padded run of AW_FM_PAD weights or more), so a work item needs one
-// dependent load before its Rf loads.  The Gaussian width enters as
-// c2 = log2(e) / (2 sigma^2): exp(-dr^2 / (2 sigma^2)) = exp2(-dr^2 c2).
+// dependent load before its Rf loads.  The Gaussian width enters through
+// sc = sqrt(log2(e) / (2 sigma^2)): the device Rf entries of an item are
+// stored pre-multiplied by sc, the item scales its four distances by sc, and
+// exp(-dr^2 / (2 sigma^2)) = exp2(-(sc r - sc rf)^2).  An entry whose
+// Gaussian is skipped (a glycine CB of the memory or of the target pair,
+// fb.cpp:4148-4149) holds AW_FM_FAR, whose Gaussian underflows to exactly 0.
+// invs = 1 / (sigma^2 sc) turns the sum of w e (sc r - sc rf) into the
+// force factor.
 #define AW_FM_PAD 16
+#define AW_FM_FAR 1.0e6
 #define AW_FM_GLY_I (1 << 30)  // flags in nmem: residue i / j is glycine
 #define AW_FM_GLY_J (1 << 29)
 #define AW_FM_NMEM_MASK ((1 << 29) - 1)
 struct FMItem {
-  float gamma, c2, invs;     // gamma, log2(e)/(2 sigma^2), 1/sigma^2
+  float gamma, sc, invs;     // gamma, sqrt(log2(e)/(2 sigma^2)), 1/(sigma^2 sc)
   int ij;                    // (i << 16) | j, replica-local residue indices
   int rf0, stride, nmem, coef0;
 };
 struct FMItemD {
-  double gamma, c2, invs;
+  double gamma, sc, invs;
   int ij;
   int rf0, stride, nmem, coef0, pad;
 };
