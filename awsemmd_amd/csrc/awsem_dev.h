@@ -110,7 +110,7 @@ struct PairMarks {
   __device__ __forceinline__ void mark(int k) { prof_mark(prof, first, k, t); }
 };
 
-template <class T>
+template <class T, bool LDSM>
 __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long long *prof, bool first)
 {
   int t0 = threadIdx.x, ts = blockDim.x;
@@ -132,7 +132,7 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
   int nparts = prof ? 3 : 1;
   for (int q = 0; q < nparts; ++q) {
     PairMarks pm{prof, first, t};
-    phase_B<T, true, PairMarks>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr);
+    phase_B<T, true, PairMarks, LDSM>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr);
     __syncthreads();
     prof_mark(prof, first, prof ? 8 + q : 10, t);
   }
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
         __syncthreads();
         prof_mark(B.prof, first, 6, t);
       }
-      force_eval_block(V, B.prof, first);
+      force_eval_block<T, LDSM>(V, B.prof, first);
       if (s >= 0) {
         t = B.prof ? wall_clock64() : 0;
         integrate2(V, gv, lg, R, s, threadIdx.x, blockDim.x);

@@ -634,9 +634,11 @@ AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B)
     });
     int np = (int)prs.size(), nmem = (int)kv.second.size();
     long long rf_base = (long long)(B.rf.size() / 4);
-    if (rf_base + (long long)np * nmem >= (1LL << 31))
-      return AwStatus::err(-8, "fragment memory table exceeds 2^31 entries");
-    if (nmem > AW_FM_NMEM_MASK) return AwStatus::err(-1, "too many memories in one fragment window");
+    // the kernels address a replica's table with 32-bit byte offsets of
+    // 32-byte (FP64) entries
+    if (rf_base + (long long)np * nmem >= (1LL << 27))
+      return AwStatus::err(-8, "fragment memory table exceeds 2^27 entries");
+    if (nmem >= (1 << 24) || np >= (1 << 19)) return AwStatus::err(-1, "too many memories or pairs in one fragment window");
     int coef0 = (int)B.coef.size();
     for (int q = 0; q < np; ++q) {
       int i = pos + prs[q].first, j = pos + prs[q].second;

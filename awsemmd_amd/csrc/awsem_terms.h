@@ -25,6 +25,13 @@ inline int aadd_i(int *p, int v) { int o = *p; *p += v; return o; }
 inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
 inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #define AW_BARRIER() ((void)0)
+#define AW_NOINLINE inline
+#define AW_SCHED_FENCE() ((void)0)
+template <bool L> inline double ld_as(const double *p) { return *p; }
+template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
+template <class ItemT> inline ItemT ld_item(const ItemT *p) { return *p; }
+template <class P> inline P *aw_uniform(P *p) { return p; }
+inline unsigned aw_umul24(unsigned a, unsigned b) { return (a & 0xffffffu) * (b & 0xffffffu); }
 // max of non-negative floats into *p (as int bits)
 inline void wave_amax(int *p, float v) { amax_i(p, f2i(v)); }
 // wave-level queue reservation (one lane here): a lane's ballot is its own
@@ -50,6 +57,52 @@ __device__ __forceinline__ int aadd_i(int *p, int v) { return atomicAdd(p, v); }
 __device__ __forceinline__ int f2i(float f) { return __float_as_int(f); }
 __device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
 #define AW_BARRIER() __syncthreads()
+// a separately allocated device function (called once per pass)
+#define AW_NOINLINE __device__ __attribute__((noinline))
+// instruction-scheduling barrier: nothing is moved across it
+#define AW_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+// typed accesses for code compiled apart from the kernel (a noinline
+// function), where a plain pointer would give flat instructions: ld_as /
+// aadd_as read / add into LDS (L) or global memory, ld_item loads a 16-byte
+// multiple struct from global memory in 16-byte vectors
+typedef __attribute__((address_space(3))) double aw_lds_double;
+typedef __attribute__((address_space(1))) double aw_global_double;
+template <bool L>
+__device__ __forceinline__ double ld_as(const double *p)
+{
+  return L ? *(const aw_lds_double *)p : *(const aw_global_double *)p;
+}
+template <bool L>
+__device__ __forceinline__ void aadd_as(double *p, double v)
+{
+  if (L)
+    __hip_atomic_fetch_add((aw_lds_double *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    __hip_atomic_fetch_add((aw_global_double *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a wave-uniform pointer moved to scalar registers (arguments of a device
+// function arrive in vector registers)
+template <class P>
+__device__ __forceinline__ P *aw_uniform(P *p)
+{
+  unsigned long long v = (unsigned long long)p;
+  unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (P *)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ unsigned aw_umul24(unsigned a, unsigned b) { return __umul24(a, b); }
+template <class ItemT>
+__device__ __forceinline__ ItemT ld_item(const ItemT *p)
+{
+  static_assert(sizeof(ItemT) % 16 == 0, "item size");
+  typedef int I4 __attribute__((ext_vector_type(4)));
+  const __attribute__((address_space(1))) I4 *q = (const __attribute__((address_space(1))) I4 *)p;
+  I4 w[sizeof(ItemT) / 16];
+#pragma unroll
+  for (int u = 0; u < (int)(sizeof(ItemT) / 16); ++u) w[u] = q[u];
+  ItemT it;
+  __builtin_memcpy(&it, w, sizeof it);
+  return it;
+}
 // max of non-negative floats into *p (as int bits): reduced across the
 // wave first (every lane active), then one atomic per wave
 __device__ __forceinline__ void wave_amax(int *p, float v)
@@ -640,88 +693,164 @@ AW_DEV void fm_gauss(const T4t &f, T w, const T *r, T &S0, T &S1, T &S2, T &S3, 
   Es += (v0 + v1) + (v2 + v3);
 }
 
-// force sink of fm_item: adds into the replica's force array
-template <class T>
-struct FmSinkView {
-  const View<T> &V;
+// force sinks of fm_run.  fm_run is compiled as a function of its own, so
+// the sinks state where their arrays live (aadd_as) for the compiler to emit
+// LDS or global atomics.
+// FmSinkForce: adds into a replica's force array f[3][n][3]
+template <bool FLDS>
+struct FmSinkForce {
+  double *f;
+  int n;
   template <class R>
-  AW_DEV void operator()(int kind, int i, R fx, R fy, R fz) const { fadd3(V, kind, i, fx, fy, fz); }
+  AW_DEV void operator()(int kind, int i, R fx, R fy, R fz) const
+  {
+    double *p = f + ((size_t)kind * n + i) * 3;
+    aadd_as<FLDS>(p + 0, (double)fx);
+    aadd_as<FLDS>(p + 1, (double)fy);
+    aadd_as<FLDS>(p + 2, (double)fz);
+  }
 };
 
-template <class T, class Sink>
-AW_DEV void fm_item_to(const View<T> &V, const typename T4<T>::item &it, EAcc &E, const Sink &sink)
+// a lane's fragment-memory items k, k + ks, ... (< k1), software-pipelined
+// so that a lane keeps its table loads in flight while it computes:
+//   - a descriptor is loaded an item ahead;
+//   - the first batch (FMB memories) of an item is loaded while the previous
+//     item computes its last batch, and an item's second batch (4 memories)
+//     is loaded before its first batch computes.
+// Memories beyond FMB + 4 (groups larger than the shipped 20 per window) are
+// computed in further batches of 4 without overlap.  Loads past nmem re-read
+// the last memory, whose weight there is the zero padding of coef (no
+// branches on the data).  Coordinates x[3][n][3] (CA, CB, O) are in LDS when
+// XLDS; forces go to sink(kind, residue, fx, fy, fz); returns the energy.
+// A function of its own (not inlined): it gets its own register allocation
+// for the batches in flight, and is called once per pass.
+template <class T>
+struct FmBatch {
+  static const int FMB = sizeof(T) == 4 ? 16 : 8;
+};
+
+// byte offsets of an item's table rows: memory m's Rf entry at ob + m * sb,
+// its weights at cb (from the replica's table bases); nm1 = nmem - 1.  The
+// tables of one replica stay below 4 GiB (checked when they are packed), so
+// 32-bit offsets from a wave-uniform base address them.
+struct FmRow {
+  unsigned ob, sb, cb;
+  int nm1;
+};
+template <class T>
+AW_DEV FmRow fm_row(const typename T4<T>::item &it)
+{
+  FmRow a;
+  a.ob = (unsigned)it.rf0 * (unsigned)sizeof(typename T4<T>::type);
+  a.sb = (unsigned)it.stride * (unsigned)sizeof(typename T4<T>::type);
+  a.cb = (unsigned)it.coef0 * (unsigned)sizeof(T);
+  a.nm1 = (it.nmem & AW_FM_NMEM_MASK) - 1;
+  return a;
+}
+
+// memories m0 .. m0 + nb - 1 of a row (nb a multiple of 4): weights from the
+// zero-padded run of the group, Rf entries with m clamped to nmem - 1
+template <class T>
+AW_DEV void fm_load(const char *rfb, const char *cfb, const FmRow &a, int m0, int nb, typename T4<T>::type *f4,
+                    T *cf)
 {
   typedef typename T4<T>::type T4t;
-  int i = it.ij >> 16, j = it.ij & 0xffff;
-  int nmem = it.nmem & AW_FM_NMEM_MASK;
-  bool gi = it.nmem & AW_FM_GLY_I, gj = it.nmem & AW_FM_GLY_J;
-  V3<T> d[4];
-  d[0] = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j));
-  d[1] = dvec<T>(XA(V, K_CA, i), XA(V, K_CB, j));
-  d[2] = dvec<T>(XA(V, K_CB, i), XA(V, K_CA, j));
-  d[3] = dvec<T>(XA(V, K_CB, i), XA(V, K_CB, j));
-  T r[4], rs[4];
-  T sc = (T)it.sc;
-  for (int k = 0; k < 4; ++k) {
-    r[k] = t_sqrt(dot(d[k], d[k]));
-    rs[k] = r[k] * sc;
+  const T4t *cf4 = (const T4t *)(cfb + a.cb) + (m0 >> 2);
+#pragma unroll
+  for (int u = 0; u < nb / 4; ++u) {
+    T4t c = ld_global4<T>(cf4 + u);
+    cf[4 * u] = c.x; cf[4 * u + 1] = c.y; cf[4 * u + 2] = c.z; cf[4 * u + 3] = c.w;
   }
-  bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
-  T S0 = 0, S1 = 0, S2 = 0, S3 = 0, Es = 0;
-  const T4t *rf = (const T4t *)V.rf + it.rf0;
-  const T4t *cf4 = (const T4t *)(V.coef + it.coef0);  // zero-padded, 4-aligned
-  int np = it.stride;
-  // memories in batches: the batch's weights, then its Rf entries, are all
-  // loaded before any is used.  Full batches of FMB step a pointer; the tail
-  // runs in batches of 4 where memories past nmem re-read the last one and
-  // carry weight 0 (no branches on the data)
-  const int FMB = sizeof(T) == 4 ? 16 : 8;
-  int m0 = 0;
-  const T4t *p = rf;
-  for (; m0 + FMB <= nmem; m0 += FMB) {
-    T4t f4[FMB];
-    T cf[FMB];
 #pragma unroll
-    for (int u = 0; u < FMB / 4; ++u) {
-      T4t c = ld_global4<T>(cf4 + (m0 >> 2) + u);
-      cf[4 * u] = c.x; cf[4 * u + 1] = c.y; cf[4 * u + 2] = c.z; cf[4 * u + 3] = c.w;
-    }
-#pragma unroll
-    for (int u = 0; u < FMB; ++u) { f4[u] = ld_global4<T>(p); p += np; }
-#pragma unroll
-    for (int u = 0; u < FMB; ++u) fm_gauss(f4[u], cf[u], rs, S0, S1, S2, S3, Es);
+  for (int u = 0; u < nb; ++u) {
+    int m = m0 + u < a.nm1 ? m0 + u : a.nm1;
+    f4[u] = ld_global4<T>((const T4t *)(rfb + (a.ob + aw_umul24((unsigned)m, a.sb))));
   }
-  for (; m0 < nmem; m0 += 4) {
-    T4t f4[4];
-    T4t c = ld_global4<T>(cf4 + (m0 >> 2));
-    T cf[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      int m = m0 + u < nmem ? m0 + u : nmem - 1;
-      f4[u] = ld_global4<T>(rf + m * np);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) fm_gauss(f4[u], cf[u], rs, S0, S1, S2, S3, Es);
-  }
-  // the weights entered with the sign of cf; the energy is -cf e
-  T g = -(T)it.gamma, invs = (T)it.invs;
-  E.fm += (double)(g * Es);
-  T f0 = g * S0 * invs / r[0];
-  T f1 = act1 ? g * S1 * invs / r[1] : (T)0;
-  T f2 = act2 ? g * S2 * invs / r[2] : (T)0;
-  T f3 = act3 ? g * S3 * invs / r[3] : (T)0;
-  sink(K_CA, i, f0 * d[0].x + f1 * d[1].x, f0 * d[0].y + f1 * d[1].y, f0 * d[0].z + f1 * d[1].z);
-  sink(K_CA, j, -(f0 * d[0].x + f2 * d[2].x), -(f0 * d[0].y + f2 * d[2].y), -(f0 * d[0].z + f2 * d[2].z));
-  if (act2) sink(K_CB, i, f2 * d[2].x + f3 * d[3].x, f2 * d[2].y + f3 * d[3].y, f2 * d[2].z + f3 * d[3].z);
-  if (act1) sink(K_CB, j, -(f1 * d[1].x + f3 * d[3].x), -(f1 * d[1].y + f3 * d[3].y), -(f1 * d[1].z + f3 * d[3].z));
 }
 
-template <class T>
-AW_DEV void fm_item(const View<T> &V, const typename T4<T>::item &it, EAcc &E)
+template <class T, bool XLDS, class Sink>
+AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
+                          const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
 {
-  fm_item_to(V, it, E, FmSinkView<T>{V});
+  typedef typename T4<T>::type T4t;
+  typedef typename T4<T>::item ItemT;
+  const int FMB = FmBatch<T>::FMB;
+  if (k >= k1) return 0.0;
+  // the bases are the same in every lane: scalar registers
+  const char *rfb = (const char *)aw_uniform(rf), *cfb = (const char *)aw_uniform(coef);
+  x = aw_uniform(x);
+  items = aw_uniform(items);
+  double efm = 0.0;
+  ItemT cur = ld_item(items + k);
+  ItemT nxt = k + ks < k1 ? ld_item(items + k + ks) : cur;
+  T4t fa[FMB];
+  T ca[FMB];
+  fm_load(rfb, cfb, fm_row<T>(cur), 0, FMB, fa, ca);
+  for (; k < k1; k += ks) {
+    int nmem = cur.nmem & AW_FM_NMEM_MASK;
+    // second batch: memories FMB..FMB+3, or (no such memories) a re-read of
+    // the first four with weight 0; loads and compute are unconditional so
+    // that the wait for this batch leaves the next item's loads in flight
+    T4t fb[4];
+    T cb[4];
+    bool tail = nmem > FMB;
+    FmRow row = fm_row<T>(cur);
+    fm_load(rfb, cfb, row, tail ? FMB : 0, 4, fb, cb);
+    AW_SCHED_FENCE();  // the loads stay ahead of the first batch's arithmetic
+    int i = cur.ij >> 16, j = cur.ij & 0xffff;
+    bool gi = cur.nmem & AW_FM_GLY_I, gj = cur.nmem & AW_FM_GLY_J;
+    double cai[3], caj[3], cbi[3], cbj[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cai[c] = ld_as<XLDS>(x + 3 * i + c);
+      caj[c] = ld_as<XLDS>(x + 3 * j + c);
+      cbi[c] = ld_as<XLDS>(x + 3 * (n + i) + c);
+      cbj[c] = ld_as<XLDS>(x + 3 * (n + j) + c);
+    }
+    V3<T> d[4];
+    d[0] = dvec<T>(cai, caj);
+    d[1] = dvec<T>(cai, cbj);
+    d[2] = dvec<T>(cbi, caj);
+    d[3] = dvec<T>(cbi, cbj);
+    T r[4], rs[4];
+    T sc = (T)cur.sc;
+    for (int q = 0; q < 4; ++q) {
+      r[q] = t_sqrt(dot(d[q], d[q]));
+      rs[q] = r[q] * sc;
+    }
+    T S0 = 0, S1 = 0, S2 = 0, S3 = 0, Es = 0;
+#pragma unroll
+    for (int u = 0; u < FMB; ++u) {
+      fm_gauss(fa[u], ca[u], rs, S0, S1, S2, S3, Es);
+      if ((u & 3) == 3) AW_SCHED_FENCE();  // bounds the interleaving (register pressure)
+    }
+    // the next item's first batch is in flight while this one finishes (past
+    // the lane's last item, nxt is a valid item loaded again and unused)
+    fm_load(rfb, cfb, fm_row<T>(nxt), 0, FMB, fa, ca);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) fm_gauss(fb[u], tail ? cb[u] : (T)0, rs, S0, S1, S2, S3, Es);
+    for (int m0 = FMB + 4; m0 < nmem; m0 += 4) {
+      fm_load(rfb, cfb, row, m0, 4, fb, cb);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fm_gauss(fb[u], cb[u], rs, S0, S1, S2, S3, Es);
+    }
+    // the weights entered with the sign of cf; the energy is -cf e
+    bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
+    T g = -(T)cur.gamma, invs = (T)cur.invs;
+    efm += (double)(g * Es);
+    T f0 = g * S0 * invs / r[0];
+    T f1 = act1 ? g * S1 * invs / r[1] : (T)0;
+    T f2 = act2 ? g * S2 * invs / r[2] : (T)0;
+    T f3 = act3 ? g * S3 * invs / r[3] : (T)0;
+    sink(K_CA, i, f0 * d[0].x + f1 * d[1].x, f0 * d[0].y + f1 * d[1].y, f0 * d[0].z + f1 * d[1].z);
+    sink(K_CA, j, -(f0 * d[0].x + f2 * d[2].x), -(f0 * d[0].y + f2 * d[2].y), -(f0 * d[0].z + f2 * d[2].z));
+    if (act2) sink(K_CB, i, f2 * d[2].x + f3 * d[3].x, f2 * d[2].y + f3 * d[3].y, f2 * d[2].z + f3 * d[3].z);
+    if (act1) sink(K_CB, j, -(f1 * d[1].x + f3 * d[3].x), -(f1 * d[1].y + f3 * d[3].y), -(f1 * d[1].z + f3 * d[3].z));
+    cur = nxt;
+    if (k + 2 * ks < k1) nxt = ld_item(items + k + 2 * ks);
+  }
+  return efm;
 }
-
 
 // ------------------------------------------------------------------------
 // switching functions
@@ -1250,24 +1379,18 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
 // Unqueued (grid-wide phase engine): every term is evaluated in place.
 // pm (clocked runs only): pm->mark(k) after each barrier of the queued pair
 // pass, k = 12 scan, 13 DSSP, 14 P_AP, 15 excluded volume
-template <class T, bool QUEUED, class PM = void>
+// XLDS: the working slab (coordinates, forces) is in LDS
+template <class T, bool QUEUED, class PM = void, bool XLDS = false>
 AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr)
 {
   EAcc E;
   E.clear();
   const FFParams *P = V.P;
   int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM) && (part & 1)) ? V.fm_items : 0;
-  if (t0 < nfm) {
-    // the next item's descriptor is loaded while the current one computes
+  if (nfm > 0) {
     typedef typename T4<T>::item ItemT;
-    const ItemT *items = (const ItemT *)V.items + V.fm_item_base;
-    ItemT cur = items[t0];
-    for (int k = t0; k < nfm; k += ts) {
-      ItemT nxt = cur;
-      if (k + ts < nfm) nxt = items[k + ts];
-      fm_item(V, cur, E);
-      cur = nxt;
-    }
+    E.fm += fm_run<T, XLDS>(V.x(), V.n, V.rf, V.coef, (const ItemT *)V.items + V.fm_item_base, t0, nfm, ts,
+                            FmSinkForce<XLDS>{V.f(), V.n});
   }
   if (part & 2)
     for (int k = t0; k < V.n; k += ts) residue_local(V, k, E);

@@ -263,7 +263,6 @@ __global__ void __launch_bounds__(64) k_w_nl_publish(BatchDev B, int s)
 
 // force sink of the wide fragment-memory kernel: a block's CA/CB forces for
 // residues [rlo, rlo + span) in LDS, flushed to the replica once
-template <class T>
 struct FmSinkLds {
   double *fl;
   int rlo, span;
@@ -271,9 +270,9 @@ struct FmSinkLds {
   __device__ void operator()(int kind, int i, R fx, R fy, R fz) const
   {
     double *p = fl + ((size_t)kind * span + (i - rlo)) * 3;
-    aadd(p + 0, (double)fx);
-    aadd(p + 1, (double)fy);
-    aadd(p + 2, (double)fz);
+    aadd_as<true>(p + 0, (double)fx);
+    aadd_as<true>(p + 1, (double)fy);
+    aadd_as<true>(p + 2, (double)fz);
   }
 };
 
@@ -315,12 +314,12 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   if (local)
     for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) fl[k] = 0.0;
   __syncthreads();
-  if (local) {
-    FmSinkLds<T> sink{fl, rlo, span};
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) fm_item_to(V, items[k], E, sink);
-  } else {
-    for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) fm_item(V, items[k], E);
-  }
+  if (local)
+    E.fm += fm_run<T, false>(V.x(), V.n, V.rf, V.coef, items, k0 + (int)threadIdx.x, k1, (int)blockDim.x,
+                             FmSinkLds{fl, rlo, span});
+  else
+    E.fm += fm_run<T, false>(V.x(), V.n, V.rf, V.coef, items, k0 + (int)threadIdx.x, k1, (int)blockDim.x,
+                             FmSinkForce<false>{V.f(), V.n});
   __syncthreads();
   if (local)
     for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) {

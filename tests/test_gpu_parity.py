@@ -82,6 +82,54 @@ def test_batch_fp32_per_term(oracle_mod, engine):
             assert eerr <= TOL[1], (r, name)
 
 
+def ragged_memory_case(seed):
+    """Fragment-memory groups of every size from 1 to 29 memories, weights of
+    both signs, and 6-residue windows beside the 9-residue ones: the table
+    loads (first batch of 16 (FP32) or 8 (FP64) memories, a second batch of
+    4, further batches of 4, clamped past the group's end) see full, partial
+    and empty batches, and lanes of one wave see different group sizes."""
+    import dataclasses
+    import tempfile
+    from awsemmd_amd import workload as W
+    s = W.make_system([64, 30], seed=seed, n_struct=29)
+    rng = np.random.default_rng(seed)
+    keep = {}
+    mems = []
+    for m in s.memories:
+        n = keep.setdefault(m.pos, int(rng.integers(1, 30)))
+        if m.struct < n:
+            mems.append(dataclasses.replace(m, weight=float(rng.uniform(-0.5, 2.0))))
+    for m in s.memories[::53]:
+        if m.pos + 6 <= s.n and m.struct < 5:
+            mems.append(dataclasses.replace(m, length=6, weight=0.7))
+    s.memories = mems
+    d = tempfile.mkdtemp(prefix="awsem_case_")
+    return s, W.write_system(s, d)
+
+
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_fragment_memory_ragged_groups(oracle_mod, engine, prec):
+    A = _gpu()
+    cases = [ragged_memory_case(91), ragged_memory_case(92)]
+    mask = 1 << A.ET["Frag_Mem"]
+    fixes = [A.FixBackbone(p["coeff"], p["seq"], p["workdir"], prec) for _, p in cases]
+    fixes[0].set_term_mask(mask)
+    b = A.Batch(fixes, prec)
+    b.set_engine(engine)
+    b.set_state(A.pack_states([s.coords() for s, _ in cases]))
+    b.compute()
+    x, v, f, e = b.get_state()
+    fs = b.split(f)
+    k = A.ET["Frag_Mem"]
+    for r, (s, p) in enumerate(cases):
+        fo, eo = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"]).compute(s.coords(), mask)
+        ferr = term_err(fs[r], fo, np.linalg.norm(fo.reshape(-1, 3), axis=1).max())
+        eerr = abs(e[r, k] - eo[k]) / max(1.0, abs(eo[k]))
+        print(f"replica {r}: {len(s.memories)} memories, E={e[r, k]:.9g} Eref={eo[k]:.9g} eerr={eerr:.2e} ferr={ferr:.2e}")
+        assert ferr <= TOL[prec] and eerr <= TOL[prec]
+
+
 @pytest.mark.parametrize("kind,arg", [(0, 0), (0, 1), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (6, 1)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_lists_bit_exact(oracle_mod, kind, arg, prec):
