@@ -240,8 +240,7 @@ template <class T>
 static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 {
   BatchDev B = to_dev(p);
-  LgParams lg;
-  lg.dt = run.dt; lg.damp = run.damp;
+  LgParams lg = lg_make(run.dt, run.damp);
   int mode = run.mode;
   if (p.nrep <= 0) return 0;
   // a kernel, not a memset node, so that the sequence can be graph-captured

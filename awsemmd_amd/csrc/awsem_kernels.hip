@@ -268,8 +268,7 @@ template <class T>
 static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 {
   BatchDev B = to_dev(p);
-  LgParams lg;
-  lg.dt = run.dt; lg.damp = run.damp;
+  LgParams lg = lg_make(run.dt, run.damp);
   int grid = run.grid > 0 ? run.grid : 1024;
   for (int r = 0; r < p.nrep; ++r) {
     hipLaunchKernelGGL(k_ph_load<T>, dim3(grid), dim3(256), 0, st, B, r);

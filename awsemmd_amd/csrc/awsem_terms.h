@@ -19,6 +19,7 @@ struct double4 { double x, y, z, w; };
 inline float aw_fast_exp(float a) { return expf(a); }
 inline float aw_fast_exp2(float a) { return exp2f(a); }
 inline float aw_fsqrt(float a) { return sqrtf(a); }
+inline float aw_frcp(float a) { return 1.0f / a; }
 inline void aadd(double *p, double v) { *p += v; }
 inline void amax_i(int *p, int v) { if (v > *p) *p = v; }
 inline int aadd_i(int *p, int v) { int o = *p; *p += v; return o; }
@@ -26,6 +27,7 @@ inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
 inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #define AW_BARRIER() ((void)0)
 #define AW_NOINLINE inline
+#define AW_HOSTDEV
 #define AW_SCHED_FENCE() ((void)0)
 template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
@@ -51,6 +53,7 @@ inline void q_reserve4(int *ctr, const int n[4], int b[4])
 __device__ __forceinline__ float aw_fast_exp(float a) { return __expf(a); }
 __device__ __forceinline__ float aw_fast_exp2(float a) { return __builtin_amdgcn_exp2f(a); }
 __device__ __forceinline__ float aw_fsqrt(float a) { return __fsqrt_rn(a); }
+__device__ __forceinline__ float aw_frcp(float a) { return __builtin_amdgcn_rcpf(a); }
 __device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void amax_i(int *p, int v) { atomicMax(p, v); }
 __device__ __forceinline__ int aadd_i(int *p, int v) { return atomicAdd(p, v); }
@@ -59,6 +62,7 @@ __device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
 #define AW_BARRIER() __syncthreads()
 // a separately allocated device function (called once per pass)
 #define AW_NOINLINE __device__ __attribute__((noinline))
+#define AW_HOSTDEV __host__ __device__
 // instruction-scheduling barrier: nothing is moved across it
 #define AW_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
 // typed accesses for code compiled apart from the kernel (a noinline
@@ -157,6 +161,9 @@ AW_DEV double t_exp(double a) { return exp(a); }
 AW_DEV float t_exp2(float a) { return aw_fast_exp2(a); }
 AW_DEV double t_exp2(double a) { return exp2(a); }
 AW_DEV float t_sqrt(float a) { return aw_fsqrt(a); }
+// a / b: FP32 through the hardware reciprocal (1 ulp), FP64 exact
+AW_DEV float t_div(float a, float b) { return a * aw_frcp(b); }
+AW_DEV double t_div(double a, double b) { return a / b; }
 AW_DEV double t_sqrt(double a) { return sqrt(a); }
 AW_DEV float t_tanh(float a) { return tanhf(a); }
 AW_DEV double t_tanh(double a) { return tanh(a); }
@@ -838,10 +845,10 @@ AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
     bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
     T g = -(T)cur.gamma, invs = (T)cur.invs;
     efm += (double)(g * Es);
-    T f0 = g * S0 * invs / r[0];
-    T f1 = act1 ? g * S1 * invs / r[1] : (T)0;
-    T f2 = act2 ? g * S2 * invs / r[2] : (T)0;
-    T f3 = act3 ? g * S3 * invs / r[3] : (T)0;
+    T f0 = t_div(g * S0 * invs, r[0]);
+    T f1 = act1 ? t_div(g * S1 * invs, r[1]) : (T)0;
+    T f2 = act2 ? t_div(g * S2 * invs, r[2]) : (T)0;
+    T f3 = act3 ? t_div(g * S3 * invs, r[3]) : (T)0;
     sink(K_CA, i, f0 * d[0].x + f1 * d[1].x, f0 * d[0].y + f1 * d[1].y, f0 * d[0].z + f1 * d[1].z);
     sink(K_CA, j, -(f0 * d[0].x + f2 * d[2].x), -(f0 * d[0].y + f2 * d[2].y), -(f0 * d[0].z + f2 * d[2].z));
     if (act2) sink(K_CB, i, f2 * d[2].x + f3 * d[3].x, f2 * d[2].y + f3 * d[3].y, f2 * d[2].z + f3 * d[3].z);
@@ -876,9 +883,9 @@ AW_DEV void well_theta<float>(double kappa, double rmin, double rmax, double r, 
 {
   float ea = aw_fast_exp((float)(-2.0 * kappa * (r - rmin))), eb = aw_fast_exp((float)(-2.0 * kappa * (rmax - r)));
   float D = (1.0f + ea) * (1.0f + eb);
-  float iD = 1.0f / D;
+  float iD = aw_frcp(D);
   th = iD;
-  prd = 2.0f * (float)kappa * (ea - eb) * iD * iD / (float)r;
+  prd = t_div(2.0f * (float)kappa * (ea - eb) * iD * iD, (float)r);
 }
 
 template <class T>
@@ -1008,8 +1015,8 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     T dNO = eNO * (T)P->sigma_NO_sqinv;
     T dHO = eHO * (T)P->sigma_HO_sqinv;
     theta[k] = t_exp((T)-0.5 * (eNO * dNO + eHO * dHO));
-    p0[k] = -dNO / (T)RNO;
-    p1[k] = -dHO / (T)RHO;
+    p0[k] = t_div(-dNO, (T)RNO);
+    p1[k] = t_div(-dHO, (T)RHO);
   }
   T ff1 = (T)P->k_dssp * theta[0];
   T Vv[4] = {lambda[0], lambda[1] * theta[1], lambda[2] * theta[2], lambda[3] * theta[3]};
@@ -1121,7 +1128,7 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
       T pw = (T)1;
       for (int q = 0; q < p - 1; ++q) pw *= dr;
       T pref = (T)((p % 2 == 0) ? lam : -lam) * pw;
-      T fp = rr > (T)0 ? -pref * (T)p / rr : (T)0;
+      T fp = rr > (T)0 ? t_div(-pref * (T)p, rr) : (T)0;
       fadd(V, ka, i, fp, d);
       fadd(V, kb, j, -fp, d);
       E.ev += (double)(pref * dr);
@@ -1145,7 +1152,7 @@ AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E)
   T pw = (T)1;
   for (int q = 0; q < p - 1; ++q) pw *= dr;
   T pref = (T)((p % 2 == 0) ? lam : -lam) * pw;
-  T fp = rr > (T)0 ? -pref * (T)p / rr : (T)0;
+  T fp = rr > (T)0 ? t_div(-pref * (T)p, rr) : (T)0;
   fadd(V, K_CB, i, fp, d);
   fadd(V, K_O, i, -fp, d);
   E.ev += (double)(pref * dr);
@@ -1485,7 +1492,7 @@ AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
   T dNO = (T)(RNOd - P->helix_NO_zero), dHO = (T)(RHOd - P->helix_HO_zero);
   T dNOs = dNO * (T)P->helix_sigma_NO_sqinv, dHOs = dHO * (T)P->helix_sigma_HO_sqinv;
   T pt = -(T)P->k_helix * prob_sum * t_exp((T)-0.5 * (dNO * dNOs + dHO * dHOs));
-  T p0 = -dNOs / RNO, p1 = -dHOs / RHO;
+  T p0 = t_div(-dNOs, RNO), p1 = t_div(-dHOs, RHO);
   double gdiff = P->helix_gamma_w - P->helix_gamma_p;
   double sg = P->helix_gamma_p + gdiff * V.sig_h()[i] * V.sig_h()[j];
   V.hxi1()[i] = gdiff * (double)pt * V.sigp_h()[i] * V.sig_h()[j];
@@ -1636,31 +1643,59 @@ AW_DEV uint64_t mix64(uint64_t z)
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
-// counter hash (seed, replica, step, atom, component) -> U[0,1)
-AW_DEV double urand(uint64_t seed, int replica, long long step, int atom, int comp)
+// counter hash (seed, replica, step, atom, component) -> U[0,1), in two
+// parts: the state after (seed, replica, component, step), then the atom
+AW_DEV uint64_t urand_step(uint64_t seed, int replica, long long step, int comp)
 {
   uint64_t h = mix64(seed ^ 0x9e3779b97f4a7c15ULL);
   h = mix64(h ^ ((uint64_t)(uint32_t)replica << 32 | (uint32_t)comp));
-  h = mix64(h ^ (uint64_t)step);
-  h = mix64(h ^ (uint64_t)(uint32_t)atom);
+  return mix64(h ^ (uint64_t)step);
+}
+AW_DEV double urand_atom(uint64_t hstep, int atom)
+{
+  uint64_t h = mix64(hstep ^ (uint64_t)(uint32_t)atom);
   return (double)(h >> 11) * (1.0 / 9007199254740992.0);
 }
+AW_DEV double urand(uint64_t seed, int replica, long long step, int atom, int comp)
+{
+  return urand_atom(urand_step(seed, replica, step, comp), atom);
+}
 
-AW_DEV double atom_mass(int kind, int ri) { return kind == K_CA ? 27.0 : (kind == K_O ? 28.0 : ((ri & RI_GLY) ? 2.0 : 60.0)); }
+// mass class of an atom: 0 CA (27), 1 CB (60), 2 O (28), 3 glycine H-beta (2)
+AW_DEV int mass_class(int kind, int ri) { return (kind == K_CB && (ri & RI_GLY)) ? 3 : kind; }
 
+// integrator constants; the per-class factors are the expressions the
+// integrator would form per atom (same operations, same order), formed once
 struct LgParams {
   double dt, damp;  // fs; fix langevin damping time (fs)
+  double dtfm[4];   // 0.5 dt ftm2v / m
+  double g1[4];     // -m / damp / ftm2v
+  double g2c[4];    // sqrt(m) sqrt(24 kB / damp / dt / mvv2e) / ftm2v (times sqrt(T))
 };
+inline AW_HOSTDEV LgParams lg_make(double dt, double damp)
+{
+  LgParams lg;
+  lg.dt = dt;
+  lg.damp = damp;
+  const double mass[4] = {27.0, 60.0, 28.0, 2.0};
+  double dtf = 0.5 * dt * AW_FTM2V;
+  for (int c = 0; c < 4; ++c) {
+    double m = mass[c];
+    lg.dtfm[c] = dtf / m;
+    lg.g1[c] = -m / damp / AW_FTM2V;
+    lg.g2c[c] = sqrt(m) * sqrt(24.0 * AW_BOLTZ / damp / dt / AW_MVV2E) / AW_FTM2V;
+  }
+  return lg;
+}
 
 // initial_integrate of fix nve
 template <class T>
 AW_DEV void integrate1(const View<T> &V, double *v, const LgParams &lg, int t0, int ts)
 {
   int n = V.n;
-  double dtf = 0.5 * lg.dt * AW_FTM2V;
   for (int a = t0; a < 3 * n; a += ts) {
-    int kind = a / n, i = a - kind * n;
-    double dtfm = dtf / atom_mass(kind, V.rinfo[i]);
+    int kind = (a >= n) + (a >= 2 * n), i = a - kind * n;
+    double dtfm = lg.dtfm[mass_class(kind, V.rinfo[i])];
     for (int l = 0; l < 3; ++l) {
       double vv = v[3 * a + l] + dtfm * V.f()[3 * a + l];
       v[3 * a + l] = vv;
@@ -1674,21 +1709,21 @@ template <class T>
 AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, const ReplicaDesc &R, int s, int t0, int ts)
 {
   int n = V.n;
-  double dtf = 0.5 * lg.dt * AW_FTM2V;
   // the target temperature ramps over the replica's whole run (which may
   // span several launches)
   double delta = R.run_steps > 0 ? (double)(R.ramp0 + s + 1) / (double)R.run_steps : 0.0;
   double Tt = R.t_start + delta * (R.t_stop - R.t_start);
   double tsq = sqrt(Tt);
   long long step = R.step0 + s;
+  // urand's hash up to the step, per component (the atom enters last)
+  uint64_t hs[3];
+  for (int l = 0; l < 3; ++l) hs[l] = urand_step(R.seed, R.rng_id, step, l);
   for (int a = t0; a < 3 * n; a += ts) {
-    int kind = a / n, i = a - kind * n;
-    double m = atom_mass(kind, V.rinfo[i]);
-    double g1 = -m / lg.damp / AW_FTM2V;
-    double g2 = sqrt(m) * sqrt(24.0 * AW_BOLTZ / lg.damp / lg.dt / AW_MVV2E) / AW_FTM2V * tsq;
-    double dtfm = dtf / m;
+    int kind = (a >= n) + (a >= 2 * n), i = a - kind * n;
+    int c = mass_class(kind, V.rinfo[i]);
+    double g1 = lg.g1[c], g2 = lg.g2c[c] * tsq, dtfm = lg.dtfm[c];
     for (int l = 0; l < 3; ++l) {
-      double fv = V.f()[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand(R.seed, R.rng_id, step, a, l) - 0.5);
+      double fv = V.f()[3 * a + l] + g1 * v[3 * a + l] + g2 * (urand_atom(hs[l], a) - 0.5);
       V.f()[3 * a + l] = fv;
       v[3 * a + l] += dtfm * fv;
     }

@@ -429,8 +429,7 @@ static int launch_wide_t(const AwBatchPtrs &p, const AwRun &run, int max_steps, 
 {
   if (p.nrep <= 0) return 0;
   BatchDev B = to_dev(p);
-  LgParams lg;
-  lg.dt = run.dt; lg.damp = run.damp;
+  LgParams lg = lg_make(run.dt, run.damp);
   int bx = run.grid > 0 ? run.grid : (2048 + p.nrep - 1) / p.nrep;
   bx = bx < 1 ? 1 : (bx > 64 ? 64 : bx);
   dim3 g(bx, p.nrep);
