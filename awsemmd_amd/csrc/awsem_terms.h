@@ -904,19 +904,83 @@ AW_DEV const double *site(const View<double> &V, int i, int ri) { return (ri & R
 
 // cP_AP::compute (sml.h:176-203) in FP64: the nu < pap_delta early exit of
 // fb.cpp:3281 is a hard threshold
+// nu = (1 + tanh a) / 2 and prd = pref (1 - tanh^2 a) / (2 dr), a formed in
+// FP64 (the P_AP and DSSP switches).  FP32: through E = exp(-2|a|), nu = 1/(1+E) (a >= 0) or E/(1+E),
+// 1 - tanh^2 = 4E/(1+E)^2, without the cancellation of 1 + tanh for a << 0
 template <class T>
-AW_DEV void pap_nu(const View<T> &V, int i, int j, double &nu, double &prd)
+AW_DEV void tanh_switch(double a, double pref, double dr, T &nu, T &prd);
+template <>
+AW_DEV void tanh_switch<double>(double a, double pref, double dr, double &nu, double &prd)
+{
+  double th = tanh(a);
+  nu = 0.5 * (1 + th);
+  prd = 0.5 * pref * (1 - th * th) / dr;
+}
+template <>
+AW_DEV void tanh_switch<float>(double a, double pref, double dr, float &nu, float &prd)
+{
+  float af = (float)a;
+  float e = aw_fast_exp(-2.0f * (af < 0.0f ? -af : af));
+  float ip = aw_frcp(1.0f + e);
+  nu = af >= 0.0f ? ip : e * ip;
+  prd = 2.0f * (float)pref * e * ip * ip * aw_frcp((float)dr);
+}
+
+template <class T>
+AW_DEV void pap_nu(const View<T> &V, int i, int j, T &nu, T &prd)
 {
   const FFParams *P = V.P;
   double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
-  if (drsq > P->pap_drmax_sq) { nu = 0.0; prd = 0.0; }
-  else if (drsq < P->pap_drmin_sq) { nu = 1.0; prd = 0.0; }
+  if (drsq > P->pap_drmax_sq) { nu = 0; prd = 0; }
+  else if (drsq < P->pap_drmin_sq) { nu = 1; prd = 0; }
   else {
     double dr = sqrt(drsq);
-    double th = tanh(P->pap_pref * (P->pap_cut - dr));
-    nu = 0.5 * (1 + th);
-    prd = 0.5 * P->pap_pref * (1 - th * th) / dr;
+    tanh_switch<T>(P->pap_pref * (P->pap_cut - dr), P->pap_pref, dr, nu, prd);
   }
+}
+
+// the nu < 1e-12 early exit of compute_P_AP_potential: an FP32 nu decides
+// unless it lies within a relative 1e-5 of the threshold, where FP64 decides
+template <class T>
+AW_DEV bool pap_keep(const View<T> &V, int i, int j, T nu)
+{
+  if (sizeof(T) == 8) return !(nu < (T)AW_PAP_DELTA);
+  double d = (double)nu - AW_PAP_DELTA;
+  if (d > 1e-5 * AW_PAP_DELTA) return true;
+  if (d < -1e-5 * AW_PAP_DELTA) return false;
+  double nud, prdd;
+  const FFParams *P = V.P;
+  double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
+  double dr = sqrt(drsq);
+  tanh_switch<double>(P->pap_pref * (P->pap_cut - dr), P->pap_pref, dr, nud, prdd);
+  return !(nud < AW_PAP_DELTA);
+}
+
+// DSSP nu switch (fb.cpp:3006-3045): nu = (1 + tanh(pref (r - d0))) / 2,
+// prd = pref nu (1 - tanh) / r; FP64 in the reference's form, FP32 through
+// tanh_switch
+template <class T>
+AW_DEV void dssp_nu(double pref, double r, double d0, T &nu, T &prd)
+{
+  if (sizeof(T) == 8) {
+    double th = tanh(pref * (r - d0));
+    double v = 0.5 * (1.0 + th);
+    nu = (T)v;
+    prd = (T)(pref * v * (1.0 - th) / r);
+  } else {
+    tanh_switch<T>(pref * (r - d0), pref, r, nu, prd);
+  }
+}
+// nu < 1e-4, decided in FP64 where an FP32 nu lies within a relative 1e-5
+// of the threshold (nu = 0 and 1 are exact)
+template <class T>
+AW_DEV bool dssp_nu_below(T nu, double pref, double r, double d0)
+{
+  if (sizeof(T) == 8) return nu < (T)AW_DSSP_NU_DELTA;
+  double d = (double)nu - AW_DSSP_NU_DELTA;
+  if (d > 1e-5 * AW_DSSP_NU_DELTA) return false;
+  if (d < -1e-5 * AW_DSSP_NU_DELTA) return true;
+  return 0.5 * (1.0 + tanh(pref * (r - d0))) < AW_DSSP_NU_DELTA;
 }
 
 // ------------------------------------------------------------------------
@@ -977,29 +1041,27 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     lambda[3] = (T)-h[3][6] - (T)h[3][7] * tsp_HB[1] - (T)(h[3][8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
     hb_class = 4;
   }
-  // nu switches in FP64: the nu < 1e-4 early exit (fb.cpp:3052) is a hard threshold
-  double nu0 = 0.0, prdnu0 = 0.0, nu1 = 0.0, prdnu1 = 0.0;
+  // nu switches (arguments formed in FP64); the nu < 1e-4 early exit
+  // (fb.cpp:3052) is a hard threshold, decided as in FP64 (dssp_nu_below)
+  T nu0 = 0, prdnu0 = 0, nu1 = 0, prdnu1 = 0;
+  double r0 = 0.0, r1 = 0.0;
   bool nu0_on = (i - 2 >= i_ch_start - 1 && i + 2 < i_ch_end && hb_class != 2);
   bool nu1_on = (j - 2 >= j_ch_start - 1 && j + 2 < j_ch_end && hb_class != 2);
   if (nu0_on) {
     double rr = rsq_exact(XA(V, K_CA, i + 2), XA(V, K_CA, i - 2));
     if (rr > P->nu_cut1_sq) {
-      rr = sqrt(rr);
-      double th = tanh(P->pref[0] * (rr - P->d_nu0));
-      nu0 = 0.5 * (1.0 + th);
-      prdnu0 = P->pref[0] * nu0 * (1.0 - th) / rr;
+      r0 = sqrt(rr);
+      dssp_nu<T>(P->pref[0], r0, P->d_nu0, nu0, prdnu0);
     }
-  } else nu0 = 1.0;
+  } else nu0 = 1;
   if (nu1_on) {
     double rr = rsq_exact(XA(V, K_CA, j + 2), XA(V, K_CA, j - 2));
     if (rr > P->nu_cut2_sq) {
-      rr = sqrt(rr);
-      double th = tanh(P->pref[1] * (rr - P->d_nu0));
-      nu1 = 0.5 * (1.0 + th);
-      prdnu1 = P->pref[1] * nu1 * (1.0 - th) / rr;
+      r1 = sqrt(rr);
+      dssp_nu<T>(P->pref[1], r1, P->d_nu0, nu1, prdnu1);
     }
-  } else nu1 = 1.0;
-  if (nu0 < AW_DSSP_NU_DELTA || nu1 < AW_DSSP_NU_DELTA) return;
+  } else nu1 = 1;
+  if (dssp_nu_below(nu0, P->pref[0], r0, P->d_nu0) || dssp_nu_below(nu1, P->pref[1], r1, P->d_nu0)) return;
 
   bool on[4] = {true, i_rep, i_AP, i_P};
   int acc[4] = {i, i, j, j}, don[4] = {j, j + 1, i, i + 2};
@@ -1021,19 +1083,19 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   T ff1 = (T)P->k_dssp * theta[0];
   T Vv[4] = {lambda[0], lambda[1] * theta[1], lambda[2] * theta[2], lambda[3] * theta[3]};
   T theta_sum = ff1 * (Vv[0] + Vv[1] + Vv[2] + Vv[3]);
-  ff1 *= (T)(nu0 * nu1);
+  ff1 *= nu0 * nu1;
   for (int k = 0; k < 4; ++k) Vv[k] *= ff1;
   T VT = Vv[0] + Vv[1] + Vv[2] + Vv[3];
   E.dssp += (double)VT;
   if (nu0_on) {
     V3<T> dn = dvec<T>(XA(V, K_CA, i + 2), XA(V, K_CA, i - 2));
-    T fq = theta_sum * (T)(prdnu0 * nu1);
+    T fq = theta_sum * (prdnu0 * nu1);
     fadd(V, K_CA, i - 2, fq, dn);
     fadd(V, K_CA, i + 2, -fq, dn);
   }
   if (nu1_on) {
     V3<T> dn = dvec<T>(XA(V, K_CA, j + 2), XA(V, K_CA, j - 2));
-    T fq = theta_sum * (T)(nu0 * prdnu1);
+    T fq = theta_sum * (nu0 * prdnu1);
     fadd(V, K_CA, j - 2, fq, dn);
     fadd(V, K_CA, j + 2, -fq, dn);
   }
@@ -1058,9 +1120,9 @@ template <class T>
 AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
 {
   const FFParams *P = V.P;
-  double nu_ij, prd_ij;
+  T nu_ij, prd_ij;
   pap_nu(V, i, j, nu_ij, prd_ij);
-  if (nu_ij < AW_PAP_DELTA) return;
+  if (!pap_keep(V, i, j, nu_ij)) return;
   const int *RI = V.rinfo;
   int n = V.n, d = P->i_diff_pap;
   int ic = rchain(RI[i]), jc = rchain(RI[j]);
@@ -1077,23 +1139,23 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
   if (med || lng) {
     double K = beta ? (med ? P->k_pap[0] : 0.0) + (lng ? P->k_pap[1] * P->k_betapred_pap : 0.0)
                     : (med ? P->k_pap[0] : 0.0) + (lng ? P->k_pap[1] : 0.0);
-    double nu2, prd2;
+    T nu2, prd2;
     pap_nu(V, i + d, j - d, nu2, prd2);
     double kk = P->k_global_pap * K;
-    E.pap += -kk * nu_ij * nu2;
+    E.pap += -kk * (double)nu_ij * (double)nu2;
     V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j - d));
-    T f0 = (T)(kk * prd_ij * nu2), f1 = (T)(kk * nu_ij * prd2);
+    T f0 = (T)kk * prd_ij * nu2, f1 = (T)kk * nu_ij * prd2;
     fadd(V, K_CA, i, -f0, d0); fadd(V, K_CA, j, f0, d0);
     fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j - d, f1, d1);
   }
   if (par) {
     double K = beta ? P->k_pap[2] * P->k_betapred_pap : P->k_pap[2];
-    double nu2, prd2;
+    T nu2, prd2;
     pap_nu(V, i + d, j + d, nu2, prd2);
     double kk = P->k_global_pap * K;
-    E.pap += -kk * nu_ij * nu2;
+    E.pap += -kk * (double)nu_ij * (double)nu2;
     V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j + d));
-    T f0 = (T)(kk * prd_ij * nu2), f1 = (T)(kk * nu_ij * prd2);
+    T f0 = (T)kk * prd_ij * nu2, f1 = (T)kk * nu_ij * prd2;
     fadd(V, K_CA, i, -f0, d0); fadd(V, K_CA, j, f0, d0);
     fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j + d, f1, d1);
   }
@@ -1119,6 +1181,8 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
       double lam = P->ev_lambda[ta][tb];
       if (lam == 0.0) continue;
       double rc = shortc ? P->ev_cut_short[ta][tb] : P->ev_cut_long[ta][tb];
+      // FP32 estimate first: beyond rc^2 (1 + margin) the exact test fails too
+      if (rsq_f(XA(V, ka, i), XA(V, kb, j)) > (float)(rc * rc) * (1.0f + AW_RSQ_MARGIN)) continue;
       double rsq = rsq_exact(XA(V, ka, i), XA(V, kb, j));
       double r = sqrt(rsq);
       if (!(r < rc)) continue;
@@ -1144,6 +1208,7 @@ AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E)
   double lam = P->ev_lambda[ta][tb];
   if (lam == 0.0) return;
   double rc = P->ev_cut_short[ta][tb];
+  if (rsq_f(XA(V, K_CB, i), XA(V, K_O, i)) > (float)(rc * rc) * (1.0f + AW_RSQ_MARGIN)) return;
   double r = sqrt(rsq_exact(XA(V, K_CB, i), XA(V, K_O, i)));
   if (!(r < rc)) return;
   V3<T> d = dvec<T>(XA(V, K_CB, i), XA(V, K_O, i));
