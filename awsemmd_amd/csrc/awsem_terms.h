@@ -202,6 +202,17 @@ AW_DEV bool rsq_lt_est(float e, const double *a, const double *b, double t2)
   return rsq_exact(a, b) < t2;
 }
 AW_DEV bool rsq_lt(const double *a, const double *b, double t2) { return rsq_lt_est(rsq_f(a, b), a, b, t2); }
+// exact (rsq_exact(a, b) < t2 && !(sqrt(rsq_exact(a, b)) > t)) for t2 = t * t
+// rounded: well inside (outside) the FP32 estimate decides, since then
+// r^2 < t2 (1 - 9e-6) implies sqrt(r^2) < t
+AW_DEV bool dist_within(const double *a, const double *b, double t, double t2)
+{
+  float e = rsq_f(a, b), tf = (float)t2;
+  if (e < tf * (1.0f - AW_RSQ_MARGIN)) return true;
+  if (e > tf * (1.0f + AW_RSQ_MARGIN)) return false;
+  double r2 = rsq_exact(a, b);
+  return r2 < t2 && !(sqrt(r2) > t);
+}
 
 AW_DEV double lin3_exact(double a, double xa, double b, double xb, double c, double xc)
 {
@@ -1166,6 +1177,11 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
 // special_bonds fene: CA(i)-CA(i+1), CA(i)-O(i), O(i)-CA(i+1), CA(i)-CB(i))
 
 AW_DEV int atype(int kind, int ri) { return kind == K_CA ? 1 : (kind == K_O ? 3 : ((ri & RI_GLY) ? 5 : 4)); }
+AW_DEV double ev_pick(const double (*tab)[6], int ta0, int ta1, int tb0, int tb1, bool gi, bool gj)
+{
+  double v00 = tab[ta0][tb0], v01 = tab[ta0][tb1], v10 = tab[ta1][tb0], v11 = tab[ta1][tb1];
+  return gi ? (gj ? v11 : v10) : (gj ? v01 : v00);
+}
 
 template <class T>
 AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
@@ -1174,13 +1190,19 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
   bool same = rchain(ri) == rchain(rj);
   bool nb = same && (j == i + 1);
   bool shortc = same && (j - i < 5);
+  bool gi = ri & RI_GLY, gj = rj & RI_GLY;
+#pragma unroll
   for (int ka = 0; ka < 3; ++ka)
+#pragma unroll
     for (int kb = 0; kb < 3; ++kb) {
       if (nb && kb == K_CA && (ka == K_CA || ka == K_O)) continue;
-      int ta = atype(ka, ri), tb = atype(kb, rj);
-      double lam = P->ev_lambda[ta][tb];
+      // parameters at wave-uniform indices (the glycine and non-glycine
+      // entries, then a per-lane choice): scalar loads
+      int ta0 = atype(ka, 0), ta1 = atype(ka, RI_GLY), tb0 = atype(kb, 0), tb1 = atype(kb, RI_GLY);
+      double lam = ev_pick(P->ev_lambda, ta0, ta1, tb0, tb1, gi, gj);
       if (lam == 0.0) continue;
-      double rc = shortc ? P->ev_cut_short[ta][tb] : P->ev_cut_long[ta][tb];
+      double rc = shortc ? ev_pick(P->ev_cut_short, ta0, ta1, tb0, tb1, gi, gj)
+                         : ev_pick(P->ev_cut_long, ta0, ta1, tb0, tb1, gi, gj);
       // FP32 estimate first: beyond rc^2 (1 + margin) the exact test fails too
       if (rsq_f(XA(V, ka, i), XA(V, kb, j)) > (float)(rc * rc) * (1.0f + AW_RSQ_MARGIN)) continue;
       double rsq = rsq_exact(XA(V, ka, i), XA(V, kb, j));
@@ -1410,14 +1432,10 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
   bool dij = false, dji = false;
   float ld = (float)P->dssp_cut + eij;
   if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
-    if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST) && rsq_lt(XA(V, K_O, i), XV(V, K_N, j), P->dssp_cut_sq)) {
-      double r2 = rsq_exact(XA(V, K_O, i), XV(V, K_N, j));
-      dij = !(sqrt(r2) > P->dssp_cut);
-    }
-    if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST) && rsq_lt(XA(V, K_O, j), XV(V, K_N, i), P->dssp_cut_sq)) {
-      double r2 = rsq_exact(XA(V, K_O, j), XV(V, K_N, i));
-      dji = !(sqrt(r2) > P->dssp_cut);
-    }
+    if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST))
+      dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), P->dssp_cut, P->dssp_cut_sq);
+    if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST))
+      dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), P->dssp_cut, P->dssp_cut_sq);
   }
   // P_AP (fb.cpp:8096-8103)
   bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), P->pap_cutoff_sq);
@@ -1500,6 +1518,36 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
 // ------------------------------------------------------------------------
 // Phase C: sigma_h and burial (SL, fb.cpp:7795-7841)
 
+// sigma = (1 - tanh x) / 2 and 1 + tanh x (fb.cpp:7802-7804): FP64, or FP32
+// through E = exp(-2|x|) without cancellation: sigma = E/(1+E) (x >= 0) or
+// 1/(1+E), 1 + tanh = 2/(1+E) or 2E/(1+E)
+template <class T>
+AW_DEV void sig_switch(double x, double &s, double &onep)
+{
+  if (sizeof(T) == 8) {
+    double th = tanh(x);
+    s = 0.5 * (1.0 - th);
+    onep = 1.0 + th;
+    return;
+  }
+  float e = aw_fast_exp(-2.0f * (float)(x < 0.0 ? -x : x));
+  float ip = aw_frcp(1.0f + e);
+  s = (double)(x >= 0.0 ? e * ip : ip);
+  onep = (double)(x >= 0.0 ? 2.0f * ip : 2.0f * e * ip);
+}
+// tanh of an FP64-formed argument for the burial switches: FP64, or FP32
+// through exp(-2|x|) (absolute error ~1e-7, the accuracy the burial wells
+// enter the energies and forces with)
+template <class T>
+AW_DEV double sw_tanh(double x)
+{
+  if (sizeof(T) == 8) return tanh(x);
+  float a = (float)(x < 0.0 ? -x : x);
+  float e = aw_fast_exp(-2.0f * a);
+  double t = (double)((1.0f - e) * aw_frcp(1.0f + e));
+  return x < 0.0 ? -t : t;
+}
+
 template <class T>
 AW_DEV void phase_C(const View<T> &V, int t0, int ts)
 {
@@ -1509,21 +1557,21 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts)
   for (int i = t0; i < V.n; i += ts) {
     int it = rtype(V.rinfo[i]);
     if (P->water_flag) {
-      double th = tanh(P->water_kappa_sigma * (V.rho_w()[i] - P->treshold));
-      double s = 0.5 * (1.0 - th);
+      double s, onep;
+      sig_switch<T>(P->water_kappa_sigma * (V.rho_w()[i] - P->treshold), s, onep);
       V.sig_w()[i] = s;
-      V.sigp_w()[i] = -P->water_kappa_sigma * s * (1.0 + th);
+      V.sigp_w()[i] = -P->water_kappa_sigma * s * onep;
     }
     if (P->helix_flag) {
-      double th = tanh(P->helix_kappa_sigma * (V.rho_h()[i] - P->helix_treshold));
-      double s = 0.5 * (1.0 - th);
+      double s, onep;
+      sig_switch<T>(P->helix_kappa_sigma * (V.rho_h()[i] - P->helix_treshold), s, onep);
       V.sig_h()[i] = s;
-      V.sigp_h()[i] = -P->helix_kappa_sigma * s * (1.0 + th);
+      V.sigp_h()[i] = -P->helix_kappa_sigma * s * onep;
     }
     if (P->burial_flag) {
       double rho = V.rho_w()[i], kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
       for (int b = 0; b < 3; ++b) {
-        double t0_ = tanh(kb * (rho - P->burial_ro_min[b])), t1_ = tanh(kb * (P->burial_ro_max[b] - rho));
+        double t0_ = sw_tanh<T>(kb * (rho - P->burial_ro_min[b])), t1_ = sw_tanh<T>(kb * (P->burial_ro_max[b] - rho));
         eb += -0.5 * P->k_burial * P->burial_gamma[it][b] * (t0_ + t1_);
         bfv += P->burial_gamma[it][b] * (t1_ * t1_ - t0_ * t0_);
       }
