@@ -111,7 +111,7 @@ struct PairMarks {
 };
 
 template <class T, bool LDSM>
-__device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long long *prof, bool first)
+__device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long long *prof, bool first, bool ef)
 {
   int t0 = threadIdx.x, ts = blockDim.x;
   unsigned long long t = prof ? wall_clock64() : 0;
@@ -132,7 +132,7 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
   int nparts = prof ? 3 : 1;
   for (int q = 0; q < nparts; ++q) {
     PairMarks pm{prof, first, t};
-    phase_B<T, true, PairMarks, LDSM>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr);
+    phase_B<T, true, PairMarks, LDSM>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr, ef);
     __syncthreads();
     prof_mark(prof, first, prof ? 8 + q : 10, t);
   }
@@ -202,7 +202,8 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
         __syncthreads();
         prof_mark(B.prof, first, 6, t);
       }
-      force_eval_block<T, LDSM>(V, B.prof, first);
+      // energies leave the launch from its last evaluation only
+      force_eval_block<T, LDSM>(V, B.prof, first, s == nst - 1);
       if (s >= 0) {
         t = B.prof ? wall_clock64() : 0;
         integrate2(V, gv, lg, R, s, threadIdx.x, blockDim.x);

@@ -90,18 +90,18 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   const AwFMPack &fm = plan.fm;
   if (fp32) {
     std::vector<FMItem> items;
-    std::vector<float> rf;
-    aw_fm_device(fm, items, rf);
+    std::vector<float> rf, coef;
+    aw_fm_device(fm, items, rf, coef);
     put(img, plan.off_items, items);
     put(img, plan.off_rf, rf);
-    put(img, plan.off_coef, std::vector<float>(fm.coef.begin(), fm.coef.end()));
+    put(img, plan.off_coef, coef);
   } else {
     std::vector<FMItemD> items;
-    std::vector<double> rf;
-    aw_fm_device(fm, items, rf);
+    std::vector<double> rf, coef;
+    aw_fm_device(fm, items, rf, coef);
     put(img, plan.off_items, items);
     put(img, plan.off_rf, rf);
-    put(img, plan.off_coef, fm.coef);
+    put(img, plan.off_coef, coef);
   }
   plan.n_items = fm.n_items();
   plan.n = n;

@@ -6,6 +6,7 @@
 #include <map>
 #include <string>
 #include <cstring>
+#include <limits>
 #include <vector>
 #include "awsem_types.h"
 
@@ -70,11 +71,30 @@ AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B);
 // it (in T, as the kernels scale the distances), skipped entries (memory or
 // target glycine CB) set to AW_FM_FAR.  Every Rf entry belongs to exactly
 // one item.
+// Weights: a group whose weights are all positive gets log2(w) (padding
+// -inf) and AW_FM_LOGW on its items, so that w exp2(-d^2) = exp2(log2 w - d^2)
+// costs no multiply; other groups keep w.
 template <class T, class ItemT>
-void aw_fm_device(const AwFMPack &B, std::vector<ItemT> &items, std::vector<T> &rf)
+void aw_fm_device(const AwFMPack &B, std::vector<ItemT> &items, std::vector<T> &rf, std::vector<T> &coef)
 {
   items.assign(B.n_items(), ItemT());
   rf.assign(B.rf.size(), (T)AW_FM_FAR);
+  coef.assign(B.coef.begin(), B.coef.end());
+  std::vector<signed char> logw(B.coef.size(), -1);  // per group start: -1 unseen, 0 no, 1 yes
+  for (int k = 0; k < B.n_items(); ++k) {
+    const int *m = &B.meta[5 * k];
+    int c0 = m[4], nm = m[3] & AW_FM_NMEM_MASK;
+    if (logw[c0] < 0) {
+      bool pos = true;
+      for (int q = 0; q < nm; ++q) pos = pos && B.coef[c0 + q] > 0.0;
+      logw[c0] = pos ? 1 : 0;
+      if (pos) {
+        int len = (nm + AW_FM_PAD - 1) / AW_FM_PAD * AW_FM_PAD;
+        for (int q = 0; q < len; ++q)
+          coef[c0 + q] = q < nm ? (T)std::log2(B.coef[c0 + q]) : -std::numeric_limits<T>::infinity();
+      }
+    }
+  }
   for (int k = 0; k < B.n_items(); ++k) {
     ItemT &it = items[k];
     std::memset(&it, 0, sizeof it);
@@ -83,7 +103,8 @@ void aw_fm_device(const AwFMPack &B, std::vector<ItemT> &items, std::vector<T> &
     it.gamma = (T)B.items[3 * k];
     it.sc = sc;
     it.invs = (T)(B.items[3 * k + 2] / (double)sc);
-    it.ij = m[0]; it.rf0 = m[1]; it.stride = m[2]; it.nmem = m[3]; it.coef0 = m[4];
+    it.ij = m[0]; it.rf0 = m[1]; it.stride = m[2]; it.nmem = m[3] | (logw[m[4]] > 0 ? AW_FM_LOGW : 0);
+    it.coef0 = m[4];
     bool gi = m[3] & AW_FM_GLY_I, gj = m[3] & AW_FM_GLY_J;
     bool live[4] = {true, !gj, !gi, !gi && !gj};
     int nmem = m[3] & AW_FM_NMEM_MASK;

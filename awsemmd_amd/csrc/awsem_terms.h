@@ -698,17 +698,20 @@ AW_DEV typename T4<T>::type ld_global4(const typename T4<T>::type *p)
 // one memory's four Gaussian wells of one pair: rf = (CA-CA, CA(i)-CB(j),
 // CA(j)-CB(i), CB-CB) pre-scaled by the item's sc, AW_FM_FAR where the
 // Gaussian is skipped
-template <class T4t, class T>
+// EF: accumulate the energy too (only the evaluation whose energies are
+// reported needs it)
+// LW: w is log2 of the weight, which then enters the exponent
+template <bool EF, bool LW, class T4t, class T>
 AW_DEV void fm_gauss(const T4t &f, T w, const T *r, T &S0, T &S1, T &S2, T &S3, T &Es)
 {
   // r and f are pre-scaled by the item's sc: exp2(-d^2) is the Gaussian
   T d0 = r[0] - f.x, d1 = r[1] - f.y, d2 = r[2] - f.z, d3 = r[3] - f.w;
-  T v0 = w * t_exp2(-(d0 * d0));
-  T v1 = w * t_exp2(-(d1 * d1));
-  T v2 = w * t_exp2(-(d2 * d2));
-  T v3 = w * t_exp2(-(d3 * d3));
+  T v0 = LW ? t_exp2(w - d0 * d0) : w * t_exp2(-(d0 * d0));
+  T v1 = LW ? t_exp2(w - d1 * d1) : w * t_exp2(-(d1 * d1));
+  T v2 = LW ? t_exp2(w - d2 * d2) : w * t_exp2(-(d2 * d2));
+  T v3 = LW ? t_exp2(w - d3 * d3) : w * t_exp2(-(d3 * d3));
   S0 += v0 * d0; S1 += v1 * d1; S2 += v2 * d2; S3 += v3 * d3;
-  Es += (v0 + v1) + (v2 + v3);
+  if (EF) Es += (v0 + v1) + (v2 + v3);
 }
 
 // force sinks of fm_run.  fm_run is compiled as a function of its own, so
@@ -739,7 +742,8 @@ struct FmSinkForce {
 // computed in further batches of 4 without overlap.  Loads past nmem re-read
 // the last memory, whose weight there is the zero padding of coef (no
 // branches on the data).  Coordinates x[3][n][3] (CA, CB, O) are in LDS when
-// XLDS; forces go to sink(kind, residue, fx, fy, fz); returns the energy.
+// XLDS; forces go to sink(kind, residue, fx, fy, fz); returns the energy
+// (EF; 0 otherwise).
 // A function of its own (not inlined): it gets its own register allocation
 // for the batches in flight, and is called once per pass.
 template <class T>
@@ -786,7 +790,7 @@ AW_DEV void fm_load(const char *rfb, const char *cfb, const FmRow &a, int m0, in
   }
 }
 
-template <class T, bool XLDS, class Sink>
+template <class T, bool XLDS, bool EF, class Sink>
 AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
                           const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
 {
@@ -837,25 +841,47 @@ AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
       rs[q] = r[q] * sc;
     }
     T S0 = 0, S1 = 0, S2 = 0, S3 = 0, Es = 0;
+    bool lw = cur.nmem & AW_FM_LOGW;
+    if (lw) {
 #pragma unroll
-    for (int u = 0; u < FMB; ++u) {
-      fm_gauss(fa[u], ca[u], rs, S0, S1, S2, S3, Es);
-      if ((u & 3) == 3) AW_SCHED_FENCE();  // bounds the interleaving (register pressure)
+      for (int u = 0; u < FMB; ++u) {
+        fm_gauss<EF, true>(fa[u], ca[u], rs, S0, S1, S2, S3, Es);
+        if ((u & 3) == 3) AW_SCHED_FENCE();  // bounds the interleaving (register pressure)
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < FMB; ++u) {
+        fm_gauss<EF, false>(fa[u], ca[u], rs, S0, S1, S2, S3, Es);
+        if ((u & 3) == 3) AW_SCHED_FENCE();
+      }
     }
     // the next item's first batch is in flight while this one finishes (past
     // the lane's last item, nxt is a valid item loaded again and unused)
     fm_load(rfb, cfb, fm_row<T>(nxt), 0, FMB, fa, ca);
+    // a second batch past the group's end weighs 0 (exp2(-inf) = 0 for log
+    // weights)
+    T w0 = lw ? -(T)INFINITY : (T)0;
+    if (lw) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) fm_gauss(fb[u], tail ? cb[u] : (T)0, rs, S0, S1, S2, S3, Es);
+      for (int u = 0; u < 4; ++u) fm_gauss<EF, true>(fb[u], tail ? cb[u] : w0, rs, S0, S1, S2, S3, Es);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fm_gauss<EF, false>(fb[u], tail ? cb[u] : w0, rs, S0, S1, S2, S3, Es);
+    }
     for (int m0 = FMB + 4; m0 < nmem; m0 += 4) {
       fm_load(rfb, cfb, row, m0, 4, fb, cb);
+      if (lw) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) fm_gauss(fb[u], cb[u], rs, S0, S1, S2, S3, Es);
+        for (int u = 0; u < 4; ++u) fm_gauss<EF, true>(fb[u], cb[u], rs, S0, S1, S2, S3, Es);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) fm_gauss<EF, false>(fb[u], cb[u], rs, S0, S1, S2, S3, Es);
+      }
     }
     // the weights entered with the sign of cf; the energy is -cf e
     bool act1 = !gj, act2 = !gi, act3 = !gi && !gj;
     T g = -(T)cur.gamma, invs = (T)cur.invs;
-    efm += (double)(g * Es);
+    if (EF) efm += (double)(g * Es);
     T f0 = t_div(g * S0 * invs, r[0]);
     T f1 = act1 ? t_div(g * S1 * invs, r[1]) : (T)0;
     T f2 = act2 ? t_div(g * S2 * invs, r[2]) : (T)0;
@@ -1469,9 +1495,11 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
 // Unqueued (grid-wide phase engine): every term is evaluated in place.
 // pm (clocked runs only): pm->mark(k) after each barrier of the queued pair
 // pass, k = 12 scan, 13 DSSP, 14 P_AP, 15 excluded volume
-// XLDS: the working slab (coordinates, forces) is in LDS
+// XLDS: the working slab (coordinates, forces) is in LDS; ef: this
+// evaluation's energies are reported (fragment memory skips its energy sums
+// otherwise)
 template <class T, bool QUEUED, class PM = void, bool XLDS = false>
-AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr)
+AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr, bool ef = true)
 {
   EAcc E;
   E.clear();
@@ -1479,8 +1507,11 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
   int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM) && (part & 1)) ? V.fm_items : 0;
   if (nfm > 0) {
     typedef typename T4<T>::item ItemT;
-    E.fm += fm_run<T, XLDS>(V.x(), V.n, V.rf, V.coef, (const ItemT *)V.items + V.fm_item_base, t0, nfm, ts,
-                            FmSinkForce<XLDS>{V.f(), V.n});
+    const ItemT *items = (const ItemT *)V.items + V.fm_item_base;
+    if (ef)
+      E.fm += fm_run<T, XLDS, true>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, FmSinkForce<XLDS>{V.f(), V.n});
+    else
+      fm_run<T, XLDS, false>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, FmSinkForce<XLDS>{V.f(), V.n});
   }
   if (part & 2)
     for (int k = t0; k < V.n; k += ts) residue_local(V, k, E);

@@ -97,7 +97,8 @@ struct FFParams {
 #define AW_FM_FAR 1.0e6
 #define AW_FM_GLY_I (1 << 30)  // flags in nmem: residue i / j is glycine
 #define AW_FM_GLY_J (1 << 29)
-#define AW_FM_NMEM_MASK ((1 << 29) - 1)
+#define AW_FM_LOGW (1 << 28)   // flag in nmem: the device weights of the group are log2(w), all w > 0
+#define AW_FM_NMEM_MASK ((1 << 28) - 1)
 struct FMItem {
   float gamma, sc, invs;     // gamma, sqrt(log2(e)/(2 sigma^2)), 1/(sigma^2 sc)
   int ij;                    // (i << 16) | j, replica-local residue indices

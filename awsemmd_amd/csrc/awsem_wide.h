@@ -314,12 +314,18 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   if (local)
     for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) fl[k] = 0.0;
   __syncthreads();
-  if (local)
-    E.fm += fm_run<T, false>(V.x(), V.n, V.rf, V.coef, items, k0 + (int)threadIdx.x, k1, (int)blockDim.x,
-                             FmSinkLds{fl, rlo, span});
-  else
-    E.fm += fm_run<T, false>(V.x(), V.n, V.rf, V.coef, items, k0 + (int)threadIdx.x, k1, (int)blockDim.x,
-                             FmSinkForce<false>{V.f(), V.n});
+  // energies leave a launch from the replica's last evaluation only
+  bool ef = s < 0 || s == R.nsteps - 1;
+  int kk = k0 + (int)threadIdx.x, ks = (int)blockDim.x;
+  if (local) {
+    FmSinkLds sink{fl, rlo, span};
+    if (ef) E.fm += fm_run<T, false, true>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    else fm_run<T, false, false>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+  } else {
+    FmSinkForce<false> sink{V.f(), V.n};
+    if (ef) E.fm += fm_run<T, false, true>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    else fm_run<T, false, false>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+  }
   __syncthreads();
   if (local)
     for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) {
