@@ -1382,12 +1382,14 @@ AW_DEV void for_pairs(const View<T> &V, int t0, int ts, F &&body)
 template <class T, class F>
 AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
 {
+  // body(i, j, queued): queued pairs passed the site-distance test of the
+  // scan, which implies the near_pair test (|CA-CA| <= |site-site| + extents)
   const int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
   if (qs) {
-    for_packed(qs, 0, V.nl_state[5], t0, ts, body);
+    for_packed(qs, 0, V.nl_state[5], t0, ts, [&](int i, int j) { body(i, j, true); });
     return;
   }
-  for_pairs(V, t0, ts, body);
+  for_pairs(V, t0, ts, [&](int i, int j) { body(i, j, false); });
 }
 
 // ------------------------------------------------------------------------
@@ -1653,13 +1655,14 @@ AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
 }
 
 template <class T>
-AW_DEV void xi_pair(const View<T> &V, int i, int j)
+AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
 {
   const FFParams *P = V.P;
   int ri = V.rinfo[i], rj = V.rinfo[j];
   if (!(rchain(ri) != rchain(rj) || j - i >= P->contact_cutoff)) return;
-  if (!near_pair(V, i, j, (float)P->dens_reach)) return;
+  if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
   double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
+  double r = sqrt(rsq);
   int it = rtype(ri), jt = rtype(rj);
   double acc_i = 0.0, acc_j = 0.0;
   for (int w = 0; w < P->n_wells; ++w) {
@@ -1668,7 +1671,7 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j)
     // fb.cpp:7925 reads !fabs(g0-g1)<delta, i.e. (!fabs(g0-g1)) < delta: true iff g0 != g1
     if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
       T th, prd;
-      wtheta<T>(P, w, sqrt(rsq), th, prd);
+      wtheta<T>(P, w, r, th, prd);
       double tg = (g1 - g0) * (double)th;
       acc_i += tg * V.sig_w()[j];
       acc_j += tg * V.sig_w()[i];
@@ -1687,7 +1690,7 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts)
   int n = V.n;
   int nh = P->helix_flag ? n : 0;
   for (int k = t0; k < nh; k += ts) helix_item(V, k, E);
-  if (P->water_flag) for_density_pairs(V, t0, ts, [&](int i, int j) { xi_pair(V, i, j); });
+  if (P->water_flag) for_density_pairs(V, t0, ts, [&](int i, int j, bool q) { xi_pair(V, i, j, q); });
   eflush(E, V.ered);
 }
 
@@ -1701,19 +1704,22 @@ AW_DEV void phase_E(const View<T> &V, int t0, int ts)
 // Phase F: DL3 density-site pair forces (fb.cpp:7996-8057, scatter 8116-8128)
 
 template <class T>
-AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
+AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
 {
   const FFParams *P = V.P;
   int ri = V.rinfo[i], rj = V.rinfo[j];
   bool same = rchain(ri) == rchain(rj);
   int dres = j - i;
   if (!(!same || dres > 1)) return;
-  if (!near_pair(V, i, j, (float)P->dens_reach)) return;
+  if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
   double rsq = rsq_exact(si, sj);
   double r = sqrt(rsq);
   T force = 0;
   bool in0 = rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0];
+  // well 0's theta'/r, shared by the water, water-xi and burial parts
+  T th0 = 0, prd0 = 0;
+  if (in0) wtheta<T>(P, 0, r, th0, prd0);
   if (P->water_flag && ON(P, AW_ET_WATER)) {
     int it = rtype(ri), jt = rtype(rj);
     double ew = 0.0;
@@ -1723,8 +1729,8 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
         double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1], sg;
         if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
         else sg = g0 + (g1 - g0) * V.sig_w()[i] * V.sig_w()[j];
-        T th, prd;
-        wtheta<T>(P, w, r, th, prd);
+        T th = th0, prd = prd0;
+        if (w != 0) wtheta<T>(P, w, r, th, prd);
         ew += -sg * (double)th;
         force += (T)sg * prd;
       }
@@ -1734,18 +1740,10 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E)
       double xi = 0.0;
       if (fabs(V.xi_w()[i]) > AW_DELTA_WATER_XI) xi += V.xi_w()[i];
       if (fabs(V.xi_w()[j]) > AW_DELTA_WATER_XI) xi += V.xi_w()[j];
-      if (xi != 0.0) {
-        T th, prd;
-        wtheta<T>(P, 0, r, th, prd);
-        force += prd * (T)xi;
-      }
+      if (xi != 0.0) force += prd0 * (T)xi;
     }
   }
-  if (P->burial_flag && in0 && ON(P, AW_ET_BURIAL)) {
-    T th, prd;
-    wtheta<T>(P, 0, r, th, prd);
-    force += (T)(V.bf()[i] + V.bf()[j]) * prd;
-  }
+  if (P->burial_flag && in0 && ON(P, AW_ET_BURIAL)) force += (T)(V.bf()[i] + V.bf()[j]) * prd0;
   if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq && ON(P, AW_ET_HELIX)) {
     int hd = P->helix_i_diff;
     double factor = V.hxi1()[i] + V.hxi1()[j];
@@ -1770,7 +1768,7 @@ AW_DEV void phase_F(const View<T> &V, int t0, int ts)
 {
   EAcc E;
   E.clear();
-  for_density_pairs(V, t0, ts, [&](int i, int j) { force_pair(V, i, j, E); });
+  for_density_pairs(V, t0, ts, [&](int i, int j, bool q) { force_pair(V, i, j, E, q); });
   eflush(E, V.ered);
 }
 
