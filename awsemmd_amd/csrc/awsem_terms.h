@@ -790,7 +790,10 @@ AW_DEV void fm_load(const char *rfb, const char *cfb, const FmRow &a, int m0, in
   }
 }
 
-template <class T, bool XLDS, bool EF, class Sink>
+// PIPE = false (lanes with one or two items, as in the grid-wide phase
+// engine): no prefetch of the next item's first batch, each item loads its
+// own
+template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true>
 AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
                           const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
 {
@@ -807,8 +810,9 @@ AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
   ItemT nxt = k + ks < k1 ? ld_item(items + k + ks) : cur;
   T4t fa[FMB];
   T ca[FMB];
-  fm_load(rfb, cfb, fm_row<T>(cur), 0, FMB, fa, ca);
+  if (PIPE) fm_load(rfb, cfb, fm_row<T>(cur), 0, FMB, fa, ca);
   for (; k < k1; k += ks) {
+    if (!PIPE) fm_load(rfb, cfb, fm_row<T>(cur), 0, FMB, fa, ca);
     int nmem = cur.nmem & AW_FM_NMEM_MASK;
     // second batch: memories FMB..FMB+3, or (no such memories) a re-read of
     // the first four with weight 0; loads and compute are unconditional so
@@ -857,7 +861,7 @@ AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
     }
     // the next item's first batch is in flight while this one finishes (past
     // the lane's last item, nxt is a valid item loaded again and unused)
-    fm_load(rfb, cfb, fm_row<T>(nxt), 0, FMB, fa, ca);
+    if (PIPE) fm_load(rfb, cfb, fm_row<T>(nxt), 0, FMB, fa, ca);
     // a second batch past the group's end weighs 0 (exp2(-inf) = 0 for log
     // weights)
     T w0 = lw ? -(T)INFINITY : (T)0;
@@ -1510,10 +1514,12 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
   if (nfm > 0) {
     typedef typename T4<T>::item ItemT;
     const ItemT *items = (const ItemT *)V.items + V.fm_item_base;
+    // the queued (resident) pass gives a lane many items: pipelined loads
+    typedef FmSinkForce<XLDS> Sk;
     if (ef)
-      E.fm += fm_run<T, XLDS, true>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, FmSinkForce<XLDS>{V.f(), V.n});
+      E.fm += fm_run<T, XLDS, true, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{V.f(), V.n});
     else
-      fm_run<T, XLDS, false>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, FmSinkForce<XLDS>{V.f(), V.n});
+      fm_run<T, XLDS, false, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{V.f(), V.n});
   }
   if (part & 2)
     for (int k = t0; k < V.n; k += ts) residue_local(V, k, E);
