@@ -84,3 +84,30 @@ def test_autotune_restores_state_and_keeps_a_candidate_skin():
     d = np.abs(b.get_state()[0] - ref.get_state()[0]).max()
     print(f"tuned vs untuned after 20 steps: {d:.2e} A")
     assert d <= 1e-6
+
+
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_energies_after_run_are_the_final_states(engine, prec):
+    """A launch sums energies in its last evaluation only: after run(n) the
+    reported per-term energies equal a fresh evaluation of the final
+    coordinates (FP64 to rounding, FP32 to 1e-6 relative; the stored forces
+    also hold the Langevin drag and noise of the last step, so they are not
+    compared)."""
+    A = _gpu()
+    cases = [make_case([L], 80 + k) for k, L in enumerate([70, 45, 110])]
+    x0 = A.pack_states([s.coords() for s, _ in cases])
+    b = _batch(A, cases, prec)
+    b.set_engine(engine)
+    b.set_state(x0, np.zeros_like(x0))
+    b.run(23, seed=7, step0=0)
+    x, v, f, e = b.get_state()
+    ref = _batch(A, cases, prec)
+    ref.set_engine(engine)
+    ref.set_state(x, v)
+    ref.compute()
+    er = ref.get_state()[3]
+    tol = 1e-12 if prec == 0 else 1e-6
+    de = np.abs(e - er) / np.maximum(1.0, np.abs(er))
+    print(f"{engine} prec {prec}: max rel dE {de.max():.2e}")
+    assert np.all(np.isfinite(e)) and de.max() <= tol
