@@ -744,8 +744,9 @@ struct FmSinkForce {
 // branches on the data).  Coordinates x[3][n][3] (CA, CB, O) are in LDS when
 // XLDS; forces go to sink(kind, residue, fx, fy, fz); returns the energy
 // (EF; 0 otherwise).
-// A function of its own (not inlined): it gets its own register allocation
-// for the batches in flight, and is called once per pass.
+// fm_run (below) is the same code as a function of its own (not inlined):
+// it gets its own register allocation for the batches in flight, and is
+// called once per pass; the wide engine inlines fm_run_body unpipelined.
 template <class T>
 struct FmBatch {
   static const int FMB = sizeof(T) == 4 ? 16 : 8;
@@ -793,13 +794,13 @@ AW_DEV void fm_load(const char *rfb, const char *cfb, const FmRow &a, int m0, in
 // PIPE = false (lanes with one or two items, as in the grid-wide phase
 // engine): no prefetch of the next item's first batch, each item loads its
 // own
-template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true>
-AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
+template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true, int FMB = FmBatch<T>::FMB>
+AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
                           const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
 {
+  static_assert(FMB % 4 == 0 && FMB <= AW_FM_PAD, "first batch: whole weight vectors within the padding");
   typedef typename T4<T>::type T4t;
   typedef typename T4<T>::item ItemT;
-  const int FMB = FmBatch<T>::FMB;
   if (k >= k1) return 0.0;
   // the bases are the same in every lane: scalar registers
   const char *rfb = (const char *)aw_uniform(rf), *cfb = (const char *)aw_uniform(coef);
@@ -898,6 +899,15 @@ AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
     if (k + 2 * ks < k1) nxt = ld_item(items + k + 2 * ks);
   }
   return efm;
+}
+
+// the same as a function of its own (the resident engine's call: its own
+// register allocation for the batches in flight)
+template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true>
+AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
+                          const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
+{
+  return fm_run_body<T, XLDS, EF, Sink, PIPE>(x, n, rf, coef, items, k, k1, ks, sink);
 }
 
 // ------------------------------------------------------------------------

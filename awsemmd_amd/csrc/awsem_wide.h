@@ -317,14 +317,20 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   // energies leave a launch from the replica's last evaluation only
   bool ef = s < 0 || s == R.nsteps - 1;
   int kk = k0 + (int)threadIdx.x, ks = (int)blockDim.x;
+  // inlined, unpipelined, first batches of 8 memories: a lane has a few
+  // items, and the kernel keeps more waves per SIMD with the smaller
+  // register footprint
+  const int W_FMB = sizeof(T) == 4 ? 8 : 4;
   if (local) {
-    FmSinkLds sink{fl, rlo, span};
-    if (ef) E.fm += fm_run<T, false, true>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
-    else fm_run<T, false, false>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    typedef FmSinkLds Sk;
+    Sk sink{fl, rlo, span};
+    if (ef) E.fm += fm_run_body<T, false, true, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    else fm_run_body<T, false, false, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
   } else {
-    FmSinkForce<false> sink{V.f(), V.n};
-    if (ef) E.fm += fm_run<T, false, true>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
-    else fm_run<T, false, false>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    typedef FmSinkForce<false> Sk;
+    Sk sink{V.f(), V.n};
+    if (ef) E.fm += fm_run_body<T, false, true, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    else fm_run_body<T, false, false, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
   }
   __syncthreads();
   if (local)

@@ -215,6 +215,29 @@ def test_batch_fp64_dynamics_matches_oracle(oracle_mod, engine):
         assert dx < 1e-8 and dv < 1e-7, r
 
 
+def test_single_chain_n100_1000_steps_against_oracle(oracle_mod):
+    """The CPU-runnable configuration end to end: one N=100 chain with every
+    term on (fragment memory: 20 memories per 9-residue window), FP64
+    Langevin for 1000 steps at 300 K, dt 2 fs, damp 1000 fs, through the
+    plain drop-in (awsem_run), against the oracle integrating the same
+    counter-based noise from the same start."""
+    A = _gpu()
+    s, p = make_case([100], 100)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    x0 = s.coords()
+    v0 = np.zeros_like(x0)
+    lg = dict(dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0, seed=11)
+    x, v, f, e = fix.run(x0, v0, 1000, step0=0, **lg)
+    fo, _ = o.compute(x0)
+    xo, vo, fo, eo = o.langevin(x0, v0, fo, 1000, replica=0, step0=0, **lg)
+    dx = np.abs(x - xo).max()
+    dv = np.abs(v - vo).max() / np.abs(vo).max()
+    de = np.max(np.abs(e - eo) / np.maximum(1.0, np.abs(eo)))
+    print(f"after 1000 steps: max|dx|={dx:.2e} A, rel dv={dv:.2e}, max rel dE={de:.2e}")
+    assert dx < 1e-6 and dv < 1e-6 and de < 1e-8
+
+
 def test_plain_fp64_run_checked_along_trajectory(oracle_mod):
     """The FP64 drop-in (awsem_run, phase engine) over 500 Langevin steps of a
     300-residue chain; every 100 steps the GPU energies and forces at the
