@@ -20,6 +20,7 @@ inline float aw_fast_exp(float a) { return expf(a); }
 inline float aw_fast_exp2(float a) { return exp2f(a); }
 inline float aw_fsqrt(float a) { return sqrtf(a); }
 inline float aw_frcp(float a) { return 1.0f / a; }
+inline float aw_frsq(float a) { return 1.0f / sqrtf(a); }
 inline void aadd(double *p, double v) { *p += v; }
 inline void amax_i(int *p, int v) { if (v > *p) *p = v; }
 inline int aadd_i(int *p, int v) { int o = *p; *p += v; return o; }
@@ -54,6 +55,7 @@ __device__ __forceinline__ float aw_fast_exp(float a) { return __expf(a); }
 __device__ __forceinline__ float aw_fast_exp2(float a) { return __builtin_amdgcn_exp2f(a); }
 __device__ __forceinline__ float aw_fsqrt(float a) { return __fsqrt_rn(a); }
 __device__ __forceinline__ float aw_frcp(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ float aw_frsq(float a) { return __builtin_amdgcn_rsqf(a); }
 __device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void amax_i(int *p, int v) { atomicMax(p, v); }
 __device__ __forceinline__ int aadd_i(int *p, int v) { return atomicAdd(p, v); }
@@ -161,6 +163,18 @@ AW_DEV double t_exp(double a) { return exp(a); }
 AW_DEV float t_exp2(float a) { return aw_fast_exp2(a); }
 AW_DEV double t_exp2(double a) { return exp2(a); }
 AW_DEV float t_sqrt(float a) { return aw_fsqrt(a); }
+// sqrt of an FP64 value that no predicate depends on: FP64 exact, or (FP32
+// path) the FP32 reciprocal square root refined by one FP64 Newton step
+// (relative error ~1e-13: enough for the FP64-formed offsets and switching
+// arguments it feeds)
+template <class T>
+AW_DEV double sqrt_t(double x)
+{
+  if (sizeof(T) == 8) return sqrt(x);
+  double y = (double)aw_frsq(fmaxf((float)x, 1e-30f));
+  y = y * (1.5 - 0.5 * x * y * y);
+  return x * y;
+}
 // a / b: FP32 through the hardware reciprocal (1 ulp), FP64 exact
 AW_DEV float t_div(float a, float b) { return a * aw_frcp(b); }
 AW_DEV double t_div(double a, double b) { return a / b; }
@@ -985,7 +999,7 @@ AW_DEV void pap_nu(const View<T> &V, int i, int j, T &nu, T &prd)
   if (drsq > P->pap_drmax_sq) { nu = 0; prd = 0; }
   else if (drsq < P->pap_drmin_sq) { nu = 1; prd = 0; }
   else {
-    double dr = sqrt(drsq);
+    double dr = sqrt_t<T>(drsq);
     tanh_switch<T>(P->pap_pref * (P->pap_cut - dr), P->pap_pref, dr, nu, prd);
   }
 }
@@ -1122,8 +1136,8 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     if (!on[k]) continue;
     // distances and their offsets from the well centres in FP64 (the
     // offsets are small differences of ~3 A quantities)
-    double RNO = sqrt(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_N, don[k])));
-    double RHO = sqrt(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_H, don[k])));
+    double RNO = sqrt_t<T>(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_N, don[k])));
+    double RHO = sqrt_t<T>(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_H, don[k])));
     T eNO = (T)(RNO - P->NO_zero), eHO = (T)(RHO - P->HO_zero);
     T dNO = eNO * (T)P->sigma_NO_sqinv;
     T dHO = eHO * (T)P->sigma_HO_sqinv;
@@ -1458,13 +1472,13 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
       double rsq = rsq_exact(si, sj);
       if (wc && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
         T th, prd;
-        wtheta<T>(P, 0, sqrt(rsq), th, prd);
+        wtheta<T>(P, 0, sqrt_t<T>(rsq), th, prd);
         aadd(&V.rho_w()[i], (double)th);
         aadd(&V.rho_w()[j], (double)th);
       }
       if (hc && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
         T th, prd;
-        htheta<T>(P, sqrt(rsq), th, prd);
+        htheta<T>(P, sqrt_t<T>(rsq), th, prd);
         aadd(&V.rho_h()[i], (double)th);
         aadd(&V.rho_h()[j], (double)th);
       }
@@ -1678,7 +1692,7 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
   if (!(rchain(ri) != rchain(rj) || j - i >= P->contact_cutoff)) return;
   if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
   double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
-  double r = sqrt(rsq);
+  double r = sqrt_t<T>(rsq);
   int it = rtype(ri), jt = rtype(rj);
   double acc_i = 0.0, acc_j = 0.0;
   for (int w = 0; w < P->n_wells; ++w) {
@@ -1730,7 +1744,7 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
   if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
   double rsq = rsq_exact(si, sj);
-  double r = sqrt(rsq);
+  double r = sqrt_t<T>(rsq);
   T force = 0;
   bool in0 = rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0];
   // well 0's theta'/r, shared by the water, water-xi and burial parts
