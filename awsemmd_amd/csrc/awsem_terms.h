@@ -808,11 +808,12 @@ AW_DEV void fm_load(const char *rfb, const char *cfb, const FmRow &a, int m0, in
 // PIPE = false (lanes with one or two items, as in the grid-wide phase
 // engine): no prefetch of the next item's first batch, each item loads its
 // own
-template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true, int FMB = FmBatch<T>::FMB>
+template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true, int FMB = FmBatch<T>::FMB, int TB = 4>
 AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
                           const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
 {
   static_assert(FMB % 4 == 0 && FMB <= AW_FM_PAD, "first batch: whole weight vectors within the padding");
+  static_assert(TB % 4 == 0 && TB <= AW_FM_PAD, "second batch: whole weight vectors");
   typedef typename T4<T>::type T4t;
   typedef typename T4<T>::item ItemT;
   if (k >= k1) return 0.0;
@@ -829,14 +830,14 @@ AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
   for (; k < k1; k += ks) {
     if (!PIPE) fm_load(rfb, cfb, fm_row<T>(cur), 0, FMB, fa, ca);
     int nmem = cur.nmem & AW_FM_NMEM_MASK;
-    // second batch: memories FMB..FMB+3, or (no such memories) a re-read of
-    // the first four with weight 0; loads and compute are unconditional so
+    // second batch: memories FMB..FMB+TB-1, or (no such memories) a re-read
+    // of the first TB with weight 0; loads and compute are unconditional so
     // that the wait for this batch leaves the next item's loads in flight
-    T4t fb[4];
-    T cb[4];
+    T4t fb[TB];
+    T cb[TB];
     bool tail = nmem > FMB;
     FmRow row = fm_row<T>(cur);
-    fm_load(rfb, cfb, row, tail ? FMB : 0, 4, fb, cb);
+    fm_load(rfb, cfb, row, tail ? FMB : 0, TB, fb, cb);
     AW_SCHED_FENCE();  // the loads stay ahead of the first batch's arithmetic
     int i = cur.ij >> 16, j = cur.ij & 0xffff;
     bool gi = cur.nmem & AW_FM_GLY_I, gj = cur.nmem & AW_FM_GLY_J;
@@ -879,15 +880,21 @@ AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
     if (PIPE) fm_load(rfb, cfb, fm_row<T>(nxt), 0, FMB, fa, ca);
     // a second batch past the group's end weighs 0 (exp2(-inf) = 0 for log
     // weights)
+    // (when the batch may reach past a group's zero-padded weight block, its
+    // memories past nmem are masked explicitly)
     T w0 = lw ? -(T)INFINITY : (T)0;
+    T wb[TB];
+#pragma unroll
+    for (int u = 0; u < TB; ++u)
+      wb[u] = (tail && (FMB + TB <= AW_FM_PAD || FMB + u < nmem)) ? cb[u] : w0;
     if (lw) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) fm_gauss<EF, true>(fb[u], tail ? cb[u] : w0, rs, S0, S1, S2, S3, Es);
+      for (int u = 0; u < TB; ++u) fm_gauss<EF, true>(fb[u], wb[u], rs, S0, S1, S2, S3, Es);
     } else {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) fm_gauss<EF, false>(fb[u], tail ? cb[u] : w0, rs, S0, S1, S2, S3, Es);
+      for (int u = 0; u < TB; ++u) fm_gauss<EF, false>(fb[u], wb[u], rs, S0, S1, S2, S3, Es);
     }
-    for (int m0 = FMB + 4; m0 < nmem; m0 += 4) {
+    for (int m0 = FMB + TB; m0 < nmem; m0 += 4) {
       fm_load(rfb, cfb, row, m0, 4, fb, cb);
       if (lw) {
 #pragma unroll
@@ -921,7 +928,11 @@ template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true>
 AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
                           const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
 {
-  return fm_run_body<T, XLDS, EF, Sink, PIPE>(x, n, rf, coef, items, k, k1, ks, sink);
+  // FP32: a first batch of 12 and a second of 8 (the shipped 20 memories
+  // per window): the second batch's arithmetic covers more of the next
+  // item's load latency
+  const int FMB = sizeof(T) == 4 ? 12 : FmBatch<T>::FMB, TB = sizeof(T) == 4 ? 8 : 4;
+  return fm_run_body<T, XLDS, EF, Sink, PIPE, FMB, TB>(x, n, rf, coef, items, k, k1, ks, sink);
 }
 
 // ------------------------------------------------------------------------
