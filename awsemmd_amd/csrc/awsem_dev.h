@@ -63,7 +63,7 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
   V.ch_len = R.ch_len;
   V.s = slab;
   V.ered = nullptr;
-  V.P = B.P;
+  V.P = (ParamPtr)B.P;
   V.fm_items = R.fm_items;
   V.fm_item_base = 0;
   V.coef = (const T *)R.coef;
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     const ReplicaDesc &R = B.rep[r];
     double *slab = LDSM ? smem : R.scratch;
     View<T> V = make_view<T>(B, R, slab);
-    V.P = Pr;  // read-only, unaliased: uniform parameter loads go through the scalar cache
+    V.P = (ParamPtr)Pr;  // read-only: invariant uniform loads through the scalar cache
     V.ered = ered;
     V.nl = R.nl_cap > 0 ? R.nl : nullptr;
     V.nlx = R.nlx;

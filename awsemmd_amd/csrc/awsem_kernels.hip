@@ -121,7 +121,7 @@ template <class T>
 __global__ void __launch_bounds__(256) k_list(BatchDev B, int r, int kind, int arg, long long *out, int cap, int *count)
 {
   View<T> V = phase_view<T>(B, r);
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int n = V.n;
   int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
   if (kind == 4) {

@@ -29,6 +29,7 @@ inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #define AW_BARRIER() ((void)0)
 #define AW_NOINLINE inline
 #define AW_HOSTDEV
+typedef const FFParams *ParamPtr;
 #define AW_SCHED_FENCE() ((void)0)
 template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
@@ -65,6 +66,11 @@ __device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
 // a separately allocated device function (called once per pass)
 #define AW_NOINLINE __device__ __attribute__((noinline))
 #define AW_HOSTDEV __host__ __device__
+// the force-field parameters through the constant address space: loads
+// from it are known invariant, so the compiler hoists them out of the pair
+// loops (a plain pointer in a struct carries no aliasing guarantee, and
+// every atomic in a loop would force them to be reloaded)
+typedef const __attribute__((address_space(4))) FFParams *ParamPtr;
 // instruction-scheduling barrier: nothing is moved across it
 #define AW_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
 // typed accesses for code compiled apart from the kernel (a noinline
@@ -266,7 +272,7 @@ struct View {
   const int *ch_len;
   double *s;            // working slab (LDS or global): see the accessors
   double *ered;         // energy accumulators [N_EACC] (LDS)
-  const FFParams *P;
+  ParamPtr P;
   // fragment memory
   int fm_items, fm_item_base;
   const T *coef;
@@ -393,7 +399,7 @@ AW_DEV int pair_count(int n) { return n * (n - 1) / 2; }
 template <class T>
 AW_DEV void phase_A(const View<T> &V, int t0, int ts)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int n = V.n;
   float dmax = 0.0f, emax = 0.0f;  // this lane's largest drift and extent
   for (int i = t0; i < n; i += ts) {
@@ -457,7 +463,7 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
 template <class T>
 AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
   if (!first && !gly) {
     V3<double> d = dvec<double>(XV(V, K_N, i), XA(V, K_CB, i));
@@ -496,7 +502,7 @@ AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
 template <class T>
 AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   double ks = (double)(P->epsilon * P->k_shake);
   bool last = ri & RI_LAST;
   if (!last) {
@@ -526,7 +532,7 @@ AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E)
 template <class T>
 AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   V3<double> a = dvec<double>(XV(V, K_CP, i), XA(V, K_CA, i));
   V3<double> b = dvec<double>(XA(V, K_CA, i), XV(V, K_N, i));
   V3<double> c = dvec<double>(XA(V, K_CA, i), XA(V, K_CB, i));
@@ -561,7 +567,7 @@ AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
 template <class T>
 AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &cs, double &sn)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   double an = P->an, bn = P->bn, cn = P->cn, ap = P->ap, bp = P->bp, cp = P->cp;
   V3<double> a, b, c;
   if (phi) {
@@ -630,7 +636,7 @@ AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &c
 template <class T>
 AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   double cphi, sphi, cpsi, spsi, dummy;
   dihedral(V, i, true, 0.0, cphi, sphi);
   dihedral(V, i, false, 0.0, cpsi, spsi);
@@ -676,7 +682,7 @@ AW_DEV void bond_one(const View<T> &V, int ka, int ia, int kb, int ib, double r0
 template <class T>
 AW_DEV void bonds_term(const View<T> &V, int i, int ri, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   bond_one(V, K_CA, i, K_O, i, P->bond_r0[1], E);
   bond_one(V, K_CA, i, K_CB, i, (ri & RI_GLY) ? P->bond_r0[4] : P->bond_r0[3], E);
   if (!(ri & RI_LAST)) {
@@ -965,12 +971,12 @@ AW_DEV void well_theta<float>(double kappa, double rmin, double rmax, double r, 
 }
 
 template <class T>
-AW_DEV void wtheta(const FFParams *P, int w, double r, T &th, T &prd)
+AW_DEV void wtheta(ParamPtr P, int w, double r, T &th, T &prd)
 {
   well_theta<T>(P->water_kappa, P->well_r_min[w], P->well_r_max[w], r, th, prd);
 }
 template <class T>
-AW_DEV void htheta(const FFParams *P, double r, T &th, T &prd)
+AW_DEV void htheta(ParamPtr P, double r, T &th, T &prd)
 {
   well_theta<T>(P->helix_kappa, P->helix_r_min, P->helix_r_max, r, th, prd);
 }
@@ -1005,7 +1011,7 @@ AW_DEV void tanh_switch<float>(double a, double pref, double dr, float &nu, floa
 template <class T>
 AW_DEV void pap_nu(const View<T> &V, int i, int j, T &nu, T &prd)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
   if (drsq > P->pap_drmax_sq) { nu = 0; prd = 0; }
   else if (drsq < P->pap_drmin_sq) { nu = 1; prd = 0; }
@@ -1025,7 +1031,7 @@ AW_DEV bool pap_keep(const View<T> &V, int i, int j, T nu)
   if (d > 1e-5 * AW_PAP_DELTA) return true;
   if (d < -1e-5 * AW_PAP_DELTA) return false;
   double nud, prdd;
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
   double dr = sqrt(drsq);
   tanh_switch<double>(P->pap_pref * (P->pap_cut - dr), P->pap_pref, dr, nud, prdd);
@@ -1065,7 +1071,7 @@ AW_DEV bool dssp_nu_below(T nu, double pref, double r, double d0)
 template <class T>
 AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   const int *RI = V.rinfo;
   int ri = RI[i], rj = RI[j];
   int ic = rchain(ri), jc = rchain(rj);
@@ -1088,7 +1094,7 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   int hb_class;
   int nr = P->n_rama_par;
   int dij = j - i < 0 ? i - j : j - i;
-  const double(*h)[9] = P->hbscl;
+  auto h = P->hbscl;
   if (ic == jc && dij < 45) {
     if (dij < 4) {
       lambda[0] = (T)-h[0][0]; lambda[1] = (T)-h[0][1];
@@ -1195,7 +1201,7 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
 template <class T>
 AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   T nu_ij, prd_ij;
   pap_nu(V, i, j, nu_ij, prd_ij);
   if (!pap_keep(V, i, j, nu_ij)) return;
@@ -1242,7 +1248,8 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
 // special_bonds fene: CA(i)-CA(i+1), CA(i)-O(i), O(i)-CA(i+1), CA(i)-CB(i))
 
 AW_DEV int atype(int kind, int ri) { return kind == K_CA ? 1 : (kind == K_O ? 3 : ((ri & RI_GLY) ? 5 : 4)); }
-AW_DEV double ev_pick(const double (*tab)[6], int ta0, int ta1, int tb0, int tb1, bool gi, bool gj)
+template <class TabPtr>
+AW_DEV double ev_pick(TabPtr tab, int ta0, int ta1, int tb0, int tb1, bool gi, bool gj)
 {
   double v00 = tab[ta0][tb0], v01 = tab[ta0][tb1], v10 = tab[ta1][tb0], v11 = tab[ta1][tb1];
   return gi ? (gj ? v11 : v10) : (gj ? v01 : v00);
@@ -1251,7 +1258,7 @@ AW_DEV double ev_pick(const double (*tab)[6], int ta0, int ta1, int tb0, int tb1
 template <class T>
 AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   bool same = rchain(ri) == rchain(rj);
   bool nb = same && (j == i + 1);
   bool shortc = same && (j - i < 5);
@@ -1290,7 +1297,7 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
 template <class T>
 AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int ta = atype(K_CB, ri), tb = 3;
   double lam = P->ev_lambda[ta][tb];
   if (lam == 0.0) return;
@@ -1437,7 +1444,7 @@ AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
 template <class T>
 AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int ri = V.rinfo[i];
   bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
   if (P->chain_flag && ON(P, AW_ET_CHAIN)) chain_term(V, i, ri, E);
@@ -1456,11 +1463,24 @@ template <class T>
 AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr,
                    int *qs = nullptr)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
+  // every parameter the scan reads, loaded before any branch: invariant and
+  // unconditional, so the loads leave the pair loop (conditional ones would
+  // be reissued, and waited for, every iteration)
+  const float reach = (float)P->pair_reach, dreach = (float)P->dens_reach, evreach = (float)P->ev_reach;
+  const bool water = P->water_flag, helix = P->helix_flag;
+  const double wmin2 = P->w_rmin_th_sq[0], wmax2 = P->w_rmax_th_sq[0], hmin2 = P->h_rmin_th_sq, hmax2 = P->h_rmax_th_sq;
+  const double wk = P->water_kappa, wr0 = P->well_r_min[0], wr1 = P->well_r_max[0];
+  const double hk = P->helix_kappa, hr0 = P->helix_r_min, hr1 = P->helix_r_max;
+  const unsigned mask = P->term_mask;  // (no short-circuit: every load unconditional)
+  const bool dssp_on = (P->dssp_flag != 0) & ((mask >> AW_ET_DSSP) & 1u);
+  const bool pap_on = (P->pap_flag != 0) & ((mask >> AW_ET_PAP) & 1u);
+  const bool ev_on = (P->ev_on != 0) & ((mask >> AW_ET_VEXCLUDED) & 1u);
+  const double dcut = P->dssp_cut, dcut2 = P->dssp_cut_sq, pcut2 = P->pap_cutoff_sq;
   // CA-CA distance (FP32) against each term's reach (the near_pair test)
   float rca = rsq_f(XA(V, K_CA, i), XA(V, K_CA, j));
   float eij = V.ext()[i] + V.ext()[j] + 1e-3f;
-  float lim = (float)P->pair_reach + eij;
+  float lim = reach + eij;
   if (!(rca <= lim * lim * 1.0001f)) return;
   int ri = V.rinfo[i], rj = V.rinfo[j];
   bool same = rchain(ri) == rchain(rj);
@@ -1471,25 +1491,20 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
   if (!same || dres > 1) {
     const double *si = site(V, i, ri), *sj = site(V, j, rj);
     float e = rsq_f(si, sj);
-    if (qs) {
-      float dr = (float)P->dens_reach;
-      dq = e < dr * dr * (1.0f + AW_RSQ_MARGIN);
-    }
-    bool wc = P->water_flag && e > (float)P->w_rmin_th_sq[0] * (1.0f - AW_RSQ_MARGIN) &&
-              e < (float)P->w_rmax_th_sq[0] * (1.0f + AW_RSQ_MARGIN);
-    bool hc = P->helix_flag && e > (float)P->h_rmin_th_sq * (1.0f - AW_RSQ_MARGIN) &&
-              e < (float)P->h_rmax_th_sq * (1.0f + AW_RSQ_MARGIN);
+    if (qs) dq = e < dreach * dreach * (1.0f + AW_RSQ_MARGIN);
+    bool wc = water && e > (float)wmin2 * (1.0f - AW_RSQ_MARGIN) && e < (float)wmax2 * (1.0f + AW_RSQ_MARGIN);
+    bool hc = helix && e > (float)hmin2 * (1.0f - AW_RSQ_MARGIN) && e < (float)hmax2 * (1.0f + AW_RSQ_MARGIN);
     if (wc || hc) {
       double rsq = rsq_exact(si, sj);
-      if (wc && rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0]) {
+      if (wc && rsq > wmin2 && rsq < wmax2) {
         T th, prd;
-        wtheta<T>(P, 0, sqrt_t<T>(rsq), th, prd);
+        well_theta<T>(wk, wr0, wr1, sqrt_t<T>(rsq), th, prd);
         aadd(&V.rho_w()[i], (double)th);
         aadd(&V.rho_w()[j], (double)th);
       }
-      if (hc && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq) {
+      if (hc && rsq > hmin2 && rsq < hmax2) {
         T th, prd;
-        htheta<T>(P, sqrt_t<T>(rsq), th, prd);
+        well_theta<T>(hk, hr0, hr1, sqrt_t<T>(rsq), th, prd);
         aadd(&V.rho_h()[i], (double)th);
         aadd(&V.rho_h()[j], (double)th);
       }
@@ -1497,17 +1512,17 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
   }
   // DSSP in both orientations (fb.cpp:8064-8094)
   bool dij = false, dji = false;
-  float ld = (float)P->dssp_cut + eij;
-  if (P->dssp_flag && ON(P, AW_ET_DSSP) && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
+  float ld = (float)dcut + eij;
+  if (dssp_on && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
     if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST))
-      dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), P->dssp_cut, P->dssp_cut_sq);
+      dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), dcut, dcut2);
     if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST))
-      dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), P->dssp_cut, P->dssp_cut_sq);
+      dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), dcut, dcut2);
   }
   // P_AP (fb.cpp:8096-8103)
-  bool pap = P->pap_flag && ON(P, AW_ET_PAP) && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), P->pap_cutoff_sq);
-  float le = (float)P->ev_reach + eij;
-  bool ev = P->ev_on && ON(P, AW_ET_VEXCLUDED) && rca <= le * le * 1.0001f;
+  bool pap = pap_on && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), pcut2);
+  float le = evreach + eij;
+  bool ev = ev_on && rca <= le * le * 1.0001f;
   if (qd) {
     // queue counters nl_state[2..5]: DSSP (both orientations), P_AP,
     // excluded volume, density pairs
@@ -1544,7 +1559,7 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
 {
   EAcc E;
   E.clear();
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM) && (part & 1)) ? V.fm_items : 0;
   if (nfm > 0) {
     typedef typename T4<T>::item ItemT;
@@ -1627,7 +1642,7 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts)
 {
   EAcc E;
   E.clear();
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   for (int i = t0; i < V.n; i += ts) {
     int it = rtype(V.rinfo[i]);
     if (P->water_flag) {
@@ -1662,7 +1677,7 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts)
 template <class T>
 AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int j = i + P->helix_i_diff;
   if (j >= V.n) return;
   int ri = V.rinfo[i], rj = V.rinfo[j];
@@ -1698,7 +1713,7 @@ AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
 template <class T>
 AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int ri = V.rinfo[i], rj = V.rinfo[j];
   if (!(rchain(ri) != rchain(rj) || j - i >= P->contact_cutoff)) return;
   if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
@@ -1727,7 +1742,7 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts)
 {
   EAcc E;
   E.clear();
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int n = V.n;
   int nh = P->helix_flag ? n : 0;
   for (int k = t0; k < nh; k += ts) helix_item(V, k, E);
@@ -1747,7 +1762,7 @@ AW_DEV void phase_E(const View<T> &V, int t0, int ts)
 template <class T>
 AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
 {
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int ri = V.rinfo[i], rj = V.rinfo[j];
   bool same = rchain(ri) == rchain(rj);
   int dres = j - i;

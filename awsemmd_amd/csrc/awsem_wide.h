@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   __syncthreads();
   EAcc E;
   E.clear();
-  const FFParams *P = V.P;
+  ParamPtr P = V.P;
   int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM)) ? R.fm_items : 0;
   typedef typename T4<T>::item ItemT;
   const ItemT *items = (const ItemT *)R.items;
