@@ -1008,33 +1008,41 @@ AW_DEV void tanh_switch<float>(double a, double pref, double dr, float &nu, floa
   prd = 2.0f * (float)pref * e * ip * ip * aw_frcp((float)dr);
 }
 
-template <class T>
-AW_DEV void pap_nu(const View<T> &V, int i, int j, T &nu, T &prd)
+// the P_AP switch's parameters, read once by the caller
+struct PapP {
+  double drmax2, drmin2, pref, cut;
+};
+AW_DEV PapP pap_p(ParamPtr P)
 {
-  ParamPtr P = V.P;
+  PapP q;
+  q.drmax2 = P->pap_drmax_sq; q.drmin2 = P->pap_drmin_sq; q.pref = P->pap_pref; q.cut = P->pap_cut;
+  return q;
+}
+template <class T>
+AW_DEV void pap_nu(const View<T> &V, const PapP &q, int i, int j, T &nu, T &prd)
+{
   double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
-  if (drsq > P->pap_drmax_sq) { nu = 0; prd = 0; }
-  else if (drsq < P->pap_drmin_sq) { nu = 1; prd = 0; }
+  if (drsq > q.drmax2) { nu = 0; prd = 0; }
+  else if (drsq < q.drmin2) { nu = 1; prd = 0; }
   else {
     double dr = sqrt_t<T>(drsq);
-    tanh_switch<T>(P->pap_pref * (P->pap_cut - dr), P->pap_pref, dr, nu, prd);
+    tanh_switch<T>(q.pref * (q.cut - dr), q.pref, dr, nu, prd);
   }
 }
 
 // the nu < 1e-12 early exit of compute_P_AP_potential: an FP32 nu decides
 // unless it lies within a relative 1e-5 of the threshold, where FP64 decides
 template <class T>
-AW_DEV bool pap_keep(const View<T> &V, int i, int j, T nu)
+AW_DEV bool pap_keep(const View<T> &V, const PapP &q, int i, int j, T nu)
 {
   if (sizeof(T) == 8) return !(nu < (T)AW_PAP_DELTA);
   double d = (double)nu - AW_PAP_DELTA;
   if (d > 1e-5 * AW_PAP_DELTA) return true;
   if (d < -1e-5 * AW_PAP_DELTA) return false;
   double nud, prdd;
-  ParamPtr P = V.P;
   double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
   double dr = sqrt(drsq);
-  tanh_switch<double>(P->pap_pref * (P->pap_cut - dr), P->pap_pref, dr, nud, prdd);
+  tanh_switch<double>(q.pref * (q.cut - dr), q.pref, dr, nud, prdd);
   return !(nud < AW_PAP_DELTA);
 }
 
@@ -1072,6 +1080,12 @@ template <class T>
 AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
 {
   ParamPtr P = V.P;
+  // the scalar parameters, read before any branch (invariant loads that
+  // leave the queue loop)
+  const double kd = P->k_dssp, NOz = P->NO_zero, HOz = P->HO_zero, sNO = P->sigma_NO_sqinv, sHO = P->sigma_HO_sqinv;
+  const double pr0 = P->pref[0], pr1 = P->pref[1], dnu0 = P->d_nu0, cut1 = P->nu_cut1_sq, cut2 = P->nu_cut2_sq;
+  const double an_ = P->an, bn_ = P->bn, cn_ = P->cn, ah_ = P->ah, bh_ = P->bh, ch_ = P->ch;
+  const int nr = P->n_rama_par;
   const int *RI = V.rinfo;
   int ri = RI[i], rj = RI[j];
   int ic = rchain(ri), jc = rchain(rj);
@@ -1091,37 +1105,31 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     if (i_P) tsp_HB[k] = (T)P->para_HB[rtype(RI[i + 1])][tj][k];
   }
   T lambda[4] = {0, 0, 0, 0};
-  int hb_class;
-  int nr = P->n_rama_par;
   int dij = j - i < 0 ? i - j : j - i;
-  auto h = P->hbscl;
-  if (ic == jc && dij < 45) {
-    if (dij < 4) {
-      lambda[0] = (T)-h[0][0]; lambda[1] = (T)-h[0][1];
-      hb_class = 1;
-    } else if (dij < 18) {
-      if (nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0) {
-        lambda[0] = (T)-h[1][0];
-        lambda[1] = (T)-h[1][1];
-        lambda[2] = (T)-h[1][2] - (T)h[1][3] * tsa_HB[0] - (T)h[1][4] * tsa_NHB[0] - (T)(h[1][5] * (P->anti_one[ti] + P->anti_one[tj]));
-        lambda[3] = (T)-h[1][6];
-      } else {
-        lambda[1] = (T)-h[1][1];
-      }
-      hb_class = 2;
+  // sequence-separation class (fb.cpp:2956-3002) and its hbscl row, read
+  // per lane
+  int cls = (ic == jc && dij < 45) ? (dij < 4 ? 0 : (dij < 18 ? 1 : 2)) : 3;
+  int hb_class = cls + 1;
+  auto hr = P->hbscl[cls];
+  double h[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) h[k] = hr[k];
+  if (cls == 0) {
+    lambda[0] = (T)-h[0]; lambda[1] = (T)-h[1];
+  } else if (cls == 1) {
+    if (nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0) {
+      lambda[0] = (T)-h[0];
+      lambda[1] = (T)-h[1];
+      lambda[2] = (T)-h[2] - (T)h[3] * tsa_HB[0] - (T)h[4] * tsa_NHB[0] - (T)(h[5] * (P->anti_one[ti] + P->anti_one[tj]));
+      lambda[3] = (T)-h[6];
     } else {
-      lambda[0] = (T)-h[2][0];
-      lambda[1] = (T)-h[2][1];
-      lambda[2] = (T)-h[2][2] - (T)h[2][3] * tsa_HB[1] - (T)h[2][4] * tsa_NHB[1] - (T)(h[2][5] * (P->anti_one[ti] + P->anti_one[tj]));
-      lambda[3] = (T)-h[2][6] - (T)h[2][7] * tsp_HB[1] - (T)(h[2][8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
-      hb_class = 3;
+      lambda[1] = (T)-h[1];
     }
   } else {
-    lambda[0] = (T)-h[3][0];
-    lambda[1] = (T)-h[3][1];
-    lambda[2] = (T)-h[3][2] - (T)h[3][3] * tsa_HB[1] - (T)h[3][4] * tsa_NHB[1] - (T)(h[3][5] * (P->anti_one[ti] + P->anti_one[tj]));
-    lambda[3] = (T)-h[3][6] - (T)h[3][7] * tsp_HB[1] - (T)(h[3][8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
-    hb_class = 4;
+    lambda[0] = (T)-h[0];
+    lambda[1] = (T)-h[1];
+    lambda[2] = (T)-h[2] - (T)h[3] * tsa_HB[1] - (T)h[4] * tsa_NHB[1] - (T)(h[5] * (P->anti_one[ti] + P->anti_one[tj]));
+    lambda[3] = (T)-h[6] - (T)h[7] * tsp_HB[1] - (T)(h[8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
   }
   // nu switches (arguments formed in FP64); the nu < 1e-4 early exit
   // (fb.cpp:3052) is a hard threshold, decided as in FP64 (dssp_nu_below)
@@ -1131,19 +1139,19 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   bool nu1_on = (j - 2 >= j_ch_start - 1 && j + 2 < j_ch_end && hb_class != 2);
   if (nu0_on) {
     double rr = rsq_exact(XA(V, K_CA, i + 2), XA(V, K_CA, i - 2));
-    if (rr > P->nu_cut1_sq) {
+    if (rr > cut1) {
       r0 = sqrt(rr);
-      dssp_nu<T>(P->pref[0], r0, P->d_nu0, nu0, prdnu0);
+      dssp_nu<T>(pr0, r0, dnu0, nu0, prdnu0);
     }
   } else nu0 = 1;
   if (nu1_on) {
     double rr = rsq_exact(XA(V, K_CA, j + 2), XA(V, K_CA, j - 2));
-    if (rr > P->nu_cut2_sq) {
+    if (rr > cut2) {
       r1 = sqrt(rr);
-      dssp_nu<T>(P->pref[1], r1, P->d_nu0, nu1, prdnu1);
+      dssp_nu<T>(pr1, r1, dnu0, nu1, prdnu1);
     }
   } else nu1 = 1;
-  if (dssp_nu_below(nu0, P->pref[0], r0, P->d_nu0) || dssp_nu_below(nu1, P->pref[1], r1, P->d_nu0)) return;
+  if (dssp_nu_below(nu0, pr0, r0, dnu0) || dssp_nu_below(nu1, pr1, r1, dnu0)) return;
 
   bool on[4] = {true, i_rep, i_AP, i_P};
   int acc[4] = {i, i, j, j}, don[4] = {j, j + 1, i, i + 2};
@@ -1155,14 +1163,14 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     // offsets are small differences of ~3 A quantities)
     double RNO = sqrt_t<T>(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_N, don[k])));
     double RHO = sqrt_t<T>(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_H, don[k])));
-    T eNO = (T)(RNO - P->NO_zero), eHO = (T)(RHO - P->HO_zero);
-    T dNO = eNO * (T)P->sigma_NO_sqinv;
-    T dHO = eHO * (T)P->sigma_HO_sqinv;
+    T eNO = (T)(RNO - NOz), eHO = (T)(RHO - HOz);
+    T dNO = eNO * (T)sNO;
+    T dHO = eHO * (T)sHO;
     theta[k] = t_exp((T)-0.5 * (eNO * dNO + eHO * dHO));
     p0[k] = t_div(-dNO, (T)RNO);
     p1[k] = t_div(-dHO, (T)RHO);
   }
-  T ff1 = (T)P->k_dssp * theta[0];
+  T ff1 = (T)kd * theta[0];
   T Vv[4] = {lambda[0], lambda[1] * theta[1], lambda[2] * theta[2], lambda[3] * theta[3]};
   T theta_sum = ff1 * (Vv[0] + Vv[1] + Vv[2] + Vv[3]);
   ff1 *= nu0 * nu1;
@@ -1182,7 +1190,7 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     fadd(V, K_CA, j + 2, -fq, dn);
   }
   Vv[0] = VT;
-  T an = (T)P->an, bn = (T)P->bn, cn = (T)P->cn, ah = (T)P->ah, bh = (T)P->bh, ch = (T)P->ch;
+  T an = (T)an_, bn = (T)bn_, cn = (T)cn_, ah = (T)ah_, bh = (T)bh_, ch = (T)ch_;
   for (int k = 0; k < 4; ++k) {
     if (!on[k]) continue;
     T q0 = p0[k] * Vv[k], q1 = p1[k] * Vv[k];
@@ -1202,28 +1210,31 @@ template <class T>
 AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
 {
   ParamPtr P = V.P;
+  // parameters read before any branch (invariant loads leave the loop)
+  const PapP q = pap_p(P);
+  const int d = P->i_diff_pap, mn = P->i_med_min, mx = P->i_med_max, nr = P->n_rama_par;
+  const double kp0 = P->k_pap[0], kp1 = P->k_pap[1], kp2 = P->k_pap[2];
+  const double kbeta = P->k_betapred_pap, kglob = P->k_global_pap;
   T nu_ij, prd_ij;
-  pap_nu(V, i, j, nu_ij, prd_ij);
-  if (!pap_keep(V, i, j, nu_ij)) return;
+  pap_nu(V, q, i, j, nu_ij, prd_ij);
+  if (!pap_keep(V, q, i, j, nu_ij)) return;
   const int *RI = V.rinfo;
-  int n = V.n, d = P->i_diff_pap;
+  int n = V.n;
   int ic = rchain(RI[i]), jc = rchain(RI[j]);
   int i_ch_end = V.ch_start[ic] + V.ch_len[ic], j_ch_end = V.ch_start[jc] + V.ch_len[jc];
-  int mn = P->i_med_min, mx = P->i_med_max;
   int jhi = (i + mx + 2 * d) < (i_ch_end - 1) ? (i + mx + 2 * d) : (i_ch_end - 1);
   bool med = ic == jc && i < i_ch_end - (mn + 2 * d) && j >= i + (mn + 2 * d) && j <= jhi;
   bool lng = (ic == jc && i < i_ch_end - (mx + 2 * d + 1) && j >= i + (mx + 2 * d + 1) && j < j_ch_end) ||
              (ic != jc && i + d < n && j - d >= 0 && rchain(RI[i + d]) == ic && rchain(RI[j - d]) == jc);
   bool par = (ic == jc && i < i_ch_end - (mx + 1 + d) && j >= i + (mx + 1) && j < i_ch_end - d) ||
              (ic != jc && i + d < n && j + d < n && rchain(RI[i + d]) == ic && rchain(RI[j + d]) == jc);
-  int nr = P->n_rama_par;
   bool beta = nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0;
   if (med || lng) {
-    double K = beta ? (med ? P->k_pap[0] : 0.0) + (lng ? P->k_pap[1] * P->k_betapred_pap : 0.0)
-                    : (med ? P->k_pap[0] : 0.0) + (lng ? P->k_pap[1] : 0.0);
+    double K = beta ? (med ? kp0 : 0.0) + (lng ? kp1 * kbeta : 0.0)
+                    : (med ? kp0 : 0.0) + (lng ? kp1 : 0.0);
     T nu2, prd2;
-    pap_nu(V, i + d, j - d, nu2, prd2);
-    double kk = P->k_global_pap * K;
+    pap_nu(V, q, i + d, j - d, nu2, prd2);
+    double kk = kglob * K;
     E.pap += -kk * (double)nu_ij * (double)nu2;
     V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j - d));
     T f0 = (T)kk * prd_ij * nu2, f1 = (T)kk * nu_ij * prd2;
@@ -1231,10 +1242,10 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
     fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j - d, f1, d1);
   }
   if (par) {
-    double K = beta ? P->k_pap[2] * P->k_betapred_pap : P->k_pap[2];
+    double K = beta ? kp2 * kbeta : kp2;
     T nu2, prd2;
-    pap_nu(V, i + d, j + d, nu2, prd2);
-    double kk = P->k_global_pap * K;
+    pap_nu(V, q, i + d, j + d, nu2, prd2);
+    double kk = kglob * K;
     E.pap += -kk * (double)nu_ij * (double)nu2;
     V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j + d));
     T f0 = (T)kk * prd_ij * nu2, f1 = (T)kk * nu_ij * prd2;
@@ -1259,6 +1270,7 @@ template <class T>
 AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
 {
   ParamPtr P = V.P;
+  const int p = P->ev_p;  // read before any branch (leaves the loop)
   bool same = rchain(ri) == rchain(rj);
   bool nb = same && (j == i + 1);
   bool shortc = same && (j - i < 5);
@@ -1282,7 +1294,6 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
       if (!(r < rc)) continue;
       V3<T> d = dvec<T>(XA(V, ka, i), XA(V, kb, j));
       T rr = (T)r, dr = rr - (T)rc;
-      int p = P->ev_p;
       T pw = (T)1;
       for (int q = 0; q < p - 1; ++q) pw *= dr;
       T pref = (T)((p % 2 == 0) ? lam : -lam) * pw;
@@ -1710,24 +1721,45 @@ AW_DEV void helix_item(const View<T> &V, int i, EAcc &E)
   fadd3(V, K_O, i, -(q0 * xNO.x + q1 * xHO.x), -(q0 * xNO.y + q1 * xHO.y), -(q0 * xNO.z + q1 * xHO.z));
 }
 
+// one water well's thresholds and range (wells 0 and 1, the shipped deck's,
+// are read before the pair loops; further wells inside them)
+struct WellP {
+  double min2, max2, r0, r1;
+  int flag;
+};
+AW_DEV WellP well_p(ParamPtr P, int w)
+{
+  WellP q;
+  q.min2 = P->w_rmin_th_sq[w]; q.max2 = P->w_rmax_th_sq[w];
+  q.r0 = P->well_r_min[w]; q.r1 = P->well_r_max[w];
+  q.flag = P->well_flag[w];
+  return q;
+}
+
 template <class T>
 AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
 {
   ParamPtr P = V.P;
+  // parameters read before any branch (invariant loads leave the loop)
+  const int cc = P->contact_cutoff, nw = P->n_wells;
+  const float dreach = (float)P->dens_reach;
+  const double wk = P->water_kappa;
+  const WellP w0 = well_p(P, 0), w1 = well_p(P, 1);
   int ri = V.rinfo[i], rj = V.rinfo[j];
-  if (!(rchain(ri) != rchain(rj) || j - i >= P->contact_cutoff)) return;
-  if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
+  if (!(rchain(ri) != rchain(rj) || j - i >= cc)) return;
+  if (!queued && !near_pair(V, i, j, dreach)) return;
   double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
   double r = sqrt_t<T>(rsq);
   int it = rtype(ri), jt = rtype(rj);
   double acc_i = 0.0, acc_j = 0.0;
-  for (int w = 0; w < P->n_wells; ++w) {
-    if (!P->well_flag[w]) continue;
+  for (int w = 0; w < nw; ++w) {
+    WellP wp = w == 0 ? w0 : (w == 1 ? w1 : well_p(P, w));
+    if (!wp.flag) continue;
     double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1];
     // fb.cpp:7925 reads !fabs(g0-g1)<delta, i.e. (!fabs(g0-g1)) < delta: true iff g0 != g1
-    if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
+    if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > wp.min2 && rsq < wp.max2) {
       T th, prd;
-      wtheta<T>(P, w, r, th, prd);
+      well_theta<T>(wk, wp.r0, wp.r1, r, th, prd);
       double tg = (g1 - g0) * (double)th;
       acc_i += tg * V.sig_w()[j];
       acc_j += tg * V.sig_w()[i];
@@ -1763,30 +1795,41 @@ template <class T>
 AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
 {
   ParamPtr P = V.P;
+  // parameters read before any branch (invariant loads leave the loop)
+  const unsigned mask = P->term_mask;
+  const bool water = (P->water_flag != 0) & ((mask >> AW_ET_WATER) & 1u);
+  const bool burial = (P->burial_flag != 0) & ((mask >> AW_ET_BURIAL) & 1u);
+  const bool helix = (P->helix_flag != 0) & ((mask >> AW_ET_HELIX) & 1u);
+  const int cc = P->contact_cutoff, nw = P->n_wells, hd = P->helix_i_diff;
+  const float dreach = (float)P->dens_reach;
+  const double wk = P->water_kappa, hk = P->helix_kappa, hr0 = P->helix_r_min, hr1 = P->helix_r_max;
+  const double hmin2 = P->h_rmin_th_sq, hmax2 = P->h_rmax_th_sq;
+  const WellP w0 = well_p(P, 0), w1 = well_p(P, 1);
   int ri = V.rinfo[i], rj = V.rinfo[j];
   bool same = rchain(ri) == rchain(rj);
   int dres = j - i;
   if (!(!same || dres > 1)) return;
-  if (!queued && !near_pair(V, i, j, (float)P->dens_reach)) return;
+  if (!queued && !near_pair(V, i, j, dreach)) return;
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
   double rsq = rsq_exact(si, sj);
   double r = sqrt_t<T>(rsq);
   T force = 0;
-  bool in0 = rsq > P->w_rmin_th_sq[0] && rsq < P->w_rmax_th_sq[0];
+  bool in0 = rsq > w0.min2 && rsq < w0.max2;
   // well 0's theta'/r, shared by the water, water-xi and burial parts
   T th0 = 0, prd0 = 0;
-  if (in0) wtheta<T>(P, 0, r, th0, prd0);
-  if (P->water_flag && ON(P, AW_ET_WATER)) {
+  if (in0) well_theta<T>(wk, w0.r0, w0.r1, r, th0, prd0);
+  if (water) {
     int it = rtype(ri), jt = rtype(rj);
     double ew = 0.0;
-    for (int w = 0; w < P->n_wells; ++w) {
-      if (!P->well_flag[w]) continue;
-      if ((!same || dres >= P->contact_cutoff) && rsq > P->w_rmin_th_sq[w] && rsq < P->w_rmax_th_sq[w]) {
+    for (int w = 0; w < nw; ++w) {
+      WellP wp = w == 0 ? w0 : (w == 1 ? w1 : well_p(P, w));
+      if (!wp.flag) continue;
+      if ((!same || dres >= cc) && rsq > wp.min2 && rsq < wp.max2) {
         double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1], sg;
         if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
         else sg = g0 + (g1 - g0) * V.sig_w()[i] * V.sig_w()[j];
         T th = th0, prd = prd0;
-        if (w != 0) wtheta<T>(P, w, r, th, prd);
+        if (w != 0) well_theta<T>(wk, wp.r0, wp.r1, r, th, prd);
         ew += -sg * (double)th;
         force += (T)sg * prd;
       }
@@ -1799,15 +1842,14 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
       if (xi != 0.0) force += prd0 * (T)xi;
     }
   }
-  if (P->burial_flag && in0 && ON(P, AW_ET_BURIAL)) force += (T)(V.bf()[i] + V.bf()[j]) * prd0;
-  if (P->helix_flag && rsq > P->h_rmin_th_sq && rsq < P->h_rmax_th_sq && ON(P, AW_ET_HELIX)) {
-    int hd = P->helix_i_diff;
+  if (burial && in0) force += (T)(V.bf()[i] + V.bf()[j]) * prd0;
+  if (helix && rsq > hmin2 && rsq < hmax2) {
     double factor = V.hxi1()[i] + V.hxi1()[j];
     if (i - hd >= 0) factor += V.hxi2()[i - hd];
     if (j - hd >= 0) factor += V.hxi2()[j - hd];
     if (fabs(factor) > AW_DELTA_HELIX_XI) {
       T th, prd;
-      htheta<T>(P, r, th, prd);
+      well_theta<T>(hk, hr0, hr1, r, th, prd);
       force += -(T)factor * prd;
     }
   }
