@@ -1466,11 +1466,12 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
   if (P->ev_on && ON(P, AW_ET_VEXCLUDED)) ev_self(V, i, ri, E);
 }
 
-// Residue-pair pass of phase B.  With queues (qd/qp/qe non-null) the
-// DSSP (directed, acceptor << 16 | donor), P_AP and excluded-volume pairs
-// that pass their cutoffs are queued and evaluated afterwards in dense
-// passes; otherwise they are evaluated in place.
-template <class T>
+// Residue-pair pass of phase B.  Q (queues qd/qp/qe given): the DSSP
+// (directed, acceptor << 16 | donor), P_AP and excluded-volume pairs that
+// pass their cutoffs are queued and evaluated afterwards in dense passes;
+// otherwise they are evaluated in place.  A template argument, so the scan
+// loop of the queued pass carries no in-place term code.
+template <class T, bool Q = false>
 AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr,
                    int *qs = nullptr)
 {
@@ -1534,7 +1535,7 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, i
   bool pap = pap_on && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), pcut2);
   float le = evreach + eij;
   bool ev = ev_on && rca <= le * le * 1.0001f;
-  if (qd) {
+  if constexpr (Q) {
     // queue counters nl_state[2..5]: DSSP (both orientations), P_AP,
     // excluded volume, density pairs
     lmask m1 = w_ballot(dij), m2 = w_ballot(dji), mp = w_ballot(pap), me = w_ballot(ev), ms = w_ballot(dq);
@@ -1594,7 +1595,7 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
         // density pairs are kept for phases D and F when the list holds
         // every pair (one chunk)
         int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
-        for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe, qs); });
+        for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
         AW_BARRIER();
         if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(12);
         int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];

@@ -358,7 +358,7 @@ __device__ __forceinline__ void w_scan(const View<T> &V)
   if (V.wq && V.nl[1]) {
     int cap = V.nl_cap;
     int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
-    for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B(V, i, j, E, qd, qp, qe, qs); });
+    for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
   } else {
     for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B(V, i, j, E); });
   }
