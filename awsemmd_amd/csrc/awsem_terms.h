@@ -328,6 +328,44 @@ AW_DEV void fadd3(const View<T> &V, int kind, int i, R fx, R fy, R fz)
   aadd(p + 2, (double)fz);
 }
 
+// Forces of one residue's local terms (chain, chi, shake, Ramachandran,
+// bonds, the CB-O excluded volume): every one of them acts on CA(i-1),
+// O(i-1), CA(i), O(i), CB(i) and CA(i+1) only, so they are summed in
+// registers and added to the force array once per residue (18 atomic adds
+// instead of ~130).  add(kind, d, ...) names the atom by its residue
+// offset d in {-1, 0, 1}; both are constants wherever the terms are inlined.
+struct LocF {
+  double f[6][3];
+  AW_DEV void clear()
+  {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) f[a][0] = f[a][1] = f[a][2] = 0.0;
+  }
+  AW_DEV static int slot(int kind, int d)
+  {
+    return d < 0 ? (kind == K_CA ? 0 : 1) : (d > 0 ? 5 : (kind == K_CA ? 2 : (kind == K_O ? 3 : 4)));
+  }
+  template <class R>
+  AW_DEV void add3(int kind, int d, R fx, R fy, R fz)
+  {
+    double *q = f[slot(kind, d)];
+    q[0] += (double)fx; q[1] += (double)fy; q[2] += (double)fz;
+  }
+  template <class R>
+  AW_DEV void add(int kind, int d, R s, const V3<R> &v) { add3(kind, d, s * v.x, s * v.y, s * v.z); }
+  // CA(i-1) and O(i-1) exist unless i is a chain's first residue, CA(i+1)
+  // unless it is the last
+  template <class T>
+  AW_DEV void flush(const View<T> &V, int i, bool first, bool last) const
+  {
+    if (!first) { fadd3(V, K_CA, i - 1, f[0][0], f[0][1], f[0][2]); fadd3(V, K_O, i - 1, f[1][0], f[1][1], f[1][2]); }
+    fadd3(V, K_CA, i, f[2][0], f[2][1], f[2][2]);
+    fadd3(V, K_O, i, f[3][0], f[3][1], f[3][2]);
+    fadd3(V, K_CB, i, f[4][0], f[4][1], f[4][2]);
+    if (!last) fadd3(V, K_CA, i + 1, f[5][0], f[5][1], f[5][2]);
+  }
+};
+
 // energy accumulators (per thread, FP64)
 struct EAcc {
   double chain, chi, rama, ev, dssp, pap, water, burial, helix, fm, bond, shake;
@@ -461,7 +499,7 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
 
 // compute_chain_potential, fb.cpp:2062-2170
 template <class T>
-AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
+AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
 {
   ParamPtr P = V.P;
   bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
@@ -470,37 +508,37 @@ AW_DEV void chain_term(const View<T> &V, int i, int ri, EAcc &E)
     double r = sqrt(dot(d, d)), dr = r - (double)P->r_ncb0;
     double force = (double)2 * (double)P->k_chain[0] * dr / r;
     E.chain += (double)((double)P->k_chain[0] * dr * dr);
-    fadd(V, K_CA, i - 1, -(double)P->an * force, d);
-    fadd(V, K_O, i - 1, -(double)P->cn * force, d);
-    fadd(V, K_CA, i, -(double)P->bn * force, d);
-    fadd(V, K_CB, i, force, d);
+    F.add(K_CA, -1, -(double)P->an * force, d);
+    F.add(K_O, -1, -(double)P->cn * force, d);
+    F.add(K_CA, 0, -(double)P->bn * force, d);
+    F.add(K_CB, 0, force, d);
   }
   if (!last && !gly) {
     V3<double> d = dvec<double>(XV(V, K_CP, i), XA(V, K_CB, i));
     double r = sqrt(dot(d, d)), dr = r - (double)P->r_cpcb0;
     double force = (double)2 * (double)P->k_chain[1] * dr / r;
     E.chain += (double)((double)P->k_chain[1] * dr * dr);
-    fadd(V, K_CA, i + 1, -(double)P->bp * force, d);
-    fadd(V, K_CA, i, -(double)P->ap * force, d);
-    fadd(V, K_O, i, -(double)P->cp * force, d);
-    fadd(V, K_CB, i, force, d);
+    F.add(K_CA, 1, -(double)P->bp * force, d);
+    F.add(K_CA, 0, -(double)P->ap * force, d);
+    F.add(K_O, 0, -(double)P->cp * force, d);
+    F.add(K_CB, 0, force, d);
   }
   if (!first && !last) {
     V3<double> d = dvec<double>(XV(V, K_N, i), XV(V, K_CP, i));
     double r = sqrt(dot(d, d)), dr = r - (double)P->r_ncp0;
     double force = (double)2 * (double)P->k_chain[2] * dr / r;
     E.chain += (double)((double)P->k_chain[2] * dr * dr);
-    fadd(V, K_CA, i - 1, -(double)P->an * force, d);
-    fadd(V, K_O, i - 1, -(double)P->cn * force, d);
-    fadd(V, K_CA, i + 1, (double)P->bp * force, d);
-    fadd(V, K_CA, i, -(double)(P->bn - P->ap) * force, d);
-    fadd(V, K_O, i, (double)P->cp * force, d);
+    F.add(K_CA, -1, -(double)P->an * force, d);
+    F.add(K_O, -1, -(double)P->cn * force, d);
+    F.add(K_CA, 1, (double)P->bp * force, d);
+    F.add(K_CA, 0, -(double)(P->bn - P->ap) * force, d);
+    F.add(K_O, 0, (double)P->cp * force, d);
   }
 }
 
 // compute_shake, fb.cpp:2172-2237
 template <class T>
-AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E)
+AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
 {
   ParamPtr P = V.P;
   double ks = (double)(P->epsilon * P->k_shake);
@@ -509,28 +547,28 @@ AW_DEV void shake_term(const View<T> &V, int i, int ri, EAcc &E)
     V3<double> d = dvec<double>(XA(V, K_CA, i), XA(V, K_CA, i + 1));
     double r = sqrt(dot(d, d)), dr = r - (double)P->r_sh1, force = (double)2 * ks * dr / r;
     E.shake += (double)(ks * dr * dr);
-    fadd(V, K_CA, i, -force, d);
-    fadd(V, K_CA, i + 1, force, d);
+    F.add(K_CA, 0, -force, d);
+    F.add(K_CA, 1, force, d);
   }
   {
     V3<double> d = dvec<double>(XA(V, K_CA, i), XA(V, K_O, i));
     double r = sqrt(dot(d, d)), dr = r - (double)P->r_sh2, force = (double)2 * ks * dr / r;
     E.shake += (double)(ks * dr * dr);
-    fadd(V, K_CA, i, -force, d);
-    fadd(V, K_O, i, force, d);
+    F.add(K_CA, 0, -force, d);
+    F.add(K_O, 0, force, d);
   }
   if (!last) {
     V3<double> d = dvec<double>(XA(V, K_O, i), XA(V, K_CA, i + 1));
     double r = sqrt(dot(d, d)), dr = r - (double)P->r_sh3, force = (double)2 * ks * dr / r;
     E.shake += (double)(ks * dr * dr);
-    fadd(V, K_O, i, -force, d);
-    fadd(V, K_CA, i + 1, force, d);
+    F.add(K_O, 0, -force, d);
+    F.add(K_CA, 1, force, d);
   }
 }
 
 // compute_chi_potential, fb.cpp:2239-2339
 template <class T>
-AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
+AW_DEV void chi_term(const View<T> &V, int i, EAcc &E, LocF &F)
 {
   ParamPtr P = V.P;
   V3<double> a = dvec<double>(XV(V, K_CP, i), XA(V, K_CA, i));
@@ -547,14 +585,14 @@ AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
   double dchi = chi - (double)P->chi0;
   double force = (double)2 * (double)P->k_chi * dchi;
   E.chi += (double)((double)P->k_chi * dchi * dchi);
-  fadd(V, K_CA, i - 1, (double)P->an * force, bp_);
-  fadd(V, K_O, i - 1, (double)P->cn * force, bp_);
-  fadd(V, K_CA, i + 1, -(double)P->bp * force, ap_);
+  F.add(K_CA, -1, (double)P->an * force, bp_);
+  F.add(K_O, -1, (double)P->cn * force, bp_);
+  F.add(K_CA, 1, -(double)P->bp * force, ap_);
   double c1 = (double)(1 - P->bn), c2 = (double)(P->ap - 1);
-  fadd3(V, K_CA, i, -(cp_.x + c1 * bp_.x + c2 * ap_.x) * force, -(cp_.y + c1 * bp_.y + c2 * ap_.y) * force,
+  F.add3(K_CA, 0, -(cp_.x + c1 * bp_.x + c2 * ap_.x) * force, -(cp_.y + c1 * bp_.y + c2 * ap_.y) * force,
         -(cp_.z + c1 * bp_.z + c2 * ap_.z) * force);
-  fadd(V, K_O, i, -(double)P->cp * force, ap_);
-  fadd(V, K_CB, i, force, cp_);
+  F.add(K_O, 0, -(double)P->cp * force, ap_);
+  F.add(K_CB, 0, force, cp_);
 }
 
 // calcDihedralAndSlopes, fb.cpp:2341-2453.  The angle and its slopes are
@@ -562,10 +600,11 @@ AW_DEV void chi_term(const View<T> &V, int i, EAcc &E)
 // dihedral(..., 0, cs, sn) returns cos and sin of the angle (X/|(X,Y)|,
 // Y/|(X,Y)|: the angle is atan2(Y, X)); dihedral(..., fscale, ...)
 // recomputes the geometry and adds fscale * (y_slope + x_slope) of each of
-// the five atoms CA(i-1), CA(i), CA(i+1), O(i-1), O(i) into the forces
-// (compute_rama_force, fb.cpp:2455-2506, summed over the wells).
+// the five atoms CA(i-1), CA(i), CA(i+1), O(i-1), O(i) into the residue's
+// local-force sums F (compute_rama_force, fb.cpp:2455-2506, summed over the
+// wells).
 template <class T>
-AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &cs, double &sn)
+AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &cs, double &sn, LocF &F)
 {
   ParamPtr P = V.P;
   double an = P->an, bn = P->bn, cn = P->cn, ap = P->ap, bp = P->bp, cp = P->cp;
@@ -595,8 +634,7 @@ AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &c
   b2 *= dx; adb *= dx; adc *= dx; bdc *= dx;
   cdbxa *= dy / bm;
   bm *= dy;
-  double *fca_m = FA(V, K_CA, i - 1), *fca_0 = FA(V, K_CA, i), *fca_p = FA(V, K_CA, i + 1);
-  double *fo_m = FA(V, K_O, i - 1), *fo_0 = FA(V, K_O, i);
+#pragma unroll
   for (int l = 0; l < 3; l++) {
     double av = l == 0 ? a.x : (l == 1 ? a.y : a.z), bv = l == 0 ? b.x : (l == 1 ? b.y : b.z);
     double cv = l == 0 ? c.x : (l == 1 ? c.y : c.z);
@@ -624,22 +662,22 @@ AW_DEV void dihedral(const View<T> &V, int i, bool phi, double fscale, double &c
       o3 = -cn * h1 + (-cn * g3);
       o4 = cp * h4 + (cn - cp) * h2 - cp * h3 + ((cn - cp) * g1 + cp * g2);
     }
-    aadd(fca_m + l, fscale * o0);
-    aadd(fca_0 + l, fscale * o1);
-    aadd(fca_p + l, fscale * o2);
-    aadd(fo_m + l, fscale * o3);
-    aadd(fo_0 + l, fscale * o4);
+    F.f[LocF::slot(K_CA, -1)][l] += fscale * o0;
+    F.f[LocF::slot(K_CA, 0)][l] += fscale * o1;
+    F.f[LocF::slot(K_CA, 1)][l] += fscale * o2;
+    F.f[LocF::slot(K_O, -1)][l] += fscale * o3;
+    F.f[LocF::slot(K_O, 0)][l] += fscale * o4;
   }
 }
 
 // compute_rama_potential / compute_rama_force, fb.cpp:2455-2546
 template <class T>
-AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
+AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
 {
   ParamPtr P = V.P;
   double cphi, sphi, cpsi, spsi, dummy;
-  dihedral(V, i, true, 0.0, cphi, sphi);
-  dihedral(V, i, false, 0.0, cpsi, spsi);
+  dihedral(V, i, true, 0.0, cphi, sphi, F);
+  dihedral(V, i, false, 0.0, cpsi, spsi, F);
   int js = 0, je = P->n_rama_par;
   if ((ri & RI_PRO) && P->rama_p_flag) { js = 6; je = 6 + P->n_rama_p_par; }
   double fphi = 0, fpsi = 0;
@@ -660,34 +698,33 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E)
     ev += -Vv;
   }
   E.rama += ev;
-  if (fphi != 0.0) dihedral(V, i, true, fphi, dummy, dummy);
-  if (fpsi != 0.0) dihedral(V, i, false, fpsi, dummy, dummy);
+  if (fphi != 0.0) dihedral(V, i, true, fphi, dummy, dummy, F);
+  if (fpsi != 0.0) dihedral(V, i, false, fpsi, dummy, dummy, F);
 }
 
 // LAMMPS bond_style harmonic on the deck's bonds (outside the reference).
 // Always FP64: r - r0 cancels for bonds near their rest length.
 template <class T>
-AW_DEV void bond_one(const View<T> &V, int ka, int ia, int kb, int ib, double r0, EAcc &E)
+AW_DEV void bond_one(const View<T> &V, int i, int ka, int da, int kb, int db, double r0, EAcc &E, LocF &F)
 {
-  V3<double> d = dvec<double>(XA(V, ka, ia), XA(V, kb, ib));
+  V3<double> d = dvec<double>(XA(V, ka, i + da), XA(V, kb, i + db));
   double r = sqrt(dot(d, d)), dr = r - r0;
   double k = V.P->bond_k;
   E.bond += k * dr * dr;
   double fb = r > 0.0 ? -2.0 * k * dr / r : 0.0;
-  double *pa = FA(V, ka, ia), *pb = FA(V, kb, ib);
-  aadd(pa + 0, fb * d.x); aadd(pa + 1, fb * d.y); aadd(pa + 2, fb * d.z);
-  aadd(pb + 0, -fb * d.x); aadd(pb + 1, -fb * d.y); aadd(pb + 2, -fb * d.z);
+  F.add(ka, da, fb, d);
+  F.add(kb, db, -fb, d);
 }
 
 template <class T>
-AW_DEV void bonds_term(const View<T> &V, int i, int ri, EAcc &E)
+AW_DEV void bonds_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
 {
   ParamPtr P = V.P;
-  bond_one(V, K_CA, i, K_O, i, P->bond_r0[1], E);
-  bond_one(V, K_CA, i, K_CB, i, (ri & RI_GLY) ? P->bond_r0[4] : P->bond_r0[3], E);
+  bond_one(V, i, K_CA, 0, K_O, 0, P->bond_r0[1], E, F);
+  bond_one(V, i, K_CA, 0, K_CB, 0, (ri & RI_GLY) ? P->bond_r0[4] : P->bond_r0[3], E, F);
   if (!(ri & RI_LAST)) {
-    bond_one(V, K_CA, i, K_CA, i + 1, P->bond_r0[0], E);
-    bond_one(V, K_O, i, K_CA, i + 1, P->bond_r0[2], E);
+    bond_one(V, i, K_CA, 0, K_CA, 1, P->bond_r0[0], E, F);
+    bond_one(V, i, K_O, 0, K_CA, 1, P->bond_r0[2], E, F);
   }
 }
 
@@ -1306,7 +1343,7 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
 
 // same-residue excluded volume (only CB-O can be unbonded within a residue)
 template <class T>
-AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E)
+AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
 {
   ParamPtr P = V.P;
   int ta = atype(K_CB, ri), tb = 3;
@@ -1323,8 +1360,8 @@ AW_DEV void ev_self(const View<T> &V, int i, int ri, EAcc &E)
   for (int q = 0; q < p - 1; ++q) pw *= dr;
   T pref = (T)((p % 2 == 0) ? lam : -lam) * pw;
   T fp = rr > (T)0 ? t_div(-pref * (T)p, rr) : (T)0;
-  fadd(V, K_CB, i, fp, d);
-  fadd(V, K_O, i, -fp, d);
+  F.add(K_CB, 0, fp, d);
+  F.add(K_O, 0, -fp, d);
   E.ev += (double)(pref * dr);
 }
 
@@ -1458,12 +1495,23 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
   ParamPtr P = V.P;
   int ri = V.rinfo[i];
   bool first = ri & RI_FIRST, last = ri & RI_LAST, gly = ri & RI_GLY;
-  if (P->chain_flag && ON(P, AW_ET_CHAIN)) chain_term(V, i, ri, E);
-  if (!first && !last && P->chi_flag && !gly && ON(P, AW_ET_CHI)) chi_term(V, i, E);
-  if (P->shake_flag && ON(P, AW_ET_SHAKE)) shake_term(V, i, ri, E);
-  if (!first && !last && P->rama_flag && !gly && ON(P, AW_ET_RAMA)) rama_term(V, i, ri, E);
-  if (P->bond_on && ON(P, AW_ET_BOND)) bonds_term(V, i, ri, E);
-  if (P->ev_on && ON(P, AW_ET_VEXCLUDED)) ev_self(V, i, ri, E);
+  // the switches, read before any branch
+  const unsigned mask = P->term_mask;
+  const bool chain_on = (P->chain_flag != 0) & ((mask >> AW_ET_CHAIN) & 1u);
+  const bool chi_on = (P->chi_flag != 0) & ((mask >> AW_ET_CHI) & 1u);
+  const bool shake_on = (P->shake_flag != 0) & ((mask >> AW_ET_SHAKE) & 1u);
+  const bool rama_on = (P->rama_flag != 0) & ((mask >> AW_ET_RAMA) & 1u);
+  const bool bond_on = (P->bond_on != 0) & ((mask >> AW_ET_BOND) & 1u);
+  const bool ev_on = (P->ev_on != 0) & ((mask >> AW_ET_VEXCLUDED) & 1u);
+  LocF F;
+  F.clear();
+  if (chain_on) chain_term(V, i, ri, E, F);
+  if (!first && !last && chi_on && !gly) chi_term(V, i, E, F);
+  if (shake_on) shake_term(V, i, ri, E, F);
+  if (!first && !last && rama_on && !gly) rama_term(V, i, ri, E, F);
+  if (bond_on) bonds_term(V, i, ri, E, F);
+  if (ev_on) ev_self(V, i, ri, E, F);
+  F.flush(V, i, first, last);
 }
 
 // Residue-pair pass of phase B.  Q (queues qd/qp/qe given): the DSSP
