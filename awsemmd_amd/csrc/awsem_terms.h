@@ -678,20 +678,34 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
   double cphi, sphi, cpsi, spsi, dummy;
   dihedral(V, i, true, 0.0, cphi, sphi, F);
   dihedral(V, i, false, 0.0, cpsi, spsi, F);
-  int js = 0, je = P->n_rama_par;
-  if ((ri & RI_PRO) && P->rama_p_flag) { js = 6; je = 6 + P->n_rama_p_par; }
+  // wells j = js + k, k < nk: the general set (js = 0) or, for a proline
+  // with Rama_P on, the proline set (js = 6).  Both sets' parameters of well
+  // k are read at wave-uniform addresses (scalar loads, all issued ahead of
+  // the wells) and chosen per lane; at most 6 wells per set.
+  const bool pro = (ri & RI_PRO) && P->rama_p_flag;
+  const int nk = pro ? P->n_rama_p_par : P->n_rama_par;
+  const double *apsr = V.aps + i * 12 + (pro ? 6 : 0);
   double fphi = 0, fpsi = 0;
   double ev = 0.0;
-  for (int j = js; j < je; j++) {
-    double apsv = V.aps[i * 12 + j];
-    if (P->ssweight[j] && apsv == 0.0) continue;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int ssw0 = P->ssweight[k], ssw1 = P->ssweight[6 + k];
+    const double w0 = P->w[k], w1 = P->w[6 + k], pw0 = P->phiw[k], pw1 = P->phiw[6 + k];
+    const double qw0 = P->psiw[k], qw1 = P->psiw[6 + k];
+    const double cf0 = P->cphi0[k], cf1 = P->cphi0[6 + k], sf0 = P->sphi0[k], sf1 = P->sphi0[6 + k];
+    const double cs0 = P->cpsi0[k], cs1 = P->cpsi0[6 + k], ss0 = P->spsi0[k], ss1 = P->spsi0[6 + k];
+    if (k >= nk) break;
+    const bool ssw = pro ? ssw1 != 0 : ssw0 != 0;
+    const double apsv = apsr[k];
+    if (ssw && apsv == 0.0) continue;
+    const double cf = pro ? cf1 : cf0, sf = pro ? sf1 : sf0, cs = pro ? cs1 : cs0, ss = pro ? ss1 : ss0;
     // cos/sin(phi + phi0) by angle addition
-    double cp_ = cphi * P->cphi0[j] - sphi * P->sphi0[j], sp_ = sphi * P->cphi0[j] + cphi * P->sphi0[j];
-    double cq_ = cpsi * P->cpsi0[j] - spsi * P->spsi0[j], sq_ = spsi * P->cpsi0[j] + cpsi * P->spsi0[j];
+    double cp_ = cphi * cf - sphi * sf, sp_ = sphi * cf + cphi * sf;
+    double cq_ = cpsi * cs - spsi * ss, sq_ = spsi * cs + cpsi * ss;
     double cph = cp_ - 1.0, cps = cq_ - 1.0;
-    double pc = P->phiw[j] * cph, sc = P->psiw[j] * cps;
-    double Vv = P->w[j] * exp(-cph * pc - cps * sc);
-    if (P->ssweight[j]) Vv *= apsv;
+    double pc = (pro ? pw1 : pw0) * cph, sc = (pro ? qw1 : qw0) * cps;
+    double Vv = (pro ? w1 : w0) * exp(-cph * pc - cps * sc);
+    if (ssw) Vv *= apsv;
     double force = 2.0 * Vv;
     fphi += force * pc * sp_;
     fpsi += force * sc * sq_;
