@@ -58,9 +58,9 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
   View<T> V;
   V.n = R.n; V.nch = R.nch;
   V.rinfo = R.rinfo;
-  V.aps = R.aps;
-  V.ch_start = R.ch_start;
-  V.ch_len = R.ch_len;
+  V.aps = (const AW_G double *)R.aps;
+  V.ch_start = (const AW_G int *)R.ch_start;
+  V.ch_len = (const AW_G int *)R.ch_len;
   V.s = slab;
   V.ered = nullptr;
   V.P = (ParamPtr)B.P;
@@ -171,8 +171,8 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     View<T> V = make_view<T>(B, R, slab);
     V.P = (ParamPtr)Pr;  // read-only: invariant uniform loads through the scalar cache
     V.ered = ered;
-    V.nl = R.nl_cap > 0 ? R.nl : nullptr;
-    V.nlx = R.nlx;
+    V.nl = R.nl_cap > 0 ? (AW_G int *)R.nl : nullptr;
+    V.nlx = (AW_G double *)R.nlx;
     V.nl_cap = R.nl_cap;
     V.nl_state = nl_state;
     V.wq = V.nl ? V.nl + AW_NL_HDR + R.nl_cap : nullptr;
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
       if ((R.flags & AW_RF_NL_RESET) && V.nl) V.nl[1] = 0;
     }
     int n = R.n;
-    double *gx = R.x, *gv = R.v, *gf = R.f;
+    AW_G double *gx = (AW_G double *)R.x, *gv = (AW_G double *)R.v, *gf = (AW_G double *)R.f;
     for (int k = threadIdx.x; k < 9 * n; k += blockDim.x) V.x()[k] = gx[k];
     int *ri = V.rinfo_copy();
     for (int k = threadIdx.x; k < n; k += blockDim.x) ri[k] = V.rinfo[k];

@@ -93,7 +93,7 @@ template <class T>
 __global__ void __launch_bounds__(256) k_ph_int1(BatchDev B, int r, LgParams lg)
 {
   View<T> V = phase_view<T>(B, r);
-  integrate1(V, B.rep[r].v, lg, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  integrate1(V, (AW_G double *)B.rep[r].v, lg, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 template <class T>
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) k_ph_int2(BatchDev B, int r, LgParams lg,
 {
   const ReplicaDesc R = B.rep[r];
   View<T> V = phase_view<T>(B, r);
-  integrate2(V, R.v, lg, R, s, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  integrate2(V, (AW_G double *)R.v, lg, R, s, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // total energy slot of one replica

@@ -31,6 +31,7 @@ inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 #define AW_HOSTDEV
 typedef const FFParams *ParamPtr;
 #define AW_SCHED_FENCE() ((void)0)
+#define AW_G
 template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
 template <class ItemT> inline ItemT ld_item(const ItemT *p) { return *p; }
@@ -73,6 +74,10 @@ __device__ __forceinline__ float i2f(int i) { return __int_as_float(i); }
 typedef const __attribute__((address_space(4))) FFParams *ParamPtr;
 // instruction-scheduling barrier: nothing is moved across it
 #define AW_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+// global-memory pointee: global_ instructions instead of flat ones (a flat
+// access also counts against the LDS wait counter, so every LDS wait after
+// it would wait for the global round trip too)
+#define AW_G __attribute__((address_space(1)))
 // typed accesses for code compiled apart from the kernel (a noinline
 // function), where a plain pointer would give flat instructions: ld_as /
 // aadd_as read / add into LDS (L) or global memory, ld_item loads a 16-byte
@@ -266,10 +271,10 @@ template <class T> AW_DEV V3<T> crs(const V3<T> &a, const V3<T> &b)
 template <class T>
 struct View {
   int n, nch;
-  const int *rinfo;     // [n] (global)
-  const double *aps;    // [n][12] (global)
-  const int *ch_start;  // [nch] 0-based
-  const int *ch_len;
+  const int *rinfo;     // [n] (global, or its copy in LDS)
+  const AW_G double *aps;    // [n][12]
+  const AW_G int *ch_start;  // [nch] 0-based
+  const AW_G int *ch_len;
   double *s;            // working slab (LDS or global): see the accessors
   double *ered;         // energy accumulators [N_EACC] (LDS)
   ParamPtr P;
@@ -279,11 +284,11 @@ struct View {
   const void *rf;        // T4 per (memory, pair)
   const void *items;     // FMItem / FMItemD per item
   // Verlet list of residue pairs (resident engine; nl == nullptr: sweep all pairs)
-  int *nl;               // global: header (AW_NL_HDR ints: [0] count, [1] valid, [2] builds), then (i<<16 | j)
+  AW_G int *nl;          // global: header (AW_NL_HDR ints: [0] count, [1] valid, [2] builds), then (i<<16 | j)
   int nl_cap;
-  double *nlx;           // global: CA positions [n][3] and extents [n] at the last build
+  AW_G double *nlx;      // global: CA positions [n][3] and extents [n] at the last build
   int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts
-  int *wq;               // global work queues beside the list: DSSP [2 cap], P_AP [cap], EV [cap], density pairs [cap]
+  AW_G int *wq;          // global work queues beside the list: DSSP [2 cap], P_AP [cap], EV [cap], density pairs [cap]
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
   // f[3][n][3], ten per-residue scalars, ext (float)
   AW_DEV double *x() const { return s; }
@@ -468,7 +473,7 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
     V.ext()[i] = ei;
     if (V.nl) {
       // CA drift plus growth of the extent since the last list build
-      const double *c0 = V.nlx + 3 * i;
+      const double c0[3] = {V.nlx[3 * i], V.nlx[3 * i + 1], V.nlx[3 * i + 2]};
       double d2 = rsq_exact(ca, c0);
       float drift = (float)sqrt(d2) * 1.0001f + 1e-4f + fmaxf(0.0f, ei - (float)V.nlx[3 * n + i]);
       dmax = fmaxf(dmax, drift);
@@ -684,7 +689,7 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
   // the wells) and chosen per lane; at most 6 wells per set.
   const bool pro = (ri & RI_PRO) && P->rama_p_flag;
   const int nk = pro ? P->n_rama_p_par : P->n_rama_par;
-  const double *apsr = V.aps + i * 12 + (pro ? 6 : 0);
+  const AW_G double *apsr = V.aps + i * 12 + (pro ? 6 : 0);
   double fphi = 0, fpsi = 0;
   double ev = 0.0;
 #pragma unroll
@@ -1451,8 +1456,8 @@ AW_DEV int pair_total(const View<T> &V)
 // body(i, j) for the packed pairs src[k0 + t0], src[k0 + t0 + ts], ...: the
 // next entry is loaded while the current pair is evaluated, so a lane does
 // not wait on a memory round trip per pair
-template <class F>
-AW_DEV void for_packed(const int *src, int k0, int k1, int t0, int ts, F &&body)
+template <class Src, class F>
+AW_DEV void for_packed(Src src, int k0, int k1, int t0, int ts, F &&body)
 {
   int k = k0 + t0;
   int cur = k < k1 ? src[k] : 0;
@@ -1466,7 +1471,7 @@ AW_DEV void for_packed(const int *src, int k0, int k1, int t0, int ts, F &&body)
 template <class T, class F>
 AW_DEV void for_pairs_range(const View<T> &V, int k0, int k1, int t0, int ts, F &&body)
 {
-  const int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
+  const AW_G int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
   if (lst) {
     for_packed(lst, k0, k1, t0, ts, body);
     return;
@@ -1492,7 +1497,7 @@ AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
 {
   // body(i, j, queued): queued pairs passed the site-distance test of the
   // scan, which implies the near_pair test (|CA-CA| <= |site-site| + extents)
-  const int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
+  const AW_G int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
   if (qs) {
     for_packed(qs, 0, V.nl_state[5], t0, ts, [&](int i, int j) { body(i, j, true); });
     return;
@@ -1534,8 +1539,8 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
 // otherwise they are evaluated in place.  A template argument, so the scan
 // loop of the queued pass carries no in-place term code.
 template <class T, bool Q = false>
-AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, int *qd = nullptr, int *qp = nullptr, int *qe = nullptr,
-                   int *qs = nullptr)
+AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullptr, AW_G int *qp = nullptr,
+                   AW_G int *qe = nullptr, AW_G int *qs = nullptr)
 {
   ParamPtr P = V.P;
   // every parameter the scan reads, loaded before any branch: invariant and
@@ -1650,13 +1655,13 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
   if (part & 4) {
     if (QUEUED) {
       int cap = V.nl_cap;
-      int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
+      AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
       int total = pair_total(V);
       for (int c0 = 0; c0 < total && cap > 0; c0 += cap) {
         int c1 = total - c0 < cap ? total : c0 + cap;
         // density pairs are kept for phases D and F when the list holds
         // every pair (one chunk)
-        int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
+        AW_G int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
         for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
         AW_BARRIER();
         if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(12);
@@ -1993,7 +1998,7 @@ inline AW_HOSTDEV LgParams lg_make(double dt, double damp)
 
 // initial_integrate of fix nve
 template <class T>
-AW_DEV void integrate1(const View<T> &V, double *v, const LgParams &lg, int t0, int ts)
+AW_DEV void integrate1(const View<T> &V, AW_G double *v, const LgParams &lg, int t0, int ts)
 {
   int n = V.n;
   for (int a = t0; a < 3 * n; a += ts) {
@@ -2009,7 +2014,7 @@ AW_DEV void integrate1(const View<T> &V, double *v, const LgParams &lg, int t0, 
 
 // fix langevin post_force + fix nve final_integrate
 template <class T>
-AW_DEV void integrate2(const View<T> &V, double *v, const LgParams &lg, const ReplicaDesc &R, int s, int t0, int ts)
+AW_DEV void integrate2(const View<T> &V, AW_G double *v, const LgParams &lg, const ReplicaDesc &R, int s, int t0, int ts)
 {
   int n = V.n;
   // the target temperature ramps over the replica's whole run (which may

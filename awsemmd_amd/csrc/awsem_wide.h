@@ -26,8 +26,8 @@ template <class T>
 __device__ __forceinline__ View<T> w_view(const BatchDev &B, const ReplicaDesc &R)
 {
   View<T> V = make_view<T>(B, R, R.scratch);
-  V.nl = R.nl_cap > 0 ? R.nl : nullptr;
-  V.nlx = R.nlx;
+  V.nl = R.nl_cap > 0 ? (AW_G int *)R.nl : nullptr;
+  V.nlx = (AW_G double *)R.nlx;
   V.nl_cap = R.nl_cap;
   V.nl_state = R.nl + 4;
   V.wq = V.nl ? V.nl + AW_NL_HDR + R.nl_cap : nullptr;
@@ -357,7 +357,7 @@ __device__ __forceinline__ void w_scan(const View<T> &V)
   E.clear();
   if (V.wq && V.nl[1]) {
     int cap = V.nl_cap;
-    int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
+    AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
     for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
   } else {
     for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B(V, i, j, E); });
@@ -373,7 +373,7 @@ __device__ __forceinline__ void w_queues(const View<T> &V)
   E.clear();
   if (V.wq && V.nl[1]) {
     int cap = V.nl_cap;
-    const int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
+    const AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
     int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
     for_packed(qd, 0, nd, W_TID, W_NT, [&](int i, int j) { dssp_term(V, i, j, E); });
     for_packed(qp, 0, npap, W_TID, W_NT, [&](int i, int j) { pap_term(V, i, j, E); });
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(256) k_w_int1(BatchDev B, LgParams lg, int s)
   const ReplicaDesc &R = B.rep[blockIdx.y];
   if (!w_active(R, s)) return;
   View<T> V = w_view<T>(B, R);
-  integrate1(V, R.v, lg, W_TID, W_NT);
+  integrate1(V, (AW_G double *)R.v, lg, W_TID, W_NT);
 }
 
 template <class T>
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_w_int2(BatchDev B, LgParams lg, int s)
   const ReplicaDesc &R = B.rep[blockIdx.y];
   if (!w_active(R, s)) return;
   View<T> V = w_view<T>(B, R);
-  integrate2(V, R.v, lg, R, s, W_TID, W_NT);
+  integrate2(V, (AW_G double *)R.v, lg, R, s, W_TID, W_NT);
 }
 
 template <class T>
