@@ -32,6 +32,7 @@ inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
 typedef const FFParams *ParamPtr;
 #define AW_SCHED_FENCE() ((void)0)
 #define AW_G
+inline void aw_hold(int &) {}
 template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
 template <class ItemT> inline ItemT ld_item(const ItemT *p) { return *p; }
@@ -78,6 +79,9 @@ typedef const __attribute__((address_space(4))) FFParams *ParamPtr;
 // access also counts against the LDS wait counter, so every LDS wait after
 // it would wait for the global round trip too)
 #define AW_G __attribute__((address_space(1)))
+// a no-op that reads x: the wait for a load into x is placed here, not at
+// x's first real use (only a register operand; nothing is loaded)
+__device__ __forceinline__ void aw_hold(int &x) { asm volatile("" : "+v"(x)); }
 // typed accesses for code compiled apart from the kernel (a noinline
 // function), where a plain pointer would give flat instructions: ld_as /
 // aadd_as read / add into LDS (L) or global memory, ld_item loads a 16-byte
@@ -1462,8 +1466,28 @@ AW_DEV void for_packed(Src src, int k0, int k1, int t0, int ts, F &&body)
   int k = k0 + t0;
   int cur = k < k1 ? src[k] : 0;
   for (; k < k1; k += ts) {
-    int nxt = k + ts < k1 ? src[k + ts] : 0;
+    // unconditional (clamped to the last entry): a branch around the load
+    // would leave a path with no younger load, and the wait for cur would
+    // then become a wait for every outstanding load at the top of the body
+    int nxt = src[k + ts < k1 ? k + ts : k1 - 1];
     body(cur >> 16, cur & 0xffff);
+    cur = nxt;
+  }
+}
+
+// for_packed for a body that stores to global memory: body(i, j, nxt)
+// gets the prefetched next entry and must aw_hold(nxt) on every path before
+// its stores.  Vector-memory operations are counted in issue order, so a
+// wait for nxt after this pair's stores would also wait for the stores.
+template <class Src, class F>
+AW_DEV void for_packed_st(Src src, int k0, int k1, int t0, int ts, F &&body)
+{
+  int k = k0 + t0;
+  int cur = k < k1 ? src[k] : 0;
+  aw_hold(cur);  // complete on entry as on the back edge: no wait in the loop
+  for (; k < k1; k += ts) {
+    int nxt = src[k + ts < k1 ? k + ts : k1 - 1];
+    body(cur >> 16, cur & 0xffff, nxt);
     cur = nxt;
   }
 }
@@ -1540,7 +1564,7 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
 // loop of the queued pass carries no in-place term code.
 template <class T, bool Q = false>
 AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullptr, AW_G int *qp = nullptr,
-                   AW_G int *qe = nullptr, AW_G int *qs = nullptr)
+                   AW_G int *qe = nullptr, AW_G int *qs = nullptr, int *hold = nullptr)
 {
   ParamPtr P = V.P;
   // every parameter the scan reads, loaded before any branch: invariant and
@@ -1560,49 +1584,56 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
   float rca = rsq_f(XA(V, K_CA, i), XA(V, K_CA, j));
   float eij = V.ext()[i] + V.ext()[j] + 1e-3f;
   float lim = reach + eij;
-  if (!(rca <= lim * lim * 1.0001f)) return;
-  int ri = V.rinfo[i], rj = V.rinfo[j];
-  bool same = rchain(ri) == rchain(rj);
-  int dres = j - i;
-  // DL1 densities (fb.cpp:7729-7786); the exact FP64 distance only for
-  // pairs near a well
+  // no early exit for pairs beyond every reach: all lanes reach the queue
+  // stores through one join (a divergent return is joined after the
+  // stores, and its path would then wait for them)
+  const bool near = rca <= lim * lim * 1.0001f;
   bool dq = false;  // a pair the water-xi and force passes (phases D, F) can use
-  if (!same || dres > 1) {
-    const double *si = site(V, i, ri), *sj = site(V, j, rj);
-    float e = rsq_f(si, sj);
-    if (qs) dq = e < dreach * dreach * (1.0f + AW_RSQ_MARGIN);
-    bool wc = water && e > (float)wmin2 * (1.0f - AW_RSQ_MARGIN) && e < (float)wmax2 * (1.0f + AW_RSQ_MARGIN);
-    bool hc = helix && e > (float)hmin2 * (1.0f - AW_RSQ_MARGIN) && e < (float)hmax2 * (1.0f + AW_RSQ_MARGIN);
-    if (wc || hc) {
-      double rsq = rsq_exact(si, sj);
-      if (wc && rsq > wmin2 && rsq < wmax2) {
-        T th, prd;
-        well_theta<T>(wk, wr0, wr1, sqrt_t<T>(rsq), th, prd);
-        aadd(&V.rho_w()[i], (double)th);
-        aadd(&V.rho_w()[j], (double)th);
-      }
-      if (hc && rsq > hmin2 && rsq < hmax2) {
-        T th, prd;
-        well_theta<T>(hk, hr0, hr1, sqrt_t<T>(rsq), th, prd);
-        aadd(&V.rho_h()[i], (double)th);
-        aadd(&V.rho_h()[j], (double)th);
+  bool dij = false, dji = false, pap = false, ev = false;
+  int ri = 0, rj = 0;
+  if (near) {
+    ri = V.rinfo[i]; rj = V.rinfo[j];
+    bool same = rchain(ri) == rchain(rj);
+    int dres = j - i;
+    // DL1 densities (fb.cpp:7729-7786); the exact FP64 distance only for
+    // pairs near a well
+    if (!same || dres > 1) {
+      const double *si = site(V, i, ri), *sj = site(V, j, rj);
+      float e = rsq_f(si, sj);
+      if (qs) dq = e < dreach * dreach * (1.0f + AW_RSQ_MARGIN);
+      bool wc = water && e > (float)wmin2 * (1.0f - AW_RSQ_MARGIN) && e < (float)wmax2 * (1.0f + AW_RSQ_MARGIN);
+      bool hc = helix && e > (float)hmin2 * (1.0f - AW_RSQ_MARGIN) && e < (float)hmax2 * (1.0f + AW_RSQ_MARGIN);
+      if (wc || hc) {
+        double rsq = rsq_exact(si, sj);
+        if (wc && rsq > wmin2 && rsq < wmax2) {
+          T th, prd;
+          well_theta<T>(wk, wr0, wr1, sqrt_t<T>(rsq), th, prd);
+          aadd(&V.rho_w()[i], (double)th);
+          aadd(&V.rho_w()[j], (double)th);
+        }
+        if (hc && rsq > hmin2 && rsq < hmax2) {
+          T th, prd;
+          well_theta<T>(hk, hr0, hr1, sqrt_t<T>(rsq), th, prd);
+          aadd(&V.rho_h()[i], (double)th);
+          aadd(&V.rho_h()[j], (double)th);
+        }
       }
     }
+    // DSSP in both orientations (fb.cpp:8064-8094)
+    float ld = (float)dcut + eij;
+    if (dssp_on && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
+      if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST))
+        dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), dcut, dcut2);
+      if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST))
+        dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), dcut, dcut2);
+    }
+    // P_AP (fb.cpp:8096-8103)
+    pap = pap_on && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), pcut2);
+    float le = evreach + eij;
+    ev = ev_on && rca <= le * le * 1.0001f;
   }
-  // DSSP in both orientations (fb.cpp:8064-8094)
-  bool dij = false, dji = false;
-  float ld = (float)dcut + eij;
-  if (dssp_on && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
-    if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST))
-      dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), dcut, dcut2);
-    if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST))
-      dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), dcut, dcut2);
-  }
-  // P_AP (fb.cpp:8096-8103)
-  bool pap = pap_on && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), pcut2);
-  float le = evreach + eij;
-  bool ev = ev_on && rca <= le * le * 1.0001f;
   if constexpr (Q) {
+    if (hold) aw_hold(*hold);  // the list scan's next entry, before the stores
     // queue counters nl_state[2..5]: DSSP (both orientations), P_AP,
     // excluded volume, density pairs
     lmask m1 = w_ballot(dij), m2 = w_ballot(dji), mp = w_ballot(pap), me = w_ballot(ev), ms = w_ballot(dq);
@@ -1662,7 +1693,11 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
         // density pairs are kept for phases D and F when the list holds
         // every pair (one chunk)
         AW_G int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
-        for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
+        if (V.nl && V.nl[1])
+          for_packed_st(V.nl + AW_NL_HDR, c0, c1, t0, ts,
+                        [&](int i, int j, int &nx) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs, &nx); });
+        else
+          for_pairs_range(V, c0, c1, t0, ts, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
         AW_BARRIER();
         if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(12);
         int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
