@@ -33,6 +33,7 @@ typedef const FFParams *ParamPtr;
 #define AW_SCHED_FENCE() ((void)0)
 #define AW_G
 inline void aw_hold(int &) {}
+inline double aw_sval(double v) { return v; }
 template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
 template <class ItemT> inline ItemT ld_item(const ItemT *p) { return *p; }
@@ -82,6 +83,15 @@ typedef const __attribute__((address_space(4))) FFParams *ParamPtr;
 // a no-op that reads x: the wait for a load into x is placed here, not at
 // x's first real use (only a register operand; nothing is loaded)
 __device__ __forceinline__ void aw_hold(int &x) { asm volatile("" : "+v"(x)); }
+// a wave-uniform value kept in scalar registers: a per-lane choice between
+// two such values then stays a choice, instead of being folded into one
+// load from a per-lane address (a vector load and its wait)
+__device__ __forceinline__ double aw_sval(double v)
+{
+  unsigned long long b = __double_as_longlong(v);
+  unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b), hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 // typed accesses for code compiled apart from the kernel (a noinline
 // function), where a plain pointer would give flat instructions: ld_as /
 // aadd_as read / add into LDS (L) or global memory, ld_item loads a 16-byte
@@ -1147,40 +1157,57 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   const double an_ = P->an, bn_ = P->bn, cn_ = P->cn, ah_ = P->ah, bh_ = P->bh, ch_ = P->ch;
   const int nr = P->n_rama_par;
   const int *RI = V.rinfo;
+  const int n = V.n;
   int ri = RI[i], rj = RI[j];
+  // neighbours' residue info; indices clamped into the system (a clamped
+  // entry is never used: i is not a chain's last residue, j not its first)
+  int rip1 = RI[i + 1 < n ? i + 1 : i], rim1 = RI[i > 0 ? i - 1 : i];
+  int rjp1 = RI[j + 1 < n ? j + 1 : j], rjm1 = RI[j > 0 ? j - 1 : j];
+  int rip2 = RI[i + 2 < n ? i + 2 : i], rim2 = RI[i >= 2 ? i - 2 : i];
+  int rjp2 = RI[j + 2 < n ? j + 2 : j], rjm2 = RI[j >= 2 ? j - 2 : j];
   int ic = rchain(ri), jc = rchain(rj);
-  int i_ch_start = V.ch_start[ic] + 1, j_ch_start = V.ch_start[jc] + 1;          // 1-based like ch_pos
-  int i_ch_end = V.ch_start[ic] + V.ch_len[ic], j_ch_end = V.ch_start[jc] + V.ch_len[jc];
-  bool jlast = rj & RI_LAST;
-  bool i_rep = !(jlast || (RI[j + (jlast ? 0 : 1)] & RI_PRO));
-  bool i_AP = !((ri & RI_FIRST) || jlast || (ri & RI_PRO));
-  bool i_P = !(i >= i_ch_end - 2 || jlast || (RI[i + 2 < V.n ? i + 2 : i] & RI_PRO));
-  int ti = rtype(ri), tj = rtype(rj);
-  T tsa_HB[2] = {0, 0}, tsa_NHB[2] = {0, 0}, tsp_HB[2] = {0, 0};
-  for (int k = 0; k < 2; ++k) {
-    if (i_AP) {
-      tsa_HB[k] = (T)(0.5 * P->anti_HB[ti][tj][k]);
-      tsa_NHB[k] = (T)(0.25 * (P->anti_NHB[rtype(RI[i + 1])][rtype(RI[j - 1])][k] + P->anti_NHB[rtype(RI[i - 1])][rtype(RI[j + 1])][k]));
-    }
-    if (i_P) tsp_HB[k] = (T)P->para_HB[rtype(RI[i + 1])][tj][k];
-  }
-  T lambda[4] = {0, 0, 0, 0};
+  // chains are contiguous index ranges: i +- 2 lie in i's chain when the
+  // residues there carry its chain id (no chain-bound table read)
+  bool ip2_in = i + 2 < n && rchain(rip2) == ic, im2_in = i >= 2 && rchain(rim2) == ic;
+  bool jp2_in = j + 2 < n && rchain(rjp2) == jc, jm2_in = j >= 2 && rchain(rjm2) == jc;
+  int ti = rtype(ri), tj = rtype(rj), tip1 = rtype(rip1), tim1 = rtype(rim1), tjp1 = rtype(rjp1), tjm1 = rtype(rjm1);
   int dij = j - i < 0 ? i - j : j - i;
-  // sequence-separation class (fb.cpp:2956-3002) and its hbscl row, read
-  // per lane
+  // sequence-separation class (fb.cpp:2956-3002)
   int cls = (ic == jc && dij < 45) ? (dij < 4 ? 0 : (dij < 18 ? 1 : 2)) : 3;
   int hb_class = cls + 1;
+  // every table entry the pair can use, read before any branch: the
+  // per-lane gathers then share one memory round trip instead of waiting
+  // one after another
+  const double aHB0 = P->anti_HB[ti][tj][0], aHB1 = P->anti_HB[ti][tj][1];
+  const double aNHB0 = P->anti_NHB[tip1][tjm1][0] + P->anti_NHB[tim1][tjp1][0];
+  const double aNHB1 = P->anti_NHB[tip1][tjm1][1] + P->anti_NHB[tim1][tjp1][1];
+  const double pHB0 = P->para_HB[tip1][tj][0], pHB1 = P->para_HB[tip1][tj][1];
+  const double aone = P->anti_one[ti] + P->anti_one[tj], pone = P->para_one[tip1] + P->para_one[tj];
+  const int nrk = nr > 0 ? nr - 1 : 0;
+  const double apsi = V.aps[i * 12 + nrk], apsj = V.aps[j * 12 + nrk];
   auto hr = P->hbscl[cls];
   double h[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) h[k] = hr[k];
+  bool jlast = rj & RI_LAST;
+  bool i_rep = !(jlast || (rjp1 & RI_PRO));
+  bool i_AP = !((ri & RI_FIRST) || jlast || (ri & RI_PRO));
+  // i >= chain end - 2 (fb.cpp) is i + 2 outside the chain
+  bool i_P = !(!ip2_in || jlast || (rip2 & RI_PRO));
+  T tsa_HB[2] = {0, 0}, tsa_NHB[2] = {0, 0}, tsp_HB[2] = {0, 0};
+  if (i_AP) {
+    tsa_HB[0] = (T)(0.5 * aHB0); tsa_HB[1] = (T)(0.5 * aHB1);
+    tsa_NHB[0] = (T)(0.25 * aNHB0); tsa_NHB[1] = (T)(0.25 * aNHB1);
+  }
+  if (i_P) { tsp_HB[0] = (T)pHB0; tsp_HB[1] = (T)pHB1; }
+  T lambda[4] = {0, 0, 0, 0};
   if (cls == 0) {
     lambda[0] = (T)-h[0]; lambda[1] = (T)-h[1];
   } else if (cls == 1) {
-    if (nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0) {
+    if (nr > 0 && apsi == 1.0 && apsj == 1.0) {
       lambda[0] = (T)-h[0];
       lambda[1] = (T)-h[1];
-      lambda[2] = (T)-h[2] - (T)h[3] * tsa_HB[0] - (T)h[4] * tsa_NHB[0] - (T)(h[5] * (P->anti_one[ti] + P->anti_one[tj]));
+      lambda[2] = (T)-h[2] - (T)h[3] * tsa_HB[0] - (T)h[4] * tsa_NHB[0] - (T)(h[5] * aone);
       lambda[3] = (T)-h[6];
     } else {
       lambda[1] = (T)-h[1];
@@ -1188,15 +1215,15 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   } else {
     lambda[0] = (T)-h[0];
     lambda[1] = (T)-h[1];
-    lambda[2] = (T)-h[2] - (T)h[3] * tsa_HB[1] - (T)h[4] * tsa_NHB[1] - (T)(h[5] * (P->anti_one[ti] + P->anti_one[tj]));
-    lambda[3] = (T)-h[6] - (T)h[7] * tsp_HB[1] - (T)(h[8] * (P->para_one[rtype(RI[i + 1])] + P->para_one[tj]));
+    lambda[2] = (T)-h[2] - (T)h[3] * tsa_HB[1] - (T)h[4] * tsa_NHB[1] - (T)(h[5] * aone);
+    lambda[3] = (T)-h[6] - (T)h[7] * tsp_HB[1] - (T)(h[8] * pone);
   }
   // nu switches (arguments formed in FP64); the nu < 1e-4 early exit
   // (fb.cpp:3052) is a hard threshold, decided as in FP64 (dssp_nu_below)
   T nu0 = 0, prdnu0 = 0, nu1 = 0, prdnu1 = 0;
   double r0 = 0.0, r1 = 0.0;
-  bool nu0_on = (i - 2 >= i_ch_start - 1 && i + 2 < i_ch_end && hb_class != 2);
-  bool nu1_on = (j - 2 >= j_ch_start - 1 && j + 2 < j_ch_end && hb_class != 2);
+  bool nu0_on = (im2_in && ip2_in && hb_class != 2);
+  bool nu1_on = (jm2_in && jp2_in && hb_class != 2);
   if (nu0_on) {
     double rr = rsq_exact(XA(V, K_CA, i + 2), XA(V, K_CA, i - 2));
     if (rr > cut1) {
@@ -1275,20 +1302,25 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
   const int d = P->i_diff_pap, mn = P->i_med_min, mx = P->i_med_max, nr = P->n_rama_par;
   const double kp0 = P->k_pap[0], kp1 = P->k_pap[1], kp2 = P->k_pap[2];
   const double kbeta = P->k_betapred_pap, kglob = P->k_global_pap;
-  T nu_ij, prd_ij;
-  pap_nu(V, q, i, j, nu_ij, prd_ij);
-  if (!pap_keep(V, q, i, j, nu_ij)) return;
+  // the pair's global reads (chain bounds, secondary-structure weights),
+  // issued before the switch so that their latency overlaps it
   const int *RI = V.rinfo;
   int n = V.n;
   int ic = rchain(RI[i]), jc = rchain(RI[j]);
-  int i_ch_end = V.ch_start[ic] + V.ch_len[ic], j_ch_end = V.ch_start[jc] + V.ch_len[jc];
+  const int ics = V.ch_start[ic], icl = V.ch_len[ic], jcs = V.ch_start[jc], jcl = V.ch_len[jc];
+  const int nrk = nr > 0 ? nr - 1 : 0;
+  const double apsi = V.aps[i * 12 + nrk], apsj = V.aps[j * 12 + nrk];
+  T nu_ij, prd_ij;
+  pap_nu(V, q, i, j, nu_ij, prd_ij);
+  if (!pap_keep(V, q, i, j, nu_ij)) return;
+  int i_ch_end = ics + icl, j_ch_end = jcs + jcl;
   int jhi = (i + mx + 2 * d) < (i_ch_end - 1) ? (i + mx + 2 * d) : (i_ch_end - 1);
   bool med = ic == jc && i < i_ch_end - (mn + 2 * d) && j >= i + (mn + 2 * d) && j <= jhi;
   bool lng = (ic == jc && i < i_ch_end - (mx + 2 * d + 1) && j >= i + (mx + 2 * d + 1) && j < j_ch_end) ||
              (ic != jc && i + d < n && j - d >= 0 && rchain(RI[i + d]) == ic && rchain(RI[j - d]) == jc);
   bool par = (ic == jc && i < i_ch_end - (mx + 1 + d) && j >= i + (mx + 1) && j < i_ch_end - d) ||
              (ic != jc && i + d < n && j + d < n && rchain(RI[i + d]) == ic && rchain(RI[j + d]) == jc);
-  bool beta = nr > 0 && V.aps[i * 12 + nr - 1] == 1.0 && V.aps[j * 12 + nr - 1] == 1.0;
+  bool beta = nr > 0 && apsi == 1.0 && apsj == 1.0;
   if (med || lng) {
     double K = beta ? (med ? kp0 : 0.0) + (lng ? kp1 * kbeta : 0.0)
                     : (med ? kp0 : 0.0) + (lng ? kp1 : 0.0);
@@ -1322,7 +1354,7 @@ AW_DEV int atype(int kind, int ri) { return kind == K_CA ? 1 : (kind == K_O ? 3 
 template <class TabPtr>
 AW_DEV double ev_pick(TabPtr tab, int ta0, int ta1, int tb0, int tb1, bool gi, bool gj)
 {
-  double v00 = tab[ta0][tb0], v01 = tab[ta0][tb1], v10 = tab[ta1][tb0], v11 = tab[ta1][tb1];
+  double v00 = aw_sval(tab[ta0][tb0]), v01 = aw_sval(tab[ta0][tb1]), v10 = aw_sval(tab[ta1][tb0]), v11 = aw_sval(tab[ta1][tb1]);
   return gi ? (gj ? v11 : v10) : (gj ? v01 : v00);
 }
 
