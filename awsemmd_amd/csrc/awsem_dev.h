@@ -160,7 +160,10 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
   for (;;) {
     if (threadIdx.x == 0) s_rep = atomicAdd(B.queue, 1);
     __syncthreads();
-    int slot = s_rep;
+    // wave-uniform (an LDS broadcast is not known to be): the descriptor,
+    // n and every slab address derived from it then live in scalar
+    // registers instead of vector registers (and their spills)
+    int slot = __builtin_amdgcn_readfirstlane(s_rep);
     __syncthreads();
     if (slot >= B.nrep) break;
     int r = B.order[slot];

@@ -704,6 +704,11 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
   const bool pro = (ri & RI_PRO) && P->rama_p_flag;
   const int nk = pro ? P->n_rama_p_par : P->n_rama_par;
   const AW_G double *apsr = V.aps + i * 12 + (pro ? 6 : 0);
+  // the residue's six weights of its set (in its 12-entry row), read
+  // together before the wells
+  double apsk[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) apsk[k] = apsr[k];
   double fphi = 0, fpsi = 0;
   double ev = 0.0;
 #pragma unroll
@@ -715,7 +720,7 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
     const double cs0 = P->cpsi0[k], cs1 = P->cpsi0[6 + k], ss0 = P->spsi0[k], ss1 = P->spsi0[6 + k];
     if (k >= nk) break;
     const bool ssw = pro ? ssw1 != 0 : ssw0 != 0;
-    const double apsv = apsr[k];
+    const double apsv = apsk[k];
     if (ssw && apsv == 0.0) continue;
     const double cf = pro ? cf1 : cf0, sf = pro ? sf1 : sf0, cs = pro ? cs1 : cs0, ss = pro ? ss1 : ss0;
     // cos/sin(phi + phi0) by angle addition
@@ -1883,14 +1888,19 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
   int ri = V.rinfo[i], rj = V.rinfo[j];
   if (!(rchain(ri) != rchain(rj) || j - i >= cc)) return;
   if (!queued && !near_pair(V, i, j, dreach)) return;
+  int it = rtype(ri), jt = rtype(rj);
+  // wells 0 and 1's gammas (per-lane gathers), issued before the distance
+  // so that their latency overlaps it
+  const double g00 = P->water_gamma[0][it][jt][0], g01 = P->water_gamma[0][it][jt][1];
+  const double g10 = P->water_gamma[1][it][jt][0], g11 = P->water_gamma[1][it][jt][1];
   double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
   double r = sqrt_t<T>(rsq);
-  int it = rtype(ri), jt = rtype(rj);
   double acc_i = 0.0, acc_j = 0.0;
   for (int w = 0; w < nw; ++w) {
     WellP wp = w == 0 ? w0 : (w == 1 ? w1 : well_p(P, w));
     if (!wp.flag) continue;
-    double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1];
+    double g0 = w == 0 ? g00 : (w == 1 ? g10 : P->water_gamma[w][it][jt][0]);
+    double g1 = w == 0 ? g01 : (w == 1 ? g11 : P->water_gamma[w][it][jt][1]);
     // fb.cpp:7925 reads !fabs(g0-g1)<delta, i.e. (!fabs(g0-g1)) < delta: true iff g0 != g1
     if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > wp.min2 && rsq < wp.max2) {
       T th, prd;
@@ -1945,6 +1955,11 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
   int dres = j - i;
   if (!(!same || dres > 1)) return;
   if (!queued && !near_pair(V, i, j, dreach)) return;
+  int it = rtype(ri), jt = rtype(rj);
+  // wells 0 and 1's gammas (per-lane gathers), issued before the distance
+  // so that their latency overlaps it
+  const double g00 = P->water_gamma[0][it][jt][0], g01 = P->water_gamma[0][it][jt][1];
+  const double g10 = P->water_gamma[1][it][jt][0], g11 = P->water_gamma[1][it][jt][1];
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
   double rsq = rsq_exact(si, sj);
   double r = sqrt_t<T>(rsq);
@@ -1954,13 +1969,13 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
   T th0 = 0, prd0 = 0;
   if (in0) well_theta<T>(wk, w0.r0, w0.r1, r, th0, prd0);
   if (water) {
-    int it = rtype(ri), jt = rtype(rj);
     double ew = 0.0;
     for (int w = 0; w < nw; ++w) {
       WellP wp = w == 0 ? w0 : (w == 1 ? w1 : well_p(P, w));
       if (!wp.flag) continue;
       if ((!same || dres >= cc) && rsq > wp.min2 && rsq < wp.max2) {
-        double g0 = P->water_gamma[w][it][jt][0], g1 = P->water_gamma[w][it][jt][1], sg;
+        double g0 = w == 0 ? g00 : (w == 1 ? g10 : P->water_gamma[w][it][jt][0]);
+        double g1 = w == 0 ? g01 : (w == 1 ? g11 : P->water_gamma[w][it][jt][1]), sg;
         if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
         else sg = g0 + (g1 - g0) * V.sig_w()[i] * V.sig_w()[j];
         T th = th0, prd = prd0;
