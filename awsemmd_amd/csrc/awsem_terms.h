@@ -1618,8 +1618,16 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
   const bool ev_on = (P->ev_on != 0) & ((mask >> AW_ET_VEXCLUDED) & 1u);
   const double dcut = P->dssp_cut, dcut2 = P->dssp_cut_sq, pcut2 = P->pap_cutoff_sq;
   // CA-CA distance (FP32) against each term's reach (the near_pair test)
-  float rca = rsq_f(XA(V, K_CA, i), XA(V, K_CA, j));
-  float eij = V.ext()[i] + V.ext()[j] + 1e-3f;
+  // the pair's first reads (CA positions, extents, residue info) issued
+  // together, ahead of any use: one LDS round trip instead of one per
+  // quantity
+  const double *pci = XA(V, K_CA, i), *pcj = XA(V, K_CA, j);
+  const double ci[3] = {pci[0], pci[1], pci[2]}, cj[3] = {pcj[0], pcj[1], pcj[2]};
+  const float exi = V.ext()[i], exj = V.ext()[j];
+  const int ri0 = V.rinfo[i], rj0 = V.rinfo[j];
+  AW_SCHED_FENCE();
+  float rca = rsq_f(ci, cj);
+  float eij = exi + exj + 1e-3f;
   float lim = reach + eij;
   // no early exit for pairs beyond every reach: all lanes reach the queue
   // stores through one join (a divergent return is joined after the
@@ -1629,7 +1637,7 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
   bool dij = false, dji = false, pap = false, ev = false;
   int ri = 0, rj = 0;
   if (near) {
-    ri = V.rinfo[i]; rj = V.rinfo[j];
+    ri = ri0; rj = rj0;
     bool same = rchain(ri) == rchain(rj);
     int dres = j - i;
     // DL1 densities (fb.cpp:7729-7786); the exact FP64 distance only for
@@ -1665,7 +1673,7 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
         dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), dcut, dcut2);
     }
     // P_AP (fb.cpp:8096-8103)
-    pap = pap_on && rsq_lt_est(rca, XA(V, K_CA, i), XA(V, K_CA, j), pcut2);
+    pap = pap_on && rsq_lt_est(rca, ci, cj, pcut2);
     float le = evreach + eij;
     ev = ev_on && rca <= le * le * 1.0001f;
   }
