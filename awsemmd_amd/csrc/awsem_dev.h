@@ -136,16 +136,16 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
     __syncthreads();
     prof_mark(prof, first, prof ? 8 + q : 10, t);
   }
-  phase_C(V, t0, ts);
+  phase_C(V, t0, ts, ef);
   __syncthreads();
   prof_mark(prof, first, 2, t);
-  phase_D(V, t0, ts);
+  phase_D(V, t0, ts, ef);
   __syncthreads();
   prof_mark(prof, first, 3, t);
   phase_E(V, t0, ts);
   __syncthreads();
   prof_mark(prof, first, 4, t);
-  phase_F(V, t0, ts);
+  phase_F(V, t0, ts, ef);
   __syncthreads();
   prof_mark(prof, first, 5, t);
 }

@@ -1716,7 +1716,8 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
   E.clear();
   ParamPtr P = V.P;
   int nfm = (P->fm_flag && ON(P, AW_ET_FRAGMEM) && (part & 1)) ? V.fm_items : 0;
-  if (nfm > 0) {
+  auto run_fm = [&]() {
+    if (nfm <= 0) return;
     typedef typename T4<T>::item ItemT;
     const ItemT *items = (const ItemT *)V.items + V.fm_item_base;
     // the queued (resident) pass gives a lane many items: pipelined loads
@@ -1725,7 +1726,13 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
       E.fm += fm_run<T, XLDS, true, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{V.f(), V.n});
     else
       fm_run<T, XLDS, false, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{V.f(), V.n});
-  }
+  };
+  // fragment memory runs after the last chunk's queue passes, before their
+  // closing barrier: a wave that is done with its queue items starts on
+  // the arithmetic-bound fragment memory while the other waves of its SIMD
+  // still wait on the queue passes' memory accesses (no pass reads what
+  // another one writes: they only add forces)
+  bool close = false;  // the last chunk's queue passes still to be closed
   if (part & 2)
     for (int k = t0; k < V.n; k += ts) residue_local(V, k, E);
   if (part & 4) {
@@ -1751,6 +1758,10 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
         for_packed(qp, 0, npap, t0, ts, [&](int i, int j) { pap_term(V, i, j, E); });
         if constexpr (!std::is_void<PM>::value) if (pm) { AW_BARRIER(); pm->mark(14); }
         for_packed(qe, 0, nev, t0, ts, [&](int i, int j) { ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E); });
+        if (c0 + cap >= total) {
+          close = true;  // closed after the fragment memory below
+          break;
+        }
         AW_BARRIER();
         if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(15);
         if (t0 == 0) V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
@@ -1760,7 +1771,19 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
       for_pairs(V, t0, ts, [&](int i, int j) { pair_B(V, i, j, E); });
     }
   }
-  eflush(E, V.ered);
+  // the energies so far are flushed before the fragment-memory call, so
+  // that no accumulator stays live across it; only an evaluation whose
+  // energies are reported (ef) reduces them at all
+  if (ef) eflush(E, V.ered);
+  E.clear();
+  run_fm();
+  if (QUEUED && close) {
+    AW_BARRIER();
+    if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(15);
+    if (t0 == 0) V.nl_state[2] = V.nl_state[3] = V.nl_state[4] = 0;
+    AW_BARRIER();
+  }
+  if (ef) eflush(E, V.ered);
 }
 
 // ------------------------------------------------------------------------
@@ -1797,7 +1820,7 @@ AW_DEV double sw_tanh(double x)
 }
 
 template <class T>
-AW_DEV void phase_C(const View<T> &V, int t0, int ts)
+AW_DEV void phase_C(const View<T> &V, int t0, int ts, bool ef = true)
 {
   EAcc E;
   E.clear();
@@ -1827,7 +1850,7 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts)
       if (ON(P, AW_ET_BURIAL)) E.burial += eb;
     }
   }
-  eflush(E, V.ered);
+  if (ef) eflush(E, V.ered);  // energies only where reported
 }
 
 // ------------------------------------------------------------------------
@@ -1923,7 +1946,7 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
 }
 
 template <class T>
-AW_DEV void phase_D(const View<T> &V, int t0, int ts)
+AW_DEV void phase_D(const View<T> &V, int t0, int ts, bool ef = true)
 {
   EAcc E;
   E.clear();
@@ -1932,7 +1955,7 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts)
   int nh = P->helix_flag ? n : 0;
   for (int k = t0; k < nh; k += ts) helix_item(V, k, E);
   if (P->water_flag) for_density_pairs(V, t0, ts, [&](int i, int j, bool q) { xi_pair(V, i, j, q); });
-  eflush(E, V.ered);
+  if (ef) eflush(E, V.ered);  // energies only where reported
 }
 
 template <class T>
@@ -2020,12 +2043,12 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
 }
 
 template <class T>
-AW_DEV void phase_F(const View<T> &V, int t0, int ts)
+AW_DEV void phase_F(const View<T> &V, int t0, int ts, bool ef = true)
 {
   EAcc E;
   E.clear();
   for_density_pairs(V, t0, ts, [&](int i, int j, bool q) { force_pair(V, i, j, E, q); });
-  eflush(E, V.ered);
+  if (ef) eflush(E, V.ered);  // energies only where reported
 }
 
 // ------------------------------------------------------------------------
