@@ -49,8 +49,9 @@ struct BatchDev {
 // bytes of a workgroup's LDS left to the resident kernel's static arrays
 #define AW_LDS_STATIC_RESERVE 256
 
-// doubles of per-replica working storage (x, xv, f, 10 scalars, ext, rinfo copy)
-__host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + 10LL * n + 2LL * ((n + 1) / 2) + 8; }
+// doubles of per-replica working storage (x, xv, f, the per-residue
+// scalars, ext, rinfo copy)
+__host__ __device__ inline long long slab_doubles(int n) { return 27LL * n + (long long)AW_SC_STRIDE * n + 2LL * ((n + 1) / 2) + 8; }
 
 template <class T>
 __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDesc &R, double *slab)

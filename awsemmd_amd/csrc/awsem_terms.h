@@ -282,6 +282,8 @@ template <class T> AW_DEV V3<T> crs(const V3<T> &a, const V3<T> &b)
 // ------------------------------------------------------------------------
 // per-replica view: every pointer is either LDS or a global scratch slab
 
+#define AW_SC_STRIDE 11  // doubles per residue for the ten per-residue scalars
+
 template <class T>
 struct View {
   int n, nch;
@@ -304,24 +306,32 @@ struct View {
   int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts
   AW_G int *wq;          // global work queues beside the list: DSSP [2 cap], P_AP [cap], EV [cap], density pairs [cap]
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
-  // f[3][n][3], ten per-residue scalars, ext (float)
+  // f[3][n][3], the ten per-residue scalars [n][AW_SC_STRIDE], ext (float)
   AW_DEV double *x() const { return s; }
   AW_DEV double *xv() const { return s + 9 * n; }
   AW_DEV double *f() const { return s + 18 * n; }
-  AW_DEV double *sc(int k) const { return s + 27 * n + k * n; }
-  AW_DEV double *rho_w() const { return sc(0); }
-  AW_DEV double *rho_h() const { return sc(1); }
-  AW_DEV double *sig_w() const { return sc(2); }
-  AW_DEV double *sigp_w() const { return sc(3); }
-  AW_DEV double *sig_h() const { return sc(4); }
-  AW_DEV double *sigp_h() const { return sc(5); }
-  AW_DEV double *xi_w() const { return sc(6); }
-  AW_DEV double *bf() const { return sc(7); }
-  AW_DEV double *hxi1() const { return sc(8); }
-  AW_DEV double *hxi2() const { return sc(9); }
-  AW_DEV float *ext() const { return (float *)sc(10); }
+  // the per-residue scalars are stored residue by residue: one base
+  // address serves all ten (the scalar's index is an instruction offset),
+  // which frees the registers ten array bases took in the pair loops; the
+  // stride of 11 doubles keeps 16 consecutive residues on distinct LDS banks
+  struct SRow {
+    double *p;
+    AW_DEV double &operator[](int i) const { return p[AW_SC_STRIDE * i]; }
+  };
+  AW_DEV SRow sc(int k) const { return SRow{s + 27 * n + k}; }
+  AW_DEV SRow rho_w() const { return sc(0); }
+  AW_DEV SRow rho_h() const { return sc(1); }
+  AW_DEV SRow sig_w() const { return sc(2); }
+  AW_DEV SRow sigp_w() const { return sc(3); }
+  AW_DEV SRow sig_h() const { return sc(4); }
+  AW_DEV SRow sigp_h() const { return sc(5); }
+  AW_DEV SRow xi_w() const { return sc(6); }
+  AW_DEV SRow bf() const { return sc(7); }
+  AW_DEV SRow hxi1() const { return sc(8); }
+  AW_DEV SRow hxi2() const { return sc(9); }
+  AW_DEV float *ext() const { return (float *)(s + (27 + AW_SC_STRIDE) * n); }
   // resident engine: a copy of rinfo after the extents
-  AW_DEV int *rinfo_copy() const { return (int *)(sc(10) + (n + 1) / 2); }
+  AW_DEV int *rinfo_copy() const { return (int *)(s + (27 + AW_SC_STRIDE) * n + (n + 1) / 2); }
 };
 
 #define XA(V, kind, i) ((V).x() + ((kind) * (V).n + (i)) * 3)
