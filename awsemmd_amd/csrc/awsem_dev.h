@@ -180,9 +180,11 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
     V.nl_cap = R.nl_cap;
     V.nl_state = nl_state;
     V.wq = V.nl ? V.nl + AW_NL_HDR + R.nl_cap : nullptr;
-    if (threadIdx.x < 8) nl_state[threadIdx.x] = 0;
+    if (threadIdx.x < 7) nl_state[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
       if ((R.flags & AW_RF_NL_RESET) && V.nl) V.nl[1] = 0;
+      nl_state[7] = 0;
+      if (V.nl) nl_state_init(V);
     }
     int n = R.n;
     AW_G double *gx = (AW_G double *)R.x, *gv = (AW_G double *)R.v, *gf = (AW_G double *)R.f;

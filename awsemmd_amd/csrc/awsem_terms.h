@@ -303,7 +303,8 @@ struct View {
   AW_G int *nl;          // global: header (AW_NL_HDR ints: [0] count, [1] valid, [2] builds), then (i<<16 | j)
   int nl_cap;
   AW_G double *nlx;      // global: CA positions [n][3] and extents [n] at the last build
-  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts
+  int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts,
+                         // [6] largest extent, [7] list pairs + 1 when valid, else 0
   AW_G int *wq;          // global work queues beside the list: DSSP [2 cap], P_AP [cap], EV [cap], density pairs [cap]
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
   // f[3][n][3], the ten per-residue scalars [n][AW_SC_STRIDE], ext (float)
@@ -1458,7 +1459,7 @@ AW_DEV bool near_pair(const View<T> &V, int i, int j, float reach)
 template <class T>
 AW_DEV bool nl_needs_build(const View<T> &V)
 {
-  return V.nl && (V.nl[1] == 0 || i2f(V.nl_state[0]) > 0.5f * (float)V.P->nl_skin);
+  return V.nl && (V.nl_state[7] == 0 || i2f(V.nl_state[0]) > 0.5f * (float)V.P->nl_skin);
 }
 
 template <class T>
@@ -1481,6 +1482,14 @@ AW_DEV void phase_nl_build(const View<T> &V, int t0, int ts)
   }
 }
 
+// launch start (one thread, after any reset of the header): the header's
+// count and validity into nl_state[7]
+template <class T>
+AW_DEV void nl_state_init(const View<T> &V)
+{
+  V.nl_state[7] = V.nl[1] ? V.nl[0] + 1 : 0;
+}
+
 // after the build's barrier: publish count and validity (one thread)
 template <class T>
 AW_DEV void nl_publish(const View<T> &V, bool built)
@@ -1490,6 +1499,9 @@ AW_DEV void nl_publish(const View<T> &V, bool built)
     V.nl[0] = c < V.nl_cap ? c : V.nl_cap;
     V.nl[1] = c <= V.nl_cap ? 1 : 0;
     V.nl[2] += 1;
+    // the same count and validity for this launch's passes, next to the
+    // queue counters (the header stays the host's record)
+    V.nl_state[7] = c <= V.nl_cap ? c + 1 : 0;
   }
   V.nl_state[0] = 0;
   V.nl_state[1] = 0;
@@ -1501,7 +1513,8 @@ AW_DEV void nl_publish(const View<T> &V, bool built)
 template <class T>
 AW_DEV int pair_total(const View<T> &V)
 {
-  return (V.nl && V.nl[1]) ? V.nl[0] : pair_count(V.n);
+  int c = V.nl ? V.nl_state[7] : 0;
+  return c > 0 ? c - 1 : pair_count(V.n);
 }
 
 // body(i, j) for the packed pairs src[k0 + t0], src[k0 + t0 + ts], ...: the
@@ -1542,7 +1555,7 @@ AW_DEV void for_packed_st(Src src, int k0, int k1, int t0, int ts, F &&body)
 template <class T, class F>
 AW_DEV void for_pairs_range(const View<T> &V, int k0, int k1, int t0, int ts, F &&body)
 {
-  const AW_G int *lst = (V.nl && V.nl[1]) ? V.nl + AW_NL_HDR : nullptr;
+  const AW_G int *lst = (V.nl && V.nl_state[7] > 0) ? V.nl + AW_NL_HDR : nullptr;
   if (lst) {
     for_packed(lst, k0, k1, t0, ts, body);
     return;
@@ -1568,7 +1581,7 @@ AW_DEV void for_density_pairs(const View<T> &V, int t0, int ts, F &&body)
 {
   // body(i, j, queued): queued pairs passed the site-distance test of the
   // scan, which implies the near_pair test (|CA-CA| <= |site-site| + extents)
-  const AW_G int *qs = (V.wq && V.nl && V.nl[1]) ? V.wq + 4 * V.nl_cap : nullptr;
+  const AW_G int *qs = (V.wq && V.nl && V.nl_state[7] > 0) ? V.wq + 4 * V.nl_cap : nullptr;
   if (qs) {
     for_packed(qs, 0, V.nl_state[5], t0, ts, [&](int i, int j) { body(i, j, true); });
     return;
@@ -1754,8 +1767,8 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
         int c1 = total - c0 < cap ? total : c0 + cap;
         // density pairs are kept for phases D and F when the list holds
         // every pair (one chunk)
-        AW_G int *qs = total <= cap && V.nl && V.nl[1] ? V.wq + 4 * cap : nullptr;
-        if (V.nl && V.nl[1])
+        AW_G int *qs = total <= cap && V.nl && V.nl_state[7] > 0 ? V.wq + 4 * cap : nullptr;
+        if (V.nl && V.nl_state[7] > 0)
           for_packed_st(V.nl + AW_NL_HDR, c0, c1, t0, ts,
                         [&](int i, int j, int &nx) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs, &nx); });
         else
@@ -2140,15 +2153,22 @@ template <class T>
 AW_DEV void integrate2(const View<T> &V, AW_G double *v, const LgParams &lg, const ReplicaDesc &R, int s, int t0, int ts)
 {
   int n = V.n;
+  // the replica's run constants, read together ahead of any use (one
+  // memory round trip)
+  const double t_start = R.t_start, t_stop = R.t_stop;
+  const unsigned long long seed = R.seed;
+  const long long step0 = R.step0;
+  const int ramp0 = R.ramp0, run_steps = R.run_steps, rng_id = R.rng_id;
+  AW_SCHED_FENCE();
   // the target temperature ramps over the replica's whole run (which may
   // span several launches)
-  double delta = R.run_steps > 0 ? (double)(R.ramp0 + s + 1) / (double)R.run_steps : 0.0;
-  double Tt = R.t_start + delta * (R.t_stop - R.t_start);
+  double delta = run_steps > 0 ? (double)(ramp0 + s + 1) / (double)run_steps : 0.0;
+  double Tt = t_start + delta * (t_stop - t_start);
   double tsq = sqrt(Tt);
-  long long step = R.step0 + s;
+  long long step = step0 + s;
   // urand's hash up to the step, per component (the atom enters last)
   uint64_t hs[3];
-  for (int l = 0; l < 3; ++l) hs[l] = urand_step(R.seed, R.rng_id, step, l);
+  for (int l = 0; l < 3; ++l) hs[l] = urand_step(seed, rng_id, step, l);
   for (int a = t0; a < 3 * n; a += ts) {
     int kind = (a >= n) + (a >= 2 * n), i = a - kind * n;
     int c = mass_class(kind, V.rinfo[i]);

@@ -52,6 +52,7 @@ __global__ void __launch_bounds__(256) k_w_begin(BatchDev B, int mode)
   if (W_TID == 0 && V.nl) {
     if (R.flags & AW_RF_NL_RESET) V.nl[1] = 0;
     for (int k = 0; k < 8; ++k) V.nl_state[k] = 0;
+    nl_state_init(V);
   }
 }
 
@@ -355,7 +356,7 @@ __device__ __forceinline__ void w_scan(const View<T> &V)
 {
   EAcc E;
   E.clear();
-  if (V.wq && V.nl[1]) {
+  if (V.wq && V.nl_state[7] > 0) {
     int cap = V.nl_cap;
     AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
     for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
@@ -371,7 +372,7 @@ __device__ __forceinline__ void w_queues(const View<T> &V)
 {
   EAcc E;
   E.clear();
-  if (V.wq && V.nl[1]) {
+  if (V.wq && V.nl_state[7] > 0) {
     int cap = V.nl_cap;
     const AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
     int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
