@@ -36,6 +36,10 @@ inline void aw_hold(int &) {}
 inline double aw_sval(double v) { return v; }
 template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
+template <bool L> struct aw_as { typedef double type; };
+template <bool L> inline double *aw_cast(double *p) { return p; }
+template <bool L> inline const double *aw_cast(const double *p) { return p; }
+inline void aadd_t(double *p, double v) { *p += v; }
 template <class ItemT> inline ItemT ld_item(const ItemT *p) { return *p; }
 template <class P> inline P *aw_uniform(P *p) { return p; }
 inline unsigned aw_umul24(unsigned a, unsigned b) { return (a & 0xffffffu) * (b & 0xffffffu); }
@@ -57,7 +61,10 @@ inline void q_reserve4(int *ctr, const int n[4], int b[4])
 #define AW_DEVCONST __device__ __constant__
 __device__ __forceinline__ float aw_fast_exp(float a) { return __expf(a); }
 __device__ __forceinline__ float aw_fast_exp2(float a) { return __builtin_amdgcn_exp2f(a); }
-__device__ __forceinline__ float aw_fsqrt(float a) { return __fsqrt_rn(a); }
+// the hardware square root: the same value as the rounded form for every
+// normal input (that form only rescales inputs below 2^-96 around it); its
+// one use is an interatomic distance
+__device__ __forceinline__ float aw_fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ float aw_frcp(float a) { return __builtin_amdgcn_rcpf(a); }
 __device__ __forceinline__ float aw_frsq(float a) { return __builtin_amdgcn_rsqf(a); }
 __device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
@@ -110,6 +117,26 @@ __device__ __forceinline__ void aadd_as(double *p, double v)
     __hip_atomic_fetch_add((aw_lds_double *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   else
     __hip_atomic_fetch_add((aw_global_double *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a base pointer converted to its address space once (aw_cast): indexing it
+// then needs no per-access conversion (a generic-to-LDS conversion carries a
+// null check, and generic addresses are 64-bit)
+template <bool L> struct aw_as { typedef aw_global_double type; };
+template <> struct aw_as<true> { typedef aw_lds_double type; };
+template <bool L>
+__device__ __forceinline__ typename aw_as<L>::type *aw_cast(double *p) { return (typename aw_as<L>::type *)p; }
+template <bool L>
+__device__ __forceinline__ const typename aw_as<L>::type *aw_cast(const double *p)
+{
+  return (const typename aw_as<L>::type *)p;
+}
+__device__ __forceinline__ void aadd_t(aw_lds_double *p, double v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void aadd_t(aw_global_double *p, double v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // a wave-uniform pointer moved to scalar registers (arguments of a device
 // function arrive in vector registers)
@@ -826,15 +853,15 @@ AW_DEV void fm_gauss(const T4t &f, T w, const T *r, T &S0, T &S1, T &S2, T &S3, 
 // FmSinkForce: adds into a replica's force array f[3][n][3]
 template <bool FLDS>
 struct FmSinkForce {
-  double *f;
+  typename aw_as<FLDS>::type *f;  // typed once (aw_cast): 32-bit LDS offsets when FLDS
   int n;
   template <class R>
   AW_DEV void operator()(int kind, int i, R fx, R fy, R fz) const
   {
-    double *p = f + ((size_t)kind * n + i) * 3;
-    aadd_as<FLDS>(p + 0, (double)fx);
-    aadd_as<FLDS>(p + 1, (double)fy);
-    aadd_as<FLDS>(p + 2, (double)fz);
+    auto *p = f + (kind * n + i) * 3;
+    aadd_t(p + 0, (double)fx);
+    aadd_t(p + 1, (double)fy);
+    aadd_t(p + 2, (double)fz);
   }
 };
 
@@ -912,6 +939,7 @@ AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
   // the bases are the same in every lane: scalar registers
   const char *rfb = (const char *)aw_uniform(rf), *cfb = (const char *)aw_uniform(coef);
   x = aw_uniform(x);
+  const auto *xs = aw_cast<XLDS>(x);  // coordinates in their own address space
   items = aw_uniform(items);
   double efm = 0.0;
   ItemT cur = ld_item(items + k);
@@ -936,10 +964,10 @@ AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
     double cai[3], caj[3], cbi[3], cbj[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      cai[c] = ld_as<XLDS>(x + 3 * i + c);
-      caj[c] = ld_as<XLDS>(x + 3 * j + c);
-      cbi[c] = ld_as<XLDS>(x + 3 * (n + i) + c);
-      cbj[c] = ld_as<XLDS>(x + 3 * (n + j) + c);
+      cai[c] = xs[3 * i + c];
+      caj[c] = xs[3 * j + c];
+      cbi[c] = xs[3 * (n + i) + c];
+      cbj[c] = xs[3 * (n + j) + c];
     }
     V3<T> d[4];
     d[0] = dvec<T>(cai, caj);
@@ -1746,9 +1774,9 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
     // the queued (resident) pass gives a lane many items: pipelined loads
     typedef FmSinkForce<XLDS> Sk;
     if (ef)
-      E.fm += fm_run<T, XLDS, true, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{V.f(), V.n});
+      E.fm += fm_run<T, XLDS, true, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{aw_cast<XLDS>(V.f()), V.n});
     else
-      fm_run<T, XLDS, false, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{V.f(), V.n});
+      fm_run<T, XLDS, false, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{aw_cast<XLDS>(V.f()), V.n});
   };
   // fragment memory runs after the last chunk's queue passes, before their
   // closing barrier: a wave that is done with its queue items starts on

@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
     else fm_run_body<T, false, false, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
   } else {
     typedef FmSinkForce<false> Sk;
-    Sk sink{V.f(), V.n};
+    Sk sink{aw_cast<false>(V.f()), V.n};
     if (ef) E.fm += fm_run_body<T, false, true, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
     else fm_run_body<T, false, false, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
   }
