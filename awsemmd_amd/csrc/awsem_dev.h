@@ -143,9 +143,6 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, unsigned long
   phase_D(V, t0, ts, ef);
   __syncthreads();
   prof_mark(prof, first, 3, t);
-  phase_E(V, t0, ts);
-  __syncthreads();
-  prof_mark(prof, first, 4, t);
   phase_F(V, t0, ts, ef);
   __syncthreads();
   prof_mark(prof, first, 5, t);

@@ -72,13 +72,6 @@ PH_KERNEL(k_ph_D, phase_D(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * 
 PH_KERNEL(k_ph_F, phase_F(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
 
 template <class T>
-__global__ void __launch_bounds__(256) k_ph_E(BatchDev B, int r)
-{
-  View<T> V = phase_view<T>(B, r);
-  phase_E(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-
-template <class T>
 __global__ void __launch_bounds__(256) k_ph_store(BatchDev B, int r, int with_x)
 {
   const ReplicaDesc &R = B.rep[r];
@@ -257,7 +250,6 @@ static void force_phases(const BatchDev &B, int r, int grid, hipStream_t st)
   hipLaunchKernelGGL(k_ph_B<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_C<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_D<T>, dim3(grid), dim3(256), 0, st, B, r);
-  hipLaunchKernelGGL(k_ph_E<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_F<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_energy_total, dim3(1), dim3(64), 0, st, B, r);
 }

@@ -2009,12 +2009,6 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts, bool ef = true)
   if (ef) eflush(E, V.ered);  // energies only where reported
 }
 
-template <class T>
-AW_DEV void phase_E(const View<T> &V, int t0, int ts)
-{
-  for (int i = t0; i < V.n; i += ts) V.xi_w()[i] *= V.sigp_w()[i];
-}
-
 // ------------------------------------------------------------------------
 // Phase F: DL3 density-site pair forces (fb.cpp:7996-8057, scatter 8116-8128)
 
@@ -2069,8 +2063,11 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
     E.water += ew;
     if (in0) {
       double xi = 0.0;
-      if (fabs(V.xi_w()[i]) > AW_DELTA_WATER_XI) xi += V.xi_w()[i];
-      if (fabs(V.xi_w()[j]) > AW_DELTA_WATER_XI) xi += V.xi_w()[j];
+      // xi times dsigma/drho (fb.cpp:7939-7950), formed here where it is
+      // read: the same product, without a pass and a barrier of its own
+      double xii = V.xi_w()[i] * V.sigp_w()[i], xij = V.xi_w()[j] * V.sigp_w()[j];
+      if (fabs(xii) > AW_DELTA_WATER_XI) xi += xii;
+      if (fabs(xij) > AW_DELTA_WATER_XI) xi += xij;
       if (xi != 0.0) force += prd0 * (T)xi;
     }
   }

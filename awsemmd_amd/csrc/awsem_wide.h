@@ -388,15 +388,6 @@ W_KERNEL(k_w_D, phase_D(V, W_TID, W_NT))
 W_KERNEL(k_w_F, phase_F(V, W_TID, W_NT))
 
 template <class T>
-__global__ void __launch_bounds__(256) k_w_E(BatchDev B, int s)
-{
-  const ReplicaDesc &R = B.rep[blockIdx.y];
-  if (!w_active(R, s)) return;
-  View<T> V = w_view<T>(B, R);
-  phase_E(V, W_TID, W_NT);
-}
-
-template <class T>
 __global__ void __launch_bounds__(256) k_w_int1(BatchDev B, LgParams lg, int s)
 {
   const ReplicaDesc &R = B.rep[blockIdx.y];
@@ -431,7 +422,6 @@ static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s)
   hipLaunchKernelGGL(k_w_B3<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_C<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_D<T>, g, dim3(256), 0, st, B, s);
-  hipLaunchKernelGGL(k_w_E<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_F<T>, g, dim3(256), 0, st, B, s);
 }
 

@@ -477,11 +477,12 @@ def main():
 
     if os.environ.get("AWSEM_PHASE_PROF"):
         ticks = batch.phase_clock() / 100.0  # microseconds
-        names = ["A", "B", "C", "D", "E", "F", "int1", "int2", "B_fm", "B_local", "B_pair", "list",
+        # clock slots; slot 4 is unused (the xi scaling is part of F)
+        names = ["A", "B", "C", "D", None, "F", "int1", "int2", "B_fm", "B_local", "B_pair", "list",
                  "pair_scan", "pair_dssp", "pair_pap", "pair_ev"]
         nsteps_all = (a.warmup + 1) + a.steps + (a.steps + a.chunk - 1) // a.chunk  # force evaluations
-        log("phase us/eval, longest replica: " + " ".join(f"{n}={t / nsteps_all:.1f}" for n, t in zip(names, ticks[:16])))
-        log("phase us/eval, mean replica:    " + " ".join(f"{n}={t / nsteps_all / a.replicas:.1f}" for n, t in zip(names, ticks[16:32])))
+        log("phase us/eval, longest replica: " + " ".join(f"{n}={t / nsteps_all:.1f}" for n, t in zip(names, ticks[:16]) if n))
+        log("phase us/eval, mean replica:    " + " ".join(f"{n}={t / nsteps_all / a.replicas:.1f}" for n, t in zip(names, ticks[16:32]) if n))
     # force error of the batched FP32 path against the FP64 path (sample)
     batch.compute()
     xs, vs, fs, es = batch.get_state()
