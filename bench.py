@@ -446,7 +446,9 @@ def main():
     tuned = None
     if not a.no_autotune:
         t0 = time.perf_counter()
-        tuned = batch.autotune(probe_steps=40, step0=a.warmup, **lg)
+        # 160-step probes: each candidate is timed over several list-rebuild
+        # cycles (a short probe from freshly built lists favours small skins)
+        tuned = batch.autotune(probe_steps=160, step0=a.warmup, **lg)
         tuned["tuning_s"] = time.perf_counter() - t0
         log(f"autotuned: {tuned['engine']} engine, skin {tuned['skin']} A, {tuned['steps_per_launch']} steps per launch "
             f"({tuned['ms_per_step']:.3f} ms/step) in {tuned['tuning_s']:.1f}s")
