@@ -499,35 +499,40 @@ AW_DEV void phase_A(const View<T> &V, int t0, int ts)
   float dmax = 0.0f, emax = 0.0f;  // this lane's largest drift and extent
   for (int i = t0; i < n; i += ts) {
     int ri = V.rinfo[i];
-    const double *ca = XA(V, K_CA, i), *o = XA(V, K_O, i), *cb = XA(V, K_CB, i);
-    double *xn = XV(V, K_N, i), *xh = XV(V, K_H, i), *xcp = XV(V, K_CP, i);
-    if (!(ri & RI_FIRST)) {
-      const double *cam = XA(V, K_CA, i - 1), *om = XA(V, K_O, i - 1);
-      for (int k = 0; k < 3; ++k) {
-        xn[k] = lin3_exact(P->an, cam[k], P->bn, ca[k], P->cn, om[k]);
-        xh[k] = lin3_exact(P->ah, cam[k], P->bh, ca[k], P->ch, om[k]);
-      }
-    } else {
-      xn[0] = xn[1] = xn[2] = 0.0;
-      xh[0] = xh[1] = xh[2] = 0.0;
+    bool first = ri & RI_FIRST, last = ri & RI_LAST;
+    // every position the residue needs, read into registers before any
+    // store (the stores into the slab could alias the reads otherwise, and
+    // each read would wait for the previous store); neighbour indices are
+    // clamped, their values unused at a chain's ends
+    int im = first ? i : i - 1, ip = last ? i : i + 1;
+    double ca[3], o[3], cb[3], cam[3], om[3], cap[3], c0[3] = {0, 0, 0}, e0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ca[k] = XA(V, K_CA, i)[k]; o[k] = XA(V, K_O, i)[k]; cb[k] = XA(V, K_CB, i)[k];
+      cam[k] = XA(V, K_CA, im)[k]; om[k] = XA(V, K_O, im)[k]; cap[k] = XA(V, K_CA, ip)[k];
     }
-    if (!(ri & RI_LAST)) {
-      const double *cap = XA(V, K_CA, i + 1);
-      for (int k = 0; k < 3; ++k) xcp[k] = lin3_exact(P->ap, ca[k], P->bp, cap[k], P->cp, o[k]);
-    } else {
-      xcp[0] = xcp[1] = xcp[2] = 0.0;
+    if (V.nl) {
+      c0[0] = V.nlx[3 * i]; c0[1] = V.nlx[3 * i + 1]; c0[2] = V.nlx[3 * i + 2];
+      e0 = V.nlx[3 * n + i];
+    }
+    double xn[3], xh[3], xcp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xn[k] = first ? 0.0 : lin3_exact(P->an, cam[k], P->bn, ca[k], P->cn, om[k]);
+      xh[k] = first ? 0.0 : lin3_exact(P->ah, cam[k], P->bh, ca[k], P->ch, om[k]);
+      xcp[k] = last ? 0.0 : lin3_exact(P->ap, ca[k], P->bp, cap[k], P->cp, o[k]);
+      XV(V, K_N, i)[k] = xn[k]; XV(V, K_H, i)[k] = xh[k]; XV(V, K_CP, i)[k] = xcp[k];
     }
     double e2 = rsq_exact(ca, cb);
     e2 = fmax(e2, rsq_exact(ca, o));
-    if (!(ri & RI_FIRST)) { e2 = fmax(e2, rsq_exact(ca, xn)); e2 = fmax(e2, rsq_exact(ca, xh)); }
-    if (!(ri & RI_LAST)) e2 = fmax(e2, rsq_exact(ca, xcp));
+    if (!first) { e2 = fmax(e2, rsq_exact(ca, xn)); e2 = fmax(e2, rsq_exact(ca, xh)); }
+    if (!last) e2 = fmax(e2, rsq_exact(ca, xcp));
     float ei = (float)(sqrt(e2) * (1.0 + 1e-6) + 1e-4);
     V.ext()[i] = ei;
     if (V.nl) {
       // CA drift plus growth of the extent since the last list build
-      const double c0[3] = {V.nlx[3 * i], V.nlx[3 * i + 1], V.nlx[3 * i + 2]};
       double d2 = rsq_exact(ca, c0);
-      float drift = (float)sqrt(d2) * 1.0001f + 1e-4f + fmaxf(0.0f, ei - (float)V.nlx[3 * n + i]);
+      float drift = (float)sqrt(d2) * 1.0001f + 1e-4f + fmaxf(0.0f, ei - (float)e0);
       dmax = fmaxf(dmax, drift);
       emax = fmaxf(emax, ei);
     }
