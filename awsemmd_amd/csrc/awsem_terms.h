@@ -741,11 +741,12 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
   dihedral(V, i, true, 0.0, cphi, sphi, F);
   dihedral(V, i, false, 0.0, cpsi, spsi, F);
   // wells j = js + k, k < nk: the general set (js = 0) or, for a proline
-  // with Rama_P on, the proline set (js = 6).  Both sets' parameters of well
-  // k are read at wave-uniform addresses (scalar loads, all issued ahead of
-  // the wells) and chosen per lane; at most 6 wells per set.
+  // with Rama_P on, the proline set (js = 6); at most 6 wells per set.  The
+  // two sets run as two loops over a wave-uniform index, each with its
+  // lanes only: every parameter is a scalar load, with no per-lane choice
+  // between the sets.
   const bool pro = (ri & RI_PRO) && P->rama_p_flag;
-  const int nk = pro ? P->n_rama_p_par : P->n_rama_par;
+  const int nr = P->n_rama_par, npr = P->n_rama_p_par;
   const AW_G double *apsr = V.aps + i * 12 + (pro ? 6 : 0);
   // the residue's six weights of its set (in its 12-entry row), read
   // together before the wells
@@ -754,29 +755,30 @@ AW_DEV void rama_term(const View<T> &V, int i, int ri, EAcc &E, LocF &F)
   for (int k = 0; k < 6; ++k) apsk[k] = apsr[k];
   double fphi = 0, fpsi = 0;
   double ev = 0.0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int ssw0 = P->ssweight[k], ssw1 = P->ssweight[6 + k];
-    const double w0 = P->w[k], w1 = P->w[6 + k], pw0 = P->phiw[k], pw1 = P->phiw[6 + k];
-    const double qw0 = P->psiw[k], qw1 = P->psiw[6 + k];
-    const double cf0 = P->cphi0[k], cf1 = P->cphi0[6 + k], sf0 = P->sphi0[k], sf1 = P->sphi0[6 + k];
-    const double cs0 = P->cpsi0[k], cs1 = P->cpsi0[6 + k], ss0 = P->spsi0[k], ss1 = P->spsi0[6 + k];
-    if (k >= nk) break;
-    const bool ssw = pro ? ssw1 != 0 : ssw0 != 0;
-    const double apsv = apsk[k];
-    if (ssw && apsv == 0.0) continue;
-    const double cf = pro ? cf1 : cf0, sf = pro ? sf1 : sf0, cs = pro ? cs1 : cs0, ss = pro ? ss1 : ss0;
+  auto well = [&](int j, double apsv) {
+    const int ssw = P->ssweight[j];
+    if (ssw && apsv == 0.0) return;
+    const double cf = P->cphi0[j], sf = P->sphi0[j], cs = P->cpsi0[j], ss = P->spsi0[j];
     // cos/sin(phi + phi0) by angle addition
     double cp_ = cphi * cf - sphi * sf, sp_ = sphi * cf + cphi * sf;
     double cq_ = cpsi * cs - spsi * ss, sq_ = spsi * cs + cpsi * ss;
     double cph = cp_ - 1.0, cps = cq_ - 1.0;
-    double pc = (pro ? pw1 : pw0) * cph, sc = (pro ? qw1 : qw0) * cps;
-    double Vv = (pro ? w1 : w0) * exp(-cph * pc - cps * sc);
+    double pc = P->phiw[j] * cph, sc = P->psiw[j] * cps;
+    double Vv = P->w[j] * exp(-cph * pc - cps * sc);
     if (ssw) Vv *= apsv;
     double force = 2.0 * Vv;
     fphi += force * pc * sp_;
     fpsi += force * sc * sq_;
     ev += -Vv;
+  };
+  if (!pro) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (k < nr) well(k, apsk[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (k < npr) well(6 + k, apsk[k]);
   }
   E.rama += ev;
   if (fphi != 0.0) dihedral(V, i, true, fphi, dummy, dummy, F);
@@ -1292,13 +1294,20 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
   bool on[4] = {true, i_rep, i_AP, i_P};
   int acc[4] = {i, i, j, j}, don[4] = {j, j + 1, i, i + 2};
   T theta[4], p0[4], p1[4];
+  V3<T> uk[4], wk[4];
   for (int k = 0; k < 4; ++k) {
     theta[k] = 0; p0[k] = 0; p1[k] = 0;
     if (!on[k]) continue;
     // distances and their offsets from the well centres in FP64 (the
     // offsets are small differences of ~3 A quantities)
-    double RNO = sqrt_t<T>(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_N, don[k])));
-    double RHO = sqrt_t<T>(rsq_exact(XA(V, K_O, acc[k]), XV(V, K_H, don[k])));
+    const double *oa = XA(V, K_O, acc[k]), *nd = XV(V, K_N, don[k]), *hd = XV(V, K_H, don[k]);
+    double RNO = sqrt_t<T>(rsq_exact(oa, nd));
+    double RHO = sqrt_t<T>(rsq_exact(oa, hd));
+    // the force directions, from the same reads (the force loop below
+    // follows atomic adds into the slab, after which they would be read
+    // and subtracted again)
+    uk[k] = dvec<T>(oa, nd);
+    wk[k] = dvec<T>(oa, hd);
     T eNO = (T)(RNO - NOz), eHO = (T)(RHO - HOz);
     T dNO = eNO * (T)sNO;
     T dHO = eHO * (T)sHO;
@@ -1331,7 +1340,7 @@ AW_DEV void dssp_term(const View<T> &V, int i, int j, EAcc &E)
     if (!on[k]) continue;
     T q0 = p0[k] * Vv[k], q1 = p1[k] * Vv[k];
     int a = acc[k], d = don[k];
-    V3<T> u = dvec<T>(XA(V, K_O, a), XV(V, K_N, d)), w = dvec<T>(XA(V, K_O, a), XV(V, K_H, d));
+    const V3<T> &u = uk[k], &w = wk[k];
     fadd3(V, K_O, a, -(q0 * u.x + q1 * w.x), -(q0 * u.y + q1 * w.y), -(q0 * u.z + q1 * w.z));
     fadd3(V, K_CA, d - 1, an * q0 * u.x + ah * q1 * w.x, an * q0 * u.y + ah * q1 * w.y, an * q0 * u.z + ah * q1 * w.z);
     fadd3(V, K_CA, d, bn * q0 * u.x + bh * q1 * w.x, bn * q0 * u.y + bh * q1 * w.y, bn * q0 * u.z + bh * q1 * w.z);
