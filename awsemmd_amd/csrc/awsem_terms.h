@@ -1425,6 +1425,14 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
   bool nb = same && (j == i + 1);
   bool shortc = same && (j - i < 5);
   bool gi = ri & RI_GLY, gj = rj & RI_GLY;
+  // the six positions, read together before any test: read inside the
+  // loop, each would wait behind the force adds of the combinations before
+  // it (the compiler cannot move a read above a possibly aliasing add)
+  double pa[3][3], pb[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { pa[k][c] = XA(V, k, i)[c]; pb[k][c] = XA(V, k, j)[c]; }
 #pragma unroll
   for (int ka = 0; ka < 3; ++ka)
 #pragma unroll
@@ -1438,11 +1446,11 @@ AW_DEV void ev_pair(const View<T> &V, int i, int j, int ri, int rj, EAcc &E)
       double rc = shortc ? ev_pick(P->ev_cut_short, ta0, ta1, tb0, tb1, gi, gj)
                          : ev_pick(P->ev_cut_long, ta0, ta1, tb0, tb1, gi, gj);
       // FP32 estimate first: beyond rc^2 (1 + margin) the exact test fails too
-      if (rsq_f(XA(V, ka, i), XA(V, kb, j)) > (float)(rc * rc) * (1.0f + AW_RSQ_MARGIN)) continue;
-      double rsq = rsq_exact(XA(V, ka, i), XA(V, kb, j));
+      if (rsq_f(pa[ka], pb[kb]) > (float)(rc * rc) * (1.0f + AW_RSQ_MARGIN)) continue;
+      double rsq = rsq_exact(pa[ka], pb[kb]);
       double r = sqrt(rsq);
       if (!(r < rc)) continue;
-      V3<T> d = dvec<T>(XA(V, ka, i), XA(V, kb, j));
+      V3<T> d = dvec<T>(pa[ka], pb[kb]);
       T rr = (T)r, dr = rr - (T)rc;
       T pw = (T)1;
       for (int q = 0; q < p - 1; ++q) pw *= dr;
