@@ -1140,9 +1140,9 @@ AW_DEV PapP pap_p(ParamPtr P)
   return q;
 }
 template <class T>
-AW_DEV void pap_nu(const View<T> &V, const PapP &q, int i, int j, T &nu, T &prd)
+AW_DEV void pap_nu(const PapP &q, const double *a, const double *b, T &nu, T &prd)
 {
-  double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
+  double drsq = rsq_exact(a, b);
   if (drsq > q.drmax2) { nu = 0; prd = 0; }
   else if (drsq < q.drmin2) { nu = 1; prd = 0; }
   else {
@@ -1154,14 +1154,14 @@ AW_DEV void pap_nu(const View<T> &V, const PapP &q, int i, int j, T &nu, T &prd)
 // the nu < 1e-12 early exit of compute_P_AP_potential: an FP32 nu decides
 // unless it lies within a relative 1e-5 of the threshold, where FP64 decides
 template <class T>
-AW_DEV bool pap_keep(const View<T> &V, const PapP &q, int i, int j, T nu)
+AW_DEV bool pap_keep(const PapP &q, const double *a, const double *b, T nu)
 {
   if (sizeof(T) == 8) return !(nu < (T)AW_PAP_DELTA);
   double d = (double)nu - AW_PAP_DELTA;
   if (d > 1e-5 * AW_PAP_DELTA) return true;
   if (d < -1e-5 * AW_PAP_DELTA) return false;
   double nud, prdd;
-  double drsq = rsq_exact(XA(V, K_CA, i), XA(V, K_CA, j));
+  double drsq = rsq_exact(a, b);
   double dr = sqrt(drsq);
   tanh_switch<double>(q.pref * (q.cut - dr), q.pref, dr, nud, prdd);
   return !(nud < AW_PAP_DELTA);
@@ -1368,9 +1368,21 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
   const int ics = V.ch_start[ic], icl = V.ch_len[ic], jcs = V.ch_start[jc], jcl = V.ch_len[jc];
   const int nrk = nr > 0 ? nr - 1 : 0;
   const double apsi = V.aps[i * 12 + nrk], apsj = V.aps[j * 12 + nrk];
+  // the CA positions the pair can use, read together before any branch
+  // (indices clamped; i + d, j - d, j + d are used only where they exist):
+  // read where used, they would wait behind the force adds before them
+  auto clampi = [n](int k) { return k < 0 ? 0 : (k >= n ? n - 1 : k); };
+  double ci[3], cj[3], cid[3], cjm[3], cjp[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ci[c] = XA(V, K_CA, i)[c]; cj[c] = XA(V, K_CA, j)[c];
+    cid[c] = XA(V, K_CA, clampi(i + d))[c];
+    cjm[c] = XA(V, K_CA, clampi(j - d))[c];
+    cjp[c] = XA(V, K_CA, clampi(j + d))[c];
+  }
   T nu_ij, prd_ij;
-  pap_nu(V, q, i, j, nu_ij, prd_ij);
-  if (!pap_keep(V, q, i, j, nu_ij)) return;
+  pap_nu(q, ci, cj, nu_ij, prd_ij);
+  if (!pap_keep(q, ci, cj, nu_ij)) return;
   int i_ch_end = ics + icl, j_ch_end = jcs + jcl;
   int jhi = (i + mx + 2 * d) < (i_ch_end - 1) ? (i + mx + 2 * d) : (i_ch_end - 1);
   bool med = ic == jc && i < i_ch_end - (mn + 2 * d) && j >= i + (mn + 2 * d) && j <= jhi;
@@ -1383,10 +1395,10 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
     double K = beta ? (med ? kp0 : 0.0) + (lng ? kp1 * kbeta : 0.0)
                     : (med ? kp0 : 0.0) + (lng ? kp1 : 0.0);
     T nu2, prd2;
-    pap_nu(V, q, i + d, j - d, nu2, prd2);
+    pap_nu(q, cid, cjm, nu2, prd2);
     double kk = kglob * K;
     E.pap += -kk * (double)nu_ij * (double)nu2;
-    V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j - d));
+    V3<T> d0 = dvec<T>(ci, cj), d1 = dvec<T>(cid, cjm);
     T f0 = (T)kk * prd_ij * nu2, f1 = (T)kk * nu_ij * prd2;
     fadd(V, K_CA, i, -f0, d0); fadd(V, K_CA, j, f0, d0);
     fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j - d, f1, d1);
@@ -1394,10 +1406,10 @@ AW_DEV void pap_term(const View<T> &V, int i, int j, EAcc &E)
   if (par) {
     double K = beta ? kp2 * kbeta : kp2;
     T nu2, prd2;
-    pap_nu(V, q, i + d, j + d, nu2, prd2);
+    pap_nu(q, cid, cjp, nu2, prd2);
     double kk = kglob * K;
     E.pap += -kk * (double)nu_ij * (double)nu2;
-    V3<T> d0 = dvec<T>(XA(V, K_CA, i), XA(V, K_CA, j)), d1 = dvec<T>(XA(V, K_CA, i + d), XA(V, K_CA, j + d));
+    V3<T> d0 = dvec<T>(ci, cj), d1 = dvec<T>(cid, cjp);
     T f0 = (T)kk * prd_ij * nu2, f1 = (T)kk * nu_ij * prd2;
     fadd(V, K_CA, i, -f0, d0); fadd(V, K_CA, j, f0, d0);
     fadd(V, K_CA, i + d, -f1, d1); fadd(V, K_CA, j + d, f1, d1);
