@@ -1725,6 +1725,15 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
     ri = ri0; rj = rj0;
     bool same = rchain(ri) == rchain(rj);
     int dres = j - i;
+    // DSSP in both orientations (fb.cpp:8064-8094); ahead of the density
+    // block, whose atomic adds would otherwise order these reads behind them
+    float ld = (float)dcut + eij;
+    if (dssp_on && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
+      if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST))
+        dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), dcut, dcut2);
+      if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST))
+        dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), dcut, dcut2);
+    }
     // DL1 densities (fb.cpp:7729-7786); the exact FP64 distance only for
     // pairs near a well
     if (!same || dres > 1) {
@@ -1748,14 +1757,6 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
           aadd(&V.rho_h()[j], (double)th);
         }
       }
-    }
-    // DSSP in both orientations (fb.cpp:8064-8094)
-    float ld = (float)dcut + eij;
-    if (dssp_on && (!same || dres > 2) && rca <= ld * ld * 1.0001f) {
-      if (!(rj & RI_PRO) && !(ri & RI_LAST) && !(rj & RI_FIRST))
-        dij = dist_within(XA(V, K_O, i), XV(V, K_N, j), dcut, dcut2);
-      if (!(ri & RI_PRO) && !(rj & RI_LAST) && !(ri & RI_FIRST))
-        dji = dist_within(XA(V, K_O, j), XV(V, K_N, i), dcut, dcut2);
     }
     // P_AP (fb.cpp:8096-8103)
     pap = pap_on && rsq_lt_est(rca, ci, cj, pcut2);
