@@ -2010,6 +2010,7 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
   // so that their latency overlaps it
   const double g00 = P->water_gamma[0][it][jt][0], g01 = P->water_gamma[0][it][jt][1];
   const double g10 = P->water_gamma[1][it][jt][0], g11 = P->water_gamma[1][it][jt][1];
+  const double swi = V.sig_w()[i], swj = V.sig_w()[j];  // with the positions
   double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
   double r = sqrt_t<T>(rsq);
   double acc_i = 0.0, acc_j = 0.0;
@@ -2023,8 +2024,8 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
       T th, prd;
       well_theta<T>(wk, wp.r0, wp.r1, r, th, prd);
       double tg = (g1 - g0) * (double)th;
-      acc_i += tg * V.sig_w()[j];
-      acc_j += tg * V.sig_w()[i];
+      acc_i += tg * swj;
+      acc_j += tg * swi;
     }
   }
   if (acc_i != 0.0) aadd(&V.xi_w()[i], acc_i);
@@ -2072,6 +2073,14 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
   const double g00 = P->water_gamma[0][it][jt][0], g01 = P->water_gamma[0][it][jt][1];
   const double g10 = P->water_gamma[1][it][jt][0], g11 = P->water_gamma[1][it][jt][1];
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
+  // the per-residue scalars the pair can use, read with the positions and
+  // before any branch (one LDS round trip; hxi2 indices clamped, their
+  // values used only where i - hd, j - hd >= 0)
+  const double swi = V.sig_w()[i], swj = V.sig_w()[j];
+  const double xii = V.xi_w()[i] * V.sigp_w()[i], xij = V.xi_w()[j] * V.sigp_w()[j];
+  const double bfij = V.bf()[i] + V.bf()[j];
+  const double h1 = V.hxi1()[i] + V.hxi1()[j];
+  const double h2i = V.hxi2()[i - hd >= 0 ? i - hd : 0], h2j = V.hxi2()[j - hd >= 0 ? j - hd : 0];
   double rsq = rsq_exact(si, sj);
   double r = sqrt_t<T>(rsq);
   T force = 0;
@@ -2088,7 +2097,7 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
         double g0 = w == 0 ? g00 : (w == 1 ? g10 : P->water_gamma[w][it][jt][0]);
         double g1 = w == 0 ? g01 : (w == 1 ? g11 : P->water_gamma[w][it][jt][1]), sg;
         if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
-        else sg = g0 + (g1 - g0) * V.sig_w()[i] * V.sig_w()[j];
+        else sg = g0 + (g1 - g0) * swi * swj;
         T th = th0, prd = prd0;
         if (w != 0) well_theta<T>(wk, wp.r0, wp.r1, r, th, prd);
         ew += -sg * (double)th;
@@ -2098,19 +2107,18 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
     E.water += ew;
     if (in0) {
       double xi = 0.0;
-      // xi times dsigma/drho (fb.cpp:7939-7950), formed here where it is
-      // read: the same product, without a pass and a barrier of its own
-      double xii = V.xi_w()[i] * V.sigp_w()[i], xij = V.xi_w()[j] * V.sigp_w()[j];
+      // xii, xij: xi times dsigma/drho (fb.cpp:7939-7950), formed where it
+      // is read: the same product, without a pass and a barrier of its own
       if (fabs(xii) > AW_DELTA_WATER_XI) xi += xii;
       if (fabs(xij) > AW_DELTA_WATER_XI) xi += xij;
       if (xi != 0.0) force += prd0 * (T)xi;
     }
   }
-  if (burial && in0) force += (T)(V.bf()[i] + V.bf()[j]) * prd0;
+  if (burial && in0) force += (T)bfij * prd0;
   if (helix && rsq > hmin2 && rsq < hmax2) {
-    double factor = V.hxi1()[i] + V.hxi1()[j];
-    if (i - hd >= 0) factor += V.hxi2()[i - hd];
-    if (j - hd >= 0) factor += V.hxi2()[j - hd];
+    double factor = h1;
+    if (i - hd >= 0) factor += h2i;
+    if (j - hd >= 0) factor += h2j;
     if (fabs(factor) > AW_DELTA_HELIX_XI) {
       T th, prd;
       well_theta<T>(hk, hr0, hr1, r, th, prd);
