@@ -1913,24 +1913,28 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts, bool ef = true)
   ParamPtr P = V.P;
   for (int i = t0; i < V.n; i += ts) {
     int it = rtype(V.rinfo[i]);
+    // the residue's reads before its stores (which could alias them) and
+    // before any branch: densities and its burial gammas (per-lane gathers)
+    const double rho_w = V.rho_w()[i], rho_h = V.rho_h()[i];
+    const double bg[3] = {P->burial_gamma[it][0], P->burial_gamma[it][1], P->burial_gamma[it][2]};
     if (P->water_flag) {
       double s, onep;
-      sig_switch<T>(P->water_kappa_sigma * (V.rho_w()[i] - P->treshold), s, onep);
+      sig_switch<T>(P->water_kappa_sigma * (rho_w - P->treshold), s, onep);
       V.sig_w()[i] = s;
       V.sigp_w()[i] = -P->water_kappa_sigma * s * onep;
     }
     if (P->helix_flag) {
       double s, onep;
-      sig_switch<T>(P->helix_kappa_sigma * (V.rho_h()[i] - P->helix_treshold), s, onep);
+      sig_switch<T>(P->helix_kappa_sigma * (rho_h - P->helix_treshold), s, onep);
       V.sig_h()[i] = s;
       V.sigp_h()[i] = -P->helix_kappa_sigma * s * onep;
     }
     if (P->burial_flag) {
-      double rho = V.rho_w()[i], kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
+      double rho = rho_w, kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
       for (int b = 0; b < 3; ++b) {
         double t0_ = sw_tanh<T>(kb * (rho - P->burial_ro_min[b])), t1_ = sw_tanh<T>(kb * (P->burial_ro_max[b] - rho));
-        eb += -0.5 * P->k_burial * P->burial_gamma[it][b] * (t0_ + t1_);
-        bfv += P->burial_gamma[it][b] * (t1_ * t1_ - t0_ * t0_);
+        eb += -0.5 * P->k_burial * bg[b] * (t0_ + t1_);
+        bfv += bg[b] * (t1_ * t1_ - t0_ * t0_);
       }
       V.bf()[i] = bfv * 0.5 * P->k_burial * kb;
       if (ON(P, AW_ET_BURIAL)) E.burial += eb;
