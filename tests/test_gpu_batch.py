@@ -111,3 +111,27 @@ def test_energies_after_run_are_the_final_states(engine, prec):
     de = np.abs(e - er) / np.maximum(1.0, np.abs(er))
     print(f"{engine} prec {prec}: max rel dE {de.max():.2e}")
     assert np.all(np.isfinite(e)) and de.max() <= tol
+
+
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+def test_split_launches_continue_one_run(engine):
+    """Dynamics split over two runs at another step continue as one run: a
+    launch starts from the forces the previous one left and re-reads the
+    Verlet list's count and validity, so the trajectory is the same (FP64;
+    only the order of atomic force adds may differ, at rounding level)."""
+    A = _gpu()
+    cases = [make_case([L], 90 + k) for k, L in enumerate([60, 85])]
+    x0 = A.pack_states([s.coords() for s, _ in cases])
+    one, two = _batch(A, cases, A.PREC_FP64), _batch(A, cases, A.PREC_FP64)
+    for bb in (one, two):
+        bb.set_engine(engine)
+        bb.set_state(x0, np.zeros_like(x0))
+    one.run(24, seed=5, step0=0)
+    two.run(7, seed=5, step0=0)
+    two.run(17, seed=5, step0=7)
+    xa, va, _, ea = one.get_state()
+    xb, vb, _, eb = two.get_state()
+    dx, dv = np.abs(xa - xb).max(), np.abs(va - vb).max()
+    de = (np.abs(ea - eb) / np.maximum(1.0, np.abs(ea))).max()
+    print(f"{engine}: one run vs 7 + 17 steps: |dx| {dx:.2e} A, |dv| {dv:.2e}, rel dE {de:.2e}")
+    assert np.all(np.isfinite(xa)) and dx <= 1e-9 and dv <= 1e-9 and de <= 1e-9
