@@ -34,7 +34,6 @@ typedef const FFParams *ParamPtr;
 #define AW_G
 inline void aw_hold(int &) {}
 inline double aw_sval(double v) { return v; }
-template <bool L> inline double ld_as(const double *p) { return *p; }
 template <bool L> inline void aadd_as(double *p, double v) { *p += v; }
 template <bool L> struct aw_as { typedef double type; };
 template <bool L> inline double *aw_cast(double *p) { return p; }
@@ -100,16 +99,11 @@ __device__ __forceinline__ double aw_sval(double v)
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 // typed accesses for code compiled apart from the kernel (a noinline
-// function), where a plain pointer would give flat instructions: ld_as /
-// aadd_as read / add into LDS (L) or global memory, ld_item loads a 16-byte
-// multiple struct from global memory in 16-byte vectors
+// function), where a plain pointer would give flat instructions: aadd_as
+// adds into LDS (L) or global memory, ld_item loads a 16-byte multiple
+// struct from global memory in 16-byte vectors
 typedef __attribute__((address_space(3))) double aw_lds_double;
 typedef __attribute__((address_space(1))) double aw_global_double;
-template <bool L>
-__device__ __forceinline__ double ld_as(const double *p)
-{
-  return L ? *(const aw_lds_double *)p : *(const aw_global_double *)p;
-}
 template <bool L>
 __device__ __forceinline__ void aadd_as(double *p, double v)
 {
@@ -855,8 +849,8 @@ AW_DEV void fm_gauss(const T4t &f, T w, const T *r, T &S0, T &S1, T &S2, T &S3, 
 }
 
 // force sinks of fm_run.  fm_run is compiled as a function of its own, so
-// the sinks state where their arrays live (aadd_as) for the compiler to emit
-// LDS or global atomics.
+// the sinks state where their arrays live (a base typed by aw_cast, or
+// aadd_as) for the compiler to emit LDS or global atomics.
 // FmSinkForce: adds into a replica's force array f[3][n][3]
 template <bool FLDS>
 struct FmSinkForce {
