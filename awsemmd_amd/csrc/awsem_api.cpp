@@ -694,10 +694,13 @@ int awsem_batch_set_skin(awsem_batch *b, double skin)
 {
   if (!b || !b->dev || !(skin > 0.0) || skin > 20.0) return AWSEM_E_ARG;
   DevBatch &D = *b->dev;
+  // a list built with a larger skin stays a superset of every selectable
+  // pair until some residue drifts by the new (smaller) skin/2, which is
+  // when the drift test rebuilds it; only a larger skin needs a rebuild now
+  if (skin > D.P.nl_skin) D.nl_dirty = true;
   D.P.nl_skin = skin;
   if (hipMemcpy(D.d_params, &D.P, sizeof(FFParams), hipMemcpyHostToDevice) != hipSuccess)
     return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "parameter upload"));
-  D.nl_dirty = true;  // lists were built with the old skin
   return AWSEM_OK;
 }
 

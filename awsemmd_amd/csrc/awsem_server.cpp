@@ -22,6 +22,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -93,6 +94,12 @@ struct awsem_server {
   std::thread worker;
   bool worker_on = false, stop_req = false;
   std::string err;
+  // set when a scheduling round fails: the device state of every job in
+  // flight is unknown, so the server refuses further work with this code
+  int fail_code = 0;
+  // test hook (AWSEM_SERVER_FAIL_ROUND=k): round k (1-based) fails before
+  // it launches anything, as a failed launch would
+  int fail_round = 0;
   // statistics
   double rounds = 0, jobs_done = 0, peak_active = 0, replica_steps = 0, device_ms = 0;
 
@@ -117,6 +124,17 @@ struct awsem_server {
   AwStatus init(const FFParams &p, const awsem_server_opts &opts);
   AwStatus round(int *n_active, int *n_queued);
   void run_worker();
+  // marks every job in flight or queued as failed with s (mu held)
+  void fail_all(const AwStatus &s)
+  {
+    fail_code = s.code;
+    err = s.msg;
+    for (Job *j : active) { j->state = s.code; j->err = s.msg; j->t_done = Clock::now(); j->dev.reset(); }
+    active.clear();
+    used = 0;
+    for (Job *j : queued) { j->state = s.code; j->err = s.msg; j->t_done = Clock::now(); }
+    queued.clear();
+  }
 };
 
 #define SV_HIPCHK(expr)                                                              \
@@ -160,6 +178,8 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
   std::vector<Job *> admit;
   {
     std::lock_guard<std::mutex> lk(mu);
+    if (fail_round > 0 && rounds + 1 >= fail_round)
+      return AwStatus::err(AWSEM_E_DEVICE, "injected failure of scheduling round " + std::to_string(fail_round));
     while (!queued.empty() && (int)(active.size() + admit.size()) < o.max_jobs) {
       Job *j = queued.front();
       if (used + j->plan.bytes > budget) break;  // arrival order: wait for memory
@@ -330,13 +350,11 @@ void awsem_server::run_worker()
     AwStatus s = round(nullptr, nullptr);
     if (!s.ok()) {
       std::lock_guard<std::mutex> lk(mu);
-      err = s.msg;
-      // fail everything in flight: the device state is unknown
-      for (Job *j : active) { j->state = s.code; j->err = s.msg; j->t_done = Clock::now(); j->dev.reset(); }
-      active.clear();
-      used = 0;
-      for (Job *j : queued) { j->state = s.code; j->err = s.msg; j->t_done = Clock::now(); }
-      queued.clear();
+      fail_all(s);
+      // the thread ends here: the server no longer counts as started, and
+      // every later call reports the failure (worker is joined by stop,
+      // start or the destructor)
+      worker_on = false;
       cv.notify_all();
       return;
     }
@@ -364,6 +382,7 @@ int awsem_server_create(const awsem_ctx *forcefield, const awsem_server_opts *op
       (opts->precision != AWSEM_PREC_FP32 && opts->precision != AWSEM_PREC_FP64))
     return AWSEM_E_ARG;
   std::unique_ptr<awsem_server> s(new awsem_server());
+  if (const char *e = std::getenv("AWSEM_SERVER_FAIL_ROUND")) s->fail_round = std::atoi(e);
   AwStatus st = s->init(S->P, *opts);
   *out = s.release();
   if (!st.ok()) { (*out)->err = st.msg; return st.code; }
@@ -398,6 +417,7 @@ int awsem_server_submit(awsem_server *s, const awsem_ctx *system, const double *
   j->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fail_code) return s->fail_code;  // s->err holds the failure
     j->id = s->next_id++;
     *job_id = j->id;
     s->queued.push_back(j.get());
@@ -412,17 +432,31 @@ int awsem_server_pump(awsem_server *s, int *n_active, int *n_queued)
   if (!s) return AWSEM_E_ARG;
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fail_code) return s->fail_code;
     if (s->worker_on) return AWSEM_E_STATE;
   }
   AwStatus st = s->round(n_active, n_queued);
-  return st.ok() ? AWSEM_OK : sfail(s, st);
+  if (!st.ok()) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fail_all(st);
+    s->cv.notify_all();
+    return st.code;
+  }
+  return AWSEM_OK;
 }
 
 int awsem_server_start(awsem_server *s)
 {
   if (!s) return AWSEM_E_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->fail_code) return s->fail_code;
   if (s->worker_on) return AWSEM_E_STATE;
+  if (s->worker.joinable()) {
+    // a worker that ended on its own (a failed round) is joined first
+    lk.unlock();
+    s->worker.join();
+    lk.lock();
+  }
   s->stop_req = false;
   s->worker_on = true;
   s->worker = std::thread([s] { s->run_worker(); });
@@ -434,10 +468,10 @@ int awsem_server_stop(awsem_server *s)
   if (!s) return AWSEM_E_ARG;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (!s->worker_on) return AWSEM_OK;
     s->stop_req = true;
   }
   s->cv.notify_all();
+  // also joins a worker that ended on its own after a failed round
   if (s->worker.joinable()) s->worker.join();
   std::lock_guard<std::mutex> lk(s->mu);
   s->worker_on = false;
@@ -462,13 +496,16 @@ int awsem_server_wait(awsem_server *s, uint64_t job_id, double timeout_s)
   auto it = s->jobs.find(job_id);
   if (it == s->jobs.end()) return AWSEM_E_ARG;
   Job *j = it->second.get();
-  auto done = [&] { return j->state == JOB_DONE || j->state < 0; };
+  // a failed server fails every job it held; a job still queued or running
+  // there could otherwise never finish
+  auto done = [&] { return j->state == JOB_DONE || j->state < 0 || s->fail_code != 0; };
   if (timeout_s > 0) {
     if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return AWSEM_E_STATE;
   } else {
     s->cv.wait(lk, done);
   }
-  return j->state < 0 ? j->state : AWSEM_OK;
+  if (j->state < 0) return j->state;
+  return j->state == JOB_DONE ? AWSEM_OK : s->fail_code;
 }
 
 int awsem_server_result(awsem_server *s, uint64_t job_id, double *xres, double *energy, double *times_ms)
