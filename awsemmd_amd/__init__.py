@@ -43,7 +43,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_server_create", "awsem_server_submit", "awsem_server_pump", "awsem_server_start",
            "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
            "awsem_server_release", "awsem_server_stats", "awsem_server_last_error", "awsem_server_destroy",
-           "awsem_batch_destroy"]
+           "awsem_batch_destroy", "awsem_compute_res_noalloc"]
 
 
 class AwsemError(RuntimeError):
@@ -79,6 +79,7 @@ def lib():
         L.awsem_set_topology.argtypes = [vp, C.c_int, ip, ip, ip]
         L.awsem_compute.argtypes = [vp, dp, dp, dp]
         L.awsem_compute_res.argtypes = [vp, dp, dp, dp]
+        L.awsem_compute_res_noalloc.argtypes = [vp, dp, dp, dp]
         L.awsem_set_vexcluded.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
         L.awsem_set_bonds.argtypes = [vp, C.c_int, C.c_double, dp]
         L.awsem_set_term_mask.argtypes = [vp, C.c_uint32]
@@ -230,6 +231,19 @@ class FixBackbone:
         f = np.zeros_like(x)
         e = np.zeros(N_ET)
         self._check(lib().awsem_compute_res(self.h, _dp(x), _dp(f), _dp(e)))
+        self.energy = e
+        return f, e
+
+    def compute_res_noalloc(self, xres: np.ndarray, f: Optional[np.ndarray] = None):
+        """compute_res without any device allocation in the call (per-call
+        buffers in a fixed 4 MiB device array; the context's device image
+        must exist, e.g. after one compute_res).  Forces are added into f."""
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        if f is None:
+            f = np.zeros_like(x)
+        assert f.dtype == np.float64 and f.flags.c_contiguous and f.shape == x.shape
+        e = np.zeros(N_ET)
+        self._check(lib().awsem_compute_res_noalloc(self.h, _dp(x), _dp(f), _dp(e)))
         self.energy = e
         return f, e
 

@@ -464,6 +464,89 @@ int awsem_compute_res(awsem_ctx *c, const double *xres, double *fres, double *en
   return s.ok() ? AWSEM_OK : fail(c, s);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the residue-layout evaluation over a caller-given device region (used by
+// awsem_compute_res_noalloc, awsem_noalloc.hip): the region holds the
+// coordinates, the forces, the energies and the phase engine's working slab,
+// laid out as [x 9n][f 9n][energy 32][slab, 256-byte aligned]
+
+static size_t noalloc_slab_off(int n) { return ((18 * (size_t)n + 32 + 31) / 32) * 32; }
+
+size_t aw_noalloc_doubles(int n) { return noalloc_slab_off(n) + (size_t)aw_slab_doubles(n); }
+
+int aw_noalloc_check(awsem_ctx *c, const double *xres, const double *fres, size_t cap_doubles)
+{
+  if (!c || c->sys.n < 0 || !xres || !fres) return AWSEM_E_ARG;
+  int n = c->sys.n;
+  if (aw_noalloc_doubles(n) > cap_doubles) {
+    int nmax = 0;
+    while (aw_noalloc_doubles(nmax + 1) <= cap_doubles) ++nmax;
+    char msg[256];
+    std::snprintf(msg, sizeof msg,
+                  "awsem_compute_res_noalloc: %d residues need %zu bytes of scratch; its fixed scratch array holds "
+                  "%zu bytes (at most %d residues)", n, aw_noalloc_doubles(n) * sizeof(double),
+                  cap_doubles * sizeof(double), nmax);
+    return fail(c, AwStatus::err(AWSEM_E_NOMEM, msg));
+  }
+  if (!c->dev || c->dirty)
+    return fail(c, AwStatus::err(AWSEM_E_STATE, "awsem_compute_res_noalloc: the context's device image is not built "
+                                                "(or its parameters changed): call awsem_compute_res, awsem_compute "
+                                                "or awsem_run once first"));
+  if (!finite_all(xres, (size_t)n * 9)) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  return AWSEM_OK;
+}
+
+int aw_compute_res_scratch(awsem_ctx *c, const double *xres, double *fres, double *energy, double *dscr)
+{
+  DevBatch &D = *c->dev;
+  int n = c->sys.n;
+  size_t n9 = (size_t)n * 9;
+  // the context's descriptor with its per-call buffers moved into the region
+  ReplicaDesc R = D.reps[0];
+  R.x = dscr;
+  R.f = dscr + n9;
+  R.energy = dscr + 2 * n9;
+  R.scratch = dscr + noalloc_slab_off(n);
+  R.nsteps = 0; R.ramp0 = 0; R.run_steps = 0; R.rng_id = 0; R.flags = 0;
+  AwRun run;
+  std::memset(&run, 0, sizeof run);
+  run.mode = 0;
+  auto dev_fail = [&](const char *what) {
+    (void)hipStreamSynchronize(D.stream);
+    // the context's own descriptor back in place for its other entry points
+    (void)hipMemcpy(D.d_rep, D.reps.data(), sizeof(ReplicaDesc), hipMemcpyHostToDevice);
+    return fail(c, AwStatus::err(AWSEM_E_DEVICE, what));
+  };
+  if (hipMemcpyAsync(R.x, xres, n9 * sizeof(double), hipMemcpyHostToDevice, D.stream) != hipSuccess ||
+      hipMemcpyAsync(D.d_rep, &R, sizeof R, hipMemcpyHostToDevice, D.stream) != hipSuccess)
+    return dev_fail("awsem_compute_res_noalloc: upload");
+  if (aw_launch_phases(D.ptrs(), run, D.stream)) return dev_fail("awsem_compute_res_noalloc: kernel launch failed");
+  // every later call of the context's other entry points re-reads its own
+  // descriptor (awsem_list reads it without rewriting it first)
+  if (hipMemcpyAsync(D.d_rep, D.reps.data(), sizeof(ReplicaDesc), hipMemcpyHostToDevice, D.stream) != hipSuccess)
+    return dev_fail("awsem_compute_res_noalloc: descriptor restore");
+  double e[AW_N_ET];
+  if (hipMemcpyAsync(e, R.energy, sizeof e, hipMemcpyDeviceToHost, D.stream) != hipSuccess)
+    return dev_fail("awsem_compute_res_noalloc: energy copy");
+  // forces added into fres through a fixed host buffer (no allocation)
+  double buf[2048];
+  for (size_t k0 = 0; k0 < n9; k0 += 2048) {
+    size_t m = std::min<size_t>(2048, n9 - k0);
+    if (hipMemcpyAsync(buf, R.f + k0, m * sizeof(double), hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
+        hipStreamSynchronize(D.stream) != hipSuccess)
+      return dev_fail("awsem_compute_res_noalloc: force copy");
+    for (size_t k = 0; k < m; ++k) fres[k0 + k] += buf[k];
+  }
+  if (hipStreamSynchronize(D.stream) != hipSuccess) return dev_fail("awsem_compute_res_noalloc: sync");
+  if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
+  if (!std::isfinite(e[0])) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy"));
+  return AWSEM_OK;
+}
+
+extern "C" {
+
 int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy)
 {
   if (!c || c->sys.n < 0 || !x || !f) return AWSEM_E_ARG;

@@ -33,3 +33,13 @@ int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
 // lists and queues in global memory); max_steps = largest replica nsteps
 int aw_launch_wide(const AwBatchPtrs &p, const AwRun &run, int max_steps, hipStream_t st);
 int aw_launch_list(const AwBatchPtrs &p, int r, int kind, int arg, long long *out, int cap, int *count, hipStream_t st);
+
+// the residue-layout evaluation over a caller-given device region
+// (awsem_compute_res_noalloc): doubles the region needs for n residues; the
+// checks of a call (arguments, size against cap_doubles, device image built,
+// finite coordinates; 0 or an AWSEM_E_* code with the context's message
+// set); the evaluation itself, fres += forces
+struct awsem_ctx;
+size_t aw_noalloc_doubles(int n);
+int aw_noalloc_check(awsem_ctx *c, const double *xres, const double *fres, size_t cap_doubles);
+int aw_compute_res_scratch(awsem_ctx *c, const double *xres, double *fres, double *energy, double *dscr);

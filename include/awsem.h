@@ -108,6 +108,19 @@ int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy);
 /* Residue-layout variant of awsem_compute (no topology needed). */
 int awsem_compute_res(awsem_ctx *c, const double *xres, double *fres, double *energy);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): awsem_compute_res with no allocation in the call.  The
+ * per-call device buffers (coordinate staging, forces, energies and the
+ * working slab) live in one fixed 4 MiB device array per GPU, held until the
+ * process ends, so a system may have at most 9196 residues (larger ones get
+ * AWSEM_E_NOMEM).  The context's constant device image must already exist:
+ * call awsem_compute_res, awsem_compute or awsem_run once first (otherwise,
+ * or after its parameters changed, AWSEM_E_STATE).  fres += forces as in
+ * awsem_compute_res.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other threads or contexts) race. */
+int awsem_compute_res_noalloc(awsem_ctx *c, const double *xres, double *fres, double *energy);
+
 /* pair_style vexcluded p rc_short rc_long + pair_coeff (pev.cpp:175-265):
  * tables are [6][6] indexed by atom type (1 CA, 2 N, 3 O, 4 CB, 5 H-beta).
  * enable=0 removes the pair style; NULL tables keep the deck defaults

@@ -170,3 +170,20 @@ def test_server_and_batch_symbols_are_bound():
     for name in ("awsem_server_create", "awsem_server_submit", "awsem_batch_autotune", "awsem_batch_set_engine",
                  "awsem_batch_use_graph", "awsem_batch_set_skin"):
         assert getattr(L, name).argtypes is not None
+
+
+def test_noalloc_rejects_systems_beyond_its_array():
+    """awsem_compute_res_noalloc refuses a system whose buffers exceed its
+    fixed 4 MiB device array, before any device call, with a message naming
+    the limit (9196 residues)."""
+    A = _A()
+    from awsemmd_amd import workload as W
+    import tempfile
+    s = W.make_system([9300], seed=1, n_struct=1)
+    d = tempfile.mkdtemp(prefix="awsem_big_")
+    p = W.write_system(s, d, fm=False)
+    fix = A.FixBackbone(p["coeff"], p["seq"], d, A.PREC_FP64)
+    x = s.coords()
+    with pytest.raises(A.AwsemError) as e:
+        fix.compute_res_noalloc(x, np.zeros_like(x))
+    assert e.value.code == -8 and "9196 residues" in str(e.value)
