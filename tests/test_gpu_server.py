@@ -122,3 +122,42 @@ def test_server_admission_by_memory_budget():
     with pytest.raises(A.AwsemError) as e:
         tiny.submit(fixes[0], cases[0][0].coords(), None, 10)
     assert e.value.code == -8
+
+
+def test_server_failed_round_is_reported(monkeypatch):
+    """A scheduling round that fails (injected from the host before it
+    launches anything, AWSEM_SERVER_FAIL_ROUND) fails the jobs it held; the
+    worker stops, and submit, pump, start and wait (with and without a time
+    limit) return the failure instead of queueing or blocking."""
+    A = _gpu()
+    s, p = make_case([30], 35)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32)
+    monkeypatch.setenv("AWSEM_SERVER_FAIL_ROUND", "1")
+    srv = A.Server(fix, A.PREC_FP32, max_jobs=4, steps_per_round=5)
+    pumped = A.Server(fix, A.PREC_FP32, max_jobs=4, steps_per_round=5)
+    monkeypatch.delenv("AWSEM_SERVER_FAIL_ROUND")
+    jid = srv.submit(fix, s.coords(), None, 10)
+    srv.start()
+    with pytest.raises(A.AwsemError) as e:
+        srv.wait(jid, 30.0)
+    assert e.value.code == -7 and "injected" in str(e.value)
+    assert srv.poll(jid) == -7
+    with pytest.raises(A.AwsemError) as e:
+        srv.wait(jid, 0.0)  # no time limit: returns at once
+    assert e.value.code == -7
+    for call in (lambda: srv.submit(fix, s.coords(), None, 10), srv.pump, srv.start):
+        with pytest.raises(A.AwsemError) as e:
+            call()
+        assert e.value.code == -7
+    srv.stop()
+    srv.close()
+    # the same on the calling thread: pump reports the failure, then refuses
+    j2 = pumped.submit(fix, s.coords(), None, 10)
+    with pytest.raises(A.AwsemError) as e:
+        pumped.pump()
+    assert e.value.code == -7 and pumped.poll(j2) == -7
+    with pytest.raises(A.AwsemError):
+        pumped.submit(fix, s.coords(), None, 10)
+    with pytest.raises(A.AwsemError):
+        pumped.wait(j2, 0.0)
+    pumped.close()
