@@ -116,6 +116,8 @@ struct or_ctx {
   double *fca, *fcb, *fo;
   double *energy;
   unsigned mask;
+  /* branch counters (or_branch_counts): which selection branches ran */
+  long long branch[OR_N_BRANCH];
 };
 
 /* ------------------------------------------------------------------ */
@@ -1333,7 +1335,11 @@ static void dssp_term(or_ctx *c, int i, int j)
       prdnu[1] = c->pref[1] * nu[1] * (1.0 - th) / rr;
     }
   } else nu[1] = 1.0;
-  if (nu[0] < DSSP_NU_DELTA || nu[1] < DSSP_NU_DELTA) return;
+  if (nu0_on && nu[0] > 0.0) c->branch[OR_BR_DSSP_NU0]++;
+  if (nu1_on && nu[1] > 0.0) c->branch[OR_BR_DSSP_NU1]++;
+  if (nu[0] < DSSP_NU_DELTA || nu[1] < DSSP_NU_DELTA) { c->branch[OR_BR_DSSP_NU_EXIT]++; return; }
+  c->branch[OR_BR_DSSP_CLASS1 + hb_class - 1]++;
+  if (hb_class == 2 && lambda[0] != 0.0) c->branch[OR_BR_DSSP_BETA]++;
 
   itheta[0] = 1; itheta[1] = i_rep; itheta[2] = i_AP; itheta[3] = i_P;
   /* geometry of the four H-bond channels: (acceptor O, donor N/H) */
@@ -1359,6 +1365,8 @@ static void dssp_term(or_ctx *c, int i, int j)
       prd_theta[k][0] = prd_theta[k][1] = 0.0;
     }
   }
+  for (int k = 1; k < 4; ++k)
+    if (itheta[k] && theta[k] > 1e-3) c->branch[OR_BR_DSSP_REP + k - 1]++;
   double ff1 = c->k_dssp * theta[0];
   V[0] = lambda[0];
   V[1] = lambda[1] * theta[1];
@@ -1412,6 +1420,10 @@ static void pap_term(or_ctx *c, int i, int j)
             (ic != jc && i + d < n && j + d < n && (c->chain_no[i + d] - 1) == ic && (c->chain_no[j + d] - 1) == jc);
   int nr = c->n_rama_par;
   int beta = nr > 0 && c->aps[nr - 1][i] == 1.0 && c->aps[nr - 1][j] == 1.0;
+  if (i_AP_med) c->branch[OR_BR_PAP_MED]++;
+  if (i_AP_long) c->branch[ic == jc ? OR_BR_PAP_LONG : OR_BR_PAP_LONG_X]++;
+  if (i_P) c->branch[ic == jc ? OR_BR_PAP_PAR : OR_BR_PAP_PAR_X]++;
+  if (beta && (i_AP_med || i_AP_long || i_P)) c->branch[OR_BR_PAP_BETA]++;
   if (!ON(OR_ET_PAP)) return;
   if (i_AP_med || i_AP_long) {
     double K = beta ? (i_AP_med ? c->k_P_AP[0] : 0.0) + (i_AP_long ? c->k_P_AP[1] * c->k_betapred_P_AP : 0.0)
@@ -1449,6 +1461,7 @@ static void helix_term(or_ctx *c, int i, int j)
   double h4i = c->h4prob[it];
   if (c->se[i] == 'P' && c->pro_accepter_flag) h4i = c->h4prob_pro_accepter;
   double prob_sum = h4i + c->h4prob[jt];
+  c->branch[OR_BR_HELIX]++;
   double dNO = R_NO - c->helix_NO_zero, dHO = R_HO - c->helix_HO_zero;
   double dNOs = dNO * c->helix_sigma_NO_sqinv, dHOs = dHO * c->helix_sigma_HO_sqinv;
   double pt = -c->k_helix * prob_sum * exp(-0.5 * (dNO * dNOs + dHO * dHOs));
@@ -1786,6 +1799,14 @@ int or_list(or_ctx *c, int kind, int arg, const double *xca, const double *xcb, 
 
 /* ------------------------------------------------------------------ */
 /* fragment-memory introspection                                        */
+
+int or_branch_counts(or_ctx *c, long long *out, int n, int reset)
+{
+  int m = n < OR_N_BRANCH ? n : OR_N_BRANCH;
+  for (int k = 0; k < m; ++k) out[k] = c->branch[k];
+  if (reset) memset(c->branch, 0, sizeof c->branch);
+  return OR_N_BRANCH;
+}
 
 int or_fm_count(const or_ctx *c) { return c->n_mems; }
 int or_fm_info(const or_ctx *c, int k, int *pos, int *len, double *weight)

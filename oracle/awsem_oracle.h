@@ -38,6 +38,22 @@ enum {
 
 typedef struct or_ctx or_ctx;
 
+/* Branch counters: how often each selection branch of the DSSP, P_AP and
+ * helix terms ran since the last reset (evidence that a test system
+ * exercises them).  DSSP: pairs evaluated per class 1..4 (fb.cpp:2956-3002),
+ * class 2 with the beta prediction, repulsive / antiparallel / parallel
+ * channels on with theta > 1e-3, nu switches of i and j evaluated, pairs
+ * leaving at nu < 1e-4 (fb.cpp:3052); P_AP: medium, long (same chain, other
+ * chain), parallel (same chain, other chain) bands and beta-predicted pairs
+ * (fb.cpp:3295-3315); helix pairs evaluated (fb.cpp:3706). */
+enum {
+  OR_BR_DSSP_CLASS1 = 0, OR_BR_DSSP_CLASS2, OR_BR_DSSP_CLASS3, OR_BR_DSSP_CLASS4, OR_BR_DSSP_BETA,
+  OR_BR_DSSP_REP, OR_BR_DSSP_AP, OR_BR_DSSP_P, OR_BR_DSSP_NU0, OR_BR_DSSP_NU1, OR_BR_DSSP_NU_EXIT,
+  OR_BR_PAP_MED, OR_BR_PAP_LONG, OR_BR_PAP_LONG_X, OR_BR_PAP_PAR, OR_BR_PAP_PAR_X, OR_BR_PAP_BETA,
+  OR_BR_HELIX, OR_N_BRANCH
+};
+int or_branch_counts(or_ctx *c, long long *out, int n, int reset);
+
 /* Reads the coeff file, the sequence file and the fixed-name side files
  * (anti_HB, anti_NHB, para_HB, para_one, anti_one, ssweight, gamma.dat,
  * burial_gamma.dat, fragment memory files) from workdir, like the

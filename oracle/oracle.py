@@ -21,6 +21,9 @@ ET_NAMES = ["Total", "Chain", "Shake", "Chi", "Rama", "Excluded", "DSSP", "P_AP"
 N_ET = 19
 ET = {name: k for k, name in enumerate(ET_NAMES)}
 ALL_TERMS = 0xFFFFFFFF
+BRANCHES = ["dssp_class1", "dssp_class2", "dssp_class3", "dssp_class4", "dssp_beta", "dssp_rep", "dssp_ap", "dssp_p",
+            "dssp_nu_i", "dssp_nu_j", "dssp_nu_exit", "pap_medium", "pap_long", "pap_long_interchain", "pap_parallel",
+            "pap_parallel_interchain", "pap_beta", "helix"]
 
 _lib = None
 
@@ -64,6 +67,8 @@ def lib():
         L.or_langevin_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_ulonglong, C.c_int, C.c_longlong, dp, dp, dp, dp]
         L.or_langevin_run.restype = C.c_int
+        L.or_branch_counts.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int,
+                                       C.c_int]
         _lib = L
     return _lib
 
@@ -124,6 +129,12 @@ class Oracle:
             if k >= 0:
                 return out[:k].copy()
             cap = -k + 16
+
+    def branches(self, reset=True) -> dict:
+        """Branch counters since the last reset (see awsem_oracle.h)."""
+        out = np.zeros(64, dtype=np.int64)
+        n = lib().or_branch_counts(self.h, out, 64, int(reset))
+        return dict(zip(BRANCHES, out[:n].tolist()))
 
     def theta_pair(self, r, helix=False, well=0):
         t, p = C.c_double(), C.c_double()
