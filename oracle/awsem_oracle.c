@@ -414,44 +414,76 @@ static char three_to_one(const char *s)
   return '?';
 }
 
-static int mem_load(or_mem *m, int pos, int fpos, int len, double weight, const char *path)
+/* The atom records of one .gro file in file order (the fields fm.cpp:60-110
+ * reads), parsed once per file and shared by every memory taken from it. */
+typedef struct {
+  int ires, kind; /* kind: 1 CA, 2 CB, 0 any other atom */
+  char one;       /* one-letter residue name ('?' when unknown, fm.cpp:270) */
+  double x, y, z; /* nm */
+} gro_rec;
+typedef struct {
+  char path[1024];
+  int ok, n;
+  gro_rec *r;
+} gro_file;
+
+static void gro_parse(const char *path, gro_file *g)
 {
-  memset(m, 0, sizeof(*m));
-  m->pos = pos; m->len = len; m->weight = weight;
-  m->se = (char *)calloc((size_t)len + 1, 1);
-  double *xca = (double *)calloc((size_t)len * 3, sizeof(double));
-  double *xcb = (double *)calloc((size_t)len * 3, sizeof(double));
+  memset(g, 0, sizeof(*g));
+  snprintf(g->path, sizeof g->path, "%s", path);
   FILE *f = fopen(path, "r");
-  if (!f) { free(xca); free(xcb); return 1; /* ERR_FILE */ }
+  if (!f) return;
+  g->ok = 1;
   char buff[201];
-  int nAtoms = 0, nca = 0, ncb = 0;
+  int nAtoms = 0;
   if (!fgets(buff, 200, f)) nAtoms = 0;
   if (fscanf(f, "%d", &nAtoms) != 1) nAtoms = 0;
   if (!fgets(buff, 200, f)) nAtoms = 0;
+  g->r = (gro_rec *)calloc((size_t)(nAtoms > 0 ? nAtoms : 1), sizeof(gro_rec));
   for (int i = 0; i < nAtoms; ++i) {
     int ires, iatom;
     char resty[64], atomty[64];
     double x, y, z;
     if (fscanf(f, "%d %63s %63s %d %lf %lf %lf", &ires, resty, atomty, &iatom, &x, &y, &z) != 7) break;
+    gro_rec *q = &g->r[g->n++];
+    q->ires = ires;
+    q->kind = strcmp(atomty, "CA") == 0 ? 1 : (strcmp(atomty, "CB") == 0 ? 2 : 0);
+    q->one = three_to_one(resty);
+    q->x = x; q->y = y; q->z = z;
+  }
+  fclose(f);
+}
+
+static int mem_load(or_mem *m, int pos, int fpos, int len, double weight, const gro_file *g)
+{
+  memset(m, 0, sizeof(*m));
+  m->pos = pos; m->len = len; m->weight = weight;
+  m->se = (char *)calloc((size_t)len + 1, 1);
+  if (!g->ok) return 1; /* ERR_FILE */
+  double *xca = (double *)calloc((size_t)len * 3, sizeof(double));
+  double *xcb = (double *)calloc((size_t)len * 3, sizeof(double));
+  int nca = 0, ncb = 0;
+  for (int i = 0; i < g->n; ++i) {
+    const gro_rec *q = &g->r[i];
+    int ires = q->ires;
     if (ires > fpos + len) break;
     if (ires > fpos && ires <= fpos + len) {
       ires -= fpos + 1;
-      x *= 10; y *= 10; z *= 10;
-      if (strcmp(atomty, "CA") == 0) {
-        if (ires >= len || nca >= len) { fclose(f); free(xca); free(xcb); return 2; }
-        m->se[ires] = three_to_one(resty);
-        if (m->se[ires] == '-') { fclose(f); free(xca); free(xcb); return 3; }
+      double x = q->x * 10, y = q->y * 10, z = q->z * 10;
+      if (q->kind == 1) {
+        if (ires >= len || nca >= len) { free(xca); free(xcb); return 2; }
+        m->se[ires] = q->one;
+        if (m->se[ires] == '-') { free(xca); free(xcb); return 3; }
         xca[3 * ires] = x; xca[3 * ires + 1] = y; xca[3 * ires + 2] = z;
         nca++;
       }
-      if (strcmp(atomty, "CB") == 0) {
-        if (ires >= len || ncb >= len) { fclose(f); free(xca); free(xcb); return 2; }
+      if (q->kind == 2) {
+        if (ires >= len || ncb >= len) { free(xca); free(xcb); return 2; }
         xcb[3 * ires] = x; xcb[3 * ires + 1] = y; xcb[3 * ires + 2] = z;
         ncb++;
       }
     }
   }
-  fclose(f);
   if (nca != len) { free(xca); free(xcb); return 2; }
   m->rf0 = (double *)calloc((size_t)len * len, sizeof(double));
   m->rf1 = (double *)calloc((size_t)len * len, sizeof(double));
@@ -506,6 +538,9 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
   int state = 0; /* 0 none, 1 target, 2 mems */
   c->n_mems = 0;
   int cap = 0;
+  int ngro = 0, capgro = 0;
+  gro_file *gros = NULL;
+  int rc = 0;
   while (fgets(ln, sizeof ln, f)) {
     char *line = trim(ln);
     if (line[0] == '#') continue;
@@ -522,23 +557,31 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
         if (nstr > 5) break;
         str[nstr] = strtok(NULL, " \t\n");
       }
-      if (nstr != 5) { fclose(f); set_err(err, errlen, "Fragment_Memory: Error reading mem file"); return -1; }
+      if (nstr != 5) { set_err(err, errlen, "Fragment_Memory: Error reading mem file"); rc = -1; break; }
       int tpos = atoi(str[1]) - 1, fpos = atoi(str[2]) - 1, len = atoi(str[3]);
       double weight = atof(str[4]);
       if (c->n_mems == cap) { cap = cap ? 2 * cap : 64; c->mems = (or_mem *)realloc(c->mems, (size_t)cap * sizeof(or_mem)); }
       char gro[1024];
       join_path(gro, sizeof gro, workdir, str[0]);
-      int e = mem_load(&c->mems[c->n_mems], tpos, fpos, len, weight, gro);
+      int gi = 0;
+      while (gi < ngro && strcmp(gros[gi].path, gro) != 0) ++gi;
+      if (gi == ngro) {
+        if (ngro == capgro) { capgro = capgro ? 2 * capgro : 16; gros = (gro_file *)realloc(gros, (size_t)capgro * sizeof(gro_file)); }
+        gro_parse(gro, &gros[ngro++]);
+      }
+      int e = mem_load(&c->mems[c->n_mems], tpos, fpos, len, weight, &gros[gi]);
       c->n_mems++;
-      if (e) { fclose(f); set_err(err, errlen, "read_mems: Fragment_Memory: Error reading memory fragment"); return -1; }
-      if (tpos + len > c->n) { fclose(f); set_err(err, errlen, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); return -1; }
+      if (e) { set_err(err, errlen, "read_mems: Fragment_Memory: Error reading memory fragment"); rc = -1; break; }
+      if (tpos + len > c->n) { set_err(err, errlen, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); rc = -1; break; }
     } else if (state == 0) {
       if (strcmp(line, "[Target]") == 0) state = 1;
       else if (strcmp(line, "[Memories]") == 0) state = 2;
     }
   }
   fclose(f);
-  return 0;
+  for (int k = 0; k < ngro; ++k) free(gros[k].r);
+  free(gros);
+  return rc;
 }
 
 /* ------------------------------------------------------------------ */
