@@ -43,7 +43,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_server_create", "awsem_server_submit", "awsem_server_pump", "awsem_server_start",
            "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
            "awsem_server_release", "awsem_server_stats", "awsem_server_last_error", "awsem_server_destroy",
-           "awsem_batch_destroy", "awsem_compute_res_noalloc"]
+           "awsem_batch_destroy", "awsem_compute_res_noalloc", "awsem_batch_lists"]
 
 
 class AwsemError(RuntimeError):
@@ -104,6 +104,7 @@ def lib():
         L.awsem_batch_set_skin.argtypes = [vp, C.c_double]
         L.awsem_batch_autotune.argtypes = [vp, C.POINTER(Langevin), C.c_int, dp, ip, ip, dp]
         L.awsem_batch_set_engine.argtypes = [vp, C.c_int]
+        L.awsem_batch_lists.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int, ip]
         u64 = C.c_uint64
         L.awsem_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
         L.awsem_server_submit.argtypes = [vp, vp, dp, dp, C.c_int, C.c_double, C.c_double, u64, C.POINTER(u64)]
@@ -364,6 +365,17 @@ class Batch:
         b, p, k = C.c_longlong(), C.c_longlong(), C.c_int()
         self._check(lib().awsem_batch_list_stats(self.h, C.byref(b), C.byref(p), C.byref(k)))
         return b.value, p.value, k.value
+
+    LIST_KINDS = {"verlet": 0, "dssp": 1, "pap": 2, "excluded": 3, "density": 4}
+
+    def lists(self, replica: int, kind: str, cap: int = 1 << 22) -> np.ndarray:
+        """The production pair selection of the last evaluation (sorted codes;
+        see awsem_batch_lists)."""
+        out = np.zeros(cap, dtype=np.int64)
+        cnt = C.c_int()
+        self._check(lib().awsem_batch_lists(self.h, replica, self.LIST_KINDS[kind],
+                                            out.ctypes.data_as(C.POINTER(C.c_int64)), cap, C.byref(cnt)))
+        return out[: cnt.value].copy()
 
     def split(self, flat: np.ndarray):
         """Splits a concatenated residue-layout array into per-replica (3, n, 3) arrays."""

@@ -51,6 +51,7 @@ struct DevBatch {
   int g_mode = -1, g_engine = -1, g_nsteps = -1;
   double g_dt = 0.0, g_damp = 0.0;
   int engine = 0;  // 0 resident (workgroup per replica), 1 wide (phase launches over all replicas)
+  int last_engine = -1;  // engine of the last completed launch (where its queue lengths are kept)
 
   ~DevBatch()
   {
@@ -734,6 +735,7 @@ static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
   hipError_t e = hipEventSynchronize(D.ev1);
   if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));
   D.f_valid = true;
+  D.last_engine = D.engine;
   float ms = 0.f;
   hipEventElapsedTime(&ms, D.ev0, D.ev1);
   b->ms_total = ms;
@@ -887,6 +889,38 @@ int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *p
   if (builds) *builds = nb;
   if (pairs) *pairs = np;
   if (n_invalid) *n_invalid = bad;
+  return AWSEM_OK;
+}
+
+int awsem_batch_lists(awsem_batch *b, int replica, int kind, int64_t *out, int cap, int *count)
+{
+  if (!b || !b->dev || !count || kind < 0 || kind > 4 || cap < 0 || (cap > 0 && !out)) return AWSEM_E_ARG;
+  DevBatch &D = *b->dev;
+  if (replica < 0 || replica >= D.nrep) return AWSEM_E_ARG;
+  if (D.last_engine < 0) return bfail(b, AwStatus::err(AWSEM_E_STATE, "no evaluation has run on this batch"));
+  if (hipStreamSynchronize(D.stream) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "sync"));
+  const ReplicaDesc &R = D.reps[replica];
+  int h[AW_NL_HDR];
+  if (R.nl_cap <= 0 || hipMemcpy(h, R.nl, sizeof h, hipMemcpyDeviceToHost) != hipSuccess)
+    return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "list header copy"));
+  if (!h[1]) return bfail(b, AwStatus::err(AWSEM_E_STATE, "the replica's Verlet list is not valid (overflowed or never built)"));
+  // queue lengths: the wide engine keeps its counters in header slots 4-11
+  // (queues at 6-9), the resident engine stores them to slots 12-15
+  int len;
+  if (kind == 0) len = h[0];
+  else len = D.last_engine == 1 ? h[5 + kind] : h[11 + kind];
+  if (len < 0 || len > (kind == 1 ? 2 * R.nl_cap : R.nl_cap)) return bfail(b, AwStatus::err(AWSEM_E_STATE, "queue length out of range"));
+  *count = len;
+  if (len > cap) return bfail(b, AwStatus::err(AWSEM_E_NOMEM, "list capacity too small"));
+  const int qoff[5] = {0, R.nl_cap, 3 * R.nl_cap, 4 * R.nl_cap, 5 * R.nl_cap};
+  std::vector<int> packed(std::max(len, 1));
+  if (len && hipMemcpy(packed.data(), R.nl + AW_NL_HDR + qoff[kind], (size_t)len * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "list copy"));
+  long long n = R.n;
+  std::vector<long long> codes(len);
+  for (int k = 0; k < len; ++k) codes[k] = (long long)(packed[k] >> 16) * n + (packed[k] & 0xffff);
+  std::sort(codes.begin(), codes.end());
+  for (int k = 0; k < len; ++k) out[k] = codes[k];
   return AWSEM_OK;
 }
 

@@ -1833,6 +1833,11 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
         AW_BARRIER();
         if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(12);
         int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
+        // the queue lengths of a one-chunk pass into the list header (slots
+        // 12-15), where awsem_batch_lists reads them after the launch
+        if (t0 == 0 && qs) {
+          V.nl[12] = nd; V.nl[13] = npap; V.nl[14] = nev; V.nl[15] = V.nl_state[5];
+        }
         for_packed(qd, 0, nd, t0, ts, [&](int i, int j) { dssp_term(V, i, j, E); });
         if constexpr (!std::is_void<PM>::value) if (pm) { AW_BARRIER(); pm->mark(13); }
         for_packed(qp, 0, npap, t0, ts, [&](int i, int j) { pap_term(V, i, j, E); });
@@ -1886,17 +1891,16 @@ AW_DEV void sig_switch(double x, double &s, double &onep)
   s = (double)(x >= 0.0 ? e * ip : ip);
   onep = (double)(x >= 0.0 ? 2.0f * ip : 2.0f * e * ip);
 }
-// tanh of an FP64-formed argument for the burial switches: FP64, or FP32
-// through exp(-2|x|) (absolute error ~1e-7, the accuracy the burial wells
-// enter the energies and forces with)
+// tanh(x) and sech^2(x) = 1 - tanh^2(x) in FP32 through E = exp(-2|x|):
+// tanh = +-(1-E)/(1+E), sech^2 = 4E/(1+E)^2 (no cancellation for |x| large)
 template <class T>
-AW_DEV double sw_tanh(double x)
+AW_DEV void sw_tanh_sech2(double x, double &t, double &s)
 {
-  if (sizeof(T) == 8) return tanh(x);
-  float a = (float)(x < 0.0 ? -x : x);
-  float e = aw_fast_exp(-2.0f * a);
-  double t = (double)((1.0f - e) * aw_frcp(1.0f + e));
-  return x < 0.0 ? -t : t;
+  float e = aw_fast_exp(-2.0f * (float)(x < 0.0 ? -x : x));
+  float ip = aw_frcp(1.0f + e);
+  float tt = (1.0f - e) * ip;
+  t = x < 0.0 ? -(double)tt : (double)tt;
+  s = (double)(4.0f * e * ip * ip);
 }
 
 template <class T>
@@ -1926,9 +1930,23 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts, bool ef = true)
     if (P->burial_flag) {
       double rho = rho_w, kb = P->burial_kappa, bfv = 0.0, eb = 0.0;
       for (int b = 0; b < 3; ++b) {
-        double t0_ = sw_tanh<T>(kb * (rho - P->burial_ro_min[b])), t1_ = sw_tanh<T>(kb * (P->burial_ro_max[b] - rho));
+        double x0 = kb * (rho - P->burial_ro_min[b]), x1 = kb * (P->burial_ro_max[b] - rho);
+        double t0_, t1_, dsq;
+        if (sizeof(T) == 8) {
+          t0_ = tanh(x0);
+          t1_ = tanh(x1);
+          dsq = t1_ * t1_ - t0_ * t0_;
+        } else {
+          // t1^2 - t0^2 = sech^2(x0) - sech^2(x1), with sech^2 = 4E/(1+E)^2,
+          // E = exp(-2|x|): exact to FP32 rounding relative to the sech^2
+          // values, where 1 - t^2 formed from t near +-1 would cancel
+          double s0, s1;
+          sw_tanh_sech2<T>(x0, t0_, s0);
+          sw_tanh_sech2<T>(x1, t1_, s1);
+          dsq = s0 - s1;
+        }
         eb += -0.5 * P->k_burial * bg[b] * (t0_ + t1_);
-        bfv += bg[b] * (t1_ * t1_ - t0_ * t0_);
+        bfv += bg[b] * dsq;
       }
       V.bf()[i] = bfv * 0.5 * P->k_burial * kb;
       if (ON(P, AW_ET_BURIAL)) E.burial += eb;

@@ -218,6 +218,16 @@ int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_ste
  * replicas whose current list is invalid (overflowed or never built). */
 int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *pairs,
                            int *n_invalid);
+/* Diagnostic: the production pair selection of one replica as the last
+ * evaluation of the batch used it (the same lists the force passes read):
+ * kind 0 the Verlet list of residue pairs (a superset of every term's pairs,
+ * i*n+j with i < j), 1 the DSSP calls (acceptor*n + donor; the pairs of
+ * fb.cpp:8064-8094 that pass |O-N| < dssp_hdrgn_cut), 2 the P_AP pairs
+ * (fb.cpp:8096-8103), 3 the residue pairs the excluded-volume pass visits,
+ * 4 the residue pairs the water-xi and force passes visit (supersets: those
+ * passes apply their exact cutoffs).  Sorted ascending; AWSEM_E_STATE when
+ * the replica's list was not valid. */
+int awsem_batch_lists(awsem_batch *b, int replica, int kind, int64_t *out, int cap, int *count);
 int awsem_batch_last_error(const awsem_batch *b, char *buf, size_t n);
 int awsem_batch_destroy(awsem_batch *b);
 
