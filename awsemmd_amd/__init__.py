@@ -43,7 +43,8 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_server_create", "awsem_server_submit", "awsem_server_pump", "awsem_server_start",
            "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
            "awsem_server_release", "awsem_server_stats", "awsem_server_last_error", "awsem_server_destroy",
-           "awsem_batch_destroy", "awsem_compute_res_noalloc", "awsem_batch_lists"]
+           "awsem_batch_destroy", "awsem_compute_res_noalloc", "awsem_batch_lists", "awsem_set_box",
+           "awsem_compute_image", "awsem_energy_log_open", "awsem_energy_log_write", "awsem_energy_log_close"]
 
 
 class AwsemError(RuntimeError):
@@ -80,6 +81,11 @@ def lib():
         L.awsem_compute.argtypes = [vp, dp, dp, dp]
         L.awsem_compute_res.argtypes = [vp, dp, dp, dp]
         L.awsem_compute_res_noalloc.argtypes = [vp, dp, dp, dp]
+        L.awsem_set_box.argtypes = [vp, ip, dp]
+        L.awsem_compute_image.argtypes = [vp, dp, C.POINTER(C.c_int32), dp, dp]
+        L.awsem_energy_log_open.argtypes = [vp, C.c_char_p]
+        L.awsem_energy_log_write.argtypes = [vp, C.c_int64, dp]
+        L.awsem_energy_log_close.argtypes = [vp]
         L.awsem_set_vexcluded.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
         L.awsem_set_bonds.argtypes = [vp, C.c_int, C.c_double, dp]
         L.awsem_set_term_mask.argtypes = [vp, C.c_uint32]
@@ -217,14 +223,38 @@ class FixBackbone:
         r = None if r0 is None else np.ascontiguousarray(r0, dtype=np.float64)
         self._check(lib().awsem_set_bonds(self.h, int(enable), k, _dp(r)))
 
-    def pre_force(self, x: np.ndarray, f: np.ndarray) -> np.ndarray:
-        """FixBackbone::pre_force: f[n_atoms,3] += forces (atom layout)."""
+    def set_box(self, periodic, prd=None):
+        """Periodic box: flags per dimension and box lengths (domain->prd)."""
+        pf = np.ascontiguousarray(periodic, dtype=np.int32)
+        pr = None if prd is None else np.ascontiguousarray(prd, dtype=np.float64)
+        self._check(lib().awsem_set_box(self.h, _ip(pf), _dp(pr)))
+
+    def pre_force(self, x: np.ndarray, f: np.ndarray, image: Optional[np.ndarray] = None) -> np.ndarray:
+        """FixBackbone::pre_force: f[n_atoms,3] += forces (atom layout); image
+        (LAMMPS image flags per atom) is needed in a periodic box."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         assert f.dtype == np.float64 and f.flags.c_contiguous
         e = np.zeros(N_ET)
-        self._check(lib().awsem_compute(self.h, _dp(x), _dp(f), _dp(e)))
+        if image is None:
+            self._check(lib().awsem_compute(self.h, _dp(x), _dp(f), _dp(e)))
+        else:
+            im = np.ascontiguousarray(image, dtype=np.int32)
+            self._check(lib().awsem_compute_image(self.h, _dp(x), im.ctypes.data_as(C.POINTER(C.c_int32)), _dp(f),
+                                                  _dp(e)))
         self.energy = e
         return e
+
+    def energy_log_open(self, path: str):
+        self._check(lib().awsem_energy_log_open(self.h, path.encode()))
+
+    def energy_log_write(self, step: int, energy: Optional[np.ndarray] = None):
+        """Appends one energy.log line: the given 19 energies, or (None) the
+        last evaluation's."""
+        e = None if energy is None else np.ascontiguousarray(energy, dtype=np.float64)
+        self._check(lib().awsem_energy_log_write(self.h, int(step), _dp(e)))
+
+    def energy_log_close(self):
+        self._check(lib().awsem_energy_log_close(self.h))
 
     def compute_res(self, xres: np.ndarray):
         """Residue layout (3, n, 3) [CA, CB, O]; returns (forces, energies)."""

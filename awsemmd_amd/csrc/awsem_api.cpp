@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -169,6 +170,16 @@ struct awsem_ctx {
   std::vector<int> atom_of;   // [3*n]: atom index of (kind, residue)
   std::unique_ptr<DevBatch> dev;
   bool dirty = true;          // parameters changed since dev was built
+  // periodic box (domain->x/y/zperiodic and prd): coordinates are unwrapped
+  // with the atoms' image flags before the evaluation (fb.cpp:7067-7112)
+  int periodic[3] = {0, 0, 0};
+  double prd[3] = {0.0, 0.0, 0.0};
+  double last_e[AW_N_ET] = {0.0};  // energies of the last evaluation
+  FILE *elog = nullptr;            // energy.log (awsem_energy_log_open)
+  ~awsem_ctx()
+  {
+    if (elog) std::fclose(elog);
+  }
 };
 
 struct awsem_batch {
@@ -453,6 +464,7 @@ static AwStatus compute_res(awsem_ctx *c, const double *xres, double *fres, doub
   D.last_ms = ms;
   for (size_t k = 0; k < f.size(); ++k) fres[k] += f[k];
   if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
+  for (int k = 0; k < AW_N_ET; ++k) c->last_e[k] = e[k];
   if (!std::isfinite(e[0])) return AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy");
   return AwStatus();
 }
@@ -542,22 +554,32 @@ int aw_compute_res_scratch(awsem_ctx *c, const double *xres, double *fres, doubl
   }
   if (hipStreamSynchronize(D.stream) != hipSuccess) return dev_fail("awsem_compute_res_noalloc: sync");
   if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
+  for (int k = 0; k < AW_N_ET; ++k) c->last_e[k] = e[k];
   if (!std::isfinite(e[0])) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy"));
   return AWSEM_OK;
 }
 
 extern "C" {
 
-int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy)
+// the atom-layout evaluation: gather CA/CB/O of every residue from the
+// caller's atom order (unwrapped with the image flags in a periodic box,
+// fb.cpp:7067-7112), evaluate, scatter-add the forces back
+static int compute_atoms(awsem_ctx *c, const double *x, const int32_t *image, double *f, double *energy)
 {
-  if (!c || c->sys.n < 0 || !x || !f) return AWSEM_E_ARG;
   if (!c->topo) return fail(c, AwStatus::err(AWSEM_E_STATE, "awsem_set_topology must be called first"));
   int n = c->sys.n;
   std::vector<double> xr((size_t)n * 9), fr((size_t)n * 9, 0.0);
   for (int k = 0; k < 3; ++k)
     for (int i = 0; i < n; ++i) {
       int a = c->atom_of[(size_t)k * n + i];
-      for (int l = 0; l < 3; ++l) xr[((size_t)k * n + i) * 3 + l] = x[(size_t)a * 3 + l];
+      double *dst = &xr[((size_t)k * n + i) * 3];
+      for (int l = 0; l < 3; ++l) dst[l] = x[(size_t)a * 3 + l];
+      if (image) {
+        int img = image[a];
+        int box[3] = {(img & 1023) - 512, (img >> 10 & 1023) - 512, (img >> 20) - 512};
+        for (int l = 0; l < 3; ++l)
+          if (c->periodic[l]) dst[l] = x[(size_t)a * 3 + l] + box[l] * c->prd[l];
+      }
     }
   if (!finite_all(xr.data(), xr.size())) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
   AwStatus s = compute_res(c, xr.data(), fr.data(), energy);
@@ -567,6 +589,78 @@ int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy)
       int a = c->atom_of[(size_t)k * n + i];
       for (int l = 0; l < 3; ++l) f[(size_t)a * 3 + l] += fr[((size_t)k * n + i) * 3 + l];
     }
+  return AWSEM_OK;
+}
+
+int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy)
+{
+  if (!c || c->sys.n < 0 || !x || !f) return AWSEM_E_ARG;
+  if (c->periodic[0] || c->periodic[1] || c->periodic[2])
+    return fail(c, AwStatus::err(AWSEM_E_UNSUPPORTED, "a periodic box needs the atoms' image flags: use awsem_compute_image"));
+  return compute_atoms(c, x, nullptr, f, energy);
+}
+
+int awsem_compute_image(awsem_ctx *c, const double *x, const int32_t *image, double *f, double *energy)
+{
+  if (!c || c->sys.n < 0 || !x || !f || !image) return AWSEM_E_ARG;
+  return compute_atoms(c, x, image, f, energy);
+}
+
+int awsem_set_box(awsem_ctx *c, const int *periodic, const double *prd)
+{
+  if (!c || c->sys.n < 0 || !periodic) return AWSEM_E_ARG;
+  for (int l = 0; l < 3; ++l)
+    if (periodic[l] && (!prd || !(prd[l] > 0.0) || !std::isfinite(prd[l])))
+      return fail(c, AwStatus::err(AWSEM_E_ARG, "a periodic dimension needs a positive box length"));
+  for (int l = 0; l < 3; ++l) {
+    c->periodic[l] = periodic[l] ? 1 : 0;
+    c->prd[l] = periodic[l] ? prd[l] : 0.0;
+  }
+  return AWSEM_OK;
+}
+
+// energy.log, fb.cpp:101-106 (header) and 7661-7672 (one line per thermo
+// step): the step, then "\t%8.6f" for the seventeen FixBackbone terms in
+// EnergyTerms order and the total.  The fix's own excluded-volume slot is
+// zero in the reference's production build (its terms run under
+// DEBUGFORCES only, fb.cpp:7396-7403); the excluded-volume pair style and
+// the bonds are LAMMPS styles that report their energies in LAMMPS' thermo
+// output, so this log leaves them out of the columns and of the total.
+int awsem_energy_log_open(awsem_ctx *c, const char *path)
+{
+  if (!c || c->sys.n < 0 || !path) return AWSEM_E_ARG;
+  if (c->elog) std::fclose(c->elog);
+  c->elog = std::fopen(path, "w");
+  if (!c->elog) return fail(c, AwStatus::err(AWSEM_E_FILE, std::string("cannot open ") + path));
+  std::fputs("Step   \tChain   \tShake   \tChi     \tRama    \tExcluded\tDSSP    \tP_AP    \tWater   \tBurial  "
+             "\tHelix   \tAMH-Go  \tFrag_Mem\tVec_FM  \tContact_Restraints  \tMembrane\tSSB     \tElectro.\tVTotal\n",
+             c->elog);
+  std::fflush(c->elog);
+  return AWSEM_OK;
+}
+
+int awsem_energy_log_write(awsem_ctx *c, int64_t step, const double *energy)
+{
+  if (!c || c->sys.n < 0) return AWSEM_E_ARG;
+  if (!c->elog) return fail(c, AwStatus::err(AWSEM_E_STATE, "awsem_energy_log_open must be called first"));
+  const double *e = energy ? energy : c->last_e;
+  double col[18], total = 0.0;
+  for (int k = 1; k < 18; ++k) {
+    col[k] = k == AW_ET_VEXCLUDED ? 0.0 : e[k];
+    total += col[k];
+  }
+  std::fprintf(c->elog, "%lld ", (long long)step);
+  for (int k = 1; k < 18; ++k) std::fprintf(c->elog, "\t%8.6f", col[k]);
+  std::fprintf(c->elog, "\t%8.6f\n", total);
+  std::fflush(c->elog);
+  return AWSEM_OK;
+}
+
+int awsem_energy_log_close(awsem_ctx *c)
+{
+  if (!c) return AWSEM_E_ARG;
+  if (c->elog) std::fclose(c->elog);
+  c->elog = nullptr;
   return AWSEM_OK;
 }
 
@@ -627,6 +721,7 @@ int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres, 
   hipEventElapsedTime(&ms, D.ev0, D.ev1);
   D.last_ms = ms;
   if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
+  for (int k = 0; k < AW_N_ET; ++k) c->last_e[k] = e[k];
   if (!std::isfinite(e[0])) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy"));
   return AWSEM_OK;
 }

@@ -105,6 +105,32 @@ int awsem_set_topology(awsem_ctx *c, int n_atoms, const int *group, const int *m
  * them in the same force evaluation. */
 int awsem_compute(awsem_ctx *c, const double *x, double *f, double *energy);
 
+/* Periodic box (LAMMPS domain->xperiodic/yperiodic/zperiodic and prd):
+ * compute_backbone unwraps CA, CB and O with the atoms' image flags before
+ * it evaluates anything (fb.cpp:7067-7112).  periodic[3] flags, prd[3] box
+ * lengths (Angstrom, positive where periodic).  Default: not periodic. */
+int awsem_set_box(awsem_ctx *c, const int *periodic, const double *prd);
+
+/* awsem_compute with the atoms' LAMMPS image flags (atom->image, 10 bits
+ * per dimension): a periodic coordinate is unwrapped as x + box * prd with
+ * box = (image & 1023) - 512, (image >> 10 & 1023) - 512, (image >> 20) - 512
+ * (fb.cpp:7067-7112); f += forces.  awsem_compute refuses a periodic box
+ * (AWSEM_E_UNSUPPORTED), since it has no image flags to unwrap with. */
+int awsem_compute_image(awsem_ctx *c, const double *x, const int32_t *image, double *f, double *energy);
+
+/* energy.log of the reference (rank 0, fb.cpp:101-106 and 7661-7672): open
+ * writes the tab-separated header; write appends "step " and then
+ * "\t%8.6f" for the seventeen FixBackbone terms (EnergyTerms order,
+ * fb.h:331-332) and their total, from energy[AWSEM_N_ET] or, when energy is
+ * NULL, from the context's last evaluation.  The Excluded column is the
+ * fix's own (zero in the reference's production build); the excluded-volume
+ * pair style and the bonds are LAMMPS styles outside this log and its
+ * total.  The caller decides the thermo steps (step % thermo_every == 0 in
+ * the reference). */
+int awsem_energy_log_open(awsem_ctx *c, const char *path);
+int awsem_energy_log_write(awsem_ctx *c, int64_t step, const double *energy);
+int awsem_energy_log_close(awsem_ctx *c);
+
 /* Residue-layout variant of awsem_compute (no topology needed). */
 int awsem_compute_res(awsem_ctx *c, const double *xres, double *fres, double *energy);
 

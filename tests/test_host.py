@@ -187,3 +187,49 @@ def test_noalloc_rejects_systems_beyond_its_array():
     with pytest.raises(A.AwsemError) as e:
         fix.compute_res_noalloc(x, np.zeros_like(x))
     assert e.value.code == -8 and "9196 residues" in str(e.value)
+
+
+def test_energy_log_byte_format(tmp_path):
+    """energy.log as the reference writes it (fb.cpp:101-106, 7661-7672):
+    the tab-separated header, then "%d " and "\\t%8.6f" per FixBackbone term
+    and the total; the excluded-volume pair style and the bonds (slots 5 and
+    18 here) belong to LAMMPS' own styles and stay out of the line."""
+    A = _A()
+    s, p = make_case([20], 2)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    path = tmp_path / "energy.log"
+    fix.energy_log_open(str(path))
+    rng = np.random.default_rng(1)
+    rows = []
+    for step in (0, 1000, 123456):
+        e = rng.normal(scale=50.0, size=A.N_ET)
+        e[0] = e[1:].sum()
+        fix.energy_log_write(step, e)
+        rows.append((step, e))
+    fix.energy_log_close()
+    head = ("Step   \tChain   \tShake   \tChi     \tRama    \tExcluded\tDSSP    \tP_AP    \tWater   \tBurial  \tHelix   "
+            "\tAMH-Go  \tFrag_Mem\tVec_FM  \tContact_Restraints  \tMembrane\tSSB     \tElectro.\tVTotal\n")
+    want = head
+    for step, e in rows:
+        cols = [0.0 if k == 5 else e[k] for k in range(1, 18)]
+        want += "%d " % step + "".join("\t%8.6f" % v for v in cols) + "\t%8.6f\n" % sum(cols)
+    assert path.read_text() == want
+
+
+def test_periodic_box_needs_image_flags():
+    """In a periodic box the drop-in needs image flags to unwrap the
+    coordinates (fb.cpp:7067-7112): awsem_compute refuses, before any device
+    work, and a periodic dimension needs a positive length."""
+    A = _A()
+    s, p = make_case([12], 3)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    n = s.n
+    fix.set_topology(np.repeat([0, 1, 2], n), np.ones(3 * n, dtype=int), np.tile(np.arange(1, n + 1), 3))
+    with pytest.raises(A.AwsemError) as e:
+        fix.set_box([1, 0, 0], [0.0, 0.0, 0.0])
+    assert e.value.code == -1
+    fix.set_box([1, 1, 0], [50.0, 60.0, 0.0])
+    x = s.coords().reshape(-1, 3)
+    with pytest.raises(A.AwsemError) as e:
+        fix.pre_force(x, np.zeros_like(x))
+    assert e.value.code == -4 and "image" in str(e.value)
