@@ -88,6 +88,8 @@ struct DevBatch {
   }
 };
 
+static AwStatus launch_seq(DevBatch &D, const AwRun &run, int engine, int nst);
+
 template <class T>
 static AwStatus upload(T **dst, const std::vector<T> &src, size_t min_elems = 1)
 {
@@ -705,10 +707,17 @@ int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres, 
   AwRun run;
   std::memset(&run, 0, sizeof run);
   run.mode = 1; run.nsteps = nsteps; run.dt = lg->dt; run.damp = lg->damp;
+  // the coordinates come from the caller: the Verlet list is rebuilt and
+  // the first forces evaluated before the first step
+  D.nl_dirty = true;
+  D.f_valid = false;
   s = D.set_schedule(nsteps, lg);
   if (!s.ok()) return fail(c, s);
   (void)hipEventRecord(D.ev0, D.stream);
-  if (aw_launch_phases(D.ptrs(), run, D.stream)) return fail(c, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
+  // the wide engine on this one system: Verlet list and work queues, every
+  // phase spread over many workgroups, the step sequence replayed from a
+  // captured graph
+  if (!(s = launch_seq(D, run, 1, nsteps)).ok()) return fail(c, s);
   (void)hipEventRecord(D.ev1, D.stream);
   double e[AW_N_ET];
   if (hipMemcpyAsync(xres, D.d_x, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
@@ -793,6 +802,38 @@ int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, doubl
   return AWSEM_OK;
 }
 
+// one launch sequence of a device batch on its stream: replayed from a
+// captured HIP graph (captured again when the mode, dt, damp, engine or the
+// wide engine's step count change), or launched directly
+static AwStatus launch_seq(DevBatch &D, const AwRun &run, int engine, int nst)
+{
+  auto launch = [&]() {
+    return engine == 1 ? aw_launch_wide(D.ptrs(), run, nst, D.stream) : aw_launch_resident(D.ptrs(), run, D.stream);
+  };
+  if (!D.use_graph) {
+    if (launch()) return AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed");
+    return AwStatus();
+  }
+  // the wide engine's launch sequence depends on the step count
+  int gkey_steps = engine == 1 ? nst : -1;
+  if (!D.graph || D.g_mode != run.mode || D.g_dt != run.dt || D.g_damp != run.damp || D.g_engine != engine ||
+      D.g_nsteps != gkey_steps) {
+    if (D.graph) { (void)hipGraphExecDestroy(D.graph); D.graph = nullptr; }
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return AwStatus::err(AWSEM_E_DEVICE, "graph capture");
+    int lrc = launch();
+    hipError_t ce = hipStreamEndCapture(D.stream, &g);
+    if (lrc || ce != hipSuccess || !g) return AwStatus::err(AWSEM_E_DEVICE, "graph capture of the launch sequence");
+    hipError_t ie = hipGraphInstantiate(&D.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) { D.graph = nullptr; return AwStatus::err(AWSEM_E_DEVICE, "graph instantiate"); }
+    D.g_mode = run.mode; D.g_dt = run.dt; D.g_damp = run.damp; D.g_engine = engine; D.g_nsteps = gkey_steps;
+  }
+  if (hipGraphLaunch(D.graph, D.stream) != hipSuccess) return AwStatus::err(AWSEM_E_DEVICE, "graph launch");
+  return AwStatus();
+}
+
 static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
 {
   DevBatch &D = *b->dev;
@@ -802,30 +843,7 @@ static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
   D.f_valid = false;
   (void)hipEventRecord(D.ev0, D.stream);
   int nst = run.mode == 1 ? run.nsteps : 0;
-  auto launch = [&]() {
-    return D.engine == 1 ? aw_launch_wide(D.ptrs(), run, nst, D.stream) : aw_launch_resident(D.ptrs(), run, D.stream);
-  };
-  if (D.use_graph) {
-    // the wide engine's launch sequence depends on the step count
-    int gkey_steps = D.engine == 1 ? nst : -1;
-    if (!D.graph || D.g_mode != run.mode || D.g_dt != run.dt || D.g_damp != run.damp || D.g_engine != D.engine ||
-        D.g_nsteps != gkey_steps) {
-      if (D.graph) { (void)hipGraphExecDestroy(D.graph); D.graph = nullptr; }
-      hipGraph_t g = nullptr;
-      if (hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
-        return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph capture"));
-      int lrc = launch();
-      hipError_t ce = hipStreamEndCapture(D.stream, &g);
-      if (lrc || ce != hipSuccess || !g) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph capture of the resident launch"));
-      hipError_t ie = hipGraphInstantiate(&D.graph, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (ie != hipSuccess) { D.graph = nullptr; return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph instantiate")); }
-      D.g_mode = run.mode; D.g_dt = run.dt; D.g_damp = run.damp; D.g_engine = D.engine; D.g_nsteps = gkey_steps;
-    }
-    if (hipGraphLaunch(D.graph, D.stream) != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "graph launch"));
-  } else if (launch()) {
-    return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed"));
-  }
+  if (!(s = launch_seq(D, run, D.engine, nst)).ok()) return bfail(b, s);
   (void)hipEventRecord(D.ev1, D.stream);
   hipError_t e = hipEventSynchronize(D.ev1);
   if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));

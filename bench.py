@@ -12,13 +12,16 @@ A "step" is one Langevin step of every replica (one batch-step).  `value`
 is replica-steps/s of the batched path (FP32 pair arithmetic, FP64
 accumulation), whole job, inputs resident in HBM.  Reported beside it:
   * the plain single-chain FP64 GPU drop-in on the same hardware (one chain
-    at a time through the phase engine), replica-steps/s on a sample of the
-    same chains;
+    at a time through awsem_run: Verlet lists, step sequence replayed from a
+    captured graph), replica-steps/s on a sample of the same chains;
   * the max relative force error of the batched path against that FP64
     path (max-norm over atoms);
   * p50 / p99 latency of one batch-step;
   * the HBM roofline of the resident kernel (algorithmic bytes / kernel time);
-  * the CPU oracle (FP64 restatement, 1 core) on a bounded sample.
+  * the CPU oracle (FP64 restatement, 1 core) on a bounded sample;
+  * the other workloads, bounded: one N=1000 chain through the FP64 drop-in,
+    oracle-checked every 100 steps; 64 replicas of an 8 x 300-residue
+    complex; the serving mix at offered arrival rates of 200-800 jobs/s.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20]
 """
@@ -168,23 +171,34 @@ def cpu_baseline(systems, budget_s=30.0, max_cores=16):
                       f"over the workload's lengths), {per:.1f} s each; rate = cores / mean seconds per replica-step"}
 
 
-def serving_run(A, W, ctx, n_jobs, rate_hz, seed, steps=500, round_steps=50, sigma=0.6):
-    """The serving mix: a Poisson stream of annealing jobs (600 -> 300 K over
-    `steps` steps, then the final per-term energies) with lengths lognormal
-    around a median of 150 residues (log-sd `sigma`), capped at 1500, each
-    generated like the single-chain workload with its own seed.  Jobs are
-    submitted at their arrival times to a background-scheduled server and
-    admitted by slots and device memory while others run."""
+def serving_pool(A, ctx, n_pool, seed, sigma=0.6):
+    """The serving mix's job pool: lengths lognormal around a median of 150
+    residues (log-sd `sigma`), capped at 1500, each system generated like the
+    single-chain workload with its own seed (sequence, trace, 20-memory
+    library)."""
     rng = np.random.default_rng(seed)
-    lengths = np.clip(np.round(rng.lognormal(np.log(150.0), sigma, n_jobs)), 20, 1500).astype(int)
+    lengths = np.clip(np.round(rng.lognormal(np.log(150.0), sigma, n_pool)), 20, 1500).astype(int)
     jobs = [(int(L), seed * 7919 + k) for k, L in enumerate(lengths)]
     t0 = time.perf_counter()
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
         systems = pool.map(_gen, jobs, chunksize=2)
     fixes = [ctx(s, A.PREC_FP32) for s in systems]
-    log(f"serving: {n_jobs} jobs generated in {time.perf_counter() - t0:.1f}s "
-        f"(L median {int(np.median(lengths))}, max {lengths.max()})")
-    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_jobs))
+    log(f"serving: pool of {n_pool} jobs generated in {time.perf_counter() - t0:.1f}s "
+        f"(L median {int(np.median(lengths))}, max {lengths.max()}, {int((lengths > 520).sum())} above the LDS cap)")
+    return systems, fixes, lengths
+
+
+def serving_run(A, systems, fixes, lengths, rate_hz, duration_s, seed, steps=500, round_steps=50, sigma=0.6):
+    """One arrival rate of the serving mix: a Poisson stream of annealing jobs
+    (600 -> 300 K over `steps` steps, then the final per-term energies) drawn
+    from the pool, submitted at their arrival times for duration_s to a
+    background-scheduled server that admits them by slots and device memory
+    while others run.  Latency runs from a job's arrival to its result."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, int(rate_hz * duration_s * 1.5) + 16))
+    arrivals = arrivals[arrivals < duration_s]
+    n_jobs = len(arrivals)
+    pick = rng.integers(0, len(systems), n_jobs)
     srv = A.Server(fixes[0], A.PREC_FP32, max_jobs=1024, steps_per_round=round_steps)
     # warm-up job (kernel load, first allocations), not counted
     w = srv.submit(fixes[0], systems[0].coords(), None, round_steps, seed=1)
@@ -198,7 +212,8 @@ def serving_run(A, W, ctx, n_jobs, rate_hz, seed, steps=500, round_steps=50, sig
 
     def submit(k):
         t_arr = t_start + arrivals[k]
-        jid = srv.submit(fixes[k], systems[k].coords(), None, steps, t_start=600.0, t_stop=300.0, seed=1000 + k)
+        q = pick[k]
+        jid = srv.submit(fixes[q], systems[q].coords(), None, steps, t_start=600.0, t_stop=300.0, seed=1000 + k)
         return jid, time.perf_counter() - t_arr
 
     t_start = time.perf_counter()
@@ -212,25 +227,39 @@ def serving_run(A, W, ctx, n_jobs, rate_hz, seed, steps=500, round_steps=50, sig
         subs = [f.result() for f in futs]
     t_sub = time.perf_counter() - t_start
     lat, energies_ok = [], True
-    for (jid, t_pack), s in zip(subs, systems):
+    for (jid, t_pack), q in zip(subs, pick):
         srv.wait(jid, 600.0)
-        x, e, t = srv.result(jid, s.n)
+        x, e, t = srv.result(jid, systems[q].n)
         energies_ok &= bool(np.all(np.isfinite(e)))
         lat.append(1e3 * t_pack + t["total_ms"])
+        srv.release(jid)
     t_all = time.perf_counter() - t_start
     st = srv.stats()
     srv.stop()
     srv.close()
     lat = np.array(lat)
-    return {"jobs": n_jobs, "arrival_rate_hz": rate_hz, "achieved_arrival_rate_hz": n_jobs / t_sub,
-            "steps_per_job": steps, "steps_per_round": round_steps,
-            "lengths": {"median": int(np.median(lengths)), "mean": float(lengths.mean()), "max": int(lengths.max()),
-                        "lognormal_sigma": sigma, "cap": 1500},
+    Ls = lengths[pick]
+    return {"offered_rate_hz": rate_hz, "jobs": n_jobs, "achieved_arrival_rate_hz": n_jobs / t_sub,
             "jobs_per_s": n_jobs / t_all, "replica_steps_per_s": n_jobs * steps / t_all,
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "max": float(lat.max())},
+            "lengths": {"median": int(np.median(Ls)), "max": int(Ls.max()), "above_lds_cap": int((Ls > 520).sum())},
             "peak_active_jobs": st["peak_active"], "rounds": st["rounds"], "device_ms": st["device_ms"],
             "final_energies_finite": energies_ok, "wall_s": t_all}
+
+
+def serving_sweep(A, ctx, rates, duration_s, seed, n_pool=600, steps=500, round_steps=50):
+    """The serving mix at several offered arrival rates, below and at
+    saturation, each for duration_s, from one job pool."""
+    systems, fixes, lengths = serving_pool(A, ctx, n_pool, seed)
+    out = []
+    for k, r in enumerate(rates):
+        res = serving_run(A, systems, fixes, lengths, r, duration_s, seed + 17 * k, steps, round_steps)
+        log(f"serving at {r:.0f} jobs/s offered: {res['jobs_per_s']:.1f} jobs/s done, latency p50 "
+            f"{res['latency_ms']['p50']:.1f} ms p99 {res['latency_ms']['p99']:.1f} ms, peak {res['peak_active_jobs']:.0f} active")
+        out.append(res)
+    return {"steps_per_job": steps, "steps_per_round": round_steps, "duration_s_per_rate": duration_s,
+            "lengths": {"median": 150, "lognormal_sigma": 0.6, "cap": 1500}, "pool": n_pool, "rates": out}
 
 
 def calib_copy(nbytes):
@@ -271,7 +300,9 @@ def force_field(A, W, system):
 
 def run_single_chain(a, A, W):
     """Plain FP64 GPU drop-in on one N=1000 chain: `steps` Langevin steps
-    through awsem_run (phase engine), and every `check_every` steps the FP64
+    through awsem_run (wide engine with Verlet lists, replayed from a captured
+    graph; one untimed call of `check_every` steps first captures it), and
+    every `check_every` steps the FP64
     per-term energies and total forces at the current coordinates against the
     oracle (per-term forces every 10th check)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -285,6 +316,8 @@ def run_single_chain(a, A, W):
     v = velocities(s, 300.0, 3)
     terms = ["Chain", "Chi", "Rama", "Excluded", "DSSP", "P_AP", "Water", "Burial", "Helix", "Frag_Mem", "Bond"]
     worst_e, worst_f, worst_tf, t_gpu, done, checks = 0.0, 0.0, 0.0, 0.0, 0, 0
+    fix.run(x.copy(), v.copy(), min(a.check_every, a.steps), dt=2.0, t_start=300.0, t_stop=300.0, damp=1000.0,
+            seed=a.seed, step0=0)  # warm-up: kernel load, graph capture
     while done < a.steps:
         k = min(a.check_every, a.steps - done)
         t0 = time.perf_counter()
@@ -380,12 +413,18 @@ def main():
     ap.add_argument("--engine", choices=["resident", "wide"], default="resident",
                     help="batch engine when not autotuned")
     ap.add_argument("--plain-sample", type=int, default=32)
-    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="CPU seconds of the host baseline, all cores")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--serving-jobs", type=int, default=512, help="jobs in the serving-mix run (0: skip)")
-    ap.add_argument("--serving-rate", type=float, default=1000.0, help="Poisson arrival rate of the serving mix (1/s)")
+    ap.add_argument("--serving-duration", type=float, default=6.0,
+                    help="seconds of arrivals per offered rate of the serving mix (0: skip)")
+    ap.add_argument("--serving-rates", default="200,400,600,800",
+                    help="offered Poisson arrival rates of the serving mix (jobs/s), comma-separated")
+    ap.add_argument("--single-steps", type=int, default=500,
+                    help="steps of the N=1000 FP64 drop-in line in the default run, oracle-checked every 100 (0: skip)")
+    ap.add_argument("--multimer-steps", type=int, default=20,
+                    help="timed steps of the 64 x (8 x 300) multimer line in the default run (0: skip)")
     ap.add_argument("--workload", choices=["batch", "single", "multimer"], default="batch",
                     help="batch: 256 chains (default, the headline); single: one N=1000 chain through the "
                          "FP64 drop-in, oracle-checked; multimer: 64 x (8 x 300) annealing")
@@ -509,9 +548,9 @@ def main():
         # plain FP64 drop-in throughput: device-resident dynamics, one chain
         xv = xs_l[r].copy()
         vv = batch.split(vs)[r].copy()
-        fx.run(xv, vv, 1, step0=0, **lg)  # warm-up
+        fx.run(xv, vv, a.plain_steps, step0=0, **lg)  # warm-up: list build, graph capture
         c0 = time.perf_counter()
-        fx.run(xv, vv, a.plain_steps, step0=1, **lg)
+        fx.run(xv, vv, a.plain_steps, step0=a.plain_steps, **lg)
         plain_time += time.perf_counter() - c0
         plain_steps += a.plain_steps
         fx.close()
@@ -529,10 +568,19 @@ def main():
     if a.calib_copy_mb:
         calib_copy(a.calib_copy_mb << 20)
     serving = None
-    if a.serving_jobs > 0:
-        serving = serving_run(A, W, ctx, a.serving_jobs, a.serving_rate, a.seed + 1)
-        log(f"serving: {serving['jobs_per_s']:.1f} jobs/s, latency p50 {serving['latency_ms']['p50']:.1f} ms "
-            f"p99 {serving['latency_ms']['p99']:.1f} ms, peak {serving['peak_active_jobs']:.0f} jobs active")
+    if a.serving_duration > 0:
+        rates = [float(r) for r in a.serving_rates.split(",") if r]
+        serving = serving_sweep(A, ctx, rates, a.serving_duration, a.seed + 1)
+    single = None
+    if a.single_steps > 0:
+        a1 = argparse.Namespace(**vars(a))
+        a1.steps, a1.check_every = a.single_steps, 100
+        single = run_single_chain(a1, A, W)
+    multimer = None
+    if a.multimer_steps > 0:
+        a3 = argparse.Namespace(**vars(a))
+        a3.replicas, a3.steps, a3.warmup, a3.chunk, a3.engine = 64, a.multimer_steps, 5, 0, "wide"
+        multimer = run_multimer(a3, A, W)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(systems, a.cpu_budget)
@@ -556,7 +604,8 @@ def main():
                    "verlet_skin_A": tuned["skin"] if tuned else 2.0, "autotuned": tuned is not None,
                    "engine": tuned["engine"] if tuned else a.engine},
         "plain_fp64_gpu": {"value": plain_value, "unit": "replica-steps/s",
-                           "sample": f"{len(sample)} chains x {a.plain_steps} steps, phase engine, one chain at a time"},
+                           "sample": f"{len(sample)} chains x {a.plain_steps} steps through awsem_run (wide engine, "
+                                     "Verlet lists, captured graph), one chain at a time"},
         "speedup_vs_plain_fp64": value / plain_value,
         "force_err_max_vs_fp64": force_err,
         "latency_ms_per_batch_step": {"p50": float(np.percentile(per_step_ms, 50)),
@@ -573,6 +622,8 @@ def main():
                      "algorithmic_bytes_per_batch_step": bytes_step},
         "cpu_baseline": cpu,
         "serving": serving,
+        "single_chain_fp64": single,
+        "multimer": multimer,
         "setup_s": {"generate": t_gen, "build": t_build},
     }
     print(json.dumps(out))

@@ -239,7 +239,7 @@ def test_single_chain_n100_1000_steps_against_oracle(oracle_mod):
 
 
 def test_plain_fp64_run_checked_along_trajectory(oracle_mod):
-    """The FP64 drop-in (awsem_run, phase engine) over 500 Langevin steps of a
+    """The FP64 drop-in (awsem_run, wide engine) over 500 Langevin steps of a
     300-residue chain; every 100 steps the GPU energies and forces at the
     trajectory's coordinates against the oracle."""
     A = _gpu()
