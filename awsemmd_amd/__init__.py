@@ -38,7 +38,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_n_residues", "awsem_pair_list_cutoff", "awsem_last_error", "awsem_destroy",
            "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
            "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_phase_clock",
-           "awsem_batch_list_stats", "awsem_batch_last_error", "awsem_batch_use_graph", "awsem_batch_set_skin",
+           "awsem_batch_list_stats", "awsem_batch_share_stats", "awsem_batch_last_error", "awsem_batch_use_graph", "awsem_batch_set_skin",
            "awsem_batch_autotune", "awsem_batch_set_engine",
            "awsem_server_create", "awsem_server_submit", "awsem_server_pump", "awsem_server_start",
            "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
@@ -106,6 +106,7 @@ def lib():
         L.awsem_batch_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.awsem_batch_phase_clock.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.awsem_batch_list_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip]
+        L.awsem_batch_share_stats.argtypes = [vp, ip, C.POINTER(C.c_longlong)]
         L.awsem_batch_use_graph.argtypes = [vp, C.c_int]
         L.awsem_batch_set_skin.argtypes = [vp, C.c_double]
         L.awsem_batch_autotune.argtypes = [vp, C.POINTER(Langevin), C.c_int, dp, ip, ip, dp]
@@ -395,6 +396,13 @@ class Batch:
         b, p, k = C.c_longlong(), C.c_longlong(), C.c_int()
         self._check(lib().awsem_batch_list_stats(self.h, C.byref(b), C.byref(p), C.byref(k)))
         return b.value, p.value, k.value
+
+    def share_stats(self):
+        """(replicas whose fragment-memory chunks idle workgroups may take,
+        chunks evaluated by another workgroup than the replica's own so far)."""
+        n, c = C.c_int(), C.c_longlong()
+        self._check(lib().awsem_batch_share_stats(self.h, C.byref(n), C.byref(c)))
+        return n.value, c.value
 
     LIST_KINDS = {"verlet": 0, "dssp": 1, "pap": 2, "excluded": 3, "density": 4}
 

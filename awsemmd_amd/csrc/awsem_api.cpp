@@ -53,11 +53,19 @@ struct DevBatch {
   double g_dt = 0.0, g_damp = 0.0;
   int engine = 0;  // 0 resident (workgroup per replica), 1 wide (phase launches over all replicas)
   int last_engine = -1;  // engine of the last completed launch (where its queue lengths are kept)
+  // fragment-memory sharing of the resident engine: one allocation holding
+  // the claim words, the list of sharing replicas and their chunk tables,
+  // position copies and accumulators
+  void *d_share = nullptr;
+  std::vector<unsigned char> share_img;  // its initial contents (restored after a failed launch)
+  int nshare = 0;
+  int *d_share_rep = nullptr, *d_share_n = nullptr;
+  unsigned long long *d_share_word = nullptr;
 
   ~DevBatch()
   {
     if (graph) (void)hipGraphExecDestroy(graph);
-    void *ptrs[] = {d_rep, d_params, d_queue, d_order, d_count, d_x, d_v, d_f, d_e, d_list, d_prof};
+    void *ptrs[] = {d_rep, d_params, d_queue, d_order, d_count, d_x, d_v, d_f, d_e, d_list, d_prof, d_share};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     sys.clear();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -70,6 +78,7 @@ struct DevBatch {
     AwBatchPtrs p;
     p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds;
     p.params = d_params; p.queue = d_queue; p.order = d_order; p.prof = d_prof;
+    p.nshare = nshare; p.share_rep = d_share_rep; p.share_n = d_share_n; p.share_word = d_share_word;
     return p;
   }
 
@@ -99,6 +108,102 @@ static AwStatus upload(T **dst, const std::vector<T> &src, size_t min_elems = 1)
   return AwStatus();
 }
 
+// fragment-memory chunks of one replica: consecutive items, at most cs of
+// them, whose residues lie in [lo, lo + AW_SHARE_SPAN); false when the
+// items do not cut into AW_SHARE_MAXCH such chunks
+static bool fm_chunks(const std::vector<int> &ij, int cs, std::vector<ShareChunk> &out)
+{
+  out.clear();
+  int nit = (int)ij.size();
+  int k0 = 0, lo = 1 << 30, hi = -1;
+  for (int k = 0; k <= nit; ++k) {
+    int i = k < nit ? ij[k] >> 16 : 0, j = k < nit ? ij[k] & 0xffff : 0;
+    bool close = k == nit || k - k0 == cs ||
+                 std::max(hi, j) - std::min(lo, i) + 1 > AW_SHARE_SPAN;
+    if (close && k > k0) {
+      ShareChunk c;
+      c.k0 = k0; c.k1 = k; c.lo = lo; c.pad = 0;
+      out.push_back(c);
+      k0 = k; lo = 1 << 30; hi = -1;
+    }
+    if (k == nit) break;
+    if (j - i + 1 > AW_SHARE_SPAN) return false;
+    lo = std::min(lo, i);
+    hi = std::max(hi, j);
+  }
+  return (int)out.size() <= AW_SHARE_MAXCH;
+}
+
+// fragment-memory sharing for a resident batch in LDS: replicas above the
+// batch's mean size (AWSEM_SHARE_MIN_N overrides; AWSEM_SHARE=0 turns it
+// off) hand chunks of AWSEM_SHARE_CHUNK (512) items to idle workgroups
+static AwStatus setup_share(DevBatch &D, const std::vector<std::vector<int>> &ij)
+{
+  const char *on = std::getenv("AWSEM_SHARE");
+  if ((on && std::atoi(on) == 0) || !D.lds || D.nrep < 2) return AwStatus();
+  // a helping workgroup's waves stage residue spans in the slab
+  if (aw_slab_doubles(D.max_n) < 8LL * 12 * AW_SHARE_SPAN) return AwStatus();
+  double min_n = (double)D.total_res / D.nrep;
+  if (const char *e = std::getenv("AWSEM_SHARE_MIN_N")) min_n = std::atof(e);
+  int cs = 512;
+  if (const char *e = std::getenv("AWSEM_SHARE_CHUNK")) cs = std::max(64, std::atoi(e));
+  std::vector<int> rep;
+  std::vector<std::vector<ShareChunk>> ch;
+  for (int r = 0; r < D.nrep; ++r) {
+    if (D.reps[r].n < min_n || ij[r].empty()) continue;
+    std::vector<ShareChunk> c;
+    if (!fm_chunks(ij[r], cs, c) || c.size() < 2) continue;
+    rep.push_back(r);
+    ch.push_back(std::move(c));
+  }
+  if (rep.empty()) return AwStatus();
+  // longest first
+  std::vector<int> ord(rep.size());
+  for (size_t q = 0; q < ord.size(); ++q) ord[q] = (int)q;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return D.reps[rep[a]].n > D.reps[rep[b]].n; });
+  int ns = (int)rep.size();
+  // layout: words, done counters, replica list, chunk counts, then per
+  // replica its chunks, positions and accumulator
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t o_word = 0, o_done = al(8 * (size_t)ns), o_rep = o_done + al(4 * (size_t)ns), o_n = o_rep + al(4 * (size_t)ns);
+  size_t cur = o_n + al(4 * (size_t)ns);
+  std::vector<size_t> o_ch(ns), o_x(ns), o_acc(ns);
+  for (int q = 0; q < ns; ++q) {
+    int r = rep[ord[q]], n = D.reps[r].n;
+    o_ch[q] = cur; cur += al(ch[ord[q]].size() * sizeof(ShareChunk));
+    o_x[q] = cur; cur += al(6 * (size_t)n * sizeof(double));
+    o_acc[q] = cur; cur += al((6 * (size_t)n + 1) * sizeof(double));
+  }
+  std::vector<unsigned char> img(cur, 0);
+  HIPCHK(hipMalloc(&D.d_share, cur));
+  unsigned char *b = (unsigned char *)D.d_share;
+  for (int q = 0; q < ns; ++q) {
+    int r = rep[ord[q]];
+    const std::vector<ShareChunk> &c = ch[ord[q]];
+    unsigned long long w = c.size();  // closed: no chunk open before the owner's first evaluation
+    std::memcpy(img.data() + o_word + 8 * q, &w, 8);
+    std::memcpy(img.data() + o_rep + 4 * q, &r, 4);
+    int nc = (int)c.size();
+    std::memcpy(img.data() + o_n + 4 * q, &nc, 4);
+    std::memcpy(img.data() + o_ch[q], c.data(), c.size() * sizeof(ShareChunk));
+    ReplicaDesc &R = D.reps[r];
+    R.sh_word = (unsigned long long *)(b + o_word) + q;
+    R.sh_done = (int *)(b + o_done) + q;
+    R.sh_chunks = (const ShareChunk *)(b + o_ch[q]);
+    R.sh_x = (double *)(b + o_x[q]);
+    R.sh_acc = (double *)(b + o_acc[q]);
+    R.sh_n = nc;
+    R.sh_q = q;
+  }
+  HIPCHK(hipMemcpy(D.d_share, img.data(), cur, hipMemcpyHostToDevice));
+  D.share_img = std::move(img);
+  D.nshare = ns;
+  D.d_share_word = (unsigned long long *)(b + o_word);
+  D.d_share_rep = (int *)(b + o_rep);
+  D.d_share_n = (int *)(b + o_n);
+  return AwStatus();
+}
+
 static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int precision, bool want_lds,
                                std::unique_ptr<DevBatch> &out)
 {
@@ -123,6 +228,7 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMemset(D->d_f, 0, st));
   HIPCHK(hipMalloc((void **)&D->d_e, (size_t)D->nrep * AW_N_ET * sizeof(double)));
   HIPCHK(hipMemset(D->d_e, 0, (size_t)D->nrep * AW_N_ET * sizeof(double)));
+  std::vector<std::vector<int>> fm_ij(D->nrep);
   for (int r = 0; r < D->nrep; ++r) {
     const AwSystem &S = *sys[r];
     AwDevPlan plan;
@@ -139,11 +245,15 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
     R.energy = D->d_e + (long long)r * AW_N_ET;
     D->reps.push_back(R);
     D->sys.push_back(std::move(ds));
+    if (want_lds) fm_ij[r] = std::move(plan.fm_ij);
   }
   AwStatus s;
+  if (want_lds && !(s = setup_share(*D, fm_ij)).ok()) return s;
   if (!(s = upload_raw((void **)&D->d_rep, D->reps.data(), D->reps.size() * sizeof(ReplicaDesc))).ok()) return s;
   if (!(s = upload_raw(&D->d_params, &D->P, sizeof(FFParams))).ok()) return s;
-  HIPCHK(hipMalloc((void **)&D->d_queue, 64));
+  // [0] work queue, [16] finished replicas, [32] sharing error flag
+  HIPCHK(hipMalloc((void **)&D->d_queue, 256));
+  HIPCHK(hipMemset(D->d_queue, 0, 256));
   HIPCHK(hipMalloc((void **)&D->d_count, 64));
   if (const char *e = std::getenv("AWSEM_PHASE_PROF")) {
     if (std::atoi(e) > 0) {
@@ -847,6 +957,20 @@ static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
   (void)hipEventRecord(D.ev1, D.stream);
   hipError_t e = hipEventSynchronize(D.ev1);
   if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));
+  if (D.nshare > 0 && D.engine == 0) {
+    // a replica that gave up waiting for chunks taken elsewhere (bounded
+    // spin) leaves this flag: the launch's forces are then incomplete
+    int flag = 0;
+    if (hipMemcpy(&flag, D.d_queue + 32, sizeof flag, hipMemcpyDeviceToHost) != hipSuccess)
+      return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "reading the sharing flag"));
+    if (flag) {
+      // claim words, counters and accumulators back to their initial state
+      (void)hipMemset(D.d_queue + 32, 0, sizeof flag);
+      (void)hipMemcpy(D.d_share, D.share_img.data(), D.share_img.size(), hipMemcpyHostToDevice);
+      return bfail(b, AwStatus::err(AWSEM_E_DEVICE, "fragment-memory sharing: a replica gave up waiting for its "
+                                                    "chunks; the launch's results are invalid"));
+    }
+  }
   D.f_valid = true;
   D.last_engine = D.engine;
   float ms = 0.f;
@@ -925,7 +1049,7 @@ int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_ste
   bool fv0 = D.f_valid;
   double skin0 = D.P.nl_skin;
   const double skins[] = {1.0, 1.5, 2.0, 2.5, 3.0};
-  const int chunks[] = {5, 10, 20, 40};
+  const int chunks[] = {5, 10, 20, 40, 80, 160};
   double best = 1e300, best_skin = skin0;
   int best_chunk = 10, best_engine = D.engine;
   int rc = AWSEM_OK;
@@ -1002,6 +1126,19 @@ int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *p
   if (builds) *builds = nb;
   if (pairs) *pairs = np;
   if (n_invalid) *n_invalid = bad;
+  return AWSEM_OK;
+}
+
+int awsem_batch_share_stats(const awsem_batch *b, int *n_shared, long long *chunks)
+{
+  if (!b || !b->dev) return AWSEM_E_ARG;
+  const DevBatch &D = *b->dev;
+  if (hipStreamSynchronize(D.stream) != hipSuccess) return AWSEM_E_DEVICE;
+  int c = 0;
+  if (D.nshare > 0 && hipMemcpy(&c, D.d_queue + 48, sizeof c, hipMemcpyDeviceToHost) != hipSuccess)
+    return AWSEM_E_DEVICE;
+  if (n_shared) *n_shared = D.nshare;
+  if (chunks) *chunks = c;
   return AWSEM_OK;
 }
 

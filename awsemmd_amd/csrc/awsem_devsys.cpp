@@ -104,6 +104,8 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
     put(img, plan.off_coef, coef);
   }
   plan.n_items = fm.n_items();
+  plan.fm_ij.resize(plan.n_items);
+  for (int k = 0; k < plan.n_items; ++k) plan.fm_ij[k] = fm.meta[5 * k];
   plan.n = n;
   plan.nch = S.nch;
   plan.fm = AwFMPack();  // packed into the image

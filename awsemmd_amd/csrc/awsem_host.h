@@ -129,6 +129,7 @@ struct AwDevPlan {
   size_t off_x = 0, off_v = 0, off_f = 0, off_e = 0, off_rinfo = 0, off_aps = 0, off_chs = 0, off_chl = 0;
   size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0, off_cells = 0;
   int cell_h = 0;
+  std::vector<int> fm_ij;  // (i << 16) | j of every fragment-memory item, in device order
   size_t bytes = 0;
   std::vector<unsigned char> image;  // host copy of the whole device image
 };

@@ -4,15 +4,23 @@
 #include <hip/hip_runtime.h>
 
 struct AwBatchPtrs {
-  const void *rep;      // ReplicaDesc[nrep] on the device
-  int nrep;
-  int max_n;            // largest replica (residues)
-  int fp32;             // pair arithmetic in FP32 (else FP64)
-  int lds_resident;     // resident engine keeps replicas in LDS
-  const void *params;   // FFParams on the device
-  int *queue;           // one int of device scratch (work-queue counter)
-  const int *order;     // [nrep] replica order for the resident engine
-  unsigned long long *prof;  // optional per-phase clock accumulators [32]
+  const void *rep = nullptr;      // ReplicaDesc[nrep] on the device
+  int nrep = 0;
+  int max_n = 0;                  // largest replica (residues)
+  int fp32 = 0;                   // pair arithmetic in FP32 (else FP64)
+  int lds_resident = 0;           // resident engine keeps replicas in LDS
+  const void *params = nullptr;   // FFParams on the device
+  int *queue = nullptr;           // device scratch: [0] work-queue counter, [16] replicas finished,
+                                  // [32] sharing error flag, [48] chunks evaluated elsewhere (the
+                                  // last three with nshare > 0 only)
+  const int *order = nullptr;     // [nrep] replica order for the resident engine
+  unsigned long long *prof = nullptr;  // optional per-phase clock accumulators [32]
+  // fragment-memory sharing (resident engine, LDS variant): the replicas
+  // whose chunks other workgroups may take
+  int nshare = 0;
+  const int *share_rep = nullptr;             // [nshare] replica index
+  const int *share_n = nullptr;               // [nshare] chunks
+  unsigned long long *share_word = nullptr;   // [nshare] claim words (ReplicaDesc::sh_word)
 };
 
 struct AwRun {

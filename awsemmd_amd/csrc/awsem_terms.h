@@ -327,6 +327,13 @@ struct View {
   int *nl_state;         // LDS: [0] max drift (float bits), [1] build counter, [2..5] queue counts,
                          // [6] largest extent, [7] list pairs + 1 when valid, else 0
   AW_G int *wq;          // global work queues beside the list: DSSP [2 cap], P_AP [cap], EV [cap], density pairs [cap]
+  // fragment-memory sharing (resident engine; sh_n == 0 elsewhere): the
+  // replica's claim word and chunk table, and (LDS) the chunks this
+  // workgroup claimed in the current evaluation
+  int sh_n;
+  unsigned long long *sh_word;
+  const ShareChunk *sh_chunks;
+  unsigned long long *sh_mask;
   // slab layout: x[3][n][3] (CA, CB, O), xv[3][n][3] (N, H, C'),
   // f[3][n][3], the ten per-residue scalars [n][AW_SC_STRIDE], ext (float)
   AW_DEV double *x() const { return s; }
@@ -928,9 +935,11 @@ AW_DEV void fm_load(const char *rfb, const char *cfb, const FmRow &a, int m0, in
 // PIPE = false (lanes with one or two items, as in the grid-wide phase
 // engine): no prefetch of the next item's first batch, each item loads its
 // own
+// off: residue index of x's first row (items carry replica residue indices;
+// a copy of a residue span is addressed as rows i - off)
 template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true, int FMB = FmBatch<T>::FMB, int TB = 4>
 AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
-                          const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
+                          const typename T4<T>::item *items, int k, int k1, int ks, Sink sink, int off = 0)
 {
   static_assert(FMB % 4 == 0 && FMB <= AW_FM_PAD, "first batch: whole weight vectors within the padding");
   static_assert(TB % 4 == 0 && TB <= AW_FM_PAD, "second batch: whole weight vectors");
@@ -960,7 +969,7 @@ AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
     FmRow row = fm_row<T>(cur);
     fm_load(rfb, cfb, row, tail ? FMB : 0, TB, fb, cb);
     AW_SCHED_FENCE();  // the loads stay ahead of the first batch's arithmetic
-    int i = cur.ij >> 16, j = cur.ij & 0xffff;
+    int i = (cur.ij >> 16) - off, j = (cur.ij & 0xffff) - off;
     bool gi = cur.nmem & AW_FM_GLY_I, gj = cur.nmem & AW_FM_GLY_J;
     double cai[3], caj[3], cbi[3], cbj[3];
 #pragma unroll
@@ -1047,14 +1056,54 @@ AW_DEV double fm_run_body(const double *x, int n, const void *rf, const T *coef,
 // register allocation for the batches in flight)
 template <class T, bool XLDS, bool EF, class Sink, bool PIPE = true>
 AW_NOINLINE double fm_run(const double *x, int n, const void *rf, const T *coef,
-                          const typename T4<T>::item *items, int k, int k1, int ks, Sink sink)
+                          const typename T4<T>::item *items, int k, int k1, int ks, Sink sink, int off = 0)
 {
   // FP32: a first batch of 12 and a second of 8 (the shipped 20 memories
   // per window): the second batch's arithmetic covers more of the next
   // item's load latency
   const int FMB = sizeof(T) == 4 ? 12 : FmBatch<T>::FMB, TB = sizeof(T) == 4 ? 8 : 4;
-  return fm_run_body<T, XLDS, EF, Sink, PIPE, FMB, TB>(x, n, rf, coef, items, k, k1, ks, sink);
+  return fm_run_body<T, XLDS, EF, Sink, PIPE, FMB, TB>(x, n, rf, coef, items, k, k1, ks, sink, off);
 }
+
+#ifndef AW_HOST_EMU
+// a shared replica's own workgroup: each wave claims chunks one at a time
+// (the claim word's low half is the next unclaimed chunk) and evaluates a
+// claimed chunk's items, a lane per item, into the replica's forces; the
+// claimed chunks are marked in the workgroup's LDS mask.  Returns the
+// energy of the lane's items (ef).
+template <class T, bool XLDS>
+AW_DEV double fm_claims(const View<T> &V, const typename T4<T>::item *items, bool ef)
+{
+  typedef FmSinkForce<XLDS> Sk;
+  int lane = threadIdx.x & 63;
+  double e = 0.0;
+  auto claim = [&]() {
+    unsigned c = 0;
+    if (lane == 0)
+      c = (unsigned)__hip_atomic_fetch_add(V.sh_word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return c;
+  };
+  unsigned cn = claim();
+  for (;;) {
+    unsigned c = __builtin_amdgcn_readfirstlane(cn);
+    if (c >= (unsigned)V.sh_n) break;
+    cn = claim();  // the next claim is in flight while this chunk computes
+    if (lane == 0)
+      __hip_atomic_fetch_or(V.sh_mask, 1ull << c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    int k0 = __builtin_amdgcn_readfirstlane(V.sh_chunks[c].k0);
+    int k1 = __builtin_amdgcn_readfirstlane(V.sh_chunks[c].k1);
+    Sk sink{aw_cast<XLDS>(V.f()), V.n};
+    if (ef)
+      e += fm_run<T, XLDS, true, Sk, true>(V.x(), V.n, V.rf, V.coef, items, k0 + lane, k1, 64, sink);
+    else
+      fm_run<T, XLDS, false, Sk, true>(V.x(), V.n, V.rf, V.coef, items, k0 + lane, k1, 64, sink);
+  }
+  return e;
+}
+#else
+template <class T, bool XLDS>
+inline double fm_claims(const View<T> &, const void *, bool) { return 0.0; }
+#endif
 
 // ------------------------------------------------------------------------
 // switching functions
@@ -1802,6 +1851,10 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
     const ItemT *items = (const ItemT *)V.items + V.fm_item_base;
     // the queued (resident) pass gives a lane many items: pipelined loads
     typedef FmSinkForce<XLDS> Sk;
+    if (QUEUED && V.sh_n > 0) {
+      E.fm += fm_claims<T, XLDS>(V, items, ef);
+      return;
+    }
     if (ef)
       E.fm += fm_run<T, XLDS, true, Sk, QUEUED>(V.x(), V.n, V.rf, V.coef, items, t0, nfm, ts, Sk{aw_cast<XLDS>(V.f()), V.n});
     else

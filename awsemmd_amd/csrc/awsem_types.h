@@ -120,6 +120,21 @@ struct FMItemD {
 // (AWSEM_CELL_MIN_N overrides it).  Below it the all-pairs build wins: its
 // pair order keeps neighbouring lanes on neighbouring residues.
 #define AW_CELL_MIN_N 4096
+// Fragment-memory sharing (resident engine): the fragment-memory items of a
+// large replica are cut into chunks of consecutive items, each touching
+// residues [lo, lo + AW_SHARE_SPAN) only.  While the replica's workgroup
+// runs its other passes, workgroups that are done with their own replicas
+// take chunks, evaluate them against a copy of the replica's CA and CB
+// positions and add the forces into the replica's accumulator; the owner
+// takes what is left, waits for the chunks taken elsewhere and adds the
+// accumulator into its forces before it integrates.
+#define AW_SHARE_SPAN 64
+#define AW_SHARE_MAXCH 64      // chunks per replica (one 64-bit claim mask)
+#define AW_SHARE_TAKE 8        // chunks one helping workgroup takes at once (one per wave)
+struct ShareChunk {
+  int k0, k1;                  // items [k0, k1)
+  int lo, pad;                 // first residue of the span
+};
 #define AW_RF_F_VALID 1   // f holds the forces of x: dynamics continue from them
 #define AW_RF_NL_RESET 2  // x was replaced: rebuild the Verlet list first
 struct ReplicaDesc {
@@ -146,4 +161,11 @@ struct ReplicaDesc {
   int nsteps;                  // steps of this replica in this launch
   int rng_id;
   int flags, pad;              // AW_RF_*
+  // fragment-memory sharing (sh_n chunks; 0: the replica evaluates its own)
+  unsigned long long *sh_word; // (energies wanted << 32) | next unclaimed chunk
+  int *sh_done;                // chunks finished elsewhere in the current evaluation
+  const ShareChunk *sh_chunks; // [sh_n]
+  double *sh_x;                // [2][n][3]: CA and CB positions of the current evaluation
+  double *sh_acc;              // [2][n][3] forces of the chunks finished elsewhere, then their energy
+  int sh_n, sh_q;              // chunks; position in the batch's list of sharing replicas
 };

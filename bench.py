@@ -23,7 +23,7 @@ accumulation), whole job, inputs resident in HBM.  Reported beside it:
     oracle-checked every 100 steps; 64 replicas of an 8 x 300-residue
     complex; the serving mix at offered arrival rates of 200-800 jobs/s.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20]
+    python bench.py [--gpus 1] [--steps 640] [--warmup 20]
 """
 from __future__ import annotations
 
@@ -405,7 +405,7 @@ def _gen_multimer(seed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=640)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--chunk", type=int, default=0, help="batch-steps per resident launch (0: autotuned)")
@@ -444,7 +444,7 @@ def main():
     A.lib()
 
     if a.workload == "single":
-        if a.steps == 200:
+        if a.steps == 640:
             a.steps = 10000
         print(json.dumps(run_single_chain(a, A, W)), flush=True)
         return

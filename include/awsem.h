@@ -235,7 +235,7 @@ int awsem_batch_set_skin(awsem_batch *b, double skin);
 int awsem_batch_set_engine(awsem_batch *b, int engine);
 /* Autotuner: times probe_steps of dynamics (from the current state, which
  * is restored afterwards) for each engine, skin in {1, 1.5, 2, 2.5, 3} and
- * steps per launch in {5, 10, 20, 40}; keeps the fastest engine and skin and
+ * steps per launch in {5, 10, 20, 40, 80, 160}; keeps the fastest engine and skin and
  * reports them with the best steps per launch and its ms per batch-step. */
 int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_steps,
                          double *skin, int *steps_per_launch, int *engine, double *ms_per_step);
@@ -244,6 +244,13 @@ int awsem_batch_autotune(awsem_batch *b, const awsem_langevin *lg, int probe_ste
  * replicas whose current list is invalid (overflowed or never built). */
 int awsem_batch_list_stats(const awsem_batch *b, long long *builds, long long *pairs,
                            int *n_invalid);
+/* Diagnostic (no counterpart in the reference, whose force loop is one
+ * process per replica): fragment-memory sharing of the resident engine.
+ * n_shared: replicas whose fragment-memory chunks idle workgroups may take
+ * (0 when sharing is off: AWSEM_SHARE=0, a batch of one replica, or one too
+ * large for LDS); chunks: chunks evaluated by another workgroup than the
+ * replica's own since the batch was created. */
+int awsem_batch_share_stats(const awsem_batch *b, int *n_shared, long long *chunks);
 /* Diagnostic: the production pair selection of one replica as the last
  * evaluation of the batch used it (the same lists the force passes read):
  * kind 0 the Verlet list of residue pairs (a superset of every term's pairs,
