@@ -824,10 +824,15 @@ int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres, 
   s = D.set_schedule(nsteps, lg);
   if (!s.ok()) return fail(c, s);
   (void)hipEventRecord(D.ev0, D.stream);
-  // the wide engine on this one system: Verlet list and work queues, every
-  // phase spread over many workgroups, the step sequence replayed from a
-  // captured graph
-  if (!(s = launch_seq(D, run, 1, nsteps)).ok()) return fail(c, s);
+  // the step sequence replayed from a captured graph, on the phase engine
+  // (all-pairs sweeps, eight launches per step) or, with
+  // AWSEM_RUN_ENGINE=wide, the wide engine (Verlet list and work queues,
+  // eleven launches per step).  For one system the phase engine is the
+  // faster: 12.5k / 7.4k / 2.9k steps/s at 200 / 1000 / 3000 residues
+  // against 9.2k / 5.4k / 2.9k (FP64, MI355X)
+  int eng = 2;
+  if (const char *e = std::getenv("AWSEM_RUN_ENGINE")) eng = std::strcmp(e, "wide") == 0 ? 1 : 2;
+  if (!(s = launch_seq(D, run, eng, nsteps)).ok()) return fail(c, s);
   (void)hipEventRecord(D.ev1, D.stream);
   double e[AW_N_ET];
   if (hipMemcpyAsync(xres, D.d_x, bytes, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
@@ -913,19 +918,21 @@ int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, doubl
 }
 
 // one launch sequence of a device batch on its stream: replayed from a
-// captured HIP graph (captured again when the mode, dt, damp, engine or the
-// wide engine's step count change), or launched directly
+// captured HIP graph (captured again when the mode, dt, damp, engine or,
+// for the wide and phase engines, the step count change), or launched
+// directly
 static AwStatus launch_seq(DevBatch &D, const AwRun &run, int engine, int nst)
 {
   auto launch = [&]() {
+    if (engine == 2) return aw_launch_phases(D.ptrs(), run, D.stream);
     return engine == 1 ? aw_launch_wide(D.ptrs(), run, nst, D.stream) : aw_launch_resident(D.ptrs(), run, D.stream);
   };
   if (!D.use_graph) {
     if (launch()) return AwStatus::err(AWSEM_E_DEVICE, "kernel launch failed");
     return AwStatus();
   }
-  // the wide engine's launch sequence depends on the step count
-  int gkey_steps = engine == 1 ? nst : -1;
+  // the wide and phase engines' launch sequences depend on the step count
+  int gkey_steps = engine >= 1 ? nst : -1;
   if (!D.graph || D.g_mode != run.mode || D.g_dt != run.dt || D.g_damp != run.damp || D.g_engine != engine ||
       D.g_nsteps != gkey_steps) {
     if (D.graph) { (void)hipGraphExecDestroy(D.graph); D.graph = nullptr; }

@@ -12,8 +12,8 @@ A "step" is one Langevin step of every replica (one batch-step).  `value`
 is replica-steps/s of the batched path (FP32 pair arithmetic, FP64
 accumulation), whole job, inputs resident in HBM.  Reported beside it:
   * the plain single-chain FP64 GPU drop-in on the same hardware (one chain
-    at a time through awsem_run: Verlet lists, step sequence replayed from a
-    captured graph), replica-steps/s on a sample of the same chains;
+    at a time through awsem_run: the phase engine's step sequence replayed
+    from a captured graph), replica-steps/s on a sample of the same chains;
   * the max relative force error of the batched path against that FP64
     path (max-norm over atoms);
   * p50 / p99 latency of one batch-step;
@@ -300,8 +300,9 @@ def force_field(A, W, system):
 
 def run_single_chain(a, A, W):
     """Plain FP64 GPU drop-in on one N=1000 chain: `steps` Langevin steps
-    through awsem_run (wide engine with Verlet lists, replayed from a captured
-    graph; one untimed call of `check_every` steps first captures it), and
+    through awsem_run (the phase engine's step sequence replayed from a
+    captured graph; one untimed call of `check_every` steps first captures
+    it), and
     every `check_every` steps the FP64
     per-term energies and total forces at the current coordinates against the
     oracle (per-term forces every 10th check)."""
@@ -604,8 +605,8 @@ def main():
                    "verlet_skin_A": tuned["skin"] if tuned else 2.0, "autotuned": tuned is not None,
                    "engine": tuned["engine"] if tuned else a.engine},
         "plain_fp64_gpu": {"value": plain_value, "unit": "replica-steps/s",
-                           "sample": f"{len(sample)} chains x {a.plain_steps} steps through awsem_run (wide engine, "
-                                     "Verlet lists, captured graph), one chain at a time"},
+                           "sample": f"{len(sample)} chains x {a.plain_steps} steps through awsem_run (phase engine, "
+                                     "captured graph), one chain at a time"},
         "speedup_vs_plain_fp64": value / plain_value,
         "force_err_max_vs_fp64": force_err,
         "latency_ms_per_batch_step": {"p50": float(np.percentile(per_step_ms, 50)),

@@ -1,8 +1,8 @@
 """GPU parity at the benchmark configurations' own shapes, against the FP64
 oracle, per term:
-  c1  one N = 1000 chain through the FP64 drop-in (awsem_run: wide engine,
-      Verlet lists, captured graph), 1000 Langevin steps, every term checked
-      every 100 steps;
+  c1  one N = 1000 chain through the FP64 drop-in (awsem_run: the phase
+      engine's step sequence replayed from a captured graph), 1000 Langevin
+      steps, every term checked every 100 steps;
   c2  chains of 300-400 residues (the upper end of L ~ U[50, 400]) through
       the resident engine with the replica in LDS, FP32 pair arithmetic; and
       the resident engine's global-slab variant, taken when a batch holds a

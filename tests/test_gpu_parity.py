@@ -238,10 +238,12 @@ def test_single_chain_n100_1000_steps_against_oracle(oracle_mod):
     assert dx < 1e-6 and dv < 1e-6 and de < 1e-8
 
 
-def test_plain_fp64_run_checked_along_trajectory(oracle_mod):
-    """The FP64 drop-in (awsem_run, wide engine) over 500 Langevin steps of a
-    300-residue chain; every 100 steps the GPU energies and forces at the
-    trajectory's coordinates against the oracle."""
+@pytest.mark.parametrize("engine", ["phase", "wide"])
+def test_plain_fp64_run_checked_along_trajectory(oracle_mod, engine):
+    """The FP64 drop-in (awsem_run on the phase engine, its default, and on
+    the wide engine) over 500 Langevin steps of a 300-residue chain; every
+    100 steps the GPU energies and forces at the trajectory's coordinates
+    against the oracle."""
     A = _gpu()
     s, p = make_case([300], 81)
     fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
@@ -249,7 +251,11 @@ def test_plain_fp64_run_checked_along_trajectory(oracle_mod):
     x = s.coords()
     v = np.random.default_rng(8).normal(0.0, 0.004, size=x.shape)
     for k in range(5):
-        x, v, f, e = fix.run(x, v, 100, seed=2, step0=100 * k)
+        os.environ["AWSEM_RUN_ENGINE"] = engine
+        try:
+            x, v, f, e = fix.run(x, v, 100, seed=2, step0=100 * k)
+        finally:
+            del os.environ["AWSEM_RUN_ENGINE"]
         assert np.all(np.isfinite(x))
         fg, eg = fix.compute_res(x)
         fo, eo = o.compute(x)
