@@ -215,7 +215,7 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   for (const AwSystem *S : sys) { D->res_off.push_back(res); res += S->n; D->max_n = std::max(D->max_n, S->n); }
   D->total_res = res;
   D->lds = want_lds && D->max_n <= aw_lds_max_residues();
-  if (want_lds) aw_resident_prepare();
+  aw_resident_prepare();
   HIPCHK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&D->ev0));
   HIPCHK(hipEventCreate(&D->ev1));

@@ -67,6 +67,8 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
   V.ch_start = (const AW_G int *)R.ch_start;
   V.ch_len = (const AW_G int *)R.ch_len;
   V.s = slab;
+  V.fb = slab + 18LL * R.n;
+  V.fs = R.n;
   V.ered = nullptr;
   V.P = (ParamPtr)B.P;
   V.fm_items = R.fm_items;

@@ -205,14 +205,23 @@ __global__ void __launch_bounds__(256) k_list(BatchDev B, int r, int kind, int a
 
 #include "awsem_launch.h"
 
-static int g_lds_max = 0;
+static int g_lds_max = 0, g_lds_lim = 0;
+
+int aw_lds_limit_bytes()
+{
+  if (!g_lds_lim) {
+    int dev = 0, lim = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
+    g_lds_lim = lim;
+  }
+  return g_lds_lim;
+}
 
 int aw_lds_max_residues()
 {
   if (!g_lds_max) {
-    int dev = 0, lim = 0;
-    hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
+    int lim = aw_lds_limit_bytes();
     long long avail = (lim - AW_LDS_STATIC_RESERVE) / 8;  // dynamic LDS, doubles
     int n = 1;
     while (slab_doubles(n + 1) <= avail) ++n;
@@ -228,6 +237,8 @@ int aw_launch_resident_f64(const AwBatchPtrs &p, const AwRun &run, hipStream_t s
 
 void aw_resident_prepare_f32();
 void aw_resident_prepare_f64();
+void aw_wide_prepare_f32();
+void aw_wide_prepare_f64();
 
 void aw_resident_prepare()
 {
@@ -235,6 +246,8 @@ void aw_resident_prepare()
   if (done) return;
   aw_resident_prepare_f32();
   aw_resident_prepare_f64();
+  aw_wide_prepare_f32();
+  aw_wide_prepare_f64();
   done = true;
 }
 

@@ -31,9 +31,11 @@ struct AwRun {
 };
 
 int aw_lds_max_residues();
+// bytes of LDS one workgroup may use (the device's limit)
+int aw_lds_limit_bytes();
 long long aw_slab_doubles(int n);
-// raises the resident kernels' dynamic-LDS limit (once per process, before
-// any launch or graph capture)
+// raises the resident and wide kernels' dynamic-LDS limits (once per
+// process, before any launch or graph capture)
 void aw_resident_prepare();
 int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);
 int aw_launch_phases(const AwBatchPtrs &p, const AwRun &run, hipStream_t st);

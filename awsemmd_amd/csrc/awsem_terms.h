@@ -313,6 +313,11 @@ struct View {
   const AW_G int *ch_start;  // [nch] 0-based
   const AW_G int *ch_len;
   double *s;            // working slab (LDS or global): see the accessors
+  // where the pair terms add forces (FA): atom (kind, i) at
+  // fb + 3 (kind fs + i).  The slab's force array (fb = f(), fs = n), or a
+  // workgroup's LDS window holding only the kinds a pass writes
+  double *fb;
+  int fs;
   double *ered;         // energy accumulators [N_EACC] (LDS)
   ParamPtr P;
   // fragment memory
@@ -365,7 +370,7 @@ struct View {
 
 #define XA(V, kind, i) ((V).x() + ((kind) * (V).n + (i)) * 3)
 #define XV(V, kind, i) ((V).xv() + ((kind) * (V).n + (i)) * 3)
-#define FA(V, kind, i) ((V).f() + ((kind) * (V).n + (i)) * 3)
+#define FA(V, kind, i) ((V).fb + ((kind) * (V).fs + (i)) * 3)
 enum { K_CA = 0, K_CB = 1, K_O = 2 };
 enum { K_N = 0, K_H = 1, K_CP = 2 };
 

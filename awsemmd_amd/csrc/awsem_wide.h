@@ -10,6 +10,7 @@
 // memory and local terms), B (list scan into the work queues), B (queues),
 // C, D, E, F, int2.  Kernel boundaries are the barriers between phases.
 #pragma once
+#include <cstdlib>
 #include "awsem_dev.h"
 
 // replica of this block; false when it has no work in evaluation s
@@ -367,25 +368,97 @@ __device__ __forceinline__ void w_scan(const View<T> &V)
 }
 W_KERNEL(k_w_B2, w_scan(V))
 
+// the queue passes, one launch per queue: DSSP (q = 0), P_AP (1),
+// excluded volume (2)
 template <class T>
-__device__ __forceinline__ void w_queues(const View<T> &V)
+__device__ __forceinline__ void w_queue(const View<T> &V, int q)
 {
   EAcc E;
   E.clear();
   if (V.wq && V.nl_state[7] > 0) {
     int cap = V.nl_cap;
-    const AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap;
-    int nd = V.nl_state[2], npap = V.nl_state[3], nev = V.nl_state[4];
-    for_packed(qd, 0, nd, W_TID, W_NT, [&](int i, int j) { dssp_term(V, i, j, E); });
-    for_packed(qp, 0, npap, W_TID, W_NT, [&](int i, int j) { pap_term(V, i, j, E); });
-    for_packed(qe, 0, nev, W_TID, W_NT, [&](int i, int j) { ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E); });
+    if (q == 0)
+      for_packed((const AW_G int *)V.wq, 0, V.nl_state[2], W_TID, W_NT, [&](int i, int j) { dssp_term(V, i, j, E); });
+    else if (q == 1)
+      for_packed((const AW_G int *)V.wq + 2 * cap, 0, V.nl_state[3], W_TID, W_NT,
+                 [&](int i, int j) { pap_term(V, i, j, E); });
+    else
+      for_packed((const AW_G int *)V.wq + 3 * cap, 0, V.nl_state[4], W_TID, W_NT,
+                 [&](int i, int j) { ev_pair(V, i, j, V.rinfo[i], V.rinfo[j], E); });
   }
   eflush(E, V.ered);
 }
-W_KERNEL(k_w_B3, w_queues(V))
+W_KERNEL(k_w_B3d, w_queue(V, 0))
+W_KERNEL(k_w_B3p, w_queue(V, 1))
+W_KERNEL(k_w_B3e, w_queue(V, 2))
 W_KERNEL(k_w_C, phase_C(V, W_TID, W_NT))
 W_KERNEL(k_w_D, phase_D(V, W_TID, W_NT))
 W_KERNEL(k_w_F, phase_F(V, W_TID, W_NT))
+
+// ---- force windows.  A global FP64 atomic add is performed at the memory
+// side (device-coherent across the XCDs' L2 caches), at about 2e10 adds/s
+// for the whole GPU; the DSSP, P_AP and density-pair force passes issue
+// about 36, 14 and 20 of them per item, which bounds them.  Their windowed
+// form gives each workgroup (1024 threads, a few per replica) an LDS copy
+// of the two atom kinds the pass writes, adds into it with LDS atomics and
+// adds the window into the replica's forces once: q = 0, the DSSP and P_AP
+// queues (CA and O forces only; O rows start at 2 fs with fs = ceil(n / 2),
+// so the two kinds fill 6 fs + 3 n doubles); q = 1, the density-pair forces
+// of water, burial and helix (CA and CB: site atoms only, 6 n doubles).
+#define W_WIN_NT 1024
+
+__host__ __device__ inline int w_win_fs(int n, int q) { return q == 0 ? (n + 1) / 2 : n; }
+__host__ __device__ inline int w_win_doubles(int n, int q) { return q == 0 ? 6 * w_win_fs(n, 0) + 3 * n : 6 * n; }
+
+template <class T>
+__global__ void __launch_bounds__(W_WIN_NT) k_w_win(BatchDev B, int s, int q)
+{
+  extern __shared__ double win[];
+  __shared__ double ered[N_EACC];
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  int n = R.n;
+  int fs = w_win_fs(n, q), wn = w_win_doubles(n, q);
+  for (int k = threadIdx.x; k < wn; k += blockDim.x) win[k] = 0.0;
+  if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
+  __syncthreads();
+  V.ered = ered;
+  AW_G double *g = (AW_G double *)V.f();  // the replica's force array [3][n][3]
+  V.fb = win;
+  V.fs = fs;
+  if (q == 0) {
+    EAcc E;
+    E.clear();
+    if (V.wq && V.nl_state[7] > 0) {
+      int cap = V.nl_cap;
+      for_packed((const AW_G int *)V.wq, 0, V.nl_state[2], W_TID, W_NT, [&](int i, int j) { dssp_term(V, i, j, E); });
+      for_packed((const AW_G int *)V.wq + 2 * cap, 0, V.nl_state[3], W_TID, W_NT,
+                 [&](int i, int j) { pap_term(V, i, j, E); });
+    }
+    eflush(E, V.ered);
+  } else {
+    phase_F(V, W_TID, W_NT);
+  }
+  __syncthreads();
+  // the window into the replica's forces: CA rows, then O (q = 0) or CB rows
+  int o2 = q == 0 ? 6 * fs : 3 * n, k2 = q == 0 ? 6 * n : 3 * n;
+  for (int k = threadIdx.x; k < 6 * n; k += blockDim.x) {
+    double v = k < 3 * n ? win[k] : win[o2 + k - 3 * n];
+    int gk = k < 3 * n ? k : k2 + k - 3 * n;
+    if (v != 0.0) __hip_atomic_fetch_add(g + gk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  energy_add(ered, R.energy);
+}
+
+template <class T>
+static void prepare_wide_t()
+{
+  int dev = 0, lim = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
+  (void)hipFuncSetAttribute((const void *)k_w_win<T>, hipFuncAttributeMaxDynamicSharedMemorySize, lim - 512);
+}
 
 template <class T>
 __global__ void __launch_bounds__(256) k_w_int1(BatchDev B, LgParams lg, int s)
@@ -405,8 +478,15 @@ __global__ void __launch_bounds__(256) k_w_int2(BatchDev B, LgParams lg, int s)
   integrate2(V, (AW_G double *)R.v, lg, R, s, W_TID, W_NT);
 }
 
+// the windowed passes' launch: wg workgroups per replica, lds bytes of
+// window (0: the direct passes, for replicas too large for the window)
+struct WWin {
+  int wg;
+  size_t lds;
+};
+
 template <class T>
-static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s)
+static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s, const WWin &w)
 {
   dim3 g1(1, g.y);
   hipLaunchKernelGGL(k_w_A<T>, g, dim3(256), 0, st, B, s);
@@ -419,10 +499,20 @@ static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s)
   hipLaunchKernelGGL(k_w_nl_publish<T>, g1, dim3(64), 0, st, B, s);
   hipLaunchKernelGGL(k_w_B1<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_B2<T>, g, dim3(256), 0, st, B, s);
-  hipLaunchKernelGGL(k_w_B3<T>, g, dim3(256), 0, st, B, s);
+  dim3 gw(w.wg, g.y);
+  if (w.lds) {
+    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds, st, B, s, 0);
+  } else {
+    hipLaunchKernelGGL(k_w_B3d<T>, g, dim3(256), 0, st, B, s);
+    hipLaunchKernelGGL(k_w_B3p<T>, g, dim3(256), 0, st, B, s);
+  }
+  hipLaunchKernelGGL(k_w_B3e<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_C<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_D<T>, g, dim3(256), 0, st, B, s);
-  hipLaunchKernelGGL(k_w_F<T>, g, dim3(256), 0, st, B, s);
+  if (w.lds)
+    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds, st, B, s, 1);
+  else
+    hipLaunchKernelGGL(k_w_F<T>, g, dim3(256), 0, st, B, s);
 }
 
 // K steps of every replica (mode 1), or one force evaluation (mode 0);
@@ -436,16 +526,28 @@ static int launch_wide_t(const AwBatchPtrs &p, const AwRun &run, int max_steps, 
   int bx = run.grid > 0 ? run.grid : (2048 + p.nrep - 1) / p.nrep;
   bx = bx < 1 ? 1 : (bx > 64 ? 64 : bx);
   dim3 g(bx, p.nrep);
+  // windowed passes: about one 1024-thread workgroup per CU in total, at
+  // most 16 per replica (each adds its whole window once); replicas whose
+  // window exceeds the LDS take the direct passes (AWSEM_WIDE_WINDOW=0:
+  // always)
+  WWin w;
+  w.wg = (256 + p.nrep - 1) / p.nrep;
+  w.wg = w.wg < 1 ? 1 : (w.wg > 16 ? 16 : w.wg);
+  w.lds = (size_t)(w_win_doubles(p.max_n, 0) > w_win_doubles(p.max_n, 1) ? w_win_doubles(p.max_n, 0)
+                                                                          : w_win_doubles(p.max_n, 1)) * sizeof(double);
+  if (w.lds > (size_t)aw_lds_limit_bytes() - 512) w.lds = 0;
+  if (const char *e = getenv("AWSEM_WIDE_WINDOW"))
+    if (atoi(e) == 0) w.lds = 0;
   hipLaunchKernelGGL(k_w_begin<T>, g, dim3(256), 0, st, B, run.mode);
   if (run.mode == 1) {
-    w_eval<T>(B, g, st, -1);
+    w_eval<T>(B, g, st, -1, w);
     for (int s = 0; s < max_steps; ++s) {
       hipLaunchKernelGGL(k_w_int1<T>, g, dim3(256), 0, st, B, lg, s);
-      w_eval<T>(B, g, st, s);
+      w_eval<T>(B, g, st, s, w);
       hipLaunchKernelGGL(k_w_int2<T>, g, dim3(256), 0, st, B, lg, s);
     }
   } else {
-    w_eval<T>(B, g, st, -2);
+    w_eval<T>(B, g, st, -2, w);
   }
   hipLaunchKernelGGL(k_w_end<T>, g, dim3(256), 0, st, B, run.mode);
   return hipGetLastError() == hipSuccess ? 0 : -1;

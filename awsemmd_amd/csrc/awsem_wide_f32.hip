@@ -5,3 +5,8 @@ int aw_launch_wide_f32(const AwBatchPtrs &p, const AwRun &run, int max_steps, hi
 {
   return launch_wide_t<float>(p, run, max_steps, st);
 }
+
+void aw_wide_prepare_f32()
+{
+  prepare_wide_t<float>();
+}

@@ -5,3 +5,8 @@ int aw_launch_wide_f64(const AwBatchPtrs &p, const AwRun &run, int max_steps, hi
 {
   return launch_wide_t<double>(p, run, max_steps, st);
 }
+
+void aw_wide_prepare_f64()
+{
+  prepare_wide_t<double>();
+}
