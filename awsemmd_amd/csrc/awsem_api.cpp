@@ -204,6 +204,26 @@ static AwStatus setup_share(DevBatch &D, const std::vector<std::vector<int>> &ij
   return AwStatus();
 }
 
+// two 64-bit hashes of a byte range (8-byte words; FNV-style multiply-xor
+// with different primes and seeds): the key under which identical
+// fragment-memory libraries are found
+static void bytes_hash2(const unsigned char *p, size_t nb, uint64_t &h1, uint64_t &h2)
+{
+  h1 = 1469598103934665603ULL;
+  h2 = 0x9E3779B97F4A7C15ULL ^ nb;
+  size_t k = 0;
+  for (; k + 8 <= nb; k += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + k, 8);
+    h1 = (h1 ^ w) * 1099511628211ULL;
+    h2 = (h2 ^ (w + (h2 >> 29))) * 0xFF51AFD7ED558CCDULL;
+  }
+  for (; k < nb; ++k) {
+    h1 = (h1 ^ p[k]) * 1099511628211ULL;
+    h2 = (h2 ^ (p[k] + (h2 >> 29))) * 0xFF51AFD7ED558CCDULL;
+  }
+}
+
 static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int precision, bool want_lds,
                                std::unique_ptr<DevBatch> &out)
 {
@@ -229,6 +249,11 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMalloc((void **)&D->d_e, (size_t)D->nrep * AW_N_ET * sizeof(double)));
   HIPCHK(hipMemset(D->d_e, 0, (size_t)D->nrep * AW_N_ET * sizeof(double)));
   std::vector<std::vector<int>> fm_ij(D->nrep);
+  // replicas of one system (the same fragment-memory library, byte for
+  // byte) read a single copy of its tables: one stream of them through the
+  // caches instead of one per replica
+  struct FmLib { size_t bytes; uint64_t h1, h2; int n, rep; };
+  std::vector<FmLib> libs;
   for (int r = 0; r < D->nrep; ++r) {
     const AwSystem &S = *sys[r];
     AwDevPlan plan;
@@ -236,9 +261,22 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
     // replicas too large for LDS, work from it
     AwStatus s = aw_devsys_plan(S, nullptr, D->fp32, want_lds || !D->lds, false, plan);
     if (!s.ok()) return s;
+    int src = -1;
+    FmLib lib{plan.off_nl - plan.off_items, 0, 0, S.n, r};
+    if (plan.n_items > 0) {
+      bytes_hash2(plan.image.data() + plan.off_items, lib.bytes, lib.h1, lib.h2);
+      for (const FmLib &L : libs)
+        if (L.bytes == lib.bytes && L.h1 == lib.h1 && L.h2 == lib.h2 && L.n == lib.n) { src = L.rep; break; }
+      if (src < 0) libs.push_back(lib);
+    }
     std::unique_ptr<AwDevSystem> ds(new AwDevSystem());
     if (!(s = aw_devsys_create(plan, *ds)).ok()) return s;
     ReplicaDesc R = ds->d;
+    if (src >= 0) {
+      R.items = D->reps[src].items;
+      R.rf = D->reps[src].rf;
+      R.coef = D->reps[src].coef;
+    }
     R.x = D->d_x + 9LL * D->res_off[r];
     R.v = D->d_v + 9LL * D->res_off[r];
     R.f = D->d_f + 9LL * D->res_off[r];

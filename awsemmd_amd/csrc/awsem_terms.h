@@ -54,6 +54,7 @@ inline void q_reserve4(int *ctr, const int n[4], int b[4])
 {
   for (int k = 0; k < 4; ++k) { b[k] = ctr[k]; ctr[k] += n[k]; }
 }
+inline int q_reserve1(int *ctr, int n) { int b = *ctr; *ctr += n; return b; }
 #else
 #include <hip/hip_runtime.h>
 #define AW_DEV __device__ __forceinline__
@@ -190,6 +191,14 @@ __device__ __forceinline__ void q_reserve4(int *ctr, const int n[4], int b[4])
   b[1] = __builtin_amdgcn_readfirstlane((int)(unsigned)(o0 >> 32));
   b[2] = __builtin_amdgcn_readfirstlane((int)(unsigned)o1);
   b[3] = __builtin_amdgcn_readfirstlane((int)(unsigned)(o1 >> 32));
+}
+// n (wave-uniform) slots of one queue: one atomic per wave
+__device__ __forceinline__ int q_reserve1(int *ctr, int n)
+{
+  int first = __ffsll((unsigned long long)__ballot(1)) - 1;
+  int o = 0;
+  if (__lane_id() == first) o = atomicAdd(ctr, n);
+  return __builtin_amdgcn_readfirstlane(o);
 }
 #endif
 
@@ -1580,9 +1589,13 @@ AW_DEV void phase_nl_build(const View<T> &V, int t0, int ts)
   for (int k = t0; k < np; k += ts) {
     int i, j;
     pair_decode(k, n, i, j);
-    if (near_pair(V, i, j, reach)) {
-      int q = aadd_i(&V.nl_state[1], 1);
-      if (q < V.nl_cap) V.nl[AW_NL_HDR + q] = (i << 16) | j;
+    // the kept pairs of a wave reserve their slots with one atomic
+    bool keep = near_pair(V, i, j, reach);
+    lmask m = w_ballot(keep);
+    if (m) {
+      int base = q_reserve1(&V.nl_state[1], w_count(m));
+      int q = base + w_rank(m);
+      if (keep && q < V.nl_cap) V.nl[AW_NL_HDR + q] = (i << 16) | j;
     }
   }
   for (int i = t0; i < n; i += ts) {
