@@ -1745,9 +1745,22 @@ AW_DEV void residue_local(const View<T> &V, int i, EAcc &E)
 // pass their cutoffs are queued and evaluated afterwards in dense passes;
 // otherwise they are evaluated in place.  A template argument, so the scan
 // loop of the queued pass carries no in-place term code.
-template <class T, bool Q = false>
+// where the list scan adds the densities rho_w (w = 0) and rho_h (w = 1):
+// the slab's per-residue scalars, or (DensWin) a workgroup's LDS window of
+// the two, rho_w of residue i at win[2 i], rho_h at win[2 i + 1]
+struct DensAdd {
+  template <class VT>
+  AW_DEV void operator()(const VT &V, int w, int i, double v) const { aadd(w == 0 ? &V.rho_w()[i] : &V.rho_h()[i], v); }
+};
+struct DensWin {
+  double *win;
+  template <class VT>
+  AW_DEV void operator()(const VT &, int w, int i, double v) const { aadd(win + 2 * i + w, v); }
+};
+
+template <class T, bool Q = false, class DS = DensAdd>
 AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullptr, AW_G int *qp = nullptr,
-                   AW_G int *qe = nullptr, AW_G int *qs = nullptr, int *hold = nullptr)
+                   AW_G int *qe = nullptr, AW_G int *qs = nullptr, int *hold = nullptr, DS dens = DS())
 {
   ParamPtr P = V.P;
   // every parameter the scan reads, loaded before any branch: invariant and
@@ -1808,14 +1821,14 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
         if (wc && rsq > wmin2 && rsq < wmax2) {
           T th, prd;
           well_theta<T>(wk, wr0, wr1, sqrt_t<T>(rsq), th, prd);
-          aadd(&V.rho_w()[i], (double)th);
-          aadd(&V.rho_w()[j], (double)th);
+          dens(V, 0, i, (double)th);
+          dens(V, 0, j, (double)th);
         }
         if (hc && rsq > hmin2 && rsq < hmax2) {
           T th, prd;
           well_theta<T>(hk, hr0, hr1, sqrt_t<T>(rsq), th, prd);
-          aadd(&V.rho_h()[i], (double)th);
-          aadd(&V.rho_h()[j], (double)th);
+          dens(V, 1, i, (double)th);
+          dens(V, 1, j, (double)th);
         }
       }
     }
@@ -2080,8 +2093,20 @@ AW_DEV WellP well_p(ParamPtr P, int w)
   return q;
 }
 
-template <class T>
-AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
+// where the water-xi pass adds xi_w: the slab's per-residue scalar, or
+// (XiWin) a workgroup's LDS window, xi_w of residue i at win[i]
+struct XiAdd {
+  template <class VT>
+  AW_DEV void operator()(const VT &V, int i, double v) const { aadd(&V.xi_w()[i], v); }
+};
+struct XiWin {
+  double *win;
+  template <class VT>
+  AW_DEV void operator()(const VT &, int i, double v) const { aadd(win + i, v); }
+};
+
+template <class T, class XS = XiAdd>
+AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued, XS xs = XS())
 {
   ParamPtr P = V.P;
   // parameters read before any branch (invariant loads leave the loop)
@@ -2115,12 +2140,12 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued)
       acc_j += tg * swi;
     }
   }
-  if (acc_i != 0.0) aadd(&V.xi_w()[i], acc_i);
-  if (acc_j != 0.0) aadd(&V.xi_w()[j], acc_j);
+  if (acc_i != 0.0) xs(V, i, acc_i);
+  if (acc_j != 0.0) xs(V, j, acc_j);
 }
 
-template <class T>
-AW_DEV void phase_D(const View<T> &V, int t0, int ts, bool ef = true)
+template <class T, class XS = XiAdd>
+AW_DEV void phase_D(const View<T> &V, int t0, int ts, bool ef = true, XS xs = XS())
 {
   EAcc E;
   E.clear();
@@ -2128,7 +2153,7 @@ AW_DEV void phase_D(const View<T> &V, int t0, int ts, bool ef = true)
   int n = V.n;
   int nh = P->helix_flag ? n : 0;
   for (int k = t0; k < nh; k += ts) helix_item(V, k, E);
-  if (P->water_flag) for_density_pairs(V, t0, ts, [&](int i, int j, bool q) { xi_pair(V, i, j, q); });
+  if (P->water_flag) for_density_pairs(V, t0, ts, [&](int i, int j, bool q) { xi_pair(V, i, j, q, xs); });
   if (ef) eflush(E, V.ered);  // energies only where reported
 }
 

@@ -353,11 +353,12 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
 // into their queues and the density pairs for D and F; without a valid
 // list every pair is visited and evaluated in place
 template <class T>
-__device__ __forceinline__ void w_scan(const View<T> &V)
+__device__ __forceinline__ void w_scan(const View<T> &V, bool listless_only)
 {
   EAcc E;
   E.clear();
   if (V.wq && V.nl_state[7] > 0) {
+    if (listless_only) return;  // the windowed scan takes this replica
     int cap = V.nl_cap;
     AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
     for_pairs(V, W_TID, W_NT, [&](int i, int j) { pair_B<T, true>(V, i, j, E, qd, qp, qe, qs); });
@@ -366,7 +367,22 @@ __device__ __forceinline__ void w_scan(const View<T> &V)
   }
   eflush(E, V.ered);
 }
-W_KERNEL(k_w_B2, w_scan(V))
+// without windows, every replica; with them (lo = 1), the replicas whose
+// list is not valid (every pair evaluated in place)
+template <class T>
+__global__ void __launch_bounds__(256) k_w_B2(BatchDev B, int s, int lo)
+{
+  __shared__ double ered[N_EACC];
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  V.ered = ered;
+  if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
+  __syncthreads();
+  w_scan(V, lo != 0);
+  __syncthreads();
+  energy_add(ered, R.energy);
+}
 
 // the queue passes, one launch per queue: DSSP (q = 0), P_AP (1),
 // excluded volume (2)
@@ -404,11 +420,17 @@ W_KERNEL(k_w_F, phase_F(V, W_TID, W_NT))
 // adds the window into the replica's forces once: q = 0, the DSSP and P_AP
 // queues (CA and O forces only; O rows start at 2 fs with fs = ceil(n / 2),
 // so the two kinds fill 6 fs + 3 n doubles); q = 1, the density-pair forces
-// of water, burial and helix (CA and CB: site atoms only, 6 n doubles).
+// of water, burial and helix (CA and CB: site atoms only, 6 n doubles);
+// q = 2, the list scan's densities rho_w and rho_h (2 n doubles); q = 3,
+// phase D: helix forces (CA and O, as q = 0) and water xi (n doubles after
+// them).
 #define W_WIN_NT 1024
 
-__host__ __device__ inline int w_win_fs(int n, int q) { return q == 0 ? (n + 1) / 2 : n; }
-__host__ __device__ inline int w_win_doubles(int n, int q) { return q == 0 ? 6 * w_win_fs(n, 0) + 3 * n : 6 * n; }
+__host__ __device__ inline int w_win_fs(int n, int q) { return (q == 0 || q == 3) ? (n + 1) / 2 : n; }
+__host__ __device__ inline int w_win_doubles(int n, int q)
+{
+  return q == 0 ? 6 * w_win_fs(n, 0) + 3 * n : q == 1 ? 6 * n : q == 2 ? 2 * n : 6 * w_win_fs(n, 3) + 4 * n;
+}
 
 template <class T>
 __global__ void __launch_bounds__(W_WIN_NT) k_w_win(BatchDev B, int s, int q)
@@ -419,6 +441,9 @@ __global__ void __launch_bounds__(W_WIN_NT) k_w_win(BatchDev B, int s, int q)
   if (!w_active(R, s)) return;
   View<T> V = w_view<T>(B, R);
   int n = R.n;
+  // a replica without a valid list: its scan is the direct kernel's
+  // (k_w_B2, launched beside this one, takes exactly those replicas)
+  if (q == 2 && !(V.wq && V.nl_state[7] > 0)) return;
   int fs = w_win_fs(n, q), wn = w_win_doubles(n, q);
   for (int k = threadIdx.x; k < wn; k += blockDim.x) win[k] = 0.0;
   if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
@@ -437,16 +462,50 @@ __global__ void __launch_bounds__(W_WIN_NT) k_w_win(BatchDev B, int s, int q)
                  [&](int i, int j) { pap_term(V, i, j, E); });
     }
     eflush(E, V.ered);
-  } else {
+  } else if (q == 1) {
     phase_F(V, W_TID, W_NT);
+  } else if (q == 2) {
+    // the launcher takes this form only for a valid list (queued scan: the
+    // densities are its only in-place sums)
+    EAcc E;
+    E.clear();
+    int cap = V.nl_cap;
+    AW_G int *qd = V.wq, *qp = V.wq + 2 * cap, *qe = V.wq + 3 * cap, *qs = V.wq + 4 * cap;
+    DensWin dw{win};
+    for_pairs(V, W_TID, W_NT,
+              [&](int i, int j) { pair_B<T, true, DensWin>(V, i, j, E, qd, qp, qe, qs, nullptr, dw); });
+    eflush(E, V.ered);
+  } else {
+    phase_D(V, W_TID, W_NT, true, XiWin{win + 6 * fs + 3 * n});
   }
   __syncthreads();
-  // the window into the replica's forces: CA rows, then O (q = 0) or CB rows
-  int o2 = q == 0 ? 6 * fs : 3 * n, k2 = q == 0 ? 6 * n : 3 * n;
-  for (int k = threadIdx.x; k < 6 * n; k += blockDim.x) {
-    double v = k < 3 * n ? win[k] : win[o2 + k - 3 * n];
-    int gk = k < 3 * n ? k : k2 + k - 3 * n;
-    if (v != 0.0) __hip_atomic_fetch_add(g + gk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (q == 2) {
+    // rho_w, rho_h into the slab's per-residue scalars
+    AW_G double *sc = (AW_G double *)(V.s + 27LL * n);
+    for (int k = threadIdx.x; k < 2 * n; k += blockDim.x) {
+      double v = win[k];
+      if (v != 0.0) __hip_atomic_fetch_add(sc + AW_SC_STRIDE * (k >> 1) + (k & 1), v, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else {
+    // the window into the replica's forces: CA rows, then O (q = 0, 3) or
+    // CB rows (q = 1)
+    bool ca_o = q != 1;
+    int o2 = ca_o ? 6 * fs : 3 * n, k2 = ca_o ? 6 * n : 3 * n;
+    for (int k = threadIdx.x; k < 6 * n; k += blockDim.x) {
+      double v = k < 3 * n ? win[k] : win[o2 + k - 3 * n];
+      int gk = k < 3 * n ? k : k2 + k - 3 * n;
+      if (v != 0.0) __hip_atomic_fetch_add(g + gk, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (q == 3) {
+      // water xi into the slab's per-residue scalars
+      AW_G double *xi = (AW_G double *)(V.s + 27LL * n + 6);
+      const double *xw = win + 6 * fs + 3 * n;
+      for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double v = xw[k];
+        if (v != 0.0) __hip_atomic_fetch_add(xi + AW_SC_STRIDE * k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
   energy_add(ered, R.energy);
 }
@@ -478,11 +537,11 @@ __global__ void __launch_bounds__(256) k_w_int2(BatchDev B, LgParams lg, int s)
   integrate2(V, (AW_G double *)R.v, lg, R, s, W_TID, W_NT);
 }
 
-// the windowed passes' launch: wg workgroups per replica, lds bytes of
-// window (0: the direct passes, for replicas too large for the window)
+// the windowed passes' launch: wg workgroups per replica, lds[q] bytes of
+// window for pass q (0: the direct kernel, for replicas too large for it)
 struct WWin {
   int wg;
-  size_t lds;
+  size_t lds[4];
 };
 
 template <class T>
@@ -498,19 +557,23 @@ static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s, const WWin 
   hipLaunchKernelGGL(k_w_cell_pairs<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_nl_publish<T>, g1, dim3(64), 0, st, B, s);
   hipLaunchKernelGGL(k_w_B1<T>, g, dim3(256), 0, st, B, s);
-  hipLaunchKernelGGL(k_w_B2<T>, g, dim3(256), 0, st, B, s);
   dim3 gw(w.wg, g.y);
-  if (w.lds) {
-    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds, st, B, s, 0);
+  hipLaunchKernelGGL(k_w_B2<T>, g, dim3(256), 0, st, B, s, w.lds[2] ? 1 : 0);
+  if (w.lds[2]) hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds[2], st, B, s, 2);
+  if (w.lds[0]) {
+    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds[0], st, B, s, 0);
   } else {
     hipLaunchKernelGGL(k_w_B3d<T>, g, dim3(256), 0, st, B, s);
     hipLaunchKernelGGL(k_w_B3p<T>, g, dim3(256), 0, st, B, s);
   }
   hipLaunchKernelGGL(k_w_B3e<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_C<T>, g, dim3(256), 0, st, B, s);
-  hipLaunchKernelGGL(k_w_D<T>, g, dim3(256), 0, st, B, s);
-  if (w.lds)
-    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds, st, B, s, 1);
+  if (w.lds[3])
+    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds[3], st, B, s, 3);
+  else
+    hipLaunchKernelGGL(k_w_D<T>, g, dim3(256), 0, st, B, s);
+  if (w.lds[1])
+    hipLaunchKernelGGL(k_w_win<T>, gw, dim3(W_WIN_NT), w.lds[1], st, B, s, 1);
   else
     hipLaunchKernelGGL(k_w_F<T>, g, dim3(256), 0, st, B, s);
 }
@@ -533,11 +596,12 @@ static int launch_wide_t(const AwBatchPtrs &p, const AwRun &run, int max_steps, 
   WWin w;
   w.wg = (256 + p.nrep - 1) / p.nrep;
   w.wg = w.wg < 1 ? 1 : (w.wg > 16 ? 16 : w.wg);
-  w.lds = (size_t)(w_win_doubles(p.max_n, 0) > w_win_doubles(p.max_n, 1) ? w_win_doubles(p.max_n, 0)
-                                                                          : w_win_doubles(p.max_n, 1)) * sizeof(double);
-  if (w.lds > (size_t)aw_lds_limit_bytes() - 512) w.lds = 0;
-  if (const char *e = getenv("AWSEM_WIDE_WINDOW"))
-    if (atoi(e) == 0) w.lds = 0;
+  bool win_on = true;
+  if (const char *e = getenv("AWSEM_WIDE_WINDOW")) win_on = atoi(e) != 0;
+  for (int q = 0; q < 4; ++q) {
+    w.lds[q] = (size_t)w_win_doubles(p.max_n, q) * sizeof(double);
+    if (!win_on || w.lds[q] > (size_t)aw_lds_limit_bytes() - 512) w.lds[q] = 0;
+  }
   hipLaunchKernelGGL(k_w_begin<T>, g, dim3(256), 0, st, B, run.mode);
   if (run.mode == 1) {
     w_eval<T>(B, g, st, -1, w);
