@@ -278,24 +278,34 @@ struct FmSinkLds {
   }
 };
 
-#define W_FM_SPAN 640  // residues of the block-local force window (30 KB of LDS)
+#define W_FM_SPAN 640  // residues of the block-local force window (3 kinds: 45 KB of LDS)
 
-// fragment memory: each block takes a contiguous range of the replica's
-// items (ordered by window), so its forces fall on a short residue window
-// accumulated in LDS and written to the replica's forces once per block;
-// then the local terms
+// fragment memory and the local terms: each block takes a contiguous range
+// of the replica's fragment-memory items (ordered by window) and the local
+// terms of a contiguous range of residues, so its forces fall on a short
+// residue window [wlo, wlo + span) of all three atom kinds, accumulated in
+// LDS and added into the replica's forces once per block (a global FP64
+// atomic is performed at the memory side; the local terms alone issue 18
+// per residue).  A block whose window would exceed W_FM_SPAN adds directly.
 template <class T>
 __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
 {
   __shared__ double ered[N_EACC];
-  __shared__ double fl[2 * W_FM_SPAN * 3];
+  __shared__ double fl[3 * W_FM_SPAN * 3];
   __shared__ int lim[2];
   const ReplicaDesc &R = B.rep[blockIdx.y];
   if (!w_active(R, s)) return;
   View<T> V = w_view<T>(B, R);
   V.ered = ered;
+  int n = V.n;
+  // the block's residues for the local terms, whose forces reach one
+  // residue either side
+  int o0 = (int)((long long)n * blockIdx.x / gridDim.x), o1 = (int)((long long)n * (blockIdx.x + 1) / gridDim.x);
   if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
-  if (threadIdx.x == 0) { lim[0] = 1 << 30; lim[1] = -1; }
+  if (threadIdx.x == 0) {
+    lim[0] = o0 < o1 ? (o0 > 0 ? o0 - 1 : 0) : 1 << 30;
+    lim[1] = o0 < o1 ? (o1 < n ? o1 : n - 1) : -1;
+  }
   __syncthreads();
   EAcc E;
   E.clear();
@@ -311,10 +321,10 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
     atomicMax(&lim[1], ij & 0xffff);
   }
   __syncthreads();
-  int rlo = lim[0], span = lim[1] - lim[0] + 1;
-  bool local = k0 < k1 && span <= W_FM_SPAN;
+  int wlo = lim[0], span = lim[1] - lim[0] + 1;
+  bool local = span > 0 && span <= W_FM_SPAN;
   if (local)
-    for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) fl[k] = 0.0;
+    for (int k = threadIdx.x; k < 9 * span; k += blockDim.x) fl[k] = 0.0;
   __syncthreads();
   // energies leave a launch from the replica's last evaluation only
   bool ef = s < 0 || s == R.nsteps - 1;
@@ -325,25 +335,32 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   const int W_FMB = sizeof(T) == 4 ? 8 : 4;
   if (local) {
     typedef FmSinkLds Sk;
-    Sk sink{fl, rlo, span};
+    Sk sink{fl, wlo, span};
     if (ef) E.fm += fm_run_body<T, false, true, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
     else fm_run_body<T, false, false, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
+    // the local terms add through the view into the same window: atom
+    // (kind, i) at fl + 3 (kind span + i - wlo), i.e. a force base 3 wlo
+    // doubles before the window (formed as an integer: the base itself lies
+    // outside it, every address used lies inside)
+    V.fb = (double *)((uintptr_t)fl - (uintptr_t)wlo * 3 * sizeof(double));
+    V.fs = span;
   } else {
     typedef FmSinkForce<false> Sk;
     Sk sink{aw_cast<false>(V.f()), V.n};
     if (ef) E.fm += fm_run_body<T, false, true, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
     else fm_run_body<T, false, false, Sk, false, W_FMB>(V.x(), V.n, V.rf, V.coef, items, kk, k1, ks, sink);
   }
+  for (int k = o0 + (int)threadIdx.x; k < o1; k += blockDim.x) residue_local(V, k, E);
   __syncthreads();
   if (local)
-    for (int k = threadIdx.x; k < 6 * span; k += blockDim.x) {
+    for (int k = threadIdx.x; k < 9 * span; k += blockDim.x) {
       double v = fl[k];
       if (v != 0.0) {
         int kind = k / (3 * span), rem = k - kind * 3 * span;
-        atomicAdd(&V.f()[((size_t)kind * V.n + rlo) * 3 + rem], v);
+        __hip_atomic_fetch_add(V.f() + ((size_t)kind * n + wlo) * 3 + rem, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-  for (int k = W_TID; k < V.n; k += W_NT) residue_local(V, k, E);
   eflush(E, V.ered);
   __syncthreads();
   energy_add(ered, R.energy);
