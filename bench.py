@@ -21,7 +21,8 @@ accumulation), whole job, inputs resident in HBM.  Reported beside it:
   * the CPU oracle (FP64 restatement, 1 core) on a bounded sample;
   * the other workloads, bounded: one N=1000 chain through the FP64 drop-in,
     oracle-checked every 100 steps; 64 replicas of an 8 x 300-residue
-    complex; the serving mix at offered arrival rates of 200-800 jobs/s.
+    complex; the serving mix at offered arrival rates of 200-1600 jobs/s
+    (below and past saturation).
 
     python bench.py [--gpus 1] [--steps 640] [--warmup 20]
 """
@@ -418,9 +419,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="CPU seconds of the host baseline, all cores")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--serving-duration", type=float, default=6.0,
+    ap.add_argument("--serving-duration", type=float, default=5.0,
                     help="seconds of arrivals per offered rate of the serving mix (0: skip)")
-    ap.add_argument("--serving-rates", default="200,400,600,800",
+    ap.add_argument("--serving-rates", default="200,400,800,1200,1600",
                     help="offered Poisson arrival rates of the serving mix (jobs/s), comma-separated")
     ap.add_argument("--single-steps", type=int, default=500,
                     help="steps of the N=1000 FP64 drop-in line in the default run, oracle-checked every 100 (0: skip)")
