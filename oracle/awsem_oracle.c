@@ -95,6 +95,14 @@ struct or_ctx {
   int n_mems;
   or_mem *mems;
   int *ilen_fm_map, **frag_mem_map;
+  /* fragment-memory table mode ([Fragment_Memory_Table], fb.cpp:310-320):
+     per (i, j - i - minSep, atom pair k) a curve of energy and force on
+     the grid r = tb_rmin + ir tb_dr, ir < tb_size; NULL where no memory
+     contributes (fb.cpp:997-1017, 4238-4323) */
+  int frag_mem_tb_flag, tb_size, tb_nbrs, fm_use_table_flag;
+  double tb_rmin, tb_rmax, tb_dr, frag_table_well_width;
+  double **tb_e, **tb_f;
+  char last_err[256];
   double pair_list_cutoff;
   /* system */
   int n;
@@ -727,7 +735,17 @@ static int parse_coeff(or_ctx *c, const char *path, char *err, int errlen)
       tk_str(&in, c->fm_gamma_file, sizeof c->fm_gamma_file);
     } else if (strcmp(sec, "[Epsilon]") == 0) {
       c->epsilon = tk_double(&in);
-    } else if (strcmp(sec, "[AMH-Go]") == 0 || strcmp(sec, "[Fragment_Memory_Table]") == 0 ||
+    } else if (strcmp(sec, "[Fragment_Memory_Table]") == 0) { /* fb.cpp:310-320 */
+      c->frag_mem_tb_flag = 1;
+      c->k_frag_mem = tk_double(&in);
+      tk_str(&in, c->frag_mems_file, sizeof c->frag_mems_file);
+      tk_str(&in, c->fm_gamma_file, sizeof c->fm_gamma_file);
+      c->tb_rmin = tk_double(&in); c->tb_rmax = tk_double(&in); c->tb_dr = tk_double(&in);
+      c->tb_size = (int)((c->tb_rmax - c->tb_rmin) / c->tb_dr) + 2;
+      c->frag_table_well_width = tk_double(&in);
+      c->fm_use_table_flag = tk_int(&in);
+      c->fm_sigma_exp = tk_double(&in);
+    } else if (strcmp(sec, "[AMH-Go]") == 0 ||
                strcmp(sec, "[Contact_Restraints]") == 0 || strcmp(sec, "[Vector_Fragment_Memory]") == 0 ||
                strcmp(sec, "[Solvent_Barrier]") == 0 || strcmp(sec, "[Membrane]") == 0 ||
                strcmp(sec, "[Fragment_Frustratometer]") == 0 || strcmp(sec, "[Tertiary_Frustratometer]") == 0 ||
@@ -767,6 +785,8 @@ static int read_table_file(const char *dir, const char *name, double *dst, int c
   if (bad) { set_err(err, errlen, "DSSP file format error"); return -1; }
   return 0;
 }
+
+static int fm_table_build(or_ctx *c, char *err, int errlen);
 
 int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
               or_ctx **out, char *err, int errlen)
@@ -890,7 +910,7 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
     free(t);
     if (bad) { or_destroy(c); set_err(err, errlen, "Burial potential burial_gamma.dat file format error"); return -1; }
   }
-  if (c->frag_mem_flag) { /* fb.cpp:749-791 */
+  if (c->frag_mem_flag || c->frag_mem_tb_flag) { /* fb.cpp:749-791 */
     join_path(path, sizeof path, workdir, c->fm_gamma_file);
     if (gm_load(&c->fmg, path, err, errlen)) { or_destroy(c); return -1; }
     if (read_mems(c, workdir, c->frag_mems_file, err, errlen)) { or_destroy(c); return -1; }
@@ -907,6 +927,7 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
       }
     }
   }
+  if (c->frag_mem_tb_flag && fm_table_build(c, err, errlen)) { or_destroy(c); return -1; }
   /* fb.cpp:1048-1066 */
   double plc = 8.0;
   if (c->dssp_hdrgn_flag) plc = fmax(plc, c->dssp_hdrgn_cut);
@@ -950,6 +971,11 @@ void or_destroy(or_ctx *c)
   if (c->frag_mem_map) for (int i = 0; i < c->n; ++i) free(c->frag_mem_map[i]);
   free(c->frag_mem_map); free(c->ilen_fm_map);
   free(c->fmg.gamma);
+  if (c->tb_e) {
+    size_t nt = (size_t)4 * c->n * c->tb_nbrs;
+    for (size_t k = 0; k < nt; ++k) { free(c->tb_e[k]); free(c->tb_f[k]); }
+    free(c->tb_e); free(c->tb_f);
+  }
   for (int k = 0; k < 12; ++k) free(c->aps[k]);
   free(c->se); free(c->chain_no);
   free(c->xn); free(c->xh); free(c->xcp);
@@ -1226,6 +1252,122 @@ static void fm_term(or_ctx *c, int i)
       }
     }
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* fragment-memory table mode                                           */
+
+/* FixBackbone::compute_fragment_memory_table, fb.cpp:4238-4323, with the
+   allocation of fb.cpp:997-1017: for residue i, every memory covering it,
+   j from i + minSep to the memory's end (capped at i + maxSep), and the
+   four atom pairs (a CB skipped when the target or memory residue is a
+   glycine), the curve of entry 4 tb_nbrs i + 4 (j - i - minSep) + k gets
+   V(r) = -eps k_fm w gamma exp(-(r - rf)^2 / (2 sigma^2)) and
+   V (r - rf) / (sigma^2 r) added at every grid point r = rmin + ir dr,
+   sigma^2 = |i - j|^(2 fm_sigma_exp) ww^2 */
+static int fm_table_build(or_ctx *c, char *err, int errlen)
+{
+  int n = c->n, minsep = c->fmg.i_sep[0], maxsep = c->fmg.i_sep[c->fmg.nseq_cl];
+  c->tb_nbrs = maxsep != -1 ? maxsep - minsep + 1 : n - minsep;
+  if (c->tb_nbrs <= 0 || c->tb_size <= 0) { set_err(err, errlen, "Fragment_Memory_Table: empty table"); return -1; }
+  size_t nt = (size_t)4 * n * c->tb_nbrs;
+  c->tb_e = (double **)calloc(nt, sizeof(double *));
+  c->tb_f = (double **)calloc(nt, sizeof(double *));
+  const int ity[4] = {FM_CA, FM_CA, FM_CB, FM_CB}, jty[4] = {FM_CA, FM_CB, FM_CA, FM_CB};
+  for (int i = 0; i < n; ++i) {
+    int itype = restype(c->se[i]);
+    for (int ifm = 0; ifm < c->ilen_fm_map[i]; ++ifm) {
+      const or_mem *m = &c->mems[c->frag_mem_map[i][ifm]];
+      double ekw = c->epsilon * c->k_frag_mem * m->weight;
+      int js = i + minsep, je = m->pos + m->len - 1;
+      if (maxsep != -1 && je > i + maxsep) je = i + maxsep;
+      if (je >= n) { set_err(err, errlen, "Missing residues in memory potential"); return -1; }
+      for (int j = js; j <= je; ++j) {
+        int jtype = restype(c->se[j]);
+        double sig2 = pow(abs(i - j), 2 * c->fm_sigma_exp);
+        sig2 = sig2 * c->frag_table_well_width * c->frag_table_well_width;
+        double g;
+        if (!c->fmg.frag_resty) g = gm_get2(&c->fmg, itype, jtype, i, j);
+        else g = gm_get4(&c->fmg, itype, jtype, restype(m->se[i - m->pos]), restype(m->se[j - m->pos]), i, j);
+        double ekwg = ekw * g;
+        for (int k = 0; k < 4; ++k) {
+          if (ity[k] == FM_CB && (c->se[i] == 'G' || m->se[i - m->pos] == 'G')) continue;
+          if (jty[k] == FM_CB && (c->se[j] == 'G' || m->se[j - m->pos] == 'G')) continue;
+          size_t itb = (size_t)4 * c->tb_nbrs * i + 4 * (j - js) + k;
+          if (!c->tb_e[itb]) {
+            c->tb_e[itb] = (double *)calloc((size_t)c->tb_size, sizeof(double));
+            c->tb_f[itb] = (double *)calloc((size_t)c->tb_size, sizeof(double));
+          }
+          double rf = mem_rf(m, i, ity[k], j, jty[k]);
+          for (int ir = 0; ir < c->tb_size; ++ir) {
+            double r = c->tb_rmin + ir * c->tb_dr;
+            double dr = r - rf, drsq = dr * dr;
+            double V = -ekwg * exp(-drsq / (2 * sig2));
+            c->tb_e[itb][ir] += V;
+            c->tb_f[itb][ir] += V * dr / (sig2 * r);
+          }
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* FixBackbone::table_fragment_memory, fb.cpp:4325-4429: linear
+   interpolation of both curves at r, force f_i += ff dx, f_j -= ff dx.  A
+   missing curve ends the pair (the reference returns, also for the atom
+   pairs after it); a distance outside [rmin, rmax] is an error. */
+static int fm_table_term(or_ctx *c, int i, int j)
+{
+  int minsep = c->fmg.i_sep[0], maxsep = c->fmg.i_sep[c->fmg.nseq_cl];
+  if (j - i < minsep) return 0;
+  if (maxsep != -1 && j - i > maxsep) return 0;
+  int tb_j = j - i - minsep;
+  size_t itb0 = (size_t)4 * c->tb_nbrs * i + 4 * tb_j;
+  if (!c->tb_e[itb0]) return 0;
+  const int ity[4] = {FM_CA, FM_CA, FM_CB, FM_CB}, jty[4] = {FM_CA, FM_CB, FM_CA, FM_CB};
+  const double *xi[4] = {XCA(i), XCA(i), XCB(i), XCB(i)}, *xj[4] = {XCA(j), XCB(j), XCA(j), XCB(j)};
+  double *fi[4] = {FCA(i), FCA(i), FCB(i), FCB(i)}, *fj[4] = {FCA(j), FCB(j), FCA(j), FCB(j)};
+  for (int k = 0; k < 4; ++k) {
+    if (c->se[i] == 'G' && ity[k] == FM_CB) continue;
+    if (c->se[j] == 'G' && jty[k] == FM_CB) continue;
+    double dx[3] = {xi[k][0] - xj[k][0], xi[k][1] - xj[k][1], xi[k][2] - xj[k][2]};
+    double r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    if (!(r >= c->tb_rmin && r <= c->tb_rmax)) {
+      snprintf(c->last_err, sizeof c->last_err, "Table Fragment Memory: r is out of computed range.");
+      return -1;
+    }
+    int ir = (int)((r - c->tb_rmin) / c->tb_dr);
+    size_t itb = itb0 + k;
+    if (!c->tb_e[itb]) return 0;
+    if (ir < 0 || ir >= c->tb_size) {
+      snprintf(c->last_err, sizeof c->last_err, "Table Fragment Memory: ir is out of range.");
+      return -1;
+    }
+    double r1 = c->tb_rmin + (double)ir * c->tb_dr, r2 = c->tb_rmin + (double)(ir + 1) * c->tb_dr;
+    double v1 = c->tb_e[itb][ir], v2 = c->tb_e[itb][ir + 1];
+    double V = ((v2 - v1) * r + v1 * r2 - v2 * r1) / (r2 - r1);
+    double f1 = c->tb_f[itb][ir], f2 = c->tb_f[itb][ir + 1];
+    double ff = ((f2 - f1) * r + f1 * r2 - f2 * r1) / (r2 - r1);
+    c->energy[OR_ET_FRAGMEM] += V;
+    fadd(fi[k], ff, dx);
+    fadd(fj[k], -ff, dx);
+  }
+  return 0;
+}
+
+/* the pair path's loop (fb.cpp:7513-7527): j from i + minSep to
+   ch_pos + ch_len - 2, the last residue of i's chain (ch_pos is 1-based,
+   fb.cpp:546), capped at i + maxSep */
+static int fm_table_terms(or_ctx *c, int i)
+{
+  int minsep = c->fmg.i_sep[0], maxsep = c->fmg.i_sep[c->fmg.nseq_cl];
+  int ch = c->chain_no[i] - 1;
+  int jr0 = i + minsep, jrn = c->ch_pos[ch] + c->ch_len[ch] - 2;
+  if (maxsep != -1 && i + maxsep < jrn) jrn = i + maxsep;
+  for (int j = jr0; j <= jrn; ++j)
+    if (fm_table_term(c, i, j)) return -1;
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1780,6 +1922,7 @@ int or_compute(or_ctx *c, const double *xca, const double *xcb, const double *xo
     if (c->shake_flag && ON(OR_ET_SHAKE)) shake_term(c, i);
     if (!first && !last && c->rama_flag && c->se[i] != 'G' && ON(OR_ET_RAMA)) rama_term(c, i);
     if (c->frag_mem_flag && ON(OR_ET_FRAGMEM)) fm_term(c, i);
+    if (c->frag_mem_tb_flag && ON(OR_ET_FRAGMEM) && fm_table_terms(c, i)) return -1;
   }
   pair_terms(c);
   if (ON(OR_ET_VEXCLUDED)) ev_terms(c, NULL, 0, NULL);
@@ -1852,6 +1995,24 @@ int or_branch_counts(or_ctx *c, long long *out, int n, int reset)
 }
 
 int or_fm_count(const or_ctx *c) { return c->n_mems; }
+
+const char *or_last_error(const or_ctx *c) { return c->last_err; }
+
+/* table mode: grid size and neighbour count (0 without table mode) */
+void or_fm_table_dims(const or_ctx *c, int *tb_size, int *tb_nbrs)
+{
+  *tb_size = c->frag_mem_tb_flag ? c->tb_size : 0;
+  *tb_nbrs = c->frag_mem_tb_flag ? c->tb_nbrs : 0;
+}
+
+/* curve itb (energy, force on the grid): 1 when it exists, else 0 */
+int or_fm_table(const or_ctx *c, int itb, double *e, double *f)
+{
+  if (!c->frag_mem_tb_flag || itb < 0 || (size_t)itb >= (size_t)4 * c->n * c->tb_nbrs || !c->tb_e[itb]) return 0;
+  memcpy(e, c->tb_e[itb], (size_t)c->tb_size * sizeof(double));
+  memcpy(f, c->tb_f[itb], (size_t)c->tb_size * sizeof(double));
+  return 1;
+}
 int or_fm_info(const or_ctx *c, int k, int *pos, int *len, double *weight)
 {
   if (k < 0 || k >= c->n_mems) return -1;
@@ -1928,7 +2089,7 @@ int or_langevin_run(or_ctx *c, int nsteps, double dt, double t_start, double t_s
     }
     /* force */
     for (int k = 0; k < 3 * na; ++k) f[k] = 0.0;
-    or_compute(c, x, x + 3 * n, x + 6 * n, f, f + 3 * n, f + 6 * n, energy, all);
+    if (or_compute(c, x, x + 3 * n, x + 6 * n, f, f + 3 * n, f + 6 * n, energy, all)) return -1;
     /* fix langevin post_force: target T ramps linearly over the run */
     double delta = nsteps > 0 ? (double)(s + 1) / (double)nsteps : 0.0;
     double T = t_start + delta * (t_stop - t_start);

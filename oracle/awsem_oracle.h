@@ -100,6 +100,11 @@ int or_list(or_ctx *c, int kind, int arg, const double *xca, const double *xcb,
 
 /* Fragment-memory introspection (fm.cpp) */
 int or_fm_count(const or_ctx *c);
+/* fragment-memory table mode: last error of or_compute (an out-of-range
+   distance), grid size / neighbour count, one curve's energy and force */
+const char *or_last_error(const or_ctx *c);
+void or_fm_table_dims(const or_ctx *c, int *tb_size, int *tb_nbrs);
+int or_fm_table(const or_ctx *c, int itb, double *e, double *f);
 int or_fm_info(const or_ctx *c, int k, int *pos, int *len, double *weight);
 double or_fm_rf(const or_ctx *c, int k, int ires, int iatom, int jres, int jatom);
 char or_fm_se(const or_ctx *c, int k, int resno);

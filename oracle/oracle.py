@@ -69,6 +69,11 @@ def lib():
         L.or_langevin_run.restype = C.c_int
         L.or_branch_counts.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int,
                                        C.c_int]
+        L.or_last_error.argtypes = [C.c_void_p]
+        L.or_last_error.restype = C.c_char_p
+        L.or_fm_table_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_fm_table.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.or_fm_table.restype = C.c_int
         _lib = L
     return _lib
 
@@ -111,10 +116,26 @@ class Oracle:
         f = np.zeros_like(x)
         e = np.zeros(N_ET)
         fca, fcb, fo = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros((self.n, 3))
-        lib().or_compute(self.h, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]), np.ascontiguousarray(x[2]),
-                         fca, fcb, fo, e, mask & 0xFFFFFFFF)
+        rc = lib().or_compute(self.h, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]),
+                              np.ascontiguousarray(x[2]), fca, fcb, fo, e, mask & 0xFFFFFFFF)
+        if rc:
+            raise RuntimeError(lib().or_last_error(self.h).decode())
         f[0], f[1], f[2] = fca, fcb, fo
         return f, e
+
+    def fm_table_dims(self):
+        """(grid points per curve, neighbours per residue) of fragment-memory
+        table mode; (0, 0) without it."""
+        a, b = C.c_int(), C.c_int()
+        lib().or_fm_table_dims(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def fm_table(self, itb: int):
+        """curve itb = 4 tb_nbrs i + 4 (j - i - minSep) + k: (energy, force)
+        on the grid, or None where no memory contributes."""
+        size, _ = self.fm_table_dims()
+        e, f = np.zeros(size), np.zeros(size)
+        return (e, f) if lib().or_fm_table(self.h, itb, e, f) else None
 
     def energy(self, coords, mask=ALL_TERMS):
         return self.compute(coords, mask)[1]
@@ -171,6 +192,8 @@ class Oracle:
         v = np.ascontiguousarray(vel, dtype=np.float64).copy()
         f = np.ascontiguousarray(force, dtype=np.float64).copy()
         e = np.zeros(N_ET)
-        lib().or_langevin_run(self.h, nsteps, dt, t_start, t_stop, damp, seed, replica, step0,
-                              x.reshape(-1), v.reshape(-1), f.reshape(-1), e)
+        rc = lib().or_langevin_run(self.h, nsteps, dt, t_start, t_stop, damp, seed, replica, step0,
+                                   x.reshape(-1), v.reshape(-1), f.reshape(-1), e)
+        if rc:
+            raise RuntimeError(lib().or_last_error(self.h).decode())
         return x, v, f, e
