@@ -31,7 +31,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 
 ERRORS = {-1: "AWSEM_E_ARG", -2: "AWSEM_E_FILE", -3: "AWSEM_E_FORMAT", -4: "AWSEM_E_UNSUPPORTED",
           -5: "AWSEM_E_TOPOLOGY", -6: "AWSEM_E_MISSING_ATOM", -7: "AWSEM_E_DEVICE", -8: "AWSEM_E_NOMEM",
-          -9: "AWSEM_E_NONFINITE", -10: "AWSEM_E_STATE"}
+          -9: "AWSEM_E_NONFINITE", -10: "AWSEM_E_STATE", -11: "AWSEM_E_RANGE"}
 
 EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_compute", "awsem_compute_res",
            "awsem_set_vexcluded", "awsem_set_bonds", "awsem_set_term_mask", "awsem_list", "awsem_run",

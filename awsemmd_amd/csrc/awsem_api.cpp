@@ -99,6 +99,28 @@ struct DevBatch {
 
 static AwStatus launch_seq(DevBatch &D, const AwRun &run, int engine, int nst);
 
+// error words the kernels raised during the last launch (table mode: a
+// distance outside the table's range), read and cleared; after the
+// launch's synchronisation
+static AwStatus take_faults(DevBatch &D)
+{
+  int any = 0;
+  for (const ReplicaDesc &R : D.reps) {
+    if (R.tb_items <= 0 || !R.fault) continue;
+    int w = 0;
+    if (hipMemcpy(&w, R.fault, sizeof w, hipMemcpyDeviceToHost) != hipSuccess)
+      return AwStatus::err(AWSEM_E_DEVICE, "reading the error word");
+    if (w) {
+      any |= w;
+      int z = 0;
+      if (hipMemcpy(R.fault, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess)
+        return AwStatus::err(AWSEM_E_DEVICE, "clearing the error word");
+    }
+  }
+  if (any & AW_FAULT_TB_RANGE) return AwStatus::err(AWSEM_E_RANGE, "Table Fragment Memory: r is out of computed range.");
+  return AwStatus();
+}
+
 template <class T>
 static AwStatus upload(T **dst, const std::vector<T> &src, size_t min_elems = 1)
 {
@@ -347,6 +369,31 @@ static int fail(awsem_ctx *c, const AwStatus &s)
   return s.code;
 }
 
+// table mode: fm_use_table set and both table files present (fb.cpp:746)
+static bool fm_tables_stored(const FFParams &P, const std::string &dir)
+{
+  if (!P.fm_tb_flag || !P.fm_use_table) return false;
+  std::string a = dir.empty() ? "fm_table.energy" : dir + "/fm_table.energy";
+  std::string b = dir.empty() ? "fm_table.force" : dir + "/fm_table.force";
+  FILE *fa = std::fopen(a.c_str(), "r"), *fb = std::fopen(b.c_str(), "r");
+  bool ok = fa && fb;
+  if (fa) std::fclose(fa);
+  if (fb) std::fclose(fb);
+  return ok;
+}
+
+// table mode, once the system is complete (fb.cpp:997-1017): the stored
+// tables, or the tables built from the memories (written out when
+// fm_use_table is set)
+static AwStatus fm_table_setup(AwSystem &S, const std::string &dir)
+{
+  if (!S.P.fm_tb_flag) return AwStatus();
+  if (fm_tables_stored(S.P, dir)) return aw_read_fm_table(dir, S);
+  AwStatus s = aw_build_fm_table(S);
+  if (s.ok() && S.P.fm_use_table) s = aw_write_fm_table(dir, S);
+  return s;
+}
+
 static AwStatus load_system(const std::string &coeff, const std::string &seq_path, const std::string &dir, AwSystem &S,
                             bool from_files)
 {
@@ -360,10 +407,12 @@ static AwStatus load_system(const std::string &coeff, const std::string &seq_pat
   if (from_files) {
     if (!(s = aw_read_ssweight(dir, S)).ok()) return s;
   }
-  if (S.P.fm_flag) {
+  if (S.P.fm_flag || S.P.fm_tb_flag) {
     std::string gp = gamma_file.empty() || gamma_file[0] == '/' ? gamma_file : dir + "/" + gamma_file;
     if (!(s = aw_read_gamma_array(gp, S.fmg)).ok()) return s;
-    if (from_files && !(s = aw_read_mems(dir, mem_file, S)).ok()) return s;
+    // table mode with stored tables reads no memories (fb.cpp:746-758)
+    if (from_files && (S.P.fm_flag || !fm_tables_stored(S.P, dir)) && !(s = aw_read_mems(dir, mem_file, S)).ok())
+      return s;
   }
   return AwStatus();
 }
@@ -379,6 +428,7 @@ int awsem_create(const char *coeff_path, const char *seq_path, const char *workd
   std::string dir = workdir ? workdir : ".";
   AwStatus s = load_system(coeff_path, seq_path, dir, c->sys, true);
   if (s.ok()) s = aw_finish_system(c->sys);
+  if (s.ok()) s = fm_table_setup(c->sys, dir);
   if (!s.ok()) {
     // keep the message reachable for the caller through a context-less path
     *out = c.release();
@@ -424,7 +474,7 @@ int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, 
           for (int k = 0; k < 12; ++k)
             if (S.P.ssweight[k]) S.aps[(size_t)i * 12 + k] = aps[(size_t)i * 12 + k];
     }
-    if (s.ok() && S.P.fm_flag) {
+    if (s.ok() && (S.P.fm_flag || S.P.fm_tb_flag)) {
       if (n_mem > 0 && (!struct_len || !struct_xyz_nm || !struct_seq || !mem_struct || !mem_tpos || !mem_fpos ||
                         !mem_len || !mem_weight))
         s = AwStatus::err(AWSEM_E_ARG, "memory arrays missing");
@@ -455,6 +505,7 @@ int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, 
     }
   }
   if (s.ok()) s = aw_finish_system(S);
+  if (s.ok()) s = fm_table_setup(S, param_dir ? param_dir : ".");
   *out = c.release();
   if (!s.ok()) {
     (*out)->err = s.msg;
@@ -609,6 +660,7 @@ static AwStatus compute_res(awsem_ctx *c, const double *xres, double *fres, doub
   double e[AW_N_ET];
   HIPCHK(hipMemcpyAsync(e, D.d_e, sizeof e, hipMemcpyDeviceToHost, D.stream));
   HIPCHK(hipStreamSynchronize(D.stream));
+  if (!(s = take_faults(D)).ok()) return s;
   float ms = 0.f;
   hipEventElapsedTime(&ms, D.ev0, D.ev1);
   D.last_ms = ms;
@@ -703,6 +755,10 @@ int aw_compute_res_scratch(awsem_ctx *c, const double *xres, double *fres, doubl
     for (size_t k = 0; k < m; ++k) fres[k0 + k] += buf[k];
   }
   if (hipStreamSynchronize(D.stream) != hipSuccess) return dev_fail("awsem_compute_res_noalloc: sync");
+  {
+    AwStatus fs = take_faults(D);
+    if (!fs.ok()) return fail(c, fs);
+  }
   if (energy) for (int k = 0; k < AW_N_ET; ++k) energy[k] = e[k];
   for (int k = 0; k < AW_N_ET; ++k) c->last_e[k] = e[k];
   if (!std::isfinite(e[0])) return fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite energy"));
@@ -879,6 +935,7 @@ int awsem_run(awsem_ctx *c, int nsteps, const awsem_langevin *lg, double *xres, 
       hipMemcpyAsync(e, D.d_e, sizeof e, hipMemcpyDeviceToHost, D.stream) != hipSuccess ||
       hipStreamSynchronize(D.stream) != hipSuccess)
     return fail(c, AwStatus::err(AWSEM_E_DEVICE, "copy back"));
+  if (!(s = take_faults(D)).ok()) return fail(c, s);
   float ms = 0.f;
   hipEventElapsedTime(&ms, D.ev0, D.ev1);
   D.last_ms = ms;
@@ -1002,6 +1059,7 @@ static int batch_launch(awsem_batch *b, AwRun run, const awsem_langevin *lg)
   (void)hipEventRecord(D.ev1, D.stream);
   hipError_t e = hipEventSynchronize(D.ev1);
   if (e != hipSuccess) return bfail(b, AwStatus::err(AWSEM_E_DEVICE, hipGetErrorString(e)));
+  if (!(s = take_faults(D)).ok()) return bfail(b, s);
   if (D.nshare > 0 && D.engine == 0) {
     // a replica that gave up waiting for chunks taken elsewhere (bounded
     // spin) leaves this flag: the launch's forces are then incomplete

@@ -73,6 +73,10 @@ __device__ __forceinline__ View<T> make_view(const BatchDev &B, const ReplicaDes
   V.P = (ParamPtr)B.P;
   V.fm_items = R.fm_items;
   V.fm_item_base = 0;
+  V.tb_items = R.tb_items;
+  V.tb_item = (const AW_G int *)R.tb_item;
+  V.tb_curve = R.tb_curve;
+  V.fault = (AW_G int *)R.fault;
   V.coef = (const T *)R.coef;
   V.rf = R.rf;
   V.items = R.items;

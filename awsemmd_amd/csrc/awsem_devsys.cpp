@@ -66,6 +66,13 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   plan.off_items = take(cur, (size_t)plan.fm.n_items() * (fp32 ? sizeof(FMItem) : sizeof(FMItemD)));
   plan.off_rf = take(cur, plan.fm.rf.size() * tsz);
   plan.off_coef = take(cur, plan.fm.coef.size() * tsz);
+  // table mode: items and curves (inside the fragment-memory range that
+  // identical replicas share), curves as (energy, force) pairs in T
+  std::vector<int> tb_items;
+  if (FP.fm_tb_flag) aw_fm_table_items(S, tb_items);
+  plan.n_tb_items = (int)(tb_items.size() / 8);
+  plan.off_tbi = take(cur, tb_items.size() * sizeof(int));
+  plan.off_tbc = take(cur, 2 * S.fmt.e.size() * tsz);
   plan.off_nl = take(cur, (AW_NL_HDR + 6 * (size_t)plan.nl_cap) * sizeof(int));
   plan.off_nlx = take(cur, 4 * (size_t)n * sizeof(double));
   // cell-list table for large systems (the wide engine builds their lists
@@ -79,6 +86,7 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
     plan.off_cells = take(cur, (2 * (size_t)h + 4 * (size_t)n) * sizeof(int));
   }
   if (scratch) plan.off_scr = take(cur, (size_t)aw_slab_doubles(n) * sizeof(double));
+  plan.off_fault = take(cur, 256);
   plan.bytes = std::max<size_t>(cur, 256);
   // the image: zeros (state, list headers, slab) and the packed constants
   std::vector<unsigned char> &img = plan.image;
@@ -102,6 +110,19 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
     put(img, plan.off_items, items);
     put(img, plan.off_rf, rf);
     put(img, plan.off_coef, coef);
+  }
+  put(img, plan.off_tbi, tb_items);
+  if (!S.fmt.e.empty()) {
+    const AwFMTable &T = S.fmt;
+    if (fp32) {
+      std::vector<float> c(2 * T.e.size());
+      for (size_t q = 0; q < T.e.size(); ++q) { c[2 * q] = (float)T.e[q]; c[2 * q + 1] = (float)T.f[q]; }
+      put(img, plan.off_tbc, c);
+    } else {
+      std::vector<double> c(2 * T.e.size());
+      for (size_t q = 0; q < T.e.size(); ++q) { c[2 * q] = T.e[q]; c[2 * q + 1] = T.f[q]; }
+      put(img, plan.off_tbc, c);
+    }
   }
   plan.n_items = fm.n_items();
   plan.fm_ij.resize(plan.n_items);
@@ -143,6 +164,10 @@ AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out)
   R.nlx = (double *)(b + plan.off_nlx);
   R.scratch = plan.scratch ? (double *)(b + plan.off_scr) : nullptr;
   R.cells = plan.cell_h ? (int *)(b + plan.off_cells) : nullptr;
+  R.tb_items = plan.n_tb_items;
+  R.tb_item = (const int *)(b + plan.off_tbi);
+  R.tb_curve = b + plan.off_tbc;
+  R.fault = (int *)(b + plan.off_fault);
   R.cell_h = plan.cell_h;
   return AwStatus();
 }

@@ -35,6 +35,17 @@ struct AwMemory {
   std::vector<double> ca, cb;   // len*3, Angstrom
 };
 
+// fragment-memory table mode: curve ids per (i, j - i - minSep, atom pair
+// k) at 4 (tb_nbrs i + j - i - minSep) + k: -1 no curve, -2 a curve of
+// zeros (read from a table file), else the index of its tb_size energies
+// and forces in e / f
+struct AwFMTable {
+  int tb_size = 0, tb_nbrs = 0;
+  std::vector<int> cid;
+  std::vector<double> e, f;
+  int n_curves() const { return tb_size ? (int)(e.size() / tb_size) : 0; }
+};
+
 struct AwSystem {
   FFParams P;
   int n = 0, nch = 0;
@@ -44,6 +55,7 @@ struct AwSystem {
   std::vector<int> rinfo;
   AwGamma fmg;
   std::vector<AwMemory> mems;
+  AwFMTable fmt;                       // table mode only
   char frag_mems_file[256] = {0}, fm_gamma_file[256] = {0};
 };
 
@@ -66,6 +78,16 @@ struct AwFMPack {
 // appends one system's FM items (residue indices local to the system)
 // P: the force field (fragment-memory strength, sigma exponent, epsilon)
 AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B);
+// table mode: the curves from the memories (fb.cpp:4238-4323), or read
+// from fm_table.energy / fm_table.force (fb.cpp:4179-4236), and written
+// there (fb.cpp:6062-6106)
+AwStatus aw_build_fm_table(AwSystem &S);
+AwStatus aw_read_fm_table(const std::string &dir, AwSystem &S);
+AwStatus aw_write_fm_table(const std::string &dir, const AwSystem &S);
+// the pair path's table items (fb.cpp:7513-7527): per pair (i, j) whose
+// CA-CA curve exists, i, j and the curve ids of its four atom pairs, -3
+// for an atom pair skipped by a target glycine CB
+void aw_fm_table_items(const AwSystem &S, std::vector<int> &items);
 // the device form of a pack in precision T: work items with the width
 // factor sc = sqrt(c2) rounded to T, and the Rf entries pre-multiplied by
 // it (in T, as the kernels scale the distances), skipped entries (memory or
@@ -128,6 +150,8 @@ struct AwDevPlan {
   int nl_cap = 0, n = 0, nch = 0, n_items = 0;
   size_t off_x = 0, off_v = 0, off_f = 0, off_e = 0, off_rinfo = 0, off_aps = 0, off_chs = 0, off_chl = 0;
   size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0, off_cells = 0;
+  size_t off_tbi = 0, off_tbc = 0, off_fault = 0;  // table items, curves, error word
+  int n_tb_items = 0;
   int cell_h = 0;
   std::vector<int> fm_ij;  // (i << 16) | j of every fragment-memory item, in device order
   size_t bytes = 0;

@@ -33,7 +33,7 @@ static std::string join(const std::string &dir, const std::string &name)
 }
 
 static const char *kUnsupported[] = {
-    "[AMH-Go]", "[Fragment_Memory_Table]", "[Contact_Restraints]", "[Vector_Fragment_Memory]",
+    "[AMH-Go]", "[Contact_Restraints]", "[Vector_Fragment_Memory]",
     "[Solvent_Barrier]", "[Membrane]", "[Fragment_Frustratometer]", "[Tertiary_Frustratometer]",
     "[Nmer_Frustratometer]", "[Phosphorylation]", "[Amylometer]", "[Selection_Temperature]",
     "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]", "[DebyeHuckel]",
@@ -132,6 +132,14 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
     } else if (sec == "[Fragment_Memory]") {
       P.fm_flag = 1;
       in >> P.k_frag_mem >> mem_file >> gamma_file;
+    } else if (sec == "[Fragment_Memory_Table]") {
+      // fb.cpp:310-320
+      P.fm_tb_flag = 1;
+      in >> P.k_frag_mem >> mem_file >> gamma_file >> P.tb_rmin >> P.tb_rmax >> P.tb_dr;
+      P.tb_size = (int)((P.tb_rmax - P.tb_rmin) / P.tb_dr) + 2;
+      in >> P.tb_well_width >> P.fm_use_table >> P.fm_sigma_exp;
+      if (!in.fail() && (!(P.tb_dr > 0.0) || P.tb_size < 2))
+        return AwStatus::err(-3, "Fragment_Memory_Table: empty distance grid");
     } else if (sec == "[Epsilon]") {
       in >> P.epsilon;
     } else if (sec == "[Shuffler]" || sec == "[Mutate_Sequence]") {
@@ -569,9 +577,11 @@ AwStatus aw_finish_system(AwSystem &S)
       if (S.se[i] == 'P') ri |= RI_PRO;
       S.rinfo[i] = ri;
     }
-  if (P.fm_flag) {
+  if (P.fm_flag || P.fm_tb_flag) {
     P.fm_min_sep = S.fmg.min_sep();
     P.fm_max_sep = S.fmg.max_sep();
+  }
+  if (P.fm_flag) {
     for (auto &m : S.mems) {
       int i0 = m.pos, i1 = m.pos + m.len - 1;
       for (int i = i0; i < i1; ++i)
@@ -673,3 +683,175 @@ AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B)
   return AwStatus();
 }
 
+
+// ---------------------------------------------------------------------------
+// Fragment-memory table mode.
+
+// compute_fragment_memory_table (fb.cpp:4238-4323) with the allocation of
+// fb.cpp:997-1017: for residue i and every memory covering it (pos <= i <
+// pos + len - minSep, the frag_mem_map of fb.cpp:775-789), j from i + minSep
+// to the memory's end (at most i + maxSep) and the four atom pairs (a CB
+// skipped when the target or the memory residue is a glycine), the curve of
+// (i, j, k) gets V(r) = -eps k_fm w gamma exp(-(r - rf)^2 / (2 sigma^2)) and
+// V (r - rf) / (sigma^2 r) added at every grid point r = rmin + ir dr, with
+// sigma^2 = |i - j|^(2 fm_sigma_exp) well_width^2.
+AwStatus aw_build_fm_table(AwSystem &S)
+{
+  const FFParams &P = S.P;
+  AwFMTable &T = S.fmt;
+  int n = S.n, minsep = S.fmg.min_sep(), maxsep = S.fmg.max_sep();
+  T.tb_size = P.tb_size;
+  T.tb_nbrs = maxsep != -1 ? maxsep - minsep + 1 : n - minsep;
+  if (T.tb_nbrs <= 0 || T.tb_size < 2) return AwStatus::err(-3, "Fragment_Memory_Table: empty table");
+  T.cid.assign((size_t)4 * n * T.tb_nbrs, -1);
+  T.e.clear();
+  T.f.clear();
+  int nc = 0;
+  for (int i = 0; i < n; ++i) {
+    int it = aw_restype(S.se[i]);
+    for (const AwMemory &m : S.mems) {
+      if (i < m.pos || i >= m.pos + m.len - minsep) continue;
+      double ekw = P.epsilon * P.k_frag_mem * m.weight;
+      int js = i + minsep, je = m.pos + m.len - 1;
+      if (maxsep != -1) je = std::min(je, i + maxsep);
+      if (je >= n) return AwStatus::err(-3, "Missing residues in memory potential");
+      for (int j = js; j <= je; ++j) {
+        double sig2 = std::pow((double)std::abs(i - j), 2 * P.fm_sigma_exp) * P.tb_well_width * P.tb_well_width;
+        double ekwg = ekw * S.fmg.get(it, aw_restype(S.se[j]), i, j);
+        int a = i - m.pos, b = j - m.pos;
+        bool gi = S.se[i] == 'G' || m.se[a] == 'G', gj = S.se[j] == 'G' || m.se[b] == 'G';
+        const double *cai = &m.ca[3 * a], *caj = &m.ca[3 * b], *cbi = &m.cb[3 * a], *cbj = &m.cb[3 * b];
+        auto R = [](const double *p, const double *q) {
+          return std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]));
+        };
+        const double rfk[4] = {R(cai, caj), gj ? 0.0 : R(cai, cbj), gi ? 0.0 : R(cbi, caj), (gi || gj) ? 0.0 : R(cbi, cbj)};
+        const bool live[4] = {true, !gj, !gi, !gi && !gj};
+        for (int k = 0; k < 4; ++k) {
+          if (!live[k]) continue;
+          size_t itb = 4 * ((size_t)T.tb_nbrs * i + (j - js)) + k;
+          if (T.cid[itb] < 0) {
+            T.cid[itb] = nc++;
+            T.e.resize((size_t)nc * T.tb_size, 0.0);
+            T.f.resize((size_t)nc * T.tb_size, 0.0);
+          }
+          double *ce = &T.e[(size_t)T.cid[itb] * T.tb_size], *cf = &T.f[(size_t)T.cid[itb] * T.tb_size];
+          for (int ir = 0; ir < T.tb_size; ++ir) {
+            double r = P.tb_rmin + ir * P.tb_dr;
+            double dr = r - rfk[k];
+            double V = -ekwg * std::exp(-dr * dr / (2 * sig2));
+            ce[ir] += V;
+            cf[ir] += V * dr / (sig2 * r);
+          }
+        }
+      }
+    }
+  }
+  return AwStatus();
+}
+
+// read_fragment_memory_table (fb.cpp:4179-4236): both files hold
+// 4 n tb_nbrs rows of tb_size values; every row read becomes a curve (a
+// row of zeros, the reference's output for a curve it never allocated, is
+// kept as a zero curve: present, contributing nothing)
+AwStatus aw_read_fm_table(const std::string &dir, AwSystem &S)
+{
+  const FFParams &P = S.P;
+  AwFMTable &T = S.fmt;
+  int n = S.n, minsep = S.fmg.min_sep(), maxsep = S.fmg.max_sep();
+  T.tb_size = P.tb_size;
+  T.tb_nbrs = maxsep != -1 ? maxsep - minsep + 1 : n - minsep;
+  if (T.tb_nbrs <= 0 || T.tb_size < 2) return AwStatus::err(-3, "Fragment_Memory_Table: empty table");
+  size_t ntb = (size_t)4 * n * T.tb_nbrs;
+  std::vector<double> ev, fv;
+  for (int pass = 0; pass < 2; ++pass) {
+    std::ifstream in(join(dir, pass ? "fm_table.force" : "fm_table.energy"));
+    if (!in) return AwStatus::err(-2, "Fragment memory table files not found!");
+    std::vector<double> &v = pass ? fv : ev;
+    double val;
+    while (in >> val) {
+      if (v.size() >= ntb * (size_t)T.tb_size) return AwStatus::err(-3, "Fragment memory table file format error!");
+      v.push_back(val);
+    }
+    if (v.empty()) return AwStatus::err(-3, "Fragment memory table file format error!");
+    // the reference's end check (fb.cpp:4210, 4235)
+    size_t itb = (v.size() - 1) / T.tb_size, ir = v.size() - itb * T.tb_size;
+    if (itb != ntb - 1 && ir != (size_t)T.tb_size) return AwStatus::err(-3, "Fragment memory table file format error!");
+  }
+  if (fv.size() > ev.size()) return AwStatus::err(-3, "Fragment memory table file format error!");
+  fv.resize(ev.size(), 0.0);
+  T.cid.assign(ntb, -1);
+  T.e.clear();
+  T.f.clear();
+  size_t rows = (ev.size() + T.tb_size - 1) / T.tb_size;
+  int nc = 0;
+  for (size_t itb = 0; itb < rows; ++itb) {
+    size_t b = itb * T.tb_size, e = std::min(ev.size(), b + T.tb_size);
+    bool zero = true;
+    for (size_t q = b; q < e; ++q) zero = zero && ev[q] == 0.0 && fv[q] == 0.0;
+    if (zero) { T.cid[itb] = -2; continue; }
+    T.cid[itb] = nc++;
+    T.e.insert(T.e.end(), ev.begin() + b, ev.begin() + e);
+    T.f.insert(T.f.end(), fv.begin() + b, fv.begin() + e);
+    T.e.resize((size_t)nc * T.tb_size, 0.0);
+    T.f.resize((size_t)nc * T.tb_size, 0.0);
+  }
+  return AwStatus();
+}
+
+// output_fragment_memory_table (fb.cpp:6062-6106): "%.12f " per value, one
+// row per curve, zeros for a curve never allocated and for an infinity
+AwStatus aw_write_fm_table(const std::string &dir, const AwSystem &S)
+{
+  const AwFMTable &T = S.fmt;
+  FILE *fe = std::fopen(join(dir, "fm_table.energy").c_str(), "w");
+  FILE *ff = std::fopen(join(dir, "fm_table.force").c_str(), "w");
+  if (!fe || !ff) {
+    if (fe) std::fclose(fe);
+    if (ff) std::fclose(ff);
+    return AwStatus::err(-2, "Fragment memory table files not found!");
+  }
+  for (size_t itb = 0; itb < T.cid.size(); ++itb) {
+    int c = T.cid[itb];
+    for (int ir = 0; ir < T.tb_size; ++ir) {
+      double e = c >= 0 ? T.e[(size_t)c * T.tb_size + ir] : 0.0, f = c >= 0 ? T.f[(size_t)c * T.tb_size + ir] : 0.0;
+      if (std::isinf(e)) e = 0.0;
+      if (std::isinf(f)) f = 0.0;
+      std::fprintf(fe, "%.12f ", e);
+      std::fprintf(ff, "%.12f ", f);
+    }
+    std::fprintf(fe, "\n");
+    std::fprintf(ff, "\n");
+  }
+  std::fclose(fe);
+  std::fclose(ff);
+  return AwStatus();
+}
+
+// the pair path's loop (fb.cpp:7513-7527): j from i + minSep to the last
+// residue of i's chain (ch_pos + ch_len - 2 with the reference's 1-based
+// chain starts, fb.cpp:546), at most i + maxSep; a pair whose CA-CA curve
+// does not exist is skipped (fb.cpp:4341); per atom pair the curve id, -3
+// where a target glycine has no CB (fb.cpp:4378-4379)
+void aw_fm_table_items(const AwSystem &S, std::vector<int> &items)
+{
+  items.clear();
+  const AwFMTable &T = S.fmt;
+  if (T.cid.empty()) return;
+  int minsep = S.fmg.min_sep(), maxsep = S.fmg.max_sep();
+  for (int c = 0; c < S.nch; ++c) {
+    int last = S.ch_start[c] + S.ch_len[c] - 1;
+    for (int i = S.ch_start[c]; i <= last; ++i) {
+      int jrn = last;
+      if (maxsep != -1) jrn = std::min(jrn, i + maxsep);
+      for (int j = i + minsep; j <= jrn; ++j) {
+        size_t base = 4 * ((size_t)T.tb_nbrs * i + (j - i - minsep));
+        if (T.cid[base] == -1) continue;
+        bool gi = S.se[i] == 'G', gj = S.se[j] == 'G';
+        const bool skip[4] = {false, gj, gi, gi || gj};
+        int it[8] = {i, j, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) it[2 + k] = skip[k] ? -3 : T.cid[base + k];
+        items.insert(items.end(), it, it + 8);
+      }
+    }
+  }
+}

@@ -316,6 +316,15 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
       if (hipMemcpy(j->x_out.data(), j->dev->d.x, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(j->energy, j->dev->d.energy, sizeof j->energy, hipMemcpyDeviceToHost) != hipSuccess)
         s = AwStatus::err(AWSEM_E_DEVICE, "job result copy");
+      // table mode: a distance outside the table's range in any round of
+      // the job (the error word is never cleared during a job)
+      if (s.ok() && j->dev->d.tb_items > 0) {
+        int w = 0;
+        if (hipMemcpy(&w, j->dev->d.fault, sizeof w, hipMemcpyDeviceToHost) != hipSuccess)
+          s = AwStatus::err(AWSEM_E_DEVICE, "job error word");
+        else if (w & AW_FAULT_TB_RANGE)
+          s = AwStatus::err(AWSEM_E_RANGE, "Table Fragment Memory: r is out of computed range.");
+      }
       j->dev.reset();
       std::lock_guard<std::mutex> lk(mu);
       used -= j->plan.bytes;

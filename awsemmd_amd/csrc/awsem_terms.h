@@ -331,6 +331,11 @@ struct View {
   ParamPtr P;
   // fragment memory
   int fm_items, fm_item_base;
+  // fragment-memory table mode: items and curves (ReplicaDesc), error word
+  int tb_items;
+  const AW_G int *tb_item;
+  const void *tb_curve;
+  AW_G int *fault;
   const T *coef;
   const void *rf;        // T4 per (memory, pair)
   const void *items;     // FMItem / FMItemD per item
@@ -1859,6 +1864,59 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
   }
 }
 
+// fragment-memory table mode, one item = one residue pair (i, j) of the pair
+// path's loop (table_fragment_memory, fb.cpp:4325-4429): for each atom pair
+// k (CA-CA, CA(i)-CB(j), CB(i)-CA(j), CB-CB) not skipped by a target
+// glycine, the FP64 distance r must lie in [rmin, rmax] (else the error word
+// is set and the pair ends: the reference aborts); a missing curve ends the
+// pair, a curve of zeros contributes nothing; otherwise energy and force
+// factor are interpolated linearly between the grid points around r, and
+// f_i += ff dx, f_j -= ff dx.  Items k, k + ks, ... (< k1) of one lane;
+// coordinates x[3][n][3], forces at fb + 3 (kind fs + i) (the view's force
+// base and stride); returns the energy.  A function of its own (not
+// inlined): table mode is rare, and its registers stay out of the passes
+// around its call.
+template <class T>
+AW_NOINLINE double fm_table_run(const double *x, int n, double *fb, int fs, ParamPtr P, const AW_G int *items,
+                                const void *curve, AW_G int *fault, int k, int k1, int ks)
+{
+  const double rmin = P->tb_rmin, rmax = P->tb_rmax, dr = P->tb_dr;
+  const int tbs = P->tb_size;
+  const int ki[4] = {K_CA, K_CA, K_CB, K_CB}, kj[4] = {K_CA, K_CB, K_CA, K_CB};
+  double e = 0.0;
+  for (; k < k1; k += ks) {
+    const AW_G int *it = items + 8 * k;
+    int i = it[0], j = it[1];
+    const int cs[4] = {it[2], it[3], it[4], it[5]};
+    for (int q = 0; q < 4; ++q) {
+      int c = cs[q];
+      if (c == -3) continue;
+      const double *a = x + (ki[q] * n + i) * 3, *b = x + (kj[q] * n + j) * 3;
+      double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+      double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      if (!(r >= rmin && r <= rmax)) {
+#ifndef AW_HOST_EMU
+        atomicOr((int *)fault, AW_FAULT_TB_RANGE);
+#endif
+        break;
+      }
+      if (c == -1) break;
+      if (c == -2) continue;
+      int ir = (int)((r - rmin) / dr);
+      ir = ir < tbs - 2 ? ir : tbs - 2;
+      T t = (T)((r - (rmin + (double)ir * dr)) / dr);
+      const T *cv = (const T *)curve + 2 * ((size_t)c * tbs + ir);
+      T e1 = cv[0], f1 = cv[1], e2 = cv[2], f2 = cv[3];
+      T ve = e1 + (e2 - e1) * t, ff = f1 + (f2 - f1) * t;
+      e += (double)ve;
+      double *pi = fb + (ki[q] * fs + i) * 3, *pj = fb + (kj[q] * fs + j) * 3;
+      aadd(pi + 0, (double)(ff * (T)d0)); aadd(pi + 1, (double)(ff * (T)d1)); aadd(pi + 2, (double)(ff * (T)d2));
+      aadd(pj + 0, -(double)(ff * (T)d0)); aadd(pj + 1, -(double)(ff * (T)d1)); aadd(pj + 2, -(double)(ff * (T)d2));
+    }
+  }
+  return e;
+}
+
 // part: bit 0 fragment memory, bit 1 local terms, bit 2 residue pairs.
 // QUEUED (resident engine, one workgroup per replica): the pair sweep runs
 // in chunks of at most nl_cap pairs; each chunk is scanned into the DSSP,
@@ -1946,6 +2004,8 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
   if (ef) eflush(E, V.ered);
   E.clear();
   run_fm();
+  if (P->fm_tb_flag && ON(P, AW_ET_FRAGMEM) && (part & 1) && V.tb_items > 0)
+    E.fm += fm_table_run<T>(V.x(), V.n, V.fb, V.fs, P, V.tb_item, V.tb_curve, V.fault, t0, V.tb_items, ts);
   if (QUEUED && close) {
     AW_BARRIER();
     if constexpr (!std::is_void<PM>::value) if (pm) pm->mark(15);

@@ -64,6 +64,10 @@ struct FFParams {
   // fragment memory
   double k_frag_mem, fm_sigma_exp;
   int fm_min_sep, fm_max_sep;
+  // fragment-memory table mode ([Fragment_Memory_Table], fb.cpp:310-320):
+  // curves on the grid r = tb_rmin + ir tb_dr, ir < tb_size
+  int fm_tb_flag, fm_use_table, tb_size, pad_tb;
+  double tb_rmin, tb_rmax, tb_dr, tb_well_width;
   // excluded volume pair style (src/pair_excluded_volume.cpp)
   int ev_on, ev_p;
   double ev_lambda[6][6], ev_cut_short[6][6], ev_cut_long[6][6];
@@ -168,4 +172,12 @@ struct ReplicaDesc {
   double *sh_x;                // [2][n][3]: CA and CB positions of the current evaluation
   double *sh_acc;              // [2][n][3] forces of the chunks finished elsewhere, then their energy
   int sh_n, sh_q;              // chunks; position in the batch's list of sharing replicas
+  // fragment-memory table mode: items of 8 ints (i, j, the curve ids of the
+  // four atom pairs: -1 none, -2 zeros, -3 skipped, 0, 0), curves of
+  // tb_size (energy, force) pairs in the pair arithmetic's type
+  int tb_items, pad_tb;
+  const int *tb_item;
+  const void *tb_curve;
+  int *fault;                  // device error word: bit 0 an FM table distance outside [rmin, rmax]
 };
+#define AW_FAULT_TB_RANGE 1

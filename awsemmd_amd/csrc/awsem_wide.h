@@ -328,6 +328,9 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   __syncthreads();
   // energies leave a launch from the replica's last evaluation only
   bool ef = s < 0 || s == R.nsteps - 1;
+  // table mode: its pairs add into the replica's forces directly
+  if (P->fm_tb_flag && ON(P, AW_ET_FRAGMEM) && R.tb_items > 0)
+    E.fm += fm_table_run<T>(V.x(), V.n, V.fb, V.fs, P, V.tb_item, V.tb_curve, V.fault, W_TID, R.tb_items, W_NT);
   int kk = k0 + (int)threadIdx.x, ks = (int)blockDim.x;
   // inlined, unpipelined, first batches of 8 memories: a lane has a few
   // items, and the kernel keeps more waves per SIMD with the smaller

@@ -40,6 +40,8 @@ extern "C" {
 #define AWSEM_E_NOMEM (-8)        /* allocation / HBM budget exceeded               */
 #define AWSEM_E_NONFINITE (-9)    /* NaN/Inf coordinates or forces                  */
 #define AWSEM_E_STATE (-10)       /* call out of order                              */
+#define AWSEM_E_RANGE (-11)       /* FM table: a distance outside [rmin, rmax]      */
+                                  /* (fb.cpp:4419-4427; the reference aborts)       */
 
 /* energy slots: FixBackbone::EnergyTerms order (fb.h:331-332) + bond style */
 enum awsem_energy_slot {
