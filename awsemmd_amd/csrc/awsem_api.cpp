@@ -76,7 +76,7 @@ struct DevBatch {
   AwBatchPtrs ptrs() const
   {
     AwBatchPtrs p;
-    p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds;
+    p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds; p.table = P.fm_tb_flag;
     p.params = d_params; p.queue = d_queue; p.order = d_order; p.prof = d_prof;
     p.nshare = nshare; p.share_rep = d_share_rep; p.share_n = d_share_n; p.share_word = d_share_word;
     return p;

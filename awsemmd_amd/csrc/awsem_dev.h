@@ -334,7 +334,7 @@ __device__ void share_help(const BatchDev &B, double *smem)
   }
 }
 
-template <class T, bool LDSM>
+template <class T, bool LDSM, bool TB>
 __device__ __forceinline__ void force_eval_block(const View<T> &V, const ReplicaDesc &R, unsigned long long *prof,
                                                  bool first, bool ef)
 {
@@ -363,7 +363,7 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
   int nparts = prof ? 3 : 1;
   for (int q = 0; q < nparts; ++q) {
     PairMarks pm{prof, first, t};
-    phase_B<T, true, PairMarks, LDSM>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr, ef);
+    phase_B<T, true, PairMarks, LDSM, TB>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr, ef);
     __syncthreads();
     prof_mark(prof, first, prof ? 8 + q : 10, t);
   }
@@ -378,7 +378,8 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
   prof_mark(prof, first, 5, t);
 }
 
-template <class T, bool LDSM, int NT>
+// TB: the instance with the fragment-memory table pass (table mode only)
+template <class T, bool LDSM, int NT, bool TB>
 __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__restrict__ Pr, LgParams lg, int mode)
 {
   static_assert(NT == 64 * AW_SHARE_TAKE, "a helping workgroup evaluates one claimed chunk per wave");
@@ -446,7 +447,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
         prof_mark(B.prof, first, 6, t);
       }
       // energies leave the launch from its last evaluation only
-      force_eval_block<T, LDSM>(V, R, B.prof, first, s == nst - 1);
+      force_eval_block<T, LDSM, TB>(V, R, B.prof, first, s == nst - 1);
       // a replica whose wait gave up evaluates its own fragment memory from
       // then on (the launch is reported as failed)
       if (V.sh_n > 0 && !share_finish(V, R, s == nst - 1, B.queue + 32, &sh_ok)) V.sh_n = 0;
@@ -483,7 +484,9 @@ static void prepare_resident_t()
   int dev = 0, lim = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
-  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lim - AW_LDS_STATIC_RESERVE);
+  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             lim - AW_LDS_STATIC_RESERVE);
 }
 
@@ -504,9 +507,12 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   if (p.lds_resident) {
     // the dynamic-LDS limit is raised once, outside any graph capture
     // (aw_resident_prepare)
-    hipLaunchKernelGGL((k_resident<T, true, 512>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+    if (p.table) hipLaunchKernelGGL((k_resident<T, true, 512, true>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+    else hipLaunchKernelGGL((k_resident<T, true, 512, false>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+  } else if (p.table) {
+    hipLaunchKernelGGL((k_resident<T, false, 512, true>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   } else {
-    hipLaunchKernelGGL((k_resident<T, false, 512>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
+    hipLaunchKernelGGL((k_resident<T, false, 512, false>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

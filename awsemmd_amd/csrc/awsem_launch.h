@@ -9,6 +9,8 @@ struct AwBatchPtrs {
   int max_n = 0;                  // largest replica (residues)
   int fp32 = 0;                   // pair arithmetic in FP32 (else FP64)
   int lds_resident = 0;           // resident engine keeps replicas in LDS
+  int table = 0;                  // fragment-memory table mode (the resident engine's
+                                  // instance with the table pass; the others have none)
   const void *params = nullptr;   // FFParams on the device
   int *queue = nullptr;           // device scratch: [0] work-queue counter, [16] replicas finished,
                                   // [32] sharing error flag, [48] chunks evaluated elsewhere (the

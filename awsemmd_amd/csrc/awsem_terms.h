@@ -1926,8 +1926,9 @@ AW_NOINLINE double fm_table_run(const double *x, int n, double *fb, int fs, Para
 // pass, k = 12 scan, 13 DSSP, 14 P_AP, 15 excluded volume
 // XLDS: the working slab (coordinates, forces) is in LDS; ef: this
 // evaluation's energies are reported (fragment memory skips its energy sums
-// otherwise)
-template <class T, bool QUEUED, class PM = void, bool XLDS = false>
+// otherwise); TB: the fragment-memory table pass is compiled in (an
+// instance without it has no call in its pair pass)
+template <class T, bool QUEUED, class PM = void, bool XLDS = false, bool TB = true>
 AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr, bool ef = true)
 {
   EAcc E;
@@ -2004,7 +2005,7 @@ AW_DEV void phase_B(const View<T> &V, int t0, int ts, int part, PM *pm = nullptr
   if (ef) eflush(E, V.ered);
   E.clear();
   run_fm();
-  if (P->fm_tb_flag && ON(P, AW_ET_FRAGMEM) && (part & 1) && V.tb_items > 0)
+  if (TB && P->fm_tb_flag && ON(P, AW_ET_FRAGMEM) && (part & 1) && V.tb_items > 0)
     E.fm += fm_table_run<T>(V.x(), V.n, V.fb, V.fs, P, V.tb_item, V.tb_curve, V.fault, t0, V.tb_items, ts);
   if (QUEUED && close) {
     AW_BARRIER();
