@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,8 @@ struct DevBatch {
   ReplicaDesc *d_rep = nullptr;
   void *d_params = nullptr;
   int *d_queue = nullptr, *d_order = nullptr, *d_count = nullptr;
+  int *d_fault = nullptr;  // error word of every replica (table mode: AW_FAULT_TB_RANGE)
+  bool has_table = false;  // some replica evaluates table pairs
   double *d_x = nullptr, *d_v = nullptr, *d_f = nullptr, *d_e = nullptr;  // concatenated state
   long long *d_list = nullptr;
   unsigned long long *d_prof = nullptr;  // phase clock (AWSEM_PHASE_PROF=1)
@@ -65,7 +68,7 @@ struct DevBatch {
   ~DevBatch()
   {
     if (graph) (void)hipGraphExecDestroy(graph);
-    void *ptrs[] = {d_rep, d_params, d_queue, d_order, d_count, d_x, d_v, d_f, d_e, d_list, d_prof, d_share};
+    void *ptrs[] = {d_rep, d_params, d_queue, d_order, d_count, d_x, d_v, d_f, d_e, d_list, d_prof, d_share, d_fault};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     sys.clear();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -99,23 +102,19 @@ struct DevBatch {
 
 static AwStatus launch_seq(DevBatch &D, const AwRun &run, int engine, int nst);
 
-// error words the kernels raised during the last launch (table mode: a
-// distance outside the table's range), read and cleared; after the
-// launch's synchronisation
+// the batch's error word (every replica's kernels raise their bits in it;
+// table mode: a distance outside the table's range), read after the
+// launch's synchronisation and cleared when set
 static AwStatus take_faults(DevBatch &D)
 {
+  if (!D.has_table) return AwStatus();
   int any = 0;
-  for (const ReplicaDesc &R : D.reps) {
-    if (R.tb_items <= 0 || !R.fault) continue;
-    int w = 0;
-    if (hipMemcpy(&w, R.fault, sizeof w, hipMemcpyDeviceToHost) != hipSuccess)
-      return AwStatus::err(AWSEM_E_DEVICE, "reading the error word");
-    if (w) {
-      any |= w;
-      int z = 0;
-      if (hipMemcpy(R.fault, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess)
-        return AwStatus::err(AWSEM_E_DEVICE, "clearing the error word");
-    }
+  if (hipMemcpy(&any, D.d_fault, sizeof any, hipMemcpyDeviceToHost) != hipSuccess)
+    return AwStatus::err(AWSEM_E_DEVICE, "reading the error word");
+  if (any) {
+    int z = 0;
+    if (hipMemcpy(D.d_fault, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess)
+      return AwStatus::err(AWSEM_E_DEVICE, "clearing the error word");
   }
   if (any & AW_FAULT_TB_RANGE) return AwStatus::err(AWSEM_E_RANGE, "Table Fragment Memory: r is out of computed range.");
   return AwStatus();
@@ -270,22 +269,31 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   HIPCHK(hipMemset(D->d_f, 0, st));
   HIPCHK(hipMalloc((void **)&D->d_e, (size_t)D->nrep * AW_N_ET * sizeof(double)));
   HIPCHK(hipMemset(D->d_e, 0, (size_t)D->nrep * AW_N_ET * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&D->d_fault, 256));
+  HIPCHK(hipMemset(D->d_fault, 0, 256));
   std::vector<std::vector<int>> fm_ij(D->nrep);
   // replicas of one system (the same fragment-memory library, byte for
   // byte) read a single copy of its tables: one stream of them through the
   // caches instead of one per replica
+  // (a context given more than once is found by address, and its later
+  // replicas carry no library of their own; separate contexts with equal
+  // libraries by a hash of the packed bytes)
   struct FmLib { size_t bytes; uint64_t h1, h2; int n, rep; };
   std::vector<FmLib> libs;
+  std::unordered_map<const AwSystem *, int> first_of;
   for (int r = 0; r < D->nrep; ++r) {
     const AwSystem &S = *sys[r];
+    auto seen = first_of.find(&S);
+    int same = seen == first_of.end() ? -1 : seen->second;
+    if (same < 0) first_of[&S] = r;
     AwDevPlan plan;
     // a global working slab for every replica: the wide engine, and
     // replicas too large for LDS, work from it
-    AwStatus s = aw_devsys_plan(S, nullptr, D->fp32, want_lds || !D->lds, false, plan);
+    AwStatus s = aw_devsys_plan(S, nullptr, D->fp32, want_lds || !D->lds, false, plan, same < 0);
     if (!s.ok()) return s;
-    int src = -1;
+    int src = same;
     FmLib lib{plan.off_nl - plan.off_items, 0, 0, S.n, r};
-    if (plan.n_items > 0) {
+    if (same < 0 && plan.n_items + plan.n_tb_items > 0) {
       bytes_hash2(plan.image.data() + plan.off_items, lib.bytes, lib.h1, lib.h2);
       for (const FmLib &L : libs)
         if (L.bytes == lib.bytes && L.h1 == lib.h1 && L.h2 == lib.h2 && L.n == lib.n) { src = L.rep; break; }
@@ -295,14 +303,17 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
     if (!(s = aw_devsys_create(plan, *ds)).ok()) return s;
     ReplicaDesc R = ds->d;
     if (src >= 0) {
-      R.items = D->reps[src].items;
-      R.rf = D->reps[src].rf;
-      R.coef = D->reps[src].coef;
+      const ReplicaDesc &Q = D->reps[src];
+      R.items = Q.items; R.rf = Q.rf; R.coef = Q.coef; R.fm_items = Q.fm_items;
+      R.tb_item = Q.tb_item; R.tb_curve = Q.tb_curve; R.tb_items = Q.tb_items;
+      if (same >= 0 && want_lds) plan.fm_ij = fm_ij[src];
     }
     R.x = D->d_x + 9LL * D->res_off[r];
     R.v = D->d_v + 9LL * D->res_off[r];
     R.f = D->d_f + 9LL * D->res_off[r];
     R.energy = D->d_e + (long long)r * AW_N_ET;
+    R.fault = D->d_fault;
+    D->has_table = D->has_table || R.tb_items > 0;
     D->reps.push_back(R);
     D->sys.push_back(std::move(ds));
     if (want_lds) fm_ij[r] = std::move(plan.fm_ij);

@@ -350,6 +350,15 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
   __syncthreads();
   if (V.sh_n > 0) share_publish(R, ef);
   prof_mark(prof, first, 0, t);
+  // table mode: the table pass runs here, where few registers are live (its
+  // call inside the pair pass costs spills there); forces are zeroed
+  if constexpr (TB) {
+    fm_table_pass(V, t0, ts, ef);
+    if (prof) {
+      __syncthreads();
+      prof_mark(prof, first, 8, t);
+    }
+  }
   if (V.nl) {
     bool build = nl_needs_build(V);
     if (build) phase_nl_build(V, t0, ts);
@@ -363,7 +372,7 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
   int nparts = prof ? 3 : 1;
   for (int q = 0; q < nparts; ++q) {
     PairMarks pm{prof, first, t};
-    phase_B<T, true, PairMarks, LDSM, TB>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr, ef);
+    phase_B<T, true, PairMarks, LDSM, false>(V, t0, ts, prof ? (1 << q) : 7, prof ? &pm : nullptr, ef);
     __syncthreads();
     prof_mark(prof, first, prof ? 8 + q : 10, t);
   }

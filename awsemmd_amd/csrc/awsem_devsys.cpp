@@ -39,12 +39,13 @@ static void put(std::vector<unsigned char> &img, size_t off, const std::vector<T
   if (!v.empty()) std::memcpy(img.data() + off, v.data(), v.size() * sizeof(T));
 }
 
-AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool scratch, bool own_state, AwDevPlan &plan)
+AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool scratch, bool own_state, AwDevPlan &plan,
+                        bool lib)
 {
   plan = AwDevPlan();
   plan.fp32 = fp32; plan.scratch = scratch; plan.own_state = own_state;
   const FFParams &FP = P ? *P : S.P;
-  if (FP.fm_flag) {
+  if (FP.fm_flag && lib) {
     AwStatus s = aw_build_fm(S, FP, plan.fm);
     if (!s.ok()) return s;
   }
@@ -69,10 +70,10 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   // table mode: items and curves (inside the fragment-memory range that
   // identical replicas share), curves as (energy, force) pairs in T
   std::vector<int> tb_items;
-  if (FP.fm_tb_flag) aw_fm_table_items(S, tb_items);
-  plan.n_tb_items = (int)(tb_items.size() / 8);
+  if (FP.fm_tb_flag && lib) aw_fm_table_items(S, tb_items);
+  plan.n_tb_items = (int)(tb_items.size() / 4);
   plan.off_tbi = take(cur, tb_items.size() * sizeof(int));
-  plan.off_tbc = take(cur, 2 * S.fmt.e.size() * tsz);
+  plan.off_tbc = take(cur, lib ? 2 * S.fmt.e.size() * tsz : 0);
   plan.off_nl = take(cur, (AW_NL_HDR + 6 * (size_t)plan.nl_cap) * sizeof(int));
   plan.off_nlx = take(cur, 4 * (size_t)n * sizeof(double));
   // cell-list table for large systems (the wide engine builds their lists
@@ -112,7 +113,7 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
     put(img, plan.off_coef, coef);
   }
   put(img, plan.off_tbi, tb_items);
-  if (!S.fmt.e.empty()) {
+  if (lib && !S.fmt.e.empty()) {
     const AwFMTable &T = S.fmt;
     if (fp32) {
       std::vector<float> c(2 * T.e.size());

@@ -84,9 +84,13 @@ AwStatus aw_build_fm(const AwSystem &S, const FFParams &P, AwFMPack &B);
 AwStatus aw_build_fm_table(AwSystem &S);
 AwStatus aw_read_fm_table(const std::string &dir, AwSystem &S);
 AwStatus aw_write_fm_table(const std::string &dir, const AwSystem &S);
-// the pair path's table items (fb.cpp:7513-7527): per pair (i, j) whose
-// CA-CA curve exists, i, j and the curve ids of its four atom pairs, -3
-// for an atom pair skipped by a target glycine CB
+// the pair path's table items (fb.cpp:7513-7527), one per atom pair that
+// table_fragment_memory evaluates (fb.cpp:4325-4429), 4 ints each: i, j,
+// the atom pair q (0 CA-CA, 1 CA(i)-CB(j), 2 CB(i)-CA(j), 3 CB-CB) and its
+// curve id, -1 for a range check only (a curve of zeros, or a missing curve:
+// then the pair's later atom pairs are not listed, the reference returns
+// there).  Pairs whose CA-CA curve is missing, and atom pairs skipped by a
+// target glycine CB, have no item.
 void aw_fm_table_items(const AwSystem &S, std::vector<int> &items);
 // the device form of a pack in precision T: work items with the width
 // factor sc = sqrt(c2) rounded to T, and the Rf entries pre-multiplied by
@@ -166,8 +170,12 @@ struct AwDevSystem {
   AwDevSystem(const AwDevSystem &) = delete;
   AwDevSystem &operator=(const AwDevSystem &) = delete;
 };
-// P: the force field to pack with (S.P when null)
-AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool scratch, bool own_state, AwDevPlan &plan);
+// P: the force field to pack with (S.P when null); lib = false leaves the
+// fragment-memory library (items, distances, coefficients, table items and
+// curves) out of the image: the caller points the descriptor at another
+// replica's copy of the same system's library
+AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool scratch, bool own_state, AwDevPlan &plan,
+                        bool lib = true);
 // allocates the device image and uploads plan.image (the plan's host copy)
 AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out);
 AwStatus upload_raw(void **dst, const void *src, size_t bytes);

@@ -848,9 +848,13 @@ void aw_fm_table_items(const AwSystem &S, std::vector<int> &items)
         if (T.cid[base] == -1) continue;
         bool gi = S.se[i] == 'G', gj = S.se[j] == 'G';
         const bool skip[4] = {false, gj, gi, gi || gj};
-        int it[8] = {i, j, 0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < 4; ++k) it[2 + k] = skip[k] ? -3 : T.cid[base + k];
-        items.insert(items.end(), it, it + 8);
+        for (int k = 0; k < 4; ++k) {
+          if (skip[k]) continue;
+          int c = T.cid[base + k];
+          const int it[4] = {i, j, k, c >= 0 ? c : -1};
+          items.insert(items.end(), it, it + 4);
+          if (c == -1) break;
+        }
       }
     }
   }

@@ -1864,57 +1864,98 @@ AW_DEV void pair_B(const View<T> &V, int i, int j, EAcc &E, AW_G int *qd = nullp
   }
 }
 
-// fragment-memory table mode, one item = one residue pair (i, j) of the pair
-// path's loop (table_fragment_memory, fb.cpp:4325-4429): for each atom pair
-// k (CA-CA, CA(i)-CB(j), CB(i)-CA(j), CB-CB) not skipped by a target
-// glycine, the FP64 distance r must lie in [rmin, rmax] (else the error word
-// is set and the pair ends: the reference aborts); a missing curve ends the
-// pair, a curve of zeros contributes nothing; otherwise energy and force
-// factor are interpolated linearly between the grid points around r, and
-// f_i += ff dx, f_j -= ff dx.  Items k, k + ks, ... (< k1) of one lane;
-// coordinates x[3][n][3], forces at fb + 3 (kind fs + i) (the view's force
+// fragment-memory table mode (table_fragment_memory, fb.cpp:4325-4429 over
+// the pair path's loop 7513-7527), one item = one atom pair (i, j, q, c) of
+// that loop (q: 0 CA-CA, 1 CA(i)-CB(j), 2 CB(i)-CA(j), 3 CB-CB; glycine
+// skips and the atom pairs after a missing curve are left out when the
+// items are built): the FP64 distance r must lie in [rmin, rmax], else the
+// error word is set (the reference aborts); c < 0 (a curve of zeros or a
+// missing one) is a range check only; otherwise energy and force factor are
+// interpolated linearly between the grid points around r, and
+// f_i += ff dx, f_j -= ff dx.  Items k, k + ks, ... (< k1) of one lane,
+// TB_NB at a time: their curve loads are issued together, so a lane waits
+// for one memory latency per batch instead of one per atom pair.
+// Coordinates x[3][n][3], forces at fb + 3 (kind fs + i) (the view's force
 // base and stride); returns the energy.  A function of its own (not
 // inlined): table mode is rare, and its registers stay out of the passes
 // around its call.
+struct alignas(16) TbItem { int i, j, q, c; };
+
 template <class T>
 AW_NOINLINE double fm_table_run(const double *x, int n, double *fb, int fs, ParamPtr P, const AW_G int *items,
                                 const void *curve, AW_G int *fault, int k, int k1, int ks)
 {
+  typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
+  constexpr int TB_NB = 4;
   const double rmin = P->tb_rmin, rmax = P->tb_rmax, dr = P->tb_dr;
   const int tbs = P->tb_size;
-  const int ki[4] = {K_CA, K_CA, K_CB, K_CB}, kj[4] = {K_CA, K_CB, K_CA, K_CB};
+  const AW_G TbItem *it4 = (const AW_G TbItem *)items;
+  const T2 *cv2 = (const T2 *)curve;
   double e = 0.0;
-  for (; k < k1; k += ks) {
-    const AW_G int *it = items + 8 * k;
-    int i = it[0], j = it[1];
-    const int cs[4] = {it[2], it[3], it[4], it[5]};
-    for (int q = 0; q < 4; ++q) {
-      int c = cs[q];
-      if (c == -3) continue;
-      const double *a = x + (ki[q] * n + i) * 3, *b = x + (kj[q] * n + j) * 3;
-      double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
-      double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-      if (!(r >= rmin && r <= rmax)) {
-#ifndef AW_HOST_EMU
-        atomicOr((int *)fault, AW_FAULT_TB_RANGE);
-#endif
-        break;
+  int bad = 0;
+  for (; k < k1; k += TB_NB * ks) {
+    int ia[TB_NB], ja[TB_NB];
+    double d[TB_NB][3];
+    T tw[TB_NB];
+    T2 lo[TB_NB], hi[TB_NB];
+    bool use[TB_NB];
+#pragma unroll
+    for (int u = 0; u < TB_NB; ++u) {
+      int kk = k + u * ks;
+      use[u] = false;
+      if (kk < k1) {
+        const AW_G TbItem &w = it4[kk];
+        int wi = w.i, wj = w.j, wq = w.q, wc = w.c;
+        int ka = wq >> 1, kb = wq & 1;  // K_CA = 0, K_CB = 1
+        const double *a = x + (ka * n + wi) * 3, *b = x + (kb * n + wj) * 3;
+        d[u][0] = a[0] - b[0]; d[u][1] = a[1] - b[1]; d[u][2] = a[2] - b[2];
+        double r = sqrt(d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2]);
+        ia[u] = ka * fs + wi;
+        ja[u] = kb * fs + wj;
+        if (!(r >= rmin && r <= rmax)) {
+          bad = 1;
+        } else if (wc >= 0) {
+          int ir = (int)((r - rmin) / dr);
+          ir = ir < tbs - 2 ? ir : tbs - 2;
+          tw[u] = (T)((r - (rmin + (double)ir * dr)) / dr);
+          const T2 *cv = cv2 + (size_t)wc * tbs + ir;
+          lo[u] = cv[0];
+          hi[u] = cv[1];
+          use[u] = true;
+        }
       }
-      if (c == -1) break;
-      if (c == -2) continue;
-      int ir = (int)((r - rmin) / dr);
-      ir = ir < tbs - 2 ? ir : tbs - 2;
-      T t = (T)((r - (rmin + (double)ir * dr)) / dr);
-      const T *cv = (const T *)curve + 2 * ((size_t)c * tbs + ir);
-      T e1 = cv[0], f1 = cv[1], e2 = cv[2], f2 = cv[3];
-      T ve = e1 + (e2 - e1) * t, ff = f1 + (f2 - f1) * t;
+    }
+#pragma unroll
+    for (int u = 0; u < TB_NB; ++u) {
+      if (!use[u]) continue;
+      T ve = lo[u].x + (hi[u].x - lo[u].x) * tw[u], ff = lo[u].y + (hi[u].y - lo[u].y) * tw[u];
       e += (double)ve;
-      double *pi = fb + (ki[q] * fs + i) * 3, *pj = fb + (kj[q] * fs + j) * 3;
-      aadd(pi + 0, (double)(ff * (T)d0)); aadd(pi + 1, (double)(ff * (T)d1)); aadd(pi + 2, (double)(ff * (T)d2));
-      aadd(pj + 0, -(double)(ff * (T)d0)); aadd(pj + 1, -(double)(ff * (T)d1)); aadd(pj + 2, -(double)(ff * (T)d2));
+      double fx = (double)(ff * (T)d[u][0]), fy = (double)(ff * (T)d[u][1]), fz = (double)(ff * (T)d[u][2]);
+      double *pi = fb + ia[u] * 3, *pj = fb + ja[u] * 3;
+      aadd(pi + 0, fx); aadd(pi + 1, fy); aadd(pi + 2, fz);
+      aadd(pj + 0, -fx); aadd(pj + 1, -fy); aadd(pj + 2, -fz);
     }
   }
+#ifndef AW_HOST_EMU
+  if (bad) atomicOr((int *)fault, AW_FAULT_TB_RANGE);
+#endif
   return e;
+}
+
+// the table pass over one replica's items by the lanes t0, t0 + ts, ...;
+// its energy into the fragment-memory accumulator when reported (ef)
+template <class T>
+AW_DEV void fm_table_pass(const View<T> &V, int t0, int ts, bool ef)
+{
+  ParamPtr P = V.P;
+  if (!(P->fm_tb_flag && ON(P, AW_ET_FRAGMEM) && V.tb_items > 0)) return;
+  double e = fm_table_run<T>(V.x(), V.n, V.fb, V.fs, P, V.tb_item, V.tb_curve, V.fault, t0, V.tb_items, ts);
+  if (ef) {
+    EAcc E;
+    E.clear();
+    E.fm = e;
+    eflush(E, V.ered);
+  }
 }
 
 // part: bit 0 fragment memory, bit 1 local terms, bit 2 residue pairs.

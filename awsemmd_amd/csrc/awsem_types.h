@@ -172,8 +172,8 @@ struct ReplicaDesc {
   double *sh_x;                // [2][n][3]: CA and CB positions of the current evaluation
   double *sh_acc;              // [2][n][3] forces of the chunks finished elsewhere, then their energy
   int sh_n, sh_q;              // chunks; position in the batch's list of sharing replicas
-  // fragment-memory table mode: items of 8 ints (i, j, the curve ids of the
-  // four atom pairs: -1 none, -2 zeros, -3 skipped, 0, 0), curves of
+  // fragment-memory table mode: one item of 4 ints per evaluated atom pair
+  // (i, j, atom pair 0..3, curve id or -1: range check only), curves of
   // tb_size (energy, force) pairs in the pair arithmetic's type
   int tb_items, pad_tb;
   const int *tb_item;

@@ -283,14 +283,21 @@ def arrays(sys: System) -> dict:
 
 
 def coeff_text(fm: bool = True, mem_file: str = "fragsLAMW.mem", gamma_file: str = "uniform.gamma",
-               base: Optional[str] = None) -> str:
-    """The shipped coeff file with the [Fragment_Memory] section enabled."""
+               base: Optional[str] = None, table: Optional[Sequence[float]] = None) -> str:
+    """The shipped coeff file with the [Fragment_Memory] section enabled, or
+    with table=(rmin, rmax, dr, well_width, use_table, sigma_exp) the
+    [Fragment_Memory_Table] section in its place (same strength, memories
+    and gamma file; fix_backbone.cpp:310-320 reads these lines)."""
     base = base or os.path.join(PARAM_DIR, "fix_backbone_coeff.data")
     txt = open(base).read()
     if fm:
-        txt = txt.replace("[Fragment_Memory]-\n1.0\n1r69.mem\n1r69.gamma",
-                          "[Fragment_Memory]\n1.0\n%s\n%s" % (mem_file, gamma_file))
-        assert "[Fragment_Memory]\n1.0\n" in txt
+        sec = "[Fragment_Memory]\n1.0\n%s\n%s" % (mem_file, gamma_file)
+        if table is not None:
+            rmin, rmax, dr, ww, use, sexp = table
+            sec = "[Fragment_Memory_Table]\n1.0\n%s\n%s\n%r %r %r\n%r\n%d\n%r" % (
+                mem_file, gamma_file, rmin, rmax, dr, ww, int(use), sexp)
+        txt = txt.replace("[Fragment_Memory]-\n1.0\n1r69.mem\n1r69.gamma", sec)
+        assert sec in txt
     return txt
 
 
@@ -313,15 +320,17 @@ def write_gro(path: str, seq: str, struct_nm: np.ndarray) -> None:
         f.write("   0.00000   0.00000   0.00000\n")
 
 
-def write_system(sys: System, workdir: str, fm: bool = True, coeff_base: Optional[str] = None) -> dict:
-    """Writes the reference file set for `sys` into workdir; returns paths."""
+def write_system(sys: System, workdir: str, fm: bool = True, coeff_base: Optional[str] = None,
+                 table: Optional[Sequence[float]] = None) -> dict:
+    """Writes the reference file set for `sys` into workdir; returns paths.
+    table: fragment memory in table mode (see coeff_text)."""
     os.makedirs(workdir, exist_ok=True)
     for name in ("anti_HB", "anti_NHB", "para_HB", "anti_one", "para_one", "gamma.dat",
                  "burial_gamma.dat", "uniform.gamma"):
         shutil.copyfile(os.path.join(PARAM_DIR, name), os.path.join(workdir, name))
     coeff = os.path.join(workdir, "fix_backbone_coeff.data")
     with open(coeff, "w") as f:
-        f.write(coeff_text(fm=fm, base=coeff_base))
+        f.write(coeff_text(fm=fm, base=coeff_base, table=table))
     seqf = os.path.join(workdir, "protein.seq")
     with open(seqf, "w") as f:
         f.write("\n".join(sys.chains) + "\n")

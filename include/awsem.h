@@ -177,7 +177,10 @@ int awsem_list(awsem_ctx *c, int kind, int arg, const double *xres, int64_t *out
  * in real units; not part of the reference) with all force terms of the
  * context, kept resident on the device for nsteps.  t_start -> t_stop ramps
  * linearly over the call.  xres/vres are updated in place; fres holds the
- * last force; energy the last step's per-term energies. */
+ * force array as LAMMPS leaves it after the last step: every force term at
+ * the final coordinates plus that step's fix langevin drag and random force
+ * (the force the next step's first half-kick uses); energy the per-term
+ * energies at the final coordinates. */
 typedef struct awsem_langevin {
   double dt;        /* fs */
   double t_start;   /* K */
@@ -199,12 +202,16 @@ int awsem_destroy(awsem_ctx *c);
 
 /* A batch owns device copies of every replica's system and state.  The
  * contexts may be destroyed after the call.  All replicas must share one
- * coeff parameter set. */
+ * coeff parameter set.  A context may be given more than once (replicas of
+ * one system): its fragment-memory library, or its table curves, are then
+ * stored on the device once. */
 int awsem_batch_create(int n_ctx, awsem_ctx *const *ctxs, int precision,
                        awsem_batch **out);
 int awsem_batch_size(const awsem_batch *b, int *n_replicas, int *n_residues_total);
 /* host <-> device state, residue layout per replica concatenated in
- * replica order ([3][n_r][3] blocks); any pointer may be NULL */
+ * replica order ([3][n_r][3] blocks); any pointer may be NULL.  f: after
+ * awsem_batch_compute the forces of x; after awsem_batch_run as awsem_run's
+ * fres (with the last step's Langevin drag and random force). */
 int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v);
 int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f,
                           double *energy /* [n_rep][AWSEM_N_ET] */);

@@ -167,3 +167,33 @@ def test_table_range_error():
             break
     print("server states", srv.poll(bad), srv.poll(good), flush=True)
     assert srv.poll(bad) == E_RANGE and srv.poll(good) == 2
+
+
+@pytest.mark.parametrize("engine", ["resident", "wide"])
+@pytest.mark.parametrize("table", [True, False])
+def test_batch_repeated_context(oracle_mod, engine, table):
+    """One context given three times (replicas of one protein): the later
+    replicas read the first one's fragment-memory library or table curves;
+    each replica, on its own perturbed coordinates, against the oracle."""
+    from conftest import make_case
+    A = _gpu()
+    s, p = make_table_case([45, 30], 58) if table else make_case([45, 30], 58)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP32)
+    rng = np.random.default_rng(8)
+    xs = [s.coords() + rng.normal(0.0, 0.05, size=s.coords().shape) for _ in range(3)]
+    b = A.Batch([fix, fix, fix], A.PREC_FP32)
+    b.set_engine(engine)
+    b.set_state(A.pack_states(xs))
+    b.compute()
+    _, _, f, e = b.get_state()
+    fs = b.split(f)
+    k = A.ET["Frag_Mem"]
+    for r in range(3):
+        fo, eo = o.compute(xs[r])
+        ferr = term_err(fs[r], fo, np.linalg.norm(fo.reshape(-1, 3), axis=1).max())
+        eerr = abs(e[r, 0] - eo[0]) / max(1.0, abs(eo[0]))
+        print(f"{engine} table={table} replica {r}: E_fm={e[r, k]:.9g} ref {eo[k]:.9g} eerr={eerr:.2e} ferr={ferr:.2e}",
+              flush=True)
+        assert abs(e[r, k] - eo[k]) / max(1.0, abs(eo[k])) <= TOL[1]
+        assert ferr <= TOL[1] and eerr <= TOL[1], r

@@ -233,3 +233,32 @@ def test_table_out_of_range_is_an_error(oracle_mod, tmp_path):
     x[0][9, 0] += 100.0
     with pytest.raises(RuntimeError, match="r is out of computed range"):
         o.compute(x)
+
+
+def test_table_converges_to_memory_mode(oracle_mod):
+    """With well width 1 the table curves are the memory-mode Gaussians
+    (fb.cpp:4279-4280 against 4126) sampled on the grid, so the table-mode
+    energy and forces converge to the memory-mode ones at second order in dr
+    (linear interpolation of smooth curves): the difference falls by about 4
+    for each halving of dr, and is small at dr = 0.005 A.  A lost
+    well-width factor, a shifted grid or a wrong interpolation weight breaks
+    the order or the limit."""
+    from conftest import make_case, make_table_case
+    k = oracle_mod.ET["Frag_Mem"]
+    fm = 1 << k
+    s, p = make_case([30], 61)
+    x = s.coords()
+    f_mem, e_mem = _o(oracle_mod, p).compute(x, fm)
+    diffs = []
+    for dr in (0.04, 0.02, 0.01, 0.005):
+        _, pt = make_table_case([30], 61, tb=(1.0, 30.0, dr), well_width=1.0)
+        f_tb, e_tb = _o(oracle_mod, pt).compute(x, fm)
+        de = abs(e_tb[k] - e_mem[k]) / abs(e_mem[k])
+        dfv = np.linalg.norm((f_tb - f_mem).reshape(-1, 3), axis=1).max() / np.linalg.norm(
+            f_mem.reshape(-1, 3), axis=1).max()
+        diffs.append((de, dfv))
+        print(f"dr={dr}: rel energy diff {de:.3e}, rel force diff {dfv:.3e}")
+    assert e_mem[k] < -10.0
+    for (e0, f0), (e1, f1) in zip(diffs, diffs[1:]):
+        assert 2.8 < e0 / e1 < 5.5 and 2.8 < f0 / f1 < 5.5
+    assert diffs[-1][0] < 1e-5 and diffs[-1][1] < 1e-4
