@@ -392,6 +392,13 @@ def _gen_multimer(seed):
     return W.make_system([300] * 8, seed=seed, compact=True)
 
 
+# phase-clock slots of the resident engine (AWSEM_PHASE_PROF=1); slot 4 is
+# unused (the xi scaling is part of F); in table mode the table pass is
+# clocked in B_fm
+PHASE_NAMES = ["A", "B", "C", "D", None, "F", "int1", "int2", "B_fm", "B_local", "B_pair", "list",
+               "pair_scan", "pair_dssp", "pair_pap", "pair_ev"]
+
+
 def run_fm_table(a, A, W):
     """Fragment-memory table mode on replicas of one deck-sized protein: a
     63-residue chain with 20 memories per 9-residue window, the shipped
@@ -416,8 +423,13 @@ def run_fm_table(a, A, W):
         t_ctx = time.perf_counter() - t0
         batch = A.Batch([fix] * a.table_replicas, A.PREC_FP32)
         batch.set_state(x0, v0)
+        clocked = bool(os.environ.get("AWSEM_PHASE_PROF"))
         batch.run(a.warmup, step0=0, **lg)
-        tuned = batch.autotune(probe_steps=160, step0=a.warmup, **lg)  # restores the state
+        if clocked or a.no_autotune:  # fixed settings (a clocked run counts every evaluation)
+            batch.set_engine("resident")
+            tuned = {"engine": "resident", "steps_per_launch": 160, "skin": 2.0}
+        else:
+            tuned = batch.autotune(probe_steps=160, step0=a.warmup, **lg)  # restores the state
         chunk = tuned["steps_per_launch"]
         step = a.warmup
         t0 = time.perf_counter()
@@ -428,6 +440,11 @@ def run_fm_table(a, A, W):
             done += k
         t = time.perf_counter() - t0
         e = batch.get_state()[3]
+        if clocked:
+            ticks = batch.phase_clock() / 100.0  # microseconds
+            ev = 1 + a.warmup + done  # force evaluations: the first launch's initial one, then one per step
+            log(f"{mode} mode, phase us/eval, mean replica: " + " ".join(
+                f"{nm}={v / ev / a.table_replicas:.1f}" for nm, v in zip(PHASE_NAMES, ticks[16:32]) if nm))
         res[mode] = dict(value=a.table_replicas * done / t, ms_per_step=1e3 * t / done, context_s=t_ctx,
                          engine=tuned["engine"], steps_per_launch=chunk, skin=tuned["skin"],
                          finite=bool(np.all(np.isfinite(e))))
@@ -596,9 +613,7 @@ def main():
 
     if os.environ.get("AWSEM_PHASE_PROF"):
         ticks = batch.phase_clock() / 100.0  # microseconds
-        # clock slots; slot 4 is unused (the xi scaling is part of F)
-        names = ["A", "B", "C", "D", None, "F", "int1", "int2", "B_fm", "B_local", "B_pair", "list",
-                 "pair_scan", "pair_dssp", "pair_pap", "pair_ev"]
+        names = PHASE_NAMES
         nsteps_all = (a.warmup + 1) + a.chunk + a.steps + (a.steps + a.chunk - 1) // a.chunk  # force evaluations
         log("phase us/eval, longest replica: " + " ".join(f"{n}={t / nsteps_all:.1f}" for n, t in zip(names, ticks[:16]) if n))
         log("phase us/eval, mean replica:    " + " ".join(f"{n}={t / nsteps_all / a.replicas:.1f}" for n, t in zip(names, ticks[16:32]) if n))
