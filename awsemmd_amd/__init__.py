@@ -44,7 +44,9 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_server_stop", "awsem_server_poll", "awsem_server_wait", "awsem_server_result",
            "awsem_server_release", "awsem_server_stats", "awsem_server_last_error", "awsem_server_destroy",
            "awsem_batch_destroy", "awsem_compute_res_noalloc", "awsem_batch_lists", "awsem_set_box",
-           "awsem_compute_image", "awsem_energy_log_open", "awsem_energy_log_write", "awsem_energy_log_close"]
+           "awsem_compute_image", "awsem_energy_log_open", "awsem_energy_log_write", "awsem_energy_log_close",
+           "awsem_deck_open", "awsem_deck_run", "awsem_deck_natoms", "awsem_deck_state",
+           "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy"]
 
 
 class AwsemError(RuntimeError):
@@ -56,6 +58,10 @@ class AwsemError(RuntimeError):
 class Langevin(C.Structure):
     _fields_ = [("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double), ("damp", C.c_double),
                 ("seed", C.c_uint64), ("step0", C.c_int64)]
+
+
+class DeckOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("quiet", C.c_int), ("max_steps", C.c_int64)]
 
 
 class ServerOpts(C.Structure):
@@ -126,6 +132,15 @@ def lib():
         L.awsem_server_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.awsem_server_destroy.argtypes = [vp]
         L.awsem_batch_destroy.argtypes = [vp]
+        i64p = C.POINTER(C.c_int64)
+        L.awsem_deck_open.argtypes = [C.c_char_p, C.POINTER(DeckOpts), C.POINTER(vp)]
+        L.awsem_deck_run.argtypes = [vp]
+        L.awsem_deck_natoms.argtypes = [vp, ip, ip]
+        L.awsem_deck_state.argtypes = [vp, i64p, dp, dp, dp, i64p]
+        L.awsem_deck_residue_atoms.argtypes = [vp, i64p]
+        L.awsem_deck_variable.argtypes = [vp, C.c_char_p, dp]
+        L.awsem_deck_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.awsem_deck_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -510,6 +525,84 @@ class Server:
         keys = ["rounds", "jobs_done", "bytes_in_use", "peak_active", "replica_steps", "device_ms", "budget_bytes",
                 "active"]
         return dict(zip(keys, s.tolist()))
+
+
+class Deck:
+    """A LAMMPS input deck of the FixBackbone path (awsem_deck_*): the script
+    is executed up to its first `run` when the object is made (no device
+    work), `run()` executes the rest on the device.  File names in the
+    script are relative to its directory, where energy.log and log.awsem
+    are written."""
+
+    def __init__(self, path: str, precision: int = PREC_FP64, max_steps: int = 0, quiet: bool = True):
+        o = DeckOpts(precision, 1 if quiet else 0, max_steps)
+        h = C.c_void_p()
+        rc = lib().awsem_deck_open(path.encode(), C.byref(o), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.last_error()
+            self.close()
+            raise AwsemError(rc, msg)
+
+    def last_error(self) -> str:
+        if not self.h:
+            return ""
+        buf = C.create_string_buffer(1024)
+        lib().awsem_deck_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AwsemError(rc, self.last_error())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().awsem_deck_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self):
+        self._check(lib().awsem_deck_run(self.h))
+
+    def natoms(self):
+        na, nr = C.c_int(), C.c_int()
+        self._check(lib().awsem_deck_natoms(self.h, C.byref(na), C.byref(nr)))
+        return na.value, nr.value
+
+    def state(self):
+        """(ids, x (n, 3), v (n, 3), energies, step), atoms in id order."""
+        na, _ = self.natoms()
+        ids = np.zeros(na, dtype=np.int64)
+        x, v, e = np.zeros((na, 3)), np.zeros((na, 3)), np.zeros(N_ET)
+        st = C.c_int64()
+        self._check(lib().awsem_deck_state(self.h, ids.ctypes.data_as(C.POINTER(C.c_int64)), _dp(x), _dp(v), _dp(e),
+                                           C.byref(st)))
+        return ids, x, v, e, st.value
+
+    def residue_atoms(self) -> np.ndarray:
+        """Atom ids of the residue layout slots, (3, n_res): CA, CB/H-beta, O."""
+        _, nr = self.natoms()
+        ids = np.zeros(3 * nr, dtype=np.int64)
+        self._check(lib().awsem_deck_residue_atoms(self.h, ids.ctypes.data_as(C.POINTER(C.c_int64))))
+        return ids.reshape(3, nr)
+
+    def residue_state(self):
+        """(x, v) in the residue layout (3, n_res, 3)."""
+        ids, x, v, _, _ = self.state()
+        pos = {int(i): k for k, i in enumerate(ids)}
+        slots = self.residue_atoms()
+        idx = np.vectorize(lambda i: pos[int(i)])(slots)
+        return x[idx], v[idx]
+
+    def variable(self, formula: str) -> float:
+        out = C.c_double()
+        self._check(lib().awsem_deck_variable(self.h, formula.encode(), C.byref(out)))
+        return out.value
 
 
 def pack_states(states: Sequence[np.ndarray]) -> np.ndarray:

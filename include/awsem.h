@@ -321,6 +321,54 @@ int awsem_server_stats(awsem_server *s, double *stats);
 int awsem_server_last_error(awsem_server *s, char *buf, size_t n);
 int awsem_server_destroy(awsem_server *s);
 
+/* ---- LAMMPS input decks (the commands the shipped decks use around
+ * fix backbone) ---------------------------------------------------------- */
+
+/* The subset of a LAMMPS input script that runs the FixBackbone path (the
+ * shipped decks, e.g. examples/examples_new/1R69_single_memory/folding.in):
+ * units real, dimension, timestep, boundary (non-periodic), neighbor /
+ * neigh_modify / atom_modify (accepted: lists are the engine's own),
+ * special_bonds fene, atom_style awsemmd, bond_style harmonic, pair_style
+ * vexcluded + pair_coeff (pev.cpp:175-265), read_data (atom_style awsemmd:
+ * id molecule residue type q x y z, src/atom_vec_awsemmd.cpp:56; Masses,
+ * Atoms, Velocities, Bond Coeffs, Bonds, Pair Coeffs), mass, bond_coeff,
+ * group (id / type / molecule lists and a:b ranges), velocity all create /
+ * zero, fix nve, fix langevin, fix backbone (fb.cpp:86-106), fix print,
+ * unfix, variable (equal / string / index; formulas with f_ID[k], emol,
+ * epair, ke, pe, temp, step ...), thermo, thermo_style one / custom, dump
+ * atom + dump_modify sort id, reset_timestep, print and run.  Anything
+ * else is refused with AWSEM_E_UNSUPPORTED naming the command and line.
+ * File names are relative to the script's directory, which is also the
+ * FixBackbone working directory: energy.log (fb.cpp:101-106, 7661-7672)
+ * and a thermo log, log.awsem, are written there.  The random streams of
+ * velocity create and fix langevin are this library's, not LAMMPS's. */
+typedef struct awsem_deck awsem_deck;
+typedef struct awsem_deck_opts {
+  int precision;       /* AWSEM_PREC_FP64 (0, default) or AWSEM_PREC_FP32 */
+  int quiet;           /* no thermo / print lines on stdout */
+  int64_t max_steps;   /* > 0: every run command runs at most this many steps */
+} awsem_deck_opts;
+/* Reads the script and executes it up to its first run command (no device
+ * work).  On an error *out still holds the deck, for awsem_deck_last_error;
+ * destroy it. */
+int awsem_deck_open(const char *input_path, const awsem_deck_opts *opts, awsem_deck **out);
+/* Executes the remaining commands: each run on the device (one replica of
+ * a batch, continued across the output intervals), with its outputs. */
+int awsem_deck_run(awsem_deck *d);
+int awsem_deck_natoms(const awsem_deck *d, int *n_atoms, int *n_residues);
+/* Atoms in ascending id order: ids[n], x[n][3], v[n][3]; the last
+ * evaluated energies [AWSEM_N_ET]; the current step.  Any pointer may be
+ * NULL. */
+int awsem_deck_state(const awsem_deck *d, int64_t *ids, double *x, double *v, double *energy,
+                     int64_t *step);
+/* Atom id of every (kind, residue) slot of the residue layout ([3][n_res]:
+ * CA, CB / H-beta, O), once fix backbone has been created. */
+int awsem_deck_residue_atoms(const awsem_deck *d, int64_t *ids);
+/* Value of an equal-style formula with the deck's current state. */
+int awsem_deck_variable(awsem_deck *d, const char *formula, double *value);
+int awsem_deck_last_error(const awsem_deck *d, char *buf, size_t n);
+int awsem_deck_destroy(awsem_deck *d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -989,6 +989,12 @@ int or_chain_of(const or_ctx *c, int i) { return c->chain_no[i] - 1; }
 char or_seq_at(const or_ctx *c, int i) { return c->se[i]; }
 double or_pair_list_cutoff(const or_ctx *c) { return c->pair_list_cutoff; }
 
+void or_set_bonds(or_ctx *c, double k, const double *r0)
+{
+  c->bond_k = k;
+  for (int q = 0; q < 5; ++q) c->bond_r0[q] = r0[q];
+}
+
 void or_set_vexcluded(or_ctx *c, int p, const double *lam, const double *cs, const double *clg)
 {
   c->ev_p = p;

@@ -77,6 +77,11 @@ double or_pair_list_cutoff(const or_ctx *c);
 void or_set_vexcluded(or_ctx *c, int p, const double *lambda6x6,
                       const double *cut_short6x6, const double *cut_long6x6);
 
+/* Harmonic bond style of a deck (bond_style harmonic + bond_coeff): one K
+ * and r0 for CA-CA, CA-O, O-CA(+1), CA-CB, CA-H-beta.  Defaults: the deck
+ * written by CoordinatesToWorkLammpsDataFile.py:289-294 (K 200). */
+void or_set_bonds(or_ctx *c, double k, const double *r0_5);
+
 /* Forces are ADDED into f* (pre_force semantics).  energy[OR_N_ET] is
  * overwritten.  term_mask selects energy slots (bit k = slot k) whose
  * forces and energies are accumulated; the coeff flags still decide which

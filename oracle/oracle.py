@@ -64,6 +64,7 @@ def lib():
         L.or_theta_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_pap_nu.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_set_vexcluded.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
+        L.or_set_bonds.argtypes = [C.c_void_p, C.c_double, dp]
         L.or_langevin_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_ulonglong, C.c_int, C.c_longlong, dp, dp, dp, dp]
         L.or_langevin_run.restype = C.c_int
@@ -105,6 +106,15 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def set_bonds(self, k: float, r0):
+        """bond_style harmonic + bond_coeff: K, and r0 of CA-CA, CA-O, O-CA(+1), CA-CB, CA-H-beta."""
+        lib().or_set_bonds(self.h, float(k), np.ascontiguousarray(r0, dtype=np.float64))
+
+    def set_vexcluded(self, p: int, lam, cut_short, cut_long):
+        """pair_style vexcluded p + pair_coeff: [6][6] tables by atom type (1 CA, 2 N, 3 O, 4 CB, 5 H-beta)."""
+        f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+        lib().or_set_vexcluded(self.h, int(p), f(lam), f(cut_short), f(cut_long))
 
     @property
     def pair_list_cutoff(self) -> float:
