@@ -64,7 +64,7 @@ def write_data(s, path):
 
 
 def deck_text(n, *, runs=(60,), thermo=20, dump=30, print_every=20, timestep=2.0, langevin=(500.0, 300.0, 100.0, 3456),
-              velocity=(300.0, 2349852), ranges=True, extra=""):
+              velocity=(300.0, 2349852), ranges=True, extra="", data="data.sys", seq="protein.seq"):
     ca, cb, o = atom_ids(n)
     if ranges:
         g = [f"group alpha_carbons id 1:{3 * n}:3", f"group beta_atoms id 3:{3 * n}:3", f"group oxygens id 2:{3 * n}:3"]
@@ -74,7 +74,7 @@ def deck_text(n, *, runs=(60,), thermo=20, dump=30, print_every=20, timestep=2.0
     t = ["# synthetic deck in the grammar of the shipped examples", "units real", f"timestep {timestep}",
          "dimension 3", "boundary s s s", "neighbor 5 bin", "neigh_modify delay 5", "atom_modify sort 0 0.0",
          "special_bonds fene", "atom_style awsemmd", "bond_style harmonic", "pair_style vexcluded 2 3.5 3.5",
-         "read_data data.sys", "pair_coeff * * 0.0"]
+         f"read_data {data}", "pair_coeff * * 0.0"]
     t += ["pair_coeff %d %d %g %g %g" % p for p in PAIR]
     if velocity:
         t.append("velocity all create %g %d" % velocity)
@@ -82,7 +82,7 @@ def deck_text(n, *, runs=(60,), thermo=20, dump=30, print_every=20, timestep=2.0
     t.append("fix 1 all nve")
     if langevin:
         t.append("fix 11 all langevin %g %g %g %d  # ramped thermostat" % langevin)
-    t += ["fix 2 alpha_carbons backbone beta_atoms oxygens fix_backbone_coeff.data protein.seq",
+    t += [f"fix 2 alpha_carbons backbone beta_atoms oxygens fix_backbone_coeff.data {seq}",
           f"thermo {thermo}", f"dump 1 all atom {dump} dump.lammpstrj", "dump_modify 1 sort id",
           "variable E_bond  equal emol", "variable E_chain equal f_2[1]", "variable E_excl  equal epair",
           "variable E_chi   equal f_2[3]", "variable E_rama  equal f_2[4]", "variable E_dssp  equal f_2[6]",
@@ -112,9 +112,32 @@ def make_deck(lengths, seed, **kw):
     return s, path, d
 
 
-def deck_oracle(oracle_mod, d):
+FIXTURE_1R69 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "deck_1r69")
+N_1R69 = 63
+
+
+def deck_1r69(**kw):
+    """(deck path, directory) of the shipped 1R69 system (tests/golden/deck_1r69,
+    written by make_deck_1r69.py) with a deck of the example's settings:
+    timestep 5 fs, Langevin 500 -> 300 K with 1000 fs damping, seed 3456,
+    velocities at 300 K (seed 2349852), thermo and energy output every 1000
+    steps, a run of 2,000,000 steps (callers cap it)."""
+    import shutil
+    d = tempfile.mkdtemp(prefix="awsem_1r69_")
+    for f in os.listdir(FIXTURE_1R69):
+        shutil.copyfile(os.path.join(FIXTURE_1R69, f), os.path.join(d, f))
+    opts = dict(runs=(2000000,), thermo=1000, dump=2000, print_every=1000, timestep=5.0,
+                langevin=(500.0, 300.0, 1000.0, 3456), data="data.1r69", seq="extended.seq")
+    opts.update(kw)
+    path = os.path.join(d, "deck.in")
+    with open(path, "w") as f:
+        f.write(deck_text(N_1R69, **opts))
+    return path, d
+
+
+def deck_oracle(oracle_mod, d, seq="protein.seq"):
     """Oracle of a deck's directory with the deck's bond and pair coefficients."""
-    o = oracle_mod.Oracle(os.path.join(d, "fix_backbone_coeff.data"), os.path.join(d, "protein.seq"), d)
+    o = oracle_mod.Oracle(os.path.join(d, "fix_backbone_coeff.data"), os.path.join(d, seq), d)
     o.set_bonds(BOND_K, BOND_R0)
     lam, cs, cl = np.zeros((6, 6)), np.zeros((6, 6)), np.zeros((6, 6))
     for i, j, l, a, b in PAIR:

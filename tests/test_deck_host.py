@@ -138,3 +138,32 @@ def test_deck_refuses_foreign_data():
     with pytest.raises(A.AwsemError) as e:
         A.Deck(path)
     assert e.value.code == -4 and "bond" in str(e.value)
+
+
+def test_deck_1r69_setup(oracle_mod):
+    """The shipped 1R69 system (data file, coeff with fragment memory in table
+    mode, single native memory) sets up through the front end: 189 atoms of
+    63 residues in the data file's order (CA, O, CB / H-beta), the data
+    file's coordinates, and finite oracle energies there with the table term
+    present."""
+    A = _A()
+    path, d = deckgen.deck_1r69()
+    dk = A.Deck(path)
+    assert dk.natoms() == (189, 63)
+    ids = dk.residue_atoms()
+    assert np.array_equal(ids, np.array(deckgen.atom_ids(63)))
+    x, v = dk.residue_state()
+    rows = {}
+    txt = open(os.path.join(d, "data.1r69")).read()
+    atoms = txt.split("Atoms")[1].split("Bond Coeffs")[0].split("\n")
+    for ln in atoms:
+        w = ln.split()
+        if len(w) >= 8:
+            rows[int(w[0])] = [float(t) for t in w[5:8]]
+    want = np.array([[rows[int(i)] for i in ids[k]] for k in range(3)])
+    assert np.array_equal(x, want)
+    o = deckgen.deck_oracle(oracle_mod, d, seq="extended.seq")
+    f, e = o.compute(x)
+    k = oracle_mod.ET["Frag_Mem"]
+    print("1R69 energies at the data file's coordinates:", dict(zip(oracle_mod.ET_NAMES, np.round(e, 4))))
+    assert np.all(np.isfinite(e)) and e[k] < -1.0

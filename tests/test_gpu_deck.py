@@ -160,3 +160,28 @@ def test_deck_fp32_energies(oracle_mod):
     row = [float(t) for t in open(os.path.join(d, "energy")).read().splitlines()[-1].split()[1:]]
     print("E_P", row[-1], flush=True)
     check_energies(row, e_ref, 1e-5)
+
+
+def test_deck_1r69_shipped_system(oracle_mod):
+    """The shipped 1R69 system (fragment memory in table mode on the decks'
+    0-50 A grid, a single native memory) with the example's settings
+    (timestep 5 fs, Langevin 500 -> 300 K over the run, damping 1000 fs),
+    the run capped at 300 steps: trajectory and energy rows against the
+    oracle."""
+    A = _A()
+    path, d = deckgen.deck_1r69(thermo=100, print_every=100, dump=300)
+    dk = A.Deck(path, A.PREC_FP64, max_steps=300)
+    x0, v0 = dk.residue_state()
+    o = deckgen.deck_oracle(oracle_mod, d, seq="extended.seq")
+    ref = oracle_run(o, x0, v0, 0, 300, events_of(0, 300, [100, 300]), 500.0, 300.0, 1000.0, 3456, 5.0)
+    dk.run()
+    x, v = dk.residue_state()
+    dx = np.abs(x - ref[300][0]).max()
+    rows = open(os.path.join(d, "energy")).read().splitlines()[1:]
+    print(f"1R69, 300 steps of 5 fs: max |dx| {dx:.2e} A; E_P {rows[0].split()[-1]} -> {rows[-1].split()[-1]}",
+          flush=True)
+    assert dx < 1e-7
+    assert [int(r.split()[0]) for r in rows] == [0, 100, 200, 300]
+    for r in rows:
+        w = r.split()
+        check_energies([float(t) for t in w[1:]], ref[int(w[0])][2], 1e-6)
