@@ -346,3 +346,112 @@ def write_system(sys: System, workdir: str, fm: bool = True, coeff_base: Optiona
             for m in sys.memories:
                 f.write("mem%03d.gro %d %d %d %g\n" % (m.struct, m.pos + 1, m.fpos + 1, m.length, m.weight))
     return dict(coeff=coeff, seq=seqf, workdir=workdir)
+
+
+# ---------------------------------------------------------------------------
+# coeff-toggled terms (src/fix_backbone.cpp:297-358, 412-421, 466-478)
+
+EXTRA_TERMS = ("amhgo", "cont_rest", "vec_fm", "ssb", "membrane", "phosph", "dh")
+# residue radii of the solvent barrier in the shipped coeff file
+SSB_RSHIFT = (0.00, 2.04, 0.57, 0.57, 0.36, 1.11, 1.17, -1.52, 0.87, 0.67, 0.79, 1.47, 1.03, 1.00, -0.10, 0.26,
+              0.37, 1.21, 1.15, 0.39)
+
+
+def _sec(name: str, *rows) -> str:
+    out = [name]
+    for r in rows:
+        out.append(" ".join(str(v) for v in r) if isinstance(r, (list, tuple)) else str(r))
+    return "\n".join(out) + "\n\n"
+
+
+def extras_text(terms: Sequence[str], amhgo=(1.0, 2, 8.0, 0), cont_rest=(2.0, 1.0), vec_fm=(1.0, 0.6),
+                ssb=(1.0, 4.5, 6.5, 1.0, 6.0, 7.0, 5.0, 13, 1), membrane=None, phosph_k=1.2, phosph_res=(),
+                dh=(1.0, 1.0, 1.0, 1.0, 10.0, 10)) -> str:
+    """Coeff sections of the given optional terms (appended to a coeff file:
+    the reader takes a section wherever its tag appears).  membrane:
+    (k, k_bin, centre xyz, length, rho0_max, rho0_distor, 3x4 gammas)."""
+    txt = ""
+    for t in terms:
+        if t == "amhgo":
+            txt += _sec("[AMH-Go]", *amhgo)
+        elif t == "cont_rest":
+            txt += _sec("[Contact_Restraints]", cont_rest[0], cont_rest[1], "contact.list")
+        elif t == "vec_fm":
+            txt += _sec("[Vector_Fragment_Memory]", *vec_fm)
+        elif t == "ssb":
+            k1, r1a, r1b, k2, r2a, r2b, kap, sep, cor = ssb
+            txt += _sec("[Solvent_Barrier]", k1, (r1a, r1b), k2, (r2a, r2b), kap, sep, cor, tuple(SSB_RSHIFT))
+        elif t == "membrane":
+            k, kb, c, ln, r0, dist, g = membrane
+            txt += _sec("[Membrane]", k, kb, tuple(c), 0, ln, r0, dist, *[tuple(row) for row in g])
+        elif t == "phosph":
+            txt += _sec("[Phosphorylation]", phosph_k, len(phosph_res), *[int(r) for r in phosph_res])
+        elif t == "dh":
+            kpp, kmm, kpm, ks, lam, sep = dh
+            txt += _sec("[DebyeHuckel]", (kpp, kmm, kpm), ks, lam, sep)
+        else:
+            raise ValueError("unknown optional term %r" % t)
+    return txt
+
+
+def write_extras(sys: System, workdir: str, terms: Sequence[str] = EXTRA_TERMS, seed: int = 0,
+                 native_noise: float = 0.5, **kw) -> dict:
+    """Enables optional FixBackbone terms on a system written by write_system
+    (appends their coeff sections and writes their side files):
+      amhgo      amh-go.gro = the structure plus Gaussian noise (native_noise,
+                 A), amh-go.gamma with two sequence-separation classes
+                 (3-4: 0.125, >= 5: 0.5);
+      cont_rest  contact.list: residue pairs whose density sites are 4-9 A
+                 apart (|i - j| >= 3), r0 = that distance plus noise, weights
+                 0.5-1.5;
+      vec_fm     the fragment memories (write_system with fm=True);
+      ssb        the shipped barrier parameters;
+      membrane   a slab through the system's centroid, zim classes 1-3;
+      phosph     up to three serines (else the first residues) phosphorylated;
+      dh         charge_on_residues.dat: K, R +1, D, E -1.
+    Returns the membrane centre and the phosphorylated residues."""
+    rng = np.random.default_rng(10007 * seed + 17)
+    n, seq = sys.n, sys.seq
+    site = np.where(np.array([c == "G" for c in seq])[:, None], sys.xca, sys.xcb)
+    out = {}
+    if "amhgo" in terms:
+        ca = sys.xca + rng.normal(scale=native_noise, size=sys.xca.shape)
+        cb = sys.xcb + rng.normal(scale=native_noise, size=sys.xcb.shape)
+        st = np.round(np.stack([ca, cb], axis=1) * 100.0) / 1000.0
+        write_gro(os.path.join(workdir, "amh-go.gro"), seq, st)
+        with open(os.path.join(workdir, "amh-go.gamma"), "w") as f:
+            f.write("# i_min i_med_min i_med_max\n3 5 inf\n#\n# ires_code   jres_code   class      gamma\n"
+                    "ALL ALL 1 0.125\nALL ALL 2 0.5\n")
+    if "cont_rest" in terms:
+        d = np.linalg.norm(site[:, None, :] - site[None, :, :], axis=2)
+        ii, jj = np.nonzero((d > 4.0) & (d < 9.0) & (np.abs(np.arange(n)[:, None] - np.arange(n)[None, :]) >= 3))
+        keep = ii < jj
+        ii, jj = ii[keep], jj[keep]
+        if len(ii) > 2 * n:
+            pick = rng.choice(len(ii), size=2 * n, replace=False)
+            ii, jj = ii[pick], jj[pick]
+        with open(os.path.join(workdir, "contact.list"), "w") as f:
+            for a, b in zip(ii, jj):
+                f.write("%d %d %.4f %.4f\n" % (b + 1, a + 1, rng.uniform(0.5, 1.5), d[a, b] + rng.normal(scale=0.7)))
+    if "membrane" in terms:
+        c = sys.xca.mean(axis=0) + rng.normal(scale=2.0, size=3)
+        g = rng.uniform(0.2, 1.5, size=(3, 4))
+        kw.setdefault("membrane", (1.0, 0.5, tuple(np.round(c, 3)), 30.0, 6.0, 0.3, g.round(3).tolist()))
+        with open(os.path.join(workdir, "zim"), "w") as f:
+            f.write("\n".join(str(int(z)) for z in rng.integers(1, 4, size=n)) + "\n")
+        out["membrane"] = kw["membrane"]
+    if "phosph" in terms:
+        ser = [i + 1 for i, c in enumerate(seq) if c == "S"][:3]
+        res = ser if ser else [1, 2]
+        kw.setdefault("phosph_res", tuple(res))
+        out["phosph_res"] = kw["phosph_res"]
+    if "dh" in terms:
+        q = {"K": 1.0, "R": 1.0, "D": -1.0, "E": -1.0}
+        charged = [(i + 1, q[c]) for i, c in enumerate(seq) if c in q]
+        with open(os.path.join(workdir, "charge_on_residues.dat"), "w") as f:
+            f.write("%d\n" % len(charged))
+            for r, v in charged:
+                f.write("%d %.1f\n" % (r, v))
+    with open(os.path.join(workdir, "fix_backbone_coeff.data"), "a") as f:
+        f.write("\n" + extras_text(terms, **kw))
+    return out

@@ -40,6 +40,7 @@ typedef struct {
   double weight;
   char *se;
   double *rf0, *rf1; /* len*len: rf0 CA-CA (i<j) / CB-CB (i>j), rf1 CA(i)-CB(j) */
+  double *vmf;       /* len*len angles between the CA->CB vectors (vector fragment memory), or NULL */
 } or_mem;
 
 typedef struct {
@@ -104,6 +105,37 @@ struct or_ctx {
   double **tb_e, **tb_f;
   char last_err[256];
   double pair_list_cutoff;
+  double list_skin;                     /* neighbour-list skin of the list-limited terms (or_set_list_skin) */
+  /* coeff-toggled terms: AMH-Go (fb.cpp:297-303), contact restraints
+     (321-327), vector fragment memory (328-333), solvent-separated barrier
+     (334-345), membrane (346-358), phosphorylation (412-421), Debye-Hueckel
+     (466-478) */
+  int amh_go_flag, cont_rest_flag, vec_frag_mem_flag, ssb_flag, memb_flag, phosph_flag, huckel_flag;
+  double k_amh_go, amh_go_p, amh_go_rc, amh_go_norm;
+  int frustration_censoring_flag;
+  or_gamma amhg;
+  or_mem amh_mem;                       /* amh-go.gro as one memory (pos 0, len n) */
+  unsigned char *censor;                /* n*n, censoring flag 1 */
+  double *rn_caca, *rn_cbcb, *rn_cacb;  /* n*n, censoring flag 2 */
+  double *go_f;                         /* 2n*3 scratch: one atom's pair forces */
+  double k_cont_rest, cr_sigma, cr_sigma_sq_inv, cr_dr_cutoff, cr_glob_cutoff_sq;
+  char cr_file[256];
+  int n_cr;
+  int *cr_i1, *cr_i2, *cr_start;        /* entries sorted by (i1, i2, file order); start per i1 (n+1) */
+  double *cr_w, *cr_r0;
+  double k_vec_frag_mem, vfm_sigma, vfm_sigma_sq;
+  double k_solventb1, ssb_rmin1, ssb_rmax1, k_solventb2, ssb_rmin2, ssb_rmax2, ssb_kappa, ssb_rshift[20];
+  int ssb_ij_sep, ssb_rad_cor;
+  double k_overall_memb, k_bin, memb_xo[3], memb_len, rho0_max, rho0_distor, g_memb[3][4];
+  int memb_pore_type;
+  int *z_res;
+  double k_hypercharge;
+  int n_phosph_res, phosph_res[20];
+  int *phosph_map;
+  double phosph_water_gamma[5][20][20][2];
+  double k_PlusPlus, k_MinusMinus, k_PlusMinus, k_screening, screening_length;
+  int debye_huckel_min_sep;
+  double *charge;
   /* system */
   int n;
   char *se;
@@ -462,7 +494,7 @@ static void gro_parse(const char *path, gro_file *g)
   fclose(f);
 }
 
-static int mem_load(or_mem *m, int pos, int fpos, int len, double weight, const gro_file *g)
+static int mem_load(or_mem *m, int pos, int fpos, int len, double weight, const gro_file *g, int vec)
 {
   memset(m, 0, sizeof(*m));
   m->pos = pos; m->len = len; m->weight = weight;
@@ -509,6 +541,15 @@ static int mem_load(or_mem *m, int pos, int fpos, int len, double weight, const 
       double e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
       m->rf1[i * len + j] = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
     }
+  if (vec) { /* Fragment_Memory::VM, fm.cpp:166, 239-243 */
+    m->vmf = (double *)calloc((size_t)len * len, sizeof(double));
+    for (int i = 0; i < len; ++i)
+      for (int j = 0; j < len; ++j) {
+        const double *a1 = xca + 3 * i, *b1 = xcb + 3 * i, *a2 = xca + 3 * j, *b2 = xcb + 3 * j;
+        double u[3] = {b1[0] - a1[0], b1[1] - a1[1], b1[2] - a1[2]}, v[3] = {b2[0] - a2[0], b2[1] - a2[1], b2[2] - a2[2]};
+        m->vmf[i * len + j] = acos(dot3(u, v) / (sqrt(dot3(u, u)) * sqrt(dot3(v, v))));
+      }
+  }
   free(xca); free(xcb);
   return 0;
 }
@@ -577,7 +618,7 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
         if (ngro == capgro) { capgro = capgro ? 2 * capgro : 16; gros = (gro_file *)realloc(gros, (size_t)capgro * sizeof(gro_file)); }
         gro_parse(gro, &gros[ngro++]);
       }
-      int e = mem_load(&c->mems[c->n_mems], tpos, fpos, len, weight, &gros[gi]);
+      int e = mem_load(&c->mems[c->n_mems], tpos, fpos, len, weight, &gros[gi], c->vec_frag_mem_flag);
       c->n_mems++;
       if (e) { set_err(err, errlen, "read_mems: Fragment_Memory: Error reading memory fragment"); rc = -1; break; }
       if (tpos + len > c->n) { set_err(err, errlen, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); rc = -1; break; }
@@ -745,14 +786,62 @@ static int parse_coeff(or_ctx *c, const char *path, char *err, int errlen)
       c->frag_table_well_width = tk_double(&in);
       c->fm_use_table_flag = tk_int(&in);
       c->fm_sigma_exp = tk_double(&in);
-    } else if (strcmp(sec, "[AMH-Go]") == 0 ||
-               strcmp(sec, "[Contact_Restraints]") == 0 || strcmp(sec, "[Vector_Fragment_Memory]") == 0 ||
-               strcmp(sec, "[Solvent_Barrier]") == 0 || strcmp(sec, "[Membrane]") == 0 ||
-               strcmp(sec, "[Fragment_Frustratometer]") == 0 || strcmp(sec, "[Tertiary_Frustratometer]") == 0 ||
-               strcmp(sec, "[Nmer_Frustratometer]") == 0 || strcmp(sec, "[Phosphorylation]") == 0 ||
+    } else if (strcmp(sec, "[AMH-Go]") == 0) { /* fb.cpp:297-303 */
+      c->amh_go_flag = 1;
+      c->k_amh_go = tk_double(&in);
+      c->amh_go_p = tk_double(&in);
+      c->amh_go_rc = tk_double(&in);
+      c->frustration_censoring_flag = tk_int(&in);
+    } else if (strcmp(sec, "[Contact_Restraints]") == 0) { /* fb.cpp:321-327 */
+      c->cont_rest_flag = 1;
+      c->k_cont_rest = tk_double(&in);
+      c->cr_sigma = tk_double(&in);
+      tk_str(&in, c->cr_file, sizeof c->cr_file);
+      c->cr_sigma_sq_inv = pow(c->cr_sigma, -2);
+    } else if (strcmp(sec, "[Vector_Fragment_Memory]") == 0) { /* fb.cpp:328-333 */
+      c->vec_frag_mem_flag = 1;
+      c->k_vec_frag_mem = tk_double(&in);
+      c->vfm_sigma = tk_double(&in);
+      c->vfm_sigma_sq = c->vfm_sigma * c->vfm_sigma;
+    } else if (strcmp(sec, "[Solvent_Barrier]") == 0) { /* fb.cpp:334-345 */
+      c->ssb_flag = 1;
+      c->k_solventb1 = tk_double(&in);
+      c->ssb_rmin1 = tk_double(&in); c->ssb_rmax1 = tk_double(&in);
+      c->k_solventb2 = tk_double(&in);
+      c->ssb_rmin2 = tk_double(&in); c->ssb_rmax2 = tk_double(&in);
+      c->ssb_kappa = tk_double(&in);
+      c->ssb_ij_sep = tk_int(&in);
+      c->ssb_rad_cor = tk_int(&in);
+      for (j = 0; j < 20; ++j) c->ssb_rshift[j] = tk_double(&in);
+    } else if (strcmp(sec, "[Membrane]") == 0) { /* fb.cpp:346-358 */
+      c->memb_flag = 1;
+      c->k_overall_memb = tk_double(&in);
+      c->k_bin = tk_double(&in);
+      for (j = 0; j < 3; ++j) c->memb_xo[j] = tk_double(&in);
+      c->memb_pore_type = tk_int(&in);
+      c->memb_len = tk_double(&in);
+      c->rho0_max = tk_double(&in);
+      c->rho0_distor = tk_double(&in);
+      for (int a = 0; a < 3; ++a)
+        for (j = 0; j < 4; ++j) c->g_memb[a][j] = tk_double(&in);
+    } else if (strcmp(sec, "[Phosphorylation]") == 0) { /* fb.cpp:412-421 */
+      if (!c->water_flag) { free(text); set_err(err, errlen, "Cannot run phosphorylation without water potential"); return -1; }
+      c->phosph_flag = 1;
+      c->k_hypercharge = tk_double(&in);
+      c->n_phosph_res = tk_int(&in);
+      if (c->n_phosph_res > 20) { free(text); set_err(err, errlen, "Number of phosphorylated residues may not exceed 20"); return -1; }
+      for (j = 0; j < c->n_phosph_res; ++j) c->phosph_res[j] = tk_int(&in);
+    } else if (strcmp(sec, "[DebyeHuckel]") == 0) { /* fb.cpp:466-478 */
+      c->huckel_flag = 1;
+      c->k_PlusPlus = tk_double(&in); c->k_MinusMinus = tk_double(&in); c->k_PlusMinus = tk_double(&in);
+      c->k_screening = tk_double(&in);
+      c->screening_length = tk_double(&in);
+      c->debye_huckel_min_sep = tk_int(&in);
+    } else if (strcmp(sec, "[Fragment_Frustratometer]") == 0 || strcmp(sec, "[Tertiary_Frustratometer]") == 0 ||
+               strcmp(sec, "[Nmer_Frustratometer]") == 0 ||
                strcmp(sec, "[Amylometer]") == 0 || strcmp(sec, "[Selection_Temperature]") == 0 ||
                strcmp(sec, "[Monte_Carlo_Seq_Opt]") == 0 || strcmp(sec, "[Optimization]") == 0 ||
-               strcmp(sec, "[Burial_Optimization]") == 0 || strcmp(sec, "[DebyeHuckel]") == 0 ||
+               strcmp(sec, "[Burial_Optimization]") == 0 ||
                strcmp(sec, "[DebyeHuckel_Optimization]") == 0) {
       free(text);
       set_err(err, errlen, "Section %s is enabled but is not part of this build's force path", sec);
@@ -788,6 +877,172 @@ static int read_table_file(const char *dir, const char *name, double *dst, int c
 
 static int fm_table_build(or_ctx *c, char *err, int errlen);
 
+/* read_contact_restraints_file, fb.cpp:1106-1170: records "i1 i2 w r0"
+ * (1-based residues) until one fails to read; w *= k_cont_rest (already
+ * scaled by epsilon); per lower residue the entries sorted by the upper
+ * one (qsort, fb.cpp:1163-1166; ties kept in file order here, the
+ * reference's qsort leaves their order unspecified); the global cutoff
+ * (r0_max + 4.29 sigma)^2 (fb.cpp:1147-1148). */
+static int read_contact_restraints(or_ctx *c, const char *workdir, char *err, int errlen)
+{
+  char path[1024];
+  join_path(path, sizeof path, workdir, c->cr_file);
+  char *t = read_file(path, NULL);
+  if (!t) { set_err(err, errlen, "File for Contact Restraints potential doesn't exist"); return -1; }
+  tokin s = {t, 0, 0};
+  int cap = 0, n = c->n;
+  double r0_max = 0.0;
+  int *i1 = NULL, *i2 = NULL;
+  double *w = NULL, *r0 = NULL;
+  c->n_cr = 0;
+  for (;;) {
+    int a = tk_int(&s), b = tk_int(&s);
+    double ww = tk_double(&s), rr = tk_double(&s);
+    if (s.fail) break;
+    int j1 = (a < b ? a : b) - 1, j2 = (a < b ? b : a) - 1;
+    if (j1 < 0 || j1 >= n || j2 < 0 || j2 >= n) {
+      free(t); free(i1); free(i2); free(w); free(r0);
+      set_err(err, errlen, "Indices out of range in Contact Restraints potential file");
+      return -1;
+    }
+    if (c->n_cr == cap) {
+      cap = cap ? 2 * cap : 64;
+      i1 = (int *)realloc(i1, cap * sizeof(int)); i2 = (int *)realloc(i2, cap * sizeof(int));
+      w = (double *)realloc(w, cap * sizeof(double)); r0 = (double *)realloc(r0, cap * sizeof(double));
+    }
+    i1[c->n_cr] = j1; i2[c->n_cr] = j2; w[c->n_cr] = ww * c->k_cont_rest; r0[c->n_cr] = rr;
+    if (rr > r0_max) r0_max = rr;
+    c->n_cr++;
+  }
+  free(t);
+  /* counting sort by i1, then insertion sort by i2 within each (stable) */
+  int m = c->n_cr;
+  c->cr_start = (int *)calloc((size_t)n + 1, sizeof(int));
+  for (int k = 0; k < m; ++k) c->cr_start[i1[k] + 1]++;
+  for (int i = 0; i < n; ++i) c->cr_start[i + 1] += c->cr_start[i];
+  int *fill = (int *)calloc((size_t)n, sizeof(int));
+  c->cr_i1 = (int *)malloc((m ? m : 1) * sizeof(int)); c->cr_i2 = (int *)malloc((m ? m : 1) * sizeof(int));
+  c->cr_w = (double *)malloc((m ? m : 1) * sizeof(double)); c->cr_r0 = (double *)malloc((m ? m : 1) * sizeof(double));
+  for (int k = 0; k < m; ++k) {
+    int q = c->cr_start[i1[k]] + fill[i1[k]]++;
+    c->cr_i1[q] = i1[k]; c->cr_i2[q] = i2[k]; c->cr_w[q] = w[k]; c->cr_r0[q] = r0[k];
+  }
+  for (int i = 0; i < n; ++i)
+    for (int q = c->cr_start[i] + 1; q < c->cr_start[i + 1]; ++q)
+      for (int p = q; p > c->cr_start[i] && c->cr_i2[p - 1] > c->cr_i2[p]; --p) {
+        int ti = c->cr_i2[p]; c->cr_i2[p] = c->cr_i2[p - 1]; c->cr_i2[p - 1] = ti;
+        double tw = c->cr_w[p]; c->cr_w[p] = c->cr_w[p - 1]; c->cr_w[p - 1] = tw;
+        double tr = c->cr_r0[p]; c->cr_r0[p] = c->cr_r0[p - 1]; c->cr_r0[p - 1] = tr;
+      }
+  free(fill); free(i1); free(i2); free(w); free(r0);
+  c->cr_dr_cutoff = c->cr_sigma * 4.29;
+  c->cr_glob_cutoff_sq = pow(r0_max + c->cr_dr_cutoff, 2);
+  return 0;
+}
+
+/* AMH-Go set-up, fb.cpp:695-743: amh-go.gamma (Gamma_Array), amh-go.gro
+ * (one memory over the whole system, pos 0, len n), the censored pairs of
+ * frustration_censored_contacts.dat (flag 1, 1-based) or the native
+ * distances of go_rnative{CACA,CBCB,CACB}.dat (flag 2, n x n each). */
+static int amhgo_setup(or_ctx *c, const char *workdir, char *err, int errlen)
+{
+  char path[1024];
+  int n = c->n;
+  join_path(path, sizeof path, workdir, "amh-go.gamma");
+  char e2[256];
+  if (gm_load(&c->amhg, path, e2, sizeof e2)) {
+    if (strstr(e2, "Cannot read")) set_err(err, errlen, "Cannot read file amh-go.gamma");
+    else set_err(err, errlen, "AMH_Go: %s", strchr(e2, ':') ? strchr(e2, ':') + 2 : e2);
+    return -1;
+  }
+  if (c->amhg.frag_resty) { set_err(err, errlen, "AMH-Go: Wrong call of getGamma() function"); return -1; }
+  join_path(path, sizeof path, workdir, "amh-go.gro");
+  gro_file g;
+  gro_parse(path, &g);
+  int e = mem_load(&c->amh_mem, 0, 0, n, 1.0, &g, 0);
+  free(g.r);
+  if (e == 1) { set_err(err, errlen, "Cannot read file amh-go.gro"); return -1; }
+  if (e == 2) { set_err(err, errlen, "AMH_Go: Wrong atom count in memory structure file"); return -1; }
+  if (e == 3) { set_err(err, errlen, "AMH_Go: Unknown residue"); return -1; }
+  if (c->frustration_censoring_flag == 1) {
+    c->censor = (unsigned char *)calloc((size_t)n * n, 1);
+    join_path(path, sizeof path, workdir, "frustration_censored_contacts.dat");
+    char *t = read_file(path, NULL);
+    if (t) {
+      tokin s = {t, 0, 0};
+      for (;;) {
+        int a = tk_int(&s), b = tk_int(&s);
+        if (s.fail) break;
+        if (a < 1 || a > n || b < 1 || b > n) { free(t); set_err(err, errlen, "AMH-Go: censored contact out of range"); return -1; }
+        c->censor[(size_t)(a - 1) * n + (b - 1)] = 1;
+      }
+      free(t);
+    }
+  } else if (c->frustration_censoring_flag == 2) {
+    const char *names[3] = {"go_rnativeCACA.dat", "go_rnativeCBCB.dat", "go_rnativeCACB.dat"};
+    double **dst[3] = {&c->rn_caca, &c->rn_cbcb, &c->rn_cacb};
+    for (int q = 0; q < 3; ++q) {
+      join_path(path, sizeof path, workdir, names[q]);
+      char *t = read_file(path, NULL);
+      if (!t) { set_err(err, errlen, "Go native distance file can't be read"); return -1; }
+      tokin s = {t, 0, 0};
+      *dst[q] = (double *)calloc((size_t)n * n, sizeof(double));
+      for (size_t k = 0; k < (size_t)n * n; ++k) (*dst[q])[k] = tk_double(&s);
+      int bad = s.fail;
+      free(t);
+      if (bad) { set_err(err, errlen, "go_rnative*.dat file format error"); return -1; }
+    }
+  }
+  c->go_f = (double *)calloc((size_t)2 * n * 3, sizeof(double));
+  return 0;
+}
+
+/* native distance of the atom pair (i, ia) - (j, ja), ia/ja FM_CA or FM_CB
+ * (fb.cpp:3838-3843, 3904-3909: the CA-CB matrix serves both mixed orders) */
+static double amhgo_rnative(const or_ctx *c, int i, int ia, int j, int ja)
+{
+  if (c->frustration_censoring_flag == 2) {
+    size_t k = (size_t)i * c->n + j;
+    if (ia == FM_CA && ja == FM_CA) return c->rn_caca[k];
+    if (ia == FM_CB && ja == FM_CB) return c->rn_cbcb[k];
+    return c->rn_cacb[k];
+  }
+  return mem_rf(&c->amh_mem, i, ia, j, ja);
+}
+
+/* compute_amhgo_normalization, fb.cpp:3787-3852: a = sum over chains ich,
+ * residues i, atoms ia (CA, and CB unless glycine) of, after each chain
+ * jch, |normi|^p with normi the running sum (not reset per jch) of the
+ * gammas of native contacts (rnative < rc, |i - j| >= minSep); a /= 8 N
+ * with N the end of the last chain. */
+static void amhgo_normalization(or_ctx *c)
+{
+  double norm = 0.0;
+  int iresn = 0;
+  for (int ich = 0; ich < c->nch; ++ich) {
+    int ires0 = c->ch_pos[ich] - 1;
+    iresn = c->ch_pos[ich] + c->ch_len[ich] - 1;
+    for (int i = ires0; i < iresn; ++i) {
+      int it = restype(c->se[i]);
+      for (int ia = FM_CA; ia <= FM_CB - (c->se[i] == 'G' ? 1 : 0); ++ia) {
+        double normi = 0.0;
+        for (int jch = 0; jch < c->nch; ++jch) {
+          int jres0 = c->ch_pos[jch] - 1, jresn = c->ch_pos[jch] + c->ch_len[jch] - 1;
+          for (int j = jres0; j < jresn; ++j) {
+            int jt = restype(c->se[j]);
+            for (int ja = FM_CA; ja <= FM_CB - (c->se[j] == 'G' ? 1 : 0); ++ja) {
+              if (abs(i - j) < c->amhg.i_sep[0]) continue;
+              if (amhgo_rnative(c, i, ia, j, ja) < c->amh_go_rc) normi += gm_get2(&c->amhg, it, jt, i, j);
+            }
+          }
+          norm += pow(fabs(normi), c->amh_go_p);
+        }
+      }
+    }
+  }
+  c->amh_go_norm = norm / (8 * iresn);
+}
+
 int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
               or_ctx **out, char *err, int errlen)
 {
@@ -807,6 +1062,7 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
   for (int j = 0; j < 3; ++j) c->k_chain[j] *= e;
   c->k_chi *= e; c->k_rama *= e; c->k_water *= e; c->k_burial *= e; c->k_helix *= e;
   c->k_dssp *= e; c->k_global_P_AP *= e;
+  c->k_amh_go *= e; c->k_cont_rest *= e;
   for (int j = 0; j < c->n_rama_par; j++) c->w[j] *= c->k_rama;
   for (int j = 0; j < c->n_rama_p_par; j++) c->w[j + 6] *= c->k_rama;
 
@@ -884,6 +1140,21 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
       for (int i = 0; i < 12; ++i) c->aps[i][j] = c->ssweight[i] ? tk_double(&s) : 0.0;
     free(t);
   }
+  if (c->cont_rest_flag && read_contact_restraints(c, workdir, err, errlen)) { or_destroy(c); return -1; }
+  if (c->memb_flag) { /* fb.cpp:611-620 */
+    join_path(path, sizeof path, workdir, "zim");
+    char *t = read_file(path, NULL);
+    if (!t) { or_destroy(c); set_err(err, errlen, "File zim doesn't exist"); return -1; }
+    tokin s = {t, 0, 0};
+    c->z_res = (int *)calloc((size_t)n, sizeof(int));
+    for (int i = 0; i < n; ++i) c->z_res[i] = tk_int(&s);
+    int bad = s.fail;
+    free(t);
+    if (bad) { or_destroy(c); set_err(err, errlen, "Membrane potential parameter file format error"); return -1; }
+    /* the reference leaves V unset for any other class (fb.cpp:6163-6172) */
+    for (int i = 0; i < n; ++i)
+      if (c->z_res[i] < 1 || c->z_res[i] > 3) { or_destroy(c); set_err(err, errlen, "Membrane potential: zim class outside 1-3"); return -1; }
+  }
   if (c->water_flag) { /* fb.cpp:623-641 */
     join_path(path, sizeof path, workdir, "gamma.dat");
     char *t = read_file(path, NULL);
@@ -898,6 +1169,28 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
           c->water_gamma[w][i][j][1] = c->water_gamma[w][j][i][1] = g1 * c->k_water;
         }
     free(t);
+  }
+  if (c->phosph_flag) { /* fb.cpp:643-682 */
+    for (int w = 0; w < c->n_wells; ++w)
+      for (int i = 0; i < 20; ++i)
+        for (int j = i; j < 20; ++j)
+          for (int k = 0; k < 2; ++k)
+            c->phosph_water_gamma[w][i][j][k] = c->phosph_water_gamma[w][j][i][k] = c->water_gamma[w][i][j][k];
+    /* serine (15) takes the hypercharged glutamate (6) column, in this
+       loop order (entries written earlier in the loop are read later) */
+    for (int w = 0; w < c->n_wells; ++w)
+      for (int i = 0; i < 20; ++i) {
+        double f = four_letter_map[i] == 1 ? c->k_hypercharge
+                 : (four_letter_map[i] == 2 || four_letter_map[i] == 3) ? pow(c->k_hypercharge, 2) : 1.0;
+        for (int k = 0; k < 2; ++k)
+          c->phosph_water_gamma[w][i][15][k] = c->phosph_water_gamma[w][15][i][k] = c->phosph_water_gamma[w][i][6][k] * f;
+      }
+    c->phosph_map = (int *)calloc((size_t)n, sizeof(int));
+    for (int j = 0; j < c->n_phosph_res; ++j)
+      if (c->phosph_res[j] != 0) {
+        if (c->phosph_res[j] < 1 || c->phosph_res[j] > n) { or_destroy(c); set_err(err, errlen, "Phosphorylation: residue index out of range"); return -1; }
+        c->phosph_map[c->phosph_res[j] - 1] = 1;
+      }
   }
   if (c->burial_flag) { /* fb.cpp:684-693 */
     if (!c->water_flag) { or_destroy(c); set_err(err, errlen, "Cannot use burial without water potential"); return -1; }
@@ -928,6 +1221,27 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
     }
   }
   if (c->frag_mem_tb_flag && fm_table_build(c, err, errlen)) { or_destroy(c); return -1; }
+  if (c->vec_frag_mem_flag && !(c->frag_mem_flag || c->frag_mem_tb_flag)) {
+    /* the reference reads the memories only for those sections (fb.cpp:749-791) */
+    or_destroy(c); set_err(err, errlen, "Vector_Fragment_Memory needs the memories of [Fragment_Memory]"); return -1;
+  }
+  if (c->amh_go_flag && amhgo_setup(c, workdir, err, errlen)) { or_destroy(c); return -1; }
+  if (c->huckel_flag) { /* fb.cpp:1018-1046 */
+    join_path(path, sizeof path, workdir, "charge_on_residues.dat");
+    char *t = read_file(path, NULL);
+    if (!t) { or_destroy(c); set_err(err, errlen, "File charge_on_residues.dat doesn't exist"); return -1; }
+    tokin s = {t, 0, 0};
+    c->charge = (double *)calloc((size_t)n, sizeof(double));
+    int total = tk_int(&s);
+    for (int k = 0; k < total && !s.fail; ++k) {
+      int rn = tk_int(&s);
+      double q = tk_double(&s);
+      if (s.fail) break;
+      if (rn < 1 || rn > n) { free(t); or_destroy(c); set_err(err, errlen, "charge_on_residues.dat: residue number out of range"); return -1; }
+      c->charge[rn - 1] = q;
+    }
+    free(t);
+  }
   /* fb.cpp:1048-1066 */
   double plc = 8.0;
   if (c->dssp_hdrgn_flag) plc = fmax(plc, c->dssp_hdrgn_cut);
@@ -936,7 +1250,17 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
   if (c->helix_flag) plc = fmax(plc, c->h_rmax_theta);
   if (c->helix_flag) plc = fmax(plc, calc_exp_helix_cutoff(c));
   if (c->helix_flag) plc = fmax(plc, c->helix_cutoff);
+  if (c->cont_rest_flag) plc = fmax(plc, sqrt(c->cr_glob_cutoff_sq));
+  if (c->huckel_flag) plc = fmax(plc, 2.0 * c->screening_length / c->k_screening);
+  if (c->ssb_flag) {
+    double cut = 0.0;
+    if (c->ssb_rad_cor)
+      for (int i = 0; i < 20; ++i) cut = fmax(cut, 2.0 * c->ssb_rshift[i]);
+    cut += fmax(c->ssb_rmax1, c->ssb_rmax2) + 10.0 / c->ssb_kappa;
+    plc = fmax(plc, cut);
+  }
   c->pair_list_cutoff = plc;
+  if (c->amh_go_flag) amhgo_normalization(c);
 
   /* excluded volume deck defaults (CoordinatesToWorkLammpsDataFile.py:314-322) */
   c->ev_p = 2;
@@ -966,7 +1290,7 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
 void or_destroy(or_ctx *c)
 {
   if (!c) return;
-  for (int k = 0; k < c->n_mems; ++k) { free(c->mems[k].se); free(c->mems[k].rf0); free(c->mems[k].rf1); }
+  for (int k = 0; k < c->n_mems; ++k) { free(c->mems[k].se); free(c->mems[k].rf0); free(c->mems[k].rf1); free(c->mems[k].vmf); }
   free(c->mems);
   if (c->frag_mem_map) for (int i = 0; i < c->n; ++i) free(c->frag_mem_map[i]);
   free(c->frag_mem_map); free(c->ilen_fm_map);
@@ -981,6 +1305,10 @@ void or_destroy(or_ctx *c)
   free(c->xn); free(c->xh); free(c->xcp);
   free(c->water_ro); free(c->helix_ro); free(c->w_sig); free(c->w_sig_prd); free(c->h_sig); free(c->h_sig_prd);
   free(c->water_xi); free(c->helix_xi_1); free(c->helix_xi_2); free(c->burial_force);
+  free(c->amhg.gamma); free(c->amh_mem.se); free(c->amh_mem.rf0); free(c->amh_mem.rf1); free(c->amh_mem.vmf);
+  free(c->censor); free(c->rn_caca); free(c->rn_cbcb); free(c->rn_cacb); free(c->go_f);
+  free(c->cr_i1); free(c->cr_i2); free(c->cr_start); free(c->cr_w); free(c->cr_r0);
+  free(c->z_res); free(c->phosph_map); free(c->charge);
   free(c);
 }
 
@@ -988,6 +1316,8 @@ int or_n_residues(const or_ctx *c) { return c->n; }
 int or_chain_of(const or_ctx *c, int i) { return c->chain_no[i] - 1; }
 char or_seq_at(const or_ctx *c, int i) { return c->se[i]; }
 double or_pair_list_cutoff(const or_ctx *c) { return c->pair_list_cutoff; }
+
+void or_set_list_skin(or_ctx *c, double skin) { c->list_skin = skin; }
 
 void or_set_bonds(or_ctx *c, double k, const double *r0)
 {
@@ -1685,6 +2015,236 @@ static int helix_pair_ok(const or_ctx *c, int i)
 }
 
 /* compute_pair, fb.cpp:7677-8135 */
+/* ------------------------------------------------------------------ */
+/* coeff-toggled terms                                                  */
+
+/* the neighbour list of the list-limited terms: every pair closer than
+ * pair_list_cutoff + skin, as a list freshly built by LAMMPS holds
+ * (fb.cpp:1805-1813; skin 0 by default: a list rebuilt every step) */
+static inline double list_r2(const or_ctx *c)
+{
+  double r = c->pair_list_cutoff + c->list_skin;
+  return r * r;
+}
+
+/* get_water_gamma / get_burial_gamma, fb.cpp:2872-2900: a pair with a
+ * phosphorylated residue reads the hypercharged table; a phosphorylated
+ * residue's burial gammas are those of type 6 */
+static inline double water_g(const or_ctx *c, int i, int j, int w, int it, int jt, int k)
+{
+  if (c->phosph_flag && (c->phosph_map[i] || c->phosph_map[j])) return c->phosph_water_gamma[w][it][jt][k];
+  return c->water_gamma[w][it][jt][k];
+}
+static inline double burial_g(const or_ctx *c, int i, int it, int b)
+{
+  if (c->phosph_flag && c->phosph_map[i]) return c->burial_gamma[6][b];
+  return c->burial_gamma[it][b];
+}
+
+/* compute_membrane_potential, fb.cpp:6107-6235 (the cylindrical pore of
+ * memb_pore_type 0; the other form is commented out in the reference) */
+static void memb_term(or_ctx *c, int i)
+{
+  const double *xi = c->se[i] == 'G' ? XCA(i) : XCB(i);
+  double *fi = c->se[i] == 'G' ? FCA(i) : FCB(i);
+  double dx = xi[0] - c->memb_xo[0], dy = xi[1] - c->memb_xo[1], dz = xi[2] - c->memb_xo[2];
+  double kb = c->k_bin, a = c->rho0_distor * c->rho0_max, b = c->memb_len / 2;
+  double rho = sqrt(dx * dx + dy * dy);
+  double rho0 = (c->rho0_max - a) + (a / c->memb_len) * (dz + b);
+  double s_per = 0.5 * (1 + tanh(kb * (dz - b)));
+  double s_mem = 0.5 * (tanh(kb * (dz + b)) + tanh(kb * (b - dz)));
+  double s_cyt = 0.5 * (1 + tanh(kb * (-b - dz)));
+  double s_por = 0.5 * (1 - tanh(kb * (rho - rho0)));
+  double s_nopor = 1 - s_por;
+  int z = c->z_res[i] - 1;
+  const double *g = c->g_memb[z];
+  /* classes 1 and 3 share the signs, class 2 has them reversed */
+  double sg = z == 1 ? -1.0 : 1.0;
+  double V = sg * (-g[0] * s_per - g[1] * s_cyt + g[2] * s_mem * s_nopor - g[3] * s_mem * s_por);
+  c->energy[OR_ET_MEMB] += c->epsilon * c->k_overall_memb * V;
+  double dz_per = 0.5 * kb * (1 - pow(tanh(kb * (dz - b)), 2));
+  double dz_mem = -0.5 * kb * pow(tanh(kb * (dz + b)), 2) + 0.5 * kb * pow(tanh(kb * (b - dz)), 2);
+  double dz_cyt = -0.5 * kb * (1 - pow(tanh(kb * (-b - dz)), 2));
+  double dr1_dz = a / c->memb_len;
+  double sech2 = 1 - pow(tanh(kb * (rho - rho0)), 2);
+  double dz_s_por = 0.5 * kb * sech2 * dr1_dz;
+  double dx_s_por = ((-0.5 * kb * dx) / rho) * sech2, dy_s_por = ((-0.5 * kb * dy) / rho) * sech2;
+  double dz_s_por_smem = s_mem * dz_s_por + dz_mem * s_por;
+  double dz_s_nopor_smem = s_mem * (-dz_s_por) + dz_mem * s_nopor;
+  double dVx = sg * (g[2] * s_mem * (-dx_s_por) - g[3] * s_mem * dx_s_por);
+  double dVy = sg * (g[2] * s_mem * (-dy_s_por) - g[3] * s_mem * dy_s_por);
+  double dVz = sg * (-g[0] * dz_per - g[1] * dz_cyt + g[2] * dz_s_nopor_smem - g[3] * dz_s_por_smem);
+  double k = -c->epsilon * c->k_overall_memb;
+  fi[0] += k * dVx; fi[1] += k * dVy; fi[2] += k * dVz;
+}
+
+/* compute_vector_fragment_memory_potential, fb.cpp:3994-4077: for the
+ * memories covering i and j from i + minSep to the memory's end (at most
+ * i + maxSep), no glycine in the target or the memory at i or j: the angle
+ * gc between the CA->CB vectors against the memory's (Fragment_Memory::VMf),
+ * V = -eps k_vfm exp(-(gc - gf)^2 / (2 sigma^2)) */
+static void vfm_term(or_ctx *c, int i)
+{
+  int n = c->n, maxsep = c->fmg.i_sep[c->fmg.nseq_cl], minsep = c->fmg.i_sep[0];
+  double ekw = c->epsilon * c->k_vec_frag_mem;
+  for (int ifm = 0; ifm < c->ilen_fm_map[i]; ++ifm) {
+    const or_mem *m = &c->mems[c->frag_mem_map[i][ifm]];
+    int js = i + minsep, je = m->pos + m->len - 1;
+    if (maxsep != -1 && je > i + maxsep) je = i + maxsep;
+    if (je >= n) continue; /* reference aborts: "Missing residues in memory potential" */
+    for (int j = js; j <= je; ++j) {
+      if (c->se[i] == 'G' || c->se[j] == 'G' || m->se[i - m->pos] == 'G' || m->se[j - m->pos] == 'G') continue;
+      double vi[3] = {XCB(i)[0] - XCA(i)[0], XCB(i)[1] - XCA(i)[1], XCB(i)[2] - XCA(i)[2]};
+      double vj[3] = {XCB(j)[0] - XCA(j)[0], XCB(j)[1] - XCA(j)[1], XCB(j)[2] - XCA(j)[2]};
+      double vmsqi = dot3(vi, vi), vmsqj = dot3(vj, vj);
+      double vmi = sqrt(vmsqi), vmj = sqrt(vmsqj), vp = dot3(vi, vj);
+      double vpn = vp / (vmi * vmj);
+      double gc = acos(vpn), gf = m->vmf[(i - m->pos) * m->len + (j - m->pos)];
+      double dg = gc - gf;
+      double V = -ekw * exp(-dg * dg / (2 * c->vfm_sigma_sq));
+      c->energy[OR_ET_VFRAGMEM] += V;
+      double force = -V * dg / (c->vfm_sigma_sq * vmi * vmj * sqrt(1 - vpn * vpn));
+      double fi[3], fj[3];
+      for (int k = 0; k < 3; ++k) {
+        fi[k] = force * (vj[k] - vi[k] * vp / vmsqi);
+        fj[k] = force * (vi[k] - vj[k] * vp / vmsqj);
+      }
+      fadd(FCA(i), -1.0, fi); fadd(FCB(i), 1.0, fi);
+      fadd(FCA(j), -1.0, fj); fadd(FCB(j), 1.0, fj);
+    }
+  }
+}
+
+/* compute_contact_restraints_potential, fb.cpp:8137-8177: the first entry
+ * for (min, max) of the pair; V = -w exp(-dr^2 / (2 sigma^2)) within
+ * |dr| < 4.29 sigma; returns the pair force factor (f_i += factor dx) */
+static double cr_force(or_ctx *c, int i, int j, double rsq)
+{
+  int lo = i < j ? i : j, hi = i < j ? j : i;
+  for (int q = c->cr_start[lo]; q < c->cr_start[lo + 1]; ++q) {
+    if (c->cr_i2[q] != hi) continue;
+    double r = sqrt(rsq), dr = r - c->cr_r0[q];
+    if (fabs(dr) < c->cr_dr_cutoff) {
+      double V = -c->cr_w[q] * exp(-0.5 * dr * dr * c->cr_sigma_sq_inv);
+      c->energy[OR_ET_CONT_REST] += V;
+      return V * dr * c->cr_sigma_sq_inv / r;
+    }
+    return 0.0;
+  }
+  return 0.0;
+}
+
+/* compute_solvent_barrier, fb.cpp:6237-6305: two tanh barriers between the
+ * CB sites (CA for glycine), radii shifted by residue type when ssb_rad_cor;
+ * theta = (tanh k(r - rmin) + tanh k(rmax - r)) / 2, E = eps (k1 theta1 +
+ * k2 theta2), zero beyond both rmax + 10/kappa */
+static void ssb_term(or_ctx *c, int i, int j)
+{
+  const double *xi = site(c, i), *xj = site(c, j);
+  double dx[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double r = sqrt(dot3(dx, dx));
+  double rmin1 = c->ssb_rmin1, rmax1 = c->ssb_rmax1, rmin2 = c->ssb_rmin2, rmax2 = c->ssb_rmax2;
+  if (c->ssb_rad_cor) {
+    double sh = c->ssb_rshift[restype(c->se[i])] + c->ssb_rshift[restype(c->se[j])];
+    rmin1 += sh; rmax1 += sh; rmin2 += sh; rmax2 += sh;
+  }
+  double kp = c->ssb_kappa;
+  if (r > rmax1 + 10 / kp && r > rmax2 + 10 / kp) return;
+  double tmin1 = tanh(kp * (r - rmin1)), tmax1 = tanh(kp * (rmax1 - r));
+  double tmin2 = tanh(kp * (r - rmin2)), tmax2 = tanh(kp * (rmax2 - r));
+  double th1 = 0.5 * (tmin1 + tmax1), th2 = 0.5 * (tmin2 + tmax2);
+  double e = c->epsilon;
+  c->energy[OR_ET_SSB] += e * c->k_solventb1 * th1;
+  c->energy[OR_ET_SSB] += e * c->k_solventb2 * th2;
+  double f1 = -e * c->k_solventb1 * kp * th1 * (tmax1 - tmin1) / r;
+  double f2 = -e * c->k_solventb2 * kp * th2 * (tmax2 - tmin2) / r;
+  fadd(fsite(c, i), f1 + f2, dx);
+  fadd(fsite(c, j), -(f1 + f2), dx);
+}
+
+/* compute_DebyeHuckel_Interaction, fb.cpp:6307-6363: screened Coulomb
+ * between the CB sites of charged residues, k++ / k-- / k+- by the signs */
+static void dh_term(or_ctx *c, int i, int j)
+{
+  if (abs(i - j) < c->debye_huckel_min_sep) return;
+  double qi = c->charge[i], qj = c->charge[j];
+  if (qi == 0 && qj == 0) return;
+  const double *xi = site(c, i), *xj = site(c, j);
+  double dx[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double r = sqrt(dot3(dx, dx));
+  double qq = 0.0;
+  if (qi > 0.0 && qj > 0.0) qq = c->k_PlusPlus * qi * qj / r;
+  else if (qi < 0.0 && qj < 0.0) qq = c->k_MinusMinus * qi * qj / r;
+  else if ((qi < 0.0 && qj > 0.0) || (qi > 0.0 && qj < 0.0)) qq = c->k_PlusMinus * qi * qj / r;
+  double E = c->epsilon * qq * exp(-c->k_screening * r / c->screening_length);
+  c->energy[OR_ET_DH] += E;
+  double ft = (E / r) * (1.0 / r + c->k_screening / c->screening_length);
+  fadd(fsite(c, i), ft, dx);
+  fadd(fsite(c, j), -ft, dx);
+}
+
+/* compute_amh_go_model, fb.cpp:3854-3992: for every CA and non-glycine CB
+ * atom i, E_i = sum over atoms j of the full neighbour list (CA or
+ * non-glycine CB, |i - j| >= minSep or another chain, not censored) with a
+ * native distance below rc and (r - rnative)^2 < 27.6 sigma^2
+ * (sigma^2 = |i - j|^0.3) of gamma exp(-(r - rnative)^2 / (2 sigma^2));
+ * E = -k/2 sum_i E_i^p / a, forces of atom i's pairs scaled by
+ * -k p E_i^(p-1) / (2 a).  The neighbour list is taken as every atom pair
+ * closer than pair_list_cutoff (a list rebuilt every step without skin). */
+static void amhgo_term(or_ctx *c)
+{
+  int n = c->n;
+  double plc2 = list_r2(c), E = 0.0;
+  int minsep = c->amhg.i_sep[0];
+  for (int i = 0; i < n; ++i) {
+    int it = restype(c->se[i]);
+    for (int ka = 0; ka < 2; ++ka) {
+      if (ka == 1 && c->se[i] == 'G') continue;
+      const double *xa = ka ? XCB(i) : XCA(i);
+      double *fa = ka ? FCB(i) : FCA(i);
+      double fself[3] = {0.0, 0.0, 0.0}, Ei = 0.0;
+      memset(c->go_f, 0, sizeof(double) * 6 * n);
+      for (int j = 0; j < n; ++j) {
+        if (c->censor && (c->censor[(size_t)i * n + j] || c->censor[(size_t)j * n + i])) continue;
+        if (!(abs(i - j) >= minsep || c->chain_no[i] != c->chain_no[j])) continue;
+        int jt = restype(c->se[j]);
+        for (int kb = 0; kb < 2; ++kb) {
+          if (kb == 1 && c->se[j] == 'G') continue;
+          if (j == i && kb == ka) continue;
+          const double *xb = kb ? XCB(j) : XCA(j);
+          double dx[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+          double rsq = dot3(dx, dx);
+          if (!(rsq < plc2)) continue;
+          double rn = amhgo_rnative(c, i, ka ? FM_CB : FM_CA, j, kb ? FM_CB : FM_CA);
+          if (!(rn < c->amh_go_rc)) continue;
+          double r = sqrt(rsq), dr = r - rn, drsq = dr * dr;
+          double s2 = pow(abs(i - j), 0.3);
+          if (!(drsq < 27.6 * s2)) continue;
+          double g = gm_get2(&c->amhg, it, jt, i, j);
+          double Eij = g * exp(-drsq / (2.0 * s2));
+          double force = Eij * dr / (s2 * r);
+          for (int k = 0; k < 3; ++k) {
+            fself[k] += force * dx[k];
+            c->go_f[(2 * j + kb) * 3 + k] -= force * dx[k];
+          }
+          Ei += Eij;
+        }
+      }
+      double factor = -0.5 * c->k_amh_go * c->amh_go_p * pow(Ei, c->amh_go_p - 1) / c->amh_go_norm;
+      for (int k = 0; k < 3; ++k) fa[k] += factor * fself[k];
+      for (int j = 0; j < n; ++j)
+        for (int kb = 0; kb < 2; ++kb) {
+          double *g3 = &c->go_f[(2 * j + kb) * 3];
+          if (g3[0] == 0.0 && g3[1] == 0.0 && g3[2] == 0.0) continue;
+          double *fb = kb ? FCB(j) : FCA(j);
+          for (int k = 0; k < 3; ++k) fb[k] += factor * g3[k];
+        }
+      E += -0.5 * c->k_amh_go * pow(Ei, c->amh_go_p) / c->amh_go_norm;
+    }
+  }
+  c->energy[OR_ET_AMHGO] += E;
+}
+
 static void pair_terms(or_ctx *c)
 {
   int n = c->n;
@@ -1727,9 +2287,9 @@ static void pair_terms(or_ctx *c)
         t[b][1] = tanh(kb * (c->burial_ro_max[b] - rho));
       }
       if (ON(OR_ET_BURIAL))
-        for (int b = 0; b < 3; ++b) c->energy[OR_ET_BURIAL] += -0.5 * c->k_burial * c->burial_gamma[it][b] * (t[b][0] + t[b][1]);
+        for (int b = 0; b < 3; ++b) c->energy[OR_ET_BURIAL] += -0.5 * c->k_burial * burial_g(c, i, it, b) * (t[b][0] + t[b][1]);
       double bf = 0.0;
-      for (int b = 0; b < 3; ++b) bf += c->burial_gamma[it][b] * (t[b][1] * t[b][1] - t[b][0] * t[b][0]);
+      for (int b = 0; b < 3; ++b) bf += burial_g(c, i, it, b) * (t[b][1] * t[b][1] - t[b][0] * t[b][0]);
       c->burial_force[i] = bf * 0.5 * c->k_burial * kb;
     }
   }
@@ -1746,7 +2306,7 @@ static void pair_terms(or_ctx *c)
         int it = restype(c->se[i]), jt = restype(c->se[j]);
         for (int w = 0; w < c->n_wells; ++w) {
           if (!c->well_flag[w]) continue;
-          double g0 = c->water_gamma[w][it][jt][0], g1 = c->water_gamma[w][it][jt][1];
+          double g0 = water_g(c, i, j, w, it, jt, 0), g1 = water_g(c, i, j, w, it, jt, 1);
           /* the reference predicate !fabs(g0-g1)<delta parses as (!fabs(g0-g1)) < delta */
           if (((double)(!fabs(g0 - g1)) < DELTA) && rsq > c->w_rmin_theta_sq[w] && rsq < c->w_rmax_theta_sq[w]) {
             double th, prd;
@@ -1775,7 +2335,7 @@ static void pair_terms(or_ctx *c)
           for (int w = 0; w < c->n_wells; ++w) {
             if (!c->well_flag[w]) continue;
             if ((!same || dres >= c->contact_cutoff) && rsq > c->w_rmin_theta_sq[w] && rsq < c->w_rmax_theta_sq[w]) {
-              double g0 = c->water_gamma[w][it][jt][0], g1 = c->water_gamma[w][it][jt][1], sg;
+              double g0 = water_g(c, i, j, w, it, jt, 0), g1 = water_g(c, i, j, w, it, jt, 1), sg;
               if (fabs(g0 - g1) < DELTA) sg = 0.5 * (g0 + g1);
               else sg = g0 + (g1 - g0) * c->w_sig[i] * c->w_sig[j];
               if (!br) { r = sqrt(rsq); br = 1; }
@@ -1810,6 +2370,8 @@ static void pair_terms(or_ctx *c)
             force += -factor * prd;
           }
         }
+        /* contact restraints, fb.cpp:8059-8061 */
+        if (c->cont_rest_flag && rsq < c->cr_glob_cutoff_sq && ON(OR_ET_CONT_REST)) force += cr_force(c, i, j, rsq);
         if (force != 0.0) {
           fadd(fsite(c, i), force, dx);
           fadd(fsite(c, j), -force, dx);
@@ -1828,6 +2390,15 @@ static void pair_terms(or_ctx *c)
       }
       /* P_AP: CA-CA pairs, fb.cpp:8096-8103 */
       if (c->p_ap_flag && rsq_of(XCA(i), XCA(j)) < c->pap_cutoff_sq) pap_term(c, i, j);
+      /* solvent barrier and Debye-Hueckel on the CA-CA pairs of the list
+       * (fb.cpp:8113-8117; the list: CA-CA closer than pair_list_cutoff).
+       * A same-chain barrier pair counts with the lower residue first, as
+       * the reference's check at fb.cpp:6239 and its list-free loop
+       * (fb.cpp:7481) have it. */
+      if ((c->ssb_flag || c->huckel_flag) && rsq_of(XCA(i), XCA(j)) < list_r2(c)) {
+        if (c->ssb_flag && (!same || dres >= c->ssb_ij_sep) && ON(OR_ET_SSB)) ssb_term(c, i, j);
+        if (c->huckel_flag && dres > 1 && ON(OR_ET_DH)) dh_term(c, i, j);
+      }
     }
 }
 
@@ -1927,10 +2498,13 @@ int or_compute(or_ctx *c, const double *xca, const double *xcb, const double *xo
     if (!first && !last && c->chi_flag && c->se[i] != 'G' && ON(OR_ET_CHI)) chi_term(c, i);
     if (c->shake_flag && ON(OR_ET_SHAKE)) shake_term(c, i);
     if (!first && !last && c->rama_flag && c->se[i] != 'G' && ON(OR_ET_RAMA)) rama_term(c, i);
+    if (c->memb_flag && ON(OR_ET_MEMB)) memb_term(c, i);
     if (c->frag_mem_flag && ON(OR_ET_FRAGMEM)) fm_term(c, i);
+    if (c->vec_frag_mem_flag && ON(OR_ET_VFRAGMEM)) vfm_term(c, i);
     if (c->frag_mem_tb_flag && ON(OR_ET_FRAGMEM) && fm_table_terms(c, i)) return -1;
   }
   pair_terms(c);
+  if (c->amh_go_flag && ON(OR_ET_AMHGO)) amhgo_term(c);
   if (ON(OR_ET_VEXCLUDED)) ev_terms(c, NULL, 0, NULL);
   if (ON(OR_ET_BOND)) bond_terms(c);
   for (int k = 1; k < OR_N_ET; ++k) energy[OR_ET_TOTAL] += energy[k];
@@ -1970,7 +2544,7 @@ int or_list(or_ctx *c, int kind, int arg, const double *xca, const double *xcb, 
           PUSH((long long)i * n + j);
         if (kind == 6 && c->water_flag && (!same || dres >= c->contact_cutoff) && arg < c->n_wells && c->well_flag[arg]) {
           int it = restype(c->se[i]), jt = restype(c->se[j]);
-          double g0 = c->water_gamma[arg][it][jt][0], g1 = c->water_gamma[arg][it][jt][1];
+          double g0 = water_g(c, i, j, arg, it, jt, 0), g1 = water_g(c, i, j, arg, it, jt, 1);
           if (((double)(!fabs(g0 - g1)) < DELTA) && rsq > c->w_rmin_theta_sq[arg] && rsq < c->w_rmax_theta_sq[arg])
             PUSH((long long)i * n + j);
         }

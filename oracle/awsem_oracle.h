@@ -70,6 +70,13 @@ char or_seq_at(const or_ctx *c, int i);
 /* pair_list_cutoff of fb.cpp:1048-1066 */
 double or_pair_list_cutoff(const or_ctx *c);
 
+/* Neighbour-list skin of the terms that take their pairs from FixBackbone's
+ * neighbour list without a cutoff of their own (Debye-Hueckel, the CA-CA
+ * pairs of the solvent barrier, AMH-Go): pairs closer than
+ * pair_list_cutoff + skin count, as in a list LAMMPS has just built
+ * (fb.cpp:1805-1813).  Default 0. */
+void or_set_list_skin(or_ctx *c, double skin);
+
 /* Excluded-volume pair style settings (pair_style vexcluded p rs rl +
  * pair_coeff per type pair, pev.cpp:175-265).  Atom types: 1 CA, 2 N,
  * 3 O, 4 CB, 5 H-beta (Gly).  Defaults are the deck written by

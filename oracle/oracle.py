@@ -65,6 +65,7 @@ def lib():
         L.or_pap_nu.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_set_vexcluded.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.or_set_bonds.argtypes = [C.c_void_p, C.c_double, dp]
+        L.or_set_list_skin.argtypes = [C.c_void_p, C.c_double]
         L.or_langevin_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_ulonglong, C.c_int, C.c_longlong, dp, dp, dp, dp]
         L.or_langevin_run.restype = C.c_int
@@ -110,6 +111,12 @@ class Oracle:
     def set_bonds(self, k: float, r0):
         """bond_style harmonic + bond_coeff: K, and r0 of CA-CA, CA-O, O-CA(+1), CA-CB, CA-H-beta."""
         lib().or_set_bonds(self.h, float(k), np.ascontiguousarray(r0, dtype=np.float64))
+
+    def set_list_skin(self, skin: float):
+        """Neighbour-list skin of the list-limited terms (Debye-Hueckel, the
+        solvent barrier's CA-CA pairs, AMH-Go): pairs closer than
+        pair_list_cutoff + skin count (a list just built by LAMMPS)."""
+        lib().or_set_list_skin(self.h, float(skin))
 
     def set_vexcluded(self, p: int, lam, cut_short, cut_long):
         """pair_style vexcluded p + pair_coeff: [6][6] tables by atom type (1 CA, 2 N, 3 O, 4 CB, 5 H-beta)."""
