@@ -34,7 +34,7 @@ ERRORS = {-1: "AWSEM_E_ARG", -2: "AWSEM_E_FILE", -3: "AWSEM_E_FORMAT", -4: "AWSE
           -9: "AWSEM_E_NONFINITE", -10: "AWSEM_E_STATE", -11: "AWSEM_E_RANGE"}
 
 EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_compute", "awsem_compute_res",
-           "awsem_set_vexcluded", "awsem_set_bonds", "awsem_set_term_mask", "awsem_list", "awsem_run",
+           "awsem_set_vexcluded", "awsem_set_bonds", "awsem_set_list_skin", "awsem_set_term_mask", "awsem_list", "awsem_run",
            "awsem_n_residues", "awsem_pair_list_cutoff", "awsem_last_error", "awsem_destroy",
            "awsem_batch_create", "awsem_batch_size", "awsem_batch_set_state", "awsem_batch_get_state",
            "awsem_batch_compute", "awsem_batch_run", "awsem_batch_timing", "awsem_batch_phase_clock",
@@ -95,6 +95,7 @@ def lib():
         L.awsem_set_vexcluded.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
         L.awsem_set_bonds.argtypes = [vp, C.c_int, C.c_double, dp]
         L.awsem_set_term_mask.argtypes = [vp, C.c_uint32]
+        L.awsem_set_list_skin.argtypes = [vp, C.c_double]
         L.awsem_list.argtypes = [vp, C.c_int, C.c_int, dp, C.POINTER(C.c_int64), C.c_int, ip]
         L.awsem_run.argtypes = [vp, C.c_int, C.POINTER(Langevin), dp, dp, dp, dp]
         L.awsem_n_residues.argtypes = [vp, ip]
@@ -230,6 +231,11 @@ class FixBackbone:
 
     def set_term_mask(self, mask: int):
         self._check(lib().awsem_set_term_mask(self.h, mask & 0xFFFFFFFF))
+
+    def set_list_skin(self, skin: float):
+        """Neighbour-list skin of the list-limited terms (Debye-Hueckel, the
+        solvent barrier, AMH-Go): pairs closer than pair_list_cutoff + skin."""
+        self._check(lib().awsem_set_list_skin(self.h, float(skin)))
 
     def set_vexcluded(self, enable: bool = True, p: int = 2, lam=None, cut_short=None, cut_long=None):
         arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(36) for a in (lam, cut_short, cut_long)]

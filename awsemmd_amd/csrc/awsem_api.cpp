@@ -80,6 +80,7 @@ struct DevBatch {
   {
     AwBatchPtrs p;
     p.rep = d_rep; p.nrep = nrep; p.max_n = max_n; p.fp32 = fp32; p.lds_resident = lds; p.table = P.fm_tb_flag;
+    p.extras = P.xt_on;
     p.params = d_params; p.queue = d_queue; p.order = d_order; p.prof = d_prof;
     p.nshare = nshare; p.share_rep = d_share_rep; p.share_n = d_share_n; p.share_word = d_share_word;
     return p;
@@ -252,6 +253,9 @@ static AwStatus make_dev_batch(const std::vector<const AwSystem *> &sys, int pre
   D->nrep = (int)sys.size();
   D->fp32 = precision == AWSEM_PREC_FP32;
   D->P = sys[0]->P;
+  for (const AwSystem *S : sys)
+    if (!aw_same_xt_params(S->P, D->P))
+      return AwStatus::err(AWSEM_E_ARG, "batch replicas must share one coeff parameter set (the optional terms differ)");
   int res = 0;
   for (const AwSystem *S : sys) { D->res_off.push_back(res); res += S->n; D->max_n = std::max(D->max_n, S->n); }
   D->total_res = res;
@@ -408,8 +412,8 @@ static AwStatus fm_table_setup(AwSystem &S, const std::string &dir)
 static AwStatus load_system(const std::string &coeff, const std::string &seq_path, const std::string &dir, AwSystem &S,
                             bool from_files)
 {
-  std::string mem_file, gamma_file;
-  AwStatus s = aw_parse_coeff(coeff, S.P, mem_file, gamma_file);
+  std::string mem_file, gamma_file, cr_file;
+  AwStatus s = aw_parse_coeff(coeff, S.P, mem_file, gamma_file, cr_file);
   if (!s.ok()) return s;
   if (from_files) {
     if (!(s = aw_read_sequence(seq_path, S)).ok()) return s;
@@ -425,6 +429,7 @@ static AwStatus load_system(const std::string &coeff, const std::string &seq_pat
     if (from_files && (S.P.fm_flag || !fm_tables_stored(S.P, dir)) && !(s = aw_read_mems(dir, mem_file, S)).ok())
       return s;
   }
+  S.cr_file = cr_file;
   return AwStatus();
 }
 
@@ -438,6 +443,7 @@ int awsem_create(const char *coeff_path, const char *seq_path, const char *workd
   c->precision = precision;
   std::string dir = workdir ? workdir : ".";
   AwStatus s = load_system(coeff_path, seq_path, dir, c->sys, true);
+  if (s.ok()) s = aw_read_extras(dir, c->sys.cr_file, c->sys);
   if (s.ok()) s = aw_finish_system(c->sys);
   if (s.ok()) s = fm_table_setup(c->sys, dir);
   if (!s.ok()) {
@@ -515,6 +521,7 @@ int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, 
       }
     }
   }
+  if (s.ok()) s = aw_read_extras(param_dir ? param_dir : ".", S.cr_file, S);
   if (s.ok()) s = aw_finish_system(S);
   if (s.ok()) s = fm_table_setup(S, param_dir ? param_dir : ".");
   *out = c.release();
@@ -578,6 +585,15 @@ int awsem_set_bonds(awsem_ctx *c, int enable, double k, const double *r0)
   }
   c->dirty = true;
   return AWSEM_OK;
+}
+
+int awsem_set_list_skin(awsem_ctx *c, double skin)
+{
+  if (!c || c->sys.n < 0 || !(skin >= 0.0)) return AWSEM_E_ARG;
+  c->sys.P.list_skin = skin;
+  AwStatus s = aw_finish_system(c->sys);
+  c->dirty = true;
+  return s.ok() ? AWSEM_OK : fail(c, s);
 }
 
 int awsem_set_term_mask(awsem_ctx *c, uint32_t mask)

@@ -4,15 +4,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "awsem_terms.h"
+#include "awsem_extra.h"
 #include "awsem_launch.h"
 
 // ------------------------------------------------------------------------
-// energy epilogue: ered[N_EACC] (LDS) -> out[AW_N_ET]
+// energy epilogue: ered[N_ERED] (LDS) -> out[AW_N_ET]
 
 // phase engine: blocks add their partial sums
 __device__ __forceinline__ void energy_add(const double *ered, double *out)
 {
-  if (threadIdx.x < N_EACC) {
+  if (threadIdx.x < N_ERED) {
     double v = ered[threadIdx.x];
     if (v != 0.0) atomicAdd(&out[eacc_slot[threadIdx.x]], v);
   }
@@ -25,9 +26,9 @@ __device__ __forceinline__ void energy_write(const double *ered, double *out)
   if (t < AW_N_ET) {
     double v = 0.0;
     if (t == 0) {
-      for (int k = 0; k < N_EACC; ++k) v += ered[k];
+      for (int k = 0; k < N_ERED; ++k) v += ered[k];
     } else {
-      for (int k = 0; k < N_EACC; ++k)
+      for (int k = 0; k < N_ERED; ++k)
         if (eacc_slot[k] == t) v = ered[k];
     }
     out[t] = v;
@@ -334,13 +335,16 @@ __device__ void share_help(const BatchDev &B, double *smem)
   }
 }
 
-template <class T, bool LDSM, bool TB>
+// VAR: bit 0 the fragment-memory table pass (table mode), bit 1 the pass of
+// the coeff-toggled terms; instances 0, 1 and 3 exist, so batches without
+// either run a kernel with neither call
+template <class T, bool LDSM, int VAR>
 __device__ __forceinline__ void force_eval_block(const View<T> &V, const ReplicaDesc &R, unsigned long long *prof,
                                                  bool first, bool ef)
 {
   int t0 = threadIdx.x, ts = blockDim.x;
   unsigned long long t = prof ? wall_clock64() : 0;
-  if (threadIdx.x < N_EACC) V.ered[threadIdx.x] = 0.0;
+  if (threadIdx.x < N_ERED) V.ered[threadIdx.x] = 0.0;
   if (V.sh_n > 0) {
     share_snapshot(R, V.x(), V.n);
     if (threadIdx.x == 0) *V.sh_mask = 0ull;
@@ -352,7 +356,7 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
   prof_mark(prof, first, 0, t);
   // table mode: the table pass runs here, where few registers are live (its
   // call inside the pair pass costs spills there); forces are zeroed
-  if constexpr (TB) {
+  if constexpr ((VAR & 1) != 0) {
     fm_table_pass(V, t0, ts, ef);
     if (prof) {
       __syncthreads();
@@ -367,6 +371,10 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
     __syncthreads();
   }
   prof_mark(prof, first, 11, t);
+  // coeff-toggled terms: positions and the list only, forces by atomics
+  // (no barrier before phase B, which touches neither their inputs nor
+  // their accumulators)
+  if constexpr ((VAR & 2) != 0) xt_pass(V, R, t0, ts, ef);
   // phase B in one part, or (clocked run) its three parts separately; one
   // call site keeps a single copy of the code
   int nparts = prof ? 3 : 1;
@@ -387,14 +395,14 @@ __device__ __forceinline__ void force_eval_block(const View<T> &V, const Replica
   prof_mark(prof, first, 5, t);
 }
 
-// TB: the instance with the fragment-memory table pass (table mode only)
-template <class T, bool LDSM, int NT, bool TB>
+// VAR: the passes compiled in (force_eval_block)
+template <class T, bool LDSM, int NT, int VAR>
 __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__restrict__ Pr, LgParams lg, int mode)
 {
   static_assert(NT == 64 * AW_SHARE_TAKE, "a helping workgroup evaluates one claimed chunk per wave");
   extern __shared__ double smem[];
   __shared__ int s_rep;
-  __shared__ double ered[N_EACC];
+  __shared__ double ered[N_ERED];
   __shared__ __attribute__((aligned(8))) int nl_state[8];  // [2..5]: two 64-bit queue-counter words
   __shared__ unsigned long long sh_mask;
   __shared__ int sh_ok;
@@ -456,7 +464,7 @@ __global__ void __launch_bounds__(NT) k_resident(BatchDev B, const FFParams *__r
         prof_mark(B.prof, first, 6, t);
       }
       // energies leave the launch from its last evaluation only
-      force_eval_block<T, LDSM, TB>(V, R, B.prof, first, s == nst - 1);
+      force_eval_block<T, LDSM, VAR>(V, R, B.prof, first, s == nst - 1);
       // a replica whose wait gave up evaluates its own fragment memory from
       // then on (the launch is reported as failed)
       if (V.sh_n > 0 && !share_finish(V, R, s == nst - 1, B.queue + 32, &sh_ok)) V.sh_n = 0;
@@ -493,9 +501,11 @@ static void prepare_resident_t()
   int dev = 0, lim = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0) lim = 65536;
-  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             lim - AW_LDS_STATIC_RESERVE);
-  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lim - AW_LDS_STATIC_RESERVE);
+  (void)hipFuncSetAttribute((const void *)k_resident<T, true, 512, 3>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             lim - AW_LDS_STATIC_RESERVE);
 }
 
@@ -516,12 +526,15 @@ static int launch_resident_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t
   if (p.lds_resident) {
     // the dynamic-LDS limit is raised once, outside any graph capture
     // (aw_resident_prepare)
-    if (p.table) hipLaunchKernelGGL((k_resident<T, true, 512, true>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
-    else hipLaunchKernelGGL((k_resident<T, true, 512, false>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+    if (p.extras) hipLaunchKernelGGL((k_resident<T, true, 512, 3>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+    else if (p.table) hipLaunchKernelGGL((k_resident<T, true, 512, 1>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+    else hipLaunchKernelGGL((k_resident<T, true, 512, 0>), dim3(grid), dim3(512), lds, st, B, B.P, lg, mode);
+  } else if (p.extras) {
+    hipLaunchKernelGGL((k_resident<T, false, 512, 3>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   } else if (p.table) {
-    hipLaunchKernelGGL((k_resident<T, false, 512, true>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
+    hipLaunchKernelGGL((k_resident<T, false, 512, 1>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   } else {
-    hipLaunchKernelGGL((k_resident<T, false, 512, false>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
+    hipLaunchKernelGGL((k_resident<T, false, 512, 0>), dim3(grid), dim3(512), 0, st, B, B.P, lg, mode);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -88,6 +88,28 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
   }
   if (scratch) plan.off_scr = take(cur, (size_t)aw_slab_doubles(n) * sizeof(double));
   plan.off_fault = take(cur, 256);
+  // coeff-toggled terms (per image: they are small and follow the
+  // system's own side files)
+  AwXtPack xt;
+  if (FP.xt_on) {
+    AwStatus s = aw_build_extras(S, FP, xt);
+    if (!s.ok()) return s;
+  }
+  plan.has_go = !xt.go_start.empty();
+  plan.has_zres = FP.memb_flag && !S.xt.zres.empty();
+  plan.n_go = (int)xt.go.size(); plan.n_cr = (int)xt.cr.size(); plan.n_dh = (int)xt.dh.size(); plan.n_vfm = (int)xt.vfm.size();
+  plan.off_gos = take(cur, xt.go_start.size() * sizeof(int));
+  plan.off_go = take(cur, xt.go.size() * sizeof(XtGo));
+  plan.off_cr = take(cur, xt.cr.size() * sizeof(XtPair));
+  plan.off_dh = take(cur, xt.dh.size() * sizeof(XtPair));
+  plan.off_vfm = take(cur, xt.vfm.size() * sizeof(XtPair));
+  plan.off_zres = take(cur, plan.has_zres ? (size_t)n * sizeof(int) : 0);
+  plan.go_norm = S.xt.go_norm;
+  // the system's own reach (its contact list may widen pair_list_cutoff)
+  // with the force field's list skin
+  double lr = S.P.pair_list_cutoff + FP.list_skin;
+  plan.list_r2 = lr * lr;
+  plan.cr_cut2 = S.P.cr_glob_cutoff_sq;
   plan.bytes = std::max<size_t>(cur, 256);
   // the image: zeros (state, list headers, slab) and the packed constants
   std::vector<unsigned char> &img = plan.image;
@@ -113,6 +135,12 @@ AwStatus aw_devsys_plan(const AwSystem &S, const FFParams *P, bool fp32, bool sc
     put(img, plan.off_coef, coef);
   }
   put(img, plan.off_tbi, tb_items);
+  put(img, plan.off_gos, xt.go_start);
+  put(img, plan.off_go, xt.go);
+  put(img, plan.off_cr, xt.cr);
+  put(img, plan.off_dh, xt.dh);
+  put(img, plan.off_vfm, xt.vfm);
+  if (plan.has_zres) put(img, plan.off_zres, S.xt.zres);
   if (lib && !S.fmt.e.empty()) {
     const AwFMTable &T = S.fmt;
     if (fp32) {
@@ -170,5 +198,15 @@ AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out)
   R.tb_curve = b + plan.off_tbc;
   R.fault = (int *)(b + plan.off_fault);
   R.cell_h = plan.cell_h;
+  R.n_go = plan.n_go; R.n_cr = plan.n_cr; R.n_dh = plan.n_dh; R.n_vfm = plan.n_vfm;
+  R.go_start = plan.has_go ? (const int *)(b + plan.off_gos) : nullptr;
+  R.go_item = b + plan.off_go;
+  R.cr_item = b + plan.off_cr;
+  R.dh_item = b + plan.off_dh;
+  R.vfm_item = b + plan.off_vfm;
+  R.zres = plan.has_zres ? (const int *)(b + plan.off_zres) : nullptr;
+  R.go_norm = plan.go_norm;
+  R.list_r2 = plan.list_r2;
+  R.cr_cut2 = plan.cr_cut2;
   return AwStatus();
 }

@@ -46,6 +46,24 @@ struct AwFMTable {
   int n_curves() const { return tb_size ? (int)(e.size() / tb_size) : 0; }
 };
 
+// coeff-toggled terms' side files (fb.cpp:607-620, 695-743, 1018-1046,
+// 1106-1170)
+struct AwExtras {
+  std::vector<int> zres;              // membrane class per residue (zim)
+  std::vector<double> charge;         // Debye-Hueckel charge per residue
+  // contact restraints: entries sorted by (lower, upper residue, file order)
+  std::vector<int> cr_i, cr_j;
+  std::vector<double> cr_w, cr_r0;    // w already times k_cont_rest
+  // AMH-Go: gamma classes, the native structure (one memory over the
+  // system), censored pairs (flag 1, n*n) or native distances (flag 2,
+  // CA-CA, CB-CB, CA-CB, n*n each), normalisation
+  AwGamma amhg;
+  AwMemory amh_mem;
+  std::vector<unsigned char> censor;
+  std::vector<double> rn[3];
+  double go_norm = 1.0;
+};
+
 struct AwSystem {
   FFParams P;
   int n = 0, nch = 0;
@@ -56,11 +74,28 @@ struct AwSystem {
   AwGamma fmg;
   std::vector<AwMemory> mems;
   AwFMTable fmt;                       // table mode only
+  AwExtras xt;                         // coeff-toggled terms
+  std::string cr_file;                 // contact-restraints file named by the coeff
   char frag_mems_file[256] = {0}, fm_gamma_file[256] = {0};
 };
 
 int aw_restype(char c);
-AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file);
+AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file,
+                        std::string &cr_file);
+// the coeff-toggled terms' side files (after the sequence, tables and
+// memories): contact list, zim, charges, AMH-Go files; their share of
+// pair_list_cutoff (fb.cpp:1058-1066) and the AMH-Go normalisation
+AwStatus aw_read_extras(const std::string &dir, const std::string &cr_file, AwSystem &S);
+// AMH-Go native distance of (i, ia) - (j, ja), ia/ja 0 CA, 1 CB
+double aw_go_rnative(const AwSystem &S, int i, int ia, int j, int ja);
+// the device items of the coeff-toggled terms (AwExtras + memories)
+struct AwXtPack {
+  std::vector<int> go_start;          // [2n + 1], empty without AMH-Go
+  std::vector<XtGo> go;
+  std::vector<XtPair> cr, dh, vfm;
+};
+AwStatus aw_build_extras(const AwSystem &S, const FFParams &P, AwXtPack &X);
+bool aw_same_xt_params(const FFParams &a, const FFParams &b);
 AwStatus aw_read_param_tables(const std::string &dir, AwSystem &S);
 AwStatus aw_read_sequence(const std::string &path, AwSystem &S);
 AwStatus aw_read_ssweight(const std::string &dir, AwSystem &S);
@@ -144,6 +179,7 @@ void aw_fm_device(const AwFMPack &B, std::vector<ItemT> &items, std::vector<T> &
   }
 }
 void aw_default_pair_styles(FFParams &P);
+double aw_ssb_cut(const FFParams &P);
 // ---------------------------------------------------------------------------
 // One system's device image (awsem_devsys.cpp): a single allocation holding
 // its per-residue constants, fragment-memory tables, Verlet list block and,
@@ -156,6 +192,12 @@ struct AwDevPlan {
   size_t off_items = 0, off_rf = 0, off_coef = 0, off_nl = 0, off_nlx = 0, off_scr = 0, off_cells = 0;
   size_t off_tbi = 0, off_tbc = 0, off_fault = 0;  // table items, curves, error word
   int n_tb_items = 0;
+  // coeff-toggled terms: AMH-Go start offsets and pairs, contact restraints,
+  // Debye-Hueckel pairs, vector-memory pairs, membrane classes
+  size_t off_gos = 0, off_go = 0, off_cr = 0, off_dh = 0, off_vfm = 0, off_zres = 0;
+  int n_go = 0, n_cr = 0, n_dh = 0, n_vfm = 0;
+  bool has_go = false, has_zres = false;
+  double go_norm = 1.0, list_r2 = 0.0, cr_cut2 = 0.0;
   int cell_h = 0;
   std::vector<int> fm_ij;  // (i << 16) | j of every fragment-memory item, in device order
   size_t bytes = 0;

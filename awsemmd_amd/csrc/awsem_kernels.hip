@@ -12,6 +12,7 @@
 //   D  helix (i,i+4) pairs (7843-7868, 3706-3785) and water xi (DL2, 7879-7937)
 //   E  xi *= dsigma/drho (7939-7950)
 //   F  water / burial / helix pair forces (DL3, 7996-8057)
+// Coeff-toggled terms (awsem_extra.h) run in their own pass after A.
 // Two engines run the same phase bodies:
 //   * resident: one workgroup owns one replica, its coordinates, forces and
 //     per-residue scalars live in LDS (or in a global scratch slab for
@@ -56,16 +57,17 @@ __global__ void __launch_bounds__(256) k_ph_A(BatchDev B, int r)
   template <class T>                                                                   \
   __global__ void __launch_bounds__(256) NAME(BatchDev B, int r)                       \
   {                                                                                    \
-    __shared__ double ered[N_EACC];                                                    \
+    __shared__ double ered[N_ERED];                                                    \
     View<T> V = phase_view<T>(B, r);                                                   \
     V.ered = ered;                                                                     \
-    if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;                                 \
+    if (threadIdx.x < N_ERED) ered[threadIdx.x] = 0.0;                                 \
     __syncthreads();                                                                   \
     CALL;                                                                              \
     __syncthreads();                                                                   \
     energy_add(ered, B.rep[r].energy);                                                 \
   }
 
+PH_KERNEL(k_ph_X, xt_pass(V, B.rep[r], blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, true))
 PH_KERNEL(k_ph_B, (phase_B<T, false>(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, 7)))
 PH_KERNEL(k_ph_C, phase_C(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
 PH_KERNEL(k_ph_D, phase_D(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x))
@@ -146,7 +148,8 @@ __global__ void __launch_bounds__(256) k_list(BatchDev B, int r, int kind, int a
         code = (long long)i * n + j;
       if (kind == 6 && P->water_flag && (!same || dres >= P->contact_cutoff) && arg < P->n_wells && P->well_flag[arg]) {
         int it = rtype(ri), jt = rtype(rj);
-        double g0 = P->water_gamma[arg][it][jt][0], g1 = P->water_gamma[arg][it][jt][1];
+        const auto wg = ((ri | rj) & RI_PHOS) ? P->phosph_gamma : P->water_gamma;
+        double g0 = wg[arg][it][jt][0], g1 = wg[arg][it][jt][1];
         if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > P->w_rmin_th_sq[arg] && rsq < P->w_rmax_th_sq[arg])
           code = (long long)i * n + j;
       }
@@ -257,9 +260,10 @@ int aw_launch_resident(const AwBatchPtrs &p, const AwRun &run, hipStream_t st)
 }
 
 template <class T>
-static void force_phases(const BatchDev &B, int r, int grid, hipStream_t st)
+static void force_phases(const BatchDev &B, int r, int grid, hipStream_t st, bool xt)
 {
   hipLaunchKernelGGL(k_ph_A<T>, dim3(grid), dim3(256), 0, st, B, r);
+  if (xt) hipLaunchKernelGGL(k_ph_X<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_B<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_C<T>, dim3(grid), dim3(256), 0, st, B, r);
   hipLaunchKernelGGL(k_ph_D<T>, dim3(grid), dim3(256), 0, st, B, r);
@@ -277,11 +281,11 @@ static int launch_phases_t(const AwBatchPtrs &p, const AwRun &run, hipStream_t s
   int grid = run.grid > 0 ? run.grid : 1024;
   for (int r = 0; r < p.nrep; ++r) {
     hipLaunchKernelGGL(k_ph_load<T>, dim3(grid), dim3(256), 0, st, B, r);
-    force_phases<T>(B, r, grid, st);
+    force_phases<T>(B, r, grid, st, p.extras != 0);
     if (run.mode == 1) {
       for (int s = 0; s < run.nsteps; ++s) {
         hipLaunchKernelGGL(k_ph_int1<T>, dim3(grid), dim3(256), 0, st, B, r, lg);
-        force_phases<T>(B, r, grid, st);
+        force_phases<T>(B, r, grid, st, p.extras != 0);
         hipLaunchKernelGGL(k_ph_int2<T>, dim3(grid), dim3(256), 0, st, B, r, lg, s);
       }
     }

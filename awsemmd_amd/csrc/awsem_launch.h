@@ -11,6 +11,7 @@ struct AwBatchPtrs {
   int lds_resident = 0;           // resident engine keeps replicas in LDS
   int table = 0;                  // fragment-memory table mode (the resident engine's
                                   // instance with the table pass; the others have none)
+  int extras = 0;                 // coeff-toggled terms on (the engines' extra pass)
   const void *params = nullptr;   // FFParams on the device
   int *queue = nullptr;           // device scratch: [0] work-queue counter, [16] replicas finished,
                                   // [32] sharing error flag, [48] chunks evaluated elsewhere (the

@@ -33,13 +33,13 @@ static std::string join(const std::string &dir, const std::string &name)
 }
 
 static const char *kUnsupported[] = {
-    "[AMH-Go]", "[Contact_Restraints]", "[Vector_Fragment_Memory]",
-    "[Solvent_Barrier]", "[Membrane]", "[Fragment_Frustratometer]", "[Tertiary_Frustratometer]",
-    "[Nmer_Frustratometer]", "[Phosphorylation]", "[Amylometer]", "[Selection_Temperature]",
-    "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]", "[DebyeHuckel]",
+    "[Fragment_Frustratometer]", "[Tertiary_Frustratometer]",
+    "[Nmer_Frustratometer]", "[Amylometer]", "[Selection_Temperature]",
+    "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]",
     "[DebyeHuckel_Optimization]", nullptr};
 
-AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file)
+AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file,
+                        std::string &cr_file)
 {
   std::ifstream in(path);
   if (!in) return AwStatus::err(-2, "Coefficient file was not found!");
@@ -142,6 +142,37 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
         return AwStatus::err(-3, "Fragment_Memory_Table: empty distance grid");
     } else if (sec == "[Epsilon]") {
       in >> P.epsilon;
+    } else if (sec == "[AMH-Go]") {  // fb.cpp:297-303
+      P.amhgo_flag = 1;
+      in >> P.k_amh_go >> P.amh_go_p >> P.amh_go_rc >> P.amh_go_censor;
+    } else if (sec == "[Contact_Restraints]") {  // fb.cpp:321-327
+      P.cr_flag = 1;
+      in >> P.k_cont_rest >> P.cr_sigma >> cr_file;
+      P.cr_sigma_sq_inv = std::pow(P.cr_sigma, -2);
+    } else if (sec == "[Vector_Fragment_Memory]") {  // fb.cpp:328-333
+      P.vfm_flag = 1;
+      in >> P.k_vfm >> P.vfm_sigma;
+      P.vfm_sigma_sq = P.vfm_sigma * P.vfm_sigma;
+    } else if (sec == "[Solvent_Barrier]") {  // fb.cpp:334-345
+      P.ssb_flag = 1;
+      in >> P.k_ssb1 >> P.ssb_rmin1 >> P.ssb_rmax1 >> P.k_ssb2 >> P.ssb_rmin2 >> P.ssb_rmax2 >> P.ssb_kappa;
+      in >> P.ssb_ij_sep >> P.ssb_rad_cor;
+      for (int j = 0; j < 20; ++j) in >> P.ssb_rshift[j];
+    } else if (sec == "[Membrane]") {  // fb.cpp:346-358
+      P.memb_flag = 1;
+      in >> P.k_memb >> P.memb_kbin >> P.memb_xo[0] >> P.memb_xo[1] >> P.memb_xo[2] >> P.memb_pore_type;
+      in >> P.memb_len >> P.memb_rho0_max >> P.memb_rho0_distor;
+      for (int a = 0; a < 3; ++a)
+        for (int j = 0; j < 4; ++j) in >> P.g_memb[a][j];
+    } else if (sec == "[Phosphorylation]") {  // fb.cpp:412-421
+      if (!P.water_flag) return AwStatus::err(-3, "Cannot run phosphorylation without water potential");
+      P.phosph_flag = 1;
+      in >> P.k_hypercharge >> P.n_phosph;
+      if (P.n_phosph > 20) return AwStatus::err(-3, "Number of phosphorylated residues may not exceed 20");
+      for (int j = 0; j < P.n_phosph; ++j) in >> P.phosph_res[j];
+    } else if (sec == "[DebyeHuckel]") {  // fb.cpp:466-478
+      P.dh_flag = 1;
+      in >> P.dh_k[0] >> P.dh_k[1] >> P.dh_k[2] >> P.dh_kscreen >> P.dh_length >> P.dh_min_sep;
     } else if (sec == "[Shuffler]" || sec == "[Mutate_Sequence]") {
       int fl = 0;
       std::string name;
@@ -164,6 +195,8 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
   P.k_helix *= e;
   P.k_dssp *= e;
   P.k_global_pap *= e;
+  P.k_amh_go *= e;
+  P.k_cont_rest *= e;
   for (int j = 0; j < P.n_rama_par; ++j) P.w[j] *= P.k_rama;
   for (int j = 6; j < 6 + P.n_rama_p_par; ++j) P.w[j] *= P.k_rama;
   for (int j = 0; j < 12; ++j) {
@@ -208,10 +241,24 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
     plc = std::fmax(plc, std::fmax(r1, r2));
     plc = std::fmax(plc, P.helix_cutoff);
   }
+  // (the contact restraints' share follows their file, aw_read_extras)
+  if (P.dh_flag) plc = std::fmax(plc, 2.0 * P.dh_length / P.dh_kscreen);
+  if (P.ssb_flag) plc = std::fmax(plc, aw_ssb_cut(P));
   P.pair_list_cutoff = plc;
+  P.xt_on = (P.amhgo_flag || P.cr_flag || P.vfm_flag || P.ssb_flag || P.memb_flag || P.dh_flag) ? 1 : 0;
   aw_default_pair_styles(P);
   P.nl_skin = 2.0;
   return AwStatus();
+}
+
+// the barrier's reach (fb.cpp:1060-1066): twice the largest radius shift,
+// plus the larger outer radius and 10 / kappa
+double aw_ssb_cut(const FFParams &P)
+{
+  double cut = 0.0;
+  if (P.ssb_rad_cor)
+    for (int i = 0; i < 20; ++i) cut = std::fmax(cut, 2.0 * P.ssb_rshift[i]);
+  return cut + std::fmax(P.ssb_rmax1, P.ssb_rmax2) + 10.0 / P.ssb_kappa;
 }
 
 void aw_default_pair_styles(FFParams &P)
@@ -276,6 +323,22 @@ AwStatus aw_read_param_tables(const std::string &dir, AwSystem &S)
           P.water_gamma[w][i][j][0] = P.water_gamma[w][j][i][0] = g0 * k_water;
           P.water_gamma[w][i][j][1] = P.water_gamma[w][j][i][1] = g1 * k_water;
         }
+  }
+  if (P.phosph_flag) {  // fb.cpp:643-669
+    for (int w = 0; w < P.n_wells; ++w)
+      for (int i = 0; i < 20; ++i)
+        for (int j = 0; j < 20; ++j)
+          for (int k = 0; k < 2; ++k) P.phosph_gamma[w][i][j][k] = P.water_gamma[w][i][j][k];
+    // serine (15) takes glutamate's (6) entries times k, k^2 or 1 by the
+    // four-letter class, row by row in this order: an entry written earlier
+    // in the loop is read later ((15, 15) from (15, 6), set at i = 6)
+    for (int w = 0; w < P.n_wells; ++w)
+      for (int i = 0; i < 20; ++i) {
+        double f = kFour[i] == 1 ? P.k_hypercharge
+                 : (kFour[i] == 2 || kFour[i] == 3) ? std::pow(P.k_hypercharge, 2) : 1.0;
+        for (int k = 0; k < 2; ++k)
+          P.phosph_gamma[w][i][15][k] = P.phosph_gamma[w][15][i][k] = P.phosph_gamma[w][i][6][k] * f;
+      }
   }
   if (P.burial_flag) {  // fb.cpp:684-693
     if (!P.water_flag) return AwStatus::err(-3, "Cannot use burial without water potential");
@@ -577,6 +640,13 @@ AwStatus aw_finish_system(AwSystem &S)
       if (S.se[i] == 'P') ri |= RI_PRO;
       S.rinfo[i] = ri;
     }
+  if (P.phosph_flag)
+    for (int j = 0; j < P.n_phosph; ++j) {
+      int r = P.phosph_res[j];
+      if (r == 0) continue;  // fb.cpp:676: a zero index names no residue
+      if (r < 1 || r > S.n) return AwStatus::err(-3, "Phosphorylation: residue index out of range");
+      S.rinfo[r - 1] |= RI_PHOS;
+    }
   if (P.fm_flag || P.fm_tb_flag) {
     P.fm_min_sep = S.fmg.min_sep();
     P.fm_max_sep = S.fmg.max_sep();
@@ -602,7 +672,11 @@ AwStatus aw_finish_system(AwSystem &S)
       for (int b = 0; b < 6; ++b)
         if (P.ev_lambda[a][b] != 0.0) evr = std::fmax(evr, std::fmax(P.ev_cut_short[a][b], P.ev_cut_long[a][b]));
   P.ev_reach = evr;
+  // the barrier sweeps the residue pairs of the Verlet list: its reach
+  if (P.ssb_flag) reach = std::fmax(reach, aw_ssb_cut(P));
   P.pair_reach = std::fmax(reach, evr);
+  double lr = P.pair_list_cutoff + P.list_skin;
+  P.list_r2 = lr * lr;
   return AwStatus();
 }
 
@@ -858,4 +932,254 @@ void aw_fm_table_items(const AwSystem &S, std::vector<int> &items)
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// coeff-toggled terms: side files
+
+// native distance of an AMH-Go pair: the go_rnative*.dat matrices with
+// censoring flag 2 (the CA-CB one for both mixed orders, fb.cpp:3904-3909),
+// else Fragment_Memory::Rf of amh-go.gro (fm.cpp:190-202)
+double aw_go_rnative(const AwSystem &S, int i, int ia, int j, int ja)
+{
+  const AwExtras &X = S.xt;
+  if (S.P.amh_go_censor == 2) {
+    size_t k = (size_t)i * S.n + j;
+    return X.rn[ia == ja ? ia : 2][k];
+  }
+  const AwMemory &m = X.amh_mem;
+  const double *a, *b;
+  if (ia == 0 && ja == 0) { int lo = std::min(i, j), hi = std::max(i, j); a = &m.ca[3 * lo]; b = &m.ca[3 * hi]; }
+  else if (ia == 1 && ja == 1) { int lo = std::min(i, j), hi = std::max(i, j); a = &m.cb[3 * hi]; b = &m.cb[3 * lo]; }
+  else if (ia == 0) { a = &m.ca[3 * i]; b = &m.cb[3 * j]; }
+  else { a = &m.ca[3 * j]; b = &m.cb[3 * i]; }
+  if (a == b) return 0.0;
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+AwStatus aw_read_extras(const std::string &dir, const std::string &cr_file, AwSystem &S)
+{
+  FFParams &P = S.P;
+  AwExtras &X = S.xt;
+  int n = S.n;
+  if (P.cr_flag) {  // read_contact_restraints_file, fb.cpp:1106-1170
+    std::ifstream in(join(dir, cr_file));
+    if (!in) return AwStatus::err(-2, "File for Contact Restraints potential doesn't exist");
+    struct E { int i, j, ord; double w, r0; };
+    std::vector<E> es;
+    double r0_max = 0.0;
+    int a, b;
+    double w, r0;
+    while (in >> a >> b >> w >> r0) {
+      int j1 = std::min(a, b) - 1, j2 = std::max(a, b) - 1;
+      if (j1 < 0 || j1 >= n || j2 < 0 || j2 >= n)
+        return AwStatus::err(-3, "Indices out of range in Contact Restraints potential file");
+      es.push_back(E{j1, j2, (int)es.size(), w * P.k_cont_rest, r0});
+      if (r0 > r0_max) r0_max = r0;
+    }
+    // by lower residue, then upper (fb.cpp:1163-1166; ties in file order)
+    std::stable_sort(es.begin(), es.end(), [](const E &x, const E &y) { return x.i != y.i ? x.i < y.i : x.j < y.j; });
+    X.cr_i.clear(); X.cr_j.clear(); X.cr_w.clear(); X.cr_r0.clear();
+    for (const E &e : es) {
+      X.cr_i.push_back(e.i); X.cr_j.push_back(e.j); X.cr_w.push_back(e.w); X.cr_r0.push_back(e.r0);
+    }
+    P.cr_dr_cutoff = P.cr_sigma * 4.29;
+    P.cr_glob_cutoff_sq = std::pow(r0_max + P.cr_dr_cutoff, 2);
+    P.pair_list_cutoff = std::fmax(P.pair_list_cutoff, std::sqrt(P.cr_glob_cutoff_sq));
+  }
+  if (P.memb_flag) {  // fb.cpp:611-620
+    std::ifstream in(join(dir, "zim"));
+    if (!in) return AwStatus::err(-2, "File zim doesn't exist");
+    X.zres.assign(n, 0);
+    for (int i = 0; i < n; ++i)
+      if (!(in >> X.zres[i])) return AwStatus::err(-3, "Membrane potential parameter file format error");
+    // the reference leaves V unset for any other class (fb.cpp:6163-6172)
+    for (int i = 0; i < n; ++i)
+      if (X.zres[i] < 1 || X.zres[i] > 3) return AwStatus::err(-3, "Membrane potential: zim class outside 1-3");
+  }
+  if (P.dh_flag) {  // fb.cpp:1018-1046
+    std::ifstream in(join(dir, "charge_on_residues.dat"));
+    if (!in) return AwStatus::err(-2, "File charge_on_residues.dat doesn't exist");
+    X.charge.assign(n, 0.0);
+    int total = 0;
+    in >> total;
+    for (int k = 0; k < total; ++k) {
+      int r;
+      double q;
+      if (!(in >> r >> q)) break;
+      if (r < 1 || r > n) return AwStatus::err(-3, "charge_on_residues.dat: residue number out of range");
+      X.charge[r - 1] = q;
+    }
+  }
+  if (P.vfm_flag && !(P.fm_flag || P.fm_tb_flag))  // the memories come from those sections (fb.cpp:749-791)
+    return AwStatus::err(-3, "Vector_Fragment_Memory needs the memories of [Fragment_Memory]");
+  if (P.amhgo_flag) {  // fb.cpp:695-743
+    AwStatus s = aw_read_gamma_array(join(dir, "amh-go.gamma"), X.amhg);
+    if (!s.ok()) {
+      if (s.code == -2) return AwStatus::err(-2, "Cannot read file amh-go.gamma");
+      size_t c = s.msg.find(": ");
+      return AwStatus::err(s.code, "AMH_Go: " + (c == std::string::npos ? s.msg : s.msg.substr(c + 2)));
+    }
+    if (X.amhg.frag_resty) return AwStatus::err(-3, "AMH-Go: Wrong call of getGamma() function");
+    if (P.amh_go_censor != 2) {
+      GroFile g = load_gro(join(dir, "amh-go.gro"));
+      int e = build_memory(g, 0, 0, n, 1.0, X.amh_mem);
+      if (e == 1) return AwStatus::err(-2, "Cannot read file amh-go.gro");
+      if (e == 2) return AwStatus::err(-3, "AMH_Go: Wrong atom count in memory structure file");
+    }
+    if (P.amh_go_censor == 1) {
+      X.censor.assign((size_t)n * n, 0);
+      std::ifstream in(join(dir, "frustration_censored_contacts.dat"));
+      int a, b;
+      while (in >> a >> b) {
+        if (a < 1 || a > n || b < 1 || b > n) return AwStatus::err(-3, "AMH-Go: censored contact out of range");
+        X.censor[(size_t)(a - 1) * n + (b - 1)] = 1;
+      }
+    } else if (P.amh_go_censor == 2) {
+      const char *names[3] = {"go_rnativeCACA.dat", "go_rnativeCBCB.dat", "go_rnativeCACB.dat"};
+      for (int q = 0; q < 3; ++q) {
+        std::ifstream in(join(dir, names[q]));
+        if (!in) return AwStatus::err(-2, "Go native distance file can't be read");
+        X.rn[q].assign((size_t)n * n, 0.0);
+        for (size_t k = 0; k < (size_t)n * n; ++k)
+          if (!(in >> X.rn[q][k])) return AwStatus::err(-3, "go_rnative*.dat file format error");
+      }
+    }
+    // compute_amhgo_normalization, fb.cpp:3787-3852: after each partner
+    // chain, |normi|^p of the running sum (not reset per chain); / 8 N
+    double norm = 0.0;
+    int iresn = 0;
+    for (int ich = 0; ich < S.nch; ++ich) {
+      int i0 = S.ch_start[ich];
+      iresn = S.ch_start[ich] + S.ch_len[ich];
+      for (int i = i0; i < iresn; ++i)
+        for (int ia = 0; ia <= (S.se[i] == 'G' ? 0 : 1); ++ia) {
+          double normi = 0.0;
+          for (int jch = 0; jch < S.nch; ++jch) {
+            for (int j = S.ch_start[jch]; j < S.ch_start[jch] + S.ch_len[jch]; ++j)
+              for (int ja = 0; ja <= (S.se[j] == 'G' ? 0 : 1); ++ja) {
+                if (std::abs(i - j) < X.amhg.min_sep()) continue;
+                if (aw_go_rnative(S, i, ia, j, ja) < P.amh_go_rc)
+                  normi += X.amhg.get(aw_restype(S.se[i]), aw_restype(S.se[j]), i, j);
+              }
+            norm += std::pow(std::fabs(normi), P.amh_go_p);
+          }
+        }
+    }
+    X.go_norm = norm / (8 * iresn);
+  }
+  return AwStatus();
+}
+
+// Device items.  AMH-Go: for every atom a = 2 i + k (CA; CB unless glycine)
+// the directed pairs a -> b the reference's force loop can count
+// (fb.cpp:3895-3950): b a CA or non-glycine CB, not censored, |i - j| >=
+// minSep or another chain, native distance < rc, nonzero gamma (a zero one
+// adds nothing).  Contact restraints: the first entry of each residue pair
+// (cr_contact_search, fb.cpp:8118-8135) whose sites are a density pair
+// (another chain or |i - j| > 1, fb.cpp:7985).  Debye-Hueckel: residue pairs
+// i < j with |i - j| > 1 (fb.cpp:8116) and >= the minimum separation
+// (6309) whose charge product is nonzero, k by the signs (6334-6343).
+// Vector fragment memory: per memory covering i (frag_mem_map), j from i +
+// minSep to the memory's end (at most i + maxSep), no glycine in the target
+// or the memory (fb.cpp:4008-4027), with the memory's angle
+// (Fragment_Memory::VM, fm.cpp:166, 239-243).
+AwStatus aw_build_extras(const AwSystem &S, const FFParams &P, AwXtPack &X)
+{
+  const AwExtras &E = S.xt;
+  int n = S.n;
+  // the force field's terms need the system's side data (a server's force
+  // field with a job context read from another coeff file)
+  if ((P.amhgo_flag && !S.P.amhgo_flag) || (P.cr_flag && !S.P.cr_flag) || (P.dh_flag && !S.P.dh_flag) ||
+      (P.memb_flag && !S.P.memb_flag))
+    return AwStatus::err(-1, "the system lacks the side files of the force field's optional terms");
+  auto chain = [&](int i) { return S.rinfo[i] >> RI_CHAIN_SHIFT; };
+  X = AwXtPack();
+  if (P.amhgo_flag) {
+    X.go_start.assign(2 * n + 1, 0);
+    int minsep = E.amhg.min_sep();
+    for (int a = 0; a < 2 * n; ++a) {
+      int i = a >> 1, ka = a & 1;
+      X.go_start[a] = (int)X.go.size();
+      if (ka == 1 && S.se[i] == 'G') continue;
+      int it = aw_restype(S.se[i]);
+      for (int j = 0; j < n; ++j) {
+        if (!E.censor.empty() && (E.censor[(size_t)i * n + j] || E.censor[(size_t)j * n + i])) continue;
+        if (!(std::abs(i - j) >= minsep || chain(i) != chain(j))) continue;
+        for (int kb = 0; kb < 2; ++kb) {
+          if (kb == 1 && S.se[j] == 'G') continue;
+          if (j == i && kb == ka) continue;
+          double rn = aw_go_rnative(S, i, ka, j, kb);
+          if (!(rn < P.amh_go_rc)) continue;
+          double g = E.amhg.get(it, aw_restype(S.se[j]), i, j);
+          if (g == 0.0) continue;
+          XtGo q;
+          q.b = 2 * j + kb; q.pad = 0;
+          q.rn = rn; q.gamma = g; q.s2 = std::pow(std::abs(i - j), 0.3);
+          X.go.push_back(q);
+        }
+      }
+    }
+    X.go_start[2 * n] = (int)X.go.size();
+  }
+  if (P.cr_flag) {
+    for (size_t k = 0; k < E.cr_i.size(); ++k) {
+      if (k > 0 && E.cr_i[k] == E.cr_i[k - 1] && E.cr_j[k] == E.cr_j[k - 1]) continue;  // not the first
+      int i = E.cr_i[k], j = E.cr_j[k];
+      if (!(chain(i) != chain(j) || j - i > 1)) continue;
+      X.cr.push_back(XtPair{i, j, E.cr_w[k], E.cr_r0[k]});
+    }
+  }
+  if (P.dh_flag) {
+    for (int i = 0; i < n; ++i) {
+      double qi = E.charge[i];
+      if (qi == 0.0) continue;
+      for (int j = i + 2; j < n; ++j) {
+        double qj = E.charge[j];
+        if (qj == 0.0 || j - i < P.dh_min_sep) continue;
+        double k = (qi > 0 && qj > 0) ? P.dh_k[0] : ((qi < 0 && qj < 0) ? P.dh_k[1] : P.dh_k[2]);
+        X.dh.push_back(XtPair{i, j, k * qi * qj, 0.0});
+      }
+    }
+  }
+  if (P.vfm_flag) {
+    int minsep = S.fmg.min_sep(), maxsep = S.fmg.max_sep();
+    for (const AwMemory &m : S.mems) {
+      for (int i = m.pos; i < m.pos + m.len - minsep; ++i) {
+        int je = m.pos + m.len - 1;
+        if (maxsep != -1) je = std::min(je, i + maxsep);
+        if (je >= n) continue;  // the reference aborts ("Missing residues in memory potential")
+        for (int j = i + minsep; j <= je; ++j) {
+          int mi = i - m.pos, mj = j - m.pos;
+          if (S.se[i] == 'G' || S.se[j] == 'G' || m.se[mi] == 'G' || m.se[mj] == 'G') continue;
+          const double *a1 = &m.ca[3 * mi], *b1 = &m.cb[3 * mi], *a2 = &m.ca[3 * mj], *b2 = &m.cb[3 * mj];
+          double u[3] = {b1[0] - a1[0], b1[1] - a1[1], b1[2] - a1[2]}, v[3] = {b2[0] - a2[0], b2[1] - a2[1], b2[2] - a2[2]};
+          double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+          double nu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), nv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+          X.vfm.push_back(XtPair{i, j, std::acos(uv / (nu * nv)), 0.0});
+        }
+      }
+    }
+  }
+  return AwStatus();
+}
+
+// the coeff-toggled terms' parameters two systems of one batch must share
+// (the kernels read them from the batch's single parameter block); the
+// per-system values (phosphorylated residues, the contact restraints'
+// global cutoff, the list reach) travel with each replica instead
+bool aw_same_xt_params(const FFParams &a, const FFParams &b)
+{
+  auto same = [](const void *x, const void *y, size_t n) { return std::memcmp(x, y, n) == 0; };
+#define AW_SAME(f) same(&a.f, &b.f, sizeof a.f)
+  return AW_SAME(amhgo_flag) && AW_SAME(cr_flag) && AW_SAME(vfm_flag) && AW_SAME(ssb_flag) && AW_SAME(memb_flag) &&
+         AW_SAME(phosph_flag) && AW_SAME(dh_flag) && AW_SAME(k_amh_go) && AW_SAME(amh_go_p) && AW_SAME(amh_go_rc) &&
+         AW_SAME(k_cont_rest) && AW_SAME(cr_sigma_sq_inv) && AW_SAME(cr_dr_cutoff) && AW_SAME(k_vfm) &&
+         AW_SAME(vfm_sigma_sq) && AW_SAME(k_ssb1) && AW_SAME(ssb_rmin1) && AW_SAME(ssb_rmax1) && AW_SAME(k_ssb2) &&
+         AW_SAME(ssb_rmin2) && AW_SAME(ssb_rmax2) && AW_SAME(ssb_kappa) && AW_SAME(ssb_rshift) &&
+         AW_SAME(ssb_ij_sep) && AW_SAME(ssb_rad_cor) && AW_SAME(k_memb) && AW_SAME(memb_kbin) && AW_SAME(memb_xo) &&
+         AW_SAME(memb_len) && AW_SAME(memb_rho0_max) && AW_SAME(memb_rho0_distor) && AW_SAME(g_memb) &&
+         AW_SAME(dh_k) && AW_SAME(dh_kscreen) && AW_SAME(dh_length) && AW_SAME(phosph_gamma) && AW_SAME(epsilon);
+#undef AW_SAME
 }

@@ -274,6 +274,7 @@ AwStatus awsem_server::round(int *n_active, int *n_queued)
       p.fp32 = fp32;
       p.lds_resident = g == 0 ? 1 : 0;
       p.table = P.fm_tb_flag;
+      p.extras = P.xt_on;
       p.params = d_params;
       p.queue = d_queue + 32 * g;
       p.order = d_order + base[g];

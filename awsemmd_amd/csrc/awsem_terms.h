@@ -327,7 +327,7 @@ struct View {
   // workgroup's LDS window holding only the kinds a pass writes
   double *fb;
   int fs;
-  double *ered;         // energy accumulators [N_EACC] (LDS)
+  double *ered;         // energy accumulators [N_ERED] (LDS)
   ParamPtr P;
   // fragment memory
   int fm_items, fm_item_base;
@@ -449,6 +449,9 @@ struct EAcc {
   AW_DEV void clear() { chain = chi = rama = ev = dssp = pap = water = burial = helix = fm = bond = shake = 0.0; }
 };
 #define N_EACC 12
+// energy accumulators of a workgroup: EAcc's twelve, then the six
+// coeff-toggled terms (awsem_extra.h)
+#define N_ERED 18
 AW_DEV double eacc_get(const EAcc &e, int k)
 {
   switch (k) {
@@ -457,9 +460,10 @@ AW_DEV double eacc_get(const EAcc &e, int k)
   case 8: return e.helix; case 9: return e.fm; case 10: return e.bond; default: return e.shake;
   }
 }
-AW_DEVCONST int eacc_slot[N_EACC] = {AW_ET_CHAIN, AW_ET_CHI, AW_ET_RAMA, AW_ET_VEXCLUDED, AW_ET_DSSP,
+AW_DEVCONST int eacc_slot[N_ERED] = {AW_ET_CHAIN, AW_ET_CHI, AW_ET_RAMA, AW_ET_VEXCLUDED, AW_ET_DSSP,
                                                  AW_ET_PAP, AW_ET_WATER, AW_ET_BURIAL, AW_ET_HELIX, AW_ET_FRAGMEM,
-                                                 AW_ET_BOND, AW_ET_SHAKE};
+                                                 AW_ET_BOND, AW_ET_SHAKE, AW_ET_AMHGO, AW_ET_VFRAGMEM,
+                                                 AW_ET_CONT_REST, AW_ET_MEMB, AW_ET_SSB, AW_ET_DH};
 
 // adds a thread's energy accumulators into ered[N_EACC] (LDS): a wave
 // reduction, then one LDS atomic per wave and term
@@ -2096,7 +2100,8 @@ AW_DEV void phase_C(const View<T> &V, int t0, int ts, bool ef = true)
   E.clear();
   ParamPtr P = V.P;
   for (int i = t0; i < V.n; i += ts) {
-    int it = rtype(V.rinfo[i]);
+    // a phosphorylated residue takes type 6's burial gammas (fb.cpp:2887-2900)
+    int it = (V.rinfo[i] & RI_PHOS) ? 6 : rtype(V.rinfo[i]);
     // the residue's reads before its stores (which could alias them) and
     // before any branch: densities and its burial gammas (per-lane gathers)
     const double rho_w = V.rho_w()[i], rho_h = V.rho_h()[i];
@@ -2222,8 +2227,10 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued, XS xs = XS())
   int it = rtype(ri), jt = rtype(rj);
   // wells 0 and 1's gammas (per-lane gathers), issued before the distance
   // so that their latency overlaps it
-  const double g00 = P->water_gamma[0][it][jt][0], g01 = P->water_gamma[0][it][jt][1];
-  const double g10 = P->water_gamma[1][it][jt][0], g11 = P->water_gamma[1][it][jt][1];
+  // a pair with a phosphorylated residue reads the hypercharged table (fb.cpp:2872-2885)
+  const auto wg = ((ri | rj) & RI_PHOS) ? P->phosph_gamma : P->water_gamma;
+  const double g00 = wg[0][it][jt][0], g01 = wg[0][it][jt][1];
+  const double g10 = wg[1][it][jt][0], g11 = wg[1][it][jt][1];
   const double swi = V.sig_w()[i], swj = V.sig_w()[j];  // with the positions
   double rsq = rsq_exact(site(V, i, ri), site(V, j, rj));
   double r = sqrt_t<T>(rsq);
@@ -2231,8 +2238,8 @@ AW_DEV void xi_pair(const View<T> &V, int i, int j, bool queued, XS xs = XS())
   for (int w = 0; w < nw; ++w) {
     WellP wp = w == 0 ? w0 : (w == 1 ? w1 : well_p(P, w));
     if (!wp.flag) continue;
-    double g0 = w == 0 ? g00 : (w == 1 ? g10 : P->water_gamma[w][it][jt][0]);
-    double g1 = w == 0 ? g01 : (w == 1 ? g11 : P->water_gamma[w][it][jt][1]);
+    double g0 = w == 0 ? g00 : (w == 1 ? g10 : wg[w][it][jt][0]);
+    double g1 = w == 0 ? g01 : (w == 1 ? g11 : wg[w][it][jt][1]);
     // fb.cpp:7925 reads !fabs(g0-g1)<delta, i.e. (!fabs(g0-g1)) < delta: true iff g0 != g1
     if (((double)(!fabs(g0 - g1)) < AW_DELTA) && rsq > wp.min2 && rsq < wp.max2) {
       T th, prd;
@@ -2284,8 +2291,9 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
   int it = rtype(ri), jt = rtype(rj);
   // wells 0 and 1's gammas (per-lane gathers), issued before the distance
   // so that their latency overlaps it
-  const double g00 = P->water_gamma[0][it][jt][0], g01 = P->water_gamma[0][it][jt][1];
-  const double g10 = P->water_gamma[1][it][jt][0], g11 = P->water_gamma[1][it][jt][1];
+  const auto wg = ((ri | rj) & RI_PHOS) ? P->phosph_gamma : P->water_gamma;
+  const double g00 = wg[0][it][jt][0], g01 = wg[0][it][jt][1];
+  const double g10 = wg[1][it][jt][0], g11 = wg[1][it][jt][1];
   const double *si = site(V, i, ri), *sj = site(V, j, rj);
   // the per-residue scalars the pair can use, read with the positions and
   // before any branch (one LDS round trip; hxi2 indices clamped, their
@@ -2308,8 +2316,8 @@ AW_DEV void force_pair(const View<T> &V, int i, int j, EAcc &E, bool queued)
       WellP wp = w == 0 ? w0 : (w == 1 ? w1 : well_p(P, w));
       if (!wp.flag) continue;
       if ((!same || dres >= cc) && rsq > wp.min2 && rsq < wp.max2) {
-        double g0 = w == 0 ? g00 : (w == 1 ? g10 : P->water_gamma[w][it][jt][0]);
-        double g1 = w == 0 ? g01 : (w == 1 ? g11 : P->water_gamma[w][it][jt][1]), sg;
+        double g0 = w == 0 ? g00 : (w == 1 ? g10 : wg[w][it][jt][0]);
+        double g1 = w == 0 ? g01 : (w == 1 ? g11 : wg[w][it][jt][1]), sg;
         if (fabs(g0 - g1) < AW_DELTA) sg = 0.5 * (g0 + g1);
         else sg = g0 + (g1 - g0) * swi * swj;
         T th = th0, prd = prd0;

@@ -20,6 +20,7 @@ enum {
 #define RI_LAST 64
 #define RI_GLY 128
 #define RI_PRO 256
+#define RI_PHOS 512   // phosphorylated ([Phosphorylation], fb.cpp:643-682)
 #define RI_CHAIN_SHIFT 16
 
 // Force-field parameters after the constructor's parsing and epsilon
@@ -74,6 +75,30 @@ struct FFParams {
   // harmonic bonds of the deck
   int bond_on;
   double bond_k, bond_r0[5];
+  // coeff-toggled terms (fb.cpp:297-358, 412-421, 466-478), strengths
+  // after the epsilon scaling of fb.cpp:502-516 (AMH-Go, contact
+  // restraints); the others carry epsilon where they are evaluated
+  int amhgo_flag, cr_flag, vfm_flag, ssb_flag, memb_flag, phosph_flag, dh_flag, xt_on;
+  double k_amh_go, amh_go_p, amh_go_rc;
+  int amh_go_censor, pad_go;
+  double k_cont_rest, cr_sigma, cr_sigma_sq_inv, cr_dr_cutoff, cr_glob_cutoff_sq;
+  double k_vfm, vfm_sigma, vfm_sigma_sq;
+  double k_ssb1, ssb_rmin1, ssb_rmax1, k_ssb2, ssb_rmin2, ssb_rmax2, ssb_kappa, ssb_rshift[20];
+  int ssb_ij_sep, ssb_rad_cor;
+  double k_memb, memb_kbin, memb_xo[3], memb_len, memb_rho0_max, memb_rho0_distor, g_memb[3][4];
+  int memb_pore_type, n_phosph;
+  double k_hypercharge;
+  int phosph_res[20];
+  double dh_k[3];            // k++, k--, k+- (fb.cpp:468)
+  double dh_kscreen, dh_length;
+  int dh_min_sep, pad_dh;
+  // the hypercharged water table of phosphorylated pairs (fb.cpp:643-669)
+  double phosph_gamma[5][20][20][2];
+  // terms whose pairs come from FixBackbone's neighbour list with no cutoff
+  // of their own (Debye-Hueckel, the barrier's CA-CA pairs, AMH-Go) count
+  // pairs closer than pair_list_cutoff + list_skin (a list LAMMPS has just
+  // built, fb.cpp:1805-1813); list_r2 is that distance squared
+  double list_skin, list_r2;
   // derived cutoffs
   double pair_list_cutoff;   // src/fix_backbone.cpp:1048-1066
   double nl_skin;            // Verlet-list skin of the resident engine (A)
@@ -179,5 +204,28 @@ struct ReplicaDesc {
   const int *tb_item;
   const void *tb_curve;
   int *fault;                  // device error word: bit 0 an FM table distance outside [rmin, rmax]
+  // coeff-toggled terms (null / 0 when off): AMH-Go items of each atom
+  // a = 2 i + (0 CA, 1 CB) at go_start[a] .. go_start[a + 1); contact
+  // restraints, Debye-Hueckel pairs and vector fragment-memory items;
+  // membrane classes per residue
+  int n_cr, n_dh, n_vfm, n_go;
+  const int *go_start;
+  const void *go_item, *cr_item, *dh_item, *vfm_item;
+  const int *zres;
+  double go_norm;
+  // the system's list reach of the list-limited terms (pair_list_cutoff +
+  // list skin, squared) and the contact restraints' global cutoff squared:
+  // both follow the system's own files
+  double list_r2, cr_cut2;
+};
+
+// coeff-toggled term items (FP64: these terms are few per residue)
+struct XtGo {                  // one directed AMH-Go pair a -> b (fb.cpp:3895-3950)
+  int b, pad;                  // atom 2 j + (0 CA, 1 CB)
+  double rn, gamma, s2;        // native distance, gamma, sigma^2 = |i - j|^0.3
+};
+struct XtPair {                // contact restraint (w, r0) / Debye-Hueckel (k q_i q_j, -) / vector FM (gf, -)
+  int i, j;
+  double a, b;
 };
 #define AW_FAULT_TB_RANGE 1

@@ -251,12 +251,12 @@ __global__ void __launch_bounds__(64) k_w_nl_publish(BatchDev B, int s)
   template <class T>                                                                   \
   __global__ void __launch_bounds__(256) NAME(BatchDev B, int s)                       \
   {                                                                                    \
-    __shared__ double ered[N_EACC];                                                    \
+    __shared__ double ered[N_ERED];                                                    \
     const ReplicaDesc &R = B.rep[blockIdx.y];                                          \
     if (!w_active(R, s)) return;                                                       \
     View<T> V = w_view<T>(B, R);                                                       \
     V.ered = ered;                                                                     \
-    if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;                                 \
+    if (threadIdx.x < N_ERED) ered[threadIdx.x] = 0.0;                                 \
     __syncthreads();                                                                   \
     BODY;                                                                              \
     __syncthreads();                                                                   \
@@ -290,7 +290,7 @@ struct FmSinkLds {
 template <class T>
 __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
 {
-  __shared__ double ered[N_EACC];
+  __shared__ double ered[N_ERED];
   __shared__ double fl[3 * W_FM_SPAN * 3];
   __shared__ int lim[2];
   const ReplicaDesc &R = B.rep[blockIdx.y];
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   // the block's residues for the local terms, whose forces reach one
   // residue either side
   int o0 = (int)((long long)n * blockIdx.x / gridDim.x), o1 = (int)((long long)n * (blockIdx.x + 1) / gridDim.x);
-  if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
+  if (threadIdx.x < N_ERED) ered[threadIdx.x] = 0.0;
   if (threadIdx.x == 0) {
     lim[0] = o0 < o1 ? (o0 > 0 ? o0 - 1 : 0) : 1 << 30;
     lim[1] = o0 < o1 ? (o1 < n ? o1 : n - 1) : -1;
@@ -369,6 +369,23 @@ __global__ void __launch_bounds__(256) k_w_B1(BatchDev B, int s)
   energy_add(ered, R.energy);
 }
 
+// the coeff-toggled terms (awsem_extra.h) over each replica's lanes, after
+// the list is published (the barrier sweeps it)
+template <class T>
+__global__ void __launch_bounds__(256) k_w_X(BatchDev B, int s)
+{
+  __shared__ double ered[N_ERED];
+  const ReplicaDesc &R = B.rep[blockIdx.y];
+  if (!w_active(R, s)) return;
+  View<T> V = w_view<T>(B, R);
+  V.ered = ered;
+  if (threadIdx.x < N_ERED) ered[threadIdx.x] = 0.0;
+  __syncthreads();
+  xt_pass(V, R, W_TID, W_NT, s < 0 || s == R.nsteps - 1);
+  __syncthreads();
+  energy_add(ered, R.energy);
+}
+
 // the list scan: densities in place, DSSP / P_AP / excluded-volume pairs
 // into their queues and the density pairs for D and F; without a valid
 // list every pair is visited and evaluated in place
@@ -392,12 +409,12 @@ __device__ __forceinline__ void w_scan(const View<T> &V, bool listless_only)
 template <class T>
 __global__ void __launch_bounds__(256) k_w_B2(BatchDev B, int s, int lo)
 {
-  __shared__ double ered[N_EACC];
+  __shared__ double ered[N_ERED];
   const ReplicaDesc &R = B.rep[blockIdx.y];
   if (!w_active(R, s)) return;
   View<T> V = w_view<T>(B, R);
   V.ered = ered;
-  if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
+  if (threadIdx.x < N_ERED) ered[threadIdx.x] = 0.0;
   __syncthreads();
   w_scan(V, lo != 0);
   __syncthreads();
@@ -456,7 +473,7 @@ template <class T>
 __global__ void __launch_bounds__(W_WIN_NT) k_w_win(BatchDev B, int s, int q)
 {
   extern __shared__ double win[];
-  __shared__ double ered[N_EACC];
+  __shared__ double ered[N_ERED];
   const ReplicaDesc &R = B.rep[blockIdx.y];
   if (!w_active(R, s)) return;
   View<T> V = w_view<T>(B, R);
@@ -466,7 +483,7 @@ __global__ void __launch_bounds__(W_WIN_NT) k_w_win(BatchDev B, int s, int q)
   if (q == 2 && !(V.wq && V.nl_state[7] > 0)) return;
   int fs = w_win_fs(n, q), wn = w_win_doubles(n, q);
   for (int k = threadIdx.x; k < wn; k += blockDim.x) win[k] = 0.0;
-  if (threadIdx.x < N_EACC) ered[threadIdx.x] = 0.0;
+  if (threadIdx.x < N_ERED) ered[threadIdx.x] = 0.0;
   __syncthreads();
   V.ered = ered;
   AW_G double *g = (AW_G double *)V.f();  // the replica's force array [3][n][3]
@@ -562,6 +579,7 @@ __global__ void __launch_bounds__(256) k_w_int2(BatchDev B, LgParams lg, int s)
 struct WWin {
   int wg;
   size_t lds[4];
+  int xt;  // the coeff-toggled terms' pass (k_w_X)
 };
 
 template <class T>
@@ -576,6 +594,7 @@ static void w_eval(const BatchDev &B, dim3 g, hipStream_t st, int s, const WWin 
   hipLaunchKernelGGL(k_w_cell_fill<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_cell_pairs<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_nl_publish<T>, g1, dim3(64), 0, st, B, s);
+  if (w.xt) hipLaunchKernelGGL(k_w_X<T>, g, dim3(256), 0, st, B, s);
   hipLaunchKernelGGL(k_w_B1<T>, g, dim3(256), 0, st, B, s);
   dim3 gw(w.wg, g.y);
   hipLaunchKernelGGL(k_w_B2<T>, g, dim3(256), 0, st, B, s, w.lds[2] ? 1 : 0);
@@ -614,6 +633,7 @@ static int launch_wide_t(const AwBatchPtrs &p, const AwRun &run, int max_steps, 
   // window exceeds the LDS take the direct passes (AWSEM_WIDE_WINDOW=0:
   // always)
   WWin w;
+  w.xt = p.extras;
   w.wg = (256 + p.nrep - 1) / p.nrep;
   w.wg = w.wg < 1 ? 1 : (w.wg > 16 ? 16 : w.wg);
   bool win_on = true;

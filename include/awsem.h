@@ -160,6 +160,15 @@ int awsem_set_vexcluded(awsem_ctx *c, int enable, int p, const double *lambda,
  * CA-Hbeta; CoordinatesToWorkLammpsDataFile.py:289-294); enable=0 removes it. */
 int awsem_set_bonds(awsem_ctx *c, int enable, double k, const double *r0_5);
 
+/* Neighbour-list skin of the terms that take their pairs from FixBackbone's
+ * neighbour list with no cutoff of their own: Debye-Hueckel and the
+ * solvent barrier (CA-CA pairs of the list, fb.cpp:8113-8117) and AMH-Go
+ * (atom pairs of the full list, fb.cpp:3880-3895).  They count pairs closer
+ * than pair_list_cutoff + skin, which is what a list LAMMPS has just built
+ * with `neighbor skin bin` holds (the list request, fb.cpp:1805-1813).
+ * Default 0 (a list rebuilt every step).  skin < 0: AWSEM_E_ARG. */
+int awsem_set_list_skin(awsem_ctx *c, double skin);
+
 /* Restrict accumulation to energy slots in mask (bit k = slot k); the
  * coeff file still decides which terms exist.  Test/debug aid. */
 int awsem_set_term_mask(awsem_ctx *c, uint32_t mask);

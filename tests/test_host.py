@@ -54,12 +54,12 @@ def test_error_codes(tmp_path):
     with pytest.raises(A.AwsemError) as e:
         A.FixBackbone(str(tmp_path / "nope.data"), p["seq"], p["workdir"])
     assert e.value.code == -2
-    txt = open(p["coeff"]).read().replace("[Solvent_Barrier]-", "[Solvent_Barrier]")
-    bad = tmp_path / "coeff_ssb"
+    txt = open(p["coeff"]).read().replace("[Tertiary_Frustratometer]-", "[Tertiary_Frustratometer]")
+    bad = tmp_path / "coeff_tf"
     bad.write_text(txt)
     with pytest.raises(A.AwsemError) as e:
         A.FixBackbone(str(bad), p["seq"], p["workdir"])
-    assert e.value.code == -4 and "Solvent_Barrier" in str(e.value)
+    assert e.value.code == -4 and "Tertiary_Frustratometer" in str(e.value)
     # gamma.dat truncated
     d2 = tmp_path / "w2"
     d2.mkdir()
@@ -233,3 +233,54 @@ def test_periodic_box_needs_image_flags():
     with pytest.raises(A.AwsemError) as e:
         fix.pre_force(x, np.zeros_like(x))
     assert e.value.code == -4 and "image" in str(e.value)
+
+
+def test_extra_term_errors(tmp_path):
+    """The coeff-toggled terms' side files: the reference's messages for a
+    missing file (AWSEM_E_FILE) and for malformed content (AWSEM_E_FORMAT),
+    and the same pair_list_cutoff as the oracle (fb.cpp:1048-1066 with the
+    contact-restraint, Debye-Hueckel and barrier shares)."""
+    import shutil
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import oracle
+    from awsemmd_amd import workload as W
+    A = _A()
+    s, p = make_case([30, 20], 3, compact=True)
+    W.write_extras(s, p["workdir"], seed=3)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"])
+    o = oracle.Oracle(p["coeff"], p["seq"], p["workdir"])
+    assert A.lib().awsem_pair_list_cutoff(fix.h) == o.pair_list_cutoff
+    fix.set_list_skin(2.5)
+    with pytest.raises(A.AwsemError) as e:
+        fix.set_list_skin(-1.0)
+    assert e.value.code == -1
+
+    def variant(name, edit):
+        d = tmp_path / name
+        shutil.copytree(p["workdir"], d)
+        edit(d)
+        with pytest.raises(A.AwsemError) as e:
+            A.FixBackbone(str(d / "fix_backbone_coeff.data"), p["seq"], str(d))
+        return e.value
+
+    err = variant("nocharge", lambda d: os.remove(d / "charge_on_residues.dat"))
+    assert err.code == -2 and "charge_on_residues.dat" in str(err)
+    err = variant("nozim", lambda d: os.remove(d / "zim"))
+    assert err.code == -2 and "zim" in str(err)
+    err = variant("zim0", lambda d: (d / "zim").write_text("0\n" * 50))
+    assert err.code == -3
+    err = variant("zimshort", lambda d: (d / "zim").write_text("1\n2\n"))
+    assert err.code == -3 and "Membrane potential parameter file format error" in str(err)
+    err = variant("cr", lambda d: (d / "contact.list").write_text("1 99 1.0 5.0\n"))
+    assert err.code == -3 and "Indices out of range in Contact Restraints potential file" in str(err)
+    err = variant("nogo", lambda d: os.remove(d / "amh-go.gro"))
+    assert err.code == -2 and "Cannot read file amh-go.gro" in str(err)
+    err = variant("nogamma", lambda d: os.remove(d / "amh-go.gamma"))
+    assert err.code == -2 and "Cannot read file amh-go.gamma" in str(err)
+
+    def phos_first(d):
+        txt = open(d / "fix_backbone_coeff.data").read()
+        (d / "fix_backbone_coeff.data").write_text("[Phosphorylation]\n1.2\n1\n3\n\n" + txt)
+    err = variant("phos", phos_first)
+    assert err.code == -3 and "phosphorylation without water" in str(err)
