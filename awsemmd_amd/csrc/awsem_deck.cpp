@@ -147,6 +147,7 @@ struct awsem_deck {
 
   // settings
   double dt = 1.0;                 // fs (LAMMPS real units default)
+  double neigh_skin = 2.0;         // A: `neighbor skin style` (LAMMPS real units default)
   char boundary[3] = {'p', 'p', 'p'};
   bool units_real = false, style_awsemmd = false, bond_harmonic = false, special_fene = false;
   bool pair_on = false;
@@ -531,8 +532,14 @@ struct awsem_deck {
         if (b[0] != 'p' && b[0] != 's' && b[0] != 'f' && b[0] != 'm') fail(AWSEM_E_FORMAT, "Illegal boundary command" + here());
         boundary[d] = b[0];
       }
-    } else if (c == "neighbor" || c == "neigh_modify" || c == "atom_modify" || c == "processors" ||
-               c == "newton" || c == "comm_modify") {
+    } else if (c == "neighbor") {
+      // the engine keeps its own lists; the skin sets how far the terms
+      // that take their pairs from FixBackbone's list reach (a list just
+      // built holds pair_list_cutoff + skin)
+      need(a, 3, c);
+      neigh_skin = to_d(a[1].s, c);
+      if (neigh_skin < 0.0) fail(AWSEM_E_FORMAT, "Illegal neighbor command" + here());
+    } else if (c == "neigh_modify" || c == "atom_modify" || c == "processors" || c == "newton" || c == "comm_modify") {
       // neighbour lists, sorting and decomposition are the engine's own
     } else if (c == "special_bonds") {
       // the excluded-volume pairs leave out exactly the bonded (1-2) pairs
@@ -990,6 +997,7 @@ struct awsem_deck {
       if (ctx) { awsem_destroy(ctx); ctx = nullptr; }
       fail(rc, "fix backbone: " + (m.empty() ? std::string("context creation failed") : m) + here());
     }
+    awsem_set_list_skin(ctx, neigh_skin);
     // Construct_Computational_Arrays: group codes per atom
     const std::vector<char> &gca = groups[fb_group], &gcb = groups[fb_beta], &go = groups[fb_oxy];
     size_t na = atoms.size();
