@@ -46,7 +46,9 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_batch_destroy", "awsem_compute_res_noalloc", "awsem_batch_lists", "awsem_set_box",
            "awsem_compute_image", "awsem_energy_log_open", "awsem_energy_log_write", "awsem_energy_log_close",
            "awsem_deck_open", "awsem_deck_run", "awsem_deck_natoms", "awsem_deck_state",
-           "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy"]
+           "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy",
+           "awsem_q_wolynes_create", "awsem_q_onuchic_create", "awsem_q_compute", "awsem_batch_q",
+           "awsem_q_last_error", "awsem_q_destroy"]
 
 
 class AwsemError(RuntimeError):
@@ -142,6 +144,12 @@ def lib():
         L.awsem_deck_variable.argtypes = [vp, C.c_char_p, dp]
         L.awsem_deck_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.awsem_deck_destroy.argtypes = [vp]
+        L.awsem_q_wolynes_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, C.POINTER(vp)]
+        L.awsem_q_onuchic_create.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_double, C.c_double, C.POINTER(vp)]
+        L.awsem_q_compute.argtypes = [vp, dp, dp]
+        L.awsem_batch_q.argtypes = [vp, vp, C.c_int, C.c_int, dp]
+        L.awsem_q_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.awsem_q_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -609,6 +617,74 @@ class Deck:
         out = C.c_double()
         self._check(lib().awsem_deck_variable(self.h, formula.encode(), C.byref(out)))
         return out.value
+
+
+Q_KINDS = {"wolynes": 0, "cutoff": 1, "cutoff/gauss": 2, "shadow": 3, "shadow/gauss": 4}
+
+
+class OrderParameter:
+    """compute qwolynes / compute qonuchic of n CAs on the device
+    (src/compute_q_wolynes.cpp, src/compute_q_onuchic.cpp).
+
+    OrderParameter.wolynes("rnative.dat", n, sep=2, sigma_exp=0.15)
+    OrderParameter.onuchic("cutoff", "native.dat", n, r_contact=8.0, factor=1.2)
+    OrderParameter.onuchic("shadow/gauss", "contacts.map", n)"""
+
+    def __init__(self, h):
+        self.h = h
+
+    @classmethod
+    def wolynes(cls, path: str, n: int, sep: int = 2, sigma_exp: float = 0.15):
+        h = C.c_void_p()
+        rc = lib().awsem_q_wolynes_create(path.encode(), n, sep, sigma_exp, C.byref(h))
+        q = cls(h)
+        if rc != 0:
+            raise AwsemError(rc, q.last_error())
+        return q
+
+    @classmethod
+    def onuchic(cls, kind: str, path: str, n: int, r_contact: float = 0.0, factor: float = 1.0):
+        h = C.c_void_p()
+        rc = lib().awsem_q_onuchic_create(Q_KINDS[kind], path.encode(), n, r_contact, factor, C.byref(h))
+        q = cls(h)
+        if rc != 0:
+            raise AwsemError(rc, q.last_error())
+        return q
+
+    def last_error(self) -> str:
+        buf = C.create_string_buffer(512)
+        if self.h.value:
+            lib().awsem_q_last_error(self.h, buf, 512)
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AwsemError(rc, self.last_error())
+
+    def compute(self, xca: np.ndarray) -> float:
+        x = np.ascontiguousarray(xca, dtype=np.float64)
+        out = np.zeros(1)
+        self._check(lib().awsem_q_compute(self.h, _dp(x), _dp(out)))
+        return float(out[0])
+
+    def batch(self, b: "Batch", first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        count = b.nrep - first if count is None else count
+        out = np.zeros(count)
+        rc = lib().awsem_batch_q(b.h, self.h, first, count, _dp(out))
+        if rc != 0:
+            raise AwsemError(rc, b.last_error())
+        return out
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().awsem_q_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def pack_states(states: Sequence[np.ndarray]) -> np.ndarray:

@@ -1025,6 +1025,24 @@ int awsem_batch_set_state(awsem_batch *b, const double *x, const double *v)
   return AWSEM_OK;
 }
 
+// compute qwolynes / qonuchic of replicas [first, first + count) at their
+// current positions on the device (no copy of the state)
+int awsem_batch_q(awsem_batch *b, awsem_q *q, int first, int count, double *values)
+{
+  if (!b || !b->dev || !q || !values || first < 0 || count <= 0 || first + count > b->dev->nrep) return AWSEM_E_ARG;
+  DevBatch &D = *b->dev;
+  std::vector<long long> off(count);
+  for (int k = 0; k < count; ++k) {
+    int r = first + k;
+    if (D.reps[r].n != aw_q_residues(q))
+      return bfail(b, AwStatus::err(AWSEM_E_ARG, "order parameter: replica residue count differs from its native"));
+    off[k] = 9LL * D.res_off[r];
+  }
+  int rc = aw_q_eval(q, D.d_x, off.data(), count, values, D.stream);
+  if (rc) return bfail(b, AwStatus::err(rc, "order parameter evaluation failed"));
+  return AWSEM_OK;
+}
+
 int awsem_batch_get_state(awsem_batch *b, double *x, double *v, double *f, double *energy)
 {
   if (!b || !b->dev) return AWSEM_E_ARG;

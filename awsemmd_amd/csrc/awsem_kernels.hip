@@ -323,3 +323,44 @@ int aw_launch_list(const AwBatchPtrs &p, int r, int kind, int arg, long long *ou
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+// ------------------------------------------------------------------------
+// order parameters (compute qwolynes, compute qonuchic): one workgroup per
+// trace sums its pair items in a fixed order (lane-strided, then a tree in
+// LDS), so a value does not depend on what else is launched
+
+__global__ void __launch_bounds__(1024) k_q(const QItem *__restrict__ items, int nitems, int gauss,
+                                           const double *__restrict__ xbase, const long long *__restrict__ xoff,
+                                           double norm, double *__restrict__ out)
+{
+  __shared__ double part[1024];
+  const double *x = xbase + xoff[blockIdx.x];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nitems; k += blockDim.x) {
+    QItem it = items[k];
+    const double *a = x + 3 * it.i, *b = x + 3 * it.j;
+    double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    if (gauss) {
+      double d = sqrt(r2) - it.a;
+      s += exp(-d * d * it.b);
+    } else if (r2 < it.a) {
+      s += 1.0;
+    }
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0] * norm;
+}
+
+int aw_launch_q(const void *items, int nitems, int gauss, const double *xbase, const long long *xoff, int count,
+                double norm, double *out, hipStream_t st)
+{
+  if (count <= 0) return 0;
+  hipLaunchKernelGGL(k_q, dim3(count), dim3(1024), 0, st, (const QItem *)items, nitems, gauss, xbase, xoff, norm, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -56,3 +56,14 @@ struct awsem_ctx;
 size_t aw_noalloc_doubles(int n);
 int aw_noalloc_check(awsem_ctx *c, const double *xres, const double *fres, size_t cap_doubles);
 int aw_compute_res_scratch(awsem_ctx *c, const double *xres, double *fres, double *energy, double *dscr);
+
+// order parameters: Q of `count` CA traces, trace r at xbase + xoff[r]
+// ([n][3] doubles); items: QItem[nitems]; gauss: 1 for the Gaussian kinds,
+// 0 for the contact counts; out[r] = norm * sum (device pointers)
+int aw_launch_q(const void *items, int nitems, int gauss, const double *xbase, const long long *xoff, int count,
+                double norm, double *out, hipStream_t st);
+// the order-parameter object (awsem_q.cpp): Q of `count` CA traces at
+// d_xbase + xoff[r] (host offsets, doubles) into values (host), on stream st
+struct awsem_q;
+int aw_q_eval(awsem_q *q, const double *d_xbase, const long long *xoff, int count, double *values, hipStream_t st);
+int aw_q_residues(const awsem_q *q);

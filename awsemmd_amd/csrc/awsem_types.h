@@ -219,6 +219,13 @@ struct ReplicaDesc {
   double list_r2, cr_cut2;
 };
 
+// order-parameter pair (compute qwolynes / qonuchic): gauss kinds add
+// exp(-(r - a)^2 b), cutoff kinds count r^2 < a
+struct QItem {
+  int i, j;
+  double a, b;
+};
+
 // coeff-toggled term items (FP64: these terms are few per residue)
 struct XtGo {                  // one directed AMH-Go pair a -> b (fb.cpp:3895-3950)
   int b, pad;                  // atom 2 j + (0 CA, 1 CB)

@@ -378,6 +378,37 @@ int awsem_deck_variable(awsem_deck *d, const char *formula, double *value);
 int awsem_deck_last_error(const awsem_deck *d, char *buf, size_t n);
 int awsem_deck_destroy(awsem_deck *d);
 
+/* ---- order parameters (compute qwolynes, compute qonuchic) ---------------
+ * Replace `compute ID alpha_carbons qwolynes rnative.dat sep exp`
+ * (src/compute_q_wolynes.cpp:51-88, compute_scalar 171-236) and
+ * `compute ID alpha_carbons qonuchic {cutoff rc native.dat factor |
+ * cutoff/gauss rc native.dat | shadow map factor | shadow/gauss map}`
+ * (src/compute_q_onuchic.cpp:35-93, 150-317) for the n CAs of a system in
+ * residue order.  Evaluated on the device, for one trace or for replicas of
+ * a batch at their current positions.  On an error *out still holds the
+ * object, for awsem_q_last_error; destroy it. */
+#define AWSEM_Q_WOLYNES 0
+#define AWSEM_Q_ONUCHIC_CUTOFF 1
+#define AWSEM_Q_ONUCHIC_CUTOFF_GAUSS 2
+#define AWSEM_Q_ONUCHIC_SHADOW 3
+#define AWSEM_Q_ONUCHIC_SHADOW_GAUSS 4
+typedef struct awsem_q awsem_q;
+int awsem_q_wolynes_create(const char *rnative_path, int n, int sep, double sigma_exp, awsem_q **out);
+/* kind AWSEM_Q_ONUCHIC_*; path: native CA coordinates ("n" then n rows
+ * of x y z) for the cutoff kinds, the contact map ("i j rn" rows, 1-based)
+ * for the shadow kinds; r_contact for the cutoff kinds, factor for cutoff
+ * and shadow */
+int awsem_q_onuchic_create(int kind, const char *path, int n, double r_contact, double factor, awsem_q **out);
+/* Q of one CA trace xca[n][3] (the coordinates as the compute reads them:
+ * compute qwolynes uses the stored, wrapped ones, its unwrap being
+ * overwritten at compute_q_wolynes.cpp:198-214) */
+int awsem_q_compute(awsem_q *q, const double *xca, double *value);
+/* Q of replicas [first, first + count) of a batch (each with the native's
+ * residue count) at their current device positions: values[count] */
+int awsem_batch_q(awsem_batch *b, awsem_q *q, int first, int count, double *values);
+int awsem_q_last_error(const awsem_q *q, char *buf, size_t n);
+int awsem_q_destroy(awsem_q *q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2690,3 +2690,78 @@ int or_langevin_run(or_ctx *c, int nsteps, double dt, double t_start, double t_s
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* order parameters: compute qwolynes and compute qonuchic              */
+
+/* ComputeQWolynes::compute_scalar (src/compute_q_wolynes.cpp:171-236):
+ * CA pairs i < j with |i - j| > sep, sigma = (1 + |i - j|)^sexp,
+ * q = sum exp(-(r - rn_ij)^2 / (2 sigma^2)) * 2 / ((n - sep)(n - sep - 1)),
+ * rn row-major n x n (readNativeDistances, 152-167).  The reference's
+ * periodic unwrap is overwritten by the stored coordinate (198-214): the
+ * coordinates are used as given. */
+double or_q_wolynes(int n, const double *xca, const double *rn, int sep, double sexp)
+{
+  double q = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      if (!(abs(j - i) > sep)) continue;
+      double s = pow(1 + abs(i - j), sexp);
+      double dx = xca[3 * i] - xca[3 * j], dy = xca[3 * i + 1] - xca[3 * j + 1], dz = xca[3 * i + 2] - xca[3 * j + 2];
+      double r = sqrt(pow(dx, 2) + pow(dy, 2) + pow(dz, 2));
+      q += exp(-pow(r - rn[(size_t)i * n + j], 2) / (2.0 * pow(s, 2)));
+    }
+  return q * (2.0 / (((double)n - (double)sep) * ((double)n - ((double)sep + 1.0))));
+}
+
+/* ComputeQOnuchic (src/compute_q_onuchic.cpp:150-317).  type 0 cutoff,
+ * 1 cutoff/gauss: native pairs are j - i > 3 with |x_i - x_j|^2 < rc^2 in
+ * the native coordinates (nat, n x 3); type 2 shadow, 3 shadow/gauss: the
+ * contact map's (i, j, rn) entries (0-based; a later entry of a pair
+ * replaces an earlier one, every entry counts in the normalisation).
+ * Cutoff types count pairs with r^2 < (factor rn)^2; the gauss types add
+ * exp(-(r - rn)^2 / (2 (1 + |i - j|)^0.3)).  Pairs are visited i < j (the
+ * atom order of a data file), so a map entry with i > j counts only in the
+ * normalisation.  q /= (number of native pairs or map entries). */
+double or_q_onuchic(int n, const double *xca, int type, const double *nat, double rc, int nmap, const int *mi,
+                    const int *mj, const double *mrn, double factor)
+{
+  unsigned char *isn = (unsigned char *)calloc((size_t)n * n, 1);
+  double *rsq = (double *)calloc((size_t)n * n, sizeof(double));
+  double norm = 0.0;
+  if (type == 0 || type == 1) {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double dx = nat[3 * j] - nat[3 * i], dy = nat[3 * j + 1] - nat[3 * i + 1], dz = nat[3 * j + 2] - nat[3 * i + 2];
+        size_t k = (size_t)i * n + j;
+        rsq[k] = dx * dx + dy * dy + dz * dz;
+        if (rsq[k] < rc * rc && j - i > 3) { isn[k] = 1; norm += 1.0; }
+        if (type == 0 && isn[k]) rsq[k] *= factor * factor;
+      }
+  } else {
+    for (int e = 0; e < nmap; ++e) {
+      size_t k = (size_t)mi[e] * n + mj[e];
+      isn[k] = 1;
+      rsq[k] = mrn[e] * mrn[e];
+      if (type == 2) rsq[k] *= factor * factor;
+      norm += 1.0;
+    }
+  }
+  double q = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      size_t k = (size_t)i * n + j;
+      if (!(isn[k] && abs(j - i) > 3)) continue;
+      double dx = xca[3 * i] - xca[3 * j], dy = xca[3 * i + 1] - xca[3 * j + 1], dz = xca[3 * i + 2] - xca[3 * j + 2];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (type == 0 || type == 2) {
+        if (r2 < rsq[k]) q += 1.0;
+      } else {
+        double s2 = pow(1 + abs(i - j), 2.0 * 0.15);
+        q += exp(-pow(sqrt(r2) - sqrt(rsq[k]), 2) / (2.0 * s2));
+      }
+    }
+  free(isn);
+  free(rsq);
+  return q * (1 / norm);
+}

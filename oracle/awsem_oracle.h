@@ -141,6 +141,15 @@ int or_langevin_run(or_ctx *c, int nsteps, double dt, double t_start,
                     int replica, long long step0, double *x, double *v,
                     double *f, double *energy);
 
+/* Order parameters of a CA trace (xca: n x 3): compute qwolynes
+ * (src/compute_q_wolynes.cpp) against n x n native distances, and compute
+ * qonuchic (src/compute_q_onuchic.cpp): type 0 cutoff / 1 cutoff/gauss
+ * from native coordinates nat (n x 3) and contact radius rc, type 2 shadow
+ * / 3 shadow/gauss from a contact map (0-based i, j, native distance). */
+double or_q_wolynes(int n, const double *xca, const double *rn, int sep, double sexp);
+double or_q_onuchic(int n, const double *xca, int type, const double *nat, double rc, int nmap, const int *mi,
+                    const int *mj, const double *mrn, double factor);
+
 #ifdef __cplusplus
 }
 #endif

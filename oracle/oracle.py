@@ -66,6 +66,11 @@ def lib():
         L.or_set_vexcluded.argtypes = [C.c_void_p, C.c_int, dp, dp, dp]
         L.or_set_bonds.argtypes = [C.c_void_p, C.c_double, dp]
         L.or_set_list_skin.argtypes = [C.c_void_p, C.c_double]
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.or_q_wolynes.argtypes = [C.c_int, dp, dp, C.c_int, C.c_double]
+        L.or_q_wolynes.restype = C.c_double
+        L.or_q_onuchic.argtypes = [C.c_int, dp, C.c_int, dp, C.c_double, C.c_int, ip, ip, dp, C.c_double]
+        L.or_q_onuchic.restype = C.c_double
         L.or_langevin_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_ulonglong, C.c_int, C.c_longlong, dp, dp, dp, dp]
         L.or_langevin_run.restype = C.c_int
@@ -214,3 +219,24 @@ class Oracle:
         if rc:
             raise RuntimeError(lib().or_last_error(self.h).decode())
         return x, v, f, e
+
+
+def q_wolynes(xca, rnative, sep=2, sigma_exp=0.15):
+    """compute qwolynes of a CA trace (n x 3) against n x n native distances."""
+    x = np.ascontiguousarray(xca, dtype=np.float64)
+    rn = np.ascontiguousarray(rnative, dtype=np.float64)
+    return lib().or_q_wolynes(len(x), x.reshape(-1), rn.reshape(-1), int(sep), float(sigma_exp))
+
+
+def q_onuchic(xca, kind, native_xyz=None, r_contact=0.0, contacts=None, factor=1.0):
+    """compute qonuchic; kind 'cutoff', 'cutoff/gauss' (native_xyz, r_contact)
+    or 'shadow', 'shadow/gauss' (contacts: (i, j, rn) 0-based)."""
+    t = {"cutoff": 0, "cutoff/gauss": 1, "shadow": 2, "shadow/gauss": 3}[kind]
+    x = np.ascontiguousarray(xca, dtype=np.float64)
+    nat = np.ascontiguousarray(native_xyz if native_xyz is not None else np.zeros((1, 3)), dtype=np.float64)
+    c = contacts if contacts is not None else []
+    mi = np.ascontiguousarray([a for a, _, _ in c] or [0], dtype=np.int32)
+    mj = np.ascontiguousarray([b for _, b, _ in c] or [0], dtype=np.int32)
+    mr = np.ascontiguousarray([r for _, _, r in c] or [0.0], dtype=np.float64)
+    return lib().or_q_onuchic(len(x), x.reshape(-1), t, nat.reshape(-1), float(r_contact), len(c), mi, mj, mr,
+                              float(factor))
