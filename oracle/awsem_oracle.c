@@ -136,6 +136,14 @@ struct or_ctx {
   double k_PlusPlus, k_MinusMinus, k_PlusMinus, k_screening, screening_length;
   int debye_huckel_min_sep;
   double *charge;
+  /* [Tertiary_Frustratometer] (fb.cpp:385-397): contact cutoff, decoys per
+     contact, output interval, mode 0 configurational / 1 mutational / 2
+     singleresidue; the configurational decoy statistics, computed once
+     (fb.cpp:5099, 5342); the rand() stream the decoys draw from */
+  int tert_frust_flag, tf_ndecoys, tf_freq, tf_mode, tf_done;
+  double tf_cutoff, tf_stats[2];
+  uint32_t rng[34];
+  int rng_k;
   /* system */
   int n;
   char *se;
@@ -837,7 +845,22 @@ static int parse_coeff(or_ctx *c, const char *path, char *err, int errlen)
       c->k_screening = tk_double(&in);
       c->screening_length = tk_double(&in);
       c->debye_huckel_min_sep = tk_int(&in);
-    } else if (strcmp(sec, "[Fragment_Frustratometer]") == 0 || strcmp(sec, "[Tertiary_Frustratometer]") == 0 ||
+    } else if (strcmp(sec, "[Tertiary_Frustratometer]") == 0) { /* fb.cpp:385-397 */
+      c->tert_frust_flag = 1;
+      c->tf_cutoff = tk_double(&in);
+      c->tf_ndecoys = tk_int(&in);
+      c->tf_freq = tk_int(&in);
+      tk_str(&in, sbuf, sizeof sbuf);
+      c->tf_done = 0;
+      if (strcmp(sbuf, "configurational") == 0) c->tf_mode = 0;
+      else if (strcmp(sbuf, "mutational") == 0) c->tf_mode = 1;
+      else if (strcmp(sbuf, "singleresidue") == 0) c->tf_mode = 2;
+      else {
+        free(text);
+        set_err(err, errlen, "Only \"configurational\", \"mutational\", \"singleresidue\" are acceptable modes for the Tertiary_Frustratometer.");
+        return -1;
+      }
+    } else if (strcmp(sec, "[Fragment_Frustratometer]") == 0 ||
                strcmp(sec, "[Nmer_Frustratometer]") == 0 ||
                strcmp(sec, "[Amylometer]") == 0 || strcmp(sec, "[Selection_Temperature]") == 0 ||
                strcmp(sec, "[Monte_Carlo_Seq_Opt]") == 0 || strcmp(sec, "[Optimization]") == 0 ||
@@ -876,6 +899,34 @@ static int read_table_file(const char *dir, const char *name, double *dst, int c
 }
 
 static int fm_table_build(or_ctx *c, char *err, int errlen);
+
+/* The C library's rand() as glibc implements it (the additive feedback
+ * generator of its random_r, TYPE_3: degree 31, separation 3): srand(seed)
+ * fills r[0] = seed, r[i] = 16807 r[i-1] mod (2^31 - 1) for i < 31, copies
+ * r[31..33] = r[0..2], then r[i] = r[i-31] + r[i-3] (mod 2^32), and the
+ * k-th rand() is r[344 + k] >> 1.  The reference draws its decoys with
+ * rand() % n (fb.cpp:5550-5556) and never seeds it. */
+void or_rng_seed(or_ctx *c, unsigned seed)
+{
+  int64_t r[34];
+  r[0] = seed ? seed : 1;
+  for (int i = 1; i < 31; ++i) r[i] = (16807 * r[i - 1]) % 2147483647;
+  for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+  for (int i = 0; i < 34; ++i) c->rng[i] = (uint32_t)r[i];
+  c->rng_k = 34;
+  for (int i = 34; i < 344; ++i) or_rng_next(c);
+}
+
+int or_rng_next(or_ctx *c)
+{
+  /* ring of the last 34 values: r[k] at rng[k % 34] */
+  int k = c->rng_k;
+  uint32_t v = c->rng[(k - 31) % 34] + c->rng[(k - 3) % 34];
+  c->rng[k % 34] = v;
+  c->rng_k = k + 1;
+  if (c->rng_k >= 34 * 1000000) c->rng_k = c->rng_k % 34 + 34;
+  return (int)(v >> 1);
+}
 
 /* read_contact_restraints_file, fb.cpp:1106-1170: records "i1 i2 w r0"
  * (1-based residues) until one fails to read; w *= k_cont_rest (already
@@ -1054,6 +1105,7 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
   c->ap = 0.4436538; c->bp = 0.2352006; c->cp = 0.3211455;
   c->ah = 0.8409657; c->bh = 0.8929599; c->ch = -0.7338894;
   c->fm_sigma_exp = 0.15;
+  or_rng_seed(c, 1);  /* rand() of a process that never called srand */
   int rc = parse_coeff(c, coeff_path, err, errlen);
   if (rc) { or_destroy(c); return rc; }
 
@@ -2764,4 +2816,192 @@ double or_q_onuchic(int n, const double *xca, int type, const double *nat, doubl
   free(isn);
   free(rsq);
   return q * (1 / norm);
+}
+
+/* ------------------------------------------------------------------ */
+/* the tertiary frustratometer (fb.cpp:5057-5412, 5444-5600)           */
+
+/* compute_water_energy (fb.cpp:5444-5476): wells 0 (direct, the mean of
+ * its two gammas) and 1 (mediated, protein/water gammas by the two
+ * residues' sigma) at distance r, densities rho_i, rho_j */
+static double tf_water(const or_ctx *c, double r, int i, int j, int it, int jt, double ri, double rj)
+{
+  double g0d = water_g(c, i, j, 0, it, jt, 0), g1d = water_g(c, i, j, 0, it, jt, 1);
+  double gpm = water_g(c, i, j, 1, it, jt, 0), gwm = water_g(c, i, j, 1, it, jt, 1);
+  double sw = 0.25 * (1.0 - tanh(c->water_kappa_sigma * (ri - c->treshold))) * (1.0 - tanh(c->water_kappa_sigma * (rj - c->treshold)));
+  double sp = 1.0 - sw;
+  double sgd = (g0d + g1d) / 2, sgm = sp * gpm + sw * gwm;
+  double k = c->water_kappa;
+  double td = 0.25 * (1.0 + tanh(k * (r - c->well_r_min[0]))) * (1.0 + tanh(k * (c->well_r_max[0] - r)));
+  double tm = 0.25 * (1.0 + tanh(k * (r - c->well_r_min[1]))) * (1.0 + tanh(k * (c->well_r_max[1] - r)));
+  return -(sgd * td + sgm * tm);
+}
+
+/* compute_burial_energy (fb.cpp:5478-5500) */
+static double tf_burial(const or_ctx *c, int i, int it, double rho)
+{
+  double e = 0.0, kb = c->burial_kappa;
+  for (int b = 0; b < 3; ++b)
+    e += -0.5 * c->k_burial * burial_g(c, i, it, b) * (tanh(kb * (rho - c->burial_ro_min[b])) + tanh(kb * (c->burial_ro_max[b] - rho)));
+  return e;
+}
+
+/* compute_electrostatic_energy (fb.cpp:5502-5548): charges by residue type
+ * (R, K +1; D, E -1), nothing below the minimum separation of i, j */
+static double tf_elec(const or_ctx *c, double r, int i, int j, int it, int jt)
+{
+  static const char one[20] = {'A', 'R', 'N', 'D', 'C', 'Q', 'E', 'G', 'H', 'I', 'L', 'K', 'M', 'F', 'P', 'S', 'T', 'W', 'Y', 'V'};
+  if (abs(i - j) < c->debye_huckel_min_sep) return 0.0;
+  double qi, qj;
+  if (one[it] == 'R' || one[it] == 'K') qi = 1.0;
+  else if (one[it] == 'D' || one[it] == 'E') qi = -1.0;
+  else return 0.0;
+  if (one[jt] == 'R' || one[jt] == 'K') qj = 1.0;
+  else if (one[jt] == 'D' || one[jt] == 'E') qj = -1.0;
+  else return 0.0;
+  double k = (qi > 0 && qj > 0) ? c->k_PlusPlus : ((qi < 0 && qj < 0) ? c->k_MinusMinus : c->k_PlusMinus);
+  return c->epsilon * (k * qi * qj / r) * exp(-c->k_screening * r / c->screening_length);
+}
+
+static double tf_dist(const or_ctx *c, int i, int j)
+{
+  return sqrt(rsq_of(site(c, i), site(c, j)));
+}
+
+/* mean and standard deviation (population) of the decoy energies
+ * (fb.cpp:5413-5442) */
+static void tf_stats(const double *e, int m, double *st)
+{
+  double mean = 0.0, var = 0.0;
+  for (int k = 0; k < m; ++k) mean += e[k];
+  mean /= (double)m;
+  for (int k = 0; k < m; ++k) var += (e[k] - mean) * (e[k] - mean);
+  st[0] = mean;
+  st[1] = sqrt(var / (double)m);
+}
+
+/* the mutational environment of residue a (its contacts k other than x,
+ * y) with a's type at and residue index ar (fb.cpp:5211-5239) */
+static double tf_env(const or_ctx *c, int a, int ar, int at, double rho_a, int x, int y, const double *rho)
+{
+  double e = 0.0;
+  for (int k = 0; k < c->n; ++k) {
+    if (k == x || k == y) continue;
+    double rak = tf_dist(c, a, k);
+    int kt = restype(c->se[k]);
+    if (rak < c->tf_cutoff) e += tf_water(c, rak, ar, k, at, kt, rho_a, rho[k]);
+    if (c->huckel_flag) e += tf_elec(c, rak, ar, k, at, kt);
+  }
+  return e;
+}
+
+/* one call of the frustratometer at the coordinates set by or_compute
+ * (densities from the well-0 pairs as cWell::compute_ro, sml.h:630-650).
+ * Rows of 11 doubles: i, j, i_chain, j_chain, r_ij, rho_i, rho_j, native,
+ * decoy mean, decoy std, frustration index (1-based residue and chain
+ * numbers; j = 0 and j_chain = 0 in singleresidue mode).  Returns the rows
+ * written, or -needed. */
+int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap)
+{
+  int n = c->n;
+  c->xca = xca; c->xcb = xcb; c->xo = xo;
+  double *rho = (double *)calloc((size_t)n, sizeof(double));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      if (!(c->chain_no[i] != c->chain_no[j] || abs(j - i) > 1)) continue;
+      double r2 = rsq_of(site(c, i), site(c, j));
+      if (r2 < c->w_rmin_theta_sq[0] || r2 > c->w_rmax_theta_sq[0]) continue;
+      double th, prd;
+      wtheta(c, 0, sqrt(r2), &th, &prd);
+      rho[i] += th;
+    }
+  int m = c->tf_ndecoys, nr = 0;
+  double *dec = (double *)calloc((size_t)(m > 0 ? m : 1), sizeof(double));
+  double cut = c->tf_cutoff;
+#define TF_ROW(a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10)                                       \
+  do {                                                                                        \
+    if (nr < cap) {                                                                           \
+      double *q = rows + 11 * (size_t)nr;                                                     \
+      q[0] = a0; q[1] = a1; q[2] = a2; q[3] = a3; q[4] = a4; q[5] = a5; q[6] = a6; q[7] = a7; \
+      q[8] = a8; q[9] = a9; q[10] = a10;                                                      \
+    }                                                                                         \
+    nr++;                                                                                     \
+  } while (0)
+  if (c->tf_mode == 2) {
+    for (int i = 0; i < n; ++i) {
+      int it = restype(c->se[i]);
+      double e = tf_burial(c, i, it, rho[i]);
+      /* compute_singleresidue_native_ixn (fb.cpp:5346-5393) */
+      for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        double r = tf_dist(c, i, j);
+        int jt = restype(c->se[j]);
+        if (r < cut && (abs(i - j) >= c->contact_cutoff || c->chain_no[i] != c->chain_no[j]))
+          e += tf_water(c, r, i, j, it, jt, rho[i], rho[j]);
+        if (c->huckel_flag) e += tf_elec(c, r, i, j, it, jt);
+      }
+      for (int d = 0; d < m; ++d) {
+        int dt = restype(c->se[or_rng_next(c) % n]);
+        double ed = tf_burial(c, i, dt, rho[i]);
+        for (int j = 0; j < n; ++j) {
+          if (j == i) continue;
+          double r = tf_dist(c, i, j);
+          int jt = restype(c->se[j]);
+          if (r < cut && (abs(i - j) >= c->contact_cutoff || c->chain_no[i] != c->chain_no[j]))
+            ed += tf_water(c, r, i, j, dt, jt, rho[i], rho[j]);
+          if (c->huckel_flag) ed += tf_elec(c, r, i, j, dt, jt);
+        }
+        dec[d] = ed;
+      }
+      double st[2];
+      tf_stats(dec, m, st);
+      TF_ROW(i + 1, 0, c->chain_no[i], 0, 0.0, rho[i], 0.0, e, st[0], st[1], (st[0] - e) / st[1]);
+    }
+  } else {
+    for (int i = 0; i < n; ++i) {
+      int it = restype(c->se[i]);
+      for (int j = i + 1; j < n; ++j) {
+        int jt = restype(c->se[j]);
+        double rij = tf_dist(c, i, j);
+        if (!(rij < cut && (abs(i - j) >= c->contact_cutoff || c->chain_no[i] != c->chain_no[j]))) continue;
+        /* compute_native_ixn, fb.cpp:5191-5247 */
+        double e = tf_water(c, rij, i, j, it, jt, rho[i], rho[j]) + tf_burial(c, i, it, rho[i]) + tf_burial(c, j, jt, rho[j]);
+        if (c->huckel_flag) e += tf_elec(c, rij, i, j, it, jt);
+        if (c->tf_mode == 1) e += tf_env(c, i, i, it, rho[i], i, j, rho) + tf_env(c, j, j, jt, rho[j], i, j, rho);
+        /* compute_decoy_ixns, fb.cpp:5249-5344 */
+        if (c->tf_mode == 1 || !c->tf_done) {
+          for (int d = 0; d < m; ++d) {
+            double r = rij, ri = rho[i], rj = rho[j];
+            if (c->tf_mode == 0) {
+              int a = or_rng_next(c) % n, b = or_rng_next(c) % n;
+              r = tf_dist(c, a, b);
+              while (r > cut || a == b) {
+                a = or_rng_next(c) % n;
+                b = or_rng_next(c) % n;
+                r = tf_dist(c, a, b);
+              }
+              a = or_rng_next(c) % n;
+              b = or_rng_next(c) % n;
+              ri = rho[a];
+              rj = rho[b];
+            }
+            int a = or_rng_next(c) % n, b = or_rng_next(c) % n;
+            int at = restype(c->se[a]), bt = restype(c->se[b]);
+            double ed = tf_water(c, r, a, b, at, bt, ri, rj) + tf_burial(c, a, at, ri) + tf_burial(c, b, bt, rj);
+            if (c->huckel_flag) ed += tf_elec(c, r, a, b, at, bt);
+            if (c->tf_mode == 1) ed += tf_env(c, i, a, at, ri, i, j, rho) + tf_env(c, j, b, bt, rj, i, j, rho);
+            dec[d] = ed;
+          }
+          tf_stats(dec, m, c->tf_stats);
+          if (c->tf_mode == 0) c->tf_done = 1;
+        }
+        TF_ROW(i + 1, j + 1, c->chain_no[i], c->chain_no[j], rij, rho[i], rho[j], e, c->tf_stats[0], c->tf_stats[1],
+               (c->tf_stats[0] - e) / c->tf_stats[1]);
+      }
+    }
+  }
+#undef TF_ROW
+  free(rho);
+  free(dec);
+  return nr <= cap ? nr : -nr;
 }

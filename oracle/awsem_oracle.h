@@ -147,6 +147,17 @@ int or_langevin_run(or_ctx *c, int nsteps, double dt, double t_start,
  * from native coordinates nat (n x 3) and contact radius rc, type 2 shadow
  * / 3 shadow/gauss from a contact map (0-based i, j, native distance). */
 double or_q_wolynes(int n, const double *xca, const double *rn, int sep, double sexp);
+
+/* The tertiary frustratometer ([Tertiary_Frustratometer], fb.cpp:5057-5412)
+ * at one coordinate set: rows of 11 doubles (i, j, i_chain, j_chain, r_ij,
+ * rho_i, rho_j, native, decoy mean, decoy std, frustration index; j = 0 in
+ * singleresidue mode); returns the rows or -needed.  The decoys draw from
+ * the context's rand() stream (glibc's, unseeded: or_rng_seed(c, 1) at
+ * creation), which continues across calls, as do the configurational decoy
+ * statistics (computed once, fb.cpp:5099, 5342). */
+int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap);
+void or_rng_seed(or_ctx *c, unsigned seed);
+int or_rng_next(or_ctx *c);
 double or_q_onuchic(int n, const double *xca, int type, const double *nat, double rc, int nmap, const int *mi,
                     const int *mj, const double *mrn, double factor);
 

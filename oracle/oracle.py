@@ -67,6 +67,11 @@ def lib():
         L.or_set_bonds.argtypes = [C.c_void_p, C.c_double, dp]
         L.or_set_list_skin.argtypes = [C.c_void_p, C.c_double]
         ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.or_rng_seed.argtypes = [C.c_void_p, C.c_uint]
+        L.or_rng_next.argtypes = [C.c_void_p]
+        L.or_rng_next.restype = C.c_int
+        L.or_tert_frust.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_int]
+        L.or_tert_frust.restype = C.c_int
         L.or_q_wolynes.argtypes = [C.c_int, dp, dp, C.c_int, C.c_double]
         L.or_q_wolynes.restype = C.c_double
         L.or_q_onuchic.argtypes = [C.c_int, dp, C.c_int, dp, C.c_double, C.c_int, ip, ip, dp, C.c_double]
@@ -144,6 +149,25 @@ class Oracle:
             raise RuntimeError(lib().or_last_error(self.h).decode())
         f[0], f[1], f[2] = fca, fcb, fo
         return f, e
+
+    def rng_seed(self, seed: int):
+        lib().or_rng_seed(self.h, seed)
+
+    def rng_next(self) -> int:
+        return lib().or_rng_next(self.h)
+
+    def tert_frust(self, coords: np.ndarray) -> np.ndarray:
+        """One tertiary-frustratometer call: rows (i, j, i_chain, j_chain,
+        r_ij, rho_i, rho_j, native, decoy mean, decoy std, index)."""
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        cap = 4096
+        while True:
+            out = np.zeros(11 * cap)
+            k = lib().or_tert_frust(self.h, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]),
+                                    np.ascontiguousarray(x[2]), out, cap)
+            if k >= 0:
+                return out[:11 * k].reshape(k, 11)
+            cap = -k + 16
 
     def fm_table_dims(self):
         """(grid points per curve, neighbours per residue) of fragment-memory
