@@ -48,7 +48,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_deck_open", "awsem_deck_run", "awsem_deck_natoms", "awsem_deck_state",
            "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy",
            "awsem_q_wolynes_create", "awsem_q_onuchic_create", "awsem_q_compute", "awsem_batch_q",
-           "awsem_q_last_error", "awsem_q_destroy"]
+           "awsem_q_last_error", "awsem_q_destroy", "awsem_tert_frust", "awsem_tert_frust_write"]
 
 
 class AwsemError(RuntimeError):
@@ -150,6 +150,8 @@ def lib():
         L.awsem_batch_q.argtypes = [vp, vp, C.c_int, C.c_int, dp]
         L.awsem_q_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.awsem_q_destroy.argtypes = [vp]
+        L.awsem_tert_frust.argtypes = [vp, dp, dp, C.c_int, C.POINTER(C.c_int)]
+        L.awsem_tert_frust_write.argtypes = [vp, dp, C.c_int64]
         _lib = L
     return _lib
 
@@ -239,6 +241,20 @@ class FixBackbone:
 
     def set_term_mask(self, mask: int):
         self._check(lib().awsem_set_term_mask(self.h, mask & 0xFFFFFFFF))
+
+    def tert_frust(self, xres: np.ndarray) -> np.ndarray:
+        """The tertiary frustratometer at xres (3, n, 3): rows (i, j, i_chain,
+        j_chain, r_ij, rho_i, rho_j, native, decoy mean, decoy std, index)."""
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        cap = 4 * self.n * self.n + 16
+        out = np.zeros(11 * cap)
+        k = C.c_int()
+        self._check(lib().awsem_tert_frust(self.h, _dp(x), _dp(out), cap, C.byref(k)))
+        return out[:11 * k.value].reshape(k.value, 11)
+
+    def tert_frust_write(self, xres: np.ndarray, step: int):
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        self._check(lib().awsem_tert_frust_write(self.h, _dp(x), step))
 
     def set_list_skin(self, skin: float):
         """Neighbour-list skin of the list-limited terms (Debye-Hueckel, the

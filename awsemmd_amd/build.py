@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libawsem_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["awsem_resident_f32.hip", "awsem_resident_f64.hip", "awsem_wide_f32.hip", "awsem_wide_f64.hip", "awsem_kernels.hip", "awsem_noalloc.hip", "awsem_parse.cpp", "awsem_devsys.cpp", "awsem_api.cpp", "awsem_server.cpp", "awsem_deck.cpp", "awsem_q.cpp"]
+SOURCES = ["awsem_resident_f32.hip", "awsem_resident_f64.hip", "awsem_wide_f32.hip", "awsem_wide_f64.hip", "awsem_kernels.hip", "awsem_noalloc.hip", "awsem_parse.cpp", "awsem_devsys.cpp", "awsem_api.cpp", "awsem_server.cpp", "awsem_deck.cpp", "awsem_q.cpp", "awsem_frust.hip"]
 ARCH = os.environ.get("AWSEM_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -363,6 +363,8 @@ struct awsem_ctx {
   double prd[3] = {0.0, 0.0, 0.0};
   double last_e[AW_N_ET] = {0.0};  // energies of the last evaluation
   FILE *elog = nullptr;            // energy.log (awsem_energy_log_open)
+  std::string dir = ".";           // working directory of the side and output files
+  AwFrustState tf;                 // the tertiary frustratometer's state
   ~awsem_ctx()
   {
     if (elog) std::fclose(elog);
@@ -442,6 +444,7 @@ int awsem_create(const char *coeff_path, const char *seq_path, const char *workd
   std::unique_ptr<awsem_ctx> c(new awsem_ctx());
   c->precision = precision;
   std::string dir = workdir ? workdir : ".";
+  c->dir = dir;
   AwStatus s = load_system(coeff_path, seq_path, dir, c->sys, true);
   if (s.ok()) s = aw_read_extras(dir, c->sys.cr_file, c->sys);
   if (s.ok()) s = aw_finish_system(c->sys);
@@ -468,6 +471,7 @@ int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, 
   std::unique_ptr<awsem_ctx> c(new awsem_ctx());
   c->precision = precision;
   AwSystem &S = c->sys;
+  c->dir = param_dir ? param_dir : ".";
   AwStatus s = load_system(coeff_path, "", param_dir ? param_dir : ".", S, false);
   if (s.ok()) {
     S.nch = nch;
@@ -666,6 +670,27 @@ static bool finite_all(const double *x, size_t n)
   for (size_t k = 0; k < n; ++k) if (!std::isfinite(x[k])) return false;
   return true;
 }
+
+}  // extern "C"
+
+AwStatus aw_ctx_device(awsem_ctx *c, const double *xres, AwCtxDev &out)
+{
+  AwStatus s = ensure_dev(c);
+  if (!s.ok()) return s;
+  DevBatch &D = *c->dev;
+  HIPCHK(hipMemcpyAsync(D.d_x, xres, (size_t)c->sys.n * 9 * sizeof(double), hipMemcpyHostToDevice, D.stream));
+  out.params = (const FFParams *)D.d_params;
+  out.rinfo = D.reps[0].rinfo;
+  out.x = D.d_x;
+  out.stream = (void *)D.stream;
+  return AwStatus();
+}
+
+AwFrustState &aw_ctx_frust(awsem_ctx *c) { return c->tf; }
+const std::string &aw_ctx_dir(const awsem_ctx *c) { return c->dir; }
+int aw_ctx_fail(awsem_ctx *c, const AwStatus &s) { return fail(c, s); }
+
+extern "C" {
 
 static AwStatus compute_res(awsem_ctx *c, const double *xres, double *fres, double *energy)
 {

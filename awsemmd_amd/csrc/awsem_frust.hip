@@ -1,0 +1,497 @@
+// awsem_frust.hip -- the tertiary frustratometer ([Tertiary_Frustratometer],
+// fb.cpp:5057-5412, energies 5444-5548) on the device.
+//
+// A call at one coordinate set: densities rho_i (k_tf_rho, the well-0 sum
+// of cWell::compute_ro, sml.h:630-650), then the host lists the rows (the
+// contacts i < j within the cutoff, or every residue in single-residue
+// mode) and draws the decoys from the context's rand() stream in the
+// reference's order; k_tf_rows evaluates every (row, entry) energy, entry 0
+// the native assignment and entries 1..m the decoys: the pair's water,
+// burial and Debye-Hueckel terms, plus (mutational and single-residue
+// modes) the environment sum over every other residue k, O(n) per entry.
+// One workgroup per row; the k-dependent, entry-independent pieces (the
+// two wells' theta at r_xk, the density switch of k, the screened 1/r) are
+// computed once per chunk of k into LDS and shared by the row's entries, so
+// the inner loop is table gathers and a few multiplies.  FP64 throughout.
+// The host reduces each row's decoys to mean and (population) standard
+// deviation in the reference's order and writes tertiary_frustration.dat /
+// .tcl as the reference does.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/awsem.h"
+#include "awsem_host.h"
+
+// ---------------------------------------------------------------------------
+// glibc's rand(): srand(seed) fills r[0] = seed, r[i] = 16807 r[i-1]
+// mod (2^31 - 1) for i < 31, r[31..33] = r[0..2]; then r[i] = r[i-31] +
+// r[i-3] (mod 2^32) and the k-th rand() is r[344 + k] >> 1 (the additive
+// feedback generator of its random_r, TYPE_3).  The reference calls rand()
+// without srand: seed 1.
+
+AwFrustState::AwFrustState() { seed(1); }
+
+AwFrustState::~AwFrustState()
+{
+  if (dat) std::fclose(dat);
+  if (tcl) std::fclose(tcl);
+}
+
+void AwFrustState::seed(unsigned s)
+{
+  long long r[34];
+  r[0] = s ? s : 1;
+  for (int i = 1; i < 31; ++i) r[i] = (16807LL * r[i - 1]) % 2147483647LL;
+  for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+  for (int i = 0; i < 34; ++i) rng[i] = (uint32_t)r[i];
+  rng_k = 34;
+  for (int i = 34; i < 344; ++i) next();
+}
+
+int AwFrustState::next()
+{
+  long long k = rng_k;
+  uint32_t v = rng[(k - 31) % 34] + rng[(k - 3) % 34];
+  rng[k % 34] = v;
+  rng_k = k + 1;
+  return (int)(v >> 1);
+}
+
+// ---------------------------------------------------------------------------
+// device side
+
+struct TfRow {
+  int kind;        // 0 pair (configurational), 1 pair + environment (mutational), 2 single residue
+  int a, b, cnt;   // residues (b unused for kind 2); entries
+  long long off;   // first entry
+  double r, ra, rb;  // pair distance and densities (kinds 1, 2; kind 0 per entry)
+};
+struct TfEnt {
+  int ar, at, br, bt;  // residue index (phosphorylation, minimum separation) and type of each side
+};
+struct TfGeo {
+  double r, ra, rb;    // kind 0: the entry's distance and densities
+};
+
+__device__ __forceinline__ const double *tf_site(const double *x, const int *rinfo, int n, int i)
+{
+  return x + (((rinfo[i] & RI_GLY) ? 0 : 1) * n + i) * 3;
+}
+
+__device__ __forceinline__ double tf_dist(const double *x, const int *rinfo, int n, int i, int j)
+{
+  const double *p = tf_site(x, rinfo, n, i), *q = tf_site(x, rinfo, n, j);
+  double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// the well-0 density of cWell::compute_ro (sml.h:630-650, 563-588): pairs
+// of another chain or |i - j| > 1, theta 0 outside (rmin_theta, rmax_theta)
+__global__ void __launch_bounds__(256) k_tf_rho(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                               const double *__restrict__ x, int n, double *__restrict__ rho)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *si = tf_site(x, rinfo, n, i);
+  double k = P->water_kappa, r0 = P->well_r_min[0], r1 = P->well_r_max[0];
+  double lo2 = P->w_rmin_th_sq[0], hi2 = P->w_rmax_th_sq[0];
+  int ci = rinfo[i] >> RI_CHAIN_SHIFT;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (!((rinfo[j] >> RI_CHAIN_SHIFT) != ci || abs(j - i) > 1)) continue;
+    const double *sj = tf_site(x, rinfo, n, j);
+    double dx = si[0] - sj[0], dy = si[1] - sj[1], dz = si[2] - sj[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < lo2 || r2 > hi2) continue;
+    double r = sqrt(r2);
+    s += 0.25 * (1.0 + tanh(k * (r - r0))) * (1.0 + tanh(k * (r1 - r)));
+  }
+  rho[i] = s;
+}
+
+__device__ __forceinline__ const double (*tf_wg(const FFParams *P, int ri, int rj))[20][20][2]
+{
+  return ((ri | rj) & RI_PHOS) ? P->phosph_gamma : P->water_gamma;
+}
+
+// compute_water_energy (fb.cpp:5444-5476) from the two wells' thetas and
+// the density switches h = 1 - tanh(kappa_sigma (rho - rho0))
+__device__ __forceinline__ double tf_water(const FFParams *P, const int *rinfo, int ir, int jr, int it, int jt,
+                                           double td, double tm, double hi, double hj)
+{
+  const double(*g)[20][20][2] = tf_wg(P, rinfo[ir], rinfo[jr]);
+  double sw = 0.25 * hi * hj, sp = 1.0 - sw;
+  double sgd = (g[0][it][jt][0] + g[0][it][jt][1]) / 2, sgm = sp * g[1][it][jt][0] + sw * g[1][it][jt][1];
+  return -(sgd * td + sgm * tm);
+}
+
+__device__ __forceinline__ void tf_thetas(const FFParams *P, double r, double &td, double &tm)
+{
+  double k = P->water_kappa;
+  td = 0.25 * (1.0 + tanh(k * (r - P->well_r_min[0]))) * (1.0 + tanh(k * (P->well_r_max[0] - r)));
+  tm = 0.25 * (1.0 + tanh(k * (r - P->well_r_min[1]))) * (1.0 + tanh(k * (P->well_r_max[1] - r)));
+}
+
+// compute_burial_energy (fb.cpp:5478-5500); a phosphorylated residue reads type 6
+__device__ __forceinline__ double tf_burial(const FFParams *P, const int *rinfo, int ir, int it, double rho)
+{
+  int t = (rinfo[ir] & RI_PHOS) ? 6 : it;
+  double kb = P->burial_kappa, e = 0.0;
+  for (int b = 0; b < 3; ++b)
+    e += -0.5 * P->k_burial * P->burial_gamma[t][b] * (tanh(kb * (rho - P->burial_ro_min[b])) + tanh(kb * (P->burial_ro_max[b] - rho)));
+  return e;
+}
+
+// charge of a residue type (fb.cpp:5510-5531): R, K +1; D, E -1
+__device__ __forceinline__ double tf_charge(int t) { return (t == 1 || t == 11) ? 1.0 : ((t == 3 || t == 6) ? -1.0 : 0.0); }
+
+// compute_electrostatic_energy (fb.cpp:5502-5548) with the screened 1/r
+// factor s = exp(-k_s r / lambda) / r given
+__device__ __forceinline__ double tf_elec(const FFParams *P, int ir, int jr, int it, int jt, double s)
+{
+  if (abs(ir - jr) < P->dh_min_sep) return 0.0;
+  double qi = tf_charge(it), qj = tf_charge(jt);
+  if (qi == 0.0 || qj == 0.0) return 0.0;
+  double k = (qi > 0 && qj > 0) ? P->dh_k[0] : ((qi < 0 && qj < 0) ? P->dh_k[1] : P->dh_k[2]);
+  return P->epsilon * k * qi * qj * s;
+}
+
+#define TF_CH 512  // residues k per LDS chunk
+
+__global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                                const double *__restrict__ x, int n, const double *__restrict__ rho,
+                                                const TfRow *__restrict__ rows, const TfEnt *__restrict__ ent,
+                                                const TfGeo *__restrict__ geo, double *__restrict__ out)
+{
+  __shared__ double c_td[2][TF_CH], c_tm[2][TF_CH], c_es[2][TF_CH], c_hk[TF_CH];
+  __shared__ int c_on[2][TF_CH], c_tk[TF_CH];
+  const TfRow R = rows[blockIdx.x];
+  const bool dh = P->dh_flag != 0;
+  const double ks = P->water_kappa_sigma, th = P->treshold;
+  // base: the pair (kinds 0, 1) or the residue's burial (kind 2)
+  for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
+    TfEnt E = ent[R.off + e];
+    double v;
+    if (R.kind == 2) {
+      v = tf_burial(P, rinfo, E.ar, E.at, R.ra);
+    } else {
+      double r = R.r, ra = R.ra, rb = R.rb;
+      if (R.kind == 0) {
+        TfGeo G = geo[R.off + e];
+        r = G.r; ra = G.ra; rb = G.rb;
+      }
+      double td, tm;
+      tf_thetas(P, r, td, tm);
+      v = tf_water(P, rinfo, E.ar, E.br, E.at, E.bt, td, tm, 1.0 - tanh(ks * (ra - th)), 1.0 - tanh(ks * (rb - th)));
+      v += tf_burial(P, rinfo, E.ar, E.at, ra) + tf_burial(P, rinfo, E.br, E.bt, rb);
+      if (dh) v += tf_elec(P, E.ar, E.br, E.at, E.bt, exp(-P->dh_kscreen * r / P->dh_length) / r);
+    }
+    out[R.off + e] = v;
+  }
+  if (R.kind == 0) return;
+  // the environment: mutational sides a and b over k != a, b (water within
+  // the cutoff, Debye-Hueckel always, fb.cpp:5211-5239); single residue a
+  // over k != a (water within the cutoff and the contact separation, fb.cpp:5362-5390)
+  const int ns = R.kind == 1 ? 2 : 1;
+  const int side[2] = {R.a, R.b};
+  const double hs[2] = {1.0 - tanh(ks * (R.ra - th)), 1.0 - tanh(ks * (R.rb - th))};
+  const double cut = P->tf_cutoff;
+  for (int k0 = 0; k0 < n; k0 += TF_CH) {
+    int kn = min(TF_CH, n - k0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < kn; q += blockDim.x) {
+      int k = k0 + q;
+      c_hk[q] = 1.0 - tanh(ks * (rho[k] - th));
+      c_tk[q] = rinfo[k] & RI_TYPE_MASK;
+      for (int s = 0; s < ns; ++s) {
+        int a = side[s];
+        bool skip = R.kind == 1 ? (k == R.a || k == R.b) : (k == a);
+        double r = tf_dist(x, rinfo, n, a, k);
+        bool wat = !skip && r < cut;
+        if (R.kind == 2)
+          wat = wat && (abs(a - k) >= P->contact_cutoff || (rinfo[a] >> RI_CHAIN_SHIFT) != (rinfo[k] >> RI_CHAIN_SHIFT));
+        double td = 0.0, tm = 0.0;
+        if (wat) tf_thetas(P, r, td, tm);
+        c_td[s][q] = td;
+        c_tm[s][q] = tm;
+        c_on[s][q] = (wat ? 1 : 0) | (!skip && dh ? 2 : 0);
+        c_es[s][q] = (!skip && dh) ? exp(-P->dh_kscreen * r / P->dh_length) / r : 0.0;
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
+      TfEnt E = ent[R.off + e];
+      int er[2] = {E.ar, E.br}, et[2] = {E.at, E.bt};
+      double acc = 0.0;
+      for (int q = 0; q < kn; ++q) {
+        int k = k0 + q, tk = c_tk[q];
+        for (int s = 0; s < ns; ++s) {
+          int on = c_on[s][q];
+          if (on & 1) acc += tf_water(P, rinfo, er[s], k, et[s], tk, c_td[s][q], c_tm[s][q], hs[s], c_hk[q]);
+          if (on & 2) acc += tf_elec(P, er[s], k, et[s], tk, c_es[s][q]);
+        }
+      }
+      out[R.off + e] += acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+#define TF_HIPCHK(expr)                                                                        \
+  do {                                                                                         \
+    if ((expr) != hipSuccess) return AwStatus::err(AWSEM_E_DEVICE, std::string("frustratometer: ") + #expr); \
+  } while (0)
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <class T>
+  AwStatus put(const std::vector<T> &v, hipStream_t st)
+  {
+    size_t b = std::max<size_t>(v.size(), 1) * sizeof(T);
+    if (hipMalloc(&p, b) != hipSuccess) return AwStatus::err(AWSEM_E_NOMEM, "frustratometer: device allocation");
+    if (!v.empty() && hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+      return AwStatus::err(AWSEM_E_DEVICE, "frustratometer: copy");
+    return AwStatus();
+  }
+};
+
+// site (CB; CA for glycine) of residue i in the residue layout
+inline const double *site(const AwSystem &S, const double *x, int i)
+{
+  return x + ((S.se[i] == 'G' ? 0 : 1) * (size_t)S.n + i) * 3;
+}
+inline double dist(const AwSystem &S, const double *x, int i, int j)
+{
+  const double *p = site(S, x, i), *q = site(S, x, j);
+  double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+void stats(const double *e, int m, double *st)
+{
+  double mean = 0.0, var = 0.0;
+  for (int k = 0; k < m; ++k) mean += e[k];
+  mean /= (double)m;
+  for (int k = 0; k < m; ++k) var += (e[k] - mean) * (e[k] - mean);
+  st[0] = mean;
+  st[1] = std::sqrt(var / (double)m);
+}
+
+}  // namespace
+
+// one call: rows of 11 doubles (i, j, i_chain, j_chain, r_ij, rho_i, rho_j,
+// native, decoy mean, decoy std, index; 1-based; j = j_chain = 0 for the
+// single-residue mode)
+static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double> &table)
+{
+  const AwSystem &S = *aw_ctx_system(c);
+  const FFParams &P = S.P;
+  if (!P.tf_flag) return AwStatus::err(AWSEM_E_STATE, "no [Tertiary_Frustratometer] section in the coeff file");
+  if (P.tf_ndecoys < 1) return AwStatus::err(AWSEM_E_FORMAT, "Tertiary_Frustratometer: no decoys");
+  AwCtxDev D;
+  AwStatus s = aw_ctx_device(c, xres, D);
+  if (!s.ok()) return s;
+  hipStream_t st = (hipStream_t)D.stream;
+  int n = S.n, m = P.tf_ndecoys;
+  AwFrustState &F = aw_ctx_frust(c);
+  // densities on the device
+  double *d_rho = nullptr;
+  TF_HIPCHK(hipMalloc((void **)&d_rho, (size_t)n * sizeof(double)));
+  DevBuf rho_buf;
+  rho_buf.p = d_rho;
+  hipLaunchKernelGGL(k_tf_rho, dim3((n + 255) / 256), dim3(256), 0, st, D.params, D.rinfo, D.x, n, d_rho);
+  std::vector<double> rho(n);
+  TF_HIPCHK(hipMemcpyAsync(rho.data(), d_rho, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+  TF_HIPCHK(hipStreamSynchronize(st));
+  auto type = [&](int i) { return S.rinfo[i] & RI_TYPE_MASK; };
+  auto chain = [&](int i) { return S.rinfo[i] >> RI_CHAIN_SHIFT; };
+  double cut = P.tf_cutoff;
+  std::vector<TfRow> rows;
+  std::vector<TfEnt> ent;
+  std::vector<TfGeo> geo;
+  struct Pair { int i, j; double r; };
+  std::vector<Pair> contacts;
+  long long off = 0;
+  // configurational: the decoy row, once per simulation (fb.cpp:5263-5290)
+  int conf_row = -1;
+  if (P.tf_mode != 2) {
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        double r = dist(S, xres, i, j);
+        if (r < cut && (std::abs(i - j) >= P.contact_cutoff || chain(i) != chain(j))) contacts.push_back(Pair{i, j, r});
+      }
+  }
+  if (P.tf_mode == 0) {
+    for (const Pair &q : contacts) {
+      rows.push_back(TfRow{0, q.i, q.j, 1, off, q.r, rho[q.i], rho[q.j]});
+      ent.push_back(TfEnt{q.i, type(q.i), q.j, type(q.j)});
+      geo.push_back(TfGeo{q.r, rho[q.i], rho[q.j]});
+      off += 1;
+    }
+    if (!F.done && !contacts.empty()) {
+      conf_row = (int)rows.size();
+      rows.push_back(TfRow{0, 0, 0, m, off, 0.0, 0.0, 0.0});
+      for (int d = 0; d < m; ++d) {
+        int a = F.next() % n, b = F.next() % n;
+        double r = dist(S, xres, a, b);
+        while (r > cut || a == b) {
+          a = F.next() % n;
+          b = F.next() % n;
+          r = dist(S, xres, a, b);
+        }
+        int a2 = F.next() % n, b2 = F.next() % n;
+        int a3 = F.next() % n, b3 = F.next() % n;
+        ent.push_back(TfEnt{a3, type(a3), b3, type(b3)});
+        geo.push_back(TfGeo{r, rho[a2], rho[b2]});
+      }
+      off += m;
+    }
+  } else if (P.tf_mode == 1) {
+    for (const Pair &q : contacts) {
+      rows.push_back(TfRow{1, q.i, q.j, m + 1, off, q.r, rho[q.i], rho[q.j]});
+      ent.push_back(TfEnt{q.i, type(q.i), q.j, type(q.j)});
+      for (int d = 0; d < m; ++d) {
+        int a = F.next() % n, b = F.next() % n;
+        ent.push_back(TfEnt{a, type(a), b, type(b)});
+      }
+      off += m + 1;
+    }
+  } else {
+    for (int i = 0; i < n; ++i) {
+      rows.push_back(TfRow{2, i, -1, m + 1, off, 0.0, rho[i], 0.0});
+      ent.push_back(TfEnt{i, type(i), 0, 0});
+      for (int d = 0; d < m; ++d) ent.push_back(TfEnt{i, type(F.next() % n), 0, 0});
+      off += m + 1;
+    }
+  }
+  std::vector<double> e((size_t)off);
+  if (!rows.empty()) {
+    DevBuf b_rows, b_ent, b_geo, b_out;
+    if (geo.empty()) geo.push_back(TfGeo{0, 0, 0});
+    if (!(s = b_rows.put(rows, st)).ok() || !(s = b_ent.put(ent, st)).ok() || !(s = b_geo.put(geo, st)).ok()) return s;
+    TF_HIPCHK(hipMalloc(&b_out.p, std::max<size_t>(e.size(), 1) * sizeof(double)));
+    hipLaunchKernelGGL(k_tf_rows, dim3(rows.size()), dim3(256), 0, st, D.params, D.rinfo, D.x, n, d_rho,
+                       (const TfRow *)b_rows.p, (const TfEnt *)b_ent.p, (const TfGeo *)b_geo.p, (double *)b_out.p);
+    TF_HIPCHK(hipGetLastError());
+    TF_HIPCHK(hipMemcpyAsync(e.data(), b_out.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    TF_HIPCHK(hipStreamSynchronize(st));
+  }
+  if (conf_row >= 0) {
+    stats(&e[rows[conf_row].off], m, F.stats);
+    F.done = true;
+  }
+  table.clear();
+  for (size_t k = 0; k < rows.size(); ++k) {
+    const TfRow &R = rows[k];
+    if ((int)k == conf_row) continue;
+    double nat = e[R.off], sd[2];
+    if (R.kind == 0) { sd[0] = F.stats[0]; sd[1] = F.stats[1]; }
+    else stats(&e[R.off + 1], m, sd);
+    double row[11] = {(double)(R.a + 1), R.kind == 2 ? 0.0 : (double)(R.b + 1), (double)(chain(R.a) + 1),
+                      R.kind == 2 ? 0.0 : (double)(chain(R.b) + 1), R.kind == 2 ? 0.0 : R.r, R.ra,
+                      R.kind == 2 ? 0.0 : R.rb, nat, sd[0], sd[1], (sd[0] - nat) / sd[1]};
+    table.insert(table.end(), row, row + 11);
+  }
+  return AwStatus();
+}
+
+extern "C" {
+
+int awsem_tert_frust(awsem_ctx *c, const double *xres, double *rows, int cap, int *n_rows)
+{
+  if (!c || !xres || !n_rows || cap < 0 || (cap > 0 && !rows)) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S || S->n <= 0) return AWSEM_E_ARG;
+  for (size_t k = 0; k < (size_t)S->n * 9; ++k)
+    if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  std::vector<double> t;
+  AwStatus s = tert_frust(c, xres, t);
+  if (!s.ok()) return aw_ctx_fail(c, s);
+  int nr = (int)(t.size() / 11);
+  *n_rows = nr;
+  if (nr > cap) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_ARG, "awsem_tert_frust: more rows than cap"));
+  if (nr) std::memcpy(rows, t.data(), t.size() * sizeof(double));
+  return AWSEM_OK;
+}
+
+// the reference's output at step % tert_frust_output_freq == 0
+// (fb.cpp:7586-7596, 5057-5189; the files opened and headed at
+// construction, fb.cpp:855-867)
+int awsem_tert_frust_write(awsem_ctx *c, const double *xres, int64_t step)
+{
+  if (!c || !xres) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S || S->n <= 0) return AWSEM_E_ARG;
+  const FFParams &P = S->P;
+  if (!P.tf_flag) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_STATE, "no [Tertiary_Frustratometer] section in the coeff file"));
+  if (P.tf_freq <= 0 || step % P.tf_freq != 0) return AWSEM_OK;
+  AwFrustState &F = aw_ctx_frust(c);
+  const std::string &dir = aw_ctx_dir(c);
+  if (!F.dat) {
+    F.dat = std::fopen((dir + "/tertiary_frustration.dat").c_str(), "w");
+    F.tcl = std::fopen((dir + "/tertiary_frustration.tcl").c_str(), "w");
+    if (!F.dat || !F.tcl) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_FILE, "cannot open tertiary_frustration.dat/.tcl"));
+    if (P.tf_mode != 2)
+      std::fprintf(F.dat, "# i j i_chain j_chain xi yi zi xj yj zj r_ij rho_i rho_j a_i a_j native_energy "
+                          "<decoy_energies> std(decoy_energies) f_ij\n");
+    else
+      std::fprintf(F.dat, "# i i_chain xi yi zi rho_i a_i native_energy <decoy_energies> std(decoy_energies) f_i\n");
+  }
+  std::vector<double> t;
+  for (size_t k = 0; k < (size_t)S->n * 9; ++k)
+    if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  AwStatus s = tert_frust(c, xres, t);
+  if (!s.ok()) return aw_ctx_fail(c, s);
+  std::fprintf(F.dat, "# timestep: %d\n", (int)step);
+  std::fprintf(F.tcl, "# timestep: %d\n", (int)step);
+  int n = S->n, atomselect = 0;
+  for (size_t k = 0; k + 11 <= t.size(); k += 11) {
+    const double *q = &t[k];
+    int i = (int)q[0] - 1;
+    const double *xi = site(*S, xres, i);
+    if (P.tf_mode != 2) {
+      int j = (int)q[1] - 1;
+      const double *xj = site(*S, xres, j);
+      std::fprintf(F.dat, "%5d %5d %3d %3d %8.3f %8.3f %8.3f %8.3f %8.3f %8.3f %8.3f %8.3f %8.3f %c %c %8.3f %8.3f %8.3f %8.3f\n",
+                   i + 1, j + 1, (int)q[2], (int)q[3], xi[0], xi[1], xi[2], xj[0], xj[1], xj[2], q[4], q[5], q[6],
+                   S->se[i], S->se[j], q[7], q[8], q[9], q[10]);
+      if (q[10] > 0.78 || q[10] < -1) {
+        std::fprintf(F.tcl, "set sel%d [atomselect top \"resid %d and name CA\"]\n", i, i + 1);
+        std::fprintf(F.tcl, "set sel%d [atomselect top \"resid %d and name CA\"]\n", j, j + 1);
+        std::fprintf(F.tcl, "lassign [atomselect%d get {x y z}] pos1\n", atomselect++);
+        std::fprintf(F.tcl, "lassign [atomselect%d get {x y z}] pos2\n", atomselect++);
+        std::fprintf(F.tcl, q[10] > 0.78 ? "draw color green\n" : "draw color red\n");
+        if (q[4] < P.well_r_max[0]) std::fprintf(F.tcl, "draw line $pos1 $pos2 style solid width 1\n");
+        else std::fprintf(F.tcl, "draw line $pos1 $pos2 style dashed width 2\n");
+      }
+    } else {
+      std::fprintf(F.dat, "%5d %5d %8.3f %8.3f %8.3f %8.3f %c %8.3f %8.3f %8.3f %8.3f\n", i + 1, (int)q[2], xi[0], xi[1],
+                   xi[2], q[5], S->se[i], q[7], q[8], q[9], q[10]);
+      atomselect += 1;
+      std::fprintf(F.tcl, "mol addrep 0\n");
+      std::fprintf(F.tcl, "mol modselect %d 0 resid %d\n", atomselect, i + 1);
+      std::fprintf(F.tcl, "mol modstyle %d 0 VDW %f 12.000000\n", atomselect, 0.5 * std::fabs(q[10]));
+      std::fprintf(F.tcl, "mol modmaterial %d 0 Transparent\n", atomselect);
+      std::fprintf(F.tcl, q[10] > 0.0 ? "mol modcolor %d 0 ColorID 7\n" : "mol modcolor %d 0 ColorID 1\n", atomselect);
+    }
+  }
+  std::fprintf(F.tcl, "mol modselect 0 top \"all\"\n");
+  std::fprintf(F.tcl, "mol modstyle 0 top newcartoon\n");
+  std::fprintf(F.tcl, "mol modcolor 0 top colorid 15\n");
+  std::fflush(F.dat);
+  std::fflush(F.tcl);
+  (void)n;
+  return AWSEM_OK;
+}
+
+}  // extern "C"

@@ -223,3 +223,34 @@ AwStatus aw_devsys_create(const AwDevPlan &plan, AwDevSystem &out);
 AwStatus upload_raw(void **dst, const void *src, size_t bytes);
 long long aw_slab_doubles(int n);
 
+
+// ---------------------------------------------------------------------------
+// the tertiary frustratometer of a context (awsem_frust.hip): glibc's rand()
+// stream the decoys draw from (the reference's unseeded rand(), continued
+// across calls), the configurational decoy statistics (computed once,
+// fb.cpp:5099, 5342) and the output files
+struct AwFrustState {
+  uint32_t rng[34];
+  long long rng_k = 0;
+  bool done = false;
+  double stats[2] = {0.0, 0.0};
+  FILE *dat = nullptr, *tcl = nullptr;
+  AwFrustState();
+  ~AwFrustState();
+  int next();
+  void seed(unsigned s);
+};
+struct awsem_ctx;
+// the context's device image built and xres (residue layout) uploaded as
+// its coordinates: FFParams, rinfo and coordinates on the device, stream
+struct AwCtxDev {
+  const FFParams *params;
+  const int *rinfo;
+  const double *x;
+  void *stream;
+};
+AwStatus aw_ctx_device(awsem_ctx *c, const double *xres, AwCtxDev &out);
+AwFrustState &aw_ctx_frust(awsem_ctx *c);
+const AwSystem *aw_ctx_system(const awsem_ctx *c);
+const std::string &aw_ctx_dir(const awsem_ctx *c);
+int aw_ctx_fail(awsem_ctx *c, const AwStatus &s);

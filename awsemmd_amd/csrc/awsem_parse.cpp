@@ -33,7 +33,7 @@ static std::string join(const std::string &dir, const std::string &name)
 }
 
 static const char *kUnsupported[] = {
-    "[Fragment_Frustratometer]", "[Tertiary_Frustratometer]",
+    "[Fragment_Frustratometer]",
     "[Nmer_Frustratometer]", "[Amylometer]", "[Selection_Temperature]",
     "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]",
     "[DebyeHuckel_Optimization]", nullptr};
@@ -170,6 +170,16 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
       in >> P.k_hypercharge >> P.n_phosph;
       if (P.n_phosph > 20) return AwStatus::err(-3, "Number of phosphorylated residues may not exceed 20");
       for (int j = 0; j < P.n_phosph; ++j) in >> P.phosph_res[j];
+    } else if (sec == "[Tertiary_Frustratometer]") {  // fb.cpp:385-397
+      std::string mode;
+      P.tf_flag = 1;
+      in >> P.tf_cutoff >> P.tf_ndecoys >> P.tf_freq >> mode;
+      if (mode == "configurational") P.tf_mode = 0;
+      else if (mode == "mutational") P.tf_mode = 1;
+      else if (mode == "singleresidue") P.tf_mode = 2;
+      else if (!in.fail())
+        return AwStatus::err(-3, "Only \"configurational\", \"mutational\", \"singleresidue\" are acceptable modes "
+                                 "for the Tertiary_Frustratometer.");
     } else if (sec == "[DebyeHuckel]") {  // fb.cpp:466-478
       P.dh_flag = 1;
       in >> P.dh_k[0] >> P.dh_k[1] >> P.dh_k[2] >> P.dh_kscreen >> P.dh_length >> P.dh_min_sep;

@@ -94,6 +94,10 @@ struct FFParams {
   int dh_min_sep, pad_dh;
   // the hypercharged water table of phosphorylated pairs (fb.cpp:643-669)
   double phosph_gamma[5][20][20][2];
+  // [Tertiary_Frustratometer] (fb.cpp:385-397): contact cutoff, decoys,
+  // output interval, mode 0 configurational / 1 mutational / 2 single residue
+  int tf_flag, tf_ndecoys, tf_freq, tf_mode;
+  double tf_cutoff;
   // terms whose pairs come from FixBackbone's neighbour list with no cutoff
   // of their own (Debye-Hueckel, the barrier's CA-CA pairs, AMH-Go) count
   // pairs closer than pair_list_cutoff + list_skin (a list LAMMPS has just

@@ -378,6 +378,24 @@ int awsem_deck_variable(awsem_deck *d, const char *formula, double *value);
 int awsem_deck_last_error(const awsem_deck *d, char *buf, size_t n);
 int awsem_deck_destroy(awsem_deck *d);
 
+/* ---- the tertiary frustratometer ([Tertiary_Frustratometer]) ------------
+ * compute_tert_frust / compute_tert_frust_singleresidue (fb.cpp:5057-5412)
+ * at one coordinate set (residue layout), with the coeff section's cutoff,
+ * decoy count and mode.  Decoys draw from the context's own copy of the C
+ * library's unseeded rand() stream (glibc's generator, seed 1), continued
+ * across calls; the configurational decoy statistics are computed once, at
+ * the first call with a contact (fb.cpp:5099, 5342).  rows: 11 doubles per
+ * contact (i, j, i_chain, j_chain, r_ij, rho_i, rho_j, native energy, decoy
+ * mean, decoy std, frustration index; 1-based) or per residue in
+ * singleresidue mode (j, j_chain, r_ij and rho_j 0); *n_rows receives the
+ * count (AWSEM_E_ARG when it exceeds cap). */
+int awsem_tert_frust(awsem_ctx *c, const double *xres, double *rows, int cap, int *n_rows);
+/* The reference's output when step % output interval == 0:
+ * tertiary_frustration.dat and .tcl in the context's directory (created
+ * with their header at the first call), "# timestep: step" and the rows in
+ * its formats (fb.cpp:855-867, 7586-7596, 5104-5124, 5168-5182). */
+int awsem_tert_frust_write(awsem_ctx *c, const double *xres, int64_t step);
+
 /* ---- order parameters (compute qwolynes, compute qonuchic) ---------------
  * Replace `compute ID alpha_carbons qwolynes rnative.dat sep exp`
  * (src/compute_q_wolynes.cpp:51-88, compute_scalar 171-236) and

@@ -1,0 +1,91 @@
+"""GPU tertiary frustratometer ([Tertiary_Frustratometer], fb.cpp:5057-5412)
+against the oracle: every mode on compact multimers (one with the
+Debye-Hueckel term and phosphorylated residues, which change the decoys'
+electrostatics and gammas), two calls in a row (the rand() stream continues;
+the configurational statistics stay those of the first call), and the
+tertiary_frustration.dat / .tcl writer.
+
+Energies are FP64 sums in another order than the oracle's: 1e-10 relative
+on energies, 1e-8 on the frustration index."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import make_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    import awsemmd_amd as A
+    A.lib()
+    return A
+
+
+def frust_case(lengths, seed, mode, ndecoys=150, extras=False):
+    from awsemmd_amd import workload as W
+    s, p = make_case(lengths, seed, compact=True)
+    if extras:
+        W.write_extras(s, p["workdir"], terms=("dh", "phosph"), seed=seed)
+    with open(p["coeff"], "a") as f:
+        f.write("\n[Tertiary_Frustratometer]\n9.5\n%d\n1\n%s\n\n" % (ndecoys, mode))
+    return s, p
+
+
+def _compare(got, want):
+    assert got.shape == want.shape, (got.shape, want.shape)
+    assert np.array_equal(got[:, :4], want[:, :4])
+    assert np.abs(got[:, 4:7] - want[:, 4:7]).max() <= 1e-12
+    e = np.abs(got[:, 7:10] - want[:, 7:10]) / np.maximum(1.0, np.abs(want[:, 7:10]))
+    assert e.max() <= 1e-10, e.max()
+    f = np.abs(got[:, 10] - want[:, 10]) / np.maximum(1.0, np.abs(want[:, 10]))
+    assert f.max() <= 1e-8, f.max()
+
+
+@pytest.mark.parametrize("mode", ["configurational", "mutational", "singleresidue"])
+@pytest.mark.parametrize("extras", [False, True])
+def test_tert_frust_against_oracle(oracle_mod, mode, extras):
+    A = _gpu()
+    s, p = frust_case([40, 30], 71 if not extras else 72, mode, extras=extras)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    x = s.coords()
+    for call in range(2):
+        got = fix.tert_frust(x)
+        want = o.tert_frust(x)
+        print(mode, extras, call, len(got), "rows; native", got[:3, 7], "index", got[:3, 10], flush=True)
+        assert len(got) > 10
+        _compare(got, want)
+        x = x + np.random.default_rng(call).normal(scale=0.05, size=x.shape)
+
+
+def test_tert_frust_files(oracle_mod):
+    """The writer: header, '# timestep: k' blocks at the output interval,
+    the rows in the reference's %8.3f format (compared with the oracle's
+    values to the printed precision), the tcl commands for the strongly
+    frustrated and minimally frustrated contacts."""
+    A = _gpu()
+    s, p = frust_case([35, 25], 73, "mutational", ndecoys=100)
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    x = s.coords()
+    want = []
+    for step in range(3):
+        fix.tert_frust_write(x, step)
+        want.append(o.tert_frust(x))
+    dat = open(os.path.join(p["workdir"], "tertiary_frustration.dat")).read().splitlines()
+    tcl = open(os.path.join(p["workdir"], "tertiary_frustration.tcl")).read().splitlines()
+    assert dat[0].startswith("# i j i_chain j_chain xi yi zi")
+    blocks = [k for k, ln in enumerate(dat) if ln.startswith("# timestep:")]
+    assert [dat[k] for k in blocks] == ["# timestep: %d" % k for k in range(3)]
+    for b, w in zip(blocks, want):
+        rows = [ln.split() for ln in dat[b + 1:b + 1 + len(w)]]
+        for r, q in zip(rows, w):
+            assert int(r[0]) == int(q[0]) and int(r[1]) == int(q[1])
+            vals = [float(v) for v in r[10:13]] + [float(v) for v in r[15:19]]
+            ref = [q[4], q[5], q[6], q[7], q[8], q[9], q[10]]
+            assert max(abs(a - b) for a, b in zip(vals, ref)) <= 0.0015, (r, q)
+    nstrong = sum(1 for w in want for q in w if q[10] > 0.78 or q[10] < -1)
+    assert sum(1 for ln in tcl if ln.startswith("draw line")) == nstrong
+    assert tcl.count('mol modstyle 0 top newcartoon') == 3
