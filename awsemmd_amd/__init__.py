@@ -48,7 +48,8 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_deck_open", "awsem_deck_run", "awsem_deck_natoms", "awsem_deck_state",
            "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy",
            "awsem_q_wolynes_create", "awsem_q_onuchic_create", "awsem_q_compute", "awsem_batch_q",
-           "awsem_q_last_error", "awsem_q_destroy", "awsem_tert_frust", "awsem_tert_frust_write"]
+           "awsem_q_last_error", "awsem_q_destroy", "awsem_tert_frust", "awsem_tert_frust_write",
+           "awsem_tert_frust_every"]
 
 
 class AwsemError(RuntimeError):
@@ -152,6 +153,7 @@ def lib():
         L.awsem_q_destroy.argtypes = [vp]
         L.awsem_tert_frust.argtypes = [vp, dp, dp, C.c_int, C.POINTER(C.c_int)]
         L.awsem_tert_frust_write.argtypes = [vp, dp, C.c_int64]
+        L.awsem_tert_frust_every.argtypes = [vp, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -255,6 +257,12 @@ class FixBackbone:
     def tert_frust_write(self, xres: np.ndarray, step: int):
         x = np.ascontiguousarray(xres, dtype=np.float64)
         self._check(lib().awsem_tert_frust_write(self.h, _dp(x), step))
+
+    def tert_frust_every(self) -> int:
+        """Output interval of the [Tertiary_Frustratometer] section (0: none)."""
+        k = C.c_int64()
+        self._check(lib().awsem_tert_frust_every(self.h, C.byref(k)))
+        return k.value
 
     def set_list_skin(self, skin: float):
         """Neighbour-list skin of the list-limited terms (Debye-Hueckel, the

@@ -173,6 +173,7 @@ struct awsem_deck {
   // fixes
   std::string fb_id, fb_group, fb_beta, fb_oxy, fb_coeff, fb_seq;
   bool have_nve = false, have_langevin = false;
+  long long tf_every = 0;  // the frustratometer's output interval (0: none)
   double lg_t0 = 0.0, lg_t1 = 0.0, lg_damp = 0.0;
   long long lg_seed = 0;
   std::vector<Dump> dumps;
@@ -1203,6 +1204,9 @@ struct awsem_deck {
     batch_check(awsem_batch_compute(batch));
     pull();
     long long s0 = step, s1 = step + n;
+    int64_t tfe = 0;
+    ctx_check(awsem_tert_frust_every(ctx, &tfe));
+    tf_every = (long long)tfe;
     outputs(s0, true, n == 0);
     double T0 = have_langevin ? lg_t0 : 0.0, T1 = have_langevin ? lg_t1 : 0.0;
     // without fix langevin the thermostat is off: an infinite damping time
@@ -1214,6 +1218,7 @@ struct awsem_deck {
         if (every > 0) next = std::min(next, (cur / every + 1) * every);
       };
       upd(thermo_every);
+      upd(tf_every);
       for (const Dump &d : dumps) upd(d.every);
       for (const PrintFix &p : prints) upd(p.every);
       // fix langevin ramps T over the run (delta = elapsed / run length);
@@ -1240,6 +1245,13 @@ struct awsem_deck {
     // FixBackbone's energy.log: every evaluation on a thermo step
     if (elog_open && thermo_every > 0 && s % thermo_every == 0) ctx_check(awsem_energy_log_write(ctx, s, energy));
     if ((thermo_every > 0 && s % thermo_every == 0) || setup || last) thermo_line(s, setup);
+    // FixBackbone's tertiary frustratometer: every evaluation on a multiple
+    // of its output interval (fb.cpp:7580-7590), the setup one included
+    if (tf_every > 0 && s % tf_every == 0) {
+      std::vector<double> x, v;
+      to_res(x, v);
+      ctx_check(awsem_tert_frust_write(ctx, x.data(), s));
+    }
     for (Dump &d : dumps)
       if (s % d.every == 0 && s != d.last) dump_frame(d, s);
     for (PrintFix &p : prints)

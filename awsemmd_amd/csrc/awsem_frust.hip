@@ -494,4 +494,13 @@ int awsem_tert_frust_write(awsem_ctx *c, const double *xres, int64_t step)
   return AWSEM_OK;
 }
 
+int awsem_tert_frust_every(const awsem_ctx *c, int64_t *every)
+{
+  if (!c || !every) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S) return AWSEM_E_ARG;
+  *every = S->P.tf_flag ? (int64_t)S->P.tf_freq : 0;
+  return AWSEM_OK;
+}
+
 }  // extern "C"

@@ -395,6 +395,10 @@ int awsem_tert_frust(awsem_ctx *c, const double *xres, double *rows, int cap, in
  * with their header at the first call), "# timestep: step" and the rows in
  * its formats (fb.cpp:855-867, 7586-7596, 5104-5124, 5168-5182). */
 int awsem_tert_frust_write(awsem_ctx *c, const double *xres, int64_t step);
+/* The output interval of the [Tertiary_Frustratometer] section (its third
+ * line, fb.cpp:385-397), 0 without the section: a driver stops at its
+ * multiples to call awsem_tert_frust_write. */
+int awsem_tert_frust_every(const awsem_ctx *c, int64_t *every);
 
 /* ---- order parameters (compute qwolynes, compute qonuchic) ---------------
  * Replace `compute ID alpha_carbons qwolynes rnative.dat sep exp`

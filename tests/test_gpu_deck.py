@@ -185,3 +185,37 @@ def test_deck_1r69_shipped_system(oracle_mod):
     for r in rows:
         w = r.split()
         check_energies([float(t) for t in w[1:]], ref[int(w[0])][2], 1e-6)
+
+
+def test_deck_tertiary_frustratometer(oracle_mod):
+    """A deck whose coeff file enables the tertiary frustratometer
+    (mutational, 80 decoys, every 5 steps): the run stops at its output
+    steps and writes tertiary_frustration.dat blocks at 0, 5, ..., 20 whose
+    rows equal the oracle's frustratometer at the oracle's trajectory (the
+    rand() stream continued across the blocks) to the printed precision."""
+    A = _A()
+    s, path, d = deckgen.make_deck([40, 30], 54, compact=True, runs=(20,), thermo=10, dump=20, print_every=10)
+    with open(os.path.join(d, "fix_backbone_coeff.data"), "a") as f:
+        f.write("\n[Tertiary_Frustratometer]\n9.5\n80\n5\nmutational\n\n")
+    dk = A.Deck(path, A.PREC_FP64)
+    x0, v0 = dk.residue_state()
+    o = deckgen.deck_oracle(oracle_mod, d)
+    ref = oracle_run(o, x0, v0, 0, 20, events_of(0, 20, [10, 20, 5]), 500.0, 300.0, 100.0, 3456, 2.0)
+    dk.run()
+    x, _ = dk.residue_state()
+    assert np.abs(x - ref[20][0]).max() < 1e-8
+    dat = open(os.path.join(d, "tertiary_frustration.dat")).read().splitlines()
+    blocks = [k for k, ln in enumerate(dat) if ln.startswith("# timestep:")]
+    assert [dat[k] for k in blocks] == ["# timestep: %d" % k for k in (0, 5, 10, 15, 20)]
+    blocks.append(len(dat))
+    for b, b1 in zip(blocks[:-1], blocks[1:]):
+        st = int(dat[b].split()[-1])
+        want = o.tert_frust(ref[st][0])
+        rows = [ln.split() for ln in dat[b + 1:b1]]
+        assert len(rows) == len(want) > 10, st
+        for r, q in zip(rows, want):
+            assert int(r[0]) == int(q[0]) and int(r[1]) == int(q[1])
+            vals = [float(v) for v in r[10:13]] + [float(v) for v in r[15:19]]
+            ref_v = [q[4], q[5], q[6], q[7], q[8], q[9], q[10]]
+            assert max(abs(a - c) for a, c in zip(vals, ref_v)) <= 0.0015, (st, r, q)
+        print(f"step {st}: {len(rows)} contacts, mean index {np.mean(want[:, 10]):.3f}", flush=True)
