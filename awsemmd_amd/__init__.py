@@ -49,7 +49,7 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy",
            "awsem_q_wolynes_create", "awsem_q_onuchic_create", "awsem_q_compute", "awsem_batch_q",
            "awsem_q_last_error", "awsem_q_destroy", "awsem_tert_frust", "awsem_tert_frust_write",
-           "awsem_tert_frust_every"]
+           "awsem_tert_frust_every", "awsem_nmer_frust", "awsem_nmer_frust_write", "awsem_nmer_frust_every"]
 
 
 class AwsemError(RuntimeError):
@@ -154,6 +154,9 @@ def lib():
         L.awsem_tert_frust.argtypes = [vp, dp, dp, C.c_int, C.POINTER(C.c_int)]
         L.awsem_tert_frust_write.argtypes = [vp, dp, C.c_int64]
         L.awsem_tert_frust_every.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.awsem_nmer_frust.argtypes = [vp, dp, dp, C.c_int, C.POINTER(C.c_int), dp, C.c_int, C.POINTER(C.c_int)]
+        L.awsem_nmer_frust_write.argtypes = [vp, dp, C.c_int64]
+        L.awsem_nmer_frust_every.argtypes = [vp, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -257,6 +260,32 @@ class FixBackbone:
     def tert_frust_write(self, xres: np.ndarray, step: int):
         x = np.ascontiguousarray(xres, dtype=np.float64)
         self._check(lib().awsem_tert_frust_write(self.h, _dp(x), step))
+
+    def nmer_frust(self, xres: np.ndarray):
+        """The n-mer frustratometer at xres (3, n, 3): (rows, traps); rows (i,
+        j, contacts, native, decoy mean, decoy std, index), traps (row,
+        first, second, k, backward, threshold, energy)."""
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        cap, tcap = self.n * self.n + 16, 4096
+        while True:
+            out, tr = np.zeros(7 * cap), np.zeros(7 * tcap)
+            k, t = C.c_int(), C.c_int()
+            rc = lib().awsem_nmer_frust(self.h, _dp(x), _dp(out), cap, C.byref(k), _dp(tr), tcap, C.byref(t))
+            if rc == -1 and (k.value > cap or t.value > tcap):
+                cap, tcap = max(cap, k.value), max(tcap, t.value)
+                continue
+            self._check(rc)
+            return out[:7 * k.value].reshape(k.value, 7), tr[:7 * t.value].reshape(t.value, 7)
+
+    def nmer_frust_write(self, xres: np.ndarray, step: int):
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        self._check(lib().awsem_nmer_frust_write(self.h, _dp(x), step))
+
+    def nmer_frust_every(self) -> int:
+        """Output interval of the [Nmer_Frustratometer] section (0: none)."""
+        k = C.c_int64()
+        self._check(lib().awsem_nmer_frust_every(self.h, C.byref(k)))
+        return k.value
 
     def tert_frust_every(self) -> int:
         """Output interval of the [Tertiary_Frustratometer] section (0: none)."""

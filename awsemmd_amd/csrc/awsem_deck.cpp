@@ -173,7 +173,7 @@ struct awsem_deck {
   // fixes
   std::string fb_id, fb_group, fb_beta, fb_oxy, fb_coeff, fb_seq;
   bool have_nve = false, have_langevin = false;
-  long long tf_every = 0;  // the frustratometer's output interval (0: none)
+  long long tf_every = 0, nf_every = 0;  // the frustratometers' output intervals (0: none)
   double lg_t0 = 0.0, lg_t1 = 0.0, lg_damp = 0.0;
   long long lg_seed = 0;
   std::vector<Dump> dumps;
@@ -1207,6 +1207,8 @@ struct awsem_deck {
     int64_t tfe = 0;
     ctx_check(awsem_tert_frust_every(ctx, &tfe));
     tf_every = (long long)tfe;
+    ctx_check(awsem_nmer_frust_every(ctx, &tfe));
+    nf_every = (long long)tfe;
     outputs(s0, true, n == 0);
     double T0 = have_langevin ? lg_t0 : 0.0, T1 = have_langevin ? lg_t1 : 0.0;
     // without fix langevin the thermostat is off: an infinite damping time
@@ -1219,6 +1221,7 @@ struct awsem_deck {
       };
       upd(thermo_every);
       upd(tf_every);
+      upd(nf_every);
       for (const Dump &d : dumps) upd(d.every);
       for (const PrintFix &p : prints) upd(p.every);
       // fix langevin ramps T over the run (delta = elapsed / run length);
@@ -1245,12 +1248,14 @@ struct awsem_deck {
     // FixBackbone's energy.log: every evaluation on a thermo step
     if (elog_open && thermo_every > 0 && s % thermo_every == 0) ctx_check(awsem_energy_log_write(ctx, s, energy));
     if ((thermo_every > 0 && s % thermo_every == 0) || setup || last) thermo_line(s, setup);
-    // FixBackbone's tertiary frustratometer: every evaluation on a multiple
-    // of its output interval (fb.cpp:7580-7590), the setup one included
-    if (tf_every > 0 && s % tf_every == 0) {
+    // FixBackbone's tertiary and n-mer frustratometers, in the reference's
+    // order: every evaluation on a multiple of their output intervals
+    // (fb.cpp:7580-7603), the setup one included
+    if ((tf_every > 0 && s % tf_every == 0) || (nf_every > 0 && s % nf_every == 0)) {
       std::vector<double> x, v;
       to_res(x, v);
-      ctx_check(awsem_tert_frust_write(ctx, x.data(), s));
+      if (tf_every > 0 && s % tf_every == 0) ctx_check(awsem_tert_frust_write(ctx, x.data(), s));
+      if (nf_every > 0 && s % nf_every == 0) ctx_check(awsem_nmer_frust_write(ctx, x.data(), s));
     }
     for (Dump &d : dumps)
       if (s % d.every == 0 && s != d.last) dump_frame(d, s);

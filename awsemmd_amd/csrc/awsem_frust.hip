@@ -40,6 +40,9 @@ AwFrustState::~AwFrustState()
 {
   if (dat) std::fclose(dat);
   if (tcl) std::fclose(tcl);
+  if (nm_dat) std::fclose(nm_dat);
+  if (nm_tcl) std::fclose(nm_tcl);
+  if (nm_trap) std::fclose(nm_trap);
 }
 
 void AwFrustState::seed(unsigned s)
@@ -242,6 +245,151 @@ __global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P,
 }
 
 // ---------------------------------------------------------------------------
+// the n-mer frustratometer (fb.cpp:5600-6061)
+
+#define NM_MAX 32  // largest n-mer size the kernels take
+
+// contacts between the n-mers at i and j (compute_nmer_contacts,
+// fb.cpp:5895-5919: the sequence-separation test only, no chain test) for
+// the start pairs i < j < n - s with j - i >= s; -1 for the others
+__global__ void __launch_bounds__(256) k_nm_contacts(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                                    const double *__restrict__ x, int n, int s, int *__restrict__ cnt)
+{
+  const int ns = n - s;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ns * ns) return;
+  int i = (int)(t / ns), j = (int)(t % ns);
+  if (!(j > i && j - i >= s)) {
+    cnt[t] = -1;
+    return;
+  }
+  int c = 0;
+  const double cut = P->nf_cutoff;
+  for (int a = i; a < i + s; ++a)
+    for (int b = j; b < j + s; ++b)
+      if (abs(a - b) >= P->contact_cutoff && tf_dist(x, rinfo, n, a, b) < cut) c++;
+  cnt[t] = c;
+}
+
+struct NmRow {
+  int a, b, cnt, ncont;  // the two n-mers' first residues; entries; contacts
+  long long off;         // first entry
+};
+// the types of n-mer a are those of residues sa + da*o, of n-mer b those
+// of sb + db*o; swap: the reference's order is b's burials first and b
+// outer in the water loop (compute_nmer_traps(j, i), fb.cpp:5622)
+struct NmEnt {
+  int sa, sb;
+  int da, db, swap, pad;
+};
+
+// every (row, entry) energy of the pairwise mode: burials of both n-mers
+// and the water of their s*s cross pairs (compute_nmer_native_ixn,
+// fb.cpp:5947-5988, the decoys 6028-6056, the traps 5822-5853).  One
+// workgroup per row; the pair geometry (both wells' theta), the density
+// switches and every member's burial for each of the 20 types are staged
+// in LDS once and shared by the row's entries.
+__global__ void __launch_bounds__(256) k_nm_pairs(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                                 const double *__restrict__ x, int n, int s,
+                                                 const double *__restrict__ rho, const NmRow *__restrict__ rows,
+                                                 const NmEnt *__restrict__ ent, double *__restrict__ out)
+{
+  __shared__ double l_td[NM_MAX * NM_MAX], l_tm[NM_MAX * NM_MAX];
+  __shared__ double l_h[2][NM_MAX];
+  __shared__ double l_bur[2][NM_MAX][20];
+  const NmRow R = rows[blockIdx.x];
+  const double ks = P->water_kappa_sigma, th = P->treshold;
+  for (int q = threadIdx.x; q < s * s; q += blockDim.x) {
+    int o = q / s, p = q % s;
+    double td, tm;
+    tf_thetas(P, tf_dist(x, rinfo, n, R.a + o, R.b + p), td, tm);
+    l_td[q] = td;
+    l_tm[q] = tm;
+  }
+  for (int q = threadIdx.x; q < 2 * s * 20; q += blockDim.x) {
+    int side = q / (s * 20), o = (q / 20) % s, t = q % 20;
+    int r = (side ? R.b : R.a) + o;
+    l_bur[side][o][t] = tf_burial(P, rinfo, r, t, rho[r]);
+  }
+  for (int q = threadIdx.x; q < 2 * s; q += blockDim.x) {
+    int side = q / s, o = q % s;
+    l_h[side][o] = 1.0 - tanh(ks * (rho[(side ? R.b : R.a) + o] - th));
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
+    const NmEnt E = ent[R.off + e];
+    double v = 0.0;
+    if (!E.swap) {
+      for (int o = 0; o < s; ++o) v += l_bur[0][o][rinfo[E.sa + E.da * o] & RI_TYPE_MASK];
+      for (int p = 0; p < s; ++p) v += l_bur[1][p][rinfo[E.sb + E.db * p] & RI_TYPE_MASK];
+      for (int o = 0; o < s; ++o) {
+        int ta = rinfo[E.sa + E.da * o] & RI_TYPE_MASK;
+        for (int p = 0; p < s; ++p) {
+          int tb = rinfo[E.sb + E.db * p] & RI_TYPE_MASK;
+          v += tf_water(P, rinfo, R.a + o, R.b + p, ta, tb, l_td[o * s + p], l_tm[o * s + p], l_h[0][o], l_h[1][p]);
+        }
+      }
+    } else {
+      for (int p = 0; p < s; ++p) v += l_bur[1][p][rinfo[E.sb + E.db * p] & RI_TYPE_MASK];
+      for (int o = 0; o < s; ++o) v += l_bur[0][o][rinfo[E.sa + E.da * o] & RI_TYPE_MASK];
+      for (int p = 0; p < s; ++p) {
+        int tb = rinfo[E.sb + E.db * p] & RI_TYPE_MASK;
+        for (int o = 0; o < s; ++o) {
+          int ta = rinfo[E.sa + E.da * o] & RI_TYPE_MASK;
+          v += tf_water(P, rinfo, R.b + p, R.a + o, tb, ta, l_td[o * s + p], l_tm[o * s + p], l_h[1][p], l_h[0][o]);
+        }
+      }
+    }
+    out[R.off + e] = v;
+  }
+}
+
+// single-n-mer mode: the energy of residue j with type t
+// (compute_singleresidue_native_ixn with nmercalc, fb.cpp:5346-5393: its
+// burial, its water with the residues k < j within the cutoff and the
+// contact separation or another chain, its Debye-Hueckel with every k < j),
+// for every (j, t): an entry is then a sum of s table values
+__global__ void __launch_bounds__(256) k_nm_single(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                                  const double *__restrict__ x, int n,
+                                                  const double *__restrict__ rho, double *__restrict__ e1)
+{
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * 20) return;
+  const int j = q / 20, t = q % 20;
+  const double ks = P->water_kappa_sigma, th = P->treshold, cut = P->nf_cutoff;
+  const bool dh = P->dh_flag != 0;
+  const double hj = 1.0 - tanh(ks * (rho[j] - th));
+  const int cj = rinfo[j] >> RI_CHAIN_SHIFT;
+  double w = 0.0, el = 0.0;
+  for (int k = 0; k < j; ++k) {
+    double r = tf_dist(x, rinfo, n, j, k);
+    int kt = rinfo[k] & RI_TYPE_MASK;
+    if (r < cut && (j - k >= P->contact_cutoff || (rinfo[k] >> RI_CHAIN_SHIFT) != cj)) {
+      double td, tm;
+      tf_thetas(P, r, td, tm);
+      w += tf_water(P, rinfo, j, k, t, kt, td, tm, hj, 1.0 - tanh(ks * (rho[k] - th)));
+    }
+    if (dh) el += tf_elec(P, j, k, t, kt, exp(-P->dh_kscreen * r / P->dh_length) / r);
+  }
+  e1[q] = w + tf_burial(P, rinfo, j, t, rho[j]) + el;
+}
+
+// entry e of row e / (m + 1): the n-mer at row's start with the types of
+// residues src[e] + o (compute_singlenmer_native_ixn / decoy_ixns,
+// fb.cpp:5708-5768)
+__global__ void __launch_bounds__(256) k_nm_single_sum(const int *__restrict__ rinfo, const double *__restrict__ e1,
+                                                      int s, int m, long long n_ent, const int *__restrict__ src,
+                                                      double *__restrict__ out)
+{
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ent) return;
+  const int i = (int)(e / (m + 1)), sr = src[e];
+  double v = 0.0;
+  for (int o = 0; o < s; ++o) v += e1[(i + o) * 20 + (rinfo[sr + o] & RI_TYPE_MASK)];
+  out[e] = v;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 #define TF_HIPCHK(expr)                                                                        \
@@ -405,6 +553,189 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
   return AwStatus();
 }
 
+// secondary-structure letters of the n-mer at st from the ssweight helix
+// and strand columns (get_nmer_secondary_structure, fb.cpp:5931-5945)
+static void nm_ss(const AwSystem &S, int st, char *ss)
+{
+  for (int o = 0; o < S.P.nf_size; ++o) {
+    bool e = S.aps[(size_t)(st + o) * 12 + 4] == 1.0, h = S.aps[(size_t)(st + o) * 12 + 3] == 1.0;
+    ss[o] = (e && h) ? '!' : (h ? 'H' : (e ? 'E' : '-'));
+  }
+  ss[S.P.nf_size] = 0;
+}
+
+// one call of the n-mer frustratometer: rows of 7 doubles (i, j, contacts,
+// native, decoy mean, decoy std, index; single n-mers i, 0, 0, ...) and, with
+// the trap flag, trap records of 7 doubles (row, first n-mer, second n-mer,
+// k, backward, threshold, energy; 1-based starts) in the reference's order
+static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double> &table, std::vector<double> &traps)
+{
+  const AwSystem &S = *aw_ctx_system(c);
+  const FFParams &P = S.P;
+  if (!P.nf_flag) return AwStatus::err(AWSEM_E_STATE, "no [Nmer_Frustratometer] section in the coeff file");
+  if (P.nf_ndecoys < 1) return AwStatus::err(AWSEM_E_FORMAT, "Nmer_Frustratometer: no decoys");
+  if (P.nf_size < 1 || P.nf_size > NM_MAX)
+    return AwStatus::err(AWSEM_E_FORMAT, "Nmer_Frustratometer: n-mer size outside 1.." + std::to_string(NM_MAX));
+  AwCtxDev D;
+  AwStatus st = aw_ctx_device(c, xres, D);
+  if (!st.ok()) return st;
+  hipStream_t stream = (hipStream_t)D.stream;
+  const int n = S.n, s = P.nf_size, m = P.nf_ndecoys;
+  AwFrustState &F = aw_ctx_frust(c);
+  DevBuf rho_buf;
+  TF_HIPCHK(hipMalloc(&rho_buf.p, (size_t)n * sizeof(double)));
+  double *d_rho = (double *)rho_buf.p;
+  hipLaunchKernelGGL(k_tf_rho, dim3((n + 255) / 256), dim3(256), 0, stream, D.params, D.rinfo, D.x, n, d_rho);
+  TF_HIPCHK(hipGetLastError());
+  std::vector<double> e;
+  table.clear();
+  traps.clear();
+  auto stats_of = [&](const double *v, double *sd) { stats(v, m, sd); };
+  if (P.nf_mode == 1) {
+    // compute_singlenmer_frust, fb.cpp:5659-5706
+    const int nr = n - s + 1;
+    if (nr <= 0) return AwStatus();
+    std::vector<int> src((size_t)nr * (m + 1));
+    for (int i = 0; i < nr; ++i) {
+      src[(size_t)i * (m + 1)] = i;
+      for (int d = 0; d < m; ++d) {
+        int jr = F.next() % n;
+        while (jr + s > n) jr = F.next() % n;
+        src[(size_t)i * (m + 1) + 1 + d] = jr;
+      }
+    }
+    DevBuf e1, b_src, b_out;
+    TF_HIPCHK(hipMalloc(&e1.p, (size_t)n * 20 * sizeof(double)));
+    hipLaunchKernelGGL(k_nm_single, dim3((n * 20 + 255) / 256), dim3(256), 0, stream, D.params, D.rinfo, D.x, n,
+                       d_rho, (double *)e1.p);
+    TF_HIPCHK(hipGetLastError());
+    if (!(st = b_src.put(src, stream)).ok()) return st;
+    TF_HIPCHK(hipMalloc(&b_out.p, src.size() * sizeof(double)));
+    long long ne = (long long)src.size();
+    hipLaunchKernelGGL(k_nm_single_sum, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, stream, D.rinfo,
+                       (const double *)e1.p, s, m, ne, (const int *)b_src.p, (double *)b_out.p);
+    TF_HIPCHK(hipGetLastError());
+    e.resize(src.size());
+    TF_HIPCHK(hipMemcpyAsync(e.data(), b_out.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    TF_HIPCHK(hipStreamSynchronize(stream));
+    for (int i = 0; i < nr; ++i) {
+      const double *v = &e[(size_t)i * (m + 1)];
+      double sd[2];
+      stats_of(v + 1, sd);
+      double row[7] = {(double)(i + 1), 0.0, 0.0, v[0], sd[0], sd[1], (sd[0] - v[0]) / sd[1]};
+      table.insert(table.end(), row, row + 7);
+    }
+    return AwStatus();
+  }
+  // compute_nmer_frust, fb.cpp:5600-5657
+  const int ns = n - s;
+  if (ns <= 0) return AwStatus();
+  DevBuf b_cnt;
+  TF_HIPCHK(hipMalloc(&b_cnt.p, (size_t)ns * ns * sizeof(int)));
+  long long np = (long long)ns * ns;
+  hipLaunchKernelGGL(k_nm_contacts, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream, D.params, D.rinfo, D.x,
+                     n, s, (int *)b_cnt.p);
+  TF_HIPCHK(hipGetLastError());
+  std::vector<int> cnt((size_t)np);
+  TF_HIPCHK(hipMemcpyAsync(cnt.data(), b_cnt.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+  TF_HIPCHK(hipStreamSynchronize(stream));
+  const int nt = P.nf_trap ? 4 : 0;  // trap entries: (i, j) and (j, i), forward and backward
+  std::vector<NmRow> rows;
+  std::vector<NmEnt> ent;
+  long long off = 0;
+  for (int i = 0; i < ns; ++i)
+    for (int j = i + 1; j < ns; ++j) {
+      int k = cnt[(size_t)i * ns + j];
+      if (!(k >= P.nf_contacts && j - i >= s)) continue;
+      rows.push_back(NmRow{i, j, 1 + m + nt, k, off});
+      ent.push_back(NmEnt{i, j, 1, 1, 0, 0});
+      for (int d = 0; d < m; ++d) {
+        int ir = F.next() % n, jr = F.next() % n;
+        if (ir > jr) std::swap(ir, jr);
+        while (ir + s > n || jr + s > n || jr - ir < s) {
+          ir = F.next() % n;
+          jr = F.next() % n;
+          if (ir > jr) std::swap(ir, jr);
+        }
+        ent.push_back(NmEnt{ir, jr, 1, 1, 0, 0});
+      }
+      if (nt) {
+        // the second n-mer's types from residues o (forward) or s - o
+        // (backward): the reference reads them without k (fb.cpp:5829, 5839)
+        ent.push_back(NmEnt{i, 0, 1, 1, 0, 0});
+        ent.push_back(NmEnt{i, s, 1, -1, 0, 0});
+        ent.push_back(NmEnt{0, j, 1, 1, 1, 0});
+        ent.push_back(NmEnt{s, j, -1, 1, 1, 0});
+      }
+      off += 1 + m + nt;
+    }
+  if (rows.empty()) return AwStatus();
+  DevBuf b_rows, b_ent, b_out;
+  if (!(st = b_rows.put(rows, stream)).ok() || !(st = b_ent.put(ent, stream)).ok()) return st;
+  TF_HIPCHK(hipMalloc(&b_out.p, (size_t)off * sizeof(double)));
+  hipLaunchKernelGGL(k_nm_pairs, dim3((unsigned)rows.size()), dim3(256), 0, stream, D.params, D.rinfo, D.x, n, s,
+                     d_rho, (const NmRow *)b_rows.p, (const NmEnt *)b_ent.p, (double *)b_out.p);
+  TF_HIPCHK(hipGetLastError());
+  e.resize((size_t)off);
+  TF_HIPCHK(hipMemcpyAsync(e.data(), b_out.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  TF_HIPCHK(hipStreamSynchronize(stream));
+  std::vector<char> ssg(s + 1), ssk(s + 1);
+  for (size_t r = 0; r < rows.size(); ++r) {
+    const NmRow &R = rows[r];
+    const double *v = &e[R.off];
+    double sd[2];
+    stats_of(v + 1, sd);
+    if (nt) {
+      // compute_nmer_traps (fb.cpp:5786-5893) for (i, j), then (j, i)
+      const double thr = v[0] - P.nf_trap_sigma * sd[1];
+      for (int dir = 0; dir < 2; ++dir) {
+        const int f = dir ? R.b : R.a, g = dir ? R.a : R.b;
+        nm_ss(S, g, ssg.data());
+        for (int bw = 0; bw < 2; ++bw) {
+          const double et = v[1 + m + 2 * dir + bw];
+          for (int k = 0; k < ns; ++k) {
+            if (std::abs(k - g) <= s || (std::abs(k - f) <= s && f != k)) continue;
+            nm_ss(S, k, ssk.data());
+            int sdist = 0;
+            for (int o = 0; o < s; ++o) sdist += ssg[o] != ssk[o];
+            if (sdist > s * (1 - P.nf_ss_frac)) continue;
+            if (et < thr) {
+              double q[7] = {(double)r, (double)(f + 1), (double)(g + 1), (double)(k + 1), (double)bw, thr, et};
+              traps.insert(traps.end(), q, q + 7);
+            }
+          }
+        }
+      }
+    }
+    double row[7] = {(double)(R.a + 1), (double)(R.b + 1), (double)R.ncont, v[0], sd[0], sd[1], (sd[0] - v[0]) / sd[1]};
+    table.insert(table.end(), row, row + 7);
+  }
+  return AwStatus();
+}
+
+// the generator's state, kept so that a call whose output does not fit
+// leaves the stream where it was
+struct TfSaved {
+  uint32_t rng[34];
+  long long k;
+  bool done;
+  double stats[2];
+  explicit TfSaved(const AwFrustState &F) : k(F.rng_k), done(F.done)
+  {
+    std::memcpy(rng, F.rng, sizeof rng);
+    stats[0] = F.stats[0];
+    stats[1] = F.stats[1];
+  }
+  void restore(AwFrustState &F) const
+  {
+    std::memcpy(F.rng, rng, sizeof rng);
+    F.rng_k = k;
+    F.done = done;
+    F.stats[0] = stats[0];
+    F.stats[1] = stats[1];
+  }
+};
+
 extern "C" {
 
 int awsem_tert_frust(awsem_ctx *c, const double *xres, double *rows, int cap, int *n_rows)
@@ -415,11 +746,15 @@ int awsem_tert_frust(awsem_ctx *c, const double *xres, double *rows, int cap, in
   for (size_t k = 0; k < (size_t)S->n * 9; ++k)
     if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
   std::vector<double> t;
+  const TfSaved saved(aw_ctx_frust(c));
   AwStatus s = tert_frust(c, xres, t);
   if (!s.ok()) return aw_ctx_fail(c, s);
   int nr = (int)(t.size() / 11);
   *n_rows = nr;
-  if (nr > cap) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_ARG, "awsem_tert_frust: more rows than cap"));
+  if (nr > cap) {
+    saved.restore(aw_ctx_frust(c));
+    return aw_ctx_fail(c, AwStatus::err(AWSEM_E_ARG, "awsem_tert_frust: more rows than cap"));
+  }
   if (nr) std::memcpy(rows, t.data(), t.size() * sizeof(double));
   return AWSEM_OK;
 }
@@ -500,6 +835,156 @@ int awsem_tert_frust_every(const awsem_ctx *c, int64_t *every)
   const AwSystem *S = aw_ctx_system(c);
   if (!S) return AWSEM_E_ARG;
   *every = S->P.tf_flag ? (int64_t)S->P.tf_freq : 0;
+  return AWSEM_OK;
+}
+
+int awsem_nmer_frust(awsem_ctx *c, const double *xres, double *rows, int cap, int *n_rows, double *traps, int tcap,
+                     int *n_traps)
+{
+  if (!c || !xres || !n_rows || !n_traps || cap < 0 || tcap < 0 || (cap > 0 && !rows) || (tcap > 0 && !traps))
+    return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S || S->n <= 0) return AWSEM_E_ARG;
+  for (size_t k = 0; k < (size_t)S->n * 9; ++k)
+    if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  std::vector<double> t, tr;
+  const TfSaved saved(aw_ctx_frust(c));
+  AwStatus s = nmer_frust(c, xres, t, tr);
+  if (!s.ok()) return aw_ctx_fail(c, s);
+  int nr = (int)(t.size() / 7), nt = (int)(tr.size() / 7);
+  *n_rows = nr;
+  *n_traps = nt;
+  if (nr > cap || nt > tcap) {
+    saved.restore(aw_ctx_frust(c));
+    return aw_ctx_fail(c, AwStatus::err(AWSEM_E_ARG, "awsem_nmer_frust: more rows or traps than cap"));
+  }
+  if (nr) std::memcpy(rows, t.data(), t.size() * sizeof(double));
+  if (nt) std::memcpy(traps, tr.data(), tr.size() * sizeof(double));
+  return AWSEM_OK;
+}
+
+// the reference's output at step % nmer_frust_output_freq == 0
+// (fb.cpp:7592-7603, 5600-5706, 5855-5890; the files opened and headed at
+// construction, fb.cpp:885-896)
+int awsem_nmer_frust_write(awsem_ctx *c, const double *xres, int64_t step)
+{
+  if (!c || !xres) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S || S->n <= 0) return AWSEM_E_ARG;
+  const FFParams &P = S->P;
+  if (!P.nf_flag) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_STATE, "no [Nmer_Frustratometer] section in the coeff file"));
+  if (P.nf_freq <= 0 || step % P.nf_freq != 0) return AWSEM_OK;
+  AwFrustState &F = aw_ctx_frust(c);
+  const std::string &dir = aw_ctx_dir(c);
+  if (!F.nm_dat) {
+    F.nm_dat = std::fopen((dir + "/nmer_frustration.dat").c_str(), "w");
+    F.nm_tcl = std::fopen((dir + "/nmer_frustration.tcl").c_str(), "w");
+    if (P.nf_trap) F.nm_trap = std::fopen((dir + "/nmer_traps.dat").c_str(), "w");
+    if (!F.nm_dat || !F.nm_tcl || (P.nf_trap && !F.nm_trap))
+      return aw_ctx_fail(c, AwStatus::err(AWSEM_E_FILE, "cannot open nmer_frustration.dat/.tcl or nmer_traps.dat"));
+    if (P.nf_mode == 0)
+      std::fprintf(F.nm_dat, "# i j ncontacts a_i a_j native_energy <decoy_energies> std(decoy_energies) f_ij\n");
+    else
+      std::fprintf(F.nm_dat, "# i a_i native_energy <decoy_energies> std(decoy_energies) f_ij\n");
+    if (P.nf_trap)
+      std::fprintf(F.nm_trap, "# i a_i ss_i j a_j ss_j threshold_energy k a_k ss_k direction trap_energy\n");
+  }
+  for (size_t k = 0; k < (size_t)S->n * 9; ++k)
+    if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  std::vector<double> t, tr;
+  AwStatus s = nmer_frust(c, xres, t, tr);
+  if (!s.ok()) return aw_ctx_fail(c, s);
+  const int ns = P.nf_size;
+  auto seq = [&](int st, int bw) {
+    std::string q(ns, ' ');
+    for (int o = 0; o < ns; ++o) q[o] = S->se[bw ? st + ns - o : st + o];
+    return q;
+  };
+  auto ss = [&](int st) {
+    std::vector<char> b(ns + 1);
+    nm_ss(*S, st, b.data());
+    return std::string(b.data());
+  };
+  FILE *dat = F.nm_dat, *tcl = F.nm_tcl;
+  std::fprintf(dat, "# timestep: %d\n", (int)step);
+  std::fprintf(tcl, "# timestep: %d\n", (int)step);
+  if (P.nf_mode == 1) {
+    std::fprintf(tcl, "mol modselect 0 top \"all\"\n");
+    std::fprintf(tcl, "mol modstyle 0 top newcartoon\n");
+    std::fprintf(tcl, "mol modcolor 0 top colorid 15\n");
+    int atomselect = 0;
+    for (size_t k = 0; k + 7 <= t.size(); k += 7) {
+      const double *q = &t[k];
+      int i = (int)q[0] - 1;
+      std::fprintf(dat, "%d %s %f %f %f %f\n", i + 1, seq(i, 0).c_str(), q[3], q[4], q[5], q[6]);
+      if (q[6] > P.nf_min_thr || q[6] < P.nf_high_thr) {
+        atomselect += 1;
+        std::fprintf(tcl, "mol addrep 0\n");
+        std::fprintf(tcl, "mol modselect %d 0 resid %d to %d\n", atomselect, i + 1, i + ns);
+        std::fprintf(tcl, "mol modstyle %d 0 VDW %f 12.000000\n", atomselect, 0.5 * std::fabs(q[6]));
+        std::fprintf(tcl, "mol modmaterial %d 0 Transparent\n", atomselect);
+        if (q[6] > P.nf_min_thr) std::fprintf(tcl, "mol modcolor %d 0 ColorID 7\n", atomselect);
+        else std::fprintf(tcl, "mol modcolor %d 0 ColorID 1\n", atomselect);
+      }
+    }
+  } else {
+    int atomselect = 0;
+    size_t tk = 0;
+    for (size_t k = 0; k + 7 <= t.size(); k += 7) {
+      const double *q = &t[k];
+      const int row = (int)(k / 7), i = (int)q[0] - 1, j = (int)q[1] - 1;
+      for (; tk + 7 <= tr.size() && (int)tr[tk] == row; tk += 7) {
+        const double *r = &tr[tk];
+        int f = (int)r[1] - 1, g = (int)r[2] - 1, kk = (int)r[3] - 1, bw = (int)r[4];
+        std::fprintf(F.nm_trap, bw ? "%d %s %s %d %s %s %f %d %s <-- %s %f \n" : "%d %s %s %d %s %s %f %d %s --> %s %f \n",
+                     f + 1, seq(f, 0).c_str(), ss(f).c_str(), g + 1, seq(g, 0).c_str(), ss(g).c_str(), r[5], kk + 1,
+                     seq(kk, bw).c_str(), ss(kk).c_str(), r[6]);
+        if (P.nf_draw_trap) {
+          if (f == kk) {
+            std::fprintf(tcl, "mol addrep 0\n");
+            std::fprintf(tcl, "mol modselect %d 0 resid %d to %d\n", F.nm_rep, f + 1, f + ns);
+            std::fprintf(tcl, "mol modcolor %d 0 ColorID 11\n", F.nm_rep);
+            std::fprintf(tcl, "mol modstyle %d 0 NewCartoon 0.350000 10.000000 4.100000 0\n", F.nm_rep);
+            F.nm_rep++;
+          } else {
+            std::fprintf(tcl, "set sel%d [atomselect top \"resid %d and name CA\"]\n", f + ns / 2, f + 1 + ns / 2);
+            std::fprintf(tcl, "set sel%d [atomselect top \"resid %d and name CA\"]\n", kk + ns / 2, kk + 1 + ns / 2);
+            std::fprintf(tcl, "lassign [atomselect%d get {x y z}] pos1\n", atomselect++);
+            std::fprintf(tcl, "lassign [atomselect%d get {x y z}] pos2\n", atomselect++);
+            std::fprintf(tcl, "draw color purple\n");
+            std::fprintf(tcl, bw ? "draw line $pos1 $pos2 style dashed width 1\n" : "draw line $pos1 $pos2 style solid width 1\n");
+          }
+        }
+      }
+      std::fprintf(dat, "%d %d %d %s %s %f %f %f %f\n", i + 1, j + 1, (int)q[2], seq(i, 0).c_str(), seq(j, 0).c_str(),
+                   q[3], q[4], q[5], q[6]);
+      if (q[6] > P.nf_min_thr || q[6] < P.nf_high_thr || P.nf_neutral) {
+        std::fprintf(tcl, "set sel%d [atomselect top \"resid %d and name CA\"]\n", i + ns / 2, i + 1 + ns / 2);
+        std::fprintf(tcl, "set sel%d [atomselect top \"resid %d and name CA\"]\n", j + ns / 2, j + 1 + ns / 2);
+        std::fprintf(tcl, "lassign [atomselect%d get {x y z}] pos1\n", atomselect++);
+        std::fprintf(tcl, "lassign [atomselect%d get {x y z}] pos2\n", atomselect++);
+        if (q[6] > P.nf_min_thr) std::fprintf(tcl, "draw color green\n");
+        else if (q[6] < P.nf_high_thr) std::fprintf(tcl, "draw color red\n");
+        else std::fprintf(tcl, "draw color blue\n");
+        std::fprintf(tcl, "draw line $pos1 $pos2 style solid width 1\n");
+      }
+    }
+    std::fprintf(tcl, "mol modselect 0 top \"all\"\n");
+    std::fprintf(tcl, "mol modstyle 0 top newcartoon\n");
+    std::fprintf(tcl, "mol modcolor 0 top colorid 15\n");
+  }
+  std::fflush(dat);
+  std::fflush(tcl);
+  if (F.nm_trap) std::fflush(F.nm_trap);
+  return AWSEM_OK;
+}
+
+int awsem_nmer_frust_every(const awsem_ctx *c, int64_t *every)
+{
+  if (!c || !every) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S) return AWSEM_E_ARG;
+  *every = S->P.nf_flag ? (int64_t)S->P.nf_freq : 0;
   return AWSEM_OK;
 }
 

@@ -235,6 +235,10 @@ struct AwFrustState {
   bool done = false;
   double stats[2] = {0.0, 0.0};
   FILE *dat = nullptr, *tcl = nullptr;
+  // the n-mer frustratometer's files and the trap drawing's representation
+  // counter (a function-static in the reference, fb.cpp:5802)
+  FILE *nm_dat = nullptr, *nm_tcl = nullptr, *nm_trap = nullptr;
+  int nm_rep = 1;
   AwFrustState();
   ~AwFrustState();
   int next();

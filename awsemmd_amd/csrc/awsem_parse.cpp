@@ -33,8 +33,7 @@ static std::string join(const std::string &dir, const std::string &name)
 }
 
 static const char *kUnsupported[] = {
-    "[Fragment_Frustratometer]",
-    "[Nmer_Frustratometer]", "[Amylometer]", "[Selection_Temperature]",
+    "[Fragment_Frustratometer]", "[Amylometer]", "[Selection_Temperature]",
     "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]",
     "[DebyeHuckel_Optimization]", nullptr};
 
@@ -180,6 +179,16 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
       else if (!in.fail())
         return AwStatus::err(-3, "Only \"configurational\", \"mutational\", \"singleresidue\" are acceptable modes "
                                  "for the Tertiary_Frustratometer.");
+    } else if (sec == "[Nmer_Frustratometer]") {  // fb.cpp:398-412
+      std::string mode;
+      P.nf_flag = 1;
+      in >> P.nf_size >> P.nf_cutoff >> P.nf_contacts >> P.nf_ndecoys >> P.nf_freq;
+      in >> P.nf_min_thr >> P.nf_high_thr >> P.nf_neutral;
+      in >> P.nf_trap >> P.nf_draw_trap >> P.nf_trap_sigma >> P.nf_ss_frac >> mode;
+      if (mode == "pairwise") P.nf_mode = 0;
+      else if (mode == "singlenmer") P.nf_mode = 1;
+      else if (!in.fail())
+        return AwStatus::err(-3, "Only \"pairwise\", \"singlenmer\" are acceptable modes for the Nmer_Frustratometer.");
     } else if (sec == "[DebyeHuckel]") {  // fb.cpp:466-478
       P.dh_flag = 1;
       in >> P.dh_k[0] >> P.dh_k[1] >> P.dh_k[2] >> P.dh_kscreen >> P.dh_length >> P.dh_min_sep;

@@ -98,6 +98,12 @@ struct FFParams {
   // output interval, mode 0 configurational / 1 mutational / 2 single residue
   int tf_flag, tf_ndecoys, tf_freq, tf_mode;
   double tf_cutoff;
+  // [Nmer_Frustratometer] (fb.cpp:398-412): n-mer size, contact cutoff,
+  // contacts needed, decoys, output interval, tcl thresholds and neutral
+  // flag, trap flags, trap width in sigmas, secondary-structure fraction,
+  // mode 0 pairwise / 1 singlenmer
+  int nf_flag, nf_size, nf_contacts, nf_ndecoys, nf_freq, nf_neutral, nf_trap, nf_draw_trap, nf_mode, pad_nf;
+  double nf_cutoff, nf_min_thr, nf_high_thr, nf_trap_sigma, nf_ss_frac;
   // terms whose pairs come from FixBackbone's neighbour list with no cutoff
   // of their own (Debye-Hueckel, the barrier's CA-CA pairs, AMH-Go) count
   // pairs closer than pair_list_cutoff + list_skin (a list LAMMPS has just

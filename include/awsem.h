@@ -388,7 +388,8 @@ int awsem_deck_destroy(awsem_deck *d);
  * contact (i, j, i_chain, j_chain, r_ij, rho_i, rho_j, native energy, decoy
  * mean, decoy std, frustration index; 1-based) or per residue in
  * singleresidue mode (j, j_chain, r_ij and rho_j 0); *n_rows receives the
- * count (AWSEM_E_ARG when it exceeds cap). */
+ * count (AWSEM_E_ARG when it exceeds cap, the rand() stream then left where
+ * it was). */
 int awsem_tert_frust(awsem_ctx *c, const double *xres, double *rows, int cap, int *n_rows);
 /* The reference's output when step % output interval == 0:
  * tertiary_frustration.dat and .tcl in the context's directory (created
@@ -399,6 +400,25 @@ int awsem_tert_frust_write(awsem_ctx *c, const double *xres, int64_t step);
  * line, fb.cpp:385-397), 0 without the section: a driver stops at its
  * multiples to call awsem_tert_frust_write. */
 int awsem_tert_frust_every(const awsem_ctx *c, int64_t *every);
+
+/* The n-mer frustratometer ([Nmer_Frustratometer], fb.cpp:398-412,
+ * 5600-6061) at xres, on the same rand() stream.  rows: 7 doubles per row
+ * (pairwise: i, j, contacts, native energy, decoy mean, decoy std,
+ * frustration index, for the n-mer pairs with enough contacts; singlenmer:
+ * i, 0, 0, ... for every n-mer; 1-based starts).  With the trap flag
+ * (pairwise), traps: 7 doubles per trap (row, first n-mer, second n-mer, k,
+ * backward, threshold energy, trap energy) in the reference's order.
+ * *n_rows / *n_traps receive the counts; AWSEM_E_ARG when either exceeds
+ * its cap, the rand() stream then left where it was. */
+int awsem_nmer_frust(awsem_ctx *c, const double *xres, double *rows, int cap, int *n_rows, double *traps, int tcap,
+                     int *n_traps);
+/* The reference's output when step % output interval == 0:
+ * nmer_frustration.dat, nmer_frustration.tcl and (trap flag) nmer_traps.dat
+ * in the context's directory (fb.cpp:885-896, 7592-7603, 5626-5706,
+ * 5855-5890). */
+int awsem_nmer_frust_write(awsem_ctx *c, const double *xres, int64_t step);
+/* The output interval of the [Nmer_Frustratometer] section, 0 without it. */
+int awsem_nmer_frust_every(const awsem_ctx *c, int64_t *every);
 
 /* ---- order parameters (compute qwolynes, compute qonuchic) ---------------
  * Replace `compute ID alpha_carbons qwolynes rnative.dat sep exp`

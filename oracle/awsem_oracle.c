@@ -142,6 +142,12 @@ struct or_ctx {
      (fb.cpp:5099, 5342); the rand() stream the decoys draw from */
   int tert_frust_flag, tf_ndecoys, tf_freq, tf_mode, tf_done;
   double tf_cutoff, tf_stats[2];
+  /* [Nmer_Frustratometer] (fb.cpp:398-412): n-mer size, contact cutoff,
+     contacts needed, decoys, output interval, the tcl thresholds and
+     neutral flag, the trap flags, trap width in sigmas, secondary-structure
+     fraction, mode 0 pairwise / 1 singlenmer */
+  int nmer_frust_flag, nf_size, nf_contacts, nf_ndecoys, nf_freq, nf_neutral, nf_trap, nf_draw_trap, nf_mode;
+  double nf_cutoff, nf_min_thr, nf_high_thr, nf_trap_sigma, nf_ss_frac;
   uint32_t rng[34];
   int rng_k;
   /* system */
@@ -860,8 +866,24 @@ static int parse_coeff(or_ctx *c, const char *path, char *err, int errlen)
         set_err(err, errlen, "Only \"configurational\", \"mutational\", \"singleresidue\" are acceptable modes for the Tertiary_Frustratometer.");
         return -1;
       }
+    } else if (strcmp(sec, "[Nmer_Frustratometer]") == 0) { /* fb.cpp:398-412 */
+      c->nmer_frust_flag = 1;
+      c->nf_size = tk_int(&in);
+      c->nf_cutoff = tk_double(&in);
+      c->nf_contacts = tk_int(&in);
+      c->nf_ndecoys = tk_int(&in);
+      c->nf_freq = tk_int(&in);
+      c->nf_min_thr = tk_double(&in); c->nf_high_thr = tk_double(&in); c->nf_neutral = tk_int(&in);
+      c->nf_trap = tk_int(&in); c->nf_draw_trap = tk_int(&in); c->nf_trap_sigma = tk_double(&in); c->nf_ss_frac = tk_double(&in);
+      tk_str(&in, sbuf, sizeof sbuf);
+      if (strcmp(sbuf, "pairwise") == 0) c->nf_mode = 0;
+      else if (strcmp(sbuf, "singlenmer") == 0) c->nf_mode = 1;
+      else {
+        free(text);
+        set_err(err, errlen, "Only \"pairwise\", \"singlenmer\" are acceptable modes for the Nmer_Frustratometer.");
+        return -1;
+      }
     } else if (strcmp(sec, "[Fragment_Frustratometer]") == 0 ||
-               strcmp(sec, "[Nmer_Frustratometer]") == 0 ||
                strcmp(sec, "[Amylometer]") == 0 || strcmp(sec, "[Selection_Temperature]") == 0 ||
                strcmp(sec, "[Monte_Carlo_Seq_Opt]") == 0 || strcmp(sec, "[Optimization]") == 0 ||
                strcmp(sec, "[Burial_Optimization]") == 0 ||
@@ -2901,10 +2923,12 @@ static double tf_env(const or_ctx *c, int a, int ar, int at, double rho_a, int x
  * decoy mean, decoy std, frustration index (1-based residue and chain
  * numbers; j = 0 and j_chain = 0 in singleresidue mode).  Returns the rows
  * written, or -needed. */
-int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap)
+/* the densities the frustratometers read (get_residue_density,
+ * fb.cpp:5579-5583: the well's ro, the well-0 sum of cWell::compute_ro,
+ * sml.h:630-650) */
+static double *tf_rho(or_ctx *c)
 {
   int n = c->n;
-  c->xca = xca; c->xcb = xcb; c->xo = xo;
   double *rho = (double *)calloc((size_t)n, sizeof(double));
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
@@ -2915,6 +2939,18 @@ int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double 
       wtheta(c, 0, sqrt(r2), &th, &prd);
       rho[i] += th;
     }
+  return rho;
+}
+
+int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap)
+{
+  int n = c->n;
+  c->xca = xca; c->xcb = xcb; c->xo = xo;
+  double *rho = tf_rho(c);
+  uint32_t rng0[34];
+  int rngk0 = c->rng_k, done0 = c->tf_done;
+  double st0[2] = {c->tf_stats[0], c->tf_stats[1]};
+  memcpy(rng0, c->rng, sizeof rng0);
   int m = c->tf_ndecoys, nr = 0;
   double *dec = (double *)calloc((size_t)(m > 0 ? m : 1), sizeof(double));
   double cut = c->tf_cutoff;
@@ -3003,5 +3039,171 @@ int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double 
 #undef TF_ROW
   free(rho);
   free(dec);
+  if (nr > cap) { /* a call that did not fit leaves the stream where it was */
+    memcpy(c->rng, rng0, sizeof rng0);
+    c->rng_k = rngk0;
+    c->tf_done = done0;
+    c->tf_stats[0] = st0[0];
+    c->tf_stats[1] = st0[1];
+  }
+  return nr <= cap ? nr : -nr;
+}
+
+/* ------------------------------------------------------------------ */
+/* the n-mer frustratometer (fb.cpp:5600-6061)                          */
+
+/* compute_singleresidue_native_ixn with nmercalc (fb.cpp:5346-5393): the
+ * burial of residue i with type it, its water with the residues k < i
+ * within the cutoff (and the contact separation, or another chain), and
+ * its Debye-Hueckel with every k < i */
+static double nf_single(const or_ctx *c, int i, int it, const double *rho)
+{
+  double w = 0.0, b = tf_burial(c, i, it, rho[i]), el = 0.0;
+  for (int k = 0; k < c->n; ++k) {
+    if (k == i || k > i) continue;
+    double r = tf_dist(c, i, k);
+    int kt = restype(c->se[k]);
+    if (r < c->nf_cutoff && (abs(i - k) >= c->contact_cutoff || c->chain_no[i] != c->chain_no[k]))
+      w += tf_water(c, r, i, k, it, kt, rho[i], rho[k]);
+    if (c->huckel_flag) el += tf_elec(c, r, i, k, it, kt);
+  }
+  return w + b + el;
+}
+
+/* the energy of n-mers a (types from residues sa + da*o) and b (types
+ * from sb + db*o): their burials, then the water of every (a_o, b_p)
+ * pair, a outer (compute_nmer_native_ixn, fb.cpp:5947-5988; the decoys
+ * 6028-6056 and the traps 5822-5853 with other type sources) */
+static double nf_pair(const or_ctx *c, int a, int sa, int da, int b, int sb, int db, const double *rho)
+{
+  int s = c->nf_size;
+  double e = 0.0;
+  for (int o = 0; o < s; ++o) e += tf_burial(c, a + o, restype(c->se[sa + da * o]), rho[a + o]);
+  for (int o = 0; o < s; ++o) e += tf_burial(c, b + o, restype(c->se[sb + db * o]), rho[b + o]);
+  for (int o = 0; o < s; ++o) {
+    int at = restype(c->se[sa + da * o]);
+    for (int p = 0; p < s; ++p) {
+      int bt = restype(c->se[sb + db * p]);
+      e += tf_water(c, tf_dist(c, a + o, b + p), a + o, b + p, at, bt, rho[a + o], rho[b + p]);
+    }
+  }
+  return e;
+}
+
+/* secondary-structure letters of an n-mer from the ssweight columns
+ * (get_nmer_secondary_structure, fb.cpp:5931-5945) */
+static void nf_ss(const or_ctx *c, int st, char *ss)
+{
+  for (int o = 0; o < c->nf_size; ++o) {
+    int e = c->aps[4][st + o] == 1.0, h = c->aps[3][st + o] == 1.0;
+    ss[o] = (e && h) ? '!' : (h ? 'H' : (e ? 'E' : '-'));
+  }
+}
+
+/* one call at the coordinates given.  Rows of 7 doubles: pairwise i, j
+ * (1-based starts), contacts, native, decoy mean, decoy std, frustration
+ * index; singlenmer i, 0, 0, native, mean, std, index.  With the trap flag
+ * (pairwise), trap records of 7 doubles in the reference's order (per row,
+ * compute_nmer_traps(i, j) then (j, i), fb.cpp:5620-5623): row, first and
+ * second n-mer (1-based), k (1-based), backward, threshold, trap energy.
+ * Returns the rows (or -needed); *ntr the trap records (or -needed). */
+int or_nmer_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap,
+                  double *traps, int tcap, int *ntr)
+{
+  int n = c->n, s = c->nf_size, m = c->nf_ndecoys, nr = 0, nt = 0;
+  c->xca = xca; c->xcb = xcb; c->xo = xo;
+  double *rho = tf_rho(c);
+  uint32_t rng0[34];
+  int rngk0 = c->rng_k;
+  memcpy(rng0, c->rng, sizeof rng0);
+  double *dec = (double *)calloc((size_t)(m > 0 ? m : 1), sizeof(double));
+  char *ssf = (char *)calloc((size_t)s + 1, 1), *ssk = (char *)calloc((size_t)s + 1, 1);
+#define NF_ROW(a0, a1, a2, a3, a4, a5, a6)                                                   \
+  do {                                                                                     \
+    if (nr < cap) {                                                                        \
+      double *q = rows + 7 * (size_t)nr;                                                   \
+      q[0] = a0; q[1] = a1; q[2] = a2; q[3] = a3; q[4] = a4; q[5] = a5; q[6] = a6;         \
+    }                                                                                      \
+    nr++;                                                                                  \
+  } while (0)
+  if (c->nf_mode == 1) {
+    /* compute_singlenmer_frust, fb.cpp:5659-5706 */
+    for (int i = 0; i < n - s + 1; ++i) {
+      double e = 0.0;
+      for (int j = i; j < i + s; ++j) e += nf_single(c, j, restype(c->se[j]), rho);
+      for (int d = 0; d < m; ++d) {
+        int jr = or_rng_next(c) % n;
+        while (jr + s > n) jr = or_rng_next(c) % n;
+        double ed = 0.0;
+        for (int j = i; j < i + s; ++j) ed += nf_single(c, j, restype(c->se[jr + j - i]), rho);
+        dec[d] = ed;
+      }
+      double st[2];
+      tf_stats(dec, m, st);
+      NF_ROW(i + 1, 0, 0, e, st[0], st[1], (st[0] - e) / st[1]);
+    }
+  } else {
+    /* compute_nmer_frust, fb.cpp:5600-5657 */
+    for (int i = 0; i < n - s; ++i)
+      for (int j = i + 1; j < n - s; ++j) {
+        int cnt = 0; /* compute_nmer_contacts, fb.cpp:5895-5919 */
+        for (int a = i; a < i + s; ++a)
+          for (int b = j; b < j + s; ++b)
+            if (abs(a - b) >= c->contact_cutoff && tf_dist(c, a, b) < c->nf_cutoff) cnt++;
+        if (!(cnt >= c->nf_contacts && j - i >= s)) continue;
+        double e = nf_pair(c, i, i, 1, j, j, 1, rho);
+        for (int d = 0; d < m; ++d) { /* compute_nmer_decoy_ixns, fb.cpp:5990-6060 */
+          int ir = or_rng_next(c) % n, jr = or_rng_next(c) % n;
+          if (ir > jr) { int t = ir; ir = jr; jr = t; }
+          while (ir + s > n || jr + s > n || jr - ir < s) {
+            ir = or_rng_next(c) % n;
+            jr = or_rng_next(c) % n;
+            if (ir > jr) { int t = ir; ir = jr; jr = t; }
+          }
+          dec[d] = nf_pair(c, i, ir, 1, j, jr, 1, rho);
+        }
+        double st[2];
+        tf_stats(dec, m, st);
+        if (c->nf_trap) {
+          /* compute_nmer_traps (fb.cpp:5786-5893) for (i, j) and (j, i):
+             the second n-mer's types come from residues o (forward) or
+             s - o (backward) -- not from k, so the energy is k's for every
+             k that passes the overlap and secondary-structure tests */
+          double thr = e - c->nf_trap_sigma * st[1];
+          for (int dir = 0; dir < 2; ++dir) {
+            int f = dir ? j : i, g = dir ? i : j;
+            nf_ss(c, g, ssf);
+            for (int bw = 0; bw < 2; ++bw) {
+              double et = nf_pair(c, f, f, 1, g, bw ? s : 0, bw ? -1 : 1, rho);
+              for (int k = 0; k < n - s; ++k) {
+                if (abs(k - g) <= s || (abs(k - f) <= s && f != k)) continue;
+                nf_ss(c, k, ssk);
+                int sd = 0;
+                for (int o = 0; o < s; ++o) sd += ssf[o] != ssk[o];
+                if (sd > s * (1 - c->nf_ss_frac)) continue;
+                if (et < thr) {
+                  if (nt < tcap) {
+                    double *q = traps + 7 * (size_t)nt;
+                    q[0] = nr; q[1] = f + 1; q[2] = g + 1; q[3] = k + 1; q[4] = bw; q[5] = thr; q[6] = et;
+                  }
+                  nt++;
+                }
+              }
+            }
+          }
+        }
+        NF_ROW(i + 1, j + 1, cnt, e, st[0], st[1], (st[0] - e) / st[1]);
+      }
+  }
+#undef NF_ROW
+  free(rho);
+  free(dec);
+  free(ssf);
+  free(ssk);
+  if (nr > cap || nt > tcap) { /* a call that did not fit leaves the stream where it was */
+    memcpy(c->rng, rng0, sizeof rng0);
+    c->rng_k = rngk0;
+  }
+  *ntr = nt <= tcap ? nt : -nt;
   return nr <= cap ? nr : -nr;
 }

@@ -156,6 +156,13 @@ double or_q_wolynes(int n, const double *xca, const double *rn, int sep, double 
  * creation), which continues across calls, as do the configurational decoy
  * statistics (computed once, fb.cpp:5099, 5342). */
 int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap);
+/* the n-mer frustratometer ([Nmer_Frustratometer], fb.cpp:5600-6061) on
+ * the same rand() stream: rows of 7 doubles (i, j, contacts, native, decoy
+ * mean, decoy std, index; singlenmer: i, 0, 0, ...) and, with the trap
+ * flag, trap records of 7 doubles (row, first, second, k, backward,
+ * threshold, energy); returns the rows (-needed beyond cap) */
+int or_nmer_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap,
+                  double *traps, int tcap, int *ntr);
 void or_rng_seed(or_ctx *c, unsigned seed);
 int or_rng_next(or_ctx *c);
 double or_q_onuchic(int n, const double *xca, int type, const double *nat, double rc, int nmap, const int *mi,

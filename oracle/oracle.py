@@ -72,6 +72,8 @@ def lib():
         L.or_rng_next.restype = C.c_int
         L.or_tert_frust.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_int]
         L.or_tert_frust.restype = C.c_int
+        L.or_nmer_frust.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_int, dp, C.c_int, C.POINTER(C.c_int)]
+        L.or_nmer_frust.restype = C.c_int
         L.or_q_wolynes.argtypes = [C.c_int, dp, dp, C.c_int, C.c_double]
         L.or_q_wolynes.restype = C.c_double
         L.or_q_onuchic.argtypes = [C.c_int, dp, C.c_int, dp, C.c_double, C.c_int, ip, ip, dp, C.c_double]
@@ -168,6 +170,21 @@ class Oracle:
             if k >= 0:
                 return out[:11 * k].reshape(k, 11)
             cap = -k + 16
+
+    def nmer_frust(self, coords: np.ndarray):
+        """One n-mer-frustratometer call: (rows, traps); rows (i, j,
+        contacts, native, decoy mean, decoy std, index), trap records (row,
+        first, second, k, backward, threshold, energy)."""
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        cap, tcap = 4096, 4096
+        while True:
+            out, tr = np.zeros(7 * cap), np.zeros(7 * tcap)
+            nt = C.c_int()
+            k = lib().or_nmer_frust(self.h, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]),
+                                    np.ascontiguousarray(x[2]), out, cap, tr, tcap, C.byref(nt))
+            if k >= 0 and nt.value >= 0:
+                return out[:7 * k].reshape(k, 7), tr[:7 * nt.value].reshape(nt.value, 7)
+            cap, tcap = max(cap, -k + 16), max(tcap, -nt.value + 16)
 
     def fm_table_dims(self):
         """(grid points per curve, neighbours per residue) of fragment-memory

@@ -187,16 +187,20 @@ def test_deck_1r69_shipped_system(oracle_mod):
         check_energies([float(t) for t in w[1:]], ref[int(w[0])][2], 1e-6)
 
 
-def test_deck_tertiary_frustratometer(oracle_mod):
+def test_deck_frustratometers(oracle_mod):
     """A deck whose coeff file enables the tertiary frustratometer
-    (mutational, 80 decoys, every 5 steps): the run stops at its output
-    steps and writes tertiary_frustration.dat blocks at 0, 5, ..., 20 whose
-    rows equal the oracle's frustratometer at the oracle's trajectory (the
-    rand() stream continued across the blocks) to the printed precision."""
+    (mutational, 80 decoys, every 5 steps) and the n-mer frustratometer
+    (single 4-mers, 30 decoys, every 10 steps): the run stops at their output
+    steps and writes tertiary_frustration.dat blocks at 0, 5, ..., 20 and
+    nmer_frustration.dat blocks at 0, 10, 20 whose rows equal the oracle's
+    at the oracle's trajectory (one rand() stream, the tertiary call first
+    at a shared step, continued across the blocks) to the printed
+    precision."""
     A = _A()
     s, path, d = deckgen.make_deck([40, 30], 54, compact=True, runs=(20,), thermo=10, dump=20, print_every=10)
     with open(os.path.join(d, "fix_backbone_coeff.data"), "a") as f:
         f.write("\n[Tertiary_Frustratometer]\n9.5\n80\n5\nmutational\n\n")
+        f.write("\n[Nmer_Frustratometer]\n4\n9.5\n1\n30\n10\n0.78 -1 0\n0 0 1 0.6\nsinglenmer\n\n")
     dk = A.Deck(path, A.PREC_FP64)
     x0, v0 = dk.residue_state()
     o = deckgen.deck_oracle(oracle_mod, d)
@@ -204,13 +208,28 @@ def test_deck_tertiary_frustratometer(oracle_mod):
     dk.run()
     x, _ = dk.residue_state()
     assert np.abs(x - ref[20][0]).max() < 1e-8
+    tert, nmer = {}, {}
+    for st in (0, 5, 10, 15, 20):
+        tert[st] = o.tert_frust(ref[st][0])
+        if st % 10 == 0:
+            nmer[st] = o.nmer_frust(ref[st][0])[0]
+    nd = open(os.path.join(d, "nmer_frustration.dat")).read().splitlines()
+    nb = [k for k, ln in enumerate(nd) if ln.startswith("# timestep:")]
+    assert [nd[k] for k in nb] == ["# timestep: %d" % k for k in (0, 10, 20)]
+    for b in nb:
+        st = int(nd[b].split()[-1])
+        rows = [ln.split() for ln in nd[b + 1:b + 1 + len(nmer[st])]]
+        assert len(rows) == len(nmer[st]) > 10
+        for r, q in zip(rows, nmer[st]):
+            assert int(r[0]) == int(q[0])
+            assert max(abs(float(a) - c) for a, c in zip(r[2:6], q[3:7])) <= 2e-6, (st, r, q)
     dat = open(os.path.join(d, "tertiary_frustration.dat")).read().splitlines()
     blocks = [k for k, ln in enumerate(dat) if ln.startswith("# timestep:")]
     assert [dat[k] for k in blocks] == ["# timestep: %d" % k for k in (0, 5, 10, 15, 20)]
     blocks.append(len(dat))
     for b, b1 in zip(blocks[:-1], blocks[1:]):
         st = int(dat[b].split()[-1])
-        want = o.tert_frust(ref[st][0])
+        want = tert[st]
         rows = [ln.split() for ln in dat[b + 1:b1]]
         assert len(rows) == len(want) > 10, st
         for r, q in zip(rows, want):

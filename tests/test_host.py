@@ -54,12 +54,12 @@ def test_error_codes(tmp_path):
     with pytest.raises(A.AwsemError) as e:
         A.FixBackbone(str(tmp_path / "nope.data"), p["seq"], p["workdir"])
     assert e.value.code == -2
-    txt = open(p["coeff"]).read().replace("[Nmer_Frustratometer]-", "[Nmer_Frustratometer]")
-    bad = tmp_path / "coeff_nmer"
+    txt = open(p["coeff"]).read().replace("[Amylometer]-", "[Amylometer]", 1)
+    bad = tmp_path / "coeff_amylometer"
     bad.write_text(txt)
     with pytest.raises(A.AwsemError) as e:
         A.FixBackbone(str(bad), p["seq"], p["workdir"])
-    assert e.value.code == -4 and "Nmer_Frustratometer" in str(e.value)
+    assert e.value.code == -4 and "Amylometer" in str(e.value)
     # gamma.dat truncated
     d2 = tmp_path / "w2"
     d2.mkdir()
