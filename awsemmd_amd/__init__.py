@@ -49,7 +49,8 @@ EXPORTS = ["awsem_create", "awsem_create_arrays", "awsem_set_topology", "awsem_c
            "awsem_deck_residue_atoms", "awsem_deck_variable", "awsem_deck_last_error", "awsem_deck_destroy",
            "awsem_q_wolynes_create", "awsem_q_onuchic_create", "awsem_q_compute", "awsem_batch_q",
            "awsem_q_last_error", "awsem_q_destroy", "awsem_tert_frust", "awsem_tert_frust_write",
-           "awsem_tert_frust_every", "awsem_nmer_frust", "awsem_nmer_frust_write", "awsem_nmer_frust_every"]
+           "awsem_tert_frust_every", "awsem_nmer_frust", "awsem_nmer_frust_write", "awsem_nmer_frust_every",
+           "awsem_frag_frust", "awsem_frag_frust_write", "awsem_frag_frust_info"]
 
 
 class AwsemError(RuntimeError):
@@ -157,6 +158,9 @@ def lib():
         L.awsem_nmer_frust.argtypes = [vp, dp, dp, C.c_int, C.POINTER(C.c_int), dp, C.c_int, C.POINTER(C.c_int)]
         L.awsem_nmer_frust_write.argtypes = [vp, dp, C.c_int64]
         L.awsem_nmer_frust_every.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.awsem_frag_frust.argtypes = [vp, dp, dp, dp, ip]
+        L.awsem_frag_frust_write.argtypes = [vp, dp, C.c_int64]
+        L.awsem_frag_frust_info.argtypes = [vp, ip, ip, ip, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -286,6 +290,31 @@ class FixBackbone:
         k = C.c_int64()
         self._check(lib().awsem_nmer_frust_every(self.h, C.byref(k)))
         return k.value
+
+    def frag_frust_info(self):
+        """(mode 0 shuffle / 1 read / -1 none, decoy calculations, decoys,
+        output interval) of the [Fragment_Frustratometer] section."""
+        m, nc, nd, ev = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        self._check(lib().awsem_frag_frust_info(self.h, C.byref(m), C.byref(nc), C.byref(nd), C.byref(ev)))
+        return m.value, nc.value, nd.value, ev.value
+
+    def frag_frust(self, xres: np.ndarray):
+        """One fragment-frustratometer calculation at xres (3, n, 3): (rows,
+        gen); rows n x 3 (index, native - mean, std per residue); gen (read
+        mode, at the call that computes the decoy statistics) (natives n,
+        decoy energies n x n_decoys), else None."""
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        _, _, nd, _ = self.frag_frust_info()
+        rows = np.zeros(3 * self.n)
+        gen = np.zeros(self.n * (nd + 1))
+        w = C.c_int()
+        self._check(lib().awsem_frag_frust(self.h, _dp(x), _dp(rows), _dp(gen), C.byref(w)))
+        g = (gen[:self.n].copy(), gen[self.n:].reshape(self.n, nd)) if w.value else None
+        return rows.reshape(self.n, 3), g
+
+    def frag_frust_write(self, xres: np.ndarray, step: int):
+        x = np.ascontiguousarray(xres, dtype=np.float64)
+        self._check(lib().awsem_frag_frust_write(self.h, _dp(x), step))
 
     def tert_frust_every(self) -> int:
         """Output interval of the [Tertiary_Frustratometer] section (0: none)."""

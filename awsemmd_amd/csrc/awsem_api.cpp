@@ -414,8 +414,8 @@ static AwStatus fm_table_setup(AwSystem &S, const std::string &dir)
 static AwStatus load_system(const std::string &coeff, const std::string &seq_path, const std::string &dir, AwSystem &S,
                             bool from_files)
 {
-  std::string mem_file, gamma_file, cr_file;
-  AwStatus s = aw_parse_coeff(coeff, S.P, mem_file, gamma_file, cr_file);
+  std::string mem_file, gamma_file, cr_file, ff_file;
+  AwStatus s = aw_parse_coeff(coeff, S.P, mem_file, gamma_file, cr_file, ff_file);
   if (!s.ok()) return s;
   if (from_files) {
     if (!(s = aw_read_sequence(seq_path, S)).ok()) return s;
@@ -432,6 +432,7 @@ static AwStatus load_system(const std::string &coeff, const std::string &seq_pat
       return s;
   }
   S.cr_file = cr_file;
+  S.ff_file = ff_file;
   return AwStatus();
 }
 
@@ -447,6 +448,7 @@ int awsem_create(const char *coeff_path, const char *seq_path, const char *workd
   c->dir = dir;
   AwStatus s = load_system(coeff_path, seq_path, dir, c->sys, true);
   if (s.ok()) s = aw_read_extras(dir, c->sys.cr_file, c->sys);
+  if (s.ok()) s = aw_read_decoy_mems(dir, c->sys);
   if (s.ok()) s = aw_finish_system(c->sys);
   if (s.ok()) s = fm_table_setup(c->sys, dir);
   if (!s.ok()) {
@@ -526,6 +528,7 @@ int awsem_create_arrays(const char *coeff_path, const char *param_dir, int nch, 
     }
   }
   if (s.ok()) s = aw_read_extras(param_dir ? param_dir : ".", S.cr_file, S);
+  if (s.ok()) s = aw_read_decoy_mems(param_dir ? param_dir : ".", S);
   if (s.ok()) s = aw_finish_system(S);
   if (s.ok()) s = fm_table_setup(S, param_dir ? param_dir : ".");
   *out = c.release();

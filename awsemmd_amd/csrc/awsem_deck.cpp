@@ -173,7 +173,7 @@ struct awsem_deck {
   // fixes
   std::string fb_id, fb_group, fb_beta, fb_oxy, fb_coeff, fb_seq;
   bool have_nve = false, have_langevin = false;
-  long long tf_every = 0, nf_every = 0;  // the frustratometers' output intervals (0: none)
+  long long tf_every = 0, nf_every = 0, ff_every = 0;  // the frustratometers' output intervals (0: none)
   double lg_t0 = 0.0, lg_t1 = 0.0, lg_damp = 0.0;
   long long lg_seed = 0;
   std::vector<Dump> dumps;
@@ -1209,6 +1209,11 @@ struct awsem_deck {
     tf_every = (long long)tfe;
     ctx_check(awsem_nmer_frust_every(ctx, &tfe));
     nf_every = (long long)tfe;
+    {
+      int fm = 0, fc = 0, fd = 0;
+      ctx_check(awsem_frag_frust_info(ctx, &fm, &fc, &fd, &tfe));
+      ff_every = (long long)tfe;
+    }
     outputs(s0, true, n == 0);
     double T0 = have_langevin ? lg_t0 : 0.0, T1 = have_langevin ? lg_t1 : 0.0;
     // without fix langevin the thermostat is off: an infinite damping time
@@ -1222,6 +1227,7 @@ struct awsem_deck {
       upd(thermo_every);
       upd(tf_every);
       upd(nf_every);
+      upd(ff_every);
       for (const Dump &d : dumps) upd(d.every);
       for (const PrintFix &p : prints) upd(p.every);
       // fix langevin ramps T over the run (delta = elapsed / run length);
@@ -1248,12 +1254,14 @@ struct awsem_deck {
     // FixBackbone's energy.log: every evaluation on a thermo step
     if (elog_open && thermo_every > 0 && s % thermo_every == 0) ctx_check(awsem_energy_log_write(ctx, s, energy));
     if ((thermo_every > 0 && s % thermo_every == 0) || setup || last) thermo_line(s, setup);
-    // FixBackbone's tertiary and n-mer frustratometers, in the reference's
-    // order: every evaluation on a multiple of their output intervals
-    // (fb.cpp:7580-7603), the setup one included
-    if ((tf_every > 0 && s % tf_every == 0) || (nf_every > 0 && s % nf_every == 0)) {
+    // FixBackbone's fragment, tertiary and n-mer frustratometers, in the
+    // reference's order: every evaluation on a multiple of their output
+    // intervals (fb.cpp:7537-7603), the setup one included
+    auto due = [&](long long every) { return every > 0 && s % every == 0; };
+    if (due(ff_every) || due(tf_every) || due(nf_every)) {
       std::vector<double> x, v;
       to_res(x, v);
+      if (due(ff_every)) ctx_check(awsem_frag_frust_write(ctx, x.data(), s));
       if (tf_every > 0 && s % tf_every == 0) ctx_check(awsem_tert_frust_write(ctx, x.data(), s));
       if (nf_every > 0 && s % nf_every == 0) ctx_check(awsem_nmer_frust_write(ctx, x.data(), s));
     }

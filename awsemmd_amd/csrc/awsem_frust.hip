@@ -43,6 +43,8 @@ AwFrustState::~AwFrustState()
   if (nm_dat) std::fclose(nm_dat);
   if (nm_tcl) std::fclose(nm_tcl);
   if (nm_trap) std::fclose(nm_trap);
+  for (FILE *f : ff_out)
+    if (f) std::fclose(f);
 }
 
 void AwFrustState::seed(unsigned s)
@@ -390,6 +392,105 @@ __global__ void __launch_bounds__(256) k_nm_single_sum(const int *__restrict__ r
 }
 
 // ---------------------------------------------------------------------------
+// the fragment frustratometer (fb.cpp:4431-4832)
+
+// one (memory, relative pair) of a library: the pair (di, dj) of the
+// memory's residues that compute_decoy_memory_potential visits (residue
+// pos + di of the map, fb.cpp:4478-4496, partner pos + dj), the memory's four
+// reference distances (CA-CA, CA(i)-CB(j), CB(i)-CA(j), CB-CB, fm.cpp:190-202)
+// and its glycine flags (bit 0 at di, bit 1 at dj)
+struct FfItem {
+  int k, di, dj, gly;
+  double rf[4];
+};
+
+// the Gaussians of one item: target residues i, j with distance r[q] per
+// atom pair q, added to E[i] and E[j] (fb.cpp:4497-4561)
+__device__ __forceinline__ void ff_pair(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                        const double *__restrict__ gsep, const FfItem &I, double w, int i, int j,
+                                        const double r[4], double *__restrict__ E)
+{
+  const int ti = rinfo[i] & RI_TYPE_MASK, tj = rinfo[j] & RI_TYPE_MASK, d = j - i;
+  double g = gsep[(size_t)d * 400 + ti * 20 + tj];
+  if (P->ff_seqsep) g *= pow((double)(d - P->fm_min_sep + 1), -P->ff_seqsep_gamma);
+  const double ekwg = P->epsilon * P->k_frag_mem * w * g;
+  double s2 = pow((double)d, 2 * P->fm_sigma_exp);
+  s2 = s2 * P->ff_ww * P->ff_ww;
+  const bool gi = (rinfo[i] & RI_GLY) || (I.gly & 1), gj = (rinfo[j] & RI_GLY) || (I.gly & 2);
+  for (int q = 0; q < 4; ++q) {
+    if ((q >= 2 && gi) || ((q & 1) && gj)) continue;
+    double dr = r[q] - I.rf[q];
+    double V = -ekwg * exp(-(dr * dr) / (2 * s2));
+    atomicAdd(&E[i], V);
+    atomicAdd(&E[j], V);
+  }
+}
+
+__device__ __forceinline__ double ff_r(const double *a, const double *b)
+{
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// a library at the current coordinates, one calculation per blockIdx.y
+// with the memories at pos[calc * nm + k]: E[calc * n + i]
+// (compute_decoy_memory_potential, fb.cpp:4431-4563)
+__global__ void __launch_bounds__(256) k_ff_mem(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                               const double *__restrict__ x, int n, const FfItem *__restrict__ items,
+                                               int n_items, const int *__restrict__ pos, const double *__restrict__ w,
+                                               int nm, const double *__restrict__ gsep, double *__restrict__ E,
+                                               int *__restrict__ err)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_items) return;
+  const int c = blockIdx.y;
+  const FfItem I = items[t];
+  const int p = pos[(size_t)c * nm + I.k], i = p + I.di, j = p + I.dj;
+  if ((rinfo[i] >> RI_CHAIN_SHIFT) != (rinfo[j] >> RI_CHAIN_SHIFT)) {
+    atomicOr(err, 1);
+    return;
+  }
+  const double *ci = x + (size_t)i * 3, *cj = x + (size_t)j * 3;
+  const double *bi = x + ((size_t)n + i) * 3, *bj = x + ((size_t)n + j) * 3;
+  const double r[4] = {ff_r(ci, cj), ff_r(ci, bj), ff_r(bi, cj), ff_r(bi, bj)};
+  ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r, E + (size_t)c * n);
+}
+
+// read mode's decoy energies: the memories at their positions compared with
+// decoy structure blockIdx.y (compute_generated_decoy_energies,
+// fb.cpp:4691-4784): the decoy's distance where it covers both residues,
+// else 0 (its Rf returns 0 there, unchecked)
+__global__ void __launch_bounds__(256) k_ff_gen(const FFParams *__restrict__ P, const int *__restrict__ rinfo, int n,
+                                               const FfItem *__restrict__ items, int n_items,
+                                               const int *__restrict__ pos, const double *__restrict__ w,
+                                               const double *__restrict__ gsep, const double *__restrict__ dxyz,
+                                               const int *__restrict__ dinfo, double *__restrict__ G,
+                                               int *__restrict__ err)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_items) return;
+  const int d = blockIdx.y;
+  const FfItem I = items[t];
+  const int p = pos[I.k], i = p + I.di, j = p + I.dj;
+  if ((rinfo[i] >> RI_CHAIN_SHIFT) != (rinfo[j] >> RI_CHAIN_SHIFT)) {
+    atomicOr(err, 1);
+    return;
+  }
+  // decoy d: first residue dpos, length dlen, its CA then CB coordinates at dxyz + off
+  const int dpos = dinfo[3 * d], dlen = dinfo[3 * d + 1], off = dinfo[3 * d + 2];
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  const int a = i - dpos, b = j - dpos;
+  if (a >= 0 && a < dlen && b >= 0 && b < dlen) {
+    const double *ca = dxyz + off, *cb = dxyz + off + 3 * dlen;
+    r[0] = ff_r(ca + 3 * a, ca + 3 * b);
+    r[1] = ff_r(ca + 3 * a, cb + 3 * b);
+    r[2] = ff_r(cb + 3 * a, ca + 3 * b);
+    r[3] = ff_r(cb + 3 * a, cb + 3 * b);
+  }
+  ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r, G + (size_t)d * n);
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 #define TF_HIPCHK(expr)                                                                        \
@@ -713,6 +814,191 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
   return AwStatus();
 }
 
+// the (memory, relative pair) items of a library (the pairs of
+// compute_decoy_memory_potential, fb.cpp:4478-4496: residue di from 0 to
+// len - minSep - 1, partner dj from di + minSep to len - 1, at most
+// di + maxSep); memory distances as Fragment_Memory::Rf (fm.cpp:146-163)
+static void ff_items(const AwSystem &S, const std::vector<AwMemory> &lib, std::vector<FfItem> &out)
+{
+  const int minsep = S.fmg.min_sep(), maxsep = S.fmg.max_sep();
+  auto R = [](const double *a, const double *b) {
+    double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  out.clear();
+  for (int k = 0; k < (int)lib.size(); ++k) {
+    const AwMemory &m = lib[k];
+    for (int di = 0; di < m.len - minsep; ++di) {
+      int je = m.len - 1;
+      if (maxsep != -1) je = std::min(je, di + maxsep);
+      for (int dj = di + minsep; dj <= je; ++dj) {
+        FfItem I;
+        I.k = k;
+        I.di = di;
+        I.dj = dj;
+        I.gly = (m.se[di] == 'G' ? 1 : 0) | (m.se[dj] == 'G' ? 2 : 0);
+        const double *ai = &m.ca[3 * di], *aj = &m.ca[3 * dj], *bi = &m.cb[3 * di], *bj = &m.cb[3 * dj];
+        I.rf[0] = R(ai, aj);
+        I.rf[1] = R(ai, bj);
+        I.rf[2] = R(aj, bi);
+        I.rf[3] = R(bj, bi);
+        out.push_back(I);
+      }
+    }
+  }
+}
+
+// one fragment-frustratometer calculation (fb.cpp:7537-7576): rows n x 3
+// (index, native - mean, std per residue); in read mode, at the call that
+// computes the decoy statistics, gen = n natives then n x n_decoys decoy
+// energies (residue-major) and *gen_written = 1
+static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double> &rows, std::vector<double> *gen,
+                           int *gen_written)
+{
+  const AwSystem &S = *aw_ctx_system(c);
+  const FFParams &P = S.P;
+  *gen_written = 0;
+  if (!P.ff_flag) return AwStatus::err(AWSEM_E_STATE, "no [Fragment_Frustratometer] section in the coeff file");
+  if (P.ff_ncalc - 1 > 65535 || (int)S.ff_mems.size() > 65535)
+    return AwStatus::err(AWSEM_E_FORMAT, "Fragment_Frustratometer: more than 65535 decoy calculations or structures");
+  AwCtxDev D;
+  AwStatus st = aw_ctx_device(c, xres, D);
+  if (!st.ok()) return st;
+  hipStream_t stream = (hipStream_t)D.stream;
+  const int n = S.n, nc = P.ff_ncalc, nfm = (int)S.mems.size(), nd = (int)S.ff_mems.size();
+  AwFrustState &F = aw_ctx_frust(c);
+  if (F.ff_pos.empty())
+    for (const AwMemory &m : S.ff_mems) F.ff_pos.push_back(m.pos);
+  // gamma by separation and types (Gamma_Array::getGamma, fm.cpp:400-424)
+  int dmax = 0;
+  for (const AwMemory &m : S.mems) dmax = std::max(dmax, m.len);
+  for (const AwMemory &m : S.ff_mems) dmax = std::max(dmax, m.len);
+  std::vector<double> gsep((size_t)(dmax + 1) * 400);
+  for (int d = 0; d <= dmax; ++d)
+    for (int a = 0; a < 20; ++a)
+      for (int b = 0; b < 20; ++b) gsep[(size_t)d * 400 + a * 20 + b] = S.fmg.get(a, b, 0, d);
+  std::vector<FfItem> nat_items, dec_items;
+  ff_items(S, S.mems, nat_items);
+  std::vector<int> nat_pos(nfm);
+  std::vector<double> nat_w(nfm), dec_w(std::max(nd, 1));
+  for (int k = 0; k < nfm; ++k) { nat_pos[k] = S.mems[k].pos; nat_w[k] = S.mems[k].weight; }
+  for (int k = 0; k < nd; ++k) dec_w[k] = S.ff_mems[k].weight;
+  DevBuf b_g, b_err, b_ni, b_np, b_nw, b_E;
+  std::vector<int> zero(1, 0);
+  if (!(st = b_g.put(gsep, stream)).ok() || !(st = b_err.put(zero, stream)).ok() || !(st = b_ni.put(nat_items, stream)).ok() ||
+      !(st = b_np.put(nat_pos, stream)).ok() || !(st = b_nw.put(nat_w, stream)).ok())
+    return st;
+  const int ncalc_dev = P.ff_mode == 0 ? nc : 1;  // read mode: the natives only
+  TF_HIPCHK(hipMalloc(&b_E.p, (size_t)ncalc_dev * n * sizeof(double)));
+  TF_HIPCHK(hipMemsetAsync(b_E.p, 0, (size_t)ncalc_dev * n * sizeof(double), stream));
+  double *E = (double *)b_E.p;
+  int *d_err = (int *)b_err.p;
+  if (!nat_items.empty()) {
+    hipLaunchKernelGGL(k_ff_mem, dim3((unsigned)((nat_items.size() + 255) / 256), 1), dim3(256), 0, stream, D.params,
+                       D.rinfo, D.x, n, (const FfItem *)b_ni.p, (int)nat_items.size(), (const int *)b_np.p,
+                       (const double *)b_nw.p, nfm, (const double *)b_g.p, E, d_err);
+    TF_HIPCHK(hipGetLastError());
+  }
+  DevBuf b_di, b_dp, b_dw, b_dx, b_dinfo, b_G;
+  if (P.ff_mode == 0) {
+    // randomize_decoys after every calculation (fb.cpp:4565-4611, 7546-7553):
+    // calculation c >= 1 sees the placement drawn after calculation c - 1
+    std::vector<int> pos((size_t)(nc - 1) * std::max(nd, 1));
+    for (int r = 0; r < nc; ++r)
+      for (int k = 0; k < nd; ++k) {
+        int p = F.next() % (n - S.ff_mems[k].len + 1);
+        F.ff_pos[k] = p;
+        if (r + 1 < nc) pos[(size_t)r * nd + k] = p;
+      }
+    ff_items(S, S.ff_mems, dec_items);
+    if (!dec_items.empty()) {
+      if (!(st = b_di.put(dec_items, stream)).ok() || !(st = b_dp.put(pos, stream)).ok() || !(st = b_dw.put(dec_w, stream)).ok())
+        return st;
+      hipLaunchKernelGGL(k_ff_mem, dim3((unsigned)((dec_items.size() + 255) / 256), (unsigned)(nc - 1)), dim3(256), 0,
+                         stream, D.params, D.rinfo, D.x, n, (const FfItem *)b_di.p, (int)dec_items.size(),
+                         (const int *)b_dp.p, (const double *)b_dw.p, nd, (const double *)b_g.p, E + n, d_err);
+      TF_HIPCHK(hipGetLastError());
+    }
+  }
+  const bool gen_now = P.ff_mode == 1 && !F.ff_gen_done;
+  if (gen_now && !nat_items.empty()) {
+    // the decoy structures' coordinates: CA then CB per decoy
+    std::vector<double> dxyz;
+    std::vector<int> dinfo;
+    for (const AwMemory &m : S.ff_mems) {
+      dinfo.push_back(m.pos);
+      dinfo.push_back(m.len);
+      dinfo.push_back((int)dxyz.size());
+      dxyz.insert(dxyz.end(), m.ca.begin(), m.ca.end());
+      dxyz.insert(dxyz.end(), m.cb.begin(), m.cb.end());
+    }
+    if (!(st = b_dx.put(dxyz, stream)).ok() || !(st = b_dinfo.put(dinfo, stream)).ok()) return st;
+    TF_HIPCHK(hipMalloc(&b_G.p, (size_t)nd * n * sizeof(double)));
+    TF_HIPCHK(hipMemsetAsync(b_G.p, 0, (size_t)nd * n * sizeof(double), stream));
+    hipLaunchKernelGGL(k_ff_gen, dim3((unsigned)((nat_items.size() + 255) / 256), (unsigned)nd), dim3(256), 0, stream,
+                       D.params, D.rinfo, n, (const FfItem *)b_ni.p, (int)nat_items.size(), (const int *)b_np.p,
+                       (const double *)b_nw.p, (const double *)b_g.p, (const double *)b_dx.p, (const int *)b_dinfo.p,
+                       (double *)b_G.p, d_err);
+    TF_HIPCHK(hipGetLastError());
+  }
+  std::vector<double> e((size_t)ncalc_dev * n), g(gen_now ? (size_t)nd * n : 0);
+  int herr = 0;
+  TF_HIPCHK(hipMemcpyAsync(e.data(), E, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (gen_now && !g.empty())
+    TF_HIPCHK(hipMemcpyAsync(g.data(), b_G.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  TF_HIPCHK(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, stream));
+  TF_HIPCHK(hipStreamSynchronize(stream));
+  if (herr) return AwStatus::err(AWSEM_E_FORMAT, "Decoy Memory: Interaction between residues of different chains");
+  rows.assign((size_t)n * 3, 0.0);
+  if (P.ff_mode == 0) {
+    // compute_fragment_frustration, fb.cpp:4613-4689
+    for (int i = 0; i < n; ++i) {
+      e[i] /= nfm;
+      for (int d = 1; d < nc; ++d) e[(size_t)d * n + i] /= nd;
+    }
+    for (int i = 0; i < n; ++i) {
+      double mean = 0.0, var = 0.0;
+      for (int d = 1; d < nc; ++d) mean += e[(size_t)d * n + i];
+      mean /= (nc - 1);
+      for (int d = 1; d < nc; ++d) var += std::pow(e[(size_t)d * n + i] - mean, 2);
+      var /= (nc - 1);
+      rows[3 * i] = (e[i] - mean) / std::sqrt(var);
+      rows[3 * i + 1] = e[i] - mean;
+      rows[3 * i + 2] = std::sqrt(var);
+    }
+    return AwStatus();
+  }
+  if (gen_now) {
+    // the per-residue decoy statistics (fb.cpp:4804-4830), once: at step 0
+    // in the reference, at the first calculation here
+    F.ff_mean.assign(n, 0.0);
+    F.ff_var.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double mean = 0.0, var = 0.0;
+      for (int d = 0; d < nd; ++d) mean += g[(size_t)d * n + i];
+      mean /= nd;
+      for (int d = 0; d < nd; ++d) var += std::pow(g[(size_t)d * n + i] - mean, 2);
+      var /= nd;
+      F.ff_mean[i] = mean;
+      F.ff_var[i] = var;
+    }
+    F.ff_gen_done = true;
+    if (gen) {
+      gen->assign((size_t)n * (nd + 1), 0.0);
+      for (int i = 0; i < n; ++i) (*gen)[i] = e[i];
+      for (int i = 0; i < n; ++i)
+        for (int d = 0; d < nd; ++d) (*gen)[n + (size_t)i * nd + d] = g[(size_t)d * n + i];
+      *gen_written = 1;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    rows[3 * i] = (e[i] - F.ff_mean[i]) / std::sqrt(F.ff_var[i]);
+    rows[3 * i + 1] = e[i] - F.ff_mean[i];
+    rows[3 * i + 2] = std::sqrt(F.ff_var[i]);
+  }
+  return AwStatus();
+}
+
 // the generator's state, kept so that a call whose output does not fit
 // leaves the stream where it was
 struct TfSaved {
@@ -985,6 +1271,76 @@ int awsem_nmer_frust_every(const awsem_ctx *c, int64_t *every)
   const AwSystem *S = aw_ctx_system(c);
   if (!S) return AWSEM_E_ARG;
   *every = S->P.nf_flag ? (int64_t)S->P.nf_freq : 0;
+  return AWSEM_OK;
+}
+
+int awsem_frag_frust_info(const awsem_ctx *c, int *mode, int *n_calcs, int *n_decoys, int64_t *every)
+{
+  if (!c || !mode || !n_calcs || !n_decoys || !every) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S) return AWSEM_E_ARG;
+  *mode = S->P.ff_flag ? S->P.ff_mode : -1;
+  *n_calcs = S->P.ff_flag ? S->P.ff_ncalc : 0;
+  *n_decoys = (int)S->ff_mems.size();
+  *every = S->P.ff_flag ? (int64_t)S->P.ff_freq : 0;
+  return AWSEM_OK;
+}
+
+int awsem_frag_frust(awsem_ctx *c, const double *xres, double *rows, double *gen, int *gen_written)
+{
+  if (!c || !xres || !rows || !gen_written) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S || S->n <= 0) return AWSEM_E_ARG;
+  for (size_t k = 0; k < (size_t)S->n * 9; ++k)
+    if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  std::vector<double> r, g;
+  AwStatus s = frag_frust(c, xres, r, gen ? &g : nullptr, gen_written);
+  if (!s.ok()) return aw_ctx_fail(c, s);
+  std::memcpy(rows, r.data(), r.size() * sizeof(double));
+  if (*gen_written) std::memcpy(gen, g.data(), g.size() * sizeof(double));
+  return AWSEM_OK;
+}
+
+// the reference's output at step % frag_frust_output_freq == 0
+// (fb.cpp:7537-7576, 4676-4688, 4786-4802; the five files opened at
+// construction, fb.cpp:794-800)
+int awsem_frag_frust_write(awsem_ctx *c, const double *xres, int64_t step)
+{
+  if (!c || !xres) return AWSEM_E_ARG;
+  const AwSystem *S = aw_ctx_system(c);
+  if (!S || S->n <= 0) return AWSEM_E_ARG;
+  const FFParams &P = S->P;
+  if (!P.ff_flag) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_STATE, "no [Fragment_Frustratometer] section in the coeff file"));
+  if (P.ff_freq <= 0 || step % P.ff_freq != 0) return AWSEM_OK;
+  AwFrustState &F = aw_ctx_frust(c);
+  const std::string &dir = aw_ctx_dir(c);
+  static const char *names[5] = {"fragment_frustration.dat", "fragment_frustration_gap.dat",
+                                 "fragment_frustration_variance.dat", "fragment_frustration_decoy.dat",
+                                 "fragment_frustration_native.dat"};
+  if (!F.ff_out[0])
+    for (int k = 0; k < 5; ++k) {
+      F.ff_out[k] = std::fopen((dir + "/" + names[k]).c_str(), "w");
+      if (!F.ff_out[k]) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_FILE, std::string("cannot open ") + names[k]));
+    }
+  for (size_t k = 0; k < (size_t)S->n * 9; ++k)
+    if (!std::isfinite(xres[k])) return aw_ctx_fail(c, AwStatus::err(AWSEM_E_NONFINITE, "non-finite coordinates"));
+  std::vector<double> r, g;
+  int gw = 0;
+  AwStatus s = frag_frust(c, xres, r, &g, &gw);
+  if (!s.ok()) return aw_ctx_fail(c, s);
+  const int n = S->n, nd = (int)S->ff_mems.size();
+  if (gw) {
+    for (int i = 0; i < n; ++i) std::fprintf(F.ff_out[4], "%f ", g[i]);
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < nd; ++d) std::fprintf(F.ff_out[3], " %f\n", g[n + (size_t)i * nd + d]);
+  }
+  for (int i = 0; i < n; ++i) {
+    std::fprintf(F.ff_out[0], "%f ", r[3 * i]);
+    std::fprintf(F.ff_out[1], "%f ", r[3 * i + 1]);
+    std::fprintf(F.ff_out[2], "%f ", r[3 * i + 2]);
+  }
+  for (int k = 0; k < 3; ++k) std::fprintf(F.ff_out[k], "\n");
+  for (FILE *f : F.ff_out) std::fflush(f);
   return AWSEM_OK;
 }
 

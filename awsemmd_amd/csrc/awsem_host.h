@@ -76,12 +76,16 @@ struct AwSystem {
   AwFMTable fmt;                       // table mode only
   AwExtras xt;                         // coeff-toggled terms
   std::string cr_file;                 // contact-restraints file named by the coeff
+  std::string ff_file;                 // the fragment frustratometer's decoy library
+  std::vector<AwMemory> ff_mems;       // its fragments (positions as read) or structures
   char frag_mems_file[256] = {0}, fm_gamma_file[256] = {0};
 };
 
 int aw_restype(char c);
 AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file,
-                        std::string &cr_file);
+                        std::string &cr_file, std::string &ff_file);
+// the fragment frustratometer's decoy library (after the memories)
+AwStatus aw_read_decoy_mems(const std::string &dir, AwSystem &S);
 // the coeff-toggled terms' side files (after the sequence, tables and
 // memories): contact list, zim, charges, AMH-Go files; their share of
 // pair_list_cutoff (fb.cpp:1058-1066) and the AMH-Go normalisation
@@ -239,6 +243,12 @@ struct AwFrustState {
   // counter (a function-static in the reference, fb.cpp:5802)
   FILE *nm_dat = nullptr, *nm_tcl = nullptr, *nm_trap = nullptr;
   int nm_rep = 1;
+  // the fragment frustratometer: its five files, the decoy fragments'
+  // current positions (shuffle), read mode's per-residue decoy statistics
+  FILE *ff_out[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::vector<int> ff_pos;
+  bool ff_gen_done = false;
+  std::vector<double> ff_mean, ff_var;
   AwFrustState();
   ~AwFrustState();
   int next();

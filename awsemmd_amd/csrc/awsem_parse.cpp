@@ -33,12 +33,12 @@ static std::string join(const std::string &dir, const std::string &name)
 }
 
 static const char *kUnsupported[] = {
-    "[Fragment_Frustratometer]", "[Amylometer]", "[Selection_Temperature]",
+    "[Amylometer]", "[Selection_Temperature]",
     "[Monte_Carlo_Seq_Opt]", "[Optimization]", "[Burial_Optimization]",
     "[DebyeHuckel_Optimization]", nullptr};
 
 AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_file, std::string &gamma_file,
-                        std::string &cr_file)
+                        std::string &cr_file, std::string &ff_file)
 {
   std::ifstream in(path);
   if (!in) return AwStatus::err(-2, "Coefficient file was not found!");
@@ -179,6 +179,22 @@ AwStatus aw_parse_coeff(const std::string &path, FFParams &P, std::string &mem_f
       else if (!in.fail())
         return AwStatus::err(-3, "Only \"configurational\", \"mutational\", \"singleresidue\" are acceptable modes "
                                  "for the Tertiary_Frustratometer.");
+    } else if (sec == "[Fragment_Frustratometer]") {  // fb.cpp:359-384
+      if (!P.fm_flag && !P.fm_tb_flag)
+        return AwStatus::err(-3, "Cannot run Fragment_Frustratometer without Fragment_Memory or Fragment_Memory_Table.");
+      std::string mode;
+      P.ff_flag = 1;
+      P.ff_ww = 1.0;
+      in >> mode;
+      if (mode == "shuffle") {
+        P.ff_mode = 0;
+        in >> ff_file >> P.ff_ncalc >> P.ff_freq;
+      } else if (mode == "read") {
+        P.ff_mode = 1;
+        in >> ff_file >> P.ff_freq >> P.ff_ww >> P.ff_seqsep >> P.ff_seqsep_gamma >> P.ff_norm;
+      } else {
+        return AwStatus::err(-3, "Only \"shuffle\" and \"read\" are acceptable modes for the Fragment_Frustratometer.");
+      }
     } else if (sec == "[Nmer_Frustratometer]") {  // fb.cpp:398-412
       std::string mode;
       P.nf_flag = 1;
@@ -598,14 +614,17 @@ static char *trim(char *s)
   return s;
 }
 
-AwStatus aw_read_mems(const std::string &dir, const std::string &mem_file, AwSystem &S)
+// read_mems (fb.cpp:1966-2042): the [Memories] lines of a .mem file, each
+// a window of a .gro structure placed at a target position
+static AwStatus read_mem_list(const std::string &dir, const std::string &mem_file, const AwSystem &S,
+                              std::vector<AwMemory> &out)
 {
   FILE *f = std::fopen(join(dir, mem_file).c_str(), "r");
   if (!f) return AwStatus::err(-2, "Fragment_Memory: Error opening mem file");
   std::unordered_map<std::string, GroFile> cache;
   char ln[500];
   int state = 0;  // 0 none, 1 target, 2 memories
-  S.mems.clear();
+  out.clear();
   while (std::fgets(ln, sizeof ln, f)) {
     char *line = trim(ln);
     if (line[0] == '#') continue;
@@ -635,13 +654,34 @@ AwStatus aw_read_mems(const std::string &dir, const std::string &mem_file, AwSys
         std::fclose(f);
         return AwStatus::err(-3, "read_mems: Fragment_Memory: Incorrectly defined memory fragment");
       }
-      S.mems.push_back(std::move(m));
+      out.push_back(std::move(m));
     } else if (state == 0) {
       if (std::strcmp(line, "[Target]") == 0) state = 1;
       else if (std::strcmp(line, "[Memories]") == 0) state = 2;
     }
   }
   std::fclose(f);
+  return AwStatus();
+}
+
+AwStatus aw_read_mems(const std::string &dir, const std::string &mem_file, AwSystem &S)
+{
+  return read_mem_list(dir, mem_file, S, S.mems);
+}
+
+// the fragment frustratometer's decoy library (fb.cpp:794-834): fragments
+// to shuffle, or decoy structures; it compares them with the memories,
+// which must be loaded
+AwStatus aw_read_decoy_mems(const std::string &dir, AwSystem &S)
+{
+  if (!S.P.ff_flag) return AwStatus();
+  if (S.mems.empty())
+    return AwStatus::err(-3, "Fragment_Frustratometer: the fragment memories are not loaded (a table read from "
+                             "fm_table.energy / fm_table.force reads none)");
+  AwStatus s = read_mem_list(dir, S.ff_file, S, S.ff_mems);
+  if (!s.ok()) return s;
+  if (S.P.ff_mode == 1) S.P.ff_ncalc = (int)S.ff_mems.size() + 1;
+  if (S.P.ff_ncalc < 2) return AwStatus::err(-3, "Fragment_Frustratometer: no decoys");
   return AwStatus();
 }
 

@@ -104,6 +104,13 @@ struct FFParams {
   // mode 0 pairwise / 1 singlenmer
   int nf_flag, nf_size, nf_contacts, nf_ndecoys, nf_freq, nf_neutral, nf_trap, nf_draw_trap, nf_mode, pad_nf;
   double nf_cutoff, nf_min_thr, nf_high_thr, nf_trap_sigma, nf_ss_frac;
+  // [Fragment_Frustratometer] (fb.cpp:359-384): mode 0 shuffle / 1 read,
+  // decoy calculations (shuffle; read: decoys + 1), output interval, the
+  // sequence-separation scaling flag and exponent and the well width (read
+  // mode; shuffle mode leaves them unset in the reference: no scaling,
+  // width 1 here), the normalisation flag (no effect in the reference)
+  int ff_flag, ff_mode, ff_ncalc, ff_freq, ff_seqsep, ff_norm;
+  double ff_ww, ff_seqsep_gamma;
   // terms whose pairs come from FixBackbone's neighbour list with no cutoff
   // of their own (Debye-Hueckel, the barrier's CA-CA pairs, AMH-Go) count
   // pairs closer than pair_list_cutoff + list_skin (a list LAMMPS has just

@@ -420,6 +420,25 @@ int awsem_nmer_frust_write(awsem_ctx *c, const double *xres, int64_t step);
 /* The output interval of the [Nmer_Frustratometer] section, 0 without it. */
 int awsem_nmer_frust_every(const awsem_ctx *c, int64_t *every);
 
+/* The fragment frustratometer ([Fragment_Frustratometer], fb.cpp:359-384,
+ * 4431-4832) at xres.  Shuffle mode: the native memory energies and
+ * n_calcs - 1 placements of the decoy fragments drawn from the rand()
+ * stream; read mode: the native memory energies against decoy-structure
+ * statistics computed at the first call (at step 0 in the reference).
+ * rows: n x 3 (frustration index (native - mean) / std, native - mean, std
+ * per residue).  gen (may be NULL; n * (n_decoys + 1) doubles): at the call
+ * that computes read mode's statistics, the n native energies then the
+ * n x n_decoys decoy energies (residue-major), and *gen_written = 1. */
+int awsem_frag_frust(awsem_ctx *c, const double *xres, double *rows, double *gen, int *gen_written);
+/* The reference's output when step % output interval == 0:
+ * fragment_frustration.dat, _gap.dat, _variance.dat (one line per call),
+ * _native.dat and _decoy.dat (read mode's generated energies) in the
+ * context's directory (fb.cpp:794-800, 4676-4688, 4786-4802). */
+int awsem_frag_frust_write(awsem_ctx *c, const double *xres, int64_t step);
+/* mode (0 shuffle, 1 read, -1 without the section), decoy calculations,
+ * decoy fragments or structures, output interval (0 without the section). */
+int awsem_frag_frust_info(const awsem_ctx *c, int *mode, int *n_calcs, int *n_decoys, int64_t *every);
+
 /* ---- order parameters (compute qwolynes, compute qonuchic) ---------------
  * Replace `compute ID alpha_carbons qwolynes rnative.dat sep exp`
  * (src/compute_q_wolynes.cpp:51-88, compute_scalar 171-236) and

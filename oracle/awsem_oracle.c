@@ -35,7 +35,7 @@ static const int four_letter_map[20] = {1, 3, 2, 2, 4, 2, 2, 1, 3, 4, 4, 3, 4, 4
 
 enum { FM_CA = 1, FM_CB = 2 };
 
-typedef struct {
+typedef struct or_mem_s {
   int pos, len;
   double weight;
   char *se;
@@ -148,6 +148,19 @@ struct or_ctx {
      fraction, mode 0 pairwise / 1 singlenmer */
   int nmer_frust_flag, nf_size, nf_contacts, nf_ndecoys, nf_freq, nf_neutral, nf_trap, nf_draw_trap, nf_mode;
   double nf_cutoff, nf_min_thr, nf_high_thr, nf_trap_sigma, nf_ss_frac;
+  /* [Fragment_Frustratometer] (fb.cpp:359-384): mode 0 shuffle / 1 read,
+     the decoy library (shuffled memories or decoy structures), decoy
+     calculations (shuffle), output interval, well width, sequence-
+     separation scaling (read mode only; shuffle mode leaves them unset in
+     the reference: width 1, no scaling here); the decoy map, and read
+     mode's per-residue decoy statistics (computed once) */
+  int frag_frust_flag, ff_mode, ff_ncalc, ff_freq, ff_seqsep, ff_norm, ff_gen_done;
+  double ff_ww, ff_seqsep_gamma;
+  char ff_file[256];
+  struct or_mem_s *ff_mems;
+  int ff_nm;
+  int *ff_ilen, **ff_map;
+  double *ff_mean, *ff_var;
   uint32_t rng[34];
   int rng_k;
   /* system */
@@ -591,7 +604,8 @@ static char *trim(char *s)
   return s;
 }
 
-static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char *err, int errlen)
+static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, or_mem **mems_out, int *nm_out, int vec,
+                     char *err, int errlen)
 {
   char path[1024];
   join_path(path, sizeof path, workdir, mems_file);
@@ -599,7 +613,8 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
   if (!f) { set_err(err, errlen, "Fragment_Memory: Error opening mem file"); return -1; }
   char ln[500];
   int state = 0; /* 0 none, 1 target, 2 mems */
-  c->n_mems = 0;
+  *nm_out = 0;
+  *mems_out = NULL;
   int cap = 0;
   int ngro = 0, capgro = 0;
   gro_file *gros = NULL;
@@ -623,7 +638,7 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
       if (nstr != 5) { set_err(err, errlen, "Fragment_Memory: Error reading mem file"); rc = -1; break; }
       int tpos = atoi(str[1]) - 1, fpos = atoi(str[2]) - 1, len = atoi(str[3]);
       double weight = atof(str[4]);
-      if (c->n_mems == cap) { cap = cap ? 2 * cap : 64; c->mems = (or_mem *)realloc(c->mems, (size_t)cap * sizeof(or_mem)); }
+      if (*nm_out == cap) { cap = cap ? 2 * cap : 64; *mems_out = (or_mem *)realloc(*mems_out, (size_t)cap * sizeof(or_mem)); }
       char gro[1024];
       join_path(gro, sizeof gro, workdir, str[0]);
       int gi = 0;
@@ -632,8 +647,8 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
         if (ngro == capgro) { capgro = capgro ? 2 * capgro : 16; gros = (gro_file *)realloc(gros, (size_t)capgro * sizeof(gro_file)); }
         gro_parse(gro, &gros[ngro++]);
       }
-      int e = mem_load(&c->mems[c->n_mems], tpos, fpos, len, weight, &gros[gi], c->vec_frag_mem_flag);
-      c->n_mems++;
+      int e = mem_load(&(*mems_out)[*nm_out], tpos, fpos, len, weight, &gros[gi], vec);
+      (*nm_out)++;
       if (e) { set_err(err, errlen, "read_mems: Fragment_Memory: Error reading memory fragment"); rc = -1; break; }
       if (tpos + len > c->n) { set_err(err, errlen, "read_mems: Fragment_Memory: Incorrectly defined memory fragment"); rc = -1; break; }
     } else if (state == 0) {
@@ -650,6 +665,24 @@ static int read_mems(or_ctx *c, const char *workdir, const char *mems_file, char
 /* ------------------------------------------------------------------ */
 /* Constructor: coeff parse (fb.cpp:165-497), scaling (502-516), side   */
 /* files (538-791), pair_list_cutoff (1048-1071)                        */
+
+/* the decoy memory map at the decoys' current positions (fb.cpp:812-834,
+ * 4588-4610): residue i lists the decoys with pos <= i < pos + len - minSep */
+static int ff_build_map(or_ctx *c)
+{
+  int n = c->n, min_sep = c->fmg.i_sep[0];
+  for (int i = 0; i < n; ++i) { free(c->ff_map[i]); c->ff_map[i] = NULL; c->ff_ilen[i] = 0; }
+  for (int k = 0; k < c->ff_nm; ++k) {
+    int pos = c->ff_mems[k].pos, len = c->ff_mems[k].len;
+    if (pos + len > n) return -1;
+    for (int i = pos; i < pos + len - min_sep; ++i) {
+      c->ff_ilen[i]++;
+      c->ff_map[i] = (int *)realloc(c->ff_map[i], (size_t)c->ff_ilen[i] * sizeof(int));
+      c->ff_map[i][c->ff_ilen[i] - 1] = k;
+    }
+  }
+  return 0;
+}
 
 static double calc_exp_helix_cutoff(const or_ctx *c) /* fb.cpp:1074-1104 */
 {
@@ -883,8 +916,34 @@ static int parse_coeff(or_ctx *c, const char *path, char *err, int errlen)
         set_err(err, errlen, "Only \"pairwise\", \"singlenmer\" are acceptable modes for the Nmer_Frustratometer.");
         return -1;
       }
-    } else if (strcmp(sec, "[Fragment_Frustratometer]") == 0 ||
-               strcmp(sec, "[Amylometer]") == 0 || strcmp(sec, "[Selection_Temperature]") == 0 ||
+    } else if (strcmp(sec, "[Fragment_Frustratometer]") == 0) { /* fb.cpp:359-384 */
+      if (!c->frag_mem_flag && !c->frag_mem_tb_flag) {
+        free(text);
+        set_err(err, errlen, "Cannot run Fragment_Frustratometer without Fragment_Memory or Fragment_Memory_Table.");
+        return -1;
+      }
+      c->frag_frust_flag = 1;
+      c->ff_ww = 1.0;
+      tk_str(&in, sbuf, sizeof sbuf);
+      if (strcmp(sbuf, "shuffle") == 0) {
+        c->ff_mode = 0;
+        tk_str(&in, c->ff_file, sizeof c->ff_file);
+        c->ff_ncalc = tk_int(&in);
+        c->ff_freq = tk_int(&in);
+      } else if (strcmp(sbuf, "read") == 0) {
+        c->ff_mode = 1;
+        tk_str(&in, c->ff_file, sizeof c->ff_file);
+        c->ff_freq = tk_int(&in);
+        c->ff_ww = tk_double(&in);
+        c->ff_seqsep = tk_int(&in);
+        c->ff_seqsep_gamma = tk_double(&in);
+        c->ff_norm = tk_int(&in);
+      } else {
+        free(text);
+        set_err(err, errlen, "Only \"shuffle\" and \"read\" are acceptable modes for the Fragment_Frustratometer.");
+        return -1;
+      }
+    } else if (strcmp(sec, "[Amylometer]") == 0 || strcmp(sec, "[Selection_Temperature]") == 0 ||
                strcmp(sec, "[Monte_Carlo_Seq_Opt]") == 0 || strcmp(sec, "[Optimization]") == 0 ||
                strcmp(sec, "[Burial_Optimization]") == 0 ||
                strcmp(sec, "[DebyeHuckel_Optimization]") == 0) {
@@ -1280,7 +1339,7 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
   if (c->frag_mem_flag || c->frag_mem_tb_flag) { /* fb.cpp:749-791 */
     join_path(path, sizeof path, workdir, c->fm_gamma_file);
     if (gm_load(&c->fmg, path, err, errlen)) { or_destroy(c); return -1; }
-    if (read_mems(c, workdir, c->frag_mems_file, err, errlen)) { or_destroy(c); return -1; }
+    if (read_mems(c, workdir, c->frag_mems_file, &c->mems, &c->n_mems, c->vec_frag_mem_flag, err, errlen)) { or_destroy(c); return -1; }
     c->ilen_fm_map = (int *)calloc((size_t)n, sizeof(int));
     c->frag_mem_map = (int **)calloc((size_t)n, sizeof(int *));
     int min_sep = c->fmg.i_sep[0];
@@ -1293,6 +1352,16 @@ int or_create(const char *coeff_path, const char *seq_path, const char *workdir,
         c->frag_mem_map[i][c->ilen_fm_map[i] - 1] = k;
       }
     }
+  }
+  if (c->frag_frust_flag) { /* fb.cpp:794-855: the decoy library and its map */
+    if (read_mems(c, workdir, c->ff_file, &c->ff_mems, &c->ff_nm, 0, err, errlen)) { or_destroy(c); return -1; }
+    if (c->ff_mode == 1) c->ff_ncalc = c->ff_nm + 1;
+    if (c->ff_ncalc < 2) { or_destroy(c); set_err(err, errlen, "Fragment_Frustratometer: fewer than one decoy calculation"); return -1; }
+    c->ff_ilen = (int *)calloc((size_t)n, sizeof(int));
+    c->ff_map = (int **)calloc((size_t)n, sizeof(int *));
+    c->ff_mean = (double *)calloc((size_t)n, sizeof(double));
+    c->ff_var = (double *)calloc((size_t)n, sizeof(double));
+    if (ff_build_map(c)) { or_destroy(c); set_err(err, errlen, "Fragment_Frustratometer: Incorrectly defined memory fragment"); return -1; }
   }
   if (c->frag_mem_tb_flag && fm_table_build(c, err, errlen)) { or_destroy(c); return -1; }
   if (c->vec_frag_mem_flag && !(c->frag_mem_flag || c->frag_mem_tb_flag)) {
@@ -1366,6 +1435,10 @@ void or_destroy(or_ctx *c)
   if (!c) return;
   for (int k = 0; k < c->n_mems; ++k) { free(c->mems[k].se); free(c->mems[k].rf0); free(c->mems[k].rf1); free(c->mems[k].vmf); }
   free(c->mems);
+  for (int k = 0; k < c->ff_nm; ++k) { free(c->ff_mems[k].se); free(c->ff_mems[k].rf0); free(c->ff_mems[k].rf1); free(c->ff_mems[k].vmf); }
+  free(c->ff_mems);
+  if (c->ff_map) for (int i = 0; i < c->n; ++i) free(c->ff_map[i]);
+  free(c->ff_map); free(c->ff_ilen); free(c->ff_mean); free(c->ff_var);
   if (c->frag_mem_map) for (int i = 0; i < c->n; ++i) free(c->frag_mem_map[i]);
   free(c->frag_mem_map); free(c->ilen_fm_map);
   free(c->fmg.gamma);
@@ -3206,4 +3279,144 @@ int or_nmer_frust(or_ctx *c, const double *xca, const double *xcb, const double 
   }
   *ntr = nt <= tcap ? nt : -nt;
   return nr <= cap ? nr : -nr;
+}
+
+/* ------------------------------------------------------------------ */
+/* the fragment frustratometer (fb.cpp:4431-4832, 7537-7576)            */
+
+/* compute_decoy_memory_potential (fb.cpp:4431-4563): residue i's memory
+ * energies with the library mems (map, ilen) at the current coordinates,
+ * added to E[i] and E[j]; with dec != NULL (compute_generated_decoy_
+ * energies, 4691-4832) the distances are the decoy structure's instead
+ * (0 where the decoy does not cover the residue: its Rf returns 0 there
+ * and the reference does not check) */
+static int ff_energy(const or_ctx *c, const or_mem *mems, int **map, const int *ilen, const or_mem *dec, double *E)
+{
+  int n = c->n, minsep = c->fmg.i_sep[0], maxsep = c->fmg.i_sep[c->fmg.nseq_cl];
+  const int ity[4] = {FM_CA, FM_CA, FM_CB, FM_CB}, jty[4] = {FM_CA, FM_CB, FM_CA, FM_CB};
+  for (int i = 0; i < n; ++i) {
+    int itype = restype(c->se[i]);
+    for (int ifm = 0; ifm < ilen[i]; ++ifm) {
+      const or_mem *m = &mems[map[i][ifm]];
+      int js = i + minsep, je = m->pos + m->len - 1;
+      if (maxsep != -1 && je > i + maxsep) je = i + maxsep;
+      double ekw = c->epsilon * c->k_frag_mem * m->weight;
+      if (je >= n) return -1; /* "Missing residues in decoy memory potential" */
+      for (int j = js; j <= je; ++j) {
+        int jtype = restype(c->se[j]);
+        if (c->chain_no[i] != c->chain_no[j]) return -2; /* "Interaction between residues of different chains" */
+        double sig2 = pow(abs(i - j), 2 * c->fm_sigma_exp);
+        sig2 = sig2 * c->ff_ww * c->ff_ww;
+        double g = gm_get2(&c->fmg, itype, jtype, i, j);
+        if (c->ff_seqsep) g *= pow((abs(i - j) - minsep + 1), -c->ff_seqsep_gamma);
+        double ekwg = ekw * g;
+        const double *xi[4] = {XCA(i), XCA(i), XCB(i), XCB(i)}, *xj[4] = {XCA(j), XCB(j), XCA(j), XCB(j)};
+        for (int k = 0; k < 4; ++k) {
+          if (ity[k] == FM_CB && (c->se[i] == 'G' || m->se[i - m->pos] == 'G')) continue;
+          if (jty[k] == FM_CB && (c->se[j] == 'G' || m->se[j - m->pos] == 'G')) continue;
+          double r;
+          if (dec) {
+            r = mem_rf(dec, i, ity[k], j, jty[k]);
+            if (isnan(r)) r = 0.0;
+          } else {
+            double dx[3] = {xi[k][0] - xj[k][0], xi[k][1] - xj[k][1], xi[k][2] - xj[k][2]};
+            r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+          }
+          double rf = mem_rf(m, i, ity[k], j, jty[k]);
+          double dr = r - rf, drsq = dr * dr;
+          /* the reference's normalizeInteraction scales V before assigning
+             it (fb.cpp:4548-4553): no effect */
+          double V = -ekwg * exp(-drsq / (2 * sig2));
+          E[i] += V;
+          E[j] += V;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* one fragment-frustratometer calculation at step `step` (fb.cpp:7537-
+ * 7576) at the given coordinates.  rows: n x 3 (frustration index
+ * (native - mean) / std, native - mean, std, per residue).  gen (read mode,
+ * may be NULL): at the calculation that computes the decoy statistics, the
+ * n native energies then the n x n_decoys decoy energies (residue-major),
+ * the values written to fragment_frustration_native.dat / _decoy.dat
+ * (4786-4802); *gen_written is 1 then.  Returns 0, or < 0 on the
+ * reference's errors. */
+int or_frag_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, long long step, double *rows,
+                  double *gen, int *gen_written)
+{
+  int n = c->n, nc = c->ff_ncalc;
+  c->xca = xca; c->xcb = xcb; c->xo = xo;
+  *gen_written = 0;
+  double *E = (double *)calloc((size_t)n * nc, sizeof(double)); /* E[calc * n + i] */
+  int rc = 0;
+  if (c->ff_mode == 0) {
+    for (int d = 0; d < nc && !rc; ++d) {
+      rc = d == 0 ? ff_energy(c, c->mems, c->frag_mem_map, c->ilen_fm_map, NULL, E)
+                  : ff_energy(c, c->ff_mems, c->ff_map, c->ff_ilen, NULL, E + (size_t)d * n);
+      /* randomize_decoys (fb.cpp:4565-4611) */
+      for (int k = 0; k < c->ff_nm; ++k) c->ff_mems[k].pos = or_rng_next(c) % (n - c->ff_mems[k].len + 1);
+      if (ff_build_map(c)) rc = -3;
+    }
+    if (!rc) {
+      /* compute_fragment_frustration, fb.cpp:4613-4689 */
+      for (int i = 0; i < n; ++i) {
+        E[i] /= c->n_mems;
+        for (int d = 1; d < nc; ++d) E[(size_t)d * n + i] /= c->ff_nm;
+      }
+      for (int i = 0; i < n; ++i) {
+        double mean = 0.0, var = 0.0;
+        for (int d = 1; d < nc; ++d) mean += E[(size_t)d * n + i];
+        mean /= (nc - 1);
+        for (int d = 1; d < nc; ++d) var += pow(E[(size_t)d * n + i] - mean, 2);
+        var /= (nc - 1);
+        rows[3 * i] = (E[i] - mean) / sqrt(var);
+        rows[3 * i + 1] = E[i] - mean;
+        rows[3 * i + 2] = sqrt(var);
+      }
+    }
+  } else {
+    rc = ff_energy(c, c->mems, c->frag_mem_map, c->ilen_fm_map, NULL, E);
+    /* the decoy statistics: at step 0 in the reference (fb.cpp:7566-7568;
+       left unset when a run starts elsewhere), here at the first call */
+    if (!rc && !c->ff_gen_done) {
+      int nd = c->ff_nm;
+      for (int d = 0; d < nd && !rc; ++d) rc = ff_energy(c, c->mems, c->frag_mem_map, c->ilen_fm_map, &c->ff_mems[d], E + (size_t)(d + 1) * n);
+      for (int i = 0; i < n && !rc; ++i) {
+        double mean = 0.0, var = 0.0;
+        for (int d = 1; d < nc; ++d) mean += E[(size_t)d * n + i];
+        mean /= nd;
+        for (int d = 1; d < nc; ++d) var += pow(E[(size_t)d * n + i] - mean, 2);
+        var /= nd;
+        c->ff_mean[i] = mean;
+        c->ff_var[i] = var;
+      }
+      if (!rc && gen) {
+        for (int i = 0; i < n; ++i) gen[i] = E[i];
+        for (int i = 0; i < n; ++i)
+          for (int d = 0; d < nd; ++d) gen[n + (size_t)i * nd + d] = E[(size_t)(d + 1) * n + i];
+        *gen_written = 1;
+      }
+      c->ff_gen_done = 1;
+    }
+    if (!rc)
+      for (int i = 0; i < n; ++i) {
+        rows[3 * i] = (E[i] - c->ff_mean[i]) / sqrt(c->ff_var[i]);
+        rows[3 * i + 1] = E[i] - c->ff_mean[i];
+        rows[3 * i + 2] = sqrt(c->ff_var[i]);
+      }
+  }
+  free(E);
+  (void)step;
+  return rc;
+}
+
+int or_frag_frust_dims(const or_ctx *c, int *mode, int *ncalc, int *ndecoys)
+{
+  *mode = c->frag_frust_flag ? c->ff_mode : -1;
+  *ncalc = c->ff_ncalc;
+  *ndecoys = c->ff_nm;
+  return 0;
 }

@@ -163,6 +163,13 @@ int or_tert_frust(or_ctx *c, const double *xca, const double *xcb, const double 
  * threshold, energy); returns the rows (-needed beyond cap) */
 int or_nmer_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, double *rows, int cap,
                   double *traps, int tcap, int *ntr);
+/* the fragment frustratometer ([Fragment_Frustratometer], fb.cpp:4431-4832)
+ * on the same rand() stream: rows n x 3 (index, native - mean, std per
+ * residue); read mode's generated energies (n natives, n x n_decoys) into
+ * gen at the call that computes them (*gen_written 1); < 0 on errors */
+int or_frag_frust(or_ctx *c, const double *xca, const double *xcb, const double *xo, long long step, double *rows,
+                  double *gen, int *gen_written);
+int or_frag_frust_dims(const or_ctx *c, int *mode, int *ncalc, int *ndecoys);
 void or_rng_seed(or_ctx *c, unsigned seed);
 int or_rng_next(or_ctx *c);
 double or_q_onuchic(int n, const double *xca, int type, const double *nat, double rc, int nmap, const int *mi,

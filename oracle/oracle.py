@@ -74,6 +74,9 @@ def lib():
         L.or_tert_frust.restype = C.c_int
         L.or_nmer_frust.argtypes = [C.c_void_p, dp, dp, dp, dp, C.c_int, dp, C.c_int, C.POINTER(C.c_int)]
         L.or_nmer_frust.restype = C.c_int
+        L.or_frag_frust.argtypes = [C.c_void_p, dp, dp, dp, C.c_longlong, dp, dp, C.POINTER(C.c_int)]
+        L.or_frag_frust.restype = C.c_int
+        L.or_frag_frust_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_q_wolynes.argtypes = [C.c_int, dp, dp, C.c_int, C.c_double]
         L.or_q_wolynes.restype = C.c_double
         L.or_q_onuchic.argtypes = [C.c_int, dp, C.c_int, dp, C.c_double, C.c_int, ip, ip, dp, C.c_double]
@@ -185,6 +188,25 @@ class Oracle:
             if k >= 0 and nt.value >= 0:
                 return out[:7 * k].reshape(k, 7), tr[:7 * nt.value].reshape(nt.value, 7)
             cap, tcap = max(cap, -k + 16), max(tcap, -nt.value + 16)
+
+    def frag_frust(self, coords: np.ndarray, step: int = 0):
+        """One fragment-frustratometer calculation: (rows, gen); rows n x 3
+        (index, native - mean, std per residue); gen (read mode, at the call
+        that computes the decoy statistics) (native energies n, decoy
+        energies n x n_decoys), else None."""
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        n = x.shape[1]
+        mode, nc, nd = C.c_int(), C.c_int(), C.c_int()
+        lib().or_frag_frust_dims(self.h, C.byref(mode), C.byref(nc), C.byref(nd))
+        rows = np.zeros(3 * n)
+        gen = np.zeros(n + n * max(nd.value, 1))
+        w = C.c_int()
+        rc = lib().or_frag_frust(self.h, np.ascontiguousarray(x[0]), np.ascontiguousarray(x[1]),
+                                 np.ascontiguousarray(x[2]), int(step), rows, gen, C.byref(w))
+        if rc:
+            raise RuntimeError("or_frag_frust: error %d" % rc)
+        g = (gen[:n].copy(), gen[n:].reshape(n, nd.value)) if w.value else None
+        return rows.reshape(n, 3), g
 
     def fm_table_dims(self):
         """(grid points per curve, neighbours per residue) of fragment-memory

@@ -188,17 +188,22 @@ def test_deck_1r69_shipped_system(oracle_mod):
 
 
 def test_deck_frustratometers(oracle_mod):
-    """A deck whose coeff file enables the tertiary frustratometer
+    """A deck whose coeff file enables the fragment frustratometer (read
+    mode, six decoy structures, every 10 steps), the tertiary frustratometer
     (mutational, 80 decoys, every 5 steps) and the n-mer frustratometer
     (single 4-mers, 30 decoys, every 10 steps): the run stops at their output
-    steps and writes tertiary_frustration.dat blocks at 0, 5, ..., 20 and
+    steps and writes fragment_frustration.dat lines at 0, 10, 20,
+    tertiary_frustration.dat blocks at 0, 5, ..., 20 and
     nmer_frustration.dat blocks at 0, 10, 20 whose rows equal the oracle's
     at the oracle's trajectory (one rand() stream, the tertiary call first
     at a shared step, continued across the blocks) to the printed
     precision."""
     A = _A()
     s, path, d = deckgen.make_deck([40, 30], 54, compact=True, runs=(20,), thermo=10, dump=20, print_every=10)
+    with open(os.path.join(d, "decoys.mem"), "w") as f:
+        f.write("[Target]\nquery\n\n[Memories]\n" + "".join("mem%03d.gro 1 1 70 1\n" % k for k in range(6)))
     with open(os.path.join(d, "fix_backbone_coeff.data"), "a") as f:
+        f.write("\n[Fragment_Frustratometer]\nread\ndecoys.mem\n10\n1.0\n0 0.0\n0\n\n")
         f.write("\n[Tertiary_Frustratometer]\n9.5\n80\n5\nmutational\n\n")
         f.write("\n[Nmer_Frustratometer]\n4\n9.5\n1\n30\n10\n0.78 -1 0\n0 0 1 0.6\nsinglenmer\n\n")
     dk = A.Deck(path, A.PREC_FP64)
@@ -208,11 +213,18 @@ def test_deck_frustratometers(oracle_mod):
     dk.run()
     x, _ = dk.residue_state()
     assert np.abs(x - ref[20][0]).max() < 1e-8
-    tert, nmer = {}, {}
+    tert, nmer, frag = {}, {}, {}
     for st in (0, 5, 10, 15, 20):
+        if st % 10 == 0:
+            frag[st] = o.frag_frust(ref[st][0], st)[0]
         tert[st] = o.tert_frust(ref[st][0])
         if st % 10 == 0:
             nmer[st] = o.nmer_frust(ref[st][0])[0]
+    fl = open(os.path.join(d, "fragment_frustration.dat")).read().splitlines()
+    assert len(fl) == 3
+    for ln, st in zip(fl, (0, 10, 20)):
+        vals = np.array([float(t) for t in ln.split()])
+        assert np.abs(vals - frag[st][:, 0]).max() <= 2e-6, st
     nd = open(os.path.join(d, "nmer_frustration.dat")).read().splitlines()
     nb = [k for k, ln in enumerate(nd) if ln.startswith("# timestep:")]
     assert [nd[k] for k in nb] == ["# timestep: %d" % k for k in (0, 10, 20)]
