@@ -1,21 +1,23 @@
-// awsem_frust.hip -- the tertiary frustratometer ([Tertiary_Frustratometer],
-// fb.cpp:5057-5412, energies 5444-5548) on the device.
+// awsem_frust.hip -- the frustratometers on the device: tertiary
+// ([Tertiary_Frustratometer], fb.cpp:5057-5412, energies 5444-5548),
+// n-mer ([Nmer_Frustratometer], fb.cpp:5600-6061) and fragment
+// ([Fragment_Frustratometer], fb.cpp:4431-4832).
 //
 // A call at one coordinate set: densities rho_i (k_tf_rho, the well-0 sum
-// of cWell::compute_ro, sml.h:630-650), then the host lists the rows (the
-// contacts i < j within the cutoff, or every residue in single-residue
-// mode) and draws the decoys from the context's rand() stream in the
-// reference's order; k_tf_rows evaluates every (row, entry) energy, entry 0
-// the native assignment and entries 1..m the decoys: the pair's water,
-// burial and Debye-Hueckel terms, plus (mutational and single-residue
-// modes) the environment sum over every other residue k, O(n) per entry.
-// One workgroup per row; the k-dependent, entry-independent pieces (the
-// two wells' theta at r_xk, the density switch of k, the screened 1/r) are
-// computed once per chunk of k into LDS and shared by the row's entries, so
-// the inner loop is table gathers and a few multiplies.  FP64 throughout.
-// The host reduces each row's decoys to mean and (population) standard
-// deviation in the reference's order and writes tertiary_frustration.dat /
-// .tcl as the reference does.
+// of cWell::compute_ro, sml.h:630-650), then the host lists the rows and
+// draws the decoys from the context's rand() stream in the reference's
+// order, and the kernels evaluate every (row, entry) energy, entry 0 the
+// native assignment and entries 1..m the decoys.  Tertiary contacts
+// (k_tf_rows): the pair's water, burial and Debye-Hueckel terms, plus in
+// mutational mode the environment sum over every other residue k, O(n) per
+// entry, one workgroup per contact with the k-dependent, entry-independent
+// pieces (the two wells' theta at r_xk, the density switch of k, the
+// screened 1/r) computed once per chunk of k into LDS and shared by the
+// contact's entries.  A single-residue entry depends on (residue, type)
+// only: one n x 20 table (k_nm_single) serves every decoy.  FP64
+// throughout.  The host reduces each row's decoys to mean and (population)
+// standard deviation in the reference's order and writes the reference's
+// output files.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -54,16 +56,18 @@ void AwFrustState::seed(unsigned s)
   for (int i = 1; i < 31; ++i) r[i] = (16807LL * r[i - 1]) % 2147483647LL;
   for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
   for (int i = 0; i < 34; ++i) rng[i] = (uint32_t)r[i];
-  rng_k = 34;
+  rng_k = 0;  // the slot of r[34], 34 mod 34
   for (int i = 34; i < 344; ++i) next();
 }
 
 int AwFrustState::next()
 {
-  long long k = rng_k;
-  uint32_t v = rng[(k - 31) % 34] + rng[(k - 3) % 34];
-  rng[k % 34] = v;
-  rng_k = k + 1;
+  // the ring holds r[k-34 .. k-1] at slots k mod 34: r[k-31] at j + 3 and
+  // r[k-3] at j + 31 (mod 34), j the slot of r[k]
+  const int j = rng_k, a = j + 3 < 34 ? j + 3 : j + 3 - 34, b = j + 31 < 34 ? j + 31 : j + 31 - 34;
+  uint32_t v = rng[a] + rng[b];
+  rng[j] = v;
+  rng_k = j + 1 < 34 ? j + 1 : 0;
   return (int)(v >> 1);
 }
 
@@ -71,10 +75,10 @@ int AwFrustState::next()
 // device side
 
 struct TfRow {
-  int kind;        // 0 pair (configurational), 1 pair + environment (mutational), 2 single residue
-  int a, b, cnt;   // residues (b unused for kind 2); entries
+  int kind;        // 0 pair (configurational), 1 pair + environment (mutational)
+  int a, b, cnt;   // residues; entries
   long long off;   // first entry
-  double r, ra, rb;  // pair distance and densities (kinds 1, 2; kind 0 per entry)
+  double r, ra, rb;  // pair distance and densities (kind 1; kind 0 per entry)
 };
 struct TfEnt {
   int ar, at, br, bt;  // residue index (phosphorylation, minimum separation) and type of each side
@@ -96,18 +100,20 @@ __device__ __forceinline__ double tf_dist(const double *x, const int *rinfo, int
 }
 
 // the well-0 density of cWell::compute_ro (sml.h:630-650, 563-588): pairs
-// of another chain or |i - j| > 1, theta 0 outside (rmin_theta, rmax_theta)
+// of another chain or |i - j| > 1, theta 0 outside (rmin_theta, rmax_theta).
+// One wave per residue: the lanes stride over j and the partial sums meet
+// in a fixed butterfly, so the value does not depend on the launch.
 __global__ void __launch_bounds__(256) k_tf_rho(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
                                                const double *__restrict__ x, int n, double *__restrict__ rho)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int i = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (i >= n) return;  // a whole wave leaves together
   const double *si = tf_site(x, rinfo, n, i);
-  double k = P->water_kappa, r0 = P->well_r_min[0], r1 = P->well_r_max[0];
-  double lo2 = P->w_rmin_th_sq[0], hi2 = P->w_rmax_th_sq[0];
-  int ci = rinfo[i] >> RI_CHAIN_SHIFT;
+  const double k = P->water_kappa, r0 = P->well_r_min[0], r1 = P->well_r_max[0];
+  const double lo2 = P->w_rmin_th_sq[0], hi2 = P->w_rmax_th_sq[0];
+  const int ci = rinfo[i] >> RI_CHAIN_SHIFT;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) {
+  for (int j = lane; j < n; j += 64) {
     if (!((rinfo[j] >> RI_CHAIN_SHIFT) != ci || abs(j - i) > 1)) continue;
     const double *sj = tf_site(x, rinfo, n, j);
     double dx = si[0] - sj[0], dy = si[1] - sj[1], dz = si[2] - sj[2];
@@ -116,7 +122,8 @@ __global__ void __launch_bounds__(256) k_tf_rho(const FFParams *__restrict__ P, 
     double r = sqrt(r2);
     s += 0.25 * (1.0 + tanh(k * (r - r0))) * (1.0 + tanh(k * (r1 - r)));
   }
-  rho[i] = s;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) rho[i] = s;
 }
 
 __device__ __forceinline__ const double (*tf_wg(const FFParams *P, int ri, int rj))[20][20][2]
@@ -181,28 +188,22 @@ __global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P,
   // base: the pair (kinds 0, 1) or the residue's burial (kind 2)
   for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
     TfEnt E = ent[R.off + e];
-    double v;
-    if (R.kind == 2) {
-      v = tf_burial(P, rinfo, E.ar, E.at, R.ra);
-    } else {
-      double r = R.r, ra = R.ra, rb = R.rb;
-      if (R.kind == 0) {
-        TfGeo G = geo[R.off + e];
-        r = G.r; ra = G.ra; rb = G.rb;
-      }
-      double td, tm;
-      tf_thetas(P, r, td, tm);
-      v = tf_water(P, rinfo, E.ar, E.br, E.at, E.bt, td, tm, 1.0 - tanh(ks * (ra - th)), 1.0 - tanh(ks * (rb - th)));
-      v += tf_burial(P, rinfo, E.ar, E.at, ra) + tf_burial(P, rinfo, E.br, E.bt, rb);
-      if (dh) v += tf_elec(P, E.ar, E.br, E.at, E.bt, exp(-P->dh_kscreen * r / P->dh_length) / r);
+    double r = R.r, ra = R.ra, rb = R.rb;
+    if (R.kind == 0) {
+      TfGeo G = geo[R.off + e];
+      r = G.r; ra = G.ra; rb = G.rb;
     }
+    double td, tm;
+    tf_thetas(P, r, td, tm);
+    double v = tf_water(P, rinfo, E.ar, E.br, E.at, E.bt, td, tm, 1.0 - tanh(ks * (ra - th)), 1.0 - tanh(ks * (rb - th)));
+    v += tf_burial(P, rinfo, E.ar, E.at, ra) + tf_burial(P, rinfo, E.br, E.bt, rb);
+    if (dh) v += tf_elec(P, E.ar, E.br, E.at, E.bt, exp(-P->dh_kscreen * r / P->dh_length) / r);
     out[R.off + e] = v;
   }
   if (R.kind == 0) return;
-  // the environment: mutational sides a and b over k != a, b (water within
-  // the cutoff, Debye-Hueckel always, fb.cpp:5211-5239); single residue a
-  // over k != a (water within the cutoff and the contact separation, fb.cpp:5362-5390)
-  const int ns = R.kind == 1 ? 2 : 1;
+  // the mutational environment: sides a and b over k != a, b (water within
+  // the cutoff, Debye-Hueckel always, fb.cpp:5211-5239)
+  const int ns = 2;
   const int side[2] = {R.a, R.b};
   const double hs[2] = {1.0 - tanh(ks * (R.ra - th)), 1.0 - tanh(ks * (R.rb - th))};
   const double cut = P->tf_cutoff;
@@ -215,11 +216,9 @@ __global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P,
       c_tk[q] = rinfo[k] & RI_TYPE_MASK;
       for (int s = 0; s < ns; ++s) {
         int a = side[s];
-        bool skip = R.kind == 1 ? (k == R.a || k == R.b) : (k == a);
+        bool skip = k == R.a || k == R.b;
         double r = tf_dist(x, rinfo, n, a, k);
         bool wat = !skip && r < cut;
-        if (R.kind == 2)
-          wat = wat && (abs(a - k) >= P->contact_cutoff || (rinfo[a] >> RI_CHAIN_SHIFT) != (rinfo[k] >> RI_CHAIN_SHIFT));
         double td = 0.0, tm = 0.0;
         if (wat) tf_thetas(P, r, td, tm);
         c_td[s][q] = td;
@@ -346,27 +345,32 @@ __global__ void __launch_bounds__(256) k_nm_pairs(const FFParams *__restrict__ P
   }
 }
 
-// single-n-mer mode: the energy of residue j with type t
-// (compute_singleresidue_native_ixn with nmercalc, fb.cpp:5346-5393: its
-// burial, its water with the residues k < j within the cutoff and the
-// contact separation or another chain, its Debye-Hueckel with every k < j),
-// for every (j, t): an entry is then a sum of s table values
+// the energy of residue j with type t for every (j, t)
+// (compute_singleresidue_native_ixn, fb.cpp:5346-5393: its burial, its
+// water with the other residues within the cutoff and the contact
+// separation or another chain, its Debye-Hueckel with every other residue;
+// with lower_only (nmercalc) the residues k < j only), summed as the
+// reference sums: water + burial + electrostatics.  A single-residue row's
+// entry is then one table value, a single n-mer's a sum of s of them.
 __global__ void __launch_bounds__(256) k_nm_single(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
                                                   const double *__restrict__ x, int n,
-                                                  const double *__restrict__ rho, double *__restrict__ e1)
+                                                  const double *__restrict__ rho, double cut, int lower_only,
+                                                  double *__restrict__ e1)
 {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * 20) return;
   const int j = q / 20, t = q % 20;
-  const double ks = P->water_kappa_sigma, th = P->treshold, cut = P->nf_cutoff;
+  const double ks = P->water_kappa_sigma, th = P->treshold;
   const bool dh = P->dh_flag != 0;
   const double hj = 1.0 - tanh(ks * (rho[j] - th));
   const int cj = rinfo[j] >> RI_CHAIN_SHIFT;
   double w = 0.0, el = 0.0;
-  for (int k = 0; k < j; ++k) {
+  const int kend = lower_only ? j : n;
+  for (int k = 0; k < kend; ++k) {
+    if (k == j) continue;
     double r = tf_dist(x, rinfo, n, j, k);
     int kt = rinfo[k] & RI_TYPE_MASK;
-    if (r < cut && (j - k >= P->contact_cutoff || (rinfo[k] >> RI_CHAIN_SHIFT) != cj)) {
+    if (r < cut && (abs(j - k) >= P->contact_cutoff || (rinfo[k] >> RI_CHAIN_SHIFT) != cj)) {
       double td, tm;
       tf_thetas(P, r, td, tm);
       w += tf_water(P, rinfo, j, k, t, kt, td, tm, hj, 1.0 - tanh(ks * (rho[k] - th)));
@@ -405,10 +409,11 @@ struct FfItem {
 };
 
 // the Gaussians of one item: target residues i, j with distance r[q] per
-// atom pair q, added to E[i] and E[j] (fb.cpp:4497-4561)
-__device__ __forceinline__ void ff_pair(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
-                                        const double *__restrict__ gsep, const FfItem &I, double w, int i, int j,
-                                        const double r[4], double *__restrict__ E)
+// atom pair q; the reference adds each to E[i] and E[j] (fb.cpp:4497-4561),
+// here their sum is returned and added to both once
+__device__ __forceinline__ double ff_pair(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                          const double *__restrict__ gsep, const FfItem &I, double w, int i, int j,
+                                          const double r[4])
 {
   const int ti = rinfo[i] & RI_TYPE_MASK, tj = rinfo[j] & RI_TYPE_MASK, d = j - i;
   double g = gsep[(size_t)d * 400 + ti * 20 + tj];
@@ -417,13 +422,13 @@ __device__ __forceinline__ void ff_pair(const FFParams *__restrict__ P, const in
   double s2 = pow((double)d, 2 * P->fm_sigma_exp);
   s2 = s2 * P->ff_ww * P->ff_ww;
   const bool gi = (rinfo[i] & RI_GLY) || (I.gly & 1), gj = (rinfo[j] & RI_GLY) || (I.gly & 2);
+  double v = 0.0;
   for (int q = 0; q < 4; ++q) {
     if ((q >= 2 && gi) || ((q & 1) && gj)) continue;
     double dr = r[q] - I.rf[q];
-    double V = -ekwg * exp(-(dr * dr) / (2 * s2));
-    atomicAdd(&E[i], V);
-    atomicAdd(&E[j], V);
+    v += -ekwg * exp(-(dr * dr) / (2 * s2));
   }
+  return v;
 }
 
 __device__ __forceinline__ double ff_r(const double *a, const double *b)
@@ -453,7 +458,46 @@ __global__ void __launch_bounds__(256) k_ff_mem(const FFParams *__restrict__ P, 
   const double *ci = x + (size_t)i * 3, *cj = x + (size_t)j * 3;
   const double *bi = x + ((size_t)n + i) * 3, *bj = x + ((size_t)n + j) * 3;
   const double r[4] = {ff_r(ci, cj), ff_r(ci, bj), ff_r(bi, cj), ff_r(bi, bj)};
-  ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r, E + (size_t)c * n);
+  const double v = ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r);
+  atomicAdd(&E[(size_t)c * n + i], v);
+  atomicAdd(&E[(size_t)c * n + j], v);
+}
+
+// the same with the calculation's n energies accumulated in LDS by each
+// workgroup over FF_CHUNK items and added to E once per workgroup: the
+// items of one calculation hit few residues, so global atomics would
+// serialise on them (n * 8 bytes of LDS, n <= FF_LDS_MAX)
+#define FF_CHUNK 2048
+#define FF_LDS_MAX 8192
+__global__ void __launch_bounds__(256) k_ff_mem_lds(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                                   const double *__restrict__ x, int n,
+                                                   const FfItem *__restrict__ items, int n_items,
+                                                   const int *__restrict__ pos, const double *__restrict__ w, int nm,
+                                                   const double *__restrict__ gsep, double *__restrict__ E,
+                                                   int *__restrict__ err)
+{
+  extern __shared__ double e_l[];
+  const int c = blockIdx.y;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) e_l[q] = 0.0;
+  __syncthreads();
+  const int t1 = min(n_items, (int)(blockIdx.x + 1) * FF_CHUNK);
+  for (int t = blockIdx.x * FF_CHUNK + threadIdx.x; t < t1; t += blockDim.x) {
+    const FfItem I = items[t];
+    const int p = pos[(size_t)c * nm + I.k], i = p + I.di, j = p + I.dj;
+    if ((rinfo[i] >> RI_CHAIN_SHIFT) != (rinfo[j] >> RI_CHAIN_SHIFT)) {
+      atomicOr(err, 1);
+      continue;
+    }
+    const double *ci = x + (size_t)i * 3, *cj = x + (size_t)j * 3;
+    const double *bi = x + ((size_t)n + i) * 3, *bj = x + ((size_t)n + j) * 3;
+    const double r[4] = {ff_r(ci, cj), ff_r(ci, bj), ff_r(bi, cj), ff_r(bi, bj)};
+    const double v = ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r);
+    atomicAdd(&e_l[i], v);
+    atomicAdd(&e_l[j], v);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += blockDim.x)
+    if (e_l[q] != 0.0) atomicAdd(&E[(size_t)c * n + q], e_l[q]);
 }
 
 // read mode's decoy energies: the memories at their positions compared with
@@ -487,7 +531,9 @@ __global__ void __launch_bounds__(256) k_ff_gen(const FFParams *__restrict__ P, 
     r[2] = ff_r(cb + 3 * a, ca + 3 * b);
     r[3] = ff_r(cb + 3 * a, cb + 3 * b);
   }
-  ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r, G + (size_t)d * n);
+  const double v = ff_pair(P, rinfo, gsep, I, w[I.k], i, j, r);
+  atomicAdd(&G[(size_t)d * n + i], v);
+  atomicAdd(&G[(size_t)d * n + j], v);
 }
 
 // ---------------------------------------------------------------------------
@@ -558,7 +604,7 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
   TF_HIPCHK(hipMalloc((void **)&d_rho, (size_t)n * sizeof(double)));
   DevBuf rho_buf;
   rho_buf.p = d_rho;
-  hipLaunchKernelGGL(k_tf_rho, dim3((n + 255) / 256), dim3(256), 0, st, D.params, D.rinfo, D.x, n, d_rho);
+  hipLaunchKernelGGL(k_tf_rho, dim3((n + 3) / 4), dim3(256), 0, st, D.params, D.rinfo, D.x, n, d_rho);
   std::vector<double> rho(n);
   TF_HIPCHK(hipMemcpyAsync(rho.data(), d_rho, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
   TF_HIPCHK(hipStreamSynchronize(st));
@@ -624,7 +670,19 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
     }
   }
   std::vector<double> e((size_t)off);
-  if (!rows.empty()) {
+  if (P.tf_mode == 2) {
+    // a single-residue entry depends on (residue, type) only: one table of
+    // n x 20 energies (compute_singleresidue_native_ixn, fb.cpp:5346-5393)
+    DevBuf b_t;
+    TF_HIPCHK(hipMalloc(&b_t.p, (size_t)n * 20 * sizeof(double)));
+    hipLaunchKernelGGL(k_nm_single, dim3((n * 20 + 255) / 256), dim3(256), 0, st, D.params, D.rinfo, D.x, n, d_rho,
+                       cut, 0, (double *)b_t.p);
+    TF_HIPCHK(hipGetLastError());
+    std::vector<double> T((size_t)n * 20);
+    TF_HIPCHK(hipMemcpyAsync(T.data(), b_t.p, T.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    TF_HIPCHK(hipStreamSynchronize(st));
+    for (size_t k = 0; k < e.size(); ++k) e[k] = T[(size_t)ent[k].ar * 20 + ent[k].at];
+  } else if (!rows.empty()) {
     DevBuf b_rows, b_ent, b_geo, b_out;
     if (geo.empty()) geo.push_back(TfGeo{0, 0, 0});
     if (!(s = b_rows.put(rows, st)).ok() || !(s = b_ent.put(ent, st)).ok() || !(s = b_geo.put(geo, st)).ok()) return s;
@@ -686,7 +744,7 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
   DevBuf rho_buf;
   TF_HIPCHK(hipMalloc(&rho_buf.p, (size_t)n * sizeof(double)));
   double *d_rho = (double *)rho_buf.p;
-  hipLaunchKernelGGL(k_tf_rho, dim3((n + 255) / 256), dim3(256), 0, stream, D.params, D.rinfo, D.x, n, d_rho);
+  hipLaunchKernelGGL(k_tf_rho, dim3((n + 3) / 4), dim3(256), 0, stream, D.params, D.rinfo, D.x, n, d_rho);
   TF_HIPCHK(hipGetLastError());
   std::vector<double> e;
   table.clear();
@@ -708,7 +766,7 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
     DevBuf e1, b_src, b_out;
     TF_HIPCHK(hipMalloc(&e1.p, (size_t)n * 20 * sizeof(double)));
     hipLaunchKernelGGL(k_nm_single, dim3((n * 20 + 255) / 256), dim3(256), 0, stream, D.params, D.rinfo, D.x, n,
-                       d_rho, (double *)e1.p);
+                       d_rho, P.nf_cutoff, 1, (double *)e1.p);
     TF_HIPCHK(hipGetLastError());
     if (!(st = b_src.put(src, stream)).ok()) return st;
     TF_HIPCHK(hipMalloc(&b_out.p, src.size() * sizeof(double)));
@@ -893,12 +951,22 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
   TF_HIPCHK(hipMemsetAsync(b_E.p, 0, (size_t)ncalc_dev * n * sizeof(double), stream));
   double *E = (double *)b_E.p;
   int *d_err = (int *)b_err.p;
-  if (!nat_items.empty()) {
-    hipLaunchKernelGGL(k_ff_mem, dim3((unsigned)((nat_items.size() + 255) / 256), 1), dim3(256), 0, stream, D.params,
-                       D.rinfo, D.x, n, (const FfItem *)b_ni.p, (int)nat_items.size(), (const int *)b_np.p,
-                       (const double *)b_nw.p, nfm, (const double *)b_g.p, E, d_err);
-    TF_HIPCHK(hipGetLastError());
-  }
+  // a library's energies for `calcs` calculations (LDS accumulation when
+  // the residues fit)
+  auto launch_mem = [&](const void *items, int n_items, const void *pos, const void *w, int nm, int calcs,
+                        double *out) {
+    if (n <= FF_LDS_MAX)
+      hipLaunchKernelGGL(k_ff_mem_lds, dim3((unsigned)((n_items + FF_CHUNK - 1) / FF_CHUNK), (unsigned)calcs),
+                         dim3(256), (size_t)n * sizeof(double), stream, D.params, D.rinfo, D.x, n,
+                         (const FfItem *)items, n_items, (const int *)pos, (const double *)w, nm,
+                         (const double *)b_g.p, out, d_err);
+    else
+      hipLaunchKernelGGL(k_ff_mem, dim3((unsigned)((n_items + 255) / 256), (unsigned)calcs), dim3(256), 0, stream,
+                         D.params, D.rinfo, D.x, n, (const FfItem *)items, n_items, (const int *)pos,
+                         (const double *)w, nm, (const double *)b_g.p, out, d_err);
+    return hipGetLastError();
+  };
+  if (!nat_items.empty()) TF_HIPCHK(launch_mem(b_ni.p, (int)nat_items.size(), b_np.p, b_nw.p, nfm, 1, E));
   DevBuf b_di, b_dp, b_dw, b_dx, b_dinfo, b_G;
   if (P.ff_mode == 0) {
     // randomize_decoys after every calculation (fb.cpp:4565-4611, 7546-7553):
@@ -914,10 +982,7 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
     if (!dec_items.empty()) {
       if (!(st = b_di.put(dec_items, stream)).ok() || !(st = b_dp.put(pos, stream)).ok() || !(st = b_dw.put(dec_w, stream)).ok())
         return st;
-      hipLaunchKernelGGL(k_ff_mem, dim3((unsigned)((dec_items.size() + 255) / 256), (unsigned)(nc - 1)), dim3(256), 0,
-                         stream, D.params, D.rinfo, D.x, n, (const FfItem *)b_di.p, (int)dec_items.size(),
-                         (const int *)b_dp.p, (const double *)b_dw.p, nd, (const double *)b_g.p, E + n, d_err);
-      TF_HIPCHK(hipGetLastError());
+      TF_HIPCHK(launch_mem(b_di.p, (int)dec_items.size(), b_dp.p, b_dw.p, nd, nc - 1, E + n));
     }
   }
   const bool gen_now = P.ff_mode == 1 && !F.ff_gen_done;
@@ -1003,7 +1068,7 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
 // leaves the stream where it was
 struct TfSaved {
   uint32_t rng[34];
-  long long k;
+  int k;
   bool done;
   double stats[2];
   explicit TfSaved(const AwFrustState &F) : k(F.rng_k), done(F.done)

@@ -235,7 +235,7 @@ long long aw_slab_doubles(int n);
 // fb.cpp:5099, 5342) and the output files
 struct AwFrustState {
   uint32_t rng[34];
-  long long rng_k = 0;
+  int rng_k = 0;  // ring slot of the next value
   bool done = false;
   double stats[2] = {0.0, 0.0};
   FILE *dat = nullptr, *tcl = nullptr;

@@ -22,7 +22,8 @@ accumulation), whole job, inputs resident in HBM.  Reported beside it:
   * the other workloads, bounded: one N=1000 chain through the FP64 drop-in;
     64 replicas of an 8 x 300-residue
     complex; the serving mix at offered arrival rates of 200-1600 jobs/s
-    (below and past saturation).
+    (below and past saturation); fragment-memory table mode; each
+    frustratometer mode (ms per call, decoy energies/s beside the oracle).
 
     python bench.py [--gpus 1] [--steps 640] [--warmup 20]
 """
@@ -480,6 +481,109 @@ def run_fm_table(a, A, W):
             "final_energies_finite": tb["finite"] and mm["finite"]}
 
 
+# the frustratometer line: (name, section text with {m} decoys / calculations, GPU m, CPU m)
+FRUST_MODES = [
+    ("tertiary configurational", "[Tertiary_Frustratometer]\n9.5\n{m}\n1\nconfigurational", 1000, 1000),
+    ("tertiary mutational", "[Tertiary_Frustratometer]\n9.5\n{m}\n1\nmutational", 1000, 200),
+    ("tertiary singleresidue", "[Tertiary_Frustratometer]\n9.5\n{m}\n1\nsingleresidue", 1000, 200),
+    ("nmer pairwise", "[Nmer_Frustratometer]\n6\n9.5\n10\n{m}\n1\n1.7 -1 0\n0 0 1 0.6\npairwise", 1000, 200),
+    ("nmer singlenmer", "[Nmer_Frustratometer]\n6\n9.5\n10\n{m}\n1\n1.7 -1 0\n0 0 1 0.6\nsinglenmer", 1000, 200),
+    ("fragment shuffle", "[Fragment_Frustratometer]\nshuffle\nfragsLAMW.mem\n{m}\n1", 101, 41),
+]
+
+
+def _frust_call(fix_or_oracle, kind, x, oracle=False):
+    """One frustratometer call; returns its row count and decoy entries."""
+    if kind.startswith("tertiary"):
+        r = fix_or_oracle.tert_frust(x)
+    elif kind.startswith("nmer"):
+        r = fix_or_oracle.nmer_frust(x)[0]
+    else:
+        r = fix_or_oracle.frag_frust(x, 0)[0] if oracle else fix_or_oracle.frag_frust(x)[0]
+    return len(r)
+
+
+def _frust_systems(W, seed):
+    """Two compact chains of 150 residues (tertiary and n-mer modes) and one
+    compact 300-residue chain whose 5,840 memories are also the shuffled
+    decoy fragments (fragment mode: decoys placed across chains are an error
+    in the reference)."""
+    return W.make_system([150, 150], seed=seed, compact=True), W.make_system([300], seed=seed + 1, compact=True)
+
+
+def _frust_workdir(W, s, section, d):
+    p = W.write_system(s, d)
+    with open(p["coeff"], "a") as f:
+        f.write("\n" + section + "\n\n")
+    return p
+
+
+def _entries(kind, rows, m, n):
+    """Decoy energies one call evaluates (the natives included)."""
+    if kind == "tertiary configurational":
+        return rows  # the decoy statistics are drawn once, at the first call
+    if kind == "fragment shuffle":
+        return m * n  # m calculations of every residue's memory energy
+    return rows * (m + 1)
+
+
+def frustratometer_cpu_baseline(W, seed):
+    """The oracle (FP64 C restatement, 1 core) on the same systems with fewer
+    decoys (CPU m per mode), one call per mode after a warm-up call: decoy
+    energies per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sysA, sysB = _frust_systems(W, seed)
+    out = {}
+    for kind, sec, _, m in FRUST_MODES:
+        s = sysB if kind.startswith("fragment") else sysA
+        p = _frust_workdir(W, s, sec.format(m=m), tempfile.mkdtemp(prefix="awsem_fcpu_"))
+        o = oracle.Oracle(p["coeff"], p["seq"], p["workdir"])
+        x = s.coords()
+        _frust_call(o, kind, x, oracle=True)
+        t0 = time.perf_counter()
+        rows = _frust_call(o, kind, x, oracle=True)
+        t = time.perf_counter() - t0
+        out[kind] = {"entries_per_s": _entries(kind, rows, m, s.n) / t, "decoys": m, "s_per_call": t}
+    return out
+
+
+def run_frustratometers(a, A, W):
+    """The analysis workload: each frustratometer mode on one GPU, FP64, the
+    steady-state wall time of one call (host decoy draws, uploads, kernels,
+    copies back) averaged over a.frust_calls calls after a warm-up call, and
+    the decoy energies per second, beside the oracle on one host core."""
+    sysA, sysB = _frust_systems(W, a.seed)
+    res = {}
+    for kind, sec, m, _ in FRUST_MODES:
+        s = sysB if kind.startswith("fragment") else sysA
+        p = _frust_workdir(W, s, sec.format(m=m), tempfile.mkdtemp(prefix="awsem_frust_"))
+        fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+        x = s.coords()
+        _frust_call(fix, kind, x)
+        t0 = time.perf_counter()
+        for _ in range(a.frust_calls):
+            rows = _frust_call(fix, kind, x)
+        t = (time.perf_counter() - t0) / a.frust_calls
+        fix.close()
+        res[kind] = {"ms_per_call": 1e3 * t, "rows": rows, "decoys": m, "residues": s.n,
+                     "entries_per_s": _entries(kind, rows, m, s.n) / t}
+        log(f"frustratometer {kind}: {1e3 * t:.2f} ms per call, {rows} rows, {m} decoys")
+    cpu = None if a.no_cpu else frustratometer_cpu_baseline(W, a.seed)
+    if cpu:
+        for kind in res:
+            res[kind]["cpu_entries_per_s"] = cpu[kind]["entries_per_s"]
+            res[kind]["speedup_vs_cpu"] = res[kind]["entries_per_s"] / cpu[kind]["entries_per_s"]
+    return {"workload": "frustratometers on 2 x 150 compact chains (tertiary, n-mer: cutoff 9.5 A, 1000 decoys; "
+                        "n-mers of 6 with 10 contacts) and one 300-residue chain (fragment shuffle: its 5,840 "
+                        "memories as decoy fragments, 100 decoy calculations)",
+            "unit": "decoy energies/s (and ms per call)", "dtype": "f64", "modes": res,
+            "cpu_baseline": {"cores": 1, "kind": "port",
+                             "sample": "oracle, same systems, 200 decoys per row (configurational 1000, its "
+                                       "statistics drawn once; fragment shuffle 40 decoy calculations), one call "
+                                       "after a warm-up call"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -507,10 +611,12 @@ def main():
                     help="timed steps of the fragment-memory table-mode line in the default run (0: skip)")
     ap.add_argument("--table-replicas", type=int, default=1024)
     ap.add_argument("--table-len", type=int, default=63)
-    ap.add_argument("--workload", choices=["batch", "single", "multimer", "table"], default="batch",
+    ap.add_argument("--frust-calls", type=int, default=5,
+                    help="timed calls per frustratometer mode in the default run (0: skip the line)")
+    ap.add_argument("--workload", choices=["batch", "single", "multimer", "table", "frust"], default="batch",
                     help="batch: 256 chains (default, the headline); single: one N=1000 chain through the "
                          "FP64 drop-in; multimer: 64 x (8 x 300) annealing; table: fragment-memory table mode "
-                         "on replicas of one 63-residue chain")
+                         "on replicas of one 63-residue chain; frust: the frustratometer modes")
     ap.add_argument("--n-single", type=int, default=1000)
     ap.add_argument("--check-every", type=int, default=100)
     ap.add_argument("--anneal-steps", type=int, default=20000)
@@ -535,6 +641,9 @@ def main():
         if a.replicas == 256:
             a.replicas = 64
         print(json.dumps(run_multimer(a, A, W)), flush=True)
+        return
+    if a.workload == "frust":
+        print(json.dumps(run_frustratometers(a, A, W)), flush=True)
         return
     if a.workload == "table":
         print(json.dumps(run_fm_table(a, A, W)), flush=True)
@@ -670,6 +779,9 @@ def main():
         a4 = argparse.Namespace(**vars(a))
         a4.warmup = 20
         table = run_fm_table(a4, A, W)
+    frust = None
+    if a.frust_calls > 0:
+        frust = run_frustratometers(a, A, W)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(systems, a.cpu_budget)
@@ -714,6 +826,7 @@ def main():
         "single_chain_fp64": single,
         "multimer": multimer,
         "fm_table": table,
+        "frustratometers": frust,
         "setup_s": {"generate": t_gen, "build": t_build},
     }
     print(json.dumps(out))
