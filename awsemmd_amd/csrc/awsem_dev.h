@@ -49,6 +49,7 @@ struct BatchDev {
   int nshare;
   const int *share_rep, *share_n;       // [nshare] replica, chunks
   unsigned long long *share_word;       // [nshare] claim words
+  int share_spread;                     // a helper's first look: the (blockIdx mod spread)-th open replica (power of 2)
 };
 
 // bytes of a workgroup's LDS left to the resident kernel's static arrays
@@ -106,6 +107,7 @@ static inline BatchDev to_dev(const AwBatchPtrs &p)
   B.share_rep = p.share_rep;
   B.share_n = p.share_n;
   B.share_word = p.share_word;
+  B.share_spread = p.share_spread;
   return B;
 }
 
@@ -248,7 +250,9 @@ __device__ void share_help(const BatchDev &B, double *smem)
     __syncthreads();  // the previous round's broadcast has been read
     if (wave == 0) {
       int pick = -1;
-      int rot = blockIdx.x & 63;
+      // the first open replica, longest first, from the (blockIdx mod
+      // spread)-th on (spread 1: strictly the longest open one)
+      int rot = blockIdx.x & (B.share_spread - 1);
       for (int b = 0; b < B.nshare && pick < 0; b += 64) {
         int q = b + ((lane + rot) & 63);
         bool open = false;

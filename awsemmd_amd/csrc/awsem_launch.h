@@ -24,6 +24,7 @@ struct AwBatchPtrs {
   const int *share_rep = nullptr;             // [nshare] replica index
   const int *share_n = nullptr;               // [nshare] chunks
   unsigned long long *share_word = nullptr;   // [nshare] claim words (ReplicaDesc::sh_word)
+  int share_spread = 1;                       // helpers start their scan at blockIdx mod this (power of 2, <= 64)
 };
 
 struct AwRun {
