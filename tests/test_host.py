@@ -284,3 +284,50 @@ def test_extra_term_errors(tmp_path):
         (d / "fix_backbone_coeff.data").write_text("[Phosphorylation]\n1.2\n1\n3\n\n" + txt)
     err = variant("phos", phos_first)
     assert err.code == -3 and "phosphorylation without water" in str(err)
+
+
+def test_frustratometer_sections_and_errors(tmp_path):
+    """The three frustratometer sections (fb.cpp:359-412): their intervals
+    and modes, the reference's messages for a bad mode and for a fragment
+    frustratometer without fragment memory, a missing decoy library, and
+    AWSEM_E_STATE from a call on a context without the section (no device
+    work)."""
+    A = _A()
+    s, p = make_case([30, 20], 5)
+    base = open(p["coeff"]).read()
+
+    def fix_with(text, fm=True):
+        c = tmp_path / ("coeff_%d" % (abs(hash(text)) % 10**8))
+        c.write_text((base if fm else base.replace("[Fragment_Memory]\n", "[Fragment_Memory]-\n")) + "\n" + text + "\n\n")
+        return A.FixBackbone(str(c), p["seq"], p["workdir"])
+
+    f = fix_with("[Tertiary_Frustratometer]\n9.5\n10\n7\nmutational\n\n"
+                 "[Nmer_Frustratometer]\n6\n9.5\n10\n10\n3\n1.7 -1 0\n0 0 1 0.6\nsinglenmer\n\n"
+                 "[Fragment_Frustratometer]\nshuffle\nfragsLAMW.mem\n11\n5")
+    assert f.tert_frust_every() == 7 and f.nmer_frust_every() == 3
+    mode, nc, nd, every = f.frag_frust_info()
+    assert (mode, nc, every) == (0, 11, 5) and nd == len(s.memories)
+    f.close()
+    g = fix_with("")
+    assert g.tert_frust_every() == 0 and g.nmer_frust_every() == 0 and g.frag_frust_info()[0] == -1
+    x = s.coords()
+    for call in (lambda: g.tert_frust(x), lambda: g.nmer_frust(x), lambda: g.frag_frust(x)):
+        with pytest.raises(A.AwsemError) as e:
+            call()
+        assert e.value.code == -10
+    g.close()
+    cases = [
+        ("[Tertiary_Frustratometer]\n9.5\n10\n1\nsideways", True,
+         'Only "configurational", "mutational", "singleresidue" are acceptable modes for the Tertiary_Frustratometer.'),
+        ("[Nmer_Frustratometer]\n6\n9.5\n10\n10\n1\n1.7 -1 0\n0 0 1 0.6\ntriple", True,
+         'Only "pairwise", "singlenmer" are acceptable modes for the Nmer_Frustratometer.'),
+        ("[Fragment_Frustratometer]\nsideways\nfragsLAMW.mem\n1", True,
+         'Only "shuffle" and "read" are acceptable modes for the Fragment_Frustratometer.'),
+        ("[Fragment_Frustratometer]\nshuffle\nfragsLAMW.mem\n10\n1", False,
+         "Cannot run Fragment_Frustratometer without Fragment_Memory or Fragment_Memory_Table."),
+        ("[Fragment_Frustratometer]\nread\nnone.mem\n1\n1.0\n0 0\n0", True, "Error opening mem file"),
+    ]
+    for text, fm, msg in cases:
+        with pytest.raises(A.AwsemError) as e:
+            fix_with(text, fm)
+        assert msg in str(e.value), (text, str(e.value))
