@@ -692,6 +692,9 @@ AwStatus aw_ctx_device(awsem_ctx *c, const double *xres, AwCtxDev &out)
   if (!s.ok()) return s;
   DevBatch &D = *c->dev;
   HIPCHK(hipMemcpyAsync(D.d_x, xres, (size_t)c->sys.n * 9 * sizeof(double), hipMemcpyHostToDevice, D.stream));
+  // the context's coordinates changed: its list and forces are stale
+  D.nl_dirty = true;
+  D.f_valid = false;
   out.params = (const FFParams *)D.d_params;
   out.rinfo = D.reps[0].rinfo;
   out.x = D.d_x;

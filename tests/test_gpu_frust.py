@@ -89,3 +89,32 @@ def test_tert_frust_files(oracle_mod):
     nstrong = sum(1 for w in want for q in w if q[10] > 0.78 or q[10] < -1)
     assert sum(1 for ln in tcl if ln.startswith("draw line")) == nstrong
     assert tcl.count('mol modstyle 0 top newcartoon') == 3
+
+
+def test_frustratometers_leave_the_force_path_alone(oracle_mod):
+    """The frustratometers upload their coordinates into the context's
+    device buffers: forces and a short Langevin run on the same context
+    before and after frustratometer calls at other coordinates agree (to
+    the order of the FP64 atomic sums), and the forces equal the oracle's."""
+    A = _gpu()
+    s, p = frust_case([40, 30], 74, "mutational", ndecoys=50)
+    with open(p["coeff"], "a") as f:
+        f.write("\n[Nmer_Frustratometer]\n4\n9.5\n2\n20\n1\n0.78 -1 0\n0 0 1 0.6\npairwise\n\n")
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    x = s.coords()
+    v = np.random.default_rng(2).normal(scale=0.01, size=x.shape)
+    f0, e0 = fix.compute_res(x)
+    r0 = fix.run(x, v, 20, seed=5)
+    x_other = x + np.random.default_rng(3).normal(scale=0.3, size=x.shape)
+    fix.tert_frust(x_other)
+    fix.nmer_frust(x_other)
+    f1, e1 = fix.compute_res(x)
+    r1 = fix.run(x, v, 20, seed=5)
+    # equal up to the order of the FP64 atomic force sums
+    tol = lambda a, b: np.abs(a - b).max() <= 1e-11 * max(1.0, np.abs(b).max())
+    assert tol(f1, f0) and tol(e1, e0)
+    for a, b in zip(r0, r1):
+        assert tol(a, b)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    fo, eo = o.compute(x)
+    assert np.abs(f1 - fo).max() <= 1e-8 * max(1.0, np.abs(fo).max())
