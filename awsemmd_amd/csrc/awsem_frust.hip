@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -252,14 +253,16 @@ __global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P,
 
 // contacts between the n-mers at i and j (compute_nmer_contacts,
 // fb.cpp:5895-5919: the sequence-separation test only, no chain test) for
-// the start pairs i < j < n - s with j - i >= s; -1 for the others
+// the start pairs i < j < n - s with j - i >= s, rows i0 <= i < i1 (entry
+// (i - i0) ns + j); -1 for the other pairs
 __global__ void __launch_bounds__(256) k_nm_contacts(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
-                                                    const double *__restrict__ x, int n, int s, int *__restrict__ cnt)
+                                                    const double *__restrict__ x, int n, int s, int i0, int i1,
+                                                    int *__restrict__ cnt)
 {
   const int ns = n - s;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)ns * ns) return;
-  int i = (int)(t / ns), j = (int)(t % ns);
+  if (t >= (long long)(i1 - i0) * ns) return;
+  int i = i0 + (int)(t / ns), j = (int)(t % ns);
   if (!(j > i && j - i >= s)) {
     cnt[t] = -1;
     return;
@@ -789,22 +792,30 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
   // compute_nmer_frust, fb.cpp:5600-5657
   const int ns = n - s;
   if (ns <= 0) return AwStatus();
+  // contact counts a block of start rows at a time (at most 2^24 pairs;
+  // AWSEM_NMER_BLOCK sets another bound)
+  long long maxp = 1 << 24;
+  if (const char *e = std::getenv("AWSEM_NMER_BLOCK")) maxp = std::max(1LL, std::atoll(e));
+  const int rb = (int)std::max(1LL, std::min((long long)ns, maxp / ns));
   DevBuf b_cnt;
-  TF_HIPCHK(hipMalloc(&b_cnt.p, (size_t)ns * ns * sizeof(int)));
-  long long np = (long long)ns * ns;
-  hipLaunchKernelGGL(k_nm_contacts, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream, D.params, D.rinfo, D.x,
-                     n, s, (int *)b_cnt.p);
-  TF_HIPCHK(hipGetLastError());
-  std::vector<int> cnt((size_t)np);
-  TF_HIPCHK(hipMemcpyAsync(cnt.data(), b_cnt.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
-  TF_HIPCHK(hipStreamSynchronize(stream));
+  TF_HIPCHK(hipMalloc(&b_cnt.p, (size_t)rb * ns * sizeof(int)));
+  std::vector<int> cnt((size_t)rb * ns);
   const int nt = P.nf_trap ? 4 : 0;  // trap entries: (i, j) and (j, i), forward and backward
   std::vector<NmRow> rows;
   std::vector<NmEnt> ent;
   long long off = 0;
-  for (int i = 0; i < ns; ++i)
+  for (int i = 0; i < ns; ++i) {
+    if (i % rb == 0) {
+      const int i1 = std::min(ns, i + rb);
+      const long long np = (long long)(i1 - i) * ns;
+      hipLaunchKernelGGL(k_nm_contacts, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream, D.params, D.rinfo,
+                         D.x, n, s, i, i1, (int *)b_cnt.p);
+      TF_HIPCHK(hipGetLastError());
+      TF_HIPCHK(hipMemcpyAsync(cnt.data(), b_cnt.p, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, stream));
+      TF_HIPCHK(hipStreamSynchronize(stream));
+    }
     for (int j = i + 1; j < ns; ++j) {
-      int k = cnt[(size_t)i * ns + j];
+      int k = cnt[(size_t)(i % rb) * ns + j];
       if (!(k >= P.nf_contacts && j - i >= s)) continue;
       rows.push_back(NmRow{i, j, 1 + m + nt, k, off});
       ent.push_back(NmEnt{i, j, 1, 1, 0, 0});
@@ -828,6 +839,7 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
       }
       off += 1 + m + nt;
     }
+  }
   if (rows.empty()) return AwStatus();
   DevBuf b_rows, b_ent, b_out;
   if (!(st = b_rows.put(rows, stream)).ok() || !(st = b_ent.put(ent, stream)).ok()) return st;

@@ -111,3 +111,19 @@ def test_nmer_frust_files(oracle_mod):
     assert sum(1 for ln in tcl if ln == "draw color purple") == ntrap_lines
     assert sum(1 for ln in tcl if ln.startswith("draw line")) == ndraw + ntrap_lines
     assert tcl.count("mol modstyle 0 top newcartoon") == 2
+
+
+def test_nmer_contact_blocks(oracle_mod, monkeypatch):
+    """The contact counts computed a few start rows at a time (the bound on
+    the count buffer, here 300 pairs per block) give the same rows, traps
+    and rand() stream as the oracle."""
+    A = _gpu()
+    monkeypatch.setenv("AWSEM_NMER_BLOCK", "300")
+    s, p = nmer_case([40, 30], 86, 4, 3, 30, 1, "pairwise")
+    fix = A.FixBackbone(p["coeff"], p["seq"], p["workdir"], A.PREC_FP64)
+    o = oracle_mod.Oracle(p["coeff"], p["seq"], p["workdir"])
+    x = s.coords()
+    for call in range(2):
+        got, want = fix.nmer_frust(x), o.nmer_frust(x)
+        assert len(want[0]) > 10
+        _compare(got, want)
