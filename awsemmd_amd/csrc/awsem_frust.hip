@@ -78,7 +78,9 @@ int AwFrustState::next()
 struct TfRow {
   int kind;        // 0 pair (configurational), 1 pair + environment (mutational)
   int a, b, cnt;   // residues; entries
-  long long off;   // first entry
+  int nat, pad;    // 1: entry 0 is the native (its decoys follow), 0: every entry is a decoy
+  long long off;   // first entry (in the energies, and for kind 0 in ent / geo)
+  long long doff;  // kind 1: the decoys' residue pairs at draws[doff + 2 d], [doff + 2 d + 1]
   double r, ra, rb;  // pair distance and densities (kind 1; kind 0 per entry)
 };
 struct TfEnt {
@@ -176,34 +178,45 @@ __device__ __forceinline__ double tf_elec(const FFParams *P, int ir, int jr, int
 
 #define TF_CH 512  // residues k per LDS chunk
 
-__global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
-                                                const double *__restrict__ x, int n, const double *__restrict__ rho,
-                                                const TfRow *__restrict__ rows, const TfEnt *__restrict__ ent,
-                                                const TfGeo *__restrict__ geo, double *__restrict__ out)
+// entry e of a row: the residues whose types (and indices) it assigns
+__device__ __forceinline__ TfEnt tf_entry(const TfRow &R, int e, const TfEnt *__restrict__ ent,
+                                          const int *__restrict__ draws, const int *__restrict__ rinfo)
 {
-  __shared__ double c_td[2][TF_CH], c_tm[2][TF_CH], c_es[2][TF_CH], c_hk[TF_CH];
-  __shared__ int c_on[2][TF_CH], c_tk[TF_CH];
-  const TfRow R = rows[blockIdx.x];
+  if (R.kind == 0) return ent[R.off + e];
+  int a = R.a, b = R.b;
+  if (e > 0) {
+    a = draws[R.doff + 2 * (e - 1)];
+    b = draws[R.doff + 2 * (e - 1) + 1];
+  }
+  return TfEnt{a, rinfo[a] & RI_TYPE_MASK, b, rinfo[b] & RI_TYPE_MASK};
+}
+
+// a workgroup's sum of one value per thread (fixed tree: deterministic)
+__device__ __forceinline__ double tf_block_sum(double v, double *red)
+{
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  double s = red[0];
+  __syncthreads();
+  return s;
+}
+
+// the mutational environment: sides a and b over k != a, b (water within
+// the cutoff, Debye-Hueckel always, fb.cpp:5211-5239), added to the row's
+// entries; the k-dependent factors staged in LDS a chunk of TF_CH at a time
+__device__ __forceinline__ void tf_env_pass(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                            const double *__restrict__ x, int n, const double *__restrict__ rho,
+                                            const TfRow &R, const TfEnt *__restrict__ ent,
+                                            const int *__restrict__ draws, double *__restrict__ out,
+                                            double (*c_td)[TF_CH], double (*c_tm)[TF_CH], double (*c_es)[TF_CH],
+                                            double *c_hk, int (*c_on)[TF_CH], int *c_tk)
+{
   const bool dh = P->dh_flag != 0;
   const double ks = P->water_kappa_sigma, th = P->treshold;
-  // base: the pair (kinds 0, 1) or the residue's burial (kind 2)
-  for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
-    TfEnt E = ent[R.off + e];
-    double r = R.r, ra = R.ra, rb = R.rb;
-    if (R.kind == 0) {
-      TfGeo G = geo[R.off + e];
-      r = G.r; ra = G.ra; rb = G.rb;
-    }
-    double td, tm;
-    tf_thetas(P, r, td, tm);
-    double v = tf_water(P, rinfo, E.ar, E.br, E.at, E.bt, td, tm, 1.0 - tanh(ks * (ra - th)), 1.0 - tanh(ks * (rb - th)));
-    v += tf_burial(P, rinfo, E.ar, E.at, ra) + tf_burial(P, rinfo, E.br, E.bt, rb);
-    if (dh) v += tf_elec(P, E.ar, E.br, E.at, E.bt, exp(-P->dh_kscreen * r / P->dh_length) / r);
-    out[R.off + e] = v;
-  }
-  if (R.kind == 0) return;
-  // the mutational environment: sides a and b over k != a, b (water within
-  // the cutoff, Debye-Hueckel always, fb.cpp:5211-5239)
   const int ns = 2;
   const int side[2] = {R.a, R.b};
   const double hs[2] = {1.0 - tanh(ks * (R.ra - th)), 1.0 - tanh(ks * (R.rb - th))};
@@ -230,7 +243,7 @@ __global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P,
     }
     __syncthreads();
     for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
-      TfEnt E = ent[R.off + e];
+      TfEnt E = tf_entry(R, e, ent, draws, rinfo);
       int er[2] = {E.ar, E.br}, et[2] = {E.at, E.bt};
       double acc = 0.0;
       for (int q = 0; q < kn; ++q) {
@@ -243,6 +256,54 @@ __global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P,
       }
       out[R.off + e] += acc;
     }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_tf_rows(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
+                                                const double *__restrict__ x, int n, const double *__restrict__ rho,
+                                                const TfRow *__restrict__ rows, const TfEnt *__restrict__ ent,
+                                                const TfGeo *__restrict__ geo, const int *__restrict__ draws,
+                                                double *__restrict__ out, double *__restrict__ stat)
+{
+  __shared__ double c_td[2][TF_CH], c_tm[2][TF_CH], c_es[2][TF_CH], c_hk[TF_CH];
+  __shared__ int c_on[2][TF_CH], c_tk[TF_CH];
+  __shared__ double red[256];
+  const TfRow R = rows[blockIdx.x];
+  const bool dh = P->dh_flag != 0;
+  const double ks = P->water_kappa_sigma, th = P->treshold;
+  // base: the pair's water, burial and electrostatics
+  for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
+    TfEnt E = tf_entry(R, e, ent, draws, rinfo);
+    double r = R.r, ra = R.ra, rb = R.rb;
+    if (R.kind == 0) {
+      TfGeo G = geo[R.off + e];
+      r = G.r; ra = G.ra; rb = G.rb;
+    }
+    double td, tm;
+    tf_thetas(P, r, td, tm);
+    double v = tf_water(P, rinfo, E.ar, E.br, E.at, E.bt, td, tm, 1.0 - tanh(ks * (ra - th)), 1.0 - tanh(ks * (rb - th)));
+    v += tf_burial(P, rinfo, E.ar, E.at, ra) + tf_burial(P, rinfo, E.br, E.bt, rb);
+    if (dh) v += tf_elec(P, E.ar, E.br, E.at, E.bt, exp(-P->dh_kscreen * r / P->dh_length) / r);
+    out[R.off + e] = v;
+  }
+  if (R.kind == 1) tf_env_pass(P, rinfo, x, n, rho, R, ent, draws, out, c_td, c_tm, c_es, c_hk, c_on, c_tk);
+  // the row's decoy statistics (compute_array_mean / _std, fb.cpp:5413-5442:
+  // population standard deviation)
+  __syncthreads();
+  const int m = R.cnt - R.nat;
+  double sum = 0.0;
+  for (int e = R.nat + threadIdx.x; e < R.cnt; e += blockDim.x) sum += out[R.off + e];
+  const double mean = m > 0 ? tf_block_sum(sum, red) / m : 0.0;
+  double sq = 0.0;
+  for (int e = R.nat + threadIdx.x; e < R.cnt; e += blockDim.x) {
+    double d = out[R.off + e] - mean;
+    sq += d * d;
+  }
+  const double var = m > 0 ? tf_block_sum(sq, red) / m : 0.0;
+  if (threadIdx.x == 0) {
+    stat[3 * (size_t)blockIdx.x] = R.nat ? out[R.off] : 0.0;
+    stat[3 * (size_t)blockIdx.x + 1] = mean;
+    stat[3 * (size_t)blockIdx.x + 2] = sqrt(var);
   }
 }
 
@@ -617,6 +678,7 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
   std::vector<TfRow> rows;
   std::vector<TfEnt> ent;
   std::vector<TfGeo> geo;
+  std::vector<int> draws;
   struct Pair { int i, j; double r; };
   std::vector<Pair> contacts;
   long long off = 0;
@@ -631,14 +693,14 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
   }
   if (P.tf_mode == 0) {
     for (const Pair &q : contacts) {
-      rows.push_back(TfRow{0, q.i, q.j, 1, off, q.r, rho[q.i], rho[q.j]});
+      rows.push_back(TfRow{0, q.i, q.j, 1, 1, 0, off, 0, q.r, rho[q.i], rho[q.j]});
       ent.push_back(TfEnt{q.i, type(q.i), q.j, type(q.j)});
       geo.push_back(TfGeo{q.r, rho[q.i], rho[q.j]});
       off += 1;
     }
     if (!F.done && !contacts.empty()) {
       conf_row = (int)rows.size();
-      rows.push_back(TfRow{0, 0, 0, m, off, 0.0, 0.0, 0.0});
+      rows.push_back(TfRow{0, 0, 0, m, 0, 0, off, 0, 0.0, 0.0, 0.0});
       for (int d = 0; d < m; ++d) {
         int a = F.next() % n, b = F.next() % n;
         double r = dist(S, xres, a, b);
@@ -655,24 +717,26 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
       off += m;
     }
   } else if (P.tf_mode == 1) {
+    // the decoys' residue pairs only: the kernel reads their types
+    draws.reserve(contacts.size() * 2 * (size_t)m);
     for (const Pair &q : contacts) {
-      rows.push_back(TfRow{1, q.i, q.j, m + 1, off, q.r, rho[q.i], rho[q.j]});
-      ent.push_back(TfEnt{q.i, type(q.i), q.j, type(q.j)});
+      rows.push_back(TfRow{1, q.i, q.j, m + 1, 1, 0, off, (long long)draws.size(), q.r, rho[q.i], rho[q.j]});
       for (int d = 0; d < m; ++d) {
         int a = F.next() % n, b = F.next() % n;
-        ent.push_back(TfEnt{a, type(a), b, type(b)});
+        draws.push_back(a);
+        draws.push_back(b);
       }
       off += m + 1;
     }
   } else {
     for (int i = 0; i < n; ++i) {
-      rows.push_back(TfRow{2, i, -1, m + 1, off, 0.0, rho[i], 0.0});
+      rows.push_back(TfRow{2, i, -1, m + 1, 1, 0, off, 0, 0.0, rho[i], 0.0});
       ent.push_back(TfEnt{i, type(i), 0, 0});
       for (int d = 0; d < m; ++d) ent.push_back(TfEnt{i, type(F.next() % n), 0, 0});
       off += m + 1;
     }
   }
-  std::vector<double> e((size_t)off);
+  std::vector<double> stat(rows.size() * 3);  // per row: native, decoy mean, decoy std
   if (P.tf_mode == 2) {
     // a single-residue entry depends on (residue, type) only: one table of
     // n x 20 energies (compute_singleresidue_native_ixn, fb.cpp:5346-5393)
@@ -684,29 +748,40 @@ static AwStatus tert_frust(awsem_ctx *c, const double *xres, std::vector<double>
     std::vector<double> T((size_t)n * 20);
     TF_HIPCHK(hipMemcpyAsync(T.data(), b_t.p, T.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     TF_HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> e((size_t)off);
     for (size_t k = 0; k < e.size(); ++k) e[k] = T[(size_t)ent[k].ar * 20 + ent[k].at];
+    for (size_t k = 0; k < rows.size(); ++k) {
+      stat[3 * k] = e[rows[k].off];
+      stats(&e[rows[k].off + 1], m, &stat[3 * k + 1]);
+    }
   } else if (!rows.empty()) {
-    DevBuf b_rows, b_ent, b_geo, b_out;
+    // every entry's energy, and each row's statistics, on the device
+    DevBuf b_rows, b_ent, b_geo, b_draws, b_out, b_stat;
     if (geo.empty()) geo.push_back(TfGeo{0, 0, 0});
-    if (!(s = b_rows.put(rows, st)).ok() || !(s = b_ent.put(ent, st)).ok() || !(s = b_geo.put(geo, st)).ok()) return s;
-    TF_HIPCHK(hipMalloc(&b_out.p, std::max<size_t>(e.size(), 1) * sizeof(double)));
+    if (!(s = b_rows.put(rows, st)).ok() || !(s = b_ent.put(ent, st)).ok() || !(s = b_geo.put(geo, st)).ok() ||
+        !(s = b_draws.put(draws, st)).ok())
+      return s;
+    TF_HIPCHK(hipMalloc(&b_out.p, std::max<long long>(off, 1) * sizeof(double)));
+    TF_HIPCHK(hipMalloc(&b_stat.p, stat.size() * sizeof(double)));
     hipLaunchKernelGGL(k_tf_rows, dim3(rows.size()), dim3(256), 0, st, D.params, D.rinfo, D.x, n, d_rho,
-                       (const TfRow *)b_rows.p, (const TfEnt *)b_ent.p, (const TfGeo *)b_geo.p, (double *)b_out.p);
+                       (const TfRow *)b_rows.p, (const TfEnt *)b_ent.p, (const TfGeo *)b_geo.p,
+                       (const int *)b_draws.p, (double *)b_out.p, (double *)b_stat.p);
     TF_HIPCHK(hipGetLastError());
-    TF_HIPCHK(hipMemcpyAsync(e.data(), b_out.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    TF_HIPCHK(hipMemcpyAsync(stat.data(), b_stat.p, stat.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     TF_HIPCHK(hipStreamSynchronize(st));
   }
   if (conf_row >= 0) {
-    stats(&e[rows[conf_row].off], m, F.stats);
+    F.stats[0] = stat[3 * conf_row + 1];
+    F.stats[1] = stat[3 * conf_row + 2];
     F.done = true;
   }
   table.clear();
   for (size_t k = 0; k < rows.size(); ++k) {
     const TfRow &R = rows[k];
     if ((int)k == conf_row) continue;
-    double nat = e[R.off], sd[2];
+    double nat = stat[3 * k], sd[2];
     if (R.kind == 0) { sd[0] = F.stats[0]; sd[1] = F.stats[1]; }
-    else stats(&e[R.off + 1], m, sd);
+    else { sd[0] = stat[3 * k + 1]; sd[1] = stat[3 * k + 2]; }
     double row[11] = {(double)(R.a + 1), R.kind == 2 ? 0.0 : (double)(R.b + 1), (double)(chain(R.a) + 1),
                       R.kind == 2 ? 0.0 : (double)(chain(R.b) + 1), R.kind == 2 ? 0.0 : R.r, R.ra,
                       R.kind == 2 ? 0.0 : R.rb, nat, sd[0], sd[1], (sd[0] - nat) / sd[1]};
