@@ -339,6 +339,8 @@ __global__ void __launch_bounds__(256) k_nm_contacts(const FFParams *__restrict_
 struct NmRow {
   int a, b, cnt, ncont;  // the two n-mers' first residues; entries; contacts
   long long off;         // first entry
+  long long doff;        // the decoys' starts at draws[doff + 2 d], [doff + 2 d + 1]
+  int m, nt;             // decoys, trap entries (0 or 4)
 };
 // the types of n-mer a are those of residues sa + da*o, of n-mer b those
 // of sb + db*o; swap: the reference's order is b's burials first and b
@@ -347,6 +349,22 @@ struct NmEnt {
   int sa, sb;
   int da, db, swap, pad;
 };
+
+// entry e of a row: 0 the native, 1..m the decoys, then the four trap
+// entries: the second n-mer's types from residues o (forward) or s - o
+// (backward), for (i, j) and, in the reference's order, (j, i)
+// (fb.cpp:5829, 5839: the trap types are read without k)
+__device__ __forceinline__ NmEnt nm_entry(const NmRow &R, int e, const int *__restrict__ draws, int s)
+{
+  if (e == 0) return NmEnt{R.a, R.b, 1, 1, 0, 0};
+  if (e <= R.m) return NmEnt{draws[R.doff + 2 * (e - 1)], draws[R.doff + 2 * (e - 1) + 1], 1, 1, 0, 0};
+  switch (e - 1 - R.m) {
+    case 0: return NmEnt{R.a, 0, 1, 1, 0, 0};
+    case 1: return NmEnt{R.a, s, 1, -1, 0, 0};
+    case 2: return NmEnt{0, R.b, 1, 1, 1, 0};
+    default: return NmEnt{s, R.b, -1, 1, 1, 0};
+  }
+}
 
 // every (row, entry) energy of the pairwise mode: burials of both n-mers
 // and the water of their s*s cross pairs (compute_nmer_native_ixn,
@@ -357,9 +375,11 @@ struct NmEnt {
 __global__ void __launch_bounds__(256) k_nm_pairs(const FFParams *__restrict__ P, const int *__restrict__ rinfo,
                                                  const double *__restrict__ x, int n, int s,
                                                  const double *__restrict__ rho, const NmRow *__restrict__ rows,
-                                                 const NmEnt *__restrict__ ent, double *__restrict__ out)
+                                                 const int *__restrict__ draws, double *__restrict__ out,
+                                                 double *__restrict__ stat)
 {
   __shared__ double l_td[NM_MAX * NM_MAX], l_tm[NM_MAX * NM_MAX];
+  __shared__ double red[256];
   __shared__ double l_h[2][NM_MAX];
   __shared__ double l_bur[2][NM_MAX][20];
   const NmRow R = rows[blockIdx.x];
@@ -382,7 +402,7 @@ __global__ void __launch_bounds__(256) k_nm_pairs(const FFParams *__restrict__ P
   }
   __syncthreads();
   for (int e = threadIdx.x; e < R.cnt; e += blockDim.x) {
-    const NmEnt E = ent[R.off + e];
+    const NmEnt E = nm_entry(R, e, draws, s);
     double v = 0.0;
     if (!E.swap) {
       for (int o = 0; o < s; ++o) v += l_bur[0][o][rinfo[E.sa + E.da * o] & RI_TYPE_MASK];
@@ -406,6 +426,24 @@ __global__ void __launch_bounds__(256) k_nm_pairs(const FFParams *__restrict__ P
       }
     }
     out[R.off + e] = v;
+  }
+  // the row's decoy mean and (population) standard deviation
+  // (fb.cpp:6058-6059), its native and trap energies: 7 values per row
+  __syncthreads();
+  double sum = 0.0;
+  for (int e = 1 + threadIdx.x; e <= R.m; e += blockDim.x) sum += out[R.off + e];
+  const double mean = tf_block_sum(sum, red) / R.m;
+  double sq = 0.0;
+  for (int e = 1 + threadIdx.x; e <= R.m; e += blockDim.x) {
+    double d = out[R.off + e] - mean;
+    sq += d * d;
+  }
+  const double var = tf_block_sum(sq, red) / R.m;
+  if (threadIdx.x < 7) {
+    const int q = threadIdx.x;
+    double v = q == 0 ? out[R.off] : (q == 1 ? mean : (q == 2 ? sqrt(var) : 0.0));
+    if (q >= 3 && q - 3 < R.nt) v = out[R.off + 1 + R.m + (q - 3)];
+    stat[7 * (size_t)blockIdx.x + q] = v;
   }
 }
 
@@ -877,7 +915,7 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
   std::vector<int> cnt((size_t)rb * ns);
   const int nt = P.nf_trap ? 4 : 0;  // trap entries: (i, j) and (j, i), forward and backward
   std::vector<NmRow> rows;
-  std::vector<NmEnt> ent;
+  std::vector<int> draws;  // the decoys' n-mer starts, two per decoy
   long long off = 0;
   for (int i = 0; i < ns; ++i) {
     if (i % rb == 0) {
@@ -892,8 +930,7 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
     for (int j = i + 1; j < ns; ++j) {
       int k = cnt[(size_t)(i % rb) * ns + j];
       if (!(k >= P.nf_contacts && j - i >= s)) continue;
-      rows.push_back(NmRow{i, j, 1 + m + nt, k, off});
-      ent.push_back(NmEnt{i, j, 1, 1, 0, 0});
+      rows.push_back(NmRow{i, j, 1 + m + nt, k, off, (long long)draws.size(), m, nt});
       for (int d = 0; d < m; ++d) {
         int ir = F.next() % n, jr = F.next() % n;
         if (ir > jr) std::swap(ir, jr);
@@ -902,35 +939,30 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
           jr = F.next() % n;
           if (ir > jr) std::swap(ir, jr);
         }
-        ent.push_back(NmEnt{ir, jr, 1, 1, 0, 0});
-      }
-      if (nt) {
-        // the second n-mer's types from residues o (forward) or s - o
-        // (backward): the reference reads them without k (fb.cpp:5829, 5839)
-        ent.push_back(NmEnt{i, 0, 1, 1, 0, 0});
-        ent.push_back(NmEnt{i, s, 1, -1, 0, 0});
-        ent.push_back(NmEnt{0, j, 1, 1, 1, 0});
-        ent.push_back(NmEnt{s, j, -1, 1, 1, 0});
+        draws.push_back(ir);
+        draws.push_back(jr);
       }
       off += 1 + m + nt;
     }
   }
   if (rows.empty()) return AwStatus();
-  DevBuf b_rows, b_ent, b_out;
-  if (!(st = b_rows.put(rows, stream)).ok() || !(st = b_ent.put(ent, stream)).ok()) return st;
+  // every entry's energy, and per row the native, decoy mean and std and
+  // the four trap energies, on the device
+  DevBuf b_rows, b_draws, b_out, b_stat;
+  if (!(st = b_rows.put(rows, stream)).ok() || !(st = b_draws.put(draws, stream)).ok()) return st;
   TF_HIPCHK(hipMalloc(&b_out.p, (size_t)off * sizeof(double)));
+  TF_HIPCHK(hipMalloc(&b_stat.p, rows.size() * 7 * sizeof(double)));
   hipLaunchKernelGGL(k_nm_pairs, dim3((unsigned)rows.size()), dim3(256), 0, stream, D.params, D.rinfo, D.x, n, s,
-                     d_rho, (const NmRow *)b_rows.p, (const NmEnt *)b_ent.p, (double *)b_out.p);
+                     d_rho, (const NmRow *)b_rows.p, (const int *)b_draws.p, (double *)b_out.p, (double *)b_stat.p);
   TF_HIPCHK(hipGetLastError());
-  e.resize((size_t)off);
-  TF_HIPCHK(hipMemcpyAsync(e.data(), b_out.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  e.resize(rows.size() * 7);
+  TF_HIPCHK(hipMemcpyAsync(e.data(), b_stat.p, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
   TF_HIPCHK(hipStreamSynchronize(stream));
   std::vector<char> ssg(s + 1), ssk(s + 1);
   for (size_t r = 0; r < rows.size(); ++r) {
     const NmRow &R = rows[r];
-    const double *v = &e[R.off];
-    double sd[2];
-    stats_of(v + 1, sd);
+    const double *v = &e[7 * r];  // native, mean, std, traps (i, j) fw / bw, (j, i) fw / bw
+    const double sd[2] = {v[1], v[2]};
     if (nt) {
       // compute_nmer_traps (fb.cpp:5786-5893) for (i, j), then (j, i)
       const double thr = v[0] - P.nf_trap_sigma * sd[1];
@@ -938,7 +970,7 @@ static AwStatus nmer_frust(awsem_ctx *c, const double *xres, std::vector<double>
         const int f = dir ? R.b : R.a, g = dir ? R.a : R.b;
         nm_ss(S, g, ssg.data());
         for (int bw = 0; bw < 2; ++bw) {
-          const double et = v[1 + m + 2 * dir + bw];
+          const double et = v[3 + 2 * dir + bw];
           for (int k = 0; k < ns; ++k) {
             if (std::abs(k - g) <= s || (std::abs(k - f) <= s && f != k)) continue;
             nm_ss(S, k, ssk.data());
