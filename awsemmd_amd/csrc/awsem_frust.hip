@@ -48,6 +48,8 @@ AwFrustState::~AwFrustState()
   if (nm_trap) std::fclose(nm_trap);
   for (FILE *f : ff_out)
     if (f) std::fclose(f);
+  for (void *p : ff_dev)
+    if (p) (void)hipFree(p);
 }
 
 void AwFrustState::seed(unsigned s)
@@ -662,6 +664,20 @@ struct DevBuf {
   }
 };
 
+// a vector copied to device memory that lives with the context's
+// frustratometer state (freed with it)
+template <class T>
+AwStatus put_static(const std::vector<T> &v, void *&p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)) != hipSuccess)
+    return AwStatus::err(AWSEM_E_NOMEM, "frustratometer: device allocation");
+  if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return AwStatus::err(AWSEM_E_DEVICE, "frustratometer: copy");
+  return AwStatus();
+}
+
 // site (CB; CA for glycine) of residue i in the residue layout
 inline const double *site(const AwSystem &S, const double *x, int i)
 {
@@ -1046,25 +1062,33 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
   AwFrustState &F = aw_ctx_frust(c);
   if (F.ff_pos.empty())
     for (const AwMemory &m : S.ff_mems) F.ff_pos.push_back(m.pos);
-  // gamma by separation and types (Gamma_Array::getGamma, fm.cpp:400-424)
-  int dmax = 0;
-  for (const AwMemory &m : S.mems) dmax = std::max(dmax, m.len);
-  for (const AwMemory &m : S.ff_mems) dmax = std::max(dmax, m.len);
-  std::vector<double> gsep((size_t)(dmax + 1) * 400);
-  for (int d = 0; d <= dmax; ++d)
-    for (int a = 0; a < 20; ++a)
-      for (int b = 0; b < 20; ++b) gsep[(size_t)d * 400 + a * 20 + b] = S.fmg.get(a, b, 0, d);
-  std::vector<FfItem> nat_items, dec_items;
-  ff_items(S, S.mems, nat_items);
-  std::vector<int> nat_pos(nfm);
-  std::vector<double> nat_w(nfm), dec_w(std::max(nd, 1));
-  for (int k = 0; k < nfm; ++k) { nat_pos[k] = S.mems[k].pos; nat_w[k] = S.mems[k].weight; }
-  for (int k = 0; k < nd; ++k) dec_w[k] = S.ff_mems[k].weight;
-  DevBuf b_g, b_err, b_ni, b_np, b_nw, b_E;
+  if (!F.ff_ready) {
+    // gamma by separation and types (Gamma_Array::getGamma, fm.cpp:400-424)
+    int dmax = 0;
+    for (const AwMemory &m : S.mems) dmax = std::max(dmax, m.len);
+    for (const AwMemory &m : S.ff_mems) dmax = std::max(dmax, m.len);
+    std::vector<double> gsep((size_t)(dmax + 1) * 400);
+    for (int d = 0; d <= dmax; ++d)
+      for (int a = 0; a < 20; ++a)
+        for (int b = 0; b < 20; ++b) gsep[(size_t)d * 400 + a * 20 + b] = S.fmg.get(a, b, 0, d);
+    std::vector<FfItem> nat_items, dec_items;
+    ff_items(S, S.mems, nat_items);
+    ff_items(S, S.ff_mems, dec_items);
+    std::vector<int> nat_pos(std::max(nfm, 1));
+    std::vector<double> nat_w(std::max(nfm, 1)), dec_w(std::max(nd, 1));
+    for (int k = 0; k < nfm; ++k) { nat_pos[k] = S.mems[k].pos; nat_w[k] = S.mems[k].weight; }
+    for (int k = 0; k < nd; ++k) dec_w[k] = S.ff_mems[k].weight;
+    if (!(st = put_static(gsep, F.ff_dev[0])).ok() || !(st = put_static(nat_items, F.ff_dev[1])).ok() ||
+        !(st = put_static(nat_pos, F.ff_dev[2])).ok() || !(st = put_static(nat_w, F.ff_dev[3])).ok() ||
+        !(st = put_static(dec_items, F.ff_dev[4])).ok() || !(st = put_static(dec_w, F.ff_dev[5])).ok())
+      return st;
+    F.ff_n_nat = (int)nat_items.size();
+    F.ff_n_dec = (int)dec_items.size();
+    F.ff_ready = true;
+  }
+  DevBuf b_err, b_E;
   std::vector<int> zero(1, 0);
-  if (!(st = b_g.put(gsep, stream)).ok() || !(st = b_err.put(zero, stream)).ok() || !(st = b_ni.put(nat_items, stream)).ok() ||
-      !(st = b_np.put(nat_pos, stream)).ok() || !(st = b_nw.put(nat_w, stream)).ok())
-    return st;
+  if (!(st = b_err.put(zero, stream)).ok()) return st;
   const int ncalc_dev = P.ff_mode == 0 ? nc : 1;  // read mode: the natives only
   TF_HIPCHK(hipMalloc(&b_E.p, (size_t)ncalc_dev * n * sizeof(double)));
   TF_HIPCHK(hipMemsetAsync(b_E.p, 0, (size_t)ncalc_dev * n * sizeof(double), stream));
@@ -1078,15 +1102,15 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
       hipLaunchKernelGGL(k_ff_mem_lds, dim3((unsigned)((n_items + FF_CHUNK - 1) / FF_CHUNK), (unsigned)calcs),
                          dim3(256), (size_t)n * sizeof(double), stream, D.params, D.rinfo, D.x, n,
                          (const FfItem *)items, n_items, (const int *)pos, (const double *)w, nm,
-                         (const double *)b_g.p, out, d_err);
+                         (const double *)F.ff_dev[0], out, d_err);
     else
       hipLaunchKernelGGL(k_ff_mem, dim3((unsigned)((n_items + 255) / 256), (unsigned)calcs), dim3(256), 0, stream,
                          D.params, D.rinfo, D.x, n, (const FfItem *)items, n_items, (const int *)pos,
-                         (const double *)w, nm, (const double *)b_g.p, out, d_err);
+                         (const double *)w, nm, (const double *)F.ff_dev[0], out, d_err);
     return hipGetLastError();
   };
-  if (!nat_items.empty()) TF_HIPCHK(launch_mem(b_ni.p, (int)nat_items.size(), b_np.p, b_nw.p, nfm, 1, E));
-  DevBuf b_di, b_dp, b_dw, b_dx, b_dinfo, b_G;
+  if (F.ff_n_nat > 0) TF_HIPCHK(launch_mem(F.ff_dev[1], F.ff_n_nat, F.ff_dev[2], F.ff_dev[3], nfm, 1, E));
+  DevBuf b_dp, b_dx, b_dinfo, b_G;
   if (P.ff_mode == 0) {
     // randomize_decoys after every calculation (fb.cpp:4565-4611, 7546-7553):
     // calculation c >= 1 sees the placement drawn after calculation c - 1
@@ -1097,15 +1121,13 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
         F.ff_pos[k] = p;
         if (r + 1 < nc) pos[(size_t)r * nd + k] = p;
       }
-    ff_items(S, S.ff_mems, dec_items);
-    if (!dec_items.empty()) {
-      if (!(st = b_di.put(dec_items, stream)).ok() || !(st = b_dp.put(pos, stream)).ok() || !(st = b_dw.put(dec_w, stream)).ok())
-        return st;
-      TF_HIPCHK(launch_mem(b_di.p, (int)dec_items.size(), b_dp.p, b_dw.p, nd, nc - 1, E + n));
+    if (F.ff_n_dec > 0) {
+      if (!(st = b_dp.put(pos, stream)).ok()) return st;
+      TF_HIPCHK(launch_mem(F.ff_dev[4], F.ff_n_dec, b_dp.p, F.ff_dev[5], nd, nc - 1, E + n));
     }
   }
   const bool gen_now = P.ff_mode == 1 && !F.ff_gen_done;
-  if (gen_now && !nat_items.empty()) {
+  if (gen_now && F.ff_n_nat > 0) {
     // the decoy structures' coordinates: CA then CB per decoy
     std::vector<double> dxyz;
     std::vector<int> dinfo;
@@ -1119,9 +1141,10 @@ static AwStatus frag_frust(awsem_ctx *c, const double *xres, std::vector<double>
     if (!(st = b_dx.put(dxyz, stream)).ok() || !(st = b_dinfo.put(dinfo, stream)).ok()) return st;
     TF_HIPCHK(hipMalloc(&b_G.p, (size_t)nd * n * sizeof(double)));
     TF_HIPCHK(hipMemsetAsync(b_G.p, 0, (size_t)nd * n * sizeof(double), stream));
-    hipLaunchKernelGGL(k_ff_gen, dim3((unsigned)((nat_items.size() + 255) / 256), (unsigned)nd), dim3(256), 0, stream,
-                       D.params, D.rinfo, n, (const FfItem *)b_ni.p, (int)nat_items.size(), (const int *)b_np.p,
-                       (const double *)b_nw.p, (const double *)b_g.p, (const double *)b_dx.p, (const int *)b_dinfo.p,
+    hipLaunchKernelGGL(k_ff_gen, dim3((unsigned)((F.ff_n_nat + 255) / 256), (unsigned)nd), dim3(256), 0, stream,
+                       D.params, D.rinfo, n, (const FfItem *)F.ff_dev[1], F.ff_n_nat, (const int *)F.ff_dev[2],
+                       (const double *)F.ff_dev[3], (const double *)F.ff_dev[0], (const double *)b_dx.p,
+                       (const int *)b_dinfo.p,
                        (double *)b_G.p, d_err);
     TF_HIPCHK(hipGetLastError());
   }

@@ -249,6 +249,12 @@ struct AwFrustState {
   std::vector<int> ff_pos;
   bool ff_gen_done = false;
   std::vector<double> ff_mean, ff_var;
+  // its call-invariant device data, built at the first call: the gamma
+  // table; the memories' items, positions and weights; the decoy
+  // fragments' items and weights
+  void *ff_dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int ff_n_nat = 0, ff_n_dec = 0;
+  bool ff_ready = false;
   AwFrustState();
   ~AwFrustState();
   int next();
